@@ -1,0 +1,417 @@
+"""Scene fixtures and workloads for the wavefront path tracer.
+
+Every fixture is generated from a recorded master seed with the PCG hash the
+renderer itself uses, so the CPU oracle and the GPU read identical bytes.
+The reference's fixtures are restated from its sources:
+  * ``cornell_v1`` — adapters/common/reference.cpp:221-280 (SetCornellBox);
+  * ``grid`` indexing — adapters/common/workloads.cpp:10-37;
+  * material records — backend/vulkan/gpu_scene.cpp:2041-2089
+    (GpuMaterialRecord, gpu_scene.hpp:94-113);
+  * mesh / material table order — the SceneExtraction plan order
+    (include/lotus/extraction.hpp:107-117): instances in mesh key order,
+    the default material at slot 0, then materials in key order.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+F32 = np.float32
+
+
+def identity() -> np.ndarray:
+    return np.eye(4, dtype=F32).reshape(-1, order="F")
+
+
+def multiply(left: np.ndarray, right: np.ndarray) -> np.ndarray:
+    """Column-major float32 product with the reference's summation order
+    (core/render-world/render_world.cpp:288-300)."""
+    out = np.zeros(16, F32)
+    for c in range(4):
+        for r in range(4):
+            s = F32(0.0)
+            for i in range(4):
+                s = F32(s + F32(left[i * 4 + r] * right[c * 4 + i]))
+            out[c * 4 + r] = s
+    return out
+
+
+# ------------------------------------------------------------- materials --
+MATERIAL_WORDS = 36
+
+
+def preview_material(base_color=(0.18, 0.18, 0.18), roughness=0.5, metallic=0.0, ior=1.5,
+                     emission=(0.0, 0.0, 0.0), specular_workflow=False,
+                     specular_color=(0.0, 0.0, 0.0), transmission=0.0,
+                     transmission_color=(1.0, 1.0, 1.0), thin_walled=False,
+                     opacity=1.0, opacity_threshold=0.0, normal=(0.0, 0.0, 1.0)) -> np.ndarray:
+    """A UsdPreviewSurface GpuMaterialRecord (gpu_scene.hpp:94-113) as 36
+    words, filled as gpu_scene.cpp:2041-2089 fills it. Defaults are
+    PreviewSurface's (include/lotus/material.hpp:157-196)."""
+    w = np.zeros(MATERIAL_WORDS, np.uint32)
+    f = w.view(np.float32)
+    flags = (1 if specular_workflow else 0) | (2 if thin_walled else 0)
+    w[0] = 0 | (flags << 8)  # model kUsdPreviewSurface
+    f[4:8] = (0.0, 0.0, opacity, opacity_threshold)
+    f[8:12] = (*normal, 0.0)
+    f[12:16] = (*base_color, roughness)
+    f[16:20] = (*emission, metallic)
+    f[20:24] = (*specular_color, ior)
+    f[24:28] = (*transmission_color, transmission)
+    return w
+
+
+# ----------------------------------------------------------------- scene --
+@dataclasses.dataclass
+class Scene:
+    """Object-space geometries placed by instances; materials as records;
+    a camera as view and projection (include/lotus/render_world.hpp:31-37)."""
+
+    geometries: List[Tuple[np.ndarray, np.ndarray]] = dataclasses.field(default_factory=list)
+    instances: List[Tuple[int, np.ndarray, int]] = dataclasses.field(default_factory=list)
+    materials: List[np.ndarray] = dataclasses.field(default_factory=lambda: [preview_material()])
+    environment: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    view: np.ndarray = dataclasses.field(default_factory=identity)
+    projection: np.ndarray = dataclasses.field(default_factory=identity)
+    name: str = "scene"
+    _packed: Optional[dict] = None
+
+    def add_mesh(self, positions, triangles, material: int, transform=None) -> int:
+        positions = np.ascontiguousarray(positions, F32).reshape(-1, 3)
+        triangles = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+        self.geometries.append((positions, triangles))
+        t = identity() if transform is None else np.ascontiguousarray(transform, F32).reshape(16)
+        self.instances.append((len(self.geometries) - 1, t, material))
+        self._packed = None
+        return len(self.instances) - 1
+
+    def add_material(self, record: np.ndarray) -> int:
+        self.materials.append(np.ascontiguousarray(record, np.uint32))
+        self._packed = None
+        return len(self.materials) - 1
+
+    @property
+    def world_to_clip(self) -> np.ndarray:
+        """camera.projection * camera.view (core/render-extraction/extraction.cpp:12-19)."""
+        return multiply(self.projection, self.view)
+
+    @property
+    def triangle_count(self) -> int:
+        return int(sum(self.geometries[g][1].shape[0] for g, _, _ in self.instances))
+
+    def packed(self) -> dict:
+        """Contiguous arrays in the layout both the renderer's C ABI and the
+        oracle read."""
+        if self._packed is not None:
+            return self._packed
+        vo, to, tc, vc = [], [], [], []
+        v = t = 0
+        for pos, tri in self.geometries:
+            vo.append(v)
+            vc.append(pos.shape[0])
+            to.append(t)
+            tc.append(tri.shape[0])
+            v += pos.shape[0]
+            t += tri.shape[0]
+        self._packed = dict(
+            geometry_count=len(self.geometries),
+            positions=np.ascontiguousarray(np.concatenate([p for p, _ in self.geometries]).astype(F32)),
+            triangles=np.ascontiguousarray(np.concatenate([tr for _, tr in self.geometries]).astype(np.uint32)),
+            geometry_vertex_offset=np.array(vo, np.uint32),
+            geometry_vertex_count=np.array(vc, np.uint32),
+            geometry_triangle_offset=np.array(to, np.uint32),
+            geometry_triangle_count=np.array(tc, np.uint32),
+            instance_count=len(self.instances),
+            instance_geometry=np.array([g for g, _, _ in self.instances], np.uint32),
+            instance_transform=np.ascontiguousarray(np.stack([tr for _, tr, _ in self.instances]).astype(F32)),
+            instance_material=np.array([m for _, _, m in self.instances], np.uint32),
+            material_count=len(self.materials),
+            materials=np.ascontiguousarray(np.stack(self.materials).astype(np.uint32)),
+            environment=np.array(self.environment, F32),
+        )
+        return self._packed
+
+
+# ------------------------------------------------------------ randomness --
+def pcg_hash(v: np.ndarray) -> np.ndarray:
+    """PCG RXS-M-XS hash (sampling.slang:9-13) over uint32 arrays."""
+    v = np.asarray(v, np.uint64) & 0xFFFFFFFF
+    state = (v * 747796405 + 2891336453) & 0xFFFFFFFF
+    word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & 0xFFFFFFFF
+    return ((word >> 22) ^ word).astype(np.uint32)
+
+
+def uniform(seed: int, stream: int, index) -> np.ndarray:
+    """Seeded uniform numbers in [0, 1) with 24 bits, indexed."""
+    key = pcg_hash(np.uint32(seed) ^ pcg_hash(np.uint32(stream)))
+    h = pcg_hash(np.asarray(index, np.uint64) ^ np.uint64(key))
+    return ((h >> 8).astype(np.float64) * (1.0 / 16777216.0))
+
+
+# --------------------------------------------------------------- cameras --
+def perspective(fovy_degrees: float, aspect: float, near: float, far: float) -> np.ndarray:
+    f = 1.0 / math.tan(math.radians(fovy_degrees) / 2.0)
+    m = np.zeros(16, F32)
+    m[0] = f / aspect
+    m[5] = f
+    m[10] = (far + near) / (near - far)
+    m[11] = -1.0
+    m[14] = 2.0 * far * near / (near - far)
+    return m
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    eye, target, up = (np.asarray(x, np.float64) for x in (eye, target, up))
+    f = target - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = s, u, -f
+    m[0, 3], m[1, 3], m[2, 3] = -s @ eye, -u @ eye, f @ eye
+    return m.astype(F32).reshape(-1, order="F")
+
+
+# --------------------------------------------------------------- Cornell --
+def _quad(corners):
+    return np.array(corners, F32), np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+UNIT_CUBE_POS = np.array([[-0.5, -0.5, -0.5], [0.5, -0.5, -0.5], [0.5, 0.5, -0.5], [-0.5, 0.5, -0.5],
+                          [-0.5, -0.5, 0.5], [0.5, -0.5, 0.5], [0.5, 0.5, 0.5], [-0.5, 0.5, 0.5]], F32)
+UNIT_CUBE_TRI = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                          [3, 7, 6], [3, 6, 2], [0, 4, 7], [0, 7, 3], [1, 2, 6], [1, 6, 5]], np.uint32)
+
+
+def block(size, degrees, centre) -> np.ndarray:
+    """reference.cpp:51-62: scale, turn about +Y, move."""
+    r = degrees * 3.14159265358979323846 / 180.0
+    c, s = F32(math.cos(r)), F32(math.sin(r))
+    sx, sy, sz = (F32(v) for v in size)
+    return np.array([c * sx, 0, -s * sx, 0, 0, sy, 0, 0, s * sz, 0, c * sz, 0,
+                     centre[0], centre[1], centre[2], 1], F32)
+
+
+def cornell_camera():
+    """reference.cpp:222-234: 3 units in front, 45 degree vertical field."""
+    focal, near, far = F32(2.41421356), F32(1.0), F32(10.0)
+    view = identity()
+    view[14] = -3.0
+    proj = np.array([focal, 0, 0, 0, 0, focal, 0, 0, 0, 0, (far + near) / (near - far), -1,
+                     0, 0, F32(2.0) * far * near / (near - far), 0], F32)
+    return view, proj
+
+
+def cornell_v1(red_scale: float = 1.0) -> Scene:
+    """The reference scene (reference.cpp:221-280): coated-diffuse walls, an
+    emissive panel (15, 13, 10), a GGX metal block and a white block."""
+    sc = Scene(name="cornell-v1")
+    sc.view, sc.projection = cornell_camera()
+    white, red, green = (0.73, 0.73, 0.73), (0.63, 0.065, 0.05), (0.14, 0.45, 0.091)
+    red = tuple(float(F32(c) * F32(red_scale)) for c in red)
+    # material table: default, then keys in order: green, light, metal, red, white
+    m_green = sc.add_material(preview_material(base_color=green))
+    m_light = sc.add_material(preview_material(base_color=(0, 0, 0), emission=(15.0, 13.0, 10.0)))
+    m_metal = sc.add_material(preview_material(base_color=(0.95, 0.85, 0.6), roughness=0.35, metallic=1.0))
+    m_red = sc.add_material(preview_material(base_color=red))
+    m_white = sc.add_material(preview_material(base_color=white))
+    # meshes in key order: back, ceiling, floor, left, light, right, short, tall
+    sc.add_mesh(*_quad([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]]), m_white)
+    sc.add_mesh(*_quad([[-1, 1, 1], [-1, 1, -1], [1, 1, -1], [1, 1, 1]]), m_white)
+    sc.add_mesh(*_quad([[-1, -1, 1], [1, -1, 1], [1, -1, -1], [-1, -1, -1]]), m_white)
+    sc.add_mesh(*_quad([[-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1]]), m_red)
+    sc.add_mesh(*_quad([[-0.35, 0.98, 0.35], [-0.35, 0.98, -0.35], [0.35, 0.98, -0.35], [0.35, 0.98, 0.35]]),
+                m_light)
+    sc.add_mesh(*_quad([[1, -1, -1], [1, -1, 1], [1, 1, 1], [1, 1, -1]]), m_green)
+    sc.add_mesh(UNIT_CUBE_POS, UNIT_CUBE_TRI, m_white, block((0.6, 0.6, 0.6), -18.0, (0.38, -0.699, 0.3)))
+    sc.add_mesh(UNIT_CUBE_POS, UNIT_CUBE_TRI, m_metal, block((0.6, 1.2, 0.6), 17.0, (-0.35, -0.399, -0.3)))
+    return sc
+
+
+def cornell_lambert() -> Scene:
+    """The CPU plain-PT workload: the reference box's five walls and two
+    blocks, Lambertian (PreviewSurface with ior 1, metallic 0), lit by a
+    0.25 x 0.25 emissive ceiling quad of radiance 15. Walls 0.73, left
+    (0.65, 0.05, 0.05), right (0.12, 0.45, 0.15); 10 + 24 + 2 triangles."""
+    sc = Scene(name="cornell-lambert")
+    sc.view, sc.projection = cornell_camera()
+    lam = lambda c: preview_material(base_color=c, ior=1.0)
+    m_white = sc.add_material(lam((0.73, 0.73, 0.73)))
+    m_left = sc.add_material(lam((0.65, 0.05, 0.05)))
+    m_right = sc.add_material(lam((0.12, 0.45, 0.15)))
+    m_light = sc.add_material(preview_material(base_color=(0, 0, 0), ior=1.0, emission=(15.0, 15.0, 15.0)))
+    sc.add_mesh(*_quad([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]]), m_white)
+    sc.add_mesh(*_quad([[-1, 1, 1], [-1, 1, -1], [1, 1, -1], [1, 1, 1]]), m_white)
+    sc.add_mesh(*_quad([[-1, -1, 1], [1, -1, 1], [1, -1, -1], [-1, -1, -1]]), m_white)
+    sc.add_mesh(*_quad([[-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1]]), m_left)
+    h = 0.125
+    sc.add_mesh(*_quad([[-h, 0.98, h], [-h, 0.98, -h], [h, 0.98, -h], [h, 0.98, h]]), m_light)
+    sc.add_mesh(*_quad([[1, -1, -1], [1, -1, 1], [1, 1, 1], [1, 1, -1]]), m_right)
+    sc.add_mesh(UNIT_CUBE_POS, UNIT_CUBE_TRI, m_white, block((0.6, 0.6, 0.6), -18.0, (0.38, -0.699, 0.3)))
+    sc.add_mesh(UNIT_CUBE_POS, UNIT_CUBE_TRI, m_white, block((0.6, 1.2, 0.6), 17.0, (-0.35, -0.399, -0.3)))
+    return sc
+
+
+# ------------------------------------------------------- closed-form scenes --
+def lambert_under_sky() -> Scene:
+    """adapters/headless/main.cpp:1525-1545: a lone Lambert triangle (ior 1)
+    under a constant environment reflects albedo * environment + emission."""
+    sc = Scene(name="lambert-sky")
+    sc.view, sc.projection = cornell_camera()
+    m = sc.add_material(preview_material(base_color=(0.5, 0.25, 0.75), ior=1.0, emission=(0.1, 0.05, 0.0)))
+    sc.add_mesh([[-0.65, -0.35, 0], [0.55, -0.45, 0], [0.15, 0.70, 0]], [[0, 1, 2]], m)
+    sc.environment = (0.8, 0.6, 0.4)
+    return sc
+
+
+def closed_box() -> Scene:
+    """adapters/headless/main.cpp:3073-3136: a closed Lambert box seen from
+    inside; L = Le (1 + a + ... + a^n), unlimited mean Le / (1 - a)."""
+    sc = Scene(name="closed-box")
+    sc.view = identity()
+    sc.projection = perspective(90.0, 1.0, 0.05, 10.0)
+    m = sc.add_material(preview_material(base_color=(0.5, 0.25, 0.75), ior=1.0, emission=(0.1, 0.15, 0.025)))
+    pos = UNIT_CUBE_POS * F32(2.0)
+    sc.add_mesh(pos, UNIT_CUBE_TRI, m)
+    sc.environment = (1.0, 1.0, 1.0)
+    return sc
+
+
+# ------------------------------------------------------------ heightfield --
+def value_noise(x: np.ndarray, z: np.ndarray, seed: int, octaves: int = 6,
+                base_frequency: float = 4.0, gain: float = 0.5, lacunarity: float = 2.0) -> np.ndarray:
+    """Seeded fractal value noise in [-1, 1]: per octave, lattice values from
+    the PCG hash interpolated with the cubic smoothstep; amplitudes scale by
+    `gain`, frequencies by `lacunarity`, normalised by the amplitude sum."""
+    total = np.zeros_like(x, np.float64)
+    norm = 0.0
+    amp, freq = 1.0, base_frequency
+    for o in range(octaves):
+        fx, fz = (x + 1.0) * 0.5 * freq, (z + 1.0) * 0.5 * freq
+        ix, iz = np.floor(fx).astype(np.int64), np.floor(fz).astype(np.int64)
+        tx, tz = fx - ix, fz - iz
+        sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+
+        def lattice(a, b):
+            key = (a & 0xFFFF) | ((b & 0xFFFF) << 16)
+            return uniform(seed, 100 + o, key) * 2.0 - 1.0
+
+        v00, v10 = lattice(ix, iz), lattice(ix + 1, iz)
+        v01, v11 = lattice(ix, iz + 1), lattice(ix + 1, iz + 1)
+        v0 = v00 + (v10 - v00) * sx
+        v1 = v01 + (v11 - v01) * sx
+        total += amp * (v0 + (v1 - v0) * sz)
+        norm += amp
+        amp *= gain
+        freq *= lacunarity
+    return total / norm
+
+
+def heightfield(cells: int = 1024, seed: int = 2026, amplitude: float = 0.2, tiles: int = 8,
+                emissive: int = 64, camera_aspect: float = 16.0 / 9.0) -> Scene:
+    """The wavefront workload: a cells x cells heightfield over [-1, 1]^2 in
+    x-z (2 triangles per cell, workloads.cpp:18-27 indexing), heights 0.2 x
+    6-octave value noise. The grid is split into tiles x tiles meshes, each
+    bound to its own seeded PreviewSurface: roughness U[0.05, 1], 20 %
+    conductors (metallic 1), 80 % dielectric-coated diffuse (ior 1.5).
+    `emissive` seeded triangles become single-triangle meshes with emission
+    U[5, 20] (grey). Triangle total: 2 * cells^2."""
+    n = cells + 1
+    lin = np.linspace(-1.0, 1.0, n)
+    gx, gz = np.meshgrid(lin, lin)  # row = z index
+    h = amplitude * value_noise(gx, gz, seed)
+    pos = np.stack([gx, h, gz], -1).reshape(-1, 3).astype(F32)
+    sc = Scene(name=f"heightfield-{cells}")
+    sc.view = look_at((0.0, 1.05, 1.75), (0.0, -0.05, 0.0))
+    sc.projection = perspective(45.0, camera_aspect, 0.05, 10.0)
+    tile_cells = cells // tiles
+    # per-tile materials
+    u = lambda stream, i: float(uniform(seed, stream, i))
+    tile_mat = []
+    for t in range(tiles * tiles):
+        rough = 0.05 + 0.95 * u(1, t)
+        if u(2, t) < 0.2:
+            bc = tuple(0.5 + 0.5 * u(3 + c, t) for c in range(3))
+            rec = preview_material(base_color=bc, roughness=rough, metallic=1.0)
+        else:
+            bc = tuple(0.1 + 0.8 * u(3 + c, t) for c in range(3))
+            rec = preview_material(base_color=bc, roughness=rough, metallic=0.0, ior=1.5)
+        tile_mat.append(sc.add_material(rec))
+    # emissive triangles: seeded distinct picks
+    total_tris = 2 * cells * cells
+    picks = []
+    k = 0
+    while len(picks) < emissive:
+        cand = int(uniform(seed, 7, k) * total_tris)
+        k += 1
+        if cand not in picks:
+            picks.append(cand)
+    pick_set = set(picks)
+    # tile meshes (each with its own vertex block)
+    for ty in range(tiles):
+        for tx in range(tiles):
+            y0, x0 = ty * tile_cells, tx * tile_cells
+            ys, xs = np.meshgrid(np.arange(y0, y0 + tile_cells), np.arange(x0, x0 + tile_cells), indexing="ij")
+            a = (ys * n + xs).reshape(-1)
+            b = a + 1
+            d = a + n
+            c = d + 1
+            tri = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], 1).reshape(-1, 3)
+            cell_index = (ys * cells + xs).reshape(-1)
+            gid = np.stack([2 * cell_index, 2 * cell_index + 1], 1).reshape(-1)
+            keep = np.array([g not in pick_set for g in gid]) if pick_set else np.ones(len(gid), bool)
+            tri = tri[keep]
+            used, inv = np.unique(tri.reshape(-1), return_inverse=True)
+            sc.add_mesh(pos[used], inv.reshape(-1, 3).astype(np.uint32), tile_mat[ty * tiles + tx])
+    for i, g in enumerate(picks):
+        cell, half = divmod(g, 2)
+        y, x = divmod(cell, cells)
+        a = y * n + x
+        b, d = a + 1, a + n
+        c = d + 1
+        tri = (a, b, c) if half == 0 else (a, c, d)
+        radiance = 5.0 + 15.0 * u(8, i)
+        m = sc.add_material(preview_material(base_color=(0.0, 0.0, 0.0), emission=(radiance,) * 3))
+        sc.add_mesh(pos[list(tri)], [[0, 1, 2]], m)
+    return sc
+
+
+def random_triangles(count: int, seed: int = 1, extent: float = 1.0, size: float = 0.2) -> Scene:
+    """Seeded random triangles in a cube, for traversal parity."""
+    idx = np.arange(count * 9)
+    centre = (uniform(seed, 1, np.arange(count * 3)).reshape(count, 1, 3) * 2 - 1) * extent
+    off = (uniform(seed, 2, idx).reshape(count, 3, 3) * 2 - 1) * size
+    pos = (centre + off).astype(F32).reshape(-1, 3)
+    sc = Scene(name=f"random-{count}")
+    m = sc.add_material(preview_material())
+    sc.add_mesh(pos, np.arange(count * 3, dtype=np.uint32).reshape(-1, 3), m)
+    sc.view = look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    sc.projection = perspective(45.0, 1.0, 0.05, 10.0)
+    return sc
+
+
+def grid(divisions: int = 64, extent: float = 1.0) -> Scene:
+    """The reference's Grid geometry (workloads.cpp:10-37): a divisions^2
+    quad grid with a small deterministic z ripple, seen from its +z side.
+    Every interior edge is shared: watertightness cases."""
+    n = divisions + 1
+    y, x = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    px = extent * (F32(2.0) * x.astype(F32) / F32(divisions) - F32(1.0))
+    py = extent * (F32(2.0) * y.astype(F32) / F32(divisions) - F32(1.0))
+    pz = F32(0.002) * ((x * 17 + y * 13) % 11).astype(F32)
+    pos = np.stack([px, py, pz], -1).reshape(-1, 3).astype(F32)
+    ys, xs = np.meshgrid(np.arange(divisions), np.arange(divisions), indexing="ij")
+    a = (ys * n + xs).reshape(-1)
+    b, d = a + 1, a + n
+    c = d + 1
+    tri = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], 1).reshape(-1, 3).astype(np.uint32)
+    sc = Scene(name=f"grid-{divisions}")
+    m = sc.add_material(preview_material())
+    sc.add_mesh(pos, tri, m)
+    sc.view = look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    sc.projection = perspective(45.0, 1.0, 0.05, 10.0)
+    return sc

@@ -1,0 +1,1391 @@
+// CPU oracle for the wavefront path tracer — TEST INFRASTRUCTURE ONLY.
+//
+// A C++ restatement of hydra-lotus's reference light transport, the loop the
+// reference integrator runs per pixel (backend/vulkan/shaders/path_trace.slang:
+// 32-40, 106-139) over common/transport.slang, common/scene.slang,
+// sampling/sampling.slang and bsdf/preview.slang, plus the host-side light
+// table (backend/vulkan/gpu_scene.cpp:144-203, 512-609) and camera constants
+// (backend/vulkan/vulkan_internal.hpp:326-336, vulkan_backend.cpp:436-478,
+// 2399-2470). Only tests, the smoke check and the CPU baseline call it.
+//
+// Arithmetic conventions. The oracle evaluates every float expression in
+// IEEE single precision in the order written, without contraction (built with
+// -ffp-contract=off). Where the reference calls a function whose rounding is
+// the GPU's own (sin, cos, rsqrt, pow), it uses a stated replacement:
+//   sin/cos(2 pi u): SinCosTurns below (quadrant reduction + Taylor
+//     polynomials to degree 9/10, absolute error < 2e-8 on [-pi/4, pi/4]);
+//   rsqrt(x) = 1 / sqrt(x);   normalize(v) = v * rsqrt(dot(v, v));
+//   pow(x, 5) = ((x*x)*(x*x))*x;   dot = (x*x' + y*y') + z*z'.
+// These differ from the reference GPU's by a few ulp, which the statistical
+// tests absorb; they make the oracle's decisions reproducible bit for bit.
+//
+// Pinning (which check holds which function; tests/test_oracle_*.py):
+//   PCG RNG                 golden values of the RXS-M-XS permutation computed
+//                           independently in numpy (uint64 arithmetic);
+//   SinCosTurns             numpy float64 sin/cos, |err| < 1e-7;
+//   camera constants        closed-form inverse of the reference Cornell camera
+//                           + reference-built Invert (oracle/_ref) where built;
+//   watertight intersection brute force in float64 (Moller-Trumbore) on random
+//                           triangles, shared-edge and shared-vertex fans;
+//   BVH traversal           oracle brute force, exact ids;
+//   BSDF                    Lambert closed form, white furnace (ior 1, albedo 1),
+//                           pdf integrates to the sampled mass, sample histogram
+//                           vs pdf, reciprocity of the layered lobes;
+//   light table             closed-form alias tables, sampling frequencies;
+//   transport               closed forms of the reference's evidence scenes
+//                           (adapters/headless/main.cpp:1525-1545, 3073-3136),
+//                           and the reference's committed Cornell images
+//                           (validation/reference/cornell-box-*.pfm) by DES-Q5.
+#include "lotus_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kFloatMax = 3.402823466e+38f;
+constexpr uint32_t kNoLights = 0xFFFFFFFFu;
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------- vectors --
+struct F3 {
+  float x, y, z;
+};
+inline F3 Make(float x, float y, float z) { return F3{x, y, z}; }
+inline F3 operator+(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline F3 operator-(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline F3 operator-(F3 a) { return F3{-a.x, -a.y, -a.z}; }
+inline F3 operator*(F3 a, F3 b) { return F3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline F3 operator*(F3 a, float s) { return F3{a.x * s, a.y * s, a.z * s}; }
+inline F3 operator*(float s, F3 a) { return F3{s * a.x, s * a.y, s * a.z}; }
+inline F3 operator/(F3 a, float s) { return F3{a.x / s, a.y / s, a.z / s}; }
+inline float Dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline F3 Cross(F3 a, F3 b) {
+  return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float Sqrt(float x) { return std::sqrt(x); }
+inline float Rsqrt(float x) { return 1.0f / std::sqrt(x); }
+inline float Length(F3 v) { return Sqrt(Dot(v, v)); }
+inline F3 Normalize(F3 v) { return v * Rsqrt(Dot(v, v)); }
+inline float MaxComponent(F3 v) { return std::max(v.x, std::max(v.y, v.z)); }
+inline bool IsFinite(F3 v) {
+  return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z);
+}
+inline float Saturate(float x) { return std::min(std::max(x, 0.0f), 1.0f); }
+inline float Comp(F3 v, int axis) { return axis == 0 ? v.x : axis == 1 ? v.y : v.z; }
+
+// Column-major 4x4 times (x, y, z, w): row r is ((m[r]x + m[4+r]y) + m[8+r]z) + m[12+r]w.
+inline void MulPoint(const float* m, float x, float y, float z, float w,
+                     float* out) {
+  for (int r = 0; r < 4; ++r)
+    out[r] = ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r] * w;
+}
+inline F3 TransformPoint(const float* m, F3 p) {
+  float h[4];
+  MulPoint(m, p.x, p.y, p.z, 1.0f, h);
+  return F3{h[0], h[1], h[2]};
+}
+inline F3 TransformVector(const float* m, F3 v) {
+  float h[4];
+  MulPoint(m, v.x, v.y, v.z, 0.0f, h);
+  return F3{h[0], h[1], h[2]};
+}
+
+// sin and cos of 2 pi u for u in [0, 1] (every sampled angle of the
+// transport is 2 pi times a random number: sampling.slang:81, preview.slang:154).
+inline void SinCosTurns(float u, float* s, float* c) {
+  const float t = u * 4.0f;          // exact
+  int q = static_cast<int>(t);       // quadrant
+  float f = t - static_cast<float>(q);  // exact, in [0, 1)
+  if (f > 0.5f) {
+    f = f - 1.0f;  // exact (Sterbenz)
+    q = q + 1;
+  }
+  const float a = f * 1.57079632679489662f;
+  const float a2 = a * a;
+  const float sn = a + a * a2 * (-1.66666666666666667e-1f +
+      a2 * (8.33333333333333333e-3f + a2 * (-1.98412698412698413e-4f +
+      a2 * 2.75573192239858907e-6f)));
+  const float cs = 1.0f + a2 * (-0.5f + a2 * (4.16666666666666667e-2f +
+      a2 * (-1.38888888888888889e-3f + a2 * (2.48015873015873016e-5f +
+      a2 * -2.75573192239858907e-7f))));
+  switch (q & 3) {
+    case 0: *s = sn; *c = cs; break;
+    case 1: *s = cs; *c = -sn; break;
+    case 2: *s = -sn; *c = -cs; break;
+    default: *s = -cs; *c = sn; break;
+  }
+}
+
+// ------------------------------------------------------------------- RNG --
+// sampling.slang:9-25.
+inline uint32_t PcgHash(uint32_t value) {
+  const uint32_t state = value * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+inline uint32_t SeedRandom(uint32_t pixel, uint32_t sample) {
+  return PcgHash(pixel ^ PcgHash(sample));
+}
+inline float NextRandom(uint32_t& state) {
+  state = state * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return static_cast<float>(((word >> 22u) ^ word) >> 8u) * (1.0f / 16777216.0f);
+}
+
+// --------------------------------------------------------------- sampling --
+struct Frame {
+  F3 t, b, n;
+  F3 ToLocal(F3 d) const { return F3{Dot(d, t), Dot(d, b), Dot(d, n)}; }
+  F3 ToWorld(F3 d) const { return ((d.x * t) + (d.y * b)) + (d.z * n); }
+};
+// sampling.slang:45-56 (Duff et al. 2017).
+inline Frame MakeShadingFrame(F3 n) {
+  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  Frame f;
+  f.t = F3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+  f.b = F3{b, sign + n.y * n.y * a, -n.y};
+  f.n = n;
+  return f;
+}
+// sampling.slang:67-76.
+inline float PowerHeuristic(float chosen, float other) {
+  if (!(chosen > 0.0f)) return 0.0f;
+  if (std::isinf(chosen)) return std::isinf(other) ? 0.5f : 1.0f;
+  const float ratio = other / chosen;
+  return 1.0f / (1.0f + ratio * ratio);
+}
+// sampling.slang:79-84.
+inline F3 SampleCosineHemisphere(float ux, float uy) {
+  const float radius = Sqrt(ux);
+  float s, c;
+  SinCosTurns(uy, &s, &c);
+  return F3{radius * c, radius * s, Sqrt(std::max(0.0f, 1.0f - ux))};
+}
+
+// ------------------------------------------------------------------- BSDF --
+// bsdf/preview.slang. UsdPreviewSurface's closure.
+constexpr float kMinAlpha = 1.0e-3f;
+struct Closure {
+  F3 base_color;
+  float alpha;
+  float metallic;
+  F3 dielectric_f0;
+  float dielectric_f90;
+  F3 f0;
+  float f90;
+  float transmission;
+  F3 transmission_color;
+  float eta;
+  bool thin_walled;
+  bool smooth;
+};
+
+// preview.slang:60-89.
+Closure MakePreviewClosure(F3 base_color, float roughness, float metallic,
+                           float ior, bool specular_workflow, F3 specular,
+                           float transmission, F3 transmission_color,
+                           bool thin_walled, bool entering) {
+  Closure b;
+  b.base_color = base_color;
+  b.alpha = std::max(roughness * roughness, kMinAlpha);
+  b.metallic = specular_workflow ? 0.0f : metallic;
+  b.transmission = (1.0f - b.metallic) * transmission;
+  b.transmission_color = transmission_color;
+  b.eta = thin_walled || entering ? ior : 1.0f / ior;
+  b.thin_walled = thin_walled;
+  b.smooth = roughness * roughness < kMinAlpha || ior == 1.0f;
+  if (b.transmission > 0.0f) {
+    b.metallic = b.transmission < 1.0f
+                     ? std::min(b.metallic / (1.0f - b.transmission), 1.0f)
+                     : 0.0f;
+  }
+  const float ratio = 1.0f - 2.0f / (ior + 1.0f);
+  const float r2 = ratio * ratio;
+  b.dielectric_f0 = specular_workflow ? specular : F3{r2, r2, r2};
+  b.dielectric_f90 = specular_workflow || ior > 1.0f ? 1.0f : 0.0f;
+  b.f0 = base_color * b.metallic + b.dielectric_f0 * (1.0f - b.metallic);
+  b.f90 = b.metallic + (1.0f - b.metallic) * b.dielectric_f90;
+  return b;
+}
+
+inline float Pow5(float x) {
+  const float x2 = x * x;
+  return (x2 * x2) * x;
+}
+// preview.slang:92-95.
+inline F3 FresnelSchlick(F3 f0, float f90, float cos_theta) {
+  const float w = Pow5(Saturate(1.0f - cos_theta));
+  const float g = f90 * w;
+  return f0 * (1.0f - w) + F3{g, g, g};
+}
+// preview.slang:100-104.
+inline F3 DielectricTransmission(const Closure& b, float cos_theta) {
+  const F3 fr = FresnelSchlick(b.dielectric_f0, b.dielectric_f90, cos_theta);
+  return F3{1.0f, 1.0f, 1.0f} + fr * -1.0f;
+}
+// preview.slang:106-112.
+inline float SpecularProbability(const Closure& b, F3 wo) {
+  if (b.metallic == 1.0f) return 1.0f;
+  if (b.dielectric_f90 == 0.0f) return b.metallic;
+  const float d = std::min(std::max(MaxComponent(FresnelSchlick(
+      b.dielectric_f0, b.dielectric_f90, wo.z)), 0.05f), 0.95f);
+  return b.metallic + (1.0f - b.metallic) * d;
+}
+// preview.slang:118-127.
+inline float FresnelDielectric(float cos_theta, float eta) {
+  cos_theta = Saturate(cos_theta);
+  const float sin2_t = (1.0f - cos_theta * cos_theta) / (eta * eta);
+  if (sin2_t >= 1.0f) return 1.0f;
+  const float cos_t = Sqrt(1.0f - sin2_t);
+  const float par = (eta * cos_theta - cos_t) / (eta * cos_theta + cos_t);
+  const float per = (cos_theta - eta * cos_t) / (cos_theta + eta * cos_t);
+  return 0.5f * (par * par + per * per);
+}
+// preview.slang:132-135.
+inline float InterfaceReflectance(const Closure& b, float cos_theta) {
+  const float r = FresnelDielectric(cos_theta, b.eta);
+  return b.thin_walled ? 2.0f * r / (1.0f + r) : r;
+}
+// preview.slang:138-142.
+inline float GgxD(F3 h, float alpha) {
+  const float a2 = alpha * alpha;
+  const float d = h.z * h.z * (a2 - 1.0f) + 1.0f;
+  return a2 / (kPi * d * d);
+}
+// preview.slang:145-148.
+inline float GgxLambda(F3 w, float alpha) {
+  const float tan2 = std::max(0.0f, 1.0f - w.z * w.z) / (w.z * w.z);
+  return 0.5f * (Sqrt(1.0f + alpha * alpha * tan2) - 1.0f);
+}
+// preview.slang:152-160 (Dupuy and Benyoub 2023).
+inline F3 SampleGgxVisibleNormal(F3 wo, float alpha, float ux, float uy) {
+  const F3 wo_std = Normalize(F3{wo.x * alpha, wo.y * alpha, wo.z});
+  float s, c;
+  SinCosTurns(ux, &s, &c);
+  const float z = (1.0f - uy) * (1.0f + wo_std.z) - wo_std.z;
+  const float sin_theta = Sqrt(Saturate(1.0f - z * z));
+  const F3 h_std = F3{sin_theta * c, sin_theta * s, z} + wo_std;
+  return Normalize(F3{h_std.x * alpha, h_std.y * alpha, h_std.z});
+}
+// preview.slang:164-170.
+inline F3 GgxEvalCos(const Closure& b, F3 wo, F3 wi) {
+  const F3 h = Normalize(wo + wi);
+  const float g2 = 1.0f / (1.0f + GgxLambda(wo, b.alpha) + GgxLambda(wi, b.alpha));
+  return FresnelSchlick(b.f0, b.f90, Dot(wo, h)) *
+         (GgxD(h, b.alpha) * g2 / (4.0f * wo.z));
+}
+// preview.slang:173-177.
+inline float GgxPdf(const Closure& b, F3 wo, F3 wi) {
+  const F3 h = Normalize(wo + wi);
+  const float g1 = 1.0f / (1.0f + GgxLambda(wo, b.alpha));
+  return GgxD(h, b.alpha) * g1 / (4.0f * wo.z);
+}
+// preview.slang:180-183.
+inline float SmithRatio(const Closure& b, F3 wo, F3 wi) {
+  const float lo = GgxLambda(wo, b.alpha);
+  return (1.0f + lo) / (1.0f + lo + GgxLambda(wi, b.alpha));
+}
+// preview.slang:187-198.
+inline F3 LayeredEvalCos(const Closure& b, F3 wo, F3 wi) {
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return F3{0, 0, 0};
+  F3 v = b.base_color * ((1.0f - b.metallic) * wi.z / kPi);
+  v = v * DielectricTransmission(b, wo.z) * DielectricTransmission(b, wi.z);
+  if (b.f90 > 0.0f) v = v + GgxEvalCos(b, wo, wi);
+  return v;
+}
+// preview.slang:202-212.
+inline float LayeredPdf(const Closure& b, F3 wo, F3 wi) {
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+  const float spec = SpecularProbability(b, wo);
+  float pdf = (1.0f - spec) * wi.z / kPi;
+  if (spec > 0.0f) pdf += spec * GgxPdf(b, wo, wi);
+  return pdf;
+}
+// preview.slang:216-222.
+inline float InterfaceReflectionEvalCos(const Closure& b, F3 wo, F3 wi) {
+  const F3 h = Normalize(wo + wi);
+  const float g2 = 1.0f / (1.0f + GgxLambda(wo, b.alpha) + GgxLambda(wi, b.alpha));
+  return InterfaceReflectance(b, Dot(wo, h)) * GgxD(h, b.alpha) * g2 / (4.0f * wo.z);
+}
+// preview.slang:227-238.
+inline F3 ReflectionEvaluate(const Closure& b, F3 wo, F3 wi) {
+  const F3 layered = LayeredEvalCos(b, wo, wi);
+  if (b.transmission == 0.0f) return layered;
+  F3 v = layered * (1.0f - b.transmission);
+  if (!b.smooth && wo.z > 0.0f && wi.z > 0.0f) {
+    const float r = b.transmission * InterfaceReflectionEvalCos(b, wo, wi);
+    v = v + F3{r, r, r};
+  }
+  return v;
+}
+// preview.slang:240-252.
+inline float ReflectionPdf(const Closure& b, F3 wo, F3 wi) {
+  const float layered = LayeredPdf(b, wo, wi);
+  if (b.transmission == 0.0f) return layered;
+  float pdf = (1.0f - b.transmission) * layered;
+  if (!b.smooth && wo.z > 0.0f && wi.z > 0.0f) {
+    const F3 h = Normalize(wo + wi);
+    pdf += b.transmission * InterfaceReflectance(b, Dot(wo, h)) * GgxPdf(b, wo, wi);
+  }
+  return pdf;
+}
+// preview.slang:262-310.
+struct Transmit {
+  float value, pdf;
+};
+inline Transmit PreviewTransmission(const Closure& b, F3 wo, F3 wi) {
+  Transmit r{0.0f, 0.0f};
+  if (b.transmission == 0.0f || b.smooth || !(wo.z > 0.0f) || !(wi.z < 0.0f))
+    return r;
+  if (b.thin_walled) {
+    const F3 refl = F3{wi.x, wi.y, -wi.z};
+    const F3 h = Normalize(wo + refl);
+    const float share = b.transmission * (1.0f - InterfaceReflectance(b, Dot(wo, h)));
+    const float g2 = 1.0f / (1.0f + GgxLambda(wo, b.alpha) + GgxLambda(refl, b.alpha));
+    r.value = share * GgxD(h, b.alpha) * g2 / (4.0f * wo.z);
+    r.pdf = share * GgxPdf(b, wo, refl);
+    return r;
+  }
+  F3 h = Normalize(wo + b.eta * wi);
+  if (h.z < 0.0f) h = -h;
+  const float cos_oh = Dot(wo, h);
+  const float cos_ih = Dot(wi, h);
+  if (!(cos_oh > 0.0f) || !(cos_ih < 0.0f)) return r;
+  const float den = cos_ih + cos_oh / b.eta;
+  const float jac = -cos_ih / (den * den);
+  const float share = b.transmission * (1.0f - InterfaceReflectance(b, cos_oh));
+  const float d = GgxD(h, b.alpha);
+  const float lo = GgxLambda(wo, b.alpha);
+  const float g1 = 1.0f / (1.0f + lo);
+  const float g2 = 1.0f / (1.0f + lo + GgxLambda(wi, b.alpha));
+  r.pdf = share * (g1 * cos_oh * d / wo.z) * jac;
+  r.value = share * (g2 * cos_oh * d / wo.z) * jac / (b.eta * b.eta);
+  return r;
+}
+// preview.slang:314-327.
+inline F3 BsdfEvaluate(const Closure& b, F3 wo, F3 wi) {
+  if (wi.z < 0.0f) return b.transmission_color * PreviewTransmission(b, wo, wi).value;
+  return ReflectionEvaluate(b, wo, wi);
+}
+inline float BsdfPdf(const Closure& b, F3 wo, F3 wi) {
+  if (wi.z < 0.0f) return PreviewTransmission(b, wo, wi).pdf;
+  return ReflectionPdf(b, wo, wi);
+}
+struct SampleResult {
+  F3 wi;
+  F3 weight;
+  float pdf;
+  bool delta;
+  bool transmitted;
+};
+// preview.slang:348-405.
+bool SampleInterface(const Closure& b, F3 wo, float ux, float uy, float uz,
+                     SampleResult& s) {
+  const F3 h = b.smooth ? F3{0.0f, 0.0f, 1.0f}
+                        : SampleGgxVisibleNormal(wo, b.alpha, ux, uy);
+  const float cos_oh = Dot(wo, h);
+  if (!(cos_oh > 0.0f)) return false;
+  const F3 reflected = (2.0f * cos_oh) * h - wo;
+  s.delta = b.smooth;
+  if (uz < InterfaceReflectance(b, cos_oh)) {
+    s.wi = reflected;
+    if (b.smooth) {
+      s.weight = F3{1.0f, 1.0f, 1.0f};
+      return true;
+    }
+    if (reflected.z <= 0.0f) return false;
+    const float r = SmithRatio(b, wo, reflected);
+    s.weight = F3{r, r, r};
+    s.pdf = ReflectionPdf(b, wo, reflected);
+    return s.pdf > 0.0f && IsFinite(s.weight);
+  }
+  s.transmitted = true;
+  if (b.thin_walled) {
+    if (!b.smooth && reflected.z <= 0.0f) return false;
+    s.wi = F3{reflected.x, reflected.y, -reflected.z};
+    s.weight = b.transmission_color * (b.smooth ? 1.0f : SmithRatio(b, wo, reflected));
+    if (!b.smooth) s.pdf = PreviewTransmission(b, wo, s.wi).pdf;
+    return IsFinite(s.weight);
+  }
+  const float sin2_t = (1.0f - cos_oh * cos_oh) / (b.eta * b.eta);
+  if (sin2_t >= 1.0f) return false;
+  const float cos_t = Sqrt(1.0f - sin2_t);
+  s.wi = -wo / b.eta + (cos_oh / b.eta - cos_t) * h;
+  if (s.wi.z >= 0.0f) return false;
+  s.weight = b.transmission_color *
+             ((b.smooth ? 1.0f : SmithRatio(b, wo, s.wi)) / (b.eta * b.eta));
+  if (!b.smooth) s.pdf = PreviewTransmission(b, wo, s.wi).pdf;
+  return IsFinite(s.weight);
+}
+// preview.slang:411-454.
+bool BsdfSample(const Closure& b, F3 wo, float ux, float uy, float uz,
+                SampleResult& s) {
+  s.wi = F3{0, 0, 0};
+  s.weight = F3{0, 0, 0};
+  s.pdf = 0.0f;
+  s.delta = false;
+  s.transmitted = false;
+  if (wo.z <= 0.0f) return false;
+  if (b.transmission > 0.0f) {
+    if (uz < b.transmission) return SampleInterface(b, wo, ux, uy, uz / b.transmission, s);
+    uz = (uz - b.transmission) / (1.0f - b.transmission);
+  }
+  if (uz < SpecularProbability(b, wo)) {
+    const F3 h = SampleGgxVisibleNormal(wo, b.alpha, ux, uy);
+    s.wi = (2.0f * Dot(wo, h)) * h - wo;
+  } else {
+    s.wi = SampleCosineHemisphere(ux, uy);
+  }
+  const float layered_pdf = LayeredPdf(b, wo, s.wi);
+  if (!(layered_pdf > 0.0f)) return false;
+  s.pdf = b.transmission > 0.0f ? ReflectionPdf(b, wo, s.wi) : layered_pdf;
+  s.weight = LayeredEvalCos(b, wo, s.wi) / layered_pdf;
+  if (b.metallic == 1.0f) {
+    const F3 h = Normalize(wo + s.wi);
+    const float lo = GgxLambda(wo, b.alpha);
+    s.weight = FresnelSchlick(b.f0, b.f90, Dot(wo, h)) *
+               ((1.0f + lo) / (1.0f + lo + GgxLambda(s.wi, b.alpha)));
+  }
+  return IsFinite(s.weight);
+}
+
+// --------------------------------------------------------------- material --
+// GpuMaterialRecord (gpu_scene.hpp:94-113): words 0 model_flags,
+// 1 lookup_base, 2-3 textured, 4-7 geometry_coverage, 8-11 geometry_normal,
+// 12-35 parameters[6][4].
+struct Material {
+  uint32_t flags;
+  F3 base_color;
+  float roughness;
+  F3 emission;
+  float metallic;
+  F3 specular;
+  float ior;
+  F3 transmission_color;
+  float transmission;
+  F3 tangent_normal;
+};
+inline float WordFloat(const uint32_t* w, int i) {
+  float f;
+  std::memcpy(&f, &w[i], 4);
+  return f;
+}
+Material DecodeMaterial(const uint32_t* w) {
+  Material m;
+  m.flags = w[0] >> 8;
+  m.tangent_normal = F3{WordFloat(w, 8), WordFloat(w, 9), WordFloat(w, 10)};
+  m.base_color = F3{WordFloat(w, 12), WordFloat(w, 13), WordFloat(w, 14)};
+  m.roughness = WordFloat(w, 15);
+  m.emission = F3{WordFloat(w, 16), WordFloat(w, 17), WordFloat(w, 18)};
+  m.metallic = WordFloat(w, 19);
+  m.specular = F3{WordFloat(w, 20), WordFloat(w, 21), WordFloat(w, 22)};
+  m.ior = WordFloat(w, 23);
+  m.transmission_color = F3{WordFloat(w, 24), WordFloat(w, 25), WordFloat(w, 26)};
+  m.transmission = WordFloat(w, 27);
+  return m;
+}
+constexpr uint32_t kSpecularWorkflow = 1u;
+constexpr uint32_t kThinWalled = 2u;
+
+// ------------------------------------------------------------------ scene --
+struct Ray {
+  F3 o;
+  float tmin;
+  F3 d;
+  float tmax;
+};
+struct Hit {
+  uint32_t instance, primitive, global;
+  float u, v, t;
+};
+
+struct Node {  // oracle BVH2 node
+  float lo[3], hi[3];
+  uint32_t left_or_first;  // interior: left child; leaf: first triangle
+  uint32_t count;          // 0: interior (right = left + 1? no: right stored)
+  uint32_t right;
+};
+
+struct LightRecord {
+  float pdf, threshold;
+  uint32_t alias, instance;
+};
+
+struct Scene {
+  std::vector<F3> v0, v1, v2;           // world corners per flattened triangle
+  std::vector<uint32_t> tri_instance;   // per flattened triangle
+  std::vector<uint32_t> tri_primitive;
+  std::vector<uint32_t> instance_base;  // first flattened triangle
+  std::vector<uint32_t> instance_material;
+  std::vector<float> instance_transform;  // 16 per instance
+  std::vector<Material> materials;
+  std::vector<uint32_t> light_offsets;
+  std::vector<LightRecord> lights;
+  F3 environment;
+  std::vector<Node> nodes;
+  std::vector<uint32_t> order;  // BVH leaf order -> flattened triangle
+};
+
+// ------------------------------------------------- watertight intersection --
+// Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection", JCGT 2013.
+struct RayPre {
+  int kx, ky, kz;
+  float sx, sy, sz;
+};
+inline RayPre Precompute(const Ray& r) {
+  RayPre p;
+  const float ax = std::fabs(r.d.x), ay = std::fabs(r.d.y), az = std::fabs(r.d.z);
+  p.kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  p.kx = p.kz + 1 == 3 ? 0 : p.kz + 1;
+  p.ky = p.kx + 1 == 3 ? 0 : p.kx + 1;
+  if (Comp(r.d, p.kz) < 0.0f) std::swap(p.kx, p.ky);
+  const float dz = Comp(r.d, p.kz);
+  p.sx = Comp(r.d, p.kx) / dz;
+  p.sy = Comp(r.d, p.ky) / dz;
+  p.sz = 1.0f / dz;
+  return p;
+}
+// Returns true with t, u (weight of v1), v (weight of v2) for a hit with
+// tmin <= t <= tmax.
+inline bool IntersectTriangle(const Ray& r, const RayPre& p, F3 a0, F3 a1,
+                              F3 a2, float& t, float& bu, float& bv) {
+  const F3 A = a0 - r.o, B = a1 - r.o, C = a2 - r.o;
+  const float Az0 = Comp(A, p.kz), Bz0 = Comp(B, p.kz), Cz0 = Comp(C, p.kz);
+  const float Ax = Comp(A, p.kx) - p.sx * Az0;
+  const float Ay = Comp(A, p.ky) - p.sy * Az0;
+  const float Bx = Comp(B, p.kx) - p.sx * Bz0;
+  const float By = Comp(B, p.ky) - p.sy * Bz0;
+  const float Cx = Comp(C, p.kx) - p.sx * Cz0;
+  const float Cy = Comp(C, p.ky) - p.sy * Cz0;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = static_cast<float>(static_cast<double>(Cx) * By - static_cast<double>(Cy) * Bx);
+    V = static_cast<float>(static_cast<double>(Ax) * Cy - static_cast<double>(Ay) * Cx);
+    W = static_cast<float>(static_cast<double>(Bx) * Ay - static_cast<double>(By) * Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+    return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float Az = p.sz * Az0, Bz = p.sz * Bz0, Cz = p.sz * Cz0;
+  const float T = (U * Az + V * Bz) + W * Cz;
+  const float inv = 1.0f / det;
+  t = T * inv;
+  if (!(t >= r.tmin) || !(t <= r.tmax)) return false;
+  bu = V * inv;
+  bv = W * inv;
+  return true;
+}
+
+// Robust slab test (Ize, "Robust BVH Ray Traversal", JCGT 2013): the far
+// distance grows by 1 + 2 gamma(3).
+inline bool IntersectBox(const Ray& r, const float* inv, const float* lo,
+                         const float* hi, float tmax, float& tnear) {
+  const float o[3] = {r.o.x, r.o.y, r.o.z};
+  float t0 = r.tmin, t1 = tmax;
+  for (int a = 0; a < 3; ++a) {
+    float tn = (lo[a] - o[a]) * inv[a];
+    float tf = (hi[a] - o[a]) * inv[a];
+    if (tn > tf) std::swap(tn, tf);
+    tf *= 1.0000003576278687f;  // 1 + 2 gamma(3)
+    t0 = tn > t0 ? tn : t0;
+    t1 = tf < t1 ? tf : t1;
+    if (t0 > t1) return false;
+  }
+  tnear = t0;
+  return true;
+}
+
+inline bool Closer(float t, uint32_t g, const Hit& best) {
+  return t < best.t || (t == best.t && g < best.global);
+}
+
+// Boxes grow by a relative margin so no rounding of their corners can cull a
+// triangle the watertight test accepts.
+inline void Inflate(float* lo, float* hi) {
+  for (int a = 0; a < 3; ++a) {
+    const float m = (std::fabs(lo[a]) + std::fabs(hi[a]) + (hi[a] - lo[a])) * 1.0e-6f + 1.0e-30f;
+    lo[a] -= m;
+    hi[a] += m;
+  }
+}
+
+void BuildBvh(Scene& s) {
+  const uint32_t n = static_cast<uint32_t>(s.v0.size());
+  s.order.resize(n);
+  for (uint32_t i = 0; i < n; ++i) s.order[i] = i;
+  s.nodes.clear();
+  if (n == 0) return;
+  std::vector<float> cen(3 * size_t(n));
+  for (uint32_t i = 0; i < n; ++i) {
+    const F3 a = s.v0[i], b = s.v1[i], c = s.v2[i];
+    cen[3 * i + 0] = (a.x + b.x + c.x) / 3.0f;
+    cen[3 * i + 1] = (a.y + b.y + c.y) / 3.0f;
+    cen[3 * i + 2] = (a.z + b.z + c.z) / 3.0f;
+  }
+  struct Task {
+    uint32_t node, first, count;
+  };
+  s.nodes.reserve(2 * size_t(n));
+  s.nodes.push_back(Node{});
+  std::vector<Task> stack{{0, 0, n}};
+  while (!stack.empty()) {
+    const Task task = stack.back();
+    stack.pop_back();
+    Node& node = s.nodes[task.node];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = task.first; k < task.first + task.count; ++k) {
+      const uint32_t i = s.order[k];
+      for (const F3 p : {s.v0[i], s.v1[i], s.v2[i]}) {
+        const float q[3] = {p.x, p.y, p.z};
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(lo[a], q[a]);
+          hi[a] = std::max(hi[a], q[a]);
+        }
+      }
+      for (int a = 0; a < 3; ++a) {
+        clo[a] = std::min(clo[a], cen[3 * i + a]);
+        chi[a] = std::max(chi[a], cen[3 * i + a]);
+      }
+    }
+    Inflate(lo, hi);
+    std::memcpy(node.lo, lo, sizeof lo);
+    std::memcpy(node.hi, hi, sizeof hi);
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+      if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+    if (task.count <= 4 || !(chi[axis] > clo[axis])) {
+      node.left_or_first = task.first;
+      node.count = task.count;
+      node.right = 0;
+      continue;
+    }
+    const uint32_t half = task.count / 2;
+    auto begin = s.order.begin() + task.first;
+    std::nth_element(begin, begin + half, begin + task.count,
+                     [&](uint32_t x, uint32_t y) {
+                       const float cx = cen[3 * x + axis], cy = cen[3 * y + axis];
+                       return cx < cy || (cx == cy && x < y);
+                     });
+    const uint32_t left = static_cast<uint32_t>(s.nodes.size());
+    s.nodes.push_back(Node{});
+    s.nodes.push_back(Node{});
+    Node& parent = s.nodes[task.node];
+    parent.left_or_first = left;
+    parent.right = left + 1;
+    parent.count = 0;
+    stack.push_back({left, task.first, half});
+    stack.push_back({left + 1, task.first + half, task.count - half});
+  }
+}
+
+// A zero direction component becomes +-1e-30 so no slab distance is 0 * inf.
+inline void InverseDirection(const Ray& r, float* inv) {
+  const float d[3] = {r.d.x, r.d.y, r.d.z};
+  for (int a = 0; a < 3; ++a)
+    inv[a] = 1.0f / (std::fabs(d[a]) > 1.0e-30f ? d[a] : std::copysign(1.0e-30f, d[a]));
+}
+
+bool TraceClosest(const Scene& s, const Ray& r, Hit& best, bool brute) {
+  best.t = INFINITY;
+  best.global = kMiss;
+  best.instance = kMiss;
+  best.primitive = kMiss;
+  best.u = best.v = 0.0f;
+  const RayPre p = Precompute(r);
+  auto test = [&](uint32_t g) {
+    float t, u, v;
+    if (IntersectTriangle(r, p, s.v0[g], s.v1[g], s.v2[g], t, u, v) &&
+        Closer(t, g, best)) {
+      best.t = t;
+      best.global = g;
+      best.u = u;
+      best.v = v;
+    }
+  };
+  if (brute) {
+    for (uint32_t g = 0; g < s.v0.size(); ++g) test(g);
+  } else if (!s.nodes.empty()) {
+    float inv[3];
+    InverseDirection(r, inv);
+    uint32_t stack[128];
+    int top = 0;
+    stack[top++] = 0;
+    while (top > 0) {
+      const Node& node = s.nodes[stack[--top]];
+      float tn;
+      const float limit = best.global == kMiss ? r.tmax : best.t;
+      if (!IntersectBox(r, inv, node.lo, node.hi, limit, tn)) continue;
+      if (node.count > 0) {
+        for (uint32_t k = 0; k < node.count; ++k) test(s.order[node.left_or_first + k]);
+      } else {
+        stack[top++] = node.right;
+        stack[top++] = node.left_or_first;
+      }
+    }
+  }
+  if (best.global == kMiss) return false;
+  best.instance = s.tri_instance[best.global];
+  best.primitive = s.tri_primitive[best.global];
+  return true;
+}
+
+bool Occluded(const Scene& s, const Ray& r, bool brute) {
+  const RayPre p = Precompute(r);
+  float t, u, v;
+  if (brute) {
+    for (uint32_t g = 0; g < s.v0.size(); ++g)
+      if (IntersectTriangle(r, p, s.v0[g], s.v1[g], s.v2[g], t, u, v)) return true;
+    return false;
+  }
+  if (s.nodes.empty()) return false;
+  float inv[3];
+  InverseDirection(r, inv);
+  uint32_t stack[128];
+  int top = 0;
+  stack[top++] = 0;
+  while (top > 0) {
+    const Node& node = s.nodes[stack[--top]];
+    float tn;
+    if (!IntersectBox(r, inv, node.lo, node.hi, r.tmax, tn)) continue;
+    if (node.count > 0) {
+      for (uint32_t k = 0; k < node.count; ++k) {
+        const uint32_t g = s.order[node.left_or_first + k];
+        if (IntersectTriangle(r, p, s.v0[g], s.v1[g], s.v2[g], t, u, v)) return true;
+      }
+    } else {
+      stack[top++] = node.right;
+      stack[top++] = node.left_or_first;
+    }
+  }
+  return false;
+}
+
+// ------------------------------------------------------------ light table --
+// gpu_scene.cpp:144-203, 512-609 (emissive triangles only).
+double EmissionEstimate(const Material& m) {
+  const double mean = (double(m.emission.x) + double(m.emission.y) + double(m.emission.z)) / 3.0;
+  return std::isfinite(mean) && mean > 0.0 ? mean : 0.0;
+}
+double WorldTriangleArea(const F3* object, const float* t) {
+  double c[3][3];
+  for (int k = 0; k < 3; ++k) {
+    const float p[3] = {object[k].x, object[k].y, object[k].z};
+    for (int row = 0; row < 3; ++row)
+      c[k][row] = double(t[row]) * p[0] + double(t[4 + row]) * p[1] +
+                  double(t[8 + row]) * p[2] + t[12 + row];
+  }
+  double a[3], b[3];
+  for (int row = 0; row < 3; ++row) {
+    a[row] = c[1][row] - c[0][row];
+    b[row] = c[2][row] - c[0][row];
+  }
+  const double x = a[1] * b[2] - a[2] * b[1];
+  const double y = a[2] * b[0] - a[0] * b[2];
+  const double z = a[0] * b[1] - a[1] * b[0];
+  const double area = 0.5 * std::sqrt(x * x + y * y + z * z);
+  return std::isfinite(area) ? area : 0.0;
+}
+
+// ---------------------------------------------------------------- camera --
+// vulkan_internal.hpp:326-336.
+void VulkanWorldToClip(const float* gl, float* vk) {
+  std::memcpy(vk, gl, 16 * sizeof(float));
+  for (int c = 0; c < 4; ++c) {
+    const float y = gl[c * 4 + 1], z = gl[c * 4 + 2], w = gl[c * 4 + 3];
+    vk[c * 4 + 1] = -y;
+    vk[c * 4 + 2] = 0.5f * (z + w);
+  }
+}
+// vulkan_backend.cpp:436-478 (Gauss-Jordan with partial pivoting in double).
+bool Invert(const float* m, float* inverse) {
+  double rows[4][8] = {};
+  for (int r = 0; r < 4; ++r) {
+    for (int c = 0; c < 4; ++c) {
+      const float v = m[c * 4 + r];
+      if (!std::isfinite(v)) return false;
+      rows[r][c] = v;
+    }
+    rows[r][r + 4] = 1.0;
+  }
+  for (int c = 0; c < 4; ++c) {
+    int pivot = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (std::fabs(rows[r][c]) > std::fabs(rows[pivot][c])) pivot = r;
+    if (rows[pivot][c] == 0.0) return false;
+    for (int i = 0; i < 8; ++i) std::swap(rows[c][i], rows[pivot][i]);
+    const double scale = rows[c][c];
+    for (double& v : rows[c]) v /= scale;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const double f = rows[r][c];
+      for (int i = 0; i < 8; ++i) rows[r][i] -= f * rows[c][i];
+    }
+  }
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      const float v = static_cast<float>(rows[r][c + 4]);
+      if (!std::isfinite(v)) return false;
+      inverse[c * 4 + r] = v;
+    }
+  return true;
+}
+// scene.slang:273-283.
+Ray CameraRay(const float* clip_to_world, float nx, float ny) {
+  float nh[4], fh[4];
+  MulPoint(clip_to_world, nx, ny, 0.0f, 1.0f, nh);
+  MulPoint(clip_to_world, nx, ny, 1.0f, 1.0f, fh);
+  Ray r;
+  r.o = F3{nh[0] / nh[3], nh[1] / nh[3], nh[2] / nh[3]};
+  r.d = Normalize(F3{fh[0], fh[1], fh[2]} * nh[3] - F3{nh[0], nh[1], nh[2]} * fh[3]);
+  r.tmin = 0.0f;
+  r.tmax = fh[3] == 0.0f ? kFloatMax
+                         : Length(F3{fh[0] / fh[3], fh[1] / fh[3], fh[2] / fh[3]} - r.o);
+  return r;
+}
+
+// --------------------------------------------------------------- shading --
+// scene.slang:1454-1469 (Waechter and Binder 2019).
+F3 OffsetRayOrigin(F3 p, F3 n) {
+  const float pos[3] = {p.x, p.y, p.z};
+  const float nor[3] = {n.x, n.y, n.z};
+  float out[3];
+  for (int a = 0; a < 3; ++a) {
+    const int offset = static_cast<int>(256.0f * nor[a]);
+    int32_t bits;
+    std::memcpy(&bits, &pos[a], 4);
+    bits = static_cast<int32_t>(static_cast<uint32_t>(bits) +
+                                static_cast<uint32_t>(pos[a] < 0.0f ? -offset : offset));
+    float moved;
+    std::memcpy(&moved, &bits, 4);
+    out[a] = std::fabs(pos[a]) < (1.0f / 32.0f) ? pos[a] + (1.0f / 65536.0f) * nor[a] : moved;
+  }
+  return F3{out[0], out[1], out[2]};
+}
+
+struct Surface {
+  F3 position, normal, shading_normal;
+  float area;
+  Closure bsdf;
+  F3 emission;
+};
+
+// scene.slang:1408-1449 with EvaluatePreview (:1184-1260) for a constant
+// UsdPreviewSurface and MappedNormal (:539-553) for a constant tangent normal.
+Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
+  const uint32_t g = hit.global;
+  const F3 p0 = s.v0[g], p1 = s.v1[g], p2 = s.v2[g];
+  Surface sf;
+  sf.position = (p0 + hit.u * (p1 - p0)) + hit.v * (p2 - p0);
+  const F3 e = Cross(p1 - p0, p2 - p0);
+  const F3 normal = Normalize(e);
+  sf.area = 0.5f * Length(e);
+  sf.normal = Dot(normal, dir) > 0.0f ? -normal : normal;
+  sf.shading_normal = sf.normal;
+  const Material& m = s.materials[s.instance_material[hit.instance]];
+  const F3 tn = m.tangent_normal;
+  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f)) {
+    const Frame basis = MakeShadingFrame(sf.shading_normal);
+    const float mag = std::max(std::max(std::fabs(tn.x), std::fabs(tn.y)), std::fabs(tn.z));
+    if (mag > 0.0f && IsFinite(tn)) {
+      F3 sh = Normalize(basis.ToWorld(tn / mag));
+      if (Dot(sh, sf.normal) < 0.0f) sh = -sh;
+      sf.shading_normal = Dot(sh, dir) < 0.0f ? sh : basis.n;
+    }
+  }
+  bool entering = true;
+  if (m.transmission > 0.0f) {
+    const float* t = &s.instance_transform[16 * size_t(hit.instance)];
+    const F3 c0 = TransformVector(t, F3{1, 0, 0});
+    const F3 c1 = TransformVector(t, F3{0, 1, 0});
+    const F3 c2 = TransformVector(t, F3{0, 0, 1});
+    const F3 outward = Dot(c0, Cross(c1, c2)) < 0.0f ? -normal : normal;
+    entering = Dot(outward, dir) < 0.0f;
+  }
+  sf.bsdf = MakePreviewClosure(m.base_color, m.roughness, m.metallic, m.ior,
+                               (m.flags & kSpecularWorkflow) != 0, m.specular,
+                               m.transmission, m.transmission_color,
+                               (m.flags & kThinWalled) != 0, entering);
+  sf.emission = m.emission;
+  return sf;
+}
+
+struct LightSample {
+  F3 direction;
+  float distance;
+  F3 radiance;
+  float pdf;
+};
+// scene.slang:894-909 with SampleTriangle (:632-677).
+LightSample SampleLight(const Scene& s, float ux, float uy, float uz, float uw,
+                        F3 origin) {
+  LightSample none{F3{0, 0, 1}, 0.0f, F3{0, 0, 0}, 0.0f};
+  const uint32_t count = static_cast<uint32_t>(s.lights.size());
+  const uint32_t column = std::min(static_cast<uint32_t>(ux * static_cast<float>(count)), count - 1);
+  const LightRecord& record = s.lights[column];
+  const uint32_t light = uy < record.threshold ? column : record.alias;
+  const LightRecord& chosen = s.lights[light];
+  const uint32_t prim = light - s.light_offsets[chosen.instance];
+  const float root = Sqrt(uz);
+  const float bu = root * (1.0f - uw), bv = root * uw;
+  const uint32_t g = s.instance_base[chosen.instance] + prim;
+  const F3 p0 = s.v0[g], p1 = s.v1[g], p2 = s.v2[g];
+  const Material& m = s.materials[s.instance_material[chosen.instance]];
+  const F3 position = (p0 + bu * (p1 - p0)) + bv * (p2 - p0);
+  const F3 e = Cross(p1 - p0, p2 - p0);
+  const F3 normal = Normalize(e);
+  const float area = 0.5f * Length(e);
+  const F3 radiance = m.emission * 1.0f;
+  const F3 toward = position - origin;
+  const float d2 = Dot(toward, toward);
+  if (!(area > 0.0f) || !(d2 > 0.0f) || MaxComponent(radiance) <= 0.0f) return none;
+  LightSample ls;
+  ls.direction = toward * Rsqrt(d2);
+  const float cosine = std::fabs(Dot(normal, ls.direction));
+  if (!(cosine > 0.0f)) return none;
+  const F3 end = OffsetRayOrigin(position, Dot(normal, ls.direction) > 0.0f ? -normal : normal);
+  ls.distance = std::max(0.0f, Dot(end - origin, ls.direction));
+  ls.radiance = radiance;
+  ls.pdf = chosen.pdf / area * d2 / cosine;
+  return ls;
+}
+// scene.slang:1004-1010.
+float EmitterPdf(const Scene& s, const Hit& hit, float area) {
+  if (s.light_offsets.empty()) return 0.0f;
+  const uint32_t first = s.light_offsets[hit.instance];
+  if (first == kNoLights || !(area > 0.0f)) return 0.0f;
+  return s.lights[first + hit.primitive].pdf / area;
+}
+
+struct Path {
+  Ray ray;
+  F3 throughput, radiance;
+  uint32_t depth, rng;
+  float last_pdf;
+  bool unweighted;
+};
+
+struct Counters {
+  uint32_t* counts;  // max_bounces + 2
+  uint64_t shadow;
+};
+
+// transport.slang:283-316, 324-336.
+F3 DirectLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo,
+               uint32_t& rng, Counters& k, bool brute) {
+  if (s.lights.empty()) return F3{0, 0, 0};
+  const float ux = NextRandom(rng), uy = NextRandom(rng), uz = NextRandom(rng),
+              uw = NextRandom(rng);
+  const F3 origin = OffsetRayOrigin(sf.position, sf.normal);
+  const LightSample ls = SampleLight(s, ux, uy, uz, uw, origin);
+  if (!(ls.pdf > 0.0f) || MaxComponent(ls.radiance) <= 0.0f ||
+      Dot(ls.direction, sf.normal) <= 0.0f)
+    return F3{0, 0, 0};
+  const F3 wi = basis.ToLocal(ls.direction);
+  if (wi.z <= 0.0f) return F3{0, 0, 0};
+  const F3 f = BsdfEvaluate(sf.bsdf, wo, wi);
+  if (MaxComponent(f) <= 0.0f) return F3{0, 0, 0};
+  Ray shadow{origin, 0.0f, ls.direction, ls.distance};
+  ++k.shadow;
+  const float transmittance = Occluded(s, shadow, brute) ? 0.0f : 1.0f;
+  if (transmittance <= 0.0f) return F3{0, 0, 0};
+  const float weight = PowerHeuristic(ls.pdf, BsdfPdf(sf.bsdf, wo, wi));
+  const F3 c = f * ls.radiance * (transmittance * weight / ls.pdf);
+  return IsFinite(c) ? c : F3{0, 0, 0};
+}
+
+// transport.slang:365-431.
+bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
+                 const Hit& hit, Counters& k, bool brute) {
+  const bool sample_lights = fr.light_sampling != 0;
+  const Surface sf = ReconstructHit(s, hit, path.ray.d);
+  F3 emission = sf.emission;
+  if (sample_lights && !path.unweighted && path.depth != 0 &&
+      MaxComponent(emission) > 0.0f) {
+    float w = 1.0f;
+    const float area_pdf = EmitterPdf(s, hit, sf.area);
+    if (area_pdf > 0.0f) {
+      const F3 toward = sf.position - path.ray.o;
+      const float cosine = std::fabs(Dot(sf.normal, path.ray.d));
+      w = PowerHeuristic(path.last_pdf, area_pdf * Dot(toward, toward) / cosine);
+    }
+    emission = emission * w;
+  }
+  path.radiance = path.radiance + path.throughput * emission;
+  if (path.depth >= fr.max_bounces) return false;
+  if (path.depth >= 3) {
+    const float survival = std::min(MaxComponent(path.throughput), 0.95f);
+    if (NextRandom(path.rng) >= survival) return false;
+    path.throughput = path.throughput / survival;
+  }
+  const Frame basis = MakeShadingFrame(sf.shading_normal);
+  const F3 wo = basis.ToLocal(-path.ray.d);
+  if (sample_lights)
+    path.radiance = path.radiance + path.throughput * DirectLight(s, sf, basis, wo, path.rng, k, brute);
+  const float ux = NextRandom(path.rng), uy = NextRandom(path.rng), uz = NextRandom(path.rng);
+  SampleResult smp;
+  if (!BsdfSample(sf.bsdf, wo, ux, uy, uz, smp)) return false;
+  const F3 wi = Normalize(basis.ToWorld(smp.wi));
+  const float side = Dot(wi, sf.normal);
+  if (smp.transmitted ? side >= 0.0f : side <= 0.0f) return false;
+  path.throughput = path.throughput * smp.weight;
+  path.last_pdf = smp.pdf;
+  path.unweighted = smp.delta || smp.transmitted;
+  path.ray.o = OffsetRayOrigin(sf.position, smp.transmitted ? -sf.normal : sf.normal);
+  path.ray.d = wi;
+  path.ray.tmin = 0.0f;
+  path.ray.tmax = kFloatMax;
+  ++path.depth;
+  return true;
+}
+
+// transport.slang:436-448 (constant environment).
+void MissPath(const Scene& s, Path& path) {
+  path.radiance = path.radiance + path.throughput * s.environment;
+}
+
+// transport.slang:254-270.
+Path StartPath(const oracle_frame& fr, uint32_t x, uint32_t y, uint32_t sample) {
+  Path p;
+  p.throughput = F3{1, 1, 1};
+  p.radiance = F3{0, 0, 0};
+  p.depth = 0;
+  p.rng = SeedRandom(y * fr.width + x, sample);
+  p.last_pdf = 0.0f;
+  p.unweighted = false;
+  const float ux = NextRandom(p.rng), uy = NextRandom(p.rng);
+  const float fx = (static_cast<float>(x) + 0.5f) + (ux - 0.5f);
+  const float fy = (static_cast<float>(y) + 0.5f) + (uy - 0.5f);
+  p.ray = CameraRay(fr.clip_to_world, fx * fr.ndc_from_pixel[0] + fr.ndc_from_pixel[2],
+                    fy * fr.ndc_from_pixel[1] + fr.ndc_from_pixel[3]);
+  return p;
+}
+
+// path_trace.slang:32-40, 114-118: returns whether the camera ray hit.
+bool RunPath(const Scene& s, const oracle_frame& fr, uint32_t x, uint32_t y,
+             uint32_t sample, Counters& k, F3& radiance, uint32_t& depth,
+             bool brute) {
+  Path path = StartPath(fr, x, y, sample);
+  ++k.counts[0];
+  Hit hit;
+  if (!TraceClosest(s, path.ray, hit, brute)) {
+    depth = 0;
+    return false;
+  }
+  while (ScatterPath(s, fr, path, hit, k, brute)) {
+    ++k.counts[path.depth];
+    if (!TraceClosest(s, path.ray, hit, brute)) {
+      MissPath(s, path);
+      break;
+    }
+  }
+  radiance = path.radiance;
+  depth = path.depth;
+  return true;
+}
+
+template <typename F>
+void ParallelFor(uint32_t count, int threads, F&& body) {
+  if (threads <= 1 || count < 64) {
+    for (uint32_t i = 0; i < count; ++i) body(i, 0);
+    return;
+  }
+  std::atomic<uint32_t> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&, t] {
+      for (;;) {
+        const uint32_t begin = next.fetch_add(64);
+        if (begin >= count) break;
+        const uint32_t end = std::min(count, begin + 64);
+        for (uint32_t i = begin; i < end; ++i) body(i, t);
+      }
+    });
+  for (auto& th : pool) th.join();
+}
+
+inline Ray LoadRay(const float* r) {
+  return Ray{F3{r[0], r[1], r[2]}, r[3], F3{r[4], r[5], r[6]}, r[7]};
+}
+
+}  // namespace
+
+struct oracle_scene {
+  Scene s;
+};
+
+extern "C" {
+
+oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
+                                  size_t error_size) {
+  auto fail = [&](const std::string& m) -> oracle_scene* {
+    if (error && error_size) {
+      std::strncpy(error, m.c_str(), error_size - 1);
+      error[error_size - 1] = 0;
+    }
+    return nullptr;
+  };
+  auto scene = std::make_unique<oracle_scene>();
+  Scene& s = scene->s;
+  for (uint32_t m = 0; m < d->material_count; ++m)
+    s.materials.push_back(DecodeMaterial(d->materials + 36 * size_t(m)));
+  s.environment = F3{d->environment[0], d->environment[1], d->environment[2]};
+  s.instance_transform.assign(d->instance_transform,
+                              d->instance_transform + 16 * size_t(d->instance_count));
+  for (uint32_t i = 0; i < d->instance_count; ++i) {
+    const uint32_t geo = d->instance_geometry[i];
+    if (geo >= d->geometry_count) return fail("instance names a missing geometry");
+    if (d->instance_material[i] >= d->material_count) return fail("instance names a missing material");
+    s.instance_base.push_back(static_cast<uint32_t>(s.v0.size()));
+    s.instance_material.push_back(d->instance_material[i]);
+    const float* t = d->instance_transform + 16 * size_t(i);
+    const uint32_t* tri = d->triangles + 3 * size_t(d->geometry_triangle_offset[geo]);
+    const float* pos = d->positions + 3 * size_t(d->geometry_vertex_offset[geo]);
+    for (uint32_t k = 0; k < d->geometry_triangle_count[geo]; ++k) {
+      F3 w[3];
+      for (int c = 0; c < 3; ++c) {
+        const float* p = pos + 3 * size_t(tri[3 * k + c]);
+        w[c] = TransformPoint(t, F3{p[0], p[1], p[2]});
+      }
+      s.v0.push_back(w[0]);
+      s.v1.push_back(w[1]);
+      s.v2.push_back(w[2]);
+      s.tri_instance.push_back(i);
+      s.tri_primitive.push_back(k);
+    }
+  }
+  // Light table (gpu_scene.cpp:512-609), emissive triangles only.
+  s.light_offsets.assign(d->instance_count, kNoLights);
+  std::vector<double> weights;
+  for (uint32_t i = 0; i < d->instance_count; ++i) {
+    const double emission = EmissionEstimate(s.materials[s.instance_material[i]]);
+    const uint32_t geo = d->instance_geometry[i];
+    if (emission <= 0.0 || d->geometry_triangle_count[geo] == 0) continue;
+    s.light_offsets[i] = static_cast<uint32_t>(s.lights.size());
+    const uint32_t* tri = d->triangles + 3 * size_t(d->geometry_triangle_offset[geo]);
+    const float* pos = d->positions + 3 * size_t(d->geometry_vertex_offset[geo]);
+    for (uint32_t k = 0; k < d->geometry_triangle_count[geo]; ++k) {
+      F3 obj[3];
+      for (int c = 0; c < 3; ++c) {
+        const float* p = pos + 3 * size_t(tri[3 * k + c]);
+        obj[c] = F3{p[0], p[1], p[2]};
+      }
+      weights.push_back(emission * WorldTriangleArea(obj, d->instance_transform + 16 * size_t(i)));
+      s.lights.push_back(LightRecord{0.0f, 1.0f, 0u, i});
+    }
+  }
+  double total = 0.0;
+  for (double w : weights) total += w;
+  if (!(total > 0.0) || !std::isfinite(total)) {
+    s.light_offsets.assign(d->instance_count, kNoLights);
+    s.lights.clear();
+  } else {
+    const size_t count = s.lights.size();
+    std::vector<double> scaled(count);
+    std::vector<uint32_t> small, large;
+    for (size_t i = 0; i < count; ++i) {
+      s.lights[i].pdf = static_cast<float>(weights[i] / total);
+      scaled[i] = weights[i] / total * static_cast<double>(count);
+      (scaled[i] < 1.0 ? small : large).push_back(static_cast<uint32_t>(i));
+    }
+    while (!small.empty() && !large.empty()) {
+      const uint32_t less = small.back();
+      small.pop_back();
+      const uint32_t more = large.back();
+      s.lights[less].threshold = static_cast<float>(scaled[less]);
+      s.lights[less].alias = more;
+      scaled[more] -= 1.0 - scaled[less];
+      if (scaled[more] < 1.0) {
+        large.pop_back();
+        small.push_back(more);
+      }
+    }
+    for (uint32_t i : small) {
+      s.lights[i].threshold = 1.0f;
+      s.lights[i].alias = i;
+    }
+    for (uint32_t i : large) {
+      s.lights[i].threshold = 1.0f;
+      s.lights[i].alias = i;
+    }
+  }
+  BuildBvh(s);
+  return scene.release();
+}
+
+void oracle_scene_destroy(oracle_scene* scene) { delete scene; }
+
+uint32_t oracle_scene_triangle_count(const oracle_scene* scene) {
+  return static_cast<uint32_t>(scene->s.v0.size());
+}
+
+void oracle_scene_world_triangles(const oracle_scene* scene, float* out) {
+  const Scene& s = scene->s;
+  for (size_t g = 0; g < s.v0.size(); ++g) {
+    const F3 c[3] = {s.v0[g], s.v1[g], s.v2[g]};
+    for (int k = 0; k < 3; ++k) {
+      out[9 * g + 3 * k + 0] = c[k].x;
+      out[9 * g + 3 * k + 1] = c[k].y;
+      out[9 * g + 3 * k + 2] = c[k].z;
+    }
+  }
+}
+
+uint32_t oracle_light_count(const oracle_scene* scene) {
+  return static_cast<uint32_t>(scene->s.lights.size());
+}
+
+void oracle_light_table(const oracle_scene* scene, uint32_t* offsets, uint32_t* records) {
+  const Scene& s = scene->s;
+  std::memcpy(offsets, s.light_offsets.data(), s.light_offsets.size() * 4);
+  std::memcpy(records, s.lights.data(), s.lights.size() * 16);
+}
+
+void oracle_trace_closest(const oracle_scene* scene, uint32_t count, const float* rays,
+                          uint32_t* hits, int brute_force, int threads) {
+  ParallelFor(count, threads, [&](uint32_t i, int) {
+    Hit h;
+    const bool hit = TraceClosest(scene->s, LoadRay(rays + 8 * size_t(i)), h, brute_force != 0);
+    uint32_t* o = hits + 5 * size_t(i);
+    o[0] = hit ? h.instance : kMiss;
+    o[1] = hit ? h.primitive : kMiss;
+    const float f[3] = {hit ? h.u : 0.0f, hit ? h.v : 0.0f, hit ? h.t : 0.0f};
+    std::memcpy(o + 2, f, 12);
+  });
+}
+
+void oracle_trace_occluded(const oracle_scene* scene, uint32_t count, const float* rays,
+                           uint32_t* occluded, int brute_force, int threads) {
+  ParallelFor(count, threads, [&](uint32_t i, int) {
+    occluded[i] = Occluded(scene->s, LoadRay(rays + 8 * size_t(i)), brute_force != 0) ? 1u : 0u;
+  });
+}
+
+int oracle_render(const oracle_scene* scene, const oracle_frame* frame, double* accum,
+                  uint32_t* counts, uint64_t* shadow, int threads) {
+  const oracle_frame& fr = *frame;
+  const uint32_t rw = fr.rect[2], rh = fr.rect[3];
+  const uint32_t stride = fr.max_bounces + 2;
+  const int nthreads = std::max(1, threads);
+  std::vector<std::vector<uint32_t>> local_counts(
+      nthreads, std::vector<uint32_t>(size_t(fr.spp) * stride, 0u));
+  std::vector<std::vector<uint64_t>> local_shadow(nthreads, std::vector<uint64_t>(fr.spp, 0u));
+  ParallelFor(rw * rh, nthreads, [&](uint32_t index, int t) {
+    const uint32_t x = fr.rect[0] + index % rw, y = fr.rect[1] + index / rw;
+    double* a = accum + 8 * (size_t(y) * fr.width + x);
+    for (uint32_t k = 0; k < fr.spp; ++k) {
+      Counters c{local_counts[t].data() + size_t(k) * stride, 0};
+      F3 rad;
+      uint32_t depth;
+      if (RunPath(scene->s, fr, x, y, fr.sample_index + k, c, rad, depth, false)) {
+        a[0] += rad.x;
+        a[1] += rad.y;
+        a[2] += rad.z;
+        a[3] += 1.0;
+        a[4] += double(rad.x) * rad.x;
+        a[5] += double(rad.y) * rad.y;
+        a[6] += double(rad.z) * rad.z;
+      }
+      local_shadow[t][k] += c.shadow;
+    }
+  });
+  for (size_t i = 0; i < size_t(fr.spp) * stride; ++i) {
+    uint32_t sum = 0;
+    for (int t = 0; t < nthreads; ++t) sum += local_counts[t][i];
+    counts[i] = sum;
+  }
+  for (uint32_t k = 0; k < fr.spp; ++k) {
+    uint64_t sum = 0;
+    for (int t = 0; t < nthreads; ++t) sum += local_shadow[t][k];
+    shadow[k] = sum;
+  }
+  return 0;
+}
+
+uint32_t oracle_trace_path(const oracle_scene* scene, const oracle_frame* frame, uint32_t x,
+                           uint32_t y, uint32_t sample_index, float* rgb) {
+  std::vector<uint32_t> counts(frame->max_bounces + 2, 0u);
+  Counters c{counts.data(), 0};
+  F3 rad{0, 0, 0};
+  uint32_t depth = 0;
+  RunPath(scene->s, *frame, x, y, sample_index, c, rad, depth, false);
+  rgb[0] = rad.x;
+  rgb[1] = rad.y;
+  rgb[2] = rad.z;
+  return depth;
+}
+
+void oracle_camera_ray(const float* clip_to_world, float nx, float ny, float* ray) {
+  const Ray r = CameraRay(clip_to_world, nx, ny);
+  const float o[8] = {r.o.x, r.o.y, r.o.z, r.tmin, r.d.x, r.d.y, r.d.z, r.tmax};
+  std::memcpy(ray, o, sizeof o);
+}
+
+int oracle_camera_constants(const float* gl, float* vk, float* clip_to_world) {
+  VulkanWorldToClip(gl, vk);
+  return Invert(vk, clip_to_world) ? 1 : 0;
+}
+
+uint32_t oracle_pcg_hash(uint32_t value) { return PcgHash(value); }
+uint32_t oracle_seed_random(uint32_t pixel, uint32_t sample) { return SeedRandom(pixel, sample); }
+void oracle_random_sequence(uint32_t state, uint32_t count, float* out) {
+  for (uint32_t i = 0; i < count; ++i) out[i] = NextRandom(state);
+}
+
+void oracle_sincos_turns(uint32_t count, const float* u, float* s, float* c) {
+  for (uint32_t i = 0; i < count; ++i) SinCosTurns(u[i], &s[i], &c[i]);
+}
+
+void oracle_bsdf_eval(const uint32_t* material, int entering, const float* wo,
+                      const float* wi, float* f, float* pdf) {
+  const Material m = DecodeMaterial(material);
+  const Closure b = MakePreviewClosure(m.base_color, m.roughness, m.metallic, m.ior,
+                                       (m.flags & kSpecularWorkflow) != 0, m.specular,
+                                       m.transmission, m.transmission_color,
+                                       (m.flags & kThinWalled) != 0, entering != 0);
+  const F3 o{wo[0], wo[1], wo[2]}, i{wi[0], wi[1], wi[2]};
+  const F3 v = BsdfEvaluate(b, o, i);
+  f[0] = v.x;
+  f[1] = v.y;
+  f[2] = v.z;
+  *pdf = BsdfPdf(b, o, i);
+}
+
+void oracle_bsdf_sample(const uint32_t* material, int entering, const float* wo,
+                        const float* u, float* out) {
+  const Material m = DecodeMaterial(material);
+  const Closure b = MakePreviewClosure(m.base_color, m.roughness, m.metallic, m.ior,
+                                       (m.flags & kSpecularWorkflow) != 0, m.specular,
+                                       m.transmission, m.transmission_color,
+                                       (m.flags & kThinWalled) != 0, entering != 0);
+  SampleResult s;
+  const bool ok = BsdfSample(b, F3{wo[0], wo[1], wo[2]}, u[0], u[1], u[2], s);
+  const float o[10] = {s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z,
+                       s.pdf, s.delta ? 1.0f : 0.0f, s.transmitted ? 1.0f : 0.0f,
+                       ok ? 1.0f : 0.0f};
+  std::memcpy(out, o, sizeof o);
+}
+
+void oracle_offset_ray_origin(const float* position, const float* normal, float* out) {
+  const F3 r = OffsetRayOrigin(F3{position[0], position[1], position[2]},
+                               F3{normal[0], normal[1], normal[2]});
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+/* CPU oracle for the wavefront path tracer — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain C++ restatement of hydra-lotus's reference transport (the Slang
+ * reference integrator), used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker. Nothing in the product links,
+ * loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference repository root). Pinning: see the header comment of
+ * lotus_oracle.cpp for which check pins which function.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The scene as the oracle reads it: object-space geometries placed by
+ * instances, with one material record per slot. The material record is the
+ * reference's GpuMaterialRecord (backend/vulkan/gpu_scene.hpp:94-113),
+ * 36 words. */
+typedef struct oracle_scene_desc {
+  uint32_t geometry_count;
+  const float* positions;           /* xyz per vertex, all geometries */
+  const uint32_t* triangles;        /* 3 indices per triangle, geometry-relative */
+  const uint32_t* geometry_vertex_offset;   /* per geometry */
+  const uint32_t* geometry_triangle_offset; /* per geometry */
+  const uint32_t* geometry_triangle_count;  /* per geometry */
+  uint32_t instance_count;
+  const uint32_t* instance_geometry; /* per instance */
+  const float* instance_transform;   /* 16 floats per instance, column-major */
+  const uint32_t* instance_material; /* per instance */
+  uint32_t material_count;
+  const uint32_t* materials;         /* 36 words per material */
+  float environment[3];
+} oracle_scene_desc;
+
+typedef struct oracle_scene oracle_scene;
+
+oracle_scene* oracle_scene_create(const oracle_scene_desc* desc, char* error,
+                                  size_t error_size);
+void oracle_scene_destroy(oracle_scene* scene);
+uint32_t oracle_scene_triangle_count(const oracle_scene* scene);
+/* world-space corners of flattened triangle g, 9 floats */
+void oracle_scene_world_triangles(const oracle_scene* scene, float* out);
+
+/* Light table (backend/vulkan/gpu_scene.cpp:512-609): per instance first
+ * entry or 0xFFFFFFFF, and per entry {pdf, threshold, alias, instance}. */
+uint32_t oracle_light_count(const oracle_scene* scene);
+void oracle_light_table(const oracle_scene* scene, uint32_t* offsets,
+                        uint32_t* records /* 4 words per entry */);
+
+/* Rays: origin xyz, tmin, direction xyz, tmax (8 floats). Hits: instance,
+ * primitive (0xFFFFFFFF on a miss), barycentric u, v, t (5 words). */
+void oracle_trace_closest(const oracle_scene* scene, uint32_t count,
+                          const float* rays, uint32_t* hits, int brute_force,
+                          int threads);
+void oracle_trace_occluded(const oracle_scene* scene, uint32_t count,
+                           const float* rays, uint32_t* occluded,
+                           int brute_force, int threads);
+
+typedef struct oracle_frame {
+  float clip_to_world[16];  /* column-major */
+  float ndc_from_pixel[4];
+  uint32_t width, height;   /* target size; pixel index = y * width + x */
+  uint32_t rect[4];         /* data window x, y, w, h */
+  uint32_t sample_index;    /* first sample's random sequence */
+  uint32_t spp;             /* samples per pixel this call */
+  uint32_t max_bounces;
+  uint32_t light_sampling;  /* 0 BSDF only, 1 NEE + MIS */
+} oracle_frame;
+
+/* Renders spp samples per pixel of the data window. accum: 8 doubles per
+ * target pixel: sum r, g, b, hit samples, sum r^2, g^2, b^2, 0. counts:
+ * spp x (max_bounces + 2) ray-queue occupancies per sample (camera rays at
+ * [0]); shadow: spp shadow-ray counts. */
+int oracle_render(const oracle_scene* scene, const oracle_frame* frame,
+                  double* accum, uint32_t* counts, uint64_t* shadow,
+                  int threads);
+
+/* One path's radiance, for single-path probes: rgb out, returns depth. */
+uint32_t oracle_trace_path(const oracle_scene* scene, const oracle_frame* frame,
+                           uint32_t x, uint32_t y, uint32_t sample_index,
+                           float* rgb);
+
+/* The camera ray through a film position (reference CameraRay). */
+void oracle_camera_ray(const float* clip_to_world, float ndc_x, float ndc_y,
+                       float* ray /* 8 floats */);
+
+/* OpenGL world_to_clip -> Vulkan world_to_clip -> inverse (float). Returns 0
+ * on a singular matrix. */
+int oracle_camera_constants(const float* world_to_clip_gl,
+                            float* world_to_clip_vk, float* clip_to_world);
+
+/* RNG probes (backend/vulkan/shaders/sampling/sampling.slang:9-25). */
+uint32_t oracle_pcg_hash(uint32_t value);
+uint32_t oracle_seed_random(uint32_t pixel, uint32_t sample);
+void oracle_random_sequence(uint32_t state, uint32_t count, float* out);
+
+/* Sine and cosine of 2 pi u, as every sampled angle uses them. */
+void oracle_sincos_turns(uint32_t count, const float* u, float* s, float* c);
+
+/* BSDF probes over one material record and a shading-frame direction pair.
+ * eval: f*|cos| rgb; pdf; sample: wi xyz, weight rgb, pdf, delta, transmitted,
+ * ok (10 floats). `entering` as EvaluatePreview decides it. */
+void oracle_bsdf_eval(const uint32_t* material, int entering, const float* wo,
+                      const float* wi, float* f, float* pdf);
+void oracle_bsdf_sample(const uint32_t* material, int entering,
+                        const float* wo, const float* u, float* out);
+
+/* OffsetRayOrigin probe. */
+void oracle_offset_ray_origin(const float* position, const float* normal,
+                              float* out);
+
+#ifdef __cplusplus
+}
+#endif
