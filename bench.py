@@ -1,0 +1,183 @@
+"""Wavefront path-tracing benchmark on one MI355X.
+
+Workload: the 1024 x 1024-cell heightfield (2,097,152 triangles, seeded
+6-octave value noise, 64 emissive triangles, per-tile GGX PreviewSurface
+materials), 1920 x 1080, 1 sample per pixel per frame, max depth 6, next
+event estimation with MIS. A step is one frame: camera rays, every bounce's
+extension and shadow rays, and the resolve into the accumulation, on scene
+data already resident in HBM.
+
+Prints one JSON line: whole-frame Mrays/s (camera + extension + shadow rays
+over the wall time of the timed frames), Mrays/s by ray kind from hipEvent
+kernel times, the dominant kernel's roofline, and the CPU oracle's rate on a
+bounded sample of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "hydra-lotus_amd"))
+
+import numpy as np  # noqa: E402
+
+from lotus_amd import hip, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 128
+TRI_BYTES = 48
+# Queue bytes per ray of each traversal kernel: camera writes a hit record
+# (48 B) and the radiance word (16 B); intersect reads a ray (32 B) and writes
+# a hit record (48 B); shadow reads a ray (32 B) and, unblocked, reads the
+# delta and read-modify-writes the radiance (48 B).
+QUEUE_BYTES = {"camera": 64, "intersect": 80, "shadow": 80}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, width, height, depth, threads, target_s=12.0):
+    """The CPU oracle on a band of rows of the same frame, 1 spp."""
+    import oracle
+
+    o = oracle.OracleScene(scene)
+    rows = 8
+
+    def run(rows):
+        y0 = max(0, height // 2 - rows // 2)
+        f = oracle.make_frame(scene.world_to_clip, width, height, spp=1, max_bounces=depth,
+                              light_sampling=1, rect=(0, y0, width, rows))
+        t0 = time.perf_counter()
+        _, counts, shadow = o.render(f, threads=threads)
+        dt = time.perf_counter() - t0
+        return dt, int(counts.sum()) + int(shadow.sum())
+
+    dt, rays = run(rows)
+    rows = int(min(height, max(rows, rows * target_s / max(dt, 1e-3))))
+    dt, rays = run(rows)
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} x {width} px band of the same 1080p frame, 1 spp, depth {depth}, NEE+MIS, "
+                      f"{rays} rays in {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=6)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by image "
+              "tiles, see the design notes)", file=sys.stderr)
+        return 2
+
+    log(f"building heightfield {args.cells}^2")
+    scene = scenes.heightfield(cells=args.cells)
+    r = hip.Renderer(0)
+    ev = r.apply(scene)
+    log(f"scene: {ev.triangle_count} tris, {ev.node_count} nodes, depth {ev.bvh_depth}, "
+        f"build {ev.build_gpu_ms:.2f} ms")
+    w2c = scene.world_to_clip
+    kw = dict(frame_count=1, max_bounces=args.depth, light_sampling=hip.LIGHT_MIS, want_color=False)
+    for _ in range(args.warmup):
+        r.render(w2c, args.width, args.height, **kw)
+    log(f"warmup done ({args.warmup} frames)")
+    tot = {k: 0.0 for k in ("camera_rays", "extension_rays", "shadow_rays", "camera_ms", "intersect_ms",
+                            "shade_ms", "shadow_ms", "resolve_ms", "frame_ms")}
+    nodes = np.zeros(4)
+    tris = np.zeros(4)
+    launches = {"intersect": 0, "shadow": 0, "camera": 0}
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = r.render(w2c, args.width, args.height, **kw)
+        for key in tot:
+            tot[key] += out[key]
+        nodes += out["nodes"]
+        tris += out["tris"]
+        launches["camera"] += 1
+        launches["intersect"] += max(out["rounds"] - 1, 0) + (1 if out["rounds"] < args.depth + 1 else 0)
+        launches["shadow"] += out["rounds"]
+    wall = time.perf_counter() - t0
+    rays = tot["camera_rays"] + tot["extension_rays"] + tot["shadow_rays"]
+    value = rays / wall / 1e6
+    by_kind = {
+        "camera": tot["camera_rays"] / tot["camera_ms"] / 1e3,
+        "extension": tot["extension_rays"] / tot["intersect_ms"] / 1e3,
+        "shadow": tot["shadow_rays"] / tot["shadow_ms"] / 1e3,
+        "frame_gpu": rays / tot["frame_ms"] / 1e3,
+    }
+    # roofline of the dominant traversal kernel: algorithmic bytes (queue
+    # records + wide nodes and triangles the traversal tests) / kernel time
+    kinds = {"camera": (0, tot["camera_ms"], tot["camera_rays"]),
+             "intersect": (1, tot["intersect_ms"], tot["extension_rays"]),
+             "shadow": (2, tot["shadow_ms"], tot["shadow_rays"])}
+    dom = max(kinds, key=lambda k: kinds[k][1])
+    idx, ms, n = kinds[dom]
+    extra = 3 if dom == "intersect" else None  # the tail traverses as intersect does
+    node_visits = nodes[idx] + (nodes[extra] if extra else 0)
+    tri_tests = tris[idx] + (tris[extra] if extra else 0)
+    bytes_total = node_visits * NODE_BYTES + tri_tests * TRI_BYTES + n * QUEUE_BYTES[dom]
+    achieved = bytes_total / (ms / 1e3) / 1e9
+    per_launch = bytes_total / max(launches[dom], 1)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "algorithmic_bytes_per_launch": int(per_launch),
+                "avg_launch_ms": round(ms / max(launches[dom], 1), 4),
+                "nodes_per_ray": round(node_visits / max(n, 1), 2),
+                "tris_per_ray": round(tri_tests / max(n, 1), 2)}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            t = json.load(f)
+        if t.get("kernel") == dom:
+            roofline["traffic"] = t.get("hbm_bytes_per_launch")
+    line = {
+        "metric": "Mrays/s by ray kind + GPU ms to reach equal relMSE (plain vs MLP cache), 1080p",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural heightfield)",
+        "config": {"workload": f"wavefront PT, {args.cells}x{args.cells} heightfield "
+                               f"({ev.triangle_count} tris), {args.width}x{args.height}, 1 spp/frame, "
+                               f"depth {args.depth}, NEE+MIS",
+                   "frames": args.steps, "bvh": f"4-wide, {ev.node_count} nodes, depth {ev.bvh_depth}"},
+        "mrays_by_kind": {k: round(v, 1) for k, v in by_kind.items()},
+        "rays_per_frame": {"camera": tot["camera_rays"] / args.steps,
+                           "extension": tot["extension_rays"] / args.steps,
+                           "shadow": tot["shadow_rays"] / args.steps},
+        "gpu_ms_per_frame": {k.replace("_ms", ""): round(tot[k] / args.steps, 3)
+                             for k in ("camera_ms", "intersect_ms", "shade_ms", "shadow_ms", "resolve_ms",
+                                       "frame_ms")},
+        "roofline": roofline,
+    }
+    r.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline(scene, args.width, args.height, args.depth, args.cpu_threads)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,595 @@
+// GPU construction of the wide BVH that replaces the driver's BLAS/TLAS
+// (backend/vulkan/gpu_scene.cpp:1899-1950, 2250-2323, 2411-2435):
+//   1. flatten instances into world-space triangles (the reference's
+//      world_from_object applied as scene.slang:1412-1417 does);
+//   2. 30-bit Morton codes of centroids, stable LSD radix sort (8-bit digits,
+//      wave64 ballot ranking);
+//   3. binary radix tree (Karras, "Maximizing Parallelism in the Construction
+//      of BVHs, Octrees, and k-d Trees", HPG 2012) and bottom-up boxes;
+//   4. top-down collapse into 4-wide nodes, opening the child of largest
+//      surface area first, leaves of at most `leaf_size` triangles.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+
+namespace lotus {
+namespace {
+
+#define LOTUS_CHECK(expr)                                                  \
+  do {                                                                     \
+    const hipError_t e__ = (expr);                                         \
+    if (e__ != hipSuccess) {                                               \
+      error = std::string(#expr) + ": " + hipGetErrorString(e__);          \
+      return false;                                                        \
+    }                                                                      \
+  } while (0)
+
+constexpr int kBlock = 256;
+
+inline uint32_t blocks_for(uint64_t n, uint32_t per_block) {
+  return static_cast<uint32_t>((n + per_block - 1) / per_block);
+}
+
+// Float bits mapped so unsigned order is float order.
+__device__ __forceinline__ uint32_t ordered(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unordered(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// Step 1: world-space triangles in flattened (instance, primitive) order, and
+// the centroid bounds (ordered-int atomics: lo[3], hi[3]).
+__global__ void __launch_bounds__(kBlock) k_flatten(BuildInput in, Tri* world, float4* centroid,
+                                                    uint32_t* bounds) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  float c[3] = {0.0f, 0.0f, 0.0f};
+  const bool valid = g < in.tri_count;
+  if (valid) {
+    uint32_t lo = 0, hi = in.instance_count;  // last instance with base <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (in.instance_base[mid] <= g) lo = mid; else hi = mid;
+    }
+    const uint32_t inst = lo, prim = g - in.instance_base[inst];
+    const uint32_t geo = in.instance_geometry[inst];
+    const uint32_t* tri = in.triangles + 3ull * (in.geometry_triangle_offset[geo] + prim);
+    const float* pos = in.positions + 3ull * in.geometry_vertex_offset[geo];
+    const float* m = in.instance_transform + 16ull * inst;
+    float w[3][4];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float* p = pos + 3ull * tri[k];
+      mul_point(m, p[0], p[1], p[2], 1.0f, w[k]);
+    }
+    Tri t;
+    t.a = make_float4(w[0][0], w[0][1], w[0][2], __uint_as_float(g));
+    t.b = make_float4(w[1][0], w[1][1], w[1][2], __uint_as_float(inst));
+    t.c = make_float4(w[2][0], w[2][1], w[2][2], __uint_as_float(prim));
+    world[g] = t;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) c[a] = (w[0][a] + w[1][a] + w[2][a]) * (1.0f / 3.0f);
+    centroid[g] = make_float4(c[0], c[1], c[2], 0.0f);
+  }
+  __shared__ uint32_t s_lo[3], s_hi[3];
+  if (threadIdx.x < 3) {
+    s_lo[threadIdx.x] = 0xFFFFFFFFu;
+    s_hi[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&s_lo[a], ordered(c[a]));
+      atomicMax(&s_hi[a], ordered(c[a]));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    atomicMin(&bounds[threadIdx.x], s_lo[threadIdx.x]);
+    atomicMax(&bounds[3 + threadIdx.x], s_hi[threadIdx.x]);
+  }
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_morton(const float4* centroid, const uint32_t* bounds,
+                                                   uint32_t n, uint32_t* keys, uint32_t* values) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const float4 c = centroid[g];
+  const float cc[3] = {c.x, c.y, c.z};
+  uint32_t q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float lo = unordered(bounds[a]), hi = unordered(bounds[3 + a]);
+    const float ext = hi - lo;
+    const float t = ext > 0.0f ? (cc[a] - lo) / ext : 0.5f;
+    q[a] = static_cast<uint32_t>(fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f));
+  }
+  keys[g] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+  values[g] = g;
+}
+
+// ------------------------------------------------------------ radix sort --
+constexpr int kSortItems = 8;                      // keys per thread per tile
+constexpr uint32_t kSortTile = kBlock * kSortItems;  // 2048
+
+__global__ void __launch_bounds__(kBlock) k_sort_histogram(const uint32_t* keys, uint32_t n, int shift,
+                                                           uint32_t tiles, uint32_t* counts) {
+  __shared__ uint32_t hist[256];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t k = base + i * kBlock + threadIdx.x;
+    if (k < n) atomicAdd(&hist[(keys[k] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  counts[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Exclusive scan of `n` counts by one block (digit-major order).
+__global__ void __launch_bounds__(1024) k_sort_scan(uint32_t* counts, uint32_t n) {
+  __shared__ uint32_t part[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  constexpr uint32_t kPer = 4;
+  for (uint32_t start = 0; start < n; start += 1024 * kPer) {
+    uint32_t v[kPer];
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t k = start + threadIdx.x * kPer + i;
+      v[i] = k < n ? counts[k] : 0u;
+      sum += v[i];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+      const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+      __syncthreads();
+      part[threadIdx.x] += add;
+      __syncthreads();
+    }
+    uint32_t run = carry + part[threadIdx.x] - sum;
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t k = start + threadIdx.x * kPer + i;
+      if (k < n) counts[k] = run;
+      run += v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+}
+
+// Stable scatter: items are ranked in tile order, 256 per round; within a
+// wave by ballot matching of the digit bits, across the block's four waves
+// by their per-digit counts in LDS.
+__global__ void __launch_bounds__(kBlock) k_sort_scatter(const uint32_t* keys_in, const uint32_t* vals_in,
+                                                         uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                                                         int shift, uint32_t tiles, const uint32_t* offsets) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wave_count[kWaves][256];
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid / 64;
+  base[tid] = offsets[tid * tiles + blockIdx.x];
+  for (int w = 0; w < kWaves; ++w) wave_count[w][tid] = 0;
+  __syncthreads();
+  const uint32_t start = blockIdx.x * kSortTile;
+  for (int round = 0; round < kSortItems; ++round) {
+    const uint32_t k = start + round * kBlock + tid;
+    const bool valid = k < n;
+    const uint32_t key = valid ? keys_in[k] : 0u;
+    const uint32_t val = valid ? vals_in[k] : 0u;
+    const uint32_t digit = (key >> shift) & 255u;
+    uint64_t same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (digit >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    const uint32_t rank = static_cast<uint32_t>(__popcll(same & ((1ull << lane) - 1ull)));
+    const bool leader = valid && rank == 0;
+    if (leader) wave_count[wave][digit] = static_cast<uint32_t>(__popcll(same));
+    __syncthreads();
+    if (valid) {
+      uint32_t before = 0;
+      for (uint32_t w = 0; w < wave; ++w) before += wave_count[w][digit];
+      const uint32_t pos = base[digit] + before + rank;
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    uint32_t total = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      total += wave_count[w][tid];
+      wave_count[w][tid] = 0;
+    }
+    base[tid] += total;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------ binary radix tree --
+__device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = keys[i], b = keys[j];
+  if (a == b) return 32 + __clz(static_cast<uint32_t>(i) ^ static_cast<uint32_t>(j));
+  return __clz(a ^ b);
+}
+
+// Internal node i in [0, n-2]; leaves are n-1+k. child[2i], child[2i+1];
+// parent of every node; range [first, last] of internal nodes.
+__global__ void __launch_bounds__(kBlock) k_karras(const uint32_t* keys, int n, uint32_t* child,
+                                                   uint32_t* parent, uint2* range) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n - 1) return;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int s = 0;
+  int t = l;
+  do {
+    t = (t + 1) / 2;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int gamma = i + s * d + (d < 0 ? -1 : 0);
+  const int first = i < j ? i : j, last = i < j ? j : i;
+  const uint32_t left = first == gamma ? static_cast<uint32_t>(n - 1 + gamma) : static_cast<uint32_t>(gamma);
+  const uint32_t right = last == gamma + 1 ? static_cast<uint32_t>(n - 1 + gamma + 1)
+                                           : static_cast<uint32_t>(gamma + 1);
+  child[2 * i] = left;
+  child[2 * i + 1] = right;
+  parent[left] = static_cast<uint32_t>(i);
+  parent[right] = static_cast<uint32_t>(i);
+  range[i] = make_uint2(static_cast<uint32_t>(first), static_cast<uint32_t>(last));
+}
+
+struct Box {
+  float lo[3], hi[3];
+};
+
+// Leaf boxes, sorted triangles, then boxes up the tree: the second of two
+// children to finish computes its parent.
+__global__ void __launch_bounds__(kBlock) k_bottom_up(const Tri* world, const uint32_t* sorted, int n,
+                                                      const uint32_t* child, const uint32_t* parent,
+                                                      uint32_t* flags, Box* boxes, Tri* tris,
+                                                      uint32_t* tri_of_global) {
+  const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= n) return;
+  const uint32_t g = sorted[k];
+  const Tri t = world[g];
+  tris[k] = t;
+  tri_of_global[g] = static_cast<uint32_t>(k);
+  Box b;
+  b.lo[0] = fminf(fminf(t.a.x, t.b.x), t.c.x);
+  b.lo[1] = fminf(fminf(t.a.y, t.b.y), t.c.y);
+  b.lo[2] = fminf(fminf(t.a.z, t.b.z), t.c.z);
+  b.hi[0] = fmaxf(fmaxf(t.a.x, t.b.x), t.c.x);
+  b.hi[1] = fmaxf(fmaxf(t.a.y, t.b.y), t.c.y);
+  b.hi[2] = fmaxf(fmaxf(t.a.z, t.b.z), t.c.z);
+  uint32_t node = static_cast<uint32_t>(n - 1 + k);
+  boxes[node] = b;
+  if (n == 1) return;
+  __threadfence();
+  uint32_t p = parent[node];
+  while (true) {
+    if (atomicAdd(&flags[p], 1u) == 0u) return;  // first child to arrive stops
+    __threadfence();
+    const Box l = boxes[child[2 * p]], r = boxes[child[2 * p + 1]];
+    Box u;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      u.lo[a] = fminf(l.lo[a], r.lo[a]);
+      u.hi[a] = fmaxf(l.hi[a], r.hi[a]);
+    }
+    boxes[p] = u;
+    __threadfence();
+    if (p == 0) return;
+    p = parent[p];
+  }
+}
+
+// ---------------------------------------------------------------- collapse --
+struct Task {
+  uint32_t binary;  // binary node to expand
+  uint32_t wide;    // wide node to write
+};
+
+__device__ __forceinline__ float half_area(const Box& b) {
+  const float x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
+  return x * y + y * z + z * x;
+}
+
+// Boxes grow by a relative margin so no rounding of their corners can cull a
+// triangle the watertight test accepts.
+__device__ __forceinline__ void inflate(float& lo, float& hi) {
+  const float m = (fabsf(lo) + fabsf(hi) + (hi - lo)) * 1.0e-6f + 1.0e-30f;
+  lo -= m;
+  hi += m;
+}
+
+__global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const uint32_t* task_count,
+                                                     Task* next, uint32_t* next_count, uint32_t* node_count,
+                                                     uint32_t* leaf_total, const uint32_t* child,
+                                                     const uint2* range, const Box* boxes, int n,
+                                                     uint32_t leaf_size, WideNode* nodes) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= *task_count) return;
+  const Task task = tasks[t];
+  uint32_t list[4];
+  int count = 0;
+  auto size_of = [&](uint32_t b) -> uint32_t {
+    if (b >= static_cast<uint32_t>(n - 1)) return 1u;
+    const uint2 r = range[b];
+    return r.y - r.x + 1u;
+  };
+  if (n == 1) {
+    list[count++] = 0;  // the single leaf is binary node 0 (n - 1 + 0)
+  } else if (size_of(task.binary) <= leaf_size) {
+    list[count++] = task.binary;  // the whole subtree is one leaf child
+  } else {
+    list[count++] = child[2 * task.binary];
+    list[count++] = child[2 * task.binary + 1];
+    while (count < 4) {
+      int pick = -1;
+      float best = -1.0f;
+      for (int c = 0; c < count; ++c) {
+        const uint32_t b = list[c];
+        if (size_of(b) > leaf_size) {
+          const float a = half_area(boxes[b]);
+          if (a > best) {
+            best = a;
+            pick = c;
+          }
+        }
+      }
+      if (pick < 0) break;
+      const uint32_t b = list[pick];
+      list[pick] = child[2 * b];
+      list[count++] = child[2 * b + 1];
+    }
+  }
+  WideNode out;
+  uint32_t leaves = 0;
+  for (int c = 0; c < 4; ++c) {
+    if (c >= count) {
+      out.lox[c] = out.loy[c] = out.loz[c] = INFINITY;
+      out.hix[c] = out.hiy[c] = out.hiz[c] = -INFINITY;
+      out.child[c] = kEmptyChild;
+      continue;
+    }
+    const uint32_t b = list[c];
+    Box box = boxes[b];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) inflate(box.lo[a], box.hi[a]);
+    out.lox[c] = box.lo[0];
+    out.loy[c] = box.lo[1];
+    out.loz[c] = box.lo[2];
+    out.hix[c] = box.hi[0];
+    out.hiy[c] = box.hi[1];
+    out.hiz[c] = box.hi[2];
+    const uint32_t sz = size_of(b);
+    if (sz <= leaf_size) {
+      const uint32_t first = b >= static_cast<uint32_t>(n - 1) ? b - static_cast<uint32_t>(n - 1) : range[b].x;
+      out.child[c] = leaf_word(first, sz);
+      ++leaves;
+    } else {
+      const uint32_t w = atomicAdd(node_count, 1u);
+      out.child[c] = w;
+      next[atomicAdd(next_count, 1u)] = Task{b, w};
+    }
+  }
+  out.pad[0] = out.pad[1] = out.pad[2] = out.pad[3] = 0;
+  nodes[task.wide] = out;
+  if (leaves) atomicAdd(leaf_total, leaves);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_empty_root(WideNode* nodes) {
+  WideNode out;
+  for (int c = 0; c < 4; ++c) {
+    out.lox[c] = out.loy[c] = out.loz[c] = INFINITY;
+    out.hix[c] = out.hiy[c] = out.hiz[c] = -INFINITY;
+    out.child[c] = kEmptyChild;
+    out.pad[c] = 0;
+  }
+  nodes[0] = out;
+}
+
+template <typename T>
+bool alloc(T*& p, size_t count, std::string& error) {
+  p = nullptr;
+  if (count == 0) count = 1;
+  const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+  if (e != hipSuccess) {
+    error = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+bool radix_sort_pairs(uint32_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits,
+                      hipStream_t stream, std::string& error) {
+  if (n <= 1) return true;
+  const uint32_t tiles = blocks_for(n, kSortTile);
+  uint32_t *k2 = nullptr, *v2 = nullptr, *counts = nullptr;
+  if (!alloc(k2, n, error) || !alloc(v2, n, error) || !alloc(counts, 256ull * tiles, error)) {
+    (void)hipFree(k2);
+    (void)hipFree(v2);
+    return false;
+  }
+  uint32_t *ka = keys, *va = values, *kb = k2, *vb = v2;
+  const int passes = static_cast<int>((key_bits + 7) / 8);
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = pass * 8;
+    k_sort_histogram<<<tiles, kBlock, 0, stream>>>(ka, n, shift, tiles, counts);
+    k_sort_scan<<<1, 1024, 0, stream>>>(counts, 256u * tiles);
+    k_sort_scatter<<<tiles, kBlock, 0, stream>>>(ka, va, kb, vb, n, shift, tiles, counts);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && ka != keys) {
+    e = hipMemcpyAsync(keys, ka, n * 4ull, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(values, va, n * 4ull, hipMemcpyDeviceToDevice, stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(k2);
+  (void)hipFree(v2);
+  (void)hipFree(counts);
+  if (e != hipSuccess) {
+    error = std::string("radix sort: ") + hipGetErrorString(e);
+    return false;
+  }
+  return true;
+}
+
+bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, double& gpu_ms,
+               std::string& error) {
+  const uint32_t n = in.tri_count;
+  hipEvent_t start, stop;
+  LOTUS_CHECK(hipEventCreate(&start));
+  LOTUS_CHECK(hipEventCreate(&stop));
+  LOTUS_CHECK(hipEventRecord(start, stream));
+  if (!alloc(out.tris, n, error) || !alloc(out.tri_of_global, n, error)) return false;
+  out.bytes += uint64_t(n) * (sizeof(Tri) + 4);
+  if (n == 0) {
+    if (!alloc(out.nodes, 1, error)) return false;
+    k_empty_root<<<1, 1, 0, stream>>>(out.nodes);
+    out.node_count = 1;
+    out.depth = 1;
+    out.leaf_count = 0;
+    out.bytes += sizeof(WideNode);
+    LOTUS_CHECK(hipStreamSynchronize(stream));
+    gpu_ms = 0.0;
+    return true;
+  }
+  Tri* world = nullptr;
+  float4* centroid = nullptr;
+  uint32_t *bounds = nullptr, *keys = nullptr, *vals = nullptr;
+  uint32_t *child = nullptr, *parent = nullptr, *flags = nullptr;
+  uint2* range = nullptr;
+  Box* boxes = nullptr;
+  Task *tasks_a = nullptr, *tasks_b = nullptr;
+  uint32_t* counters = nullptr;  // 0: task count a, 1: task count b, 2: node count, 3: leaves
+  const uint32_t internal = n > 1 ? n - 1 : 1;
+  bool ok = alloc(world, n, error) && alloc(centroid, n, error) && alloc(bounds, 6, error) &&
+            alloc(keys, n, error) && alloc(vals, n, error) && alloc(child, 2ull * internal, error) &&
+            alloc(parent, 2ull * n, error) && alloc(flags, internal, error) && alloc(range, internal, error) &&
+            alloc(boxes, 2ull * n, error) && alloc(tasks_a, n, error) && alloc(tasks_b, n, error) &&
+            alloc(counters, 4, error);
+  auto release = [&]() {
+    (void)hipFree(world);
+    (void)hipFree(centroid);
+    (void)hipFree(bounds);
+    (void)hipFree(keys);
+    (void)hipFree(vals);
+    (void)hipFree(child);
+    (void)hipFree(parent);
+    (void)hipFree(flags);
+    (void)hipFree(range);
+    (void)hipFree(boxes);
+    (void)hipFree(tasks_a);
+    (void)hipFree(tasks_b);
+    (void)hipFree(counters);
+  };
+  if (!ok) {
+    release();
+    return false;
+  }
+  // upper bound of wide nodes: every internal binary node
+  const uint32_t max_nodes = internal + 1;
+  if (!alloc(out.nodes, max_nodes, error)) {
+    release();
+    return false;
+  }
+  const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+  hipError_t e = hipMemcpyAsync(bounds, init, sizeof init, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) {
+    k_flatten<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(in, world, centroid, bounds);
+    k_morton<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(centroid, bounds, n, keys, vals);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    error = std::string("flatten: ") + hipGetErrorString(e);
+    release();
+    return false;
+  }
+  if (!radix_sort_pairs(keys, vals, n, 30, stream, error)) {
+    release();
+    return false;
+  }
+  if (n > 1) {
+    k_karras<<<blocks_for(n - 1, kBlock), kBlock, 0, stream>>>(keys, static_cast<int>(n), child, parent, range);
+    k_fill_u32<<<blocks_for(internal, kBlock), kBlock, 0, stream>>>(flags, 0u, internal);
+  }
+  k_bottom_up<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(world, vals, static_cast<int>(n), child, parent,
+                                                            flags, boxes, out.tris, out.tri_of_global);
+  // collapse, level by level; the root task expands binary node 0
+  const Task root{n > 1 ? 0u : 0u, 0u};
+  const uint32_t init_counts[4] = {1u, 0u, 1u, 0u};
+  e = hipMemcpyAsync(tasks_a, &root, sizeof root, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(counters, init_counts, sizeof init_counts, hipMemcpyHostToDevice, stream);
+  uint32_t pending = 1, depth = 0;
+  Task *cur = tasks_a, *nxt = tasks_b;
+  int cur_slot = 0;
+  while (e == hipSuccess && pending > 0) {
+    ++depth;
+    const int nxt_slot = 1 - cur_slot;
+    k_fill_u32<<<1, 1, 0, stream>>>(counters + nxt_slot, 0u, 1u);
+    k_collapse<<<blocks_for(pending, kBlock), kBlock, 0, stream>>>(
+        cur, counters + cur_slot, nxt, counters + nxt_slot, counters + 2, counters + 3, child, range, boxes,
+        static_cast<int>(n), in.leaf_size, out.nodes);
+    e = hipMemcpyAsync(&pending, counters + nxt_slot, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    std::swap(cur, nxt);
+    cur_slot = nxt_slot;
+  }
+  uint32_t final_counts[4] = {0, 0, 0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(final_counts, counters, sizeof final_counts, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipEventRecord(stop, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    error = std::string("BVH build: ") + hipGetErrorString(e);
+    release();
+    return false;
+  }
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, start, stop);
+  (void)hipEventDestroy(start);
+  (void)hipEventDestroy(stop);
+  gpu_ms = ms;
+  out.node_count = final_counts[2];
+  out.leaf_count = final_counts[3];
+  out.depth = depth;
+  out.bytes += uint64_t(max_nodes) * sizeof(WideNode);
+  release();
+  return true;
+}
+
+}  // namespace lotus

@@ -1,0 +1,79 @@
+// Internal host/device interfaces of liblotus_hip: the resident GPU scene and
+// the launchers each translation unit exports to the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "scene.hpp"
+
+namespace lotus {
+
+// One emissive triangle of the light table (GpuLightRecord, gpu_scene.hpp:120-126).
+struct LightRecord {
+  float pdf;
+  float threshold;
+  uint32_t alias;
+  uint32_t instance;
+};
+
+// Everything the kernels read of the scene, as device pointers.
+struct SceneView {
+  const WideNode* nodes;
+  const Tri* tris;                  // BVH order
+  const uint32_t* tri_of_global;    // flattened id -> BVH order
+  const uint32_t* instance_base;    // first flattened id per instance
+  const uint32_t* instance_material;
+  const float* instance_transform;  // 16 per instance
+  const uint32_t* materials;        // 36 words per material
+  const uint32_t* light_offsets;    // per instance
+  const LightRecord* lights;
+  uint32_t light_count;
+  uint32_t tri_count;
+  uint32_t instance_count;
+  float environment[3];
+};
+
+// Device memory that an apply owns until the next apply.
+struct SceneBuffers {
+  WideNode* nodes = nullptr;
+  Tri* tris = nullptr;
+  uint32_t* tri_of_global = nullptr;
+  uint32_t* instance_base = nullptr;
+  uint32_t* instance_material = nullptr;
+  float* instance_transform = nullptr;
+  uint32_t* materials = nullptr;
+  uint32_t* light_offsets = nullptr;
+  LightRecord* lights = nullptr;
+  uint32_t node_count = 0;
+  uint32_t depth = 0;
+  uint32_t leaf_count = 0;
+  uint64_t bytes = 0;
+};
+
+// Inputs of the GPU build, already on the device.
+struct BuildInput {
+  const float* positions;
+  const uint32_t* triangles;
+  const uint32_t* geometry_vertex_offset;
+  const uint32_t* geometry_triangle_offset;
+  const uint32_t* instance_geometry;
+  const float* instance_transform;
+  const uint32_t* instance_base;  // prefix sum of instance triangle counts, instance_count + 1
+  uint32_t instance_count;
+  uint32_t tri_count;
+  uint32_t leaf_size;
+};
+
+// Builds tris, tri_of_global and the wide BVH into `out` (bvh_build.hip).
+bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, double& gpu_ms,
+               std::string& error);
+
+// Test hook of the sort (bvh_build.hip): stable sort of keys with values.
+bool radix_sort_pairs(uint32_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits,
+                      hipStream_t stream, std::string& error);
+
+}  // namespace lotus

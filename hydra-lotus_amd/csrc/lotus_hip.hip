@@ -1,0 +1,846 @@
+// Host side of liblotus_hip: the C ABI of include/lotus_hip.h. It plays the
+// part of the reference's VulkanOffscreenRenderer for Integrator::Wavefront
+// (backend/vulkan/vulkan_backend.cpp:1092-1399, 2399-2470, 2543-2619,
+// 3745-3856) and of GpuScene::Apply (backend/vulkan/gpu_scene.cpp:1606-2450):
+// scene upload and GPU BVH build, accumulation and restart rules, the
+// wavefront schedule, per-kernel hipEvent timings and evidence.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lotus_hip.h"
+#include "kernels.hpp"
+
+using namespace lotus;
+
+namespace {
+
+void set_detail(char* detail, size_t size, const std::string& s) {
+  if (!detail || size == 0) return;
+  std::strncpy(detail, s.c_str(), size - 1);
+  detail[size - 1] = 0;
+}
+
+template <typename T>
+hipError_t dev_alloc(T*& p, size_t count) {
+  p = nullptr;
+  return hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T));
+}
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// vulkan_internal.hpp:326-336.
+void vulkan_world_to_clip(const float* gl, float* vk) {
+  std::memcpy(vk, gl, 64);
+  for (int c = 0; c < 4; ++c) {
+    const float y = gl[c * 4 + 1], z = gl[c * 4 + 2], w = gl[c * 4 + 3];
+    vk[c * 4 + 1] = -y;
+    vk[c * 4 + 2] = 0.5f * (z + w);
+  }
+}
+
+// vulkan_backend.cpp:436-478: Gauss-Jordan with partial pivoting in double.
+bool invert(const float* m, float* inverse) {
+  double rows[4][8] = {};
+  for (int r = 0; r < 4; ++r) {
+    for (int c = 0; c < 4; ++c) {
+      const float v = m[c * 4 + r];
+      if (!std::isfinite(v)) return false;
+      rows[r][c] = v;
+    }
+    rows[r][r + 4] = 1.0;
+  }
+  for (int c = 0; c < 4; ++c) {
+    int pivot = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (std::fabs(rows[r][c]) > std::fabs(rows[pivot][c])) pivot = r;
+    if (rows[pivot][c] == 0.0) return false;
+    for (int i = 0; i < 8; ++i) std::swap(rows[c][i], rows[pivot][i]);
+    const double scale = rows[c][c];
+    for (double& v : rows[c]) v /= scale;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const double f = rows[r][c];
+      for (int i = 0; i < 8; ++i) rows[r][i] -= f * rows[c][i];
+    }
+  }
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      const float v = static_cast<float>(rows[r][c + 4]);
+      if (!std::isfinite(v)) return false;
+      inverse[c * 4 + r] = v;
+    }
+  return true;
+}
+
+// vulkan_backend.cpp:369-385: bounces whose queue held at least 1/16 of the
+// camera rays run as rounds; the tail follows the rest.
+uint32_t wavefront_rounds(const std::vector<uint32_t>& counts) {
+  const uint32_t camera = counts.empty() ? 0 : counts.front();
+  const uint32_t threshold = std::max(1u, (camera + 15u) / 16u);
+  uint32_t rounds = 1;
+  while (rounds < counts.size() && counts[rounds] >= threshold) ++rounds;
+  return rounds;
+}
+
+// Rounds of a sample with no previous one to go by.
+constexpr uint32_t kFirstRounds = 32;
+
+struct AccumulationKey {  // vulkan_backend.cpp:419-434
+  std::array<float, 16> world_to_clip{};
+  uint32_t width = 0, height = 0;
+  std::array<float, 4> display{};
+  std::array<uint32_t, 4> data{};
+  uint32_t light_sampling = 0, sample_index = 0, max_bounces = 0;
+  uint64_t scene_generation = 0;
+  bool operator==(const AccumulationKey& o) const {
+    return world_to_clip == o.world_to_clip && width == o.width && height == o.height && display == o.display &&
+           data == o.data && light_sampling == o.light_sampling && sample_index == o.sample_index &&
+           max_bounces == o.max_bounces && scene_generation == o.scene_generation;
+  }
+};
+
+enum Timer { kTCamera, kTIntersect, kTShade, kTShadow, kTTail, kTResolve, kTimers };
+
+}  // namespace
+
+struct lotus_hip_renderer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t cus = 0;
+  std::string failure;
+
+  SceneBuffers scene;
+  SceneView view{};
+  bool has_scene = false;
+  uint64_t scene_generation = 0;
+  std::vector<uint32_t> light_offsets_host;
+  std::vector<LightRecord> lights_host;
+
+  uint32_t width = 0, height = 0;
+  Queues q{};
+  uint32_t* counters = nullptr;
+  uint32_t counter_words = 0;
+  unsigned long long* stats = nullptr;  // 8: node visits / triangle tests per traversal kernel
+  float* depth = nullptr;
+  uint2* spill = nullptr;
+  size_t spill_entries = 0;  // per lane
+  uint32_t trace_blocks = 0, shade_blocks = 0;
+
+  AccumulationKey key;
+  bool key_valid = false;
+  uint32_t accumulated = 0;
+  uint32_t rounds_hint = 0;
+
+  std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+void free_target(lotus_hip_renderer* r) {
+  Queues& q = r->q;
+  dev_free(q.thr);
+  dev_free(q.rad);
+  dev_free(q.ray_o);
+  dev_free(q.ray_d);
+  dev_free(q.hit_o);
+  dev_free(q.hit_d);
+  dev_free(q.hit_h);
+  dev_free(q.sh_o);
+  dev_free(q.sh_d);
+  dev_free(q.sh_c);
+  dev_free(q.accum);
+  dev_free(q.color);
+  dev_free(r->depth);
+  r->width = r->height = 0;
+}
+
+void free_scene(lotus_hip_renderer* r) {
+  SceneBuffers& s = r->scene;
+  dev_free(s.nodes);
+  dev_free(s.tris);
+  dev_free(s.tri_of_global);
+  dev_free(s.instance_base);
+  dev_free(s.instance_material);
+  dev_free(s.instance_transform);
+  dev_free(s.materials);
+  dev_free(s.light_offsets);
+  dev_free(s.lights);
+  s = SceneBuffers{};
+  r->has_scene = false;
+}
+
+// Persistent grids and the traversal stacks' spill area, sized for the BVH's depth.
+hipError_t ensure_launch(lotus_hip_renderer* r) {
+  r->trace_blocks = r->cus * 4;
+  r->shade_blocks = r->cus * 4;
+  const size_t need = std::max<size_t>(4, 3 * size_t(r->scene.depth) + 4 - std::min<size_t>(kStackRegs, 3 * size_t(r->scene.depth) + 4));
+  if (r->spill && r->spill_entries >= need) return hipSuccess;
+  dev_free(r->spill);
+  r->spill_entries = need;
+  return dev_alloc(r->spill, need * size_t(r->trace_blocks) * kTraceBlock);
+}
+
+hipError_t ensure_target(lotus_hip_renderer* r, uint32_t width, uint32_t height) {
+  if (r->width == width && r->height == height && r->q.thr) return hipSuccess;
+  free_target(r);
+  const size_t P = size_t(width) * height;
+  Queues& q = r->q;
+  hipError_t e = hipSuccess;
+  for (float4** p : {&q.thr, &q.rad, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.accum,
+                     &q.color})
+    if (e == hipSuccess) e = dev_alloc(*p, P);
+  if (e == hipSuccess) e = dev_alloc(q.hit_h, P);
+  if (e == hipSuccess) e = dev_alloc(r->depth, P);
+  if (e != hipSuccess) {
+    free_target(r);
+    return e;
+  }
+  r->width = width;
+  r->height = height;
+  r->key_valid = false;
+  return hipSuccess;
+}
+
+std::string latch(lotus_hip_renderer* r, const std::string& what, hipError_t e) {
+  r->failure = what + ": " + hipGetErrorString(e);
+  return r->failure;
+}
+
+// gpu_scene.cpp:144-203, 512-609: the light table of emissive triangles,
+// weights = mean emission x world area in double, Vose's alias method.
+void build_light_table(const lotus_hip_scene_plan* p, std::vector<uint32_t>& offsets,
+                       std::vector<LightRecord>& lights) {
+  offsets.assign(p->instance_count, kNoLights);
+  lights.clear();
+  std::vector<double> weights;
+  for (uint32_t i = 0; i < p->instance_count; ++i) {
+    const uint32_t* m = p->materials + 36ull * p->instance_material[i];
+    float e[3];
+    std::memcpy(e, m + 16, 12);
+    double mean = (double(e[0]) + double(e[1]) + double(e[2])) / 3.0;
+    if (!(std::isfinite(mean) && mean > 0.0)) mean = 0.0;
+    const uint32_t geo = p->instance_geometry[i];
+    const uint32_t count = p->geometry_triangle_count[geo];
+    if (mean <= 0.0 || count == 0) continue;
+    offsets[i] = static_cast<uint32_t>(lights.size());
+    const uint32_t* tri = p->triangles + 3ull * p->geometry_triangle_offset[geo];
+    const float* pos = p->positions + 3ull * p->geometry_vertex_offset[geo];
+    const float* t = p->instance_transform + 16ull * i;
+    for (uint32_t k = 0; k < count; ++k) {
+      double c[3][3];
+      for (int v = 0; v < 3; ++v) {
+        const float* q = pos + 3ull * tri[3 * k + v];
+        for (int row = 0; row < 3; ++row)
+          c[v][row] = double(t[row]) * q[0] + double(t[4 + row]) * q[1] + double(t[8 + row]) * q[2] + t[12 + row];
+      }
+      double a[3], b[3];
+      for (int row = 0; row < 3; ++row) {
+        a[row] = c[1][row] - c[0][row];
+        b[row] = c[2][row] - c[0][row];
+      }
+      const double x = a[1] * b[2] - a[2] * b[1], y = a[2] * b[0] - a[0] * b[2], z = a[0] * b[1] - a[1] * b[0];
+      double area = 0.5 * std::sqrt(x * x + y * y + z * z);
+      if (!std::isfinite(area)) area = 0.0;
+      weights.push_back(mean * area);
+      lights.push_back(LightRecord{0.0f, 1.0f, 0u, i});
+    }
+  }
+  double total = 0.0;
+  for (double w : weights) total += w;
+  if (!(total > 0.0) || !std::isfinite(total)) {
+    offsets.assign(p->instance_count, kNoLights);
+    lights.clear();
+    return;
+  }
+  const size_t count = lights.size();
+  std::vector<double> scaled(count);
+  std::vector<uint32_t> small, large;
+  for (size_t i = 0; i < count; ++i) {
+    lights[i].pdf = static_cast<float>(weights[i] / total);
+    scaled[i] = weights[i] / total * static_cast<double>(count);
+    (scaled[i] < 1.0 ? small : large).push_back(static_cast<uint32_t>(i));
+  }
+  while (!small.empty() && !large.empty()) {
+    const uint32_t less = small.back();
+    small.pop_back();
+    const uint32_t more = large.back();
+    lights[less].threshold = static_cast<float>(scaled[less]);
+    lights[less].alias = more;
+    scaled[more] -= 1.0 - scaled[less];
+    if (scaled[more] < 1.0) {
+      large.pop_back();
+      small.push_back(more);
+    }
+  }
+  for (uint32_t i : small) {
+    lights[i].threshold = 1.0f;
+    lights[i].alias = i;
+  }
+  for (uint32_t i : large) {
+    lights[i].threshold = 1.0f;
+    lights[i].alias = i;
+  }
+}
+
+std::string validate_plan(const lotus_hip_scene_plan* p) {
+  if (!p) return "no scene plan";
+  if (p->material_count == 0 || !p->materials) return "the material table is empty";
+  if (p->bvh_leaf_size > 8) return "BVH leaves hold at most 8 triangles";
+  uint64_t tris = 0;
+  for (uint32_t g = 0; g < p->geometry_count; ++g) {
+    const uint32_t vc = p->geometry_vertex_count[g];
+    const uint32_t* tri = p->triangles + 3ull * p->geometry_triangle_offset[g];
+    for (uint64_t k = 0; k < 3ull * p->geometry_triangle_count[g]; ++k)
+      if (tri[k] >= vc) return "geometry " + std::to_string(g) + " indexes a missing vertex";
+    const float* pos = p->positions + 3ull * p->geometry_vertex_offset[g];
+    for (uint64_t k = 0; k < 3ull * vc; ++k)
+      if (!std::isfinite(pos[k])) return "geometry " + std::to_string(g) + " has a non-finite position";
+  }
+  for (uint32_t i = 0; i < p->instance_count; ++i) {
+    if (p->instance_geometry[i] >= p->geometry_count) return "an instance names a missing geometry";
+    if (p->instance_material[i] >= p->material_count) return "an instance names a missing material slot";
+    for (int k = 0; k < 16; ++k)
+      if (!std::isfinite(p->instance_transform[16ull * i + k])) return "an instance transform is not finite";
+    tris += p->geometry_triangle_count[p->instance_geometry[i]];
+  }
+  if (tris >= (1ull << 27)) return "the scene holds more than 2^27 triangles";
+  for (uint32_t m = 0; m < p->material_count; ++m) {
+    const uint32_t* w = p->materials + 36ull * m;
+    if ((w[0] & 0xFFu) != 0) return "material " + std::to_string(m) + ": only UsdPreviewSurface is supported";
+    if (w[2] != 0 || w[3] != 0) return "material " + std::to_string(m) + ": texture inputs are not supported";
+    float opacity;
+    std::memcpy(&opacity, w + 6, 4);
+    if (opacity != 1.0f) return "material " + std::to_string(m) + ": fractional opacity is not supported";
+  }
+  return {};
+}
+
+}  // namespace
+
+extern "C" {
+
+int lotus_hip_probe(char* detail, size_t detail_size) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    set_detail(detail, detail_size, "no HIP device");
+    return LOTUS_HIP_SKIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+    set_detail(detail, detail_size, "hipGetDeviceProperties failed");
+    return LOTUS_HIP_SKIP;
+  }
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_detail(detail, detail_size, std::string("device is ") + prop.gcnArchName + ", the code objects are gfx950");
+    return LOTUS_HIP_SKIP;
+  }
+  set_detail(detail, detail_size, std::string(prop.name) + " (" + prop.gcnArchName + ")");
+  return LOTUS_HIP_PASS;
+}
+
+lotus_hip_renderer* lotus_hip_create(int device, int* status, char* detail, size_t detail_size) {
+  char probe[256];
+  const int p = lotus_hip_probe(probe, sizeof probe);
+  if (p != LOTUS_HIP_PASS) {
+    if (status) *status = p;
+    set_detail(detail, detail_size, probe);
+    return nullptr;
+  }
+  int count = 0;
+  (void)hipGetDeviceCount(&count);
+  if (device < 0 || device >= count) {
+    if (status) *status = LOTUS_HIP_FAIL;
+    set_detail(detail, detail_size, "device " + std::to_string(device) + " does not exist");
+    return nullptr;
+  }
+  auto* r = new lotus_hip_renderer;
+  r->device = device;
+  hipError_t e = hipSetDevice(device);
+  hipDeviceProp_t prop;
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    r->cus = static_cast<uint32_t>(prop.multiProcessorCount);
+    r->events.resize(1024);
+    for (auto& ev : r->events)
+      if (e == hipSuccess) e = hipEventCreate(&ev);
+  }
+  if (e == hipSuccess) e = dev_alloc(r->counters, 4ull * (LOTUS_HIP_MAX_BOUNCES + 4));
+  if (e == hipSuccess) e = dev_alloc(r->stats, 8);
+  r->counter_words = 4u * (LOTUS_HIP_MAX_BOUNCES + 4);
+  if (e != hipSuccess) {
+    if (status) *status = LOTUS_HIP_FAIL;
+    set_detail(detail, detail_size, std::string("device setup: ") + hipGetErrorString(e));
+    lotus_hip_destroy(r);
+    return nullptr;
+  }
+  if (status) *status = LOTUS_HIP_PASS;
+  set_detail(detail, detail_size, probe);
+  return r;
+}
+
+void lotus_hip_destroy(lotus_hip_renderer* r) {
+  if (!r) return;
+  (void)hipSetDevice(r->device);
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  free_target(r);
+  free_scene(r);
+  dev_free(r->counters);
+  dev_free(r->stats);
+  dev_free(r->spill);
+  for (auto& ev : r->events)
+    if (ev) (void)hipEventDestroy(ev);
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+}
+
+int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, lotus_hip_scene_evidence* ev,
+                          char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  const std::string invalid = validate_plan(p);
+  if (!invalid.empty()) {
+    set_detail(detail, detail_size, invalid);
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<uint32_t> base(p->instance_count + 1, 0);
+  for (uint32_t i = 0; i < p->instance_count; ++i)
+    base[i + 1] = base[i] + p->geometry_triangle_count[p->instance_geometry[i]];
+  const uint32_t tri_count = base[p->instance_count];
+  uint64_t vertex_words = 0, index_words = 0;
+  for (uint32_t g = 0; g < p->geometry_count; ++g) {
+    vertex_words = std::max<uint64_t>(vertex_words, 3ull * (uint64_t(p->geometry_vertex_offset[g]) + p->geometry_vertex_count[g]));
+    index_words = std::max<uint64_t>(index_words, 3ull * (uint64_t(p->geometry_triangle_offset[g]) + p->geometry_triangle_count[g]));
+  }
+  std::vector<uint32_t> offsets;
+  std::vector<LightRecord> lights;
+  build_light_table(p, offsets, lights);
+
+  free_scene(r);
+  SceneBuffers& s = r->scene;
+  float* positions = nullptr;
+  uint32_t *triangles = nullptr, *gvo = nullptr, *gto = nullptr, *igeo = nullptr;
+  hipError_t e = dev_alloc(positions, vertex_words);
+  if (e == hipSuccess) e = dev_alloc(triangles, index_words);
+  if (e == hipSuccess) e = dev_alloc(gvo, p->geometry_count);
+  if (e == hipSuccess) e = dev_alloc(gto, p->geometry_count);
+  if (e == hipSuccess) e = dev_alloc(igeo, p->instance_count);
+  if (e == hipSuccess) e = dev_alloc(s.instance_base, p->instance_count + 1);
+  if (e == hipSuccess) e = dev_alloc(s.instance_material, p->instance_count);
+  if (e == hipSuccess) e = dev_alloc(s.instance_transform, 16ull * p->instance_count);
+  if (e == hipSuccess) e = dev_alloc(s.materials, 36ull * p->material_count);
+  if (e == hipSuccess) e = dev_alloc(s.light_offsets, p->instance_count);
+  if (e == hipSuccess) e = dev_alloc(s.lights, lights.size());
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, r->stream);
+  };
+  up(positions, p->positions, vertex_words * 4);
+  up(triangles, p->triangles, index_words * 4);
+  up(gvo, p->geometry_vertex_offset, p->geometry_count * 4ull);
+  up(gto, p->geometry_triangle_offset, p->geometry_count * 4ull);
+  up(igeo, p->instance_geometry, p->instance_count * 4ull);
+  up(s.instance_base, base.data(), base.size() * 4);
+  up(s.instance_material, p->instance_material, p->instance_count * 4ull);
+  up(s.instance_transform, p->instance_transform, 64ull * p->instance_count);
+  up(s.materials, p->materials, 144ull * p->material_count);
+  up(s.light_offsets, offsets.data(), offsets.size() * 4);
+  up(s.lights, lights.data(), lights.size() * sizeof(LightRecord));
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  const auto t1 = std::chrono::steady_clock::now();
+  std::string error;
+  double build_ms = 0.0;
+  bool ok = e == hipSuccess;
+  if (ok) {
+    BuildInput in{positions, triangles, gvo, gto, igeo, s.instance_transform, s.instance_base,
+                  p->instance_count, tri_count, p->bvh_leaf_size ? p->bvh_leaf_size : 4u};
+    ok = build_bvh(in, s, r->stream, build_ms, error);
+  } else {
+    error = std::string("scene upload: ") + hipGetErrorString(e);
+  }
+  dev_free(positions);
+  dev_free(triangles);
+  dev_free(gvo);
+  dev_free(gto);
+  dev_free(igeo);
+  if (ok) {
+    e = ensure_launch(r);
+    if (e != hipSuccess) {
+      ok = false;
+      error = std::string("spill area: ") + hipGetErrorString(e);
+    }
+  }
+  if (!ok) {
+    free_scene(r);
+    r->failure = error;
+    set_detail(detail, detail_size, error);
+    return LOTUS_HIP_FAIL;
+  }
+  s.bytes += vertex_words * 0 + (p->instance_count + 1) * 4ull + p->instance_count * 68ull + 144ull * p->material_count +
+             p->instance_count * 4ull + lights.size() * sizeof(LightRecord);
+  SceneView& v = r->view;
+  v.nodes = s.nodes;
+  v.tris = s.tris;
+  v.tri_of_global = s.tri_of_global;
+  v.instance_base = s.instance_base;
+  v.instance_material = s.instance_material;
+  v.instance_transform = s.instance_transform;
+  v.materials = s.materials;
+  v.light_offsets = s.light_offsets;
+  v.lights = s.lights;
+  v.light_count = static_cast<uint32_t>(lights.size());
+  v.tri_count = tri_count;
+  v.instance_count = p->instance_count;
+  for (int c = 0; c < 3; ++c) v.environment[c] = p->environment[c];
+  r->light_offsets_host = offsets;
+  r->lights_host = lights;
+  r->has_scene = true;
+  ++r->scene_generation;
+  r->rounds_hint = 0;
+  if (ev) {
+    ev->triangle_count = tri_count;
+    ev->instance_count = p->instance_count;
+    ev->material_count = p->material_count;
+    ev->light_count = static_cast<uint32_t>(lights.size());
+    ev->node_count = s.node_count;
+    ev->bvh_depth = s.depth;
+    ev->leaf_count = s.leaf_count;
+    ev->bvh_width = kWidth;
+    ev->upload_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ev->build_gpu_ms = build_ms;
+    ev->device_bytes = s.bytes;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res,
+                     char* detail, size_t detail_size) {
+  if (!r || !P) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  // Settings errors fail without latching (vulkan_backend.cpp:1034-1090).
+  if (P->output > LOTUS_HIP_OUTPUT_ROUGHNESS_METALLIC) {
+    set_detail(detail, detail_size, "unknown scene output");
+    return LOTUS_HIP_FAIL;
+  }
+  if (P->output == LOTUS_HIP_OUTPUT_RADIANCE && P->max_bounces > LOTUS_HIP_MAX_BOUNCES) {
+    set_detail(detail, detail_size, "the wavefront integrator traces at most 1024 bounces");
+    return LOTUS_HIP_FAIL;
+  }
+  if (P->light_sampling > LOTUS_HIP_LIGHT_MIS) {
+    set_detail(detail, detail_size, "unknown light sampling");
+    return LOTUS_HIP_FAIL;
+  }
+  if (P->width == 0 || P->height == 0) {
+    set_detail(detail, detail_size, "the target is empty");
+    return LOTUS_HIP_FAIL;
+  }
+  if (!r->has_scene) {
+    set_detail(detail, detail_size, "no scene has been applied");
+    return LOTUS_HIP_FAIL;
+  }
+  FrameConst F{};
+  vulkan_world_to_clip(P->world_to_clip, F.world_to_clip);
+  if (!invert(F.world_to_clip, F.clip_to_world)) {
+    set_detail(detail, detail_size, "ray-query camera is singular or non-finite");
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  // DisplayViewport / DataScissor: empty windows mean the whole image.
+  const bool whole_display = P->display_window[2] == 0.0f || P->display_window[3] == 0.0f;
+  const float vx = whole_display ? 0.0f : P->display_window[0], vy = whole_display ? 0.0f : P->display_window[1];
+  const float vw = whole_display ? static_cast<float>(P->width) : P->display_window[2];
+  const float vh = whole_display ? static_cast<float>(P->height) : P->display_window[3];
+  F.ndc_from_pixel[0] = 2.0f / vw;
+  F.ndc_from_pixel[1] = 2.0f / vh;
+  F.ndc_from_pixel[2] = -2.0f * vx / vw - 1.0f;
+  F.ndc_from_pixel[3] = -2.0f * vy / vh - 1.0f;
+  uint32_t rect[4] = {0, 0, P->width, P->height};
+  if (P->data_window[2] != 0 && P->data_window[3] != 0) {
+    const uint32_t x0 = std::min(P->data_window[0], P->width), y0 = std::min(P->data_window[1], P->height);
+    rect[0] = x0;
+    rect[1] = y0;
+    rect[2] = std::min(P->data_window[2], P->width - x0);
+    rect[3] = std::min(P->data_window[3], P->height - y0);
+  }
+  std::memcpy(F.rect, rect, sizeof rect);
+  std::memcpy(F.background, P->clear_color, 16);
+  F.width = P->width;
+  F.height = P->height;
+  F.max_bounces = P->max_bounces;
+  F.light_sampling = P->light_sampling;
+  F.output = P->output;
+
+  hipError_t e = ensure_target(r, P->width, P->height);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "target allocation", e));
+    return LOTUS_HIP_FAIL;
+  }
+  const uint32_t npix = P->width * P->height;
+  Launch L{r->stream, r->trace_blocks, r->shade_blocks, r->spill, r->stats};
+  launch_zero(reinterpret_cast<uint32_t*>(r->stats), 16, r->stream);
+  launch_fill4(r->q.color, npix, make_float4(P->clear_color[0], P->clear_color[1], P->clear_color[2], P->clear_color[3]),
+               r->stream);
+  if (res && res->depth) launch_fill1(r->depth, npix, 1.0f, r->stream);
+
+  double t_ms[kTimers] = {0, 0, 0, 0, 0, 0};
+  double frame_ms = 0.0;
+  uint32_t added = 0;
+  std::vector<uint32_t> counts;
+  uint64_t camera_rays = 0, extension_rays = 0, shadow_rays = 0;
+  uint32_t rounds_used = 0;
+  if (P->output == LOTUS_HIP_OUTPUT_RADIANCE) {
+    AccumulationKey key;
+    std::memcpy(key.world_to_clip.data(), F.world_to_clip, 64);
+    key.width = P->width;
+    key.height = P->height;
+    std::memcpy(key.display.data(), P->display_window, 16);
+    std::memcpy(key.data.data(), P->data_window, 16);
+    key.light_sampling = P->light_sampling;
+    key.sample_index = P->sample_index;
+    key.max_bounces = P->max_bounces;
+    key.scene_generation = r->scene_generation;
+    if (!r->key_valid || !(r->key == key)) {
+      r->key = key;
+      r->key_valid = true;
+      r->accumulated = 0;
+    }
+    const uint32_t room = P->max_samples == 0 ? P->frame_count
+                                              : P->max_samples - std::min(P->max_samples, r->accumulated);
+    const uint32_t to_add = std::min(P->frame_count, room);
+    const uint32_t submissions = std::max(to_add, 1u);
+    const uint32_t bounces = P->max_bounces + 1;
+    const uint32_t stride = P->max_bounces + 2;
+    for (uint32_t sample = 0; sample < submissions && e == hipSuccess; ++sample) {
+      const uint32_t add = sample < to_add ? 1u : 0u;
+      F.sample_index = P->sample_index + r->accumulated;
+      F.accumulated = r->accumulated;
+      uint32_t rounds = r->rounds_hint == 0 ? std::min(bounces, kFirstRounds) : std::min(r->rounds_hint, bounces);
+      rounds = std::max(rounds, 1u);
+      size_t ev = 0;
+      std::vector<std::pair<int, size_t>> pairs;
+      auto mark = [&](int timer) {
+        pairs.emplace_back(timer, ev);
+        (void)hipEventRecord(r->events[ev++], r->stream);
+      };
+      auto done = [&]() { (void)hipEventRecord(r->events[ev++], r->stream); };
+      launch_zero(r->counters, 4 * stride, r->stream);
+      if (add) {
+        mark(kTCamera);
+        launch_camera(L, r->view, F, r->q, r->counters);
+        done();
+        for (uint32_t b = 0; b < rounds; ++b) {
+          if (b >= 1) {
+            mark(kTIntersect);
+            launch_intersect(L, r->view, F, r->q, r->counters, b);
+            done();
+          }
+          mark(kTShade);
+          launch_shade(L, r->view, F, r->q, r->counters, b);
+          done();
+          if (P->light_sampling != 0) {
+            mark(kTShadow);
+            launch_shadow(L, r->view, r->q, r->counters, b);
+            done();
+          }
+        }
+        if (rounds < bounces) {
+          mark(kTTail);
+          launch_tail(L, r->view, F, r->q, r->counters, rounds);
+          done();
+        }
+      }
+      mark(kTResolve);
+      launch_resolve(L, F, r->q, add);
+      done();
+      std::vector<uint32_t> host(4 * stride);
+      e = hipGetLastError();
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(host.data(), r->counters, host.size() * 4, hipMemcpyDeviceToHost, r->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+      if (e != hipSuccess) break;
+      for (const auto& [timer, first] : pairs) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, r->events[first], r->events[first + 1]);
+        t_ms[timer] += ms;
+      }
+      if (ev >= 2) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, r->events[0], r->events[ev - 1]);
+        frame_ms += ms;
+      }
+      if (add) {
+        counts.assign(stride, 0u);
+        size_t last = 0;
+        for (uint32_t b = 0; b < stride; ++b) {
+          counts[b] = host[4 * b];
+          if (counts[b]) last = b;
+          if (b >= 1) extension_rays += host[4 * b];
+          shadow_rays += host[4 * b + 2];
+        }
+        camera_rays += host[0];
+        counts.resize(host[0] ? last + 1 : 0);
+        r->rounds_hint = wavefront_rounds(counts);
+        rounds_used = rounds;
+        ++r->accumulated;
+        ++added;
+      }
+    }
+    if (e != hipSuccess) {
+      set_detail(detail, detail_size, latch(r, "wavefront frame", e));
+      return LOTUS_HIP_FAIL;
+    }
+  }
+  // The pixel-centre camera pass: depth, and the colour of the other outputs.
+  const bool diag = P->output != LOTUS_HIP_OUTPUT_RADIANCE;
+  if (diag || (res && res->depth)) {
+    F.sample_index = 0;
+    launch_camera_pass(L, r->view, F, r->q, (res && res->depth) ? r->depth : nullptr, diag ? 1u : 0u);
+  }
+  if (res) {
+    if (e == hipSuccess && res->color)
+      e = hipMemcpyAsync(res->color, r->q.color, npix * 16ull, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess && res->depth)
+      e = hipMemcpyAsync(res->depth, r->depth, npix * 4ull, hipMemcpyDeviceToHost, r->stream);
+  }
+  unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(stats, r->stats, sizeof stats, hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "frame readback", e));
+    return LOTUS_HIP_FAIL;
+  }
+  if (res) {
+    res->path_counts_size = static_cast<uint32_t>(std::min<size_t>(counts.size(), res->path_counts_capacity));
+    if (res->path_counts)
+      std::memcpy(res->path_counts, counts.data(), res->path_counts_size * 4ull);
+    res->wavefront_rounds = counts.empty() ? 0 : rounds_used;
+    res->samples_per_pixel = P->output == LOTUS_HIP_OUTPUT_RADIANCE ? r->accumulated : 0;
+    res->frames_rendered = std::max(added, 1u);
+    res->camera_rays = camera_rays;
+    res->extension_rays = extension_rays;
+    res->shadow_rays = shadow_rays;
+    res->camera_gpu_ms = t_ms[kTCamera];
+    res->intersect_gpu_ms = t_ms[kTIntersect] + t_ms[kTTail];
+    res->shade_gpu_ms = t_ms[kTShade];
+    res->shadow_gpu_ms = t_ms[kTShadow];
+    res->resolve_gpu_ms = t_ms[kTResolve];
+    res->frame_gpu_ms = frame_ms;
+    for (int k = 0; k < 4; ++k) {
+      res->traversal_nodes[k] = stats[2 * k];
+      res->traversal_tris[k] = stats[2 * k + 1];
+    }
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_trace_device(lotus_hip_renderer* r, uint32_t count, const void* rays, void* hits, int any_hit,
+                           void* stream, char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (!r->has_scene) {
+    set_detail(detail, detail_size, "no scene has been applied");
+    return LOTUS_HIP_FAIL;
+  }
+  if (count == 0) return LOTUS_HIP_PASS;
+  if (!rays || !hits) {
+    set_detail(detail, detail_size, "null ray or hit buffer");
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  Launch L{stream ? static_cast<hipStream_t>(stream) : r->stream, r->trace_blocks, r->shade_blocks, r->spill,
+           r->stats};
+  launch_trace(L, r->view, static_cast<const float*>(rays), static_cast<uint32_t*>(hits), count,
+               any_hit ? 1u : 0u);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "trace launch", e));
+    return LOTUS_HIP_FAIL;
+  }
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_trace(lotus_hip_renderer* r, uint32_t count, const float* rays, uint32_t* hits, int any_hit,
+                    char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (count == 0) return LOTUS_HIP_PASS;
+  (void)hipSetDevice(r->device);
+  float* d_rays = nullptr;
+  uint32_t* d_hits = nullptr;
+  const size_t out_words = any_hit ? count : 5ull * count;
+  hipError_t e = dev_alloc(d_rays, 8ull * count);
+  if (e == hipSuccess) e = dev_alloc(d_hits, out_words);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_rays, rays, 32ull * count, hipMemcpyHostToDevice, r->stream);
+  if (e == hipSuccess) {
+    const int s = lotus_hip_trace_device(r, count, d_rays, d_hits, any_hit, r->stream, detail, detail_size);
+    if (s != LOTUS_HIP_PASS) {
+      dev_free(d_rays);
+      dev_free(d_hits);
+      return s;
+    }
+    e = hipMemcpyAsync(hits, d_hits, out_words * 4, hipMemcpyDeviceToHost, r->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  dev_free(d_rays);
+  dev_free(d_hits);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "trace", e));
+    return LOTUS_HIP_FAIL;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_light_table(lotus_hip_renderer* r, uint32_t* offsets, uint32_t* records, uint32_t* count,
+                          char* detail, size_t detail_size) {
+  if (!r || !r->has_scene) {
+    set_detail(detail, detail_size, "no scene has been applied");
+    return LOTUS_HIP_FAIL;
+  }
+  if (count) *count = static_cast<uint32_t>(r->lights_host.size());
+  if (offsets) std::memcpy(offsets, r->light_offsets_host.data(), r->light_offsets_host.size() * 4);
+  if (records) std::memcpy(records, r->lights_host.data(), r->lights_host.size() * sizeof(LightRecord));
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_read_accumulation(lotus_hip_renderer* r, float* sums, uint32_t* samples, char* detail,
+                                size_t detail_size) {
+  if (!r || !r->q.accum) {
+    set_detail(detail, detail_size, "no frame has been rendered");
+    return LOTUS_HIP_FAIL;
+  }
+  if (samples) *samples = r->accumulated;
+  if (sums) {
+    const hipError_t e = hipMemcpy(sums, r->q.accum, size_t(r->width) * r->height * 16, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      set_detail(detail, detail_size, latch(r, "accumulation readback", e));
+      return LOTUS_HIP_FAIL;
+    }
+  }
+  return LOTUS_HIP_PASS;
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+// One path step on the device: hit reconstruction, emission with MIS,
+// Russian roulette, next event estimation and BSDF sampling. Behaviour follows
+// backend/vulkan/shaders/common/transport.slang:164-247 and the scene access
+// functions of common/scene.slang it calls (cited per function).
+#pragma once
+
+#include "bsdf.hpp"
+#include "internal.hpp"
+
+namespace lotus {
+
+constexpr uint32_t kRouletteBounces = 3;  // transport.slang:222
+constexpr float kMaxSurvival = 0.95f;     // transport.slang:223
+
+// What every kernel of a frame reads besides the scene.
+struct FrameConst {
+  float clip_to_world[16];
+  float world_to_clip[16];  // Vulkan clip space, for depth
+  float ndc_from_pixel[4];
+  float background[4];
+  uint32_t width, height;
+  uint32_t rect[4];
+  uint32_t sample_index;
+  uint32_t max_bounces;
+  uint32_t light_sampling;
+  uint32_t accumulated;  // samples already in the accumulation
+  uint32_t output;
+};
+
+// The path state a kernel holds in registers.
+struct PathRegs {
+  V3 o, d;          // current ray
+  V3 throughput;
+  V3 radiance;
+  uint32_t rng;
+  float last_pdf;   // < 0: the last direction was delta or transmitted
+  uint32_t depth;
+};
+
+struct Surface {
+  V3 position, normal, shading_normal;
+  float area;
+  V3 base_color;
+  float roughness, metallic;
+  Closure bsdf;
+  V3 emission;
+};
+
+__device__ __forceinline__ float word_float(const uint32_t* w, int i) { return __uint_as_float(w[i]); }
+
+// scene.slang:273-283.
+__device__ __forceinline__ Ray camera_ray(const float* clip_to_world, float nx, float ny) {
+  float nh[4], fh[4];
+  mul_point(clip_to_world, nx, ny, 0.0f, 1.0f, nh);
+  mul_point(clip_to_world, nx, ny, 1.0f, 1.0f, fh);
+  Ray r;
+  r.o = V3{nh[0] / nh[3], nh[1] / nh[3], nh[2] / nh[3]};
+  r.d = normalize3(V3{fh[0], fh[1], fh[2]} * nh[3] - V3{nh[0], nh[1], nh[2]} * fh[3]);
+  r.tmin = 0.0f;
+  r.tmax = fh[3] == 0.0f ? kFloatMax : length3(V3{fh[0] / fh[3], fh[1] / fh[3], fh[2] / fh[3]} - r.o);
+  return r;
+}
+
+// transport.slang:254-270 (StartPath): box-filtered film position.
+__device__ __forceinline__ Ray start_path(const FrameConst& f, uint32_t x, uint32_t y, uint32_t sample,
+                                          uint32_t& rng) {
+  rng = seed_random(y * f.width + x, sample);
+  const float ux = next_random(rng), uy = next_random(rng);
+  const float fx = (static_cast<float>(x) + 0.5f) + (ux - 0.5f);
+  const float fy = (static_cast<float>(y) + 0.5f) + (uy - 0.5f);
+  return camera_ray(f.clip_to_world, fx * f.ndc_from_pixel[0] + f.ndc_from_pixel[2],
+                    fy * f.ndc_from_pixel[1] + f.ndc_from_pixel[3]);
+}
+
+// scene.slang:1408-1449 with EvaluatePreview (:1184-1260) for constant
+// UsdPreviewSurface inputs and MappedNormal (:539-553) for a constant
+// tangent-space normal.
+__device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t tri, uint32_t instance,
+                                                   float u, float v, V3 dir) {
+  const Tri t = S.tris[tri];
+  const V3 p0 = xyz(t.a), p1 = xyz(t.b), p2 = xyz(t.c);
+  Surface sf;
+  sf.position = (p0 + u * (p1 - p0)) + v * (p2 - p0);
+  const V3 e = cross3(p1 - p0, p2 - p0);
+  const V3 normal = normalize3(e);
+  sf.area = 0.5f * length3(e);
+  sf.normal = dot3(normal, dir) > 0.0f ? -normal : normal;
+  sf.shading_normal = sf.normal;
+  const uint32_t* m = S.materials + 36ull * S.instance_material[instance];
+  const uint32_t flags = m[0] >> 8;
+  const V3 tn = V3{word_float(m, 8), word_float(m, 9), word_float(m, 10)};
+  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f)) {
+    // MakeShadingFrame (sampling.slang:45-56) then MappedNormal.
+    const V3 n = sf.shading_normal;
+    const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    const V3 bt = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    const V3 bb = V3{b, sign + n.y * n.y * a, -n.y};
+    const float mag = maxf(maxf(fabsf(tn.x), fabsf(tn.y)), fabsf(tn.z));
+    if (mag > 0.0f && finite3(tn)) {
+      const V3 q = tn / mag;
+      V3 sh = normalize3(((q.x * bt) + (q.y * bb)) + (q.z * n));
+      if (dot3(sh, sf.normal) < 0.0f) sh = -sh;
+      sf.shading_normal = dot3(sh, dir) < 0.0f ? sh : n;
+    }
+  }
+  sf.base_color = V3{word_float(m, 12), word_float(m, 13), word_float(m, 14)};
+  sf.roughness = word_float(m, 15);
+  sf.emission = V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)};
+  sf.metallic = word_float(m, 19);
+  const V3 specular = V3{word_float(m, 20), word_float(m, 21), word_float(m, 22)};
+  const float ior = word_float(m, 23);
+  const V3 tcolor = V3{word_float(m, 24), word_float(m, 25), word_float(m, 26)};
+  const float transmission = word_float(m, 27);
+  bool entering = true;
+  if (transmission > 0.0f) {
+    const float* x = S.instance_transform + 16ull * instance;
+    float c0[4], c1[4], c2[4];
+    mul_point(x, 1.0f, 0.0f, 0.0f, 0.0f, c0);
+    mul_point(x, 0.0f, 1.0f, 0.0f, 0.0f, c1);
+    mul_point(x, 0.0f, 0.0f, 1.0f, 0.0f, c2);
+    const V3 outward = dot3(V3{c0[0], c0[1], c0[2]}, cross3(V3{c1[0], c1[1], c1[2]}, V3{c2[0], c2[1], c2[2]})) < 0.0f
+                           ? -normal
+                           : normal;
+    entering = dot3(outward, dir) < 0.0f;
+  }
+  sf.bsdf = make_preview_closure(sf.base_color, sf.roughness, sf.metallic, ior, (flags & 1u) != 0, specular,
+                                 transmission, tcolor, (flags & 2u) != 0, entering);
+  return sf;
+}
+
+struct LightSample {
+  V3 direction;
+  float distance;
+  V3 radiance;
+  float pdf;
+};
+
+// scene.slang:894-909 (SampleLight) with SampleTriangle (:632-677).
+__device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux, float uy, float uz, float uw,
+                                                    V3 origin) {
+  LightSample none{V3{0.0f, 0.0f, 1.0f}, 0.0f, V3{0.0f, 0.0f, 0.0f}, 0.0f};
+  const uint32_t count = S.light_count;
+  const uint32_t c = static_cast<uint32_t>(ux * static_cast<float>(count));
+  const uint32_t column = c < count - 1 ? c : count - 1;
+  const LightRecord rec = S.lights[column];
+  const uint32_t light = uy < rec.threshold ? column : rec.alias;
+  const LightRecord chosen = S.lights[light];
+  const uint32_t prim = light - S.light_offsets[chosen.instance];
+  const float root = sqrtf(uz);
+  const float bu = root * (1.0f - uw), bv = root * uw;
+  const uint32_t g = S.instance_base[chosen.instance] + prim;
+  const Tri t = S.tris[S.tri_of_global[g]];
+  const V3 p0 = xyz(t.a), p1 = xyz(t.b), p2 = xyz(t.c);
+  const uint32_t* m = S.materials + 36ull * S.instance_material[chosen.instance];
+  const V3 position = (p0 + bu * (p1 - p0)) + bv * (p2 - p0);
+  const V3 e = cross3(p1 - p0, p2 - p0);
+  const V3 normal = normalize3(e);
+  const float area = 0.5f * length3(e);
+  const V3 radiance = V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)} * 1.0f;
+  const V3 toward = position - origin;
+  const float d2 = dot3(toward, toward);
+  if (!(area > 0.0f) || !(d2 > 0.0f) || max3(radiance) <= 0.0f) return none;
+  LightSample ls;
+  ls.direction = toward * rsqrt_rn(d2);
+  const float cosine = fabsf(dot3(normal, ls.direction));
+  if (!(cosine > 0.0f)) return none;
+  const V3 end = offset_ray_origin(position, dot3(normal, ls.direction) > 0.0f ? -normal : normal);
+  ls.distance = maxf(0.0f, dot3(end - origin, ls.direction));
+  ls.radiance = radiance;
+  ls.pdf = chosen.pdf / area * d2 / cosine;
+  return ls;
+}
+
+// scene.slang:1004-1010.
+__device__ __forceinline__ float emitter_pdf(const SceneView& S, uint32_t instance, uint32_t primitive, float area) {
+  if (S.light_count == 0) return 0.0f;
+  const uint32_t first = S.light_offsets[instance];
+  if (first == kNoLights || !(area > 0.0f)) return 0.0f;
+  return S.lights[first + primitive].pdf / area;
+}
+
+// sampling.slang:67-76.
+__device__ __forceinline__ float power_heuristic(float chosen, float other) {
+  if (!(chosen > 0.0f)) return 0.0f;
+  if (isinf(chosen)) return isinf(other) ? 0.5f : 1.0f;
+  const float ratio = other / chosen;
+  return 1.0f / (1.0f + ratio * ratio);
+}
+
+// The outcome of one scattering event.
+struct Scatter {
+  bool continues;      // path.o/d hold the continuation ray
+  bool shadow;         // a shadow ray carries `delta` if it reaches the light
+  Ray shadow_ray;
+  V3 delta;            // throughput * contribution, added when unoccluded
+};
+
+// transport.slang:365-431 (ScatterPath) with DirectLight (:324-336) and
+// ShadowedLight (:283-316) up to the shadow ray, which the caller traces.
+__device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameConst& F, PathRegs& p,
+                                                uint32_t tri, uint32_t instance, uint32_t primitive, float u,
+                                                float v) {
+  Scatter out;
+  out.continues = false;
+  out.shadow = false;
+  const bool sample_lights = F.light_sampling != 0;
+  const Surface sf = reconstruct_hit(S, tri, instance, u, v, p.d);
+  V3 emission = sf.emission;
+  if (sample_lights && !(p.last_pdf < 0.0f) && p.depth != 0 && max3(emission) > 0.0f) {
+    float w = 1.0f;
+    const float area_pdf = emitter_pdf(S, instance, primitive, sf.area);
+    if (area_pdf > 0.0f) {
+      const V3 toward = sf.position - p.o;
+      const float cosine = fabsf(dot3(sf.normal, p.d));
+      w = power_heuristic(p.last_pdf, area_pdf * dot3(toward, toward) / cosine);
+    }
+    emission = emission * w;
+  }
+  p.radiance = p.radiance + p.throughput * emission;
+  if (p.depth >= F.max_bounces) return out;
+  if (p.depth >= kRouletteBounces) {
+    const float survival = minf(max3(p.throughput), kMaxSurvival);
+    if (next_random(p.rng) >= survival) return out;
+    p.throughput = p.throughput / survival;
+  }
+  // MakeShadingFrame (sampling.slang:45-56)
+  const V3 n = sf.shading_normal;
+  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  const V3 ft = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+  const V3 fb = V3{b, sign + n.y * n.y * a, -n.y};
+  const V3 md = -p.d;
+  const V3 wo = V3{dot3(md, ft), dot3(md, fb), dot3(md, n)};
+  if (sample_lights && S.light_count != 0) {
+    const float ux = next_random(p.rng), uy = next_random(p.rng), uz = next_random(p.rng),
+                uw = next_random(p.rng);
+    const V3 origin = offset_ray_origin(sf.position, sf.normal);
+    const LightSample ls = sample_light(S, ux, uy, uz, uw, origin);
+    if ((ls.pdf > 0.0f) && !(max3(ls.radiance) <= 0.0f) && !(dot3(ls.direction, sf.normal) <= 0.0f)) {
+      const V3 wi = V3{dot3(ls.direction, ft), dot3(ls.direction, fb), dot3(ls.direction, n)};
+      if (!(wi.z <= 0.0f)) {
+        const V3 f = bsdf_evaluate(sf.bsdf, wo, wi);
+        if (!(max3(f) <= 0.0f)) {
+          const float weight = power_heuristic(ls.pdf, bsdf_pdf(sf.bsdf, wo, wi));
+          const V3 c = f * ls.radiance * (1.0f * weight / ls.pdf);
+          out.shadow = true;
+          out.shadow_ray = Ray{origin, 0.0f, ls.direction, ls.distance};
+          out.delta = p.throughput * (finite3(c) ? c : V3{0.0f, 0.0f, 0.0f});
+        }
+      }
+    }
+  }
+  const float ux = next_random(p.rng), uy = next_random(p.rng), uz = next_random(p.rng);
+  BsdfSample s;
+  if (!bsdf_sample(sf.bsdf, wo, ux, uy, uz, s)) return out;
+  const V3 wi = normalize3(((s.wi.x * ft) + (s.wi.y * fb)) + (s.wi.z * n));
+  const float side = dot3(wi, sf.normal);
+  if (s.transmitted ? side >= 0.0f : side <= 0.0f) return out;
+  p.throughput = p.throughput * s.weight;
+  p.last_pdf = (s.delta || s.transmitted) ? -1.0f : s.pdf;
+  p.o = offset_ray_origin(sf.position, s.transmitted ? -sf.normal : sf.normal);
+  p.d = wi;
+  ++p.depth;
+  out.continues = true;
+  return out;
+}
+
+}  // namespace lotus

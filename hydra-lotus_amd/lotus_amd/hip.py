@@ -1,0 +1,286 @@
+"""ctypes binding of liblotus_hip.so (include/lotus_hip.h).
+
+The library is the product; this module only marshals arrays through its
+C ABI. Loading fails loudly when the library has not been built: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "liblotus_hip.so")
+
+PASS, FAIL, SKIP = 0, 1, 77
+MISS = 0xFFFFFFFF
+OUTPUT_RADIANCE, OUTPUT_BARYCENTRICS, OUTPUT_SHADING_NORMAL, OUTPUT_ALBEDO, OUTPUT_ROUGHNESS_METALLIC = range(5)
+LIGHT_BSDF, LIGHT_MIS = 0, 1
+
+SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hip_scene_apply",
+           "lotus_hip_render", "lotus_hip_trace", "lotus_hip_trace_device", "lotus_hip_light_table",
+           "lotus_hip_read_accumulation"]
+
+_u32p = C.POINTER(C.c_uint32)
+_f32p = C.POINTER(C.c_float)
+
+
+class ScenePlan(C.Structure):
+    _fields_ = [
+        ("geometry_count", C.c_uint32),
+        ("positions", _f32p),
+        ("triangles", _u32p),
+        ("geometry_vertex_offset", _u32p),
+        ("geometry_vertex_count", _u32p),
+        ("geometry_triangle_offset", _u32p),
+        ("geometry_triangle_count", _u32p),
+        ("instance_count", C.c_uint32),
+        ("instance_geometry", _u32p),
+        ("instance_transform", _f32p),
+        ("instance_material", _u32p),
+        ("material_count", C.c_uint32),
+        ("materials", _u32p),
+        ("environment", C.c_float * 3),
+        ("bvh_leaf_size", C.c_uint32),
+    ]
+
+
+class SceneEvidence(C.Structure):
+    _fields_ = [
+        ("triangle_count", C.c_uint32),
+        ("instance_count", C.c_uint32),
+        ("material_count", C.c_uint32),
+        ("light_count", C.c_uint32),
+        ("node_count", C.c_uint32),
+        ("bvh_depth", C.c_uint32),
+        ("leaf_count", C.c_uint32),
+        ("bvh_width", C.c_uint32),
+        ("upload_ms", C.c_double),
+        ("build_gpu_ms", C.c_double),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [
+        ("world_to_clip", C.c_float * 16),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("display_window", C.c_float * 4),
+        ("data_window", C.c_uint32 * 4),
+        ("clear_color", C.c_float * 4),
+        ("frame_count", C.c_uint32),
+        ("output", C.c_uint32),
+        ("light_sampling", C.c_uint32),
+        ("sample_index", C.c_uint32),
+        ("max_bounces", C.c_uint32),
+        ("max_samples", C.c_uint32),
+    ]
+
+
+class FrameResult(C.Structure):
+    _fields_ = [
+        ("color", _f32p),
+        ("depth", _f32p),
+        ("path_counts", _u32p),
+        ("path_counts_capacity", C.c_uint32),
+        ("path_counts_size", C.c_uint32),
+        ("wavefront_rounds", C.c_uint32),
+        ("samples_per_pixel", C.c_uint32),
+        ("frames_rendered", C.c_uint32),
+        ("camera_rays", C.c_uint64),
+        ("extension_rays", C.c_uint64),
+        ("shadow_rays", C.c_uint64),
+        ("camera_gpu_ms", C.c_double),
+        ("intersect_gpu_ms", C.c_double),
+        ("shade_gpu_ms", C.c_double),
+        ("shadow_gpu_ms", C.c_double),
+        ("resolve_gpu_ms", C.c_double),
+        ("frame_gpu_ms", C.c_double),
+        ("traversal_nodes", C.c_uint64 * 4),
+        ("traversal_tris", C.c_uint64 * 4),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C-ABI library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with hydra-lotus_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        L.lotus_hip_probe.restype = C.c_int
+        L.lotus_hip_probe.argtypes = [C.c_char_p, C.c_size_t]
+        L.lotus_hip_create.restype = C.c_void_p
+        L.lotus_hip_create.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]
+        L.lotus_hip_destroy.argtypes = [C.c_void_p]
+        L.lotus_hip_scene_apply.restype = C.c_int
+        L.lotus_hip_scene_apply.argtypes = [C.c_void_p, C.POINTER(ScenePlan), C.POINTER(SceneEvidence),
+                                            C.c_char_p, C.c_size_t]
+        L.lotus_hip_render.restype = C.c_int
+        L.lotus_hip_render.argtypes = [C.c_void_p, C.POINTER(FrameParams), C.POINTER(FrameResult),
+                                       C.c_char_p, C.c_size_t]
+        L.lotus_hip_trace.restype = C.c_int
+        L.lotus_hip_trace.argtypes = [C.c_void_p, C.c_uint32, _f32p, _u32p, C.c_int, C.c_char_p, C.c_size_t]
+        L.lotus_hip_trace_device.restype = C.c_int
+        L.lotus_hip_trace_device.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int,
+                                             C.c_void_p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_light_table.restype = C.c_int
+        L.lotus_hip_light_table.argtypes = [C.c_void_p, _u32p, _u32p, _u32p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_read_accumulation.restype = C.c_int
+        L.lotus_hip_read_accumulation.argtypes = [C.c_void_p, _f32p, _u32p, C.c_char_p, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class LotusError(RuntimeError):
+    def __init__(self, status, detail):
+        super().__init__(f"status {status}: {detail}")
+        self.status = status
+        self.detail = detail
+
+
+def probe() -> tuple:
+    buf = C.create_string_buffer(256)
+    s = lib().lotus_hip_probe(buf, 256)
+    return s, buf.value.decode()
+
+
+class Renderer:
+    """One OffscreenRenderer-like handle over the C ABI."""
+
+    def __init__(self, device: int = 0):
+        st = C.c_int(0)
+        buf = C.create_string_buffer(256)
+        self.handle = lib().lotus_hip_create(device, C.byref(st), buf, 256)
+        if not self.handle:
+            raise LotusError(st.value, buf.value.decode())
+        self.device_detail = buf.value.decode()
+        self._scene_keep = None
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().lotus_hip_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def apply(self, scene, leaf_size: int = 0) -> SceneEvidence:
+        p = scene.packed()
+        plan = ScenePlan()
+        plan.geometry_count = p["geometry_count"]
+        plan.positions = _p(p["positions"], C.c_float)
+        plan.triangles = _p(p["triangles"], C.c_uint32)
+        plan.geometry_vertex_offset = _p(p["geometry_vertex_offset"], C.c_uint32)
+        plan.geometry_vertex_count = _p(p["geometry_vertex_count"], C.c_uint32)
+        plan.geometry_triangle_offset = _p(p["geometry_triangle_offset"], C.c_uint32)
+        plan.geometry_triangle_count = _p(p["geometry_triangle_count"], C.c_uint32)
+        plan.instance_count = p["instance_count"]
+        plan.instance_geometry = _p(p["instance_geometry"], C.c_uint32)
+        plan.instance_transform = _p(p["instance_transform"], C.c_float)
+        plan.instance_material = _p(p["instance_material"], C.c_uint32)
+        plan.material_count = p["material_count"]
+        plan.materials = _p(p["materials"], C.c_uint32)
+        for i in range(3):
+            plan.environment[i] = float(p["environment"][i])
+        plan.bvh_leaf_size = leaf_size
+        ev = SceneEvidence()
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_scene_apply(self.handle, C.byref(plan), C.byref(ev), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        self._scene_keep = p
+        self.instance_count = p["instance_count"]
+        return ev
+
+    def render(self, world_to_clip, width, height, frame_count=1, sample_index=0, max_bounces=8,
+               light_sampling=LIGHT_MIS, output=OUTPUT_RADIANCE, clear_color=(0.0, 0.0, 0.0, 0.0),
+               data_window=(0, 0, 0, 0), display_window=(0.0, 0.0, 0.0, 0.0), max_samples=0,
+               want_color=True, want_depth=False, count_capacity=None):
+        fp = FrameParams()
+        w2c = np.ascontiguousarray(world_to_clip, np.float32).reshape(16)
+        for i in range(16):
+            fp.world_to_clip[i] = float(w2c[i])
+        fp.width, fp.height = width, height
+        for i in range(4):
+            fp.display_window[i] = display_window[i]
+            fp.data_window[i] = data_window[i]
+            fp.clear_color[i] = clear_color[i]
+        fp.frame_count = frame_count
+        fp.output = output
+        fp.light_sampling = light_sampling
+        fp.sample_index = sample_index
+        fp.max_bounces = max_bounces
+        fp.max_samples = max_samples
+        res = FrameResult()
+        color = np.zeros((height, width, 4), np.float32) if want_color else None
+        depth = np.zeros((height, width), np.float32) if want_depth else None
+        cap = count_capacity or (max_bounces + 2)
+        counts = np.zeros(cap, np.uint32)
+        if color is not None:
+            res.color = _p(color, C.c_float)
+        if depth is not None:
+            res.depth = _p(depth, C.c_float)
+        res.path_counts = _p(counts, C.c_uint32)
+        res.path_counts_capacity = cap
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_render(self.handle, C.byref(fp), C.byref(res), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        return dict(color=color, depth=depth, counts=counts[:res.path_counts_size].copy(),
+                    rounds=res.wavefront_rounds, spp=res.samples_per_pixel, frames=res.frames_rendered,
+                    camera_rays=res.camera_rays, extension_rays=res.extension_rays,
+                    shadow_rays=res.shadow_rays, camera_ms=res.camera_gpu_ms,
+                    intersect_ms=res.intersect_gpu_ms, shade_ms=res.shade_gpu_ms,
+                    shadow_ms=res.shadow_gpu_ms, resolve_ms=res.resolve_gpu_ms, frame_ms=res.frame_gpu_ms,
+                    nodes=list(res.traversal_nodes), tris=list(res.traversal_tris))
+
+    def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros(n if any_hit else (n, 5), np.uint32)
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_trace(self.handle, n, _p(rays, C.c_float), _p(hits, C.c_uint32),
+                                  int(any_hit), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        return hits
+
+    def trace_device(self, n: int, rays_ptr: int, hits_ptr: int, any_hit: bool, stream_ptr: int = 0):
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_trace_device(self.handle, n, C.c_void_p(rays_ptr), C.c_void_p(hits_ptr),
+                                         int(any_hit), C.c_void_p(stream_ptr), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    def light_table(self):
+        count = np.zeros(1, np.uint32)
+        buf = C.create_string_buffer(256)
+        lib().lotus_hip_light_table(self.handle, None, None, _p(count, C.c_uint32), buf, 256)
+        offsets = np.zeros(self.instance_count, np.uint32)
+        records = np.zeros((max(int(count[0]), 1), 4), np.uint32)
+        s = lib().lotus_hip_light_table(self.handle, _p(offsets, C.c_uint32), _p(records, C.c_uint32),
+                                        _p(count, C.c_uint32), buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        return offsets, records[:int(count[0])]
+
+    def accumulation(self, width, height):
+        sums = np.zeros((height, width, 4), np.float32)
+        n = np.zeros(1, np.uint32)
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_read_accumulation(self.handle, _p(sums, C.c_float), _p(n, C.c_uint32), buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        return sums, int(n[0])

@@ -1,0 +1,194 @@
+/* lotus_hip.h — C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * The drop-in boundary for hydra-lotus's Integrator::Wavefront path. Every
+ * entry point replaces a reference interface, cited per function:
+ *   include/lotus/vulkan_backend.hpp:753-883  OffscreenRenderer and its factory
+ *   backend/vulkan/gpu_scene.hpp:248-510       GpuScene (reached via UpdateScene)
+ *   backend/vulkan/shaders/common/scene.slang  Trace / ShadowTransmittance
+ * Plain pointers and sizes only; no torch or HIP types cross it. Buffers are
+ * caller-allocated. A handle is single-threaded (one frame in flight, as
+ * vulkan_backend.hpp:744-752 states for the reference). Every call returns a
+ * status: LOTUS_HIP_PASS, LOTUS_HIP_FAIL or LOTUS_HIP_SKIP (no device), with a
+ * NUL-terminated detail string, mirroring FrameStatus {Pass, Fail, Skip}
+ * (vulkan_backend.hpp:23-27). After a failed GPU submission the handle stays
+ * failed ("the renderer failed earlier: ..."), as vulkan_backend.cpp:1095-1096
+ * latches; invalid settings fail without latching.
+ */
+#ifndef LOTUS_HIP_H
+#define LOTUS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LOTUS_HIP_PASS 0
+#define LOTUS_HIP_FAIL 1
+#define LOTUS_HIP_SKIP 77
+
+/* The wavefront integrator traces at most this many bounces
+ * (kMaxWavefrontBounces, vulkan_backend.hpp:694). */
+#define LOTUS_HIP_MAX_BOUNCES 1024u
+
+/* Hit word for a ray that found nothing. */
+#define LOTUS_HIP_MISS 0xFFFFFFFFu
+
+typedef struct lotus_hip_renderer lotus_hip_renderer;
+
+/* The GPU scene's content, one plan replacing the previous one in full
+ * (SceneUpdate, include/lotus/extraction.hpp:91-132, applied by
+ * GpuScene::Apply, backend/vulkan/gpu_scene.cpp:1606): object-space
+ * geometries placed by instances, and the material table as
+ * GpuMaterialRecord words (gpu_scene.hpp:94-113, 36 words each). */
+typedef struct lotus_hip_scene_plan {
+  uint32_t geometry_count;
+  const float* positions;                   /* xyz per vertex, all geometries */
+  const uint32_t* triangles;                /* 3 geometry-relative indices per triangle */
+  const uint32_t* geometry_vertex_offset;   /* per geometry, into positions */
+  const uint32_t* geometry_vertex_count;    /* per geometry */
+  const uint32_t* geometry_triangle_offset; /* per geometry, into triangles */
+  const uint32_t* geometry_triangle_count;  /* per geometry */
+  uint32_t instance_count;
+  const uint32_t* instance_geometry;        /* per instance */
+  const float* instance_transform;          /* 16 floats per instance, column-major */
+  const uint32_t* instance_material;        /* per instance: material table slot */
+  uint32_t material_count;
+  const uint32_t* materials;                /* 36 words per material */
+  float environment[3];                     /* constant environment radiance */
+  uint32_t bvh_leaf_size;                   /* 0: default (4); at most 8 */
+} lotus_hip_scene_plan;
+
+/* GpuSceneEvidence / GpuSceneStats counterpart (vulkan_backend.hpp:412-506). */
+typedef struct lotus_hip_scene_evidence {
+  uint32_t triangle_count;
+  uint32_t instance_count;
+  uint32_t material_count;
+  uint32_t light_count;      /* light table entries */
+  uint32_t node_count;       /* wide BVH nodes */
+  uint32_t bvh_depth;        /* wide levels, root = 1 */
+  uint32_t leaf_count;
+  uint32_t bvh_width;        /* children per node */
+  double upload_ms;          /* host to device copies, wall */
+  double build_gpu_ms;       /* BVH build kernels, hipEvent */
+  uint64_t device_bytes;     /* scene allocations */
+} lotus_hip_scene_evidence;
+
+/* SceneOutput (vulkan_backend.hpp:624-637). */
+#define LOTUS_HIP_OUTPUT_RADIANCE 0u
+#define LOTUS_HIP_OUTPUT_BARYCENTRICS 1u
+#define LOTUS_HIP_OUTPUT_SHADING_NORMAL 2u
+#define LOTUS_HIP_OUTPUT_ALBEDO 3u
+#define LOTUS_HIP_OUTPUT_ROUGHNESS_METALLIC 4u
+
+/* LightSampling (vulkan_backend.hpp:652-674). */
+#define LOTUS_HIP_LIGHT_BSDF 0u
+#define LOTUS_HIP_LIGHT_MIS 1u
+
+/* One RenderScene call (vulkan_backend.hpp:789-791): DrawSummary::
+ * world_to_clip, OffscreenTarget (vulkan_backend.hpp:253-275) and
+ * PathTracingSettings (vulkan_backend.hpp:697-738). */
+typedef struct lotus_hip_frame_params {
+  float world_to_clip[16];    /* OpenGL clip space, column-major */
+  uint32_t width, height;     /* target */
+  float display_window[4];    /* x, y, w, h; all zero = the whole image */
+  uint32_t data_window[4];    /* x, y, w, h; all zero = the whole image */
+  float clear_color[4];
+  uint32_t frame_count;       /* samples this call adds */
+  uint32_t output;            /* LOTUS_HIP_OUTPUT_* */
+  uint32_t light_sampling;    /* LOTUS_HIP_LIGHT_* */
+  uint32_t sample_index;      /* first sample's random sequence */
+  uint32_t max_bounces;       /* scattering events after the camera hit */
+  uint32_t max_samples;       /* 0 = no limit */
+} lotus_hip_frame_params;
+
+/* GpuFrameEvidence subset (vulkan_backend.hpp:202-249) plus rays by kind.
+ * Output buffers are optional (NULL skips the readback). */
+typedef struct lotus_hip_frame_result {
+  float* color;                 /* width*height*4, RGBA32F, rows from the top */
+  float* depth;                 /* width*height, D32 (pixel-centre hit), 1 on a miss */
+  uint32_t* path_counts;        /* capacity entries: last sample's ray-queue counts */
+  uint32_t path_counts_capacity;
+  uint32_t path_counts_size;    /* written: ends at the last bounce that traced */
+  uint32_t wavefront_rounds;    /* bounces run as kernel rounds */
+  uint32_t samples_per_pixel;   /* accumulated after the call */
+  uint32_t frames_rendered;
+  uint64_t camera_rays;         /* over all samples of the call */
+  uint64_t extension_rays;
+  uint64_t shadow_rays;
+  double camera_gpu_ms;         /* WavefrontTimings (vulkan_backend.hpp:52-68) */
+  double intersect_gpu_ms;
+  double shade_gpu_ms;
+  double shadow_gpu_ms;
+  double resolve_gpu_ms;
+  double frame_gpu_ms;          /* first kernel start to last kernel end */
+  /* Traversal work over the call's samples, per kernel: camera, extension
+   * (intersect), shadow, tail. Wide nodes tested and triangles tested. */
+  uint64_t traversal_nodes[4];
+  uint64_t traversal_tris[4];
+} lotus_hip_frame_result;
+
+/* ProbeVulkanBackend (vulkan_backend.hpp:742): PASS when a gfx950 device is
+ * visible, SKIP otherwise. */
+int lotus_hip_probe(char* detail, size_t detail_size);
+
+/* CreateOffscreenRenderer (vulkan_backend.hpp:879-883) on device `device`.
+ * Returns NULL with *status SKIP (no device) or FAIL. */
+lotus_hip_renderer* lotus_hip_create(int device, int* status, char* detail,
+                                     size_t detail_size);
+void lotus_hip_destroy(lotus_hip_renderer* renderer);
+
+/* OffscreenRenderer::UpdateScene (vulkan_backend.hpp:816-817): uploads the
+ * plan, builds the GPU scene (flattened world-space triangles, light table,
+ * wide BVH built on the GPU) and returns after it is resident. */
+int lotus_hip_scene_apply(lotus_hip_renderer* renderer,
+                          const lotus_hip_scene_plan* plan,
+                          lotus_hip_scene_evidence* evidence, char* detail,
+                          size_t detail_size);
+
+/* OffscreenRenderer::RenderScene with Integrator::Wavefront
+ * (vulkan_backend.hpp:789-791). Accumulation restarts when the camera,
+ * target, sample_index, max_bounces, light sampling or scene change
+ * (vulkan_backend.cpp:1186-1205). */
+int lotus_hip_render(lotus_hip_renderer* renderer,
+                     const lotus_hip_frame_params* params,
+                     lotus_hip_frame_result* result, char* detail,
+                     size_t detail_size);
+
+/* Scene.slang Trace (:424-439) and ShadowTransmittance (:445-460) over a
+ * caller's rays: origin xyz, tmin, direction xyz, tmax (8 floats per ray).
+ * Closest hits are instance, primitive (LOTUS_HIP_MISS on a miss),
+ * barycentric weights of corners 1 and 2, t (5 words); any-hit results
+ * are 1 (blocked) or 0. Host pointers. Ties in t resolve to the lowest
+ * (instance, primitive). */
+int lotus_hip_trace(lotus_hip_renderer* renderer, uint32_t count,
+                    const float* rays, uint32_t* hits, int any_hit,
+                    char* detail, size_t detail_size);
+
+/* The same over device-resident rays and hits (hipMalloc'd by the caller),
+ * enqueued on `stream` (a hipStream_t, NULL for the default stream);
+ * returns without synchronizing. */
+int lotus_hip_trace_device(lotus_hip_renderer* renderer, uint32_t count,
+                           const void* rays_device, void* hits_device,
+                           int any_hit, void* stream, char* detail,
+                           size_t detail_size);
+
+/* The light table as built (gpu_scene.cpp:512-609): per instance its first
+ * entry or 0xFFFFFFFF, then 4 words per entry {pdf, threshold, alias,
+ * instance}. Returns the entry count in *count; arrays may be NULL. */
+int lotus_hip_light_table(lotus_hip_renderer* renderer, uint32_t* offsets,
+                          uint32_t* records, uint32_t* count, char* detail,
+                          size_t detail_size);
+
+/* The accumulation as sums: 4 floats per pixel (rgb, hit samples), rows from
+ * the top, and the count of samples added. */
+int lotus_hip_read_accumulation(lotus_hip_renderer* renderer, float* sums,
+                                uint32_t* samples, char* detail,
+                                size_t detail_size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOTUS_HIP_H */
