@@ -272,7 +272,7 @@ struct Box {
 __global__ void __launch_bounds__(kBlock) k_bottom_up(const Tri* world, const uint32_t* sorted, int n,
                                                       const uint32_t* child, const uint32_t* parent,
                                                       uint32_t* flags, Box* boxes, Tri* tris,
-                                                      uint32_t* tri_of_global) {
+                                                      uint32_t* tri_of_global, uint32_t* fault) {
   const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (k >= n) return;
   const uint32_t g = sorted[k];
@@ -291,7 +291,10 @@ __global__ void __launch_bounds__(kBlock) k_bottom_up(const Tri* world, const ui
   if (n == 1) return;
   __threadfence();
   uint32_t p = parent[node];
-  while (true) {
+  // A valid tree reaches the root within 30 + 27 levels; a longer walk means
+  // a broken tree, reported instead of followed.
+  for (int step = 0; step < 128; ++step) {
+    if (p >= static_cast<uint32_t>(n - 1)) break;
     if (atomicAdd(&flags[p], 1u) == 0u) return;  // first child to arrive stops
     __threadfence();
     const Box l = boxes[child[2 * p]], r = boxes[child[2 * p + 1]];
@@ -306,6 +309,7 @@ __global__ void __launch_bounds__(kBlock) k_bottom_up(const Tri* world, const ui
     if (p == 0) return;
     p = parent[p];
   }
+  atomicOr(fault, 1u);
 }
 
 // ---------------------------------------------------------------- collapse --
@@ -331,7 +335,7 @@ __global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const ui
                                                      Task* next, uint32_t* next_count, uint32_t* node_count,
                                                      uint32_t* leaf_total, const uint32_t* child,
                                                      const uint2* range, const Box* boxes, int n,
-                                                     uint32_t leaf_size, WideNode* nodes) {
+                                                     uint32_t leaf_size, WideNode* nodes, uint32_t* fault) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= *task_count) return;
   const Task task = tasks[t];
@@ -378,6 +382,14 @@ __global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const ui
       continue;
     }
     const uint32_t b = list[c];
+    const uint32_t first_b = b >= static_cast<uint32_t>(n - 1) ? b - static_cast<uint32_t>(n - 1) : range[b].x;
+    if (b >= static_cast<uint32_t>(2 * n - 1) || first_b + size_of(b) > static_cast<uint32_t>(n)) {
+      atomicOr(fault, 2u);  // a broken tree: leave the child empty and report
+      out.lox[c] = out.loy[c] = out.loz[c] = INFINITY;
+      out.hix[c] = out.hiy[c] = out.hiz[c] = -INFINITY;
+      out.child[c] = kEmptyChild;
+      continue;
+    }
     Box box = boxes[b];
 #pragma unroll
     for (int a = 0; a < 3; ++a) inflate(box.lo[a], box.hi[a]);
@@ -389,8 +401,7 @@ __global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const ui
     out.hiz[c] = box.hi[2];
     const uint32_t sz = size_of(b);
     if (sz <= leaf_size) {
-      const uint32_t first = b >= static_cast<uint32_t>(n - 1) ? b - static_cast<uint32_t>(n - 1) : range[b].x;
-      out.child[c] = leaf_word(first, sz);
+      out.child[c] = leaf_word(first_b, sz);
       ++leaves;
     } else {
       const uint32_t w = atomicAdd(node_count, 1u);
@@ -496,13 +507,13 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   uint2* range = nullptr;
   Box* boxes = nullptr;
   Task *tasks_a = nullptr, *tasks_b = nullptr;
-  uint32_t* counters = nullptr;  // 0: task count a, 1: task count b, 2: node count, 3: leaves
+  uint32_t* counters = nullptr;  // 0: task count a, 1: task count b, 2: node count, 3: leaves, 4: fault
   const uint32_t internal = n > 1 ? n - 1 : 1;
   bool ok = alloc(world, n, error) && alloc(centroid, n, error) && alloc(bounds, 6, error) &&
             alloc(keys, n, error) && alloc(vals, n, error) && alloc(child, 2ull * internal, error) &&
             alloc(parent, 2ull * n, error) && alloc(flags, internal, error) && alloc(range, internal, error) &&
             alloc(boxes, 2ull * n, error) && alloc(tasks_a, n, error) && alloc(tasks_b, n, error) &&
-            alloc(counters, 4, error);
+            alloc(counters, 5, error);
   auto release = [&]() {
     (void)hipFree(world);
     (void)hipFree(centroid);
@@ -549,28 +560,29 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
     k_fill_u32<<<blocks_for(internal, kBlock), kBlock, 0, stream>>>(flags, 0u, internal);
   }
   k_bottom_up<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(world, vals, static_cast<int>(n), child, parent,
-                                                            flags, boxes, out.tris, out.tri_of_global);
+                                                            flags, boxes, out.tris, out.tri_of_global,
+                                                            counters + 4);
   // collapse, level by level; the root task expands binary node 0
   const Task root{n > 1 ? 0u : 0u, 0u};
-  const uint32_t init_counts[4] = {1u, 0u, 1u, 0u};
+  const uint32_t init_counts[5] = {1u, 0u, 1u, 0u, 0u};
   e = hipMemcpyAsync(tasks_a, &root, sizeof root, hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipMemcpyAsync(counters, init_counts, sizeof init_counts, hipMemcpyHostToDevice, stream);
   uint32_t pending = 1, depth = 0;
   Task *cur = tasks_a, *nxt = tasks_b;
   int cur_slot = 0;
-  while (e == hipSuccess && pending > 0) {
+  while (e == hipSuccess && pending > 0 && depth < 256) {
     ++depth;
     const int nxt_slot = 1 - cur_slot;
     k_fill_u32<<<1, 1, 0, stream>>>(counters + nxt_slot, 0u, 1u);
     k_collapse<<<blocks_for(pending, kBlock), kBlock, 0, stream>>>(
         cur, counters + cur_slot, nxt, counters + nxt_slot, counters + 2, counters + 3, child, range, boxes,
-        static_cast<int>(n), in.leaf_size, out.nodes);
+        static_cast<int>(n), in.leaf_size, out.nodes, counters + 4);
     e = hipMemcpyAsync(&pending, counters + nxt_slot, 4, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     std::swap(cur, nxt);
     cur_slot = nxt_slot;
   }
-  uint32_t final_counts[4] = {0, 0, 0, 0};
+  uint32_t final_counts[5] = {0, 0, 0, 0, 0};
   if (e == hipSuccess) e = hipMemcpyAsync(final_counts, counters, sizeof final_counts, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipEventRecord(stop, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -584,6 +596,11 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   (void)hipEventDestroy(start);
   (void)hipEventDestroy(stop);
   gpu_ms = ms;
+  if (final_counts[4] != 0 || pending > 0) {
+    error = "BVH build produced an invalid tree (code " + std::to_string(final_counts[4]) + ")";
+    release();
+    return false;
+  }
   out.node_count = final_counts[2];
   out.leaf_count = final_counts[3];
   out.depth = depth;

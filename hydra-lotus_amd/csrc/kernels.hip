@@ -18,9 +18,9 @@ namespace lotus {
 namespace {
 
 template <int N>
-__device__ __forceinline__ TraversalStack<N> make_stack(uint2* spill) {
+__device__ __forceinline__ TraversalStack<N> make_stack(uint2* spill, int cap) {
   TraversalStack<N> st;
-  st.init(spill, gridDim.x * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x);
+  st.init(spill, gridDim.x * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x, cap);
   return st;
 }
 
@@ -60,10 +60,10 @@ __device__ __forceinline__ void push_hit(const Queues& q, uint32_t k, uint32_t s
 // camera ray traced at once; a hit is queued for shading, a miss leaves no
 // sample.
 __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                        uint2* spill, unsigned long long* stats) {
+                                                        uint2* spill, int spill_cap, unsigned long long* stats) {
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
   const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill);
+  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
   for (uint32_t i = gtid(); i < total; i += gstride()) {
     const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
     const uint32_t slot = y * F.width + x;
@@ -86,10 +86,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
 // wavefront.slang:230-262: continuation rays of bounce b; a miss ends the
 // path with the environment (transport.slang:436-448).
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                           uint2* spill, uint32_t bounce, unsigned long long* stats) {
+                                                           uint2* spill, int spill_cap, uint32_t bounce, unsigned long long* stats) {
   const uint32_t count = *ray_count(counters, bounce);
   const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill);
+  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
   const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
   for (uint32_t i = gtid(); i < count; i += gstride()) {
     const float4 o4 = q.ray_o[i], d4 = q.ray_d[i];
@@ -154,11 +154,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
 
 // ShadowTransmittance (scene.slang:445-460) for the shadow queue of bounce
 // b: an unblocked ray adds its light to its path's radiance.
-__global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView S, Queues q, uint32_t* counters, uint2* spill,
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView S, Queues q, uint32_t* counters, uint2* spill, int spill_cap,
                                                         uint32_t bounce, unsigned long long* stats) {
   const uint32_t count = *shadow_count(counters, bounce);
   const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill);
+  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
   for (uint32_t i = gtid(); i < count; i += gstride()) {
     const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
     const Ray r{xyz(o4), 0.0f, xyz(d4), d4.w};
@@ -179,10 +179,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView S, Queues q, u
 // wavefront.slang:287-309: the paths still queued after the recorded rounds,
 // each followed to its end in one invocation, shadow rays traced inline.
 __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                      uint2* spill, uint32_t bounce, unsigned long long* stats) {
+                                                      uint2* spill, int spill_cap, uint32_t bounce, unsigned long long* stats) {
   const uint32_t count = *ray_count(counters, bounce);
   const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill);
+  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
   const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
   for (uint32_t i = gtid(); i < count; i += gstride()) {
     const float4 o4 = q.ray_o[i], d4 = q.ray_d[i];
@@ -247,10 +247,10 @@ __global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_
 // depth and barycentric weights or a surface diagnostic. Misses keep the
 // clears (colour = background, depth = 1).
 __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameConst F, Queues q, float* depth,
-                                                             uint32_t write_color, uint2* spill) {
+                                                             uint32_t write_color, uint2* spill, int spill_cap) {
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
   const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill);
+  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
   for (uint32_t i = gtid(); i < total; i += gstride()) {
     const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
     const uint32_t slot = y * F.width + x;
@@ -279,9 +279,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameC
 
 // Trace / ShadowTransmittance over caller rays (8 floats in, 5 words or 1 out).
 __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, const float4* rays, uint32_t* hits,
-                                                       uint32_t count, uint32_t any_hit, uint2* spill) {
+                                                       uint32_t count, uint32_t any_hit, uint2* spill, int spill_cap) {
   const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill);
+  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
   for (uint32_t i = gtid(); i < count; i += gstride()) {
     const float4 o4 = rays[2 * i], d4 = rays[2 * i + 1];
     const Ray r{xyz(o4), o4.w, xyz(d4), d4.w};
@@ -331,34 +331,34 @@ void launch_fill1(float* p, uint32_t n, float v, hipStream_t s) {
   k_fill1<<<(n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024, 256, 0, s>>>(p, n, v);
 }
 void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
-  k_camera<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.stats);
+  k_camera<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_intersect(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                       uint32_t bounce) {
-  k_intersect<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, bounce, L.stats);
+  k_intersect<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce) {
   k_shade<<<L.shade_blocks, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce);
 }
 void launch_shadow(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce) {
-  k_shadow<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, bounce, L.stats);
+  k_shadow<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, L.stats);
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce) {
-  k_tail<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, bounce, L.stats);
+  k_tail<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
 }
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample) {
   k_resolve<<<L.shade_blocks, 256, 0, L.stream>>>(F, q, add_sample);
 }
 void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, float* depth,
                         uint32_t write_color) {
-  k_camera_pass<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, depth, write_color, L.spill);
+  k_camera_pass<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, depth, write_color, L.spill, L.spill_cap);
 }
 void launch_trace(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
                   uint32_t any_hit) {
   k_trace<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, reinterpret_cast<const float4*>(rays), hits, count,
-                                                         any_hit, L.spill);
+                                                         any_hit, L.spill, L.spill_cap);
 }
 
 }  // namespace lotus

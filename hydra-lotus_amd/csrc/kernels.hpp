@@ -31,6 +31,7 @@ struct Launch {
   uint32_t trace_blocks;  // persistent grid of the traversal kernels
   uint32_t shade_blocks;
   uint2* spill;           // traversal stack spill, trace_blocks * kTraceBlock lanes
+  int spill_cap;          // spill entries per lane
   unsigned long long* stats;  // node visits, triangle tests: camera, intersect, shadow, tail
 };
 

@@ -93,8 +93,10 @@ uint32_t wavefront_rounds(const std::vector<uint32_t>& counts) {
   return rounds;
 }
 
-// Rounds of a sample with no previous one to go by.
+// Rounds of a sample with no previous one to go by, and the most a sample
+// records (the tail follows longer paths; the event pool holds their timers).
 constexpr uint32_t kFirstRounds = 32;
+constexpr uint32_t kMaxRounds = 128;
 
 struct AccumulationKey {  // vulkan_backend.cpp:419-434
   std::array<float, 16> world_to_clip{};
@@ -595,7 +597,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     return LOTUS_HIP_FAIL;
   }
   const uint32_t npix = P->width * P->height;
-  Launch L{r->stream, r->trace_blocks, r->shade_blocks, r->spill, r->stats};
+  Launch L{r->stream, r->trace_blocks, r->shade_blocks, r->spill, static_cast<int>(r->spill_entries), r->stats};
   launch_zero(reinterpret_cast<uint32_t*>(r->stats), 16, r->stream);
   launch_fill4(r->q.color, npix, make_float4(P->clear_color[0], P->clear_color[1], P->clear_color[2], P->clear_color[3]),
                r->stream);
@@ -634,7 +636,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       F.sample_index = P->sample_index + r->accumulated;
       F.accumulated = r->accumulated;
       uint32_t rounds = r->rounds_hint == 0 ? std::min(bounces, kFirstRounds) : std::min(r->rounds_hint, bounces);
-      rounds = std::max(rounds, 1u);
+      rounds = std::min(std::max(rounds, 1u), kMaxRounds);
       size_t ev = 0;
       std::vector<std::pair<int, size_t>> pairs;
       auto mark = [&](int timer) {
@@ -772,7 +774,7 @@ int lotus_hip_trace_device(lotus_hip_renderer* r, uint32_t count, const void* ra
   }
   (void)hipSetDevice(r->device);
   Launch L{stream ? static_cast<hipStream_t>(stream) : r->stream, r->trace_blocks, r->shade_blocks, r->spill,
-           r->stats};
+           static_cast<int>(r->spill_entries), r->stats};
   launch_trace(L, r->view, static_cast<const float*>(rays), static_cast<uint32_t*>(hits), count,
                any_hit ? 1u : 0u);
   const hipError_t e = hipGetLastError();

@@ -21,6 +21,9 @@ namespace lotus {
 constexpr int kWidth = 4;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
+// Traversal steps after which a ray gives up: far beyond any valid tree's
+// need (each node is visited at most once), it bounds a broken one.
+constexpr uint32_t kMaxSteps = 1u << 22;
 
 struct alignas(16) WideNode {
   float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
@@ -130,10 +133,12 @@ struct TraversalStack {
   uint32_t lane;
   uint32_t visits;  // wide nodes tested
   uint32_t tests;   // triangles tested
+  int capacity;     // spill entries per lane
 
-  __device__ __forceinline__ void init(uint2* spill_area, uint32_t lane_count, uint32_t lane_id) {
+  __device__ __forceinline__ void init(uint2* spill_area, uint32_t lane_count, uint32_t lane_id, int cap) {
     count = 0;
     spilled = 0;
+    capacity = cap;
     visits = 0;
     tests = 0;
     spill = spill_area;
@@ -142,8 +147,12 @@ struct TraversalStack {
   }
   __device__ __forceinline__ void push(uint32_t n, float d) {
     if (count == N) {
-      spill[static_cast<size_t>(spilled) * lanes + lane] = make_uint2(node[N - 1], __float_as_uint(dist[N - 1]));
-      ++spilled;
+      // The area holds 3 x (tree depth) entries, the most a nearest-first
+      // walk of a 4-wide tree keeps; the bound keeps every store in it.
+      if (spilled < capacity) {
+        spill[static_cast<size_t>(spilled) * lanes + lane] = make_uint2(node[N - 1], __float_as_uint(dist[N - 1]));
+        ++spilled;
+      }
     } else {
       ++count;
     }
@@ -235,7 +244,10 @@ struct Traverser {
     uint32_t cur = 0;
     float cur_t = r.tmin;
     bool alive = true;
-    while (alive) {
+    uint32_t guard = 0;
+    st.count = 0;
+    st.spilled = 0;
+    while (alive && ++guard < kMaxSteps) {
       if ((cur & kLeafBit) == 0) {
         float tn[4];
         uint32_t ch[4];
@@ -301,7 +313,10 @@ struct Traverser {
     const V3 inv = V3{safe_inverse(r.d.x), safe_inverse(r.d.y), safe_inverse(r.d.z)};
     uint32_t cur = 0;
     bool alive = true;
-    while (alive) {
+    uint32_t guard = 0;
+    st.count = 0;
+    st.spilled = 0;
+    while (alive && ++guard < kMaxSteps) {
       if ((cur & kLeafBit) == 0) {
         float tn[4];
         uint32_t ch[4];

@@ -116,17 +116,20 @@ class Scene:
             tc.append(tri.shape[0])
             v += pos.shape[0]
             t += tri.shape[0]
+        pos = [p for p, _ in self.geometries] or [np.zeros((0, 3), F32)]
+        tri = [tr for _, tr in self.geometries] or [np.zeros((0, 3), np.uint32)]
+        xf = [tr for _, tr, _ in self.instances] or [np.zeros(16, F32)]
         self._packed = dict(
             geometry_count=len(self.geometries),
-            positions=np.ascontiguousarray(np.concatenate([p for p, _ in self.geometries]).astype(F32)),
-            triangles=np.ascontiguousarray(np.concatenate([tr for _, tr in self.geometries]).astype(np.uint32)),
+            positions=np.ascontiguousarray(np.concatenate(pos).astype(F32)),
+            triangles=np.ascontiguousarray(np.concatenate(tri).astype(np.uint32)),
             geometry_vertex_offset=np.array(vo, np.uint32),
             geometry_vertex_count=np.array(vc, np.uint32),
             geometry_triangle_offset=np.array(to, np.uint32),
             geometry_triangle_count=np.array(tc, np.uint32),
             instance_count=len(self.instances),
             instance_geometry=np.array([g for g, _, _ in self.instances], np.uint32),
-            instance_transform=np.ascontiguousarray(np.stack([tr for _, tr, _ in self.instances]).astype(F32)),
+            instance_transform=np.ascontiguousarray(np.stack(xf).astype(F32)),
             instance_material=np.array([m for _, _, m in self.instances], np.uint32),
             material_count=len(self.materials),
             materials=np.ascontiguousarray(np.stack(self.materials).astype(np.uint32)),
