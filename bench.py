@@ -43,26 +43,26 @@ def log(msg: str) -> None:
 
 
 def cpu_baseline(scene, width, height, depth, threads, target_s=12.0):
-    """The CPU oracle on a band of rows of the same frame, 1 spp."""
+    """The CPU oracle on the same frame: enough samples per pixel for about
+    `target_s` seconds of host work (first a 64-row band calibrates)."""
     import oracle
 
     o = oracle.OracleScene(scene)
-    rows = 8
 
-    def run(rows):
+    def run(rows, spp):
         y0 = max(0, height // 2 - rows // 2)
-        f = oracle.make_frame(scene.world_to_clip, width, height, spp=1, max_bounces=depth,
+        f = oracle.make_frame(scene.world_to_clip, width, height, spp=spp, max_bounces=depth,
                               light_sampling=1, rect=(0, y0, width, rows))
         t0 = time.perf_counter()
         _, counts, shadow = o.render(f, threads=threads)
         dt = time.perf_counter() - t0
         return dt, int(counts.sum()) + int(shadow.sum())
 
-    dt, rays = run(rows)
-    rows = int(min(height, max(rows, rows * target_s / max(dt, 1e-3))))
-    dt, rays = run(rows)
+    dt, _ = run(64, 1)
+    spp = int(max(1, min(64, target_s / max(dt * height / 64, 1e-3))))
+    dt, rays = run(height, spp)
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{rows} x {width} px band of the same 1080p frame, 1 spp, depth {depth}, NEE+MIS, "
+            "sample": f"the full {width}x{height} frame, {spp} spp, depth {depth}, NEE+MIS: "
                       f"{rays} rays in {dt:.1f} s"}
 
 
@@ -94,11 +94,12 @@ def main() -> int:
     for _ in range(args.warmup):
         r.render(w2c, args.width, args.height, **kw)
     log(f"warmup done ({args.warmup} frames)")
-    tot = {k: 0.0 for k in ("camera_rays", "extension_rays", "shadow_rays", "camera_ms", "intersect_ms",
-                            "shade_ms", "shadow_ms", "resolve_ms", "frame_ms")}
+    keys = ("camera_rays", "extension_rays", "shadow_rays", "camera_ms", "intersect_ms", "shade_ms",
+            "shadow_ms", "trace_ms", "tail_ms", "resolve_ms", "frame_ms")
+    tot = {k: 0.0 for k in keys}
     nodes = np.zeros(4)
     tris = np.zeros(4)
-    launches = {"intersect": 0, "shadow": 0, "camera": 0}
+    trace_launches = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         out = r.render(w2c, args.width, args.height, **kw)
@@ -106,42 +107,49 @@ def main() -> int:
             tot[key] += out[key]
         nodes += out["nodes"]
         tris += out["tris"]
-        launches["camera"] += 1
-        launches["intersect"] += max(out["rounds"] - 1, 0) + (1 if out["rounds"] < args.depth + 1 else 0)
-        launches["shadow"] += out["rounds"]
+        trace_launches += out["rounds"]
     wall = time.perf_counter() - t0
     rays = tot["camera_rays"] + tot["extension_rays"] + tot["shadow_rays"]
     value = rays / wall / 1e6
+    # rays by kind: the same frames with extension and shadow rays in launches
+    # of their own, so each kind's kernel time is its own (not in the timed loop)
+    split = {k: 0.0 for k in keys}
+    for k in range(min(16, args.steps)):
+        out = r.render(w2c, args.width, args.height, split_kinds=True, **kw)
+        for key in split:
+            split[key] += out[key]
     by_kind = {
-        "camera": tot["camera_rays"] / tot["camera_ms"] / 1e3,
-        "extension": tot["extension_rays"] / tot["intersect_ms"] / 1e3,
-        "shadow": tot["shadow_rays"] / tot["shadow_ms"] / 1e3,
+        "camera": split["camera_rays"] / split["camera_ms"] / 1e3,
+        "extension": split["extension_rays"] / split["intersect_ms"] / 1e3,
+        "shadow": split["shadow_rays"] / split["shadow_ms"] / 1e3,
         "frame_gpu": rays / tot["frame_ms"] / 1e3,
     }
-    # roofline of the dominant traversal kernel: algorithmic bytes (queue
-    # records + wide nodes and triangles the traversal tests) / kernel time
-    kinds = {"camera": (0, tot["camera_ms"], tot["camera_rays"]),
-             "intersect": (1, tot["intersect_ms"], tot["extension_rays"]),
-             "shadow": (2, tot["shadow_ms"], tot["shadow_rays"])}
-    dom = max(kinds, key=lambda k: kinds[k][1])
-    idx, ms, n = kinds[dom]
-    extra = 3 if dom == "intersect" else None  # the tail traverses as intersect does
-    node_visits = nodes[idx] + (nodes[extra] if extra else 0)
-    tri_tests = tris[idx] + (tris[extra] if extra else 0)
-    bytes_total = node_visits * NODE_BYTES + tri_tests * TRI_BYTES + n * QUEUE_BYTES[dom]
+    # roofline of the dominant kernel, k_trace (extension + shadow rays):
+    # algorithmic bytes = queue records + the wide nodes and triangles the
+    # traversal tests (counted by the traversal itself in the timed frames),
+    # over the kernel's hipEvent time
+    ext, sh = tot["extension_rays"], tot["shadow_rays"]
+    node_visits = nodes[1] + nodes[2]
+    tri_tests = tris[1] + tris[2]
+    bytes_total = node_visits * NODE_BYTES + tri_tests * TRI_BYTES + ext * QUEUE_BYTES["intersect"] + \
+        sh * QUEUE_BYTES["shadow"]
+    ms = tot["trace_ms"]
     achieved = bytes_total / (ms / 1e3) / 1e9
-    per_launch = bytes_total / max(launches[dom], 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "algorithmic_bytes_per_launch": int(per_launch),
-                "avg_launch_ms": round(ms / max(launches[dom], 1), 4),
-                "nodes_per_ray": round(node_visits / max(n, 1), 2),
-                "tris_per_ray": round(tri_tests / max(n, 1), 2)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_trace",
+                "algorithmic_bytes_per_launch": int(bytes_total / max(trace_launches, 1)),
+                "avg_launch_ms": round(ms / max(trace_launches, 1), 4),
+                "nodes_per_ray": {"camera": round(nodes[0] / max(tot["camera_rays"], 1), 2),
+                                  "extension": round(nodes[1] / max(ext, 1), 2),
+                                  "shadow": round(nodes[2] / max(sh, 1), 2)},
+                "tris_per_ray": {"camera": round(tris[0] / max(tot["camera_rays"], 1), 2),
+                                 "extension": round(tris[1] / max(ext, 1), 2),
+                                 "shadow": round(tris[2] / max(sh, 1), 2)}}
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             t = json.load(f)
-        if t.get("kernel") == dom:
+        if t.get("kernel") == roofline["kernel"]:
             roofline["traffic"] = t.get("hbm_bytes_per_launch")
     line = {
         "metric": "Mrays/s by ray kind + GPU ms to reach equal relMSE (plain vs MLP cache), 1080p",
@@ -165,8 +173,7 @@ def main() -> int:
                            "extension": tot["extension_rays"] / args.steps,
                            "shadow": tot["shadow_rays"] / args.steps},
         "gpu_ms_per_frame": {k.replace("_ms", ""): round(tot[k] / args.steps, 3)
-                             for k in ("camera_ms", "intersect_ms", "shade_ms", "shadow_ms", "resolve_ms",
-                                       "frame_ms")},
+                             for k in ("camera_ms", "shade_ms", "trace_ms", "tail_ms", "resolve_ms", "frame_ms")},
         "roofline": roofline,
     }
     r.close()

@@ -108,11 +108,16 @@ __global__ void __launch_bounds__(kBlock) k_morton(const float4* centroid, const
   if (g >= n) return;
   const float4 c = centroid[g];
   const float cc[3] = {c.x, c.y, c.z};
+  // One quantum for every axis (the largest extent over 1024): a flat scene
+  // keeps few bits along its thin axis, so the top of the tree splits it
+  // across its width rather than into overlapping layers.
+  float ext = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) ext = fmaxf(ext, unordered(bounds[3 + a]) - unordered(bounds[a]));
   uint32_t q[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float lo = unordered(bounds[a]), hi = unordered(bounds[3 + a]);
-    const float ext = hi - lo;
+    const float lo = unordered(bounds[a]);
     const float t = ext > 0.0f ? (cc[a] - lo) / ext : 0.5f;
     q[a] = static_cast<uint32_t>(fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f));
   }

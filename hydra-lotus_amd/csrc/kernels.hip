@@ -4,44 +4,113 @@
 // plus the pixel-centre camera pass (path_trace.slang:69-97) and a plain
 // trace kernel over caller rays.
 //
-// Scheduling differs from the reference by design: HIP has no indirect
-// dispatch, so every kernel is a persistent grid (a few workgroups per CU)
-// striding over a queue whose length it reads from device memory. Queues are
-// structure-of-arrays rays and hits in queue order, appended with one atomic
-// per wave (ballot + popcount prefix), so a kernel's loads of its queue are
-// contiguous. Next event estimation's shadow rays get their own queue and
-// kernel, between shading and the next intersection, so the radiance sums
-// keep the reference's order.
+// Scheduling differs from the reference by design. HIP has no indirect
+// dispatch, so every kernel is a persistent grid, sized from its occupancy to
+// fill every CU, striding over a queue whose length it reads from device
+// memory. Queues are structure-of-arrays records in queue order, appended with
+// one atomic per wave (ballot + popcount prefix), so a kernel's loads of its
+// queue are contiguous. One traversal launch per bounce takes both the
+// extension rays of bounce b and the shadow rays that shading at bounce b-1
+// sent: a shadow ray that reaches its light leaves its contribution pending in
+// its slot, and an escaping ray leaves its environment term pending; shading
+// at the next bounce, the tail or the resolve adds them, so every path's
+// radiance is summed in the reference's order (emission, direct light, next
+// emission ..., environment).
 #include "kernels.hpp"
 
 namespace lotus {
 namespace {
 
-template <int N>
-__device__ __forceinline__ TraversalStack<N> make_stack(uint2* spill, int cap) {
-  TraversalStack<N> st;
-  st.init(spill, gridDim.x * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x, cap);
-  return st;
-}
-
-// Adds a lane's traversal counts to stats[2 kind], stats[2 kind + 1]
-// (wave sum, one atomic per wave).
-template <int N>
-__device__ __forceinline__ void flush_stats(unsigned long long* stats, int kind, const TraversalStack<N>& st) {
-  uint32_t v = st.visits, t = st.tests;
+// Adds per-lane traversal counts to stats[2 kind], stats[2 kind + 1] (wave
+// sum, one atomic per wave).
+__device__ __forceinline__ void flush_stats(unsigned long long* stats, int kind, uint32_t v, uint32_t t) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     v += __shfl_xor(v, off);
     t += __shfl_xor(t, off);
   }
-  if (__lane_id() == 0) {
+  if (__lane_id() == 0 && (v | t) != 0) {
     atomicAdd(stats + 2 * kind, static_cast<unsigned long long>(v));
     atomicAdd(stats + 2 * kind + 1, static_cast<unsigned long long>(t));
   }
 }
 
+// Persistent walks: every lane holds one ray's walk and steps it. A lane
+// whose walk ends waits until a quarter of its wave (or all of it) is ready;
+// then the ready lanes hand their results to finish(item, walk) together and
+// take as many new items from the launch's work counter, with one atomic per
+// wave, so lanes neither wait for the slowest ray of their wave nor pay the
+// hand-over on every step. Every wave leaves the loop together once the queue
+// is drained and its lanes have handed over their results.
+template <class Fetch, class Finish>
+__device__ __forceinline__ void persistent_walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total,
+                                                 uint32_t* work, Fetch fetch, Finish finish) {
+  Walk w;
+  bool working = false;  // a walk in progress
+  bool holding = false;  // a finished walk whose result awaits hand-over
+  bool drained = false;
+  uint32_t item = 0;
+  const uint32_t lane = __lane_id();
+  while (true) {
+    const uint64_t active = __ballot(true);
+    const uint64_t ready = __ballot(!working);
+    const bool last = drained && __ballot(working) == 0;
+    if (ready != 0 && (__popcll(ready) >= kRefill || ready == active || last)) {
+      if (holding) {
+        finish(item, w);
+        holding = false;
+      }
+      if (last) break;
+      if (!drained) {
+        const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(ready)) - 1);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(work, static_cast<uint32_t>(__popcll(ready)));
+        base = __shfl(base, static_cast<int>(leader));
+        if (base + static_cast<uint32_t>(__popcll(ready)) >= total) drained = true;
+        if (!working) {
+          const uint32_t idx = base + static_cast<uint32_t>(__popcll(ready & ((1ull << lane) - 1ull)));
+          if (idx < total) {
+            item = idx;
+            fetch(idx, w);
+            st.clear();
+            working = true;
+          }
+        }
+      }
+    }
+    if (working) {
+      w.step(S.nodes, S.tris, st);
+      if (w.done()) {
+        working = false;
+        holding = true;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
+
+// The same walks without hand-over: each lane follows one ray to its end,
+// striding over the queue.
+template <class Fetch, class Finish>
+__device__ __forceinline__ void strided_walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total, Fetch fetch,
+                                              Finish finish) {
+  for (uint32_t i = gtid(); i < total; i += gstride()) {
+    Walk w;
+    fetch(i, w);
+    st.clear();
+    while (!w.done()) w.step(S.nodes, S.tris, st);
+    finish(i, w);
+  }
+}
+
+template <bool kDynamic, class Fetch, class Finish>
+__device__ __forceinline__ void walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total, uint32_t* work,
+                                      Fetch fetch, Finish finish) {
+  if constexpr (kDynamic) persistent_walks(S, st, total, work, fetch, finish);
+  else strided_walks(S, st, total, fetch, finish);
+}
 
 // Counters per bounce b: [4b] ray-queue entries (camera rays at b = 0),
 // [4b+1] hit-queue entries, [4b+2] shadow-queue entries.
@@ -58,58 +127,47 @@ __device__ __forceinline__ void push_hit(const Queues& q, uint32_t k, uint32_t s
 
 // wavefront.slang:206-225: a camera path per pixel of the data window, its
 // camera ray traced at once; a hit is queued for shading, a miss leaves no
-// sample.
+// sample. Clears the slot's pending terms.
+template <bool kDynamic>
 __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                        uint2* spill, int spill_cap, unsigned long long* stats) {
+                                                        uint32_t* work, uint2* spill, int spill_cap,
+                                                        unsigned long long* stats) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  LdsStack<kRing> st;
+  st.init(ring, spill, spill_cap);
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
-  const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
-  for (uint32_t i = gtid(); i < total; i += gstride()) {
-    const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
-    const uint32_t slot = y * F.width + x;
-    uint32_t rng;
-    const Ray r = start_path(F, x, y, F.sample_index, rng);
-    HitRecord h;
-    const bool hit = tr.closest(r, st, h);
-    wave_count(ray_count(counters, 0), true);
-    const uint32_t k = wave_append(hit_count(counters, 0), hit);
-    if (hit) {
-      push_hit(q, k, slot, r.o, r.d, h);
-      q.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng));
-    } else {
-      q.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-  }
-  flush_stats(stats, 0, st);
+  if (gtid() == 0) *ray_count(counters, 0) = total;
+  uint32_t visits = 0, tests = 0;
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  walks<kDynamic>(
+      S, st, total, work,
+      [&](uint32_t i, Walk& w) {
+        uint32_t rng;
+        w.start(start_path(F, F.rect[0] + i % rw, F.rect[1] + i / rw, F.sample_index, rng), false);
+      },
+      [&](uint32_t i, const Walk& w) {
+        const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
+        const uint32_t slot = y * F.width + x;
+        const bool hit = w.hit();
+        visits += w.visits;
+        tests += w.tests;
+        const uint32_t k = wave_append(hit_count(counters, 0), hit);
+        if (hit) {
+          uint32_t rng;
+          (void)start_path(F, x, y, F.sample_index, rng);  // the path's random state after its film sample
+          push_hit(q, k, slot, w.r.o, w.r.d, w.record(S.tris));
+          q.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng));
+        } else {
+          q.rad[slot] = zero;
+        }
+        q.nee[slot] = zero;
+        q.esc[slot] = zero;
+      });
+  flush_stats(stats, 0, visits, tests);
 }
 
-// wavefront.slang:230-262: continuation rays of bounce b; a miss ends the
-// path with the environment (transport.slang:436-448).
-__global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                           uint2* spill, int spill_cap, uint32_t bounce, unsigned long long* stats) {
-  const uint32_t count = *ray_count(counters, bounce);
-  const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
-  const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
-  for (uint32_t i = gtid(); i < count; i += gstride()) {
-    const float4 o4 = q.ray_o[i], d4 = q.ray_d[i];
-    const uint32_t slot = __float_as_uint(o4.w);
-    const Ray r{xyz(o4), 0.0f, xyz(d4), d4.w};
-    HitRecord h;
-    const bool hit = tr.closest(r, st, h);
-    const uint32_t k = wave_append(hit_count(counters, bounce), hit);
-    if (hit) {
-      push_hit(q, k, slot, r.o, r.d, h);
-    } else {
-      const float4 t4 = q.thr[slot], r4 = q.rad[slot];
-      const V3 rad = xyz(r4) + xyz(t4) * env;
-      q.rad[slot] = make_float4(rad.x, rad.y, rad.z, 1.0f);
-    }
-  }
-  flush_stats(stats, 1, st);
-}
-
-// wavefront.slang:264-282: one scattering event per queued hit.
+// wavefront.slang:264-282: one scattering event per queued hit. The light a
+// shadow ray of the previous bounce found is added first.
 __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                        uint32_t bounce) {
   const uint32_t count = *hit_count(counters, bounce);
@@ -121,6 +179,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
     p.o = xyz(o4);
     p.d = xyz(d4);
     p.depth = bounce;
+    const float4 r4 = q.rad[slot];
+    p.radiance = xyz(r4);
+    p.rng = __float_as_uint(r4.w);
     if (bounce == 0) {
       p.throughput = V3{1.0f, 1.0f, 1.0f};
       p.last_pdf = 0.0f;
@@ -128,10 +189,12 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
       const float4 t4 = q.thr[slot];
       p.throughput = xyz(t4);
       p.last_pdf = t4.w;
+      const float4 n4 = q.nee[slot];
+      if (n4.w != 0.0f) {
+        p.radiance = p.radiance + xyz(n4);
+        q.nee[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
     }
-    const float4 r4 = q.rad[slot];
-    p.radiance = xyz(r4);
-    p.rng = __float_as_uint(r4.w);
     const Scatter s = scatter_path(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z),
                                    __uint_as_float(h4.w));
     const uint32_t ks = wave_append(shadow_count(counters, bounce), s.shadow);
@@ -152,71 +215,106 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
   }
 }
 
-// ShadowTransmittance (scene.slang:445-460) for the shadow queue of bounce
-// b: an unblocked ray adds its light to its path's radiance.
-__global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView S, Queues q, uint32_t* counters, uint2* spill, int spill_cap,
-                                                        uint32_t bounce, unsigned long long* stats) {
-  const uint32_t count = *shadow_count(counters, bounce);
-  const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
-  for (uint32_t i = gtid(); i < count; i += gstride()) {
-    const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
-    const Ray r{xyz(o4), 0.0f, xyz(d4), d4.w};
-    if (!tr.occluded(r, st)) {
-      const uint32_t slot = __float_as_uint(o4.w);
-      const float4 c4 = q.sh_c[i];
-      float4 r4 = q.rad[slot];
-      const V3 rad = xyz(r4) + xyz(c4);
-      r4.x = rad.x;
-      r4.y = rad.y;
-      r4.z = rad.z;
-      q.rad[slot] = r4;
-    }
-  }
-  flush_stats(stats, 2, st);
+// wavefront.slang:230-262 and ShadowTransmittance (scene.slang:445-460) in one
+// grid: items [0, n_ext) are the extension rays of bounce b (a hit is queued
+// for shading, a miss leaves its environment term pending, transport.slang:
+// 436-448), items [n_ext, n_ext + n_shadow) the shadow rays of bounce b - 1
+// (an unblocked ray leaves its light pending).
+template <bool kDynamic>
+__global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, uint32_t* counters, uint32_t* work,
+                                                       uint2* spill, int spill_cap, uint32_t bounce, uint32_t kinds,
+                                                       unsigned long long* stats) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  LdsStack<kRing> st;
+  st.init(ring, spill, spill_cap);
+  const uint32_t n_ext = (kinds & kTraceExtension) ? *ray_count(counters, bounce) : 0u;
+  const uint32_t n_sh = (kinds & kTraceShadow) ? *shadow_count(counters, bounce - 1) : 0u;
+  const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
+  uint32_t visits[2] = {0, 0}, tests[2] = {0, 0};
+  walks<kDynamic>(
+      S, st, n_ext + n_sh, work,
+      [&](uint32_t i, Walk& w) {
+        const bool ext = i < n_ext;
+        const uint32_t j = ext ? i : i - n_ext;
+        const float4 o4 = ext ? q.ray_o[j] : q.sh_o[j];
+        const float4 d4 = ext ? q.ray_d[j] : q.sh_d[j];
+        w.start(Ray{xyz(o4), 0.0f, xyz(d4), d4.w}, !ext);
+      },
+      [&](uint32_t i, const Walk& w) {
+        const bool ext = i < n_ext;
+        const uint32_t j = ext ? i : i - n_ext;
+        const uint32_t slot = __float_as_uint(ext ? q.ray_o[j].w : q.sh_o[j].w);
+        const bool hit = w.hit();
+        visits[ext ? 0 : 1] += w.visits;
+        tests[ext ? 0 : 1] += w.tests;
+        const uint32_t k = wave_append(hit_count(counters, bounce), ext && hit);
+        if (ext && hit) {
+          push_hit(q, k, slot, w.r.o, w.r.d, w.record(S.tris));
+        } else if (ext) {
+          const V3 e = xyz(q.thr[slot]) * env;
+          q.esc[slot] = make_float4(e.x, e.y, e.z, 1.0f);
+          float4 r4 = q.rad[slot];
+          r4.w = 1.0f;
+          q.rad[slot] = r4;
+        } else if (!hit) {
+          const float4 c4 = q.sh_c[j];
+          q.nee[slot] = make_float4(c4.x, c4.y, c4.z, 1.0f);
+        }
+      });
+  flush_stats(stats, 1, visits[0], tests[0]);
+  flush_stats(stats, 2, visits[1], tests[1]);
 }
 
 // wavefront.slang:287-309: the paths still queued after the recorded rounds,
 // each followed to its end in one invocation, shadow rays traced inline.
 __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                      uint2* spill, int spill_cap, uint32_t bounce, unsigned long long* stats) {
+                                                      uint2* spill, int spill_cap, uint32_t bounce,
+                                                      unsigned long long* stats) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  LdsStack<kRing> st;
+  st.init(ring, spill, spill_cap);
   const uint32_t count = *ray_count(counters, bounce);
-  const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
   const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
+  uint32_t visits = 0, tests = 0;
   for (uint32_t i = gtid(); i < count; i += gstride()) {
     const float4 o4 = q.ray_o[i], d4 = q.ray_d[i];
     const uint32_t slot = __float_as_uint(o4.w);
-    const float4 t4 = q.thr[slot], r4 = q.rad[slot];
+    const float4 t4 = q.thr[slot], r4 = q.rad[slot], n4 = q.nee[slot];
     PathRegs p;
     p.o = xyz(o4);
     p.d = xyz(d4);
     p.throughput = xyz(t4);
     p.last_pdf = t4.w;
     p.radiance = xyz(r4);
+    if (n4.w != 0.0f) {
+      p.radiance = p.radiance + xyz(n4);
+      q.nee[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     p.rng = __float_as_uint(r4.w);
     p.depth = bounce;
     while (true) {
       HitRecord h;
-      if (!tr.closest(Ray{p.o, 0.0f, p.d, kFloatMax}, st, h)) {
+      if (!walk_closest(S.nodes, S.tris, st, Ray{p.o, 0.0f, p.d, kFloatMax}, h, visits, tests)) {
         p.radiance = p.radiance + p.throughput * env;
         break;
       }
       const Scatter s = scatter_path(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v);
       if (s.shadow) {
         atomicAdd(shadow_count(counters, p.depth - (s.continues ? 1u : 0u)), 1u);
-        if (!tr.occluded(s.shadow_ray, st)) p.radiance = p.radiance + s.delta;
+        if (!walk_occluded(S.nodes, S.tris, st, s.shadow_ray, visits, tests))
+          p.radiance = p.radiance + s.delta;
       }
       if (!s.continues) break;
       atomicAdd(ray_count(counters, p.depth), 1u);
     }
     q.rad[slot] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 1.0f);
   }
-  flush_stats(stats, 3, st);
+  flush_stats(stats, 3, visits, tests);
 }
 
-// path_trace.slang:106-139 (kResolvePass): adds each slot's sample to the
-// pixel's accumulation and writes the mean, a missed sample counting as the
+// path_trace.slang:106-139 (kResolvePass): each slot's sample (its radiance,
+// then the pending direct light, then the pending environment) added to the
+// pixel's accumulation, and the mean written, a missed sample counting as the
 // background; pixels no sample has hit keep the clear colour.
 __global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_t add_sample) {
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
@@ -225,10 +323,13 @@ __global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_
     const uint32_t slot = y * F.width + x;
     float4 sum = F.accumulated != 0 ? q.accum[slot] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (add_sample) {
-      const float4 s = q.rad[slot];
-      sum.x = sum.x + s.x;
-      sum.y = sum.y + s.y;
-      sum.z = sum.z + s.z;
+      const float4 s = q.rad[slot], n = q.nee[slot], e = q.esc[slot];
+      V3 v = xyz(s);
+      if (n.w != 0.0f) v = v + xyz(n);
+      if (e.w != 0.0f) v = v + xyz(e);
+      sum.x = sum.x + v.x;
+      sum.y = sum.y + v.y;
+      sum.z = sum.z + v.z;
       sum.w = sum.w + s.w;
       q.accum[slot] = sum;
     }
@@ -248,9 +349,11 @@ __global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_
 // clears (colour = background, depth = 1).
 __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameConst F, Queues q, float* depth,
                                                              uint32_t write_color, uint2* spill, int spill_cap) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  LdsStack<kRing> st;
+  st.init(ring, spill, spill_cap);
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
-  const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
+  uint32_t visits = 0, tests = 0;
   for (uint32_t i = gtid(); i < total; i += gstride()) {
     const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
     const uint32_t slot = y * F.width + x;
@@ -258,7 +361,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameC
     const Ray r = camera_ray(F.clip_to_world, fx * F.ndc_from_pixel[0] + F.ndc_from_pixel[2],
                              fy * F.ndc_from_pixel[1] + F.ndc_from_pixel[3]);
     HitRecord h;
-    if (!tr.closest(r, st, h)) continue;
+    if (!walk_closest(S.nodes, S.tris, st, r, h, visits, tests)) continue;
     const V3 pos = r.o + r.d * h.t;
     float clip[4];
     mul_point(F.world_to_clip, pos.x, pos.y, pos.z, 1.0f, clip);
@@ -278,33 +381,39 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameC
 }
 
 // Trace / ShadowTransmittance over caller rays (8 floats in, 5 words or 1 out).
-__global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, const float4* rays, uint32_t* hits,
-                                                       uint32_t count, uint32_t any_hit, uint2* spill, int spill_cap) {
-  const Traverser<kStackRegs> tr{S.nodes, S.tris};
-  TraversalStack<kStackRegs> st = make_stack<kStackRegs>(spill, spill_cap);
-  for (uint32_t i = gtid(); i < count; i += gstride()) {
-    const float4 o4 = rays[2 * i], d4 = rays[2 * i + 1];
-    const Ray r{xyz(o4), o4.w, xyz(d4), d4.w};
-    if (any_hit) {
-      hits[i] = tr.occluded(r, st) ? 1u : 0u;
-      continue;
-    }
-    HitRecord h;
-    uint32_t* o = hits + 5ull * i;
-    if (tr.closest(r, st, h)) {
-      o[0] = h.instance;
-      o[1] = h.primitive;
-      o[2] = __float_as_uint(h.u);
-      o[3] = __float_as_uint(h.v);
-      o[4] = __float_as_uint(h.t);
-    } else {
-      o[0] = kMiss;
-      o[1] = kMiss;
-      o[2] = 0u;
-      o[3] = 0u;
-      o[4] = 0u;
-    }
-  }
+__global__ void __launch_bounds__(kTraceBlock) k_rays(SceneView S, const float4* rays, uint32_t* hits,
+                                                      uint32_t count, uint32_t any_hit, uint32_t* work, uint2* spill,
+                                                      int spill_cap) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  LdsStack<kRing> st;
+  st.init(ring, spill, spill_cap);
+  persistent_walks(
+      S, st, count, work,
+      [&](uint32_t i, Walk& w) {
+        const float4 o4 = rays[2 * i], d4 = rays[2 * i + 1];
+        w.start(Ray{xyz(o4), o4.w, xyz(d4), d4.w}, any_hit != 0);
+      },
+      [&](uint32_t i, const Walk& w) {
+        if (any_hit) {
+          hits[i] = w.hit() ? 1u : 0u;
+          return;
+        }
+        uint32_t* o = hits + 5ull * i;
+        if (w.hit()) {
+          const HitRecord h = w.record(S.tris);
+          o[0] = h.instance;
+          o[1] = h.primitive;
+          o[2] = __float_as_uint(h.u);
+          o[3] = __float_as_uint(h.v);
+          o[4] = __float_as_uint(h.t);
+        } else {
+          o[0] = kMiss;
+          o[1] = kMiss;
+          o[2] = 0u;
+          o[3] = 0u;
+          o[4] = 0u;
+        }
+      });
 }
 
 __global__ void k_zero(uint32_t* p, uint32_t n) {
@@ -319,46 +428,72 @@ __global__ void k_fill1(float* p, uint32_t n, float v) {
   for (uint32_t i = gtid(); i < n; i += gstride()) p[i] = v;
 }
 
+// Workgroups that fill every CU at the kernel's occupancy.
+template <typename K>
+uint32_t persistent_blocks(K kernel, int block, uint32_t cus) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+    per_cu = 2;
+  return cus * static_cast<uint32_t>(per_cu);
+}
+
+uint32_t small_grid(uint32_t n) { return (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024; }
+
 }  // namespace
 
-void launch_zero(uint32_t* p, uint32_t n, hipStream_t s) {
-  k_zero<<<(n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024, 256, 0, s>>>(p, n);
+GridShape grid_shape(uint32_t cus) {
+  GridShape g;
+  g.camera = std::max(persistent_blocks(k_camera<true>, kTraceBlock, cus),
+                      persistent_blocks(k_camera<false>, kTraceBlock, cus));
+  g.trace = std::max(persistent_blocks(k_trace<true>, kTraceBlock, cus),
+                     persistent_blocks(k_trace<false>, kTraceBlock, cus));
+  g.tail = persistent_blocks(k_tail, kTraceBlock, cus);
+  g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
+  g.rays = persistent_blocks(k_rays, kTraceBlock, cus);
+  g.shade = persistent_blocks(k_shade, kShadeBlock, cus);
+  g.resolve = cus * 4;
+  g.max_trace_lanes = std::max({g.camera, g.trace, g.tail, g.camera_pass, g.rays}) * kTraceBlock;
+  return g;
 }
-void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s) {
-  k_fill4<<<(n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024, 256, 0, s>>>(p, n, v);
-}
-void launch_fill1(float* p, uint32_t n, float v, hipStream_t s) {
-  k_fill1<<<(n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024, 256, 0, s>>>(p, n, v);
-}
+
+void launch_zero(uint32_t* p, uint32_t n, hipStream_t s) { k_zero<<<small_grid(n), 256, 0, s>>>(p, n); }
+void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s) { k_fill4<<<small_grid(n), 256, 0, s>>>(p, n, v); }
+void launch_fill1(float* p, uint32_t n, float v, hipStream_t s) { k_fill1<<<small_grid(n), 256, 0, s>>>(p, n, v); }
 void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
-  k_camera<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
-}
-void launch_intersect(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
-                      uint32_t bounce) {
-  k_intersect<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+  if (L.dynamic_camera)
+    k_camera<true><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.work, L.spill, L.spill_cap, L.stats);
+  else
+    k_camera<false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.work, L.spill, L.spill_cap, L.stats);
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce) {
-  k_shade<<<L.shade_blocks, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce);
+  k_shade<<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce);
 }
-void launch_shadow(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce) {
-  k_shadow<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, L.stats);
+void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
+                  uint32_t kinds) {
+  if (L.dynamic_trace)
+    k_trace<true><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.work, L.spill, L.spill_cap, bounce, kinds,
+                                                                L.stats);
+  else
+    k_trace<false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.work, L.spill, L.spill_cap, bounce, kinds,
+                                                                 L.stats);
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce) {
-  k_tail<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+  k_tail<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
 }
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample) {
-  k_resolve<<<L.shade_blocks, 256, 0, L.stream>>>(F, q, add_sample);
+  k_resolve<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, add_sample);
 }
 void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, float* depth,
                         uint32_t write_color) {
-  k_camera_pass<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, F, q, depth, write_color, L.spill, L.spill_cap);
+  k_camera_pass<<<L.grid.camera_pass, kTraceBlock, 0, L.stream>>>(S, F, q, depth, write_color, L.spill,
+                                                                   L.spill_cap);
 }
-void launch_trace(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
-                  uint32_t any_hit) {
-  k_trace<<<L.trace_blocks, kTraceBlock, 0, L.stream>>>(S, reinterpret_cast<const float4*>(rays), hits, count,
-                                                         any_hit, L.spill, L.spill_cap);
+void launch_rays(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
+                 uint32_t any_hit) {
+  k_rays<<<L.grid.rays, kTraceBlock, 0, L.stream>>>(S, reinterpret_cast<const float4*>(rays), hits, count, any_hit,
+                                                     L.work, L.spill, L.spill_cap);
 }
 
 }  // namespace lotus

@@ -1,19 +1,25 @@
 // Launch interface of the wavefront kernels (kernels.hip).
 #pragma once
 
+#include <algorithm>
+
 #include "transport.hpp"
 
 namespace lotus {
 
-// Registers of the traversal stack before it spills, and block sizes.
-constexpr int kStackRegs = 8;
+// LDS ring entries of a lane's traversal stack before it spills; idle lanes
+// of a wave that trigger a refill; block sizes.
+constexpr int kRing = 12;
+constexpr int kRefill = 16;
 constexpr int kTraceBlock = 256;
 constexpr int kShadeBlock = 256;
 
 // Per-target path state and queues, P = width * height entries each.
 struct Queues {
   float4* thr;    // per slot: throughput, last pdf (< 0: unweighted)
-  float4* rad;    // per slot: radiance, rng bits; once ended, sample and weight
+  float4* rad;    // per slot: radiance, rng bits; once ended, radiance and weight
+  float4* nee;    // per slot: a shadow ray's light awaiting addition, flag
+  float4* esc;    // per slot: the environment an escaping ray found, flag
   float4* ray_o;  // ray queue: origin, slot
   float4* ray_d;  //            direction, tmax
   float4* hit_o;  // hit queue: ray origin, slot
@@ -26,30 +32,43 @@ struct Queues {
   float4* color;  // per pixel: mean (RGBA32F product)
 };
 
+// Persistent grid of each kernel: workgroups filling every CU at its occupancy.
+struct GridShape {
+  uint32_t camera, trace, tail, camera_pass, rays, shade, resolve;
+  uint32_t max_trace_lanes;  // the most traversal lanes any grid holds
+};
+GridShape grid_shape(uint32_t cus);
+
 struct Launch {
   hipStream_t stream;
-  uint32_t trace_blocks;  // persistent grid of the traversal kernels
-  uint32_t shade_blocks;
-  uint2* spill;           // traversal stack spill, trace_blocks * kTraceBlock lanes
-  int spill_cap;          // spill entries per lane
-  unsigned long long* stats;  // node visits, triangle tests: camera, intersect, shadow, tail
+  GridShape grid;
+  uint2* spill;               // traversal stack spill, max_trace_lanes lanes
+  int spill_cap;              // spill entries per lane
+  unsigned long long* stats;  // node visits, triangle tests: camera, extension, shadow, tail
+  uint32_t* work;             // this launch's work counter (zero before it)
+  bool dynamic_camera;        // lanes refill from the queue (else one ray per lane per pass)
+  bool dynamic_trace;
 };
+
+// k_trace item kinds.
+constexpr uint32_t kTraceExtension = 1u;
+constexpr uint32_t kTraceShadow = 2u;
 
 void launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s);
 void launch_fill1(float* p, uint32_t n, float v, hipStream_t s);
 void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c);
-void launch_intersect(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
-                      uint32_t bounce);
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce);
-void launch_shadow(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce);
+// Extension rays of bounce `bounce` and/or shadow rays of bounce `bounce - 1`.
+void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
+                  uint32_t kinds);
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce);
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample);
 void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, float* depth,
                         uint32_t write_color);
-void launch_trace(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
-                  uint32_t any_hit);
+void launch_rays(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
+                 uint32_t any_hit);
 
 }  // namespace lotus

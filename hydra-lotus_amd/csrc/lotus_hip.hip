@@ -83,20 +83,26 @@ bool invert(const float* m, float* inverse) {
   return true;
 }
 
-// vulkan_backend.cpp:369-385: bounces whose queue held at least 1/16 of the
-// camera rays run as rounds; the tail follows the rest.
+// The rounds after a sample whose ray queues held `counts` paths per bounce
+// (the rule of vulkan_backend.cpp:369-385 with its threshold lowered): every
+// bounce up to the last that traced a ray, plus one, runs as a wavefront
+// round. A round here costs two launches of about 10 us, while the megakernel
+// tail runs its paths' bounces serially at low occupancy, so only paths longer
+// than any of the previous sample's, and than kMaxRounds, go to the tail.
 uint32_t wavefront_rounds(const std::vector<uint32_t>& counts) {
-  const uint32_t camera = counts.empty() ? 0 : counts.front();
-  const uint32_t threshold = std::max(1u, (camera + 15u) / 16u);
   uint32_t rounds = 1;
-  while (rounds < counts.size() && counts[rounds] >= threshold) ++rounds;
-  return rounds;
+  for (uint32_t b = 1; b < counts.size(); ++b)
+    if (counts[b] != 0) rounds = b + 1;
+  return rounds + 1;
 }
 
 // Rounds of a sample with no previous one to go by, and the most a sample
 // records (the tail follows longer paths; the event pool holds their timers).
 constexpr uint32_t kFirstRounds = 32;
-constexpr uint32_t kMaxRounds = 128;
+constexpr uint32_t kMaxRounds = 64;
+// Work counters of a frame's persistent launches: the camera kernel and at
+// most two trace launches per round.
+constexpr uint32_t kWorkCounters = 2 * kMaxRounds + 8;
 
 struct AccumulationKey {  // vulkan_backend.cpp:419-434
   std::array<float, 16> world_to_clip{};
@@ -112,7 +118,7 @@ struct AccumulationKey {  // vulkan_backend.cpp:419-434
   }
 };
 
-enum Timer { kTCamera, kTIntersect, kTShade, kTShadow, kTTail, kTResolve, kTimers };
+enum Timer { kTCamera, kTIntersect, kTShade, kTShadow, kTTrace, kTTail, kTResolve, kTimers };
 
 }  // namespace
 
@@ -134,10 +140,12 @@ struct lotus_hip_renderer {
   uint32_t* counters = nullptr;
   uint32_t counter_words = 0;
   unsigned long long* stats = nullptr;  // 8: node visits / triangle tests per traversal kernel
+  uint32_t* work = nullptr;              // kWorkCounters: one per persistent launch of a frame
   float* depth = nullptr;
   uint2* spill = nullptr;
   size_t spill_entries = 0;  // per lane
-  uint32_t trace_blocks = 0, shade_blocks = 0;
+  GridShape grid{};
+  lotus_hip_tuning tuning{0u, 0u};
 
   AccumulationKey key;
   bool key_valid = false;
@@ -153,6 +161,8 @@ void free_target(lotus_hip_renderer* r) {
   Queues& q = r->q;
   dev_free(q.thr);
   dev_free(q.rad);
+  dev_free(q.nee);
+  dev_free(q.esc);
   dev_free(q.ray_o);
   dev_free(q.ray_d);
   dev_free(q.hit_o);
@@ -183,14 +193,19 @@ void free_scene(lotus_hip_renderer* r) {
 }
 
 // Persistent grids and the traversal stacks' spill area, sized for the BVH's depth.
+// A nearest-first walk of a 4-wide tree keeps at most 3 entries per level.
 hipError_t ensure_launch(lotus_hip_renderer* r) {
-  r->trace_blocks = r->cus * 4;
-  r->shade_blocks = r->cus * 4;
-  const size_t need = std::max<size_t>(4, 3 * size_t(r->scene.depth) + 4 - std::min<size_t>(kStackRegs, 3 * size_t(r->scene.depth) + 4));
+  const size_t stack = 3 * size_t(r->scene.depth) + 4;
+  const size_t need = std::max<size_t>(4, stack > size_t(kRing) ? stack - kRing : 0);
   if (r->spill && r->spill_entries >= need) return hipSuccess;
   dev_free(r->spill);
   r->spill_entries = need;
-  return dev_alloc(r->spill, need * size_t(r->trace_blocks) * kTraceBlock);
+  return dev_alloc(r->spill, need * size_t(r->grid.max_trace_lanes));
+}
+
+Launch launch_for(lotus_hip_renderer* r, hipStream_t stream) {
+  return Launch{stream, r->grid, r->spill, static_cast<int>(r->spill_entries), r->stats, r->work,
+                r->tuning.camera_dynamic != 0, r->tuning.trace_dynamic != 0};
 }
 
 hipError_t ensure_target(lotus_hip_renderer* r, uint32_t width, uint32_t height) {
@@ -199,7 +214,7 @@ hipError_t ensure_target(lotus_hip_renderer* r, uint32_t width, uint32_t height)
   const size_t P = size_t(width) * height;
   Queues& q = r->q;
   hipError_t e = hipSuccess;
-  for (float4** p : {&q.thr, &q.rad, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.accum,
+  for (float4** p : {&q.thr, &q.rad, &q.nee, &q.esc, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.accum,
                      &q.color})
     if (e == hipSuccess) e = dev_alloc(*p, P);
   if (e == hipSuccess) e = dev_alloc(q.hit_h, P);
@@ -374,12 +389,14 @@ lotus_hip_renderer* lotus_hip_create(int device, int* status, char* detail, size
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
   if (e == hipSuccess) {
     r->cus = static_cast<uint32_t>(prop.multiProcessorCount);
+    r->grid = grid_shape(r->cus);
     r->events.resize(1024);
     for (auto& ev : r->events)
       if (e == hipSuccess) e = hipEventCreate(&ev);
   }
   if (e == hipSuccess) e = dev_alloc(r->counters, 4ull * (LOTUS_HIP_MAX_BOUNCES + 4));
   if (e == hipSuccess) e = dev_alloc(r->stats, 8);
+  if (e == hipSuccess) e = dev_alloc(r->work, kWorkCounters);
   r->counter_words = 4u * (LOTUS_HIP_MAX_BOUNCES + 4);
   if (e != hipSuccess) {
     if (status) *status = LOTUS_HIP_FAIL;
@@ -400,6 +417,7 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   free_scene(r);
   dev_free(r->counters);
   dev_free(r->stats);
+  dev_free(r->work);
   dev_free(r->spill);
   for (auto& ev : r->events)
     if (ev) (void)hipEventDestroy(ev);
@@ -597,13 +615,14 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     return LOTUS_HIP_FAIL;
   }
   const uint32_t npix = P->width * P->height;
-  Launch L{r->stream, r->trace_blocks, r->shade_blocks, r->spill, static_cast<int>(r->spill_entries), r->stats};
+  Launch L = launch_for(r, r->stream);
+  const bool split = (P->flags & LOTUS_HIP_FRAME_SPLIT_KINDS) != 0;
   launch_zero(reinterpret_cast<uint32_t*>(r->stats), 16, r->stream);
   launch_fill4(r->q.color, npix, make_float4(P->clear_color[0], P->clear_color[1], P->clear_color[2], P->clear_color[3]),
                r->stream);
   if (res && res->depth) launch_fill1(r->depth, npix, 1.0f, r->stream);
 
-  double t_ms[kTimers] = {0, 0, 0, 0, 0, 0};
+  double t_ms[kTimers] = {0, 0, 0, 0, 0, 0, 0};
   double frame_ms = 0.0;
   uint32_t added = 0;
   std::vector<uint32_t> counts;
@@ -645,22 +664,34 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       };
       auto done = [&]() { (void)hipEventRecord(r->events[ev++], r->stream); };
       launch_zero(r->counters, 4 * stride, r->stream);
+      launch_zero(r->work, kWorkCounters, r->stream);
+      uint32_t next_work = 0;  // each persistent launch takes its own work counter
       if (add) {
         mark(kTCamera);
+        L.work = r->work + next_work++;
         launch_camera(L, r->view, F, r->q, r->counters);
         done();
+        // round b: shade bounce b, then trace the extension rays of bounce
+        // b + 1 (unless the tail takes them) with the shadow rays of bounce b
         for (uint32_t b = 0; b < rounds; ++b) {
-          if (b >= 1) {
-            mark(kTIntersect);
-            launch_intersect(L, r->view, F, r->q, r->counters, b);
-            done();
-          }
           mark(kTShade);
           launch_shade(L, r->view, F, r->q, r->counters, b);
           done();
-          if (P->light_sampling != 0) {
-            mark(kTShadow);
-            launch_shadow(L, r->view, r->q, r->counters, b);
+          const uint32_t kinds = (b + 1 < rounds ? kTraceExtension : 0u) |
+                                 (P->light_sampling != 0 ? kTraceShadow : 0u);
+          if (kinds == 0) continue;
+          if (split) {
+            for (const uint32_t kind : {kTraceExtension, kTraceShadow}) {
+              if (!(kinds & kind)) continue;
+              mark(kind == kTraceExtension ? kTIntersect : kTShadow);
+              L.work = r->work + next_work++;
+              launch_trace(L, r->view, r->q, r->counters, b + 1, kind);
+              done();
+            }
+          } else {
+            mark(kTTrace);
+            L.work = r->work + next_work++;
+            launch_trace(L, r->view, r->q, r->counters, b + 1, kinds);
             done();
           }
         }
@@ -742,7 +773,9 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     res->extension_rays = extension_rays;
     res->shadow_rays = shadow_rays;
     res->camera_gpu_ms = t_ms[kTCamera];
-    res->intersect_gpu_ms = t_ms[kTIntersect] + t_ms[kTTail];
+    res->intersect_gpu_ms = t_ms[kTIntersect];
+    res->trace_gpu_ms = t_ms[kTTrace];
+    res->tail_gpu_ms = t_ms[kTTail];
     res->shade_gpu_ms = t_ms[kTShade];
     res->shadow_gpu_ms = t_ms[kTShadow];
     res->resolve_gpu_ms = t_ms[kTResolve];
@@ -773,9 +806,9 @@ int lotus_hip_trace_device(lotus_hip_renderer* r, uint32_t count, const void* ra
     return LOTUS_HIP_FAIL;
   }
   (void)hipSetDevice(r->device);
-  Launch L{stream ? static_cast<hipStream_t>(stream) : r->stream, r->trace_blocks, r->shade_blocks, r->spill,
-           static_cast<int>(r->spill_entries), r->stats};
-  launch_trace(L, r->view, static_cast<const float*>(rays), static_cast<uint32_t*>(hits), count,
+  Launch L = launch_for(r, stream ? static_cast<hipStream_t>(stream) : r->stream);
+  launch_zero(r->work, 1, L.stream);
+  launch_rays(L, r->view, static_cast<const float*>(rays), static_cast<uint32_t*>(hits), count,
                any_hit ? 1u : 0u);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -813,6 +846,22 @@ int lotus_hip_trace(lotus_hip_renderer* r, uint32_t count, const float* rays, ui
     return LOTUS_HIP_FAIL;
   }
   set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_set_tuning(lotus_hip_renderer* r, const lotus_hip_tuning* tuning, char* detail, size_t detail_size) {
+  if (!r || !tuning) {
+    set_detail(detail, detail_size, "no renderer or tuning");
+    return LOTUS_HIP_FAIL;
+  }
+  r->tuning = *tuning;
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_get_tuning(lotus_hip_renderer* r, lotus_hip_tuning* tuning) {
+  if (!r || !tuning) return LOTUS_HIP_FAIL;
+  *tuning = r->tuning;
   return LOTUS_HIP_PASS;
 }
 

@@ -120,70 +120,60 @@ __device__ __forceinline__ float safe_inverse(float d) {
   return 1.0f / (fabsf(d) > 1.0e-30f ? d : copysignf(1.0e-30f, d));
 }
 
-// Register stack of N (node, distance) entries with spill to a lane-
-// interleaved global area when full (entry k of lane l at spill[k * lanes + l]).
-template <int N>
-struct TraversalStack {
-  uint32_t node[N];
-  float dist[N];
-  int count;
-  int spilled;
+// Traversal stack of one lane: a ring of S (node, distance) entries in LDS,
+// laid out entry-major ([S][block lanes]) so a wave's pushes and pops touch
+// consecutive words, spilling its oldest entry to a lane-interleaved global
+// area (entry k of lane l at spill[k * lanes + l]) when the ring is full.
+template <int S>
+struct LdsStack {
+  uint2* ring;      // this block's ring: [S][blockDim.x]
+  uint32_t slot;    // threadIdx.x
+  uint32_t stride;  // blockDim.x
   uint2* spill;
-  uint32_t lanes;
-  uint32_t lane;
-  uint32_t visits;  // wide nodes tested
-  uint32_t tests;   // triangles tested
+  uint32_t lanes, lane;
   int capacity;     // spill entries per lane
+  int base, count, spilled;
 
-  __device__ __forceinline__ void init(uint2* spill_area, uint32_t lane_count, uint32_t lane_id, int cap) {
+  __device__ __forceinline__ void init(uint2* lds, uint2* spill_area, int cap) {
+    ring = lds;
+    slot = threadIdx.x;
+    stride = blockDim.x;
+    spill = spill_area;
+    lanes = gridDim.x * blockDim.x;
+    lane = blockIdx.x * blockDim.x + threadIdx.x;
+    capacity = cap;
+    clear();
+  }
+  __device__ __forceinline__ void clear() {
+    base = 0;
     count = 0;
     spilled = 0;
-    capacity = cap;
-    visits = 0;
-    tests = 0;
-    spill = spill_area;
-    lanes = lane_count;
-    lane = lane_id;
   }
+  __device__ __forceinline__ uint2& at(int i) { return ring[(i >= S ? i - S : i) * stride + slot]; }
   __device__ __forceinline__ void push(uint32_t n, float d) {
-    if (count == N) {
+    if (count == S) {
       // The area holds 3 x (tree depth) entries, the most a nearest-first
       // walk of a 4-wide tree keeps; the bound keeps every store in it.
-      if (spilled < capacity) {
-        spill[static_cast<size_t>(spilled) * lanes + lane] = make_uint2(node[N - 1], __float_as_uint(dist[N - 1]));
-        ++spilled;
-      }
-    } else {
-      ++count;
+      if (spilled < capacity) spill[static_cast<size_t>(spilled++) * lanes + lane] = at(base);
+      base = base + 1 == S ? 0 : base + 1;
+      --count;
     }
-#pragma unroll
-    for (int i = N - 1; i > 0; --i) {
-      node[i] = node[i - 1];
-      dist[i] = dist[i - 1];
-    }
-    node[0] = n;
-    dist[0] = d;
+    at(base + count) = make_uint2(n, __float_as_uint(d));
+    ++count;
   }
   __device__ __forceinline__ bool pop(uint32_t& n, float& d) {
+    uint2 e;
     if (count > 0) {
-      n = node[0];
-      d = dist[0];
-#pragma unroll
-      for (int i = 0; i < N - 1; ++i) {
-        node[i] = node[i + 1];
-        dist[i] = dist[i + 1];
-      }
       --count;
-      return true;
+      e = at(base + count);
+    } else if (spilled > 0) {
+      e = spill[static_cast<size_t>(--spilled) * lanes + lane];
+    } else {
+      return false;
     }
-    if (spilled > 0) {
-      --spilled;
-      const uint2 e = spill[static_cast<size_t>(spilled) * lanes + lane];
-      n = e.x;
-      d = __uint_as_float(e.y);
-      return true;
-    }
-    return false;
+    n = e.x;
+    d = __uint_as_float(e.y);
+    return true;
   }
 };
 
@@ -228,129 +218,146 @@ __device__ __forceinline__ void sort4(float t[4], uint32_t c[4]) {
 #undef LOTUS_CSWAP
 }
 
-template <int N>
-struct Traverser {
-  const WideNode* __restrict__ nodes;
-  const Tri* __restrict__ tris;
+constexpr uint32_t kWalkDone = 0xFFFFFFFEu;  // no node left (never a node or leaf word)
 
-  // Closest hit; ties in t go to the lowest flattened triangle id, so the
-  // result does not depend on the order in which nodes are visited.
-  __device__ __forceinline__ bool closest(const Ray& r, TraversalStack<N>& st, HitRecord& hit) const {
-    const RayPre pre = precompute(r);
-    const V3 inv = V3{safe_inverse(r.d.x), safe_inverse(r.d.y), safe_inverse(r.d.z)};
-    float best_t = r.tmax;
-    uint32_t best_g = kMiss, best_tri = kMiss;
-    float best_u = 0.0f, best_v = 0.0f;
-    uint32_t cur = 0;
-    float cur_t = r.tmin;
-    bool alive = true;
-    uint32_t guard = 0;
-    st.count = 0;
-    st.spilled = 0;
-    while (alive && ++guard < kMaxSteps) {
-      if ((cur & kLeafBit) == 0) {
-        float tn[4];
-        uint32_t ch[4];
-        test_children(nodes, cur, r.o, inv, r.tmin, best_t, tn, ch);
-        ++st.visits;
-        sort4(tn, ch);
-        if (tn[0] == INFINITY) {
-          alive = false;
-        } else {
-          if (tn[3] != INFINITY) st.push(ch[3], tn[3]);
-          if (tn[2] != INFINITY) st.push(ch[2], tn[2]);
-          if (tn[1] != INFINITY) st.push(ch[1], tn[1]);
-          cur = ch[0];
-          cur_t = tn[0];
-          continue;
-        }
-      } else {
-        const uint32_t first = leaf_first(cur), n = leaf_count(cur);
-        st.tests += n;
-        for (uint32_t k = 0; k < n; ++k) {
-          const Tri tr = tris[first + k];
-          float t, u, v;
-          if (intersect_triangle(r, pre, xyz(tr.a), xyz(tr.b), xyz(tr.c), t, u, v)) {
-            const uint32_t g = __float_as_uint(tr.a.w);
-            if (t < best_t || (t == best_t && g < best_g)) {
-              best_t = t;
-              best_g = g;
-              best_tri = first + k;
-              best_u = u;
-              best_v = v;
-            }
-          }
-        }
-        alive = false;
-      }
-      // next entry no farther than the best hit
-      uint32_t n;
-      float d;
-      while (st.pop(n, d)) {
-        if (d <= best_t) {
-          cur = n;
-          cur_t = d;
-          alive = true;
-          break;
-        }
+// One ray's resumable walk of the wide BVH: each step tests one node's four
+// children (nearest-first descent) or one leaf's triangles. Closest hit:
+// ties in t go to the lowest flattened triangle id, so the result does not
+// depend on the order nodes are visited. Any hit (ShadowTransmittance for
+// opaque surfaces): the walk ends at the first triangle in [tmin, tmax].
+struct Walk {
+  Ray r;
+  RayPre pre;
+  V3 inv;
+  uint32_t cur;
+  float best_t;
+  uint32_t best_g, best_tri;
+  float best_u, best_v;
+  bool any;
+  uint32_t steps;
+  uint32_t visits;  // wide nodes tested
+  uint32_t tests;   // triangles tested
+
+  __device__ __forceinline__ void start(const Ray& ray, bool any_hit) {
+    r = ray;
+    pre = precompute(ray);
+    inv = V3{safe_inverse(ray.d.x), safe_inverse(ray.d.y), safe_inverse(ray.d.z)};
+    cur = 0;
+    best_t = ray.tmax;
+    best_g = kMiss;
+    best_tri = kMiss;
+    best_u = best_v = 0.0f;
+    any = any_hit;
+    steps = 0;
+    visits = 0;
+    tests = 0;
+  }
+  __device__ __forceinline__ bool done() const { return cur == kWalkDone; }
+  __device__ __forceinline__ bool hit() const { return best_g != kMiss; }
+
+  // The next entry no farther than the best hit (any entry for any hit).
+  template <int S>
+  __device__ __forceinline__ void next(LdsStack<S>& st) {
+    uint32_t n;
+    float d;
+    while (st.pop(n, d)) {
+      if (d <= best_t) {
+        cur = n;
+        return;
       }
     }
-    (void)cur_t;
-    if (best_g == kMiss) return false;
-    const Tri tr = tris[best_tri];
-    hit.instance = __float_as_uint(tr.b.w);
-    hit.primitive = __float_as_uint(tr.c.w);
-    hit.tri = best_tri;
-    hit.u = best_u;
-    hit.v = best_v;
-    hit.t = best_t;
-    return true;
+    cur = kWalkDone;
   }
 
-  // Any hit in [tmin, tmax] (ShadowTransmittance for opaque surfaces).
-  __device__ __forceinline__ bool occluded(const Ray& r, TraversalStack<N>& st) const {
-    const RayPre pre = precompute(r);
-    const V3 inv = V3{safe_inverse(r.d.x), safe_inverse(r.d.y), safe_inverse(r.d.z)};
-    uint32_t cur = 0;
-    bool alive = true;
-    uint32_t guard = 0;
-    st.count = 0;
-    st.spilled = 0;
-    while (alive && ++guard < kMaxSteps) {
-      if ((cur & kLeafBit) == 0) {
-        float tn[4];
-        uint32_t ch[4];
-        test_children(nodes, cur, r.o, inv, r.tmin, r.tmax, tn, ch);
-        ++st.visits;
-        sort4(tn, ch);
-        if (tn[0] != INFINITY) {
-          if (tn[3] != INFINITY) st.push(ch[3], tn[3]);
-          if (tn[2] != INFINITY) st.push(ch[2], tn[2]);
-          if (tn[1] != INFINITY) st.push(ch[1], tn[1]);
-          cur = ch[0];
-          continue;
+  template <int S>
+  __device__ __forceinline__ void step(const WideNode* __restrict__ nodes, const Tri* __restrict__ tris,
+                                       LdsStack<S>& st) {
+    if (++steps >= kMaxSteps) {
+      st.clear();
+      cur = kWalkDone;
+      return;
+    }
+    if ((cur & kLeafBit) == 0) {
+      float tn[4];
+      uint32_t ch[4];
+      test_children(nodes, cur, r.o, inv, r.tmin, best_t, tn, ch);
+      ++visits;
+      sort4(tn, ch);
+      if (tn[0] != INFINITY) {
+        if (tn[3] != INFINITY) st.push(ch[3], tn[3]);
+        if (tn[2] != INFINITY) st.push(ch[2], tn[2]);
+        if (tn[1] != INFINITY) st.push(ch[1], tn[1]);
+        cur = ch[0];
+        return;
+      }
+      next(st);
+      return;
+    }
+    const uint32_t first = leaf_first(cur), n = leaf_count(cur);
+    for (uint32_t k = 0; k < n; ++k) {
+      const Tri t4 = tris[first + k];
+      float t, u, v;
+      ++tests;
+      if (intersect_triangle(r, pre, xyz(t4.a), xyz(t4.b), xyz(t4.c), t, u, v)) {
+        const uint32_t g = __float_as_uint(t4.a.w);
+        if (any) {
+          best_g = g;
+          best_tri = first + k;
+          best_t = t;
+          st.clear();
+          cur = kWalkDone;
+          return;
         }
-      } else {
-        const uint32_t first = leaf_first(cur), n = leaf_count(cur);
-        for (uint32_t k = 0; k < n; ++k) {
-          const Tri tr = tris[first + k];
-          float t, u, v;
-          ++st.tests;
-          if (intersect_triangle(r, pre, xyz(tr.a), xyz(tr.b), xyz(tr.c), t, u, v)) {
-            // drain the stack so the spill bookkeeping ends clean
-            uint32_t dn;
-            float dd;
-            while (st.pop(dn, dd)) {
-            }
-            return true;
-          }
+        if (t < best_t || (t == best_t && g < best_g)) {
+          best_t = t;
+          best_g = g;
+          best_tri = first + k;
+          best_u = u;
+          best_v = v;
         }
       }
-      float d;
-      alive = st.pop(cur, d);
     }
-    return false;
+    next(st);
+  }
+
+  // The closest hit's record (call when done() and hit()).
+  __device__ __forceinline__ HitRecord record(const Tri* __restrict__ tris) const {
+    const Tri t4 = tris[best_tri];
+    HitRecord h;
+    h.instance = __float_as_uint(t4.b.w);
+    h.primitive = __float_as_uint(t4.c.w);
+    h.tri = best_tri;
+    h.u = best_u;
+    h.v = best_v;
+    h.t = best_t;
+    return h;
   }
 };
+
+// A walk followed to its end by one lane.
+template <int S>
+__device__ __forceinline__ bool walk_closest(const WideNode* nodes, const Tri* tris, LdsStack<S>& st,
+                                             const Ray& r, HitRecord& h, uint32_t& visits, uint32_t& tests) {
+  Walk w;
+  w.start(r, false);
+  st.clear();
+  while (!w.done()) w.step(nodes, tris, st);
+  visits += w.visits;
+  tests += w.tests;
+  if (!w.hit()) return false;
+  h = w.record(tris);
+  return true;
+}
+template <int S>
+__device__ __forceinline__ bool walk_occluded(const WideNode* nodes, const Tri* tris, LdsStack<S>& st,
+                                              const Ray& r, uint32_t& visits, uint32_t& tests) {
+  Walk w;
+  w.start(r, true);
+  st.clear();
+  while (!w.done()) w.step(nodes, tris, st);
+  visits += w.visits;
+  tests += w.tests;
+  return w.hit();
+}
 
 }  // namespace lotus

@@ -19,10 +19,11 @@ PASS, FAIL, SKIP = 0, 1, 77
 MISS = 0xFFFFFFFF
 OUTPUT_RADIANCE, OUTPUT_BARYCENTRICS, OUTPUT_SHADING_NORMAL, OUTPUT_ALBEDO, OUTPUT_ROUGHNESS_METALLIC = range(5)
 LIGHT_BSDF, LIGHT_MIS = 0, 1
+FRAME_SPLIT_KINDS = 1
 
 SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hip_scene_apply",
            "lotus_hip_render", "lotus_hip_trace", "lotus_hip_trace_device", "lotus_hip_light_table",
-           "lotus_hip_read_accumulation"]
+           "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning"]
 
 _u32p = C.POINTER(C.c_uint32)
 _f32p = C.POINTER(C.c_float)
@@ -64,6 +65,10 @@ class SceneEvidence(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("camera_dynamic", C.c_uint32), ("trace_dynamic", C.c_uint32)]
+
+
 class FrameParams(C.Structure):
     _fields_ = [
         ("world_to_clip", C.c_float * 16),
@@ -78,6 +83,7 @@ class FrameParams(C.Structure):
         ("sample_index", C.c_uint32),
         ("max_bounces", C.c_uint32),
         ("max_samples", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -98,6 +104,8 @@ class FrameResult(C.Structure):
         ("intersect_gpu_ms", C.c_double),
         ("shade_gpu_ms", C.c_double),
         ("shadow_gpu_ms", C.c_double),
+        ("trace_gpu_ms", C.c_double),
+        ("tail_gpu_ms", C.c_double),
         ("resolve_gpu_ms", C.c_double),
         ("frame_gpu_ms", C.c_double),
         ("traversal_nodes", C.c_uint64 * 4),
@@ -133,6 +141,10 @@ def lib():
                                              C.c_void_p, C.c_char_p, C.c_size_t]
         L.lotus_hip_light_table.restype = C.c_int
         L.lotus_hip_light_table.argtypes = [C.c_void_p, _u32p, _u32p, _u32p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_set_tuning.restype = C.c_int
+        L.lotus_hip_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning), C.c_char_p, C.c_size_t]
+        L.lotus_hip_get_tuning.restype = C.c_int
+        L.lotus_hip_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.lotus_hip_read_accumulation.restype = C.c_int
         L.lotus_hip_read_accumulation.argtypes = [C.c_void_p, _f32p, _u32p, C.c_char_p, C.c_size_t]
         _lib = L
@@ -207,7 +219,7 @@ class Renderer:
     def render(self, world_to_clip, width, height, frame_count=1, sample_index=0, max_bounces=8,
                light_sampling=LIGHT_MIS, output=OUTPUT_RADIANCE, clear_color=(0.0, 0.0, 0.0, 0.0),
                data_window=(0, 0, 0, 0), display_window=(0.0, 0.0, 0.0, 0.0), max_samples=0,
-               want_color=True, want_depth=False, count_capacity=None):
+               want_color=True, want_depth=False, count_capacity=None, split_kinds=False):
         fp = FrameParams()
         w2c = np.ascontiguousarray(world_to_clip, np.float32).reshape(16)
         for i in range(16):
@@ -223,6 +235,7 @@ class Renderer:
         fp.sample_index = sample_index
         fp.max_bounces = max_bounces
         fp.max_samples = max_samples
+        fp.flags = FRAME_SPLIT_KINDS if split_kinds else 0
         res = FrameResult()
         color = np.zeros((height, width, 4), np.float32) if want_color else None
         depth = np.zeros((height, width), np.float32) if want_depth else None
@@ -243,7 +256,8 @@ class Renderer:
                     camera_rays=res.camera_rays, extension_rays=res.extension_rays,
                     shadow_rays=res.shadow_rays, camera_ms=res.camera_gpu_ms,
                     intersect_ms=res.intersect_gpu_ms, shade_ms=res.shade_gpu_ms,
-                    shadow_ms=res.shadow_gpu_ms, resolve_ms=res.resolve_gpu_ms, frame_ms=res.frame_gpu_ms,
+                    shadow_ms=res.shadow_gpu_ms, trace_ms=res.trace_gpu_ms, tail_ms=res.tail_gpu_ms,
+                    resolve_ms=res.resolve_gpu_ms, frame_ms=res.frame_gpu_ms,
                     nodes=list(res.traversal_nodes), tris=list(res.traversal_tris))
 
     def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
@@ -275,6 +289,21 @@ class Renderer:
         if s != PASS:
             raise LotusError(s, buf.value.decode())
         return offsets, records[:int(count[0])]
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        lib().lotus_hip_get_tuning(self.handle, C.byref(t))
+        return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic}
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        lib().lotus_hip_get_tuning(self.handle, C.byref(t))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_set_tuning(self.handle, C.byref(t), buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
 
     def accumulation(self, width, height):
         sums = np.zeros((height, width, 4), np.float32)

@@ -101,7 +101,13 @@ typedef struct lotus_hip_frame_params {
   uint32_t sample_index;      /* first sample's random sequence */
   uint32_t max_bounces;       /* scattering events after the camera hit */
   uint32_t max_samples;       /* 0 = no limit */
+  uint32_t flags;             /* LOTUS_HIP_FRAME_* */
 } lotus_hip_frame_params;
+
+/* Trace extension and shadow rays in separate launches, so each kind's
+ * kernel time is its own (rays by kind); by default one launch per bounce
+ * takes both. Results are the same either way. */
+#define LOTUS_HIP_FRAME_SPLIT_KINDS 1u
 
 /* GpuFrameEvidence subset (vulkan_backend.hpp:202-249) plus rays by kind.
  * Output buffers are optional (NULL skips the readback). */
@@ -118,9 +124,11 @@ typedef struct lotus_hip_frame_result {
   uint64_t extension_rays;
   uint64_t shadow_rays;
   double camera_gpu_ms;         /* WavefrontTimings (vulkan_backend.hpp:52-68) */
-  double intersect_gpu_ms;
+  double intersect_gpu_ms;      /* extension rays alone (split launches) */
   double shade_gpu_ms;
-  double shadow_gpu_ms;
+  double shadow_gpu_ms;         /* shadow rays alone (split launches) */
+  double trace_gpu_ms;          /* extension + shadow rays in one launch */
+  double tail_gpu_ms;           /* paths beyond the recorded rounds */
   double resolve_gpu_ms;
   double frame_gpu_ms;          /* first kernel start to last kernel end */
   /* Traversal work over the call's samples, per kernel: camera, extension
@@ -173,6 +181,18 @@ int lotus_hip_trace_device(lotus_hip_renderer* renderer, uint32_t count,
                            const void* rays_device, void* hits_device,
                            int any_hit, void* stream, char* detail,
                            size_t detail_size);
+
+/* Launch shapes the autotuner chooses (no reference counterpart: the
+ * reference's driver schedules its own dispatches). Results never depend on
+ * them. */
+typedef struct lotus_hip_tuning {
+  uint32_t camera_dynamic; /* 1: persistent lanes refill from the queue; 0: one ray per lane per pass */
+  uint32_t trace_dynamic;  /* the same for extension and shadow rays */
+} lotus_hip_tuning;
+
+int lotus_hip_set_tuning(lotus_hip_renderer* renderer, const lotus_hip_tuning* tuning, char* detail,
+                         size_t detail_size);
+int lotus_hip_get_tuning(lotus_hip_renderer* renderer, lotus_hip_tuning* tuning);
 
 /* The light table as built (gpu_scene.cpp:512-609): per instance its first
  * entry or 0xFFFFFFFF, then 4 words per entry {pdf, threshold, alias,

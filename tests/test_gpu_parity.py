@@ -273,3 +273,36 @@ def test_deep_paths_use_the_tail(renderer):
     n = len(out["counts"])
     assert np.array_equal(out["counts"], counts[0][:n]) and not counts[0][n:].any()
     assert out["shadow_rays"] == shadow[0]
+
+
+def test_split_kind_launches_give_the_same_frame(renderer):
+    # extension and shadow rays in one launch or in launches of their own:
+    # the same counts and the same radiance, bit for bit
+    sc = scenes.cornell_lambert()
+    renderer.apply(sc)
+    a = renderer.render(sc.world_to_clip, 64, 64, frame_count=1, max_bounces=8, light_sampling=hip.LIGHT_MIS,
+                        sample_index=3)
+    # another first sample index restarts the accumulation, so b starts over at 3
+    renderer.render(sc.world_to_clip, 64, 64, frame_count=1, max_bounces=8, light_sampling=hip.LIGHT_MIS,
+                    sample_index=4)
+    b = renderer.render(sc.world_to_clip, 64, 64, frame_count=1, max_bounces=8, light_sampling=hip.LIGHT_MIS,
+                        sample_index=3, split_kinds=True)
+    assert np.array_equal(a["counts"], b["counts"])
+    assert a["shadow_rays"] == b["shadow_rays"]
+    assert np.array_equal(a["color"], b["color"])
+    assert a["trace_ms"] > 0 and b["trace_ms"] == 0 and b["intersect_ms"] > 0 and b["shadow_ms"] > 0
+
+
+def test_tuning_does_not_change_results(renderer):
+    sc = scenes.cornell_lambert()
+    renderer.apply(sc)
+    outs = []
+    for cam, tr in [(1, 1), (0, 0), (0, 1)]:
+        renderer.set_tuning(camera_dynamic=cam, trace_dynamic=tr)
+        renderer.render(sc.world_to_clip, 64, 64, frame_count=1, max_bounces=8, sample_index=50)  # restart
+        outs.append(renderer.render(sc.world_to_clip, 64, 64, frame_count=1, max_bounces=8, sample_index=5,
+                                    light_sampling=hip.LIGHT_MIS))
+    renderer.set_tuning(camera_dynamic=0, trace_dynamic=0)
+    for o in outs[1:]:
+        assert np.array_equal(o["counts"], outs[0]["counts"])
+        assert np.array_equal(o["color"], outs[0]["color"])
