@@ -42,7 +42,7 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, width, height, depth, threads, target_s=12.0):
+def cpu_baseline(scene, width, height, depth, threads, target_s=25.0):
     """The CPU oracle on the same frame: enough samples per pixel for about
     `target_s` seconds of host work (first a 64-row band calibrates)."""
     import oracle
@@ -77,7 +77,22 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kind-frames", type=int, default=16,
+                    help="frames rendered after the timed ones with each ray kind in launches of its own")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times: the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(256 * 1024 * 1024, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by image "
               "tiles, see the design notes)", file=sys.stderr)
@@ -114,14 +129,14 @@ def main() -> int:
     # rays by kind: the same frames with extension and shadow rays in launches
     # of their own, so each kind's kernel time is its own (not in the timed loop)
     split = {k: 0.0 for k in keys}
-    for k in range(min(16, args.steps)):
+    for k in range(args.kind_frames):
         out = r.render(w2c, args.width, args.height, split_kinds=True, **kw)
         for key in split:
             split[key] += out[key]
     by_kind = {
-        "camera": split["camera_rays"] / split["camera_ms"] / 1e3,
-        "extension": split["extension_rays"] / split["intersect_ms"] / 1e3,
-        "shadow": split["shadow_rays"] / split["shadow_ms"] / 1e3,
+        "camera": tot["camera_rays"] / tot["camera_ms"] / 1e3,
+        "extension": split["extension_rays"] / split["intersect_ms"] / 1e3 if split["intersect_ms"] else None,
+        "shadow": split["shadow_rays"] / split["shadow_ms"] / 1e3 if split["shadow_ms"] else None,
         "frame_gpu": rays / tot["frame_ms"] / 1e3,
     }
     # roofline of the dominant kernel, k_trace (extension + shadow rays):
@@ -168,7 +183,7 @@ def main() -> int:
                                f"({ev.triangle_count} tris), {args.width}x{args.height}, 1 spp/frame, "
                                f"depth {args.depth}, NEE+MIS",
                    "frames": args.steps, "bvh": f"4-wide, {ev.node_count} nodes, depth {ev.bvh_depth}"},
-        "mrays_by_kind": {k: round(v, 1) for k, v in by_kind.items()},
+        "mrays_by_kind": {k: (round(v, 1) if v is not None else None) for k, v in by_kind.items()},
         "rays_per_frame": {"camera": tot["camera_rays"] / args.steps,
                            "extension": tot["extension_rays"] / args.steps,
                            "shadow": tot["shadow_rays"] / args.steps},

@@ -488,7 +488,7 @@ int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, 
   bool ok = e == hipSuccess;
   if (ok) {
     BuildInput in{positions, triangles, gvo, gto, igeo, s.instance_transform, s.instance_base,
-                  p->instance_count, tri_count, p->bvh_leaf_size ? p->bvh_leaf_size : 4u};
+                  p->instance_count, tri_count, p->bvh_leaf_size ? p->bvh_leaf_size : 2u};
     ok = build_bvh(in, s, r->stream, build_ms, error);
   } else {
     error = std::string("scene upload: ") + hipGetErrorString(e);

@@ -57,7 +57,7 @@ typedef struct lotus_hip_scene_plan {
   uint32_t material_count;
   const uint32_t* materials;                /* 36 words per material */
   float environment[3];                     /* constant environment radiance */
-  uint32_t bvh_leaf_size;                   /* 0: default (4); at most 8 */
+  uint32_t bvh_leaf_size;                   /* 0: default (2); at most 8 */
 } lotus_hip_scene_plan;
 
 /* GpuSceneEvidence / GpuSceneStats counterpart (vulkan_backend.hpp:412-506). */
