@@ -9,8 +9,12 @@ data already resident in HBM.
 
 Prints one JSON line: whole-frame Mrays/s (camera + extension + shadow rays
 over the wall time of the timed frames), Mrays/s by ray kind from hipEvent
-kernel times, the dominant kernel's roofline, and the CPU oracle's rate on a
-bounded sample of the same frame.
+kernel times, the dominant kernel's roofline, the CPU oracle's rate on a
+bounded sample of the same frame, and the equal-relMSE race: the GPU ms the
+radiance-cache form (a fresh cache trained online, 65536 training paths per
+frame, Adam lr 1e-2) needs to reach the relMSE that K plain 1-spp frames
+reach, against a high-spp plain reference, on the same heightfield at 1080p
+and on the Cornell box (1024^2, depth 8).
 """
 from __future__ import annotations
 
@@ -66,6 +70,66 @@ def cpu_baseline(scene, width, height, depth, threads, target_s=25.0):
                       f"{rays} rays in {dt:.1f} s"}
 
 
+def relmse(img, ref):
+    """Mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2)."""
+    return float(np.mean((img[..., :3].astype(np.float64) - ref) ** 2 / (ref * ref + 1e-2)))
+
+
+def relmse_race(r, scene, width, height, depth, frames, ref_frames, cache_cfg, checkpoints=(4, 16, 64)):
+    """Plain frames and radiance-cache frames, each accumulated from the start
+    (the cache from a fresh initialisation, its training inside every frame's
+    GPU time), scored against a ref_frames-spp plain reference."""
+    w2c = scene.world_to_clip
+    kw = dict(max_bounces=depth, light_sampling=hip.LIGHT_MIS)
+    for _ in range(ref_frames):
+        r.render(w2c, width, height, sample_index=1 << 24, want_color=False, **kw)
+    ref = r.render(w2c, width, height, sample_index=1 << 24, frame_count=0, **kw)["color"][..., :3]
+    ref = ref.astype(np.float64)
+
+    def curve(cache):
+        t, out_c = 0.0, []
+        last = None
+        for f in range(frames):
+            out = r.render(w2c, width, height, sample_index=0, cache=cache, **kw)
+            t += out["frame_ms"]
+            out_c.append((t, relmse(out["color"], ref)))
+            last = out
+        return out_c, last
+
+    plain, _ = curve(False)
+    r.cache_configure(**cache_cfg)
+    cached, last = curve(True)
+    race = {}
+    for k in checkpoints:
+        if k > frames:
+            continue
+        t_plain, level = plain[k - 1]
+        hit = next((i for i, (_, e) in enumerate(cached) if e <= level), None)
+        race[f"plain_{k}_frames"] = {
+            "relmse": round(level, 6), "plain_gpu_ms": round(t_plain, 2),
+            "cache_gpu_ms": round(cached[hit][0], 2) if hit is not None else None,
+            "cache_frames": hit + 1 if hit is not None else None,
+            "ratio": round(cached[hit][0] / t_plain, 3) if hit is not None else None}
+    q, smp = last["cache_queries"], last["cache_samples"]
+    steps = cache_cfg.get("train_steps", hip.CACHE_DEFAULTS["train_steps"])
+    # MFMA work: 2 x 16384 multiply-adds per query forwards; a training
+    # sample costs a forward, the backward and the weight gradient (3x)
+    infer_tflops = 2 * 16384 * q / (last["cache_infer_ms"] * 1e-3) / 1e12 if last["cache_infer_ms"] else None
+    train_tflops = 3 * 2 * 16384 * min(smp, 4 * cache_cfg.get("batch_capacity", 65536)) / \
+        (last["cache_train_ms"] * 1e-3) / 1e12 if last["cache_train_ms"] else None
+    return {"reference_spp": ref_frames, "frames": frames, "race": race,
+            "plain_relmse": [round(e, 6) for _, e in plain[:: max(1, frames // 16)]],
+            "cache_relmse": [round(e, 6) for _, e in cached[:: max(1, frames // 16)]],
+            "plain_ms_per_frame": round(plain[-1][0] / frames, 3),
+            "cache_ms_per_frame": round(cached[-1][0] / frames, 3),
+            "cache_last_frame": {"queries": q, "samples": smp, "train_steps": steps,
+                                 "infer_ms": round(last["cache_infer_ms"], 3),
+                                 "train_ms": round(last["cache_train_ms"], 3), "loss": round(last["cache_loss"], 5),
+                                 "infer_mfma_tflops": round(infer_tflops, 2) if infer_tflops else None,
+                                 "train_mfma_tflops": round(train_tflops, 2) if train_tflops else None,
+                                 "mfma_peak_tflops": 2500.0}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +143,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kind-frames", type=int, default=16,
                     help="frames rendered after the timed ones with each ray kind in launches of its own")
+    ap.add_argument("--race-frames", type=int, default=64,
+                    help="frames of each form in the equal-relMSE race (0 skips it)")
+    ap.add_argument("--race-reference-spp", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times: the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -191,6 +258,17 @@ def main() -> int:
                              for k in ("camera_ms", "shade_ms", "trace_ms", "tail_ms", "resolve_ms", "frame_ms")},
         "roofline": roofline,
     }
+    if args.race_frames > 0:
+        cache_cfg = dict(log2_table=19, train_paths=65536, batch_capacity=65536, train_steps=4, learning_rate=1e-2)
+        log("equal-relMSE race: heightfield ...")
+        race = {"heightfield_1080p": relmse_race(r, scene, args.width, args.height, args.depth, args.race_frames,
+                                                 args.race_reference_spp, cache_cfg)}
+        log("equal-relMSE race: Cornell box ...")
+        box = scenes.cornell_lambert()
+        r.apply(box)
+        race["cornell_1024"] = relmse_race(r, box, 1024, 1024, 8, args.race_frames, args.race_reference_spp,
+                                           cache_cfg)
+        line["equal_relmse"] = race
     r.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")

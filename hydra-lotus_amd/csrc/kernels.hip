@@ -166,8 +166,25 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
   flush_stats(stats, 0, visits, tests);
 }
 
+// A cache query for a surface reached along `dir`: position in the cache's
+// unit cube, direction towards the previous vertex, shading normal, material.
+__device__ __forceinline__ void cache_query(const FrameConst& F, const Surface& sf, V3 dir, float4* out) {
+  const float px = (sf.position.x - F.cache_lo[0]) * F.cache_scale[0];
+  const float py = (sf.position.y - F.cache_lo[1]) * F.cache_scale[1];
+  const float pz = (sf.position.z - F.cache_lo[2]) * F.cache_scale[2];
+  const V3 n = sf.shading_normal;
+  out[0] = make_float4(px, py, pz, -dir.x);
+  out[1] = make_float4(-dir.y, -dir.z, n.x, n.y);
+  out[2] = make_float4(n.z, sf.roughness, sf.base_color.x, sf.base_color.y);
+  out[3] = make_float4(sf.base_color.z, sf.metallic, 0.0f, 0.0f);
+}
+
 // wavefront.slang:264-282: one scattering event per queued hit. The light a
-// shadow ray of the previous bounce found is added first.
+// shadow ray of the previous bounce found is added first. In radiance cache
+// frames a path other than a training path ends at bounce F.cache_depth after
+// its emission term, leaving a cache query with its throughput, and a
+// training path records each vertex (query, throughput on arrival, radiance
+// including the vertex's emission).
 __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                        uint32_t bounce) {
   const uint32_t count = *hit_count(counters, bounce);
@@ -195,8 +212,38 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
         q.nee[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
     }
-    const Scatter s = scatter_path(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z),
-                                   __uint_as_float(h4.w));
+    bool train = false, query = false;
+    uint32_t path = 0;
+    if (F.cache_depth != 0) {
+      const uint32_t key = slot + F.train_offset;
+      train = key % F.train_stride == 0;
+      path = key / F.train_stride;
+      query = !train && bounce == F.cache_depth;
+    }
+    const V3 arrival = p.throughput, dir = p.d;
+    Scatter s;
+    Surface sf;
+    if (query) {
+      sf = reconstruct_hit(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z), __uint_as_float(h4.w), p.d);
+      add_emission(S, F, p, sf, h4.x, h4.y);
+      s.continues = false;
+      s.shadow = false;
+    } else {
+      s = scatter_path(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z), __uint_as_float(h4.w),
+                       train ? &sf : nullptr);
+    }
+    const uint32_t kq = wave_append(cache_query_count(counters), query);
+    if (query) {
+      cache_query(F, sf, dir, q.cq + 4ull * kq);
+      q.cq_thr[kq] = make_float4(p.throughput.x, p.throughput.y, p.throughput.z, __uint_as_float(slot));
+    }
+    if (train && bounce < kCacheRecordDepth && path < F.train_capacity) {
+      float4* rec = q.crec + 6ull * (static_cast<size_t>(bounce) * F.train_capacity + path);
+      cache_query(F, sf, dir, rec);
+      rec[4] = make_float4(arrival.x, arrival.y, arrival.z, 1.0f);
+      rec[5] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
+      q.crec_n[path] = bounce + 1;
+    }
     const uint32_t ks = wave_append(shadow_count(counters, bounce), s.shadow);
     if (s.shadow) {
       q.sh_o[ks] = make_float4(s.shadow_ray.o.x, s.shadow_ray.o.y, s.shadow_ray.o.z, __uint_as_float(slot));
@@ -211,6 +258,53 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
       q.ray_d[kr] = make_float4(p.d.x, p.d.y, p.d.z, kFloatMax);
     } else {
       q.rad[slot] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 1.0f);
+    }
+  }
+}
+
+// Training samples of a radiance cache frame, after the resolve inputs are
+// final: a training path's sample value L (radiance, pending direct light,
+// pending environment, summed as the resolve sums them) gives each recorded
+// vertex the target (L - R) / T, R the radiance summed up to and including
+// the vertex's emission and T the throughput on arrival. Vertices whose
+// throughput has a component below kCacheMinThroughput, and non-finite
+// targets, are skipped.
+__global__ void __launch_bounds__(256) k_cache_samples(FrameConst F, Queues q, uint32_t* counters,
+                                                       uint32_t capacity) {
+  const uint32_t P = F.width * F.height;
+  for (uint32_t path = gtid(); path < F.train_capacity; path += gstride()) {
+    const uint32_t key = path * F.train_stride;
+    const uint32_t n = q.crec_n[path];
+    V3 L = V3{0.0f, 0.0f, 0.0f};
+    if (n != 0 && key >= F.train_offset && key - F.train_offset < P) {
+      const uint32_t slot = key - F.train_offset;
+      const float4 s = q.rad[slot], ne = q.nee[slot], e = q.esc[slot];
+      L = xyz(s);
+      if (ne.w != 0.0f) L = L + xyz(ne);
+      if (e.w != 0.0f) L = L + xyz(e);
+    }
+    for (uint32_t b = 0; b < kCacheRecordDepth; ++b) {
+      bool ok = b < n;
+      float4 rec[6];
+      V3 target = V3{0.0f, 0.0f, 0.0f};
+      if (ok) {
+        const float4* r = q.crec + 6ull * (static_cast<size_t>(b) * F.train_capacity + path);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) rec[k] = r[k];
+        const V3 T = xyz(rec[4]), R = xyz(rec[5]);
+        ok = T.x >= kCacheMinThroughput && T.y >= kCacheMinThroughput && T.z >= kCacheMinThroughput;
+        if (ok) {
+          target = V3{(L.x - R.x) / T.x, (L.y - R.y) / T.y, (L.z - R.z) / T.z};
+          ok = finite3(target);
+        }
+      }
+      const uint32_t k = wave_append(cache_sample_count(counters), ok);
+      if (ok && k < capacity) {
+        float4* dst = reinterpret_cast<float4*>(q.samples + static_cast<size_t>(k) * kSampleFloats);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dst[j] = rec[j];
+        dst[4] = make_float4(target.x, target.y, target.z, 0.0f);
+      }
     }
   }
 }
@@ -489,6 +583,9 @@ void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F
                         uint32_t write_color) {
   k_camera_pass<<<L.grid.camera_pass, kTraceBlock, 0, L.stream>>>(S, F, q, depth, write_color, L.spill,
                                                                    L.spill_cap);
+}
+void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q, uint32_t* c, uint32_t capacity) {
+  k_cache_samples<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, c, capacity);
 }
 void launch_rays(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
                  uint32_t any_hit) {

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 
+#include "nrc.hpp"
 #include "transport.hpp"
 
 namespace lotus {
@@ -30,7 +31,19 @@ struct Queues {
   float4* sh_c;   //               radiance delta if unblocked
   float4* accum;  // per pixel: radiance sum, hit samples
   float4* color;  // per pixel: mean (RGBA32F product)
+  // Radiance cache frames.
+  float4* cq;        // cache queries, 4 float4 each (position, direction, normal, material)
+  float4* cq_thr;    // per query: path throughput, slot
+  float4* crec;      // training vertices [bounce][path], 6 float4 each: query, throughput, radiance so far
+  uint32_t* crec_n;  // per training path: vertices recorded
+  float* samples;    // training samples, kSampleFloats each
 };
+
+// Training-path vertices recorded per path (deeper ones only contribute to
+// the targets of these).
+constexpr uint32_t kCacheRecordDepth = 8;
+// Smallest throughput component a training vertex may divide by.
+constexpr float kCacheMinThroughput = 1e-3f;
 
 // Persistent grid of each kernel: workgroups filling every CU at its occupancy.
 struct GridShape {
@@ -70,5 +83,11 @@ void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F
                         uint32_t write_color);
 void launch_rays(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
                  uint32_t any_hit);
+// Radiance cache frames: targets (L - R) / T of every recorded vertex of the
+// training paths, appended to q.samples (counter word 7), at most `capacity`.
+void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q, uint32_t* c, uint32_t capacity);
+// Cache query and sample counters.
+__host__ __device__ inline uint32_t* cache_query_count(uint32_t* c) { return c + 3; }
+__host__ __device__ inline uint32_t* cache_sample_count(uint32_t* c) { return c + 7; }
 
 }  // namespace lotus

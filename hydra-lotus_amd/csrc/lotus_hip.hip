@@ -111,14 +111,15 @@ struct AccumulationKey {  // vulkan_backend.cpp:419-434
   std::array<uint32_t, 4> data{};
   uint32_t light_sampling = 0, sample_index = 0, max_bounces = 0;
   uint64_t scene_generation = 0;
+  uint32_t cache_frames = 0;  // radiance cache frames accumulate apart
   bool operator==(const AccumulationKey& o) const {
     return world_to_clip == o.world_to_clip && width == o.width && height == o.height && display == o.display &&
            data == o.data && light_sampling == o.light_sampling && sample_index == o.sample_index &&
-           max_bounces == o.max_bounces && scene_generation == o.scene_generation;
+           max_bounces == o.max_bounces && scene_generation == o.scene_generation && cache_frames == o.cache_frames;
   }
 };
 
-enum Timer { kTCamera, kTIntersect, kTShade, kTShadow, kTTrace, kTTail, kTResolve, kTimers };
+enum Timer { kTCamera, kTIntersect, kTShade, kTShadow, kTTrace, kTTail, kTResolve, kTCacheInfer, kTCacheTrain, kTimers };
 
 }  // namespace
 
@@ -153,6 +154,14 @@ struct lotus_hip_renderer {
   uint32_t rounds_hint = 0;
 
   std::vector<hipEvent_t> events;
+
+  // Radiance cache and its frame buffers.
+  CacheState cache;
+  lotus_hip_cache_config cache_config{};
+  float scene_lo[3] = {0, 0, 0}, scene_scale[3] = {1, 1, 1};
+  bool bounds_valid = false;
+  uint32_t cache_pixels = 0;       // target size the frame buffers were made for
+  uint32_t train_stride = 1, train_capacity = 0, sample_capacity = 0;
 };
 
 namespace {
@@ -173,8 +182,14 @@ void free_target(lotus_hip_renderer* r) {
   dev_free(q.sh_c);
   dev_free(q.accum);
   dev_free(q.color);
+  dev_free(q.cq);
+  dev_free(q.cq_thr);
+  dev_free(q.crec);
+  dev_free(q.crec_n);
+  dev_free(q.samples);
   dev_free(r->depth);
   r->width = r->height = 0;
+  r->cache_pixels = 0;
 }
 
 void free_scene(lotus_hip_renderer* r) {
@@ -227,6 +242,67 @@ hipError_t ensure_target(lotus_hip_renderer* r, uint32_t width, uint32_t height)
   r->height = height;
   r->key_valid = false;
   return hipSuccess;
+}
+
+// Radiance cache frame buffers: queries for every pixel, training paths one
+// pixel in train_stride, their recorded vertices and samples.
+hipError_t ensure_cache_target(lotus_hip_renderer* r) {
+  const uint32_t P = r->width * r->height;
+  const uint32_t paths = std::max(1u, r->cache_config.train_paths);
+  const uint32_t stride = std::max(1u, P / paths);
+  if (r->q.cq && r->cache_pixels == P && r->train_stride == stride) return hipSuccess;
+  Queues& q = r->q;
+  dev_free(q.cq);
+  dev_free(q.cq_thr);
+  dev_free(q.crec);
+  dev_free(q.crec_n);
+  dev_free(q.samples);
+  r->cache_pixels = 0;
+  r->train_stride = stride;
+  r->train_capacity = P / stride + 2;
+  r->sample_capacity = r->train_capacity * kCacheRecordDepth;
+  hipError_t e = dev_alloc(q.cq, 4ull * P);
+  if (e == hipSuccess) e = dev_alloc(q.cq_thr, P);
+  if (e == hipSuccess) e = dev_alloc(q.crec, 6ull * kCacheRecordDepth * r->train_capacity);
+  if (e == hipSuccess) e = dev_alloc(q.crec_n, r->train_capacity);
+  if (e == hipSuccess) e = dev_alloc(q.samples, size_t(kSampleFloats) * r->sample_capacity);
+  if (e == hipSuccess) r->cache_pixels = P;
+  return e;
+}
+
+// The scene's box (the root's child boxes) mapped to the cache's unit cube.
+hipError_t ensure_bounds(lotus_hip_renderer* r) {
+  if (r->bounds_valid) return hipSuccess;
+  float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+  if (r->scene.node_count != 0) {
+    WideNode root;
+    const hipError_t e = hipMemcpy(&root, r->scene.nodes, sizeof root, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    const float* los[3] = {root.lox, root.loy, root.loz};
+    const float* his[3] = {root.hix, root.hiy, root.hiz};
+    for (int a = 0; a < 3; ++a) {
+      float l = INFINITY, h = -INFINITY;
+      for (int c = 0; c < 4; ++c)
+        if (los[a][c] <= his[a][c]) l = std::min(l, los[a][c]), h = std::max(h, his[a][c]);
+      if (l <= h) lo[a] = l, hi[a] = h;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const float ext = hi[a] - lo[a];
+    r->scene_lo[a] = lo[a] - 1e-4f * std::max(ext, 1e-3f);
+    r->scene_scale[a] = 1.0f / (std::max(ext, 1e-3f) * 1.0002f);
+  }
+  r->bounds_valid = true;
+  return hipSuccess;
+}
+
+uint32_t mix_host(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
 }
 
 std::string latch(lotus_hip_renderer* r, const std::string& what, hipError_t e) {
@@ -415,6 +491,7 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   if (r->stream) (void)hipStreamSynchronize(r->stream);
   free_target(r);
   free_scene(r);
+  cache_free(r->cache);
   dev_free(r->counters);
   dev_free(r->stats);
   dev_free(r->work);
@@ -530,6 +607,7 @@ int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, 
   r->light_offsets_host = offsets;
   r->lights_host = lights;
   r->has_scene = true;
+  r->bounds_valid = false;
   ++r->scene_generation;
   r->rounds_hint = 0;
   if (ev) {
@@ -577,6 +655,11 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     set_detail(detail, detail_size, "no scene has been applied");
     return LOTUS_HIP_FAIL;
   }
+  const bool cache_frame = (P->flags & LOTUS_HIP_FRAME_RADIANCE_CACHE) != 0 && P->output == LOTUS_HIP_OUTPUT_RADIANCE;
+  if (cache_frame && !r->cache.ready) {
+    set_detail(detail, detail_size, "radiance cache frames need lotus_hip_cache_configure first");
+    return LOTUS_HIP_FAIL;
+  }
   FrameConst F{};
   vulkan_world_to_clip(P->world_to_clip, F.world_to_clip);
   if (!invert(F.world_to_clip, F.clip_to_world)) {
@@ -610,10 +693,23 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
   F.output = P->output;
 
   hipError_t e = ensure_target(r, P->width, P->height);
+  if (e == hipSuccess && cache_frame) e = ensure_cache_target(r);
+  if (e == hipSuccess && cache_frame) e = ensure_bounds(r);
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "target allocation", e));
     return LOTUS_HIP_FAIL;
   }
+  if (cache_frame) {
+    F.cache_depth = r->cache_config.query_depth;
+    F.train_stride = r->train_stride;
+    F.train_capacity = r->train_capacity;
+    for (int a = 0; a < 3; ++a) {
+      F.cache_lo[a] = r->scene_lo[a];
+      F.cache_scale[a] = r->scene_scale[a];
+    }
+  }
+  uint64_t cache_queries = 0, cache_samples = 0, cache_used = 0;
+  double cache_loss_sum = 0.0;
   const uint32_t npix = P->width * P->height;
   Launch L = launch_for(r, r->stream);
   const bool split = (P->flags & LOTUS_HIP_FRAME_SPLIT_KINDS) != 0;
@@ -622,7 +718,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
                r->stream);
   if (res && res->depth) launch_fill1(r->depth, npix, 1.0f, r->stream);
 
-  double t_ms[kTimers] = {0, 0, 0, 0, 0, 0, 0};
+  double t_ms[kTimers] = {};
   double frame_ms = 0.0;
   uint32_t added = 0;
   std::vector<uint32_t> counts;
@@ -639,6 +735,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     key.sample_index = P->sample_index;
     key.max_bounces = P->max_bounces;
     key.scene_generation = r->scene_generation;
+    key.cache_frames = cache_frame ? 1u : 0u;
     if (!r->key_valid || !(r->key == key)) {
       r->key = key;
       r->key_valid = true;
@@ -656,6 +753,10 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       F.accumulated = r->accumulated;
       uint32_t rounds = r->rounds_hint == 0 ? std::min(bounces, kFirstRounds) : std::min(r->rounds_hint, bounces);
       rounds = std::min(std::max(rounds, 1u), kMaxRounds);
+      if (cache_frame) {
+        rounds = std::min(std::max(rounds, F.cache_depth + 1), std::min(bounces, kMaxRounds));
+        F.train_offset = mix_host(F.sample_index * 0x9e3779b9u + 0x632be5abu) % r->train_stride;
+      }
       size_t ev = 0;
       std::vector<std::pair<int, size_t>> pairs;
       auto mark = [&](int timer) {
@@ -665,6 +766,10 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       auto done = [&]() { (void)hipEventRecord(r->events[ev++], r->stream); };
       launch_zero(r->counters, 4 * stride, r->stream);
       launch_zero(r->work, kWorkCounters, r->stream);
+      if (cache_frame) {
+        launch_zero(r->q.crec_n, r->train_capacity, r->stream);
+        launch_fill1(r->cache.loss, 1, 0.0f, r->stream);
+      }
       uint32_t next_work = 0;  // each persistent launch takes its own work counter
       if (add) {
         mark(kTCamera);
@@ -700,10 +805,34 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
           launch_tail(L, r->view, F, r->q, r->counters, rounds);
           done();
         }
+        if (cache_frame) {
+          mark(kTCacheInfer);
+          cache_infer_add(r->cache, r->q.cq, r->q.cq_thr, cache_query_count(r->counters), r->q.rad, r->stream);
+          done();
+        }
       }
       mark(kTResolve);
       launch_resolve(L, F, r->q, add);
       done();
+      if (cache_frame && add) {
+        // train on this frame's paths: the samples, then train_steps Adam
+        // steps over interleaved subsets of them
+        const uint32_t steps = std::max(1u, r->cache_config.train_steps);
+        mark(kTCacheTrain);
+        launch_cache_samples(L, F, r->q, r->counters, r->sample_capacity);
+        for (uint32_t k = 0; k < steps; ++k) {
+          cache_backprop(r->cache, r->q.samples, r->sample_capacity, cache_sample_count(r->counters), k, steps,
+                         r->stream);
+          cache_adam(r->cache, r->sample_capacity, cache_sample_count(r->counters), k, steps, r->stream);
+        }
+        done();
+      }
+      float loss_host = 0.0f;
+      if (cache_frame && add) {
+        e = hipGetLastError();
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(&loss_host, r->cache.loss, sizeof(float), hipMemcpyDeviceToHost, r->stream);
+      }
       std::vector<uint32_t> host(4 * stride);
       e = hipGetLastError();
       if (e == hipSuccess)
@@ -719,6 +848,15 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, r->events[0], r->events[ev - 1]);
         frame_ms += ms;
+      }
+      if (cache_frame && add) {
+        const uint32_t total = std::min(host[7], r->sample_capacity);
+        const uint32_t steps = std::max(1u, r->cache_config.train_steps);
+        for (uint32_t k = 0; k < steps; ++k)
+          if (total > k) cache_used += std::min((total - k + steps - 1) / steps, r->cache_config.batch_capacity);
+        cache_queries += host[3];
+        cache_samples += host[7];
+        cache_loss_sum += loss_host;
       }
       if (add) {
         counts.assign(stride, 0u);
@@ -784,6 +922,11 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       res->traversal_nodes[k] = stats[2 * k];
       res->traversal_tris[k] = stats[2 * k + 1];
     }
+    res->cache_queries = cache_queries;
+    res->cache_samples = cache_samples;
+    res->cache_loss = cache_used ? cache_loss_sum / (3.0 * static_cast<double>(cache_used)) : 0.0;
+    res->cache_infer_gpu_ms = t_ms[kTCacheInfer];
+    res->cache_train_gpu_ms = t_ms[kTCacheTrain];
   }
   set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;
@@ -892,6 +1035,153 @@ int lotus_hip_read_accumulation(lotus_hip_renderer* r, float* sums, uint32_t* sa
     }
   }
   return LOTUS_HIP_PASS;
+}
+
+namespace {
+
+bool cache_usable(lotus_hip_renderer* r, char* detail, size_t detail_size) {
+  if (!r) return false;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return false;
+  }
+  if (!r->cache.ready) {
+    set_detail(detail, detail_size, "the radiance cache is not configured");
+    return false;
+  }
+  (void)hipSetDevice(r->device);
+  return true;
+}
+
+int cache_finish(lotus_hip_renderer* r, hipError_t e, const char* what, char* detail, size_t detail_size) {
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, what, e));
+    return LOTUS_HIP_FAIL;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+}  // namespace
+
+int lotus_hip_cache_configure(lotus_hip_renderer* r, lotus_hip_cache_config* config, char* detail,
+                              size_t detail_size) {
+  if (!r || !config) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (config->query_depth < 1 || config->train_paths < 1 || config->train_steps < 1 || config->train_steps > 64) {
+    set_detail(detail, detail_size, "query_depth and train_paths must be at least 1, train_steps in [1, 64]");
+    return LOTUS_HIP_FAIL;
+  }
+  CacheConfig c;
+  c.log2_table = config->log2_table;
+  c.min_resolution = config->min_resolution;
+  c.max_resolution = config->max_resolution;
+  c.batch_capacity = config->batch_capacity;
+  c.learning_rate = config->learning_rate;
+  c.beta1 = config->beta1;
+  c.beta2 = config->beta2;
+  c.epsilon = config->epsilon;
+  c.seed = config->seed;
+  (void)hipSetDevice(r->device);
+  (void)hipStreamSynchronize(r->stream);
+  std::string err;
+  if (!cache_init(r->cache, c, r->stream, err)) {
+    set_detail(detail, detail_size, err);
+    return LOTUS_HIP_FAIL;
+  }
+  for (int l = 0; l < kCacheLevels; ++l) config->resolution[l] = r->cache.resolution[l];
+  config->dense_levels = r->cache.dense_mask;
+  r->cache_config = *config;
+  r->key_valid = false;
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_cache_infer(lotus_hip_renderer* r, const void* queries_device, uint32_t count, void* radiance_device,
+                          char* detail, size_t detail_size) {
+  if (!cache_usable(r, detail, detail_size)) return LOTUS_HIP_FAIL;
+  if (count == 0) return LOTUS_HIP_PASS;
+  if (!queries_device || !radiance_device) {
+    set_detail(detail, detail_size, "queries and radiance must be device buffers");
+    return LOTUS_HIP_FAIL;
+  }
+  cache_infer(r->cache, static_cast<const float*>(queries_device), count, static_cast<float*>(radiance_device),
+              r->stream);
+  return cache_finish(r, hipSuccess, "cache inference", detail, detail_size);
+}
+
+int lotus_hip_cache_gradients(lotus_hip_renderer* r, const void* samples_device, uint32_t count, float* weight_grad,
+                              float* grid_grad, float* loss, char* detail, size_t detail_size) {
+  if (!cache_usable(r, detail, detail_size)) return LOTUS_HIP_FAIL;
+  if (count == 0 || count > r->cache.config.batch_capacity || !samples_device) {
+    set_detail(detail, detail_size, "count must lie in [1, batch_capacity] with a device sample buffer");
+    return LOTUS_HIP_FAIL;
+  }
+  CacheState& c = r->cache;
+  launch_fill1(c.loss, 1, 0.0f, r->stream);
+  cache_backprop(c, static_cast<const float*>(samples_device), count, nullptr, 0, 1, r->stream);
+  std::vector<float> wg(kCacheParams);
+  float loss_sum = 0.0f;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(wg.data(), c.wgrad, sizeof(float) * kCacheParams, hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess && grid_grad)
+    e = hipMemcpyAsync(grid_grad, c.ggrad, sizeof(float) * c.grid_entries, hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&loss_sum, c.loss, sizeof(float), hipMemcpyDeviceToHost, r->stream);
+  if (e == hipSuccess) cache_clear_grid_grad(c, r->stream);
+  const int status = cache_finish(r, e, "cache gradients", detail, detail_size);
+  if (status != LOTUS_HIP_PASS) return status;
+  const float inv = 1.0f / static_cast<float>(count);
+  if (weight_grad)
+    for (int i = 0; i < kCacheParams; ++i) weight_grad[i] = wg[i] * inv;
+  if (grid_grad)
+    for (uint64_t i = 0; i < c.grid_entries; ++i) grid_grad[i] *= inv;
+  if (loss) *loss = loss_sum / (3.0f * static_cast<float>(count));
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_cache_train(lotus_hip_renderer* r, const void* samples_device, uint32_t count, float* loss,
+                          char* detail, size_t detail_size) {
+  if (!cache_usable(r, detail, detail_size)) return LOTUS_HIP_FAIL;
+  if (count == 0 || count > r->cache.config.batch_capacity || !samples_device) {
+    set_detail(detail, detail_size, "count must lie in [1, batch_capacity] with a device sample buffer");
+    return LOTUS_HIP_FAIL;
+  }
+  CacheState& c = r->cache;
+  launch_fill1(c.loss, 1, 0.0f, r->stream);
+  cache_backprop(c, static_cast<const float*>(samples_device), count, nullptr, 0, 1, r->stream);
+  cache_adam(c, count, nullptr, 0, 1, r->stream);
+  float loss_sum = 0.0f;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&loss_sum, c.loss, sizeof(float), hipMemcpyDeviceToHost, r->stream);
+  const int status = cache_finish(r, e, "cache training", detail, detail_size);
+  if (status == LOTUS_HIP_PASS && loss) *loss = loss_sum / (3.0f * static_cast<float>(count));
+  return status;
+}
+
+int lotus_hip_cache_parameters(lotus_hip_renderer* r, float* weights, float* grid, int write, char* detail,
+                               size_t detail_size) {
+  if (!cache_usable(r, detail, detail_size)) return LOTUS_HIP_FAIL;
+  CacheState& c = r->cache;
+  const hipMemcpyKind kind = write ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  hipError_t e = hipSuccess;
+  if (weights)
+    e = write ? hipMemcpyAsync(c.w32, weights, sizeof(float) * kCacheParams, kind, r->stream)
+              : hipMemcpyAsync(weights, c.w32, sizeof(float) * kCacheParams, kind, r->stream);
+  if (e == hipSuccess && grid)
+    e = write ? hipMemcpyAsync(c.grid32, grid, sizeof(float) * c.grid_entries, kind, r->stream)
+              : hipMemcpyAsync(grid, c.grid32, sizeof(float) * c.grid_entries, kind, r->stream);
+  if (e == hipSuccess && write) {
+    cache_reset_moments(c, r->stream);
+    cache_sync_half(c, r->stream);
+    r->key_valid = false;
+  }
+  return cache_finish(r, e, "cache parameters", detail, detail_size);
 }
 
 }  // extern "C"

@@ -9,8 +9,8 @@
 
 namespace lotus {
 
-constexpr uint32_t kRouletteBounces = 3;  // transport.slang:222
-constexpr float kMaxSurvival = 0.95f;     // transport.slang:223
+constexpr uint32_t kRouletteBounces = 3;  // transport.slang:21
+constexpr float kMaxSurvival = 0.95f;     // transport.slang:22
 
 // What every kernel of a frame reads besides the scene.
 struct FrameConst {
@@ -25,6 +25,13 @@ struct FrameConst {
   uint32_t light_sampling;
   uint32_t accumulated;  // samples already in the accumulation
   uint32_t output;
+  // Radiance cache (cache_depth 0: off). A path whose pixel index plus
+  // train_offset is a multiple of train_stride is a training path (index
+  // (pixel + offset) / stride); every other path ends at bounce cache_depth
+  // with a cache query. Positions map to [0, 1]^3 by (p - lo) * scale.
+  uint32_t cache_depth;
+  uint32_t train_stride, train_offset, train_capacity;
+  float cache_lo[3], cache_scale[3];
 };
 
 // The path state a kernel holds in registers.
@@ -61,7 +68,7 @@ __device__ __forceinline__ Ray camera_ray(const float* clip_to_world, float nx, 
   return r;
 }
 
-// transport.slang:254-270 (StartPath): box-filtered film position.
+// transport.slang:53-69 (StartPath): box-filtered film position.
 __device__ __forceinline__ Ray start_path(const FrameConst& f, uint32_t x, uint32_t y, uint32_t sample,
                                           uint32_t& rng) {
   rng = seed_random(y * f.width + x, sample);
@@ -197,18 +204,13 @@ struct Scatter {
   V3 delta;            // throughput * contribution, added when unoccluded
 };
 
-// transport.slang:365-431 (ScatterPath) with DirectLight (:324-336) and
-// ShadowedLight (:283-316) up to the shadow ray, which the caller traces.
-__device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameConst& F, PathRegs& p,
-                                                uint32_t tri, uint32_t instance, uint32_t primitive, float u,
-                                                float v) {
-  Scatter out;
-  out.continues = false;
-  out.shadow = false;
-  const bool sample_lights = F.light_sampling != 0;
-  const Surface sf = reconstruct_hit(S, tri, instance, u, v, p.d);
+// transport.slang:164-170 with EmissionWeight (:150-158): the surface's emission, MIS-weighted against
+// light sampling unless the path started here or arrived by a delta or
+// transmitted direction, added to the path's radiance.
+__device__ __forceinline__ void add_emission(const SceneView& S, const FrameConst& F, PathRegs& p, const Surface& sf,
+                                             uint32_t instance, uint32_t primitive) {
   V3 emission = sf.emission;
-  if (sample_lights && !(p.last_pdf < 0.0f) && p.depth != 0 && max3(emission) > 0.0f) {
+  if (F.light_sampling != 0 && !(p.last_pdf < 0.0f) && p.depth != 0 && max3(emission) > 0.0f) {
     float w = 1.0f;
     const float area_pdf = emitter_pdf(S, instance, primitive, sf.area);
     if (area_pdf > 0.0f) {
@@ -219,6 +221,21 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
     emission = emission * w;
   }
   p.radiance = p.radiance + p.throughput * emission;
+}
+
+// transport.slang:164-230 (ScatterPath) with DirectLight (:123-135) and
+// ShadowedLight (:82-115) up to the shadow ray, which the caller traces.
+// `keep`, when given, receives the reconstructed surface.
+__device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameConst& F, PathRegs& p,
+                                                uint32_t tri, uint32_t instance, uint32_t primitive, float u,
+                                                float v, Surface* keep = nullptr) {
+  Scatter out;
+  out.continues = false;
+  out.shadow = false;
+  const bool sample_lights = F.light_sampling != 0;
+  const Surface sf = reconstruct_hit(S, tri, instance, u, v, p.d);
+  if (keep) *keep = sf;
+  add_emission(S, F, p, sf, instance, primitive);
   if (p.depth >= F.max_bounces) return out;
   if (p.depth >= kRouletteBounces) {
     const float survival = minf(max3(p.throughput), kMaxSurvival);

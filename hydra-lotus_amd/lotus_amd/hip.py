@@ -20,10 +20,17 @@ MISS = 0xFFFFFFFF
 OUTPUT_RADIANCE, OUTPUT_BARYCENTRICS, OUTPUT_SHADING_NORMAL, OUTPUT_ALBEDO, OUTPUT_ROUGHNESS_METALLIC = range(5)
 LIGHT_BSDF, LIGHT_MIS = 0, 1
 FRAME_SPLIT_KINDS = 1
+FRAME_RADIANCE_CACHE = 2
+CACHE_PARAMS = 16384
+CACHE_LEVELS = 16
+# weight matrices [out][in] of the cache MLP, in parameter order
+CACHE_SHAPES = [(64, 48), (64, 64), (64, 64), (64, 64), (16, 64)]
 
 SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hip_scene_apply",
            "lotus_hip_render", "lotus_hip_trace", "lotus_hip_trace_device", "lotus_hip_light_table",
-           "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning"]
+           "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning",
+           "lotus_hip_cache_configure", "lotus_hip_cache_infer", "lotus_hip_cache_gradients",
+           "lotus_hip_cache_train", "lotus_hip_cache_parameters"]
 
 _u32p = C.POINTER(C.c_uint32)
 _f32p = C.POINTER(C.c_float)
@@ -110,7 +117,36 @@ class FrameResult(C.Structure):
         ("frame_gpu_ms", C.c_double),
         ("traversal_nodes", C.c_uint64 * 4),
         ("traversal_tris", C.c_uint64 * 4),
+        ("cache_queries", C.c_uint64),
+        ("cache_samples", C.c_uint64),
+        ("cache_loss", C.c_double),
+        ("cache_infer_gpu_ms", C.c_double),
+        ("cache_train_gpu_ms", C.c_double),
     ]
+
+
+class CacheConfig(C.Structure):
+    _fields_ = [
+        ("log2_table", C.c_uint32),
+        ("min_resolution", C.c_uint32),
+        ("max_resolution", C.c_uint32),
+        ("batch_capacity", C.c_uint32),
+        ("learning_rate", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("epsilon", C.c_float),
+        ("seed", C.c_uint32),
+        ("query_depth", C.c_uint32),
+        ("train_paths", C.c_uint32),
+        ("train_steps", C.c_uint32),
+        ("resolution", C.c_uint32 * 16),
+        ("dense_levels", C.c_uint32),
+    ]
+
+
+CACHE_DEFAULTS = dict(log2_table=19, min_resolution=16, max_resolution=2048, batch_capacity=65536,
+                      learning_rate=5e-3, beta1=0.9, beta2=0.99, epsilon=1e-15, seed=1, query_depth=1,
+                      train_paths=16384, train_steps=4)
 
 
 _lib = None
@@ -147,6 +183,18 @@ def lib():
         L.lotus_hip_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.lotus_hip_read_accumulation.restype = C.c_int
         L.lotus_hip_read_accumulation.argtypes = [C.c_void_p, _f32p, _u32p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_cache_configure.restype = C.c_int
+        L.lotus_hip_cache_configure.argtypes = [C.c_void_p, C.POINTER(CacheConfig), C.c_char_p, C.c_size_t]
+        L.lotus_hip_cache_infer.restype = C.c_int
+        L.lotus_hip_cache_infer.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_char_p,
+                                            C.c_size_t]
+        L.lotus_hip_cache_gradients.restype = C.c_int
+        L.lotus_hip_cache_gradients.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, _f32p, _f32p, _f32p,
+                                                C.c_char_p, C.c_size_t]
+        L.lotus_hip_cache_train.restype = C.c_int
+        L.lotus_hip_cache_train.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, _f32p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_cache_parameters.restype = C.c_int
+        L.lotus_hip_cache_parameters.argtypes = [C.c_void_p, _f32p, _f32p, C.c_int, C.c_char_p, C.c_size_t]
         _lib = L
     return _lib
 
@@ -219,7 +267,7 @@ class Renderer:
     def render(self, world_to_clip, width, height, frame_count=1, sample_index=0, max_bounces=8,
                light_sampling=LIGHT_MIS, output=OUTPUT_RADIANCE, clear_color=(0.0, 0.0, 0.0, 0.0),
                data_window=(0, 0, 0, 0), display_window=(0.0, 0.0, 0.0, 0.0), max_samples=0,
-               want_color=True, want_depth=False, count_capacity=None, split_kinds=False):
+               want_color=True, want_depth=False, count_capacity=None, split_kinds=False, cache=False):
         fp = FrameParams()
         w2c = np.ascontiguousarray(world_to_clip, np.float32).reshape(16)
         for i in range(16):
@@ -235,7 +283,7 @@ class Renderer:
         fp.sample_index = sample_index
         fp.max_bounces = max_bounces
         fp.max_samples = max_samples
-        fp.flags = FRAME_SPLIT_KINDS if split_kinds else 0
+        fp.flags = (FRAME_SPLIT_KINDS if split_kinds else 0) | (FRAME_RADIANCE_CACHE if cache else 0)
         res = FrameResult()
         color = np.zeros((height, width, 4), np.float32) if want_color else None
         depth = np.zeros((height, width), np.float32) if want_depth else None
@@ -258,7 +306,9 @@ class Renderer:
                     intersect_ms=res.intersect_gpu_ms, shade_ms=res.shade_gpu_ms,
                     shadow_ms=res.shadow_gpu_ms, trace_ms=res.trace_gpu_ms, tail_ms=res.tail_gpu_ms,
                     resolve_ms=res.resolve_gpu_ms, frame_ms=res.frame_gpu_ms,
-                    nodes=list(res.traversal_nodes), tris=list(res.traversal_tris))
+                    nodes=list(res.traversal_nodes), tris=list(res.traversal_tris),
+                    cache_queries=res.cache_queries, cache_samples=res.cache_samples, cache_loss=res.cache_loss,
+                    cache_infer_ms=res.cache_infer_gpu_ms, cache_train_ms=res.cache_train_gpu_ms)
 
     def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -313,3 +363,71 @@ class Renderer:
         if s != PASS:
             raise LotusError(s, buf.value.decode())
         return sums, int(n[0])
+
+    # radiance cache ---------------------------------------------------------
+    def cache_configure(self, **kw) -> dict:
+        """Creates the cache; returns the configuration as applied (with each
+        level's resolution and the dense-level mask)."""
+        cfg = CacheConfig()
+        opts = dict(CACHE_DEFAULTS)
+        opts.update(kw)
+        for k, v in opts.items():
+            setattr(cfg, k, v)
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_cache_configure(self.handle, C.byref(cfg), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        out = dict(opts)
+        out["resolution"] = [int(x) for x in cfg.resolution]
+        out["dense_levels"] = int(cfg.dense_levels)
+        self.cache_config = out
+        return out
+
+    def cache_infer(self, queries_ptr: int, n: int, out_ptr: int):
+        """Device buffers: queries [n][16] float32 in, radiance [n][3] float32 out."""
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_cache_infer(self.handle, C.c_void_p(queries_ptr), n, C.c_void_p(out_ptr), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    def cache_gradients(self, samples_ptr: int, n: int, want_grid: bool = True):
+        """Mean loss gradients over device samples [n][20]: (weights, grid, loss)."""
+        wg = np.zeros(CACHE_PARAMS, np.float32)
+        gg = np.zeros(self._grid_shape(), np.float32) if want_grid else None
+        loss = np.zeros(1, np.float32)
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_cache_gradients(self.handle, C.c_void_p(samples_ptr), n, _p(wg, C.c_float),
+                                            _p(gg, C.c_float) if gg is not None else None,
+                                            _p(loss, C.c_float), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        return wg, gg, float(loss[0])
+
+    def cache_train(self, samples_ptr: int, n: int) -> float:
+        loss = np.zeros(1, np.float32)
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_cache_train(self.handle, C.c_void_p(samples_ptr), n, _p(loss, C.c_float), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        return float(loss[0])
+
+    def _grid_shape(self):
+        return (CACHE_LEVELS, 1 << self.cache_config["log2_table"], 2)
+
+    def cache_parameters(self, weights=None, grid=None):
+        """With arrays: writes them (float32, [16384] and [16][T][2]); without:
+        returns the current (weights, grid)."""
+        buf = C.create_string_buffer(512)
+        if weights is None and grid is None:
+            w = np.zeros(CACHE_PARAMS, np.float32)
+            g = np.zeros(self._grid_shape(), np.float32)
+            s = lib().lotus_hip_cache_parameters(self.handle, _p(w, C.c_float), _p(g, C.c_float), 0, buf, 512)
+            if s != PASS:
+                raise LotusError(s, buf.value.decode())
+            return w, g
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32).reshape(CACHE_PARAMS)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32).reshape(self._grid_shape())
+        s = lib().lotus_hip_cache_parameters(self.handle, _p(w, C.c_float) if w is not None else None,
+                                             _p(g, C.c_float) if g is not None else None, 1, buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())

@@ -108,6 +108,10 @@ typedef struct lotus_hip_frame_params {
  * kernel time is its own (rays by kind); by default one launch per bounce
  * takes both. Results are the same either way. */
 #define LOTUS_HIP_FRAME_SPLIT_KINDS 1u
+/* Radiance cache frame (lotus_hip_cache_configure first): a subset of paths
+ * trains the cache, every other path ends at the cache's query depth with a
+ * cache query. */
+#define LOTUS_HIP_FRAME_RADIANCE_CACHE 2u
 
 /* GpuFrameEvidence subset (vulkan_backend.hpp:202-249) plus rays by kind.
  * Output buffers are optional (NULL skips the readback). */
@@ -135,6 +139,13 @@ typedef struct lotus_hip_frame_result {
    * (intersect), shadow, tail. Wide nodes tested and triangles tested. */
   uint64_t traversal_nodes[4];
   uint64_t traversal_tris[4];
+  /* Radiance cache frames: queries answered, training samples made, their
+   * mean relative L2 loss before the frame's steps, and kernel times. */
+  uint64_t cache_queries;
+  uint64_t cache_samples;
+  double cache_loss;
+  double cache_infer_gpu_ms;
+  double cache_train_gpu_ms;
 } lotus_hip_frame_result;
 
 /* ProbeVulkanBackend (vulkan_backend.hpp:742): PASS when a gfx950 device is
@@ -206,6 +217,59 @@ int lotus_hip_light_table(lotus_hip_renderer* renderer, uint32_t* offsets,
 int lotus_hip_read_accumulation(lotus_hip_renderer* renderer, float* sums,
                                 uint32_t* samples, char* detail,
                                 size_t detail_size);
+
+/* Radiance cache: a 16-level hash-grid encoding (2 FP16 features per level)
+ * and a 4 x 64 ReLU MLP without biases, trained online with Adam on the
+ * relative L2 loss (pred - y)^2 / (pred^2 + 0.01) (Mueller et al. 2021,
+ * 2022). No reference counterpart: the reference lists a neural radiance
+ * cache only as a roadmap candidate (docs/design/ROADMAP_POLICY.md:526-535).
+ *
+ * Queries are 16 floats: position mapped to [0, 1]^3, unit direction towards
+ * the previous path vertex, shading normal, roughness, base colour (3),
+ * metallic, 2 unused. Training samples are a query, an RGB target and 1
+ * unused float (20 floats). Parameters are FP32 masters: weights [16384] =
+ * W0[64][48], W1..W3[64][64], W4[16][64] (rows 3..15 unused), the grid
+ * [16][2^log2_table][2]. */
+typedef struct lotus_hip_cache_config {
+  uint32_t log2_table;      /* entries per level, 2^log2_table, 10..22 */
+  uint32_t min_resolution;  /* cells per axis of the coarsest level */
+  uint32_t max_resolution;  /* cells per axis of the finest level */
+  uint32_t batch_capacity;  /* most samples of one optimizer step */
+  float learning_rate, beta1, beta2, epsilon;
+  uint32_t seed;            /* parameter initialisation */
+  uint32_t query_depth;     /* frames: bounce at which other paths query (>= 1) */
+  uint32_t train_paths;     /* frames: training paths per frame */
+  uint32_t train_steps;     /* frames: optimizer steps per frame */
+  uint32_t resolution[16];  /* written: cells per axis of each level */
+  uint32_t dense_levels;    /* written: bit l set when level l is indexed densely */
+} lotus_hip_cache_config;
+
+/* (Re)creates the cache with freshly initialised parameters. */
+int lotus_hip_cache_configure(lotus_hip_renderer* renderer, lotus_hip_cache_config* config, char* detail,
+                              size_t detail_size);
+
+/* radiance[i][3] = network(queries[i][16]) for i < count; device pointers,
+ * synchronous. */
+int lotus_hip_cache_infer(lotus_hip_renderer* renderer, const void* queries_device, uint32_t count,
+                          void* radiance_device, char* detail, size_t detail_size);
+
+/* Mean gradients of the loss over samples[count][20] (device pointer) without
+ * a step: weights [16384] and grid [16][2^log2_table][2] (host arrays, either
+ * may be NULL), and the mean loss. */
+int lotus_hip_cache_gradients(lotus_hip_renderer* renderer, const void* samples_device, uint32_t count,
+                              float* weight_grad, float* grid_grad, float* loss, char* detail,
+                              size_t detail_size);
+
+/* One Adam step over samples[count][20] (device pointer); `loss` (may be
+ * NULL) receives the mean loss before the step. The grid's moments advance
+ * only where a sample touched it. */
+int lotus_hip_cache_train(lotus_hip_renderer* renderer, const void* samples_device, uint32_t count, float* loss,
+                          char* detail, size_t detail_size);
+
+/* Reads (write = 0) or writes (write = 1) the FP32 parameters through host
+ * arrays (either may be NULL). Writing resets the Adam moments and step. */
+int lotus_hip_cache_parameters(lotus_hip_renderer* renderer, float* weights, float* grid, int write, char* detail,
+                               size_t detail_size);
 
 #ifdef __cplusplus
 }

@@ -984,7 +984,7 @@ struct Counters {
   uint64_t shadow;
 };
 
-// transport.slang:283-316, 324-336.
+// transport.slang:82-115, 123-135.
 F3 DirectLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo,
                uint32_t& rng, Counters& k, bool brute) {
   if (s.lights.empty()) return F3{0, 0, 0};
@@ -1008,7 +1008,7 @@ F3 DirectLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo,
   return IsFinite(c) ? c : F3{0, 0, 0};
 }
 
-// transport.slang:365-431.
+// transport.slang:164-230.
 bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
                  const Hit& hit, Counters& k, bool brute) {
   const bool sample_lights = fr.light_sampling != 0;
@@ -1053,12 +1053,12 @@ bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
   return true;
 }
 
-// transport.slang:436-448 (constant environment).
+// transport.slang:235-247 (constant environment).
 void MissPath(const Scene& s, Path& path) {
   path.radiance = path.radiance + path.throughput * s.environment;
 }
 
-// transport.slang:254-270.
+// transport.slang:53-69.
 Path StartPath(const oracle_frame& fr, uint32_t x, uint32_t y, uint32_t sample) {
   Path p;
   p.throughput = F3{1, 1, 1};

@@ -100,11 +100,92 @@ def loss_and_grads(x, extra, targets, grid, weights, resolutions, outputs=3, eps
     return loss, dW, dgrid, pred
 
 
-def adam(param, grad, m, v, step, lr, beta1=0.9, beta2=0.99, eps=1e-15):
-    """One Adam step (Kingma and Ba 2015) with bias correction."""
-    m[:] = beta1 * m + (1 - beta1) * grad
-    v[:] = beta2 * v + (1 - beta2) * grad * grad
-    mh = m / (1 - beta1 ** step)
-    vh = v / (1 - beta2 ** step)
-    param -= lr * mh / (np.sqrt(vh) + eps)
+def adam(param, grad, m, v, step, lr, beta1=0.9, beta2=0.99, eps=1e-15, sparse=False):
+    """One Adam step (Kingma and Ba 2015) with bias correction; `sparse`
+    leaves entries with an exactly zero gradient (and their moments) alone."""
+    sel = grad != 0 if sparse else np.ones(grad.shape, bool)
+    m[sel] = beta1 * m[sel] + (1 - beta1) * grad[sel]
+    v[sel] = beta2 * v[sel] + (1 - beta2) * grad[sel] * grad[sel]
+    mh = m[sel] / (1 - beta1 ** step)
+    vh = v[sel] / (1 - beta2 ** step)
+    param[sel] -= lr * mh / (np.sqrt(vh) + eps)
     return param
+
+
+# --- the device network's shapes and FP16 rounding points -----------------
+# Queries are 16 floats (position in [0, 1]^3, direction, normal, roughness,
+# base colour, metallic, 2 unused); the network's 16 extra inputs are query
+# floats 3..13, then 1, then 4 zeros. Weights are one flat vector of
+# [out][in] matrices. The device rounds to FP16: the grid features it reads,
+# each level's interpolated features, the extra inputs, the weights, every
+# hidden activation (after ReLU) and every back-propagated output gradient
+# (after the ReLU mask), the last clamped per sample to +-DELTA_CLAMP first.
+SHAPES = [(64, 48), (64, 64), (64, 64), (64, 64), (16, 64)]
+DELTA_CLAMP = 1024.0
+
+
+def f16(x):
+    return np.clip(x, -65504.0, 65504.0).astype(np.float16).astype(np.float64)
+
+
+def unflatten(w):
+    out, o = [], 0
+    for r, c in SHAPES:
+        out.append(np.asarray(w[o:o + r * c], np.float64).reshape(r, c))
+        o += r * c
+    return out
+
+
+def query_inputs(q):
+    q = np.asarray(q, np.float64)
+    x = np.clip(q[:, 0:3], 0.0, 1.0)
+    extra = np.zeros((q.shape[0], 16))
+    extra[:, 0:11] = q[:, 3:14]
+    extra[:, 11] = 1.0
+    return x, extra
+
+
+def device_forward(queries, weights, grid, resolutions):
+    """The device network's output [n, 3] and what backprop needs."""
+    x, extra = query_inputs(queries)
+    feats, cache = encode(x, f16(grid), resolutions)
+    inputs = np.concatenate([f16(feats), f16(extra)], axis=1)
+    Ws = [f16(W) for W in unflatten(weights)]
+    acts = [inputs]
+    h = inputs
+    for li, W in enumerate(Ws):
+        z = h @ W.T
+        h = f16(np.maximum(z, 0.0)) if li < len(Ws) - 1 else z
+        acts.append(h)
+    return acts[-1][:, :3], dict(acts=acts, Ws=Ws, cache=cache, x=x)
+
+
+def device_loss_and_grads(samples, weights, grid, resolutions):
+    """Mean relative L2 loss over samples [n, 20] and its mean gradients with
+    respect to the flat weights and the grid, at the device's rounding points
+    (see above). The device sums per-sample gradients and divides by n."""
+    samples = np.asarray(samples, np.float64)
+    n = samples.shape[0]
+    pred, st = device_forward(samples[:, :16], weights, grid, resolutions)
+    y = samples[:, 16:19]
+    den = pred * pred + 0.01
+    loss = np.sum((pred - y) ** 2 / den) / (3 * n)
+    d = np.clip(2.0 * (pred - y) / den / 3.0, -DELTA_CLAMP, DELTA_CLAMP)
+    acts, Ws = st["acts"], st["Ws"]
+    dz = np.zeros((n, 16))
+    dz[:, :3] = f16(d)
+    dW = [None] * 5
+    for li in range(4, -1, -1):
+        dW[li] = dz.T @ acts[li]
+        if li == 0:
+            break
+        dz = f16((dz @ Ws[li]) * (acts[li] > 0))
+    dfeat = dz @ Ws[0][:, :32]
+    dgrid = np.zeros_like(np.asarray(grid, np.float64))
+    L, T, F = dgrid.shape
+    for l in range(L):
+        idx, wts = st["cache"][l]
+        for corner in range(8):
+            np.add.at(dgrid[l], idx[:, corner], wts[:, corner, None] * dfeat[:, l * F:(l + 1) * F])
+    flat = np.concatenate([g.reshape(-1) for g in dW]) / n
+    return loss, flat, dgrid / n, pred

@@ -49,9 +49,11 @@ def test_struct_sizes_match_header():
 #include <stddef.h>
 #include "lotus_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(lotus_hip_scene_plan), sizeof(lotus_hip_scene_evidence),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lotus_hip_scene_plan), sizeof(lotus_hip_scene_evidence),
          sizeof(lotus_hip_frame_params), sizeof(lotus_hip_frame_result),
-         offsetof(lotus_hip_frame_result, traversal_nodes), offsetof(lotus_hip_scene_plan, bvh_leaf_size));
+         offsetof(lotus_hip_frame_result, traversal_nodes), offsetof(lotus_hip_scene_plan, bvh_leaf_size),
+         sizeof(lotus_hip_cache_config), offsetof(lotus_hip_cache_config, resolution),
+         offsetof(lotus_hip_frame_result, cache_loss));
   return 0;
 }'''
     import tempfile
@@ -67,3 +69,6 @@ int main(void) {
     assert sizes[3] == C.sizeof(hip.FrameResult)
     assert sizes[4] == hip.FrameResult.traversal_nodes.offset
     assert sizes[5] == hip.ScenePlan.bvh_leaf_size.offset
+    assert sizes[6] == C.sizeof(hip.CacheConfig)
+    assert sizes[7] == hip.CacheConfig.resolution.offset
+    assert sizes[8] == hip.FrameResult.cache_loss.offset

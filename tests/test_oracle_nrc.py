@@ -85,3 +85,43 @@ def test_adam_first_step_moves_by_lr():
     nrc.adam(p, g, m, v, 1, 0.01)
     # bias-corrected first step: p -= lr * sign(g)
     assert np.allclose(p, [0.99, -1.99, 2.99])
+
+
+def test_adam_sparse_skips_untouched_entries():
+    p = np.array([1.0, 2.0])
+    g = np.array([0.0, 1.0])
+    m = np.array([0.5, 0.0])
+    v = np.array([0.25, 0.0])
+    nrc.adam(p, g, m, v, 1, 0.1, sparse=True)
+    assert p[0] == 1.0 and m[0] == 0.5 and v[0] == 0.25
+    assert np.isclose(p[1], 1.9)
+
+
+def _device_setup(seed=0, n=64, table=256):
+    rng = np.random.default_rng(seed)
+    res = nrc.level_resolutions(16, 2, 40)
+    grid = rng.uniform(-1, 1, (16, table, 2))
+    w = np.concatenate([rng.uniform(-1, 1, r * c) * np.sqrt(6.0 / (r + c)) for r, c in nrc.SHAPES])
+    q = np.zeros((n, 20))
+    q[:, 0:3] = rng.uniform(0, 1, (n, 3))
+    d = rng.normal(size=(n, 3))
+    q[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    nn = rng.normal(size=(n, 3))
+    q[:, 6:9] = nn / np.linalg.norm(nn, axis=1, keepdims=True)
+    q[:, 9:14] = rng.uniform(0, 1, (n, 5))
+    q[:, 16:19] = rng.uniform(0, 2, (n, 3))
+    return q, w, grid, res
+
+
+def test_device_rounding_model_tracks_exact_gradients():
+    # The FP16 rounding points change the gradient by about FP16's precision:
+    # the device model's gradients match the float64 ones to a few 1e-3.
+    q, w, grid, res = _device_setup()
+    loss_h, dw_h, dg_h, pred_h = nrc.device_loss_and_grads(q, w, grid, res)
+    x, extra = nrc.query_inputs(q[:, :16])
+    loss, dW, dgrid, pred = nrc.loss_and_grads(x, extra, q[:, 16:19], grid, nrc.unflatten(w), res)
+    dw = np.concatenate([g.reshape(-1) for g in dW])
+    assert abs(loss_h - loss) < 1e-2 * loss
+    assert np.linalg.norm(pred_h - pred) < 1e-2 * np.linalg.norm(pred)
+    assert np.linalg.norm(dw_h - dw) < 2e-2 * np.linalg.norm(dw)
+    assert np.linalg.norm(dg_h - dgrid) < 2e-2 * np.linalg.norm(dgrid)
