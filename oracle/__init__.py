@@ -62,6 +62,8 @@ class Frame(C.Structure):
         ("spp", C.c_uint32),
         ("max_bounces", C.c_uint32),
         ("light_sampling", C.c_uint32),
+        ("restir_candidates", C.c_uint32),
+        ("restir_proposal", C.c_uint32),
     ]
 
 
@@ -91,6 +93,17 @@ def lib():
         L.oracle_trace_path.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_uint32, C.c_uint32,
                                         C.c_uint32, _f32p]
         L.oracle_camera_ray.argtypes = [_f32p, C.c_float, C.c_float, _f32p]
+        L.oracle_reservoir_stream.restype = C.c_int
+        L.oracle_reservoir_stream.argtypes = [_u32p, C.c_uint32, C.c_float, C.c_float, C.c_float]
+        L.oracle_reservoir_merge.restype = C.c_int
+        L.oracle_reservoir_merge.argtypes = [_u32p, _u32p, C.c_float, C.c_float, C.c_uint32, C.c_float]
+        L.oracle_reservoir_normalization.restype = C.c_float
+        L.oracle_reservoir_normalization.argtypes = [_u32p]
+        L.oracle_reuse_normalization.restype = C.c_float
+        L.oracle_reuse_normalization.argtypes = [_u32p, C.c_uint32]
+        L.oracle_restir_initial.restype = C.c_int
+        L.oracle_restir_initial.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_uint32, C.c_uint32, C.c_uint32,
+                                            _u32p]
         L.oracle_camera_constants.restype = C.c_int
         L.oracle_camera_constants.argtypes = [_f32p, _f32p, _f32p]
         L.oracle_pcg_hash.restype = C.c_uint32
@@ -209,6 +222,13 @@ class OracleScene:
                             ptr(counts, C.c_uint32), ptr(shadow, C.c_uint64), threads)
         return acc, counts, shadow
 
+    def restir_initial(self, frame: Frame, x: int, y: int, sample_index: int):
+        """The 12-word initial reservoir record of pixel (x, y), or None on a
+        camera miss."""
+        rec = np.zeros(12, np.uint32)
+        hit = lib().oracle_restir_initial(self.handle, C.byref(frame), x, y, sample_index, ptr(rec, C.c_uint32))
+        return rec if hit else None
+
 
 def camera_constants(world_to_clip_gl):
     """(world_to_clip_vk, clip_to_world) as float32[16] column-major."""
@@ -222,7 +242,7 @@ def camera_constants(world_to_clip_gl):
 
 
 def make_frame(world_to_clip_gl, width, height, spp=1, sample_index=0, max_bounces=8,
-               light_sampling=1, rect=None) -> Frame:
+               light_sampling=1, rect=None, restir_candidates=8, restir_proposal=0) -> Frame:
     _, inv = camera_constants(world_to_clip_gl)
     f = Frame()
     for i in range(16):
@@ -241,6 +261,8 @@ def make_frame(world_to_clip_gl, width, height, spp=1, sample_index=0, max_bounc
     f.spp = spp
     f.max_bounces = max_bounces
     f.light_sampling = light_sampling
+    f.restir_candidates = restir_candidates
+    f.restir_proposal = restir_proposal
     return f
 
 

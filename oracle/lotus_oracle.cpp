@@ -930,14 +930,10 @@ struct LightSample {
   F3 radiance;
   float pdf;
 };
-// scene.slang:894-909 with SampleTriangle (:632-677).
-LightSample SampleLight(const Scene& s, float ux, float uy, float uz, float uw,
-                        F3 origin) {
+// scene.slang:632-677 (SampleTriangle): a point uniformly by area on light
+// `light` of the table, (uz, uw) the unit-square coordinates.
+LightSample SampleTriangleLight(const Scene& s, uint32_t light, float uz, float uw, F3 origin) {
   LightSample none{F3{0, 0, 1}, 0.0f, F3{0, 0, 0}, 0.0f};
-  const uint32_t count = static_cast<uint32_t>(s.lights.size());
-  const uint32_t column = std::min(static_cast<uint32_t>(ux * static_cast<float>(count)), count - 1);
-  const LightRecord& record = s.lights[column];
-  const uint32_t light = uy < record.threshold ? column : record.alias;
   const LightRecord& chosen = s.lights[light];
   const uint32_t prim = light - s.light_offsets[chosen.instance];
   const float root = Sqrt(uz);
@@ -963,12 +959,141 @@ LightSample SampleLight(const Scene& s, float ux, float uy, float uz, float uw,
   ls.pdf = chosen.pdf / area * d2 / cosine;
   return ls;
 }
+
+// scene.slang:894-909 (SampleLight): the alias table's column and threshold
+// choose the light, then SampleTriangle.
+LightSample SampleLight(const Scene& s, float ux, float uy, float uz, float uw,
+                        F3 origin) {
+  const uint32_t count = static_cast<uint32_t>(s.lights.size());
+  const uint32_t column = std::min(static_cast<uint32_t>(ux * static_cast<float>(count)), count - 1);
+  const LightRecord& record = s.lights[column];
+  const uint32_t light = uy < record.threshold ? column : record.alias;
+  return SampleTriangleLight(s, light, uz, uw, origin);
+}
 // scene.slang:1004-1010.
 float EmitterPdf(const Scene& s, const Hit& hit, float area) {
   if (s.light_offsets.empty()) return 0.0f;
   const uint32_t first = s.light_offsets[hit.instance];
   if (first == kNoLights || !(area > 0.0f)) return 0.0f;
   return s.lights[first + hit.primitive].pdf / area;
+}
+
+// ---------------------------------------------------------------- ReSTIR DI
+// reservoir.slang:6-103 (the GPU float form of reservoir.cpp).
+constexpr uint32_t kNoReservoirSample = 0xFFFFFFFFu;
+struct Reservoir {
+  uint32_t sample = kNoReservoirSample;
+  float weight_sum = 0.0f;
+  float selected_target = 0.0f;
+  uint32_t count = 0;
+};
+
+// reservoir.slang:23-56.
+bool StreamCandidate(Reservoir& r, uint32_t sample, float target, float proposal, float random,
+                     bool& selected) {
+  selected = false;
+  const bool valid_zero = r.weight_sum == 0.0f && r.sample == kNoReservoirSample && r.selected_target == 0.0f;
+  const bool valid_positive = r.weight_sum > 0.0f && r.sample != kNoReservoirSample && r.selected_target > 0.0f &&
+                              r.count != 0;
+  if (!std::isfinite(r.weight_sum) || !std::isfinite(r.selected_target) || !(valid_zero || valid_positive) ||
+      !std::isfinite(target) || target < 0.0f || !std::isfinite(proposal) || proposal <= 0.0f ||
+      !std::isfinite(random) || random < 0.0f || random >= 1.0f || (target > 0.0f && sample == kNoReservoirSample) ||
+      r.count == 0xFFFFFFFFu)
+    return false;
+  const float weight = target / proposal;
+  const float sum = r.weight_sum + weight;
+  if (!std::isfinite(weight) || !std::isfinite(sum) || (target > 0.0f && weight == 0.0f)) return false;
+  selected = weight > 0.0f && (r.weight_sum == 0.0f || random < weight / sum);
+  r.weight_sum = sum;
+  ++r.count;
+  if (selected) {
+    r.sample = sample;
+    r.selected_target = target;
+  }
+  return true;
+}
+
+// reservoir.slang:58-62.
+float ReservoirNormalization(const Reservoir& r) {
+  return r.weight_sum > 0.0f ? (r.weight_sum / static_cast<float>(r.count)) / r.selected_target : 0.0f;
+}
+
+// reservoir.slang:64-97.
+bool MergeReevaluated(Reservoir& d, const Reservoir& src, float normalization, float target, uint32_t cap,
+                      float random, bool& selected) {
+  selected = false;
+  const uint32_t count = std::min(src.count, cap);
+  const bool zero = src.weight_sum == 0.0f && src.sample == kNoReservoirSample && src.selected_target == 0.0f &&
+                    normalization == 0.0f && target == 0.0f;
+  const bool positive = src.weight_sum > 0.0f && src.selected_target > 0.0f && src.sample != kNoReservoirSample &&
+                        src.count != 0 && normalization > 0.0f;
+  const bool valid_destination = d.weight_sum == 0.0f
+                                     ? d.sample == kNoReservoirSample && d.selected_target == 0.0f
+                                     : d.weight_sum > 0.0f && d.selected_target > 0.0f &&
+                                           d.sample != kNoReservoirSample && d.count != 0;
+  if (!valid_destination || !std::isfinite(d.weight_sum) || !std::isfinite(d.selected_target) ||
+      !std::isfinite(src.weight_sum) || !std::isfinite(src.selected_target) || !(zero || positive) ||
+      !std::isfinite(normalization) || !std::isfinite(target) || target < 0.0f || cap == 0 ||
+      !std::isfinite(random) || random < 0.0f || random >= 1.0f || count > 0xFFFFFFFFu - d.count)
+    return false;
+  const float weight = (target * normalization) * static_cast<float>(count);
+  const float sum = d.weight_sum + weight;
+  if (!std::isfinite(weight) || !std::isfinite(sum) || (target > 0.0f && weight == 0.0f)) return false;
+  selected = weight > 0.0f && (d.weight_sum == 0.0f || random < weight / sum);
+  d.weight_sum = sum;
+  d.count += count;
+  if (selected) {
+    d.sample = src.sample;
+    d.selected_target = target;
+  }
+  return true;
+}
+
+// reservoir.slang:99-103.
+float ReuseNormalization(const Reservoir& r, uint32_t supporting) {
+  return r.weight_sum > 0.0f ? (r.weight_sum / static_cast<float>(supporting)) / r.selected_target : 0.0f;
+}
+
+// restir.slang:11-17 (RestirPixel, 48 bytes).
+struct RestirPixel {
+  Reservoir r;
+  float u = 0.0f, v = 0.0f;
+  float proposal = 0.0f;
+  uint32_t flags = 0;
+  F3 contribution{0, 0, 0};
+  float visibility = 0.0f;
+};
+
+// scene.slang:919-943 (ChooseReservoirLight, ReservoirProposalPdf):
+// proposal 0 power table, 1 uniform, 2 their mixture.
+uint32_t ChooseReservoirLight(const Scene& s, uint32_t mode, float ux, float uy) {
+  const uint32_t count = static_cast<uint32_t>(s.lights.size());
+  const uint32_t column = std::min(static_cast<uint32_t>(ux * static_cast<float>(count)), count - 1);
+  if (mode == 1) return column;
+  if (mode == 2) {
+    if (uy < 0.5f) return column;
+    uy = (uy - 0.5f) * 2.0f;
+  }
+  const LightRecord& record = s.lights[column];
+  return uy < record.threshold ? column : record.alias;
+}
+float ReservoirProposalPdf(const Scene& s, uint32_t mode, uint32_t index) {
+  const float power = s.lights[index].pdf;
+  const float uniform = 1.0f / static_cast<float>(s.lights.size());
+  if (mode == 1) return uniform;
+  if (mode == 2) return 0.5f * (power + uniform);
+  return power;
+}
+
+// restir.slang:66-74: reflection-side BSDF values only.
+F3 RestirEvaluate(const Closure& b, F3 wo, F3 wi) { return wi.z > 0.0f ? BsdfEvaluate(b, wo, wi) : F3{0, 0, 0}; }
+float RestirPdf(const Closure& b, F3 wo, F3 wi) { return wi.z > 0.0f ? BsdfPdf(b, wo, wi) : 0.0f; }
+
+// restir.slang:157-161 (CandidateContribution): f in the light's unit-square
+// measure, the power selection `selection` cancelled.
+F3 CandidateContribution(const Surface& sf, const Frame& basis, F3 wo, const LightSample& ls, float selection) {
+  if (!(ls.pdf > 0.0f) || Dot(ls.direction, sf.normal) <= 0.0f) return F3{0, 0, 0};
+  return RestirEvaluate(sf.bsdf, wo, basis.ToLocal(ls.direction)) * ls.radiance * (selection / ls.pdf);
 }
 
 struct Path {
@@ -1008,9 +1133,72 @@ F3 DirectLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo,
   return IsFinite(c) ? c : F3{0, 0, 0};
 }
 
+// restir.slang:163-233 (WriteInitialReservoir) at a camera path's first hit:
+// restir_candidates light draws from their own random stream, streamed by
+// the maximum component of their unshadowed contribution, and one visibility
+// ray for the selected one.
+RestirPixel InitialReservoir(const Scene& s, const oracle_frame& fr, uint32_t slot, uint32_t sample,
+                             const Ray& ray, const Hit& hit, Counters& k, bool brute) {
+  RestirPixel px;
+  px.flags = 1;
+  const Surface sf = ReconstructHit(s, hit, ray.d);
+  const Frame basis = MakeShadingFrame(sf.shading_normal);
+  const F3 wo = basis.ToLocal(-ray.d);
+  const F3 origin = OffsetRayOrigin(sf.position, sf.normal);
+  uint32_t rng = SeedRandom(slot, sample ^ 0xD1B54A35u);
+  if (fr.max_bounces != 0 && !s.lights.empty()) {
+    for (uint32_t i = 0; i < fr.restir_candidates; ++i) {
+      const float cx = NextRandom(rng), cy = NextRandom(rng);
+      const float u = NextRandom(rng), v = NextRandom(rng);
+      const float random = NextRandom(rng);
+      const uint32_t light = ChooseReservoirLight(s, fr.restir_proposal, cx, cy);
+      const float proposal = ReservoirProposalPdf(s, fr.restir_proposal, light);
+      const LightSample ls = SampleTriangleLight(s, light, u, v, origin);
+      const F3 f = CandidateContribution(sf, basis, wo, ls, s.lights[light].pdf);
+      bool selected = false;
+      if (!std::isfinite(ls.pdf) || !IsFinite(f) ||
+          !StreamCandidate(px.r, light, MaxComponent(f), proposal, random, selected)) {
+        px.flags |= 2;
+        break;
+      }
+      if (selected) {
+        px.u = u;
+        px.v = v;
+        px.proposal = proposal;
+      }
+    }
+  }
+  if ((px.flags & 2) == 0 && px.r.weight_sum > 0.0f) {
+    const LightSample ls = SampleTriangleLight(s, px.r.sample, px.u, px.v, origin);
+    ++k.shadow;
+    const float visibility = Occluded(s, Ray{origin, 0.0f, ls.direction, ls.distance}, brute) ? 0.0f : 1.0f;
+    const float normalization = ReservoirNormalization(px.r);
+    const F3 estimate = CandidateContribution(sf, basis, wo, ls, s.lights[px.r.sample].pdf) *
+                        (visibility * normalization);
+    if (!std::isfinite(normalization) || !IsFinite(estimate)) {
+      px.flags |= 2;
+    } else {
+      px.contribution = estimate;
+      px.visibility = visibility;
+    }
+  }
+  return px;
+}
+
+// restir.slang:235-254 (InitialDirectLight): the selected light's estimate,
+// weighted against BSDF sampling by the power heuristic of the light
+// table's density.
+F3 InitialDirectLight(const Scene& s, const RestirPixel& px, const Surface& sf, const Frame& basis, F3 wo) {
+  if ((px.flags & 2) != 0 || !(px.r.weight_sum > 0.0f) || px.r.sample >= s.lights.size()) return F3{0, 0, 0};
+  const F3 origin = OffsetRayOrigin(sf.position, sf.normal);
+  const LightSample ls = SampleTriangleLight(s, px.r.sample, px.u, px.v, origin);
+  const float weight = PowerHeuristic(ls.pdf, RestirPdf(sf.bsdf, wo, basis.ToLocal(ls.direction)));
+  return px.contribution * weight;
+}
+
 // transport.slang:164-230.
 bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
-                 const Hit& hit, Counters& k, bool brute) {
+                 const Hit& hit, Counters& k, bool brute, const RestirPixel* restir = nullptr) {
   const bool sample_lights = fr.light_sampling != 0;
   const Surface sf = ReconstructHit(s, hit, path.ray.d);
   F3 emission = sf.emission;
@@ -1034,7 +1222,9 @@ bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
   }
   const Frame basis = MakeShadingFrame(sf.shading_normal);
   const F3 wo = basis.ToLocal(-path.ray.d);
-  if (sample_lights)
+  if (fr.light_sampling == 2 && path.depth == 0)
+    path.radiance = path.radiance + path.throughput * InitialDirectLight(s, *restir, sf, basis, wo);
+  else if (sample_lights)
     path.radiance = path.radiance + path.throughput * DirectLight(s, sf, basis, wo, path.rng, k, brute);
   const float ux = NextRandom(path.rng), uy = NextRandom(path.rng), uz = NextRandom(path.rng);
   SampleResult smp;
@@ -1086,7 +1276,9 @@ bool RunPath(const Scene& s, const oracle_frame& fr, uint32_t x, uint32_t y,
     depth = 0;
     return false;
   }
-  while (ScatterPath(s, fr, path, hit, k, brute)) {
+  RestirPixel restir;
+  if (fr.light_sampling == 2) restir = InitialReservoir(s, fr, y * fr.width + x, sample, path.ray, hit, k, brute);
+  while (ScatterPath(s, fr, path, hit, k, brute, &restir)) {
     ++k.counts[path.depth];
     if (!TraceClosest(s, path.ray, hit, brute)) {
       MissPath(s, path);
@@ -1327,6 +1519,68 @@ uint32_t oracle_trace_path(const oracle_scene* scene, const oracle_frame* frame,
   rgb[1] = rad.y;
   rgb[2] = rad.z;
   return depth;
+}
+
+namespace {
+Reservoir LoadReservoir(const uint32_t* w) {
+  Reservoir r;
+  r.sample = w[0];
+  std::memcpy(&r.weight_sum, w + 1, 4);
+  std::memcpy(&r.selected_target, w + 2, 4);
+  r.count = w[3];
+  return r;
+}
+void StoreReservoir(const Reservoir& r, uint32_t* w) {
+  w[0] = r.sample;
+  std::memcpy(w + 1, &r.weight_sum, 4);
+  std::memcpy(w + 2, &r.selected_target, 4);
+  w[3] = r.count;
+}
+}  // namespace
+
+int oracle_reservoir_stream(uint32_t* reservoir, uint32_t sample, float target, float proposal, float random) {
+  Reservoir r = LoadReservoir(reservoir);
+  bool selected = false;
+  if (!StreamCandidate(r, sample, target, proposal, random, selected)) return -1;
+  StoreReservoir(r, reservoir);
+  return selected ? 1 : 0;
+}
+
+int oracle_reservoir_merge(uint32_t* destination, const uint32_t* source, float normalization, float target,
+                           uint32_t cap, float random) {
+  Reservoir d = LoadReservoir(destination);
+  bool selected = false;
+  if (!MergeReevaluated(d, LoadReservoir(source), normalization, target, cap, random, selected)) return -1;
+  StoreReservoir(d, destination);
+  return selected ? 1 : 0;
+}
+
+float oracle_reservoir_normalization(const uint32_t* reservoir) {
+  return ReservoirNormalization(LoadReservoir(reservoir));
+}
+
+float oracle_reuse_normalization(const uint32_t* reservoir, uint32_t supporting) {
+  return ReuseNormalization(LoadReservoir(reservoir), supporting);
+}
+
+int oracle_restir_initial(const oracle_scene* scene, const oracle_frame* frame, uint32_t x, uint32_t y,
+                          uint32_t sample_index, uint32_t* record) {
+  const oracle_frame& fr = *frame;
+  Path path = StartPath(fr, x, y, sample_index);
+  std::memset(record, 0, 48);
+  record[0] = kNoReservoirSample;
+  Hit hit;
+  if (!TraceClosest(scene->s, path.ray, hit, false)) return 0;
+  std::vector<uint32_t> counts(fr.max_bounces + 2, 0u);
+  Counters c{counts.data(), 0};
+  const RestirPixel px = InitialReservoir(scene->s, fr, y * fr.width + x, sample_index, path.ray, hit, c, false);
+  const float f[12] = {0, px.r.weight_sum, px.r.selected_target, 0, px.u, px.v, px.proposal, 0,
+                       px.contribution.x, px.contribution.y, px.contribution.z, px.visibility};
+  std::memcpy(record, f, 48);
+  record[0] = px.r.sample;
+  record[3] = px.r.count;
+  record[7] = px.flags;
+  return 1;
 }
 
 void oracle_camera_ray(const float* clip_to_world, float nx, float ny, float* ray) {

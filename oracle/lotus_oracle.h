@@ -69,7 +69,9 @@ typedef struct oracle_frame {
   uint32_t sample_index;    /* first sample's random sequence */
   uint32_t spp;             /* samples per pixel this call */
   uint32_t max_bounces;
-  uint32_t light_sampling;  /* 0 BSDF only, 1 NEE + MIS */
+  uint32_t light_sampling;  /* 0 BSDF only, 1 NEE + MIS, 2 ReSTIR initial at the first hit */
+  uint32_t restir_candidates; /* light draws per first hit (light_sampling 2) */
+  uint32_t restir_proposal;   /* 0 power table, 1 uniform, 2 mixture */
 } oracle_frame;
 
 /* Renders spp samples per pixel of the data window. accum: 8 doubles per
@@ -84,6 +86,21 @@ int oracle_render(const oracle_scene* scene, const oracle_frame* frame,
 uint32_t oracle_trace_path(const oracle_scene* scene, const oracle_frame* frame,
                            uint32_t x, uint32_t y, uint32_t sample_index,
                            float* rgb);
+
+/* Reservoir arithmetic in the GPU's float form (reservoir.slang:23-103) on
+ * 4-word reservoirs {sample, weight_sum, selected_target, M}: stream and
+ * merge return 1 selected, 0 not, -1 rejected (reservoir unchanged). */
+int oracle_reservoir_stream(uint32_t* reservoir, uint32_t sample, float target, float proposal, float random);
+int oracle_reservoir_merge(uint32_t* destination, const uint32_t* source, float normalization, float target,
+                           uint32_t cap, float random);
+float oracle_reservoir_normalization(const uint32_t* reservoir);
+float oracle_reuse_normalization(const uint32_t* reservoir, uint32_t supporting);
+
+/* The initial reservoir of pixel (x, y) for a sample (restir.slang:163-233):
+ * RestirPixel's 12 words {sample, weight_sum, target, M, u, v, proposal,
+ * flags, contribution rgb, visibility}. Returns whether the camera ray hit. */
+int oracle_restir_initial(const oracle_scene* scene, const oracle_frame* frame, uint32_t x, uint32_t y,
+                          uint32_t sample_index, uint32_t* record);
 
 /* The camera ray through a film position (reference CameraRay). */
 void oracle_camera_ray(const float* clip_to_world, float ndc_x, float ndc_y,

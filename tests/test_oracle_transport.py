@@ -86,8 +86,10 @@ def _cornell_reference():
     return mean, var
 
 
-@pytest.mark.parametrize("light_sampling", [0, 1])
+@pytest.mark.parametrize("light_sampling", [0, 1, 2])
 def test_cornell_matches_reference_images(light_sampling):
+    # light_sampling 2: ReSTIR's initial reservoir (8 candidates) for the
+    # first hit's direct light, partitioned against BSDF sampling.
     # validation/reference/cornell-box-{mean,variance}.pfm: 128^2, 1024 spp,
     # sample index 0 (docs/reference/SCENE.md:1014-1024); DES-Q5 at 64 spp.
     mean, var = _cornell_reference()
