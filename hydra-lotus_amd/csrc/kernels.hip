@@ -162,8 +162,95 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
         }
         q.nee[slot] = zero;
         q.esc[slot] = zero;
+        if (F.light_sampling == kLightRestir) {
+          // restir.slang:171-176: a failure of an earlier sample of the
+          // accumulation stays flagged
+          const uint32_t failed = F.accumulated != 0 ? (__float_as_uint(q.restir[3ull * slot + 1].w) & 2u) : 0u;
+          q.restir[3ull * slot] = make_float4(__uint_as_float(kNoReservoirSample), 0.0f, 0.0f, 0.0f);
+          q.restir[3ull * slot + 1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(failed));
+          q.restir[3ull * slot + 2] = zero;
+        }
       });
   flush_stats(stats, 0, visits, tests);
+}
+
+// restir.slang:163-233 (WriteInitialReservoir) over the first hits: each
+// draws restir_candidates lights from its own random stream (seeded from the
+// pixel and sample ^ 0xD1B54A35, so the path's own draws are unchanged),
+// streams them by the maximum component of their unshadowed contribution,
+// and traces one visibility ray for the selection (counted apart from the
+// queued shadow rays).
+__global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, FrameConst F, Queues q,
+                                                                uint32_t* counters, uint2* spill, int spill_cap,
+                                                                unsigned long long* stats) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  LdsStack<kRing> st;
+  st.init(ring, spill, spill_cap);
+  const uint32_t count = *hit_count(counters, 0);
+  uint32_t visits = 0, tests = 0;
+  for (uint32_t i = gtid(); i < count; i += gstride()) {
+    const float4 o4 = q.hit_o[i], d4 = q.hit_d[i];
+    const uint4 h4 = q.hit_h[i];
+    const uint32_t slot = __float_as_uint(o4.w);
+    const V3 dir = xyz(d4);
+    const Surface sf = reconstruct_hit(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z), __uint_as_float(h4.w),
+                                       dir);
+    const V3 n = sf.shading_normal;
+    const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    const V3 ft = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    const V3 fb = V3{b, sign + n.y * n.y * a, -n.y};
+    const V3 md = -dir;
+    const V3 wo = V3{dot3(md, ft), dot3(md, fb), dot3(md, n)};
+    const V3 origin = offset_ray_origin(sf.position, sf.normal);
+    Reservoir r = empty_reservoir();
+    float ru = 0.0f, rv = 0.0f, rproposal = 0.0f;
+    uint32_t flags = 1u | (F.accumulated != 0 ? (__float_as_uint(q.restir[3ull * slot + 1].w) & 2u) : 0u);
+    uint32_t rng = seed_random(slot, F.sample_index ^ 0xD1B54A35u);
+    if (F.max_bounces != 0 && S.light_count != 0) {
+      for (uint32_t k = 0; k < F.restir_candidates; ++k) {
+        const float cx = next_random(rng), cy = next_random(rng);
+        const float u = next_random(rng), v = next_random(rng);
+        const float random = next_random(rng);
+        const uint32_t light = choose_reservoir_light(S, F.restir_proposal, cx, cy);
+        const float proposal = reservoir_proposal_pdf(S, F.restir_proposal, light);
+        const LightSample ls = sample_triangle_light(S, light, u, v, origin);
+        const V3 f = candidate_contribution(sf, ft, fb, wo, ls, S.lights[light].pdf);
+        bool selected = false;
+        if (!isfinite(ls.pdf) || !finite3(f) || !stream_candidate(r, light, max3(f), proposal, random, selected)) {
+          flags |= 2u;
+          break;
+        }
+        if (selected) {
+          ru = u;
+          rv = v;
+          rproposal = proposal;
+        }
+      }
+    }
+    float4 contribution = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool trace = (flags & 2u) == 0u && r.weight_sum > 0.0f;
+    wave_count(restir_ray_count(counters), trace);
+    if (trace) {
+      const LightSample ls = sample_triangle_light(S, r.sample, ru, rv, origin);
+      const float visibility =
+          walk_occluded(S.nodes, S.tris, st, Ray{origin, 0.0f, ls.direction, ls.distance}, visits, tests) ? 0.0f
+                                                                                                          : 1.0f;
+      const float normalization = reservoir_normalization(r);
+      const V3 estimate = candidate_contribution(sf, ft, fb, wo, ls, S.lights[r.sample].pdf) *
+                          (visibility * normalization);
+      if (!isfinite(normalization) || !finite3(estimate))
+        flags |= 2u;
+      else
+        contribution = make_float4(estimate.x, estimate.y, estimate.z, visibility);
+    }
+    q.restir[3ull * slot] = make_float4(__uint_as_float(r.sample), r.weight_sum, r.selected_target,
+                                        __uint_as_float(r.count));
+    q.restir[3ull * slot + 1] = make_float4(ru, rv, rproposal, __uint_as_float(flags));
+    q.restir[3ull * slot + 2] = contribution;
+  }
+  flush_stats(stats, 2, visits, tests);
 }
 
 // A cache query for a surface reached along `dir`: position in the cache's
@@ -230,7 +317,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
       s.shadow = false;
     } else {
       s = scatter_path(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z), __uint_as_float(h4.w),
-                       train ? &sf : nullptr);
+                       train ? &sf : nullptr, slot);
     }
     const uint32_t kq = wave_append(cache_query_count(counters), query);
     if (query) {
@@ -583,6 +670,9 @@ void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F
                         uint32_t write_color) {
   k_camera_pass<<<L.grid.camera_pass, kTraceBlock, 0, L.stream>>>(S, F, q, depth, write_color, L.spill,
                                                                    L.spill_cap);
+}
+void launch_restir_initial(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
+  k_restir_initial<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q, uint32_t* c, uint32_t capacity) {
   k_cache_samples<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, c, capacity);

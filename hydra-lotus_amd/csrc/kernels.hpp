@@ -37,6 +37,8 @@ struct Queues {
   float4* crec;      // training vertices [bounce][path], 6 float4 each: query, throughput, radiance so far
   uint32_t* crec_n;  // per training path: vertices recorded
   float* samples;    // training samples, kSampleFloats each
+  // ReSTIR DI records, 3 float4 per pixel (initial reservoirs).
+  float4* restir;
 };
 
 // Training-path vertices recorded per path (deeper ones only contribute to
@@ -83,11 +85,20 @@ void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F
                         uint32_t write_color);
 void launch_rays(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
                  uint32_t any_hit);
+// ReSTIR DI's initial reservoirs at the first hits (queued by the camera
+// kernel), with their visibility rays traced inline.
+void launch_restir_initial(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c);
 // Radiance cache frames: targets (L - R) / T of every recorded vertex of the
 // training paths, appended to q.samples (counter word 7), at most `capacity`.
 void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q, uint32_t* c, uint32_t capacity);
 // Cache query and sample counters.
 __host__ __device__ inline uint32_t* cache_query_count(uint32_t* c) { return c + 3; }
 __host__ __device__ inline uint32_t* cache_sample_count(uint32_t* c) { return c + 7; }
+// ReSTIR visibility rays traced inline by the reservoir passes (not queued).
+__host__ __device__ inline uint32_t* restir_ray_count(uint32_t* c) { return c + 11; }
+// Counter words a frame clears: 4 per bounce, at least the three above.
+__host__ __device__ inline uint32_t frame_counter_words(uint32_t bounce_stride) {
+  return 4u * bounce_stride > 12u ? 4u * bounce_stride : 12u;
+}
 
 }  // namespace lotus

@@ -112,14 +112,18 @@ struct AccumulationKey {  // vulkan_backend.cpp:419-434
   uint32_t light_sampling = 0, sample_index = 0, max_bounces = 0;
   uint64_t scene_generation = 0;
   uint32_t cache_frames = 0;  // radiance cache frames accumulate apart
+  uint32_t restir_candidates = 0, restir_proposal = 0;
   bool operator==(const AccumulationKey& o) const {
     return world_to_clip == o.world_to_clip && width == o.width && height == o.height && display == o.display &&
            data == o.data && light_sampling == o.light_sampling && sample_index == o.sample_index &&
-           max_bounces == o.max_bounces && scene_generation == o.scene_generation && cache_frames == o.cache_frames;
+           max_bounces == o.max_bounces && scene_generation == o.scene_generation && cache_frames == o.cache_frames &&
+           restir_candidates == o.restir_candidates && restir_proposal == o.restir_proposal;
   }
 };
 
-enum Timer { kTCamera, kTIntersect, kTShade, kTShadow, kTTrace, kTTail, kTResolve, kTCacheInfer, kTCacheTrain, kTimers };
+enum Timer {
+  kTCamera, kTIntersect, kTShade, kTShadow, kTTrace, kTTail, kTResolve, kTCacheInfer, kTCacheTrain, kTRestir, kTimers
+};
 
 }  // namespace
 
@@ -187,6 +191,7 @@ void free_target(lotus_hip_renderer* r) {
   dev_free(q.crec);
   dev_free(q.crec_n);
   dev_free(q.samples);
+  dev_free(q.restir);
   dev_free(r->depth);
   r->width = r->height = 0;
   r->cache_pixels = 0;
@@ -643,8 +648,16 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     set_detail(detail, detail_size, "the wavefront integrator traces at most 1024 bounces");
     return LOTUS_HIP_FAIL;
   }
-  if (P->light_sampling > LOTUS_HIP_LIGHT_MIS) {
+  if (P->light_sampling > LOTUS_HIP_LIGHT_RESTIR) {
     set_detail(detail, detail_size, "unknown light sampling");
+    return LOTUS_HIP_FAIL;
+  }
+  if (P->light_sampling == LOTUS_HIP_LIGHT_RESTIR && (P->restir_candidates == 0 || P->restir_candidates > 64)) {
+    set_detail(detail, detail_size, "initial reservoir candidates must be in [1, 64]");
+    return LOTUS_HIP_FAIL;
+  }
+  if (P->restir_proposal > 2) {
+    set_detail(detail, detail_size, "unknown reservoir proposal");
     return LOTUS_HIP_FAIL;
   }
   if (P->width == 0 || P->height == 0) {
@@ -694,6 +707,8 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
 
   hipError_t e = ensure_target(r, P->width, P->height);
   if (e == hipSuccess && cache_frame) e = ensure_cache_target(r);
+  const bool restir = P->light_sampling == LOTUS_HIP_LIGHT_RESTIR && P->output == LOTUS_HIP_OUTPUT_RADIANCE;
+  if (e == hipSuccess && restir && !r->q.restir) e = dev_alloc(r->q.restir, 3ull * P->width * P->height);
   if (e == hipSuccess && cache_frame) e = ensure_bounds(r);
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "target allocation", e));
@@ -708,6 +723,9 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       F.cache_scale[a] = r->scene_scale[a];
     }
   }
+  F.restir_candidates = P->restir_candidates;
+  F.restir_proposal = P->restir_proposal;
+  F.restir = r->q.restir;
   uint64_t cache_queries = 0, cache_samples = 0, cache_used = 0;
   double cache_loss_sum = 0.0;
   const uint32_t npix = P->width * P->height;
@@ -736,6 +754,8 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     key.max_bounces = P->max_bounces;
     key.scene_generation = r->scene_generation;
     key.cache_frames = cache_frame ? 1u : 0u;
+    key.restir_candidates = restir ? P->restir_candidates : 0u;
+    key.restir_proposal = restir ? P->restir_proposal : 0u;
     if (!r->key_valid || !(r->key == key)) {
       r->key = key;
       r->key_valid = true;
@@ -764,7 +784,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
         (void)hipEventRecord(r->events[ev++], r->stream);
       };
       auto done = [&]() { (void)hipEventRecord(r->events[ev++], r->stream); };
-      launch_zero(r->counters, 4 * stride, r->stream);
+      launch_zero(r->counters, frame_counter_words(stride), r->stream);
       launch_zero(r->work, kWorkCounters, r->stream);
       if (cache_frame) {
         launch_zero(r->q.crec_n, r->train_capacity, r->stream);
@@ -776,6 +796,11 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
         L.work = r->work + next_work++;
         launch_camera(L, r->view, F, r->q, r->counters);
         done();
+        if (restir) {
+          mark(kTRestir);
+          launch_restir_initial(L, r->view, F, r->q, r->counters);
+          done();
+        }
         // round b: shade bounce b, then trace the extension rays of bounce
         // b + 1 (unless the tail takes them) with the shadow rays of bounce b
         for (uint32_t b = 0; b < rounds; ++b) {
@@ -833,7 +858,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
         if (e == hipSuccess)
           e = hipMemcpyAsync(&loss_host, r->cache.loss, sizeof(float), hipMemcpyDeviceToHost, r->stream);
       }
-      std::vector<uint32_t> host(4 * stride);
+      std::vector<uint32_t> host(frame_counter_words(stride));
       e = hipGetLastError();
       if (e == hipSuccess)
         e = hipMemcpyAsync(host.data(), r->counters, host.size() * 4, hipMemcpyDeviceToHost, r->stream);
@@ -868,6 +893,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
           shadow_rays += host[4 * b + 2];
         }
         camera_rays += host[0];
+        shadow_rays += *restir_ray_count(host.data());
         counts.resize(host[0] ? last + 1 : 0);
         r->rounds_hint = wavefront_rounds(counts);
         rounds_used = rounds;
@@ -891,6 +917,8 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       e = hipMemcpyAsync(res->color, r->q.color, npix * 16ull, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess && res->depth)
       e = hipMemcpyAsync(res->depth, r->depth, npix * 4ull, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess && res->restir_records && restir && r->q.restir)
+      e = hipMemcpyAsync(res->restir_records, r->q.restir, npix * 48ull, hipMemcpyDeviceToHost, r->stream);
   }
   unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (e == hipSuccess) e = hipGetLastError();
@@ -927,6 +955,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     res->cache_loss = cache_used ? cache_loss_sum / (3.0 * static_cast<double>(cache_used)) : 0.0;
     res->cache_infer_gpu_ms = t_ms[kTCacheInfer];
     res->cache_train_gpu_ms = t_ms[kTCacheTrain];
+    res->restir_gpu_ms = t_ms[kTRestir];
   }
   set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;

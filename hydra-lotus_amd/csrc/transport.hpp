@@ -32,6 +32,11 @@ struct FrameConst {
   uint32_t cache_depth;
   uint32_t train_stride, train_offset, train_capacity;
   float cache_lo[3], cache_scale[3];
+  // ReSTIR DI (light_sampling kLightRestir): draws per first hit, proposal
+  // (0 power table, 1 uniform, 2 mixture) and the records the first hit's
+  // direct light reads (the last reuse pass's).
+  uint32_t restir_candidates, restir_proposal;
+  const float4* restir;
 };
 
 // The path state a kernel holds in registers.
@@ -144,15 +149,11 @@ struct LightSample {
   float pdf;
 };
 
-// scene.slang:894-909 (SampleLight) with SampleTriangle (:632-677).
-__device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux, float uy, float uz, float uw,
-                                                    V3 origin) {
+// scene.slang:632-677 (SampleTriangle): a point uniformly by area on light
+// `light` of the table, (uz, uw) its unit-square coordinates.
+__device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S, uint32_t light, float uz, float uw,
+                                                             V3 origin) {
   LightSample none{V3{0.0f, 0.0f, 1.0f}, 0.0f, V3{0.0f, 0.0f, 0.0f}, 0.0f};
-  const uint32_t count = S.light_count;
-  const uint32_t c = static_cast<uint32_t>(ux * static_cast<float>(count));
-  const uint32_t column = c < count - 1 ? c : count - 1;
-  const LightRecord rec = S.lights[column];
-  const uint32_t light = uy < rec.threshold ? column : rec.alias;
   const LightRecord chosen = S.lights[light];
   const uint32_t prim = light - S.light_offsets[chosen.instance];
   const float root = sqrtf(uz);
@@ -178,6 +179,94 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux
   ls.radiance = radiance;
   ls.pdf = chosen.pdf / area * d2 / cosine;
   return ls;
+}
+
+// scene.slang:894-909 (SampleLight): the alias table's column and threshold
+// choose the light.
+__device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux, float uy, float uz, float uw,
+                                                    V3 origin) {
+  const uint32_t count = S.light_count;
+  const uint32_t c = static_cast<uint32_t>(ux * static_cast<float>(count));
+  const uint32_t column = c < count - 1 ? c : count - 1;
+  const LightRecord rec = S.lights[column];
+  const uint32_t light = uy < rec.threshold ? column : rec.alias;
+  return sample_triangle_light(S, light, uz, uw, origin);
+}
+
+// ------------------------------------------------------------- ReSTIR DI
+// A reservoir record (restir.slang:11-17, 48 bytes) as 3 float4: {sample,
+// weight_sum, selected_target, M}, {u, v, proposal, flags}, {contribution
+// rgb, visibility}. flags: bit 0 the camera ray hit, bit 1 a failed stream.
+constexpr uint32_t kNoReservoirSample = 0xFFFFFFFFu;
+constexpr uint32_t kLightRestir = 2u;  // light_sampling: ReSTIR's initial reservoir at the first hit
+
+struct Reservoir {
+  uint32_t sample;
+  float weight_sum, selected_target;
+  uint32_t count;
+};
+
+__device__ __forceinline__ Reservoir empty_reservoir() { return Reservoir{kNoReservoirSample, 0.0f, 0.0f, 0u}; }
+
+// reservoir.slang:23-56 (StreamReservoirCandidate).
+__device__ __forceinline__ bool stream_candidate(Reservoir& r, uint32_t sample, float target, float proposal,
+                                                 float random, bool& selected) {
+  selected = false;
+  const bool valid_zero = r.weight_sum == 0.0f && r.sample == kNoReservoirSample && r.selected_target == 0.0f;
+  const bool valid_positive = r.weight_sum > 0.0f && r.sample != kNoReservoirSample && r.selected_target > 0.0f &&
+                              r.count != 0u;
+  if (!isfinite(r.weight_sum) || !isfinite(r.selected_target) || !(valid_zero || valid_positive) ||
+      !isfinite(target) || target < 0.0f || !isfinite(proposal) || proposal <= 0.0f || !isfinite(random) ||
+      random < 0.0f || random >= 1.0f || (target > 0.0f && sample == kNoReservoirSample) || r.count == 0xFFFFFFFFu)
+    return false;
+  const float weight = target / proposal;
+  const float sum = r.weight_sum + weight;
+  if (!isfinite(weight) || !isfinite(sum) || (target > 0.0f && weight == 0.0f)) return false;
+  selected = weight > 0.0f && (r.weight_sum == 0.0f || random < weight / sum);
+  r.weight_sum = sum;
+  ++r.count;
+  if (selected) {
+    r.sample = sample;
+    r.selected_target = target;
+  }
+  return true;
+}
+
+// reservoir.slang:58-62.
+__device__ __forceinline__ float reservoir_normalization(const Reservoir& r) {
+  return r.weight_sum > 0.0f ? (r.weight_sum / static_cast<float>(r.count)) / r.selected_target : 0.0f;
+}
+
+// scene.slang:919-943 (ChooseReservoirLight, ReservoirProposalPdf).
+__device__ __forceinline__ uint32_t choose_reservoir_light(const SceneView& S, uint32_t mode, float ux, float uy) {
+  const uint32_t count = S.light_count;
+  const uint32_t c = static_cast<uint32_t>(ux * static_cast<float>(count));
+  const uint32_t column = c < count - 1 ? c : count - 1;
+  if (mode == 1u) return column;
+  if (mode == 2u) {
+    if (uy < 0.5f) return column;
+    uy = (uy - 0.5f) * 2.0f;
+  }
+  const LightRecord rec = S.lights[column];
+  return uy < rec.threshold ? column : rec.alias;
+}
+__device__ __forceinline__ float reservoir_proposal_pdf(const SceneView& S, uint32_t mode, uint32_t light) {
+  const float power = S.lights[light].pdf;
+  const float uniform = 1.0f / static_cast<float>(S.light_count);
+  if (mode == 1u) return uniform;
+  if (mode == 2u) return 0.5f * (power + uniform);
+  return power;
+}
+
+// restir.slang:66-74 and 157-161: reflection-side values only; f in the
+// light's unit-square measure with the power selection cancelled.
+__device__ __forceinline__ V3 candidate_contribution(const Surface& sf, const V3& ft, const V3& fb, V3 wo,
+                                                     const LightSample& ls, float selection) {
+  if (!(ls.pdf > 0.0f) || dot3(ls.direction, sf.normal) <= 0.0f) return V3{0.0f, 0.0f, 0.0f};
+  const V3 n = sf.shading_normal;
+  const V3 wi = V3{dot3(ls.direction, ft), dot3(ls.direction, fb), dot3(ls.direction, n)};
+  if (!(wi.z > 0.0f)) return V3{0.0f, 0.0f, 0.0f};
+  return bsdf_evaluate(sf.bsdf, wo, wi) * ls.radiance * (selection / ls.pdf);
 }
 
 // scene.slang:1004-1010.
@@ -223,12 +312,30 @@ __device__ __forceinline__ void add_emission(const SceneView& S, const FrameCons
   p.radiance = p.radiance + p.throughput * emission;
 }
 
+// restir.slang:235-254 (InitialDirectLight): the first hit's direct light
+// from its reservoir record, weighted against BSDF sampling by the power
+// heuristic of the light table's density.
+__device__ __forceinline__ V3 initial_direct_light(const SceneView& S, const float4* rec, const Surface& sf,
+                                                   const V3& ft, const V3& fb, V3 wo) {
+  const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+  const uint32_t light = __float_as_uint(r0.x);
+  if ((__float_as_uint(r1.w) & 2u) != 0u || !(r0.y > 0.0f) || light >= S.light_count)
+    return V3{0.0f, 0.0f, 0.0f};
+  const V3 origin = offset_ray_origin(sf.position, sf.normal);
+  const LightSample ls = sample_triangle_light(S, light, r1.x, r1.y, origin);
+  const V3 n = sf.shading_normal;
+  const V3 wi = V3{dot3(ls.direction, ft), dot3(ls.direction, fb), dot3(ls.direction, n)};
+  const float bsdf = wi.z > 0.0f ? bsdf_pdf(sf.bsdf, wo, wi) : 0.0f;
+  return xyz(r2) * power_heuristic(ls.pdf, bsdf);
+}
+
 // transport.slang:164-230 (ScatterPath) with DirectLight (:123-135) and
 // ShadowedLight (:82-115) up to the shadow ray, which the caller traces.
-// `keep`, when given, receives the reconstructed surface.
+// `keep`, when given, receives the reconstructed surface; `slot` indexes the
+// ReSTIR record of the first hit.
 __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameConst& F, PathRegs& p,
                                                 uint32_t tri, uint32_t instance, uint32_t primitive, float u,
-                                                float v, Surface* keep = nullptr) {
+                                                float v, Surface* keep = nullptr, uint32_t slot = 0) {
   Scatter out;
   out.continues = false;
   out.shadow = false;
@@ -251,7 +358,9 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
   const V3 fb = V3{b, sign + n.y * n.y * a, -n.y};
   const V3 md = -p.d;
   const V3 wo = V3{dot3(md, ft), dot3(md, fb), dot3(md, n)};
-  if (sample_lights && S.light_count != 0) {
+  if (F.light_sampling == kLightRestir && p.depth == 0) {
+    p.radiance = p.radiance + p.throughput * initial_direct_light(S, F.restir + 3ull * slot, sf, ft, fb, wo);
+  } else if (sample_lights && S.light_count != 0) {
     const float ux = next_random(p.rng), uy = next_random(p.rng), uz = next_random(p.rng),
                 uw = next_random(p.rng);
     const V3 origin = offset_ray_origin(sf.position, sf.normal);

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG, "liblotus_hip.so")
 PASS, FAIL, SKIP = 0, 1, 77
 MISS = 0xFFFFFFFF
 OUTPUT_RADIANCE, OUTPUT_BARYCENTRICS, OUTPUT_SHADING_NORMAL, OUTPUT_ALBEDO, OUTPUT_ROUGHNESS_METALLIC = range(5)
-LIGHT_BSDF, LIGHT_MIS = 0, 1
+LIGHT_BSDF, LIGHT_MIS, LIGHT_RESTIR = 0, 1, 2
 FRAME_SPLIT_KINDS = 1
 FRAME_RADIANCE_CACHE = 2
 CACHE_PARAMS = 16384
@@ -91,6 +91,8 @@ class FrameParams(C.Structure):
         ("max_bounces", C.c_uint32),
         ("max_samples", C.c_uint32),
         ("flags", C.c_uint32),
+        ("restir_candidates", C.c_uint32),
+        ("restir_proposal", C.c_uint32),
     ]
 
 
@@ -122,6 +124,8 @@ class FrameResult(C.Structure):
         ("cache_loss", C.c_double),
         ("cache_infer_gpu_ms", C.c_double),
         ("cache_train_gpu_ms", C.c_double),
+        ("restir_records", _u32p),
+        ("restir_gpu_ms", C.c_double),
     ]
 
 
@@ -267,7 +271,8 @@ class Renderer:
     def render(self, world_to_clip, width, height, frame_count=1, sample_index=0, max_bounces=8,
                light_sampling=LIGHT_MIS, output=OUTPUT_RADIANCE, clear_color=(0.0, 0.0, 0.0, 0.0),
                data_window=(0, 0, 0, 0), display_window=(0.0, 0.0, 0.0, 0.0), max_samples=0,
-               want_color=True, want_depth=False, count_capacity=None, split_kinds=False, cache=False):
+               want_color=True, want_depth=False, count_capacity=None, split_kinds=False, cache=False,
+               restir_candidates=8, restir_proposal=0, want_restir=False):
         fp = FrameParams()
         w2c = np.ascontiguousarray(world_to_clip, np.float32).reshape(16)
         for i in range(16):
@@ -284,6 +289,8 @@ class Renderer:
         fp.max_bounces = max_bounces
         fp.max_samples = max_samples
         fp.flags = (FRAME_SPLIT_KINDS if split_kinds else 0) | (FRAME_RADIANCE_CACHE if cache else 0)
+        fp.restir_candidates = restir_candidates
+        fp.restir_proposal = restir_proposal
         res = FrameResult()
         color = np.zeros((height, width, 4), np.float32) if want_color else None
         depth = np.zeros((height, width), np.float32) if want_depth else None
@@ -294,6 +301,9 @@ class Renderer:
         if depth is not None:
             res.depth = _p(depth, C.c_float)
         res.path_counts = _p(counts, C.c_uint32)
+        restir = np.zeros((height, width, 12), np.uint32) if want_restir else None
+        if restir is not None:
+            res.restir_records = _p(restir, C.c_uint32)
         res.path_counts_capacity = cap
         buf = C.create_string_buffer(512)
         s = lib().lotus_hip_render(self.handle, C.byref(fp), C.byref(res), buf, 512)
@@ -308,7 +318,8 @@ class Renderer:
                     resolve_ms=res.resolve_gpu_ms, frame_ms=res.frame_gpu_ms,
                     nodes=list(res.traversal_nodes), tris=list(res.traversal_tris),
                     cache_queries=res.cache_queries, cache_samples=res.cache_samples, cache_loss=res.cache_loss,
-                    cache_infer_ms=res.cache_infer_gpu_ms, cache_train_ms=res.cache_train_gpu_ms)
+                    cache_infer_ms=res.cache_infer_gpu_ms, cache_train_ms=res.cache_train_gpu_ms,
+                    restir=restir, restir_ms=res.restir_gpu_ms)
 
     def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

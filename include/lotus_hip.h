@@ -85,6 +85,10 @@ typedef struct lotus_hip_scene_evidence {
 /* LightSampling (vulkan_backend.hpp:652-674). */
 #define LOTUS_HIP_LIGHT_BSDF 0u
 #define LOTUS_HIP_LIGHT_MIS 1u
+/* LightSampling::RestirInitial (vulkan_backend.hpp:650-660, scene.slang:249):
+ * the first hit's direct light from a ReSTIR initial reservoir of
+ * restir_candidates light draws (restir.slang:163-254); later hits as MIS. */
+#define LOTUS_HIP_LIGHT_RESTIR 2u
 
 /* One RenderScene call (vulkan_backend.hpp:789-791): DrawSummary::
  * world_to_clip, OffscreenTarget (vulkan_backend.hpp:253-275) and
@@ -102,6 +106,8 @@ typedef struct lotus_hip_frame_params {
   uint32_t max_bounces;       /* scattering events after the camera hit */
   uint32_t max_samples;       /* 0 = no limit */
   uint32_t flags;             /* LOTUS_HIP_FRAME_* */
+  uint32_t restir_candidates; /* LOTUS_HIP_LIGHT_RESTIR: light draws per first hit, 1..64 */
+  uint32_t restir_proposal;   /* 0 power table, 1 uniform, 2 their mixture (scene.slang:919-943) */
 } lotus_hip_frame_params;
 
 /* Trace extension and shadow rays in separate launches, so each kind's
@@ -146,6 +152,12 @@ typedef struct lotus_hip_frame_result {
   double cache_loss;
   double cache_infer_gpu_ms;
   double cache_train_gpu_ms;
+  /* LOTUS_HIP_LIGHT_RESTIR: the last sample's reservoir records (RestirPixel,
+   * vulkan_backend.hpp:152-164: 12 words per pixel {sample, weight_sum,
+   * selected_target, M, u, v, proposal, flags, contribution rgb, visibility}),
+   * optional, and the reservoir passes' GPU time. */
+  uint32_t* restir_records;
+  double restir_gpu_ms;
 } lotus_hip_frame_result;
 
 /* ProbeVulkanBackend (vulkan_backend.hpp:742): PASS when a gfx950 device is

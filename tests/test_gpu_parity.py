@@ -121,7 +121,7 @@ def _oracle_frame(sc, w, h, spp, bounces, light):
     return oracle.make_frame(sc.world_to_clip, w, h, spp=spp, max_bounces=bounces, light_sampling=light)
 
 
-@pytest.mark.parametrize("light", [hip.LIGHT_BSDF, hip.LIGHT_MIS])
+@pytest.mark.parametrize("light", [hip.LIGHT_BSDF, hip.LIGHT_MIS, hip.LIGHT_RESTIR])
 def test_cornell_lambert_counts_and_radiance(renderer, light):
     # The CPU plain-PT workload: 256x256, 16 spp, max depth 8.
     _say(f"cornell lambert light={light}")
@@ -152,7 +152,7 @@ def test_cornell_lambert_counts_and_radiance(renderer, light):
     assert np.array_equal(sums[..., 3], acc[..., 3].astype(np.float32))
 
 
-@pytest.mark.parametrize("light", [hip.LIGHT_BSDF, hip.LIGHT_MIS])
+@pytest.mark.parametrize("light", [hip.LIGHT_BSDF, hip.LIGHT_MIS, hip.LIGHT_RESTIR])
 def test_cornell_v1_matches_reference_images(renderer, light):
     _say(f"cornell-v1 DES-Q5 light={light}")
     mean = desq5.read_pfm(f"{GOLDEN}/cornell-box-mean.pfm")
@@ -234,9 +234,46 @@ def test_heightfield_counts_and_primary_hits(renderer):
     assert np.all(bary["depth"].reshape(-1)[hit] < 1.0)
 
 
+def test_restir_initial_records_match_oracle(renderer):
+    # restir.slang:163-233 on the reference Cornell scene (GGX metal block,
+    # coated walls): every pixel's initial reservoir, 16 candidates, the
+    # power-table proposal and the mixture.
+    sc = scenes.cornell_v1()
+    renderer.apply(sc)
+    o = oracle.OracleScene(sc)
+    w = h = 64
+    for proposal in (0, 2):
+        _say(f"restir initial records proposal={proposal}")
+        out = renderer.render(sc.world_to_clip, w, h, frame_count=1, sample_index=11, max_bounces=8,
+                              light_sampling=hip.LIGHT_RESTIR, restir_candidates=16, restir_proposal=proposal,
+                              want_restir=True)
+        # visibility rays are the frame's only bounce-0 shadow rays
+        f = oracle.make_frame(sc.world_to_clip, w, h, spp=1, sample_index=11, max_bounces=8, light_sampling=2,
+                              restir_candidates=16, restir_proposal=proposal)
+        g = out["restir"]
+        exact = 0
+        for y in range(h):
+            for x in range(w):
+                ref = o.restir_initial(f, x, y, 11)
+                rec = g[y, x]
+                assert ref is not None
+                assert rec[0] == ref[0] and rec[3] == ref[3] and rec[7] == ref[7], (x, y, rec, ref)
+                exact += np.array_equal(rec, ref)
+                fw = [1, 2, 4, 5, 6, 8, 9, 10, 11]  # the float words
+                assert np.allclose(rec.view(np.float32)[fw], ref.view(np.float32)[fw], rtol=1e-5, atol=1e-7)
+        _say(f"  bit-exact records {exact}/{w * h}")
+        assert exact >= 0.999 * w * h
+        acc, counts, shadow = o.render(f)
+        assert out["shadow_rays"] == shadow[0]
+
+
 def test_settings_errors_do_not_latch(renderer):
     sc = scenes.cornell_lambert()
     renderer.apply(sc)
+    with pytest.raises(hip.LotusError, match=r"candidates must be in \[1, 64\]"):
+        renderer.render(sc.world_to_clip, 16, 16, light_sampling=hip.LIGHT_RESTIR, restir_candidates=0)
+    with pytest.raises(hip.LotusError, match="unknown reservoir proposal"):
+        renderer.render(sc.world_to_clip, 16, 16, light_sampling=hip.LIGHT_RESTIR, restir_proposal=3)
     with pytest.raises(hip.LotusError, match="at most 1024 bounces"):
         renderer.render(sc.world_to_clip, 16, 16, max_bounces=1025)
     with pytest.raises(hip.LotusError, match="singular"):
