@@ -16,7 +16,7 @@
 // at the next bounce, the tail or the resolve adds them, so every path's
 // radiance is summed in the reference's order (emission, direct light, next
 // emission ..., environment).
-#include "kernels.hpp"
+#include "temporal.hpp"
 
 namespace lotus {
 namespace {
@@ -165,10 +165,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
         if (F.light_sampling == kLightRestir) {
           // restir.slang:171-176: a failure of an earlier sample of the
           // accumulation stays flagged
-          const uint32_t failed = F.accumulated != 0 ? (__float_as_uint(q.restir[3ull * slot + 1].w) & 2u) : 0u;
-          q.restir[3ull * slot] = make_float4(__uint_as_float(kNoReservoirSample), 0.0f, 0.0f, 0.0f);
-          q.restir[3ull * slot + 1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(failed));
-          q.restir[3ull * slot + 2] = zero;
+          const uint32_t failed = F.accumulated != 0 ? (__float_as_uint(q.restir[4ull * slot + 1].w) & 2u) : 0u;
+          q.restir[4ull * slot] = make_float4(__uint_as_float(kNoReservoirSample), 0.0f, 0.0f, 0.0f);
+          q.restir[4ull * slot + 1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(failed));
+          q.restir[4ull * slot + 2] = zero;
         }
       });
   flush_stats(stats, 0, visits, tests);
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
     const V3 origin = offset_ray_origin(sf.position, sf.normal);
     Reservoir r = empty_reservoir();
     float ru = 0.0f, rv = 0.0f, rproposal = 0.0f;
-    uint32_t flags = 1u | (F.accumulated != 0 ? (__float_as_uint(q.restir[3ull * slot + 1].w) & 2u) : 0u);
+    uint32_t flags = 1u | (F.accumulated != 0 ? (__float_as_uint(q.restir[4ull * slot + 1].w) & 2u) : 0u);
     uint32_t rng = seed_random(slot, F.sample_index ^ 0xD1B54A35u);
     if (F.max_bounces != 0 && S.light_count != 0) {
       for (uint32_t k = 0; k < F.restir_candidates; ++k) {
@@ -245,10 +245,307 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
       else
         contribution = make_float4(estimate.x, estimate.y, estimate.z, visibility);
     }
-    q.restir[3ull * slot] = make_float4(__uint_as_float(r.sample), r.weight_sum, r.selected_target,
+    q.restir[4ull * slot] = make_float4(__uint_as_float(r.sample), r.weight_sum, r.selected_target,
                                         __uint_as_float(r.count));
-    q.restir[3ull * slot + 1] = make_float4(ru, rv, rproposal, __uint_as_float(flags));
-    q.restir[3ull * slot + 2] = contribution;
+    q.restir[4ull * slot + 1] = make_float4(ru, rv, rproposal, __uint_as_float(flags));
+    q.restir[4ull * slot + 2] = contribution;
+    if (F.restir_surf) {
+      float view[4];
+      mul_point(F.view, sf.position.x, sf.position.y, sf.position.z, 1.0f, view);
+      store_reuse_surface(F.restir_surf + 8ull * slot, sf, origin, -view[2] / view[3], wo,
+                          stable_instance_id(F, h4.x), stable_material_id(F, S.instance_material, h4.x));
+    }
+  }
+  flush_stats(stats, 2, visits, tests);
+}
+
+// Every pixel's initial record emptied (temporal frames: the reuse passes
+// read records outside the data window too).
+__global__ void __launch_bounds__(256) k_restir_clear(float4* restir, uint32_t pixels) {
+  for (uint32_t i = gtid(); i < pixels; i += gstride()) {
+    restir[4ull * i] = make_float4(__uint_as_float(kNoReservoirSample), 0.0f, 0.0f, 0.0f);
+    restir[4ull * i + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    restir[4ull * i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    restir[4ull * i + 3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+// A reuse record in registers: the reservoir record and {normalization,
+// supporting candidates, previous index, age}.
+struct ReuseRecord {
+  Reservoir r;
+  float u, v, proposal;
+  uint32_t flags;
+  float4 contribution;
+  float normalization;
+  uint32_t supporting, previous, age;
+};
+
+__device__ __forceinline__ ReuseRecord load_reuse(const float4* p) {
+  const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+  ReuseRecord x;
+  x.r = Reservoir{__float_as_uint(a.x), a.y, a.z, __float_as_uint(a.w)};
+  x.u = b.x;
+  x.v = b.y;
+  x.proposal = b.z;
+  x.flags = __float_as_uint(b.w);
+  x.contribution = c;
+  x.normalization = d.x;
+  x.supporting = __float_as_uint(d.y);
+  x.previous = __float_as_uint(d.z);
+  x.age = __float_as_uint(d.w);
+  return x;
+}
+
+__device__ __forceinline__ void store_reuse(float4* p, const ReuseRecord& x) {
+  p[0] = make_float4(__uint_as_float(x.r.sample), x.r.weight_sum, x.r.selected_target, __uint_as_float(x.r.count));
+  p[1] = make_float4(x.u, x.v, x.proposal, __uint_as_float(x.flags));
+  p[2] = x.contribution;
+  p[3] = make_float4(x.normalization, __uint_as_float(x.supporting), __uint_as_float(x.previous),
+                     __uint_as_float(x.age));
+}
+
+// reservoir.slang:64-97 (MergeReevaluatedReservoir).
+__device__ __forceinline__ bool merge_reevaluated(Reservoir& d, const Reservoir& src, float normalization,
+                                                  float target, uint32_t cap, float random, bool& selected) {
+  selected = false;
+  const uint32_t count = src.count < cap ? src.count : cap;
+  const bool zero = src.weight_sum == 0.0f && src.sample == kNoReservoirSample && src.selected_target == 0.0f &&
+                    normalization == 0.0f && target == 0.0f;
+  const bool positive = src.weight_sum > 0.0f && src.selected_target > 0.0f && src.sample != kNoReservoirSample &&
+                        src.count != 0u && normalization > 0.0f;
+  const bool valid_destination = d.weight_sum == 0.0f
+                                     ? d.sample == kNoReservoirSample && d.selected_target == 0.0f
+                                     : d.weight_sum > 0.0f && d.selected_target > 0.0f &&
+                                           d.sample != kNoReservoirSample && d.count != 0u;
+  if (!valid_destination || !isfinite(d.weight_sum) || !isfinite(d.selected_target) || !isfinite(src.weight_sum) ||
+      !isfinite(src.selected_target) || !(zero || positive) || !isfinite(normalization) || !isfinite(target) ||
+      target < 0.0f || cap == 0u || !isfinite(random) || random < 0.0f || random >= 1.0f ||
+      count > 0xFFFFFFFFu - d.count)
+    return false;
+  const float weight = (target * normalization) * static_cast<float>(count);
+  const float sum = d.weight_sum + weight;
+  if (!isfinite(weight) || !isfinite(sum) || (target > 0.0f && weight == 0.0f)) return false;
+  selected = weight > 0.0f && (d.weight_sum == 0.0f || random < weight / sum);
+  d.weight_sum = sum;
+  d.count += count;
+  if (selected) {
+    d.sample = src.sample;
+    d.selected_target = target;
+  }
+  return true;
+}
+
+// reservoir.slang:99-103.
+__device__ __forceinline__ float reuse_normalization(const Reservoir& r, uint32_t supporting) {
+  return r.weight_sum > 0.0f ? (r.weight_sum / static_cast<float>(supporting)) / r.selected_target : 0.0f;
+}
+
+__device__ __forceinline__ bool same_sample(const ReuseRecord& a, const ReuseRecord& b) {
+  return a.r.sample == b.r.sample && a.u == b.u && a.v == b.v;
+}
+
+__device__ __forceinline__ V3 reuse_contribution_uv(const SceneView& S, const ReuseSurface& r, uint32_t light, float u,
+                                                    float v) {
+  return reuse_contribution(S, r, light, sample_triangle_light(S, light, u, v, r.origin));
+}
+
+// restir.slang:278-287 (HistoryTarget): the selection's target at a source's
+// surface, stored when the source selected the same sample.
+__device__ __forceinline__ float history_target(const SceneView& S, const ReuseRecord& source,
+                                                const ReuseSurface& surface, const ReuseRecord& selected,
+                                                bool& valid) {
+  if (same_sample(source, selected)) {
+    valid = isfinite(source.r.selected_target);
+    return source.r.selected_target;
+  }
+  const V3 c = reuse_contribution_uv(S, surface, selected.r.sample, selected.u, selected.v);
+  valid = finite3(c);
+  return max3(c);
+}
+
+// The final estimate at the receiver (restir.slang:289-300, 372-385):
+// visibility reused from the same pixel's record when it selected the same
+// sample, else traced.
+template <int R>
+__device__ __forceinline__ void resolve_estimate(const SceneView& S, LdsStack<R>& st, const ReuseSurface& receiver,
+                                                 ReuseRecord& result, const ReuseRecord& own, uint32_t& rays,
+                                                 uint32_t& visits, uint32_t& tests) {
+  if ((result.flags & 2u) != 0u || !(result.r.weight_sum > 0.0f)) return;
+  const LightSample ls = sample_triangle_light(S, result.r.sample, result.u, result.v, receiver.origin);
+  float visibility;
+  if (same_sample(result, own)) {
+    visibility = own.contribution.w;
+  } else {
+    ++rays;
+    visibility = walk_occluded(S.nodes, S.tris, st, Ray{receiver.origin, 0.0f, ls.direction, ls.distance}, visits,
+                               tests)
+                     ? 0.0f
+                     : 1.0f;
+  }
+  const V3 estimate = reuse_contribution(S, receiver, result.r.sample, ls) * (visibility * result.normalization);
+  if (!isfinite(result.normalization) || result.normalization <= 0.0f || !finite3(estimate))
+    result.flags |= 2u;
+  else
+    result.contribution = make_float4(estimate.x, estimate.y, estimate.z, visibility);
+}
+
+// restir.slang:302-387 (WriteTemporalReservoir) per pixel of the target.
+__global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, FrameConst F, RestirReuseArgs A,
+                                                                 uint32_t* counters, uint2* spill, int spill_cap,
+                                                                 unsigned long long* stats) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  LdsStack<kRing> st;
+  st.init(ring, spill, spill_cap);
+  const uint32_t pixels = F.width * F.height;
+  uint32_t visits = 0, tests = 0;
+  for (uint32_t slot = gtid(); slot < pixels; slot += gstride()) {
+    const ReuseRecord init = load_reuse(A.initial + 4ull * slot);
+    ReuseRecord result = init;
+    result.previous = 0xFFFFFFFFu;
+    result.age = 0u;
+    result.normalization = reservoir_normalization(init.r);
+    result.supporting = init.r.count;
+    uint32_t rays = 0;
+    if ((init.flags & 1u) != 0u) {
+      const ReuseSurface receiver = load_reuse_surface(A.surf + 8ull * slot);
+      // TemporalSampleHistory (temporal.slang:73-79): a valid reprojection
+      // onto the same placement as this receiver
+      const bool history = A.history != 0u && (result.flags & 2u) == 0u && A.validity[slot].valid != 0u &&
+                           A.tsurf[slot].instance_id == receiver.instance_id &&
+                           A.tsurf[slot].material_id == receiver.material_id;
+      if (history) {
+        const TemporalValidityPixel vd = A.validity[slot];
+        const uint32_t pi = vd.previous_index;
+        const ReuseRecord src = load_reuse(A.prev_hist + 4ull * pi);
+        const ReuseSurface psurf = load_reuse_surface(A.prev_surf + 8ull * pi);
+        if ((src.flags & 1u) != 0u && (src.flags & 2u) == 0u && psurf.instance_id == receiver.instance_id &&
+            psurf.material_id == receiver.material_id) {
+          const uint32_t count = src.r.count < A.max_history ? src.r.count : A.max_history;
+          V3 received = V3{0.0f, 0.0f, 0.0f};
+          if (src.r.weight_sum > 0.0f) {
+            if (src.r.sample >= S.light_count) result.flags |= 2u;
+            else received = reuse_contribution_uv(S, receiver, src.r.sample, src.u, src.v);
+          }
+          uint32_t rng = seed_random(slot, F.sample_index ^ 0x94D049BBu);
+          bool selected = false;
+          if (!finite3(received) || (result.flags & 2u) != 0u ||
+              !merge_reevaluated(result.r, src.r, src.normalization, max3(received), A.max_history,
+                                 next_random(rng), selected)) {
+            result.flags |= 2u;
+          } else {
+            result.previous = pi;
+            if (selected) {
+              result.u = src.u;
+              result.v = src.v;
+              result.proposal = src.proposal;
+              result.age = (src.age < 65535u ? src.age : 65535u) + 1u;
+            }
+            if (result.r.weight_sum > 0.0f) {
+              bool valid = true;
+              const float support = history_target(S, src, psurf, result, valid);
+              if (!valid) result.flags |= 2u;
+              result.supporting = init.r.count + (support > 0.0f ? count : 0u);
+            } else {
+              result.supporting = 0u;
+            }
+            result.normalization = reuse_normalization(result.r, result.supporting);
+          }
+        }
+      }
+      resolve_estimate(S, st, receiver, result, init, rays, visits, tests);
+    }
+    wave_count(restir_ray_count(counters), rays != 0u);
+    store_reuse(A.temporal_out + 4ull * slot, result);
+  }
+  flush_stats(stats, 2, visits, tests);
+}
+
+// restir.slang:392-400 (SpatialCompatible): geometry alone.
+__device__ __forceinline__ bool spatial_compatible(const ReuseSurface& receiver, uint32_t record_flags,
+                                                   const ReuseSurface& source) {
+  return (record_flags & 3u) == 1u && receiver.depth > 0.0f && source.depth > 0.0f &&
+         receiver.material_id == source.material_id &&
+         fabsf(receiver.depth - source.depth) <= 0.01f + 0.02f * maxf(receiver.depth, source.depth) &&
+         dot3(receiver.normal, source.normal) >= 0.9f && dot3(receiver.shading_normal, source.shading_normal) >= 0.9f;
+}
+
+// restir.slang:402-491 (WriteSpatialReservoir): the four axis neighbours at
+// the radius.
+__global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, FrameConst F, RestirReuseArgs A,
+                                                                uint32_t* counters, uint2* spill, int spill_cap,
+                                                                unsigned long long* stats) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  LdsStack<kRing> st;
+  st.init(ring, spill, spill_cap);
+  const uint32_t pixels = F.width * F.height;
+  const int radius = static_cast<int>(A.radius);
+  uint32_t visits = 0, tests = 0;
+  for (uint32_t slot = gtid(); slot < pixels; slot += gstride()) {
+    const ReuseRecord centre = load_reuse(A.temporal_out + 4ull * slot);
+    ReuseRecord result = centre;
+    uint32_t src_index[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    uint32_t rays = 0;
+    if ((centre.flags & 3u) == 1u) {
+      const ReuseSurface receiver = load_reuse_surface(A.surf + 8ull * slot);
+      const int px = static_cast<int>(slot % F.width), py = static_cast<int>(slot / F.width);
+      const int ox[4] = {-radius, radius, 0, 0}, oy[4] = {0, 0, -radius, radius};
+      uint32_t accepted = 0;
+      for (int i = 0; i < 4; ++i) {
+        const int nx = px + ox[i], ny = py + oy[i];
+        if (nx < 0 || ny < 0 || nx >= static_cast<int>(F.width) || ny >= static_cast<int>(F.height)) continue;
+        const uint32_t index = static_cast<uint32_t>(ny) * F.width + static_cast<uint32_t>(nx);
+        const uint32_t nflags = __float_as_uint(A.temporal_out[4ull * index + 1].w);
+        if (spatial_compatible(receiver, nflags, load_reuse_surface(A.surf + 8ull * index))) {
+          src_index[i] = index;
+          ++accepted;
+        }
+      }
+      if (accepted != 0u) {
+        result.r = empty_reservoir();
+        bool selected = false;
+        if (!merge_reevaluated(result.r, centre.r, centre.normalization, centre.r.selected_target, 0xFFFFFFFFu, 0.0f,
+                               selected))
+          result.flags |= 2u;
+        uint32_t rng = seed_random(slot, F.sample_index ^ 0x369DEA0Fu);
+        for (int i = 0; i < 4; ++i) {
+          if (src_index[i] == 0xFFFFFFFFu) continue;
+          const ReuseRecord src = load_reuse(A.temporal_out + 4ull * src_index[i]);
+          V3 received = V3{0.0f, 0.0f, 0.0f};
+          if (src.r.weight_sum > 0.0f) received = reuse_contribution_uv(S, receiver, src.r.sample, src.u, src.v);
+          if (!finite3(received) || !merge_reevaluated(result.r, src.r, src.normalization, max3(received),
+                                                       A.max_history, next_random(rng), selected)) {
+            result.flags |= 2u;
+            break;
+          }
+          if (selected) {
+            result.u = src.u;
+            result.v = src.v;
+            result.proposal = src.proposal;
+            result.age = src.age;
+          }
+        }
+        result.supporting = 0u;
+        result.contribution = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if ((result.flags & 2u) == 0u && result.r.weight_sum > 0.0f) {
+          result.supporting = centre.r.count;
+          for (int i = 0; i < 4; ++i) {
+            if (src_index[i] == 0xFFFFFFFFu) continue;
+            const ReuseRecord src = load_reuse(A.temporal_out + 4ull * src_index[i]);
+            bool valid = true;
+            const float support =
+                history_target(S, src, load_reuse_surface(A.surf + 8ull * src_index[i]), result, valid);
+            if (!valid) result.flags |= 2u;
+            if (support > 0.0f) result.supporting += src.r.count < A.max_history ? src.r.count : A.max_history;
+          }
+        }
+        result.normalization = reuse_normalization(result.r, result.supporting);
+        resolve_estimate(S, st, receiver, result, centre, rays, visits, tests);
+      }
+    }
+    wave_count(restir_ray_count(counters), rays != 0u);
+    store_reuse(A.spatial_out + 4ull * slot, result);
+    A.spatial_sources[slot] = make_uint4(src_index[0], src_index[1], src_index[2], src_index[3]);
   }
   flush_stats(stats, 2, visits, tests);
 }
@@ -673,6 +970,17 @@ void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F
 }
 void launch_restir_initial(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
   k_restir_initial<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+}
+void launch_restir_clear(const Launch& L, float4* restir, uint32_t pixels) {
+  k_restir_clear<<<L.grid.resolve, 256, 0, L.stream>>>(restir, pixels);
+}
+void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
+                            uint32_t* c) {
+  k_restir_temporal<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
+}
+void launch_restir_spatial(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
+                           uint32_t* c) {
+  k_restir_spatial<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q, uint32_t* c, uint32_t capacity) {
   k_cache_samples<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, c, capacity);

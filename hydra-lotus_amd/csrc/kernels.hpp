@@ -88,6 +88,29 @@ void launch_rays(const Launch& L, const SceneView& S, const float* rays, uint32_
 // ReSTIR DI's initial reservoirs at the first hits (queued by the camera
 // kernel), with their visibility rays traced inline.
 void launch_restir_initial(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c);
+struct TemporalPixel;
+struct TemporalValidityPixel;
+// ReSTIR reuse passes of a temporal frame. Records: 4 float4 per pixel;
+// surfaces: 8 float4 per pixel.
+struct RestirReuseArgs {
+  const float4* initial;
+  const float4* surf;          // this frame's receivers
+  const float4* prev_surf;     // the previous frame's receivers
+  const float4* prev_hist;     // the previous frame's final records
+  float4* temporal_out;
+  float4* spatial_out;
+  uint4* spatial_sources;      // the spatial pass's accepted neighbours
+  const TemporalPixel* tsurf;  // this frame's pixel-centre surfaces
+  const TemporalValidityPixel* validity;
+  uint32_t history;            // the previous frame's records are usable
+  uint32_t max_history;        // restir_max_history_candidates
+  uint32_t radius;             // spatial radius in pixels
+};
+void launch_restir_clear(const Launch& L, float4* restir, uint32_t pixels);
+void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
+                            uint32_t* c);
+void launch_restir_spatial(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
+                           uint32_t* c);
 // Radiance cache frames: targets (L - R) / T of every recorded vertex of the
 // training paths, appended to q.samples (counter word 7), at most `capacity`.
 void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q, uint32_t* c, uint32_t capacity);

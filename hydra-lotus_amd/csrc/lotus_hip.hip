@@ -16,7 +16,7 @@
 #include <vector>
 
 #include "../../include/lotus_hip.h"
-#include "kernels.hpp"
+#include "temporal.hpp"
 
 using namespace lotus;
 
@@ -122,7 +122,23 @@ struct AccumulationKey {  // vulkan_backend.cpp:419-434
 };
 
 enum Timer {
-  kTCamera, kTIntersect, kTShade, kTShadow, kTTrace, kTTail, kTResolve, kTCacheInfer, kTCacheTrain, kTRestir, kTimers
+  kTCamera, kTIntersect, kTShade, kTShadow, kTTrace, kTTail, kTResolve, kTCacheInfer, kTCacheTrain, kTRestir,
+  kTTemporal, kTimers
+};
+
+// One temporal frame's device work and buffers (lotus_hip_render_temporal).
+struct TemporalFrameState {
+  TemporalConst T{};
+  TemporalInstances I{};
+  float view[16];
+  bool reuse = false, spatial = false, accumulate = false;
+  RestirReuseArgs A{};
+  TemporalPixel* surface = nullptr;
+  const TemporalPixel* previous_surface = nullptr;
+  TemporalValidityPixel* validity = nullptr;
+  TemporalRadiancePixel* radiance = nullptr;
+  const TemporalRadiancePixel* old_radiance = nullptr;
+  float4* filter = nullptr;
 };
 
 }  // namespace
@@ -158,6 +174,29 @@ struct lotus_hip_renderer {
   uint32_t rounds_hint = 0;
 
   std::vector<hipEvent_t> events;
+
+  // Temporal frames: ping-pong history buffers (index cur is this frame's)
+  // and what the next frame compares against.
+  struct Temporal {
+    uint32_t width = 0, height = 0;
+    TemporalPixel* surf[2] = {nullptr, nullptr};
+    TemporalValidityPixel* validity = nullptr;
+    TemporalRadiancePixel* rad[2] = {nullptr, nullptr};
+    float4* filter = nullptr;
+    float4* rsurf[2] = {nullptr, nullptr};
+    float4* rhist[2] = {nullptr, nullptr};
+    float4* rspin = nullptr;
+    uint4* rsources = nullptr;
+    uint32_t cur = 0;
+    bool previous = false;      // the last successful temporal frame's state below
+    uint64_t frame_index = 0, history_key = 0, scene_generation = 0;
+    float view[16] = {}, world_to_clip[16] = {}, display[4] = {};
+    uint32_t data[4] = {};
+    uint32_t light_sampling = 0, max_bounces = 0, candidates = 0, proposal = 0;
+    bool reservoirs = false, radiance = false;
+  } temporal;
+  float4* svgf_filter = nullptr;   // lotus_hip_svgf's scratch
+  uint32_t svgf_pixels = 0;
 
   // Radiance cache and its frame buffers.
   CacheState cache;
@@ -308,6 +347,47 @@ uint32_t mix_host(uint32_t x) {
   x *= 0x846ca68bu;
   x ^= x >> 16;
   return x;
+}
+
+void free_temporal(lotus_hip_renderer* r) {
+  auto& t = r->temporal;
+  for (int k = 0; k < 2; ++k) {
+    dev_free(t.surf[k]);
+    dev_free(t.rad[k]);
+    dev_free(t.rsurf[k]);
+    dev_free(t.rhist[k]);
+  }
+  dev_free(t.validity);
+  dev_free(t.filter);
+  dev_free(t.rspin);
+  dev_free(t.rsources);
+  t.width = t.height = 0;
+  t.previous = false;
+}
+
+hipError_t ensure_temporal(lotus_hip_renderer* r, uint32_t width, uint32_t height) {
+  auto& t = r->temporal;
+  if (t.width == width && t.height == height && t.surf[0]) return hipSuccess;
+  free_temporal(r);
+  const size_t P = size_t(width) * height;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    e = dev_alloc(t.surf[k], P);
+    if (e == hipSuccess) e = dev_alloc(t.rad[k], P);
+    if (e == hipSuccess) e = dev_alloc(t.rsurf[k], 8 * P);
+    if (e == hipSuccess) e = dev_alloc(t.rhist[k], 4 * P);
+  }
+  if (e == hipSuccess) e = dev_alloc(t.validity, P);
+  if (e == hipSuccess) e = dev_alloc(t.filter, 3 * P);
+  if (e == hipSuccess) e = dev_alloc(t.rspin, 4 * P);
+  if (e == hipSuccess) e = dev_alloc(t.rsources, P);
+  if (e != hipSuccess) {
+    free_temporal(r);
+    return e;
+  }
+  t.width = width;
+  t.height = height;
+  return hipSuccess;
 }
 
 std::string latch(lotus_hip_renderer* r, const std::string& what, hipError_t e) {
@@ -497,6 +577,8 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   free_target(r);
   free_scene(r);
   cache_free(r->cache);
+  free_temporal(r);
+  dev_free(r->svgf_filter);
   dev_free(r->counters);
   dev_free(r->stats);
   dev_free(r->work);
@@ -632,8 +714,21 @@ int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, 
   return LOTUS_HIP_PASS;
 }
 
+namespace {
+int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res, char* detail,
+                size_t detail_size, TemporalFrameState* tf);
+}  // namespace
+
 int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res,
                      char* detail, size_t detail_size) {
+  return render_impl(r, P, res, detail, detail_size, nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res, char* detail,
+                size_t detail_size, TemporalFrameState* tf) {
   if (!r || !P) return LOTUS_HIP_FAIL;
   if (!r->failure.empty()) {
     set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
@@ -708,7 +803,7 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
   hipError_t e = ensure_target(r, P->width, P->height);
   if (e == hipSuccess && cache_frame) e = ensure_cache_target(r);
   const bool restir = P->light_sampling == LOTUS_HIP_LIGHT_RESTIR && P->output == LOTUS_HIP_OUTPUT_RADIANCE;
-  if (e == hipSuccess && restir && !r->q.restir) e = dev_alloc(r->q.restir, 3ull * P->width * P->height);
+  if (e == hipSuccess && restir && !r->q.restir) e = dev_alloc(r->q.restir, 4ull * P->width * P->height);
   if (e == hipSuccess && cache_frame) e = ensure_bounds(r);
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "target allocation", e));
@@ -726,6 +821,14 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
   F.restir_candidates = P->restir_candidates;
   F.restir_proposal = P->restir_proposal;
   F.restir = r->q.restir;
+  const float4* final_records = r->q.restir;
+  if (tf) {
+    std::memcpy(F.view, tf->view, 64);
+    F.restir_surf = tf->reuse ? const_cast<float4*>(tf->A.surf) : nullptr;
+    F.instance_ids = tf->I.instance_id;
+    F.material_ids = tf->I.material_id;
+    r->key_valid = false;  // every temporal call traces a fresh batch
+  }
   uint64_t cache_queries = 0, cache_samples = 0, cache_used = 0;
   double cache_loss_sum = 0.0;
   const uint32_t npix = P->width * P->height;
@@ -738,6 +841,15 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
 
   double t_ms[kTimers] = {};
   double frame_ms = 0.0;
+  if (tf) {
+    // the pixel-centre surfaces and their reprojection, before any pass reads them
+    hipEvent_t* ev = r->events.data() + r->events.size() - 4;
+    (void)hipEventRecord(ev[0], r->stream);
+    launch_fill4(reinterpret_cast<float4*>(tf->surface), 4u * npix, make_float4(0.0f, 0.0f, 0.0f, 0.0f), r->stream);
+    launch_temporal_surface(L, r->view, F, tf->T, tf->I, tf->surface);
+    launch_temporal_reproject(L, tf->T, tf->surface, tf->previous_surface, tf->validity);
+    (void)hipEventRecord(ev[1], r->stream);
+  }
   uint32_t added = 0;
   std::vector<uint32_t> counts;
   uint64_t camera_rays = 0, extension_rays = 0, shadow_rays = 0;
@@ -792,13 +904,22 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       }
       uint32_t next_work = 0;  // each persistent launch takes its own work counter
       if (add) {
+        if (tf && tf->reuse) launch_restir_clear(L, r->q.restir, npix);
         mark(kTCamera);
         L.work = r->work + next_work++;
         launch_camera(L, r->view, F, r->q, r->counters);
         done();
         if (restir) {
           mark(kTRestir);
+          F.restir = r->q.restir;
+          if (tf) tf->A.initial = r->q.restir;
           launch_restir_initial(L, r->view, F, r->q, r->counters);
+          if (tf && tf->reuse) {
+            launch_restir_temporal(L, r->view, F, tf->A, r->counters);
+            if (tf->spatial) launch_restir_spatial(L, r->view, F, tf->A, r->counters);
+            F.restir = tf->spatial ? tf->A.spatial_out : tf->A.temporal_out;
+            final_records = F.restir;
+          }
           done();
         }
         // round b: shade bounce b, then trace the extension rays of bounce
@@ -906,6 +1027,16 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       return LOTUS_HIP_FAIL;
     }
   }
+  if (tf && tf->accumulate) {
+    // SVGF over this call's batch (the resolved colour): accumulation,
+    // variance, a-trous iterations and the composite back into the colour
+    hipEvent_t* ev = r->events.data() + r->events.size() - 4;
+    (void)hipEventRecord(ev[2], r->stream);
+    launch_temporal_accumulate(L, tf->T, r->q.color, tf->surface, tf->validity, tf->old_radiance, tf->radiance);
+    launch_temporal_variance(L, tf->T, tf->surface, tf->radiance, tf->filter);
+    launch_temporal_filter(L, tf->T, tf->surface, tf->radiance, tf->filter, r->q.color);
+    (void)hipEventRecord(ev[3], r->stream);
+  }
   // The pixel-centre camera pass: depth, and the colour of the other outputs.
   const bool diag = P->output != LOTUS_HIP_OUTPUT_RADIANCE;
   if (diag || (res && res->depth)) {
@@ -917,8 +1048,8 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
       e = hipMemcpyAsync(res->color, r->q.color, npix * 16ull, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess && res->depth)
       e = hipMemcpyAsync(res->depth, r->depth, npix * 4ull, hipMemcpyDeviceToHost, r->stream);
-    if (e == hipSuccess && res->restir_records && restir && r->q.restir)
-      e = hipMemcpyAsync(res->restir_records, r->q.restir, npix * 48ull, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess && res->restir_records && restir && final_records)
+      e = hipMemcpy2DAsync(res->restir_records, 48, final_records, 64, 48, npix, hipMemcpyDeviceToHost, r->stream);
   }
   unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (e == hipSuccess) e = hipGetLastError();
@@ -956,10 +1087,20 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
     res->cache_infer_gpu_ms = t_ms[kTCacheInfer];
     res->cache_train_gpu_ms = t_ms[kTCacheTrain];
     res->restir_gpu_ms = t_ms[kTRestir];
+    if (tf) {
+      hipEvent_t* ev = r->events.data() + r->events.size() - 4;
+      float a = 0.0f, b = 0.0f;
+      (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+      if (tf->accumulate) (void)hipEventElapsedTime(&b, ev[2], ev[3]);
+      res->frame_gpu_ms += a + b;
+    }
   }
   set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;
 }
+}  // namespace
+
+extern "C" {
 
 int lotus_hip_trace_device(lotus_hip_renderer* r, uint32_t count, const void* rays, void* hits, int any_hit,
                            void* stream, char* detail, size_t detail_size) {
@@ -1211,6 +1352,244 @@ int lotus_hip_cache_parameters(lotus_hip_renderer* r, float* weights, float* gri
     r->key_valid = false;
   }
   return cache_finish(r, e, "cache parameters", detail, detail_size);
+}
+
+}  // extern "C"
+
+namespace {
+
+void resolved_windows(const lotus_hip_frame_params* P, float* display, uint32_t* data) {
+  const bool whole = P->display_window[2] == 0.0f || P->display_window[3] == 0.0f;
+  display[0] = whole ? 0.0f : P->display_window[0];
+  display[1] = whole ? 0.0f : P->display_window[1];
+  display[2] = whole ? static_cast<float>(P->width) : P->display_window[2];
+  display[3] = whole ? static_cast<float>(P->height) : P->display_window[3];
+  data[0] = 0;
+  data[1] = 0;
+  data[2] = P->width;
+  data[3] = P->height;
+  if (P->data_window[2] != 0 && P->data_window[3] != 0) {
+    data[0] = std::min(P->data_window[0], P->width);
+    data[1] = std::min(P->data_window[1], P->height);
+    data[2] = std::min(P->data_window[2], P->width - data[0]);
+    data[3] = std::min(P->data_window[3], P->height - data[1]);
+  }
+}
+
+void default_validation(const float* in, float* out) {
+  const bool given = in[0] != 0.0f || in[1] != 0.0f || in[2] != 0.0f || in[3] != 0.0f;
+  const float defaults[4] = {0.01f, 0.01f, 0.9f, 32.0f};  // temporal.hpp:28-33
+  std::memcpy(out, given ? in : defaults, 16);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_params* P,
+                              lotus_hip_temporal_settings* ts, lotus_hip_frame_result* res, char* detail,
+                              size_t detail_size) {
+  if (!r || !P || !ts) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  // TemporalFrameSettings rules (temporal.hpp:58-103, vulkan_backend.cpp:1066-1076)
+  if (P->output != LOTUS_HIP_OUTPUT_RADIANCE) {
+    set_detail(detail, detail_size, "temporal frames render radiance");
+    return LOTUS_HIP_FAIL;
+  }
+  if (ts->accumulate_radiance && (ts->max_history_samples == 0 || ts->max_history_samples > 65536)) {
+    set_detail(detail, detail_size, "max_history_samples must be in [1, 65536]");
+    return LOTUS_HIP_FAIL;
+  }
+  if (ts->filter_iterations > 8 || (ts->filter_iterations != 0 && !ts->accumulate_radiance)) {
+    set_detail(detail, detail_size, "filter_iterations needs accumulate_radiance and is at most 8");
+    return LOTUS_HIP_FAIL;
+  }
+  const bool restir = P->light_sampling == LOTUS_HIP_LIGHT_RESTIR;
+  if (ts->restir_temporal && (!restir || P->frame_count != 1 || ts->restir_max_history_candidates == 0 ||
+                              ts->restir_max_history_candidates > 65536)) {
+    set_detail(detail, detail_size,
+               "temporal reservoirs require RenderTemporalScene, RestirInitial, one sample and history candidates in [1, 65536]");
+    return LOTUS_HIP_FAIL;
+  }
+  if (ts->restir_spatial && (!restir || P->frame_count != 1 || ts->restir_max_history_candidates == 0 ||
+                             ts->restir_max_history_candidates > 65536 || ts->restir_spatial_radius == 0 ||
+                             ts->restir_spatial_radius > 32)) {
+    set_detail(detail, detail_size,
+               "spatial reservoirs require RenderTemporalScene, RestirInitial, one sample, history candidates in [1, 65536] and radius in [1, 32]");
+    return LOTUS_HIP_FAIL;
+  }
+  auto& t = r->temporal;
+  if (t.previous && !ts->camera_cut && ts->frame_index <= t.frame_index) {
+    set_detail(detail, detail_size, "temporal frame indices must increase");
+    return LOTUS_HIP_FAIL;
+  }
+  // TemporalHistory::Advance (temporal.cpp:114-147): why history is dropped
+  uint32_t reason = 0;
+  if (!t.previous) reason = 1;                                    // FirstFrame
+  else if (ts->camera_cut) reason = 3;                            // CameraCut
+  else if (ts->frame_index != t.frame_index + 1) reason = 4;      // FrameGap
+  else if (P->width != t.width || P->height != t.height) reason = 5;  // TargetChanged
+  else if (ts->history_key != t.history_key) reason = 6;          // HistoryKeyChanged
+  else if (P->light_sampling != t.light_sampling || P->max_bounces != t.max_bounces ||
+           (restir && (P->restir_candidates != t.candidates || P->restir_proposal != t.proposal)))
+    reason = 7;                                                   // RenderSettingsChanged
+  else if (r->scene_generation != t.scene_generation) reason = 8;  // SceneChanged
+  (void)hipSetDevice(r->device);
+  hipError_t e = ensure_temporal(r, P->width, P->height);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "temporal allocation", e));
+    return LOTUS_HIP_FAIL;
+  }
+  const bool available = reason == 0 && t.previous;
+  const uint32_t cur = t.cur, prev = 1u - t.cur;
+  TemporalFrameState tf;
+  std::memcpy(tf.view, ts->view, 64);
+  TemporalConst& T = tf.T;
+  std::memcpy(T.current_view, ts->view, 64);
+  std::memcpy(T.previous_view, t.view, 64);
+  std::memcpy(T.previous_world_to_clip, t.world_to_clip, 64);
+  std::memcpy(T.previous_display, t.display, 16);
+  std::memcpy(T.previous_data, t.data, 16);
+  default_validation(ts->validation, T.validation);
+  T.available = available ? 1u : 0u;
+  T.width = P->width;
+  T.height = P->height;
+  T.fresh_samples = std::max(P->frame_count, 1u);
+  T.cap = ts->accumulate_radiance ? ts->max_history_samples : 1u;
+  T.iterations = ts->accumulate_radiance ? ts->filter_iterations : 0u;
+  tf.I = TemporalInstances{ts->previous_from_current, ts->instance_ids, ts->material_ids, ts->instance_flags};
+  tf.surface = t.surf[cur];
+  tf.previous_surface = t.surf[prev];
+  tf.validity = t.validity;
+  tf.radiance = t.rad[cur];
+  tf.old_radiance = t.rad[prev];
+  tf.filter = t.filter;
+  tf.accumulate = ts->accumulate_radiance != 0;
+  tf.reuse = restir && (ts->restir_temporal || ts->restir_spatial);
+  tf.spatial = restir && ts->restir_spatial;
+  RestirReuseArgs& A = tf.A;
+  A.surf = t.rsurf[cur];
+  A.prev_surf = t.rsurf[prev];
+  A.prev_hist = t.rhist[prev];
+  A.temporal_out = tf.spatial ? t.rspin : t.rhist[cur];
+  A.spatial_out = t.rhist[cur];
+  A.spatial_sources = t.rsources;
+  A.tsurf = t.surf[cur];
+  A.validity = t.validity;
+  A.history = ts->restir_temporal && available && t.reservoirs ? 1u : 0u;
+  A.max_history = ts->restir_max_history_candidates;
+  A.radius = ts->restir_spatial_radius;
+  if (tf.accumulate && !(available && t.radiance))
+    launch_fill4(reinterpret_cast<float4*>(t.rad[prev]), 2u * P->width * P->height,
+                 make_float4(0.0f, 0.0f, 0.0f, 0.0f), r->stream);
+  const int status = render_impl(r, P, res, detail, detail_size, &tf);
+  if (status != LOTUS_HIP_PASS) return status;
+  const size_t npix = size_t(P->width) * P->height;
+  if (ts->surface) e = hipMemcpy(ts->surface, t.surf[cur], npix * sizeof(TemporalPixel), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ts->validity)
+    e = hipMemcpy(ts->validity, t.validity, npix * sizeof(TemporalValidityPixel), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ts->radiance && tf.accumulate)
+    e = hipMemcpy(ts->radiance, t.rad[cur], npix * sizeof(TemporalRadiancePixel), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "temporal readback", e));
+    return LOTUS_HIP_FAIL;
+  }
+  t.previous = true;
+  t.frame_index = ts->frame_index;
+  t.history_key = ts->history_key;
+  t.scene_generation = r->scene_generation;
+  std::memcpy(t.view, ts->view, 64);
+  vulkan_world_to_clip(P->world_to_clip, t.world_to_clip);
+  resolved_windows(P, t.display, t.data);
+  t.light_sampling = P->light_sampling;
+  t.max_bounces = P->max_bounces;
+  t.candidates = P->restir_candidates;
+  t.proposal = P->restir_proposal;
+  t.reservoirs = tf.reuse;
+  t.radiance = tf.accumulate;
+  t.cur = prev;
+  ts->reset_reason = reason;
+  ts->history_available = available ? 1u : 0u;
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_svgf(lotus_hip_renderer* r, const void* surface, void* radiance, uint32_t width, uint32_t height,
+                   uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations, void* color,
+                   char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (!surface || !radiance || width == 0 || height == 0 || batch_samples == 0 || max_history_samples == 0 ||
+      iterations > 8 || (iterations != 0 && !color)) {
+    set_detail(detail, detail_size, "invalid filter input");
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  const uint32_t P = width * height;
+  hipError_t e = hipSuccess;
+  if (r->svgf_pixels < P) {
+    dev_free(r->svgf_filter);
+    r->svgf_pixels = 0;
+    e = dev_alloc(r->svgf_filter, 3ull * P);
+    if (e == hipSuccess) r->svgf_pixels = P;
+  }
+  if (e == hipSuccess) {
+    TemporalConst T{};
+    T.width = width;
+    T.height = height;
+    T.fresh_samples = batch_samples;
+    T.cap = max_history_samples;
+    T.iterations = iterations;
+    const Launch L = launch_for(r, r->stream);
+    auto* surf = static_cast<const TemporalPixel*>(surface);
+    auto* rad = static_cast<TemporalRadiancePixel*>(radiance);
+    launch_temporal_variance(L, T, surf, rad, r->svgf_filter);
+    if (iterations != 0) launch_temporal_filter(L, T, surf, rad, r->svgf_filter, static_cast<float4*>(color));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "filter", e));
+    return LOTUS_HIP_FAIL;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_reproject(lotus_hip_renderer* r, const void* current, const void* previous, uint32_t width,
+                        uint32_t height, const float* validation, void* validity, char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (!current || !validity || width == 0 || height == 0) {
+    set_detail(detail, detail_size, "invalid reprojection input");
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  TemporalConst T{};
+  const float zero[4] = {0, 0, 0, 0};
+  default_validation(validation ? validation : zero, T.validation);
+  T.available = previous ? 1u : 0u;
+  T.width = width;
+  T.height = height;
+  const Launch L = launch_for(r, r->stream);
+  launch_temporal_reproject(L, T, static_cast<const TemporalPixel*>(current), static_cast<const TemporalPixel*>(previous),
+                            static_cast<TemporalValidityPixel*>(validity));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "reprojection", e));
+    return LOTUS_HIP_FAIL;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
 }
 
 }  // extern "C"

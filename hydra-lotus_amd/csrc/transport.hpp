@@ -37,7 +37,21 @@ struct FrameConst {
   // direct light reads (the last reuse pass's).
   uint32_t restir_candidates, restir_proposal;
   const float4* restir;
+  // Temporal frames with reservoir reuse: receiver surfaces the initial pass
+  // writes (8 float4 per pixel) and the camera's world -> view matrix.
+  float4* restir_surf;
+  float view[16];
+  const uint64_t* instance_ids;  // stable ids (null: instance + 1, material slot + 1)
+  const uint64_t* material_ids;
 };
+
+__device__ __forceinline__ uint64_t stable_instance_id(const FrameConst& F, uint32_t instance) {
+  return F.instance_ids ? F.instance_ids[instance] : static_cast<uint64_t>(instance) + 1u;
+}
+__device__ __forceinline__ uint64_t stable_material_id(const FrameConst& F, const uint32_t* instance_material,
+                                                       uint32_t instance) {
+  return F.material_ids ? F.material_ids[instance] : static_cast<uint64_t>(instance_material[instance]) + 1u;
+}
 
 // The path state a kernel holds in registers.
 struct PathRegs {
@@ -197,6 +211,8 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux
 // A reservoir record (restir.slang:11-17, 48 bytes) as 3 float4: {sample,
 // weight_sum, selected_target, M}, {u, v, proposal, flags}, {contribution
 // rgb, visibility}. flags: bit 0 the camera ray hit, bit 1 a failed stream.
+// Records are stored 4 float4 apart: the reuse record (restir.slang:21-28)
+// adds {normalization, supporting candidates, previous index, age}.
 constexpr uint32_t kNoReservoirSample = 0xFFFFFFFFu;
 constexpr uint32_t kLightRestir = 2u;  // light_sampling: ReSTIR's initial reservoir at the first hit
 
@@ -312,6 +328,78 @@ __device__ __forceinline__ void add_emission(const SceneView& S, const FrameCons
   p.radiance = p.radiance + p.throughput * emission;
 }
 
+// A receiver surface for reuse (restir.slang:30-62, RestirSurface): what a
+// reservoir's target needs at it, 8 float4 = {origin, view depth},
+// {geometric normal, instance id low}, {shading normal, instance id high},
+// {wo in the shading frame, dielectric f90}, {base colour, alpha},
+// {dielectric f0, metallic}, {f0, f90}, {transmission (negated: thin),
+// eta (negated: smooth), material id low, high}.
+__device__ __forceinline__ void store_reuse_surface(float4* out, const Surface& sf, V3 origin, float depth, V3 wo,
+                                                    uint64_t instance_id, uint64_t material_id) {
+  const Closure& b = sf.bsdf;
+  out[0] = make_float4(origin.x, origin.y, origin.z, depth);
+  out[1] = make_float4(sf.normal.x, sf.normal.y, sf.normal.z, __uint_as_float(static_cast<uint32_t>(instance_id)));
+  out[2] = make_float4(sf.shading_normal.x, sf.shading_normal.y, sf.shading_normal.z,
+                       __uint_as_float(static_cast<uint32_t>(instance_id >> 32)));
+  out[3] = make_float4(wo.x, wo.y, wo.z, b.dielectric_f90);
+  out[4] = make_float4(b.base_color.x, b.base_color.y, b.base_color.z, b.alpha);
+  out[5] = make_float4(b.dielectric_f0.x, b.dielectric_f0.y, b.dielectric_f0.z, b.metallic);
+  out[6] = make_float4(b.f0.x, b.f0.y, b.f0.z, b.f90);
+  out[7] = make_float4(b.thin_walled ? -b.transmission : b.transmission, b.smooth ? -b.eta : b.eta,
+                       __uint_as_float(static_cast<uint32_t>(material_id)),
+                       __uint_as_float(static_cast<uint32_t>(material_id >> 32)));
+}
+
+struct ReuseSurface {
+  V3 origin, normal, shading_normal, wo;
+  float depth;
+  uint64_t instance_id, material_id;
+  Closure bsdf;
+};
+
+// restir.slang:117-141 (LoadClosure): targets are reflection-side, so the
+// transmission colour is white.
+__device__ __forceinline__ ReuseSurface load_reuse_surface(const float4* in) {
+  const float4 s0 = in[0], s1 = in[1], s2 = in[2], s3 = in[3], s4 = in[4], s5 = in[5], s6 = in[6], s7 = in[7];
+  ReuseSurface r;
+  r.origin = xyz(s0);
+  r.depth = s0.w;
+  r.normal = xyz(s1);
+  r.shading_normal = xyz(s2);
+  r.wo = xyz(s3);
+  r.instance_id = static_cast<uint64_t>(__float_as_uint(s1.w)) | (static_cast<uint64_t>(__float_as_uint(s2.w)) << 32);
+  r.material_id = static_cast<uint64_t>(__float_as_uint(s7.z)) | (static_cast<uint64_t>(__float_as_uint(s7.w)) << 32);
+  Closure& b = r.bsdf;
+  b.base_color = xyz(s4);
+  b.alpha = s4.w;
+  b.dielectric_f0 = xyz(s5);
+  b.metallic = s5.w;
+  b.f0 = xyz(s6);
+  b.f90 = s6.w;
+  b.dielectric_f90 = s3.w;
+  b.transmission = fabsf(s7.x);
+  b.thin_walled = s7.x < 0.0f;
+  b.eta = fabsf(s7.y);
+  b.smooth = s7.y < 0.0f;
+  b.transmission_color = V3{1.0f, 1.0f, 1.0f};
+  return r;
+}
+
+// restir.slang:256-268 (HistoryContribution) for a reconstructed sample.
+__device__ __forceinline__ V3 reuse_contribution(const SceneView& S, const ReuseSurface& r, uint32_t light,
+                                                 const LightSample& ls) {
+  if (!(ls.pdf > 0.0f) || dot3(ls.direction, r.normal) <= 0.0f) return V3{0.0f, 0.0f, 0.0f};
+  const V3 n = r.shading_normal;
+  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  const V3 ft = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+  const V3 fb = V3{b, sign + n.y * n.y * a, -n.y};
+  const V3 wi = V3{dot3(ls.direction, ft), dot3(ls.direction, fb), dot3(ls.direction, n)};
+  if (!(wi.z > 0.0f)) return V3{0.0f, 0.0f, 0.0f};
+  return bsdf_evaluate(r.bsdf, r.wo, wi) * ls.radiance * (S.lights[light].pdf / ls.pdf);
+}
+
 // restir.slang:235-254 (InitialDirectLight): the first hit's direct light
 // from its reservoir record, weighted against BSDF sampling by the power
 // heuristic of the light table's density.
@@ -359,7 +447,7 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
   const V3 md = -p.d;
   const V3 wo = V3{dot3(md, ft), dot3(md, fb), dot3(md, n)};
   if (F.light_sampling == kLightRestir && p.depth == 0) {
-    p.radiance = p.radiance + p.throughput * initial_direct_light(S, F.restir + 3ull * slot, sf, ft, fb, wo);
+    p.radiance = p.radiance + p.throughput * initial_direct_light(S, F.restir + 4ull * slot, sf, ft, fb, wo);
   } else if (sample_lights && S.light_count != 0) {
     const float ux = next_random(p.rng), uy = next_random(p.rng), uz = next_random(p.rng),
                 uw = next_random(p.rng);

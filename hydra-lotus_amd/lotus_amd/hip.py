@@ -30,7 +30,17 @@ SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hi
            "lotus_hip_render", "lotus_hip_trace", "lotus_hip_trace_device", "lotus_hip_light_table",
            "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning",
            "lotus_hip_cache_configure", "lotus_hip_cache_infer", "lotus_hip_cache_gradients",
-           "lotus_hip_cache_train", "lotus_hip_cache_parameters"]
+           "lotus_hip_cache_train", "lotus_hip_cache_parameters", "lotus_hip_render_temporal", "lotus_hip_svgf",
+           "lotus_hip_reproject"]
+
+# The reference's temporal records (include/lotus/temporal.hpp:193-245).
+TEMPORAL_PIXEL = np.dtype([("normal_depth", "<f4", 4), ("motion_depth", "<f4", 4), ("instance_id", "<u8"),
+                           ("material_id", "<u8"), ("flags", "<u4"), ("albedo", "<u4"), ("shading_normal", "<u4"),
+                           ("reserved", "<u4")])
+TEMPORAL_VALIDITY = np.dtype([("valid", "<u4"), ("rejection", "<u4"), ("previous_index", "<u4"),
+                              ("reserved", "<u4")])
+TEMPORAL_RADIANCE = np.dtype([("color", "<f4", 4), ("sample_count", "<u4"), ("first_moment", "<f4"),
+                              ("second_moment", "<f4"), ("variance", "<f4")])
 
 _u32p = C.POINTER(C.c_uint32)
 _f32p = C.POINTER(C.c_float)
@@ -129,6 +139,32 @@ class FrameResult(C.Structure):
     ]
 
 
+class TemporalSettings(C.Structure):
+    _fields_ = [
+        ("frame_index", C.c_uint64),
+        ("history_key", C.c_uint64),
+        ("camera_cut", C.c_uint32),
+        ("accumulate_radiance", C.c_uint32),
+        ("max_history_samples", C.c_uint32),
+        ("filter_iterations", C.c_uint32),
+        ("view", C.c_float * 16),
+        ("validation", C.c_float * 4),
+        ("restir_temporal", C.c_uint32),
+        ("restir_spatial", C.c_uint32),
+        ("restir_spatial_radius", C.c_uint32),
+        ("restir_max_history_candidates", C.c_uint32),
+        ("instance_ids", C.POINTER(C.c_uint64)),
+        ("material_ids", C.POINTER(C.c_uint64)),
+        ("instance_flags", _u32p),
+        ("previous_from_current", _f32p),
+        ("surface", C.c_void_p),
+        ("validity", C.c_void_p),
+        ("radiance", C.c_void_p),
+        ("reset_reason", C.c_uint32),
+        ("history_available", C.c_uint32),
+    ]
+
+
 class CacheConfig(C.Structure):
     _fields_ = [
         ("log2_table", C.c_uint32),
@@ -187,6 +223,15 @@ def lib():
         L.lotus_hip_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.lotus_hip_read_accumulation.restype = C.c_int
         L.lotus_hip_read_accumulation.argtypes = [C.c_void_p, _f32p, _u32p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_render_temporal.restype = C.c_int
+        L.lotus_hip_render_temporal.argtypes = [C.c_void_p, C.POINTER(FrameParams), C.POINTER(TemporalSettings),
+                                                C.POINTER(FrameResult), C.c_char_p, C.c_size_t]
+        L.lotus_hip_svgf.restype = C.c_int
+        L.lotus_hip_svgf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_uint32, C.c_uint32, C.c_void_p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_reproject.restype = C.c_int
+        L.lotus_hip_reproject.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, _f32p,
+                                          C.c_void_p, C.c_char_p, C.c_size_t]
         L.lotus_hip_cache_configure.restype = C.c_int
         L.lotus_hip_cache_configure.argtypes = [C.c_void_p, C.POINTER(CacheConfig), C.c_char_p, C.c_size_t]
         L.lotus_hip_cache_infer.restype = C.c_int
@@ -272,7 +317,9 @@ class Renderer:
                light_sampling=LIGHT_MIS, output=OUTPUT_RADIANCE, clear_color=(0.0, 0.0, 0.0, 0.0),
                data_window=(0, 0, 0, 0), display_window=(0.0, 0.0, 0.0, 0.0), max_samples=0,
                want_color=True, want_depth=False, count_capacity=None, split_kinds=False, cache=False,
-               restir_candidates=8, restir_proposal=0, want_restir=False):
+               restir_candidates=8, restir_proposal=0, want_restir=False, temporal=None):
+        """One RenderScene call; with `temporal` (a TemporalSettings), one
+        RenderTemporalScene frame."""
         fp = FrameParams()
         w2c = np.ascontiguousarray(world_to_clip, np.float32).reshape(16)
         for i in range(16):
@@ -306,7 +353,10 @@ class Renderer:
             res.restir_records = _p(restir, C.c_uint32)
         res.path_counts_capacity = cap
         buf = C.create_string_buffer(512)
-        s = lib().lotus_hip_render(self.handle, C.byref(fp), C.byref(res), buf, 512)
+        if temporal is None:
+            s = lib().lotus_hip_render(self.handle, C.byref(fp), C.byref(res), buf, 512)
+        else:
+            s = lib().lotus_hip_render_temporal(self.handle, C.byref(fp), C.byref(temporal), C.byref(res), buf, 512)
         if s != PASS:
             raise LotusError(s, buf.value.decode())
         return dict(color=color, depth=depth, counts=counts[:res.path_counts_size].copy(),
@@ -440,5 +490,53 @@ class Renderer:
         g = None if grid is None else np.ascontiguousarray(grid, np.float32).reshape(self._grid_shape())
         s = lib().lotus_hip_cache_parameters(self.handle, _p(w, C.c_float) if w is not None else None,
                                              _p(g, C.c_float) if g is not None else None, 1, buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    # temporal frames ------------------------------------------------------
+    def render_temporal(self, world_to_clip, view, width, height, frame_index, sample_index=None,
+                        accumulate=False, max_history=32, filter_iterations=0, restir_temporal=False,
+                        restir_spatial=False, radius=30, max_history_candidates=20, history_key=0,
+                        camera_cut=False, want_products=False, **kw):
+        """One temporal frame; returns render()'s dict plus the reset reason,
+        history availability and, with want_products, the surface, validity
+        and radiance records."""
+        t = TemporalSettings()
+        t.frame_index, t.history_key, t.camera_cut = frame_index, history_key, int(camera_cut)
+        t.accumulate_radiance, t.max_history_samples, t.filter_iterations = int(accumulate), max_history, \
+            filter_iterations
+        v = np.ascontiguousarray(view, np.float32).reshape(16)
+        for i in range(16):
+            t.view[i] = float(v[i])
+        t.restir_temporal, t.restir_spatial = int(restir_temporal), int(restir_spatial)
+        t.restir_spatial_radius, t.restir_max_history_candidates = radius, max_history_candidates
+        prod = {}
+        if want_products:
+            prod = dict(surface=np.zeros((height, width), TEMPORAL_PIXEL),
+                        validity=np.zeros((height, width), TEMPORAL_VALIDITY),
+                        radiance=np.zeros((height, width), TEMPORAL_RADIANCE))
+            t.surface = prod["surface"].ctypes.data
+            t.validity = prod["validity"].ctypes.data
+            t.radiance = prod["radiance"].ctypes.data
+        out = self.render(world_to_clip, width, height,
+                          sample_index=frame_index if sample_index is None else sample_index, temporal=t, **kw)
+        out.update(prod)
+        out["reset_reason"] = t.reset_reason
+        out["history_available"] = t.history_available
+        return out
+
+    def svgf(self, surface_ptr, radiance_ptr, width, height, batch_samples, max_history, iterations, color_ptr=0):
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_svgf(self.handle, C.c_void_p(surface_ptr), C.c_void_p(radiance_ptr), width, height,
+                                 batch_samples, max_history, iterations, C.c_void_p(color_ptr), buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    def reproject(self, current_ptr, previous_ptr, width, height, validity_ptr, validation=None):
+        buf = C.create_string_buffer(256)
+        v = None if validation is None else np.ascontiguousarray(validation, np.float32)
+        s = lib().lotus_hip_reproject(self.handle, C.c_void_p(current_ptr), C.c_void_p(previous_ptr or None),
+                                      width, height, _p(v, C.c_float) if v is not None else None,
+                                      C.c_void_p(validity_ptr), buf, 256)
         if s != PASS:
             raise LotusError(s, buf.value.decode())

@@ -283,6 +283,63 @@ int lotus_hip_cache_train(lotus_hip_renderer* renderer, const void* samples_devi
 int lotus_hip_cache_parameters(lotus_hip_renderer* renderer, float* weights, float* grid, int write, char* detail,
                                size_t detail_size);
 
+/* Temporal frames: OffscreenRenderer::RenderTemporalScene
+ * (vulkan_backend.hpp:807-810) with TemporalFrameSettings
+ * (include/lotus/temporal.hpp:58-103). One call is one temporal frame: a
+ * fresh batch of frame_count samples, the pixel-centre surface product and
+ * its reprojection onto the previous frame (temporal.slang:497-627), ReSTIR
+ * DI reuse (restir.slang:302-491) and SVGF (temporal.slang:229-493). */
+typedef struct lotus_hip_temporal_settings {
+  uint64_t frame_index;          /* must increase; a gap drops the history */
+  uint64_t history_key;          /* changing it drops the history */
+  uint32_t camera_cut;
+  uint32_t accumulate_radiance;  /* SVGF temporal accumulation */
+  uint32_t max_history_samples;  /* [1, 65536] */
+  uint32_t filter_iterations;    /* a-trous iterations, at most 8 */
+  float view[16];                /* world -> view of this frame's camera, column-major */
+  float validation[4];           /* depth abs / rel tolerance, min normal cosine, max motion px; zeros = defaults */
+  uint32_t restir_temporal;      /* PathTracingSettings::restir_temporal (vulkan_backend.hpp:719-724) */
+  uint32_t restir_spatial;
+  uint32_t restir_spatial_radius;          /* [1, 32] */
+  uint32_t restir_max_history_candidates;  /* [1, 65536] */
+  /* Optional per-instance identity (temporal.slang:32-40); NULL: instance + 1,
+   * material slot + 1, rigid and unchanged with a static placement. */
+  const uint64_t* instance_ids;
+  const uint64_t* material_ids;
+  const uint32_t* instance_flags;          /* bit 1 rigid correspondence, bit 2 material unchanged */
+  const float* previous_from_current;      /* 16 floats per instance, column-major */
+  /* Optional products, host buffers of width*height records in the
+   * reference's layouts (temporal.hpp:193-245): TemporalPixel (64 B),
+   * TemporalValidityPixel (16 B), TemporalRadiancePixel (32 B). */
+  void* surface;
+  void* validity;
+  void* radiance;
+  uint32_t reset_reason;       /* written: TemporalResetReason (temporal.hpp:16-26) */
+  uint32_t history_available;  /* written */
+} lotus_hip_temporal_settings;
+
+int lotus_hip_render_temporal(lotus_hip_renderer* renderer, const lotus_hip_frame_params* params,
+                              lotus_hip_temporal_settings* temporal, lotus_hip_frame_result* result, char* detail,
+                              size_t detail_size);
+
+/* SVGF's variance estimate and filter over caller device buffers of
+ * width*height TemporalPixel and TemporalRadiancePixel records: writes each
+ * radiance record's variance word (EstimateTemporalVariance,
+ * temporal.cpp:375-421) and, for iterations > 0, the filtered, remodulated
+ * colour (FilterTemporalRadiance, temporal.cpp:424-512) as RGBA32F. */
+int lotus_hip_svgf(lotus_hip_renderer* renderer, const void* surface_device, void* radiance_device, uint32_t width,
+                   uint32_t height, uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations,
+                   void* color_device, char* detail, size_t detail_size);
+
+/* History reprojection over caller device buffers (ReprojectTemporalHistory,
+ * temporal.cpp:258-300): TemporalValidityPixel records from current and
+ * previous TemporalPixel records (previous NULL: no history); validation
+ * holds the four tolerances of lotus_hip_temporal_settings, NULL for the
+ * defaults. */
+int lotus_hip_reproject(lotus_hip_renderer* renderer, const void* current_device, const void* previous_device,
+                        uint32_t width, uint32_t height, const float* validation, void* validity_device,
+                        char* detail, size_t detail_size);
+
 #ifdef __cplusplus
 }
 #endif

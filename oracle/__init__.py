@@ -139,6 +139,15 @@ def ref():
         R.ref_merge_reevaluated.argtypes = [_u32p, _u32p, C.c_double, C.c_float, C.c_uint32, C.c_float]
         R.ref_reservoir_normalization.restype = C.c_double
         R.ref_reservoir_normalization.argtypes = [_u32p]
+        R.ref_estimate_variance.restype = C.c_float
+        R.ref_estimate_variance.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, C.c_uint32]
+        R.ref_filter_radiance.restype = C.c_int
+        R.ref_filter_radiance.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, _f32p]
+        R.ref_reproject.restype = None
+        R.ref_reproject.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, _f32p,
+                                    _u32p]
         R.ref_reuse_normalization.restype = C.c_double
         R.ref_reuse_normalization.argtypes = [_u32p, C.c_uint32]
         _ref = R

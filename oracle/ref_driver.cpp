@@ -10,6 +10,7 @@
 #include <vector>
 
 #include <lotus/render_world.hpp>
+#include <lotus/temporal.hpp>
 
 #include "reference.hpp"
 #include "reservoir.hpp"
@@ -157,6 +158,49 @@ double ref_reuse_normalization(const uint32_t* reservoir, uint32_t supporting) {
   } catch (const std::exception&) {
     return -1.0;
   }
+}
+
+// backend/vulkan/temporal.cpp:375-421 (EstimateTemporalVariance) over
+// width*height TemporalPixel (64 B) and TemporalRadiancePixel (32 B) records.
+float ref_estimate_variance(const void* surface, const void* radiance, uint32_t x, uint32_t y, uint32_t width,
+                            uint32_t height, uint32_t batch_samples, uint32_t max_history_samples) {
+  const size_t n = size_t(width) * height;
+  std::span<const Lotus::TemporalPixel> s(static_cast<const Lotus::TemporalPixel*>(surface), n);
+  std::span<const Lotus::TemporalRadiancePixel> r(static_cast<const Lotus::TemporalRadiancePixel*>(radiance), n);
+  return Lotus::EstimateTemporalVariance(s, r, x, y, width, height, batch_samples, max_history_samples);
+}
+
+// temporal.cpp:424-512 (FilterTemporalRadiance): RGBA per pixel into out.
+// Returns 0 when the oracle rejects the input.
+int ref_filter_radiance(const void* surface, const void* radiance, uint32_t width, uint32_t height,
+                        uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations, float* out) {
+  const size_t n = size_t(width) * height;
+  std::span<const Lotus::TemporalPixel> s(static_cast<const Lotus::TemporalPixel*>(surface), n);
+  std::span<const Lotus::TemporalRadiancePixel> r(static_cast<const Lotus::TemporalRadiancePixel*>(radiance), n);
+  const auto v = Lotus::FilterTemporalRadiance(s, r, width, height, batch_samples, max_history_samples, iterations);
+  if (v.size() != n) return 0;
+  std::memcpy(out, v.data(), n * 16);
+  return 1;
+}
+
+// temporal.cpp:258-300 (ReprojectTemporalHistory): the validity record of
+// pixel (x, y) against width*height previous records; settings: absolute and
+// relative depth tolerance, minimum normal cosine, max motion pixels.
+void ref_reproject(const void* current, uint32_t x, uint32_t y, uint32_t width, uint32_t height,
+                   const void* previous, const float* settings, uint32_t* out) {
+  Lotus::TemporalValidationSettings v;
+  v.absolute_depth_tolerance = settings[0];
+  v.relative_depth_tolerance = settings[1];
+  v.minimum_normal_cosine = settings[2];
+  v.max_motion_pixels = settings[3];
+  const size_t n = previous ? size_t(width) * height : 0;
+  std::span<const Lotus::TemporalPixel> p(static_cast<const Lotus::TemporalPixel*>(previous), n);
+  const auto r = Lotus::ReprojectTemporalHistory(*static_cast<const Lotus::TemporalPixel*>(current), x, y, width,
+                                                 height, p, v);
+  out[0] = r.valid;
+  out[1] = static_cast<uint32_t>(r.rejection);
+  out[2] = r.previous_index;
+  out[3] = r.reserved;
 }
 
 }  // extern "C"

@@ -1,0 +1,237 @@
+"""Temporal frames on the GPU: SVGF's variance estimate and a-trous filter
+and the history reprojection against the reference's own CPU oracles
+(backend/vulkan/temporal.cpp, built into oracle/_ref), temporal
+accumulation against progressive accumulation, and ReSTIR DI reuse
+(temporal + spatial) against plain next event estimation."""
+import numpy as np
+import pytest
+
+import oracle
+from lotus_amd import hip, scenes
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref not built (no reference sources)")]
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    r = hip.Renderer(0)
+    yield r
+    r.close()
+
+
+def _pack_normal(n):
+    """Octahedral snorm16 packing (the surface product's shading normal)."""
+    s = np.abs(n).sum(-1, keepdims=True)
+    e = n[..., :2] / s
+    neg = n[..., 2] < 0
+    sign = np.where(e >= 0, 1.0, -1.0)
+    fold = (1.0 - np.abs(e[..., ::-1])) * sign
+    e = np.where(neg[..., None], fold, e)
+    q = np.rint(np.clip(e, -1, 1) * 32767).astype(np.int64)
+    return ((q[..., 0] & 0xFFFF) | ((q[..., 1] & 0xFFFF) << 16)).astype(np.uint32)
+
+
+def _synthetic(w, h, seed, empty=0.1):
+    rng = np.random.default_rng(seed)
+    ys, xs = np.mgrid[0:h, 0:w]
+    surf = np.zeros((h, w), hip.TEMPORAL_PIXEL)
+    inst = np.where(xs < w // 3, 1, np.where(xs < 2 * w // 3, 2, 3)) + np.where(ys > h // 2, 10, 0)
+    depth = 2.0 + 0.01 * xs + 0.02 * ys + 0.5 * (inst % 2)
+    n = np.stack([0.1 * np.sin(xs / 5.0), 0.1 * np.cos(ys / 7.0), np.ones_like(depth)], -1)
+    n += rng.normal(0, 0.02, n.shape)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    surf["normal_depth"][..., :3] = n
+    surf["normal_depth"][..., 3] = depth
+    surf["instance_id"] = inst
+    surf["material_id"] = inst
+    surf["flags"] = 7
+    surf["albedo"] = rng.integers(0, 1 << 24, (h, w), dtype=np.uint32)
+    surf["albedo"][rng.random((h, w)) < 0.05] &= 0xFFFF00  # zero albedo channels
+    sn = n + rng.normal(0, 0.05, n.shape)
+    surf["shading_normal"] = _pack_normal(sn / np.linalg.norm(sn, axis=-1, keepdims=True))
+    rad = np.zeros((h, w), hip.TEMPORAL_RADIANCE)
+    count = rng.integers(1, 60, (h, w)).astype(np.uint32)
+    count[rng.random((h, w)) < empty] = 0
+    rad["sample_count"] = count
+    rad["color"][..., :3] = rng.lognormal(-1, 1, (h, w, 3))
+    rad["color"][..., 3] = 1.0
+    m1 = rng.lognormal(-1, 0.7, (h, w))
+    rad["first_moment"] = m1
+    rad["second_moment"] = m1 * m1 + rng.lognormal(-3, 1, (h, w))
+    return surf, rad
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).reshape(-1).copy()).cuda()
+
+
+def _host(t, dtype, shape):
+    return t.cpu().numpy().view(dtype).reshape(shape)
+
+
+@pytest.mark.parametrize("batch,cap", [(1, 32), (4, 16)])
+def test_svgf_variance_and_filter_match_reference(renderer, batch, cap):
+    R = oracle.ref()
+    w, h = 61, 47
+    surf, rad = _synthetic(w, h, batch * 7 + cap)
+    ds, dr = _dev(surf), _dev(rad)
+    color = torch.zeros(w * h * 4, dtype=torch.float32, device="cuda")
+    renderer.svgf(ds.data_ptr(), dr.data_ptr(), w, h, batch, cap, 5, color.data_ptr())
+    out = _host(dr, hip.TEMPORAL_RADIANCE, (h, w))
+    ref_var = np.array([[R.ref_estimate_variance(surf.ctypes.data, rad.ctypes.data, x, y, w, h, batch, cap)
+                         for x in range(w)] for y in range(h)], np.float32)
+    err = np.abs(out["variance"] - ref_var)
+    print(f"variance: max rel err {np.max(err / np.maximum(ref_var, 1e-12)):.3g}")
+    assert np.all(err <= 2e-5 * np.abs(ref_var) + 1e-9)
+    # the filter against the reference's on the same variance words
+    rad2 = rad.copy()
+    rad2["variance"] = out["variance"]
+    ref_col = np.zeros((h, w, 4), np.float32)
+    assert R.ref_filter_radiance(surf.ctypes.data, rad2.ctypes.data, w, h, batch, cap, 5,
+                                 ref_col.ctypes.data_as(oracle.C.POINTER(oracle.C.c_float)))
+    got = color.cpu().numpy().reshape(h, w, 4)
+    rel = np.abs(got - ref_col) / np.maximum(np.abs(ref_col), 1e-6)
+    print(f"filter: max rel err {rel.max():.3g}, mean {rel.mean():.3g}")
+    assert np.all(np.isfinite(got))
+    assert rel.max() < 1e-3
+
+
+def test_reprojection_matches_reference(renderer):
+    R = oracle.ref()
+    w, h = 53, 41
+    rng = np.random.default_rng(5)
+    cur, _ = _synthetic(w, h, 1)
+    prev, _ = _synthetic(w, h, 2)
+    cur["motion_depth"][..., 0] = rng.uniform(-45, 45, (h, w)).astype(np.float32)
+    cur["motion_depth"][..., 1] = rng.uniform(-45, 45, (h, w)).astype(np.float32)
+    cur["motion_depth"][..., 2] = prev["normal_depth"][..., 3].mean() * rng.choice([1.0, 1.001, 1.5], (h, w))
+    cur["motion_depth"][..., 3] = rng.choice([1.0, 0.0], (h, w), p=[0.9, 0.1])
+    cur["flags"] = rng.choice([7, 7, 7, 5, 3, 0], (h, w))
+    cur["instance_id"] = rng.choice([1, 2], (h, w))
+    prev["instance_id"] = rng.choice([1, 2], (h, w))
+    prev["normal_depth"][..., 3] = prev["normal_depth"][..., 3].mean()
+    prev["flags"] = rng.choice([7, 0], (h, w), p=[0.95, 0.05])
+    val = np.zeros((h, w), hip.TEMPORAL_VALIDITY)
+    dc, dp, dv = _dev(cur), _dev(prev), _dev(val)
+    settings = np.array([0.01, 0.01, 0.9, 32.0], np.float32)
+    renderer.reproject(dc.data_ptr(), dp.data_ptr(), w, h, dv.data_ptr(), settings)
+    got = _host(dv, hip.TEMPORAL_VALIDITY, (h, w))
+    ref = np.zeros(4, np.uint32)
+    codes = np.zeros(11, int)
+    for y in range(h):
+        for x in range(w):
+            R.ref_reproject(cur[y, x:x + 1].ctypes.data, x, y, w, h, prev.ctypes.data,
+                            settings.ctypes.data_as(oracle.C.POINTER(oracle.C.c_float)),
+                            ref.ctypes.data_as(oracle.C.POINTER(oracle.C.c_uint32)))
+            g = got[y, x]
+            assert (g["valid"], g["rejection"], g["previous_index"]) == tuple(ref[:3]), (x, y, g, ref)
+            codes[ref[1]] += 1
+    print("rejection codes", codes)
+    assert np.count_nonzero(codes) >= 6  # the cases are exercised
+    # without history every hit is HistoryUnavailable
+    renderer.reproject(dc.data_ptr(), 0, w, h, dv.data_ptr(), settings)
+    got = _host(dv, hip.TEMPORAL_VALIDITY, (h, w))
+    assert np.all(got["rejection"][(cur["flags"] & 1) != 0] == 8)
+
+
+def _cornell(renderer):
+    sc = scenes.cornell_lambert()
+    renderer.apply(sc)
+    return sc
+
+
+def test_static_temporal_accumulation_is_the_progressive_mean(renderer):
+    # A still camera reprojects every hit onto itself; with a cap above the
+    # frame count the accumulated colour is the mean of the fresh batches,
+    # i.e. the progressive accumulation of the same samples.
+    sc = _cornell(renderer)
+    w = h = 64
+    n = 12
+    for f in range(n):
+        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=100 + f, sample_index=f,
+                                       accumulate=True, max_history=64, max_bounces=8, want_products=True)
+        assert out["reset_reason"] == (1 if f == 0 else 0)
+    hit = (out["surface"]["flags"] & 1) != 0
+    assert hit.mean() > 0.9
+    assert np.all(out["validity"]["valid"][hit] == 1)
+    assert np.all(np.abs(out["surface"]["motion_depth"][hit][:, :2]) < 1e-3)  # projection rounding only
+    assert np.all(out["radiance"]["sample_count"][hit] == n)
+    prog = renderer.render(sc.world_to_clip, w, h, frame_count=n, sample_index=0, max_bounces=8)["color"]
+    assert np.allclose(out["color"][hit], prog[hit], rtol=2e-5, atol=1e-6)
+
+
+def test_moving_camera_validity_matches_reference(renderer):
+    R = oracle.ref()
+    sc = _cornell(renderer)
+    w = h = 96
+    surfs, vals = [], []
+    for f in range(3):
+        view = scenes.look_at((0.02 * f, 0.0, 3.4 - 0.03 * f), (0.0, 0.0, 0.0))
+        w2c = scenes.multiply(sc.projection, view)
+        out = renderer.render_temporal(w2c, view, w, h, frame_index=200 + f, max_bounces=4, want_products=True)
+        surfs.append(out["surface"])
+        vals.append(out["validity"])
+    cur, prev, val = surfs[2], surfs[1], vals[2]
+    hit = (cur["flags"] & 1) != 0
+    moved = np.abs(cur["motion_depth"][..., :2]).sum(-1)
+    assert np.mean(moved[hit] > 0) > 0.9
+    assert np.mean(val["valid"][hit]) > 0.8
+    settings = np.array([0.01, 0.01, 0.9, 32.0], np.float32)
+    ref = np.zeros(4, np.uint32)
+    for y in range(0, h, 2):
+        for x in range(0, w, 2):
+            R.ref_reproject(cur[y, x:x + 1].ctypes.data, x, y, w, h, prev.ctypes.data,
+                            settings.ctypes.data_as(oracle.C.POINTER(oracle.C.c_float)),
+                            ref.ctypes.data_as(oracle.C.POINTER(oracle.C.c_uint32)))
+            g = val[y, x]
+            assert (g["valid"], g["rejection"], g["previous_index"]) == tuple(ref[:3]), (x, y)
+
+
+def _relmse(img, ref):
+    return float(np.mean((img[..., :3] - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def test_restir_reuse_is_consistent_and_reduces_error(renderer):
+    # Temporal + spatial reservoir reuse on the reference Cornell scene: no
+    # failed records, reuse grows the represented candidate counts, the
+    # image mean stays that of plain next event estimation, and one frame's
+    # error against a converged image falls below the initial-only frame's.
+    sc = scenes.cornell_v1()
+    renderer.apply(sc)
+    w = h = 96
+    ref = renderer.render(sc.world_to_clip, w, h, frame_count=1024, sample_index=1 << 20, max_bounces=8)["color"]
+    ref = ref[..., :3].astype(np.float64)
+    kw = dict(max_bounces=8, light_sampling=hip.LIGHT_RESTIR, restir_candidates=8, want_restir=True)
+    initial_err, reuse_err, means = [], [], []
+    for f in range(24):
+        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=300 + f, sample_index=f,
+                                       restir_temporal=True, restir_spatial=True, radius=8,
+                                       max_history_candidates=20, **kw)
+        rec = out["restir"]
+        assert np.all(rec[..., 7] & 2 == 0)
+        means.append(out["color"][..., :3].mean())
+        if f >= 8:
+            reuse_err.append(_relmse(out["color"], ref))
+    assert rec[..., 3].max() > 8  # reuse represents more than the initial candidates
+    for f in range(16):
+        out = renderer.render(sc.world_to_clip, w, h, sample_index=500 + f, **kw)
+        initial_err.append(_relmse(out["color"], ref))
+    m_reuse, m_ref = np.mean(means[8:]), ref.mean()
+    print(f"mean reuse {m_reuse:.5f} ref {m_ref:.5f}; relMSE initial {np.mean(initial_err):.4g} "
+          f"reuse {np.mean(reuse_err):.4g}")
+    assert abs(m_reuse - m_ref) < 0.03 * m_ref
+    assert np.mean(reuse_err) < np.mean(initial_err)
+
+
+def test_temporal_settings_errors(renderer):
+    sc = _cornell(renderer)
+    with pytest.raises(hip.LotusError, match="temporal reservoirs require"):
+        renderer.render_temporal(sc.world_to_clip, sc.view, 16, 16, frame_index=900, restir_temporal=True)
+    with pytest.raises(hip.LotusError, match="spatial reservoirs require"):
+        renderer.render_temporal(sc.world_to_clip, sc.view, 16, 16, frame_index=901, restir_spatial=True,
+                                 light_sampling=hip.LIGHT_RESTIR, radius=0)
+    with pytest.raises(hip.LotusError, match="filter_iterations"):
+        renderer.render_temporal(sc.world_to_clip, sc.view, 16, 16, frame_index=902, filter_iterations=3)
