@@ -130,6 +130,44 @@ def relmse_race(r, scene, width, height, depth, frames, ref_frames, cache_cfg, c
                                  "mfma_peak_tflops": 2500.0}}
 
 
+def restir_svgf_run(r, width, height, frames, depth=8, candidates=32, history=20, radius=30):
+    """ReSTIR DI (32 initial candidates, temporal reuse capped at 20, spatial
+    reuse of the four axis neighbours at 30 px) and SVGF (32-sample history,
+    5 a-trous iterations) over a seeded camera path through the Cornell box
+    with 100k emissive ceiling triangles; GPU ms per temporal frame by pass
+    and, at the last camera, relMSE of the output and of a plain NEE+MIS
+    frame against a 512-spp plain reference."""
+    sc = scenes.cornell_many_lights(camera_aspect=width / height)
+    r.apply(sc)
+    tot = {"frame_ms": 0.0, "restir_ms": 0.0, "trace_ms": 0.0, "shade_ms": 0.0, "camera_ms": 0.0}
+    rays = 0
+    out = None
+    for f in range(frames):
+        view = scenes.camera_spline(f, frames)
+        w2c = scenes.multiply(sc.projection, view)
+        out = r.render_temporal(w2c, view, width, height, frame_index=f + 1, sample_index=f, accumulate=True,
+                                max_history=32, filter_iterations=5, restir_temporal=True, restir_spatial=True,
+                                radius=radius, max_history_candidates=history, max_bounces=depth,
+                                light_sampling=hip.LIGHT_RESTIR, restir_candidates=candidates)
+        for k in tot:
+            tot[k] += out[k]
+        rays += out["camera_rays"] + out["extension_rays"] + out["shadow_rays"]
+    final = out["color"]
+    ref = None
+    for _ in range(512):
+        ref = r.render(w2c, width, height, sample_index=1 << 22, max_bounces=depth, want_color=False)
+    ref = r.render(w2c, width, height, sample_index=1 << 22, frame_count=0, max_bounces=depth)["color"][..., :3]
+    ref = ref.astype(np.float64)
+    plain = r.render(w2c, width, height, sample_index=77, max_bounces=depth)["color"]
+    return {"workload": f"Cornell box + {sc.triangle_count - 34} emissive ceiling tris, {width}x{height}, "
+                        f"{frames}-frame camera path, depth {depth}",
+            "frames": frames,
+            "gpu_ms_per_frame": {k.replace("_ms", ""): round(v / frames, 3) for k, v in tot.items()},
+            "mrays_per_s": round(rays / (tot["frame_ms"] * 1e3), 1),
+            "relmse_last_frame": {"restir_svgf": round(relmse(final, ref), 5), "plain_mis_1spp": round(relmse(plain, ref), 5)},
+            "reference_spp": 512}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +184,8 @@ def main() -> int:
     ap.add_argument("--race-frames", type=int, default=64,
                     help="frames of each form in the equal-relMSE race (0 skips it)")
     ap.add_argument("--race-reference-spp", type=int, default=4096)
+    ap.add_argument("--restir-frames", type=int, default=120,
+                    help="temporal frames of the ReSTIR + SVGF run (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times: the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -269,6 +309,9 @@ def main() -> int:
         race["cornell_1024"] = relmse_race(r, box, 1024, 1024, 8, args.race_frames, args.race_reference_spp,
                                            cache_cfg)
         line["equal_relmse"] = race
+    if args.restir_frames > 0:
+        log("ReSTIR + SVGF run ...")
+        line["restir_svgf"] = restir_svgf_run(r, args.width, args.height, args.restir_frames)
     r.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")

@@ -260,6 +260,63 @@ def cornell_lambert() -> Scene:
 
 
 # ------------------------------------------------------- closed-form scenes --
+def cornell_many_lights(count: int = 100_000, seed: int = 2027, buckets: int = 64,
+                        camera_aspect: float = 16.0 / 9.0) -> Scene:
+    """The ReSTIR + SVGF workload: the config-1 box and blocks (Lambertian),
+    the panel light removed, and `count` seeded emissive triangles on a
+    ceiling plane just below the ceiling, facing down, each of the same
+    area, with power log-uniform over [1e-2, 1e2] (radiance bucketed into
+    `buckets` log-spaced levels, one mesh per level)."""
+    sc = Scene(name=f"cornell-lights-{count}")
+    view, _ = cornell_camera()
+    sc.view = view
+    sc.projection = perspective(45.0, camera_aspect, 1.0, 10.0)
+    lam = lambda c: preview_material(base_color=c, ior=1.0)
+    m_white = sc.add_material(lam((0.73, 0.73, 0.73)))
+    m_left = sc.add_material(lam((0.65, 0.05, 0.05)))
+    m_right = sc.add_material(lam((0.12, 0.45, 0.15)))
+    sc.add_mesh(*_quad([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]]), m_white)
+    sc.add_mesh(*_quad([[-1, 1, 1], [-1, 1, -1], [1, 1, -1], [1, 1, 1]]), m_white)
+    sc.add_mesh(*_quad([[-1, -1, 1], [1, -1, 1], [1, -1, -1], [-1, -1, -1]]), m_white)
+    sc.add_mesh(*_quad([[-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1]]), m_left)
+    sc.add_mesh(*_quad([[1, -1, -1], [1, -1, 1], [1, 1, 1], [1, 1, -1]]), m_right)
+    sc.add_mesh(UNIT_CUBE_POS, UNIT_CUBE_TRI, m_white, block((0.6, 0.6, 0.6), -18.0, (0.38, -0.699, 0.3)))
+    sc.add_mesh(UNIT_CUBE_POS, UNIT_CUBE_TRI, m_white, block((0.6, 1.2, 0.6), 17.0, (-0.35, -0.399, -0.3)))
+    i = np.arange(count)
+    cx = uniform(seed, 1, i) * 1.9 - 0.95
+    cz = uniform(seed, 2, i) * 1.9 - 0.95
+    ang = uniform(seed, 3, i) * 2.0 * math.pi
+    size = 0.004  # circumradius: every triangle is equilateral with the same area
+    area = 3.0 * math.sqrt(3.0) / 4.0 * size * size
+    corners = []
+    for k in (0, 2, 1):  # wound so the face normal points down, into the box
+        a = ang + k * 2.0 * math.pi / 3.0
+        corners.append(np.stack([cx + size * np.cos(a), np.full(count, 0.995), cz - size * np.sin(a)], -1))
+    pos = np.stack(corners, 1).astype(F32)  # [count, 3, 3]
+    power = 10.0 ** (uniform(seed, 4, i) * 4.0 - 2.0)
+    level = np.minimum((np.log10(power) + 2.0) / 4.0 * buckets, buckets - 1).astype(np.int64)
+    for b in range(buckets):
+        sel = np.nonzero(level == b)[0]
+        if sel.size == 0:
+            continue
+        p = 10.0 ** ((b + 0.5) / buckets * 4.0 - 2.0)  # the bucket's power
+        radiance = p / (area * math.pi)
+        m = sc.add_material(preview_material(base_color=(0.0, 0.0, 0.0), ior=1.0, emission=(radiance,) * 3))
+        tri = np.arange(sel.size * 3, dtype=np.uint32).reshape(-1, 3)
+        sc.add_mesh(pos[sel].reshape(-1, 3), tri, m)
+    return sc
+
+
+def camera_spline(frame: int, frames: int = 120, seed: int = 7):
+    """A seeded smooth camera path in front of the box: the eye drifts on a
+    Lissajous curve around (0, 0, 3), looking at the box centre."""
+    t = frame / max(frames - 1, 1)
+    ph = [uniform(seed, 9, np.arange(3)) * 2 * math.pi][0]
+    eye = (0.35 * math.sin(2 * math.pi * t + ph[0]), 0.2 * math.sin(4 * math.pi * t + ph[1]),
+           3.0 + 0.25 * math.sin(2 * math.pi * t + ph[2]))
+    return look_at(eye, (0.0, 0.0, -0.2))
+
+
 def lambert_under_sky() -> Scene:
     """adapters/headless/main.cpp:1525-1545: a lone Lambert triangle (ior 1)
     under a constant environment reflects albedo * environment + emission."""
