@@ -168,6 +168,49 @@ def restir_svgf_run(r, width, height, frames, depth=8, candidates=32, history=20
             "reference_spp": 512}
 
 
+def batch_run(views, frames):
+    """64 seeded views of 64^2..512^2 over their own icosphere rigs (1k..200k
+    triangles, depth 2..12): each view autotuned, then frames rendered view
+    by view (one call each), as one batch, and as one hipGraph per frame."""
+    rs = []
+    t_build = time.perf_counter()
+    for v in views:
+        r = hip.Renderer(0)
+        r.apply(scenes.icosphere_rig(v["scene_triangles"], v["scene_seed"]))
+        rs.append(r)
+    build_s = time.perf_counter() - t_build
+    kws = [dict(world_to_clip=v["world_to_clip"], width=v["size"], height=v["size"], max_bounces=v["depth"],
+                light_sampling=hip.LIGHT_MIS) for v in views]
+    tunings = [r.autotune(trials=1, **kw) for r, kw in zip(rs, kws)]
+    res = {}
+    for mode in ("per_view", "batch", "graph"):
+        for r, kw in zip(rs, kws):  # warm-up frame: rounds and targets settle
+            r.render(want_color=False, sample_index=0, **kw)
+        rays, gpu_ms = 0, 0.0
+        t0 = time.perf_counter()
+        for f in range(frames):
+            if mode == "per_view":
+                for r, kw in zip(rs, kws):
+                    out = r.render(want_color=False, sample_index=f + 1, **kw)
+                    rays += out["camera_rays"] + out["extension_rays"] + out["shadow_rays"]
+                    gpu_ms += out["frame_ms"]
+            else:
+                outs, ms = hip.render_batch(rs, [dict(kw, sample_index=f + 1) for kw in kws], graph=mode == "graph",
+                                            want_color=False)
+                rays += sum(o["camera_rays"] + o["extension_rays"] + o["shadow_rays"] for o in outs)
+                gpu_ms += ms
+        wall = time.perf_counter() - t0
+        res[mode] = {"gpu_ms_per_frame": round(gpu_ms / frames, 3), "wall_ms_per_frame": round(wall / frames * 1e3, 3),
+                     "mrays_per_s_gpu": round(rays / (gpu_ms * 1e3), 1), "views_per_s_wall": round(len(rs) * frames / wall, 1)}
+    for r in rs:
+        r.close()
+    return {"workload": f"{len(views)} views {min(v['size'] for v in views)}^2..{max(v['size'] for v in views)}^2 over "
+                        f"their own icosphere rigs ({min(v['scene_triangles'] for v in views)}.."
+                        f"{max(v['scene_triangles'] for v in views)} tris), depth 2..12, NEE+MIS",
+            "frames": frames, "scene_build_s": round(build_s, 2),
+            "autotuned_dynamic_trace": sum(t["trace_dynamic"] for t in tunings), **res}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +227,8 @@ def main() -> int:
     ap.add_argument("--race-frames", type=int, default=64,
                     help="frames of each form in the equal-relMSE race (0 skips it)")
     ap.add_argument("--race-reference-spp", type=int, default=4096)
+    ap.add_argument("--batch-frames", type=int, default=16,
+                    help="frames of the 64-view batched workload (0 skips it)")
     ap.add_argument("--restir-frames", type=int, default=120,
                     help="temporal frames of the ReSTIR + SVGF run (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -309,6 +354,9 @@ def main() -> int:
         race["cornell_1024"] = relmse_race(r, box, 1024, 1024, 8, args.race_frames, args.race_reference_spp,
                                            cache_cfg)
         line["equal_relmse"] = race
+    if args.batch_frames > 0:
+        log("batched views ...")
+        line["batched_views"] = batch_run(scenes.batch_views(), args.batch_frames)
     if args.restir_frames > 0:
         log("ReSTIR + SVGF run ...")
         line["restir_svgf"] = restir_svgf_run(r, args.width, args.height, args.restir_frames)

@@ -195,6 +195,13 @@ struct lotus_hip_renderer {
     uint32_t light_sampling = 0, max_bounces = 0, candidates = 0, proposal = 0;
     bool reservoirs = false, radiance = false;
   } temporal;
+  // A batched frame's sample whose counters are read after the batch.
+  struct Deferred {
+    bool pending = false;
+    bool add = false;
+    uint32_t stride = 0, rounds = 0;
+  } deferred;
+  hipEvent_t join = nullptr;       // this renderer's branch of a batch ends
   float4* svgf_filter = nullptr;   // lotus_hip_svgf's scratch
   uint32_t svgf_pixels = 0;
 
@@ -585,6 +592,7 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   dev_free(r->spill);
   for (auto& ev : r->events)
     if (ev) (void)hipEventDestroy(ev);
+  if (r->join) (void)hipEventDestroy(r->join);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
 }
@@ -716,7 +724,7 @@ int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, 
 
 namespace {
 int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res, char* detail,
-                size_t detail_size, TemporalFrameState* tf);
+                size_t detail_size, TemporalFrameState* tf, bool deferred = false);
 }  // namespace
 
 int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res,
@@ -727,8 +735,11 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
 }  // extern "C"
 
 namespace {
+// `deferred` (batched frames): one sample is enqueued on the renderer's
+// stream with the previous sample's rounds, without events, readbacks or a
+// synchronisation; batch_finish reads its counters afterwards.
 int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res, char* detail,
-                size_t detail_size, TemporalFrameState* tf) {
+                size_t detail_size, TemporalFrameState* tf, bool deferred) {
   if (!r || !P) return LOTUS_HIP_FAIL;
   if (!r->failure.empty()) {
     set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
@@ -892,10 +903,13 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
       size_t ev = 0;
       std::vector<std::pair<int, size_t>> pairs;
       auto mark = [&](int timer) {
+        if (deferred) return;
         pairs.emplace_back(timer, ev);
         (void)hipEventRecord(r->events[ev++], r->stream);
       };
-      auto done = [&]() { (void)hipEventRecord(r->events[ev++], r->stream); };
+      auto done = [&]() {
+        if (!deferred) (void)hipEventRecord(r->events[ev++], r->stream);
+      };
       launch_zero(r->counters, frame_counter_words(stride), r->stream);
       launch_zero(r->work, kWorkCounters, r->stream);
       if (cache_frame) {
@@ -978,6 +992,10 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
         e = hipGetLastError();
         if (e == hipSuccess)
           e = hipMemcpyAsync(&loss_host, r->cache.loss, sizeof(float), hipMemcpyDeviceToHost, r->stream);
+      }
+      if (deferred) {
+        r->deferred = {true, add != 0, stride, rounds};
+        return LOTUS_HIP_PASS;
       }
       std::vector<uint32_t> host(frame_counter_words(stride));
       e = hipGetLastError();
@@ -1588,6 +1606,175 @@ int lotus_hip_reproject(lotus_hip_renderer* r, const void* current, const void* 
     set_detail(detail, detail_size, latch(r, "reprojection", e));
     return LOTUS_HIP_FAIL;
   }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The bookkeeping of a deferred sample once its batch has completed: ray
+// counts, the next sample's rounds, the accumulation count, the colour.
+int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res, double gpu_ms,
+                 char* detail, size_t detail_size) {
+  auto& d = r->deferred;
+  d.pending = false;
+  std::vector<uint32_t> host(frame_counter_words(d.stride));
+  hipError_t e = hipMemcpy(host.data(), r->counters, host.size() * 4, hipMemcpyDeviceToHost);
+  const uint32_t npix = P->width * P->height;
+  if (e == hipSuccess && res && res->color)
+    e = hipMemcpy(res->color, r->q.color, npix * 16ull, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "batch readback", e));
+    return LOTUS_HIP_FAIL;
+  }
+  std::vector<uint32_t> counts;
+  uint64_t ext = 0, sh = 0;
+  if (d.add) {
+    counts.assign(d.stride, 0u);
+    size_t last = 0;
+    for (uint32_t b = 0; b < d.stride; ++b) {
+      counts[b] = host[4 * b];
+      if (counts[b]) last = b;
+      if (b >= 1) ext += host[4 * b];
+      sh += host[4 * b + 2];
+    }
+    counts.resize(host[0] ? last + 1 : 0);
+    r->rounds_hint = wavefront_rounds(counts);
+    ++r->accumulated;
+  }
+  if (res) {
+    res->path_counts_size = static_cast<uint32_t>(std::min<size_t>(counts.size(), res->path_counts_capacity));
+    if (res->path_counts) std::memcpy(res->path_counts, counts.data(), res->path_counts_size * 4ull);
+    res->wavefront_rounds = counts.empty() ? 0 : d.rounds;
+    res->samples_per_pixel = r->accumulated;
+    res->frames_rendered = 1;
+    res->camera_rays = d.add ? host[0] : 0;
+    res->extension_rays = ext;
+    res->shadow_rays = sh;
+    res->frame_gpu_ms = gpu_ms;
+  }
+  return LOTUS_HIP_PASS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip_frame_params* params,
+                           lotus_hip_frame_result* results, uint32_t count, uint32_t flags, double* gpu_ms,
+                           char* detail, size_t detail_size) {
+  if (!renderers || !params || count == 0) {
+    set_detail(detail, detail_size, "a batch needs at least one view");
+    return LOTUS_HIP_FAIL;
+  }
+  for (uint32_t i = 0; i < count; ++i) {
+    lotus_hip_renderer* r = renderers[i];
+    const lotus_hip_frame_params& P = params[i];
+    if (!r) return LOTUS_HIP_FAIL;
+    for (uint32_t j = 0; j < i; ++j)
+      if (renderers[j] == r) {
+        set_detail(detail, detail_size, "each view of a batch needs its own renderer");
+        return LOTUS_HIP_FAIL;
+      }
+    if (!r->failure.empty()) {
+      set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+      return LOTUS_HIP_FAIL;
+    }
+    if (r->device != renderers[0]->device) {
+      set_detail(detail, detail_size, "a batch runs on one device");
+      return LOTUS_HIP_FAIL;
+    }
+    if (P.output != LOTUS_HIP_OUTPUT_RADIANCE || P.frame_count != 1 || P.light_sampling > LOTUS_HIP_LIGHT_MIS ||
+        (P.flags & ~LOTUS_HIP_FRAME_SPLIT_KINDS) != 0) {
+      set_detail(detail, detail_size, "batched views render one radiance sample with BSDF or MIS light sampling");
+      return LOTUS_HIP_FAIL;
+    }
+  }
+  lotus_hip_renderer* root = renderers[0];
+  (void)hipSetDevice(root->device);
+  hipError_t e = hipSuccess;
+  // targets first: nothing allocates once the frame is being captured
+  for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+    e = ensure_target(renderers[i], params[i].width, params[i].height);
+    if (e == hipSuccess && !renderers[i]->join) e = hipEventCreateWithFlags(&renderers[i]->join, hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(root, "batch allocation", e));
+    return LOTUS_HIP_FAIL;
+  }
+  const bool graph = (flags & LOTUS_HIP_BATCH_GRAPH) != 0;
+  hipEvent_t start = root->events[root->events.size() - 2], stop = root->events[root->events.size() - 1];
+  hipEvent_t fork = root->join;
+  if (graph) e = hipStreamBeginCapture(root->stream, hipStreamCaptureModeRelaxed);
+  if (!graph && e == hipSuccess) e = hipEventRecord(start, root->stream);
+  if (e == hipSuccess) e = hipEventRecord(fork, root->stream);
+  for (uint32_t i = 1; i < count && e == hipSuccess; ++i) e = hipStreamWaitEvent(renderers[i]->stream, fork, 0);
+  int status = LOTUS_HIP_PASS;
+  for (uint32_t i = 0; i < count && e == hipSuccess && status == LOTUS_HIP_PASS; ++i)
+    status = render_impl(renderers[i], &params[i], nullptr, detail, detail_size, nullptr, true);
+  for (uint32_t i = 1; i < count && e == hipSuccess; ++i) {
+    e = hipEventRecord(renderers[i]->join, renderers[i]->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(root->stream, renderers[i]->join, 0);
+  }
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (graph) {
+    const hipError_t end = hipStreamEndCapture(root->stream, &g);
+    if (e == hipSuccess) e = end;
+    if (e == hipSuccess && status == LOTUS_HIP_PASS) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (e == hipSuccess && status == LOTUS_HIP_PASS) e = hipEventRecord(start, root->stream);
+    if (e == hipSuccess && status == LOTUS_HIP_PASS) e = hipGraphLaunch(exec, root->stream);
+  }
+  if (e == hipSuccess) e = hipEventRecord(stop, root->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(root->stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess || status != LOTUS_HIP_PASS) {
+    for (uint32_t i = 0; i < count; ++i) renderers[i]->deferred.pending = false;
+    if (e != hipSuccess) set_detail(detail, detail_size, latch(root, "batched frame", e));
+    return LOTUS_HIP_FAIL;
+  }
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, start, stop);
+  if (gpu_ms) *gpu_ms = ms;
+  for (uint32_t i = 0; i < count; ++i) {
+    status = batch_finish(renderers[i], &params[i], results ? &results[i] : nullptr, ms, detail, detail_size);
+    if (status != LOTUS_HIP_PASS) return status;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_autotune(lotus_hip_renderer* r, const lotus_hip_frame_params* params, uint32_t trials,
+                       lotus_hip_tuning* chosen, char* detail, size_t detail_size) {
+  if (!r || !params) return LOTUS_HIP_FAIL;
+  const lotus_hip_tuning candidates[4] = {{0u, 0u}, {1u, 1u}, {0u, 1u}, {1u, 0u}};
+  double best = 1e300;
+  lotus_hip_tuning pick = r->tuning;
+  lotus_hip_frame_params P = *params;
+  P.frame_count = 1;
+  P.max_samples = 0;
+  trials = std::max(trials, 1u);
+  for (const auto& c : candidates) {
+    r->tuning = c;
+    double total = 0.0;
+    for (uint32_t k = 0; k <= trials; ++k) {
+      lotus_hip_frame_result res{};
+      r->key_valid = false;  // trial frames never join an accumulation
+      const int s = lotus_hip_render(r, &P, &res, detail, detail_size);
+      if (s != LOTUS_HIP_PASS) return s;
+      if (k > 0) total += res.frame_gpu_ms;  // the first frame sizes the rounds
+    }
+    if (total < best) {
+      best = total;
+      pick = c;
+    }
+  }
+  r->tuning = pick;
+  r->key_valid = false;
+  if (chosen) *chosen = pick;
   set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;
 }

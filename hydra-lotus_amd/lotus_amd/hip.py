@@ -31,7 +31,8 @@ SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hi
            "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning",
            "lotus_hip_cache_configure", "lotus_hip_cache_infer", "lotus_hip_cache_gradients",
            "lotus_hip_cache_train", "lotus_hip_cache_parameters", "lotus_hip_render_temporal", "lotus_hip_svgf",
-           "lotus_hip_reproject"]
+           "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune"]
+BATCH_GRAPH = 1
 
 # The reference's temporal records (include/lotus/temporal.hpp:193-245).
 TEMPORAL_PIXEL = np.dtype([("normal_depth", "<f4", 4), ("motion_depth", "<f4", 4), ("instance_id", "<u8"),
@@ -232,6 +233,12 @@ def lib():
         L.lotus_hip_reproject.restype = C.c_int
         L.lotus_hip_reproject.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, _f32p,
                                           C.c_void_p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_render_batch.restype = C.c_int
+        L.lotus_hip_render_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(FrameParams), C.POINTER(FrameResult),
+                                             C.c_uint32, C.c_uint32, C.POINTER(C.c_double), C.c_char_p, C.c_size_t]
+        L.lotus_hip_autotune.restype = C.c_int
+        L.lotus_hip_autotune.argtypes = [C.c_void_p, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Tuning),
+                                         C.c_char_p, C.c_size_t]
         L.lotus_hip_cache_configure.restype = C.c_int
         L.lotus_hip_cache_configure.argtypes = [C.c_void_p, C.POINTER(CacheConfig), C.c_char_p, C.c_size_t]
         L.lotus_hip_cache_infer.restype = C.c_int
@@ -263,6 +270,63 @@ def probe() -> tuple:
     buf = C.create_string_buffer(256)
     s = lib().lotus_hip_probe(buf, 256)
     return s, buf.value.decode()
+
+
+def frame_params(world_to_clip, width, height, frame_count=1, sample_index=0, max_bounces=8,
+                 light_sampling=LIGHT_MIS, output=OUTPUT_RADIANCE, clear_color=(0.0, 0.0, 0.0, 0.0),
+                 data_window=(0, 0, 0, 0), display_window=(0.0, 0.0, 0.0, 0.0), max_samples=0, split_kinds=False,
+                 cache=False, restir_candidates=8, restir_proposal=0) -> FrameParams:
+    fp = FrameParams()
+    w2c = np.ascontiguousarray(world_to_clip, np.float32).reshape(16)
+    for i in range(16):
+        fp.world_to_clip[i] = float(w2c[i])
+    fp.width, fp.height = width, height
+    for i in range(4):
+        fp.display_window[i] = display_window[i]
+        fp.data_window[i] = data_window[i]
+        fp.clear_color[i] = clear_color[i]
+    fp.frame_count = frame_count
+    fp.output = output
+    fp.light_sampling = light_sampling
+    fp.sample_index = sample_index
+    fp.max_bounces = max_bounces
+    fp.max_samples = max_samples
+    fp.flags = (FRAME_SPLIT_KINDS if split_kinds else 0) | (FRAME_RADIANCE_CACHE if cache else 0)
+    fp.restir_candidates = restir_candidates
+    fp.restir_proposal = restir_proposal
+    return fp
+
+
+def render_batch(renderers, views, graph=False, want_color=True):
+    """One batched frame: views[i] (frame_params() keyword dicts) through
+    renderers[i]. Returns (per-view results, GPU ms)."""
+    n = len(renderers)
+    handles = (C.c_void_p * n)(*[r.handle for r in renderers])
+    ps = (FrameParams * n)()
+    rs = (FrameResult * n)()
+    colors, counts = [], []
+    for i, v in enumerate(views):
+        ps[i] = frame_params(**v)
+        cap = v.get("max_bounces", 8) + 2
+        c = np.zeros((v["height"], v["width"], 4), np.float32) if want_color else None
+        k = np.zeros(cap, np.uint32)
+        if c is not None:
+            rs[i].color = _p(c, C.c_float)
+        rs[i].path_counts = _p(k, C.c_uint32)
+        rs[i].path_counts_capacity = cap
+        colors.append(c)
+        counts.append(k)
+    ms = C.c_double(0.0)
+    buf = C.create_string_buffer(512)
+    s = lib().lotus_hip_render_batch(handles, ps, rs, n, BATCH_GRAPH if graph else 0, C.byref(ms), buf, 512)
+    if s != PASS:
+        raise LotusError(s, buf.value.decode())
+    out = []
+    for i in range(n):
+        out.append(dict(color=colors[i], counts=counts[i][:rs[i].path_counts_size].copy(), spp=rs[i].samples_per_pixel,
+                        camera_rays=rs[i].camera_rays, extension_rays=rs[i].extension_rays,
+                        shadow_rays=rs[i].shadow_rays, rounds=rs[i].wavefront_rounds))
+    return out, ms.value
 
 
 class Renderer:
@@ -320,24 +384,9 @@ class Renderer:
                restir_candidates=8, restir_proposal=0, want_restir=False, temporal=None):
         """One RenderScene call; with `temporal` (a TemporalSettings), one
         RenderTemporalScene frame."""
-        fp = FrameParams()
-        w2c = np.ascontiguousarray(world_to_clip, np.float32).reshape(16)
-        for i in range(16):
-            fp.world_to_clip[i] = float(w2c[i])
-        fp.width, fp.height = width, height
-        for i in range(4):
-            fp.display_window[i] = display_window[i]
-            fp.data_window[i] = data_window[i]
-            fp.clear_color[i] = clear_color[i]
-        fp.frame_count = frame_count
-        fp.output = output
-        fp.light_sampling = light_sampling
-        fp.sample_index = sample_index
-        fp.max_bounces = max_bounces
-        fp.max_samples = max_samples
-        fp.flags = (FRAME_SPLIT_KINDS if split_kinds else 0) | (FRAME_RADIANCE_CACHE if cache else 0)
-        fp.restir_candidates = restir_candidates
-        fp.restir_proposal = restir_proposal
+        fp = frame_params(world_to_clip, width, height, frame_count, sample_index, max_bounces, light_sampling,
+                          output, clear_color, data_window, display_window, max_samples, split_kinds, cache,
+                          restir_candidates, restir_proposal)
         res = FrameResult()
         color = np.zeros((height, width, 4), np.float32) if want_color else None
         depth = np.zeros((height, width), np.float32) if want_depth else None
@@ -540,3 +589,13 @@ class Renderer:
                                       C.c_void_p(validity_ptr), buf, 256)
         if s != PASS:
             raise LotusError(s, buf.value.decode())
+
+    def autotune(self, trials=2, **view) -> dict:
+        """Times each traversal driver choice on `view` and keeps the fastest."""
+        fp = frame_params(**view)
+        t = Tuning()
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_autotune(self.handle, C.byref(fp), trials, C.byref(t), buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic}

@@ -317,6 +317,89 @@ def camera_spline(frame: int, frames: int = 120, seed: int = 7):
     return look_at(eye, (0.0, 0.0, -0.2))
 
 
+def _icosphere(level: int):
+    """Unit icosphere: positions and triangles after `level` midpoint splits."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(level):
+        edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        key = np.sort(edges, axis=1)
+        uniq, inv = np.unique(key, axis=0, return_inverse=True)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        m = inv.reshape(3, -1).T + base  # midpoints of edges 01, 12, 20
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = m[:, 0], m[:, 1], m[:, 2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([ab, b, bc], 1), np.stack([ca, bc, c], 1),
+                            np.stack([ab, bc, ca], 1)])
+    return v, f
+
+
+def icosphere_rig(triangles: int, seed: int) -> Scene:
+    """A batched-view scene: displaced icospheres (about `triangles` in all)
+    inside a closed box lit by two emissive ceiling panels; seeded
+    PreviewSurface materials."""
+    sc = Scene(name=f"rig-{triangles}-{seed}")
+    u = lambda stream, i: float(uniform(seed, stream, i))
+    m_wall = sc.add_material(preview_material(base_color=(0.7, 0.7, 0.68), ior=1.0))
+    m_light = sc.add_material(preview_material(base_color=(0.0, 0.0, 0.0), ior=1.0, emission=(12.0, 11.0, 9.0)))
+    e = 2.0
+    sc.add_mesh(*_quad([[-e, -e, -e], [e, -e, -e], [e, e, -e], [-e, e, -e]]), m_wall)
+    sc.add_mesh(*_quad([[-e, e, e], [-e, e, -e], [e, e, -e], [e, e, e]]), m_wall)
+    sc.add_mesh(*_quad([[-e, -e, e], [e, -e, e], [e, -e, -e], [-e, -e, -e]]), m_wall)
+    sc.add_mesh(*_quad([[-e, -e, e], [-e, -e, -e], [-e, e, -e], [-e, e, e]]), m_wall)
+    sc.add_mesh(*_quad([[e, -e, -e], [e, -e, e], [e, e, e], [e, e, -e]]), m_wall)
+    sc.add_mesh(*_quad([[-e, -e, e], [-e, e, e], [e, e, e], [e, -e, e]]), m_wall)
+    for k, x0 in enumerate((-0.9, 0.5)):
+        h = 0.35
+        sc.add_mesh(*_quad([[x0 - h, e - 0.01, h], [x0 - h, e - 0.01, -h], [x0 + h, e - 0.01, -h],
+                            [x0 + h, e - 0.01, h]]), m_light)
+    spheres = 1 + int(u(1, 0) * 6)
+    per = max(triangles // spheres, 80)
+    level = max(1, min(7, int(round(math.log(per / 20.0, 4)))))
+    base_v, base_f = _icosphere(level)
+    for s_ in range(spheres):
+        rough = 0.05 + 0.9 * u(2, s_)
+        metal = 1.0 if u(3, s_) < 0.25 else 0.0
+        col = tuple(0.2 + 0.7 * u(4 + c, s_) for c in range(3))
+        m = sc.add_material(preview_material(base_color=col, roughness=rough, metallic=metal, ior=1.5))
+        radius = 0.25 + 0.35 * u(8, s_)
+        centre = np.array([u(9, s_) * 2.4 - 1.2, u(10, s_) * 1.6 - 1.4, u(11, s_) * 2.4 - 1.2])
+        d = base_v
+        bump = np.sin(5.0 * d[:, 0] + 7.0 * u(12, s_)) * np.sin(4.0 * d[:, 1] + 3.0 * u(13, s_)) * \
+            np.sin(6.0 * d[:, 2] + 5.0 * u(14, s_))
+        pos = centre + d * (radius * (1.0 + 0.25 * bump))[:, None]
+        sc.add_mesh(pos.astype(F32), base_f.astype(np.uint32), m)
+    return sc
+
+
+def batch_views(count: int = 64, seed: int = 4, min_tris: int = 1000, max_tris: int = 200_000,
+                min_size: int = 64, max_size: int = 512):
+    """The batched-frame workload: `count` seeded views, each over its own
+    icosphere rig of log-uniform triangle count, log-uniform square size,
+    depth uniform in 2..12 and a camera looking into the rig."""
+    views = []
+    for i in range(count):
+        u = lambda stream: float(uniform(seed, stream, i))
+        tris = int(math.exp(math.log(min_tris) + u(1) * (math.log(max_tris) - math.log(min_tris))))
+        size = int(math.exp(math.log(min_size) + u(2) * (math.log(max_size) - math.log(min_size))))
+        depth = 2 + int(u(3) * 11)
+        ang = u(4) * 2 * math.pi
+        eye = (1.7 * math.cos(ang), 0.3 + 0.8 * u(5), 1.7 * math.sin(ang))
+        view = look_at(eye, (0.0, -0.5, 0.0))
+        proj = perspective(60.0, 1.0, 0.05, 20.0)
+        views.append(dict(scene_triangles=tris, scene_seed=seed * 1000 + i, size=size, depth=depth,
+                          view=view, world_to_clip=multiply(proj, view)))
+    return views
+
+
 def lambert_under_sky() -> Scene:
     """adapters/headless/main.cpp:1525-1545: a lone Lambert triangle (ior 1)
     under a constant environment reflects albedo * environment + emission."""

@@ -340,6 +340,27 @@ int lotus_hip_reproject(lotus_hip_renderer* renderer, const void* current_device
                         uint32_t width, uint32_t height, const float* validation, void* validity_device,
                         char* detail, size_t detail_size);
 
+/* Batched frames (no reference counterpart: the reference renders one target
+ * per call with a blocking submit, vulkan_backend.cpp:1238-1276). View i
+ * renders one radiance sample through renderers[i] (each view its own
+ * renderer and scene; BSDF or MIS light sampling) under the accumulation
+ * rules of lotus_hip_render, every view on its own stream with no host
+ * synchronisation between views. LOTUS_HIP_BATCH_GRAPH captures the whole
+ * frame, one branch per view, as one hipGraph and launches it. results
+ * (optional) receive each view's counts and colour; gpu_ms the frame's GPU
+ * time. Results are the same as unbatched frames. */
+#define LOTUS_HIP_BATCH_GRAPH 1u
+int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip_frame_params* params,
+                           lotus_hip_frame_result* results, uint32_t count, uint32_t flags, double* gpu_ms,
+                           char* detail, size_t detail_size);
+
+/* Autotuner (no reference counterpart): times trials frames of `params`
+ * under each traversal driver choice of lotus_hip_tuning and keeps the
+ * fastest; writes it to `chosen`. Trial frames do not join an
+ * accumulation, and the next frame restarts it. */
+int lotus_hip_autotune(lotus_hip_renderer* renderer, const lotus_hip_frame_params* params, uint32_t trials,
+                       lotus_hip_tuning* chosen, char* detail, size_t detail_size);
+
 #ifdef __cplusplus
 }
 #endif
