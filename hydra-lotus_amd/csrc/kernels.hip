@@ -35,47 +35,64 @@ __device__ __forceinline__ void flush_stats(unsigned long long* stats, int kind,
   }
 }
 
-// Persistent walks: every lane holds one ray's walk and steps it. A lane
-// whose walk ends waits until a quarter of its wave (or all of it) is ready;
-// then the ready lanes hand their results to finish(item, walk) together and
-// take as many new items from the launch's work counter, with one atomic per
-// wave, so lanes neither wait for the slowest ray of their wave nor pay the
-// hand-over on every step. Every wave leaves the loop together once the queue
-// is drained and its lanes have handed over their results.
+// Traversal counts of a workgroup, kept in LDS rather than in the walkers'
+// registers (the four per-lane sums of the combined traversal kernel cost it
+// a wave of occupancy per SIMD): a lane adds its finished walk's counts with
+// LDS atomics, and the workgroup adds its totals to stats[2 (first + k)],
+// stats[2 (first + k) + 1] once, at its end. Every thread of the workgroup
+// must call init() and flush().
+template <int kKinds>
+struct BlockStats {
+  unsigned long long* c;  // [2 kKinds] in LDS
+
+  __device__ __forceinline__ void init(unsigned long long* lds) {
+    c = lds;
+    if (threadIdx.x < 2 * kKinds) c[threadIdx.x] = 0;
+    __syncthreads();
+  }
+  __device__ __forceinline__ void add(int kind, uint32_t v, uint32_t t) {
+    atomicAdd(c + 2 * kind, static_cast<unsigned long long>(v));
+    atomicAdd(c + 2 * kind + 1, static_cast<unsigned long long>(t));
+  }
+  __device__ __forceinline__ void flush(unsigned long long* stats, int first) {
+    __syncthreads();
+    if (threadIdx.x < 2 * kKinds && c[threadIdx.x] != 0) atomicAdd(stats + 2 * first + threadIdx.x, c[threadIdx.x]);
+  }
+};
+
+__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
+
+// Walks with hand-over from fixed sequences: lane g takes items g, g + G,
+// g + 2G, ... (G lanes in the grid) like the strided walks, but a lane whose
+// walk ends does not wait for the slowest ray of its wave: once a quarter of
+// the wave (or all of it) is ready, the ready lanes hand their results to
+// finish(item, walk) together and start their next items. No atomics: the
+// sequences are fixed, so neighbouring lanes keep neighbouring items.
 template <class Fetch, class Finish>
-__device__ __forceinline__ void persistent_walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total,
-                                                 uint32_t* work, Fetch fetch, Finish finish) {
+__device__ __forceinline__ void sequence_walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total,
+                                               Fetch fetch, Finish finish) {
   Walk w;
   bool working = false;  // a walk in progress
   bool holding = false;  // a finished walk whose result awaits hand-over
-  bool drained = false;
   uint32_t item = 0;
-  const uint32_t lane = __lane_id();
+  uint32_t next = gtid();
+  const uint32_t stride = gstride();
   while (true) {
     const uint64_t active = __ballot(true);
     const uint64_t ready = __ballot(!working);
-    const bool last = drained && __ballot(working) == 0;
-    if (ready != 0 && (__popcll(ready) >= kRefill || ready == active || last)) {
+    if (__ballot(working || holding || next < total) == 0) break;
+    if (ready != 0 && (__popcll(ready) >= kRefill || ready == active)) {
       if (holding) {
         finish(item, w);
         holding = false;
       }
-      if (last) break;
-      if (!drained) {
-        const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(ready)) - 1);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(work, static_cast<uint32_t>(__popcll(ready)));
-        base = __shfl(base, static_cast<int>(leader));
-        if (base + static_cast<uint32_t>(__popcll(ready)) >= total) drained = true;
-        if (!working) {
-          const uint32_t idx = base + static_cast<uint32_t>(__popcll(ready & ((1ull << lane) - 1ull)));
-          if (idx < total) {
-            item = idx;
-            fetch(idx, w);
-            st.clear();
-            working = true;
-          }
-        }
+      if (!working && next < total) {
+        item = next;
+        next += stride;
+        fetch(item, w);
+        st.clear();
+        working = true;
       }
     }
     if (working) {
@@ -88,9 +105,6 @@ __device__ __forceinline__ void persistent_walks(const SceneView& S, LdsStack<kR
   }
 }
 
-__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
-__device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
-
 // The same walks without hand-over: each lane follows one ray to its end,
 // striding over the queue.
 template <class Fetch, class Finish>
@@ -100,15 +114,15 @@ __device__ __forceinline__ void strided_walks(const SceneView& S, LdsStack<kRing
     Walk w;
     fetch(i, w);
     st.clear();
-    while (!w.done()) w.step(S.nodes, S.tris, st);
+    w.run(S.nodes, S.tris, st);
     finish(i, w);
   }
 }
 
 template <bool kDynamic, class Fetch, class Finish>
-__device__ __forceinline__ void walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total, uint32_t* work,
-                                      Fetch fetch, Finish finish) {
-  if constexpr (kDynamic) persistent_walks(S, st, total, work, fetch, finish);
+__device__ __forceinline__ void walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total, Fetch fetch,
+                                      Finish finish) {
+  if constexpr (kDynamic) sequence_walks(S, st, total, fetch, finish);
   else strided_walks(S, st, total, fetch, finish);
 }
 
@@ -130,17 +144,18 @@ __device__ __forceinline__ void push_hit(const Queues& q, uint32_t k, uint32_t s
 // sample. Clears the slot's pending terms.
 template <bool kDynamic>
 __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                        uint32_t* work, uint2* spill, int spill_cap,
-                                                        unsigned long long* stats) {
+                                                        uint2* spill, int spill_cap, unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
   LdsStack<kRing> st;
   st.init(ring, spill, spill_cap);
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
   if (gtid() == 0) *ray_count(counters, 0) = total;
-  uint32_t visits = 0, tests = 0;
+  __shared__ unsigned long long counts[2];
+  BlockStats<1> bs;
+  bs.init(counts);
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   walks<kDynamic>(
-      S, st, total, work,
+      S, st, total,
       [&](uint32_t i, Walk& w) {
         uint32_t rng;
         w.start(start_path(F, F.rect[0] + i % rw, F.rect[1] + i / rw, F.sample_index, rng), false);
@@ -149,8 +164,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
         const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
         const uint32_t slot = y * F.width + x;
         const bool hit = w.hit();
-        visits += w.visits;
-        tests += w.tests;
+        bs.add(0, w.visits, w.tests);
         const uint32_t k = wave_append(hit_count(counters, 0), hit);
         if (hit) {
           uint32_t rng;
@@ -171,7 +185,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
           q.restir[4ull * slot + 2] = zero;
         }
       });
-  flush_stats(stats, 0, visits, tests);
+  bs.flush(stats, 0);
 }
 
 // restir.slang:163-233 (WriteInitialReservoir) over the first hits: each
@@ -699,8 +713,8 @@ __global__ void __launch_bounds__(256) k_cache_samples(FrameConst F, Queues q, u
 // 436-448), items [n_ext, n_ext + n_shadow) the shadow rays of bounce b - 1
 // (an unblocked ray leaves its light pending).
 template <bool kDynamic>
-__global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, uint32_t* counters, uint32_t* work,
-                                                       uint2* spill, int spill_cap, uint32_t bounce, uint32_t kinds,
+__global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, uint32_t* counters, uint2* spill,
+                                                       int spill_cap, uint32_t bounce, uint32_t kinds,
                                                        unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
   LdsStack<kRing> st;
@@ -708,9 +722,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
   const uint32_t n_ext = (kinds & kTraceExtension) ? *ray_count(counters, bounce) : 0u;
   const uint32_t n_sh = (kinds & kTraceShadow) ? *shadow_count(counters, bounce - 1) : 0u;
   const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
-  uint32_t visits[2] = {0, 0}, tests[2] = {0, 0};
+  __shared__ unsigned long long counts[4];
+  BlockStats<2> bs;
+  bs.init(counts);
   walks<kDynamic>(
-      S, st, n_ext + n_sh, work,
+      S, st, n_ext + n_sh,
       [&](uint32_t i, Walk& w) {
         const bool ext = i < n_ext;
         const uint32_t j = ext ? i : i - n_ext;
@@ -723,8 +739,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
         const uint32_t j = ext ? i : i - n_ext;
         const uint32_t slot = __float_as_uint(ext ? q.ray_o[j].w : q.sh_o[j].w);
         const bool hit = w.hit();
-        visits[ext ? 0 : 1] += w.visits;
-        tests[ext ? 0 : 1] += w.tests;
+        bs.add(ext ? 0 : 1, w.visits, w.tests);
         const uint32_t k = wave_append(hit_count(counters, bounce), ext && hit);
         if (ext && hit) {
           push_hit(q, k, slot, w.r.o, w.r.d, w.record(S.tris));
@@ -739,8 +754,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
           q.nee[slot] = make_float4(c4.x, c4.y, c4.z, 1.0f);
         }
       });
-  flush_stats(stats, 1, visits[0], tests[0]);
-  flush_stats(stats, 2, visits[1], tests[1]);
+  bs.flush(stats, 1);
 }
 
 // wavefront.slang:287-309: the paths still queued after the recorded rounds,
@@ -860,13 +874,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameC
 
 // Trace / ShadowTransmittance over caller rays (8 floats in, 5 words or 1 out).
 __global__ void __launch_bounds__(kTraceBlock) k_rays(SceneView S, const float4* rays, uint32_t* hits,
-                                                      uint32_t count, uint32_t any_hit, uint32_t* work, uint2* spill,
-                                                      int spill_cap) {
+                                                      uint32_t count, uint32_t any_hit, uint2* spill, int spill_cap) {
   __shared__ uint2 ring[kRing * kTraceBlock];
   LdsStack<kRing> st;
   st.init(ring, spill, spill_cap);
-  persistent_walks(
-      S, st, count, work,
+  strided_walks(
+      S, st, count,
       [&](uint32_t i, Walk& w) {
         const float4 o4 = rays[2 * i], d4 = rays[2 * i + 1];
         w.start(Ray{xyz(o4), o4.w, xyz(d4), d4.w}, any_hit != 0);
@@ -939,9 +952,9 @@ void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s) { k_fill4<<<sm
 void launch_fill1(float* p, uint32_t n, float v, hipStream_t s) { k_fill1<<<small_grid(n), 256, 0, s>>>(p, n, v); }
 void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
   if (L.dynamic_camera)
-    k_camera<true><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.work, L.spill, L.spill_cap, L.stats);
+    k_camera<true><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
   else
-    k_camera<false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.work, L.spill, L.spill_cap, L.stats);
+    k_camera<false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce) {
@@ -950,10 +963,10 @@ void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, cons
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds) {
   if (L.dynamic_trace)
-    k_trace<true><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.work, L.spill, L.spill_cap, bounce, kinds,
+    k_trace<true><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
                                                                 L.stats);
   else
-    k_trace<false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.work, L.spill, L.spill_cap, bounce, kinds,
+    k_trace<false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
                                                                  L.stats);
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
@@ -988,7 +1001,7 @@ void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q,
 void launch_rays(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,
                  uint32_t any_hit) {
   k_rays<<<L.grid.rays, kTraceBlock, 0, L.stream>>>(S, reinterpret_cast<const float4*>(rays), hits, count, any_hit,
-                                                     L.work, L.spill, L.spill_cap);
+                                                     L.spill, L.spill_cap);
 }
 
 }  // namespace lotus

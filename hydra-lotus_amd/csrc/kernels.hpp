@@ -60,8 +60,7 @@ struct Launch {
   uint2* spill;               // traversal stack spill, max_trace_lanes lanes
   int spill_cap;              // spill entries per lane
   unsigned long long* stats;  // node visits, triangle tests: camera, extension, shadow, tail
-  uint32_t* work;             // this launch's work counter (zero before it)
-  bool dynamic_camera;        // lanes refill from the queue (else one ray per lane per pass)
+  bool dynamic_camera;        // lanes hand over finished rays (else one ray per lane per pass)
   bool dynamic_trace;
 };
 

@@ -84,25 +84,30 @@ bool invert(const float* m, float* inverse) {
 }
 
 // The rounds after a sample whose ray queues held `counts` paths per bounce
-// (the rule of vulkan_backend.cpp:369-385 with its threshold lowered): every
-// bounce up to the last that traced a ray, plus one, runs as a wavefront
-// round. A round here costs two launches of about 10 us, while the megakernel
-// tail runs its paths' bounces serially at low occupancy, so only paths longer
-// than any of the previous sample's, and than kMaxRounds, go to the tail.
-uint32_t wavefront_rounds(const std::vector<uint32_t>& counts) {
+// (vulkan_backend.cpp:369-385): every bounce before the first whose queue
+// held fewer than 1/divisor of the camera rays, and at least the camera
+// rays' bounce, runs as a wavefront round; the megakernel tail follows the
+// rest. A round's traversal launch lasts as long as its slowest ray, however
+// few rays it holds (on the heightfield, grazing shadow rays walk hundreds of
+// nodes), while the tail pays each path's slowest walks only once. Divisor 0:
+// every bounce up to the last that traced a ray, plus one.
+uint32_t wavefront_rounds(const std::vector<uint32_t>& counts, uint32_t divisor) {
   uint32_t rounds = 1;
-  for (uint32_t b = 1; b < counts.size(); ++b)
-    if (counts[b] != 0) rounds = b + 1;
-  return rounds + 1;
+  if (divisor == 0) {
+    for (uint32_t b = 1; b < counts.size(); ++b)
+      if (counts[b] != 0) rounds = b + 1;
+    return rounds + 1;
+  }
+  const uint32_t camera = counts.empty() ? 0u : counts.front();
+  const uint32_t threshold = std::max(1u, static_cast<uint32_t>((uint64_t{camera} + divisor - 1) / divisor));
+  while (rounds < counts.size() && counts[rounds] >= threshold) ++rounds;
+  return rounds;
 }
 
 // Rounds of a sample with no previous one to go by, and the most a sample
 // records (the tail follows longer paths; the event pool holds their timers).
 constexpr uint32_t kFirstRounds = 32;
 constexpr uint32_t kMaxRounds = 64;
-// Work counters of a frame's persistent launches: the camera kernel and at
-// most two trace launches per round.
-constexpr uint32_t kWorkCounters = 2 * kMaxRounds + 8;
 
 struct AccumulationKey {  // vulkan_backend.cpp:419-434
   std::array<float, 16> world_to_clip{};
@@ -161,12 +166,11 @@ struct lotus_hip_renderer {
   uint32_t* counters = nullptr;
   uint32_t counter_words = 0;
   unsigned long long* stats = nullptr;  // 8: node visits / triangle tests per traversal kernel
-  uint32_t* work = nullptr;              // kWorkCounters: one per persistent launch of a frame
   float* depth = nullptr;
   uint2* spill = nullptr;
   size_t spill_entries = 0;  // per lane
   GridShape grid{};
-  lotus_hip_tuning tuning{0u, 0u};
+  lotus_hip_tuning tuning{0u, 0u, 16u};
 
   AccumulationKey key;
   bool key_valid = false;
@@ -270,7 +274,7 @@ hipError_t ensure_launch(lotus_hip_renderer* r) {
 }
 
 Launch launch_for(lotus_hip_renderer* r, hipStream_t stream) {
-  return Launch{stream, r->grid, r->spill, static_cast<int>(r->spill_entries), r->stats, r->work,
+  return Launch{stream, r->grid, r->spill, static_cast<int>(r->spill_entries), r->stats,
                 r->tuning.camera_dynamic != 0, r->tuning.trace_dynamic != 0};
 }
 
@@ -564,7 +568,6 @@ lotus_hip_renderer* lotus_hip_create(int device, int* status, char* detail, size
   }
   if (e == hipSuccess) e = dev_alloc(r->counters, 4ull * (LOTUS_HIP_MAX_BOUNCES + 4));
   if (e == hipSuccess) e = dev_alloc(r->stats, 8);
-  if (e == hipSuccess) e = dev_alloc(r->work, kWorkCounters);
   r->counter_words = 4u * (LOTUS_HIP_MAX_BOUNCES + 4);
   if (e != hipSuccess) {
     if (status) *status = LOTUS_HIP_FAIL;
@@ -588,7 +591,6 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   dev_free(r->svgf_filter);
   dev_free(r->counters);
   dev_free(r->stats);
-  dev_free(r->work);
   dev_free(r->spill);
   for (auto& ev : r->events)
     if (ev) (void)hipEventDestroy(ev);
@@ -911,16 +913,13 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
         if (!deferred) (void)hipEventRecord(r->events[ev++], r->stream);
       };
       launch_zero(r->counters, frame_counter_words(stride), r->stream);
-      launch_zero(r->work, kWorkCounters, r->stream);
       if (cache_frame) {
         launch_zero(r->q.crec_n, r->train_capacity, r->stream);
         launch_fill1(r->cache.loss, 1, 0.0f, r->stream);
       }
-      uint32_t next_work = 0;  // each persistent launch takes its own work counter
       if (add) {
         if (tf && tf->reuse) launch_restir_clear(L, r->q.restir, npix);
         mark(kTCamera);
-        L.work = r->work + next_work++;
         launch_camera(L, r->view, F, r->q, r->counters);
         done();
         if (restir) {
@@ -949,13 +948,11 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
             for (const uint32_t kind : {kTraceExtension, kTraceShadow}) {
               if (!(kinds & kind)) continue;
               mark(kind == kTraceExtension ? kTIntersect : kTShadow);
-              L.work = r->work + next_work++;
               launch_trace(L, r->view, r->q, r->counters, b + 1, kind);
               done();
             }
           } else {
             mark(kTTrace);
-            L.work = r->work + next_work++;
             launch_trace(L, r->view, r->q, r->counters, b + 1, kinds);
             done();
           }
@@ -1034,7 +1031,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
         camera_rays += host[0];
         shadow_rays += *restir_ray_count(host.data());
         counts.resize(host[0] ? last + 1 : 0);
-        r->rounds_hint = wavefront_rounds(counts);
+        r->rounds_hint = wavefront_rounds(counts, r->tuning.tail_divisor);
         rounds_used = rounds;
         ++r->accumulated;
         ++added;
@@ -1138,7 +1135,6 @@ int lotus_hip_trace_device(lotus_hip_renderer* r, uint32_t count, const void* ra
   }
   (void)hipSetDevice(r->device);
   Launch L = launch_for(r, stream ? static_cast<hipStream_t>(stream) : r->stream);
-  launch_zero(r->work, 1, L.stream);
   launch_rays(L, r->view, static_cast<const float*>(rays), static_cast<uint32_t*>(hits), count,
                any_hit ? 1u : 0u);
   const hipError_t e = hipGetLastError();
@@ -1186,6 +1182,7 @@ int lotus_hip_set_tuning(lotus_hip_renderer* r, const lotus_hip_tuning* tuning, 
     return LOTUS_HIP_FAIL;
   }
   r->tuning = *tuning;
+  r->rounds_hint = 0;  // the next sample sizes the rounds under the new rule
   set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;
 }
@@ -1641,7 +1638,7 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
       sh += host[4 * b + 2];
     }
     counts.resize(host[0] ? last + 1 : 0);
-    r->rounds_hint = wavefront_rounds(counts);
+    r->rounds_hint = wavefront_rounds(counts, r->tuning.tail_divisor);
     ++r->accumulated;
   }
   if (res) {
@@ -1750,7 +1747,8 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
 int lotus_hip_autotune(lotus_hip_renderer* r, const lotus_hip_frame_params* params, uint32_t trials,
                        lotus_hip_tuning* chosen, char* detail, size_t detail_size) {
   if (!r || !params) return LOTUS_HIP_FAIL;
-  const lotus_hip_tuning candidates[4] = {{0u, 0u}, {1u, 1u}, {0u, 1u}, {1u, 0u}};
+  const lotus_hip_tuning candidates[8] = {{0u, 0u, 16u}, {1u, 1u, 16u}, {0u, 1u, 16u}, {1u, 0u, 16u},
+                                          {0u, 0u, 0u},  {1u, 1u, 0u},  {0u, 1u, 0u},  {1u, 0u, 0u}};
   double best = 1e300;
   lotus_hip_tuning pick = r->tuning;
   lotus_hip_frame_params P = *params;

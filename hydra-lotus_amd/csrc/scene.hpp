@@ -177,20 +177,35 @@ struct LdsStack {
   }
 };
 
-// The slab test of one node's four children against [tmin, tmax].
-__device__ __forceinline__ void test_children(const WideNode* __restrict__ nodes, uint32_t index,
-                                              V3 o, V3 inv, float tmin, float tmax, float tn[4],
-                                              uint32_t child[4]) {
+// A wide node's words as loaded: the six box planes (axis-major) and the
+// four child words.
+struct NodeWords {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  uint4 ch;
+};
+__device__ __forceinline__ NodeWords load_node(const WideNode* __restrict__ nodes, uint32_t index) {
   const float4* p = reinterpret_cast<const float4*>(nodes + index);
-  const float4 lox = p[0], hix = p[1], loy = p[2], hiy = p[3], loz = p[4], hiz = p[5];
-  const uint4 ch = reinterpret_cast<const uint4*>(p)[6];
-  const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-  const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-  const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-  child[0] = ch.x;
-  child[1] = ch.y;
-  child[2] = ch.z;
-  child[3] = ch.w;
+  NodeWords n;
+  n.lox = p[0];
+  n.hix = p[1];
+  n.loy = p[2];
+  n.hiy = p[3];
+  n.loz = p[4];
+  n.hiz = p[5];
+  n.ch = reinterpret_cast<const uint4*>(p)[6];
+  return n;
+}
+
+// The slab test of one node's four children against [tmin, tmax].
+__device__ __forceinline__ void test_children(const NodeWords& n, V3 o, V3 inv, float tmin, float tmax,
+                                              float tn[4], uint32_t child[4]) {
+  const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w}, hx[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
+  const float ly[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w}, hy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w};
+  const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w}, hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
+  child[0] = n.ch.x;
+  child[1] = n.ch.y;
+  child[2] = n.ch.z;
+  child[3] = n.ch.w;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     float x0 = (lx[c] - o.x) * inv.x, x1 = (hx[c] - o.x) * inv.x;
@@ -201,6 +216,11 @@ __device__ __forceinline__ void test_children(const WideNode* __restrict__ nodes
                       1.0000003576278687f;
     tn[c] = child[c] != kEmptyChild && near <= fminf(far, tmax) ? near : INFINITY;
   }
+}
+__device__ __forceinline__ void test_children(const WideNode* __restrict__ nodes, uint32_t index,
+                                              V3 o, V3 inv, float tmin, float tmax, float tn[4],
+                                              uint32_t child[4]) {
+  test_children(load_node(nodes, index), o, inv, tmin, tmax, tn, child);
 }
 
 // Orders up to four (distance, child) pairs ascending; misses are INFINITY.
@@ -269,31 +289,28 @@ struct Walk {
     cur = kWalkDone;
   }
 
+  // One inner node: its children nearest first, the nearest taken now and
+  // the others pushed; with none hit, the next entry.
   template <int S>
-  __device__ __forceinline__ void step(const WideNode* __restrict__ nodes, const Tri* __restrict__ tris,
-                                       LdsStack<S>& st) {
-    if (++steps >= kMaxSteps) {
-      st.clear();
-      cur = kWalkDone;
+  __device__ __forceinline__ void visit(const WideNode* __restrict__ nodes, LdsStack<S>& st) {
+    float tn[4];
+    uint32_t ch[4];
+    test_children(nodes, cur, r.o, inv, r.tmin, best_t, tn, ch);
+    ++visits;
+    sort4(tn, ch);
+    if (tn[0] != INFINITY) {
+      if (tn[3] != INFINITY) st.push(ch[3], tn[3]);
+      if (tn[2] != INFINITY) st.push(ch[2], tn[2]);
+      if (tn[1] != INFINITY) st.push(ch[1], tn[1]);
+      cur = ch[0];
       return;
     }
-    if ((cur & kLeafBit) == 0) {
-      float tn[4];
-      uint32_t ch[4];
-      test_children(nodes, cur, r.o, inv, r.tmin, best_t, tn, ch);
-      ++visits;
-      sort4(tn, ch);
-      if (tn[0] != INFINITY) {
-        if (tn[3] != INFINITY) st.push(ch[3], tn[3]);
-        if (tn[2] != INFINITY) st.push(ch[2], tn[2]);
-        if (tn[1] != INFINITY) st.push(ch[1], tn[1]);
-        cur = ch[0];
-        return;
-      }
-      next(st);
-      return;
-    }
-    const uint32_t first = leaf_first(cur), n = leaf_count(cur);
+    next(st);
+  }
+
+  // The triangles of leaf `word`; true when an any-hit walk has found its hit.
+  __device__ __forceinline__ bool test_leaf(const Tri* __restrict__ tris, uint32_t word) {
+    const uint32_t first = leaf_first(word), n = leaf_count(word);
     for (uint32_t k = 0; k < n; ++k) {
       const Tri t4 = tris[first + k];
       float t, u, v;
@@ -304,9 +321,7 @@ struct Walk {
           best_g = g;
           best_tri = first + k;
           best_t = t;
-          st.clear();
-          cur = kWalkDone;
-          return;
+          return true;
         }
         if (t < best_t || (t == best_t && g < best_g)) {
           best_t = t;
@@ -317,7 +332,62 @@ struct Walk {
         }
       }
     }
+    return false;
+  }
+
+  // One node or one leaf (resumable: the drivers that hand lanes new rays
+  // step each walk separately).
+  template <int S>
+  __device__ __forceinline__ void step(const WideNode* __restrict__ nodes, const Tri* __restrict__ tris,
+                                       LdsStack<S>& st) {
+    if (++steps >= kMaxSteps) {
+      st.clear();
+      cur = kWalkDone;
+      return;
+    }
+    if ((cur & kLeafBit) == 0) {
+      visit(nodes, st);
+      return;
+    }
+    if (test_leaf(tris, cur)) {
+      st.clear();
+      cur = kWalkDone;
+      return;
+    }
     next(st);
+  }
+
+  // The walk to its end, the wave's lanes together. A step of the wave
+  // visits inner nodes or tests leaves, not both: lanes that reach a leaf
+  // wait until as many lanes wait at leaves as walk inner nodes, then the
+  // waiting lanes test their leaves together, so the leaf code runs for many
+  // lanes at once instead of in every step (one of the two paths of a step
+  // otherwise runs for a few lanes while the rest of the wave idles). Results
+  // do not depend on the order of steps.
+  template <int S>
+  __device__ __forceinline__ void run(const WideNode* __restrict__ nodes, const Tri* __restrict__ tris,
+                                      LdsStack<S>& st) {
+    while (true) {
+      const bool live = cur != kWalkDone;
+      const bool at_leaf = live && (cur & kLeafBit) != 0;
+      const uint64_t leaves = __ballot(at_leaf), inners = __ballot(live && !at_leaf);
+      if ((leaves | inners) == 0) break;
+      if (++steps >= kMaxSteps) {
+        st.clear();
+        cur = kWalkDone;
+        continue;
+      }
+      if (inners != 0 && __popcll(leaves) < __popcll(inners)) {
+        if (live && !at_leaf) visit(nodes, st);
+      } else if (at_leaf) {
+        if (test_leaf(tris, cur)) {
+          st.clear();
+          cur = kWalkDone;
+        } else {
+          next(st);
+        }
+      }
+    }
   }
 
   // The closest hit's record (call when done() and hit()).
@@ -341,7 +411,7 @@ __device__ __forceinline__ bool walk_closest(const WideNode* nodes, const Tri* t
   Walk w;
   w.start(r, false);
   st.clear();
-  while (!w.done()) w.step(nodes, tris, st);
+  w.run(nodes, tris, st);
   visits += w.visits;
   tests += w.tests;
   if (!w.hit()) return false;
@@ -354,7 +424,7 @@ __device__ __forceinline__ bool walk_occluded(const WideNode* nodes, const Tri* 
   Walk w;
   w.start(r, true);
   st.clear();
-  while (!w.done()) w.step(nodes, tris, st);
+  w.run(nodes, tris, st);
   visits += w.visits;
   tests += w.tests;
   return w.hit();

@@ -84,7 +84,7 @@ class SceneEvidence(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("camera_dynamic", C.c_uint32), ("trace_dynamic", C.c_uint32)]
+    _fields_ = [("camera_dynamic", C.c_uint32), ("trace_dynamic", C.c_uint32), ("tail_divisor", C.c_uint32)]
 
 
 class FrameParams(C.Structure):
@@ -453,7 +453,8 @@ class Renderer:
     def tuning(self) -> dict:
         t = Tuning()
         lib().lotus_hip_get_tuning(self.handle, C.byref(t))
-        return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic}
+        return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic,
+                "tail_divisor": t.tail_divisor}
 
     def set_tuning(self, **kw):
         t = Tuning()
@@ -591,11 +592,12 @@ class Renderer:
             raise LotusError(s, buf.value.decode())
 
     def autotune(self, trials=2, **view) -> dict:
-        """Times each traversal driver choice on `view` and keeps the fastest."""
+        """Times each traversal driver and tail rule choice on `view` and keeps the fastest."""
         fp = frame_params(**view)
         t = Tuning()
         buf = C.create_string_buffer(256)
         s = lib().lotus_hip_autotune(self.handle, C.byref(fp), trials, C.byref(t), buf, 256)
         if s != PASS:
             raise LotusError(s, buf.value.decode())
-        return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic}
+        return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic,
+                "tail_divisor": t.tail_divisor}

@@ -209,8 +209,13 @@ int lotus_hip_trace_device(lotus_hip_renderer* renderer, uint32_t count,
  * reference's driver schedules its own dispatches). Results never depend on
  * them. */
 typedef struct lotus_hip_tuning {
-  uint32_t camera_dynamic; /* 1: persistent lanes refill from the queue; 0: one ray per lane per pass */
+  uint32_t camera_dynamic; /* 1: a lane whose ray ends takes its next one without waiting for its wave; 0: the
+                              wave's lanes take their next rays together */
   uint32_t trace_dynamic;  /* the same for extension and shadow rays */
+  uint32_t tail_divisor;   /* bounces whose ray queue held at least 1/tail_divisor of the camera rays in the
+                              previous sample run as wavefront rounds, the megakernel tail follows the rest
+                              (vulkan_backend.cpp:369-385, whose divisor is 16, the default); 0: every bounce
+                              that traced a ray runs as a round */
 } lotus_hip_tuning;
 
 int lotus_hip_set_tuning(lotus_hip_renderer* renderer, const lotus_hip_tuning* tuning, char* detail,

@@ -49,12 +49,14 @@ def test_struct_sizes_match_header():
 #include <stddef.h>
 #include "lotus_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lotus_hip_scene_plan), sizeof(lotus_hip_scene_evidence),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lotus_hip_scene_plan),
+         sizeof(lotus_hip_scene_evidence),
          sizeof(lotus_hip_frame_params), sizeof(lotus_hip_frame_result),
          offsetof(lotus_hip_frame_result, traversal_nodes), offsetof(lotus_hip_scene_plan, bvh_leaf_size),
          sizeof(lotus_hip_cache_config), offsetof(lotus_hip_cache_config, resolution),
          offsetof(lotus_hip_frame_result, cache_loss), sizeof(lotus_hip_temporal_settings),
-         offsetof(lotus_hip_temporal_settings, reset_reason));
+         offsetof(lotus_hip_temporal_settings, reset_reason), sizeof(lotus_hip_tuning),
+         offsetof(lotus_hip_tuning, tail_divisor));
   return 0;
 }'''
     import tempfile
@@ -75,4 +77,6 @@ int main(void) {
     assert sizes[8] == hip.FrameResult.cache_loss.offset
     assert sizes[9] == C.sizeof(hip.TemporalSettings)
     assert sizes[10] == hip.TemporalSettings.reset_reason.offset
+    assert sizes[11] == C.sizeof(hip.Tuning)
+    assert sizes[12] == hip.Tuning.tail_divisor.offset
     assert hip.TEMPORAL_PIXEL.itemsize == 64 and hip.TEMPORAL_RADIANCE.itemsize == 32
