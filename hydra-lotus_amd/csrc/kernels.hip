@@ -126,11 +126,9 @@ __device__ __forceinline__ void walks(const SceneView& S, LdsStack<kRing>& st, u
   else strided_walks(S, st, total, fetch, finish);
 }
 
-// Counters per bounce b: [4b] ray-queue entries (camera rays at b = 0),
-// [4b+1] hit-queue entries, [4b+2] shadow-queue entries.
-__device__ __forceinline__ uint32_t* ray_count(uint32_t* c, uint32_t b) { return c + 4 * b; }
-__device__ __forceinline__ uint32_t* hit_count(uint32_t* c, uint32_t b) { return c + 4 * b + 1; }
-__device__ __forceinline__ uint32_t* shadow_count(uint32_t* c, uint32_t b) { return c + 4 * b + 2; }
+__device__ __forceinline__ uint32_t* ray_count(uint32_t* c, uint32_t b) { return c + counter_word(b, 0); }
+__device__ __forceinline__ uint32_t* hit_count(uint32_t* c, uint32_t b) { return c + counter_word(b, 1); }
+__device__ __forceinline__ uint32_t* shadow_count(uint32_t* c, uint32_t b) { return c + counter_word(b, 2); }
 
 __device__ __forceinline__ void push_hit(const Queues& q, uint32_t k, uint32_t slot, V3 o, V3 d,
                                          const HitRecord& h) {

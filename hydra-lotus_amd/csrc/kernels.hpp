@@ -111,16 +111,28 @@ void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameCons
 void launch_restir_spatial(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                            uint32_t* c);
 // Radiance cache frames: targets (L - R) / T of every recorded vertex of the
-// training paths, appended to q.samples (counter word 7), at most `capacity`.
+// training paths, appended to q.samples (cache_sample_count), at most `capacity`.
 void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q, uint32_t* c, uint32_t capacity);
-// Cache query and sample counters.
-__host__ __device__ inline uint32_t* cache_query_count(uint32_t* c) { return c + 3; }
-__host__ __device__ inline uint32_t* cache_sample_count(uint32_t* c) { return c + 7; }
+// A frame's counters: per bounce b, kind 0 the ray-queue entries (camera rays
+// at b = 0), kind 1 the hit-queue entries, kind 2 the shadow-queue entries;
+// kind 3 of bounces 0, 1, 2 counts cache queries, cache samples and ReSTIR
+// visibility rays. Every counter has a 256-byte line of its own: the queues
+// are appended with one atomic per wave, and counters sharing a line would
+// queue behind one another at one atomic unit.
+#ifndef LOTUS_COUNTER_SPREAD
+#define LOTUS_COUNTER_SPREAD 64u
+#endif
+constexpr uint32_t kCounterSpread = LOTUS_COUNTER_SPREAD;
+__host__ __device__ constexpr uint32_t counter_word(uint32_t bounce, uint32_t kind) {
+  return (4u * bounce + kind) * kCounterSpread;
+}
+__host__ __device__ inline uint32_t* cache_query_count(uint32_t* c) { return c + counter_word(0, 3); }
+__host__ __device__ inline uint32_t* cache_sample_count(uint32_t* c) { return c + counter_word(1, 3); }
 // ReSTIR visibility rays traced inline by the reservoir passes (not queued).
-__host__ __device__ inline uint32_t* restir_ray_count(uint32_t* c) { return c + 11; }
-// Counter words a frame clears: 4 per bounce, at least the three above.
+__host__ __device__ inline uint32_t* restir_ray_count(uint32_t* c) { return c + counter_word(2, 3); }
+// Counter words a frame clears: 4 counters per bounce, at least the three above.
 __host__ __device__ inline uint32_t frame_counter_words(uint32_t bounce_stride) {
-  return 4u * bounce_stride > 12u ? 4u * bounce_stride : 12u;
+  return counter_word(bounce_stride > 3u ? bounce_stride : 3u, 0);
 }
 
 }  // namespace lotus

@@ -164,7 +164,6 @@ struct lotus_hip_renderer {
   uint32_t width = 0, height = 0;
   Queues q{};
   uint32_t* counters = nullptr;
-  uint32_t counter_words = 0;
   unsigned long long* stats = nullptr;  // 8: node visits / triangle tests per traversal kernel
   float* depth = nullptr;
   uint2* spill = nullptr;
@@ -566,9 +565,8 @@ lotus_hip_renderer* lotus_hip_create(int device, int* status, char* detail, size
     for (auto& ev : r->events)
       if (e == hipSuccess) e = hipEventCreate(&ev);
   }
-  if (e == hipSuccess) e = dev_alloc(r->counters, 4ull * (LOTUS_HIP_MAX_BOUNCES + 4));
+  if (e == hipSuccess) e = dev_alloc(r->counters, frame_counter_words(LOTUS_HIP_MAX_BOUNCES + 4));
   if (e == hipSuccess) e = dev_alloc(r->stats, 8);
-  r->counter_words = 4u * (LOTUS_HIP_MAX_BOUNCES + 4);
   if (e != hipSuccess) {
     if (status) *status = LOTUS_HIP_FAIL;
     set_detail(detail, detail_size, std::string("device setup: ") + hipGetErrorString(e));
@@ -1011,22 +1009,22 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
         frame_ms += ms;
       }
       if (cache_frame && add) {
-        const uint32_t total = std::min(host[7], r->sample_capacity);
+        const uint32_t total = std::min(host[counter_word(1, 3)], r->sample_capacity);
         const uint32_t steps = std::max(1u, r->cache_config.train_steps);
         for (uint32_t k = 0; k < steps; ++k)
           if (total > k) cache_used += std::min((total - k + steps - 1) / steps, r->cache_config.batch_capacity);
-        cache_queries += host[3];
-        cache_samples += host[7];
+        cache_queries += host[counter_word(0, 3)];
+        cache_samples += host[counter_word(1, 3)];
         cache_loss_sum += loss_host;
       }
       if (add) {
         counts.assign(stride, 0u);
         size_t last = 0;
         for (uint32_t b = 0; b < stride; ++b) {
-          counts[b] = host[4 * b];
+          counts[b] = host[counter_word(b, 0)];
           if (counts[b]) last = b;
-          if (b >= 1) extension_rays += host[4 * b];
-          shadow_rays += host[4 * b + 2];
+          if (b >= 1) extension_rays += host[counter_word(b, 0)];
+          shadow_rays += host[counter_word(b, 2)];
         }
         camera_rays += host[0];
         shadow_rays += *restir_ray_count(host.data());
@@ -1632,10 +1630,10 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
     counts.assign(d.stride, 0u);
     size_t last = 0;
     for (uint32_t b = 0; b < d.stride; ++b) {
-      counts[b] = host[4 * b];
+      counts[b] = host[counter_word(b, 0)];
       if (counts[b]) last = b;
-      if (b >= 1) ext += host[4 * b];
-      sh += host[4 * b + 2];
+      if (b >= 1) ext += host[counter_word(b, 0)];
+      sh += host[counter_word(b, 2)];
     }
     counts.resize(host[0] ? last + 1 : 0);
     r->rounds_hint = wavefront_rounds(counts, r->tuning.tail_divisor);
