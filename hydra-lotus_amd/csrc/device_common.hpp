@@ -125,4 +125,20 @@ __device__ __forceinline__ void wave_count(uint32_t* counter, bool active) {
     atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
 }
 
+// Increments of counters that may differ between lanes: one atomic per
+// distinct counter among the active lanes (a few per wave where lanes are at
+// different bounces), instead of one per lane on a handful of addresses.
+__device__ __forceinline__ void wave_count_each(uint32_t* counter, bool active) {
+  uint64_t todo = __ballot(active);
+  const uint32_t lane = __lane_id();
+  const unsigned long long mine = reinterpret_cast<unsigned long long>(counter);
+  while (todo != 0) {
+    const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+    const unsigned long long theirs = __shfl(mine, leader);
+    const uint64_t same = __ballot(active && mine == theirs) & todo;
+    if (lane == static_cast<uint32_t>(leader)) atomicAdd(counter, static_cast<uint32_t>(__popcll(same)));
+    todo &= ~same;
+  }
+}
+
 }  // namespace lotus

@@ -789,13 +789,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
         break;
       }
       const Scatter s = scatter_path(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v);
-      if (s.shadow) {
-        atomicAdd(shadow_count(counters, p.depth - (s.continues ? 1u : 0u)), 1u);
-        if (!walk_occluded(S.nodes, S.tris, st, s.shadow_ray, visits, tests))
-          p.radiance = p.radiance + s.delta;
-      }
+      wave_count_each(shadow_count(counters, p.depth - (s.continues ? 1u : 0u)), s.shadow);
+      if (s.shadow && !walk_occluded(S.nodes, S.tris, st, s.shadow_ray, visits, tests))
+        p.radiance = p.radiance + s.delta;
+      wave_count_each(ray_count(counters, p.depth), s.continues);
       if (!s.continues) break;
-      atomicAdd(ray_count(counters, p.depth), 1u);
     }
     q.rad[slot] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 1.0f);
   }
