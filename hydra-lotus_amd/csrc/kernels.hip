@@ -70,7 +70,7 @@ __device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
 // finish(item, walk) together and start their next items. No atomics: the
 // sequences are fixed, so neighbouring lanes keep neighbouring items.
 template <class Fetch, class Finish>
-__device__ __forceinline__ void sequence_walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total,
+__device__ __forceinline__ void sequence_walks(const SceneView& S, TraceStack& st, uint32_t total,
                                                Fetch fetch, Finish finish) {
   Walk w;
   bool working = false;  // a walk in progress
@@ -108,7 +108,7 @@ __device__ __forceinline__ void sequence_walks(const SceneView& S, LdsStack<kRin
 // The same walks without hand-over: each lane follows one ray to its end,
 // striding over the queue.
 template <class Fetch, class Finish>
-__device__ __forceinline__ void strided_walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total, Fetch fetch,
+__device__ __forceinline__ void strided_walks(const SceneView& S, TraceStack& st, uint32_t total, Fetch fetch,
                                               Finish finish) {
   for (uint32_t i = gtid(); i < total; i += gstride()) {
     Walk w;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void strided_walks(const SceneView& S, LdsStack<kRing
 }
 
 template <bool kDynamic, class Fetch, class Finish>
-__device__ __forceinline__ void walks(const SceneView& S, LdsStack<kRing>& st, uint32_t total, Fetch fetch,
+__device__ __forceinline__ void walks(const SceneView& S, TraceStack& st, uint32_t total, Fetch fetch,
                                       Finish finish) {
   if constexpr (kDynamic) sequence_walks(S, st, total, fetch, finish);
   else strided_walks(S, st, total, fetch, finish);
@@ -144,7 +144,7 @@ template <bool kDynamic>
 __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                         uint2* spill, int spill_cap, unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
   if (gtid() == 0) *ray_count(counters, 0) = total;
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
                                                                 uint32_t* counters, uint2* spill, int spill_cap,
                                                                 unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t count = *hit_count(counters, 0);
   uint32_t visits = 0, tests = 0;
@@ -379,8 +379,7 @@ __device__ __forceinline__ float history_target(const SceneView& S, const ReuseR
 // The final estimate at the receiver (restir.slang:289-300, 372-385):
 // visibility reused from the same pixel's record when it selected the same
 // sample, else traced.
-template <int R>
-__device__ __forceinline__ void resolve_estimate(const SceneView& S, LdsStack<R>& st, const ReuseSurface& receiver,
+__device__ __forceinline__ void resolve_estimate(const SceneView& S, TraceStack& st, const ReuseSurface& receiver,
                                                  ReuseRecord& result, const ReuseRecord& own, uint32_t& rays,
                                                  uint32_t& visits, uint32_t& tests) {
   if ((result.flags & 2u) != 0u || !(result.r.weight_sum > 0.0f)) return;
@@ -407,7 +406,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, Fr
                                                                  uint32_t* counters, uint2* spill, int spill_cap,
                                                                  unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t pixels = F.width * F.height;
   uint32_t visits = 0, tests = 0;
@@ -488,7 +487,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
                                                                 uint32_t* counters, uint2* spill, int spill_cap,
                                                                 unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t pixels = F.width * F.height;
   const int radius = static_cast<int>(A.radius);
@@ -715,7 +714,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
                                                        int spill_cap, uint32_t bounce, uint32_t kinds,
                                                        unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t n_ext = (kinds & kTraceExtension) ? *ray_count(counters, bounce) : 0u;
   const uint32_t n_sh = (kinds & kTraceShadow) ? *shadow_count(counters, bounce - 1) : 0u;
@@ -761,7 +760,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
                                                       uint2* spill, int spill_cap, uint32_t bounce,
                                                       unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t count = *ray_count(counters, bounce);
   const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
@@ -838,7 +837,7 @@ __global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_
 __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameConst F, Queues q, float* depth,
                                                              uint32_t write_color, uint2* spill, int spill_cap) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
   uint32_t visits = 0, tests = 0;
@@ -872,7 +871,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameC
 __global__ void __launch_bounds__(kTraceBlock) k_rays(SceneView S, const float4* rays, uint32_t* hits,
                                                       uint32_t count, uint32_t any_hit, uint2* spill, int spill_cap) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   strided_walks(
       S, st, count,
@@ -930,16 +929,17 @@ uint32_t small_grid(uint32_t n) { return (n + 255) / 256 < 1024 ? (n + 255) / 25
 
 GridShape grid_shape(uint32_t cus) {
   GridShape g;
-  g.camera = std::max(persistent_blocks(k_camera<true>, kTraceBlock, cus),
-                      persistent_blocks(k_camera<false>, kTraceBlock, cus));
-  g.trace = std::max(persistent_blocks(k_trace<true>, kTraceBlock, cus),
-                     persistent_blocks(k_trace<false>, kTraceBlock, cus));
+  g.camera = persistent_blocks(k_camera<false>, kTraceBlock, cus);
+  g.camera_dynamic = persistent_blocks(k_camera<true>, kTraceBlock, cus);
+  g.trace = persistent_blocks(k_trace<false>, kTraceBlock, cus);
+  g.trace_dynamic = persistent_blocks(k_trace<true>, kTraceBlock, cus);
   g.tail = persistent_blocks(k_tail, kTraceBlock, cus);
   g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
   g.rays = persistent_blocks(k_rays, kTraceBlock, cus);
   g.shade = persistent_blocks(k_shade, kShadeBlock, cus);
   g.resolve = cus * 4;
-  g.max_trace_lanes = std::max({g.camera, g.trace, g.tail, g.camera_pass, g.rays}) * kTraceBlock;
+  g.max_trace_lanes =
+      std::max({g.camera, g.camera_dynamic, g.trace, g.trace_dynamic, g.tail, g.camera_pass, g.rays}) * kTraceBlock;
   return g;
 }
 
@@ -948,7 +948,7 @@ void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s) { k_fill4<<<sm
 void launch_fill1(float* p, uint32_t n, float v, hipStream_t s) { k_fill1<<<small_grid(n), 256, 0, s>>>(p, n, v); }
 void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
   if (L.dynamic_camera)
-    k_camera<true><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+    k_camera<true><<<L.grid.camera_dynamic, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
   else
     k_camera<false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
@@ -959,7 +959,7 @@ void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, cons
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds) {
   if (L.dynamic_trace)
-    k_trace<true><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
+    k_trace<true><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
                                                                 L.stats);
   else
     k_trace<false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,

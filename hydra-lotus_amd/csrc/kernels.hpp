@@ -14,6 +14,8 @@ constexpr int kRing = 12;
 constexpr int kRefill = 16;
 constexpr int kTraceBlock = 256;
 constexpr int kShadeBlock = 256;
+// A traversal kernel's per-lane stacks (launched with kTraceBlock lanes).
+using TraceStack = LdsStack<kRing, kTraceBlock>;
 
 // Per-target path state and queues, P = width * height entries each.
 struct Queues {
@@ -48,8 +50,11 @@ constexpr uint32_t kCacheRecordDepth = 8;
 constexpr float kCacheMinThroughput = 1e-3f;
 
 // Persistent grid of each kernel: workgroups filling every CU at its occupancy.
+// Workgroups of each persistent launch: every CU at the kernel's own
+// occupancy (a grid sized for another variant's occupancy would leave
+// workgroups waiting for a CU, and a strided launch then ends a pass late).
 struct GridShape {
-  uint32_t camera, trace, tail, camera_pass, rays, shade, resolve;
+  uint32_t camera, camera_dynamic, trace, trace_dynamic, tail, camera_pass, rays, shade, resolve;
   uint32_t max_trace_lanes;  // the most traversal lanes any grid holds
 };
 GridShape grid_shape(uint32_t cus);

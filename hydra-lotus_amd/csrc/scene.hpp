@@ -121,26 +121,30 @@ __device__ __forceinline__ float safe_inverse(float d) {
 }
 
 // Traversal stack of one lane: a ring of S (node, distance) entries in LDS,
-// laid out entry-major ([S][block lanes]) so a wave's pushes and pops touch
-// consecutive words, spilling its oldest entry to a lane-interleaved global
-// area (entry k of lane l at spill[k * lanes + l]) when the ring is full.
-template <int S>
+// laid out entry-major ([S][W lanes of the workgroup]) so a wave's pushes and
+// pops touch consecutive words, spilling its oldest entries to a
+// lane-interleaved global area (entry k of lane l at spill[k * lanes + l])
+// when the ring is full. W is the workgroup size, a constant, so an entry's
+// address is a shift and an add.
+template <int S, int W>
 struct LdsStack {
-  uint2* ring;      // this block's ring: [S][blockDim.x]
-  uint32_t slot;    // threadIdx.x
-  uint32_t stride;  // blockDim.x
-  uint2* spill;
+  // An entry is one 64-bit word, node in the low half, distance bits in the
+  // high half. Pops read the ring top with a volatile LDS load before they
+  // choose between it and the spill area: a load merged from an LDS and a
+  // global address would be a flat access that also waits for the lane's
+  // outstanding global loads.
+  typedef __attribute__((address_space(3))) unsigned long long LdsEntry;
+  LdsEntry* mine;   // this lane's column of the workgroup's ring
+  unsigned long long* spill;
   uint32_t lanes, lane;
   int capacity;     // spill entries per lane
   int base, count, spilled;
 
   __device__ __forceinline__ void init(uint2* lds, uint2* spill_area, int cap) {
-    ring = lds;
-    slot = threadIdx.x;
-    stride = blockDim.x;
-    spill = spill_area;
-    lanes = gridDim.x * blockDim.x;
-    lane = blockIdx.x * blockDim.x + threadIdx.x;
+    mine = (LdsEntry*)(lds) + threadIdx.x;  // an address space cast: lds is the workgroup's ring
+    spill = reinterpret_cast<unsigned long long*>(spill_area);
+    lanes = gridDim.x * W;
+    lane = blockIdx.x * W + threadIdx.x;
     capacity = cap;
     clear();
   }
@@ -149,30 +153,47 @@ struct LdsStack {
     count = 0;
     spilled = 0;
   }
-  __device__ __forceinline__ uint2& at(int i) { return ring[(i >= S ? i - S : i) * stride + slot]; }
-  __device__ __forceinline__ void push(uint32_t n, float d) {
-    if (count == S) {
-      // The area holds 3 x (tree depth) entries, the most a nearest-first
-      // walk of a 4-wide tree keeps; the bound keeps every store in it.
+  static __device__ __forceinline__ unsigned long long entry(uint32_t n, float d) {
+    return static_cast<unsigned long long>(n) | (static_cast<unsigned long long>(__float_as_uint(d)) << 32);
+  }
+  // Entry i of the ring (i < 2S).
+  __device__ __forceinline__ LdsEntry& at(int i) { return mine[(i >= S ? i - S : i) * W]; }
+  // Frees room for k more entries by spilling the oldest. The area holds
+  // 3 x (tree depth) entries, the most a nearest-first walk of a 4-wide tree
+  // keeps; the bound keeps every store in it.
+  __device__ __forceinline__ void reserve(int k) {
+    while (count + k > S) {
       if (spilled < capacity) spill[static_cast<size_t>(spilled++) * lanes + lane] = at(base);
       base = base + 1 == S ? 0 : base + 1;
       --count;
     }
-    at(base + count) = make_uint2(n, __float_as_uint(d));
+  }
+  __device__ __forceinline__ void push(uint32_t n, float d) {
+    reserve(1);
+    at(base + count) = entry(n, d);
     ++count;
   }
+  // Pushes children 1 .. k of a node's sorted children (k <= 3), farthest
+  // first, so child 1 is popped next.
+  __device__ __forceinline__ void push_sorted(int k, const uint32_t ch[4], const float tn[4]) {
+    reserve(k);
+    const int top = base + count + k;
+#pragma unroll
+    for (int m = 1; m <= 3; ++m)
+      if (m <= k) at(top - m) = entry(ch[m], tn[m]);
+    count += k;
+  }
   __device__ __forceinline__ bool pop(uint32_t& n, float& d) {
-    uint2 e;
+    unsigned long long e = *static_cast<volatile LdsEntry*>(&at(base + count + (count > 0 ? -1 : S - 1)));
     if (count > 0) {
       --count;
-      e = at(base + count);
     } else if (spilled > 0) {
       e = spill[static_cast<size_t>(--spilled) * lanes + lane];
     } else {
       return false;
     }
-    n = e.x;
-    d = __uint_as_float(e.y);
+    n = static_cast<uint32_t>(e);
+    d = __uint_as_float(static_cast<uint32_t>(e >> 32));
     return true;
   }
 };
@@ -276,8 +297,8 @@ struct Walk {
   __device__ __forceinline__ bool hit() const { return best_g != kMiss; }
 
   // The next entry no farther than the best hit (any entry for any hit).
-  template <int S>
-  __device__ __forceinline__ void next(LdsStack<S>& st) {
+  template <int S, int W>
+  __device__ __forceinline__ void next(LdsStack<S, W>& st) {
     uint32_t n;
     float d;
     while (st.pop(n, d)) {
@@ -291,17 +312,17 @@ struct Walk {
 
   // One inner node: its children nearest first, the nearest taken now and
   // the others pushed; with none hit, the next entry.
-  template <int S>
-  __device__ __forceinline__ void visit(const WideNode* __restrict__ nodes, LdsStack<S>& st) {
+  template <int S, int W>
+  __device__ __forceinline__ void visit(const WideNode* __restrict__ nodes, LdsStack<S, W>& st) {
     float tn[4];
     uint32_t ch[4];
     test_children(nodes, cur, r.o, inv, r.tmin, best_t, tn, ch);
     ++visits;
     sort4(tn, ch);
     if (tn[0] != INFINITY) {
-      if (tn[3] != INFINITY) st.push(ch[3], tn[3]);
-      if (tn[2] != INFINITY) st.push(ch[2], tn[2]);
-      if (tn[1] != INFINITY) st.push(ch[1], tn[1]);
+      st.push_sorted(static_cast<int>(tn[1] != INFINITY) + static_cast<int>(tn[2] != INFINITY) +
+                         static_cast<int>(tn[3] != INFINITY),
+                     ch, tn);
       cur = ch[0];
       return;
     }
@@ -337,9 +358,9 @@ struct Walk {
 
   // One node or one leaf (resumable: the drivers that hand lanes new rays
   // step each walk separately).
-  template <int S>
+  template <int S, int W>
   __device__ __forceinline__ void step(const WideNode* __restrict__ nodes, const Tri* __restrict__ tris,
-                                       LdsStack<S>& st) {
+                                       LdsStack<S, W>& st) {
     if (++steps >= kMaxSteps) {
       st.clear();
       cur = kWalkDone;
@@ -364,9 +385,9 @@ struct Walk {
   // lanes at once instead of in every step (one of the two paths of a step
   // otherwise runs for a few lanes while the rest of the wave idles). Results
   // do not depend on the order of steps.
-  template <int S>
+  template <int S, int W>
   __device__ __forceinline__ void run(const WideNode* __restrict__ nodes, const Tri* __restrict__ tris,
-                                      LdsStack<S>& st) {
+                                      LdsStack<S, W>& st) {
     while (true) {
       const bool live = cur != kWalkDone;
       const bool at_leaf = live && (cur & kLeafBit) != 0;
@@ -405,8 +426,8 @@ struct Walk {
 };
 
 // A walk followed to its end by one lane.
-template <int S>
-__device__ __forceinline__ bool walk_closest(const WideNode* nodes, const Tri* tris, LdsStack<S>& st,
+template <int S, int W>
+__device__ __forceinline__ bool walk_closest(const WideNode* nodes, const Tri* tris, LdsStack<S, W>& st,
                                              const Ray& r, HitRecord& h, uint32_t& visits, uint32_t& tests) {
   Walk w;
   w.start(r, false);
@@ -418,8 +439,8 @@ __device__ __forceinline__ bool walk_closest(const WideNode* nodes, const Tri* t
   h = w.record(tris);
   return true;
 }
-template <int S>
-__device__ __forceinline__ bool walk_occluded(const WideNode* nodes, const Tri* tris, LdsStack<S>& st,
+template <int S, int W>
+__device__ __forceinline__ bool walk_occluded(const WideNode* nodes, const Tri* tris, LdsStack<S, W>& st,
                                               const Ray& r, uint32_t& visits, uint32_t& tests) {
   Walk w;
   w.start(r, true);

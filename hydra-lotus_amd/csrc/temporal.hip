@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_temporal_surface(SceneView S, F
                                                                   TemporalInstances I, TemporalPixel* out,
                                                                   uint2* spill, int spill_cap) {
   __shared__ uint2 ring[kRing * kTraceBlock];
-  LdsStack<kRing> st;
+  TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
   uint32_t visits = 0, tests = 0;
