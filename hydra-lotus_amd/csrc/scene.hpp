@@ -60,30 +60,40 @@ struct HitRecord {
 // ----------------------------------------------- watertight intersection --
 // Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1),
 // 2013, with its double-precision fallback for zero edge functions.
+// The axis permutation is packed in one word (kx | ky << 2 | kz << 4) to
+// keep a walk's registers few.
 struct RayPre {
-  int kx, ky, kz;
+  uint32_t k;
   float sx, sy, sz;
+  __device__ __forceinline__ int kx() const { return static_cast<int>(k & 3u); }
+  __device__ __forceinline__ int ky() const { return static_cast<int>((k >> 2) & 3u); }
+  __device__ __forceinline__ int kz() const { return static_cast<int>(k >> 4); }
 };
 __device__ __forceinline__ RayPre precompute(const Ray& r) {
   RayPre p;
   const float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
-  p.kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-  p.kx = p.kz + 1 == 3 ? 0 : p.kz + 1;
-  p.ky = p.kx + 1 == 3 ? 0 : p.kx + 1;
-  const float dz = comp(r.d, p.kz);
+  const int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  const float dz = comp(r.d, kz);
   if (dz < 0.0f) {
-    const int t = p.kx;
-    p.kx = p.ky;
-    p.ky = t;
+    const int t = kx;
+    kx = ky;
+    ky = t;
   }
-  p.sx = comp(r.d, p.kx) / dz;
-  p.sy = comp(r.d, p.ky) / dz;
+  p.k = static_cast<uint32_t>(kx) | (static_cast<uint32_t>(ky) << 2) | (static_cast<uint32_t>(kz) << 4);
+  p.sx = comp(r.d, kx) / dz;
+  p.sy = comp(r.d, ky) / dz;
   p.sz = 1.0f / dz;
   return p;
 }
 
-__device__ __forceinline__ bool intersect_triangle(const Ray& r, const RayPre& p, V3 a0, V3 a1, V3 a2,
+__device__ __forceinline__ bool intersect_triangle(const Ray& r, const RayPre& pk, V3 a0, V3 a1, V3 a2,
                                                    float& t, float& bu, float& bv) {
+  const struct {
+    int kx, ky, kz;
+    float sx, sy, sz;
+  } p = {pk.kx(), pk.ky(), pk.kz(), pk.sx, pk.sy, pk.sz};
   const V3 A = a0 - r.o, B = a1 - r.o, C = a2 - r.o;
   const float Az0 = comp(A, p.kz), Bz0 = comp(B, p.kz), Cz0 = comp(C, p.kz);
   const float Ax = comp(A, p.kx) - p.sx * Az0;
@@ -273,7 +283,6 @@ struct Walk {
   uint32_t cur;
   float best_t;
   uint32_t best_g, best_tri;
-  float best_u, best_v;
   bool any;
   uint32_t steps;
   uint32_t visits;  // wide nodes tested
@@ -287,7 +296,6 @@ struct Walk {
     best_t = ray.tmax;
     best_g = kMiss;
     best_tri = kMiss;
-    best_u = best_v = 0.0f;
     any = any_hit;
     steps = 0;
     visits = 0;
@@ -348,8 +356,6 @@ struct Walk {
           best_t = t;
           best_g = g;
           best_tri = first + k;
-          best_u = u;
-          best_v = v;
         }
       }
     }
@@ -411,15 +417,17 @@ struct Walk {
     }
   }
 
-  // The closest hit's record (call when done() and hit()).
+  // The closest hit's record (call when done() and hit()); its barycentrics
+  // come from the same test again, which gives the same values.
   __device__ __forceinline__ HitRecord record(const Tri* __restrict__ tris) const {
     const Tri t4 = tris[best_tri];
     HitRecord h;
     h.instance = __float_as_uint(t4.b.w);
     h.primitive = __float_as_uint(t4.c.w);
     h.tri = best_tri;
-    h.u = best_u;
-    h.v = best_v;
+    float t = best_t;
+    h.u = h.v = 0.0f;
+    (void)intersect_triangle(r, pre, xyz(t4.a), xyz(t4.b), xyz(t4.c), t, h.u, h.v);
     h.t = best_t;
     return h;
   }
