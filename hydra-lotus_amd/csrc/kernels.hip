@@ -148,9 +148,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
   st.init(ring, spill, spill_cap);
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
   if (gtid() == 0) *ray_count(counters, 0) = total;
-  __shared__ unsigned long long counts[2];
-  BlockStats<1> bs;
-  bs.init(counts);
+  uint32_t visits = 0, tests = 0;  // registers to spare here: the camera kernel's peak is in its ray setup
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   walks<kDynamic>(
       S, st, total,
@@ -162,12 +160,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
         const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
         const uint32_t slot = y * F.width + x;
         const bool hit = w.hit();
-        bs.add(0, w.visits, w.tests);
+        visits += w.visits;
+        tests += w.tests;
         const uint32_t k = wave_append(hit_count(counters, 0), hit);
         if (hit) {
           uint32_t rng;
-          (void)start_path(F, x, y, F.sample_index, rng);  // the path's random state after its film sample
-          push_hit(q, k, slot, w.r.o, w.r.d, w.record(S.tris));
+          const Ray r = start_path(F, x, y, F.sample_index, rng);  // again: the ray, and the path's random state
+          push_hit(q, k, slot, r.o, r.d, w.record(S.tris));
           q.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng));
         } else {
           q.rad[slot] = zero;
@@ -183,7 +182,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
           q.restir[4ull * slot + 2] = zero;
         }
       });
-  bs.flush(stats, 0);
+  flush_stats(stats, 0, visits, tests);
 }
 
 // restir.slang:163-233 (WriteInitialReservoir) over the first hits: each
@@ -755,7 +754,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
 }
 
 // wavefront.slang:287-309: the paths still queued after the recorded rounds,
-// each followed to its end in one invocation, shadow rays traced inline.
+// each followed to its end in one invocation. A path has a pair of lanes:
+// the even lane holds the path and walks its extension rays, the odd lane
+// walks the shadow ray its previous bounce sent, both at once, so a bounce
+// costs the longer of its two walks rather than their sum. The direct light
+// of bounce b is added before anything bounce b + 1 adds, as in a serial
+// walk, so radiance is summed in the reference's order.
+__device__ __forceinline__ float pair_float(float v) { return __shfl_xor(v, 1); }
+
 __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                       uint2* spill, int spill_cap, uint32_t bounce,
                                                       unsigned long long* stats) {
@@ -764,37 +770,78 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
   st.init(ring, spill, spill_cap);
   const uint32_t count = *ray_count(counters, bounce);
   const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
+  const bool path_lane = (threadIdx.x & 1u) == 0;
   uint32_t visits = 0, tests = 0;
-  for (uint32_t i = gtid(); i < count; i += gstride()) {
-    const float4 o4 = q.ray_o[i], d4 = q.ray_d[i];
-    const uint32_t slot = __float_as_uint(o4.w);
-    const float4 t4 = q.thr[slot], r4 = q.rad[slot], n4 = q.nee[slot];
-    PathRegs p;
-    p.o = xyz(o4);
-    p.d = xyz(d4);
-    p.throughput = xyz(t4);
-    p.last_pdf = t4.w;
-    p.radiance = xyz(r4);
-    if (n4.w != 0.0f) {
-      p.radiance = p.radiance + xyz(n4);
-      q.nee[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    p.rng = __float_as_uint(r4.w);
+  // both lanes of a pair take the same items (the grid's workgroups hold
+  // whole pairs), so the pair enters and leaves the loops together
+  for (uint32_t i = gtid() >> 1; i < count; i += gstride() >> 1) {
+    PathRegs p{};
     p.depth = bounce;
-    while (true) {
-      HitRecord h;
-      if (!walk_closest(S.nodes, S.tris, st, Ray{p.o, 0.0f, p.d, kFloatMax}, h, visits, tests)) {
-        p.radiance = p.radiance + p.throughput * env;
-        break;
+    uint32_t slot = 0;
+    if (path_lane) {
+      const float4 o4 = q.ray_o[i], d4 = q.ray_d[i];
+      slot = __float_as_uint(o4.w);
+      const float4 t4 = q.thr[slot], r4 = q.rad[slot], n4 = q.nee[slot];
+      p.o = xyz(o4);
+      p.d = xyz(d4);
+      p.throughput = xyz(t4);
+      p.last_pdf = t4.w;
+      p.radiance = xyz(r4);
+      if (n4.w != 0.0f) {
+        p.radiance = p.radiance + xyz(n4);
+        q.nee[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
-      const Scatter s = scatter_path(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v);
-      wave_count_each(shadow_count(counters, p.depth - (s.continues ? 1u : 0u)), s.shadow);
-      if (s.shadow && !walk_occluded(S.nodes, S.tris, st, s.shadow_ray, visits, tests))
-        p.radiance = p.radiance + s.delta;
-      wave_count_each(ray_count(counters, p.depth), s.continues);
-      if (!s.continues) break;
+      p.rng = __float_as_uint(r4.w);
     }
-    q.rad[slot] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 1.0f);
+    bool extend = path_lane;  // the path lane has an extension ray to walk
+    bool shadow = false;      // a shadow ray of the previous bounce is pending
+    Ray sray{};               // that ray (sent to the odd lane)
+    V3 delta{};               // its contribution (kept in the path lane)
+    while (true) {
+      // the pair's state, as the path lane holds it
+      const bool pair_extend = __shfl_xor(static_cast<int>(extend), 1) != 0 || extend;
+      const bool pair_shadow = __shfl_xor(static_cast<int>(shadow), 1) != 0 || shadow;
+      if (!pair_extend && !pair_shadow) break;
+      // the odd lane receives the shadow ray
+      const Ray sr{V3{pair_float(sray.o.x), pair_float(sray.o.y), pair_float(sray.o.z)}, pair_float(sray.tmin),
+                   V3{pair_float(sray.d.x), pair_float(sray.d.y), pair_float(sray.d.z)}, pair_float(sray.tmax)};
+      Walk w;
+      if (path_lane) {
+        w.start(Ray{p.o, 0.0f, p.d, kFloatMax}, false);
+        if (!extend) w.cur = kWalkDone;
+      } else {
+        w.start(sr, true);
+        if (!pair_shadow) w.cur = kWalkDone;
+      }
+      st.clear();
+      w.run(S.nodes, S.tris, st);
+      visits += w.visits;
+      tests += w.tests;
+      const bool blocked = __shfl_xor(static_cast<int>(w.hit()), 1) != 0;  // the odd lane's answer
+      if (path_lane && shadow && !blocked) p.radiance = p.radiance + delta;
+      shadow = false;
+      Scatter sc{};
+      bool hit = false;
+      if (path_lane && extend) {
+        hit = w.hit();
+        if (!hit) {
+          p.radiance = p.radiance + p.throughput * env;
+        } else {
+          const HitRecord h = w.record(S.tris);
+          sc = scatter_path(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v);
+        }
+      }
+      const bool shaded = path_lane && extend && hit;
+      wave_count_each(shadow_count(counters, p.depth - (sc.continues ? 1u : 0u)), shaded && sc.shadow);
+      wave_count_each(ray_count(counters, p.depth), shaded && sc.continues);
+      if (shaded && sc.shadow) {
+        shadow = true;
+        sray = sc.shadow_ray;
+        delta = sc.delta;
+      }
+      extend = shaded && sc.continues;
+    }
+    if (path_lane) q.rad[slot] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 1.0f);
   }
   flush_stats(stats, 3, visits, tests);
 }
