@@ -35,6 +35,27 @@ __device__ __forceinline__ void flush_stats(unsigned long long* stats, int kind,
   }
 }
 
+// Traversal counts of one kind kept per lane in the LDS word just past the
+// lane's stack ring (its workgroup's ring is declared one entry deeper): no
+// atomics, and no register beyond the stack's own address. Visits count in
+// the low half, tests in the high half. Each wave adds its sums to
+// stats[2 kind], stats[2 kind + 1] at its end.
+struct LaneStats {
+  TraceStack::LdsEntry* word;
+
+  __device__ __forceinline__ void init(TraceStack& st) {
+    word = st.mine + kRing * kTraceBlock;
+    *word = 0;
+  }
+  __device__ __forceinline__ void add(uint32_t v, uint32_t t) {
+    *word += static_cast<unsigned long long>(v) | (static_cast<unsigned long long>(t) << 32);
+  }
+  __device__ __forceinline__ void flush(unsigned long long* stats, int kind) {
+    const unsigned long long w = *word;
+    flush_stats(stats, kind, static_cast<uint32_t>(w), static_cast<uint32_t>(w >> 32));
+  }
+};
+
 // Traversal counts of a workgroup, kept in LDS rather than in the walkers'
 // registers (the four per-lane sums of the combined traversal kernel cost it
 // a wave of occupancy per SIMD): a lane adds its finished walk's counts with
@@ -92,7 +113,11 @@ __device__ __forceinline__ void sequence_walks(const SceneView& S, TraceStack& s
         next += stride;
         fetch(item, w);
         st.clear();
-        working = true;
+        // an item with nothing to walk (fetch may start a finished walk)
+        // goes straight to hand-over: a step of a finished walk would read
+        // its end marker as a leaf
+        if (w.done()) holding = true;
+        else working = true;
       }
     }
     if (working) {
@@ -143,25 +168,44 @@ __device__ __forceinline__ void push_hit(const Queues& q, uint32_t k, uint32_t s
 template <bool kDynamic>
 __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                         uint2* spill, int spill_cap, unsigned long long* stats) {
-  __shared__ uint2 ring[kRing * kTraceBlock];
+  __shared__ uint2 ring[(kRing + 1) * kTraceBlock];  // + the lanes' traversal counts
   TraceStack st;
   st.init(ring, spill, spill_cap);
-  const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
-  if (gtid() == 0) *ray_count(counters, 0) = total;
-  uint32_t visits = 0, tests = 0;  // registers to spare here: the camera kernel's peak is in its ray setup
+  const uint32_t rw = F.rect[2], rh = F.rect[3];
+  if (gtid() == 0) *ray_count(counters, 0) = rw * rh;
+  // Items walk the data window in 8 x 8 pixel tiles, a wave's 64 rays one
+  // tile, so a wave's rays stay close together through the tree (and so do
+  // the hits and the next rays they queue). Items past the window's edge in
+  // its last tiles are skipped.
+  const uint32_t tiles_x = (rw + 7) / 8, tiles = tiles_x * ((rh + 7) / 8);
+  const auto pixel = [&](uint32_t i, uint32_t& x, uint32_t& y) {
+    const uint32_t tile = i >> 6, within = i & 63u;
+    x = (tile % tiles_x) * 8 + (within & 7u);
+    y = (tile / tiles_x) * 8 + (within >> 3);
+    return x < rw && y < rh;
+  };
+  LaneStats ls;
+  ls.init(st);
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   walks<kDynamic>(
-      S, st, total,
+      S, st, tiles * 64,
       [&](uint32_t i, Walk& w) {
-        uint32_t rng;
-        w.start(start_path(F, F.rect[0] + i % rw, F.rect[1] + i / rw, F.sample_index, rng), false);
+        uint32_t rng, x, y;
+        if (!pixel(i, x, y)) {
+          w.start(Ray{V3{0.0f, 0.0f, 0.0f}, 0.0f, V3{0.0f, 0.0f, 1.0f}, 0.0f}, false);
+          w.cur = kWalkDone;
+          return;
+        }
+        w.start(start_path(F, F.rect[0] + x, F.rect[1] + y, F.sample_index, rng), false);
       },
       [&](uint32_t i, const Walk& w) {
-        const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
+        uint32_t x, y;
+        if (!pixel(i, x, y)) return;
+        x += F.rect[0];
+        y += F.rect[1];
         const uint32_t slot = y * F.width + x;
         const bool hit = w.hit();
-        visits += w.visits;
-        tests += w.tests;
+        ls.add(w.visits, w.tests);
         const uint32_t k = wave_append(hit_count(counters, 0), hit);
         if (hit) {
           uint32_t rng;
@@ -182,7 +226,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
           q.restir[4ull * slot + 2] = zero;
         }
       });
-  flush_stats(stats, 0, visits, tests);
+  ls.flush(stats, 0);
 }
 
 // restir.slang:163-233 (WriteInitialReservoir) over the first hits: each

@@ -367,6 +367,7 @@ struct Walk {
   template <int S, int W>
   __device__ __forceinline__ void step(const WideNode* __restrict__ nodes, const Tri* __restrict__ tris,
                                        LdsStack<S, W>& st) {
+    if (cur == kWalkDone) return;
     if (++steps >= kMaxSteps) {
       st.clear();
       cur = kWalkDone;
