@@ -1308,6 +1308,7 @@ int lotus_hip_cache_gradients(lotus_hip_renderer* r, const void* samples_device,
   CacheState& c = r->cache;
   launch_fill1(c.loss, 1, 0.0f, r->stream);
   cache_backprop(c, static_cast<const float*>(samples_device), count, nullptr, 0, 1, r->stream);
+  if (grid_grad) cache_merge_grid_grad(c, r->stream);
   std::vector<float> wg(kCacheParams);
   float loss_sum = 0.0f;
   hipError_t e = hipGetLastError();

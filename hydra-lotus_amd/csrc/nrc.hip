@@ -40,7 +40,8 @@ __host__ __device__ constexpr int kslot(int kb, int g, int j) { return 32 * kb +
 
 struct GridArgs {
   const h2* grid;
-  float* grad;
+  float* grad;  // dense levels
+  h2* grad16;   // hashed levels
   uint32_t res[kCacheLevels];
   uint32_t dense_mask;
   uint32_t table;  // entries per level, a power of two
@@ -127,13 +128,28 @@ __device__ __forceinline__ void scatter_level(const GridArgs& G, float px, float
   bool dense;
   lane_level<I>(G, g, level, res, dense);
   const Cell c = level_cell(px, py, pz, res);
-  float* base = G.grad + 2ull * static_cast<size_t>(level) * G.table;
+  if (dense) {
+    // a dense cell's few entries gather thousands of samples: FP32 sums
+    float* base = G.grad + 2ull * static_cast<size_t>(level) * G.table;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float w = corner_weight(c, k);
+      const uint32_t e = corner_index(c.x + (k & 1), c.y + ((k >> 1) & 1), c.z + ((k >> 2) & 1), res, true, G.table);
+      unsafeAtomicAdd(base + 2 * e, w * d0);
+      unsafeAtomicAdd(base + 2 * e + 1, w * d1);
+    }
+    return;
+  }
+  // a hashed entry gathers a few samples: both features in one packed FP16
+  // atomic, half the atomic traffic of FP32 pairs (the sums are not yet
+  // divided by the batch size, so they stay far from FP16's lower range)
+  __attribute__((address_space(1))) h2* base =
+      (__attribute__((address_space(1))) h2*)(G.grad16 + static_cast<size_t>(level) * G.table);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const float w = corner_weight(c, k);
-    const uint32_t e = corner_index(c.x + (k & 1), c.y + ((k >> 1) & 1), c.z + ((k >> 2) & 1), res, dense, G.table);
-    unsafeAtomicAdd(base + 2 * e, w * d0);
-    unsafeAtomicAdd(base + 2 * e + 1, w * d1);
+    const uint32_t e = corner_index(c.x + (k & 1), c.y + ((k >> 1) & 1), c.z + ((k >> 2) & 1), res, false, G.table);
+    (void)__builtin_amdgcn_global_atomic_fadd_v2f16(base + e, h2{to_half(w * d0), to_half(w * d1)});
   }
 }
 
@@ -537,7 +553,8 @@ __global__ void __launch_bounds__(256) k_cache_adam_w(float* w, const float* gra
 
 // Sparse Adam of the grid: entries no sample touched (zero gradient) keep
 // their moments; touched gradients are cleared for the next batch.
-__global__ void __launch_bounds__(256) k_cache_adam_grid(float* p, _Float16* p16, float* grad, float* m, float* v,
+__global__ void __launch_bounds__(256) k_cache_adam_grid(float* p, _Float16* p16, float* grad, _Float16* grad16,
+                                                         uint32_t dense_mask, uint32_t level_shift, float* m, float* v,
                                                          uint64_t count, AdamArgs a, uint32_t n_host,
                                                          const uint32_t* n_dev, uint32_t first, uint32_t stride,
                                                          uint32_t cap) {
@@ -545,9 +562,11 @@ __global__ void __launch_bounds__(256) k_cache_adam_grid(float* p, _Float16* p16
   const float inv = n ? 1.0f / static_cast<float>(n) : 0.0f;
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const float g = grad[i];
+    const bool dense = ((dense_mask >> static_cast<uint32_t>(i >> level_shift)) & 1u) != 0;
+    const float g = dense ? grad[i] : static_cast<float>(grad16[i]);
     if (g == 0.0f) continue;
-    grad[i] = 0.0f;
+    if (dense) grad[i] = 0.0f;
+    else grad16[i] = static_cast<_Float16>(0.0f);
     if (n == 0) continue;
     float x = p[i], mi = m[i], vi = v[i];
     adam_update(x, g * inv, mi, vi, a);
@@ -632,10 +651,26 @@ __global__ void __launch_bounds__(256) k_cache_zero(float* p, uint64_t count) {
     p[i] = 0.0f;
 }
 
+__global__ void __launch_bounds__(256) k_cache_zero16(_Float16* p, uint64_t count) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = static_cast<_Float16>(0.0f);
+}
+
+// The hashed levels' FP16 sums copied into the FP32 gradient array, which
+// then holds every level (for reading the gradients out).
+__global__ void __launch_bounds__(256) k_cache_grad_merge(float* grad, const _Float16* grad16, uint32_t dense_mask,
+                                                          uint32_t level_shift, uint64_t count) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < count;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    if (((dense_mask >> static_cast<uint32_t>(i >> level_shift)) & 1u) == 0) grad[i] = static_cast<float>(grad16[i]);
+}
+
 GridArgs grid_args(const CacheState& c) {
   GridArgs G;
   G.grid = reinterpret_cast<const h2*>(c.grid16);
   G.grad = c.ggrad;
+  G.grad16 = reinterpret_cast<h2*>(c.ggrad16);
   for (int l = 0; l < kCacheLevels; ++l) G.res[l] = c.resolution[l];
   G.dense_mask = c.dense_mask;
   G.table = 1u << c.config.log2_table;
@@ -676,7 +711,7 @@ hipError_t alloc(T*& p, size_t count) {
 void cache_free(CacheState& c) {
   for (float** p : {&c.w32, &c.wgrad, &c.wm, &c.wv, &c.grid32, &c.ggrad, &c.gm, &c.gv, &c.partials, &c.loss})
     if (*p) (void)hipFree(*p), *p = nullptr;
-  for (_Float16** p : {&c.wf16, &c.wb16, &c.grid16, &c.acts, &c.deltas})
+  for (_Float16** p : {&c.wf16, &c.wb16, &c.grid16, &c.ggrad16, &c.acts, &c.deltas})
     if (*p) (void)hipFree(*p), *p = nullptr;
   if (c.count) (void)hipFree(c.count), c.count = nullptr;
   c.ready = false;
@@ -725,6 +760,7 @@ bool cache_init(CacheState& c, const CacheConfig& cfg, hipStream_t s, std::strin
   for (float** p : {&c.grid32, &c.ggrad, &c.gm, &c.gv})
     if (e == hipSuccess) e = alloc(*p, c.grid_entries);
   if (e == hipSuccess) e = alloc(c.grid16, c.grid_entries);
+  if (e == hipSuccess) e = alloc(c.ggrad16, c.grid_entries);
   if (e == hipSuccess) e = alloc(c.acts, static_cast<size_t>(kActRows) * c.ld);
   if (e == hipSuccess) e = alloc(c.deltas, static_cast<size_t>(kDeltaRows) * c.ld);
   if (e == hipSuccess) e = alloc(c.partials, static_cast<size_t>(c.chunks) * kCacheParams);
@@ -759,6 +795,7 @@ bool cache_init(CacheState& c, const CacheConfig& cfg, hipStream_t s, std::strin
   if (e == hipSuccess) {
     k_cache_init_grid<<<stream_blocks(c.grid_entries), 256, 0, s>>>(c.grid32, c.grid16, c.gm, c.gv, c.ggrad,
                                                                    c.grid_entries, cfg.seed);
+    k_cache_zero16<<<stream_blocks(c.grid_entries), 256, 0, s>>>(c.ggrad16, c.grid_entries);
     k_cache_pack<<<(kFwdHalves + kBwdHalves + 255) / 256, 256, 0, s>>>(c.w32, c.wf16, c.wb16);
     e = hipStreamSynchronize(s);
   }
@@ -806,7 +843,8 @@ void cache_adam(CacheState& c, uint32_t n, const uint32_t* n_dev, uint32_t first
   const uint32_t cap = c.config.batch_capacity;
   stride = stride ? stride : 1u;
   k_cache_adam_w<<<kCacheParams / 256, 256, 0, s>>>(c.w32, c.wgrad, c.wm, c.wv, a, n, n_dev, first, stride, cap);
-  k_cache_adam_grid<<<stream_blocks(c.grid_entries), 256, 0, s>>>(c.grid32, c.grid16, c.ggrad, c.gm, c.gv,
+  k_cache_adam_grid<<<stream_blocks(c.grid_entries), 256, 0, s>>>(c.grid32, c.grid16, c.ggrad, c.ggrad16,
+                                                                 c.dense_mask, c.config.log2_table + 1, c.gm, c.gv,
                                                                  c.grid_entries, a, n, n_dev, first, stride, cap);
   k_cache_pack<<<(kFwdHalves + kBwdHalves + 255) / 256, 256, 0, s>>>(c.w32, c.wf16, c.wb16);
 }
@@ -818,11 +856,18 @@ void cache_sync_half(CacheState& c, hipStream_t s) {
 
 void cache_clear_grid_grad(CacheState& c, hipStream_t s) {
   k_cache_zero<<<stream_blocks(c.grid_entries), 256, 0, s>>>(c.ggrad, c.grid_entries);
+  k_cache_zero16<<<stream_blocks(c.grid_entries), 256, 0, s>>>(c.ggrad16, c.grid_entries);
+}
+
+void cache_merge_grid_grad(CacheState& c, hipStream_t s) {
+  k_cache_grad_merge<<<stream_blocks(c.grid_entries), 256, 0, s>>>(c.ggrad, c.ggrad16, c.dense_mask,
+                                                                   c.config.log2_table + 1, c.grid_entries);
 }
 
 void cache_reset_moments(CacheState& c, hipStream_t s) {
   for (float* p : {c.gm, c.gv, c.ggrad})
     k_cache_zero<<<stream_blocks(c.grid_entries), 256, 0, s>>>(p, c.grid_entries);
+  k_cache_zero16<<<stream_blocks(c.grid_entries), 256, 0, s>>>(c.ggrad16, c.grid_entries);
   for (float* p : {c.wm, c.wv}) k_cache_zero<<<kCacheParams / 256, 256, 0, s>>>(p, kCacheParams);
   c.step = 0;
 }

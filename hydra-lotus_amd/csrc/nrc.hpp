@@ -56,7 +56,8 @@ struct CacheState {
   _Float16* wb16 = nullptr;   // backward operand image
   float* grid32 = nullptr;    // [level][entry][feature] master
   _Float16* grid16 = nullptr;
-  float* ggrad = nullptr;
+  float* ggrad = nullptr;      // gradient sums of the dense levels (FP32)
+  _Float16* ggrad16 = nullptr; // gradient sums of the hashed levels (FP16 pairs)
   float* gm = nullptr;
   float* gv = nullptr;
   _Float16* acts = nullptr;   // layer inputs, [row][sample], rows 48 + 4 x 64
@@ -90,6 +91,8 @@ void cache_adam(CacheState& c, uint32_t n, const uint32_t* n_dev, uint32_t first
 // Rewrites the FP16 operand images and grid copy from the FP32 masters.
 void cache_sync_half(CacheState& c, hipStream_t s);
 void cache_clear_grid_grad(CacheState& c, hipStream_t s);
+// The hashed levels' FP16 gradient sums into ggrad, which then holds every level.
+void cache_merge_grid_grad(CacheState& c, hipStream_t s);
 // Zeroes the Adam moments and grid gradients and restarts the step count.
 void cache_reset_moments(CacheState& c, hipStream_t s);
 

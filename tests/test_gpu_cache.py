@@ -110,7 +110,10 @@ def test_cache_gradients_match_oracle(small):
     # a second call starts from cleared gradients
     wg2, gg2, loss2 = r.cache_gradients(sd.data_ptr(), n)
     assert np.allclose(wg2, wg, rtol=1e-5, atol=1e-7 * np.abs(wg).max())
-    assert np.allclose(gg2, gg, rtol=1e-4, atol=1e-6 * np.abs(gg).max())
+    # hashed levels sum in packed FP16 atomics, whose order varies: the two
+    # calls are a few FP16 roundings of the summed terms apart (an entry whose
+    # terms cancel keeps their absolute rounding), far from twice the sums
+    assert np.allclose(gg2, gg, rtol=4e-3, atol=1e-3 * np.abs(gg).max())
 
 
 def test_cache_adam_first_step(small):
