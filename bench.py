@@ -266,6 +266,7 @@ def main() -> int:
     tot = {k: 0.0 for k in keys}
     nodes = np.zeros(4)
     tris = np.zeros(4)
+    walked = np.zeros(4)
     trace_launches = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -274,6 +275,7 @@ def main() -> int:
             tot[key] += out[key]
         nodes += out["nodes"]
         tris += out["tris"]
+        walked += out["walked"]
         trace_launches += out["rounds"]
     wall = time.perf_counter() - t0
     rays = tot["camera_rays"] + tot["extension_rays"] + tot["shadow_rays"]
@@ -291,11 +293,12 @@ def main() -> int:
         "shadow": split["shadow_rays"] / split["shadow_ms"] / 1e3 if split["shadow_ms"] else None,
         "frame_gpu": rays / tot["frame_ms"] / 1e3,
     }
-    # roofline of the dominant kernel, k_trace (extension + shadow rays):
-    # algorithmic bytes = queue records + the wide nodes and triangles the
-    # traversal tests (counted by the traversal itself in the timed frames),
-    # over the kernel's hipEvent time
-    ext, sh = tot["extension_rays"], tot["shadow_rays"]
+    # roofline of the dominant kernel, k_trace (the rounds' extension + shadow
+    # rays; the tail kernel walks the later bounces): algorithmic bytes =
+    # queue records + the wide nodes and triangles the traversal tests
+    # (counted by the traversal itself in the timed frames), over the
+    # kernel's hipEvent time
+    ext, sh = walked[1], walked[2]
     node_visits = nodes[1] + nodes[2]
     tri_tests = tris[1] + tris[2]
     bytes_total = node_visits * NODE_BYTES + tri_tests * TRI_BYTES + ext * QUEUE_BYTES["intersect"] + \
@@ -306,12 +309,14 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_trace",
                 "algorithmic_bytes_per_launch": int(bytes_total / max(trace_launches, 1)),
                 "avg_launch_ms": round(ms / max(trace_launches, 1), 4),
-                "nodes_per_ray": {"camera": round(nodes[0] / max(tot["camera_rays"], 1), 2),
+                "nodes_per_ray": {"camera": round(nodes[0] / max(walked[0], 1), 2),
                                   "extension": round(nodes[1] / max(ext, 1), 2),
-                                  "shadow": round(nodes[2] / max(sh, 1), 2)},
-                "tris_per_ray": {"camera": round(tris[0] / max(tot["camera_rays"], 1), 2),
+                                  "shadow": round(nodes[2] / max(sh, 1), 2),
+                                  "tail": round(nodes[3] / max(walked[3], 1), 2)},
+                "tris_per_ray": {"camera": round(tris[0] / max(walked[0], 1), 2),
                                  "extension": round(tris[1] / max(ext, 1), 2),
-                                 "shadow": round(tris[2] / max(sh, 1), 2)}}
+                                 "shadow": round(tris[2] / max(sh, 1), 2),
+                                 "tail": round(tris[3] / max(walked[3], 1), 2)}}
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:

@@ -864,6 +864,8 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
   uint32_t added = 0;
   std::vector<uint32_t> counts;
   uint64_t camera_rays = 0, extension_rays = 0, shadow_rays = 0;
+  uint64_t round_extension_rays = 0, round_shadow_rays = 0;  // walked by the rounds' traversal launches
+  uint64_t restir_rays = 0;
   uint32_t rounds_used = 0;
   if (P->output == LOTUS_HIP_OUTPUT_RADIANCE) {
     AccumulationKey key;
@@ -1025,9 +1027,12 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
           if (counts[b]) last = b;
           if (b >= 1) extension_rays += host[counter_word(b, 0)];
           shadow_rays += host[counter_word(b, 2)];
+          if (b >= 1 && b < rounds) round_extension_rays += host[counter_word(b, 0)];
+          if (b < rounds) round_shadow_rays += host[counter_word(b, 2)];
         }
         camera_rays += host[0];
         shadow_rays += *restir_ray_count(host.data());
+        restir_rays += *restir_ray_count(host.data());
         counts.resize(host[0] ? last + 1 : 0);
         r->rounds_hint = wavefront_rounds(counts, r->tuning.tail_divisor);
         rounds_used = rounds;
@@ -1094,6 +1099,11 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
       res->traversal_nodes[k] = stats[2 * k];
       res->traversal_tris[k] = stats[2 * k + 1];
     }
+    res->traversal_rays[0] = camera_rays;
+    res->traversal_rays[1] = round_extension_rays;
+    res->traversal_rays[2] = round_shadow_rays;
+    res->traversal_rays[3] =
+        (extension_rays - round_extension_rays) + (shadow_rays - restir_rays - round_shadow_rays);
     res->cache_queries = cache_queries;
     res->cache_samples = cache_samples;
     res->cache_loss = cache_used ? cache_loss_sum / (3.0 * static_cast<double>(cache_used)) : 0.0;
@@ -1626,7 +1636,7 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
     return LOTUS_HIP_FAIL;
   }
   std::vector<uint32_t> counts;
-  uint64_t ext = 0, sh = 0;
+  uint64_t ext = 0, sh = 0, round_ext = 0, round_sh = 0;
   if (d.add) {
     counts.assign(d.stride, 0u);
     size_t last = 0;
@@ -1635,6 +1645,8 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
       if (counts[b]) last = b;
       if (b >= 1) ext += host[counter_word(b, 0)];
       sh += host[counter_word(b, 2)];
+      if (b >= 1 && b < d.rounds) round_ext += host[counter_word(b, 0)];
+      if (b < d.rounds) round_sh += host[counter_word(b, 2)];
     }
     counts.resize(host[0] ? last + 1 : 0);
     r->rounds_hint = wavefront_rounds(counts, r->tuning.tail_divisor);
@@ -1649,6 +1661,10 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
     res->camera_rays = d.add ? host[0] : 0;
     res->extension_rays = ext;
     res->shadow_rays = sh;
+    res->traversal_rays[0] = d.add ? host[0] : 0;
+    res->traversal_rays[1] = round_ext;
+    res->traversal_rays[2] = round_sh;
+    res->traversal_rays[3] = (ext - round_ext) + (sh - round_sh);
     res->frame_gpu_ms = gpu_ms;
   }
   return LOTUS_HIP_PASS;

@@ -145,6 +145,9 @@ typedef struct lotus_hip_frame_result {
    * (intersect), shadow, tail. Wide nodes tested and triangles tested. */
   uint64_t traversal_nodes[4];
   uint64_t traversal_tris[4];
+  /* The rays those traversals walked: camera, extension and shadow rays of
+   * the rounds, and the tail's extension and shadow rays. */
+  uint64_t traversal_rays[4];
   /* Radiance cache frames: queries answered, training samples made, their
    * mean relative L2 loss before the frame's steps, and kernel times. */
   uint64_t cache_queries;

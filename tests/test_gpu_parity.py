@@ -209,6 +209,10 @@ def test_heightfield_counts_and_primary_hits(renderer):
     n = len(out["counts"])
     assert np.array_equal(out["counts"], counts[0][:n]) and not counts[0][n:].any()
     assert out["shadow_rays"] == shadow[0]
+    # every ray walked by exactly one traversal kernel: camera, the rounds, the tail
+    walked = out["walked"]
+    assert walked[0] == out["camera_rays"]
+    assert walked[1] + walked[2] + walked[3] == out["extension_rays"] + out["shadow_rays"]
     ref1 = acc[..., :3].astype(np.float32)
     assert np.mean(np.all(out["color"][..., :3] == ref1, -1)) > 0.999
     # camera-pass barycentrics vs the oracle's pixel-centre rays
@@ -302,14 +306,23 @@ def test_accumulation_restart_and_max_samples(renderer):
 
 def test_deep_paths_use_the_tail(renderer):
     # max_bounces beyond the recorded rounds: the tail kernel finishes paths,
-    # counts still exact against the oracle.
+    # counts still exact against the oracle. The second sample's rounds follow
+    # the first sample's counts, so its later bounces go to the tail.
     sc = scenes.closed_box()
     renderer.apply(sc)
-    out = renderer.render(sc.world_to_clip, 32, 32, frame_count=1, max_bounces=200, light_sampling=hip.LIGHT_MIS)
-    acc, counts, shadow = oracle.OracleScene(sc).render(_oracle_frame(sc, 32, 32, 1, 200, hip.LIGHT_MIS))
+    out = renderer.render(sc.world_to_clip, 32, 32, frame_count=2, max_bounces=200, light_sampling=hip.LIGHT_MIS)
+    acc, counts, shadow = oracle.OracleScene(sc).render(_oracle_frame(sc, 32, 32, 2, 200, hip.LIGHT_MIS))
     n = len(out["counts"])
-    assert np.array_equal(out["counts"], counts[0][:n]) and not counts[0][n:].any()
-    assert out["shadow_rays"] == shadow[0]
+    assert np.array_equal(out["counts"], counts[1][:n]) and not counts[1][n:].any()
+    assert out["shadow_rays"] == shadow[0] + shadow[1]
+    assert out["rounds"] < n
+    walked = out["walked"]
+    assert walked[0] == out["camera_rays"] and walked[3] > 0
+    assert walked[1] + walked[2] + walked[3] == out["extension_rays"] + out["shadow_rays"]
+    # every ray walked by exactly one traversal kernel: camera, the rounds, the tail
+    walked = out["walked"]
+    assert walked[0] == out["camera_rays"] and walked[3] > 0
+    assert walked[1] + walked[2] + walked[3] == out["extension_rays"] + out["shadow_rays"]
 
 
 def test_split_kind_launches_give_the_same_frame(renderer):
