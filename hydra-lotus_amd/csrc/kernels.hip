@@ -806,6 +806,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
 // walk, so radiance is summed in the reference's order.
 __device__ __forceinline__ float pair_float(float v) { return __shfl_xor(v, 1); }
 
+template <bool kCache>  // a radiance cache frame (training vertices recorded)
 __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                       uint2* spill, int spill_cap, uint32_t bounce,
                                                       unsigned long long* stats) {
@@ -836,6 +837,16 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
         q.nee[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
       p.rng = __float_as_uint(r4.w);
+    }
+    // radiance cache frames: only training paths reach the tail (the others
+    // end at the cache depth, inside the rounds); their vertices are recorded
+    // as the shading kernel records them
+    bool train = false;
+    uint32_t path = 0;
+    if (kCache && path_lane) {
+      const uint32_t key = slot + F.train_offset;
+      train = key % F.train_stride == 0;
+      path = key / F.train_stride;
     }
     bool extend = path_lane;  // the path lane has an extension ray to walk
     bool shadow = false;      // a shadow ray of the previous bounce is pending
@@ -872,7 +883,18 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
           p.radiance = p.radiance + p.throughput * env;
         } else {
           const HitRecord h = w.record(S.tris);
-          sc = scatter_path(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v);
+          const V3 arrival = p.throughput, dir = p.d;
+          const uint32_t b = p.depth;
+          Surface sf;
+          sc = scatter_path(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v, kCache && train ? &sf : nullptr,
+                            slot);
+          if (kCache && train && b < kCacheRecordDepth && path < F.train_capacity) {
+            float4* rec = q.crec + 6ull * (static_cast<size_t>(b) * F.train_capacity + path);
+            cache_query(F, sf, dir, rec);
+            rec[4] = make_float4(arrival.x, arrival.y, arrival.z, 1.0f);
+            rec[5] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
+            q.crec_n[path] = b + 1;
+          }
         }
       }
       const bool shaded = path_lane && extend && hit;
@@ -1024,13 +1046,14 @@ GridShape grid_shape(uint32_t cus) {
   g.camera_dynamic = persistent_blocks(k_camera<true>, kTraceBlock, cus);
   g.trace = persistent_blocks(k_trace<false>, kTraceBlock, cus);
   g.trace_dynamic = persistent_blocks(k_trace<true>, kTraceBlock, cus);
-  g.tail = persistent_blocks(k_tail, kTraceBlock, cus);
+  g.tail = persistent_blocks(k_tail<false>, kTraceBlock, cus);
+  g.tail_cache = persistent_blocks(k_tail<true>, kTraceBlock, cus);
   g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
   g.rays = persistent_blocks(k_rays, kTraceBlock, cus);
   g.shade = persistent_blocks(k_shade, kShadeBlock, cus);
   g.resolve = cus * 4;
   g.max_trace_lanes =
-      std::max({g.camera, g.camera_dynamic, g.trace, g.trace_dynamic, g.tail, g.camera_pass, g.rays}) * kTraceBlock;
+      std::max({g.camera, g.camera_dynamic, g.trace, g.trace_dynamic, g.tail, g.tail_cache, g.camera_pass, g.rays}) * kTraceBlock;
   return g;
 }
 
@@ -1058,7 +1081,10 @@ void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce) {
-  k_tail<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+  if (F.cache_depth != 0)
+    k_tail<true><<<L.grid.tail_cache, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+  else
+    k_tail<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
 }
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample) {
   k_resolve<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, add_sample);

@@ -44,8 +44,11 @@ struct Queues {
 };
 
 // Training-path vertices recorded per path (deeper ones only contribute to
-// the targets of these).
-constexpr uint32_t kCacheRecordDepth = 8;
+// the targets of these). Two: the cache answers queries at a path's second
+// vertex, and in the equal-relMSE race on the Cornell box, recording 4 or 8
+// vertices cost more training time (0.44 and 0.46 of plain's GPU time at 4
+// frames) than it gained over 2 (0.36).
+constexpr uint32_t kCacheRecordDepth = 2;
 // Smallest throughput component a training vertex may divide by.
 constexpr float kCacheMinThroughput = 1e-3f;
 
@@ -54,7 +57,7 @@ constexpr float kCacheMinThroughput = 1e-3f;
 // occupancy (a grid sized for another variant's occupancy would leave
 // workgroups waiting for a CU, and a strided launch then ends a pass late).
 struct GridShape {
-  uint32_t camera, camera_dynamic, trace, trace_dynamic, tail, camera_pass, rays, shade, resolve;
+  uint32_t camera, camera_dynamic, trace, trace_dynamic, tail, tail_cache, camera_pass, rays, shade, resolve;
   uint32_t max_trace_lanes;  // the most traversal lanes any grid holds
 };
 GridShape grid_shape(uint32_t cus);

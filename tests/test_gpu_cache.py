@@ -244,3 +244,34 @@ def test_trained_cache_beats_plain_and_terminated_paths():
     assert e_cache < e_direct
     assert abs(m_cache - m_ref) < 0.25 * abs(m_direct - m_ref)
     r.close()
+
+
+def test_cache_frame_samples_do_not_depend_on_the_tail():
+    # Training paths record the same vertices whether their later bounces run
+    # as rounds or in the tail kernel: the same training samples, loss and
+    # frame. The first frame sizes the rounds of the second; the network is
+    # set to the same seeded weights before the second (training sums in
+    # atomics of varying order, so trained weights differ in their last bits).
+    hip = _hip()
+    from lotus_amd import scenes
+    sc = scenes.cornell_lambert()
+    W = H = 64
+    outs = []
+    for div in (0, 1):
+        r = hip.Renderer(0)
+        r.apply(sc)
+        r.set_tuning(tail_divisor=div)
+        cfg = r.cache_configure(log2_table=12, min_resolution=4, max_resolution=128, train_paths=512,
+                                query_depth=1, train_steps=1)  # the loss before any of the frame's updates
+        r.render(sc.world_to_clip, W, H, max_bounces=8, cache=True, want_color=False, sample_index=3)
+        w, grid = _params(cfg, 11, grid_scale=1e-2)
+        r.cache_parameters(w * 0.1, grid)
+        outs.append(r.render(sc.world_to_clip, W, H, max_bounces=8, cache=True, sample_index=9))
+        r.close()
+    a, b = outs
+    print(f"rounds {a['rounds']} vs {b['rounds']}, samples {a['cache_samples']} vs {b['cache_samples']}, "
+          f"loss {a['cache_loss']:.6g} vs {b['cache_loss']:.6g}")
+    assert b["rounds"] < a["rounds"]
+    assert a["cache_samples"] == b["cache_samples"] > 0
+    assert abs(a["cache_loss"] - b["cache_loss"]) <= 1e-5 * abs(a["cache_loss"])
+    assert np.array_equal(a["color"], b["color"])
