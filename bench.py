@@ -283,14 +283,18 @@ def main() -> int:
     # rays by kind: the same frames with extension and shadow rays in launches
     # of their own, so each kind's kernel time is its own (not in the timed loop)
     split = {k: 0.0 for k in keys}
+    split_walked = np.zeros(4)
     for k in range(args.kind_frames):
         out = r.render(w2c, args.width, args.height, split_kinds=True, **kw)
         for key in split:
             split[key] += out[key]
+        split_walked += out["walked"]
+    # each kind's rays walked by its own launches (the tail's are its own)
     by_kind = {
         "camera": tot["camera_rays"] / tot["camera_ms"] / 1e3,
-        "extension": split["extension_rays"] / split["intersect_ms"] / 1e3 if split["intersect_ms"] else None,
-        "shadow": split["shadow_rays"] / split["shadow_ms"] / 1e3 if split["shadow_ms"] else None,
+        "extension": split_walked[1] / split["intersect_ms"] / 1e3 if split["intersect_ms"] else None,
+        "shadow": split_walked[2] / split["shadow_ms"] / 1e3 if split["shadow_ms"] else None,
+        "tail": (walked[3] / tot["tail_ms"] / 1e3) if tot["tail_ms"] else None,
         "frame_gpu": rays / tot["frame_ms"] / 1e3,
     }
     # roofline of the dominant kernel, k_trace (the rounds' extension + shadow

@@ -624,7 +624,7 @@ __device__ __forceinline__ void cache_query(const FrameConst& F, const Surface& 
 // training path records each vertex (query, throughput on arrival, radiance
 // including the vertex's emission).
 __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                       uint32_t bounce) {
+                                                       uint32_t bounce, uint32_t link) {
   const uint32_t count = *hit_count(counters, bounce);
   for (uint32_t i = gtid(); i < count; i += gstride()) {
     const float4 o4 = q.hit_o[i], d4 = q.hit_d[i];
@@ -686,14 +686,18 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
     if (s.shadow) {
       q.sh_o[ks] = make_float4(s.shadow_ray.o.x, s.shadow_ray.o.y, s.shadow_ray.o.z, __uint_as_float(slot));
       q.sh_d[ks] = make_float4(s.shadow_ray.d.x, s.shadow_ray.d.y, s.shadow_ray.d.z, s.shadow_ray.tmax);
-      q.sh_c[ks] = make_float4(s.delta.x, s.delta.y, s.delta.z, 0.0f);
+      // w: the tail walks this ray with its path (1) or on its own (0)
+      q.sh_c[ks] = make_float4(s.delta.x, s.delta.y, s.delta.z, __uint_as_float(link && s.continues ? 1u : 0u));
     }
     const uint32_t kr = wave_append(ray_count(counters, bounce + 1), s.continues);
     if (s.continues) {
       q.thr[slot] = make_float4(p.throughput.x, p.throughput.y, p.throughput.z, p.last_pdf);
       q.rad[slot] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, __uint_as_float(p.rng));
       q.ray_o[kr] = make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(slot));
-      q.ray_d[kr] = make_float4(p.d.x, p.d.y, p.d.z, kFloatMax);
+      // before the tail, w links the path's shadow ray of this bounce (the
+      // tail's rays need no tmax), else the extension ray's tmax
+      q.ray_d[kr] = make_float4(p.d.x, p.d.y, p.d.z,
+                                link ? __uint_as_float(s.shadow ? ks : kNoLink) : kFloatMax);
     } else {
       q.rad[slot] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 1.0f);
     }
@@ -814,12 +818,37 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
   TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t count = *ray_count(counters, bounce);
+  const uint32_t n_shadow = *shadow_count(counters, bounce - 1);  // the last round's, linked or on their own
   const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
   const bool path_lane = (threadIdx.x & 1u) == 0;
   uint32_t visits = 0, tests = 0;
   // both lanes of a pair take the same items (the grid's workgroups hold
-  // whole pairs), so the pair enters and leaves the loops together
-  for (uint32_t i = gtid() >> 1; i < count; i += gstride() >> 1) {
+  // whole pairs), so the pair enters and leaves the loops together: items
+  // [0, count) are paths, then the last round's shadow rays
+  for (uint32_t i = gtid() >> 1; i < count + n_shadow; i += gstride() >> 1) {
+    if (i >= count) {
+      // a shadow ray of a path that ended at the last round: the odd lane
+      // walks it, and an unblocked ray leaves its light pending (as k_trace)
+      const uint32_t j = i - count;
+      const float4 c4 = q.sh_c[j];
+      const bool alone = __float_as_uint(c4.w) == 0u;
+      Walk w;
+      uint32_t sslot = 0;
+      if (!path_lane && alone) {
+        const float4 o4 = q.sh_o[j], d4 = q.sh_d[j];
+        sslot = __float_as_uint(o4.w);
+        w.start(Ray{xyz(o4), 0.0f, xyz(d4), d4.w}, true);
+      } else {
+        w.start(Ray{V3{0.0f, 0.0f, 0.0f}, 0.0f, V3{0.0f, 0.0f, 1.0f}, 0.0f}, true);
+        w.cur = kWalkDone;
+      }
+      st.clear();
+      w.run(S.nodes, S.tris, st);
+      visits += w.visits;
+      tests += w.tests;
+      if (!path_lane && alone && !w.hit()) q.nee[sslot] = make_float4(c4.x, c4.y, c4.z, 1.0f);
+      continue;
+    }
     PathRegs p{};
     p.depth = bounce;
     uint32_t slot = 0;
@@ -852,6 +881,16 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
     bool shadow = false;      // a shadow ray of the previous bounce is pending
     Ray sray{};               // that ray (sent to the odd lane)
     V3 delta{};               // its contribution (kept in the path lane)
+    if (path_lane) {
+      // the shadow ray the last round's shading linked to this path
+      const uint32_t link = __float_as_uint(q.ray_d[i].w);
+      if (link != kNoLink) {
+        const float4 o4 = q.sh_o[link], d4 = q.sh_d[link], c4 = q.sh_c[link];
+        shadow = true;
+        sray = Ray{xyz(o4), 0.0f, xyz(d4), d4.w};
+        delta = xyz(c4);
+      }
+    }
     while (true) {
       // the pair's state, as the path lane holds it
       const bool pair_extend = __shfl_xor(static_cast<int>(extend), 1) != 0 || extend;
@@ -1067,8 +1106,8 @@ void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, con
     k_camera<false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
-                  uint32_t bounce) {
-  k_shade<<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce);
+                  uint32_t bounce, bool link) {
+  k_shade<<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
 }
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds) {

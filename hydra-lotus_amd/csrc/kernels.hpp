@@ -80,8 +80,12 @@ void launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s);
 void launch_fill1(float* p, uint32_t n, float v, hipStream_t s);
 void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c);
+// An extension ray of the tail's first bounce whose path sent no shadow ray.
+constexpr uint32_t kNoLink = 0xFFFFFFFFu;
+// link: the last round before the tail, whose shadow rays the tail walks (a
+// continuing path's extension ray carries its shadow ray's index in d.w).
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
-                  uint32_t bounce);
+                  uint32_t bounce, bool link = false);
 // Extension rays of bounce `bounce` and/or shadow rays of bounce `bounce - 1`.
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds);

@@ -936,13 +936,18 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
           done();
         }
         // round b: shade bounce b, then trace the extension rays of bounce
-        // b + 1 (unless the tail takes them) with the shadow rays of bounce b
+        // b + 1 (unless the tail takes them) with the shadow rays of bounce b.
+        // Before a tail, the last round's shadow rays go to the tail too: a
+        // path's pair walks it beside the path's next extension ray, and the
+        // tail walks the shadow rays of paths that end there on their own.
+        const bool tail = rounds < bounces;
         for (uint32_t b = 0; b < rounds; ++b) {
+          const bool link = tail && b + 1 == rounds;
           mark(kTShade);
-          launch_shade(L, r->view, F, r->q, r->counters, b);
+          launch_shade(L, r->view, F, r->q, r->counters, b, link);
           done();
           const uint32_t kinds = (b + 1 < rounds ? kTraceExtension : 0u) |
-                                 (P->light_sampling != 0 ? kTraceShadow : 0u);
+                                 (P->light_sampling != 0 && !link ? kTraceShadow : 0u);
           if (kinds == 0) continue;
           if (split) {
             for (const uint32_t kind : {kTraceExtension, kTraceShadow}) {
@@ -957,7 +962,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
             done();
           }
         }
-        if (rounds < bounces) {
+        if (tail) {
           mark(kTTail);
           launch_tail(L, r->view, F, r->q, r->counters, rounds);
           done();
@@ -1028,7 +1033,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
           if (b >= 1) extension_rays += host[counter_word(b, 0)];
           shadow_rays += host[counter_word(b, 2)];
           if (b >= 1 && b < rounds) round_extension_rays += host[counter_word(b, 0)];
-          if (b < rounds) round_shadow_rays += host[counter_word(b, 2)];
+          if (b + 1 < rounds || (b < rounds && rounds >= bounces)) round_shadow_rays += host[counter_word(b, 2)];
         }
         camera_rays += host[0];
         shadow_rays += *restir_ray_count(host.data());
@@ -1646,7 +1651,7 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
       if (b >= 1) ext += host[counter_word(b, 0)];
       sh += host[counter_word(b, 2)];
       if (b >= 1 && b < d.rounds) round_ext += host[counter_word(b, 0)];
-      if (b < d.rounds) round_sh += host[counter_word(b, 2)];
+      if (b + 1 < d.rounds || (b < d.rounds && d.rounds + 1 >= d.stride)) round_sh += host[counter_word(b, 2)];
     }
     counts.resize(host[0] ? last + 1 : 0);
     r->rounds_hint = wavefront_rounds(counts, r->tuning.tail_divisor);
