@@ -276,7 +276,7 @@ def main() -> int:
         nodes += out["nodes"]
         tris += out["tris"]
         walked += out["walked"]
-        trace_launches += out["rounds"]
+        trace_launches += out["trace_launches"]
     wall = time.perf_counter() - t0
     rays = tot["camera_rays"] + tot["extension_rays"] + tot["shadow_rays"]
     value = rays / wall / 1e6

@@ -203,6 +203,7 @@ struct lotus_hip_renderer {
     bool pending = false;
     bool add = false;
     uint32_t stride = 0, rounds = 0;
+    uint64_t launches = 0;  // its traversal launches
   } deferred;
   hipEvent_t join = nullptr;       // this renderer's branch of a batch ends
   float4* svgf_filter = nullptr;   // lotus_hip_svgf's scratch
@@ -866,6 +867,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
   uint64_t camera_rays = 0, extension_rays = 0, shadow_rays = 0;
   uint64_t round_extension_rays = 0, round_shadow_rays = 0;  // walked by the rounds' traversal launches
   uint64_t restir_rays = 0;
+  uint64_t trace_launches = 0;
   uint32_t rounds_used = 0;
   if (P->output == LOTUS_HIP_OUTPUT_RADIANCE) {
     AccumulationKey key;
@@ -955,11 +957,13 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
               mark(kind == kTraceExtension ? kTIntersect : kTShadow);
               launch_trace(L, r->view, r->q, r->counters, b + 1, kind);
               done();
+              if (add) ++trace_launches;
             }
           } else {
             mark(kTTrace);
             launch_trace(L, r->view, r->q, r->counters, b + 1, kinds);
             done();
+            if (add) ++trace_launches;
           }
         }
         if (tail) {
@@ -996,7 +1000,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
           e = hipMemcpyAsync(&loss_host, r->cache.loss, sizeof(float), hipMemcpyDeviceToHost, r->stream);
       }
       if (deferred) {
-        r->deferred = {true, add != 0, stride, rounds};
+        r->deferred = {true, add != 0, stride, rounds, trace_launches};
         return LOTUS_HIP_PASS;
       }
       std::vector<uint32_t> host(frame_counter_words(stride));
@@ -1104,6 +1108,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
       res->traversal_nodes[k] = stats[2 * k];
       res->traversal_tris[k] = stats[2 * k + 1];
     }
+    res->trace_launches = trace_launches;
     res->traversal_rays[0] = camera_rays;
     res->traversal_rays[1] = round_extension_rays;
     res->traversal_rays[2] = round_shadow_rays;
@@ -1666,6 +1671,7 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
     res->camera_rays = d.add ? host[0] : 0;
     res->extension_rays = ext;
     res->shadow_rays = sh;
+    res->trace_launches = d.launches;
     res->traversal_rays[0] = d.add ? host[0] : 0;
     res->traversal_rays[1] = round_ext;
     res->traversal_rays[2] = round_sh;

@@ -131,6 +131,7 @@ class FrameResult(C.Structure):
         ("traversal_nodes", C.c_uint64 * 4),
         ("traversal_tris", C.c_uint64 * 4),
         ("traversal_rays", C.c_uint64 * 4),
+        ("trace_launches", C.c_uint64),
         ("cache_queries", C.c_uint64),
         ("cache_samples", C.c_uint64),
         ("cache_loss", C.c_double),
@@ -417,7 +418,7 @@ class Renderer:
                     shadow_ms=res.shadow_gpu_ms, trace_ms=res.trace_gpu_ms, tail_ms=res.tail_gpu_ms,
                     resolve_ms=res.resolve_gpu_ms, frame_ms=res.frame_gpu_ms,
                     nodes=list(res.traversal_nodes), tris=list(res.traversal_tris),
-                    walked=list(res.traversal_rays),
+                    walked=list(res.traversal_rays), trace_launches=res.trace_launches,
                     cache_queries=res.cache_queries, cache_samples=res.cache_samples, cache_loss=res.cache_loss,
                     cache_infer_ms=res.cache_infer_gpu_ms, cache_train_ms=res.cache_train_gpu_ms,
                     restir=restir, restir_ms=res.restir_gpu_ms)

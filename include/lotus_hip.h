@@ -148,6 +148,7 @@ typedef struct lotus_hip_frame_result {
   /* The rays those traversals walked: camera, extension and shadow rays of
    * the rounds, and the tail's extension and shadow rays. */
   uint64_t traversal_rays[4];
+  uint64_t trace_launches;      /* the rounds' traversal launches over the call's samples */
   /* Radiance cache frames: queries answered, training samples made, their
    * mean relative L2 loss before the frame's steps, and kernel times. */
   uint64_t cache_queries;

@@ -319,10 +319,18 @@ def test_deep_paths_use_the_tail(renderer):
     walked = out["walked"]
     assert walked[0] == out["camera_rays"] and walked[3] > 0
     assert walked[1] + walked[2] + walked[3] == out["extension_rays"] + out["shadow_rays"]
+    # a round launches the traversal once, except the last round before the
+    # tail (the tail walks its shadow rays): 31 for the first sample's 32
+    # rounds, rounds - 1 for the second's
+    assert out["trace_launches"] == 31 + out["rounds"] - 1
     # every ray walked by exactly one traversal kernel: camera, the rounds, the tail
     walked = out["walked"]
     assert walked[0] == out["camera_rays"] and walked[3] > 0
     assert walked[1] + walked[2] + walked[3] == out["extension_rays"] + out["shadow_rays"]
+    # a round launches the traversal once, except the last round before the
+    # tail (the tail walks its shadow rays): 31 for the first sample's 32
+    # rounds, rounds - 1 for the second's
+    assert out["trace_launches"] == 31 + out["rounds"] - 1
 
 
 def test_split_kind_launches_give_the_same_frame(renderer):
