@@ -238,11 +238,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
 __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, FrameConst F, Queues q,
                                                                 uint32_t* counters, uint2* spill, int spill_cap,
                                                                 unsigned long long* stats) {
-  __shared__ uint2 ring[kRing * kTraceBlock];
+  __shared__ uint2 ring[(kRing + 1) * kTraceBlock];  // + the lanes' traversal counts
   TraceStack st;
   st.init(ring, spill, spill_cap);
+  LaneStats ls_counts;  // in LDS: the kernel sits one register above an occupancy step
+  ls_counts.init(st);
   const uint32_t count = *hit_count(counters, 0);
-  uint32_t visits = 0, tests = 0;
   for (uint32_t i = gtid(); i < count; i += gstride()) {
     const float4 o4 = q.hit_o[i], d4 = q.hit_d[i];
     const uint4 h4 = q.hit_h[i];
@@ -289,9 +290,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
     wave_count(restir_ray_count(counters), trace);
     if (trace) {
       const LightSample ls = sample_triangle_light(S, r.sample, ru, rv, origin);
+      uint32_t visits = 0, tests = 0;
       const float visibility =
           walk_occluded(S.nodes, S.tris, st, Ray{origin, 0.0f, ls.direction, ls.distance}, visits, tests) ? 0.0f
                                                                                                           : 1.0f;
+      ls_counts.add(visits, tests);
       const float normalization = reservoir_normalization(r);
       const V3 estimate = candidate_contribution(sf, ft, fb, wo, ls, S.lights[r.sample].pdf) *
                           (visibility * normalization);
@@ -311,7 +314,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
                           stable_instance_id(F, h4.x), stable_material_id(F, S.instance_material, h4.x));
     }
   }
-  flush_stats(stats, 2, visits, tests);
+  ls_counts.flush(stats, 2);
 }
 
 // Every pixel's initial record emptied (temporal frames: the reuse passes
