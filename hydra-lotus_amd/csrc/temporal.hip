@@ -260,14 +260,50 @@ __device__ __forceinline__ float depth_step(const TemporalConst& T, const Tempor
 }
 
 // temporal.slang:313-370 (EstimateTemporalVariance; CPU temporal.cpp:375-421).
+// A workgroup takes 16 x 16 pixels and first stages what the 7 x 7 spatial
+// estimate reads of its pixels and a 3-pixel apron (usable flag, instance id,
+// normal and depth, moments) in LDS, so the 49 taps of each pixel read LDS
+// rather than 96 bytes of records each through the L1. The taps run in the
+// same order with the same arithmetic.
+constexpr int kVarTile = 16, kVarApron = 3, kVarSpan = kVarTile + 2 * kVarApron;
+
 __global__ void __launch_bounds__(256) k_temporal_variance(TemporalConst T, const TemporalPixel* surface,
                                                            TemporalRadiancePixel* radiance, float4* filter) {
+  __shared__ float4 nd[kVarSpan * kVarSpan];
+  __shared__ float2 mom[kVarSpan * kVarSpan];
+  __shared__ unsigned long long inst[kVarSpan * kVarSpan];
+  __shared__ uint32_t ok[kVarSpan * kVarSpan];
   const uint32_t total = T.width * T.height;
+  const int W = static_cast<int>(T.width), H = static_cast<int>(T.height);
   const float batch = static_cast<float>(T.fresh_samples < T.cap ? T.fresh_samples : T.cap);
-  for (uint32_t i = tid(); i < total; i += tstride()) {
+  const uint32_t tiles_x = (T.width + kVarTile - 1) / kVarTile;
+  const uint32_t tiles = tiles_x * ((T.height + kVarTile - 1) / kVarTile);
+  const int lx = static_cast<int>(threadIdx.x % kVarTile), ly = static_cast<int>(threadIdx.x / kVarTile);
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const int x0 = static_cast<int>(t % tiles_x) * kVarTile - kVarApron;
+    const int y0 = static_cast<int>(t / tiles_x) * kVarTile - kVarApron;
+    __syncthreads();  // the previous tile's taps are done
+    for (int k = static_cast<int>(threadIdx.x); k < kVarSpan * kVarSpan; k += static_cast<int>(blockDim.x)) {
+      const int gx = x0 + k % kVarSpan, gy = y0 + k / kVarSpan;
+      uint32_t usable_here = 0u;
+      if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+        const uint32_t q = static_cast<uint32_t>(gy) * T.width + static_cast<uint32_t>(gx);
+        usable_here = radiance[q].sample_count != 0u ? 1u : 0u;
+        mom[k] = make_float2(radiance[q].first_moment, radiance[q].second_moment);
+        nd[k] = surface[q].normal_depth;
+        inst[k] = surface[q].instance_id;
+      }
+      ok[k] = usable_here;
+    }
+    __syncthreads();
+    const int x = x0 + kVarApron + lx, y = y0 + kVarApron + ly;
+    if (x >= W || y >= H) continue;
+    const uint32_t i = static_cast<uint32_t>(y) * T.width + static_cast<uint32_t>(x);
+    const int centre = (ly + kVarApron) * kVarSpan + (lx + kVarApron);
+    // usable(): a pixel of the image with history on the same instance
+    const auto usable = [&](int k, uint64_t instance) { return ok[k] != 0u && inst[k] == instance; };
     const TemporalRadiancePixel c = radiance[i];
     float variance = 0.0f;
-    const int x = static_cast<int>(i % T.width), y = static_cast<int>(i / T.width);
     const TemporalPixel s = surface[i];
     if (c.sample_count != 0u) {
       const float batches = static_cast<float>(c.sample_count) / batch;
@@ -275,24 +311,32 @@ __global__ void __launch_bounds__(256) k_temporal_variance(TemporalConst T, cons
         variance = maxf(0.0f, c.second_moment - c.first_moment * c.first_moment);
       } else {
         const float depth = s.normal_depth.w;
-        const float phi = maxf(depth_step(T, surface, radiance, x, y, 1, 0, depth, s.instance_id) +
-                                   depth_step(T, surface, radiance, x, y, 0, 1, depth, s.instance_id),
-                               1.0e-8f) * 3.0f;
+        // depth_step(): the smaller depth step to a usable neighbour along one axis
+        const auto step = [&](int off) {
+          float st = -1.0f;
+          if (usable(centre - off, s.instance_id)) st = fabsf(depth - nd[centre - off].w);
+          if (usable(centre + off, s.instance_id)) {
+            const float other = fabsf(depth - nd[centre + off].w);
+            st = st < 0.0f ? other : minf(st, other);
+          }
+          return maxf(st, 0.0f);
+        };
+        const float phi = maxf(step(1) + step(kVarSpan), 1.0e-8f) * 3.0f;
         float ws = 0.0f, s1 = 0.0f, s2 = 0.0f;
         for (int dy = -3; dy <= 3; ++dy) {
           for (int dx = -3; dx <= 3; ++dx) {
-            if (!usable(T, surface, radiance, x + dx, y + dy, s.instance_id)) continue;
-            const uint32_t q = static_cast<uint32_t>(y + dy) * T.width + static_cast<uint32_t>(x + dx);
-            const float4 n = surface[q].normal_depth;
-            const TemporalRadiancePixel nb = radiance[q];
+            const int k = centre + dy * kVarSpan + dx;
+            if (!usable(k, s.instance_id)) continue;
+            const float4 n = nd[k];
+            const float2 nb = mom[k];
             const float distance = sqrtf(static_cast<float>(dx * dx + dy * dy));
             const float wz = dx == 0 && dy == 0 ? 0.0f : fabsf(depth - n.w) / (phi * distance);
             const float cosine = (s.normal_depth.x * n.x + s.normal_depth.y * n.y) + s.normal_depth.z * n.z;
             const float wn = pow128(minf(maxf(cosine, 0.0f), 1.0f));
-            const float wl = fabsf(c.first_moment - nb.first_moment) / 10.0f;
+            const float wl = fabsf(c.first_moment - nb.x) / 10.0f;
             const float w = expf(-(wl + wz)) * wn;
-            s1 += w * nb.first_moment;
-            s2 += w * nb.second_moment;
+            s1 += w * nb.x;
+            s2 += w * nb.y;
             ws += w;
           }
         }
