@@ -41,6 +41,32 @@ __device__ __forceinline__ float unordered(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
+// Flattened triangle g in world space: its instance is the last one whose
+// first flattened id is at most g.
+__device__ __forceinline__ Tri world_triangle(const BuildInput& in, uint32_t g) {
+  uint32_t lo = 0, hi = in.instance_count;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (in.instance_base[mid] <= g) lo = mid; else hi = mid;
+  }
+  const uint32_t inst = lo, prim = g - in.instance_base[inst];
+  const uint32_t geo = in.instance_geometry[inst];
+  const uint32_t* tri = in.triangles + 3ull * (in.geometry_triangle_offset[geo] + prim);
+  const float* pos = in.positions + 3ull * in.geometry_vertex_offset[geo];
+  const float* m = in.instance_transform + 16ull * inst;
+  float w[3][4];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float* p = pos + 3ull * tri[k];
+    mul_point(m, p[0], p[1], p[2], 1.0f, w[k]);
+  }
+  Tri t;
+  t.a = make_float4(w[0][0], w[0][1], w[0][2], __uint_as_float(g));
+  t.b = make_float4(w[1][0], w[1][1], w[1][2], __uint_as_float(inst));
+  t.c = make_float4(w[2][0], w[2][1], w[2][2], __uint_as_float(prim));
+  return t;
+}
+
 // Step 1: world-space triangles in flattened (instance, primitive) order, and
 // the centroid bounds (ordered-int atomics: lo[3], hi[3]).
 __global__ void __launch_bounds__(kBlock) k_flatten(BuildInput in, Tri* world, float4* centroid,
@@ -49,29 +75,11 @@ __global__ void __launch_bounds__(kBlock) k_flatten(BuildInput in, Tri* world, f
   float c[3] = {0.0f, 0.0f, 0.0f};
   const bool valid = g < in.tri_count;
   if (valid) {
-    uint32_t lo = 0, hi = in.instance_count;  // last instance with base <= g
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (in.instance_base[mid] <= g) lo = mid; else hi = mid;
-    }
-    const uint32_t inst = lo, prim = g - in.instance_base[inst];
-    const uint32_t geo = in.instance_geometry[inst];
-    const uint32_t* tri = in.triangles + 3ull * (in.geometry_triangle_offset[geo] + prim);
-    const float* pos = in.positions + 3ull * in.geometry_vertex_offset[geo];
-    const float* m = in.instance_transform + 16ull * inst;
-    float w[3][4];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float* p = pos + 3ull * tri[k];
-      mul_point(m, p[0], p[1], p[2], 1.0f, w[k]);
-    }
-    Tri t;
-    t.a = make_float4(w[0][0], w[0][1], w[0][2], __uint_as_float(g));
-    t.b = make_float4(w[1][0], w[1][1], w[1][2], __uint_as_float(inst));
-    t.c = make_float4(w[2][0], w[2][1], w[2][2], __uint_as_float(prim));
+    const Tri t = world_triangle(in, g);
     world[g] = t;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) c[a] = (w[0][a] + w[1][a] + w[2][a]) * (1.0f / 3.0f);
+    c[0] = (t.a.x + t.b.x + t.c.x) * (1.0f / 3.0f);
+    c[1] = (t.a.y + t.b.y + t.c.y) * (1.0f / 3.0f);
+    c[2] = (t.a.z + t.b.z + t.c.z) * (1.0f / 3.0f);
     centroid[g] = make_float4(c[0], c[1], c[2], 0.0f);
   }
   __shared__ uint32_t s_lo[3], s_hi[3];
@@ -419,6 +427,77 @@ __global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const ui
   if (leaves) atomicAdd(leaf_total, leaves);
 }
 
+// ------------------------------------------------------------------ refit --
+// Triangles re-placed from new positions or transforms, each to its slot in
+// leaf order.
+__global__ void __launch_bounds__(kBlock) k_reflatten(BuildInput in, const uint32_t* tri_of_global, Tri* tris) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= in.tri_count) return;
+  tris[tri_of_global[g]] = world_triangle(in, g);
+}
+
+// One level's nodes [first, first + count): each child's box from its
+// triangles (a leaf) or from the exact box its node wrote one launch earlier,
+// with the build's margin; the node's exact box is their union.
+__global__ void __launch_bounds__(kBlock) k_refit(WideNode* nodes, float4* exact, const Tri* tris, uint32_t first,
+                                                  uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t w = first + i;
+  WideNode node = nodes[w];
+  Box all;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    all.lo[a] = INFINITY;
+    all.hi[a] = -INFINITY;
+  }
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t word = node.child[c];
+    if (word == kEmptyChild) continue;
+    Box b;
+    if (word & kLeafBit) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        b.lo[a] = INFINITY;
+        b.hi[a] = -INFINITY;
+      }
+      const uint32_t f = leaf_first(word), n = leaf_count(word);
+      for (uint32_t k = 0; k < n; ++k) {
+        const Tri t = tris[f + k];
+        b.lo[0] = fminf(b.lo[0], fminf(fminf(t.a.x, t.b.x), t.c.x));
+        b.lo[1] = fminf(b.lo[1], fminf(fminf(t.a.y, t.b.y), t.c.y));
+        b.lo[2] = fminf(b.lo[2], fminf(fminf(t.a.z, t.b.z), t.c.z));
+        b.hi[0] = fmaxf(b.hi[0], fmaxf(fmaxf(t.a.x, t.b.x), t.c.x));
+        b.hi[1] = fmaxf(b.hi[1], fmaxf(fmaxf(t.a.y, t.b.y), t.c.y));
+        b.hi[2] = fmaxf(b.hi[2], fmaxf(fmaxf(t.a.z, t.b.z), t.c.z));
+      }
+    } else {
+      const float4 lo = exact[2ull * word], hi = exact[2ull * word + 1];
+      b.lo[0] = lo.x;
+      b.lo[1] = lo.y;
+      b.lo[2] = lo.z;
+      b.hi[0] = hi.x;
+      b.hi[1] = hi.y;
+      b.hi[2] = hi.z;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      all.lo[a] = fminf(all.lo[a], b.lo[a]);
+      all.hi[a] = fmaxf(all.hi[a], b.hi[a]);
+      inflate(b.lo[a], b.hi[a]);
+    }
+    node.lox[c] = b.lo[0];
+    node.loy[c] = b.lo[1];
+    node.loz[c] = b.lo[2];
+    node.hix[c] = b.hi[0];
+    node.hiy[c] = b.hi[1];
+    node.hiz[c] = b.hi[2];
+  }
+  nodes[w] = node;
+  exact[2ull * w] = make_float4(all.lo[0], all.lo[1], all.lo[2], 0.0f);
+  exact[2ull * w + 1] = make_float4(all.hi[0], all.hi[1], all.hi[2], 0.0f);
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -499,6 +578,7 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
     k_empty_root<<<1, 1, 0, stream>>>(out.nodes);
     out.node_count = 1;
     out.depth = 1;
+    out.level_sizes.assign(1, 1u);
     out.leaf_count = 0;
     out.bytes += sizeof(WideNode);
     LOTUS_CHECK(hipStreamSynchronize(stream));
@@ -575,8 +655,10 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   uint32_t pending = 1, depth = 0;
   Task *cur = tasks_a, *nxt = tasks_b;
   int cur_slot = 0;
+  out.level_sizes.clear();
   while (e == hipSuccess && pending > 0 && depth < 256) {
     ++depth;
+    out.level_sizes.push_back(pending);
     const int nxt_slot = 1 - cur_slot;
     k_fill_u32<<<1, 1, 0, stream>>>(counters + nxt_slot, 0u, 1u);
     k_collapse<<<blocks_for(pending, kBlock), kBlock, 0, stream>>>(
@@ -611,6 +693,49 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   out.depth = depth;
   out.bytes += uint64_t(max_nodes) * sizeof(WideNode);
   release();
+  return true;
+}
+
+bool refit_bvh(const BuildInput& in, SceneBuffers& s, hipStream_t stream, double& gpu_ms, std::string& error) {
+  gpu_ms = 0.0;
+  const uint32_t n = in.tri_count;
+  if (n == 0) return true;  // the empty root has no box to move
+  uint64_t listed = 0;
+  for (uint32_t size : s.level_sizes) listed += size;
+  if (listed != s.node_count) {
+    error = "BVH refit: the level sizes do not cover the nodes";
+    return false;
+  }
+  if (!s.exact) {
+    if (!alloc(s.exact, 2ull * s.node_count, error)) return false;
+    s.bytes += 2ull * s.node_count * sizeof(float4);
+  }
+  hipEvent_t start, stop;
+  LOTUS_CHECK(hipEventCreate(&start));
+  LOTUS_CHECK(hipEventCreate(&stop));
+  hipError_t e = hipEventRecord(start, stream);
+  if (e == hipSuccess) {
+    k_reflatten<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(in, s.tri_of_global, s.tris);
+    // Deepest level first: a node's inner children sit one level below it.
+    uint32_t first = s.node_count;
+    for (size_t d = s.level_sizes.size(); d-- > 0;) {
+      const uint32_t count = s.level_sizes[d];
+      first -= count;
+      k_refit<<<blocks_for(count, kBlock), kBlock, 0, stream>>>(s.nodes, s.exact, s.tris, first, count);
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(stop, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  float ms = 0.0f;
+  if (e == hipSuccess) (void)hipEventElapsedTime(&ms, start, stop);
+  (void)hipEventDestroy(start);
+  (void)hipEventDestroy(stop);
+  if (e != hipSuccess) {
+    error = std::string("BVH refit: ") + hipGetErrorString(e);
+    return false;
+  }
+  gpu_ms = ms;
   return true;
 }
 

@@ -37,7 +37,7 @@ struct SceneView {
   float environment[3];
 };
 
-// Device memory that an apply owns until the next apply.
+// Device memory of the resident scene, kept until a plan needs a new BVH.
 struct SceneBuffers {
   WideNode* nodes = nullptr;
   Tri* tris = nullptr;
@@ -48,9 +48,24 @@ struct SceneBuffers {
   uint32_t* materials = nullptr;
   uint32_t* light_offsets = nullptr;
   LightRecord* lights = nullptr;
+  // The plan's geometry as uploaded: refits re-place its triangles and
+  // read-back copies it.
+  float* positions = nullptr;
+  uint32_t* triangles = nullptr;
+  uint32_t* geometry_vertex_offset = nullptr;
+  uint32_t* geometry_triangle_offset = nullptr;
+  uint32_t* instance_geometry = nullptr;
+  uint64_t vertex_words = 0, index_words = 0;
+  uint32_t material_count = 0, light_count = 0;
   uint32_t node_count = 0;
   uint32_t depth = 0;
   uint32_t leaf_count = 0;
+  // Wide nodes per level, the root's first: the build numbers nodes level by
+  // level, so level d holds the level_sizes[d] nodes after the levels above.
+  std::vector<uint32_t> level_sizes;
+  // Refit scratch, allocated by the first refit: each node's box before the
+  // margin (lo.xyz, hi.xyz as two float4).
+  float4* exact = nullptr;
   uint64_t bytes = 0;
 };
 
@@ -71,6 +86,12 @@ struct BuildInput {
 // Builds tris, tri_of_global and the wide BVH into `out` (bvh_build.hip).
 bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, double& gpu_ms,
                std::string& error);
+
+// Re-places every triangle from the (changed) positions and transforms in
+// `in`, keeping the BVH's topology and leaf order, and recomputes its boxes
+// level by level from the leaves up. The boxes equal those a build of the
+// same topology writes (bvh_build.hip).
+bool refit_bvh(const BuildInput& in, SceneBuffers& s, hipStream_t stream, double& gpu_ms, std::string& error);
 
 // Test hook of the sort (bvh_build.hip): stable sort of keys with values.
 bool radix_sort_pairs(uint32_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits,

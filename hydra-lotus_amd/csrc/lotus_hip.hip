@@ -157,9 +157,19 @@ struct lotus_hip_renderer {
   SceneBuffers scene;
   SceneView view{};
   bool has_scene = false;
-  uint64_t scene_generation = 0;
+  uint64_t scene_generation = 0;  // any change: restarts the accumulation
+  uint64_t scene_epoch = 0;       // a change temporal history cannot carry over
   std::vector<uint32_t> light_offsets_host;
   std::vector<LightRecord> lights_host;
+  // The applied plan, to find what the next one changes.
+  struct PlanRecord {
+    std::vector<float> positions, transforms;
+    std::vector<uint32_t> triangles, gvo, gvc, gto, gtc, igeo, imat, materials;
+    float environment[3] = {0, 0, 0};
+    uint32_t leaf_size = 0;
+  } plan;
+  // Lifetime scene counts (lotus_hip_scene_evidence).
+  uint64_t bvh_builds = 0, bvh_refits = 0, material_writes = 0, light_writes = 0, uploaded_bytes = 0;
 
   uint32_t width = 0, height = 0;
   Queues q{};
@@ -192,7 +202,7 @@ struct lotus_hip_renderer {
     uint4* rsources = nullptr;
     uint32_t cur = 0;
     bool previous = false;      // the last successful temporal frame's state below
-    uint64_t frame_index = 0, history_key = 0, scene_generation = 0;
+    uint64_t frame_index = 0, history_key = 0, scene_generation = 0, scene_epoch = 0;
     float view[16] = {}, world_to_clip[16] = {}, display[4] = {};
     uint32_t data[4] = {};
     uint32_t light_sampling = 0, max_bounces = 0, candidates = 0, proposal = 0;
@@ -258,7 +268,14 @@ void free_scene(lotus_hip_renderer* r) {
   dev_free(s.materials);
   dev_free(s.light_offsets);
   dev_free(s.lights);
+  dev_free(s.positions);
+  dev_free(s.triangles);
+  dev_free(s.geometry_vertex_offset);
+  dev_free(s.geometry_triangle_offset);
+  dev_free(s.instance_geometry);
+  dev_free(s.exact);
   s = SceneBuffers{};
+  r->plan = lotus_hip_renderer::PlanRecord{};
   r->has_scene = false;
 }
 
@@ -515,6 +532,308 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   return {};
 }
 
+// The extents of a plan's arrays.
+struct PlanShape {
+  uint32_t geometries = 0, instances = 0, materials = 0, tri_count = 0, leaf_size = 2;
+  uint64_t vertex_words = 0, index_words = 0;
+  std::vector<uint32_t> base;  // first flattened triangle per instance, instances + 1
+};
+
+PlanShape plan_shape(const lotus_hip_scene_plan* p) {
+  PlanShape sh;
+  sh.geometries = p->geometry_count;
+  sh.instances = p->instance_count;
+  sh.materials = p->material_count;
+  sh.leaf_size = p->bvh_leaf_size ? p->bvh_leaf_size : 2u;
+  sh.base.assign(sh.instances + 1, 0);
+  for (uint32_t i = 0; i < sh.instances; ++i)
+    sh.base[i + 1] = sh.base[i] + p->geometry_triangle_count[p->instance_geometry[i]];
+  sh.tri_count = sh.base[sh.instances];
+  for (uint32_t g = 0; g < sh.geometries; ++g) {
+    sh.vertex_words =
+        std::max<uint64_t>(sh.vertex_words, 3ull * (uint64_t(p->geometry_vertex_offset[g]) + p->geometry_vertex_count[g]));
+    sh.index_words = std::max<uint64_t>(
+        sh.index_words, 3ull * (uint64_t(p->geometry_triangle_offset[g]) + p->geometry_triangle_count[g]));
+  }
+  return sh;
+}
+
+template <typename T>
+bool same_words(const std::vector<T>& held, const T* src, uint64_t n) {
+  return held.size() == n && (n == 0 || std::memcmp(held.data(), src, n * sizeof(T)) == 0);
+}
+
+// An instance of this material enters the light table (gpu_scene.cpp:144-203).
+bool material_emits(const uint32_t* m) {
+  float e[3];
+  std::memcpy(e, m + 16, 12);
+  const double mean = (double(e[0]) + double(e[1]) + double(e[2])) / 3.0;
+  return std::isfinite(mean) && mean > 0.0;
+}
+
+// What a plan changes against the applied one (GpuScene::Apply's retain and
+// TLAS-current tests, gpu_scene.cpp:1638-1725).
+struct Changes {
+  uint32_t update = LOTUS_HIP_SCENE_BUILD;
+  std::vector<uint32_t> moved;  // geometries whose positions differ
+  bool transforms = false, materials = false, instance_materials = false, environment = false;
+  bool refit = false;
+  bool temporal_compatible = false;
+};
+
+Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const PlanShape& sh) {
+  Changes c;
+  const auto& old = r->plan;
+  // A retained geometry keeps its vertex count and exactly its triangle
+  // indices (gpu_scene.cpp:1651-1656); every instance keeps its geometry.
+  const bool topology =
+      r->has_scene && old.leaf_size == sh.leaf_size && same_words(old.gvo, p->geometry_vertex_offset, sh.geometries) &&
+      same_words(old.gvc, p->geometry_vertex_count, sh.geometries) &&
+      same_words(old.gto, p->geometry_triangle_offset, sh.geometries) &&
+      same_words(old.gtc, p->geometry_triangle_count, sh.geometries) &&
+      same_words(old.igeo, p->instance_geometry, sh.instances) && old.positions.size() == sh.vertex_words &&
+      same_words(old.triangles, p->triangles, sh.index_words);
+  if (!topology) return c;
+  if (!same_words(old.positions, p->positions, sh.vertex_words)) {
+    for (uint32_t g = 0; g < sh.geometries; ++g) {
+      const uint64_t first = 3ull * p->geometry_vertex_offset[g], words = 3ull * p->geometry_vertex_count[g];
+      if (words && std::memcmp(old.positions.data() + first, p->positions + first, words * 4) != 0) c.moved.push_back(g);
+    }
+  }
+  c.transforms = !same_words(old.transforms, p->instance_transform, 16ull * sh.instances);
+  c.materials = !same_words(old.materials, p->materials, 36ull * sh.materials);
+  c.instance_materials = !same_words(old.imat, p->instance_material, sh.instances);
+  c.environment = std::memcmp(old.environment, p->environment, sizeof old.environment) != 0;
+  c.refit = !c.moved.empty() || c.transforms;
+  if (!c.refit && !c.materials && !c.instance_materials && !c.environment) {
+    c.update = LOTUS_HIP_SCENE_UNCHANGED;
+    c.temporal_compatible = true;
+    return c;
+  }
+  c.update = c.refit ? LOTUS_HIP_SCENE_REFIT : LOTUS_HIP_SCENE_TABLES;
+  // TemporalSceneCompatible (vulkan_backend.cpp:262-289): the environment,
+  // the material table and the instances' slots are kept, and an emitting
+  // instance keeps its transform and its geometry's positions.
+  c.temporal_compatible = !c.materials && !c.instance_materials && !c.environment;
+  if (c.temporal_compatible && c.refit) {
+    std::vector<char> moved(sh.geometries, 0);
+    for (uint32_t g : c.moved) moved[g] = 1;
+    for (uint32_t i = 0; i < sh.instances && c.temporal_compatible; ++i) {
+      if (!material_emits(p->materials + 36ull * p->instance_material[i])) continue;
+      if (moved[p->instance_geometry[i]] ||
+          std::memcmp(old.transforms.data() + 16ull * i, p->instance_transform + 16ull * i, 64) != 0)
+        c.temporal_compatible = false;
+    }
+  }
+  return c;
+}
+
+void record_plan(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const PlanShape& sh, const Changes& c) {
+  auto& d = r->plan;
+  const bool all = c.update == LOTUS_HIP_SCENE_BUILD;
+  if (all) {
+    d.positions.assign(p->positions, p->positions + sh.vertex_words);
+    d.triangles.assign(p->triangles, p->triangles + sh.index_words);
+    d.gvo.assign(p->geometry_vertex_offset, p->geometry_vertex_offset + sh.geometries);
+    d.gvc.assign(p->geometry_vertex_count, p->geometry_vertex_count + sh.geometries);
+    d.gto.assign(p->geometry_triangle_offset, p->geometry_triangle_offset + sh.geometries);
+    d.gtc.assign(p->geometry_triangle_count, p->geometry_triangle_count + sh.geometries);
+    d.igeo.assign(p->instance_geometry, p->instance_geometry + sh.instances);
+  } else {
+    for (uint32_t g : c.moved) {
+      const uint64_t first = 3ull * p->geometry_vertex_offset[g], words = 3ull * p->geometry_vertex_count[g];
+      std::memcpy(d.positions.data() + first, p->positions + first, words * 4);
+    }
+  }
+  if (all || c.transforms) d.transforms.assign(p->instance_transform, p->instance_transform + 16ull * sh.instances);
+  if (all || c.materials) d.materials.assign(p->materials, p->materials + 36ull * sh.materials);
+  if (all || c.instance_materials) d.imat.assign(p->instance_material, p->instance_material + sh.instances);
+  std::memcpy(d.environment, p->environment, sizeof d.environment);
+  d.leaf_size = sh.leaf_size;
+}
+
+void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environment) {
+  const SceneBuffers& s = r->scene;
+  SceneView& v = r->view;
+  v.nodes = s.nodes;
+  v.tris = s.tris;
+  v.tri_of_global = s.tri_of_global;
+  v.instance_base = s.instance_base;
+  v.instance_material = s.instance_material;
+  v.instance_transform = s.instance_transform;
+  v.materials = s.materials;
+  v.light_offsets = s.light_offsets;
+  v.lights = s.lights;
+  v.light_count = s.light_count;
+  v.tri_count = sh.tri_count;
+  v.instance_count = sh.instances;
+  for (int c = 0; c < 3; ++c) v.environment[c] = environment[c];
+}
+
+uint64_t scene_device_bytes(const lotus_hip_renderer* r) {
+  const SceneBuffers& s = r->scene;
+  const uint64_t I = r->view.instance_count, G = r->plan.gvo.size();
+  return s.bytes + 4 * (s.vertex_words + s.index_words) + 4 * (2 * G + I) + 4 * (I + 1) + 68 * I +
+         144ull * s.material_count + 4 * I + uint64_t(s.light_count) * sizeof(LightRecord);
+}
+
+void fill_evidence(const lotus_hip_renderer* r, lotus_hip_scene_evidence* ev, uint32_t update, uint32_t moved,
+                   double upload_ms, double build_ms) {
+  if (!ev) return;
+  const SceneBuffers& s = r->scene;
+  ev->triangle_count = r->view.tri_count;
+  ev->instance_count = r->view.instance_count;
+  ev->material_count = s.material_count;
+  ev->light_count = s.light_count;
+  ev->node_count = s.node_count;
+  ev->bvh_depth = s.depth;
+  ev->leaf_count = s.leaf_count;
+  ev->bvh_width = kWidth;
+  ev->upload_ms = upload_ms;
+  ev->build_gpu_ms = build_ms;
+  ev->device_bytes = scene_device_bytes(r);
+  ev->update = update;
+  ev->moved_geometries = moved;
+  ev->bvh_builds = r->bvh_builds;
+  ev->bvh_refits = r->bvh_refits;
+  ev->material_writes = r->material_writes;
+  ev->light_writes = r->light_writes;
+  ev->uploaded_bytes = r->uploaded_bytes;
+}
+
+// A new scene: every buffer uploaded, a new BVH built.
+bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const PlanShape& sh, double& upload_ms,
+                 double& build_ms, std::string& error) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<uint32_t> offsets;
+  std::vector<LightRecord> lights;
+  build_light_table(p, offsets, lights);
+  free_scene(r);
+  SceneBuffers& s = r->scene;
+  const uint32_t G = sh.geometries, I = sh.instances, M = sh.materials;
+  hipError_t e = dev_alloc(s.positions, sh.vertex_words);
+  if (e == hipSuccess) e = dev_alloc(s.triangles, sh.index_words);
+  if (e == hipSuccess) e = dev_alloc(s.geometry_vertex_offset, G);
+  if (e == hipSuccess) e = dev_alloc(s.geometry_triangle_offset, G);
+  if (e == hipSuccess) e = dev_alloc(s.instance_geometry, I);
+  if (e == hipSuccess) e = dev_alloc(s.instance_base, I + 1);
+  if (e == hipSuccess) e = dev_alloc(s.instance_material, I);
+  if (e == hipSuccess) e = dev_alloc(s.instance_transform, 16ull * I);
+  if (e == hipSuccess) e = dev_alloc(s.materials, 36ull * M);
+  if (e == hipSuccess) e = dev_alloc(s.light_offsets, I);
+  if (e == hipSuccess) e = dev_alloc(s.lights, lights.size());
+  uint64_t bytes = 0;
+  auto up = [&](void* dst, const void* src, size_t n) {
+    if (e == hipSuccess && n) {
+      e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, r->stream);
+      bytes += n;
+    }
+  };
+  up(s.positions, p->positions, sh.vertex_words * 4);
+  up(s.triangles, p->triangles, sh.index_words * 4);
+  up(s.geometry_vertex_offset, p->geometry_vertex_offset, G * 4ull);
+  up(s.geometry_triangle_offset, p->geometry_triangle_offset, G * 4ull);
+  up(s.instance_geometry, p->instance_geometry, I * 4ull);
+  up(s.instance_base, sh.base.data(), sh.base.size() * 4);
+  up(s.instance_material, p->instance_material, I * 4ull);
+  up(s.instance_transform, p->instance_transform, 64ull * I);
+  up(s.materials, p->materials, 144ull * M);
+  up(s.light_offsets, offsets.data(), offsets.size() * 4);
+  up(s.lights, lights.data(), lights.size() * sizeof(LightRecord));
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (e != hipSuccess) {
+    error = std::string("scene upload: ") + hipGetErrorString(e);
+    return false;
+  }
+  s.vertex_words = sh.vertex_words;
+  s.index_words = sh.index_words;
+  s.material_count = M;
+  s.light_count = static_cast<uint32_t>(lights.size());
+  const BuildInput in{s.positions, s.triangles, s.geometry_vertex_offset, s.geometry_triangle_offset,
+                      s.instance_geometry, s.instance_transform, s.instance_base, I, sh.tri_count, sh.leaf_size};
+  if (!build_bvh(in, s, r->stream, build_ms, error)) return false;
+  e = ensure_launch(r);
+  if (e != hipSuccess) {
+    error = std::string("spill area: ") + hipGetErrorString(e);
+    return false;
+  }
+  set_view(r, sh, p->environment);
+  r->light_offsets_host = std::move(offsets);
+  r->lights_host = std::move(lights);
+  r->has_scene = true;
+  ++r->bvh_builds;
+  ++r->material_writes;
+  ++r->light_writes;
+  r->uploaded_bytes += bytes;
+  return true;
+}
+
+// The same topology: only changed sections are uploaded (moved geometries'
+// positions, transforms, tables), then a refit when anything moved.
+bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const PlanShape& sh, const Changes& c,
+                   double& upload_ms, double& build_ms, std::string& error) {
+  const auto t0 = std::chrono::steady_clock::now();
+  SceneBuffers& s = r->scene;
+  const uint32_t I = sh.instances, M = sh.materials;
+  std::vector<uint32_t> offsets;
+  std::vector<LightRecord> lights;
+  build_light_table(p, offsets, lights);
+  const bool lights_changed =
+      offsets != r->light_offsets_host || lights.size() != r->lights_host.size() ||
+      (!lights.empty() && std::memcmp(lights.data(), r->lights_host.data(), lights.size() * sizeof(LightRecord)) != 0);
+  hipError_t e = hipSuccess;
+  uint64_t bytes = 0;
+  auto up = [&](void* dst, const void* src, size_t n) {
+    if (e == hipSuccess && n) {
+      e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, r->stream);
+      bytes += n;
+    }
+  };
+  for (uint32_t g : c.moved) {
+    const uint64_t first = 3ull * p->geometry_vertex_offset[g];
+    up(s.positions + first, p->positions + first, 12ull * p->geometry_vertex_count[g]);
+  }
+  if (c.transforms) up(s.instance_transform, p->instance_transform, 64ull * I);
+  if (c.instance_materials) up(s.instance_material, p->instance_material, 4ull * I);
+  if (c.materials) {
+    if (s.material_count != M) {
+      dev_free(s.materials);
+      e = dev_alloc(s.materials, 36ull * M);
+      s.material_count = M;
+    }
+    up(s.materials, p->materials, 144ull * M);
+  }
+  if (lights_changed) {
+    if (s.light_count != lights.size() && e == hipSuccess) {
+      dev_free(s.lights);
+      e = dev_alloc(s.lights, lights.size());
+      s.light_count = static_cast<uint32_t>(lights.size());
+    }
+    up(s.light_offsets, offsets.data(), offsets.size() * 4);
+    up(s.lights, lights.data(), lights.size() * sizeof(LightRecord));
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (e != hipSuccess) {
+    error = std::string("scene upload: ") + hipGetErrorString(e);
+    return false;
+  }
+  if (c.refit) {
+    const BuildInput in{s.positions, s.triangles, s.geometry_vertex_offset, s.geometry_triangle_offset,
+                        s.instance_geometry, s.instance_transform, s.instance_base, I, sh.tri_count, sh.leaf_size};
+    if (!refit_bvh(in, s, r->stream, build_ms, error)) return false;
+    ++r->bvh_refits;
+  }
+  if (c.materials) ++r->material_writes;
+  if (lights_changed) ++r->light_writes;
+  r->uploaded_bytes += bytes;
+  r->light_offsets_host = std::move(offsets);
+  r->lights_host = std::move(lights);
+  set_view(r, sh, p->environment);
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -611,114 +930,32 @@ int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, 
     return LOTUS_HIP_FAIL;
   }
   (void)hipSetDevice(r->device);
-  const auto t0 = std::chrono::steady_clock::now();
-  std::vector<uint32_t> base(p->instance_count + 1, 0);
-  for (uint32_t i = 0; i < p->instance_count; ++i)
-    base[i + 1] = base[i] + p->geometry_triangle_count[p->instance_geometry[i]];
-  const uint32_t tri_count = base[p->instance_count];
-  uint64_t vertex_words = 0, index_words = 0;
-  for (uint32_t g = 0; g < p->geometry_count; ++g) {
-    vertex_words = std::max<uint64_t>(vertex_words, 3ull * (uint64_t(p->geometry_vertex_offset[g]) + p->geometry_vertex_count[g]));
-    index_words = std::max<uint64_t>(index_words, 3ull * (uint64_t(p->geometry_triangle_offset[g]) + p->geometry_triangle_count[g]));
+  const PlanShape shape = plan_shape(p);
+  const Changes c = compare_plan(r, p, shape);
+  if (c.update == LOTUS_HIP_SCENE_UNCHANGED) {
+    fill_evidence(r, ev, c.update, 0, 0.0, 0.0);
+    set_detail(detail, detail_size, "");
+    return LOTUS_HIP_PASS;
   }
-  std::vector<uint32_t> offsets;
-  std::vector<LightRecord> lights;
-  build_light_table(p, offsets, lights);
-
-  free_scene(r);
-  SceneBuffers& s = r->scene;
-  float* positions = nullptr;
-  uint32_t *triangles = nullptr, *gvo = nullptr, *gto = nullptr, *igeo = nullptr;
-  hipError_t e = dev_alloc(positions, vertex_words);
-  if (e == hipSuccess) e = dev_alloc(triangles, index_words);
-  if (e == hipSuccess) e = dev_alloc(gvo, p->geometry_count);
-  if (e == hipSuccess) e = dev_alloc(gto, p->geometry_count);
-  if (e == hipSuccess) e = dev_alloc(igeo, p->instance_count);
-  if (e == hipSuccess) e = dev_alloc(s.instance_base, p->instance_count + 1);
-  if (e == hipSuccess) e = dev_alloc(s.instance_material, p->instance_count);
-  if (e == hipSuccess) e = dev_alloc(s.instance_transform, 16ull * p->instance_count);
-  if (e == hipSuccess) e = dev_alloc(s.materials, 36ull * p->material_count);
-  if (e == hipSuccess) e = dev_alloc(s.light_offsets, p->instance_count);
-  if (e == hipSuccess) e = dev_alloc(s.lights, lights.size());
-  auto up = [&](void* dst, const void* src, size_t bytes) {
-    if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, r->stream);
-  };
-  up(positions, p->positions, vertex_words * 4);
-  up(triangles, p->triangles, index_words * 4);
-  up(gvo, p->geometry_vertex_offset, p->geometry_count * 4ull);
-  up(gto, p->geometry_triangle_offset, p->geometry_count * 4ull);
-  up(igeo, p->instance_geometry, p->instance_count * 4ull);
-  up(s.instance_base, base.data(), base.size() * 4);
-  up(s.instance_material, p->instance_material, p->instance_count * 4ull);
-  up(s.instance_transform, p->instance_transform, 64ull * p->instance_count);
-  up(s.materials, p->materials, 144ull * p->material_count);
-  up(s.light_offsets, offsets.data(), offsets.size() * 4);
-  up(s.lights, lights.data(), lights.size() * sizeof(LightRecord));
-  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-  const auto t1 = std::chrono::steady_clock::now();
   std::string error;
-  double build_ms = 0.0;
-  bool ok = e == hipSuccess;
-  if (ok) {
-    BuildInput in{positions, triangles, gvo, gto, igeo, s.instance_transform, s.instance_base,
-                  p->instance_count, tri_count, p->bvh_leaf_size ? p->bvh_leaf_size : 2u};
-    ok = build_bvh(in, s, r->stream, build_ms, error);
-  } else {
-    error = std::string("scene upload: ") + hipGetErrorString(e);
-  }
-  dev_free(positions);
-  dev_free(triangles);
-  dev_free(gvo);
-  dev_free(gto);
-  dev_free(igeo);
-  if (ok) {
-    e = ensure_launch(r);
-    if (e != hipSuccess) {
-      ok = false;
-      error = std::string("spill area: ") + hipGetErrorString(e);
-    }
-  }
+  double build_ms = 0.0, upload_ms = 0.0;
+  const bool ok = c.update == LOTUS_HIP_SCENE_BUILD ? apply_build(r, p, shape, upload_ms, build_ms, error)
+                                                     : apply_changes(r, p, shape, c, upload_ms, build_ms, error);
   if (!ok) {
+    // A partly applied plan leaves the scene unknown (vulkan_backend.cpp:2629-2633).
     free_scene(r);
     r->failure = error;
     set_detail(detail, detail_size, error);
     return LOTUS_HIP_FAIL;
   }
-  s.bytes += vertex_words * 0 + (p->instance_count + 1) * 4ull + p->instance_count * 68ull + 144ull * p->material_count +
-             p->instance_count * 4ull + lights.size() * sizeof(LightRecord);
-  SceneView& v = r->view;
-  v.nodes = s.nodes;
-  v.tris = s.tris;
-  v.tri_of_global = s.tri_of_global;
-  v.instance_base = s.instance_base;
-  v.instance_material = s.instance_material;
-  v.instance_transform = s.instance_transform;
-  v.materials = s.materials;
-  v.light_offsets = s.light_offsets;
-  v.lights = s.lights;
-  v.light_count = static_cast<uint32_t>(lights.size());
-  v.tri_count = tri_count;
-  v.instance_count = p->instance_count;
-  for (int c = 0; c < 3; ++c) v.environment[c] = p->environment[c];
-  r->light_offsets_host = offsets;
-  r->lights_host = lights;
-  r->has_scene = true;
-  r->bounds_valid = false;
+  record_plan(r, p, shape, c);
+  // Any change restarts the accumulation (vulkan_backend.cpp:2621-2626);
+  // temporal history survives only a compatible one (:262-289, 1782-1788).
   ++r->scene_generation;
+  if (!c.temporal_compatible) ++r->scene_epoch;
   r->rounds_hint = 0;
-  if (ev) {
-    ev->triangle_count = tri_count;
-    ev->instance_count = p->instance_count;
-    ev->material_count = p->material_count;
-    ev->light_count = static_cast<uint32_t>(lights.size());
-    ev->node_count = s.node_count;
-    ev->bvh_depth = s.depth;
-    ev->leaf_count = s.leaf_count;
-    ev->bvh_width = kWidth;
-    ev->upload_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    ev->build_gpu_ms = build_ms;
-    ev->device_bytes = s.bytes;
-  }
+  if (c.update != LOTUS_HIP_SCENE_TABLES) r->bounds_valid = false;
+  fill_evidence(r, ev, c.update, static_cast<uint32_t>(c.moved.size()), upload_ms, build_ms);
   set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;
 }
@@ -1217,9 +1454,67 @@ int lotus_hip_light_table(lotus_hip_renderer* r, uint32_t* offsets, uint32_t* re
     set_detail(detail, detail_size, "no scene has been applied");
     return LOTUS_HIP_FAIL;
   }
-  if (count) *count = static_cast<uint32_t>(r->lights_host.size());
-  if (offsets) std::memcpy(offsets, r->light_offsets_host.data(), r->light_offsets_host.size() * 4);
-  if (records) std::memcpy(records, r->lights_host.data(), r->lights_host.size() * sizeof(LightRecord));
+  // The tables the kernels read, copied from the device.
+  const SceneBuffers& s = r->scene;
+  if (count) *count = s.light_count;
+  (void)hipSetDevice(r->device);
+  hipError_t e = hipSuccess;
+  if (offsets && r->view.instance_count)
+    e = hipMemcpy(offsets, s.light_offsets, size_t(r->view.instance_count) * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && records && s.light_count)
+    e = hipMemcpy(records, s.lights, size_t(s.light_count) * sizeof(LightRecord), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "light table readback", e));
+    return LOTUS_HIP_FAIL;
+  }
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_readback_scene(lotus_hip_renderer* r, lotus_hip_scene_contents* out, char* detail,
+                             size_t detail_size) {
+  if (!r || !out) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (!r->has_scene) {
+    set_detail(detail, detail_size, "no scene has been applied");
+    return LOTUS_HIP_FAIL;
+  }
+  const SceneBuffers& s = r->scene;
+  const auto& plan = r->plan;
+  const uint32_t G = static_cast<uint32_t>(plan.gvo.size()), I = r->view.instance_count;
+  out->geometry_count = G;
+  out->instance_count = I;
+  out->material_count = s.material_count;
+  out->triangle_count = r->view.tri_count;
+  out->node_count = s.node_count;
+  out->light_count = s.light_count;
+  out->position_floats = s.vertex_words;
+  out->index_words = s.index_words;
+  for (int c = 0; c < 3; ++c) out->environment[c] = r->view.environment[c];
+  (void)hipSetDevice(r->device);
+  hipError_t e = hipSuccess;
+  auto down = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess && dst && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  };
+  down(out->positions, s.positions, s.vertex_words * 4);
+  down(out->triangles, s.triangles, s.index_words * 4);
+  down(out->geometry_vertex_offset, s.geometry_vertex_offset, G * 4ull);
+  down(out->geometry_triangle_offset, s.geometry_triangle_offset, G * 4ull);
+  down(out->instance_geometry, s.instance_geometry, I * 4ull);
+  down(out->instance_transform, s.instance_transform, I * 64ull);
+  down(out->instance_material, s.instance_material, I * 4ull);
+  down(out->materials, s.materials, s.material_count * 144ull);
+  down(out->bvh_nodes, s.nodes, size_t(s.node_count) * sizeof(WideNode));
+  down(out->bvh_triangles, s.tris, size_t(r->view.tri_count) * sizeof(Tri));
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "scene readback", e));
+    return LOTUS_HIP_FAIL;
+  }
+  if (out->geometry_vertex_count && G) std::memcpy(out->geometry_vertex_count, plan.gvc.data(), G * 4ull);
+  if (out->geometry_triangle_count && G) std::memcpy(out->geometry_triangle_count, plan.gtc.data(), G * 4ull);
+  set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;
 }
 
@@ -1470,7 +1765,11 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   else if (P->light_sampling != t.light_sampling || P->max_bounces != t.max_bounces ||
            (restir && (P->restir_candidates != t.candidates || P->restir_proposal != t.proposal)))
     reason = 7;                                                   // RenderSettingsChanged
-  else if (r->scene_generation != t.scene_generation) reason = 8;  // SceneChanged
+  // SceneChanged (vulkan_backend.cpp:1782-1788): an incompatible scene change,
+  // or any change while temporal reservoirs reuse light samples of an emitting scene.
+  else if (r->scene_epoch != t.scene_epoch ||
+           (r->scene_generation != t.scene_generation && restir && ts->restir_temporal && r->view.light_count != 0))
+    reason = 8;
   (void)hipSetDevice(r->device);
   hipError_t e = ensure_temporal(r, P->width, P->height);
   if (e != hipSuccess) {
@@ -1535,6 +1834,7 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   t.frame_index = ts->frame_index;
   t.history_key = ts->history_key;
   t.scene_generation = r->scene_generation;
+  t.scene_epoch = r->scene_epoch;
   std::memcpy(t.view, ts->view, 64);
   vulkan_world_to_clip(P->world_to_clip, t.world_to_clip);
   resolved_windows(P, t.display, t.data);

@@ -31,8 +31,10 @@ SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hi
            "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning",
            "lotus_hip_cache_configure", "lotus_hip_cache_infer", "lotus_hip_cache_gradients",
            "lotus_hip_cache_train", "lotus_hip_cache_parameters", "lotus_hip_render_temporal", "lotus_hip_svgf",
-           "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune"]
+           "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune", "lotus_hip_readback_scene"]
 BATCH_GRAPH = 1
+# What a scene apply did (lotus_hip_scene_evidence.update).
+SCENE_UNCHANGED, SCENE_TABLES, SCENE_REFIT, SCENE_BUILD = range(4)
 
 # The reference's temporal records (include/lotus/temporal.hpp:193-245).
 TEMPORAL_PIXEL = np.dtype([("normal_depth", "<f4", 4), ("motion_depth", "<f4", 4), ("instance_id", "<u8"),
@@ -80,6 +82,39 @@ class SceneEvidence(C.Structure):
         ("upload_ms", C.c_double),
         ("build_gpu_ms", C.c_double),
         ("device_bytes", C.c_uint64),
+        ("update", C.c_uint32),
+        ("moved_geometries", C.c_uint32),
+        ("bvh_builds", C.c_uint64),
+        ("bvh_refits", C.c_uint64),
+        ("material_writes", C.c_uint64),
+        ("light_writes", C.c_uint64),
+        ("uploaded_bytes", C.c_uint64),
+    ]
+
+
+class SceneContents(C.Structure):
+    _fields_ = [
+        ("geometry_count", C.c_uint32),
+        ("instance_count", C.c_uint32),
+        ("material_count", C.c_uint32),
+        ("triangle_count", C.c_uint32),
+        ("node_count", C.c_uint32),
+        ("light_count", C.c_uint32),
+        ("position_floats", C.c_uint64),
+        ("index_words", C.c_uint64),
+        ("environment", C.c_float * 3),
+        ("positions", _f32p),
+        ("triangles", _u32p),
+        ("geometry_vertex_offset", _u32p),
+        ("geometry_vertex_count", _u32p),
+        ("geometry_triangle_offset", _u32p),
+        ("geometry_triangle_count", _u32p),
+        ("instance_geometry", _u32p),
+        ("instance_transform", _f32p),
+        ("instance_material", _u32p),
+        ("materials", _u32p),
+        ("bvh_nodes", _u32p),
+        ("bvh_triangles", _f32p),
     ]
 
 
@@ -218,6 +253,8 @@ def lib():
         L.lotus_hip_trace_device.restype = C.c_int
         L.lotus_hip_trace_device.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int,
                                              C.c_void_p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_readback_scene.restype = C.c_int
+        L.lotus_hip_readback_scene.argtypes = [C.c_void_p, C.POINTER(SceneContents), C.c_char_p, C.c_size_t]
         L.lotus_hip_light_table.restype = C.c_int
         L.lotus_hip_light_table.argtypes = [C.c_void_p, _u32p, _u32p, _u32p, C.c_char_p, C.c_size_t]
         L.lotus_hip_set_tuning.restype = C.c_int
@@ -440,6 +477,32 @@ class Renderer:
                                          int(any_hit), C.c_void_p(stream_ptr), buf, 512)
         if s != PASS:
             raise LotusError(s, buf.value.decode())
+
+    def readback_scene(self) -> dict:
+        """ReadBackScene: the resident scene copied from the device."""
+        c = SceneContents()
+        buf = C.create_string_buffer(512)
+        s = lib().lotus_hip_readback_scene(self.handle, C.byref(c), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        G, I, M = c.geometry_count, c.instance_count, c.material_count
+        out = dict(
+            positions=np.zeros(c.position_floats, np.float32), triangles=np.zeros(c.index_words, np.uint32),
+            geometry_vertex_offset=np.zeros(G, np.uint32), geometry_vertex_count=np.zeros(G, np.uint32),
+            geometry_triangle_offset=np.zeros(G, np.uint32), geometry_triangle_count=np.zeros(G, np.uint32),
+            instance_geometry=np.zeros(I, np.uint32), instance_transform=np.zeros((I, 16), np.float32),
+            instance_material=np.zeros(I, np.uint32), materials=np.zeros((M, 36), np.uint32),
+            bvh_nodes=np.zeros((c.node_count, 32), np.uint32),
+            bvh_triangles=np.zeros((c.triangle_count, 12), np.float32))
+        for name, a in out.items():
+            t = C.c_float if a.dtype == np.float32 else C.c_uint32
+            setattr(c, name, _p(a, t) if a.size else None)
+        s = lib().lotus_hip_readback_scene(self.handle, C.byref(c), buf, 512)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+        out["environment"] = np.array(list(c.environment), np.float32)
+        out["light_count"] = c.light_count
+        return out
 
     def light_table(self):
         count = np.zeros(1, np.uint32)

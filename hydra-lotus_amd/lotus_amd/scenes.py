@@ -93,6 +93,30 @@ class Scene:
         self._packed = None
         return len(self.materials) - 1
 
+    def copy(self) -> "Scene":
+        """An independent copy: editing its arrays leaves this scene as it is."""
+        return Scene(geometries=[(p.copy(), t.copy()) for p, t in self.geometries],
+                     instances=[(g, t.copy(), m) for g, t, m in self.instances],
+                     materials=[m.copy() for m in self.materials], environment=tuple(self.environment),
+                     view=self.view.copy(), projection=self.projection.copy(), name=self.name)
+
+    def set_positions(self, geometry: int, positions) -> None:
+        """Moves a geometry's vertices, keeping its vertex count and indices."""
+        pos = np.ascontiguousarray(positions, F32).reshape(-1, 3)
+        if pos.shape != self.geometries[geometry][0].shape:
+            raise ValueError("a moved geometry keeps its vertex count")
+        self.geometries[geometry] = (pos, self.geometries[geometry][1])
+        self._packed = None
+
+    def set_transform(self, instance: int, transform) -> None:
+        g, _, m = self.instances[instance]
+        self.instances[instance] = (g, np.ascontiguousarray(transform, F32).reshape(16), m)
+        self._packed = None
+
+    def set_material(self, slot: int, record) -> None:
+        self.materials[slot] = np.ascontiguousarray(record, np.uint32)
+        self._packed = None
+
     @property
     def world_to_clip(self) -> np.ndarray:
         """camera.projection * camera.view (core/render-extraction/extraction.cpp:12-19)."""
@@ -521,6 +545,18 @@ def heightfield(cells: int = 1024, seed: int = 2026, amplitude: float = 0.2, til
         m = sc.add_material(preview_material(base_color=(0.0, 0.0, 0.0), emission=(radiance,) * 3))
         sc.add_mesh(pos[list(tri)], [[0, 1, 2]], m)
     return sc
+
+
+def heightfield_wave(sc: Scene, time: float, amplitude: float = 0.05) -> Scene:
+    """The heightfield animated: every vertex raised by amplitude x
+    sin(4x + 3z + time), the same vertices, triangles and instances (a
+    deformation that keeps each mesh's topology)."""
+    out = sc.copy()
+    for g, (pos, _) in enumerate(out.geometries):
+        moved = pos.copy()
+        moved[:, 1] += F32(amplitude) * np.sin(F32(4.0) * pos[:, 0] + F32(3.0) * pos[:, 2] + F32(time)).astype(F32)
+        out.set_positions(g, moved)
+    return out
 
 
 def random_triangles(count: int, seed: int = 1, extent: float = 1.0, size: float = 0.2) -> Scene:

@@ -41,7 +41,9 @@ typedef struct lotus_hip_renderer lotus_hip_renderer;
  * (SceneUpdate, include/lotus/extraction.hpp:91-132, applied by
  * GpuScene::Apply, backend/vulkan/gpu_scene.cpp:1606): object-space
  * geometries placed by instances, and the material table as
- * GpuMaterialRecord words (gpu_scene.hpp:94-113, 36 words each). */
+ * GpuMaterialRecord words (gpu_scene.hpp:94-113, 36 words each). The
+ * renderer compares a plan with the resident one to apply only what changed
+ * (see lotus_hip_scene_evidence.update). */
 typedef struct lotus_hip_scene_plan {
   uint32_t geometry_count;
   const float* positions;                   /* xyz per vertex, all geometries */
@@ -60,6 +62,26 @@ typedef struct lotus_hip_scene_plan {
   uint32_t bvh_leaf_size;                   /* 0: default (2); at most 8 */
 } lotus_hip_scene_plan;
 
+/* What a lotus_hip_scene_apply did, the counterpart of GpuScene::Apply's
+ * retain / refit / rebuild decisions (gpu_scene.cpp:1638-1725; rules in
+ * docs/reference/SCENE.md "Acceleration structures"). The plan is compared
+ * with the resident one: */
+#define LOTUS_HIP_SCENE_UNCHANGED 0u /* an equal plan: nothing is written, the
+                                        accumulation continues (an empty
+                                        SceneUpdate, vulkan_backend.cpp:2621-2626) */
+#define LOTUS_HIP_SCENE_TABLES 1u    /* same geometry, indices, instances and
+                                        transforms: the material table, the
+                                        instances' material slots, the light
+                                        table or the environment rewritten, the
+                                        BVH kept */
+#define LOTUS_HIP_SCENE_REFIT 2u     /* every geometry keeps its vertex count and
+                                        exact triangle indices and every instance
+                                        its geometry: changed positions and
+                                        transforms re-place the triangles and
+                                        refit the BVH's boxes in place (the
+                                        BLAS point refit and TLAS update) */
+#define LOTUS_HIP_SCENE_BUILD 3u     /* anything else (or the first plan): a new BVH */
+
 /* GpuSceneEvidence / GpuSceneStats counterpart (vulkan_backend.hpp:412-506). */
 typedef struct lotus_hip_scene_evidence {
   uint32_t triangle_count;
@@ -71,8 +93,18 @@ typedef struct lotus_hip_scene_evidence {
   uint32_t leaf_count;
   uint32_t bvh_width;        /* children per node */
   double upload_ms;          /* host to device copies, wall */
-  double build_gpu_ms;       /* BVH build kernels, hipEvent */
+  double build_gpu_ms;       /* this call's BVH build or refit kernels, hipEvent; 0 if neither ran */
   uint64_t device_bytes;     /* scene allocations */
+  uint32_t update;           /* LOTUS_HIP_SCENE_* */
+  uint32_t moved_geometries; /* geometries whose positions this call re-uploaded (refits) */
+  /* Lifetime counts since the renderer was created (GpuSceneStats
+   * blas_builds / blas_updates, material_writes, light_writes,
+   * uploaded_bytes); an unchanged plan changes none of them. */
+  uint64_t bvh_builds;
+  uint64_t bvh_refits;
+  uint64_t material_writes;
+  uint64_t light_writes;
+  uint64_t uploaded_bytes;
 } lotus_hip_scene_evidence;
 
 /* SceneOutput (vulkan_backend.hpp:624-637). */
@@ -232,6 +264,40 @@ int lotus_hip_get_tuning(lotus_hip_renderer* renderer, lotus_hip_tuning* tuning)
 int lotus_hip_light_table(lotus_hip_renderer* renderer, uint32_t* offsets,
                           uint32_t* records, uint32_t* count, char* detail,
                           size_t detail_size);
+
+/* OffscreenRenderer::ReadBackScene (vulkan_backend.hpp:821, GpuSceneContents
+ * :600-615): the resident scene as the device holds it (geometry, instance
+ * records, material table, the wide BVH and its triangles), copied back.
+ * Counts are always written; each array is filled when it is not NULL and
+ * must hold what the counts say. Per-geometry vertex and triangle counts are
+ * the applied plan's (the device keeps only offsets). */
+typedef struct lotus_hip_scene_contents {
+  uint32_t geometry_count;
+  uint32_t instance_count;
+  uint32_t material_count;
+  uint32_t triangle_count;    /* flattened (instance, primitive) triangles */
+  uint32_t node_count;        /* wide BVH nodes */
+  uint32_t light_count;       /* light table entries */
+  uint64_t position_floats;   /* length of `positions` (3 per vertex) */
+  uint64_t index_words;       /* length of `triangles` (3 per triangle) */
+  float environment[3];
+  float* positions;
+  uint32_t* triangles;
+  uint32_t* geometry_vertex_offset;   /* geometry_count each */
+  uint32_t* geometry_vertex_count;
+  uint32_t* geometry_triangle_offset;
+  uint32_t* geometry_triangle_count;
+  uint32_t* instance_geometry;        /* instance_count */
+  float* instance_transform;          /* 16 per instance, column-major */
+  uint32_t* instance_material;        /* instance_count */
+  uint32_t* materials;                /* 36 words per material */
+  uint32_t* bvh_nodes;                /* 32 words per node: child boxes lo/hi x, y, z (4 each), children, padding */
+  float* bvh_triangles;               /* 12 per triangle in leaf order: corners xyz with the flattened id,
+                                         instance and primitive as the w words' bits */
+} lotus_hip_scene_contents;
+
+int lotus_hip_readback_scene(lotus_hip_renderer* renderer, lotus_hip_scene_contents* contents, char* detail,
+                             size_t detail_size);
 
 /* The accumulation as sums: 4 floats per pixel (rgb, hit samples), rows from
  * the top, and the count of samples added. */

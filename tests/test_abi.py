@@ -49,14 +49,16 @@ def test_struct_sizes_match_header():
 #include <stddef.h>
 #include "lotus_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lotus_hip_scene_plan),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lotus_hip_scene_plan),
          sizeof(lotus_hip_scene_evidence),
          sizeof(lotus_hip_frame_params), sizeof(lotus_hip_frame_result),
          offsetof(lotus_hip_frame_result, traversal_nodes), offsetof(lotus_hip_scene_plan, bvh_leaf_size),
          sizeof(lotus_hip_cache_config), offsetof(lotus_hip_cache_config, resolution),
          offsetof(lotus_hip_frame_result, cache_loss), sizeof(lotus_hip_temporal_settings),
          offsetof(lotus_hip_temporal_settings, reset_reason), sizeof(lotus_hip_tuning),
-         offsetof(lotus_hip_tuning, tail_divisor));
+         offsetof(lotus_hip_tuning, tail_divisor), offsetof(lotus_hip_scene_evidence, update),
+         sizeof(lotus_hip_scene_contents), offsetof(lotus_hip_scene_contents, environment),
+         offsetof(lotus_hip_scene_contents, bvh_triangles));
   return 0;
 }'''
     import tempfile
@@ -79,4 +81,8 @@ int main(void) {
     assert sizes[10] == hip.TemporalSettings.reset_reason.offset
     assert sizes[11] == C.sizeof(hip.Tuning)
     assert sizes[12] == hip.Tuning.tail_divisor.offset
+    assert sizes[13] == hip.SceneEvidence.update.offset
+    assert sizes[14] == C.sizeof(hip.SceneContents)
+    assert sizes[15] == hip.SceneContents.environment.offset
+    assert sizes[16] == hip.SceneContents.bvh_triangles.offset
     assert hip.TEMPORAL_PIXEL.itemsize == 64 and hip.TEMPORAL_RADIANCE.itemsize == 32

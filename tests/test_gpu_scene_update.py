@@ -1,0 +1,268 @@
+"""Incremental scene updates through the C ABI (GpuScene::Apply's retain /
+refit / rebuild rules, backend/vulkan/gpu_scene.cpp:1638-1725, and
+docs/reference/SCENE.md "Acceleration structures").
+
+Bars: after a refit, closest-hit ids and any-hit results equal the oracle's
+brute force on the moved scene exactly, a rendered frame equals a fresh
+renderer's frame of the same plan bit for bit (the same triangles in world
+space give the same hits whatever the tree), and a refit back to the first
+positions reproduces the first build's nodes and triangles bit for bit.
+Update kinds, lifetime counters and the accumulation/temporal restarts follow
+the reference's rules (vulkan_backend.cpp:262-289, 1782-1788, 2621-2626).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from lotus_amd import hip, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    status, detail = hip.probe()
+    assert status == hip.PASS, detail
+    r = hip.Renderer(0)
+    yield r
+    r.close()
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rays(n, seed, box=False):
+    """Heightfield: from above the field towards points under it, so nearly
+    every ray crosses the surface. Box: from inside the box, any direction."""
+    u = scenes.uniform(seed, 11, np.arange(n * 6)).reshape(n, 6)
+    if box:
+        o = (u[:, :3] * 2 - 1) * 0.9
+        d = u[:, 3:] * 2 - 1
+    else:
+        o = np.stack([(u[:, 0] * 2 - 1) * 1.2, 1.0 + 2.0 * u[:, 1], (u[:, 2] * 2 - 1) * 1.2], -1)
+        t = np.stack([(u[:, 3] * 2 - 1) * 0.9, np.full(n, -0.6), (u[:, 5] * 2 - 1) * 0.9], -1)
+        d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, 3.0e38
+    return rays
+
+
+def _check_against_oracle(renderer, sc, seed, box=False):
+    o = oracle.OracleScene(sc)
+    rays = _rays(6000, seed, box)
+    g, c = renderer.trace(rays), o.trace(rays)
+    assert np.array_equal(g[:, :2], c[:, :2]), f"{np.sum(np.any(g[:, :2] != c[:, :2], 1))} id mismatches"
+    hit = c[:, 0] != hip.MISS
+    assert hit.mean() > (0.7 if box else 0.9)  # the box is open at its front
+    assert np.array_equal(g[hit, 2:], c[hit, 2:])
+    assert np.array_equal(renderer.trace(rays, any_hit=True), o.occluded(rays))
+    off_g, rec_g = renderer.light_table()
+    off_c, rec_c = o.light_table()
+    assert np.array_equal(off_g, off_c) and np.array_equal(rec_g, rec_c)
+
+
+def _frame(r, sc):
+    return r.render(sc.world_to_clip, 96, 54, frame_count=1, max_bounces=6, light_sampling=hip.LIGHT_MIS)
+
+
+def test_refit_matches_oracle_and_a_fresh_build(renderer):
+    _say("refit: heightfield 64 cells, wave deformation and a moved light")
+    a = scenes.heightfield(cells=64, emissive=16, tiles=4)
+    ev = renderer.apply(a)
+    assert ev.update == hip.SCENE_BUILD
+    builds, refits = ev.bvh_builds, ev.bvh_refits
+    first = renderer.readback_scene()
+    pa = a.packed()
+    assert np.array_equal(first["positions"], pa["positions"].reshape(-1))
+    assert np.array_equal(first["triangles"], pa["triangles"].reshape(-1))
+    assert np.array_equal(first["materials"], pa["materials"])
+
+    b = scenes.heightfield_wave(a, 0.7)
+    light = len(b.instances) - 1  # an emissive single-triangle mesh
+    t = scenes.identity()
+    t[13] = 0.03  # column-major translation y
+    b.set_transform(light, t)
+    ev = renderer.apply(b)
+    assert ev.update == hip.SCENE_REFIT
+    assert ev.bvh_builds == builds and ev.bvh_refits == refits + 1
+    assert ev.moved_geometries == len(b.geometries)
+    assert ev.node_count == first["bvh_nodes"].shape[0]
+    assert ev.build_gpu_ms > 0
+    _check_against_oracle(renderer, b, seed=5)
+    got = renderer.readback_scene()
+    pb = b.packed()
+    assert np.array_equal(got["positions"], pb["positions"].reshape(-1))
+    assert np.array_equal(got["instance_transform"], pb["instance_transform"])
+    # leaf order kept: the same flattened ids in the same slots
+    assert np.array_equal(got["bvh_triangles"][:, 3].view(np.uint32), first["bvh_triangles"][:, 3].view(np.uint32))
+    # every node box holds its children's triangles
+    _check_boxes(got["bvh_nodes"], got["bvh_triangles"])
+
+    refit_frame = _frame(renderer, b)
+    fresh = hip.Renderer(0)
+    try:
+        ev_fresh = fresh.apply(b)
+        assert ev_fresh.update == hip.SCENE_BUILD
+        built_frame = _frame(fresh, b)
+    finally:
+        fresh.close()
+    assert np.array_equal(refit_frame["counts"], built_frame["counts"])
+    assert refit_frame["shadow_rays"] == built_frame["shadow_rays"]
+    assert np.array_equal(refit_frame["color"], built_frame["color"])
+
+    # back to the first positions and transforms: the first build, bit for bit
+    ev = renderer.apply(a)
+    assert ev.update == hip.SCENE_REFIT and ev.bvh_refits == refits + 2
+    back = renderer.readback_scene()
+    assert np.array_equal(back["bvh_nodes"], first["bvh_nodes"])
+    assert np.array_equal(back["bvh_triangles"].view(np.uint32), first["bvh_triangles"].view(np.uint32))
+    _check_against_oracle(renderer, a, seed=6)
+
+
+def _check_boxes(nodes, tris):
+    """Each non-empty child box contains its subtree's triangles (CPU walk)."""
+    corners = tris[:, :12].reshape(-1, 3, 4)[:, :, :3]
+    lo_t, hi_t = corners.min(1), corners.max(1)
+    child = nodes[:, 24:28]
+    boxes = nodes[:, :24].view(np.float32).reshape(-1, 6, 4)  # lox hix loy hiy loz hiz
+    lo = np.stack([boxes[:, 0], boxes[:, 2], boxes[:, 4]], -1)  # node, child, axis
+    hi = np.stack([boxes[:, 1], boxes[:, 3], boxes[:, 5]], -1)
+    sub_lo = np.full((nodes.shape[0], 3), np.inf, np.float32)
+    sub_hi = np.full((nodes.shape[0], 3), -np.inf, np.float32)
+    for w in range(nodes.shape[0] - 1, -1, -1):  # children have larger indices
+        for c in range(4):
+            word = int(child[w, c])
+            if word == 0xFFFFFFFF:
+                continue
+            if word & 0x80000000:
+                f, n = word & 0x07FFFFFF, ((word >> 27) & 15) + 1
+                blo, bhi = lo_t[f:f + n].min(0), hi_t[f:f + n].max(0)
+            else:
+                assert word > w
+                blo, bhi = sub_lo[word], sub_hi[word]
+            assert np.all(lo[w, c] <= blo) and np.all(hi[w, c] >= bhi), (w, c)
+            sub_lo[w] = np.minimum(sub_lo[w], blo)
+            sub_hi[w] = np.maximum(sub_hi[w], bhi)
+
+
+def test_unchanged_plan_keeps_the_accumulation(renderer):
+    _say("unchanged plan")
+    sc = scenes.cornell_lambert()
+    renderer.apply(sc)
+    a = renderer.render(sc.world_to_clip, 24, 24, frame_count=2, max_bounces=4)
+    assert a["spp"] == 2
+    ev0 = renderer.apply(sc.copy())
+    assert ev0.update == hip.SCENE_UNCHANGED
+    b = renderer.render(sc.world_to_clip, 24, 24, frame_count=2, max_bounces=4)
+    assert b["spp"] == 4  # an empty update leaves the accumulation running
+    ev1 = renderer.apply(sc)
+    assert ev1.update == hip.SCENE_UNCHANGED
+    assert (ev1.bvh_builds, ev1.bvh_refits, ev1.material_writes, ev1.light_writes, ev1.uploaded_bytes) == \
+        (ev0.bvh_builds, ev0.bvh_refits, ev0.material_writes, ev0.light_writes, ev0.uploaded_bytes)
+    # an environment change is a change: the accumulation restarts
+    env = sc.copy()
+    env.environment = (0.1, 0.2, 0.3)
+    env._packed = None
+    ev2 = renderer.apply(env)
+    assert ev2.update == hip.SCENE_TABLES and ev2.bvh_builds == ev1.bvh_builds
+    c = renderer.render(env.world_to_clip, 24, 24, frame_count=1, max_bounces=4)
+    assert c["spp"] == 1
+
+
+@pytest.mark.parametrize("edit", ["base_color", "emission", "slot"])
+def test_material_edits_rewrite_tables_only(renderer, edit):
+    _say(f"material edit {edit}")
+    sc = scenes.cornell_lambert()
+    ev0 = renderer.apply(sc)
+    nodes = renderer.readback_scene()["bvh_nodes"]
+    b = sc.copy()
+    if edit == "base_color":
+        b.set_material(1, scenes.preview_material(base_color=(0.2, 0.6, 0.3), ior=1.0))
+    elif edit == "emission":
+        b.set_material(4, scenes.preview_material(base_color=(0, 0, 0), ior=1.0, emission=(30.0, 20.0, 10.0)))
+    else:  # the right wall takes the left wall's material slot
+        g, t, _ = b.instances[5]
+        b.instances[5] = (g, t, 2)
+        b._packed = None
+    ev = renderer.apply(b)
+    assert ev.update == hip.SCENE_TABLES
+    assert ev.bvh_builds == ev0.bvh_builds and ev.bvh_refits == ev0.bvh_refits
+    assert np.array_equal(renderer.readback_scene()["bvh_nodes"], nodes)
+    got = renderer.readback_scene()
+    pb = b.packed()
+    assert np.array_equal(got["materials"], pb["materials"])
+    assert np.array_equal(got["instance_material"], pb["instance_material"])
+    off_g, rec_g = renderer.light_table()
+    off_c, rec_c = oracle.OracleScene(b).light_table()
+    assert np.array_equal(off_g, off_c) and np.array_equal(rec_g, rec_c)
+    frame = _frame(renderer, b)
+    fresh = hip.Renderer(0)
+    try:
+        fresh.apply(b)
+        ref = _frame(fresh, b)
+    finally:
+        fresh.close()
+    assert np.array_equal(frame["counts"], ref["counts"])
+    assert np.array_equal(frame["color"], ref["color"])
+
+
+def test_topology_changes_rebuild(renderer):
+    _say("topology changes")
+    sc = scenes.cornell_lambert()
+    ev0 = renderer.apply(sc)
+    b = sc.copy()
+    pos, tri = b.geometries[0]
+    b.geometries[0] = (pos, tri[:, [0, 2, 1]].copy())  # same vertices, other indices
+    b._packed = None
+    ev = renderer.apply(b)
+    assert ev.update == hip.SCENE_BUILD and ev.bvh_builds == ev0.bvh_builds + 1
+    _check_against_oracle(renderer, b, seed=8, box=True)
+    c = sc.copy()
+    pos, tri = c.geometries[6]
+    c.geometries[6] = (np.concatenate([pos, pos[:1]]), tri)  # one more vertex
+    c._packed = None
+    ev = renderer.apply(c)
+    assert ev.update == hip.SCENE_BUILD and ev.bvh_builds == ev0.bvh_builds + 2
+    d = sc.copy()
+    d.instances = d.instances[:-1]  # one instance fewer
+    d._packed = None
+    ev = renderer.apply(d)
+    assert ev.update == hip.SCENE_BUILD and ev.instance_count == len(sc.instances) - 1
+    _check_against_oracle(renderer, d, seed=9, box=True)
+
+
+def test_temporal_history_across_scene_updates(renderer):
+    # TemporalSceneCompatible (vulkan_backend.cpp:262-289): moving a block that
+    # does not emit keeps the history; moving the light or editing a material
+    # drops it with SceneChanged (8).
+    _say("temporal history across updates")
+    sc = scenes.cornell_lambert()
+    renderer.apply(sc)
+    w, h = 32, 24
+    f = 5000
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f, camera_cut=True, max_bounces=3)
+    moved = sc.copy()
+    t = scenes.block((0.6, 0.6, 0.6), -10.0, (0.38, -0.699, 0.3))
+    moved.set_transform(6, t)
+    assert renderer.apply(moved).update == hip.SCENE_REFIT
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 1, max_bounces=3)
+    assert out["reset_reason"] == 0 and out["history_available"]
+    light = moved.copy()
+    lt = scenes.identity()
+    lt[12] = 0.1
+    light.set_transform(4, lt)
+    assert renderer.apply(light).update == hip.SCENE_REFIT
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 2, max_bounces=3)
+    assert out["reset_reason"] == 8
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 3, max_bounces=3)
+    assert out["reset_reason"] == 0
+    mat = light.copy()
+    mat.set_material(1, scenes.preview_material(base_color=(0.3, 0.3, 0.9), ior=1.0))
+    assert renderer.apply(mat).update == hip.SCENE_TABLES
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 4, max_bounces=3)
+    assert out["reset_reason"] == 8
