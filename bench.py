@@ -168,6 +168,51 @@ def restir_svgf_run(r, width, height, frames, depth=8, candidates=32, history=20
             "reference_spp": 512}
 
 
+def scene_update_run(r, scene, width, height, depth, steps, frames=4):
+    """The heightfield animated (scenes.heightfield_wave): every step's plan
+    keeps each mesh's vertex count and indices, so the renderer re-places the
+    triangles and refits the BVH instead of building one (GpuScene::Apply's
+    point refit, gpu_scene.cpp:1638-1671). Reports the refit's GPU time beside
+    a build of the same plan, and frame times on the refitted tree beside a
+    tree built for the last step's heights (the refit's quality cost)."""
+    kw = dict(frame_count=1, max_bounces=depth, light_sampling=hip.LIGHT_MIS, want_color=False)
+    refit_ms, upload_ms, uploaded = [], [], []
+    moved = scene
+    for k in range(steps):
+        moved = scenes.heightfield_wave(scene, 0.35 * (k + 1))
+        before = r.apply(moved)
+        if before.update != hip.SCENE_REFIT:
+            raise RuntimeError(f"animated step {k} was applied as update {before.update}, not as a refit")
+        refit_ms.append(before.build_gpu_ms)
+        upload_ms.append(before.upload_ms)
+        uploaded.append(before.uploaded_bytes)
+        r.render(moved.world_to_clip, width, height, **kw)
+
+    def frame_ms(renderer):
+        renderer.render(moved.world_to_clip, width, height, **kw)  # rounds from a previous sample
+        t = 0.0
+        for _ in range(frames):
+            t += renderer.render(moved.world_to_clip, width, height, **kw)["frame_ms"]
+        return t / frames
+
+    refit_frame = frame_ms(r)
+    fresh = hip.Renderer(0)
+    try:
+        ev = fresh.apply(moved)
+        built_frame = frame_ms(fresh)
+    finally:
+        fresh.close()
+    r.apply(scene)  # back to the still heightfield (a refit)
+    return {"workload": f"heightfield {ev.triangle_count} tris, {steps} animated steps "
+                        f"(heights + 0.05 sin(4x + 3z + t)), {width}x{height}, depth {depth}",
+            "build_gpu_ms": round(ev.build_gpu_ms, 3),
+            "refit_gpu_ms": round(float(np.mean(refit_ms)), 3),
+            "refit_upload_wall_ms": round(float(np.mean(upload_ms)), 2),
+            "refit_uploaded_mb": round((uploaded[-1] - uploaded[0]) / max(len(uploaded) - 1, 1) / 1e6, 1),
+            "frame_gpu_ms_refitted_tree": round(refit_frame, 3),
+            "frame_gpu_ms_built_tree": round(built_frame, 3)}
+
+
 def batch_run(views, frames):
     """64 seeded views of 64^2..512^2 over their own icosphere rigs (1k..200k
     triangles, depth 2..12): each view autotuned, then frames rendered view
@@ -231,6 +276,8 @@ def main() -> int:
                     help="frames of the 64-view batched workload (0 skips it)")
     ap.add_argument("--restir-frames", type=int, default=120,
                     help="temporal frames of the ReSTIR + SVGF run (0 skips it)")
+    ap.add_argument("--update-steps", type=int, default=8,
+                    help="animated heightfield steps applied as BVH refits (0 skips them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times: the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -352,6 +399,9 @@ def main() -> int:
                              for k in ("camera_ms", "shade_ms", "trace_ms", "tail_ms", "resolve_ms", "frame_ms")},
         "roofline": roofline,
     }
+    if args.update_steps > 0:
+        log("scene updates (refits) ...")
+        line["scene_update"] = scene_update_run(r, scene, args.width, args.height, args.depth, args.update_steps)
     if args.race_frames > 0:
         cache_cfg = dict(log2_table=19, train_paths=65536, batch_capacity=65536, train_steps=4, learning_rate=1e-2)
         log("equal-relMSE race: heightfield ...")

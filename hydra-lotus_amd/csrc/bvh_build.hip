@@ -5,7 +5,10 @@
 //   2. 30-bit Morton codes of centroids, stable LSD radix sort (8-bit digits,
 //      wave64 ballot ranking);
 //   3. binary radix tree (Karras, "Maximizing Parallelism in the Construction
-//      of BVHs, Octrees, and k-d Trees", HPG 2012) and bottom-up boxes;
+//      of BVHs, Octrees, and k-d Trees", HPG 2012); a node's box is the union
+//      of the leaf boxes its key range covers, read from a sparse table of
+//      power-of-two window unions (two lookups per node, no atomics or
+//      device-wide fences);
 //   4. top-down collapse into 4-wide nodes, opening the child of largest
 //      surface area first, leaves of at most `leaf_size` triangles.
 #include "internal.hpp"
@@ -151,19 +154,22 @@ __global__ void __launch_bounds__(kBlock) k_sort_histogram(const uint32_t* keys,
   counts[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of `n` counts by one block (digit-major order).
-__global__ void __launch_bounds__(1024) k_sort_scan(uint32_t* counts, uint32_t n) {
+// Exclusive scan of the digit-major counts in two steps: block d scans digit
+// d's `tiles` counts in place and writes their total; then every count gains
+// the totals of the smaller digits.
+__global__ void __launch_bounds__(1024) k_scan_digit(uint32_t* counts, uint32_t tiles, uint32_t* totals) {
   __shared__ uint32_t part[1024];
   __shared__ uint32_t carry;
+  uint32_t* seg = counts + size_t(blockIdx.x) * tiles;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   constexpr uint32_t kPer = 4;
-  for (uint32_t start = 0; start < n; start += 1024 * kPer) {
+  for (uint32_t start = 0; start < tiles; start += 1024 * kPer) {
     uint32_t v[kPer];
     uint32_t sum = 0;
     for (uint32_t i = 0; i < kPer; ++i) {
       const uint32_t k = start + threadIdx.x * kPer + i;
-      v[i] = k < n ? counts[k] : 0u;
+      v[i] = k < tiles ? seg[k] : 0u;
       sum += v[i];
     }
     part[threadIdx.x] = sum;
@@ -177,13 +183,26 @@ __global__ void __launch_bounds__(1024) k_sort_scan(uint32_t* counts, uint32_t n
     uint32_t run = carry + part[threadIdx.x] - sum;
     for (uint32_t i = 0; i < kPer; ++i) {
       const uint32_t k = start + threadIdx.x * kPer + i;
-      if (k < n) counts[k] = run;
+      if (k < tiles) seg[k] = run;
       run += v[i];
     }
     __syncthreads();
     if (threadIdx.x == 1023) carry += part[1023];
     __syncthreads();
   }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_scan_add(uint32_t* counts, uint32_t tiles, const uint32_t* totals) {
+  __shared__ uint32_t base;
+  if (threadIdx.x == 0) {
+    uint32_t b = 0;
+    for (uint32_t d = 0; d < blockIdx.x; ++d) b += totals[d];
+    base = b;
+  }
+  __syncthreads();
+  uint32_t* seg = counts + size_t(blockIdx.x) * tiles;
+  for (uint32_t k = threadIdx.x; k < tiles; k += blockDim.x) seg[k] += base;
 }
 
 // Stable scatter: items are ranked in tile order, 256 per round; within a
@@ -244,9 +263,8 @@ __device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j) 
 }
 
 // Internal node i in [0, n-2]; leaves are n-1+k. child[2i], child[2i+1];
-// parent of every node; range [first, last] of internal nodes.
-__global__ void __launch_bounds__(kBlock) k_karras(const uint32_t* keys, int n, uint32_t* child,
-                                                   uint32_t* parent, uint2* range) {
+// range [first, last] of internal nodes.
+__global__ void __launch_bounds__(kBlock) k_karras(const uint32_t* keys, int n, uint32_t* child, uint2* range) {
   const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n - 1) return;
   const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -271,8 +289,6 @@ __global__ void __launch_bounds__(kBlock) k_karras(const uint32_t* keys, int n, 
                                            : static_cast<uint32_t>(gamma + 1);
   child[2 * i] = left;
   child[2 * i + 1] = right;
-  parent[left] = static_cast<uint32_t>(i);
-  parent[right] = static_cast<uint32_t>(i);
   range[i] = make_uint2(static_cast<uint32_t>(first), static_cast<uint32_t>(last));
 }
 
@@ -280,18 +296,25 @@ struct Box {
   float lo[3], hi[3];
 };
 
-// Leaf boxes, sorted triangles, then boxes up the tree: the second of two
-// children to finish computes its parent.
-__global__ void __launch_bounds__(kBlock) k_bottom_up(const Tri* world, const uint32_t* sorted, int n,
-                                                      const uint32_t* child, const uint32_t* parent,
-                                                      uint32_t* flags, Box* boxes, Tri* tris,
-                                                      uint32_t* tri_of_global, uint32_t* fault) {
-  const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ Box unite(const Box& a, const Box& b) {
+  Box u;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    u.lo[k] = fminf(a.lo[k], b.lo[k]);
+    u.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+  }
+  return u;
+}
+
+// Triangles in sorted order, and their boxes: level 0 of the sparse table.
+__global__ void __launch_bounds__(kBlock) k_leaf_boxes(const Tri* world, const uint32_t* sorted, uint32_t n,
+                                                       Box* table, Tri* tris, uint32_t* tri_of_global) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const uint32_t g = sorted[k];
   const Tri t = world[g];
   tris[k] = t;
-  tri_of_global[g] = static_cast<uint32_t>(k);
+  tri_of_global[g] = k;
   Box b;
   b.lo[0] = fminf(fminf(t.a.x, t.b.x), t.c.x);
   b.lo[1] = fminf(fminf(t.a.y, t.b.y), t.c.y);
@@ -299,30 +322,41 @@ __global__ void __launch_bounds__(kBlock) k_bottom_up(const Tri* world, const ui
   b.hi[0] = fmaxf(fmaxf(t.a.x, t.b.x), t.c.x);
   b.hi[1] = fmaxf(fmaxf(t.a.y, t.b.y), t.c.y);
   b.hi[2] = fmaxf(fmaxf(t.a.z, t.b.z), t.c.z);
-  uint32_t node = static_cast<uint32_t>(n - 1 + k);
-  boxes[node] = b;
-  if (n == 1) return;
-  __threadfence();
-  uint32_t p = parent[node];
-  // A valid tree reaches the root within 30 + 27 levels; a longer walk means
-  // a broken tree, reported instead of followed.
-  for (int step = 0; step < 128; ++step) {
-    if (p >= static_cast<uint32_t>(n - 1)) break;
-    if (atomicAdd(&flags[p], 1u) == 0u) return;  // first child to arrive stops
-    __threadfence();
-    const Box l = boxes[child[2 * p]], r = boxes[child[2 * p + 1]];
-    Box u;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      u.lo[a] = fminf(l.lo[a], r.lo[a]);
-      u.hi[a] = fmaxf(l.hi[a], r.hi[a]);
-    }
-    boxes[p] = u;
-    __threadfence();
-    if (p == 0) return;
-    p = parent[p];
+  table[k] = b;
+}
+
+// Sparse table level j from level j - 1: next[k] is the union of the 2^j
+// leaf boxes from k, for every k whose window fits (k + 2^j <= n).
+__global__ void __launch_bounds__(kBlock) k_table_level(const Box* prev, Box* next, uint32_t n, uint32_t half) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k + 2 * half > n) return;
+  next[k] = unite(prev[k], prev[k + half]);
+}
+
+// Every binary node's box: a leaf's from level 0, an internal node's over its
+// range [f, l] as the union of the two windows of 2^j, j = floor(log2(l-f+1)),
+// that start at f and end at l. Min and max are exact, so this is the box a
+// bottom-up union of its children would give.
+__global__ void __launch_bounds__(kBlock) k_node_boxes(const uint2* range, const Box* table, uint32_t n,
+                                                       uint32_t levels, Box* boxes, uint32_t* fault) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * n - 1) return;
+  if (i >= n - 1) {
+    boxes[i] = table[i - (n - 1)];
+    return;
   }
-  atomicOr(fault, 1u);
+  const uint2 r = range[i];
+  if (r.x >= r.y || r.y >= n) {  // a broken tree: reported, not followed
+    atomicOr(fault, 1u);
+    return;
+  }
+  const uint32_t j = 31u - static_cast<uint32_t>(__clz(r.y - r.x + 1u));
+  if (j >= levels) {
+    atomicOr(fault, 1u);
+    return;
+  }
+  const Box* level = table + size_t(j) * n;
+  boxes[i] = unite(level[r.x], level[r.y + 1u - (1u << j)]);
 }
 
 // ---------------------------------------------------------------- collapse --
@@ -344,14 +378,18 @@ __device__ __forceinline__ void inflate(float& lo, float& hi) {
   hi += m;
 }
 
-__global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const uint32_t* task_count,
-                                                     Task* next, uint32_t* next_count, uint32_t* node_count,
-                                                     uint32_t* leaf_total, const uint32_t* child,
-                                                     const uint2* range, const Box* boxes, int n,
-                                                     uint32_t leaf_size, WideNode* nodes, uint32_t* fault) {
+// One level of tasks: each expands its binary node into at most four
+// children, opening the child of largest surface area first. Inner children
+// take the next level's task slots (one reservation per wave), and a task's
+// wide node index is the next level's first index plus its slot, so every
+// level's nodes follow the previous level's.
+__global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const uint32_t* task_count, Task* next,
+                                                     uint32_t* next_count, uint32_t next_base, uint32_t* leaf_total,
+                                                     const uint32_t* child, const uint2* range, const Box* boxes,
+                                                     int n, uint32_t leaf_size, WideNode* nodes, uint32_t* fault) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= *task_count) return;
-  const Task task = tasks[t];
+  const bool valid = t < *task_count;  // every lane reaches the reservations below
+  const Task task = valid ? tasks[t] : Task{0u, 0u};
   uint32_t list[4];
   int count = 0;
   auto size_of = [&](uint32_t b) -> uint32_t {
@@ -359,7 +397,8 @@ __global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const ui
     const uint2 r = range[b];
     return r.y - r.x + 1u;
   };
-  if (n == 1) {
+  if (!valid) {
+  } else if (n == 1) {
     list[count++] = 0;  // the single leaf is binary node 0 (n - 1 + 0)
   } else if (size_of(task.binary) <= leaf_size) {
     list[count++] = task.binary;  // the whole subtree is one leaf child
@@ -385,24 +424,31 @@ __global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const ui
       list[count++] = child[2 * b + 1];
     }
   }
+  // 0: empty or broken, 1: leaf, 2: inner
+  uint32_t kind[4] = {0u, 0u, 0u, 0u}, inner = 0, leaves = 0;
+  for (int c = 0; c < count; ++c) {
+    const uint32_t b = list[c];
+    const uint32_t first_b = b >= static_cast<uint32_t>(n - 1) ? b - static_cast<uint32_t>(n - 1) : range[b].x;
+    if (b >= static_cast<uint32_t>(2 * n - 1) || first_b + size_of(b) > static_cast<uint32_t>(n)) {
+      atomicOr(fault, 2u);  // a broken tree: the child stays empty
+      continue;
+    }
+    kind[c] = size_of(b) <= leaf_size ? 1u : 2u;
+    inner += kind[c] == 2u;
+    leaves += kind[c] == 1u;
+  }
+  uint32_t slot = wave_reserve(next_count, inner);
+  (void)wave_reserve(leaf_total, leaves);
+  if (!valid) return;
   WideNode out;
-  uint32_t leaves = 0;
   for (int c = 0; c < 4; ++c) {
-    if (c >= count) {
+    if (kind[c] == 0u) {
       out.lox[c] = out.loy[c] = out.loz[c] = INFINITY;
       out.hix[c] = out.hiy[c] = out.hiz[c] = -INFINITY;
       out.child[c] = kEmptyChild;
       continue;
     }
     const uint32_t b = list[c];
-    const uint32_t first_b = b >= static_cast<uint32_t>(n - 1) ? b - static_cast<uint32_t>(n - 1) : range[b].x;
-    if (b >= static_cast<uint32_t>(2 * n - 1) || first_b + size_of(b) > static_cast<uint32_t>(n)) {
-      atomicOr(fault, 2u);  // a broken tree: leave the child empty and report
-      out.lox[c] = out.loy[c] = out.loz[c] = INFINITY;
-      out.hix[c] = out.hiy[c] = out.hiz[c] = -INFINITY;
-      out.child[c] = kEmptyChild;
-      continue;
-    }
     Box box = boxes[b];
 #pragma unroll
     for (int a = 0; a < 3; ++a) inflate(box.lo[a], box.hi[a]);
@@ -412,19 +458,22 @@ __global__ void __launch_bounds__(kBlock) k_collapse(const Task* tasks, const ui
     out.hix[c] = box.hi[0];
     out.hiy[c] = box.hi[1];
     out.hiz[c] = box.hi[2];
-    const uint32_t sz = size_of(b);
-    if (sz <= leaf_size) {
-      out.child[c] = leaf_word(first_b, sz);
-      ++leaves;
+    if (kind[c] == 1u) {
+      const uint32_t first_b = b >= static_cast<uint32_t>(n - 1) ? b - static_cast<uint32_t>(n - 1) : range[b].x;
+      out.child[c] = leaf_word(first_b, size_of(b));
     } else {
-      const uint32_t w = atomicAdd(node_count, 1u);
+      const uint32_t w = next_base + slot;
       out.child[c] = w;
-      next[atomicAdd(next_count, 1u)] = Task{b, w};
+      next[slot++] = Task{b, w};
     }
   }
   out.pad[0] = out.pad[1] = out.pad[2] = out.pad[3] = 0;
   nodes[task.wide] = out;
-  if (leaves) atomicAdd(leaf_total, leaves);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // ------------------------------------------------------------------ refit --
@@ -498,11 +547,6 @@ __global__ void __launch_bounds__(kBlock) k_refit(WideNode* nodes, float4* exact
   exact[2ull * w + 1] = make_float4(all.hi[0], all.hi[1], all.hi[2], 0.0f);
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 __global__ void k_empty_root(WideNode* nodes) {
   WideNode out;
   for (int c = 0; c < 4; ++c) {
@@ -533,7 +577,7 @@ bool radix_sort_pairs(uint32_t* keys, uint32_t* values, uint32_t n, uint32_t key
   if (n <= 1) return true;
   const uint32_t tiles = blocks_for(n, kSortTile);
   uint32_t *k2 = nullptr, *v2 = nullptr, *counts = nullptr;
-  if (!alloc(k2, n, error) || !alloc(v2, n, error) || !alloc(counts, 256ull * tiles, error)) {
+  if (!alloc(k2, n, error) || !alloc(v2, n, error) || !alloc(counts, 256ull * tiles + 256, error)) {
     (void)hipFree(k2);
     (void)hipFree(v2);
     return false;
@@ -543,7 +587,8 @@ bool radix_sort_pairs(uint32_t* keys, uint32_t* values, uint32_t n, uint32_t key
   for (int pass = 0; pass < passes; ++pass) {
     const int shift = pass * 8;
     k_sort_histogram<<<tiles, kBlock, 0, stream>>>(ka, n, shift, tiles, counts);
-    k_sort_scan<<<1, 1024, 0, stream>>>(counts, 256u * tiles);
+    k_scan_digit<<<256, 1024, 0, stream>>>(counts, tiles, counts + 256ull * tiles);
+    k_scan_add<<<256, 256, 0, stream>>>(counts, tiles, counts + 256ull * tiles);
     k_sort_scatter<<<tiles, kBlock, 0, stream>>>(ka, va, kb, vb, n, shift, tiles, counts);
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -588,16 +633,18 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   Tri* world = nullptr;
   float4* centroid = nullptr;
   uint32_t *bounds = nullptr, *keys = nullptr, *vals = nullptr;
-  uint32_t *child = nullptr, *parent = nullptr, *flags = nullptr;
+  uint32_t* child = nullptr;
   uint2* range = nullptr;
-  Box* boxes = nullptr;
+  Box *boxes = nullptr, *table = nullptr;
   Task *tasks_a = nullptr, *tasks_b = nullptr;
-  uint32_t* counters = nullptr;  // 0: task count a, 1: task count b, 2: node count, 3: leaves, 4: fault
+  uint32_t* counters = nullptr;  // 0: task count a, 1: task count b, 2: unused, 3: leaves, 4: fault
   const uint32_t internal = n > 1 ? n - 1 : 1;
+  // sparse table levels 0 .. floor(log2 n): windows of 1, 2, 4, ... leaves
+  const uint32_t levels = 32u - static_cast<uint32_t>(__builtin_clz(n));
   bool ok = alloc(world, n, error) && alloc(centroid, n, error) && alloc(bounds, 6, error) &&
             alloc(keys, n, error) && alloc(vals, n, error) && alloc(child, 2ull * internal, error) &&
-            alloc(parent, 2ull * n, error) && alloc(flags, internal, error) && alloc(range, internal, error) &&
-            alloc(boxes, 2ull * n, error) && alloc(tasks_a, n, error) && alloc(tasks_b, n, error) &&
+            alloc(range, internal, error) && alloc(boxes, 2ull * n, error) &&
+            alloc(table, uint64_t(n) * levels, error) && alloc(tasks_a, n, error) && alloc(tasks_b, n, error) &&
             alloc(counters, 5, error);
   auto release = [&]() {
     (void)hipFree(world);
@@ -606,8 +653,7 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
     (void)hipFree(keys);
     (void)hipFree(vals);
     (void)hipFree(child);
-    (void)hipFree(parent);
-    (void)hipFree(flags);
+    (void)hipFree(table);
     (void)hipFree(range);
     (void)hipFree(boxes);
     (void)hipFree(tasks_a);
@@ -640,19 +686,25 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
     release();
     return false;
   }
-  if (n > 1) {
-    k_karras<<<blocks_for(n - 1, kBlock), kBlock, 0, stream>>>(keys, static_cast<int>(n), child, parent, range);
-    k_fill_u32<<<blocks_for(internal, kBlock), kBlock, 0, stream>>>(flags, 0u, internal);
-  }
-  k_bottom_up<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(world, vals, static_cast<int>(n), child, parent,
-                                                            flags, boxes, out.tris, out.tri_of_global,
-                                                            counters + 4);
-  // collapse, level by level; the root task expands binary node 0
-  const Task root{n > 1 ? 0u : 0u, 0u};
+  // the collapse's counters and the fault word start before the box kernels
+  const Task root{0u, 0u};  // the root task expands binary node 0
   const uint32_t init_counts[5] = {1u, 0u, 1u, 0u, 0u};
   e = hipMemcpyAsync(tasks_a, &root, sizeof root, hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipMemcpyAsync(counters, init_counts, sizeof init_counts, hipMemcpyHostToDevice, stream);
-  uint32_t pending = 1, depth = 0;
+  if (e == hipSuccess) {
+    if (n > 1) k_karras<<<blocks_for(n - 1, kBlock), kBlock, 0, stream>>>(keys, static_cast<int>(n), child, range);
+    k_leaf_boxes<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(world, vals, n, table, out.tris, out.tri_of_global);
+    for (uint32_t j = 1; j < levels; ++j) {
+      const uint32_t half = 1u << (j - 1);
+      k_table_level<<<blocks_for(n - 2 * half + 1, kBlock), kBlock, 0, stream>>>(
+          table + uint64_t(j - 1) * n, table + uint64_t(j) * n, n, half);
+    }
+    k_node_boxes<<<blocks_for(2ull * n - 1, kBlock), kBlock, 0, stream>>>(range, table, n, levels, boxes,
+                                                                          counters + 4);
+    e = hipGetLastError();
+  }
+  // collapse, level by level
+  uint32_t pending = 1, depth = 0, allocated = 1;  // the root is node 0
   Task *cur = tasks_a, *nxt = tasks_b;
   int cur_slot = 0;
   out.level_sizes.clear();
@@ -662,10 +714,12 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
     const int nxt_slot = 1 - cur_slot;
     k_fill_u32<<<1, 1, 0, stream>>>(counters + nxt_slot, 0u, 1u);
     k_collapse<<<blocks_for(pending, kBlock), kBlock, 0, stream>>>(
-        cur, counters + cur_slot, nxt, counters + nxt_slot, counters + 2, counters + 3, child, range, boxes,
+        cur, counters + cur_slot, nxt, counters + nxt_slot, allocated, counters + 3, child, range, boxes,
         static_cast<int>(n), in.leaf_size, out.nodes, counters + 4);
     e = hipMemcpyAsync(&pending, counters + nxt_slot, 4, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    allocated += pending;
+    if (allocated > max_nodes) break;  // reported below
     std::swap(cur, nxt);
     cur_slot = nxt_slot;
   }
@@ -683,12 +737,12 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   (void)hipEventDestroy(start);
   (void)hipEventDestroy(stop);
   gpu_ms = ms;
-  if (final_counts[4] != 0 || pending > 0) {
+  if (final_counts[4] != 0 || pending > 0 || allocated > max_nodes) {
     error = "BVH build produced an invalid tree (code " + std::to_string(final_counts[4]) + ")";
     release();
     return false;
   }
-  out.node_count = final_counts[2];
+  out.node_count = allocated;
   out.leaf_count = final_counts[3];
   out.depth = depth;
   out.bytes += uint64_t(max_nodes) * sizeof(WideNode);

@@ -141,4 +141,21 @@ __device__ __forceinline__ void wave_count_each(uint32_t* counter, bool active) 
   }
 }
 
+// Each lane of a fully active wave reserves `k` consecutive slots of
+// *counter and gets its first one: a wave-wide prefix sum and one atomic.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, uint32_t k) {
+  const uint32_t lane = __lane_id();
+  uint32_t incl = k;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t v = __shfl_up(incl, d);
+    if (lane >= static_cast<uint32_t>(d)) incl += v;
+  }
+  const uint32_t total = __shfl(incl, 63);
+  uint32_t base = 0;
+  if (lane == 63 && total != 0) base = atomicAdd(counter, total);
+  base = __shfl(base, 63);
+  return base + incl - k;
+}
+
 }  // namespace lotus
