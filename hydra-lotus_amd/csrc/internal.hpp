@@ -20,6 +20,41 @@ struct LightRecord {
   uint32_t instance;
 };
 
+// A light-table entry whose `instance` has this bit is an analytic light,
+// the other bits its record's index (gpu_scene.hpp:128-131).
+constexpr uint32_t kAnalyticLight = 0x80000000u;
+
+// AnalyticLightRecord shapes (gpu_scene.hpp:136-142).
+constexpr uint32_t kSphereLight = 0, kRectLight = 1, kDiskLight = 2, kDistantLight = 3, kParallelLight = 4;
+
+// One analytic light in world space (GpuAnalyticLightRecord,
+// gpu_scene.hpp:144-166): position (sphere radius in w), half axes u and v
+// (a distant light's unit direction towards it in u, its cone's
+// 1 - cos(theta_max) in u.w), the emitting side's unit normal (area in w),
+// radiance (the light table's probability of the light in w), shape.
+struct alignas(16) AnalyticRecord {
+  float position[4];
+  float axis_u[4];
+  float axis_v[4];
+  float normal[4];
+  float radiance[4];
+  uint32_t shape;
+  uint32_t reserved[3];
+};
+static_assert(sizeof(AnalyticRecord) == 96, "analytic light record is 96 bytes");
+
+// One interior node of the binary crossing hierarchy over the sphere, rect
+// and disk lights (GpuLightNode, gpu_scene.hpp:175-185): per child its
+// bounds, the child word in low.w (kLightLeaf | light, a node, or
+// kNoLightChild).
+struct alignas(16) LightNode {
+  float low0[4], high0[4], low1[4], high1[4];
+};
+static_assert(sizeof(LightNode) == 64, "light node is 64 bytes");
+constexpr uint32_t kLightLeaf = 0x80000000u;
+constexpr uint32_t kNoLightChild = 0xFFFFFFFFu;
+constexpr int kLightStack = 32;  // a median split's depth is at most ceil(log2(count))
+
 // Everything the kernels read of the scene, as device pointers.
 struct SceneView {
   const WideNode* nodes;
@@ -31,9 +66,14 @@ struct SceneView {
   const uint32_t* materials;        // 36 words per material
   const uint32_t* light_offsets;    // per instance
   const LightRecord* lights;
+  const AnalyticRecord* analytic;   // analytic_count records
+  const LightNode* light_nodes;     // light_node_count nodes, then the distant lights' indices
+  const uint32_t* distant;          // distant_count indices of distant (cone) lights
   uint32_t light_count;
   uint32_t tri_count;
   uint32_t instance_count;
+  uint32_t analytic_count, light_node_count, distant_count;
+  uint32_t parallel;                // the parallel light's index + 1 (BSDF sampling traces it), 0 without one
   float environment[3];
 };
 
@@ -48,6 +88,10 @@ struct SceneBuffers {
   uint32_t* materials = nullptr;
   uint32_t* light_offsets = nullptr;
   LightRecord* lights = nullptr;
+  AnalyticRecord* analytic = nullptr;
+  LightNode* light_nodes = nullptr;
+  uint32_t* distant = nullptr;
+  uint32_t analytic_count = 0, light_node_count = 0, distant_count = 0, parallel_count = 0, parallel = 0;
   // The plan's geometry as uploaded: refits re-place its triangles and
   // read-back copies it.
   float* positions = nullptr;

@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
         const float random = next_random(rng);
         const uint32_t light = choose_reservoir_light(S, F.restir_proposal, cx, cy);
         const float proposal = reservoir_proposal_pdf(S, F.restir_proposal, light);
-        const LightSample ls = sample_triangle_light(S, light, u, v, origin);
+        const LightSample ls = reconstruct_reservoir_light(S, light, u, v, origin);
         const V3 f = candidate_contribution(sf, ft, fb, wo, ls, S.lights[light].pdf);
         bool selected = false;
         if (!isfinite(ls.pdf) || !finite3(f) || !stream_candidate(r, light, max3(f), proposal, random, selected)) {
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
     const bool trace = (flags & 2u) == 0u && r.weight_sum > 0.0f;
     wave_count(restir_ray_count(counters), trace);
     if (trace) {
-      const LightSample ls = sample_triangle_light(S, r.sample, ru, rv, origin);
+      const LightSample ls = reconstruct_reservoir_light(S, r.sample, ru, rv, origin);
       uint32_t visits = 0, tests = 0;
       const float visibility =
           walk_occluded(S.nodes, S.tris, st, Ray{origin, 0.0f, ls.direction, ls.distance}, visits, tests) ? 0.0f
@@ -405,7 +405,7 @@ __device__ __forceinline__ bool same_sample(const ReuseRecord& a, const ReuseRec
 
 __device__ __forceinline__ V3 reuse_contribution_uv(const SceneView& S, const ReuseSurface& r, uint32_t light, float u,
                                                     float v) {
-  return reuse_contribution(S, r, light, sample_triangle_light(S, light, u, v, r.origin));
+  return reuse_contribution(S, r, light, reconstruct_reservoir_light(S, light, u, v, r.origin));
 }
 
 // restir.slang:278-287 (HistoryTarget): the selection's target at a source's
@@ -429,7 +429,7 @@ __device__ __forceinline__ void resolve_estimate(const SceneView& S, TraceStack&
                                                  ReuseRecord& result, const ReuseRecord& own, uint32_t& rays,
                                                  uint32_t& visits, uint32_t& tests) {
   if ((result.flags & 2u) != 0u || !(result.r.weight_sum > 0.0f)) return;
-  const LightSample ls = sample_triangle_light(S, result.r.sample, result.u, result.v, receiver.origin);
+  const LightSample ls = reconstruct_reservoir_light(S, result.r.sample, result.u, result.v, receiver.origin);
   float visibility;
   if (same_sample(result, own)) {
     visibility = own.contribution.w;
@@ -626,6 +626,7 @@ __device__ __forceinline__ void cache_query(const FrameConst& F, const Surface& 
 // its emission term, leaving a cache query with its throughput, and a
 // training path records each vertex (query, throughput on arrival, radiance
 // including the vertex's emission).
+template <bool kAnalytic>  // the scene has analytic lights
 __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                        uint32_t bounce, uint32_t link) {
   const uint32_t count = *hit_count(counters, bounce);
@@ -670,7 +671,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
       s.continues = false;
       s.shadow = false;
     } else {
-      s = scatter_path(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z), __uint_as_float(h4.w),
+      s = scatter_path<kAnalytic>(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z), __uint_as_float(h4.w),
                        train ? &sf : nullptr, slot);
     }
     const uint32_t kq = wave_append(cache_query_count(counters), query);
@@ -727,6 +728,7 @@ __global__ void __launch_bounds__(256) k_cache_samples(FrameConst F, Queues q, u
       L = xyz(s);
       if (ne.w != 0.0f) L = L + xyz(ne);
       if (e.w != 0.0f) L = L + xyz(e);
+      if (e.w == 2.0f) L = L + xyz(q.esc2[slot]);
     }
     for (uint32_t b = 0; b < kCacheRecordDepth; ++b) {
       bool ok = b < n;
@@ -759,7 +761,7 @@ __global__ void __launch_bounds__(256) k_cache_samples(FrameConst F, Queues q, u
 // for shading, a miss leaves its environment term pending, transport.slang:
 // 436-448), items [n_ext, n_ext + n_shadow) the shadow rays of bounce b - 1
 // (an unblocked ray leaves its light pending).
-template <bool kDynamic>
+template <bool kDynamic, bool kAnalytic>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, uint32_t* counters, uint2* spill,
                                                        int spill_cap, uint32_t bounce, uint32_t kinds,
                                                        unsigned long long* stats) {
@@ -768,7 +770,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
   st.init(ring, spill, spill_cap);
   const uint32_t n_ext = (kinds & kTraceExtension) ? *ray_count(counters, bounce) : 0u;
   const uint32_t n_sh = (kinds & kTraceShadow) ? *shadow_count(counters, bounce - 1) : 0u;
-  const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
+  const bool weigh = (kinds & kTraceWeigh) != 0;
   __shared__ unsigned long long counts[4];
   BlockStats<2> bs;
   bs.init(counts);
@@ -791,8 +793,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
         if (ext && hit) {
           push_hit(q, k, slot, w.r.o, w.r.d, w.record(S.tris));
         } else if (ext) {
-          const V3 e = xyz(q.thr[slot]) * env;
-          q.esc[slot] = make_float4(e.x, e.y, e.z, 1.0f);
+          const float4 t4 = q.thr[slot];
+          V3 e, crossed;
+          escape_terms<kAnalytic>(S, weigh, w.r.o, w.r.d, xyz(t4), t4.w, e, crossed);
+          const bool more = kAnalytic && S.analytic_count != 0;
+          q.esc[slot] = make_float4(e.x, e.y, e.z, more ? 2.0f : 1.0f);
+          if (more) q.esc2[slot] = make_float4(crossed.x, crossed.y, crossed.z, 0.0f);
           float4 r4 = q.rad[slot];
           r4.w = 1.0f;
           q.rad[slot] = r4;
@@ -813,7 +819,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
 // walk, so radiance is summed in the reference's order.
 __device__ __forceinline__ float pair_float(float v) { return __shfl_xor(v, 1); }
 
-template <bool kCache>  // a radiance cache frame (training vertices recorded)
+template <bool kCache, bool kAnalytic>  // a radiance cache frame (training vertices recorded); analytic lights
 __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                       uint2* spill, int spill_cap, uint32_t bounce,
                                                       unsigned long long* stats) {
@@ -822,7 +828,6 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
   st.init(ring, spill, spill_cap);
   const uint32_t count = *ray_count(counters, bounce);
   const uint32_t n_shadow = *shadow_count(counters, bounce - 1);  // the last round's, linked or on their own
-  const V3 env = V3{S.environment[0], S.environment[1], S.environment[2]};
   const bool path_lane = (threadIdx.x & 1u) == 0;
   uint32_t visits = 0, tests = 0;
   // both lanes of a pair take the same items (the grid's workgroups hold
@@ -922,13 +927,16 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
       if (path_lane && extend) {
         hit = w.hit();
         if (!hit) {
-          p.radiance = p.radiance + p.throughput * env;
+          V3 e, crossed;
+          escape_terms<kAnalytic>(S, F.light_sampling != 0, p.o, p.d, p.throughput, p.last_pdf, e, crossed);
+          p.radiance = p.radiance + e;
+          if (kAnalytic && S.analytic_count != 0) p.radiance = p.radiance + crossed;
         } else {
           const HitRecord h = w.record(S.tris);
           const V3 arrival = p.throughput, dir = p.d;
           const uint32_t b = p.depth;
           Surface sf;
-          sc = scatter_path(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v, kCache && train ? &sf : nullptr,
+          sc = scatter_path<kAnalytic>(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v, kCache && train ? &sf : nullptr,
                             slot);
           if (kCache && train && b < kCacheRecordDepth && path < F.train_capacity) {
             float4* rec = q.crec + 6ull * (static_cast<size_t>(b) * F.train_capacity + path);
@@ -969,6 +977,7 @@ __global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_
       V3 v = xyz(s);
       if (n.w != 0.0f) v = v + xyz(n);
       if (e.w != 0.0f) v = v + xyz(e);
+      if (e.w == 2.0f) v = v + xyz(q.esc2[slot]);
       sum.x = sum.x + v.x;
       sum.y = sum.y + v.y;
       sum.z = sum.z + v.z;
@@ -1086,13 +1095,14 @@ GridShape grid_shape(uint32_t cus) {
   GridShape g;
   g.camera = persistent_blocks(k_camera<false>, kTraceBlock, cus);
   g.camera_dynamic = persistent_blocks(k_camera<true>, kTraceBlock, cus);
-  g.trace = persistent_blocks(k_trace<false>, kTraceBlock, cus);
-  g.trace_dynamic = persistent_blocks(k_trace<true>, kTraceBlock, cus);
-  g.tail = persistent_blocks(k_tail<false>, kTraceBlock, cus);
-  g.tail_cache = persistent_blocks(k_tail<true>, kTraceBlock, cus);
+  // sized for the variants of scenes without analytic lights
+  g.trace = persistent_blocks(k_trace<false, false>, kTraceBlock, cus);
+  g.trace_dynamic = persistent_blocks(k_trace<true, false>, kTraceBlock, cus);
+  g.tail = persistent_blocks(k_tail<false, false>, kTraceBlock, cus);
+  g.tail_cache = persistent_blocks(k_tail<true, false>, kTraceBlock, cus);
   g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
   g.rays = persistent_blocks(k_rays, kTraceBlock, cus);
-  g.shade = persistent_blocks(k_shade, kShadeBlock, cus);
+  g.shade = persistent_blocks(k_shade<false>, kShadeBlock, cus);
   g.resolve = cus * 4;
   g.max_trace_lanes =
       std::max({g.camera, g.camera_dynamic, g.trace, g.trace_dynamic, g.tail, g.tail_cache, g.camera_pass, g.rays}) * kTraceBlock;
@@ -1110,23 +1120,41 @@ void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, con
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce, bool link) {
-  k_shade<<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
+  if (S.analytic_count != 0)
+    k_shade<true><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
+  else
+    k_shade<false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
 }
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds) {
-  if (L.dynamic_trace)
-    k_trace<true><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
-                                                                L.stats);
+  const bool analytic = S.analytic_count != 0;
+  if (L.dynamic_trace && analytic)
+    k_trace<true, true><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce,
+                                                                           kinds, L.stats);
+  else if (L.dynamic_trace)
+    k_trace<true, false><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce,
+                                                                            kinds, L.stats);
+  else if (analytic)
+    k_trace<false, true><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
+                                                                     L.stats);
   else
-    k_trace<false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
-                                                                 L.stats);
+    k_trace<false, false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
+                                                                      L.stats);
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce) {
-  if (F.cache_depth != 0)
-    k_tail<true><<<L.grid.tail_cache, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+  const bool analytic = S.analytic_count != 0;
+  if (F.cache_depth != 0 && analytic)
+    k_tail<true, true><<<L.grid.tail_cache, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
+                                                                       L.stats);
+  else if (F.cache_depth != 0)
+    k_tail<true, false><<<L.grid.tail_cache, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
+                                                                        L.stats);
+  else if (analytic)
+    k_tail<false, true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
   else
-    k_tail<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+    k_tail<false, false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
+                                                                    L.stats);
 }
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample) {
   k_resolve<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, add_sample);

@@ -22,7 +22,8 @@ struct Queues {
   float4* thr;    // per slot: throughput, last pdf (< 0: unweighted)
   float4* rad;    // per slot: radiance, rng bits; once ended, radiance and weight
   float4* nee;    // per slot: a shadow ray's light awaiting addition, flag
-  float4* esc;    // per slot: the environment an escaping ray found, flag
+  float4* esc;    // per slot: the environment an escaping ray found; w: 1, or 2 when esc2 follows
+  float4* esc2;   // per slot: the analytic lights that ray crossed (read only when esc.w is 2)
   float4* ray_o;  // ray queue: origin, slot
   float4* ray_d;  //            direction, tmax
   float4* hit_o;  // hit queue: ray origin, slot
@@ -75,6 +76,9 @@ struct Launch {
 // k_trace item kinds.
 constexpr uint32_t kTraceExtension = 1u;
 constexpr uint32_t kTraceShadow = 2u;
+// Light sampling is on: escaping rays weigh the analytic lights they cross
+// against it.
+constexpr uint32_t kTraceWeigh = 4u;
 
 void launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s);

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -146,6 +147,23 @@ struct TemporalFrameState {
   float4* filter = nullptr;
 };
 
+// The light table and the analytic lights as the device holds them.
+struct LightTableHost {
+  std::vector<uint32_t> offsets;  // per instance: its first entry or kNoLights
+  std::vector<LightRecord> lights;
+  std::vector<AnalyticRecord> analytic;
+  std::vector<LightNode> nodes;
+  std::vector<uint32_t> distant;
+  uint32_t parallel_count = 0, parallel = 0;  // parallel lights; the first one's index + 1
+  bool operator==(const LightTableHost& o) const {
+    auto same = [](const auto& a, const auto& b) {
+      return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(a[0])) == 0);
+    };
+    return same(offsets, o.offsets) && same(lights, o.lights) && same(analytic, o.analytic) && same(nodes, o.nodes) &&
+           same(distant, o.distant);
+  }
+};
+
 }  // namespace
 
 struct lotus_hip_renderer {
@@ -159,12 +177,12 @@ struct lotus_hip_renderer {
   bool has_scene = false;
   uint64_t scene_generation = 0;  // any change: restarts the accumulation
   uint64_t scene_epoch = 0;       // a change temporal history cannot carry over
-  std::vector<uint32_t> light_offsets_host;
-  std::vector<LightRecord> lights_host;
+  LightTableHost light_table;     // as uploaded
   // The applied plan, to find what the next one changes.
   struct PlanRecord {
     std::vector<float> positions, transforms;
     std::vector<uint32_t> triangles, gvo, gvc, gto, gtc, igeo, imat, materials;
+    std::vector<lotus_hip_light> lights;
     float environment[3] = {0, 0, 0};
     uint32_t leaf_size = 0;
   } plan;
@@ -236,6 +254,7 @@ void free_target(lotus_hip_renderer* r) {
   dev_free(q.rad);
   dev_free(q.nee);
   dev_free(q.esc);
+  dev_free(q.esc2);
   dev_free(q.ray_o);
   dev_free(q.ray_d);
   dev_free(q.hit_o);
@@ -268,6 +287,9 @@ void free_scene(lotus_hip_renderer* r) {
   dev_free(s.materials);
   dev_free(s.light_offsets);
   dev_free(s.lights);
+  dev_free(s.analytic);
+  dev_free(s.light_nodes);
+  dev_free(s.distant);
   dev_free(s.positions);
   dev_free(s.triangles);
   dev_free(s.geometry_vertex_offset);
@@ -276,6 +298,7 @@ void free_scene(lotus_hip_renderer* r) {
   dev_free(s.exact);
   s = SceneBuffers{};
   r->plan = lotus_hip_renderer::PlanRecord{};
+  r->light_table = LightTableHost{};
   r->has_scene = false;
 }
 
@@ -301,7 +324,7 @@ hipError_t ensure_target(lotus_hip_renderer* r, uint32_t width, uint32_t height)
   const size_t P = size_t(width) * height;
   Queues& q = r->q;
   hipError_t e = hipSuccess;
-  for (float4** p : {&q.thr, &q.rad, &q.nee, &q.esc, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.accum,
+  for (float4** p : {&q.thr, &q.rad, &q.nee, &q.esc, &q.esc2, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.accum,
                      &q.color})
     if (e == hipSuccess) e = dev_alloc(*p, P);
   if (e == hipSuccess) e = dev_alloc(q.hit_h, P);
@@ -423,12 +446,196 @@ std::string latch(lotus_hip_renderer* r, const std::string& what, hipError_t e) 
   return r->failure;
 }
 
-// gpu_scene.cpp:144-203, 512-609: the light table of emissive triangles,
-// weights = mean emission x world area in double, Vose's alias method.
-void build_light_table(const lotus_hip_scene_plan* p, std::vector<uint32_t>& offsets,
-                       std::vector<LightRecord>& lights) {
-  offsets.assign(p->instance_count, kNoLights);
-  lights.clear();
+void store4(float* out, double x, double y, double z, double w) {
+  out[0] = static_cast<float>(x);
+  out[1] = static_cast<float>(y);
+  out[2] = static_cast<float>(z);
+  out[3] = static_cast<float>(w);
+}
+
+// gpu_scene.cpp:233-270 (SceneRadius): half the diagonal of the world box of
+// the instances' object-space bounds' corners, in double.
+double scene_radius(const lotus_hip_scene_plan* p) {
+  const double inf = std::numeric_limits<double>::infinity();
+  std::vector<std::array<double, 6>> object(p->geometry_count, {inf, inf, inf, -inf, -inf, -inf});
+  for (uint32_t g = 0; g < p->geometry_count; ++g) {
+    const float* pos = p->positions + 3ull * p->geometry_vertex_offset[g];
+    for (uint32_t v = 0; v < p->geometry_vertex_count[g]; ++v)
+      for (int a = 0; a < 3; ++a) {
+        object[g][a] = std::min(object[g][a], double(pos[3 * v + a]));
+        object[g][3 + a] = std::max(object[g][3 + a], double(pos[3 * v + a]));
+      }
+  }
+  std::array<double, 6> world{inf, inf, inf, -inf, -inf, -inf};
+  for (uint32_t i = 0; i < p->instance_count; ++i) {
+    const auto& b = object[p->instance_geometry[i]];
+    if (!(b[0] <= b[3])) continue;
+    const float* m = p->instance_transform + 16ull * i;
+    for (int corner = 0; corner < 8; ++corner) {
+      const double q[3] = {b[(corner & 1) ? 3 : 0], b[(corner & 2) ? 4 : 1], b[(corner & 4) ? 5 : 2]};
+      for (int r = 0; r < 3; ++r) {
+        const double v = double(m[r]) * q[0] + double(m[4 + r]) * q[1] + double(m[8 + r]) * q[2] + double(m[12 + r]);
+        world[r] = std::min(world[r], v);
+        world[3 + r] = std::max(world[3 + r], v);
+      }
+    }
+  }
+  if (!(world[0] <= world[3])) return 0.0;
+  const double dx = world[3] - world[0], dy = world[4] - world[1], dz = world[5] - world[2];
+  const double radius = 0.5 * std::sqrt(dx * dx + dy * dy + dz * dz);
+  return std::isfinite(radius) ? radius : 0.0;
+}
+
+// gpu_scene.cpp:271-369 (MakeAnalyticLight): the world-space record and the
+// light-table weight, the estimated power over 2 pi (a distant light's over
+// a disk of the scene's radius). False for a light that emits nothing.
+bool make_analytic_light(const lotus_hip_light& L, double scene_r, AnalyticRecord& rec, double& weight) {
+  constexpr double kPiD = 3.14159265358979323846;
+  const float* m = L.world_from_light;
+  const double x[3] = {m[0], m[1], m[2]}, y[3] = {m[4], m[5], m[6]}, z[3] = {m[8], m[9], m[10]},
+               o[3] = {m[12], m[13], m[14]};
+  auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  auto cross = [](const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  double yz[3];
+  cross(y, z, yz);
+  const double det = dot(x, yz);
+  rec = AnalyticRecord{};
+  double extent = 0.0, scale = 1.0;
+  if (L.shape == LOTUS_HIP_LIGHT_SPHERE) {
+    const double radius = double(L.radius) * std::cbrt(std::fabs(det));
+    extent = 4.0 * kPiD * radius * radius;
+    rec.shape = kSphereLight;
+    store4(rec.position, o[0], o[1], o[2], radius);
+    store4(rec.normal, 0.0, 0.0, 0.0, extent);
+  } else if (L.shape == LOTUS_HIP_LIGHT_RECT || L.shape == LOTUS_HIP_LIGHT_DISK) {
+    const bool rect = L.shape == LOTUS_HIP_LIGHT_RECT;
+    const double su = rect ? 0.5 * double(L.width) : double(L.radius);
+    const double sv = rect ? 0.5 * double(L.height) : double(L.radius);
+    const double u[3] = {x[0] * su, x[1] * su, x[2] * su}, v[3] = {y[0] * sv, y[1] * sv, y[2] * sv};
+    double n[3];
+    cross(u, v, n);
+    const double len = std::sqrt(dot(n, n));
+    extent = (rect ? 4.0 : kPiD) * len;
+    rec.shape = rect ? kRectLight : kDiskLight;
+    store4(rec.position, o[0], o[1], o[2], 0.0);
+    store4(rec.axis_u, u[0], u[1], u[2], 0.0);
+    store4(rec.axis_v, v[0], v[1], v[2], 0.0);
+    if (len > 0.0) {
+      const double k = (det < 0.0 ? 1.0 : -1.0) / len;  // the -Z side, through the inverse transpose
+      store4(rec.normal, n[0] * k, n[1] * k, n[2] * k, extent);
+    }
+  } else {  // distant
+    const double len = std::sqrt(dot(z, z));
+    if (!(len > 0.0)) return false;
+    const double theta = std::min(double(L.angle) * kPiD / 360.0, kPiD);
+    const double half = std::sin(0.5 * theta);
+    const double one_minus_cos = 2.0 * half * half;
+    const double sin2 = std::sin(theta) * std::sin(theta);
+    store4(rec.axis_u, z[0] / len, z[1] / len, z[2] / len, one_minus_cos);
+    rec.shape = kParallelLight;
+    extent = 1.0;
+    if (theta > 0.0) {
+      extent = theta <= 0.5 * kPiD ? kPiD * sin2 : kPiD;
+      if (L.normalize) scale = 1.0 / (theta <= 0.5 * kPiD ? kPiD * sin2 : (2.0 - sin2) * kPiD);
+      if (static_cast<float>(one_minus_cos) >= std::numeric_limits<float>::min()) {
+        rec.shape = kDistantLight;
+      } else {  // too narrow a cone for float densities: one direction
+        scale *= extent;
+        extent = 1.0;
+      }
+    }
+  }
+  if (L.normalize && L.shape != LOTUS_HIP_LIGHT_DISTANT) scale = 1.0 / extent;
+  double radiance[3];
+  for (int c = 0; c < 3; ++c) radiance[c] = double(L.radiance[c]) * scale;
+  const double mean = (radiance[0] + radiance[1] + radiance[2]) / 3.0;
+  const double fmax = std::numeric_limits<float>::max();
+  if (!(extent > 0.0) || !std::isfinite(extent) || !(mean > 0.0) || !(radiance[0] <= fmax) ||
+      !(radiance[1] <= fmax) || !(radiance[2] <= fmax))
+    return false;
+  store4(rec.radiance, radiance[0], radiance[1], radiance[2], 0.0);
+  weight = mean * extent / 2.0;
+  if (L.shape == LOTUS_HIP_LIGHT_DISTANT) weight *= scene_r * scene_r;
+  if (!std::isfinite(weight)) weight = std::numeric_limits<double>::max();
+  return true;
+}
+
+// gpu_scene.cpp:379-442: a bounded light's box, widened by 1e-4 of its extent
+// plus 1e-5 of its coordinates' magnitude and rounded outwards.
+struct LightBox {
+  float low[3], high[3];
+  double centre[3];
+  uint32_t index;
+};
+
+LightBox bound_light(const AnalyticRecord& l, uint32_t index) {
+  LightBox b;
+  b.index = index;
+  for (int a = 0; a < 3; ++a) {
+    const double c = l.position[a], u = l.axis_u[a], v = l.axis_v[a];
+    const double extent = l.shape == kSphereLight ? double(l.position[3])
+                          : l.shape == kRectLight ? std::fabs(u) + std::fabs(v)
+                                                  : std::sqrt(u * u + v * v);
+    const double margin = 1e-4 * extent + 1e-5 * (std::fabs(c) + extent);
+    b.centre[a] = c;
+    b.low[a] = std::nextafter(static_cast<float>(c - extent - margin), -std::numeric_limits<float>::infinity());
+    b.high[a] = std::nextafter(static_cast<float>(c + extent + margin), std::numeric_limits<float>::infinity());
+  }
+  return b;
+}
+
+void store_bounds(float* low, float* high, const LightBox* first, const LightBox* last, uint32_t child) {
+  for (int a = 0; a < 3; ++a) {
+    low[a] = std::numeric_limits<float>::max();
+    high[a] = -std::numeric_limits<float>::max();
+    for (const LightBox* b = first; b != last; ++b) {
+      low[a] = std::min(low[a], b->low[a]);
+      high[a] = std::max(high[a], b->high[a]);
+    }
+  }
+  std::memcpy(&low[3], &child, 4);
+}
+
+// A median split along the axis the centres spread most over, ties by index
+// (gpu_scene.cpp:405-441): the hierarchy depends only on the lights.
+uint32_t build_light_node(LightBox* first, LightBox* last, std::vector<LightNode>& nodes) {
+  if (last - first == 1) return kLightLeaf | first->index;
+  const uint32_t node = static_cast<uint32_t>(nodes.size());
+  nodes.emplace_back();
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) lo[a] = hi[a] = first->centre[a];
+  for (const LightBox* b = first; b != last; ++b)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], b->centre[a]);
+      hi[a] = std::max(hi[a], b->centre[a]);
+    }
+  int axis = 0;
+  for (int a = 1; a < 3; ++a)
+    if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
+  LightBox* middle = first + (last - first) / 2;
+  std::nth_element(first, middle, last, [axis](const LightBox& a, const LightBox& b) {
+    return a.centre[axis] != b.centre[axis] ? a.centre[axis] < b.centre[axis] : a.index < b.index;
+  });
+  const uint32_t left = build_light_node(first, middle, nodes);
+  const uint32_t right = build_light_node(middle, last, nodes);
+  LightNode& n = nodes[node];
+  store_bounds(n.low0, n.high0, first, middle, left);
+  store_bounds(n.low1, n.high1, middle, last, right);
+  return node;
+}
+
+// gpu_scene.cpp:144-203, 512-609 (BuildLightTable): one entry per triangle
+// of every instance whose material emits (mean emission x world area, in
+// double), then one per analytic light of positive weight, chosen by Vose's
+// alias method; the crossing hierarchy over the bounded lights and the
+// distant lights' indices (gpu_scene.cpp:448-471).
+void build_light_table(const lotus_hip_scene_plan* p, LightTableHost& t) {
+  t = LightTableHost{};
+  t.offsets.assign(p->instance_count, kNoLights);
   std::vector<double> weights;
   for (uint32_t i = 0; i < p->instance_count; ++i) {
     const uint32_t* m = p->materials + 36ull * p->instance_material[i];
@@ -439,16 +646,16 @@ void build_light_table(const lotus_hip_scene_plan* p, std::vector<uint32_t>& off
     const uint32_t geo = p->instance_geometry[i];
     const uint32_t count = p->geometry_triangle_count[geo];
     if (mean <= 0.0 || count == 0) continue;
-    offsets[i] = static_cast<uint32_t>(lights.size());
+    t.offsets[i] = static_cast<uint32_t>(t.lights.size());
     const uint32_t* tri = p->triangles + 3ull * p->geometry_triangle_offset[geo];
     const float* pos = p->positions + 3ull * p->geometry_vertex_offset[geo];
-    const float* t = p->instance_transform + 16ull * i;
+    const float* tm = p->instance_transform + 16ull * i;
     for (uint32_t k = 0; k < count; ++k) {
       double c[3][3];
       for (int v = 0; v < 3; ++v) {
         const float* q = pos + 3ull * tri[3 * k + v];
         for (int row = 0; row < 3; ++row)
-          c[v][row] = double(t[row]) * q[0] + double(t[4 + row]) * q[1] + double(t[8 + row]) * q[2] + t[12 + row];
+          c[v][row] = double(tm[row]) * q[0] + double(tm[4 + row]) * q[1] + double(tm[8 + row]) * q[2] + tm[12 + row];
       }
       double a[3], b[3];
       for (int row = 0; row < 3; ++row) {
@@ -459,21 +666,54 @@ void build_light_table(const lotus_hip_scene_plan* p, std::vector<uint32_t>& off
       double area = 0.5 * std::sqrt(x * x + y * y + z * z);
       if (!std::isfinite(area)) area = 0.0;
       weights.push_back(mean * area);
-      lights.push_back(LightRecord{0.0f, 1.0f, 0u, i});
+      t.lights.push_back(LightRecord{0.0f, 1.0f, 0u, i});
     }
+  }
+  bool any_distant = false;
+  for (uint32_t l = 0; l < p->light_count; ++l) any_distant |= p->lights[l].shape == LOTUS_HIP_LIGHT_DISTANT;
+  const double radius = any_distant ? scene_radius(p) : 0.0;
+  std::vector<uint32_t> analytic_entries;
+  for (uint32_t l = 0; l < p->light_count; ++l) {
+    AnalyticRecord rec;
+    double weight = 0.0;
+    if (!make_analytic_light(p->lights[l], radius, rec, weight)) continue;
+    if (weight > 0.0) {
+      analytic_entries.push_back(static_cast<uint32_t>(t.lights.size()));
+      weights.push_back(weight);
+      t.lights.push_back(LightRecord{0.0f, 1.0f, 0u, kAnalyticLight | static_cast<uint32_t>(t.analytic.size())});
+    }
+    t.analytic.push_back(rec);
+  }
+  std::vector<LightBox> bounded;
+  for (uint32_t a = 0; a < t.analytic.size(); ++a) {
+    const uint32_t shape = t.analytic[a].shape;
+    if (shape == kDistantLight) {
+      t.distant.push_back(a);
+    } else if (shape == kParallelLight) {
+      if (t.parallel_count++ == 0) t.parallel = a + 1;
+    } else {
+      bounded.push_back(bound_light(t.analytic[a], a));
+    }
+  }
+  if (bounded.size() == 1) {
+    LightNode& root = t.nodes.emplace_back();
+    store_bounds(root.low0, root.high0, bounded.data(), bounded.data() + 1, kLightLeaf | bounded[0].index);
+    store_bounds(root.low1, root.high1, nullptr, nullptr, kNoLightChild);
+  } else if (bounded.size() > 1) {
+    build_light_node(bounded.data(), bounded.data() + bounded.size(), t.nodes);
   }
   double total = 0.0;
   for (double w : weights) total += w;
   if (!(total > 0.0) || !std::isfinite(total)) {
-    offsets.assign(p->instance_count, kNoLights);
-    lights.clear();
+    t.offsets.assign(p->instance_count, kNoLights);
+    t.lights.clear();
     return;
   }
-  const size_t count = lights.size();
+  const size_t count = t.lights.size();
   std::vector<double> scaled(count);
   std::vector<uint32_t> small, large;
   for (size_t i = 0; i < count; ++i) {
-    lights[i].pdf = static_cast<float>(weights[i] / total);
+    t.lights[i].pdf = static_cast<float>(weights[i] / total);
     scaled[i] = weights[i] / total * static_cast<double>(count);
     (scaled[i] < 1.0 ? small : large).push_back(static_cast<uint32_t>(i));
   }
@@ -481,8 +721,8 @@ void build_light_table(const lotus_hip_scene_plan* p, std::vector<uint32_t>& off
     const uint32_t less = small.back();
     small.pop_back();
     const uint32_t more = large.back();
-    lights[less].threshold = static_cast<float>(scaled[less]);
-    lights[less].alias = more;
+    t.lights[less].threshold = static_cast<float>(scaled[less]);
+    t.lights[less].alias = more;
     scaled[more] -= 1.0 - scaled[less];
     if (scaled[more] < 1.0) {
       large.pop_back();
@@ -490,13 +730,15 @@ void build_light_table(const lotus_hip_scene_plan* p, std::vector<uint32_t>& off
     }
   }
   for (uint32_t i : small) {
-    lights[i].threshold = 1.0f;
-    lights[i].alias = i;
+    t.lights[i].threshold = 1.0f;
+    t.lights[i].alias = i;
   }
   for (uint32_t i : large) {
-    lights[i].threshold = 1.0f;
-    lights[i].alias = i;
+    t.lights[i].threshold = 1.0f;
+    t.lights[i].alias = i;
   }
+  // each analytic record keeps its entry's probability (gpu_scene.cpp:600-604)
+  for (uint32_t e : analytic_entries) t.analytic[t.lights[e].instance & ~kAnalyticLight].radiance[3] = t.lights[e].pdf;
 }
 
 std::string validate_plan(const lotus_hip_scene_plan* p) {
@@ -521,6 +763,20 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
     tris += p->geometry_triangle_count[p->instance_geometry[i]];
   }
   if (tris >= (1ull << 27)) return "the scene holds more than 2^27 triangles";
+  if (p->light_count != 0 && !p->lights) return "the plan names lights but holds none";
+  if (p->light_count > (1u << 24)) return "the scene holds more than 2^24 analytic lights";
+  for (uint32_t l = 0; l < p->light_count; ++l) {
+    const lotus_hip_light& L = p->lights[l];
+    const std::string which = "light " + std::to_string(l);
+    if (L.shape > LOTUS_HIP_LIGHT_DISTANT) return which + " has an unknown shape";
+    for (int k = 0; k < 16; ++k)
+      if (!std::isfinite(L.world_from_light[k])) return which + ": its transform is not finite";
+    for (int c = 0; c < 3; ++c)
+      if (!(L.radiance[c] >= 0.0f) || !std::isfinite(L.radiance[c])) return which + ": radiance must be finite and non-negative";
+    if (!(L.radius >= 0.0f) || !(L.width >= 0.0f) || !(L.height >= 0.0f) || !(L.angle >= 0.0f) ||
+        !std::isfinite(L.radius) || !std::isfinite(L.width) || !std::isfinite(L.height) || !std::isfinite(L.angle))
+      return which + ": sizes must be finite and non-negative";
+  }
   for (uint32_t m = 0; m < p->material_count; ++m) {
     const uint32_t* w = p->materials + 36ull * m;
     if ((w[0] & 0xFFu) != 0) return "material " + std::to_string(m) + ": only UsdPreviewSurface is supported";
@@ -576,7 +832,7 @@ bool material_emits(const uint32_t* m) {
 struct Changes {
   uint32_t update = LOTUS_HIP_SCENE_BUILD;
   std::vector<uint32_t> moved;  // geometries whose positions differ
-  bool transforms = false, materials = false, instance_materials = false, environment = false;
+  bool transforms = false, materials = false, instance_materials = false, environment = false, lights = false;
   bool refit = false;
   bool temporal_compatible = false;
 };
@@ -604,8 +860,9 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
   c.materials = !same_words(old.materials, p->materials, 36ull * sh.materials);
   c.instance_materials = !same_words(old.imat, p->instance_material, sh.instances);
   c.environment = std::memcmp(old.environment, p->environment, sizeof old.environment) != 0;
+  c.lights = !same_words(old.lights, p->lights, p->light_count);
   c.refit = !c.moved.empty() || c.transforms;
-  if (!c.refit && !c.materials && !c.instance_materials && !c.environment) {
+  if (!c.refit && !c.materials && !c.instance_materials && !c.environment && !c.lights) {
     c.update = LOTUS_HIP_SCENE_UNCHANGED;
     c.temporal_compatible = true;
     return c;
@@ -614,7 +871,7 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
   // TemporalSceneCompatible (vulkan_backend.cpp:262-289): the environment,
   // the material table and the instances' slots are kept, and an emitting
   // instance keeps its transform and its geometry's positions.
-  c.temporal_compatible = !c.materials && !c.instance_materials && !c.environment;
+  c.temporal_compatible = !c.materials && !c.instance_materials && !c.environment && !c.lights;
   if (c.temporal_compatible && c.refit) {
     std::vector<char> moved(sh.geometries, 0);
     for (uint32_t g : c.moved) moved[g] = 1;
@@ -647,6 +904,7 @@ void record_plan(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   }
   if (all || c.transforms) d.transforms.assign(p->instance_transform, p->instance_transform + 16ull * sh.instances);
   if (all || c.materials) d.materials.assign(p->materials, p->materials + 36ull * sh.materials);
+  if (all || c.lights) d.lights.assign(p->lights, p->lights + p->light_count);
   if (all || c.instance_materials) d.imat.assign(p->instance_material, p->instance_material + sh.instances);
   std::memcpy(d.environment, p->environment, sizeof d.environment);
   d.leaf_size = sh.leaf_size;
@@ -665,6 +923,13 @@ void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environme
   v.light_offsets = s.light_offsets;
   v.lights = s.lights;
   v.light_count = s.light_count;
+  v.analytic = s.analytic;
+  v.light_nodes = s.light_nodes;
+  v.distant = s.distant;
+  v.analytic_count = s.analytic_count;
+  v.light_node_count = s.light_node_count;
+  v.distant_count = s.distant_count;
+  v.parallel = s.parallel;
   v.tri_count = sh.tri_count;
   v.instance_count = sh.instances;
   for (int c = 0; c < 3; ++c) v.environment[c] = environment[c];
@@ -674,7 +939,9 @@ uint64_t scene_device_bytes(const lotus_hip_renderer* r) {
   const SceneBuffers& s = r->scene;
   const uint64_t I = r->view.instance_count, G = r->plan.gvo.size();
   return s.bytes + 4 * (s.vertex_words + s.index_words) + 4 * (2 * G + I) + 4 * (I + 1) + 68 * I +
-         144ull * s.material_count + 4 * I + uint64_t(s.light_count) * sizeof(LightRecord);
+         144ull * s.material_count + 4 * I + uint64_t(s.light_count) * sizeof(LightRecord) +
+         uint64_t(s.analytic_count) * sizeof(AnalyticRecord) + uint64_t(s.light_node_count) * sizeof(LightNode) +
+         4ull * s.distant_count;
 }
 
 void fill_evidence(const lotus_hip_renderer* r, lotus_hip_scene_evidence* ev, uint32_t update, uint32_t moved,
@@ -701,13 +968,42 @@ void fill_evidence(const lotus_hip_renderer* r, lotus_hip_scene_evidence* ev, ui
   ev->uploaded_bytes = r->uploaded_bytes;
 }
 
+// The light table's device buffers: sized to `t` (reallocated when a size
+// changes) and filled; `up` enqueues a copy.
+template <typename Up>
+hipError_t upload_light_table(SceneBuffers& s, const LightTableHost& t, Up&& up) {
+  hipError_t e = hipSuccess;
+  auto fit = [&](auto*& ptr, size_t have, size_t need) {
+    if (e == hipSuccess && (ptr == nullptr || have != need)) {
+      dev_free(ptr);
+      e = dev_alloc(ptr, need);
+    }
+  };
+  fit(s.lights, s.light_count, t.lights.size());
+  fit(s.analytic, s.analytic_count, t.analytic.size());
+  fit(s.light_nodes, s.light_node_count, t.nodes.size());
+  fit(s.distant, s.distant_count, t.distant.size());
+  if (e != hipSuccess) return e;
+  s.light_count = static_cast<uint32_t>(t.lights.size());
+  s.analytic_count = static_cast<uint32_t>(t.analytic.size());
+  s.light_node_count = static_cast<uint32_t>(t.nodes.size());
+  s.distant_count = static_cast<uint32_t>(t.distant.size());
+  s.parallel_count = t.parallel_count;
+  s.parallel = t.parallel;
+  up(s.light_offsets, t.offsets.data(), t.offsets.size() * 4);
+  up(s.lights, t.lights.data(), t.lights.size() * sizeof(LightRecord));
+  up(s.analytic, t.analytic.data(), t.analytic.size() * sizeof(AnalyticRecord));
+  up(s.light_nodes, t.nodes.data(), t.nodes.size() * sizeof(LightNode));
+  up(s.distant, t.distant.data(), t.distant.size() * 4);
+  return hipSuccess;
+}
+
 // A new scene: every buffer uploaded, a new BVH built.
 bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const PlanShape& sh, double& upload_ms,
                  double& build_ms, std::string& error) {
   const auto t0 = std::chrono::steady_clock::now();
-  std::vector<uint32_t> offsets;
-  std::vector<LightRecord> lights;
-  build_light_table(p, offsets, lights);
+  LightTableHost table;
+  build_light_table(p, table);
   free_scene(r);
   SceneBuffers& s = r->scene;
   const uint32_t G = sh.geometries, I = sh.instances, M = sh.materials;
@@ -721,7 +1017,6 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   if (e == hipSuccess) e = dev_alloc(s.instance_transform, 16ull * I);
   if (e == hipSuccess) e = dev_alloc(s.materials, 36ull * M);
   if (e == hipSuccess) e = dev_alloc(s.light_offsets, I);
-  if (e == hipSuccess) e = dev_alloc(s.lights, lights.size());
   uint64_t bytes = 0;
   auto up = [&](void* dst, const void* src, size_t n) {
     if (e == hipSuccess && n) {
@@ -738,8 +1033,7 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   up(s.instance_material, p->instance_material, I * 4ull);
   up(s.instance_transform, p->instance_transform, 64ull * I);
   up(s.materials, p->materials, 144ull * M);
-  up(s.light_offsets, offsets.data(), offsets.size() * 4);
-  up(s.lights, lights.data(), lights.size() * sizeof(LightRecord));
+  if (e == hipSuccess) e = upload_light_table(s, table, up);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (e != hipSuccess) {
@@ -749,7 +1043,6 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   s.vertex_words = sh.vertex_words;
   s.index_words = sh.index_words;
   s.material_count = M;
-  s.light_count = static_cast<uint32_t>(lights.size());
   const BuildInput in{s.positions, s.triangles, s.geometry_vertex_offset, s.geometry_triangle_offset,
                       s.instance_geometry, s.instance_transform, s.instance_base, I, sh.tri_count, sh.leaf_size};
   if (!build_bvh(in, s, r->stream, build_ms, error)) return false;
@@ -759,8 +1052,7 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
     return false;
   }
   set_view(r, sh, p->environment);
-  r->light_offsets_host = std::move(offsets);
-  r->lights_host = std::move(lights);
+  r->light_table = std::move(table);
   r->has_scene = true;
   ++r->bvh_builds;
   ++r->material_writes;
@@ -776,12 +1068,9 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
   const auto t0 = std::chrono::steady_clock::now();
   SceneBuffers& s = r->scene;
   const uint32_t I = sh.instances, M = sh.materials;
-  std::vector<uint32_t> offsets;
-  std::vector<LightRecord> lights;
-  build_light_table(p, offsets, lights);
-  const bool lights_changed =
-      offsets != r->light_offsets_host || lights.size() != r->lights_host.size() ||
-      (!lights.empty() && std::memcmp(lights.data(), r->lights_host.data(), lights.size() * sizeof(LightRecord)) != 0);
+  LightTableHost table;
+  build_light_table(p, table);
+  const bool lights_changed = !(table == r->light_table);
   hipError_t e = hipSuccess;
   uint64_t bytes = 0;
   auto up = [&](void* dst, const void* src, size_t n) {
@@ -804,15 +1093,7 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
     }
     up(s.materials, p->materials, 144ull * M);
   }
-  if (lights_changed) {
-    if (s.light_count != lights.size() && e == hipSuccess) {
-      dev_free(s.lights);
-      e = dev_alloc(s.lights, lights.size());
-      s.light_count = static_cast<uint32_t>(lights.size());
-    }
-    up(s.light_offsets, offsets.data(), offsets.size() * 4);
-    up(s.lights, lights.data(), lights.size() * sizeof(LightRecord));
-  }
+  if (lights_changed && e == hipSuccess) e = upload_light_table(s, table, up);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (e != hipSuccess) {
@@ -828,8 +1109,7 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
   if (c.materials) ++r->material_writes;
   if (lights_changed) ++r->light_writes;
   r->uploaded_bytes += bytes;
-  r->light_offsets_host = std::move(offsets);
-  r->lights_host = std::move(lights);
+  r->light_table = std::move(table);
   set_view(r, sh, p->environment);
   return true;
 }
@@ -994,6 +1274,10 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
   }
   if (P->light_sampling > LOTUS_HIP_LIGHT_RESTIR) {
     set_detail(detail, detail_size, "unknown light sampling");
+    return LOTUS_HIP_FAIL;
+  }
+  if (P->light_sampling == LOTUS_HIP_LIGHT_BSDF && r->scene.parallel_count > 1) {
+    set_detail(detail, detail_size, "BSDF sampling traces at most one parallel light");
     return LOTUS_HIP_FAIL;
   }
   if (P->light_sampling == LOTUS_HIP_LIGHT_RESTIR && (P->restir_candidates == 0 || P->restir_candidates > 64)) {
@@ -1185,20 +1469,23 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
           mark(kTShade);
           launch_shade(L, r->view, F, r->q, r->counters, b, link);
           done();
-          const uint32_t kinds = (b + 1 < rounds ? kTraceExtension : 0u) |
-                                 (P->light_sampling != 0 && !link ? kTraceShadow : 0u);
+          // shadow rays: next event estimation, or under BSDF sampling a
+          // parallel light's
+          const bool shadows = P->light_sampling != 0 || r->view.parallel != 0;
+          const uint32_t kinds = (b + 1 < rounds ? kTraceExtension : 0u) | (shadows && !link ? kTraceShadow : 0u);
           if (kinds == 0) continue;
+          const uint32_t weigh = P->light_sampling != 0 ? kTraceWeigh : 0u;
           if (split) {
             for (const uint32_t kind : {kTraceExtension, kTraceShadow}) {
               if (!(kinds & kind)) continue;
               mark(kind == kTraceExtension ? kTIntersect : kTShadow);
-              launch_trace(L, r->view, r->q, r->counters, b + 1, kind);
+              launch_trace(L, r->view, r->q, r->counters, b + 1, kind | weigh);
               done();
               if (add) ++trace_launches;
             }
           } else {
             mark(kTTrace);
-            launch_trace(L, r->view, r->q, r->counters, b + 1, kinds);
+            launch_trace(L, r->view, r->q, r->counters, b + 1, kinds | weigh);
             done();
             if (add) ++trace_launches;
           }

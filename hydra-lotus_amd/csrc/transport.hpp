@@ -156,11 +156,20 @@ __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t 
   return sf;
 }
 
+// sampling.slang:67-76.
+__device__ __forceinline__ float power_heuristic(float chosen, float other) {
+  if (!(chosen > 0.0f)) return 0.0f;
+  if (isinf(chosen)) return isinf(other) ? 0.5f : 1.0f;
+  const float ratio = other / chosen;
+  return 1.0f / (1.0f + ratio * ratio);
+}
+
 struct LightSample {
   V3 direction;
   float distance;
   V3 radiance;
   float pdf;
+  bool delta;  // a parallel light: only next event estimation finds it
 };
 
 // scene.slang:632-677 (SampleTriangle): a point uniformly by area on light
@@ -185,6 +194,7 @@ __device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S,
   const float d2 = dot3(toward, toward);
   if (!(area > 0.0f) || !(d2 > 0.0f) || max3(radiance) <= 0.0f) return none;
   LightSample ls;
+  ls.delta = false;
   ls.direction = toward * rsqrt_rn(d2);
   const float cosine = fabsf(dot3(normal, ls.direction));
   if (!(cosine > 0.0f)) return none;
@@ -195,8 +205,128 @@ __device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S,
   return ls;
 }
 
+// MakeShadingFrame (sampling.slang:45-56, Duff et al. 2017): tangent and
+// bitangent of unit n.
+__device__ __forceinline__ void shading_frame(V3 n, V3& ft, V3& fb) {
+  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  ft = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+  fb = V3{b, sign + n.y * n.y * a, -n.y};
+}
+__device__ __forceinline__ V3 to_world(V3 v, V3 ft, V3 fb, V3 n) { return ((v.x * ft) + (v.y * fb)) + (v.z * n); }
+
+// scene.slang:682-689: the solid-angle density of a sphere's cone, given
+// sin^2 of its half angle (1 - cos without cancellation for small spheres).
+__device__ __forceinline__ float sphere_cone_pdf(float sin2_max) {
+  const float one_minus_cos = sin2_max < 0.00068523f ? 0.5f * sin2_max : 1.0f - sqrtf(maxf(0.0f, 1.0f - sin2_max));
+  return 1.0f / ((2.0f * kPi) * one_minus_cos);
+}
+
+// sampling.slang:89-104 (SampleConcentricDisk; Shirley and Chiu), the angle
+// as a fraction of a turn.
+__device__ __forceinline__ void concentric_disk(float ux, float uy, float& x, float& y) {
+  const float ox = 2.0f * ux - 1.0f, oy = 2.0f * uy - 1.0f;
+  if (ox == 0.0f && oy == 0.0f) {
+    x = y = 0.0f;
+    return;
+  }
+  float radius, turns;
+  if (fabsf(ox) > fabsf(oy)) {
+    radius = ox;
+    turns = 0.125f * (oy / ox);
+  } else {
+    radius = oy;
+    turns = 0.25f - 0.125f * (ox / oy);
+  }
+  float sn, cs;
+  sincos_turns(turns, &sn, &cs);
+  x = radius * cs;
+  y = radius * sn;
+}
+
+// scene.slang:693-760 (SampleAnalytic): a direction towards analytic light
+// `L` chosen with probability `selection`: by solid angle over a sphere's
+// cone or a distant light's, by area on a rect or disk; a parallel light's
+// one direction.
+__device__ __forceinline__ LightSample sample_analytic(const AnalyticRecord& L, float selection, float ux, float uy,
+                                                       V3 origin) {
+  const LightSample none{V3{0.0f, 0.0f, 1.0f}, 0.0f, V3{0.0f, 0.0f, 0.0f}, 0.0f, false};
+  LightSample ls = none;
+  ls.radiance = V3{L.radiance[0], L.radiance[1], L.radiance[2]};
+  const uint32_t shape = L.shape;
+  if (shape == kSphereLight) {
+    // uniform over the cone, then the cone direction's point on the near
+    // side (Pharr et al., Physically Based Rendering, 4th ed., 6.2.4)
+    const V3 centre = V3{L.position[0], L.position[1], L.position[2]};
+    const float radius = L.position[3];
+    const V3 to_centre = centre - origin;
+    const float d2 = dot3(to_centre, to_centre);
+    if (!(d2 > radius * radius)) return none;
+    const float sin2_max = radius * radius / d2;
+    const float sin_max = sqrtf(sin2_max);
+    const float cos_max = sqrtf(maxf(0.0f, 1.0f - sin2_max));
+    float cos_theta = (cos_max - 1.0f) * ux + 1.0f;
+    float sin2_theta = 1.0f - cos_theta * cos_theta;
+    if (sin2_max < 0.00068523f) {
+      sin2_theta = sin2_max * ux;
+      cos_theta = sqrtf(1.0f - sin2_theta);
+    }
+    const float cos_alpha = sin2_theta / sin_max + cos_theta * sqrtf(maxf(0.0f, 1.0f - sin2_theta / sin2_max));
+    const float sin_alpha = sqrtf(maxf(0.0f, 1.0f - cos_alpha * cos_alpha));
+    float sp, cp;
+    sincos_turns(uy, &sp, &cp);
+    const V3 n = normalize3(to_centre);
+    V3 ft, fb;
+    shading_frame(n, ft, fb);
+    const V3 outward = to_world(-V3{sin_alpha * cp, sin_alpha * sp, cos_alpha}, ft, fb, n);
+    const V3 toward = (centre + radius * outward) - origin;
+    ls.distance = length3(toward);
+    ls.direction = toward / ls.distance;
+    ls.pdf = selection * sphere_cone_pdf(sin2_max);
+  } else if (shape == kRectLight || shape == kDiskLight) {
+    float a, b;
+    if (shape == kRectLight) {
+      a = 2.0f * ux - 1.0f;
+      b = 2.0f * uy - 1.0f;
+    } else {
+      concentric_disk(ux, uy, a, b);
+    }
+    const V3 toward = ((V3{L.position[0], L.position[1], L.position[2]} + a * V3{L.axis_u[0], L.axis_u[1], L.axis_u[2]}) +
+                       b * V3{L.axis_v[0], L.axis_v[1], L.axis_v[2]}) -
+                      origin;
+    const float d2 = dot3(toward, toward);
+    if (!(d2 > 0.0f)) return none;
+    ls.distance = sqrtf(d2);
+    ls.direction = toward / ls.distance;
+    const float cosine = -dot3(V3{L.normal[0], L.normal[1], L.normal[2]}, ls.direction);
+    if (!(cosine > 0.0f)) return none;  // light leaves only its emitting side
+    ls.pdf = selection * d2 / (cosine * L.normal[3]);
+  } else if (shape == kDistantLight) {
+    // SampleUniformCone (sampling.slang:109-116) about the light's direction
+    const V3 n = V3{L.axis_u[0], L.axis_u[1], L.axis_u[2]};
+    const float one_minus_cos = ux * L.axis_u[3];
+    const float sin_theta = sqrtf(maxf(0.0f, one_minus_cos * (2.0f - one_minus_cos)));
+    float sp, cp;
+    sincos_turns(uy, &sp, &cp);
+    V3 ft, fb;
+    shading_frame(n, ft, fb);
+    ls.direction = normalize3(to_world(V3{sin_theta * cp, sin_theta * sp, 1.0f - one_minus_cos}, ft, fb, n));
+    ls.distance = kFloatMax;
+    ls.pdf = selection / ((2.0f * kPi) * L.axis_u[3]);
+  } else {
+    ls.direction = V3{L.axis_u[0], L.axis_u[1], L.axis_u[2]};
+    ls.distance = kFloatMax;
+    ls.pdf = selection;
+    ls.delta = true;
+  }
+  return ls;
+}
+
 // scene.slang:894-909 (SampleLight): the alias table's column and threshold
-// choose the light.
+// choose the light, an emissive triangle or (kAnalytic) an analytic light.
+// Kernels of scenes without analytic lights compile the triangle case alone.
+template <bool kAnalytic = true>
 __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux, float uy, float uz, float uw,
                                                     V3 origin) {
   const uint32_t count = S.light_count;
@@ -204,7 +334,168 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux
   const uint32_t column = c < count - 1 ? c : count - 1;
   const LightRecord rec = S.lights[column];
   const uint32_t light = uy < rec.threshold ? column : rec.alias;
+  if constexpr (kAnalytic) {
+    const LightRecord chosen = S.lights[light];
+    if (chosen.instance & kAnalyticLight)
+      return sample_analytic(S.analytic[chosen.instance & ~kAnalyticLight], chosen.pdf, uz, uw, origin);
+  }
   return sample_triangle_light(S, light, uz, uw, origin);
+}
+
+// scene.slang:939-971 (ReconstructReservoirLight): a light-table entry and
+// its unit-square coordinates as a sample; a sphere's candidate is a point
+// uniformly by area over the whole sphere (its far side contributing 0).
+template <bool kAnalytic = true>
+__device__ __forceinline__ LightSample reconstruct_reservoir_light(const SceneView& S, uint32_t index, float u,
+                                                                   float v, V3 origin) {
+  if constexpr (!kAnalytic) return sample_triangle_light(S, index, u, v, origin);
+  const LightRecord chosen = S.lights[index];
+  if ((chosen.instance & kAnalyticLight) == 0) return sample_triangle_light(S, index, u, v, origin);
+  const AnalyticRecord& L = S.analytic[chosen.instance & ~kAnalyticLight];
+  if (L.shape != kSphereLight) return sample_analytic(L, chosen.pdf, u, v, origin);
+  const LightSample none{V3{0.0f, 0.0f, 1.0f}, 0.0f, V3{0.0f, 0.0f, 0.0f}, 0.0f, false};
+  const float z = 1.0f - 2.0f * u;
+  const float rxy = sqrtf(maxf(0.0f, 1.0f - z * z));
+  float sp, cp;
+  sincos_turns(v, &sp, &cp);
+  const V3 normal = V3{rxy * cp, rxy * sp, z};
+  const float radius = L.position[3];
+  const V3 toward = (V3{L.position[0], L.position[1], L.position[2]} + radius * normal) - origin;
+  const float d2 = dot3(toward, toward);
+  if (!(d2 > 0.0f)) return none;
+  LightSample ls = none;
+  ls.distance = sqrtf(d2);
+  ls.direction = toward / ls.distance;
+  const float cosine = -dot3(normal, ls.direction);
+  if (!(cosine > 0.0f)) return none;
+  ls.radiance = V3{L.radiance[0], L.radiance[1], L.radiance[2]};
+  ls.pdf = chosen.pdf * d2 / ((((cosine * 4.0f) * kPi) * radius) * radius);
+  return ls;
+}
+
+// scene.slang:973-986 (ReservoirPartitionPdf): the power table's density of
+// the direction, a sphere's as its cone would sample it.
+template <bool kAnalytic = true>
+__device__ __forceinline__ float reservoir_partition_pdf(const SceneView& S, uint32_t index, const LightSample& ls,
+                                                         V3 origin) {
+  if constexpr (!kAnalytic) return ls.pdf;
+  const LightRecord chosen = S.lights[index];
+  if (chosen.instance & kAnalyticLight) {
+    const AnalyticRecord& L = S.analytic[chosen.instance & ~kAnalyticLight];
+    if (L.shape == kSphereLight) {
+      const V3 offset = V3{L.position[0], L.position[1], L.position[2]} - origin;
+      return chosen.pdf * sphere_cone_pdf(L.position[3] * L.position[3] / dot3(offset, offset));
+    }
+  }
+  return ls.pdf;
+}
+
+// scene.slang:1019-1093 (CrossedLight): the emission of analytic light L if
+// the ray crosses it before `distance`, weighed against light sampling by
+// the power heuristic when `weigh`.
+__device__ __forceinline__ V3 crossed_light(const AnalyticRecord& L, V3 o, V3 d, float distance, bool escaped,
+                                            bool weigh, float pdf) {
+  const V3 zero{0.0f, 0.0f, 0.0f};
+  const uint32_t shape = L.shape;
+  float light_pdf = 0.0f;
+  if (shape == kSphereLight) {
+    // the nearer root without cancellation (Haines et al., Ray Tracing Gems, 2019)
+    const float radius = L.position[3];
+    const V3 offset = o - V3{L.position[0], L.position[1], L.position[2]};
+    const float c = dot3(offset, offset) - radius * radius;
+    const float b = -dot3(offset, d);
+    const V3 perpendicular = offset + b * d;
+    const float disc = radius * radius - dot3(perpendicular, perpendicular);
+    if (!(c > 0.0f) || !(b > 0.0f) || disc < 0.0f) return zero;
+    const float t = c / (b + sqrtf(disc));
+    if (!(t < distance)) return zero;
+    light_pdf = sphere_cone_pdf(radius * radius / dot3(offset, offset));
+  } else if (shape == kRectLight || shape == kDiskLight) {
+    const V3 n = V3{L.normal[0], L.normal[1], L.normal[2]};
+    const float facing = dot3(d, n);
+    if (!(facing < 0.0f)) return zero;
+    const V3 p = V3{L.position[0], L.position[1], L.position[2]};
+    const float t = dot3(p - o, n) / facing;
+    if (!(t > 0.0f) || !(t < distance)) return zero;
+    const V3 q = (o + t * d) - p;
+    const V3 u = V3{L.axis_u[0], L.axis_u[1], L.axis_u[2]}, v = V3{L.axis_v[0], L.axis_v[1], L.axis_v[2]};
+    const float uu = dot3(u, u), uv = dot3(u, v), vv = dot3(v, v), qu = dot3(q, u), qv = dot3(q, v);
+    const float det = uu * vv - uv * uv;
+    const float a = (qu * vv - qv * uv) / det;
+    const float b = (qv * uu - qu * uv) / det;
+    if (shape == kRectLight ? !(fabsf(a) <= 1.0f && fabsf(b) <= 1.0f) : !(a * a + b * b <= 1.0f)) return zero;
+    light_pdf = t * t / (-facing * L.normal[3]);
+  } else if (shape == kDistantLight && escaped) {
+    const V3 toward = V3{L.axis_u[0], L.axis_u[1], L.axis_u[2]};
+    const float one_minus_cos = L.axis_u[3];
+    const float cosine = dot3(d, toward);
+    if (one_minus_cos <= 1.0f) {  // within the cone, by its sine where the cosine is too coarse
+      const V3 across = cross3(d, toward);
+      if (!(cosine > 0.0f) || !(dot3(across, across) <= one_minus_cos * (2.0f - one_minus_cos))) return zero;
+    } else if (!(cosine >= 1.0f - one_minus_cos)) {
+      return zero;
+    }
+    light_pdf = 1.0f / ((2.0f * kPi) * one_minus_cos);
+  } else {
+    return zero;
+  }
+  const float weight = weigh ? power_heuristic(pdf, L.radiance[3] * light_pdf) : 1.0f;
+  return V3{L.radiance[0], L.radiance[1], L.radiance[2]} * weight;
+}
+
+// scene.slang:1095-1108 (OverlapsBox).
+__device__ __forceinline__ bool overlaps_box(const float* lo, const float* hi, V3 o, V3 inv, float distance) {
+  const float tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
+  const float ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
+  const float tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
+  const float nx = minf(tx0, tx1), ny = minf(ty0, ty1), nz = minf(tz0, tz1);
+  const float fx = maxf(tx0, tx1), fy = maxf(ty0, ty1), fz = maxf(tz0, tz1);
+  return maxf(maxf(nx, ny), maxf(nz, 0.0f)) <= minf(minf(fx, fy), minf(fz, distance));
+}
+
+// scene.slang:1110-1182 (CrossedLights): the emission of every analytic
+// light the ray crosses before `distance`: the sphere, rect and disk lights
+// through the crossing hierarchy in its order, then, for a ray that
+// escaped, the distant lights.
+__device__ __forceinline__ V3 crossed_lights(const SceneView& S, V3 o, V3 d, float distance, bool escaped, bool weigh,
+                                             float pdf) {
+  V3 sum{0.0f, 0.0f, 0.0f};
+  if (S.light_node_count != 0) {
+    auto safe = [](float x) { return fabsf(x) > 1e-30f ? x : (x < 0.0f ? -1e-30f : 1e-30f); };
+    const V3 inv = V3{1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z)};
+    uint32_t stack[kLightStack];
+    uint32_t depth = 0, node = 0;
+    while (true) {
+      const LightNode cur = S.light_nodes[node];
+      const uint32_t child0 = __float_as_uint(cur.low0[3]), child1 = __float_as_uint(cur.low1[3]);
+      const bool enter0 = overlaps_box(cur.low0, cur.high0, o, inv, distance);
+      const bool enter1 = child1 != kNoLightChild && overlaps_box(cur.low1, cur.high1, o, inv, distance);
+      uint32_t next = 0xFFFFFFFFu;
+      if (enter0) {
+        if (child0 & kLightLeaf)
+          sum = sum + crossed_light(S.analytic[child0 & ~kLightLeaf], o, d, distance, escaped, weigh, pdf);
+        else
+          next = child0;
+      }
+      if (enter1) {
+        if (child1 & kLightLeaf)
+          sum = sum + crossed_light(S.analytic[child1 & ~kLightLeaf], o, d, distance, escaped, weigh, pdf);
+        else if (next == 0xFFFFFFFFu)
+          next = child1;
+        else if (depth < kLightStack)
+          stack[depth++] = child1;
+      }
+      if (next == 0xFFFFFFFFu) {
+        if (depth == 0) break;
+        next = stack[--depth];
+      }
+      node = next;
+    }
+  }
+  if (escaped)
+    for (uint32_t i = 0; i < S.distant_count; ++i)
+      sum = sum + crossed_light(S.analytic[S.distant[i]], o, d, distance, escaped, weigh, pdf);
+  return sum;
 }
 
 // ------------------------------------------------------------- ReSTIR DI
@@ -291,14 +582,6 @@ __device__ __forceinline__ float emitter_pdf(const SceneView& S, uint32_t instan
   const uint32_t first = S.light_offsets[instance];
   if (first == kNoLights || !(area > 0.0f)) return 0.0f;
   return S.lights[first + primitive].pdf / area;
-}
-
-// sampling.slang:67-76.
-__device__ __forceinline__ float power_heuristic(float chosen, float other) {
-  if (!(chosen > 0.0f)) return 0.0f;
-  if (isinf(chosen)) return isinf(other) ? 0.5f : 1.0f;
-  const float ratio = other / chosen;
-  return 1.0f / (1.0f + ratio * ratio);
 }
 
 // The outcome of one scattering event.
@@ -403,6 +686,7 @@ __device__ __forceinline__ V3 reuse_contribution(const SceneView& S, const Reuse
 // restir.slang:235-254 (InitialDirectLight): the first hit's direct light
 // from its reservoir record, weighted against BSDF sampling by the power
 // heuristic of the light table's density.
+template <bool kAnalytic>
 __device__ __forceinline__ V3 initial_direct_light(const SceneView& S, const float4* rec, const Surface& sf,
                                                    const V3& ft, const V3& fb, V3 wo) {
   const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
@@ -410,17 +694,31 @@ __device__ __forceinline__ V3 initial_direct_light(const SceneView& S, const flo
   if ((__float_as_uint(r1.w) & 2u) != 0u || !(r0.y > 0.0f) || light >= S.light_count)
     return V3{0.0f, 0.0f, 0.0f};
   const V3 origin = offset_ray_origin(sf.position, sf.normal);
-  const LightSample ls = sample_triangle_light(S, light, r1.x, r1.y, origin);
+  const LightSample ls = reconstruct_reservoir_light<kAnalytic>(S, light, r1.x, r1.y, origin);
   const V3 n = sf.shading_normal;
   const V3 wi = V3{dot3(ls.direction, ft), dot3(ls.direction, fb), dot3(ls.direction, n)};
   const float bsdf = wi.z > 0.0f ? bsdf_pdf(sf.bsdf, wo, wi) : 0.0f;
-  return xyz(r2) * power_heuristic(ls.pdf, bsdf);
+  const float weight = ls.delta ? 1.0f : power_heuristic(reservoir_partition_pdf<kAnalytic>(S, light, ls, origin), bsdf);
+  return xyz(r2) * weight;
+}
+
+// transport.slang:235-247 (MissPath) for a continuation ray that leaves the
+// scene: the constant environment, then the analytic lights it crossed
+// (`crossed`, zero without them), each times the throughput.
+template <bool kAnalytic>
+__device__ __forceinline__ void escape_terms(const SceneView& S, bool sample_lights, V3 o, V3 d, V3 throughput,
+                                             float last_pdf, V3& env, V3& crossed) {
+  env = throughput * V3{S.environment[0], S.environment[1], S.environment[2]};
+  crossed = V3{0.0f, 0.0f, 0.0f};
+  if (kAnalytic && S.analytic_count != 0)
+    crossed = throughput * crossed_lights(S, o, d, kFloatMax, true, sample_lights && !(last_pdf < 0.0f), last_pdf);
 }
 
 // transport.slang:164-230 (ScatterPath) with DirectLight (:123-135) and
 // ShadowedLight (:82-115) up to the shadow ray, which the caller traces.
 // `keep`, when given, receives the reconstructed surface; `slot` indexes the
 // ReSTIR record of the first hit.
+template <bool kAnalytic>
 __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameConst& F, PathRegs& p,
                                                 uint32_t tri, uint32_t instance, uint32_t primitive, float u,
                                                 float v, Surface* keep = nullptr, uint32_t slot = 0) {
@@ -431,6 +729,11 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
   const Surface sf = reconstruct_hit(S, tri, instance, u, v, p.d);
   if (keep) *keep = sf;
   add_emission(S, F, p, sf, instance, primitive);
+  // the analytic lights the continuation ray crossed before the hit
+  // (transport.slang:171-178); camera rays do not see them
+  if (kAnalytic && p.depth != 0 && S.analytic_count != 0)
+    p.radiance = p.radiance + p.throughput * crossed_lights(S, p.o, p.d, dot3(sf.position - p.o, p.d), false,
+                                                            sample_lights && !(p.last_pdf < 0.0f), p.last_pdf);
   if (p.depth >= F.max_bounces) return out;
   if (p.depth >= kRouletteBounces) {
     const float survival = minf(max3(p.throughput), kMaxSurvival);
@@ -447,18 +750,28 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
   const V3 md = -p.d;
   const V3 wo = V3{dot3(md, ft), dot3(md, fb), dot3(md, n)};
   if (F.light_sampling == kLightRestir && p.depth == 0) {
-    p.radiance = p.radiance + p.throughput * initial_direct_light(S, F.restir + 4ull * slot, sf, ft, fb, wo);
-  } else if (sample_lights && S.light_count != 0) {
-    const float ux = next_random(p.rng), uy = next_random(p.rng), uz = next_random(p.rng),
-                uw = next_random(p.rng);
+    p.radiance = p.radiance + p.throughput * initial_direct_light<kAnalytic>(S, F.restir + 4ull * slot, sf, ft, fb, wo);
+  } else if ((sample_lights && S.light_count != 0) || (kAnalytic && !sample_lights && S.parallel != 0)) {
+    // next event estimation (transport.slang:123-135), or under BSDF sampling
+    // the parallel light no sampled direction finds (:139-146; one such
+    // light, the renderer checks), through ShadowedLight (:82-115)
     const V3 origin = offset_ray_origin(sf.position, sf.normal);
-    const LightSample ls = sample_light(S, ux, uy, uz, uw, origin);
+    LightSample ls{};
+    if (sample_lights) {
+      const float ux = next_random(p.rng), uy = next_random(p.rng), uz = next_random(p.rng),
+                  uw = next_random(p.rng);
+      ls = sample_light<kAnalytic>(S, ux, uy, uz, uw, origin);
+    } else if constexpr (kAnalytic) {
+      const AnalyticRecord& L = S.analytic[S.parallel - 1];
+      ls = LightSample{V3{L.axis_u[0], L.axis_u[1], L.axis_u[2]}, kFloatMax,
+                       V3{L.radiance[0], L.radiance[1], L.radiance[2]}, 1.0f, true};
+    }
     if ((ls.pdf > 0.0f) && !(max3(ls.radiance) <= 0.0f) && !(dot3(ls.direction, sf.normal) <= 0.0f)) {
       const V3 wi = V3{dot3(ls.direction, ft), dot3(ls.direction, fb), dot3(ls.direction, n)};
       if (!(wi.z <= 0.0f)) {
         const V3 f = bsdf_evaluate(sf.bsdf, wo, wi);
         if (!(max3(f) <= 0.0f)) {
-          const float weight = power_heuristic(ls.pdf, bsdf_pdf(sf.bsdf, wo, wi));
+          const float weight = ls.delta ? 1.0f : power_heuristic(ls.pdf, bsdf_pdf(sf.bsdf, wo, wi));
           const V3 c = f * ls.radiance * (1.0f * weight / ls.pdf);
           out.shadow = true;
           out.shadow_ray = Ray{origin, 0.0f, ls.direction, ls.distance};

@@ -66,6 +66,8 @@ class ScenePlan(C.Structure):
         ("materials", _u32p),
         ("environment", C.c_float * 3),
         ("bvh_leaf_size", C.c_uint32),
+        ("light_count", C.c_uint32),
+        ("lights", C.c_void_p),
     ]
 
 
@@ -407,6 +409,8 @@ class Renderer:
         for i in range(3):
             plan.environment[i] = float(p["environment"][i])
         plan.bvh_leaf_size = leaf_size
+        plan.light_count = len(p["lights"])
+        plan.lights = p["lights"].ctypes.data if len(p["lights"]) else None
         ev = SceneEvidence()
         buf = C.create_string_buffer(512)
         s = lib().lotus_hip_scene_apply(self.handle, C.byref(plan), C.byref(ev), buf, 512)

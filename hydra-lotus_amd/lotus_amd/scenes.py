@@ -65,6 +65,26 @@ def preview_material(base_color=(0.18, 0.18, 0.18), roughness=0.5, metallic=0.0,
 
 
 # ----------------------------------------------------------------- scene --
+# lotus_hip_light (Light, include/lotus/render_world.hpp:89-114).
+LIGHT_SPHERE, LIGHT_RECT, LIGHT_DISK, LIGHT_DISTANT = range(4)
+LIGHT_DTYPE = np.dtype([("shape", "<u4"), ("normalize", "<u4"), ("world_from_light", "<f4", 16),
+                        ("radiance", "<f4", 3), ("radius", "<f4"), ("width", "<f4"), ("height", "<f4"),
+                        ("angle", "<f4")])
+
+
+def analytic_light(shape, transform=None, radiance=(1.0, 1.0, 1.0), radius=0.5, width=1.0, height=1.0,
+                   angle=0.53, normalize=False) -> np.ndarray:
+    """One analytic light record: a sphere centred on its frame's origin, a
+    rect or disk in its frame's XY plane emitting from its -Z side, or a
+    distant light shining along its frame's -Z axis (angle 0: parallel)."""
+    rec = np.zeros((), LIGHT_DTYPE)
+    rec["shape"], rec["normalize"] = shape, int(normalize)
+    rec["world_from_light"] = identity() if transform is None else np.asarray(transform, F32).reshape(16)
+    rec["radiance"] = radiance
+    rec["radius"], rec["width"], rec["height"], rec["angle"] = radius, width, height, angle
+    return rec
+
+
 @dataclasses.dataclass
 class Scene:
     """Object-space geometries placed by instances; materials as records;
@@ -74,6 +94,7 @@ class Scene:
     instances: List[Tuple[int, np.ndarray, int]] = dataclasses.field(default_factory=list)
     materials: List[np.ndarray] = dataclasses.field(default_factory=lambda: [preview_material()])
     environment: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    lights: List[np.ndarray] = dataclasses.field(default_factory=list)
     view: np.ndarray = dataclasses.field(default_factory=identity)
     projection: np.ndarray = dataclasses.field(default_factory=identity)
     name: str = "scene"
@@ -88,6 +109,11 @@ class Scene:
         self._packed = None
         return len(self.instances) - 1
 
+    def add_light(self, record: np.ndarray) -> int:
+        self.lights.append(np.array(record, LIGHT_DTYPE))
+        self._packed = None
+        return len(self.lights) - 1
+
     def add_material(self, record: np.ndarray) -> int:
         self.materials.append(np.ascontiguousarray(record, np.uint32))
         self._packed = None
@@ -98,6 +124,7 @@ class Scene:
         return Scene(geometries=[(p.copy(), t.copy()) for p, t in self.geometries],
                      instances=[(g, t.copy(), m) for g, t, m in self.instances],
                      materials=[m.copy() for m in self.materials], environment=tuple(self.environment),
+                     lights=[l.copy() for l in self.lights],
                      view=self.view.copy(), projection=self.projection.copy(), name=self.name)
 
     def set_positions(self, geometry: int, positions) -> None:
@@ -158,6 +185,7 @@ class Scene:
             material_count=len(self.materials),
             materials=np.ascontiguousarray(np.stack(self.materials).astype(np.uint32)),
             environment=np.array(self.environment, F32),
+            lights=np.ascontiguousarray(np.array(self.lights, LIGHT_DTYPE).reshape(-1)),
         )
         return self._packed
 
@@ -433,6 +461,39 @@ def lambert_under_sky() -> Scene:
     sc.add_mesh([[-0.65, -0.35, 0], [0.55, -0.45, 0], [0.15, 0.70, 0]], [[0, 1, 2]], m)
     sc.environment = (0.8, 0.6, 0.4)
     return sc
+
+
+def lit_floor(light: np.ndarray, albedo: float = 0.5, height: float = 2.0) -> Scene:
+    """A Lambert floor (ior 1) of half-size 50 in y = 0, seen straight down
+    through a 1-degree field of view from `height`, lit only by `light`: the
+    closed-form scenes of the analytic lights (each pixel sees a point within
+    0.02 of the origin, where the light's irradiance is known)."""
+    sc = Scene(name="lit-floor")
+    m = sc.add_material(preview_material(base_color=(albedo, albedo, albedo), ior=1.0))
+    h = 50.0
+    sc.add_mesh([[-h, 0.0, -h], [h, 0.0, -h], [h, 0.0, h], [-h, 0.0, h]], [[0, 2, 1], [0, 3, 2]], m)
+    sc.view = look_at((0.0, height, 0.0), (0.0, 0.0, 0.0), up=(0.0, 0.0, 1.0))
+    sc.projection = perspective(1.0, 1.0, 0.05, 10.0)
+    sc.add_light(light)
+    return sc
+
+
+def translation(x: float, y: float, z: float, scale: float = 1.0) -> np.ndarray:
+    m = identity() * F32(scale)
+    m[15] = 1.0
+    m[12:15] = (x, y, z)
+    return m
+
+
+def facing_down(x: float, y: float, z: float) -> np.ndarray:
+    """A light frame at (x, y, z) whose -Z axis points down (-Y): a rect or
+    disk there emits towards the floor, a distant light shines down."""
+    m = np.zeros(16, F32)
+    m[0:3] = (1.0, 0.0, 0.0)    # X
+    m[4:7] = (0.0, 0.0, -1.0)   # Y
+    m[8:11] = (0.0, 1.0, 0.0)   # Z: up, so -Z is down
+    m[12:16] = (x, y, z, 1.0)
+    return m
 
 
 def closed_box() -> Scene:

@@ -44,6 +44,26 @@ typedef struct lotus_hip_renderer lotus_hip_renderer;
  * GpuMaterialRecord words (gpu_scene.hpp:94-113, 36 words each). The
  * renderer compares a plan with the resident one to apply only what changed
  * (see lotus_hip_scene_evidence.update). */
+/* An analytic light (Light, include/lotus/render_world.hpp:89-114). A
+ * sphere is centred on its frame's origin, its radius scaled by the cube
+ * root of the frame's absolute determinant; a rect or disk lies in its
+ * frame's XY plane and emits from its -Z side; a distant light shines along
+ * its frame's -Z axis from beyond the scene, `angle` its angular diameter in
+ * degrees (0: a parallel light, whose radiance is its irradiance). */
+#define LOTUS_HIP_LIGHT_SPHERE 0u
+#define LOTUS_HIP_LIGHT_RECT 1u
+#define LOTUS_HIP_LIGHT_DISK 2u
+#define LOTUS_HIP_LIGHT_DISTANT 3u
+typedef struct lotus_hip_light {
+  uint32_t shape;             /* LOTUS_HIP_LIGHT_* */
+  uint32_t normalize;         /* UsdLux normalize: radiance over area (or the distant light's size factor) */
+  float world_from_light[16]; /* column-major */
+  float radiance[3];          /* linear RGB, non-negative */
+  float radius;               /* sphere, disk */
+  float width, height;        /* rect, along X and Y */
+  float angle;                /* distant, degrees */
+} lotus_hip_light;
+
 typedef struct lotus_hip_scene_plan {
   uint32_t geometry_count;
   const float* positions;                   /* xyz per vertex, all geometries */
@@ -60,6 +80,8 @@ typedef struct lotus_hip_scene_plan {
   const uint32_t* materials;                /* 36 words per material */
   float environment[3];                     /* constant environment radiance */
   uint32_t bvh_leaf_size;                   /* 0: default (2); at most 8 */
+  uint32_t light_count;                     /* analytic lights, in the light table after the triangles */
+  const lotus_hip_light* lights;
 } lotus_hip_scene_plan;
 
 /* What a lotus_hip_scene_apply did, the counterpart of GpuScene::Apply's

@@ -48,6 +48,9 @@ class SceneDesc(C.Structure):
         ("material_count", C.c_uint32),
         ("materials", _u32p),
         ("environment", C.c_float * 3),
+        ("geometry_vertex_count", _u32p),
+        ("light_count", C.c_uint32),
+        ("lights", C.c_void_p),
     ]
 
 
@@ -179,6 +182,10 @@ class OracleScene:
         d.materials = ptr(p["materials"], C.c_uint32)
         for i in range(3):
             d.environment[i] = float(p["environment"][i])
+        d.geometry_vertex_count = ptr(p["geometry_vertex_count"], C.c_uint32)
+        lights = p.get("lights")
+        d.light_count = 0 if lights is None else len(lights)
+        d.lights = lights.ctypes.data if d.light_count else None
         err = C.create_string_buffer(256)
         self.handle = lib().oracle_scene_create(C.byref(d), err, 256)
         if not self.handle:

@@ -522,7 +522,24 @@ struct LightRecord {
   uint32_t alias, instance;
 };
 
+// An analytic light in world space (GpuAnalyticLightRecord,
+// gpu_scene.hpp:144-166) and a node of the crossing hierarchy over the
+// sphere, rect and disk lights (GpuLightNode, :175-185).
+constexpr uint32_t kAnalyticLight = 0x80000000u;
+constexpr uint32_t kSphereLight = 0, kRectLight = 1, kDiskLight = 2, kDistantLight = 3, kParallelLight = 4;
+constexpr uint32_t kLightLeaf = 0x80000000u, kNoLightChild = 0xFFFFFFFFu;
+struct Analytic {
+  float position[4], axis_u[4], axis_v[4], normal[4], radiance[4];
+  uint32_t shape;
+};
+struct LightNode {
+  float low0[4], high0[4], low1[4], high1[4];
+};
+
 struct Scene {
+  std::vector<Analytic> analytic;
+  std::vector<LightNode> light_nodes;
+  std::vector<uint32_t> distant;
   std::vector<F3> v0, v1, v2;           // world corners per flattened triangle
   std::vector<uint32_t> tri_instance;   // per flattened triangle
   std::vector<uint32_t> tri_primitive;
@@ -929,6 +946,7 @@ struct LightSample {
   float distance;
   F3 radiance;
   float pdf;
+  bool delta = false;  // a parallel light, which only next event estimation finds
 };
 // scene.slang:632-677 (SampleTriangle): a point uniformly by area on light
 // `light` of the table, (uz, uw) the unit-square coordinates.
@@ -960,15 +978,252 @@ LightSample SampleTriangleLight(const Scene& s, uint32_t light, float uz, float 
   return ls;
 }
 
+inline F3 Xyz(const float* v) { return F3{v[0], v[1], v[2]}; }
+inline LightSample NoLight() { return LightSample{F3{0, 0, 1}, 0.0f, F3{0, 0, 0}, 0.0f, false}; }
+
+// scene.slang:682-689 (SphereConePdf).
+float SphereConePdf(float sin2_max) {
+  const float one_minus_cos =
+      sin2_max < 0.00068523f ? 0.5f * sin2_max : 1.0f - Sqrt(std::max(0.0f, 1.0f - sin2_max));
+  return 1.0f / ((2.0f * kPi) * one_minus_cos);
+}
+
+// sampling.slang:89-104 (SampleConcentricDisk), angles as turns for SinCosTurns.
+void ConcentricDisk(float ux, float uy, float& x, float& y) {
+  const float ox = 2.0f * ux - 1.0f, oy = 2.0f * uy - 1.0f;
+  if (ox == 0.0f && oy == 0.0f) {
+    x = y = 0.0f;
+    return;
+  }
+  float radius, turns;
+  if (std::fabs(ox) > std::fabs(oy)) {
+    radius = ox;
+    turns = 0.125f * (oy / ox);
+  } else {
+    radius = oy;
+    turns = 0.25f - 0.125f * (ox / oy);
+  }
+  float sn, cs;
+  SinCosTurns(turns, &sn, &cs);
+  x = radius * cs;
+  y = radius * sn;
+}
+
+// scene.slang:693-760 (SampleAnalytic).
+LightSample SampleAnalytic(const Analytic& L, float selection, float ux, float uy, F3 origin) {
+  LightSample sample = NoLight();
+  sample.radiance = Xyz(L.radiance);
+  if (L.shape == kSphereLight) {
+    const F3 centre = Xyz(L.position);
+    const float radius = L.position[3];
+    const F3 to_centre = centre - origin;
+    const float distance2 = Dot(to_centre, to_centre);
+    if (!(distance2 > radius * radius)) return NoLight();
+    const float sin2_max = radius * radius / distance2;
+    const float sin_max = Sqrt(sin2_max);
+    const float cos_max = Sqrt(std::max(0.0f, 1.0f - sin2_max));
+    float cos_theta = (cos_max - 1.0f) * ux + 1.0f;
+    float sin2_theta = 1.0f - cos_theta * cos_theta;
+    if (sin2_max < 0.00068523f) {
+      sin2_theta = sin2_max * ux;
+      cos_theta = Sqrt(1.0f - sin2_theta);
+    }
+    const float cos_alpha =
+        sin2_theta / sin_max + cos_theta * Sqrt(std::max(0.0f, 1.0f - sin2_theta / sin2_max));
+    const float sin_alpha = Sqrt(std::max(0.0f, 1.0f - cos_alpha * cos_alpha));
+    float sp, cp;
+    SinCosTurns(uy, &sp, &cp);
+    const Frame frame = MakeShadingFrame(Normalize(to_centre));
+    const F3 outward = frame.ToWorld(-F3{sin_alpha * cp, sin_alpha * sp, cos_alpha});
+    const F3 toward = (centre + radius * outward) - origin;
+    sample.distance = Length(toward);
+    sample.direction = toward / sample.distance;
+    sample.pdf = selection * SphereConePdf(sin2_max);
+  } else if (L.shape == kRectLight || L.shape == kDiskLight) {
+    float a, b;
+    if (L.shape == kRectLight) {
+      a = 2.0f * ux - 1.0f;
+      b = 2.0f * uy - 1.0f;
+    } else {
+      ConcentricDisk(ux, uy, a, b);
+    }
+    const F3 toward = ((Xyz(L.position) + a * Xyz(L.axis_u)) + b * Xyz(L.axis_v)) - origin;
+    const float distance2 = Dot(toward, toward);
+    if (!(distance2 > 0.0f)) return NoLight();
+    sample.distance = Sqrt(distance2);
+    sample.direction = toward / sample.distance;
+    const float cosine = -Dot(Xyz(L.normal), sample.direction);
+    if (!(cosine > 0.0f)) return NoLight();
+    sample.pdf = selection * distance2 / (cosine * L.normal[3]);
+  } else if (L.shape == kDistantLight) {
+    // SampleUniformCone (sampling.slang:109-116)
+    const float one_minus_cos = ux * L.axis_u[3];
+    const float sin_theta = Sqrt(std::max(0.0f, one_minus_cos * (2.0f - one_minus_cos)));
+    float sp, cp;
+    SinCosTurns(uy, &sp, &cp);
+    const Frame frame = MakeShadingFrame(Xyz(L.axis_u));
+    sample.direction = Normalize(frame.ToWorld(F3{sin_theta * cp, sin_theta * sp, 1.0f - one_minus_cos}));
+    sample.distance = kFloatMax;
+    sample.pdf = selection / ((2.0f * kPi) * L.axis_u[3]);
+  } else {
+    sample.direction = Xyz(L.axis_u);
+    sample.distance = kFloatMax;
+    sample.pdf = selection;
+    sample.delta = true;
+  }
+  return sample;
+}
+
 // scene.slang:894-909 (SampleLight): the alias table's column and threshold
-// choose the light, then SampleTriangle.
+// choose the light, an emissive triangle or an analytic light.
 LightSample SampleLight(const Scene& s, float ux, float uy, float uz, float uw,
                         F3 origin) {
   const uint32_t count = static_cast<uint32_t>(s.lights.size());
   const uint32_t column = std::min(static_cast<uint32_t>(ux * static_cast<float>(count)), count - 1);
   const LightRecord& record = s.lights[column];
   const uint32_t light = uy < record.threshold ? column : record.alias;
+  const LightRecord& chosen = s.lights[light];
+  if ((chosen.instance & kAnalyticLight) != 0)
+    return SampleAnalytic(s.analytic[chosen.instance & ~kAnalyticLight], chosen.pdf, uz, uw, origin);
   return SampleTriangleLight(s, light, uz, uw, origin);
+}
+
+// scene.slang:939-971 (ReconstructReservoirLight).
+LightSample ReconstructReservoirLight(const Scene& s, uint32_t index, float u, float v, F3 origin) {
+  const LightRecord& chosen = s.lights[index];
+  if ((chosen.instance & kAnalyticLight) == 0) return SampleTriangleLight(s, index, u, v, origin);
+  const Analytic& L = s.analytic[chosen.instance & ~kAnalyticLight];
+  if (L.shape != kSphereLight) return SampleAnalytic(L, chosen.pdf, u, v, origin);
+  const float z = 1.0f - 2.0f * u;
+  const float radius_xy = Sqrt(std::max(0.0f, 1.0f - z * z));
+  float sp, cp;
+  SinCosTurns(v, &sp, &cp);
+  const F3 normal{radius_xy * cp, radius_xy * sp, z};
+  const F3 toward = (Xyz(L.position) + L.position[3] * normal) - origin;
+  const float distance2 = Dot(toward, toward);
+  if (!(distance2 > 0.0f)) return NoLight();
+  LightSample sample = NoLight();
+  sample.distance = Sqrt(distance2);
+  sample.direction = toward / sample.distance;
+  const float cosine = -Dot(normal, sample.direction);
+  if (!(cosine > 0.0f)) return NoLight();
+  sample.radiance = Xyz(L.radiance);
+  sample.pdf = chosen.pdf * distance2 / ((((cosine * 4.0f) * kPi) * L.position[3]) * L.position[3]);
+  return sample;
+}
+
+// scene.slang:973-986 (ReservoirPartitionPdf).
+float ReservoirPartitionPdf(const Scene& s, uint32_t index, const LightSample& sample, F3 origin) {
+  const LightRecord& chosen = s.lights[index];
+  if ((chosen.instance & kAnalyticLight) != 0) {
+    const Analytic& L = s.analytic[chosen.instance & ~kAnalyticLight];
+    if (L.shape == kSphereLight) {
+      const F3 offset = Xyz(L.position) - origin;
+      return chosen.pdf * SphereConePdf(L.position[3] * L.position[3] / Dot(offset, offset));
+    }
+  }
+  return sample.pdf;
+}
+
+// scene.slang:1019-1093 (CrossedLight).
+F3 CrossedLight(const Analytic& L, F3 origin, F3 direction, float distance, bool escaped, bool weigh, float pdf) {
+  float light_pdf = 0.0f;
+  if (L.shape == kSphereLight) {
+    const float radius = L.position[3];
+    const F3 offset = origin - Xyz(L.position);
+    const float c = Dot(offset, offset) - radius * radius;
+    const float b = -Dot(offset, direction);
+    const F3 perpendicular = offset + b * direction;
+    const float discriminant = radius * radius - Dot(perpendicular, perpendicular);
+    if (!(c > 0.0f) || !(b > 0.0f) || discriminant < 0.0f) return F3{0, 0, 0};
+    const float t = c / (b + Sqrt(discriminant));
+    if (!(t < distance)) return F3{0, 0, 0};
+    light_pdf = SphereConePdf(radius * radius / Dot(offset, offset));
+  } else if (L.shape == kRectLight || L.shape == kDiskLight) {
+    const float facing = Dot(direction, Xyz(L.normal));
+    if (!(facing < 0.0f)) return F3{0, 0, 0};
+    const float t = Dot(Xyz(L.position) - origin, Xyz(L.normal)) / facing;
+    if (!(t > 0.0f) || !(t < distance)) return F3{0, 0, 0};
+    const F3 q = (origin + t * direction) - Xyz(L.position);
+    const F3 u = Xyz(L.axis_u), v = Xyz(L.axis_v);
+    const float uu = Dot(u, u), uv = Dot(u, v), vv = Dot(v, v), qu = Dot(q, u), qv = Dot(q, v);
+    const float determinant = uu * vv - uv * uv;
+    const float a = (qu * vv - qv * uv) / determinant;
+    const float b = (qv * uu - qu * uv) / determinant;
+    if (L.shape == kRectLight ? !(std::fabs(a) <= 1.0f && std::fabs(b) <= 1.0f) : !(a * a + b * b <= 1.0f))
+      return F3{0, 0, 0};
+    light_pdf = t * t / (-facing * L.normal[3]);
+  } else if (L.shape == kDistantLight && escaped) {
+    const F3 toward = Xyz(L.axis_u);
+    const float one_minus_cos = L.axis_u[3];
+    const float cosine = Dot(direction, toward);
+    if (one_minus_cos <= 1.0f) {
+      const F3 across = Cross(direction, toward);
+      if (!(cosine > 0.0f) || !(Dot(across, across) <= one_minus_cos * (2.0f - one_minus_cos))) return F3{0, 0, 0};
+    } else if (!(cosine >= 1.0f - one_minus_cos)) {
+      return F3{0, 0, 0};
+    }
+    light_pdf = 1.0f / ((2.0f * kPi) * one_minus_cos);
+  } else {
+    return F3{0, 0, 0};
+  }
+  const float weight = weigh ? PowerHeuristic(pdf, L.radiance[3] * light_pdf) : 1.0f;
+  return Xyz(L.radiance) * weight;
+}
+
+// scene.slang:1095-1108 (OverlapsBox).
+bool OverlapsBox(const float* low, const float* high, F3 origin, F3 inverse, float distance) {
+  const float x0 = (low[0] - origin.x) * inverse.x, x1 = (high[0] - origin.x) * inverse.x;
+  const float y0 = (low[1] - origin.y) * inverse.y, y1 = (high[1] - origin.y) * inverse.y;
+  const float z0 = (low[2] - origin.z) * inverse.z, z1 = (high[2] - origin.z) * inverse.z;
+  const float nx = std::min(x0, x1), ny = std::min(y0, y1), nz = std::min(z0, z1);
+  const float fx = std::max(x0, x1), fy = std::max(y0, y1), fz = std::max(z0, z1);
+  return std::max(std::max(nx, ny), std::max(nz, 0.0f)) <= std::min(std::min(fx, fy), std::min(fz, distance));
+}
+
+// scene.slang:1110-1182 (CrossedLights): hierarchy leaves in traversal order,
+// then for an escaping ray the distant lights.
+F3 CrossedLights(const Scene& s, F3 origin, F3 direction, float distance, bool escaped, bool weigh, float pdf) {
+  F3 sum{0, 0, 0};
+  if (!s.light_nodes.empty()) {
+    auto safe = [](float x) { return std::fabs(x) > 1e-30f ? x : (x < 0.0f ? -1e-30f : 1e-30f); };
+    const F3 inverse{1.0f / safe(direction.x), 1.0f / safe(direction.y), 1.0f / safe(direction.z)};
+    std::vector<uint32_t> stack;
+    uint32_t node = 0;
+    for (;;) {
+      const LightNode& current = s.light_nodes[node];
+      uint32_t child0, child1;
+      std::memcpy(&child0, &current.low0[3], 4);
+      std::memcpy(&child1, &current.low1[3], 4);
+      const bool enter0 = OverlapsBox(current.low0, current.high0, origin, inverse, distance);
+      const bool enter1 = child1 != kNoLightChild && OverlapsBox(current.low1, current.high1, origin, inverse, distance);
+      uint32_t next = 0xFFFFFFFFu;
+      if (enter0) {
+        if ((child0 & kLightLeaf) != 0)
+          sum = sum + CrossedLight(s.analytic[child0 & ~kLightLeaf], origin, direction, distance, escaped, weigh, pdf);
+        else
+          next = child0;
+      }
+      if (enter1) {
+        if ((child1 & kLightLeaf) != 0)
+          sum = sum + CrossedLight(s.analytic[child1 & ~kLightLeaf], origin, direction, distance, escaped, weigh, pdf);
+        else if (next == 0xFFFFFFFFu)
+          next = child1;
+        else
+          stack.push_back(child1);
+      }
+      if (next == 0xFFFFFFFFu) {
+        if (stack.empty()) break;
+        next = stack.back();
+        stack.pop_back();
+      }
+      node = next;
+    }
+  }
+  if (escaped)
+    for (uint32_t index : s.distant)
+      sum = sum + CrossedLight(s.analytic[index], origin, direction, distance, escaped, weigh, pdf);
+  return sum;
 }
 // scene.slang:1004-1010.
 float EmitterPdf(const Scene& s, const Hit& hit, float area) {
@@ -1109,14 +1364,9 @@ struct Counters {
   uint64_t shadow;
 };
 
-// transport.slang:82-115, 123-135.
-F3 DirectLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo,
-               uint32_t& rng, Counters& k, bool brute) {
-  if (s.lights.empty()) return F3{0, 0, 0};
-  const float ux = NextRandom(rng), uy = NextRandom(rng), uz = NextRandom(rng),
-              uw = NextRandom(rng);
-  const F3 origin = OffsetRayOrigin(sf.position, sf.normal);
-  const LightSample ls = SampleLight(s, ux, uy, uz, uw, origin);
+// transport.slang:82-115 (ShadowedLight).
+F3 ShadowedLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo, F3 origin, const LightSample& ls,
+                 Counters& k, bool brute) {
   if (!(ls.pdf > 0.0f) || MaxComponent(ls.radiance) <= 0.0f ||
       Dot(ls.direction, sf.normal) <= 0.0f)
     return F3{0, 0, 0};
@@ -1128,9 +1378,32 @@ F3 DirectLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo,
   ++k.shadow;
   const float transmittance = Occluded(s, shadow, brute) ? 0.0f : 1.0f;
   if (transmittance <= 0.0f) return F3{0, 0, 0};
-  const float weight = PowerHeuristic(ls.pdf, BsdfPdf(sf.bsdf, wo, wi));
+  const float weight = ls.delta ? 1.0f : PowerHeuristic(ls.pdf, BsdfPdf(sf.bsdf, wo, wi));
   const F3 c = f * ls.radiance * (transmittance * weight / ls.pdf);
   return IsFinite(c) ? c : F3{0, 0, 0};
+}
+
+// transport.slang:139-146 (ParallelLights): under BSDF sampling, each
+// parallel light through its own shadow ray, drawing no random numbers.
+F3 ParallelLights(const Scene& s, const Surface& sf, const Frame& basis, F3 wo, Counters& k, bool brute) {
+  const F3 origin = OffsetRayOrigin(sf.position, sf.normal);
+  F3 sum{0, 0, 0};
+  for (const Analytic& L : s.analytic) {
+    if (L.shape != kParallelLight) continue;  // ParallelLight returns NoLight for the others (scene.slang:988-1001)
+    const LightSample ls{Xyz(L.axis_u), kFloatMax, Xyz(L.radiance), 1.0f, true};
+    sum = sum + ShadowedLight(s, sf, basis, wo, origin, ls, k, brute);
+  }
+  return sum;
+}
+
+// transport.slang:123-135 (DirectLight).
+F3 DirectLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo,
+               uint32_t& rng, Counters& k, bool brute) {
+  if (s.lights.empty()) return F3{0, 0, 0};
+  const float ux = NextRandom(rng), uy = NextRandom(rng), uz = NextRandom(rng),
+              uw = NextRandom(rng);
+  const F3 origin = OffsetRayOrigin(sf.position, sf.normal);
+  return ShadowedLight(s, sf, basis, wo, origin, SampleLight(s, ux, uy, uz, uw, origin), k, brute);
 }
 
 // restir.slang:163-233 (WriteInitialReservoir) at a camera path's first hit:
@@ -1153,7 +1426,7 @@ RestirPixel InitialReservoir(const Scene& s, const oracle_frame& fr, uint32_t sl
       const float random = NextRandom(rng);
       const uint32_t light = ChooseReservoirLight(s, fr.restir_proposal, cx, cy);
       const float proposal = ReservoirProposalPdf(s, fr.restir_proposal, light);
-      const LightSample ls = SampleTriangleLight(s, light, u, v, origin);
+      const LightSample ls = ReconstructReservoirLight(s, light, u, v, origin);
       const F3 f = CandidateContribution(sf, basis, wo, ls, s.lights[light].pdf);
       bool selected = false;
       if (!std::isfinite(ls.pdf) || !IsFinite(f) ||
@@ -1169,7 +1442,7 @@ RestirPixel InitialReservoir(const Scene& s, const oracle_frame& fr, uint32_t sl
     }
   }
   if ((px.flags & 2) == 0 && px.r.weight_sum > 0.0f) {
-    const LightSample ls = SampleTriangleLight(s, px.r.sample, px.u, px.v, origin);
+    const LightSample ls = ReconstructReservoirLight(s, px.r.sample, px.u, px.v, origin);
     ++k.shadow;
     const float visibility = Occluded(s, Ray{origin, 0.0f, ls.direction, ls.distance}, brute) ? 0.0f : 1.0f;
     const float normalization = ReservoirNormalization(px.r);
@@ -1191,8 +1464,10 @@ RestirPixel InitialReservoir(const Scene& s, const oracle_frame& fr, uint32_t sl
 F3 InitialDirectLight(const Scene& s, const RestirPixel& px, const Surface& sf, const Frame& basis, F3 wo) {
   if ((px.flags & 2) != 0 || !(px.r.weight_sum > 0.0f) || px.r.sample >= s.lights.size()) return F3{0, 0, 0};
   const F3 origin = OffsetRayOrigin(sf.position, sf.normal);
-  const LightSample ls = SampleTriangleLight(s, px.r.sample, px.u, px.v, origin);
-  const float weight = PowerHeuristic(ls.pdf, RestirPdf(sf.bsdf, wo, basis.ToLocal(ls.direction)));
+  const LightSample ls = ReconstructReservoirLight(s, px.r.sample, px.u, px.v, origin);
+  const float weight = ls.delta ? 1.0f
+                                : PowerHeuristic(ReservoirPartitionPdf(s, px.r.sample, ls, origin),
+                                                 RestirPdf(sf.bsdf, wo, basis.ToLocal(ls.direction)));
   return px.contribution * weight;
 }
 
@@ -1214,6 +1489,12 @@ bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
     emission = emission * w;
   }
   path.radiance = path.radiance + path.throughput * emission;
+  // the analytic lights the continuation ray crossed before the hit; camera
+  // rays do not see them (transport.slang:171-178)
+  if (path.depth != 0 && !s.analytic.empty())
+    path.radiance = path.radiance + path.throughput * CrossedLights(s, path.ray.o, path.ray.d,
+                                                                    Dot(sf.position - path.ray.o, path.ray.d), false,
+                                                                    sample_lights && !path.unweighted, path.last_pdf);
   if (path.depth >= fr.max_bounces) return false;
   if (path.depth >= 3) {
     const float survival = std::min(MaxComponent(path.throughput), 0.95f);
@@ -1226,6 +1507,8 @@ bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
     path.radiance = path.radiance + path.throughput * InitialDirectLight(s, *restir, sf, basis, wo);
   else if (sample_lights)
     path.radiance = path.radiance + path.throughput * DirectLight(s, sf, basis, wo, path.rng, k, brute);
+  else if (!s.analytic.empty())
+    path.radiance = path.radiance + path.throughput * ParallelLights(s, sf, basis, wo, k, brute);
   const float ux = NextRandom(path.rng), uy = NextRandom(path.rng), uz = NextRandom(path.rng);
   SampleResult smp;
   if (!BsdfSample(sf.bsdf, wo, ux, uy, uz, smp)) return false;
@@ -1243,9 +1526,14 @@ bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
   return true;
 }
 
-// transport.slang:235-247 (constant environment).
-void MissPath(const Scene& s, Path& path) {
+// transport.slang:235-247 (MissPath): the constant environment, then the
+// analytic lights the escaping ray crossed.
+void MissPath(const Scene& s, const oracle_frame& fr, Path& path) {
   path.radiance = path.radiance + path.throughput * s.environment;
+  if (!s.analytic.empty())
+    path.radiance = path.radiance + path.throughput * CrossedLights(s, path.ray.o, path.ray.d, kFloatMax, true,
+                                                                    fr.light_sampling != 0 && !path.unweighted,
+                                                                    path.last_pdf);
 }
 
 // transport.slang:53-69.
@@ -1281,7 +1569,7 @@ bool RunPath(const Scene& s, const oracle_frame& fr, uint32_t x, uint32_t y,
   while (ScatterPath(s, fr, path, hit, k, brute, &restir)) {
     ++k.counts[path.depth];
     if (!TraceClosest(s, path.ray, hit, brute)) {
-      MissPath(s, path);
+      MissPath(s, fr, path);
       break;
     }
   }
@@ -1312,6 +1600,172 @@ void ParallelFor(uint32_t count, int threads, F&& body) {
 
 inline Ray LoadRay(const float* r) {
   return Ray{F3{r[0], r[1], r[2]}, r[3], F3{r[4], r[5], r[6]}, r[7]};
+}
+
+// gpu_scene.cpp:233-270 (SceneRadius) in double.
+double SceneRadius(const oracle_scene_desc* d) {
+  const double inf = std::numeric_limits<double>::infinity();
+  double world[6] = {inf, inf, inf, -inf, -inf, -inf};
+  for (uint32_t i = 0; i < d->instance_count; ++i) {
+    const uint32_t g = d->instance_geometry[i];
+    double b[6] = {inf, inf, inf, -inf, -inf, -inf};
+    const float* pos = d->positions + 3 * size_t(d->geometry_vertex_offset[g]);
+    for (uint32_t v = 0; v < d->geometry_vertex_count[g]; ++v)
+      for (int a = 0; a < 3; ++a) {
+        b[a] = std::min(b[a], double(pos[3 * v + a]));
+        b[3 + a] = std::max(b[3 + a], double(pos[3 * v + a]));
+      }
+    if (!(b[0] <= b[3])) continue;
+    const float* m = d->instance_transform + 16 * size_t(i);
+    for (int corner = 0; corner < 8; ++corner) {
+      const double p[3] = {b[(corner & 1) ? 3 : 0], b[(corner & 2) ? 4 : 1], b[(corner & 4) ? 5 : 2]};
+      for (int r = 0; r < 3; ++r) {
+        const double v = double(m[r]) * p[0] + double(m[4 + r]) * p[1] + double(m[8 + r]) * p[2] + double(m[12 + r]);
+        world[r] = std::min(world[r], v);
+        world[3 + r] = std::max(world[3 + r], v);
+      }
+    }
+  }
+  if (!(world[0] <= world[3])) return 0.0;
+  const double x = world[3] - world[0], y = world[4] - world[1], z = world[5] - world[2];
+  const double radius = 0.5 * std::sqrt(x * x + y * y + z * z);
+  return std::isfinite(radius) ? radius : 0.0;
+}
+
+void Store(float* out, double x, double y, double z, double w) {
+  out[0] = static_cast<float>(x);
+  out[1] = static_cast<float>(y);
+  out[2] = static_cast<float>(z);
+  out[3] = static_cast<float>(w);
+}
+
+// gpu_scene.cpp:271-369 (MakeAnalyticLight).
+bool MakeAnalyticLight(const oracle_light& light, double scene_radius, Analytic& rec, double& weight) {
+  constexpr double kPiD = 3.14159265358979323846;
+  const float* m = light.world_from_light;
+  const double x[3] = {m[0], m[1], m[2]}, y[3] = {m[4], m[5], m[6]}, z[3] = {m[8], m[9], m[10]};
+  const double det = x[0] * (y[1] * z[2] - y[2] * z[1]) + x[1] * (y[2] * z[0] - y[0] * z[2]) +
+                     x[2] * (y[0] * z[1] - y[1] * z[0]);
+  rec = Analytic{};
+  double extent = 0.0, scale = 1.0;
+  switch (light.shape) {
+    case 0: {
+      const double radius = light.radius * std::cbrt(std::fabs(det));
+      extent = 4.0 * kPiD * radius * radius;
+      rec.shape = kSphereLight;
+      Store(rec.position, m[12], m[13], m[14], radius);
+      Store(rec.normal, 0, 0, 0, extent);
+      break;
+    }
+    case 1:
+    case 2: {
+      const bool rect = light.shape == 1;
+      const double su = rect ? 0.5 * light.width : light.radius, sv = rect ? 0.5 * light.height : light.radius;
+      const double u[3] = {x[0] * su, x[1] * su, x[2] * su}, v[3] = {y[0] * sv, y[1] * sv, y[2] * sv};
+      const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+      const double length = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      extent = (rect ? 4.0 : kPiD) * length;
+      rec.shape = rect ? kRectLight : kDiskLight;
+      Store(rec.position, m[12], m[13], m[14], 0.0);
+      Store(rec.axis_u, u[0], u[1], u[2], 0.0);
+      Store(rec.axis_v, v[0], v[1], v[2], 0.0);
+      if (length > 0.0) {
+        const double k = (det < 0.0 ? 1.0 : -1.0) / length;
+        Store(rec.normal, n[0] * k, n[1] * k, n[2] * k, extent);
+      }
+      break;
+    }
+    default: {
+      const double length = std::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+      if (!(length > 0.0)) return false;
+      const double theta = std::min(static_cast<double>(light.angle) * kPiD / 360.0, kPiD);
+      const double half = std::sin(0.5 * theta);
+      const double one_minus_cos = 2.0 * half * half;
+      const double sin2 = std::sin(theta) * std::sin(theta);
+      Store(rec.axis_u, z[0] / length, z[1] / length, z[2] / length, one_minus_cos);
+      rec.shape = kParallelLight;
+      extent = 1.0;
+      if (theta > 0.0) {
+        extent = theta <= 0.5 * kPiD ? kPiD * sin2 : kPiD;
+        if (light.normalize) scale = 1.0 / (theta <= 0.5 * kPiD ? kPiD * sin2 : (2.0 - sin2) * kPiD);
+        if (static_cast<float>(one_minus_cos) >= std::numeric_limits<float>::min()) {
+          rec.shape = kDistantLight;
+        } else {
+          scale *= extent;
+          extent = 1.0;
+        }
+      }
+    }
+  }
+  if (light.normalize && light.shape != 3) scale = 1.0 / extent;
+  const double r[3] = {light.radiance[0] * scale, light.radiance[1] * scale, light.radiance[2] * scale};
+  const double mean = (r[0] + r[1] + r[2]) / 3.0;
+  const double fmax = std::numeric_limits<float>::max();
+  if (!(extent > 0.0) || !std::isfinite(extent) || !(mean > 0.0) || !(r[0] <= fmax) || !(r[1] <= fmax) ||
+      !(r[2] <= fmax))
+    return false;
+  Store(rec.radiance, r[0], r[1], r[2], 0.0);
+  weight = mean * extent / 2.0;
+  if (light.shape == 3) weight *= scene_radius * scene_radius;
+  if (!std::isfinite(weight)) weight = std::numeric_limits<double>::max();
+  return true;
+}
+
+// gpu_scene.cpp:371-471 (BoundLight, BuildLightNode, BuildLightHierarchy).
+struct Bounds {
+  float low[3], high[3];
+  double centre[3];
+  uint32_t index;
+};
+Bounds BoundLight(const Analytic& l, uint32_t index) {
+  Bounds b;
+  b.index = index;
+  for (int a = 0; a < 3; ++a) {
+    const double centre = l.position[a], u = l.axis_u[a], v = l.axis_v[a];
+    const double extent = l.shape == kSphereLight ? double(l.position[3])
+                          : l.shape == kRectLight ? std::fabs(u) + std::fabs(v)
+                                                  : std::sqrt(u * u + v * v);
+    const double margin = 1e-4 * extent + 1e-5 * (std::fabs(centre) + extent);
+    b.centre[a] = centre;
+    b.low[a] = std::nextafter(static_cast<float>(centre - extent - margin), -std::numeric_limits<float>::infinity());
+    b.high[a] = std::nextafter(static_cast<float>(centre + extent + margin), std::numeric_limits<float>::infinity());
+  }
+  return b;
+}
+void StoreBounds(float* low, float* high, const Bounds* first, const Bounds* last, uint32_t child) {
+  for (int a = 0; a < 3; ++a) {
+    low[a] = std::numeric_limits<float>::max();
+    high[a] = -std::numeric_limits<float>::max();
+    for (const Bounds* b = first; b != last; ++b) {
+      low[a] = std::min(low[a], b->low[a]);
+      high[a] = std::max(high[a], b->high[a]);
+    }
+  }
+  std::memcpy(&low[3], &child, 4);
+}
+uint32_t BuildLightNode(Bounds* first, Bounds* last, std::vector<LightNode>& nodes) {
+  if (last - first == 1) return kLightLeaf | first->index;
+  const uint32_t node = static_cast<uint32_t>(nodes.size());
+  nodes.emplace_back();
+  double lo[3] = {first->centre[0], first->centre[1], first->centre[2]};
+  double hi[3] = {lo[0], lo[1], lo[2]};
+  for (const Bounds* b = first; b != last; ++b)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], b->centre[a]);
+      hi[a] = std::max(hi[a], b->centre[a]);
+    }
+  int axis = 0;
+  for (int a = 1; a < 3; ++a)
+    if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
+  Bounds* middle = first + (last - first) / 2;
+  std::nth_element(first, middle, last, [axis](const Bounds& a, const Bounds& b) {
+    return a.centre[axis] != b.centre[axis] ? a.centre[axis] < b.centre[axis] : a.index < b.index;
+  });
+  const uint32_t left = BuildLightNode(first, middle, nodes);
+  const uint32_t right = BuildLightNode(middle, last, nodes);
+  StoreBounds(nodes[node].low0, nodes[node].high0, first, middle, left);
+  StoreBounds(nodes[node].low1, nodes[node].high1, middle, last, right);
+  return node;
 }
 
 }  // namespace
@@ -1380,6 +1834,37 @@ oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
       s.lights.push_back(LightRecord{0.0f, 1.0f, 0u, i});
     }
   }
+  bool distant = false;
+  for (uint32_t l = 0; l < d->light_count; ++l) distant |= d->lights[l].shape == 3;
+  const double scene_radius = distant ? SceneRadius(d) : 0.0;
+  std::vector<uint32_t> analytic_entries;
+  for (uint32_t l = 0; l < d->light_count; ++l) {
+    if (d->lights[l].shape > 3) return fail("a light has an unknown shape");
+    Analytic rec;
+    double weight = 0.0;
+    if (!MakeAnalyticLight(d->lights[l], scene_radius, rec, weight)) continue;
+    if (weight > 0.0) {
+      analytic_entries.push_back(static_cast<uint32_t>(s.lights.size()));
+      weights.push_back(weight);
+      s.lights.push_back(LightRecord{0.0f, 1.0f, 0u, kAnalyticLight | static_cast<uint32_t>(s.analytic.size())});
+    }
+    s.analytic.push_back(rec);
+  }
+  std::vector<Bounds> bounded;
+  for (uint32_t a = 0; a < s.analytic.size(); ++a) {
+    if (s.analytic[a].shape == kDistantLight)
+      s.distant.push_back(a);
+    else if (s.analytic[a].shape != kParallelLight)
+      bounded.push_back(BoundLight(s.analytic[a], a));
+  }
+  if (bounded.size() == 1) {
+    LightNode root{};
+    StoreBounds(root.low0, root.high0, bounded.data(), bounded.data() + 1, kLightLeaf | bounded[0].index);
+    StoreBounds(root.low1, root.high1, nullptr, nullptr, kNoLightChild);
+    s.light_nodes.push_back(root);
+  } else if (bounded.size() > 1) {
+    BuildLightNode(bounded.data(), bounded.data() + bounded.size(), s.light_nodes);
+  }
   double total = 0.0;
   for (double w : weights) total += w;
   if (!(total > 0.0) || !std::isfinite(total)) {
@@ -1414,6 +1899,7 @@ oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
       s.lights[i].threshold = 1.0f;
       s.lights[i].alias = i;
     }
+    for (uint32_t e : analytic_entries) s.analytic[s.lights[e].instance & ~kAnalyticLight].radiance[3] = s.lights[e].pdf;
   }
   BuildBvh(s);
   return scene.release();

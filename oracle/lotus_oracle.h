@@ -21,6 +21,16 @@ extern "C" {
  * instances, with one material record per slot. The material record is the
  * reference's GpuMaterialRecord (backend/vulkan/gpu_scene.hpp:94-113),
  * 36 words. */
+/* An analytic light (Light, include/lotus/render_world.hpp:89-114): shape 0
+ * sphere, 1 rect, 2 disk, 3 distant (angle 0: parallel). */
+typedef struct oracle_light {
+  uint32_t shape;
+  uint32_t normalize;
+  float world_from_light[16]; /* column-major */
+  float radiance[3];
+  float radius, width, height, angle;
+} oracle_light;
+
 typedef struct oracle_scene_desc {
   uint32_t geometry_count;
   const float* positions;           /* xyz per vertex, all geometries */
@@ -35,6 +45,9 @@ typedef struct oracle_scene_desc {
   uint32_t material_count;
   const uint32_t* materials;         /* 36 words per material */
   float environment[3];
+  const uint32_t* geometry_vertex_count;    /* per geometry */
+  uint32_t light_count;
+  const oracle_light* lights;
 } oracle_scene_desc;
 
 typedef struct oracle_scene oracle_scene;
