@@ -213,6 +213,38 @@ def scene_update_run(r, scene, width, height, depth, steps, frames=4):
             "frame_gpu_ms_built_tree": round(built_frame, 3)}
 
 
+def lights_run(r, scene, width, height, depth, frames):
+    """The heightfield lit by analytic lights as well as its emissive
+    triangles: 16 seeded sphere lights above it and a sun (distant light of
+    0.53 degrees), NEE+MIS. Every light is in the light table and every
+    continuation ray tests the crossing hierarchy and, escaping, the sun."""
+    lit = scene.copy()
+    rng = np.random.default_rng(11)
+    for k in range(16):
+        x, z = rng.uniform(-0.9, 0.9, 2)
+        lit.add_light(scenes.analytic_light(scenes.LIGHT_SPHERE, scenes.translation(x, rng.uniform(0.3, 0.6), z),
+                                            radiance=tuple(rng.uniform(2.0, 8.0, 3)), radius=rng.uniform(0.01, 0.04)))
+    sun = scenes.identity()
+    sun[8:11] = (0.3, 1.0, 0.4)  # the frame's Z: towards the sun
+    lit.add_light(scenes.analytic_light(scenes.LIGHT_DISTANT, sun, radiance=(3.0, 2.8, 2.5), angle=0.53,
+                                        normalize=True))
+    r.apply(lit)
+    kw = dict(frame_count=1, max_bounces=depth, light_sampling=hip.LIGHT_MIS, want_color=False)
+    r.render(lit.world_to_clip, width, height, **kw)
+    tot = {"rays": 0, "frame_ms": 0.0, "shade_ms": 0.0, "trace_ms": 0.0, "tail_ms": 0.0, "camera_ms": 0.0}
+    for _ in range(frames):
+        out = r.render(lit.world_to_clip, width, height, **kw)
+        tot["rays"] += out["camera_rays"] + out["extension_rays"] + out["shadow_rays"]
+        for k in ("frame_ms", "shade_ms", "trace_ms", "tail_ms", "camera_ms"):
+            tot[k] += out[k]
+    r.apply(scene)  # back to the triangle lights alone (a table rewrite)
+    return {"workload": f"heightfield {scene.triangle_count} tris + 16 sphere lights + a 0.53-degree sun, "
+                        f"{width}x{height}, depth {depth}, NEE+MIS",
+            "frames": frames,
+            "mrays_per_s_gpu": round(tot["rays"] / tot["frame_ms"] / 1e3, 1),
+            "gpu_ms_per_frame": {k.replace("_ms", ""): round(v / frames, 3) for k, v in tot.items() if k != "rays"}}
+
+
 def batch_run(views, frames):
     """64 seeded views of 64^2..512^2 over their own icosphere rigs (1k..200k
     triangles, depth 2..12): each view autotuned, then frames rendered view
@@ -278,6 +310,8 @@ def main() -> int:
                     help="temporal frames of the ReSTIR + SVGF run (0 skips it)")
     ap.add_argument("--update-steps", type=int, default=8,
                     help="animated heightfield steps applied as BVH refits (0 skips them)")
+    ap.add_argument("--lights-frames", type=int, default=16,
+                    help="frames of the heightfield with analytic lights (0 skips them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times: the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -402,6 +436,9 @@ def main() -> int:
     if args.update_steps > 0:
         log("scene updates (refits) ...")
         line["scene_update"] = scene_update_run(r, scene, args.width, args.height, args.depth, args.update_steps)
+    if args.lights_frames > 0:
+        log("analytic lights ...")
+        line["analytic_lights"] = lights_run(r, scene, args.width, args.height, args.depth, args.lights_frames)
     if args.race_frames > 0:
         cache_cfg = dict(log2_table=19, train_paths=65536, batch_capacity=65536, train_steps=4, learning_rate=1e-2)
         log("equal-relMSE race: heightfield ...")
