@@ -80,6 +80,31 @@ __device__ __forceinline__ void sincos_turns(float u, float* s, float* c) {
   *c = k == 0 ? cs : (k == 1 ? -sn : (k == 2 ? -cs : sn));
 }
 
+// atan2(y, x) / pi in [-1, 1]: reduction to |t| <= tan(pi/8) (swap of the
+// arguments, then atan(r) = pi/4 + atan((r - 1) / (r + 1))) and the Taylor
+// series of atan to t^15, error below 2e-8 rad. It replaces the reference
+// GPU's atan2 and acos (scene.slang:805-809) so the oracle matches bit for bit.
+__device__ __forceinline__ float atan2_pi(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  if (ax == 0.0f && ay == 0.0f) return 0.0f;
+  const bool swap = ay > ax;
+  const float r = swap ? ax / ay : ay / ax;
+  const bool shift = r > 0.41421356237309505f;
+  const float t = shift ? (r - 1.0f) / (r + 1.0f) : r;
+  const float t2 = t * t;
+  const float p = t + t * t2 * (-3.33333333333333333e-1f + t2 * (2.0e-1f + t2 * (-1.42857142857142857e-1f +
+      t2 * (1.11111111111111111e-1f + t2 * (-9.09090909090909091e-2f + t2 * (7.69230769230769231e-2f +
+      t2 * -6.66666666666666667e-2f))))));
+  float a = (shift ? 0.785398163397448310f + p : p) * 0.318309886183790672f;
+  if (swap) a = 0.5f - a;
+  if (x < 0.0f) a = 1.0f - a;
+  return y < 0.0f ? -a : a;
+}
+// acos(y) / pi for y in [-1, 1].
+__device__ __forceinline__ float acos_pi(float y) {
+  return atan2_pi(sqrtf(fmaxf(0.0f, (1.0f - y) * (1.0f + y))), y);
+}
+
 // PCG RXS-M-XS hash and generator (sampling.slang:9-25).
 __device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
   const uint32_t state = v * 747796405u + 2891336453u;

@@ -75,7 +75,26 @@ struct SceneView {
   uint32_t analytic_count, light_node_count, distant_count;
   uint32_t parallel;                // the parallel light's index + 1 (BSDF sampling traces it), 0 without one
   float environment[3];
+  // The environment image (env_width 0: none): texels, then its sampling
+  // distribution's floats (cell densities, the rows' cumulative
+  // distribution, each row's), the frame's axes as rows, the scale, and the
+  // probability its light-table entry has.
+  const float4* env_texels;
+  const float* env_dist;
+  uint32_t env_width, env_height, env_columns, env_rows;
+  float env_frame[9];
+  float env_scale[3];
+  float env_selection;
 };
+
+// The light-table entry of the environment image (gpu_scene.hpp:132-134).
+constexpr uint32_t kEnvironmentLight = 0xFFFFFFFFu;
+
+// Whether the kernels' variants with analytic lights and the environment
+// image run.
+__host__ __device__ inline bool scene_has_lights(const SceneView& S) {
+  return S.analytic_count != 0 || S.env_width != 0;
+}
 
 // Device memory of the resident scene, kept until a plan needs a new BVH.
 struct SceneBuffers {
@@ -92,6 +111,12 @@ struct SceneBuffers {
   LightNode* light_nodes = nullptr;
   uint32_t* distant = nullptr;
   uint32_t analytic_count = 0, light_node_count = 0, distant_count = 0, parallel_count = 0, parallel = 0;
+  float4* env_texels = nullptr;
+  float* env_dist = nullptr;
+  uint32_t env_width = 0, env_height = 0, env_columns = 0, env_rows = 0;
+  float env_frame[9] = {};
+  float env_scale[3] = {};
+  float env_selection = 0.0f;
   // The plan's geometry as uploaded: refits re-place its triangles and
   // read-back copies it.
   float* positions = nullptr;

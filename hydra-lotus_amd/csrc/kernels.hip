@@ -235,6 +235,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
 // streams them by the maximum component of their unshadowed contribution,
 // and traces one visibility ray for the selection (counted apart from the
 // queued shadow rays).
+template <bool kLights>  // the scene has analytic lights or an environment image
 __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, FrameConst F, Queues q,
                                                                 uint32_t* counters, uint2* spill, int spill_cap,
                                                                 unsigned long long* stats) {
@@ -271,7 +272,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
         const float random = next_random(rng);
         const uint32_t light = choose_reservoir_light(S, F.restir_proposal, cx, cy);
         const float proposal = reservoir_proposal_pdf(S, F.restir_proposal, light);
-        const LightSample ls = reconstruct_reservoir_light(S, light, u, v, origin);
+        const LightSample ls = reconstruct_reservoir_light<kLights>(S, light, u, v, origin);
         const V3 f = candidate_contribution(sf, ft, fb, wo, ls, S.lights[light].pdf);
         bool selected = false;
         if (!isfinite(ls.pdf) || !finite3(f) || !stream_candidate(r, light, max3(f), proposal, random, selected)) {
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
     const bool trace = (flags & 2u) == 0u && r.weight_sum > 0.0f;
     wave_count(restir_ray_count(counters), trace);
     if (trace) {
-      const LightSample ls = reconstruct_reservoir_light(S, r.sample, ru, rv, origin);
+      const LightSample ls = reconstruct_reservoir_light<kLights>(S, r.sample, ru, rv, origin);
       uint32_t visits = 0, tests = 0;
       const float visibility =
           walk_occluded(S.nodes, S.tris, st, Ray{origin, 0.0f, ls.direction, ls.distance}, visits, tests) ? 0.0f
@@ -403,13 +404,15 @@ __device__ __forceinline__ bool same_sample(const ReuseRecord& a, const ReuseRec
   return a.r.sample == b.r.sample && a.u == b.u && a.v == b.v;
 }
 
+template <bool kLights>
 __device__ __forceinline__ V3 reuse_contribution_uv(const SceneView& S, const ReuseSurface& r, uint32_t light, float u,
                                                     float v) {
-  return reuse_contribution(S, r, light, reconstruct_reservoir_light(S, light, u, v, r.origin));
+  return reuse_contribution(S, r, light, reconstruct_reservoir_light<kLights>(S, light, u, v, r.origin));
 }
 
 // restir.slang:278-287 (HistoryTarget): the selection's target at a source's
 // surface, stored when the source selected the same sample.
+template <bool kLights>
 __device__ __forceinline__ float history_target(const SceneView& S, const ReuseRecord& source,
                                                 const ReuseSurface& surface, const ReuseRecord& selected,
                                                 bool& valid) {
@@ -417,7 +420,7 @@ __device__ __forceinline__ float history_target(const SceneView& S, const ReuseR
     valid = isfinite(source.r.selected_target);
     return source.r.selected_target;
   }
-  const V3 c = reuse_contribution_uv(S, surface, selected.r.sample, selected.u, selected.v);
+  const V3 c = reuse_contribution_uv<kLights>(S, surface, selected.r.sample, selected.u, selected.v);
   valid = finite3(c);
   return max3(c);
 }
@@ -425,11 +428,12 @@ __device__ __forceinline__ float history_target(const SceneView& S, const ReuseR
 // The final estimate at the receiver (restir.slang:289-300, 372-385):
 // visibility reused from the same pixel's record when it selected the same
 // sample, else traced.
+template <bool kLights>
 __device__ __forceinline__ void resolve_estimate(const SceneView& S, TraceStack& st, const ReuseSurface& receiver,
                                                  ReuseRecord& result, const ReuseRecord& own, uint32_t& rays,
                                                  uint32_t& visits, uint32_t& tests) {
   if ((result.flags & 2u) != 0u || !(result.r.weight_sum > 0.0f)) return;
-  const LightSample ls = reconstruct_reservoir_light(S, result.r.sample, result.u, result.v, receiver.origin);
+  const LightSample ls = reconstruct_reservoir_light<kLights>(S, result.r.sample, result.u, result.v, receiver.origin);
   float visibility;
   if (same_sample(result, own)) {
     visibility = own.contribution.w;
@@ -448,6 +452,7 @@ __device__ __forceinline__ void resolve_estimate(const SceneView& S, TraceStack&
 }
 
 // restir.slang:302-387 (WriteTemporalReservoir) per pixel of the target.
+template <bool kLights>  // the scene has analytic lights or an environment image
 __global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, FrameConst F, RestirReuseArgs A,
                                                                  uint32_t* counters, uint2* spill, int spill_cap,
                                                                  unsigned long long* stats) {
@@ -482,7 +487,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, Fr
           V3 received = V3{0.0f, 0.0f, 0.0f};
           if (src.r.weight_sum > 0.0f) {
             if (src.r.sample >= S.light_count) result.flags |= 2u;
-            else received = reuse_contribution_uv(S, receiver, src.r.sample, src.u, src.v);
+            else received = reuse_contribution_uv<kLights>(S, receiver, src.r.sample, src.u, src.v);
           }
           uint32_t rng = seed_random(slot, F.sample_index ^ 0x94D049BBu);
           bool selected = false;
@@ -500,7 +505,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, Fr
             }
             if (result.r.weight_sum > 0.0f) {
               bool valid = true;
-              const float support = history_target(S, src, psurf, result, valid);
+              const float support = history_target<kLights>(S, src, psurf, result, valid);
               if (!valid) result.flags |= 2u;
               result.supporting = init.r.count + (support > 0.0f ? count : 0u);
             } else {
@@ -510,7 +515,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, Fr
           }
         }
       }
-      resolve_estimate(S, st, receiver, result, init, rays, visits, tests);
+      resolve_estimate<kLights>(S, st, receiver, result, init, rays, visits, tests);
     }
     wave_count(restir_ray_count(counters), rays != 0u);
     store_reuse(A.temporal_out + 4ull * slot, result);
@@ -529,6 +534,7 @@ __device__ __forceinline__ bool spatial_compatible(const ReuseSurface& receiver,
 
 // restir.slang:402-491 (WriteSpatialReservoir): the four axis neighbours at
 // the radius.
+template <bool kLights>  // the scene has analytic lights or an environment image
 __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, FrameConst F, RestirReuseArgs A,
                                                                 uint32_t* counters, uint2* spill, int spill_cap,
                                                                 unsigned long long* stats) {
@@ -569,7 +575,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
           if (src_index[i] == 0xFFFFFFFFu) continue;
           const ReuseRecord src = load_reuse(A.temporal_out + 4ull * src_index[i]);
           V3 received = V3{0.0f, 0.0f, 0.0f};
-          if (src.r.weight_sum > 0.0f) received = reuse_contribution_uv(S, receiver, src.r.sample, src.u, src.v);
+          if (src.r.weight_sum > 0.0f) received = reuse_contribution_uv<kLights>(S, receiver, src.r.sample, src.u, src.v);
           if (!finite3(received) || !merge_reevaluated(result.r, src.r, src.normalization, max3(received),
                                                        A.max_history, next_random(rng), selected)) {
             result.flags |= 2u;
@@ -591,13 +597,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
             const ReuseRecord src = load_reuse(A.temporal_out + 4ull * src_index[i]);
             bool valid = true;
             const float support =
-                history_target(S, src, load_reuse_surface(A.surf + 8ull * src_index[i]), result, valid);
+                history_target<kLights>(S, src, load_reuse_surface(A.surf + 8ull * src_index[i]), result, valid);
             if (!valid) result.flags |= 2u;
             if (support > 0.0f) result.supporting += src.r.count < A.max_history ? src.r.count : A.max_history;
           }
         }
         result.normalization = reuse_normalization(result.r, result.supporting);
-        resolve_estimate(S, st, receiver, result, centre, rays, visits, tests);
+        resolve_estimate<kLights>(S, st, receiver, result, centre, rays, visits, tests);
       }
     }
     wave_count(restir_ray_count(counters), rays != 0u);
@@ -1120,14 +1126,14 @@ void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, con
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce, bool link) {
-  if (S.analytic_count != 0)
+  if (scene_has_lights(S))
     k_shade<true><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
   else
     k_shade<false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
 }
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds) {
-  const bool analytic = S.analytic_count != 0;
+  const bool analytic = scene_has_lights(S);
   if (L.dynamic_trace && analytic)
     k_trace<true, true><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce,
                                                                            kinds, L.stats);
@@ -1143,7 +1149,7 @@ void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce) {
-  const bool analytic = S.analytic_count != 0;
+  const bool analytic = scene_has_lights(S);
   if (F.cache_depth != 0 && analytic)
     k_tail<true, true><<<L.grid.tail_cache, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
                                                                        L.stats);
@@ -1165,18 +1171,27 @@ void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F
                                                                    L.spill_cap);
 }
 void launch_restir_initial(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
-  k_restir_initial<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+  if (scene_has_lights(S))
+    k_restir_initial<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+  else
+    k_restir_initial<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_restir_clear(const Launch& L, float4* restir, uint32_t pixels) {
   k_restir_clear<<<L.grid.resolve, 256, 0, L.stream>>>(restir, pixels);
 }
 void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                             uint32_t* c) {
-  k_restir_temporal<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
+  if (scene_has_lights(S))
+    k_restir_temporal<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
+  else
+    k_restir_temporal<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_restir_spatial(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                            uint32_t* c) {
-  k_restir_spatial<<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
+  if (scene_has_lights(S))
+    k_restir_spatial<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
+  else
+    k_restir_spatial<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_cache_samples(const Launch& L, const FrameConst& F, const Queues& q, uint32_t* c, uint32_t capacity) {
   k_cache_samples<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, c, capacity);

@@ -155,12 +155,22 @@ struct LightTableHost {
   std::vector<LightNode> nodes;
   std::vector<uint32_t> distant;
   uint32_t parallel_count = 0, parallel = 0;  // parallel lights; the first one's index + 1
+  // The environment image (width 0: none): texels, its sampling
+  // distribution (gpu_scene.cpp:631-704), its frame's axes, scale, and its
+  // entry's probability.
+  uint32_t env_width = 0, env_height = 0, env_columns = 0, env_rows = 0;
+  std::vector<float4> env_texels;
+  std::vector<float> env_dist;
+  float env_frame[9] = {}, env_scale[3] = {}, env_selection = 0.0f;
   bool operator==(const LightTableHost& o) const {
     auto same = [](const auto& a, const auto& b) {
       return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(a[0])) == 0);
     };
     return same(offsets, o.offsets) && same(lights, o.lights) && same(analytic, o.analytic) && same(nodes, o.nodes) &&
-           same(distant, o.distant);
+           same(distant, o.distant) && env_width == o.env_width && env_height == o.env_height &&
+           same(env_texels, o.env_texels) && same(env_dist, o.env_dist) &&
+           std::memcmp(env_frame, o.env_frame, sizeof env_frame) == 0 &&
+           std::memcmp(env_scale, o.env_scale, sizeof env_scale) == 0 && env_selection == o.env_selection;
   }
 };
 
@@ -183,6 +193,10 @@ struct lotus_hip_renderer {
     std::vector<float> positions, transforms;
     std::vector<uint32_t> triangles, gvo, gvc, gto, gtc, igeo, imat, materials;
     std::vector<lotus_hip_light> lights;
+    bool has_env = false;
+    uint32_t env_width = 0, env_height = 0;
+    std::vector<float> env_texels;
+    float env_scale[3] = {0, 0, 0}, env_frame[16] = {};
     float environment[3] = {0, 0, 0};
     uint32_t leaf_size = 0;
   } plan;
@@ -290,6 +304,8 @@ void free_scene(lotus_hip_renderer* r) {
   dev_free(s.analytic);
   dev_free(s.light_nodes);
   dev_free(s.distant);
+  dev_free(s.env_texels);
+  dev_free(s.env_dist);
   dev_free(s.positions);
   dev_free(s.triangles);
   dev_free(s.geometry_vertex_offset);
@@ -628,6 +644,85 @@ uint32_t build_light_node(LightBox* first, LightBox* last, std::vector<LightNode
   return node;
 }
 
+// render_world.cpp:314-345 (EnvironmentFrame): the frame's X, Y, Z columns
+// orthonormalized in that order; false when they are not independent.
+bool environment_frame(const float* m, double axes[3][3]) {
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(m[k])) return false;
+  for (int column = 0; column < 3; ++column) {
+    double axis[3] = {m[4 * column], m[4 * column + 1], m[4 * column + 2]};
+    const double length = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    for (int previous = 0; previous < column; ++previous) {
+      const double along = axis[0] * axes[previous][0] + axis[1] * axes[previous][1] + axis[2] * axes[previous][2];
+      for (int r = 0; r < 3; ++r) axis[r] -= along * axes[previous][r];
+    }
+    const double remainder = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    if (!std::isfinite(length) || !(remainder > 1e-6 * length)) return false;
+    for (int r = 0; r < 3; ++r) axes[column][r] = axis[r] / remainder;
+  }
+  return true;
+}
+
+constexpr uint32_t kMaxEnvironmentColumns = 2048, kMaxEnvironmentRows = 1024;  // gpu_scene.hpp:211-212
+
+// gpu_scene.cpp:631-704 (BuildEnvironmentDistribution): cells of the mean
+// scaled radiance times sin(theta) at the row's centre; their densities
+// over the mean, the rows' cumulative distribution and each row's. Returns
+// the mean cell value (0 for a black image).
+double environment_distribution(const lotus_hip_environment_map& e, uint32_t& columns, uint32_t& rows,
+                                std::vector<float>& dist) {
+  constexpr double kPiD = 3.14159265358979323846;
+  columns = std::min(e.width, kMaxEnvironmentColumns);
+  rows = std::min(e.height, kMaxEnvironmentRows);
+  const size_t cells = size_t(columns) * rows;
+  std::vector<double> sums(cells, 0.0);
+  std::vector<uint32_t> counts(cells, 0);
+  for (uint32_t y = 0; y < e.height; ++y) {
+    const size_t row = uint64_t(y) * rows / e.height;
+    for (uint32_t x = 0; x < e.width; ++x) {
+      const size_t column = uint64_t(x) * columns / e.width;
+      const float* t = e.texels + 4 * (size_t(y) * e.width + x);
+      double radiance = 0.0;
+      for (int c = 0; c < 3; ++c) radiance += std::max(double(t[c]), 0.0) * e.scale[c];
+      sums[row * columns + column] += radiance / 3.0;
+      ++counts[row * columns + column];
+    }
+  }
+  std::vector<double> values(cells), row_sums(rows, 0.0);
+  double total = 0.0;
+  for (uint32_t row = 0; row < rows; ++row) {
+    const double sin_theta = std::sin(kPiD * (row + 0.5) / rows);
+    for (uint32_t column = 0; column < columns; ++column) {
+      const size_t cell = size_t(row) * columns + column;
+      values[cell] = sums[cell] / counts[cell] * sin_theta;
+      row_sums[row] += values[cell];
+    }
+    total += row_sums[row];
+  }
+  const double mean = total / static_cast<double>(cells);
+  dist.assign(cells + (rows + 1) + size_t(rows) * (columns + 1), 0.0f);
+  if (!(mean > 0.0) || !std::isfinite(mean)) return 0.0;
+  float* density = dist.data();
+  float* marginal = density + cells;
+  float* conditional = marginal + rows + 1;
+  for (size_t cell = 0; cell < cells; ++cell) density[cell] = static_cast<float>(values[cell] / mean);
+  double cumulative = 0.0;
+  for (uint32_t row = 0; row < rows; ++row) {
+    cumulative += row_sums[row];
+    marginal[row + 1] = static_cast<float>(cumulative / total);
+    float* cdf = conditional + size_t(row) * (columns + 1);
+    double along = 0.0;
+    for (uint32_t column = 0; column < columns; ++column) {
+      along += values[size_t(row) * columns + column];
+      cdf[column + 1] = row_sums[row] > 0.0 ? static_cast<float>(along / row_sums[row])
+                                            : static_cast<float>(column + 1) / columns;
+    }
+    cdf[columns] = 1.0f;
+  }
+  marginal[rows] = 1.0f;
+  return mean;
+}
+
 // gpu_scene.cpp:144-203, 512-609 (BuildLightTable): one entry per triangle
 // of every instance whose material emits (mean emission x world area, in
 // double), then one per analytic light of positive weight, chosen by Vose's
@@ -669,7 +764,21 @@ void build_light_table(const lotus_hip_scene_plan* p, LightTableHost& t) {
       t.lights.push_back(LightRecord{0.0f, 1.0f, 0u, i});
     }
   }
-  bool any_distant = false;
+  double env_mean = 0.0;
+  if (p->environment_map) {
+    const lotus_hip_environment_map& e = *p->environment_map;
+    double axes[3][3];
+    environment_frame(e.world_from_environment, axes);  // validated
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) t.env_frame[3 * a + c] = static_cast<float>(axes[a][c]);
+    for (int c = 0; c < 3; ++c) t.env_scale[c] = e.scale[c];
+    t.env_width = e.width;
+    t.env_height = e.height;
+    t.env_texels.resize(size_t(e.width) * e.height);
+    std::memcpy(t.env_texels.data(), e.texels, t.env_texels.size() * sizeof(float4));
+    env_mean = environment_distribution(e, t.env_columns, t.env_rows, t.env_dist);
+  }
+  bool any_distant = env_mean > 0.0;
   for (uint32_t l = 0; l < p->light_count; ++l) any_distant |= p->lights[l].shape == LOTUS_HIP_LIGHT_DISTANT;
   const double radius = any_distant ? scene_radius(p) : 0.0;
   std::vector<uint32_t> analytic_entries;
@@ -701,6 +810,18 @@ void build_light_table(const lotus_hip_scene_plan* p, LightTableHost& t) {
     store_bounds(root.low1, root.high1, nullptr, nullptr, kNoLightChild);
   } else if (bounded.size() > 1) {
     build_light_node(bounded.data(), bounded.data() + bounded.size(), t.nodes);
+  }
+  // gpu_scene.cpp:502-510 (EnvironmentWeight): pi R^2 times the image's
+  // integral over the sphere, 2 pi^2 times the mean cell value, over 2 pi
+  double env_weight = 0.0;
+  if (env_mean > 0.0) {
+    constexpr double kPiD = 3.14159265358979323846;
+    env_weight = kPiD * kPiD * radius * radius * env_mean;
+    if (!std::isfinite(env_weight)) env_weight = std::numeric_limits<double>::max();
+  }
+  if (env_weight > 0.0) {
+    weights.push_back(env_weight);
+    t.lights.push_back(LightRecord{0.0f, 1.0f, 0u, kEnvironmentLight});
   }
   double total = 0.0;
   for (double w : weights) total += w;
@@ -739,6 +860,7 @@ void build_light_table(const lotus_hip_scene_plan* p, LightTableHost& t) {
   }
   // each analytic record keeps its entry's probability (gpu_scene.cpp:600-604)
   for (uint32_t e : analytic_entries) t.analytic[t.lights[e].instance & ~kAnalyticLight].radiance[3] = t.lights[e].pdf;
+  if (env_weight > 0.0) t.env_selection = t.lights.back().pdf;
 }
 
 std::string validate_plan(const lotus_hip_scene_plan* p) {
@@ -764,6 +886,17 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   }
   if (tris >= (1ull << 27)) return "the scene holds more than 2^27 triangles";
   if (p->light_count != 0 && !p->lights) return "the plan names lights but holds none";
+  if (p->environment_map) {
+    const lotus_hip_environment_map& e = *p->environment_map;
+    if (e.width == 0 || e.height == 0 || e.width > 16384 || e.height > 16384 || !e.texels)
+      return "the environment image must be 1 to 16384 texels on a side";
+    for (size_t k = 0; k < 4ull * e.width * e.height; ++k)
+      if (!std::isfinite(e.texels[k])) return "the environment image has a non-finite texel";
+    for (int c = 0; c < 3; ++c)
+      if (!(e.scale[c] >= 0.0f) || !std::isfinite(e.scale[c])) return "the environment scale must be finite and non-negative";
+    double axes[3][3];
+    if (!environment_frame(e.world_from_environment, axes)) return "the environment frame's axes are not independent";
+  }
   if (p->light_count > (1u << 24)) return "the scene holds more than 2^24 analytic lights";
   for (uint32_t l = 0; l < p->light_count; ++l) {
     const lotus_hip_light& L = p->lights[l];
@@ -861,6 +994,14 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
   c.instance_materials = !same_words(old.imat, p->instance_material, sh.instances);
   c.environment = std::memcmp(old.environment, p->environment, sizeof old.environment) != 0;
   c.lights = !same_words(old.lights, p->lights, p->light_count);
+  const lotus_hip_environment_map* e = p->environment_map;
+  const bool env_same =
+      old.has_env == (e != nullptr) &&
+      (!e || (old.env_width == e->width && old.env_height == e->height &&
+              std::memcmp(old.env_scale, e->scale, sizeof old.env_scale) == 0 &&
+              std::memcmp(old.env_frame, e->world_from_environment, sizeof old.env_frame) == 0 &&
+              same_words(old.env_texels, e->texels, 4ull * e->width * e->height)));
+  c.lights = c.lights || !env_same;
   c.refit = !c.moved.empty() || c.transforms;
   if (!c.refit && !c.materials && !c.instance_materials && !c.environment && !c.lights) {
     c.update = LOTUS_HIP_SCENE_UNCHANGED;
@@ -904,7 +1045,20 @@ void record_plan(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   }
   if (all || c.transforms) d.transforms.assign(p->instance_transform, p->instance_transform + 16ull * sh.instances);
   if (all || c.materials) d.materials.assign(p->materials, p->materials + 36ull * sh.materials);
-  if (all || c.lights) d.lights.assign(p->lights, p->lights + p->light_count);
+  if (all || c.lights) {
+    d.lights.assign(p->lights, p->lights + p->light_count);
+    const lotus_hip_environment_map* e = p->environment_map;
+    d.has_env = e != nullptr;
+    d.env_width = e ? e->width : 0;
+    d.env_height = e ? e->height : 0;
+    if (e) {
+      d.env_texels.assign(e->texels, e->texels + 4ull * e->width * e->height);
+      std::memcpy(d.env_scale, e->scale, sizeof d.env_scale);
+      std::memcpy(d.env_frame, e->world_from_environment, sizeof d.env_frame);
+    } else {
+      d.env_texels.clear();
+    }
+  }
   if (all || c.instance_materials) d.imat.assign(p->instance_material, p->instance_material + sh.instances);
   std::memcpy(d.environment, p->environment, sizeof d.environment);
   d.leaf_size = sh.leaf_size;
@@ -930,6 +1084,15 @@ void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environme
   v.light_node_count = s.light_node_count;
   v.distant_count = s.distant_count;
   v.parallel = s.parallel;
+  v.env_texels = s.env_texels;
+  v.env_dist = s.env_dist;
+  v.env_width = s.env_width;
+  v.env_height = s.env_height;
+  v.env_columns = s.env_columns;
+  v.env_rows = s.env_rows;
+  std::memcpy(v.env_frame, s.env_frame, sizeof v.env_frame);
+  std::memcpy(v.env_scale, s.env_scale, sizeof v.env_scale);
+  v.env_selection = s.env_selection;
   v.tri_count = sh.tri_count;
   v.instance_count = sh.instances;
   for (int c = 0; c < 3; ++c) v.environment[c] = environment[c];
@@ -941,7 +1104,9 @@ uint64_t scene_device_bytes(const lotus_hip_renderer* r) {
   return s.bytes + 4 * (s.vertex_words + s.index_words) + 4 * (2 * G + I) + 4 * (I + 1) + 68 * I +
          144ull * s.material_count + 4 * I + uint64_t(s.light_count) * sizeof(LightRecord) +
          uint64_t(s.analytic_count) * sizeof(AnalyticRecord) + uint64_t(s.light_node_count) * sizeof(LightNode) +
-         4ull * s.distant_count;
+         4ull * s.distant_count + 16ull * s.env_width * s.env_height +
+         4ull * (s.env_columns == 0 ? 0 : uint64_t(s.env_columns) * s.env_rows + s.env_rows + 1 +
+                                              uint64_t(s.env_rows) * (s.env_columns + 1));
 }
 
 void fill_evidence(const lotus_hip_renderer* r, lotus_hip_scene_evidence* ev, uint32_t update, uint32_t moved,
@@ -983,7 +1148,20 @@ hipError_t upload_light_table(SceneBuffers& s, const LightTableHost& t, Up&& up)
   fit(s.analytic, s.analytic_count, t.analytic.size());
   fit(s.light_nodes, s.light_node_count, t.nodes.size());
   fit(s.distant, s.distant_count, t.distant.size());
+  fit(s.env_texels, size_t(s.env_width) * s.env_height, t.env_texels.size());
+  fit(s.env_dist, s.env_columns == 0 ? 0 : size_t(s.env_columns) * s.env_rows + s.env_rows + 1 +
+                                               size_t(s.env_rows) * (s.env_columns + 1),
+      t.env_dist.size());
   if (e != hipSuccess) return e;
+  s.env_width = t.env_width;
+  s.env_height = t.env_height;
+  s.env_columns = t.env_columns;
+  s.env_rows = t.env_rows;
+  std::memcpy(s.env_frame, t.env_frame, sizeof s.env_frame);
+  std::memcpy(s.env_scale, t.env_scale, sizeof s.env_scale);
+  s.env_selection = t.env_selection;
+  up(s.env_texels, t.env_texels.data(), t.env_texels.size() * sizeof(float4));
+  up(s.env_dist, t.env_dist.data(), t.env_dist.size() * 4);
   s.light_count = static_cast<uint32_t>(t.lights.size());
   s.analytic_count = static_cast<uint32_t>(t.analytic.size());
   s.light_node_count = static_cast<uint32_t>(t.nodes.size());

@@ -323,6 +323,125 @@ __device__ __forceinline__ LightSample sample_analytic(const AnalyticRecord& L, 
   return ls;
 }
 
+// ------------------------------------------------------- environment image
+// scene.slang:805-809 (EnvironmentUv): longitude +pi at u = 0 to -pi at 1,
+// +Y at v = 0 to -Y at 1.
+__device__ __forceinline__ void environment_uv(V3 local, float& u, float& v) {
+  const float uu = 0.5f - atan2_pi(local.x, local.z) * 0.5f;
+  u = uu - floorf(uu);
+  v = acos_pi(fminf(fmaxf(local.y, -1.0f), 1.0f));
+}
+
+// sin and cos of pi x for x in [-1, 1].
+__device__ __forceinline__ void sincos_pi(float x, float* s, float* c) {
+  const float turns = 0.5f * x;
+  sincos_turns(turns < 0.0f ? turns + 1.0f : turns, s, c);
+}
+
+// scene.slang:811-817 (EnvironmentDirection).
+__device__ __forceinline__ V3 environment_direction(float u, float v) {
+  float st, ct, sp, cp;
+  sincos_pi(v, &st, &ct);
+  sincos_pi(1.0f - 2.0f * u, &sp, &cp);
+  return V3{st * sp, ct, st * cp};
+}
+
+// scene.slang:819-827 (EnvironmentLookup): the bilinear texel, repeating
+// across longitudes and clamped at the poles, times the scale.
+__device__ __forceinline__ V3 environment_lookup(const SceneView& S, float u, float v) {
+  const int W = static_cast<int>(S.env_width), H = static_cast<int>(S.env_height);
+  const float x = u * static_cast<float>(W) - 0.5f, y = v * static_cast<float>(H) - 0.5f;
+  const float fx0 = floorf(x), fy0 = floorf(y);
+  const float fx = x - fx0, fy = y - fy0;
+  int i0 = static_cast<int>(fx0) % W;
+  if (i0 < 0) i0 += W;
+  const int i1 = i0 + 1 == W ? 0 : i0 + 1;
+  const int j = static_cast<int>(fy0);
+  const int j0 = j < 0 ? 0 : (j > H - 1 ? H - 1 : j), j1 = j + 1 < 0 ? 0 : (j + 1 > H - 1 ? H - 1 : j + 1);
+  const float4 t00 = S.env_texels[j0 * W + i0], t10 = S.env_texels[j0 * W + i1];
+  const float4 t01 = S.env_texels[j1 * W + i0], t11 = S.env_texels[j1 * W + i1];
+  auto lerp2 = [&](float a, float b, float c, float d) {
+    return ((1.0f - fx) * a + fx * b) * (1.0f - fy) + ((1.0f - fx) * c + fx * d) * fy;
+  };
+  const V3 texel{lerp2(t00.x, t10.x, t01.x, t11.x), lerp2(t00.y, t10.y, t01.y, t11.y),
+                 lerp2(t00.z, t10.z, t01.z, t11.z)};
+  return V3{maxf(texel.x, 0.0f) * S.env_scale[0], maxf(texel.y, 0.0f) * S.env_scale[1],
+            maxf(texel.z, 0.0f) * S.env_scale[2]};
+}
+
+// scene.slang:829-843 (EnvironmentDirectionPdf): the cell's density per unit
+// image area over the 2 pi^2 sin(theta) steradians per unit area.
+__device__ __forceinline__ float environment_direction_pdf(const SceneView& S, float u, float v) {
+  const uint32_t cu = static_cast<uint32_t>(u * static_cast<float>(S.env_columns));
+  const uint32_t cv = static_cast<uint32_t>(v * static_cast<float>(S.env_rows));
+  const uint32_t column = cu < S.env_columns - 1 ? cu : S.env_columns - 1;
+  const uint32_t row = cv < S.env_rows - 1 ? cv : S.env_rows - 1;
+  float st, ct;
+  sincos_pi(v, &st, &ct);
+  if (!(st > 0.0f)) return 0.0f;
+  return S.env_dist[row * S.env_columns + column] / ((2.0f * kPi * kPi) * st);
+}
+
+// scene.slang:775-803 (FindInterval, SampleCumulative): a point of [0, 1)
+// distributed as the `count` cells of the cumulative distribution at `cdf`.
+__device__ __forceinline__ float sample_cumulative(const float* cdf, uint32_t count, float u, uint32_t& cell) {
+  uint32_t low = 0, length = count + 1;
+  while (length > 0) {
+    const uint32_t half = length >> 1, middle = low + half;
+    if (cdf[middle] <= u) {
+      low = middle + 1;
+      length -= half + 1;
+    } else {
+      length = half;
+    }
+  }
+  cell = (low < 1u ? 1u : (low > count ? count : low)) - 1u;
+  const float lo = cdf[cell];
+  const float width = cdf[cell + 1] - lo;
+  const float offset = width > 0.0f ? (u - lo) / width : 0.5f;
+  return fminf((static_cast<float>(cell) + offset) / static_cast<float>(count), 0.99999994f);
+}
+
+__device__ __forceinline__ V3 environment_to_world(const SceneView& S, V3 local) {
+  const float* f = S.env_frame;
+  return normalize3(((local.x * V3{f[0], f[1], f[2]}) + (local.y * V3{f[3], f[4], f[5]})) +
+                    (local.z * V3{f[6], f[7], f[8]}));
+}
+
+// scene.slang:845-875 (SampleEnvironment): a row by ux, a cell of the row by
+// uy, uniformly within the cell.
+__device__ __forceinline__ LightSample sample_environment(const SceneView& S, float selection, float ux, float uy) {
+  const LightSample none{V3{0.0f, 0.0f, 1.0f}, 0.0f, V3{0.0f, 0.0f, 0.0f}, 0.0f, false};
+  const uint32_t cells = S.env_columns * S.env_rows;
+  uint32_t row, column;
+  const float v = sample_cumulative(S.env_dist + cells, S.env_rows, ux, row);
+  const float u = sample_cumulative(S.env_dist + cells + S.env_rows + 1 + row * (S.env_columns + 1), S.env_columns,
+                                    uy, column);
+  float st, ct;
+  sincos_pi(v, &st, &ct);
+  const float density = S.env_dist[row * S.env_columns + column];
+  if (!(st > 0.0f) || !(density > 0.0f)) return none;
+  LightSample ls;
+  ls.direction = environment_to_world(S, environment_direction(u, v));
+  ls.distance = kFloatMax;
+  ls.radiance = environment_lookup(S, u, v);
+  ls.pdf = selection * density / ((2.0f * kPi * kPi) * st);
+  ls.delta = false;
+  return ls;
+}
+
+// scene.slang:877-890 (EnvironmentRadiance): the image's radiance along unit
+// world `d`, weighed against light sampling when `weigh`.
+__device__ __forceinline__ V3 environment_radiance(const SceneView& S, V3 d, bool weigh, float pdf) {
+  const float* f = S.env_frame;
+  const V3 local{dot3(V3{f[0], f[1], f[2]}, d), dot3(V3{f[3], f[4], f[5]}, d), dot3(V3{f[6], f[7], f[8]}, d)};
+  float u, v;
+  environment_uv(local, u, v);
+  const V3 radiance = environment_lookup(S, u, v);
+  if (!weigh || !(S.env_selection > 0.0f)) return radiance;
+  return radiance * power_heuristic(pdf, S.env_selection * environment_direction_pdf(S, u, v));
+}
+
 // scene.slang:894-909 (SampleLight): the alias table's column and threshold
 // choose the light, an emissive triangle or (kAnalytic) an analytic light.
 // Kernels of scenes without analytic lights compile the triangle case alone.
@@ -336,6 +455,7 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux
   const uint32_t light = uy < rec.threshold ? column : rec.alias;
   if constexpr (kAnalytic) {
     const LightRecord chosen = S.lights[light];
+    if (chosen.instance == kEnvironmentLight) return sample_environment(S, chosen.pdf, uz, uw);
     if (chosen.instance & kAnalyticLight)
       return sample_analytic(S.analytic[chosen.instance & ~kAnalyticLight], chosen.pdf, uz, uw, origin);
   }
@@ -350,6 +470,7 @@ __device__ __forceinline__ LightSample reconstruct_reservoir_light(const SceneVi
                                                                    float v, V3 origin) {
   if constexpr (!kAnalytic) return sample_triangle_light(S, index, u, v, origin);
   const LightRecord chosen = S.lights[index];
+  if (chosen.instance == kEnvironmentLight) return sample_environment(S, chosen.pdf, u, v);
   if ((chosen.instance & kAnalyticLight) == 0) return sample_triangle_light(S, index, u, v, origin);
   const AnalyticRecord& L = S.analytic[chosen.instance & ~kAnalyticLight];
   if (L.shape != kSphereLight) return sample_analytic(L, chosen.pdf, u, v, origin);
@@ -380,7 +501,7 @@ __device__ __forceinline__ float reservoir_partition_pdf(const SceneView& S, uin
                                                          V3 origin) {
   if constexpr (!kAnalytic) return ls.pdf;
   const LightRecord chosen = S.lights[index];
-  if (chosen.instance & kAnalyticLight) {
+  if ((chosen.instance & kAnalyticLight) && chosen.instance != kEnvironmentLight) {
     const AnalyticRecord& L = S.analytic[chosen.instance & ~kAnalyticLight];
     if (L.shape == kSphereLight) {
       const V3 offset = V3{L.position[0], L.position[1], L.position[2]} - origin;
@@ -708,7 +829,10 @@ __device__ __forceinline__ V3 initial_direct_light(const SceneView& S, const flo
 template <bool kAnalytic>
 __device__ __forceinline__ void escape_terms(const SceneView& S, bool sample_lights, V3 o, V3 d, V3 throughput,
                                              float last_pdf, V3& env, V3& crossed) {
-  env = throughput * V3{S.environment[0], S.environment[1], S.environment[2]};
+  V3 environment = V3{S.environment[0], S.environment[1], S.environment[2]};
+  const bool weigh = sample_lights && !(last_pdf < 0.0f);
+  if (kAnalytic && S.env_width != 0) environment = environment + environment_radiance(S, d, weigh, last_pdf);
+  env = throughput * environment;
   crossed = V3{0.0f, 0.0f, 0.0f};
   if (kAnalytic && S.analytic_count != 0)
     crossed = throughput * crossed_lights(S, o, d, kFloatMax, true, sample_lights && !(last_pdf < 0.0f), last_pdf);

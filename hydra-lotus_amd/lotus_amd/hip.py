@@ -68,7 +68,27 @@ class ScenePlan(C.Structure):
         ("bvh_leaf_size", C.c_uint32),
         ("light_count", C.c_uint32),
         ("lights", C.c_void_p),
+        ("environment_map", C.c_void_p),
     ]
+
+
+class EnvironmentMap(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p), ("scale", C.c_float * 3),
+                ("world_from_environment", C.c_float * 16)]
+
+
+def environment_struct(env):
+    """The C record of a packed environment image (None: NULL)."""
+    if env is None:
+        return None
+    m = EnvironmentMap()
+    m.height, m.width = env["texels"].shape[:2]
+    m.texels = env["texels"].ctypes.data
+    for i in range(3):
+        m.scale[i] = float(env["scale"][i])
+    for i in range(16):
+        m.world_from_environment[i] = float(env["transform"][i])
+    return m
 
 
 class SceneEvidence(C.Structure):
@@ -411,6 +431,8 @@ class Renderer:
         plan.bvh_leaf_size = leaf_size
         plan.light_count = len(p["lights"])
         plan.lights = p["lights"].ctypes.data if len(p["lights"]) else None
+        env = environment_struct(p.get("environment_map"))
+        plan.environment_map = C.addressof(env) if env is not None else None
         ev = SceneEvidence()
         buf = C.create_string_buffer(512)
         s = lib().lotus_hip_scene_apply(self.handle, C.byref(plan), C.byref(ev), buf, 512)

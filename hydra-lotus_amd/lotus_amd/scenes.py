@@ -95,6 +95,9 @@ class Scene:
     materials: List[np.ndarray] = dataclasses.field(default_factory=lambda: [preview_material()])
     environment: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     lights: List[np.ndarray] = dataclasses.field(default_factory=list)
+    # a latitude-longitude image: dict(texels=float32 [height, width, 4] rows
+    # from the top, scale=(r, g, b), transform=16 floats), or None
+    environment_map: Optional[dict] = None
     view: np.ndarray = dataclasses.field(default_factory=identity)
     projection: np.ndarray = dataclasses.field(default_factory=identity)
     name: str = "scene"
@@ -125,6 +128,9 @@ class Scene:
                      instances=[(g, t.copy(), m) for g, t, m in self.instances],
                      materials=[m.copy() for m in self.materials], environment=tuple(self.environment),
                      lights=[l.copy() for l in self.lights],
+                     environment_map=None if self.environment_map is None else
+                     {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
+                      for k, v in self.environment_map.items()},
                      view=self.view.copy(), projection=self.projection.copy(), name=self.name)
 
     def set_positions(self, geometry: int, positions) -> None:
@@ -186,6 +192,10 @@ class Scene:
             materials=np.ascontiguousarray(np.stack(self.materials).astype(np.uint32)),
             environment=np.array(self.environment, F32),
             lights=np.ascontiguousarray(np.array(self.lights, LIGHT_DTYPE).reshape(-1)),
+            environment_map=None if self.environment_map is None else dict(
+                texels=np.ascontiguousarray(self.environment_map["texels"], F32),
+                scale=np.asarray(self.environment_map.get("scale", (1.0, 1.0, 1.0)), F32),
+                transform=np.asarray(self.environment_map.get("transform", identity()), F32).reshape(16)),
         )
         return self._packed
 
@@ -463,6 +473,20 @@ def lambert_under_sky() -> Scene:
     return sc
 
 
+def sky_image(width: int = 64, height: int = 32, sun=(0.3, 0.25), sun_radiance=60.0, seed: int = 5) -> dict:
+    """A seeded latitude-longitude sky: a blue gradient with noise and a
+    small bright sun disc around image coordinates `sun` (u, v)."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    uu, vv = np.meshgrid(u, v)
+    base = np.stack([0.3 + 0.4 * vv, 0.5 + 0.3 * vv, 0.9 - 0.2 * vv], -1)
+    base = base * (0.8 + 0.4 * uniform(seed, 21, np.arange(width * height)).reshape(height, width, 1))
+    d = np.hypot((uu - sun[0]) * 2.0, vv - sun[1])
+    base = base + (d < 0.05)[..., None] * sun_radiance
+    texels = np.concatenate([base, np.ones((height, width, 1))], -1).astype(F32)
+    return dict(texels=texels, scale=(1.0, 1.0, 1.0), transform=identity())
+
+
 def lit_floor(light: np.ndarray, albedo: float = 0.5, height: float = 2.0) -> Scene:
     """A Lambert floor (ior 1) of half-size 50 in y = 0, seen straight down
     through a 1-degree field of view from `height`, lit only by `light`: the
@@ -474,7 +498,8 @@ def lit_floor(light: np.ndarray, albedo: float = 0.5, height: float = 2.0) -> Sc
     sc.add_mesh([[-h, 0.0, -h], [h, 0.0, -h], [h, 0.0, h], [-h, 0.0, h]], [[0, 2, 1], [0, 3, 2]], m)
     sc.view = look_at((0.0, height, 0.0), (0.0, 0.0, 0.0), up=(0.0, 0.0, 1.0))
     sc.projection = perspective(1.0, 1.0, 0.05, 10.0)
-    sc.add_light(light)
+    if light is not None:
+        sc.add_light(light)
     return sc
 
 

@@ -64,6 +64,18 @@ typedef struct lotus_hip_light {
   float angle;                /* distant, degrees */
 } lotus_hip_light;
 
+/* A latitude-longitude environment image (EnvironmentMap,
+ * include/lotus/render_world.hpp:116-133): RGBA32F texels, rows from the
+ * top (+Y) to the bottom (-Y), column 0 at longitude +pi; its radiance is
+ * the texels' RGB times `scale`, and only the frame's orientation counts
+ * (its axes orthonormalized in X, Y, Z order). */
+typedef struct lotus_hip_environment_map {
+  uint32_t width, height;
+  const float* texels;          /* 4 floats per texel */
+  float scale[3];
+  float world_from_environment[16]; /* column-major */
+} lotus_hip_environment_map;
+
 typedef struct lotus_hip_scene_plan {
   uint32_t geometry_count;
   const float* positions;                   /* xyz per vertex, all geometries */
@@ -82,6 +94,7 @@ typedef struct lotus_hip_scene_plan {
   uint32_t bvh_leaf_size;                   /* 0: default (2); at most 8 */
   uint32_t light_count;                     /* analytic lights, in the light table after the triangles */
   const lotus_hip_light* lights;
+  const lotus_hip_environment_map* environment_map; /* NULL: none; added to the constant environment */
 } lotus_hip_scene_plan;
 
 /* What a lotus_hip_scene_apply did, the counterpart of GpuScene::Apply's

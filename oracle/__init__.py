@@ -51,6 +51,7 @@ class SceneDesc(C.Structure):
         ("geometry_vertex_count", _u32p),
         ("light_count", C.c_uint32),
         ("lights", C.c_void_p),
+        ("environment_map", C.c_void_p),
     ]
 
 
@@ -186,6 +187,9 @@ class OracleScene:
         lights = p.get("lights")
         d.light_count = 0 if lights is None else len(lights)
         d.lights = lights.ctypes.data if d.light_count else None
+        from lotus_amd.hip import environment_struct
+        self._env = environment_struct(p.get("environment_map"))
+        d.environment_map = C.addressof(self._env) if self._env is not None else None
         err = C.create_string_buffer(256)
         self.handle = lib().oracle_scene_create(C.byref(d), err, 256)
         if not self.handle:

@@ -125,6 +125,27 @@ inline void SinCosTurns(float u, float* s, float* c) {
   }
 }
 
+// atan2(y, x) / pi and acos(y) / pi, replacing the reference GPU's atan2 and
+// acos (scene.slang:805-809): reduction to |t| <= tan(pi/8) and the Taylor
+// series of atan to t^15 (error below 2e-8 rad).
+inline float Atan2Pi(float y, float x) {
+  const float ax = std::fabs(x), ay = std::fabs(y);
+  if (ax == 0.0f && ay == 0.0f) return 0.0f;
+  const bool swap = ay > ax;
+  const float r = swap ? ax / ay : ay / ax;
+  const bool shift = r > 0.41421356237309505f;
+  const float t = shift ? (r - 1.0f) / (r + 1.0f) : r;
+  const float t2 = t * t;
+  const float p = t + t * t2 * (-3.33333333333333333e-1f + t2 * (2.0e-1f + t2 * (-1.42857142857142857e-1f +
+      t2 * (1.11111111111111111e-1f + t2 * (-9.09090909090909091e-2f + t2 * (7.69230769230769231e-2f +
+      t2 * -6.66666666666666667e-2f))))));
+  float a = (shift ? 0.785398163397448310f + p : p) * 0.318309886183790672f;
+  if (swap) a = 0.5f - a;
+  if (x < 0.0f) a = 1.0f - a;
+  return y < 0.0f ? -a : a;
+}
+inline float AcosPi(float y) { return Atan2Pi(Sqrt(std::max(0.0f, (1.0f - y) * (1.0f + y))), y); }
+
 // ------------------------------------------------------------------- RNG --
 // sampling.slang:9-25.
 inline uint32_t PcgHash(uint32_t value) {
@@ -536,7 +557,20 @@ struct LightNode {
   float low0[4], high0[4], low1[4], high1[4];
 };
 
+constexpr uint32_t kEnvironmentLight = 0xFFFFFFFFu;
+// The environment image: texels (RGBA), its sampling distribution
+// (densities, the rows' cumulative distribution, each row's), the frame's
+// axes, scale and the probability of its light-table entry.
+struct Environment {
+  uint32_t width = 0, height = 0, columns = 0, rows = 0;
+  std::vector<float> texels, dist;
+  F3 axes[3];
+  F3 scale;
+  float selection = 0.0f;
+};
+
 struct Scene {
+  Environment env;
   std::vector<Analytic> analytic;
   std::vector<LightNode> light_nodes;
   std::vector<uint32_t> distant;
@@ -1074,6 +1108,104 @@ LightSample SampleAnalytic(const Analytic& L, float selection, float ux, float u
   return sample;
 }
 
+// ----------------------------------------------------- environment image
+// scene.slang:805-809 (EnvironmentUv).
+void EnvironmentUv(F3 local, float& u, float& v) {
+  const float uu = 0.5f - Atan2Pi(local.x, local.z) * 0.5f;
+  u = uu - std::floor(uu);
+  v = AcosPi(std::min(std::max(local.y, -1.0f), 1.0f));
+}
+// sin and cos of pi x, x in [-1, 1].
+void SinCosPi(float x, float* s, float* c) {
+  const float turns = 0.5f * x;
+  SinCosTurns(turns < 0.0f ? turns + 1.0f : turns, s, c);
+}
+// scene.slang:811-817 (EnvironmentDirection).
+F3 EnvironmentDirection(float u, float v) {
+  float st, ct, sp, cp;
+  SinCosPi(v, &st, &ct);
+  SinCosPi(1.0f - 2.0f * u, &sp, &cp);
+  return F3{st * sp, ct, st * cp};
+}
+// scene.slang:819-827 (EnvironmentLookup): bilinear, repeating across
+// longitudes and clamped at the poles (the sampler gpu_scene.cpp:2212-2214
+// names), times the scale.
+F3 EnvironmentLookup(const Environment& e, float u, float v) {
+  const int W = static_cast<int>(e.width), H = static_cast<int>(e.height);
+  const float x = u * static_cast<float>(W) - 0.5f, y = v * static_cast<float>(H) - 0.5f;
+  const float x0 = std::floor(x), y0 = std::floor(y);
+  const float fx = x - x0, fy = y - y0;
+  int i0 = static_cast<int>(x0) % W;
+  if (i0 < 0) i0 += W;
+  const int i1 = (i0 + 1) % W;
+  const int j = static_cast<int>(y0);
+  const int j0 = std::min(std::max(j, 0), H - 1), j1 = std::min(std::max(j + 1, 0), H - 1);
+  auto t = [&](int i, int jj, int c) { return e.texels[4 * (size_t(jj) * W + i) + c]; };
+  float rgb[3];
+  for (int c = 0; c < 3; ++c) {
+    const float value = ((1.0f - fx) * t(i0, j0, c) + fx * t(i1, j0, c)) * (1.0f - fy) +
+                        ((1.0f - fx) * t(i0, j1, c) + fx * t(i1, j1, c)) * fy;
+    rgb[c] = std::max(value, 0.0f);
+  }
+  return F3{rgb[0] * e.scale.x, rgb[1] * e.scale.y, rgb[2] * e.scale.z};
+}
+// scene.slang:829-843 (EnvironmentDirectionPdf).
+float EnvironmentDirectionPdf(const Environment& e, float u, float v) {
+  const uint32_t column = std::min(static_cast<uint32_t>(u * static_cast<float>(e.columns)), e.columns - 1);
+  const uint32_t row = std::min(static_cast<uint32_t>(v * static_cast<float>(e.rows)), e.rows - 1);
+  float st, ct;
+  SinCosPi(v, &st, &ct);
+  if (!(st > 0.0f)) return 0.0f;
+  return e.dist[size_t(row) * e.columns + column] / ((2.0f * kPi * kPi) * st);
+}
+// scene.slang:775-803 (FindInterval, SampleCumulative).
+float SampleCumulative(const float* cdf, uint32_t count, float u, uint32_t& cell) {
+  uint32_t low = 0, length = count + 1;
+  while (length > 0) {
+    const uint32_t half = length >> 1, middle = low + half;
+    if (cdf[middle] <= u) {
+      low = middle + 1;
+      length -= half + 1;
+    } else {
+      length = half;
+    }
+  }
+  cell = std::min(std::max(low, 1u), count) - 1;
+  const float low_value = cdf[cell];
+  const float width = cdf[cell + 1] - low_value;
+  const float offset = width > 0.0f ? (u - low_value) / width : 0.5f;
+  return std::min((static_cast<float>(cell) + offset) / static_cast<float>(count), 0.99999994f);
+}
+// scene.slang:845-875 (SampleEnvironment).
+LightSample SampleEnvironment(const Environment& e, float selection, float ux, float uy) {
+  const size_t cells = size_t(e.columns) * e.rows;
+  uint32_t row, column;
+  const float v = SampleCumulative(e.dist.data() + cells, e.rows, ux, row);
+  const float u = SampleCumulative(e.dist.data() + cells + e.rows + 1 + size_t(row) * (e.columns + 1), e.columns,
+                                   uy, column);
+  float st, ct;
+  SinCosPi(v, &st, &ct);
+  const float density = e.dist[size_t(row) * e.columns + column];
+  if (!(st > 0.0f) || !(density > 0.0f)) return NoLight();
+  const F3 local = EnvironmentDirection(u, v);
+  LightSample sample;
+  sample.direction = Normalize(((local.x * e.axes[0]) + (local.y * e.axes[1])) + (local.z * e.axes[2]));
+  sample.distance = kFloatMax;
+  sample.radiance = EnvironmentLookup(e, u, v);
+  sample.pdf = selection * density / ((2.0f * kPi * kPi) * st);
+  sample.delta = false;
+  return sample;
+}
+// scene.slang:877-890 (EnvironmentRadiance).
+F3 EnvironmentRadiance(const Environment& e, F3 direction, bool weigh, float pdf) {
+  const F3 local{Dot(e.axes[0], direction), Dot(e.axes[1], direction), Dot(e.axes[2], direction)};
+  float u, v;
+  EnvironmentUv(local, u, v);
+  const F3 radiance = EnvironmentLookup(e, u, v);
+  if (!weigh || !(e.selection > 0.0f)) return radiance;
+  return radiance * PowerHeuristic(pdf, e.selection * EnvironmentDirectionPdf(e, u, v));
+}
+
 // scene.slang:894-909 (SampleLight): the alias table's column and threshold
 // choose the light, an emissive triangle or an analytic light.
 LightSample SampleLight(const Scene& s, float ux, float uy, float uz, float uw,
@@ -1083,6 +1215,7 @@ LightSample SampleLight(const Scene& s, float ux, float uy, float uz, float uw,
   const LightRecord& record = s.lights[column];
   const uint32_t light = uy < record.threshold ? column : record.alias;
   const LightRecord& chosen = s.lights[light];
+  if (chosen.instance == kEnvironmentLight) return SampleEnvironment(s.env, chosen.pdf, uz, uw);
   if ((chosen.instance & kAnalyticLight) != 0)
     return SampleAnalytic(s.analytic[chosen.instance & ~kAnalyticLight], chosen.pdf, uz, uw, origin);
   return SampleTriangleLight(s, light, uz, uw, origin);
@@ -1091,6 +1224,7 @@ LightSample SampleLight(const Scene& s, float ux, float uy, float uz, float uw,
 // scene.slang:939-971 (ReconstructReservoirLight).
 LightSample ReconstructReservoirLight(const Scene& s, uint32_t index, float u, float v, F3 origin) {
   const LightRecord& chosen = s.lights[index];
+  if (chosen.instance == kEnvironmentLight) return SampleEnvironment(s.env, chosen.pdf, u, v);
   if ((chosen.instance & kAnalyticLight) == 0) return SampleTriangleLight(s, index, u, v, origin);
   const Analytic& L = s.analytic[chosen.instance & ~kAnalyticLight];
   if (L.shape != kSphereLight) return SampleAnalytic(L, chosen.pdf, u, v, origin);
@@ -1115,7 +1249,7 @@ LightSample ReconstructReservoirLight(const Scene& s, uint32_t index, float u, f
 // scene.slang:973-986 (ReservoirPartitionPdf).
 float ReservoirPartitionPdf(const Scene& s, uint32_t index, const LightSample& sample, F3 origin) {
   const LightRecord& chosen = s.lights[index];
-  if ((chosen.instance & kAnalyticLight) != 0) {
+  if ((chosen.instance & kAnalyticLight) != 0 && chosen.instance != kEnvironmentLight) {
     const Analytic& L = s.analytic[chosen.instance & ~kAnalyticLight];
     if (L.shape == kSphereLight) {
       const F3 offset = Xyz(L.position) - origin;
@@ -1529,7 +1663,11 @@ bool ScatterPath(const Scene& s, const oracle_frame& fr, Path& path,
 // transport.slang:235-247 (MissPath): the constant environment, then the
 // analytic lights the escaping ray crossed.
 void MissPath(const Scene& s, const oracle_frame& fr, Path& path) {
-  path.radiance = path.radiance + path.throughput * s.environment;
+  F3 environment = s.environment;
+  if (s.env.width != 0)
+    environment = environment + EnvironmentRadiance(s.env, path.ray.d, fr.light_sampling != 0 && !path.unweighted,
+                                                    path.last_pdf);
+  path.radiance = path.radiance + path.throughput * environment;
   if (!s.analytic.empty())
     path.radiance = path.radiance + path.throughput * CrossedLights(s, path.ray.o, path.ray.d, kFloatMax, true,
                                                                     fr.light_sampling != 0 && !path.unweighted,
@@ -1768,6 +1906,77 @@ uint32_t BuildLightNode(Bounds* first, Bounds* last, std::vector<LightNode>& nod
   return node;
 }
 
+// render_world.cpp:314-345 (EnvironmentFrame).
+bool EnvironmentFrame(const float* m, double axes[3][3]) {
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(m[k])) return false;
+  for (int column = 0; column < 3; ++column) {
+    double axis[3] = {m[4 * column], m[4 * column + 1], m[4 * column + 2]};
+    const double length = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    for (int previous = 0; previous < column; ++previous) {
+      const double along = axis[0] * axes[previous][0] + axis[1] * axes[previous][1] + axis[2] * axes[previous][2];
+      for (int r = 0; r < 3; ++r) axis[r] -= along * axes[previous][r];
+    }
+    const double remainder = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    if (!std::isfinite(length) || !(remainder > 1e-6 * length)) return false;
+    for (int r = 0; r < 3; ++r) axes[column][r] = axis[r] / remainder;
+  }
+  return true;
+}
+
+// gpu_scene.cpp:631-704 (BuildEnvironmentDistribution); returns the mean
+// cell value.
+double BuildEnvironmentDistribution(const oracle_environment_map& m, Environment& e) {
+  const double pi = 3.14159265358979323846;
+  e.columns = std::min(m.width, 2048u);
+  e.rows = std::min(m.height, 1024u);
+  const size_t cells = size_t(e.columns) * e.rows;
+  std::vector<double> sums(cells, 0.0), values(cells), row_sums(e.rows, 0.0);
+  std::vector<uint32_t> counts(cells, 0);
+  for (uint32_t y = 0; y < m.height; ++y) {
+    const size_t row = uint64_t(y) * e.rows / m.height;
+    for (uint32_t x = 0; x < m.width; ++x) {
+      const size_t column = uint64_t(x) * e.columns / m.width;
+      double radiance = 0.0;
+      for (int c = 0; c < 3; ++c)
+        radiance += std::max(double(m.texels[4 * (size_t(y) * m.width + x) + c]), 0.0) * m.scale[c];
+      sums[row * e.columns + column] += radiance / 3.0;
+      ++counts[row * e.columns + column];
+    }
+  }
+  double total = 0.0;
+  for (uint32_t row = 0; row < e.rows; ++row) {
+    const double sin_theta = std::sin(pi * (row + 0.5) / e.rows);
+    for (uint32_t column = 0; column < e.columns; ++column) {
+      const size_t cell = size_t(row) * e.columns + column;
+      values[cell] = sums[cell] / counts[cell] * sin_theta;
+      row_sums[row] += values[cell];
+    }
+    total += row_sums[row];
+  }
+  const double mean = total / static_cast<double>(cells);
+  e.dist.assign(cells + e.rows + 1 + size_t(e.rows) * (e.columns + 1), 0.0f);
+  if (!(mean > 0.0) || !std::isfinite(mean)) return 0.0;
+  for (size_t cell = 0; cell < cells; ++cell) e.dist[cell] = static_cast<float>(values[cell] / mean);
+  float* marginal = e.dist.data() + cells;
+  float* conditional = marginal + e.rows + 1;
+  double cumulative = 0.0;
+  for (uint32_t row = 0; row < e.rows; ++row) {
+    cumulative += row_sums[row];
+    marginal[row + 1] = static_cast<float>(cumulative / total);
+    double along = 0.0;
+    float* cdf = conditional + size_t(row) * (e.columns + 1);
+    for (uint32_t column = 0; column < e.columns; ++column) {
+      along += values[size_t(row) * e.columns + column];
+      cdf[column + 1] = row_sums[row] > 0.0 ? static_cast<float>(along / row_sums[row])
+                                            : static_cast<float>(column + 1) / e.columns;
+    }
+    cdf[e.columns] = 1.0f;
+  }
+  marginal[e.rows] = 1.0f;
+  return mean;
+}
+
 }  // namespace
 
 struct oracle_scene {
@@ -1834,7 +2043,21 @@ oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
       s.lights.push_back(LightRecord{0.0f, 1.0f, 0u, i});
     }
   }
-  bool distant = false;
+  double env_mean = 0.0;
+  if (d->environment_map) {
+    const oracle_environment_map& m = *d->environment_map;
+    double axes[3][3];
+    if (m.width == 0 || m.height == 0 || !EnvironmentFrame(m.world_from_environment, axes))
+      return fail("invalid environment image");
+    s.env.width = m.width;
+    s.env.height = m.height;
+    s.env.texels.assign(m.texels, m.texels + 4 * size_t(m.width) * m.height);
+    for (int a = 0; a < 3; ++a)
+      s.env.axes[a] = F3{static_cast<float>(axes[a][0]), static_cast<float>(axes[a][1]), static_cast<float>(axes[a][2])};
+    s.env.scale = F3{m.scale[0], m.scale[1], m.scale[2]};
+    env_mean = BuildEnvironmentDistribution(m, s.env);
+  }
+  bool distant = env_mean > 0.0;
   for (uint32_t l = 0; l < d->light_count; ++l) distant |= d->lights[l].shape == 3;
   const double scene_radius = distant ? SceneRadius(d) : 0.0;
   std::vector<uint32_t> analytic_entries;
@@ -1864,6 +2087,15 @@ oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
     s.light_nodes.push_back(root);
   } else if (bounded.size() > 1) {
     BuildLightNode(bounded.data(), bounded.data() + bounded.size(), s.light_nodes);
+  }
+  // gpu_scene.cpp:502-510 (EnvironmentWeight), the entry last
+  double env_weight = 0.0;
+  if (env_mean > 0.0) {
+    const double pi = 3.14159265358979323846;
+    env_weight = pi * pi * scene_radius * scene_radius * env_mean;
+    if (!std::isfinite(env_weight)) env_weight = std::numeric_limits<double>::max();
+    weights.push_back(env_weight);
+    s.lights.push_back(LightRecord{0.0f, 1.0f, 0u, kEnvironmentLight});
   }
   double total = 0.0;
   for (double w : weights) total += w;
@@ -1900,6 +2132,7 @@ oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
       s.lights[i].alias = i;
     }
     for (uint32_t e : analytic_entries) s.analytic[s.lights[e].instance & ~kAnalyticLight].radiance[3] = s.lights[e].pdf;
+    if (env_weight > 0.0) s.env.selection = s.lights.back().pdf;
   }
   BuildBvh(s);
   return scene.release();

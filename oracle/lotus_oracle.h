@@ -31,6 +31,15 @@ typedef struct oracle_light {
   float radius, width, height, angle;
 } oracle_light;
 
+/* A latitude-longitude environment image (EnvironmentMap,
+ * include/lotus/render_world.hpp:116-133), RGBA32F rows from the top. */
+typedef struct oracle_environment_map {
+  uint32_t width, height;
+  const float* texels;
+  float scale[3];
+  float world_from_environment[16];
+} oracle_environment_map;
+
 typedef struct oracle_scene_desc {
   uint32_t geometry_count;
   const float* positions;           /* xyz per vertex, all geometries */
@@ -48,6 +57,7 @@ typedef struct oracle_scene_desc {
   const uint32_t* geometry_vertex_count;    /* per geometry */
   uint32_t light_count;
   const oracle_light* lights;
+  const oracle_environment_map* environment_map; /* NULL: none */
 } oracle_scene_desc;
 
 typedef struct oracle_scene oracle_scene;

@@ -122,3 +122,78 @@ def test_bsdf_sampling_takes_one_parallel_light(renderer):
         renderer.render(sc.world_to_clip, 16, 16, light_sampling=hip.LIGHT_BSDF)
     out = renderer.render(sc.world_to_clip, 16, 16, light_sampling=hip.LIGHT_MIS)  # not latched
     assert out["spp"] == 1
+
+
+def sky_cornell():
+    """The lit Cornell box under a sky image: every escaping ray looks it up,
+    and light sampling chooses it from the light table."""
+    sc = lit_cornell()
+    sc.environment_map = scenes.sky_image()
+    sc.environment = (0.05, 0.05, 0.05)
+    return sc
+
+
+def test_environment_light_table_matches_oracle(renderer):
+    sc = sky_cornell()
+    renderer.apply(sc)
+    off_g, rec_g = renderer.light_table()
+    off_c, rec_c = oracle.OracleScene(sc).light_table()
+    assert rec_c[-1, 3] == 0xFFFFFFFF
+    assert np.array_equal(off_g, off_c) and np.array_equal(rec_g, rec_c)
+    # a new image is a table rewrite; removing it too
+    brighter = sc.copy()
+    brighter.environment_map["scale"] = (2.0, 2.0, 2.0)
+    brighter._packed = None
+    assert renderer.apply(brighter).update == hip.SCENE_TABLES
+    assert np.array_equal(renderer.light_table()[1], oracle.OracleScene(brighter).light_table()[1])
+    none = sc.copy()
+    none.environment_map = None
+    none._packed = None
+    assert renderer.apply(none).update == hip.SCENE_TABLES
+    assert np.array_equal(renderer.light_table()[1], oracle.OracleScene(none).light_table()[1])
+
+
+@pytest.mark.parametrize("light", [hip.LIGHT_BSDF, hip.LIGHT_MIS, hip.LIGHT_RESTIR])
+def test_sky_cornell_counts_and_radiance(renderer, light):
+    _say(f"sky cornell light={light}")
+    sc = sky_cornell()
+    renderer.apply(sc)
+    w = h = 96
+    o = oracle.OracleScene(sc)
+    acc, counts, shadow = o.render(oracle.make_frame(sc.world_to_clip, w, h, spp=3, max_bounces=8,
+                                                     light_sampling=light))
+    first = None
+    for k in range(3):
+        out = renderer.render(sc.world_to_clip, w, h, frame_count=1, max_bounces=8, light_sampling=light)
+        n = len(out["counts"])
+        assert np.array_equal(out["counts"], counts[k][:n]) and not counts[k][n:].any(), (k, out["counts"], counts[k])
+        assert out["shadow_rays"] == shadow[k], (k, out["shadow_rays"], shadow[k])
+        if first is None:
+            first = out["color"]
+    acc1, _, _ = o.render(oracle.make_frame(sc.world_to_clip, w, h, spp=1, max_bounces=8, light_sampling=light))
+    ref1 = acc1[..., :3].astype(np.float32)
+    hit = acc1[..., 3] > 0
+    exact = np.mean(np.all(first[..., :3][hit] == ref1[hit], -1))
+    _say(f"  1-spp radiance bit-exact fraction {exact:.6f}")
+    assert exact > 0.999
+    assert np.allclose(first[..., :3], ref1, rtol=1e-4, atol=1e-5)
+    sums, n = renderer.accumulation(w, h)
+    assert n == 3
+    assert np.allclose(sums[..., :3], acc[..., :3], rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("light_sampling", [hip.LIGHT_BSDF, hip.LIGHT_MIS])
+def test_floor_under_constant_environment_on_gpu(renderer, light_sampling):
+    sc = scenes.lit_floor(None)
+    tex = np.ones((16, 32, 4), np.float32)
+    tex[..., :3] = (0.7, 0.5, 0.3)
+    sc.environment_map = dict(texels=tex, scale=(2.0, 1.0, 0.5), transform=scenes.identity())
+    renderer.apply(sc)
+    expect = 0.5 * np.array([0.7, 0.5, 0.3]) * np.array([2.0, 1.0, 0.5])
+    out = renderer.render(sc.world_to_clip, 16, 16, frame_count=256, max_bounces=1, light_sampling=light_sampling)
+    px = out["color"][..., :3].reshape(-1, 3).astype(np.float64)
+    if light_sampling == hip.LIGHT_BSDF:
+        assert np.allclose(px, expect, rtol=1e-5)
+    else:
+        se = px.std(0) / math.sqrt(px.shape[0])
+        assert np.all(np.abs(px.mean(0) - expect) <= 4 * se + 1e-6), (px.mean(0), expect, se)

@@ -128,3 +128,69 @@ def test_light_table_weights_closed_form():
     for i in range(n):
         p[alias[i]] += (1 - thr[i]) / n
     assert np.allclose(p, pdf, atol=1e-6)
+
+
+def _constant_sky(c=(0.7, 0.5, 0.3), scale=(2.0, 1.0, 0.5)):
+    tex = np.ones((16, 32, 4), np.float32)
+    tex[..., :3] = c
+    return dict(texels=tex, scale=scale, transform=scenes.identity())
+
+
+def test_floor_under_constant_environment_image():
+    # A constant image is a constant sky: the floor reflects albedo * c *
+    # scale, exactly under BSDF sampling (every escaping direction sees c)
+    # and within four standard errors under MIS, whose light-table half
+    # samples the image's distribution (density ~ sin(theta), the uniform
+    # sphere) and weighs both halves by the power heuristic.
+    sc = scenes.lit_floor(None)
+    sc.environment_map = _constant_sky()
+    expect = ALBEDO * np.array([0.7, 0.5, 0.3]) * np.array([2.0, 1.0, 0.5])
+    o = oracle.OracleScene(sc)
+    px, shadow = _pixel_means(o, sc, 0, spp=64)
+    assert np.allclose(px, expect, rtol=1e-6)
+    assert shadow.sum() == 0
+    px, shadow = _pixel_means(o, sc, 1, spp=256)
+    se = px.std(0) / math.sqrt(px.shape[0])
+    assert np.all(np.abs(px.mean(0) - expect) <= 4 * se), (px.mean(0), expect, se)
+    # light sampling sends a shadow ray for the upper half of the sphere only
+    assert 0.45 < shadow.sum() / (256 * 256) < 0.55
+
+
+def test_environment_image_estimators_agree():
+    # a sky with a sun 60 times brighter than the rest: BSDF sampling and MIS
+    # must agree, which a wrong direction pdf, uv mapping or frame fails
+    sc = scenes.lit_floor(None)
+    sc.environment_map = scenes.sky_image()
+    o = oracle.OracleScene(sc)
+    bsdf, _ = _pixel_means(o, sc, 0, spp=2048)
+    mis, _ = _pixel_means(o, sc, 1, spp=256)
+    se = np.hypot(bsdf.std(0) / math.sqrt(bsdf.shape[0]), mis.std(0) / math.sqrt(mis.shape[0]))
+    assert np.all(np.abs(bsdf.mean(0) - mis.mean(0)) <= 4 * se), (bsdf.mean(0), mis.mean(0), se)
+    # and a tilted frame moves the sun's elevation: the estimate changes (a
+    # turn about +Y would not: the floor's irradiance ignores azimuth)
+    turned = sc.copy()
+    t = scenes.identity()
+    t[4:7], t[8:11] = (0.0, 0.0, 1.0), (0.0, -1.0, 0.0)  # a quarter turn about +X
+    turned.environment_map["transform"] = t
+    turned._packed = None
+    mis2, _ = _pixel_means(oracle.OracleScene(turned), turned, 1, spp=256)
+    assert not np.allclose(mis2.mean(0), mis.mean(0), rtol=0.02)
+
+
+def test_environment_light_table_entry():
+    # gpu_scene.cpp:502-510, 631-704: the image's entry comes last, weighted
+    # pi^2 R^2 times the mean cell value (cells of mean radiance x sin theta
+    # at the row centre), here against an emissive triangle of weight 3.
+    sc = scenes.lit_floor(None)
+    env = _constant_sky(c=(1.0, 1.0, 1.0), scale=(1.0, 1.0, 1.0))
+    sc.environment_map = env
+    glow = sc.add_material(scenes.preview_material(base_color=(0, 0, 0), ior=1.0, emission=(3.0, 3.0, 3.0)))
+    sc.add_mesh([[0, 0.5, 0], [1, 0.5, 0], [0, 0.5, 2]], [[0, 1, 2]], glow)
+    offsets, records = oracle.OracleScene(sc).light_table()
+    rows = 16
+    mean_cell = np.mean(np.sin(math.pi * (np.arange(rows) + 0.5) / rows))
+    radius = 0.5 * math.sqrt(100.0 ** 2 + 0.5 ** 2 + 100.0 ** 2)
+    w_env = math.pi ** 2 * radius ** 2 * mean_cell
+    assert records[-1, 3] == 0xFFFFFFFF
+    pdf = records[:, 0].view(np.float32)
+    assert np.allclose(pdf, np.array([3.0, w_env]) / (3.0 + w_env), rtol=2e-6)
