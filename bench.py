@@ -237,12 +237,25 @@ def lights_run(r, scene, width, height, depth, frames):
         tot["rays"] += out["camera_rays"] + out["extension_rays"] + out["shadow_rays"]
         for k in ("frame_ms", "shade_ms", "trace_ms", "tail_ms", "camera_ms"):
             tot[k] += out[k]
+    result = {"workload": f"heightfield {scene.triangle_count} tris + 16 sphere lights + a 0.53-degree sun, "
+                          f"{width}x{height}, depth {depth}, NEE+MIS",
+              "frames": frames,
+              "mrays_per_s_gpu": round(tot["rays"] / tot["frame_ms"] / 1e3, 1),
+              "gpu_ms_per_frame": {k.replace("_ms", ""): round(v / frames, 3) for k, v in tot.items() if k != "rays"}}
+    # the same heightfield under a 1024 x 512 sky image instead of the sun
+    sky = scene.copy()
+    sky.environment_map = scenes.sky_image(1024, 512, sun=(0.3, 0.3), sun_radiance=200.0)
+    r.apply(sky)
+    r.render(sky.world_to_clip, width, height, **kw)
+    rays, ms = 0, 0.0
+    for _ in range(frames):
+        out = r.render(sky.world_to_clip, width, height, **kw)
+        rays += out["camera_rays"] + out["extension_rays"] + out["shadow_rays"]
+        ms += out["frame_ms"]
+    result["sky_image"] = {"workload": "the heightfield under a 1024x512 sky image with a sun, NEE+MIS",
+                           "mrays_per_s_gpu": round(rays / ms / 1e3, 1), "gpu_ms_per_frame": round(ms / frames, 3)}
     r.apply(scene)  # back to the triangle lights alone (a table rewrite)
-    return {"workload": f"heightfield {scene.triangle_count} tris + 16 sphere lights + a 0.53-degree sun, "
-                        f"{width}x{height}, depth {depth}, NEE+MIS",
-            "frames": frames,
-            "mrays_per_s_gpu": round(tot["rays"] / tot["frame_ms"] / 1e3, 1),
-            "gpu_ms_per_frame": {k.replace("_ms", ""): round(v / frames, 3) for k, v in tot.items() if k != "rays"}}
+    return result
 
 
 def batch_run(views, frames):
