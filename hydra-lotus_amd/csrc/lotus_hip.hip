@@ -2291,6 +2291,15 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
     e = hipMemcpy(ts->validity, t.validity, npix * sizeof(TemporalValidityPixel), hipMemcpyDeviceToHost);
   if (e == hipSuccess && ts->radiance && tf.accumulate)
     e = hipMemcpy(ts->radiance, t.rad[cur], npix * sizeof(TemporalRadiancePixel), hipMemcpyDeviceToHost);
+  // the reuse passes' records: the temporal pass writes the final records
+  // alone, or the spatial pass's input when that pass follows
+  if (e == hipSuccess && ts->restir_temporal_records && restir && ts->restir_temporal)
+    e = hipMemcpy(ts->restir_temporal_records, tf.spatial ? t.rspin : t.rhist[cur], npix * 4 * sizeof(float4),
+                  hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ts->restir_spatial_records && tf.spatial)
+    e = hipMemcpy(ts->restir_spatial_records, t.rhist[cur], npix * 4 * sizeof(float4), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ts->restir_spatial_sources && tf.spatial)
+    e = hipMemcpy(ts->restir_spatial_sources, t.rsources, npix * sizeof(uint4), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "temporal readback", e));
     return LOTUS_HIP_FAIL;

@@ -222,6 +222,9 @@ class TemporalSettings(C.Structure):
         ("radiance", C.c_void_p),
         ("reset_reason", C.c_uint32),
         ("history_available", C.c_uint32),
+        ("restir_temporal_records", C.c_void_p),
+        ("restir_spatial_records", C.c_void_p),
+        ("restir_spatial_sources", C.c_void_p),
     ]
 
 
@@ -656,10 +659,16 @@ class Renderer:
         if want_products:
             prod = dict(surface=np.zeros((height, width), TEMPORAL_PIXEL),
                         validity=np.zeros((height, width), TEMPORAL_VALIDITY),
-                        radiance=np.zeros((height, width), TEMPORAL_RADIANCE))
+                        radiance=np.zeros((height, width), TEMPORAL_RADIANCE),
+                        restir_temporal=np.zeros((height, width, 16), np.uint32),
+                        restir_spatial=np.zeros((height, width, 16), np.uint32),
+                        restir_sources=np.zeros((height, width, 4), np.uint32))
             t.surface = prod["surface"].ctypes.data
             t.validity = prod["validity"].ctypes.data
             t.radiance = prod["radiance"].ctypes.data
+            t.restir_temporal_records = prod["restir_temporal"].ctypes.data
+            t.restir_spatial_records = prod["restir_spatial"].ctypes.data
+            t.restir_spatial_sources = prod["restir_sources"].ctypes.data
         out = self.render(world_to_clip, width, height,
                           sample_index=frame_index if sample_index is None else sample_index, temporal=t, **kw)
         out.update(prod)

@@ -426,6 +426,15 @@ typedef struct lotus_hip_temporal_settings {
   void* radiance;
   uint32_t reset_reason;       /* written: TemporalResetReason (temporal.hpp:16-26) */
   uint32_t history_available;  /* written */
+  /* Optional ReSTIR products (RestirProducts, vulkan_backend.hpp:175-200),
+   * host buffers of width*height records: the temporal pass's and the
+   * spatial pass's RestirReusePixel (64 B: the RestirPixel, normalization,
+   * supporting candidates, previous index, age) and the spatial pass's
+   * accepted neighbours (4 pixel indices, 0xFFFFFFFF for none). Written when
+   * the frame ran that pass. */
+  void* restir_temporal_records;
+  void* restir_spatial_records;
+  void* restir_spatial_sources;
 } lotus_hip_temporal_settings;
 
 int lotus_hip_render_temporal(lotus_hip_renderer* renderer, const lotus_hip_frame_params* params,

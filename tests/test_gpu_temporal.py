@@ -226,6 +226,47 @@ def test_restir_reuse_is_consistent_and_reduces_error(renderer):
     assert np.mean(reuse_err) < np.mean(initial_err)
 
 
+def test_restir_reuse_products(renderer):
+    # RestirProducts (vulkan_backend.hpp:175-200): the temporal and spatial
+    # passes' records as the frame leaves them (restir.slang:21-28). Each
+    # record's stored normalization is ReservoirReuseNormalization
+    # (reservoir.cpp:102-112, built from the reference) of its reservoir and
+    # supporting count; no stream failed; previous indices name pixels; the
+    # spatial pass's accepted neighbours lie exactly `radius` pixels away
+    # along an axis (restir.slang:408-410).
+    R = oracle.ref()
+    sc = scenes.cornell_v1()
+    renderer.apply(sc)
+    w, h, radius = 48, 40, 6
+    for f in range(4):
+        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=700 + f, sample_index=f,
+                                       restir_temporal=True, restir_spatial=True, radius=radius,
+                                       max_history_candidates=20, max_bounces=8, light_sampling=hip.LIGHT_RESTIR,
+                                       restir_candidates=8, want_products=True)
+    checked = 0
+    for name in ("restir_temporal", "restir_spatial"):
+        words = out[name].reshape(-1, 16)
+        assert np.all(words[:, 7] & 2 == 0), name
+        prev = words[:, 14]
+        assert np.all((prev == 0xFFFFFFFF) | (prev < w * h)), name
+        live = words[:, 1].view(np.float32) > 0
+        assert live.mean() > 0.5, name
+        for i in np.flatnonzero(live)[::7]:
+            res = np.ascontiguousarray(words[i, :4])
+            ref = R.ref_reuse_normalization(res.ctypes.data_as(oracle.C.POINTER(oracle.C.c_uint32)), int(words[i, 13]))
+            assert np.isclose(words[i, 12].view(np.float32), ref, rtol=1e-6), (name, i, words[i], ref)
+            checked += 1
+    assert checked > 100
+    src = out["restir_sources"].reshape(h, w, 4)
+    ys, xs = np.mgrid[0:h, 0:w]
+    for k in range(4):
+        s = src[..., k]
+        ok = s != 0xFFFFFFFF
+        assert ok.any()
+        dx, dy = np.abs(s[ok] % w - xs[ok]), np.abs(s[ok] // w - ys[ok])
+        assert np.all(((dx == radius) & (dy == 0)) | ((dx == 0) & (dy == radius)))
+
+
 def test_temporal_settings_errors(renderer):
     sc = _cornell(renderer)
     with pytest.raises(hip.LotusError, match="temporal reservoirs require"):
