@@ -767,7 +767,7 @@ __global__ void __launch_bounds__(256) k_cache_samples(FrameConst F, Queues q, u
 // for shading, a miss leaves its environment term pending, transport.slang:
 // 436-448), items [n_ext, n_ext + n_shadow) the shadow rays of bounce b - 1
 // (an unblocked ray leaves its light pending).
-template <bool kDynamic, bool kAnalytic>
+template <bool kDynamic>
 __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, uint32_t* counters, uint2* spill,
                                                        int spill_cap, uint32_t bounce, uint32_t kinds,
                                                        unsigned long long* stats) {
@@ -776,7 +776,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
   st.init(ring, spill, spill_cap);
   const uint32_t n_ext = (kinds & kTraceExtension) ? *ray_count(counters, bounce) : 0u;
   const uint32_t n_sh = (kinds & kTraceShadow) ? *shadow_count(counters, bounce - 1) : 0u;
-  const bool weigh = (kinds & kTraceWeigh) != 0;
+  const bool lights = scene_has_lights(S);  // escapes go to k_escape
   __shared__ unsigned long long counts[4];
   BlockStats<2> bs;
   bs.init(counts);
@@ -796,15 +796,17 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
         const bool hit = w.hit();
         bs.add(ext ? 0 : 1, w.visits, w.tests);
         const uint32_t k = wave_append(hit_count(counters, bounce), ext && hit);
+        const uint32_t m = wave_append(escape_count(counters, bounce), ext && !hit && lights);
         if (ext && hit) {
           push_hit(q, k, slot, w.r.o, w.r.d, w.record(S.tris));
         } else if (ext) {
-          const float4 t4 = q.thr[slot];
-          V3 e, crossed;
-          escape_terms<kAnalytic>(S, weigh, w.r.o, w.r.d, xyz(t4), t4.w, e, crossed);
-          const bool more = kAnalytic && S.analytic_count != 0;
-          q.esc[slot] = make_float4(e.x, e.y, e.z, more ? 2.0f : 1.0f);
-          if (more) q.esc2[slot] = make_float4(crossed.x, crossed.y, crossed.z, 0.0f);
+          if (lights) {
+            q.esc_o[m] = make_float4(w.r.o.x, w.r.o.y, w.r.o.z, __uint_as_float(slot));
+            q.esc_d[m] = make_float4(w.r.d.x, w.r.d.y, w.r.d.z, 0.0f);
+          } else {
+            const V3 e = xyz(q.thr[slot]) * V3{S.environment[0], S.environment[1], S.environment[2]};
+            q.esc[slot] = make_float4(e.x, e.y, e.z, 1.0f);
+          }
           float4 r4 = q.rad[slot];
           r4.w = 1.0f;
           q.rad[slot] = r4;
@@ -814,6 +816,27 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
         }
       });
   bs.flush(stats, 1);
+}
+
+// transport.slang:235-247 (MissPath) for the escaped extension rays of a
+// round in a scene with analytic lights or an environment image: each
+// slot's pending environment term (the constant plus the image, weighed
+// against light sampling) and the analytic lights the ray crossed, kept apart
+// so radiance is summed in the reference's order. Kept out of k_trace, whose
+// registers then stay those of the scenes without lights.
+__global__ void __launch_bounds__(256) k_escape(SceneView S, Queues q, uint32_t* counters, uint32_t bounce,
+                                                uint32_t weigh) {
+  const uint32_t n = *escape_count(counters, bounce);
+  for (uint32_t i = gtid(); i < n; i += gstride()) {
+    const float4 o4 = q.esc_o[i], d4 = q.esc_d[i];
+    const uint32_t slot = __float_as_uint(o4.w);
+    const float4 t4 = q.thr[slot];
+    V3 e, crossed;
+    escape_terms<true>(S, weigh != 0, xyz(o4), xyz(d4), xyz(t4), t4.w, e, crossed);
+    const bool more = S.analytic_count != 0;
+    q.esc[slot] = make_float4(e.x, e.y, e.z, more ? 2.0f : 1.0f);
+    if (more) q.esc2[slot] = make_float4(crossed.x, crossed.y, crossed.z, 0.0f);
+  }
 }
 
 // wavefront.slang:287-309: the paths still queued after the recorded rounds,
@@ -1102,8 +1125,8 @@ GridShape grid_shape(uint32_t cus) {
   g.camera = persistent_blocks(k_camera<false>, kTraceBlock, cus);
   g.camera_dynamic = persistent_blocks(k_camera<true>, kTraceBlock, cus);
   // sized for the variants of scenes without analytic lights
-  g.trace = persistent_blocks(k_trace<false, false>, kTraceBlock, cus);
-  g.trace_dynamic = persistent_blocks(k_trace<true, false>, kTraceBlock, cus);
+  g.trace = persistent_blocks(k_trace<false>, kTraceBlock, cus);
+  g.trace_dynamic = persistent_blocks(k_trace<true>, kTraceBlock, cus);
   g.tail = persistent_blocks(k_tail<false, false>, kTraceBlock, cus);
   g.tail_cache = persistent_blocks(k_tail<true, false>, kTraceBlock, cus);
   g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
@@ -1133,19 +1156,15 @@ void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, cons
 }
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds) {
-  const bool analytic = scene_has_lights(S);
-  if (L.dynamic_trace && analytic)
-    k_trace<true, true><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce,
-                                                                           kinds, L.stats);
-  else if (L.dynamic_trace)
-    k_trace<true, false><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce,
-                                                                            kinds, L.stats);
-  else if (analytic)
-    k_trace<false, true><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
-                                                                     L.stats);
+  if (L.dynamic_trace)
+    k_trace<true><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
+                                                                L.stats);
   else
-    k_trace<false, false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
-                                                                      L.stats);
+    k_trace<false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
+                                                                 L.stats);
+}
+void launch_escape(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce, bool weigh) {
+  k_escape<<<L.grid.resolve, 256, 0, L.stream>>>(S, q, c, bounce, weigh ? 1u : 0u);
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce) {

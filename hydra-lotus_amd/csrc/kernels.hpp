@@ -24,6 +24,8 @@ struct Queues {
   float4* nee;    // per slot: a shadow ray's light awaiting addition, flag
   float4* esc;    // per slot: the environment an escaping ray found; w: 1, or 2 when esc2 follows
   float4* esc2;   // per slot: the analytic lights that ray crossed (read only when esc.w is 2)
+  float4* esc_o;  // escaped rays of a scene with lights, in escape order: origin + slot
+  float4* esc_d;  //   and direction
   float4* ray_o;  // ray queue: origin, slot
   float4* ray_d;  //            direction, tmax
   float4* hit_o;  // hit queue: ray origin, slot
@@ -95,6 +97,9 @@ void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t
                   uint32_t kinds);
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce);
+// The pending environment and crossed analytic lights of bounce `bounce`'s
+// escaped rays (scenes with lights; `weigh`: light sampling is on).
+void launch_escape(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce, bool weigh);
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample);
 void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, float* depth,
                         uint32_t write_color);
@@ -146,9 +151,13 @@ __host__ __device__ inline uint32_t* cache_query_count(uint32_t* c) { return c +
 __host__ __device__ inline uint32_t* cache_sample_count(uint32_t* c) { return c + counter_word(1, 3); }
 // ReSTIR visibility rays traced inline by the reservoir passes (not queued).
 __host__ __device__ inline uint32_t* restir_ray_count(uint32_t* c) { return c + counter_word(2, 3); }
-// Counter words a frame clears: 4 counters per bounce, at least the three above.
+// Extension rays of bounce b (b >= 1) that escaped a scene with analytic
+// lights or an environment image: kind 3 of bounce b + 2.
+__host__ __device__ inline uint32_t* escape_count(uint32_t* c, uint32_t bounce) { return c + counter_word(bounce + 2, 3); }
+// Counter words a frame clears: 4 counters per bounce (and the escape counts
+// one bounce further), at least the three above.
 __host__ __device__ inline uint32_t frame_counter_words(uint32_t bounce_stride) {
-  return counter_word(bounce_stride > 3u ? bounce_stride : 3u, 0);
+  return counter_word((bounce_stride > 3u ? bounce_stride : 3u) + 1u, 0);
 }
 
 }  // namespace lotus

@@ -269,6 +269,8 @@ void free_target(lotus_hip_renderer* r) {
   dev_free(q.nee);
   dev_free(q.esc);
   dev_free(q.esc2);
+  dev_free(q.esc_o);
+  dev_free(q.esc_d);
   dev_free(q.ray_o);
   dev_free(q.ray_d);
   dev_free(q.hit_o);
@@ -340,7 +342,7 @@ hipError_t ensure_target(lotus_hip_renderer* r, uint32_t width, uint32_t height)
   const size_t P = size_t(width) * height;
   Queues& q = r->q;
   hipError_t e = hipSuccess;
-  for (float4** p : {&q.thr, &q.rad, &q.nee, &q.esc, &q.esc2, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.accum,
+  for (float4** p : {&q.thr, &q.rad, &q.nee, &q.esc, &q.esc2, &q.esc_o, &q.esc_d, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.accum,
                      &q.color})
     if (e == hipSuccess) e = dev_alloc(*p, P);
   if (e == hipSuccess) e = dev_alloc(q.hit_h, P);
@@ -1658,12 +1660,16 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
               if (!(kinds & kind)) continue;
               mark(kind == kTraceExtension ? kTIntersect : kTShadow);
               launch_trace(L, r->view, r->q, r->counters, b + 1, kind | weigh);
+              if (kind == kTraceExtension && scene_has_lights(r->view))
+                launch_escape(L, r->view, r->q, r->counters, b + 1, weigh != 0);
               done();
               if (add) ++trace_launches;
             }
           } else {
             mark(kTTrace);
             launch_trace(L, r->view, r->q, r->counters, b + 1, kinds | weigh);
+            if ((kinds & kTraceExtension) && scene_has_lights(r->view))
+              launch_escape(L, r->view, r->q, r->counters, b + 1, weigh != 0);
             done();
             if (add) ++trace_launches;
           }
