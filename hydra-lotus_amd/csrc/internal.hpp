@@ -85,15 +85,24 @@ struct SceneView {
   float env_frame[9];
   float env_scale[3];
   float env_selection;
+  // Material texture lookups: each texture's {first texel, width, height},
+  // the texels as linear RGBA, the binding table (24 words per binding),
+  // per-corner texture coordinates and each instance's first corner
+  // (0xFFFFFFFF: none).
+  const uint4* textures;
+  const float4* texels;
+  const uint32_t* bindings;
+  const float2* texcoords;
+  const uint32_t* instance_texcoords;
 };
 
 // The light-table entry of the environment image (gpu_scene.hpp:132-134).
 constexpr uint32_t kEnvironmentLight = 0xFFFFFFFFu;
 
-// Whether the kernels' variants with analytic lights and the environment
-// image run.
-__host__ __device__ inline bool scene_has_lights(const SceneView& S) {
-  return S.analytic_count != 0 || S.env_width != 0;
+// Whether the kernels' variants with analytic lights, the environment image
+// and material texture lookups run (the others compile none of that code).
+__host__ __device__ inline bool scene_has_extras(const SceneView& S) {
+  return S.analytic_count != 0 || S.env_width != 0 || S.bindings != nullptr;
 }
 
 // Device memory of the resident scene, kept until a plan needs a new BVH.
@@ -117,6 +126,13 @@ struct SceneBuffers {
   float env_frame[9] = {};
   float env_scale[3] = {};
   float env_selection = 0.0f;
+  uint4* textures = nullptr;
+  float4* texels = nullptr;
+  uint32_t* bindings = nullptr;
+  float2* texcoords = nullptr;
+  uint32_t* instance_texcoords = nullptr;
+  uint32_t texture_count = 0, binding_count = 0;
+  uint64_t texel_count = 0, texcoord_count = 0;
   // The plan's geometry as uploaded: refits re-place its triangles and
   // read-back copies it.
   float* positions = nullptr;

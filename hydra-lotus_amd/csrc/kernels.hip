@@ -250,8 +250,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
     const uint4 h4 = q.hit_h[i];
     const uint32_t slot = __float_as_uint(o4.w);
     const V3 dir = xyz(d4);
-    const Surface sf = reconstruct_hit(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z), __uint_as_float(h4.w),
-                                       dir);
+    const Surface sf = reconstruct_hit<kLights>(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z),
+                                                __uint_as_float(h4.w), dir);
     const V3 n = sf.shading_normal;
     const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
     const float a = -1.0f / (sign + n.z);
@@ -672,7 +672,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
     Scatter s;
     Surface sf;
     if (query) {
-      sf = reconstruct_hit(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z), __uint_as_float(h4.w), p.d);
+      sf = reconstruct_hit<kAnalytic>(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z), __uint_as_float(h4.w), p.d);
       add_emission(S, F, p, sf, h4.x, h4.y);
       s.continues = false;
       s.shadow = false;
@@ -776,7 +776,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
   st.init(ring, spill, spill_cap);
   const uint32_t n_ext = (kinds & kTraceExtension) ? *ray_count(counters, bounce) : 0u;
   const uint32_t n_sh = (kinds & kTraceShadow) ? *shadow_count(counters, bounce - 1) : 0u;
-  const bool lights = scene_has_lights(S);  // escapes go to k_escape
+  const bool lights = scene_has_extras(S);  // escapes go to k_escape
   __shared__ unsigned long long counts[4];
   BlockStats<2> bs;
   bs.init(counts);
@@ -1149,7 +1149,7 @@ void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, con
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce, bool link) {
-  if (scene_has_lights(S))
+  if (scene_has_extras(S))
     k_shade<true><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
   else
     k_shade<false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
@@ -1168,7 +1168,7 @@ void launch_escape(const Launch& L, const SceneView& S, const Queues& q, uint32_
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce) {
-  const bool analytic = scene_has_lights(S);
+  const bool analytic = scene_has_extras(S);
   if (F.cache_depth != 0 && analytic)
     k_tail<true, true><<<L.grid.tail_cache, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
                                                                        L.stats);
@@ -1190,7 +1190,7 @@ void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F
                                                                    L.spill_cap);
 }
 void launch_restir_initial(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
-  if (scene_has_lights(S))
+  if (scene_has_extras(S))
     k_restir_initial<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
   else
     k_restir_initial<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
@@ -1200,14 +1200,14 @@ void launch_restir_clear(const Launch& L, float4* restir, uint32_t pixels) {
 }
 void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                             uint32_t* c) {
-  if (scene_has_lights(S))
+  if (scene_has_extras(S))
     k_restir_temporal<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
   else
     k_restir_temporal<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_restir_spatial(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                            uint32_t* c) {
-  if (scene_has_lights(S))
+  if (scene_has_extras(S))
     k_restir_spatial<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
   else
     k_restir_spatial<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);

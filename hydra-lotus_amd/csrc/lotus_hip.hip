@@ -147,6 +147,28 @@ struct TemporalFrameState {
   float4* filter = nullptr;
 };
 
+// gpu_scene.cpp:144-179 (EmissionEstimate): the mean emission of a
+// UsdPreviewSurface record, its lookup's max(scale, 0) + max(bias, 0) (or
+// fallback) when its emission is textured; 0 unless finite and positive.
+double emission_estimate(const uint32_t* m, const uint32_t* bindings) {
+  double e[3];
+  if ((m[2] & (1u << 5)) == 0) {
+    for (int c = 0; c < 3; ++c) {
+      float v;
+      std::memcpy(&v, m + 16 + c, 4);
+      e[c] = v;
+    }
+  } else {
+    const uint32_t* b = bindings + 24ull * (m[1] + static_cast<uint32_t>(__builtin_popcount(m[2] & 31u)));
+    float f[24];
+    std::memcpy(f, b, sizeof f);
+    for (int c = 0; c < 3; ++c)
+      e[c] = b[3] == 2u ? std::max(0.0f, f[12 + c]) : double(std::max(0.0f, f[4 + c])) + std::max(0.0f, f[8 + c]);
+  }
+  const double mean = (e[0] + e[1] + e[2]) / 3.0;
+  return std::isfinite(mean) && mean > 0.0 ? mean : 0.0;
+}
+
 // The light table and the analytic lights as the device holds them.
 struct LightTableHost {
   std::vector<uint32_t> offsets;  // per instance: its first entry or kNoLights
@@ -193,6 +215,11 @@ struct lotus_hip_renderer {
     std::vector<float> positions, transforms;
     std::vector<uint32_t> triangles, gvo, gvc, gto, gtc, igeo, imat, materials;
     std::vector<lotus_hip_light> lights;
+    // texture lookups: texture sizes and formats, texel bytes, bindings,
+    // texture coordinates and each geometry's first corner
+    std::vector<uint32_t> texture_shape, bindings, texcoord_offset;
+    std::vector<uint8_t> texel_bytes;
+    std::vector<float> texcoords;
     bool has_env = false;
     uint32_t env_width = 0, env_height = 0;
     std::vector<float> env_texels;
@@ -308,6 +335,11 @@ void free_scene(lotus_hip_renderer* r) {
   dev_free(s.distant);
   dev_free(s.env_texels);
   dev_free(s.env_dist);
+  dev_free(s.textures);
+  dev_free(s.texels);
+  dev_free(s.bindings);
+  dev_free(s.texcoords);
+  dev_free(s.instance_texcoords);
   dev_free(s.positions);
   dev_free(s.triangles);
   dev_free(s.geometry_vertex_offset);
@@ -736,10 +768,7 @@ void build_light_table(const lotus_hip_scene_plan* p, LightTableHost& t) {
   std::vector<double> weights;
   for (uint32_t i = 0; i < p->instance_count; ++i) {
     const uint32_t* m = p->materials + 36ull * p->instance_material[i];
-    float e[3];
-    std::memcpy(e, m + 16, 12);
-    double mean = (double(e[0]) + double(e[1]) + double(e[2])) / 3.0;
-    if (!(std::isfinite(mean) && mean > 0.0)) mean = 0.0;
+    const double mean = emission_estimate(m, p->bindings);
     const uint32_t geo = p->instance_geometry[i];
     const uint32_t count = p->geometry_triangle_count[geo];
     if (mean <= 0.0 || count == 0) continue;
@@ -888,6 +917,30 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   }
   if (tris >= (1ull << 27)) return "the scene holds more than 2^27 triangles";
   if (p->light_count != 0 && !p->lights) return "the plan names lights but holds none";
+  if ((p->texture_count != 0 && !p->textures) || (p->binding_count != 0 && !p->bindings))
+    return "the plan names textures or bindings but holds none";
+  for (uint32_t k = 0; k < p->texture_count; ++k) {
+    const lotus_hip_texture& x = p->textures[k];
+    if (x.width == 0 || x.height == 0 || x.width > 16384 || x.height > 16384 || !x.texels ||
+        x.format > LOTUS_HIP_TEXTURE_RGBA32F)
+      return "texture " + std::to_string(k) + " must be 1 to 16384 texels on a side in a known format";
+  }
+  for (uint32_t k = 0; k < p->binding_count; ++k) {
+    const uint32_t* b = p->bindings + 24ull * k;
+    if (b[3] > 2u || b[2] > 3u || b[1] > 15u || (b[3] == 1u && b[0] >= p->texture_count))
+      return "binding " + std::to_string(k) + " names a missing texture or an unknown mode";
+    for (int w = 4; w < 24; ++w) {
+      float f;
+      std::memcpy(&f, b + w, 4);
+      if (!std::isfinite(f)) return "binding " + std::to_string(k) + " holds a non-finite value";
+    }
+  }
+  if (p->geometry_texcoord_offset) {
+    for (uint32_t g = 0; g < p->geometry_count; ++g) {
+      const uint32_t first = p->geometry_texcoord_offset[g];
+      if (first != 0xFFFFFFFFu && !p->texcoords) return "the plan names texture coordinates but holds none";
+    }
+  }
   if (p->environment_map) {
     const lotus_hip_environment_map& e = *p->environment_map;
     if (e.width == 0 || e.height == 0 || e.width > 16384 || e.height > 16384 || !e.texels)
@@ -915,7 +968,10 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   for (uint32_t m = 0; m < p->material_count; ++m) {
     const uint32_t* w = p->materials + 36ull * m;
     if ((w[0] & 0xFFu) != 0) return "material " + std::to_string(m) + ": only UsdPreviewSurface is supported";
-    if (w[2] != 0 || w[3] != 0) return "material " + std::to_string(m) + ": texture inputs are not supported";
+    if (w[3] != 0 || (w[2] & ~0x7Cu) != 0)
+      return "material " + std::to_string(m) + ": only base colour, roughness, metallic, emission and specular colour lookups are supported";
+    if (w[2] != 0 && (!p->bindings || uint64_t(w[1]) + __builtin_popcount(w[2]) > p->binding_count))
+      return "material " + std::to_string(m) + ": its lookups lie outside the binding table";
     float opacity;
     std::memcpy(&opacity, w + 6, 4);
     if (opacity != 1.0f) return "material " + std::to_string(m) + ": fractional opacity is not supported";
@@ -955,11 +1011,35 @@ bool same_words(const std::vector<T>& held, const T* src, uint64_t n) {
 }
 
 // An instance of this material enters the light table (gpu_scene.cpp:144-203).
-bool material_emits(const uint32_t* m) {
-  float e[3];
-  std::memcpy(e, m + 16, 12);
-  const double mean = (double(e[0]) + double(e[1]) + double(e[2])) / 3.0;
-  return std::isfinite(mean) && mean > 0.0;
+bool material_emits(const uint32_t* m, const uint32_t* bindings) { return emission_estimate(m, bindings) > 0.0; }
+
+// The plan's texture lookup data in one comparable form: texture shapes,
+// texel bytes, bindings, texture coordinates (the corners the geometries
+// name) and each geometry's first corner.
+struct TextureData {
+  std::vector<uint32_t> shape, bindings, texcoord_offset;
+  std::vector<uint8_t> texel_bytes;
+  std::vector<float> texcoords;
+};
+TextureData texture_data(const lotus_hip_scene_plan* p) {
+  TextureData d;
+  for (uint32_t k = 0; k < p->texture_count; ++k) {
+    const lotus_hip_texture& x = p->textures[k];
+    d.shape.insert(d.shape.end(), {x.width, x.height, x.format});
+    const size_t bytes = size_t(x.width) * x.height * (x.format == LOTUS_HIP_TEXTURE_RGBA32F ? 16 : 4);
+    const uint8_t* t = static_cast<const uint8_t*>(x.texels);
+    d.texel_bytes.insert(d.texel_bytes.end(), t, t + bytes);
+  }
+  if (p->binding_count) d.bindings.assign(p->bindings, p->bindings + 24ull * p->binding_count);
+  if (p->geometry_texcoord_offset) {
+    d.texcoord_offset.assign(p->geometry_texcoord_offset, p->geometry_texcoord_offset + p->geometry_count);
+    uint64_t corners = 0;
+    for (uint32_t g = 0; g < p->geometry_count; ++g)
+      if (d.texcoord_offset[g] != 0xFFFFFFFFu)
+        corners = std::max<uint64_t>(corners, uint64_t(d.texcoord_offset[g]) + 3ull * p->geometry_triangle_count[g]);
+    d.texcoords.assign(p->texcoords, p->texcoords + 2 * corners);
+  }
+  return d;
 }
 
 // What a plan changes against the applied one (GpuScene::Apply's retain and
@@ -985,6 +1065,11 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
       same_words(old.igeo, p->instance_geometry, sh.instances) && old.positions.size() == sh.vertex_words &&
       same_words(old.triangles, p->triangles, sh.index_words);
   if (!topology) return c;
+  // texture lookups are uploaded with a new scene (a change of them builds)
+  const TextureData tex = texture_data(p);
+  if (tex.shape != old.texture_shape || tex.bindings != old.bindings || tex.texcoord_offset != old.texcoord_offset ||
+      tex.texel_bytes != old.texel_bytes || tex.texcoords != old.texcoords)
+    return c;
   if (!same_words(old.positions, p->positions, sh.vertex_words)) {
     for (uint32_t g = 0; g < sh.geometries; ++g) {
       const uint64_t first = 3ull * p->geometry_vertex_offset[g], words = 3ull * p->geometry_vertex_count[g];
@@ -1019,7 +1104,7 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
     std::vector<char> moved(sh.geometries, 0);
     for (uint32_t g : c.moved) moved[g] = 1;
     for (uint32_t i = 0; i < sh.instances && c.temporal_compatible; ++i) {
-      if (!material_emits(p->materials + 36ull * p->instance_material[i])) continue;
+      if (!material_emits(p->materials + 36ull * p->instance_material[i], p->bindings)) continue;
       if (moved[p->instance_geometry[i]] ||
           std::memcmp(old.transforms.data() + 16ull * i, p->instance_transform + 16ull * i, 64) != 0)
         c.temporal_compatible = false;
@@ -1032,6 +1117,12 @@ void record_plan(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   auto& d = r->plan;
   const bool all = c.update == LOTUS_HIP_SCENE_BUILD;
   if (all) {
+    TextureData tex = texture_data(p);
+    d.texture_shape = std::move(tex.shape);
+    d.bindings = std::move(tex.bindings);
+    d.texcoord_offset = std::move(tex.texcoord_offset);
+    d.texel_bytes = std::move(tex.texel_bytes);
+    d.texcoords = std::move(tex.texcoords);
     d.positions.assign(p->positions, p->positions + sh.vertex_words);
     d.triangles.assign(p->triangles, p->triangles + sh.index_words);
     d.gvo.assign(p->geometry_vertex_offset, p->geometry_vertex_offset + sh.geometries);
@@ -1095,6 +1186,11 @@ void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environme
   std::memcpy(v.env_frame, s.env_frame, sizeof v.env_frame);
   std::memcpy(v.env_scale, s.env_scale, sizeof v.env_scale);
   v.env_selection = s.env_selection;
+  v.textures = s.textures;
+  v.texels = s.texels;
+  v.bindings = s.binding_count ? s.bindings : nullptr;
+  v.texcoords = s.texcoords;
+  v.instance_texcoords = s.instance_texcoords;
   v.tri_count = sh.tri_count;
   v.instance_count = sh.instances;
   for (int c = 0; c < 3; ++c) v.environment[c] = environment[c];
@@ -1108,7 +1204,8 @@ uint64_t scene_device_bytes(const lotus_hip_renderer* r) {
          uint64_t(s.analytic_count) * sizeof(AnalyticRecord) + uint64_t(s.light_node_count) * sizeof(LightNode) +
          4ull * s.distant_count + 16ull * s.env_width * s.env_height +
          4ull * (s.env_columns == 0 ? 0 : uint64_t(s.env_columns) * s.env_rows + s.env_rows + 1 +
-                                              uint64_t(s.env_rows) * (s.env_columns + 1));
+                                              uint64_t(s.env_rows) * (s.env_columns + 1)) +
+         16ull * (s.texture_count + s.texel_count) + 96ull * s.binding_count + 8ull * s.texcoord_count + 4ull * I;
 }
 
 void fill_evidence(const lotus_hip_renderer* r, lotus_hip_scene_evidence* ev, uint32_t update, uint32_t moved,
@@ -1214,6 +1311,50 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   up(s.instance_transform, p->instance_transform, 64ull * I);
   up(s.materials, p->materials, 144ull * M);
   if (e == hipSuccess) e = upload_light_table(s, table, up);
+  // texture lookups: texels decoded to linear RGBA, {first texel, width,
+  // height} per texture, the binding table, texture coordinates
+  std::vector<float4> texels;
+  std::vector<uint4> textures;
+  for (uint32_t k = 0; k < p->texture_count; ++k) {
+    const lotus_hip_texture& x = p->textures[k];
+    textures.push_back(make_uint4(static_cast<uint32_t>(texels.size()), x.width, x.height, 0u));
+    const size_t n = size_t(x.width) * x.height;
+    if (x.format == LOTUS_HIP_TEXTURE_RGBA32F) {
+      const float4* f = static_cast<const float4*>(x.texels);
+      texels.insert(texels.end(), f, f + n);
+      continue;
+    }
+    const uint8_t* b = static_cast<const uint8_t*>(x.texels);
+    for (size_t i = 0; i < n; ++i) {
+      float c[4];
+      for (int k2 = 0; k2 < 4; ++k2) {
+        double v = b[4 * i + k2] / 255.0;
+        // sRGB texels decode their colour, not alpha (TexelRgb, gpu_scene.cpp:612-629)
+        if (x.format == LOTUS_HIP_TEXTURE_RGBA8_SRGB && k2 < 3)
+          v = v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4);
+        c[k2] = static_cast<float>(v);
+      }
+      texels.push_back(make_float4(c[0], c[1], c[2], c[3]));
+    }
+  }
+  std::vector<uint32_t> instance_texcoords(I, 0xFFFFFFFFu);
+  const TextureData tex = texture_data(p);
+  if (p->geometry_texcoord_offset)
+    for (uint32_t i = 0; i < I; ++i) instance_texcoords[i] = p->geometry_texcoord_offset[p->instance_geometry[i]];
+  s.texture_count = p->texture_count;
+  s.binding_count = p->binding_count;
+  s.texel_count = texels.size();
+  s.texcoord_count = tex.texcoords.size() / 2;
+  if (e == hipSuccess) e = dev_alloc(s.textures, textures.size());
+  if (e == hipSuccess) e = dev_alloc(s.texels, texels.size());
+  if (e == hipSuccess) e = dev_alloc(s.bindings, tex.bindings.size());
+  if (e == hipSuccess) e = dev_alloc(s.texcoords, tex.texcoords.size() / 2);
+  if (e == hipSuccess) e = dev_alloc(s.instance_texcoords, I);
+  up(s.textures, textures.data(), textures.size() * sizeof(uint4));
+  up(s.texels, texels.data(), texels.size() * sizeof(float4));
+  up(s.bindings, tex.bindings.data(), tex.bindings.size() * 4);
+  up(s.texcoords, tex.texcoords.data(), tex.texcoords.size() * 4);
+  up(s.instance_texcoords, instance_texcoords.data(), instance_texcoords.size() * 4);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (e != hipSuccess) {
@@ -1660,7 +1801,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
               if (!(kinds & kind)) continue;
               mark(kind == kTraceExtension ? kTIntersect : kTShadow);
               launch_trace(L, r->view, r->q, r->counters, b + 1, kind | weigh);
-              if (kind == kTraceExtension && scene_has_lights(r->view))
+              if (kind == kTraceExtension && scene_has_extras(r->view))
                 launch_escape(L, r->view, r->q, r->counters, b + 1, weigh != 0);
               done();
               if (add) ++trace_launches;
@@ -1668,7 +1809,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
           } else {
             mark(kTTrace);
             launch_trace(L, r->view, r->q, r->counters, b + 1, kinds | weigh);
-            if ((kinds & kTraceExtension) && scene_has_lights(r->view))
+            if ((kinds & kTraceExtension) && scene_has_extras(r->view))
               launch_escape(L, r->view, r->q, r->counters, b + 1, weigh != 0);
             done();
             if (add) ++trace_launches;

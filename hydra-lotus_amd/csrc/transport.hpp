@@ -98,9 +98,101 @@ __device__ __forceinline__ Ray start_path(const FrameConst& f, uint32_t x, uint3
                     fy * f.ndc_from_pixel[1] + f.ndc_from_pixel[3]);
 }
 
+// ---------------------------------------------------- material textures
+// Texture-input slots (scene.slang:59-67), lookup modes (gpu_scene.hpp:56-59)
+// and wrap modes (TextureWrap, material.hpp:43-50).
+constexpr uint32_t kBaseColorSlot = 2, kRoughnessSlot = 3, kMetallicSlot = 4, kEmissionSlot = 5,
+                   kSpecularColorSlot = 6;
+constexpr uint32_t kPreviewLookupSlots = (1u << kBaseColorSlot) | (1u << kRoughnessSlot) | (1u << kMetallicSlot) |
+                                         (1u << kEmissionSlot) | (1u << kSpecularColorSlot);
+constexpr uint32_t kFallbackLookup = 2;
+constexpr uint32_t kWrapBlack = 0, kWrapRepeat = 2, kWrapMirror = 3;
+
+// A texel index under a wrap mode; `black` is set outside a black border.
+__device__ __forceinline__ int wrap_texel(int i, int n, uint32_t mode, bool& black) {
+  if (mode == kWrapRepeat) {
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+  }
+  if (mode == kWrapMirror) {
+    int r = i % (2 * n);
+    if (r < 0) r += 2 * n;
+    return r < n ? r : 2 * n - 1 - r;
+  }
+  if (i < 0 || i >= n) {
+    if (mode == kWrapBlack) black = true;
+    return i < 0 ? 0 : n - 1;
+  }
+  return i;
+}
+
+// scene.slang:318-329 (Lookup): the bilinear texel at (s', 1 - t') of the
+// binding's coordinate transform, times its scale plus its bias; or its
+// fallback. Binding words: texture, sampler (wrap_s * 4 + wrap_t), channel,
+// mode, scale[4], bias[4], fallback[4], st_s[4], st_t[4].
+__device__ __forceinline__ float4 texture_lookup(const SceneView& S, const uint32_t* b, float s, float t) {
+  const float* f = reinterpret_cast<const float*>(b);
+  if (b[3] == kFallbackLookup) return make_float4(f[12], f[13], f[14], f[15]);
+  const float ss = (f[16] * s + f[17] * t) + f[18];
+  const float tt = 1.0f - ((f[20] * s + f[21] * t) + f[22]);
+  const uint4 tex = S.textures[b[0]];
+  const int W = static_cast<int>(tex.y), H = static_cast<int>(tex.z);
+  const float x = ss * static_cast<float>(W) - 0.5f, y = tt * static_cast<float>(H) - 0.5f;
+  const float x0 = floorf(x), y0 = floorf(y);
+  const float fx = x - x0, fy = y - y0;
+  const uint32_t ws = b[1] >> 2, wt = b[1] & 3u;
+  auto texel = [&](int i, int j) {
+    bool black = false;
+    const int xi = wrap_texel(i, W, ws, black), yj = wrap_texel(j, H, wt, black);
+    return black ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : S.texels[tex.x + static_cast<uint32_t>(yj * W + xi)];
+  };
+  const int i0 = static_cast<int>(x0), j0 = static_cast<int>(y0);
+  const float4 t00 = texel(i0, j0), t10 = texel(i0 + 1, j0), t01 = texel(i0, j0 + 1), t11 = texel(i0 + 1, j0 + 1);
+  auto mix = [&](float a, float bb, float c, float d) {
+    return ((1.0f - fx) * a + fx * bb) * (1.0f - fy) + ((1.0f - fx) * c + fx * d) * fy;
+  };
+  return make_float4(mix(t00.x, t10.x, t01.x, t11.x) * f[4] + f[8], mix(t00.y, t10.y, t01.y, t11.y) * f[5] + f[9],
+                     mix(t00.z, t10.z, t01.z, t11.z) * f[6] + f[10], mix(t00.w, t10.w, t01.w, t11.w) * f[7] + f[11]);
+}
+
+// scene.slang:345-362 (MaterialInput): slot `slot`'s binding, or null for a
+// constant input.
+__device__ __forceinline__ const uint32_t* material_input(const SceneView& S, const uint32_t* m, uint32_t slot) {
+  const uint32_t bit = 1u << slot;
+  if ((m[2] & bit) == 0) return nullptr;
+  return S.bindings + 24ull * (m[1] + static_cast<uint32_t>(__popc(m[2] & (bit - 1u))));
+}
+
+// scene.slang:364-374 (HitTexcoords): the instance's corners interpolated
+// with the barycentrics, or the material's fallback coordinates.
+__device__ __forceinline__ void hit_texcoords(const SceneView& S, const uint32_t* m, uint32_t instance,
+                                              uint32_t primitive, float u, float v, float& s, float& t) {
+  const uint32_t base = S.instance_texcoords[instance];
+  if (base == 0xFFFFFFFFu) {
+    s = word_float(m, 4);
+    t = word_float(m, 5);
+    return;
+  }
+  const float2 t0 = S.texcoords[base + 3ull * primitive], t1 = S.texcoords[base + 3ull * primitive + 1],
+               t2 = S.texcoords[base + 3ull * primitive + 2];
+  s = (t0.x + u * (t1.x - t0.x)) + v * (t2.x - t0.x);
+  t = (t0.y + u * (t1.y - t0.y)) + v * (t2.y - t0.y);
+}
+
+__device__ __forceinline__ float channel_of(float4 v, uint32_t c) { return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w; }
+
+// scene.slang:572-577 (PreviewEmission) at texture coordinates (s, t).
+__device__ __forceinline__ V3 preview_emission(const SceneView& S, const uint32_t* m, float s, float t) {
+  const uint32_t* b = material_input(S, m, kEmissionSlot);
+  if (!b) return V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)};
+  const float4 e = texture_lookup(S, b, s, t);
+  return V3{maxf(e.x, 0.0f), maxf(e.y, 0.0f), maxf(e.z, 0.0f)};
+}
+
 // scene.slang:1408-1449 with EvaluatePreview (:1184-1260) for constant
 // UsdPreviewSurface inputs and MappedNormal (:539-553) for a constant
 // tangent-space normal.
+template <bool kExtras = true>  // texture lookups compiled in
 __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t tri, uint32_t instance,
                                                    float u, float v, V3 dir) {
   const Tri t = S.tris[tri];
@@ -135,7 +227,26 @@ __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t 
   sf.roughness = word_float(m, 15);
   sf.emission = V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)};
   sf.metallic = word_float(m, 19);
-  const V3 specular = V3{word_float(m, 20), word_float(m, 21), word_float(m, 22)};
+  V3 specular = V3{word_float(m, 20), word_float(m, 21), word_float(m, 22)};
+  if (kExtras && (m[2] & kPreviewLookupSlots)) {
+    // a lookup replaces its input's constant, clamped into its range
+    float s, tc;
+    hit_texcoords(S, m, instance, __float_as_uint(t.c.w), u, v, s, tc);
+    if (const uint32_t* b = material_input(S, m, kBaseColorSlot)) {
+      const float4 c = texture_lookup(S, b, s, tc);
+      sf.base_color = V3{saturate(c.x), saturate(c.y), saturate(c.z)};
+    }
+    if (const uint32_t* b = material_input(S, m, kRoughnessSlot))
+      sf.roughness = saturate(channel_of(texture_lookup(S, b, s, tc), b[2]));
+    if (const uint32_t* b = material_input(S, m, kMetallicSlot))
+      sf.metallic = saturate(channel_of(texture_lookup(S, b, s, tc), b[2]));
+    sf.emission = preview_emission(S, m, s, tc);
+    if (const uint32_t* b = material_input(S, m, kSpecularColorSlot)) {
+      const float4 c = texture_lookup(S, b, s, tc);
+      const V3 value{c.x, c.y, c.z};
+      if (finite3(value)) specular = V3{saturate(value.x), saturate(value.y), saturate(value.z)};
+    }
+  }
   const float ior = word_float(m, 23);
   const V3 tcolor = V3{word_float(m, 24), word_float(m, 25), word_float(m, 26)};
   const float transmission = word_float(m, 27);
@@ -174,6 +285,7 @@ struct LightSample {
 
 // scene.slang:632-677 (SampleTriangle): a point uniformly by area on light
 // `light` of the table, (uz, uw) its unit-square coordinates.
+template <bool kExtras = true>
 __device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S, uint32_t light, float uz, float uw,
                                                              V3 origin) {
   LightSample none{V3{0.0f, 0.0f, 1.0f}, 0.0f, V3{0.0f, 0.0f, 0.0f}, 0.0f};
@@ -189,7 +301,13 @@ __device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S,
   const V3 e = cross3(p1 - p0, p2 - p0);
   const V3 normal = normalize3(e);
   const float area = 0.5f * length3(e);
-  const V3 radiance = V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)} * 1.0f;
+  V3 emission = V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)};
+  if (kExtras && (m[2] & (1u << kEmissionSlot))) {
+    float s, tc;
+    hit_texcoords(S, m, chosen.instance, prim, bu, bv, s, tc);
+    emission = preview_emission(S, m, s, tc);
+  }
+  const V3 radiance = emission * 1.0f;
   const V3 toward = position - origin;
   const float d2 = dot3(toward, toward);
   if (!(area > 0.0f) || !(d2 > 0.0f) || max3(radiance) <= 0.0f) return none;
@@ -459,7 +577,7 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux
     if (chosen.instance & kAnalyticLight)
       return sample_analytic(S.analytic[chosen.instance & ~kAnalyticLight], chosen.pdf, uz, uw, origin);
   }
-  return sample_triangle_light(S, light, uz, uw, origin);
+  return sample_triangle_light<kAnalytic>(S, light, uz, uw, origin);
 }
 
 // scene.slang:939-971 (ReconstructReservoirLight): a light-table entry and
@@ -468,7 +586,7 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux
 template <bool kAnalytic = true>
 __device__ __forceinline__ LightSample reconstruct_reservoir_light(const SceneView& S, uint32_t index, float u,
                                                                    float v, V3 origin) {
-  if constexpr (!kAnalytic) return sample_triangle_light(S, index, u, v, origin);
+  if constexpr (!kAnalytic) return sample_triangle_light<false>(S, index, u, v, origin);
   const LightRecord chosen = S.lights[index];
   if (chosen.instance == kEnvironmentLight) return sample_environment(S, chosen.pdf, u, v);
   if ((chosen.instance & kAnalyticLight) == 0) return sample_triangle_light(S, index, u, v, origin);
@@ -850,7 +968,7 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
   out.continues = false;
   out.shadow = false;
   const bool sample_lights = F.light_sampling != 0;
-  const Surface sf = reconstruct_hit(S, tri, instance, u, v, p.d);
+  const Surface sf = reconstruct_hit<kAnalytic>(S, tri, instance, u, v, p.d);
   if (keep) *keep = sf;
   add_emission(S, F, p, sf, instance, primitive);
   // the analytic lights the continuation ray crossed before the hit

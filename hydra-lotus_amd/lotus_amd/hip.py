@@ -69,7 +69,44 @@ class ScenePlan(C.Structure):
         ("light_count", C.c_uint32),
         ("lights", C.c_void_p),
         ("environment_map", C.c_void_p),
+        ("texture_count", C.c_uint32),
+        ("textures", C.c_void_p),
+        ("binding_count", C.c_uint32),
+        ("bindings", C.c_void_p),
+        ("texcoords", C.c_void_p),
+        ("geometry_texcoord_offset", C.c_void_p),
     ]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32), ("texels", C.c_void_p)]
+
+
+def texture_structs(textures):
+    """A C array of texture records for packed textures (None when empty)."""
+    if not textures:
+        return None
+    arr = (Texture * len(textures))()
+    for i, t in enumerate(textures):
+        arr[i].height, arr[i].width = t["texels"].shape[:2]
+        arr[i].format = t["format"]
+        arr[i].texels = t["texels"].ctypes.data
+    return arr
+
+
+def set_texture_fields(desc, p):
+    """Fills a plan's (or the oracle's) texture lookup fields; returns what
+    must stay alive while it is used."""
+    tex = texture_structs(p.get("textures"))
+    desc.texture_count = len(p.get("textures") or [])
+    desc.textures = C.addressof(tex) if tex is not None else None
+    b = p.get("bindings")
+    desc.binding_count = 0 if b is None else len(b) // 24
+    desc.bindings = b.ctypes.data if desc.binding_count else None
+    off = p.get("geometry_texcoord_offset")
+    desc.texcoords = p["texcoords"].ctypes.data if off is not None else None
+    desc.geometry_texcoord_offset = off.ctypes.data if off is not None else None
+    return tex
 
 
 class EnvironmentMap(C.Structure):
@@ -436,6 +473,7 @@ class Renderer:
         plan.lights = p["lights"].ctypes.data if len(p["lights"]) else None
         env = environment_struct(p.get("environment_map"))
         plan.environment_map = C.addressof(env) if env is not None else None
+        keep = set_texture_fields(plan, p)  # noqa: F841 (alive through the call)
         ev = SceneEvidence()
         buf = C.create_string_buffer(512)
         s = lib().lotus_hip_scene_apply(self.handle, C.byref(plan), C.byref(ev), buf, 512)

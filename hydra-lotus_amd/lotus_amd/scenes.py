@@ -85,6 +85,35 @@ def analytic_light(shape, transform=None, radiance=(1.0, 1.0, 1.0), radius=0.5, 
     return rec
 
 
+# Material texture lookups (gpu_scene.hpp:56-84; material.hpp:15-50).
+TEXTURE_RGBA8_UNORM, TEXTURE_RGBA8_SRGB, TEXTURE_RGBA32F = range(3)
+WRAP_BLACK, WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = range(4)
+SLOT_BASE_COLOR, SLOT_ROUGHNESS, SLOT_METALLIC, SLOT_EMISSION, SLOT_SPECULAR_COLOR = 2, 3, 4, 5, 6
+
+
+def texture_binding(texture: int, wrap=(WRAP_REPEAT, WRAP_REPEAT), channel: int = 0, scale=(1, 1, 1, 1),
+                    bias=(0, 0, 0, 0), fallback=(0, 0, 0, 1), st_s=(1.0, 0.0, 0.0), st_t=(0.0, 1.0, 0.0),
+                    fallback_only: bool = False) -> np.ndarray:
+    """One GpuTextureInputRecord (24 words): texture, sampler (wrap_s * 4 +
+    wrap_t), channel, mode (1 lookup, 2 fallback), scale, bias, fallback and
+    the coordinate rows s' = st_s . (s, t, 1), t' = st_t . (s, t, 1)."""
+    w = np.zeros(24, np.uint32)
+    f = w.view(F32)
+    w[0], w[1], w[2], w[3] = texture, wrap[0] * 4 + wrap[1], channel, 2 if fallback_only else 1
+    f[4:8], f[8:12], f[12:16] = scale, bias, fallback
+    f[16:19], f[20:23] = st_s, st_t
+    return w
+
+
+def with_lookups(record: np.ndarray, first_binding: int, slots) -> np.ndarray:
+    """A material record whose `slots` read bindings first_binding,
+    first_binding + 1, ... in slot order (words 1 and 2)."""
+    out = np.array(record, np.uint32)
+    out[1] = first_binding
+    out[2] = sum(1 << s for s in slots)
+    return out
+
+
 @dataclasses.dataclass
 class Scene:
     """Object-space geometries placed by instances; materials as records;
@@ -98,6 +127,12 @@ class Scene:
     # a latitude-longitude image: dict(texels=float32 [height, width, 4] rows
     # from the top, scale=(r, g, b), transform=16 floats), or None
     environment_map: Optional[dict] = None
+    # texture lookups: textures as dict(texels, format) with texels
+    # [height, width, 4] (uint8 or float32), binding records, and per
+    # geometry its per-corner texture coordinates [triangles * 3, 2] or None
+    textures: List[dict] = dataclasses.field(default_factory=list)
+    bindings: List[np.ndarray] = dataclasses.field(default_factory=list)
+    texcoords: Dict[int, np.ndarray] = dataclasses.field(default_factory=dict)
     view: np.ndarray = dataclasses.field(default_factory=identity)
     projection: np.ndarray = dataclasses.field(default_factory=identity)
     name: str = "scene"
@@ -111,6 +146,23 @@ class Scene:
         self.instances.append((len(self.geometries) - 1, t, material))
         self._packed = None
         return len(self.instances) - 1
+
+    def add_texture(self, texels: np.ndarray, format: int) -> int:
+        self.textures.append(dict(texels=np.ascontiguousarray(texels), format=format))
+        self._packed = None
+        return len(self.textures) - 1
+
+    def add_binding(self, record: np.ndarray) -> int:
+        self.bindings.append(np.array(record, np.uint32))
+        self._packed = None
+        return len(self.bindings) - 1
+
+    def set_texcoords(self, geometry: int, corners) -> None:
+        """Per-corner texture coordinates of a geometry, 3 per triangle."""
+        c = np.ascontiguousarray(corners, F32).reshape(-1, 2)
+        assert c.shape[0] == 3 * self.geometries[geometry][1].shape[0]
+        self.texcoords[geometry] = c
+        self._packed = None
 
     def add_light(self, record: np.ndarray) -> int:
         self.lights.append(np.array(record, LIGHT_DTYPE))
@@ -128,6 +180,8 @@ class Scene:
                      instances=[(g, t.copy(), m) for g, t, m in self.instances],
                      materials=[m.copy() for m in self.materials], environment=tuple(self.environment),
                      lights=[l.copy() for l in self.lights],
+                     textures=[dict(t) for t in self.textures], bindings=[b.copy() for b in self.bindings],
+                     texcoords={g: c.copy() for g, c in self.texcoords.items()},
                      environment_map=None if self.environment_map is None else
                      {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
                       for k, v in self.environment_map.items()},
@@ -149,6 +203,20 @@ class Scene:
     def set_material(self, slot: int, record) -> None:
         self.materials[slot] = np.ascontiguousarray(record, np.uint32)
         self._packed = None
+
+    def _packed_texcoords(self) -> dict:
+        if not self.texcoords:
+            return dict(texcoords=np.zeros(2, F32), geometry_texcoord_offset=None)
+        offset, parts, n = [], [], 0
+        for g in range(len(self.geometries)):
+            if g in self.texcoords:
+                offset.append(n)
+                parts.append(self.texcoords[g])
+                n += self.texcoords[g].shape[0]
+            else:
+                offset.append(0xFFFFFFFF)
+        return dict(texcoords=np.ascontiguousarray(np.concatenate(parts).astype(F32)),
+                    geometry_texcoord_offset=np.array(offset, np.uint32))
 
     @property
     def world_to_clip(self) -> np.ndarray:
@@ -192,6 +260,9 @@ class Scene:
             materials=np.ascontiguousarray(np.stack(self.materials).astype(np.uint32)),
             environment=np.array(self.environment, F32),
             lights=np.ascontiguousarray(np.array(self.lights, LIGHT_DTYPE).reshape(-1)),
+            textures=list(self.textures),
+            bindings=np.ascontiguousarray(np.array(self.bindings, np.uint32).reshape(-1)),
+            **self._packed_texcoords(),
             environment_map=None if self.environment_map is None else dict(
                 texels=np.ascontiguousarray(self.environment_map["texels"], F32),
                 scale=np.asarray(self.environment_map.get("scale", (1.0, 1.0, 1.0)), F32),

@@ -76,6 +76,17 @@ typedef struct lotus_hip_environment_map {
   float world_from_environment[16]; /* column-major */
 } lotus_hip_environment_map;
 
+/* A texture of the material bindings (Texture, include/lotus/material.hpp:
+ * 15-41): texels row-major from the image's top; 8-bit sRGB channels are
+ * decoded to linear values (alpha stays linear). */
+#define LOTUS_HIP_TEXTURE_RGBA8_UNORM 0u
+#define LOTUS_HIP_TEXTURE_RGBA8_SRGB 1u
+#define LOTUS_HIP_TEXTURE_RGBA32F 2u
+typedef struct lotus_hip_texture {
+  uint32_t width, height, format;
+  const void* texels; /* 4 bytes or 4 floats per texel */
+} lotus_hip_texture;
+
 typedef struct lotus_hip_scene_plan {
   uint32_t geometry_count;
   const float* positions;                   /* xyz per vertex, all geometries */
@@ -95,6 +106,19 @@ typedef struct lotus_hip_scene_plan {
   uint32_t light_count;                     /* analytic lights, in the light table after the triangles */
   const lotus_hip_light* lights;
   const lotus_hip_environment_map* environment_map; /* NULL: none; added to the constant environment */
+  /* Texture lookups of the materials (gpu_scene.hpp:56-84): the textures,
+   * the binding table of GpuTextureInputRecord words (24 per binding; a
+   * material record's words 1-3 give its first binding and its textured
+   * slots), and per-corner texture coordinates (2 floats per triangle
+   * corner) with each geometry's first corner, 0xFFFFFFFF for a geometry
+   * without them. Base colour, roughness, metallic, emission and specular
+   * colour take lookups; normal and opacity lookups are rejected. */
+  uint32_t texture_count;
+  const lotus_hip_texture* textures;
+  uint32_t binding_count;
+  const uint32_t* bindings;
+  const float* texcoords;
+  const uint32_t* geometry_texcoord_offset; /* per geometry; NULL: no geometry has texture coordinates */
 } lotus_hip_scene_plan;
 
 /* What a lotus_hip_scene_apply did, the counterpart of GpuScene::Apply's

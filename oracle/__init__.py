@@ -52,6 +52,12 @@ class SceneDesc(C.Structure):
         ("light_count", C.c_uint32),
         ("lights", C.c_void_p),
         ("environment_map", C.c_void_p),
+        ("texture_count", C.c_uint32),
+        ("textures", C.c_void_p),
+        ("binding_count", C.c_uint32),
+        ("bindings", C.c_void_p),
+        ("texcoords", C.c_void_p),
+        ("geometry_texcoord_offset", C.c_void_p),
     ]
 
 
@@ -103,6 +109,8 @@ def lib():
         L.oracle_reservoir_merge.argtypes = [_u32p, _u32p, C.c_float, C.c_float, C.c_uint32, C.c_float]
         L.oracle_reservoir_normalization.restype = C.c_float
         L.oracle_reservoir_normalization.argtypes = [_u32p]
+        L.oracle_material_inputs.restype = None
+        L.oracle_material_inputs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float, _f32p]
         L.oracle_reuse_normalization.restype = C.c_float
         L.oracle_reuse_normalization.argtypes = [_u32p, C.c_uint32]
         L.oracle_restir_initial.restype = C.c_int
@@ -190,6 +198,8 @@ class OracleScene:
         from lotus_amd.hip import environment_struct
         self._env = environment_struct(p.get("environment_map"))
         d.environment_map = C.addressof(self._env) if self._env is not None else None
+        from lotus_amd.hip import set_texture_fields
+        self._tex = set_texture_fields(d, p)
         err = C.create_string_buffer(256)
         self.handle = lib().oracle_scene_create(C.byref(d), err, 256)
         if not self.handle:
@@ -241,6 +251,12 @@ class OracleScene:
         lib().oracle_render(self.handle, C.byref(frame), ptr(acc, C.c_double),
                             ptr(counts, C.c_uint32), ptr(shadow, C.c_uint64), threads)
         return acc, counts, shadow
+
+    def material_inputs(self, instance: int, primitive: int, u: float, v: float) -> np.ndarray:
+        """Base colour, roughness, emission, metallic, specular colour at a hit."""
+        out = np.zeros(12, np.float32)
+        lib().oracle_material_inputs(self.handle, instance, primitive, u, v, ptr(out, C.c_float))
+        return out
 
     def restir_initial(self, frame: Frame, x: int, y: int, sample_index: int):
         """The 12-word initial reservoir record of pixel (x, y), or None on a

@@ -496,6 +496,8 @@ struct Material {
   F3 transmission_color;
   float transmission;
   F3 tangent_normal;
+  uint32_t lookup_base, textured;  // the binding table's first entry, the slots with lookups
+  float fallback_st[2];            // texture coordinates without a set
 };
 inline float WordFloat(const uint32_t* w, int i) {
   float f;
@@ -514,6 +516,10 @@ Material DecodeMaterial(const uint32_t* w) {
   m.ior = WordFloat(w, 23);
   m.transmission_color = F3{WordFloat(w, 24), WordFloat(w, 25), WordFloat(w, 26)};
   m.transmission = WordFloat(w, 27);
+  m.lookup_base = w[1];
+  m.textured = w[2];
+  m.fallback_st[0] = WordFloat(w, 4);
+  m.fallback_st[1] = WordFloat(w, 5);
   return m;
 }
 constexpr uint32_t kSpecularWorkflow = 1u;
@@ -569,7 +575,19 @@ struct Environment {
   float selection = 0.0f;
 };
 
+// Material texture lookups: per texture {first texel, width, height}, the
+// texels as linear RGBA, the binding table (24 words each), per-corner
+// texture coordinates and each instance's first corner (0xFFFFFFFF: none).
+struct Textures {
+  std::vector<uint32_t> records;  // 3 per texture
+  std::vector<float> texels;      // 4 per texel
+  std::vector<uint32_t> bindings;
+  std::vector<float> texcoords;
+  std::vector<uint32_t> instance_texcoords;
+};
+
 struct Scene {
+  Textures tex;
   Environment env;
   std::vector<Analytic> analytic;
   std::vector<LightNode> light_nodes;
@@ -828,8 +846,17 @@ bool Occluded(const Scene& s, const Ray& r, bool brute) {
 
 // ------------------------------------------------------------ light table --
 // gpu_scene.cpp:144-203, 512-609 (emissive triangles only).
-double EmissionEstimate(const Material& m) {
-  const double mean = (double(m.emission.x) + double(m.emission.y) + double(m.emission.z)) / 3.0;
+// gpu_scene.cpp:144-179 (EmissionEstimate): a textured emission's estimate
+// is its lookup's max(scale, 0) + max(bias, 0), or its fallback.
+double EmissionEstimate(const Material& m, const std::vector<uint32_t>& bindings) {
+  double e[3] = {m.emission.x, m.emission.y, m.emission.z};
+  if ((m.textured & (1u << 5)) != 0) {
+    const uint32_t* b = bindings.data() + 24 * size_t(m.lookup_base + __builtin_popcount(m.textured & 31u));
+    for (int c = 0; c < 3; ++c)
+      e[c] = b[3] == 2u ? std::max(0.0f, WordFloat(b, 12 + c))
+                        : double(std::max(0.0f, WordFloat(b, 4 + c))) + std::max(0.0f, WordFloat(b, 8 + c));
+  }
+  const double mean = (e[0] + e[1] + e[2]) / 3.0;
   return std::isfinite(mean) && mean > 0.0 ? mean : 0.0;
 }
 double WorldTriangleArea(const F3* object, const float* t) {
@@ -937,6 +964,86 @@ struct Surface {
 
 // scene.slang:1408-1449 with EvaluatePreview (:1184-1260) for a constant
 // UsdPreviewSurface and MappedNormal (:539-553) for a constant tangent normal.
+// ------------------------------------------------------ texture lookups
+// A texel index under TextureWrap (material.hpp:43-50: black, clamp,
+// repeat, mirror); `black` is set outside a black border.
+int WrapTexel(int i, int n, uint32_t mode, bool& black) {
+  if (mode == 2) {
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+  }
+  if (mode == 3) {
+    int r = i % (2 * n);
+    if (r < 0) r += 2 * n;
+    return r < n ? r : 2 * n - 1 - r;
+  }
+  if (i < 0 || i >= n) {
+    if (mode == 0) black = true;
+    return i < 0 ? 0 : n - 1;
+  }
+  return i;
+}
+
+// scene.slang:318-329 (Lookup): bilinear at (s', 1 - t'), times scale plus
+// bias; or the fallback.
+void Lookup(const Textures& T, const uint32_t* b, float s, float t, float out[4]) {
+  if (b[3] == 2u) {
+    for (int c = 0; c < 4; ++c) out[c] = WordFloat(b, 12 + c);
+    return;
+  }
+  const float ss = (WordFloat(b, 16) * s + WordFloat(b, 17) * t) + WordFloat(b, 18);
+  const float tt = 1.0f - ((WordFloat(b, 20) * s + WordFloat(b, 21) * t) + WordFloat(b, 22));
+  const uint32_t* rec = T.records.data() + 3 * size_t(b[0]);
+  const int W = static_cast<int>(rec[1]), H = static_cast<int>(rec[2]);
+  const float x = ss * static_cast<float>(W) - 0.5f, y = tt * static_cast<float>(H) - 0.5f;
+  const float x0 = std::floor(x), y0 = std::floor(y);
+  const float fx = x - x0, fy = y - y0;
+  const uint32_t ws = b[1] >> 2, wt = b[1] & 3u;
+  const int i0 = static_cast<int>(x0), j0 = static_cast<int>(y0);
+  auto texel = [&](int i, int j, int c) {
+    bool black = false;
+    const int xi = WrapTexel(i, W, ws, black), yj = WrapTexel(j, H, wt, black);
+    return black ? 0.0f : T.texels[4 * (size_t(rec[0]) + size_t(yj) * W + xi) + c];
+  };
+  for (int c = 0; c < 4; ++c) {
+    const float v = ((1.0f - fx) * texel(i0, j0, c) + fx * texel(i0 + 1, j0, c)) * (1.0f - fy) +
+                    ((1.0f - fx) * texel(i0, j0 + 1, c) + fx * texel(i0 + 1, j0 + 1, c)) * fy;
+    out[c] = v * WordFloat(b, 4 + c) + WordFloat(b, 8 + c);
+  }
+}
+
+// scene.slang:345-362 (MaterialInput): the slot's binding or null.
+const uint32_t* MaterialInput(const Textures& T, const Material& m, uint32_t slot) {
+  const uint32_t bit = 1u << slot;
+  if ((m.textured & bit) == 0) return nullptr;
+  return T.bindings.data() + 24 * size_t(m.lookup_base + __builtin_popcount(m.textured & (bit - 1u)));
+}
+
+// scene.slang:364-374 (HitTexcoords).
+void HitTexcoords(const Scene& s, const Material& m, uint32_t instance, uint32_t primitive, float u, float v,
+                  float& st_s, float& st_t) {
+  const uint32_t base = s.tex.instance_texcoords.empty() ? 0xFFFFFFFFu : s.tex.instance_texcoords[instance];
+  if (base == 0xFFFFFFFFu) {
+    st_s = m.fallback_st[0];
+    st_t = m.fallback_st[1];
+    return;
+  }
+  const float* t0 = &s.tex.texcoords[2 * (size_t(base) + 3 * size_t(primitive))];
+  const float* t1 = t0 + 2;
+  const float* t2 = t0 + 4;
+  st_s = (t0[0] + u * (t1[0] - t0[0])) + v * (t2[0] - t0[0]);
+  st_t = (t0[1] + u * (t1[1] - t0[1])) + v * (t2[1] - t0[1]);
+}
+
+// scene.slang:572-577 (PreviewEmission).
+F3 PreviewEmission(const Scene& s, const Material& m, float st_s, float st_t) {
+  const uint32_t* b = MaterialInput(s.tex, m, 5);
+  if (!b) return m.emission;
+  float e[4];
+  Lookup(s.tex, b, st_s, st_t, e);
+  return F3{std::max(e[0], 0.0f), std::max(e[1], 0.0f), std::max(e[2], 0.0f)};
+}
+
 Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
   const uint32_t g = hit.global;
   const F3 p0 = s.v0[g], p1 = s.v1[g], p2 = s.v2[g];
@@ -967,11 +1074,37 @@ Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
     const F3 outward = Dot(c0, Cross(c1, c2)) < 0.0f ? -normal : normal;
     entering = Dot(outward, dir) < 0.0f;
   }
-  sf.bsdf = MakePreviewClosure(m.base_color, m.roughness, m.metallic, m.ior,
-                               (m.flags & kSpecularWorkflow) != 0, m.specular,
+  // EvaluatePreview (scene.slang:1197-1221): a lookup replaces its input's
+  // constant, clamped into its range
+  F3 base_color = m.base_color, specular = m.specular, emission = m.emission;
+  float roughness = m.roughness, metallic = m.metallic;
+  if ((m.textured & 0x7Cu) != 0) {
+    float st_s, st_t, v[4];
+    HitTexcoords(s, m, hit.instance, hit.primitive, hit.u, hit.v, st_s, st_t);
+    if (const uint32_t* b = MaterialInput(s.tex, m, 2)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      base_color = F3{Saturate(v[0]), Saturate(v[1]), Saturate(v[2])};
+    }
+    if (const uint32_t* b = MaterialInput(s.tex, m, 3)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      roughness = Saturate(v[b[2]]);
+    }
+    if (const uint32_t* b = MaterialInput(s.tex, m, 4)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      metallic = Saturate(v[b[2]]);
+    }
+    emission = PreviewEmission(s, m, st_s, st_t);
+    if (const uint32_t* b = MaterialInput(s.tex, m, 6)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      const F3 value{v[0], v[1], v[2]};
+      if (IsFinite(value)) specular = F3{Saturate(v[0]), Saturate(v[1]), Saturate(v[2])};
+    }
+  }
+  sf.bsdf = MakePreviewClosure(base_color, roughness, metallic, m.ior,
+                               (m.flags & kSpecularWorkflow) != 0, specular,
                                m.transmission, m.transmission_color,
                                (m.flags & kThinWalled) != 0, entering);
-  sf.emission = m.emission;
+  sf.emission = emission;
   return sf;
 }
 
@@ -997,7 +1130,13 @@ LightSample SampleTriangleLight(const Scene& s, uint32_t light, float uz, float 
   const F3 e = Cross(p1 - p0, p2 - p0);
   const F3 normal = Normalize(e);
   const float area = 0.5f * Length(e);
-  const F3 radiance = m.emission * 1.0f;
+  F3 emission = m.emission;
+  if ((m.textured & (1u << 5)) != 0) {
+    float st_s, st_t;
+    HitTexcoords(s, m, chosen.instance, prim, bu, bv, st_s, st_t);
+    emission = PreviewEmission(s, m, st_s, st_t);
+  }
+  const F3 radiance = emission * 1.0f;
   const F3 toward = position - origin;
   const float d2 = Dot(toward, toward);
   if (!(area > 0.0f) || !(d2 > 0.0f) || MaxComponent(radiance) <= 0.0f) return none;
@@ -2023,11 +2162,39 @@ oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
       s.tri_primitive.push_back(k);
     }
   }
+  // texture lookups: texels decoded to linear RGBA as the samplers decode
+  // them (sRGB colour channels, gpu_scene.cpp:612-629), bindings, corners
+  for (uint32_t k = 0; k < d->texture_count; ++k) {
+    const oracle_texture& x = d->textures[k];
+    s.tex.records.insert(s.tex.records.end(), {static_cast<uint32_t>(s.tex.texels.size() / 4), x.width, x.height});
+    const size_t n = size_t(x.width) * x.height;
+    if (x.format == 2) {
+      const float* f = static_cast<const float*>(x.texels);
+      s.tex.texels.insert(s.tex.texels.end(), f, f + 4 * n);
+      continue;
+    }
+    const uint8_t* b = static_cast<const uint8_t*>(x.texels);
+    for (size_t i = 0; i < 4 * n; ++i) {
+      double v = b[i] / 255.0;
+      if (x.format == 1 && i % 4 != 3) v = v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4);
+      s.tex.texels.push_back(static_cast<float>(v));
+    }
+  }
+  if (d->binding_count) s.tex.bindings.assign(d->bindings, d->bindings + 24 * size_t(d->binding_count));
+  if (d->geometry_texcoord_offset) {
+    uint64_t corners = 0;
+    for (uint32_t g = 0; g < d->geometry_count; ++g)
+      if (d->geometry_texcoord_offset[g] != 0xFFFFFFFFu)
+        corners = std::max<uint64_t>(corners, uint64_t(d->geometry_texcoord_offset[g]) + 3ull * d->geometry_triangle_count[g]);
+    s.tex.texcoords.assign(d->texcoords, d->texcoords + 2 * corners);
+    for (uint32_t i = 0; i < d->instance_count; ++i)
+      s.tex.instance_texcoords.push_back(d->geometry_texcoord_offset[d->instance_geometry[i]]);
+  }
   // Light table (gpu_scene.cpp:512-609), emissive triangles only.
   s.light_offsets.assign(d->instance_count, kNoLights);
   std::vector<double> weights;
   for (uint32_t i = 0; i < d->instance_count; ++i) {
-    const double emission = EmissionEstimate(s.materials[s.instance_material[i]]);
+    const double emission = EmissionEstimate(s.materials[s.instance_material[i]], s.tex.bindings);
     const uint32_t geo = d->instance_geometry[i];
     if (emission <= 0.0 || d->geometry_triangle_count[geo] == 0) continue;
     s.light_offsets[i] = static_cast<uint32_t>(s.lights.size());
@@ -2311,6 +2478,41 @@ void oracle_camera_ray(const float* clip_to_world, float nx, float ny, float* ra
 int oracle_camera_constants(const float* gl, float* vk, float* clip_to_world) {
   VulkanWorldToClip(gl, vk);
   return Invert(vk, clip_to_world) ? 1 : 0;
+}
+
+void oracle_material_inputs(const oracle_scene* scene, uint32_t instance, uint32_t primitive, float u, float v,
+                            float* out) {
+  // EvaluatePreview's inputs at a hit (scene.slang:1184-1221): base colour,
+  // roughness, emission, metallic, specular colour (out[0..11])
+  const Scene& s = scene->s;
+  Hit hit{instance, primitive, s.instance_base[instance] + primitive, u, v, 1.0f};
+  const Material& m = s.materials[s.instance_material[instance]];
+  F3 base_color = m.base_color, specular = m.specular, emission = m.emission;
+  float roughness = m.roughness, metallic = m.metallic;
+  if ((m.textured & 0x7Cu) != 0) {
+    float st_s, st_t, w[4];
+    HitTexcoords(s, m, hit.instance, hit.primitive, hit.u, hit.v, st_s, st_t);
+    if (const uint32_t* b = MaterialInput(s.tex, m, 2)) {
+      Lookup(s.tex, b, st_s, st_t, w);
+      base_color = F3{Saturate(w[0]), Saturate(w[1]), Saturate(w[2])};
+    }
+    if (const uint32_t* b = MaterialInput(s.tex, m, 3)) {
+      Lookup(s.tex, b, st_s, st_t, w);
+      roughness = Saturate(w[b[2]]);
+    }
+    if (const uint32_t* b = MaterialInput(s.tex, m, 4)) {
+      Lookup(s.tex, b, st_s, st_t, w);
+      metallic = Saturate(w[b[2]]);
+    }
+    emission = PreviewEmission(s, m, st_s, st_t);
+    if (const uint32_t* b = MaterialInput(s.tex, m, 6)) {
+      Lookup(s.tex, b, st_s, st_t, w);
+      if (IsFinite(F3{w[0], w[1], w[2]})) specular = F3{Saturate(w[0]), Saturate(w[1]), Saturate(w[2])};
+    }
+  }
+  const float vals[12] = {base_color.x, base_color.y, base_color.z, roughness, emission.x, emission.y,
+                          emission.z, metallic, specular.x, specular.y, specular.z, 0.0f};
+  std::memcpy(out, vals, sizeof vals);
 }
 
 uint32_t oracle_pcg_hash(uint32_t value) { return PcgHash(value); }

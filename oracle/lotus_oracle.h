@@ -40,6 +40,13 @@ typedef struct oracle_environment_map {
   float world_from_environment[16];
 } oracle_environment_map;
 
+/* A material texture (Texture, include/lotus/material.hpp:15-41): format 0
+ * RGBA8 linear, 1 RGBA8 sRGB, 2 RGBA32F; rows from the top. */
+typedef struct oracle_texture {
+  uint32_t width, height, format;
+  const void* texels;
+} oracle_texture;
+
 typedef struct oracle_scene_desc {
   uint32_t geometry_count;
   const float* positions;           /* xyz per vertex, all geometries */
@@ -58,6 +65,12 @@ typedef struct oracle_scene_desc {
   uint32_t light_count;
   const oracle_light* lights;
   const oracle_environment_map* environment_map; /* NULL: none */
+  uint32_t texture_count;
+  const oracle_texture* textures;
+  uint32_t binding_count;
+  const uint32_t* bindings;                 /* 24 words per GpuTextureInputRecord */
+  const float* texcoords;                   /* 2 per triangle corner */
+  const uint32_t* geometry_texcoord_offset; /* per geometry: first corner or 0xFFFFFFFF; NULL: none */
 } oracle_scene_desc;
 
 typedef struct oracle_scene oracle_scene;
@@ -135,6 +148,10 @@ int oracle_camera_constants(const float* world_to_clip_gl,
                             float* world_to_clip_vk, float* clip_to_world);
 
 /* RNG probes (backend/vulkan/shaders/sampling/sampling.slang:9-25). */
+/* EvaluatePreview's inputs at a hit: base colour, roughness, emission,
+ * metallic, specular colour, 0 (12 floats), texture lookups applied. */
+void oracle_material_inputs(const oracle_scene* scene, uint32_t instance, uint32_t primitive, float u, float v,
+                            float* out);
 uint32_t oracle_pcg_hash(uint32_t value);
 uint32_t oracle_seed_random(uint32_t pixel, uint32_t sample);
 void oracle_random_sequence(uint32_t state, uint32_t count, float* out);
