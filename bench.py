@@ -254,7 +254,34 @@ def lights_run(r, scene, width, height, depth, frames):
         ms += out["frame_ms"]
     result["sky_image"] = {"workload": "the heightfield under a 1024x512 sky image with a sun, NEE+MIS",
                            "mrays_per_s_gpu": round(rays / ms / 1e3, 1), "gpu_ms_per_frame": round(ms / frames, 3)}
-    r.apply(scene)  # back to the triangle lights alone (a table rewrite)
+    # the same heightfield with every tile's base colour and roughness read
+    # from one 1024x1024 sRGB texture through per-corner coordinates
+    tex = scene.copy()
+    texels = np.random.default_rng(12).integers(0, 256, (1024, 1024, 4), dtype=np.uint8)
+    t = tex.add_texture(texels, scenes.TEXTURE_RGBA8_SRGB)
+    first = tex.add_binding(scenes.texture_binding(t, st_s=(4.0, 0.0, 0.0), st_t=(0.0, 4.0, 0.0)))
+    tex.add_binding(scenes.texture_binding(t, channel=1, scale=(0.8,) * 4, bias=(0.1,) * 4))
+    tiled = set()
+    for g, _, m in tex.instances:
+        if not tex.materials[m][2] and tex.geometries[g][1].shape[0] > 1:  # the tiles, not the emitters
+            tex.materials[m] = scenes.with_lookups(tex.materials[m], first,
+                                                   [scenes.SLOT_BASE_COLOR, scenes.SLOT_ROUGHNESS])
+            tiled.add(g)
+    for g in tiled:
+        pos, tri = tex.geometries[g]
+        tex.set_texcoords(g, pos[tri.reshape(-1)][:, [0, 2]] * 0.5 + 0.5)
+    tex._packed = None
+    r.apply(tex)
+    r.render(tex.world_to_clip, width, height, **kw)
+    rays, ms = 0, 0.0
+    for _ in range(frames):
+        out = r.render(tex.world_to_clip, width, height, **kw)
+        rays += out["camera_rays"] + out["extension_rays"] + out["shadow_rays"]
+        ms += out["frame_ms"]
+    result["textured"] = {"workload": "the heightfield with base colour and roughness looked up in a 1024x1024 "
+                                      "sRGB texture (per-corner coordinates), NEE+MIS",
+                          "mrays_per_s_gpu": round(rays / ms / 1e3, 1), "gpu_ms_per_frame": round(ms / frames, 3)}
+    r.apply(scene)  # back to the untextured scene (a rebuild)
     return result
 
 
