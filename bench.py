@@ -285,6 +285,45 @@ def lights_run(r, scene, width, height, depth, frames):
     return result
 
 
+def render_buffer_run(r, width, height, calls=200):
+    """The colour product written into a host's UNorm8 render buffer (rows
+    flipped, RGBA32F to UNorm8) and the depth product into a Float32 one,
+    each call synchronised; algorithmic bytes: 16 + 4 and 4 + 4 per pixel."""
+    import ctypes
+    rt = ctypes.CDLL("libamdhip64.so")
+    px = width * height
+    ptrs = []
+
+    def alloc(nbytes, fill):
+        p = ctypes.c_void_p()
+        if rt.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0 or \
+                rt.hipMemset(p, fill, ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc / hipMemset failed")
+        ptrs.append(p)
+        return p.value
+
+    # 0x3e bytes: every float 0.186 (inside [0, 1], so the conversion rounds)
+    colour, unorm = alloc(px * 16, 0x3E), alloc(px * 4, 0)
+    depth, dbuf = alloc(px * 4, 0x3E), alloc(px * 4, 0)
+    res = {}
+    try:
+        for name, args, nbytes in (("colour_rgba32f_to_unorm8", (colour, hip.BUFFER_FLOAT32X4, unorm,
+                                                                 hip.BUFFER_UNORM8X4), 20),
+                                   ("depth_float32", (depth, hip.BUFFER_FLOAT32, dbuf, hip.BUFFER_FLOAT32), 8)):
+            src, sf, dst, df = args
+            r.write_render_buffer(src, sf, width, height, dst, df)
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                r.write_render_buffer(src, sf, width, height, dst, df)
+            us = (time.perf_counter() - t0) / calls * 1e6
+            res[name] = {"us_per_call": round(us, 2), "gb_per_s": round(nbytes * px / us / 1e3, 1)}
+    finally:
+        for p in ptrs:
+            rt.hipFree(p)
+    return {"workload": f"{width}x{height} products into bottom-up render buffers, host-synchronised calls",
+            **res}
+
+
 def batch_run(views, frames):
     """64 seeded views of 64^2..512^2 over their own icosphere rigs (1k..200k
     triangles, depth 2..12): each view autotuned, then frames rendered view
@@ -479,6 +518,8 @@ def main() -> int:
     if args.lights_frames > 0:
         log("analytic lights ...")
         line["analytic_lights"] = lights_run(r, scene, args.width, args.height, args.depth, args.lights_frames)
+    if args.lights_frames > 0:
+        line["render_buffer_writes"] = render_buffer_run(r, args.width, args.height)
     if args.race_frames > 0:
         cache_cfg = dict(log2_table=19, train_paths=65536, batch_capacity=65536, train_steps=4, learning_rate=1e-2)
         log("equal-relMSE race: heightfield ...")

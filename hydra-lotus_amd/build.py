@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblotus_hip.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["bvh_build.hip", "kernels.hip", "nrc.hip", "temporal.hip", "lotus_hip.hip"]
+SOURCES = ["bvh_build.hip", "kernels.hip", "nrc.hip", "temporal.hip", "products.hip", "lotus_hip.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/lotus_hip.h"
+#include "products.hpp"
 #include "temporal.hpp"
 
 using namespace lotus;
@@ -2510,6 +2511,73 @@ int lotus_hip_svgf(lotus_hip_renderer* r, const void* surface, void* radiance, u
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "filter", e));
+    return LOTUS_HIP_FAIL;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_write_render_buffer(lotus_hip_renderer* r, const void* product, uint32_t product_format,
+                                  uint32_t source_width, uint32_t source_height, void* buffer, uint32_t buffer_format,
+                                  uint32_t buffer_width, uint32_t buffer_height, char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  const bool colour_pair = (product_format == LOTUS_HIP_BUFFER_FLOAT32X4 && buffer_format == LOTUS_HIP_BUFFER_UNORM8X4) ||
+                           (product_format == LOTUS_HIP_BUFFER_UNORM8X4 && buffer_format == LOTUS_HIP_BUFFER_FLOAT32X4);
+  if (buffer_pixel_bytes(product_format) == 0 || buffer_pixel_bytes(buffer_format) == 0 ||
+      (product_format != buffer_format && !colour_pair)) {
+    set_detail(detail, detail_size, "the product's format does not convert to the render buffer's");
+    return LOTUS_HIP_FAIL;
+  }
+  if (source_width != buffer_width || source_height != buffer_height) {
+    set_detail(detail, detail_size, "the product's size differs from the render buffer's");
+    return LOTUS_HIP_FAIL;
+  }
+  if (source_width == 0 || source_height == 0) {
+    set_detail(detail, detail_size, "");
+    return LOTUS_HIP_PASS;
+  }
+  if (!product || !buffer) {
+    set_detail(detail, detail_size, "invalid render buffer input");
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  launch_write_rows_flipped(product, product_format, buffer, buffer_format, source_width, source_height, r->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "render buffer write", e));
+    return LOTUS_HIP_FAIL;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+
+int lotus_hip_write_ids(lotus_hip_renderer* r, void* buffer, uint32_t buffer_format, uint32_t width, uint32_t height,
+                        int32_t value, char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (buffer_format != LOTUS_HIP_BUFFER_INT32) {
+    set_detail(detail, detail_size, "ids are written to Int32 render buffers only");
+    return LOTUS_HIP_FAIL;
+  }
+  const uint64_t count = uint64_t(width) * height;
+  if (count != 0 && !buffer) {
+    set_detail(detail, detail_size, "invalid render buffer input");
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  launch_fill_ids(static_cast<int32_t*>(buffer), count, value, r->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "id write", e));
     return LOTUS_HIP_FAIL;
   }
   set_detail(detail, detail_size, "");

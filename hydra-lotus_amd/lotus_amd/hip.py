@@ -31,8 +31,11 @@ SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hi
            "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning",
            "lotus_hip_cache_configure", "lotus_hip_cache_infer", "lotus_hip_cache_gradients",
            "lotus_hip_cache_train", "lotus_hip_cache_parameters", "lotus_hip_render_temporal", "lotus_hip_svgf",
-           "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune", "lotus_hip_readback_scene"]
+           "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune", "lotus_hip_readback_scene",
+           "lotus_hip_write_render_buffer", "lotus_hip_write_ids"]
 BATCH_GRAPH = 1
+# Render-buffer formats (HdFormat UNorm8Vec4, Float32Vec4, Float32, Int32).
+BUFFER_UNORM8X4, BUFFER_FLOAT32X4, BUFFER_FLOAT32, BUFFER_INT32 = range(4)
 # What a scene apply did (lotus_hip_scene_evidence.update).
 SCENE_UNCHANGED, SCENE_TABLES, SCENE_REFIT, SCENE_BUILD = range(4)
 
@@ -328,6 +331,13 @@ def lib():
         L.lotus_hip_render_temporal.restype = C.c_int
         L.lotus_hip_render_temporal.argtypes = [C.c_void_p, C.POINTER(FrameParams), C.POINTER(TemporalSettings),
                                                 C.POINTER(FrameResult), C.c_char_p, C.c_size_t]
+        L.lotus_hip_write_render_buffer.restype = C.c_int
+        L.lotus_hip_write_render_buffer.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                    C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_char_p,
+                                                    C.c_size_t]
+        L.lotus_hip_write_ids.restype = C.c_int
+        L.lotus_hip_write_ids.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32,
+                                          C.c_char_p, C.c_size_t]
         L.lotus_hip_svgf.restype = C.c_int
         L.lotus_hip_svgf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.c_uint32, C.c_uint32, C.c_void_p, C.c_char_p, C.c_size_t]
@@ -718,6 +728,24 @@ class Renderer:
         buf = C.create_string_buffer(256)
         s = lib().lotus_hip_svgf(self.handle, C.c_void_p(surface_ptr), C.c_void_p(radiance_ptr), width, height,
                                  batch_samples, max_history, iterations, C.c_void_p(color_ptr), buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    def write_render_buffer(self, product_ptr, product_format, width, height, buffer_ptr, buffer_format,
+                            buffer_width=None, buffer_height=None):
+        """A top-down product into a bottom-up render buffer, both device pointers
+        (BUFFER_* formats; the colour formats convert into each other)."""
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_write_render_buffer(self.handle, C.c_void_p(product_ptr), product_format, width, height,
+                                                C.c_void_p(buffer_ptr), buffer_format,
+                                                width if buffer_width is None else buffer_width,
+                                                height if buffer_height is None else buffer_height, buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    def write_ids(self, buffer_ptr, buffer_format, width, height, value):
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_write_ids(self.handle, C.c_void_p(buffer_ptr), buffer_format, width, height, value, buf, 256)
         if s != PASS:
             raise LotusError(s, buf.value.decode())
 

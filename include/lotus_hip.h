@@ -504,6 +504,31 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
 int lotus_hip_autotune(lotus_hip_renderer* renderer, const lotus_hip_frame_params* params, uint32_t trials,
                        lotus_hip_tuning* chosen, char* detail, size_t detail_size);
 
+/* Render-buffer writes of the products, on the device. Formats of a product
+ * and of a host's render buffer, as HdFormat names them. */
+#define LOTUS_HIP_BUFFER_UNORM8X4 0u  /* HdFormatUNorm8Vec4 ("rgba8-unorm") */
+#define LOTUS_HIP_BUFFER_FLOAT32X4 1u /* HdFormatFloat32Vec4 ("rgba32-sfloat") */
+#define LOTUS_HIP_BUFFER_FLOAT32 2u   /* HdFormatFloat32 (depth) */
+#define LOTUS_HIP_BUFFER_INT32 3u     /* HdFormatInt32 (ids) */
+
+/* Replaces HdLotusRenderBuffer::WriteColor and WriteDepth
+ * (adapters/hydra2/src/adapter.cpp:2544-2596) with WriteRowsFlippedLocked
+ * (:2713-2731): copies a top-down product of source_width x source_height
+ * pixels into a bottom-up render buffer on the device. A colour product
+ * converts between RGBA32F and UNorm8 (lround(clamp(v, 0, 1) * 255), and
+ * v / 255); every other pair must be one format. A buffer of another size, or
+ * a pair it does not convert, fails without writing, as WriteColor returns
+ * false. Ordered on the renderer's stream; returns once written. */
+int lotus_hip_write_render_buffer(lotus_hip_renderer* renderer, const void* product_device, uint32_t product_format,
+                                  uint32_t source_width, uint32_t source_height, void* buffer_device,
+                                  uint32_t buffer_format, uint32_t buffer_width, uint32_t buffer_height,
+                                  char* detail, size_t detail_size);
+
+/* Replaces HdLotusRenderBuffer::WriteIds (adapter.cpp:2733-2743): fills an
+ * Int32 render buffer of width x height pixels with `value`. */
+int lotus_hip_write_ids(lotus_hip_renderer* renderer, void* buffer_device, uint32_t buffer_format, uint32_t width,
+                        uint32_t height, int32_t value, char* detail, size_t detail_size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,9 @@ def lib():
         L.oracle_reservoir_merge.argtypes = [_u32p, _u32p, C.c_float, C.c_float, C.c_uint32, C.c_float]
         L.oracle_reservoir_normalization.restype = C.c_float
         L.oracle_reservoir_normalization.argtypes = [_u32p]
+        L.oracle_write_render_buffer.restype = C.c_int
+        L.oracle_write_render_buffer.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                                 C.c_uint32]
         L.oracle_material_inputs.restype = None
         L.oracle_material_inputs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float, _f32p]
         L.oracle_reuse_normalization.restype = C.c_float
@@ -310,3 +313,13 @@ def random_sequence(state: int, count: int) -> np.ndarray:
     out = np.zeros(count, np.float32)
     lib().oracle_random_sequence(state, count, ptr(out, C.c_float))
     return out
+
+
+def write_render_buffer(product: np.ndarray, product_format: int, buffer_format: int):
+    """The reference's render-buffer write of a top-down [height, width, ...]
+    product: the bottom-up buffer as bytes, or None where it refuses."""
+    product = np.ascontiguousarray(product)
+    h, w = product.shape[:2]
+    out = np.zeros(h * w * (16 if buffer_format == 1 else 4), np.uint8)
+    ok = lib().oracle_write_render_buffer(product.ctypes.data, product_format, w, h, out.ctypes.data, buffer_format)
+    return out if ok else None

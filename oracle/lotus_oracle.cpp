@@ -2564,3 +2564,39 @@ void oracle_offset_ray_origin(const float* position, const float* normal, float*
 }
 
 }  // extern "C"
+
+// Render-buffer writes (adapters/hydra2/src/adapter.cpp:2544-2596 WriteColor
+// and WriteDepth, :2713-2731 WriteRowsFlippedLocked): formats 0 UNorm8Vec4,
+// 1 Float32Vec4, 2 Float32, 3 Int32. Returns 0 where the reference returns
+// false, without writing.
+extern "C" int oracle_write_render_buffer(const void* source, uint32_t source_format, uint32_t width, uint32_t height,
+                                          void* buffer, uint32_t buffer_format) {
+  auto pixel = [](uint32_t f) -> size_t { return f == 1 ? 16 : (f <= 3 ? 4 : 0); };
+  if (!pixel(source_format) || !pixel(buffer_format)) return 0;
+  const size_t count = size_t(width) * height;
+  std::vector<uint8_t> converted;
+  const uint8_t* src = static_cast<const uint8_t*>(source);
+  if (source_format != buffer_format) {
+    if (buffer_format == 0 && source_format == 1) {
+      converted.resize(count * 4);
+      for (size_t i = 0; i < converted.size(); ++i) {
+        float v;
+        std::memcpy(&v, src + i * 4, 4);
+        converted[i] = static_cast<uint8_t>(std::lround(std::clamp(v, 0.0f, 1.0f) * 255.0f));
+      }
+    } else if (buffer_format == 1 && source_format == 0) {
+      converted.resize(count * 16);
+      for (size_t i = 0; i < count * 4; ++i) {
+        const float v = src[i] / 255.0f;
+        std::memcpy(converted.data() + i * 4, &v, 4);
+      }
+    } else {
+      return 0;
+    }
+    src = converted.data();
+  }
+  const size_t row = size_t(width) * pixel(buffer_format);
+  for (size_t y = 0; y < height; ++y)
+    std::memcpy(static_cast<uint8_t*>(buffer) + (height - 1 - y) * row, src + y * row, row);
+  return 1;
+}

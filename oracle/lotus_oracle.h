@@ -150,6 +150,11 @@ int oracle_camera_constants(const float* world_to_clip_gl,
 /* RNG probes (backend/vulkan/shaders/sampling/sampling.slang:9-25). */
 /* EvaluatePreview's inputs at a hit: base colour, roughness, emission,
  * metallic, specular colour, 0 (12 floats), texture lookups applied. */
+/* Render-buffer writes: a top-down product into a bottom-up buffer of the
+ * same size (formats 0 UNorm8Vec4, 1 Float32Vec4, 2 Float32, 3 Int32);
+ * 0 where the reference's WriteColor returns false. */
+int oracle_write_render_buffer(const void* source, uint32_t source_format, uint32_t width, uint32_t height,
+                               void* buffer, uint32_t buffer_format);
 void oracle_material_inputs(const oracle_scene* scene, uint32_t instance, uint32_t primitive, float u, float v,
                             float* out);
 uint32_t oracle_pcg_hash(uint32_t value);
