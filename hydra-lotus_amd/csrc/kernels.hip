@@ -319,13 +319,18 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
 }
 
 // Every pixel's initial record emptied (temporal frames: the reuse passes
-// read records outside the data window too).
-__global__ void __launch_bounds__(256) k_restir_clear(float4* restir, uint32_t pixels) {
+// read records outside the data window too), and with `surf` every
+// receiver surface zeroed, as restir.slang:306-307 starts each receiver
+// empty (a pixel whose camera ray misses keeps the zero surface).
+__global__ void __launch_bounds__(256) k_restir_clear(float4* restir, float4* surf, uint32_t pixels) {
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   for (uint32_t i = gtid(); i < pixels; i += gstride()) {
     restir[4ull * i] = make_float4(__uint_as_float(kNoReservoirSample), 0.0f, 0.0f, 0.0f);
-    restir[4ull * i + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    restir[4ull * i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    restir[4ull * i + 3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    restir[4ull * i + 1] = zero;
+    restir[4ull * i + 2] = zero;
+    restir[4ull * i + 3] = zero;
+    if (surf)
+      for (int k = 0; k < 8; ++k) surf[8ull * i + k] = zero;
   }
 }
 
@@ -1195,8 +1200,8 @@ void launch_restir_initial(const Launch& L, const SceneView& S, const FrameConst
   else
     k_restir_initial<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
-void launch_restir_clear(const Launch& L, float4* restir, uint32_t pixels) {
-  k_restir_clear<<<L.grid.resolve, 256, 0, L.stream>>>(restir, pixels);
+void launch_restir_clear(const Launch& L, float4* restir, float4* surf, uint32_t pixels) {
+  k_restir_clear<<<L.grid.resolve, 256, 0, L.stream>>>(restir, surf, pixels);
 }
 void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                             uint32_t* c) {

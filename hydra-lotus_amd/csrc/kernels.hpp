@@ -126,7 +126,7 @@ struct RestirReuseArgs {
   uint32_t max_history;        // restir_max_history_candidates
   uint32_t radius;             // spatial radius in pixels
 };
-void launch_restir_clear(const Launch& L, float4* restir, uint32_t pixels);
+void launch_restir_clear(const Launch& L, float4* restir, float4* surf, uint32_t pixels);
 void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                             uint32_t* c);
 void launch_restir_spatial(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,

@@ -1763,7 +1763,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
         launch_fill1(r->cache.loss, 1, 0.0f, r->stream);
       }
       if (add) {
-        if (tf && tf->reuse) launch_restir_clear(L, r->q.restir, npix);
+        if (tf && tf->reuse) launch_restir_clear(L, r->q.restir, const_cast<float4*>(tf->A.surf), npix);
         mark(kTCamera);
         launch_camera(L, r->view, F, r->q, r->counters);
         done();
@@ -2448,6 +2448,8 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
     e = hipMemcpy(ts->restir_spatial_records, t.rhist[cur], npix * 4 * sizeof(float4), hipMemcpyDeviceToHost);
   if (e == hipSuccess && ts->restir_spatial_sources && tf.spatial)
     e = hipMemcpy(ts->restir_spatial_sources, t.rsources, npix * sizeof(uint4), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ts->restir_surfaces && tf.reuse)
+    e = hipMemcpy(ts->restir_surfaces, t.rsurf[cur], npix * 8 * sizeof(float4), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "temporal readback", e));
     return LOTUS_HIP_FAIL;

@@ -265,6 +265,7 @@ class TemporalSettings(C.Structure):
         ("restir_temporal_records", C.c_void_p),
         ("restir_spatial_records", C.c_void_p),
         ("restir_spatial_sources", C.c_void_p),
+        ("restir_surfaces", C.c_void_p),
     ]
 
 
@@ -710,13 +711,15 @@ class Renderer:
                         radiance=np.zeros((height, width), TEMPORAL_RADIANCE),
                         restir_temporal=np.zeros((height, width, 16), np.uint32),
                         restir_spatial=np.zeros((height, width, 16), np.uint32),
-                        restir_sources=np.zeros((height, width, 4), np.uint32))
+                        restir_sources=np.zeros((height, width, 4), np.uint32),
+                        restir_surfaces=np.zeros((height, width, 32), np.float32))
             t.surface = prod["surface"].ctypes.data
             t.validity = prod["validity"].ctypes.data
             t.radiance = prod["radiance"].ctypes.data
             t.restir_temporal_records = prod["restir_temporal"].ctypes.data
             t.restir_spatial_records = prod["restir_spatial"].ctypes.data
             t.restir_spatial_sources = prod["restir_sources"].ctypes.data
+            t.restir_surfaces = prod["restir_surfaces"].ctypes.data
         out = self.render(world_to_clip, width, height,
                           sample_index=frame_index if sample_index is None else sample_index, temporal=t, **kw)
         out.update(prod)

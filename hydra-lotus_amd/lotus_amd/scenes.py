@@ -751,3 +751,23 @@ def grid(divisions: int = 64, extent: float = 1.0) -> Scene:
     sc.view = look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
     sc.projection = perspective(45.0, 1.0, 0.05, 10.0)
     return sc
+
+
+def restir_plane(colors=((2.0, 0.5, 1.0), (0.2, 2.0, 1.5), (1.0, 1.0, 1.0)), zoom: float = 1.0) -> Scene:
+    """The reservoir-arithmetic scene of validation/restir-gpu-test.cpp:30-51,
+    124-134: a 24 x 24 diffuse plane (base colour (0.5, 0.75, 1), ior 1) at
+    z = -4 under the camera's identity view, lit by parallel lights of the
+    given radiances shining along -Z; the third shines along +Z, from below
+    the plane, so its target is zero. Every light's unshadowed target is a
+    closed form: max over channels of base colour x radiance / pi."""
+    sc = Scene(name="restir-plane")
+    sc.view = identity()
+    sc.projection = np.array([zoom, 0, 0, 0, 0, zoom, 0, 0, 0, 0, -1.2, -1, 0, 0, -2.2, 0], F32)
+    m = sc.add_material(preview_material(base_color=(0.5, 0.75, 1.0), ior=1.0))
+    sc.add_mesh([[-12, -12, -4], [12, -12, -4], [12, 12, -4], [-12, 12, -4]], [[0, 1, 2], [0, 2, 3]], m)
+    for i, c in enumerate(colors):
+        t = identity()
+        if i == 2:
+            t[10] = -1.0
+        sc.add_light(analytic_light(LIGHT_DISTANT, transform=t, radiance=c, angle=0.0))
+    return sc

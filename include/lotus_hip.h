@@ -459,6 +459,15 @@ typedef struct lotus_hip_temporal_settings {
   void* restir_temporal_records;
   void* restir_spatial_records;
   void* restir_spatial_sources;
+  /* Optional: the reuse passes' receiver surfaces (RestirSurface,
+   * restir.slang:30-62, in this renderer's 128-byte form: {offset origin,
+   * view depth}, {geometric normal, instance id low bits}, {shading normal,
+   * instance id high bits}, {wo in the shading frame, dielectric f90},
+   * {base colour, alpha}, {dielectric f0, metallic}, {f0, f90},
+   * {transmission (negated: thin wall), eta (negated: smooth), material id
+   * low, high bits}; zero where the camera ray missed), width*height of
+   * them. Written when the frame ran a reuse pass. */
+  void* restir_surfaces;
 } lotus_hip_temporal_settings;
 
 int lotus_hip_render_temporal(lotus_hip_renderer* renderer, const lotus_hip_frame_params* params,

@@ -130,6 +130,13 @@ def lib():
         L.oracle_bsdf_eval.argtypes = [_u32p, C.c_int, _f32p, _f32p, _f32p, _f32p]
         L.oracle_bsdf_sample.argtypes = [_u32p, C.c_int, _f32p, _f32p, _f32p]
         L.oracle_offset_ray_origin.argtypes = [_f32p, _f32p, _f32p]
+        L.oracle_restir_temporal.restype = C.c_int
+        L.oracle_restir_temporal.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_uint32, C.c_uint32, C.c_uint32,
+                                             _f32p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64,
+                                             C.c_uint64, _u32p, _f32p, C.c_uint32, _u32p, _f32p]
+        L.oracle_restir_spatial.restype = None
+        L.oracle_restir_spatial.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, C.c_uint32, _u32p, _f32p, _u32p, _u32p]
         _lib = L
     return _lib
 
@@ -267,6 +274,36 @@ class OracleScene:
         rec = np.zeros(12, np.uint32)
         hit = lib().oracle_restir_initial(self.handle, C.byref(frame), x, y, sample_index, ptr(rec, C.c_uint32))
         return rec if hit else None
+
+    def restir_temporal(self, frame: Frame, x: int, y: int, sample_index: int, view, history_valid: bool,
+                        previous_index: int, centre_ids, previous_record, previous_surface, max_history: int,
+                        instance_ids=None, material_ids=None):
+        """WriteTemporalReservoir at pixel (x, y): (record[16] words,
+        receiver surface[32] floats, hit)."""
+        view = np.ascontiguousarray(view, np.float32)
+        prec = np.ascontiguousarray(previous_record if previous_record is not None else np.zeros(16), np.uint32)
+        psurf = np.ascontiguousarray(previous_surface if previous_surface is not None else np.zeros(32), np.float32)
+        rec = np.zeros(16, np.uint32)
+        surf = np.zeros(32, np.float32)
+        iid = None if instance_ids is None else np.ascontiguousarray(instance_ids, np.uint64)
+        mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.uint64)
+        hit = lib().oracle_restir_temporal(
+            self.handle, C.byref(frame), x, y, sample_index, ptr(view, C.c_float),
+            None if iid is None else iid.ctypes.data, None if mid is None else mid.ctypes.data,
+            int(bool(history_valid)), int(previous_index) & 0xFFFFFFFF, int(centre_ids[0]), int(centre_ids[1]),
+            ptr(prec, C.c_uint32), ptr(psurf, C.c_float), max_history, ptr(rec, C.c_uint32), ptr(surf, C.c_float))
+        return rec, surf, bool(hit)
+
+    def restir_spatial(self, width: int, height: int, x: int, y: int, sample_index: int, radius: int,
+                       max_history: int, records: np.ndarray, surfaces: np.ndarray):
+        """WriteSpatialReservoir at pixel (x, y) over the temporal pass's
+        records [h*w, 16] and surfaces [h*w, 32]: (record[16], sources[4])."""
+        rec = np.zeros(16, np.uint32)
+        src = np.zeros(4, np.uint32)
+        lib().oracle_restir_spatial(self.handle, width, height, x, y, sample_index, radius, max_history,
+                                    ptr(records, C.c_uint32), ptr(surfaces, C.c_float), ptr(rec, C.c_uint32),
+                                    ptr(src, C.c_uint32))
+        return rec, src
 
 
 def camera_constants(world_to_clip_gl):

@@ -2422,6 +2422,178 @@ void StoreReservoir(const Reservoir& r, uint32_t* w) {
   std::memcpy(w + 2, &r.selected_target, 4);
   w[3] = r.count;
 }
+
+// ------------------------------------------------------- ReSTIR DI reuse
+// restir.slang:22-28 (RestirReuse, 64 bytes): the RestirPixel, then its
+// normalization, supporting candidates, previous index and age.
+struct ReuseRecord {
+  RestirPixel px;
+  float normalization = 0.0f;
+  uint32_t supporting = 0, previous = 0xFFFFFFFFu, age = 0;
+};
+
+ReuseRecord LoadReuse(const uint32_t* w) {
+  ReuseRecord x;
+  x.px.r = LoadReservoir(w);
+  std::memcpy(&x.px.u, w + 4, 4);
+  std::memcpy(&x.px.v, w + 5, 4);
+  std::memcpy(&x.px.proposal, w + 6, 4);
+  x.px.flags = w[7];
+  std::memcpy(&x.px.contribution.x, w + 8, 4);
+  std::memcpy(&x.px.contribution.y, w + 9, 4);
+  std::memcpy(&x.px.contribution.z, w + 10, 4);
+  std::memcpy(&x.px.visibility, w + 11, 4);
+  std::memcpy(&x.normalization, w + 12, 4);
+  x.supporting = w[13];
+  x.previous = w[14];
+  x.age = w[15];
+  return x;
+}
+
+void StoreReuse(const ReuseRecord& x, uint32_t* w) {
+  StoreReservoir(x.px.r, w);
+  std::memcpy(w + 4, &x.px.u, 4);
+  std::memcpy(w + 5, &x.px.v, 4);
+  std::memcpy(w + 6, &x.px.proposal, 4);
+  w[7] = x.px.flags;
+  std::memcpy(w + 8, &x.px.contribution.x, 4);
+  std::memcpy(w + 9, &x.px.contribution.y, 4);
+  std::memcpy(w + 10, &x.px.contribution.z, 4);
+  std::memcpy(w + 11, &x.px.visibility, 4);
+  std::memcpy(w + 12, &x.normalization, 4);
+  w[13] = x.supporting;
+  w[14] = x.previous;
+  w[15] = x.age;
+}
+
+// restir.slang:30-62 (RestirSurface) in the renderer's 32-float form:
+// {offset origin, view depth}, {geometric normal, instance id low},
+// {shading normal, instance id high}, {wo in the shading frame, dielectric
+// f90}, {base colour, alpha}, {dielectric f0, metallic}, {f0, f90},
+// {transmission (negated: thin wall), eta (negated: smooth), material id
+// low, high}. StoreClosure / LoadClosure (restir.slang:80-141) for the
+// preview closure: targets are reflection-side, so a loaded closure's
+// transmission colour is white.
+struct ReuseSurface {
+  F3 origin{0, 0, 0};
+  float depth = 0.0f;
+  F3 normal{0, 0, 0}, shading_normal{0, 0, 0}, wo{0, 0, 0};
+  uint64_t instance_id = 0, material_id = 0;
+  Closure bsdf{};
+};
+
+inline float BitsFloat(uint32_t w) {
+  float f;
+  std::memcpy(&f, &w, 4);
+  return f;
+}
+inline uint32_t FloatBits(float f) {
+  uint32_t w;
+  std::memcpy(&w, &f, 4);
+  return w;
+}
+
+void StoreSurface(const ReuseSurface& r, float* out) {
+  const Closure& b = r.bsdf;
+  const float v[32] = {r.origin.x, r.origin.y, r.origin.z, r.depth,
+                       r.normal.x, r.normal.y, r.normal.z, BitsFloat(static_cast<uint32_t>(r.instance_id)),
+                       r.shading_normal.x, r.shading_normal.y, r.shading_normal.z,
+                       BitsFloat(static_cast<uint32_t>(r.instance_id >> 32)),
+                       r.wo.x, r.wo.y, r.wo.z, b.dielectric_f90,
+                       b.base_color.x, b.base_color.y, b.base_color.z, b.alpha,
+                       b.dielectric_f0.x, b.dielectric_f0.y, b.dielectric_f0.z, b.metallic,
+                       b.f0.x, b.f0.y, b.f0.z, b.f90,
+                       b.thin_walled ? -b.transmission : b.transmission, b.smooth ? -b.eta : b.eta,
+                       BitsFloat(static_cast<uint32_t>(r.material_id)),
+                       BitsFloat(static_cast<uint32_t>(r.material_id >> 32))};
+  std::memcpy(out, v, sizeof v);
+}
+
+ReuseSurface LoadSurface(const float* s) {
+  ReuseSurface r;
+  r.origin = F3{s[0], s[1], s[2]};
+  r.depth = s[3];
+  r.normal = F3{s[4], s[5], s[6]};
+  r.shading_normal = F3{s[8], s[9], s[10]};
+  r.wo = F3{s[12], s[13], s[14]};
+  r.instance_id = static_cast<uint64_t>(FloatBits(s[7])) | (static_cast<uint64_t>(FloatBits(s[11])) << 32);
+  r.material_id = static_cast<uint64_t>(FloatBits(s[30])) | (static_cast<uint64_t>(FloatBits(s[31])) << 32);
+  Closure& b = r.bsdf;
+  b.dielectric_f90 = s[15];
+  b.base_color = F3{s[16], s[17], s[18]};
+  b.alpha = s[19];
+  b.dielectric_f0 = F3{s[20], s[21], s[22]};
+  b.metallic = s[23];
+  b.f0 = F3{s[24], s[25], s[26]};
+  b.f90 = s[27];
+  b.transmission = std::fabs(s[28]);
+  b.thin_walled = s[28] < 0.0f;
+  b.eta = std::fabs(s[29]);
+  b.smooth = s[29] < 0.0f;
+  b.transmission_color = F3{1, 1, 1};
+  return r;
+}
+
+// restir.slang:256-268 (HistoryContribution): the sample's unshadowed
+// contribution at a stored surface, the power selection cancelled.
+F3 HistoryContribution(const Scene& s, const ReuseSurface& src, uint32_t light, const LightSample& ls) {
+  if (!(ls.pdf > 0.0f) || Dot(ls.direction, src.normal) <= 0.0f) return F3{0, 0, 0};
+  const Frame basis = MakeShadingFrame(src.shading_normal);
+  return RestirEvaluate(src.bsdf, src.wo, basis.ToLocal(ls.direction)) * ls.radiance *
+         (s.lights[light].pdf / ls.pdf);
+}
+F3 HistoryContributionUv(const Scene& s, const ReuseSurface& src, uint32_t light, float u, float v) {
+  return HistoryContribution(s, src, light, ReconstructReservoirLight(s, light, u, v, src.origin));
+}
+
+// restir.slang:270-272 (SameSample).
+bool SameSample(const RestirPixel& a, const RestirPixel& b) {
+  return a.r.sample == b.r.sample && a.u == b.u && a.v == b.v;
+}
+
+// restir.slang:276-287 (HistoryTarget).
+float HistoryTarget(const Scene& s, const ReuseRecord& source, const ReuseSurface& surface,
+                    const RestirPixel& selected, bool& valid) {
+  if (SameSample(source.px, selected)) {
+    valid = std::isfinite(source.px.r.selected_target);
+    return source.px.r.selected_target;
+  }
+  const F3 c = HistoryContributionUv(s, surface, selected.r.sample, selected.u, selected.v);
+  valid = IsFinite(c);
+  return MaxComponent(c);
+}
+
+// restir.slang:289-300 (ReceiverVisibility): the same pixel's traced
+// visibility when it selected the same sample, else a shadow ray.
+float ReceiverVisibility(const Scene& s, const RestirPixel& selected, const RestirPixel& current, F3 origin,
+                         const LightSample& ls) {
+  if (SameSample(selected, current)) return current.visibility;
+  return Occluded(s, Ray{origin, 0.0f, ls.direction, ls.distance}, false) ? 0.0f : 1.0f;
+}
+
+// restir.slang:372-385, 475-485: the selection's estimate at the receiver.
+void ResolveEstimate(const Scene& s, const ReuseSurface& receiver, const RestirPixel& own, ReuseRecord& result) {
+  if ((result.px.flags & 2u) != 0 || !(result.px.r.weight_sum > 0.0f)) return;
+  const LightSample ls =
+      ReconstructReservoirLight(s, result.px.r.sample, result.px.u, result.px.v, receiver.origin);
+  const float visibility = ReceiverVisibility(s, result.px, own, receiver.origin, ls);
+  const F3 estimate =
+      HistoryContribution(s, receiver, result.px.r.sample, ls) * (visibility * result.normalization);
+  if (!std::isfinite(result.normalization) || result.normalization <= 0.0f || !IsFinite(estimate)) {
+    result.px.flags |= 2u;
+  } else {
+    result.px.contribution = estimate;
+    result.px.visibility = visibility;
+  }
+}
+
+// restir.slang:392-400 (SpatialCompatible): geometry alone.
+bool SpatialCompatible(const ReuseSurface& receiver, uint32_t record_flags, const ReuseSurface& source) {
+  return (record_flags & 3u) == 1u && receiver.depth > 0.0f && source.depth > 0.0f &&
+         receiver.material_id == source.material_id &&
+         std::fabs(receiver.depth - source.depth) <= 0.01f + 0.02f * std::max(receiver.depth, source.depth) &&
+         Dot(receiver.normal, source.normal) >= 0.9f && Dot(receiver.shading_normal, source.shading_normal) >= 0.9f;
+}
 }  // namespace
 
 int oracle_reservoir_stream(uint32_t* reservoir, uint32_t sample, float target, float proposal, float random) {
@@ -2467,6 +2639,165 @@ int oracle_restir_initial(const oracle_scene* scene, const oracle_frame* frame, 
   record[3] = px.r.count;
   record[7] = px.flags;
   return 1;
+}
+
+// restir.slang:302-387 (WriteTemporalReservoir) for pixel (x, y).
+int oracle_restir_temporal(const oracle_scene* scene, const oracle_frame* frame, uint32_t x, uint32_t y,
+                           uint32_t sample_index, const float* view, const uint64_t* instance_ids,
+                           const uint64_t* material_ids, uint32_t history_valid, uint32_t previous_index,
+                           uint64_t centre_instance_id, uint64_t centre_material_id,
+                           const uint32_t* previous_record, const float* previous_surface, uint32_t max_history,
+                           uint32_t* record, float* surface) {
+  const Scene& s = scene->s;
+  const oracle_frame& fr = *frame;
+  const uint32_t slot = y * fr.width + x;
+  Path path = StartPath(fr, x, y, sample_index);
+  std::vector<uint32_t> counts(fr.max_bounces + 2, 0u);
+  Counters k{counts.data(), 0};
+  ReuseRecord result;
+  ReuseSurface receiver;
+  Hit hit;
+  const bool hit_found = TraceClosest(s, path.ray, hit, false);
+  if (hit_found) result.px = InitialReservoir(s, fr, slot, sample_index, path.ray, hit, k, false);
+  else result.px.r = Reservoir{};
+  const RestirPixel initial = result.px;
+  result.previous = 0xFFFFFFFFu;
+  result.normalization = ReservoirNormalization(result.px.r);
+  result.supporting = result.px.r.count;
+  if (hit_found) {
+    const Surface sf = ReconstructHit(s, hit, path.ray.d);
+    const Frame basis = MakeShadingFrame(sf.shading_normal);
+    receiver.origin = OffsetRayOrigin(sf.position, sf.normal);
+    float h[4];
+    MulPoint(view, sf.position.x, sf.position.y, sf.position.z, 1.0f, h);
+    receiver.depth = -h[2] / h[3];  // temporal.slang:27-30 (CurrentViewDepth)
+    receiver.normal = sf.normal;
+    receiver.shading_normal = sf.shading_normal;
+    receiver.wo = basis.ToLocal(-path.ray.d);
+    receiver.bsdf = sf.bsdf;
+    // temporal.slang:67-72 (TemporalInstanceIdentity)
+    receiver.instance_id = instance_ids ? instance_ids[hit.instance] : uint64_t(hit.instance) + 1u;
+    receiver.material_id =
+        material_ids ? material_ids[hit.instance] : uint64_t(s.instance_material[hit.instance]) + 1u;
+    // temporal.slang:74-79 (TemporalSampleHistory)
+    const bool history = history_valid != 0 && centre_instance_id == receiver.instance_id &&
+                         centre_material_id == receiver.material_id;
+    if (history && (result.px.flags & 2u) == 0) {
+      const ReuseRecord source = LoadReuse(previous_record);
+      const ReuseSurface psurf = LoadSurface(previous_surface);
+      if ((source.px.flags & 1u) != 0 && (source.px.flags & 2u) == 0 && psurf.instance_id == receiver.instance_id &&
+          psurf.material_id == receiver.material_id) {
+        const uint32_t count = std::min(source.px.r.count, max_history);
+        F3 received{0, 0, 0};
+        if (source.px.r.weight_sum > 0.0f) {
+          if (source.px.r.sample >= s.lights.size()) result.px.flags |= 2u;
+          else received = HistoryContributionUv(s, receiver, source.px.r.sample, source.px.u, source.px.v);
+        }
+        uint32_t rng = SeedRandom(slot, sample_index ^ 0x94D049BBu);
+        bool selected = false;
+        if (!IsFinite(received) || (result.px.flags & 2u) != 0 ||
+            !MergeReevaluated(result.px.r, source.px.r, source.normalization, MaxComponent(received), max_history,
+                              NextRandom(rng), selected)) {
+          result.px.flags |= 2u;
+        } else {
+          result.previous = previous_index;
+          if (selected) {
+            result.px.u = source.px.u;
+            result.px.v = source.px.v;
+            result.px.proposal = source.px.proposal;
+            result.age = std::min(source.age, 65535u) + 1u;
+          }
+          if (result.px.r.weight_sum > 0.0f) {
+            bool valid = true;
+            const float support = HistoryTarget(s, source, psurf, result.px, valid);
+            if (!valid) result.px.flags |= 2u;
+            result.supporting = initial.r.count + (support > 0.0f ? count : 0u);
+          } else {
+            result.supporting = 0;
+          }
+          result.normalization = ReuseNormalization(result.px.r, result.supporting);
+        }
+      }
+    }
+    ResolveEstimate(s, receiver, initial, result);
+  }
+  StoreReuse(result, record);
+  StoreSurface(receiver, surface);
+  return hit_found ? 1 : 0;
+}
+
+// restir.slang:402-491 (WriteSpatialReservoir) for pixel (x, y) over the
+// temporal pass's records and receiver surfaces of the whole target (16
+// words and 32 floats per pixel): the four axis neighbours at `radius`.
+void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
+                           uint32_t sample_index, uint32_t radius, uint32_t max_history, const uint32_t* records,
+                           const float* surfaces, uint32_t* record, uint32_t* sources) {
+  const Scene& s = scene->s;
+  const uint32_t slot = y * width + x;
+  const ReuseRecord centre = LoadReuse(records + 16ull * slot);
+  const ReuseSurface receiver = LoadSurface(surfaces + 32ull * slot);
+  ReuseRecord result = centre;
+  uint32_t src[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+  const int r = static_cast<int>(radius);
+  const int ox[4] = {-r, r, 0, 0}, oy[4] = {0, 0, -r, r};
+  if ((centre.px.flags & 3u) == 1u) {
+    uint32_t accepted = 0;
+    for (int i = 0; i < 4; ++i) {
+      const int nx = static_cast<int>(x) + ox[i], ny = static_cast<int>(y) + oy[i];
+      if (nx < 0 || ny < 0 || nx >= static_cast<int>(width) || ny >= static_cast<int>(height)) continue;
+      const uint32_t index = static_cast<uint32_t>(ny) * width + static_cast<uint32_t>(nx);
+      if (SpatialCompatible(receiver, records[16ull * index + 7], LoadSurface(surfaces + 32ull * index))) {
+        src[i] = index;
+        ++accepted;
+      }
+    }
+    if (accepted != 0) {
+      result.px.r = Reservoir{};
+      bool selected = false;
+      if (!MergeReevaluated(result.px.r, centre.px.r, centre.normalization, centre.px.r.selected_target,
+                            0xFFFFFFFFu, 0.0f, selected))
+        result.px.flags |= 2u;
+      uint32_t rng = SeedRandom(slot, sample_index ^ 0x369DEA0Fu);
+      for (int i = 0; i < 4; ++i) {
+        if (src[i] == 0xFFFFFFFFu) continue;
+        const ReuseRecord source = LoadReuse(records + 16ull * src[i]);
+        F3 received{0, 0, 0};
+        if (source.px.r.weight_sum > 0.0f)
+          received = HistoryContributionUv(s, receiver, source.px.r.sample, source.px.u, source.px.v);
+        if (!IsFinite(received) || !MergeReevaluated(result.px.r, source.px.r, source.normalization,
+                                                     MaxComponent(received), max_history, NextRandom(rng),
+                                                     selected)) {
+          result.px.flags |= 2u;
+          break;
+        }
+        if (selected) {
+          result.px.u = source.px.u;
+          result.px.v = source.px.v;
+          result.px.proposal = source.px.proposal;
+          result.age = source.age;
+        }
+      }
+      result.supporting = 0;
+      result.px.contribution = F3{0, 0, 0};
+      result.px.visibility = 0.0f;
+      if ((result.px.flags & 2u) == 0 && result.px.r.weight_sum > 0.0f) {
+        result.supporting = centre.px.r.count;
+        for (int i = 0; i < 4; ++i) {
+          if (src[i] == 0xFFFFFFFFu) continue;
+          const ReuseRecord source = LoadReuse(records + 16ull * src[i]);
+          bool valid = true;
+          const float support =
+              HistoryTarget(s, source, LoadSurface(surfaces + 32ull * src[i]), result.px, valid);
+          if (!valid) result.px.flags |= 2u;
+          if (support > 0.0f) result.supporting += std::min(source.px.r.count, max_history);
+        }
+      }
+      result.normalization = ReuseNormalization(result.px.r, result.supporting);
+      ResolveEstimate(s, receiver, centre.px, result);
+    }
+  }
+  StoreReuse(result, record);
+  for (int i = 0; i < 4; ++i) sources[i] = src[i];
 }
 
 void oracle_camera_ray(const float* clip_to_world, float nx, float ny, float* ray) {

@@ -138,6 +138,34 @@ float oracle_reuse_normalization(const uint32_t* reservoir, uint32_t supporting)
 int oracle_restir_initial(const oracle_scene* scene, const oracle_frame* frame, uint32_t x, uint32_t y,
                           uint32_t sample_index, uint32_t* record);
 
+/* ReSTIR DI reuse (restir.slang:302-491). A reuse record is RestirReusePixel's
+ * 16 words {RestirPixel's 12, normalization, supporting candidates, previous
+ * index, age}; a receiver surface is 32 floats: {offset origin, view depth},
+ * {geometric normal, instance id low bits}, {shading normal, instance id high
+ * bits}, {wo in the shading frame, dielectric f90}, {base colour, alpha},
+ * {dielectric f0, metallic}, {f0, f90}, {transmission (negated: thin wall),
+ * eta (negated: smooth), material id low, high}.
+ *
+ * Temporal pass at pixel (x, y): the oracle's own initial reservoir and
+ * receiver surface of the frame's camera ray, merged with the previous frame's
+ * record and surface at previous_index when history_valid (the reprojection
+ * accepted the pixel) and the pixel-centre surface's ids equal the receiver's
+ * (TemporalSampleHistory, temporal.slang:74-79). view: world -> view,
+ * column-major; ids NULL: instance + 1 and material slot + 1. Writes the
+ * record and the receiver surface; returns whether the camera ray hit. */
+int oracle_restir_temporal(const oracle_scene* scene, const oracle_frame* frame, uint32_t x, uint32_t y,
+                           uint32_t sample_index, const float* view, const uint64_t* instance_ids,
+                           const uint64_t* material_ids, uint32_t history_valid, uint32_t previous_index,
+                           uint64_t centre_instance_id, uint64_t centre_material_id,
+                           const uint32_t* previous_record, const float* previous_surface, uint32_t max_history,
+                           uint32_t* record, float* surface);
+/* Spatial pass at pixel (x, y) over the temporal pass's records and surfaces
+ * of the whole width x height target: writes the record and the four
+ * accepted neighbours (0xFFFFFFFF: none). */
+void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
+                           uint32_t sample_index, uint32_t radius, uint32_t max_history, const uint32_t* records,
+                           const float* surfaces, uint32_t* record, uint32_t* sources);
+
 /* The camera ray through a film position (reference CameraRay). */
 void oracle_camera_ray(const float* clip_to_world, float ndc_x, float ndc_y,
                        float* ray /* 8 floats */);

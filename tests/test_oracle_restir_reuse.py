@@ -1,0 +1,176 @@
+"""Pins the oracle's ReSTIR DI reuse passes (WriteTemporalReservoir and
+WriteSpatialReservoir, restir.slang:302-491) the way the reference pins its
+own GPU passes (validation/restir-gpu-test.cpp:124-206, 435-515): on a
+diffuse plane under three parallel lights, whose targets are closed forms
+(1/pi, 1.5/pi and 0), every record is re-derived from the pass's inputs by
+the reference's own MergeReevaluatedReservoir and ReservoirReuseNormalization
+(backend/vulkan/reservoir.cpp, in double, built into oracle/_ref), with the
+reference's random-number offsets. A plausible slip (a dropped cap, the
+wrong seed offset, support counted without the target test, the centre not
+re-streamed) changes a selection or a count and fails here."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from lotus_amd import scenes
+
+import restir_reuse as rr
+
+needs_ref = pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref not built (no reference sources)")
+
+SIZE = 64
+CAP = 3
+BASE = (0.5, 0.75, 1.0)
+
+
+def _target(light):
+    return (1.0 / math.pi) if light == 0 else (1.5 / math.pi) if light == 1 else 0.0
+
+
+def _plane(colors):
+    sc = scenes.restir_plane(colors)
+    o = oracle.OracleScene(sc)
+    fr = oracle.make_frame(sc.world_to_clip, SIZE, SIZE, max_bounces=1, light_sampling=2, restir_candidates=2)
+    return sc, o, fr
+
+
+def _still(h, w, frame):
+    validity = np.zeros((h, w, 2), np.uint32)
+    validity[..., 0] = 1 if frame else 0
+    validity[..., 1] = np.arange(h * w, dtype=np.uint32).reshape(h, w)
+    centre = np.zeros((h, w, 2), np.uint64)
+    centre[..., 0] = 1  # the plane: instance 0, material slot 1
+    centre[..., 1] = 2
+    return validity, centre
+
+
+@needs_ref
+def test_temporal_reuse_rederived_by_the_reference():
+    colors = ((2, 1, 0.5), (1, 2, 0.5), (0.5, 1, 2))
+    sc, o, fr = _plane(colors)
+    prev_rec = prev_surf = None
+    zero_sources = 0
+    for frame in range(12):
+        sample = 17 + frame
+        validity, centre = _still(SIZE, SIZE, frame)
+        rec, surf, hits = rr.oracle_temporal_frame(o, fr, sample, sc.view, prev_rec, prev_surf, validity, centre,
+                                                   frame != 0, CAP)
+        assert hits.all()
+        for slot in range(SIZE * SIZE):
+            x, y = slot % SIZE, slot // SIZE
+            initial = o.restir_initial(fr, x, y, sample)
+            expected = np.array(initial[:4], np.uint32)
+            uv = initial[4:6].copy()
+            supporting = int(initial[3])
+            if frame:
+                old = prev_rec.reshape(-1, 16)[slot]
+                if rr.f32(old[1:2])[0] == 0:
+                    zero_sources += 1
+                rng = rr.Rng(slot, sample ^ rr.TEMPORAL_SEED)
+                sel = rr.ref_merge(expected, old, rr.f32(old[12:13])[0], _target(int(old[0])), CAP, rng.next())
+                assert sel is not None
+                if sel:
+                    uv = old[4:6].copy()
+                supporting = supporting + min(int(old[3]), CAP) if rr.f32(expected[1:2])[0] > 0 else 0
+            r = rec[y, x]
+            assert r[0] == expected[0] and r[3] == expected[3] and np.array_equal(r[4:6], uv), (frame, slot)
+            assert r[13] == supporting and r[14] == (slot if frame else rr.NO), (frame, slot)
+            assert np.isclose(rr.f32(r[1:2])[0], rr.f32(expected[1:2])[0], rtol=4e-6, atol=1e-12)
+            w = rr.ref_reuse_normalization(expected, supporting) if frame else rr.ref_normalization(expected)
+            assert np.isclose(rr.f32(r[12:13])[0], w, rtol=4e-6, atol=1e-12), (frame, slot)
+            if rr.f32(expected[1:2])[0] > 0:
+                rgb = np.array(BASE) * np.array(colors[int(expected[0])]) / math.pi * w
+                assert np.allclose(rr.f32(r[8:11]), rgb, rtol=5e-6, atol=1e-7), (frame, slot)
+            assert r[3] <= 5
+        prev_rec, prev_surf = rec, surf
+    assert zero_sources != 0
+
+
+@needs_ref
+@pytest.mark.parametrize("temporal", [False, True])
+def test_spatial_reuse_rederived_by_the_reference(temporal):
+    colors = ((2, 0.5, 1), (0.2, 2, 1.5), (1, 1, 1))
+    sc, o, fr = _plane(colors)
+    prev_rec = prev_surf = None
+    zero_sources = 0
+    for frame in range(4 if temporal else 2):
+        sample = 17 + frame
+        validity, centre = _still(SIZE, SIZE, frame)
+        trec, tsurf, _ = rr.oracle_temporal_frame(o, fr, sample, sc.view, prev_rec, prev_surf, validity, centre,
+                                                  temporal and frame != 0, CAP)
+        srec, src = rr.oracle_spatial_frame(o, SIZE, SIZE, sample, 1, CAP, trec, tsurf)
+        flat = trec.reshape(-1, 16)
+        for slot in range(SIZE * SIZE):
+            x, y = slot % SIZE, slot // SIZE
+            centre_rec = flat[slot]
+            expected = np.array([rr.NO, 0, 0, 0], np.uint32)
+            uv = centre_rec[4:6].copy()
+            sel = rr.ref_merge(expected, centre_rec, rr.f32(centre_rec[12:13])[0], rr.f32(centre_rec[2:3])[0],
+                               0xFFFFFFFF, 0.0)
+            assert sel is not None
+            rng = rr.Rng(slot, sample ^ rr.SPATIAL_SEED)
+            neighbours = [slot - 1 if x else rr.NO, slot + 1 if x != SIZE - 1 else rr.NO,
+                          slot - SIZE if y else rr.NO, slot + SIZE if y != SIZE - 1 else rr.NO]
+            assert list(src[y, x]) == neighbours, (frame, slot)
+            supporting = int(centre_rec[3])
+            for n in neighbours:
+                if n == rr.NO:
+                    continue
+                s = flat[n]
+                if rr.f32(s[1:2])[0] == 0:
+                    zero_sources += 1
+                got = rr.ref_merge(expected, s, rr.f32(s[12:13])[0], _target(int(s[0])), CAP, rng.next())
+                assert got is not None
+                if got:
+                    uv = s[4:6].copy()
+                supporting += min(int(s[3]), CAP)
+            if rr.f32(expected[1:2])[0] == 0:
+                supporting = 0
+            r = srec[y, x]
+            assert r[0] == expected[0] and r[3] == expected[3] and np.array_equal(r[4:6], uv), (frame, slot)
+            assert r[13] == supporting, (frame, slot, r[13], supporting)
+            assert np.isclose(rr.f32(r[1:2])[0], rr.f32(expected[1:2])[0], rtol=2e-5, atol=1e-12)
+            w = rr.ref_reuse_normalization(expected, supporting)
+            assert np.isclose(rr.f32(r[12:13])[0], w, rtol=8e-6, atol=1e-12), (frame, slot)
+            if rr.f32(expected[1:2])[0] > 0:
+                rgb = np.array(BASE) * np.array(colors[int(expected[0])]) / math.pi * w
+                assert np.allclose(rr.f32(r[8:11]), rgb, rtol=1e-5, atol=1e-7), (frame, slot)
+            assert r[3] <= (17 if temporal else 10)
+        prev_rec, prev_surf = srec, tsurf
+    assert zero_sources != 0
+
+
+@needs_ref
+def test_temporal_support_counts_only_sources_that_can_hold_the_selection():
+    # restir.slang:361-370: the source's capped count joins the supporting
+    # candidates only where the final selection's target at the SOURCE's
+    # surface is positive. History surfaces turned away from every light
+    # (their normals negated) give a zero target for any sample they did not
+    # select themselves, and their own stored target where they did.
+    colors = ((2, 1, 0.5), (1, 2, 0.5), (0.5, 1, 2))
+    sc, o, fr = _plane(colors)
+    validity, centre = _still(SIZE, SIZE, 0)
+    rec0, surf0, _ = rr.oracle_temporal_frame(o, fr, 17, sc.view, None, None, validity, centre, False, CAP)
+    away = surf0.copy()
+    away[..., 4:7] *= -1.0   # geometric normal
+    away[..., 8:11] *= -1.0  # shading normal
+    validity, centre = _still(SIZE, SIZE, 1)
+    rec1, _, _ = rr.oracle_temporal_frame(o, fr, 18, sc.view, rec0, away, validity, centre, True, CAP)
+    kept = dropped = 0
+    for slot in range(SIZE * SIZE):
+        x, y = slot % SIZE, slot // SIZE
+        initial = o.restir_initial(fr, x, y, 18)
+        old = rec0.reshape(-1, 16)[slot]
+        expected = np.array(initial[:4], np.uint32)
+        rng = rr.Rng(slot, 18 ^ rr.TEMPORAL_SEED)
+        sel = rr.ref_merge(expected, old, rr.f32(old[12:13])[0], _target(int(old[0])), CAP, rng.next())
+        same = sel or (int(expected[0]) == int(old[0]) and np.array_equal(initial[4:6], old[4:6]))
+        supporting = int(initial[3]) + (min(int(old[3]), CAP) if same and rr.f32(old[2:3])[0] > 0 else 0)
+        if rr.f32(expected[1:2])[0] == 0:
+            supporting = 0
+        assert rec1[y, x, 13] == supporting, (slot, rec1[y, x, 13], supporting)
+        kept += supporting > int(initial[3])
+        dropped += 0 < supporting == int(initial[3])
+    assert kept > 100 and dropped > 100
