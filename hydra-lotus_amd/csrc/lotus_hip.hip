@@ -13,6 +13,8 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -278,6 +280,8 @@ struct lotus_hip_renderer {
   hipEvent_t join = nullptr;       // this renderer's branch of a batch ends
   float4* svgf_filter = nullptr;   // lotus_hip_svgf's scratch
   uint32_t svgf_pixels = 0;
+  float4* static_scratch = nullptr;  // this device's copy of `scratch` (lotus_hip_svgf_static)
+  std::string static_scratch_error;
 
   // Radiance cache and its frame buffers.
   CacheState cache;
@@ -1438,6 +1442,41 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
 
 }  // namespace
 
+namespace {
+
+// lotus_hip_svgf_static's scratch: SVGF's filter buffers (the a-trous
+// iterations' two halves and the per-pixel guides, 48 B per pixel) for at
+// most kStaticSvgfPixels pixels, 96 MiB in all. The array lives in this
+// library's code object: each GPU the library runs on holds its own copy
+// until the process ends. Host code never passes its name to a kernel: the
+// name is a host-side stand-in, and a kernel given it would fault. Each
+// device's address comes from hipGetSymbolAddress, called once per device
+// with that device current, and only that address is used.
+constexpr uint32_t kStaticSvgfPixels = 1u << 21;  // 2,097,152: 1920 x 1080 fits
+static __device__ __align__(256) float4 scratch[3ull * kStaticSvgfPixels];
+
+std::mutex g_scratch_mutex;
+std::map<int, std::pair<float4*, std::string>> g_scratch;  // device -> (address, error)
+
+// The current device's copy of `scratch`, resolved on the device's first call.
+void resolve_scratch(int device, float4*& address, std::string& error) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch.find(device);
+  if (it == g_scratch.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    std::string message;
+    if (e != hipSuccess || p == nullptr)
+      message = std::string("the static scratch buffer has no device address: ") + hipGetErrorString(e);
+    it = g_scratch.emplace(device, std::make_pair(e == hipSuccess ? static_cast<float4*>(p) : nullptr, message))
+             .first;
+  }
+  address = it->second.first;
+  error = it->second.second;
+}
+
+}  // namespace
+
 extern "C" {
 
 int lotus_hip_probe(char* detail, size_t detail_size) {
@@ -1489,6 +1528,7 @@ lotus_hip_renderer* lotus_hip_create(int device, int* status, char* detail, size
   }
   if (e == hipSuccess) e = dev_alloc(r->counters, frame_counter_words(LOTUS_HIP_MAX_BOUNCES + 4));
   if (e == hipSuccess) e = dev_alloc(r->stats, 8);
+  if (e == hipSuccess) resolve_scratch(device, r->static_scratch, r->static_scratch_error);
   if (e != hipSuccess) {
     if (status) *status = LOTUS_HIP_FAIL;
     set_detail(detail, detail_size, std::string("device setup: ") + hipGetErrorString(e));
@@ -2474,42 +2514,27 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   return LOTUS_HIP_PASS;
 }
 
-int lotus_hip_svgf(lotus_hip_renderer* r, const void* surface, void* radiance, uint32_t width, uint32_t height,
-                   uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations, void* color,
-                   char* detail, size_t detail_size) {
-  if (!r) return LOTUS_HIP_FAIL;
-  if (!r->failure.empty()) {
-    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
-    return LOTUS_HIP_FAIL;
-  }
-  if (!surface || !radiance || width == 0 || height == 0 || batch_samples == 0 || max_history_samples == 0 ||
-      iterations > 8 || (iterations != 0 && !color)) {
-    set_detail(detail, detail_size, "invalid filter input");
-    return LOTUS_HIP_FAIL;
-  }
-  (void)hipSetDevice(r->device);
-  const uint32_t P = width * height;
-  hipError_t e = hipSuccess;
-  if (r->svgf_pixels < P) {
-    dev_free(r->svgf_filter);
-    r->svgf_pixels = 0;
-    e = dev_alloc(r->svgf_filter, 3ull * P);
-    if (e == hipSuccess) r->svgf_pixels = P;
-  }
-  if (e == hipSuccess) {
-    TemporalConst T{};
-    T.width = width;
-    T.height = height;
-    T.fresh_samples = batch_samples;
-    T.cap = max_history_samples;
-    T.iterations = iterations;
-    const Launch L = launch_for(r, r->stream);
-    auto* surf = static_cast<const TemporalPixel*>(surface);
-    auto* rad = static_cast<TemporalRadiancePixel*>(radiance);
-    launch_temporal_variance(L, T, surf, rad, r->svgf_filter);
-    if (iterations != 0) launch_temporal_filter(L, T, surf, rad, r->svgf_filter, static_cast<float4*>(color));
-    e = hipGetLastError();
-  }
+}  // extern "C"
+
+namespace {
+
+// EstimateTemporalVariance and FilterTemporalRadiance over caller device
+// buffers with `filter` (3 float4 per pixel) as the scratch.
+int svgf_run(lotus_hip_renderer* r, const void* surface, void* radiance, uint32_t width, uint32_t height,
+             uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations, void* color, float4* filter,
+             char* detail, size_t detail_size) {
+  TemporalConst T{};
+  T.width = width;
+  T.height = height;
+  T.fresh_samples = batch_samples;
+  T.cap = max_history_samples;
+  T.iterations = iterations;
+  const Launch L = launch_for(r, r->stream);
+  auto* surf = static_cast<const TemporalPixel*>(surface);
+  auto* rad = static_cast<TemporalRadiancePixel*>(radiance);
+  launch_temporal_variance(L, T, surf, rad, filter);
+  if (iterations != 0) launch_temporal_filter(L, T, surf, rad, filter, static_cast<float4*>(color));
+  hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "filter", e));
@@ -2517,6 +2542,70 @@ int lotus_hip_svgf(lotus_hip_renderer* r, const void* surface, void* radiance, u
   }
   set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;
+}
+
+bool svgf_usable(lotus_hip_renderer* r, const void* surface, const void* radiance, uint32_t width, uint32_t height,
+                 uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations, const void* color,
+                 char* detail, size_t detail_size) {
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return false;
+  }
+  if (!surface || !radiance || width == 0 || height == 0 || batch_samples == 0 || max_history_samples == 0 ||
+      iterations > 8 || (iterations != 0 && !color) || uint64_t(width) * height > 0xFFFFFFFFull / 3) {
+    set_detail(detail, detail_size, "invalid filter input");
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lotus_hip_svgf(lotus_hip_renderer* r, const void* surface, void* radiance, uint32_t width, uint32_t height,
+                   uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations, void* color,
+                   char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!svgf_usable(r, surface, radiance, width, height, batch_samples, max_history_samples, iterations, color, detail,
+                   detail_size))
+    return LOTUS_HIP_FAIL;
+  (void)hipSetDevice(r->device);
+  const uint32_t P = width * height;
+  if (r->svgf_pixels < P) {
+    dev_free(r->svgf_filter);
+    r->svgf_pixels = 0;
+    const hipError_t e = dev_alloc(r->svgf_filter, 3ull * P);
+    if (e != hipSuccess) {
+      set_detail(detail, detail_size, latch(r, "filter", e));
+      return LOTUS_HIP_FAIL;
+    }
+    r->svgf_pixels = P;
+  }
+  return svgf_run(r, surface, radiance, width, height, batch_samples, max_history_samples, iterations, color,
+                  r->svgf_filter, detail, detail_size);
+}
+
+int lotus_hip_svgf_static(lotus_hip_renderer* r, const void* surface, void* radiance, uint32_t width,
+                          uint32_t height, uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations,
+                          void* color, char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!svgf_usable(r, surface, radiance, width, height, batch_samples, max_history_samples, iterations, color, detail,
+                   detail_size))
+    return LOTUS_HIP_FAIL;
+  if (uint64_t(width) * height > kStaticSvgfPixels) {
+    set_detail(detail, detail_size,
+               "the static filter scratch holds at most " + std::to_string(kStaticSvgfPixels) + " pixels, not " +
+                   std::to_string(uint64_t(width) * height));
+    return LOTUS_HIP_FAIL;
+  }
+  if (!r->static_scratch) {
+    set_detail(detail, detail_size, r->static_scratch_error);
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  return svgf_run(r, surface, radiance, width, height, batch_samples, max_history_samples, iterations, color,
+                  r->static_scratch, detail, detail_size);
 }
 
 int lotus_hip_write_render_buffer(lotus_hip_renderer* r, const void* product, uint32_t product_format,

@@ -31,6 +31,7 @@ SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hi
            "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning",
            "lotus_hip_cache_configure", "lotus_hip_cache_infer", "lotus_hip_cache_gradients",
            "lotus_hip_cache_train", "lotus_hip_cache_parameters", "lotus_hip_render_temporal", "lotus_hip_svgf",
+           "lotus_hip_svgf_static",
            "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune", "lotus_hip_readback_scene",
            "lotus_hip_write_render_buffer", "lotus_hip_write_ids"]
 BATCH_GRAPH = 1
@@ -339,9 +340,10 @@ def lib():
         L.lotus_hip_write_ids.restype = C.c_int
         L.lotus_hip_write_ids.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32,
                                           C.c_char_p, C.c_size_t]
-        L.lotus_hip_svgf.restype = C.c_int
-        L.lotus_hip_svgf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
-                                     C.c_uint32, C.c_uint32, C.c_void_p, C.c_char_p, C.c_size_t]
+        for name in ("lotus_hip_svgf", "lotus_hip_svgf_static"):
+            getattr(L, name).restype = C.c_int
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_void_p, C.c_char_p, C.c_size_t]
         L.lotus_hip_reproject.restype = C.c_int
         L.lotus_hip_reproject.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, _f32p,
                                           C.c_void_p, C.c_char_p, C.c_size_t]
@@ -727,10 +729,14 @@ class Renderer:
         out["history_available"] = t.history_available
         return out
 
-    def svgf(self, surface_ptr, radiance_ptr, width, height, batch_samples, max_history, iterations, color_ptr=0):
+    def svgf(self, surface_ptr, radiance_ptr, width, height, batch_samples, max_history, iterations, color_ptr=0,
+             static=False):
+        """SVGF over device buffers; static=True takes lotus_hip_svgf_static,
+        which allocates nothing."""
         buf = C.create_string_buffer(256)
-        s = lib().lotus_hip_svgf(self.handle, C.c_void_p(surface_ptr), C.c_void_p(radiance_ptr), width, height,
-                                 batch_samples, max_history, iterations, C.c_void_p(color_ptr), buf, 256)
+        fn = lib().lotus_hip_svgf_static if static else lib().lotus_hip_svgf
+        s = fn(self.handle, C.c_void_p(surface_ptr), C.c_void_p(radiance_ptr), width, height,
+               batch_samples, max_history, iterations, C.c_void_p(color_ptr), buf, 256)
         if s != PASS:
             raise LotusError(s, buf.value.decode())
 

@@ -483,6 +483,20 @@ int lotus_hip_svgf(lotus_hip_renderer* renderer, const void* surface_device, voi
                    uint32_t height, uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations,
                    void* color_device, char* detail, size_t detail_size);
 
+/* lotus_hip_svgf without allocating: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same kernels and
+ * results as lotus_hip_svgf, with the filter's scratch (48 B per pixel) in a
+ * buffer of the library's own, compiled into it, of at most 2,097,152 pixels
+ * (1920 x 1080 fits); a larger image fails with a message and nothing
+ * written. Calls no hipMalloc, hipFree or allocating routine. Every call on
+ * one GPU shares that GPU's buffer, so calls that overlap on one GPU (from
+ * several renderers or threads) race; the buffer keeps the last call's
+ * contents, which no call reads before writing. */
+int lotus_hip_svgf_static(lotus_hip_renderer* renderer, const void* surface_device, void* radiance_device,
+                          uint32_t width, uint32_t height, uint32_t batch_samples, uint32_t max_history_samples,
+                          uint32_t iterations, void* color_device, char* detail, size_t detail_size);
+
 /* History reprojection over caller device buffers (ReprojectTemporalHistory,
  * temporal.cpp:258-300): TemporalValidityPixel records from current and
  * previous TemporalPixel records (previous NULL: no history); validation
