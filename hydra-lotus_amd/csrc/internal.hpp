@@ -94,15 +94,25 @@ struct SceneView {
   const uint32_t* bindings;
   const float2* texcoords;
   const uint32_t* instance_texcoords;
+  // Authored shading normals (3 floats per corner) and vertex colours (3
+  // floats per entry): per instance its first corner, 0xFFFFFFFF for none
+  // (a colour set's bit 31: one colour for the whole mesh). Null arrays: no
+  // instance has them.
+  const float* normals;
+  const uint32_t* instance_normals;
+  const float* colors;
+  const uint32_t* instance_colors;
 };
 
 // The light-table entry of the environment image (gpu_scene.hpp:132-134).
 constexpr uint32_t kEnvironmentLight = 0xFFFFFFFFu;
 
-// Whether the kernels' variants with analytic lights, the environment image
-// and material texture lookups run (the others compile none of that code).
+// Whether the kernels' variants with analytic lights, the environment image,
+// material texture lookups, texture coordinates, authored normals and vertex
+// colours run (the others compile none of that code).
 __host__ __device__ inline bool scene_has_extras(const SceneView& S) {
-  return S.analytic_count != 0 || S.env_width != 0 || S.bindings != nullptr;
+  return S.analytic_count != 0 || S.env_width != 0 || S.bindings != nullptr || S.texcoords != nullptr ||
+         S.instance_normals != nullptr || S.instance_colors != nullptr;
 }
 
 // Device memory of the resident scene, kept until a plan needs a new BVH.
@@ -131,6 +141,11 @@ struct SceneBuffers {
   uint32_t* bindings = nullptr;
   float2* texcoords = nullptr;
   uint32_t* instance_texcoords = nullptr;
+  float* normals = nullptr;
+  uint32_t* instance_normals = nullptr;
+  float* colors = nullptr;
+  uint32_t* instance_colors = nullptr;
+  uint64_t normal_count = 0, color_count = 0;
   uint32_t texture_count = 0, binding_count = 0;
   uint64_t texel_count = 0, texcoord_count = 0;
   // The plan's geometry as uploaded: refits re-place its triangles and

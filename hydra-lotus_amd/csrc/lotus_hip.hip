@@ -218,11 +218,12 @@ struct lotus_hip_renderer {
     std::vector<float> positions, transforms;
     std::vector<uint32_t> triangles, gvo, gvc, gto, gtc, igeo, imat, materials;
     std::vector<lotus_hip_light> lights;
-    // texture lookups: texture sizes and formats, texel bytes, bindings,
-    // texture coordinates and each geometry's first corner
-    std::vector<uint32_t> texture_shape, bindings, texcoord_offset;
+    // texture lookups and per-corner data: texture sizes and formats, texel
+    // bytes, bindings, each instance's first texture-coordinate, normal and
+    // colour corner, and the coordinates, normals and colours
+    std::vector<uint32_t> texture_shape, bindings, instance_texcoords, instance_normals, instance_colors;
     std::vector<uint8_t> texel_bytes;
-    std::vector<float> texcoords;
+    std::vector<float> texcoords, normals, colors;
     bool has_env = false;
     uint32_t env_width = 0, env_height = 0;
     std::vector<float> env_texels;
@@ -345,6 +346,10 @@ void free_scene(lotus_hip_renderer* r) {
   dev_free(s.bindings);
   dev_free(s.texcoords);
   dev_free(s.instance_texcoords);
+  dev_free(s.normals);
+  dev_free(s.instance_normals);
+  dev_free(s.colors);
+  dev_free(s.instance_colors);
   dev_free(s.positions);
   dev_free(s.triangles);
   dev_free(s.geometry_vertex_offset);
@@ -899,6 +904,51 @@ void build_light_table(const lotus_hip_scene_plan* p, LightTableHost& t) {
   if (env_weight > 0.0) t.env_selection = t.lights.back().pdf;
 }
 
+// The plan's per-corner data in one comparable form: texture shapes, texel
+// bytes and bindings; per instance the first corner of the texture
+// coordinates its material reads, of its geometry's shading normals and of
+// its colour set (bit 31: one colour for the whole mesh), 0xFFFFFFFF for
+// none; and the coordinates, normals and colours those name.
+constexpr uint32_t kNoCorner = 0xFFFFFFFFu;
+constexpr uint32_t kConstantColor = 0x80000000u;
+struct TextureData {
+  std::vector<uint32_t> shape, bindings, instance_texcoords, instance_normals, instance_colors;
+  std::vector<uint8_t> texel_bytes;
+  std::vector<float> texcoords, normals, colors;
+};
+
+// Per instance its first texture-coordinate, normal and colour corner.
+void instance_corners(const lotus_hip_scene_plan* p, std::vector<uint32_t>& tc, std::vector<uint32_t>& nm,
+                      std::vector<uint32_t>& col) {
+  const uint32_t I = p->instance_count;
+  tc.assign(I, kNoCorner);
+  nm.assign(I, kNoCorner);
+  col.assign(I, kNoCorner);
+  for (uint32_t i = 0; i < I; ++i) {
+    const uint32_t g = p->instance_geometry[i];
+    if (p->instance_texcoord_offset) tc[i] = p->instance_texcoord_offset[i];
+    else if (p->geometry_texcoord_offset) tc[i] = p->geometry_texcoord_offset[g];
+    if (p->geometry_normal_offset) nm[i] = p->geometry_normal_offset[g];
+    if (p->instance_color_offset && p->instance_color_offset[i] != kNoCorner)
+      col[i] = p->instance_color_offset[i] |
+               (p->instance_color_constant && p->instance_color_constant[i] ? kConstantColor : 0u);
+  }
+}
+
+// The corners the instances name: one past the last texture coordinate,
+// normal and colour entry read.
+void corner_extents(const lotus_hip_scene_plan* p, const std::vector<uint32_t>& tc, const std::vector<uint32_t>& nm,
+                    const std::vector<uint32_t>& col, uint64_t& tcs, uint64_t& nms, uint64_t& cols) {
+  tcs = nms = cols = 0;
+  for (uint32_t i = 0; i < p->instance_count; ++i) {
+    const uint64_t corners = 3ull * p->geometry_triangle_count[p->instance_geometry[i]];
+    if (tc[i] != kNoCorner) tcs = std::max<uint64_t>(tcs, uint64_t(tc[i]) + corners);
+    if (nm[i] != kNoCorner) nms = std::max<uint64_t>(nms, uint64_t(nm[i]) + corners);
+    if (col[i] != kNoCorner)
+      cols = std::max<uint64_t>(cols, uint64_t(col[i] & ~kConstantColor) + ((col[i] & kConstantColor) ? 1 : corners));
+  }
+}
+
 std::string validate_plan(const lotus_hip_scene_plan* p) {
   if (!p) return "no scene plan";
   if (p->material_count == 0 || !p->materials) return "the material table is empty";
@@ -940,11 +990,25 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
       if (!std::isfinite(f)) return "binding " + std::to_string(k) + " holds a non-finite value";
     }
   }
-  if (p->geometry_texcoord_offset) {
-    for (uint32_t g = 0; g < p->geometry_count; ++g) {
-      const uint32_t first = p->geometry_texcoord_offset[g];
-      if (first != 0xFFFFFFFFu && !p->texcoords) return "the plan names texture coordinates but holds none";
-    }
+  uint64_t texels = 0;
+  for (uint32_t k = 0; k < p->texture_count; ++k) texels += uint64_t(p->textures[k].width) * p->textures[k].height;
+  if (texels >= (1ull << 32)) return "the textures hold 2^32 texels or more";
+  {
+    std::vector<uint32_t> tc, nm, col;
+    instance_corners(p, tc, nm, col);
+    uint64_t tcs, nms, cols;
+    corner_extents(p, tc, nm, col, tcs, nms, cols);
+    if (tcs && !p->texcoords) return "the plan names texture coordinates but holds none";
+    if (nms && !p->normals) return "the plan names shading normals but holds none";
+    if (cols && !p->colors) return "the plan names colours but holds none";
+    if (tcs >= (1ull << 32) || nms >= (1ull << 32) || cols >= (1ull << 31))
+      return "the plan's per-corner data holds too many corners";
+    for (uint64_t k = 0; k < 2 * tcs; ++k)
+      if (!std::isfinite(p->texcoords[k])) return "the plan has a non-finite texture coordinate";
+    if (p->instance_color_offset)
+      for (uint32_t i = 0; i < p->instance_count; ++i)
+        if (p->instance_color_offset[i] != kNoCorner && p->instance_color_offset[i] >= kConstantColor)
+          return "a colour set's first entry lies beyond 2^31";
   }
   if (p->environment_map) {
     const lotus_hip_environment_map& e = *p->environment_map;
@@ -973,8 +1037,8 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   for (uint32_t m = 0; m < p->material_count; ++m) {
     const uint32_t* w = p->materials + 36ull * m;
     if ((w[0] & 0xFFu) != 0) return "material " + std::to_string(m) + ": only UsdPreviewSurface is supported";
-    if (w[3] != 0 || (w[2] & ~0x7Cu) != 0)
-      return "material " + std::to_string(m) + ": only base colour, roughness, metallic, emission and specular colour lookups are supported";
+    if (w[3] != 0 || (w[2] & ~0x7Eu) != 0)
+      return "material " + std::to_string(m) + ": only normal, base colour, roughness, metallic, emission and specular colour lookups are supported";
     if (w[2] != 0 && (!p->bindings || uint64_t(w[1]) + __builtin_popcount(w[2]) > p->binding_count))
       return "material " + std::to_string(m) + ": its lookups lie outside the binding table";
     float opacity;
@@ -1018,14 +1082,6 @@ bool same_words(const std::vector<T>& held, const T* src, uint64_t n) {
 // An instance of this material enters the light table (gpu_scene.cpp:144-203).
 bool material_emits(const uint32_t* m, const uint32_t* bindings) { return emission_estimate(m, bindings) > 0.0; }
 
-// The plan's texture lookup data in one comparable form: texture shapes,
-// texel bytes, bindings, texture coordinates (the corners the geometries
-// name) and each geometry's first corner.
-struct TextureData {
-  std::vector<uint32_t> shape, bindings, texcoord_offset;
-  std::vector<uint8_t> texel_bytes;
-  std::vector<float> texcoords;
-};
 TextureData texture_data(const lotus_hip_scene_plan* p) {
   TextureData d;
   for (uint32_t k = 0; k < p->texture_count; ++k) {
@@ -1036,14 +1092,12 @@ TextureData texture_data(const lotus_hip_scene_plan* p) {
     d.texel_bytes.insert(d.texel_bytes.end(), t, t + bytes);
   }
   if (p->binding_count) d.bindings.assign(p->bindings, p->bindings + 24ull * p->binding_count);
-  if (p->geometry_texcoord_offset) {
-    d.texcoord_offset.assign(p->geometry_texcoord_offset, p->geometry_texcoord_offset + p->geometry_count);
-    uint64_t corners = 0;
-    for (uint32_t g = 0; g < p->geometry_count; ++g)
-      if (d.texcoord_offset[g] != 0xFFFFFFFFu)
-        corners = std::max<uint64_t>(corners, uint64_t(d.texcoord_offset[g]) + 3ull * p->geometry_triangle_count[g]);
-    d.texcoords.assign(p->texcoords, p->texcoords + 2 * corners);
-  }
+  instance_corners(p, d.instance_texcoords, d.instance_normals, d.instance_colors);
+  uint64_t tcs, nms, cols;
+  corner_extents(p, d.instance_texcoords, d.instance_normals, d.instance_colors, tcs, nms, cols);
+  if (tcs) d.texcoords.assign(p->texcoords, p->texcoords + 2 * tcs);
+  if (nms) d.normals.assign(p->normals, p->normals + 3 * nms);
+  if (cols) d.colors.assign(p->colors, p->colors + 3 * cols);
   return d;
 }
 
@@ -1072,8 +1126,10 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
   if (!topology) return c;
   // texture lookups are uploaded with a new scene (a change of them builds)
   const TextureData tex = texture_data(p);
-  if (tex.shape != old.texture_shape || tex.bindings != old.bindings || tex.texcoord_offset != old.texcoord_offset ||
-      tex.texel_bytes != old.texel_bytes || tex.texcoords != old.texcoords)
+  if (tex.shape != old.texture_shape || tex.bindings != old.bindings ||
+      tex.instance_texcoords != old.instance_texcoords || tex.instance_normals != old.instance_normals ||
+      tex.instance_colors != old.instance_colors || tex.texel_bytes != old.texel_bytes ||
+      tex.texcoords != old.texcoords || tex.normals != old.normals || tex.colors != old.colors)
     return c;
   if (!same_words(old.positions, p->positions, sh.vertex_words)) {
     for (uint32_t g = 0; g < sh.geometries; ++g) {
@@ -1125,9 +1181,13 @@ void record_plan(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
     TextureData tex = texture_data(p);
     d.texture_shape = std::move(tex.shape);
     d.bindings = std::move(tex.bindings);
-    d.texcoord_offset = std::move(tex.texcoord_offset);
+    d.instance_texcoords = std::move(tex.instance_texcoords);
+    d.instance_normals = std::move(tex.instance_normals);
+    d.instance_colors = std::move(tex.instance_colors);
     d.texel_bytes = std::move(tex.texel_bytes);
     d.texcoords = std::move(tex.texcoords);
+    d.normals = std::move(tex.normals);
+    d.colors = std::move(tex.colors);
     d.positions.assign(p->positions, p->positions + sh.vertex_words);
     d.triangles.assign(p->triangles, p->triangles + sh.index_words);
     d.gvo.assign(p->geometry_vertex_offset, p->geometry_vertex_offset + sh.geometries);
@@ -1194,8 +1254,12 @@ void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environme
   v.textures = s.textures;
   v.texels = s.texels;
   v.bindings = s.binding_count ? s.bindings : nullptr;
-  v.texcoords = s.texcoords;
+  v.texcoords = s.texcoord_count ? s.texcoords : nullptr;
   v.instance_texcoords = s.instance_texcoords;
+  v.normals = s.normals;
+  v.instance_normals = s.normal_count ? s.instance_normals : nullptr;
+  v.colors = s.colors;
+  v.instance_colors = s.color_count ? s.instance_colors : nullptr;
   v.tri_count = sh.tri_count;
   v.instance_count = sh.instances;
   for (int c = 0; c < 3; ++c) v.environment[c] = environment[c];
@@ -1210,7 +1274,8 @@ uint64_t scene_device_bytes(const lotus_hip_renderer* r) {
          4ull * s.distant_count + 16ull * s.env_width * s.env_height +
          4ull * (s.env_columns == 0 ? 0 : uint64_t(s.env_columns) * s.env_rows + s.env_rows + 1 +
                                               uint64_t(s.env_rows) * (s.env_columns + 1)) +
-         16ull * (s.texture_count + s.texel_count) + 96ull * s.binding_count + 8ull * s.texcoord_count + 4ull * I;
+         16ull * (s.texture_count + s.texel_count) + 96ull * s.binding_count + 8ull * s.texcoord_count + 4ull * I +
+         12ull * (s.normal_count + s.color_count) + 8ull * I;
 }
 
 void fill_evidence(const lotus_hip_renderer* r, lotus_hip_scene_evidence* ev, uint32_t update, uint32_t moved,
@@ -1342,24 +1407,31 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
       texels.push_back(make_float4(c[0], c[1], c[2], c[3]));
     }
   }
-  std::vector<uint32_t> instance_texcoords(I, 0xFFFFFFFFu);
   const TextureData tex = texture_data(p);
-  if (p->geometry_texcoord_offset)
-    for (uint32_t i = 0; i < I; ++i) instance_texcoords[i] = p->geometry_texcoord_offset[p->instance_geometry[i]];
   s.texture_count = p->texture_count;
   s.binding_count = p->binding_count;
   s.texel_count = texels.size();
   s.texcoord_count = tex.texcoords.size() / 2;
+  s.normal_count = tex.normals.size() / 3;
+  s.color_count = tex.colors.size() / 3;
   if (e == hipSuccess) e = dev_alloc(s.textures, textures.size());
   if (e == hipSuccess) e = dev_alloc(s.texels, texels.size());
   if (e == hipSuccess) e = dev_alloc(s.bindings, tex.bindings.size());
   if (e == hipSuccess) e = dev_alloc(s.texcoords, tex.texcoords.size() / 2);
   if (e == hipSuccess) e = dev_alloc(s.instance_texcoords, I);
+  if (e == hipSuccess) e = dev_alloc(s.normals, tex.normals.size());
+  if (e == hipSuccess) e = dev_alloc(s.instance_normals, I);
+  if (e == hipSuccess) e = dev_alloc(s.colors, tex.colors.size());
+  if (e == hipSuccess) e = dev_alloc(s.instance_colors, I);
   up(s.textures, textures.data(), textures.size() * sizeof(uint4));
   up(s.texels, texels.data(), texels.size() * sizeof(float4));
   up(s.bindings, tex.bindings.data(), tex.bindings.size() * 4);
   up(s.texcoords, tex.texcoords.data(), tex.texcoords.size() * 4);
-  up(s.instance_texcoords, instance_texcoords.data(), instance_texcoords.size() * 4);
+  up(s.instance_texcoords, tex.instance_texcoords.data(), tex.instance_texcoords.size() * 4);
+  up(s.normals, tex.normals.data(), tex.normals.size() * 4);
+  up(s.instance_normals, tex.instance_normals.data(), tex.instance_normals.size() * 4);
+  up(s.colors, tex.colors.data(), tex.colors.size() * 4);
+  up(s.instance_colors, tex.instance_colors.data(), tex.instance_colors.size() * 4);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (e != hipSuccess) {

@@ -101,10 +101,12 @@ __device__ __forceinline__ Ray start_path(const FrameConst& f, uint32_t x, uint3
 // ---------------------------------------------------- material textures
 // Texture-input slots (scene.slang:59-67), lookup modes (gpu_scene.hpp:56-59)
 // and wrap modes (TextureWrap, material.hpp:43-50).
-constexpr uint32_t kBaseColorSlot = 2, kRoughnessSlot = 3, kMetallicSlot = 4, kEmissionSlot = 5,
+constexpr uint32_t kNormalSlot = 1, kBaseColorSlot = 2, kRoughnessSlot = 3, kMetallicSlot = 4, kEmissionSlot = 5,
                    kSpecularColorSlot = 6;
-constexpr uint32_t kPreviewLookupSlots = (1u << kBaseColorSlot) | (1u << kRoughnessSlot) | (1u << kMetallicSlot) |
-                                         (1u << kEmissionSlot) | (1u << kSpecularColorSlot);
+constexpr uint32_t kPreviewLookupSlots = (1u << kNormalSlot) | (1u << kBaseColorSlot) | (1u << kRoughnessSlot) |
+                                         (1u << kMetallicSlot) | (1u << kEmissionSlot) | (1u << kSpecularColorSlot);
+constexpr uint32_t kNoCorner = 0xFFFFFFFFu;
+constexpr uint32_t kConstantColor = 0x80000000u;
 constexpr uint32_t kFallbackLookup = 2;
 constexpr uint32_t kWrapBlack = 0, kWrapRepeat = 2, kWrapMirror = 3;
 
@@ -189,10 +191,96 @@ __device__ __forceinline__ V3 preview_emission(const SceneView& S, const uint32_
   return V3{maxf(e.x, 0.0f), maxf(e.y, 0.0f), maxf(e.z, 0.0f)};
 }
 
-// scene.slang:1408-1449 with EvaluatePreview (:1184-1260) for constant
-// UsdPreviewSurface inputs and MappedNormal (:539-553) for a constant
-// tangent-space normal.
-template <bool kExtras = true>  // texture lookups compiled in
+// scene.slang:467-488 (ShadingNormal): the corner normals at `corner`
+// interpolated at the hit, carried to world space by the cofactor matrix of
+// the instance's linear part, turned to the geometric normal's side; the
+// geometric normal where the result vanishes or faces away from the ray.
+__device__ __forceinline__ V3 authored_normal(const SceneView& S, uint32_t corner, float u, float v, uint32_t instance,
+                                              V3 geometric, V3 dir) {
+  const float* n = S.normals + 3ull * corner;
+  const V3 n0{n[0], n[1], n[2]}, n1{n[3], n[4], n[5]}, n2{n[6], n[7], n[8]};
+  const V3 object = (n0 + u * (n1 - n0)) + v * (n2 - n0);
+  const float* x = S.instance_transform + 16ull * instance;
+  float a[4], b[4], c[4];
+  mul_point(x, 1.0f, 0.0f, 0.0f, 0.0f, a);
+  mul_point(x, 0.0f, 1.0f, 0.0f, 0.0f, b);
+  mul_point(x, 0.0f, 0.0f, 1.0f, 0.0f, c);
+  const V3 c0{a[0], a[1], a[2]}, c1{b[0], b[1], b[2]}, c2{c[0], c[1], c[2]};
+  const V3 world = (cross3(c1, c2) * object.x + cross3(c2, c0) * object.y) + cross3(c0, c1) * object.z;
+  const float length2 = dot3(world, world);
+  if (!(length2 > 0.0f) || isinf(length2)) return geometric;
+  V3 shading = world * rsqrt_rn(length2);
+  if (dot3(shading, geometric) < 0.0f) shading = -shading;
+  return dot3(shading, dir) < 0.0f ? shading : geometric;
+}
+
+// scene.slang:376-385 (HitColor): the instance's colour set at the hit.
+__device__ __forceinline__ V3 hit_color(const SceneView& S, uint32_t set, uint32_t primitive, float u, float v) {
+  if (set & kConstantColor) {
+    const float* c = S.colors + 3ull * (set & ~kConstantColor);
+    return V3{c[0], c[1], c[2]};
+  }
+  const float* c = S.colors + 3ull * (uint64_t(set) + 3ull * primitive);
+  const V3 c0{c[0], c[1], c[2]}, c1{c[3], c[4], c[5]}, c2{c[6], c[7], c[8]};
+  return (c0 + u * (c1 - c0)) + v * (c2 - c0);
+}
+
+// scene.slang:490-537 (TangentFrame): the shading frame about `normal` whose
+// tangent follows the texture coordinates' s direction (the normal map's
+// transform applied to their differentials), or the normal-only frame
+// (MakeShadingFrame, sampling.slang:45-56) without usable coordinates.
+__device__ __forceinline__ void tangent_frame(const SceneView& S, const uint32_t* input, uint32_t instance,
+                                              uint32_t primitive, V3 edge1, V3 edge2, V3 normal, V3& ft, V3& fb) {
+  {
+    const V3 n = normal;
+    const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    ft = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    fb = V3{b, sign + n.y * n.y * a, -n.y};
+  }
+  const uint32_t first = S.instance_texcoords ? S.instance_texcoords[instance] : kNoCorner;
+  if (first == kNoCorner) return;
+  const float2* t = S.texcoords + (uint64_t(first) + 3ull * primitive);
+  const float2 st0 = t[0];
+  float u1x = t[1].x - st0.x, u1y = t[1].y - st0.y;
+  float u2x = t[2].x - st0.x, u2y = t[2].y - st0.y;
+  if (input) {
+    const float* f = reinterpret_cast<const float*>(input);
+    const float a1 = f[16] * u1x + f[17] * u1y, b1 = f[20] * u1x + f[21] * u1y;
+    const float a2 = f[16] * u2x + f[17] * u2y, b2 = f[20] * u2x + f[21] * u2y;
+    u1x = a1;
+    u1y = b1;
+    u2x = a2;
+    u2y = b2;
+  }
+  const float uv_scale = maxf(maxf(fabsf(u1x), fabsf(u1y)), maxf(fabsf(u2x), fabsf(u2y)));
+  const float edge_scale = maxf(maxf(maxf(fabsf(edge1.x), fabsf(edge1.y)), fabsf(edge1.z)),
+                                maxf(maxf(fabsf(edge2.x), fabsf(edge2.y)), fabsf(edge2.z)));
+  if (!(uv_scale > 0.0f) || isinf(uv_scale) || !(edge_scale > 0.0f) || isinf(edge_scale)) return;
+  u1x = u1x / uv_scale;
+  u1y = u1y / uv_scale;
+  u2x = u2x / uv_scale;
+  u2y = u2y / uv_scale;
+  edge1 = edge1 / edge_scale;
+  edge2 = edge2 / edge_scale;
+  const float determinant = u1x * u2y - u1y * u2x;
+  if (!(fabsf(determinant) > 1.0e-8f)) return;
+  const float orientation = determinant < 0.0f ? -1.0f : 1.0f;
+  V3 tangent = orientation * (edge1 * u2y - edge2 * u1y);
+  const V3 bitangent = orientation * (edge2 * u1x - edge1 * u2x);
+  tangent = tangent - normal * dot3(normal, tangent);
+  const float length2 = dot3(tangent, tangent);
+  if (!(length2 > 1.0e-16f) || isinf(length2)) return;
+  ft = tangent * rsqrt_rn(length2);
+  fb = cross3(normal, ft);
+  if (dot3(fb, bitangent) < 0.0f) fb = -fb;
+}
+
+// scene.slang:1408-1449 with EvaluatePreview (:1184-1260): authored shading
+// normals, vertex colours, texture lookups (with kExtras) and MappedNormal
+// (:541-553) for the tangent-space normal, constant or mapped.
+template <bool kExtras = true>  // normals, colours and texture lookups compiled in
 __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t tri, uint32_t instance,
                                                    float u, float v, V3 dir) {
   const Tri t = S.tris[tri];
@@ -204,34 +292,32 @@ __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t 
   sf.area = 0.5f * length3(e);
   sf.normal = dot3(normal, dir) > 0.0f ? -normal : normal;
   sf.shading_normal = sf.normal;
+  const uint32_t primitive = __float_as_uint(t.c.w);
+  if (kExtras && S.instance_normals) {
+    const uint32_t first = S.instance_normals[instance];
+    if (first != kNoCorner)
+      sf.shading_normal = authored_normal(S, first + 3u * primitive, u, v, instance, sf.normal, dir);
+  }
   const uint32_t* m = S.materials + 36ull * S.instance_material[instance];
   const uint32_t flags = m[0] >> 8;
-  const V3 tn = V3{word_float(m, 8), word_float(m, 9), word_float(m, 10)};
-  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f)) {
-    // MakeShadingFrame (sampling.slang:45-56) then MappedNormal.
-    const V3 n = sf.shading_normal;
-    const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
-    const float a = -1.0f / (sign + n.z);
-    const float b = n.x * n.y * a;
-    const V3 bt = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
-    const V3 bb = V3{b, sign + n.y * n.y * a, -n.y};
-    const float mag = maxf(maxf(fabsf(tn.x), fabsf(tn.y)), fabsf(tn.z));
-    if (mag > 0.0f && finite3(tn)) {
-      const V3 q = tn / mag;
-      V3 sh = normalize3(((q.x * bt) + (q.y * bb)) + (q.z * n));
-      if (dot3(sh, sf.normal) < 0.0f) sh = -sh;
-      sf.shading_normal = dot3(sh, dir) < 0.0f ? sh : n;
+  V3 tn = V3{word_float(m, 8), word_float(m, 9), word_float(m, 10)};
+  sf.base_color = V3{word_float(m, 12), word_float(m, 13), word_float(m, 14)};
+  if (kExtras && S.instance_colors) {
+    const uint32_t set = S.instance_colors[instance];
+    if (set != kNoCorner) {
+      const V3 c = hit_color(S, set, primitive, u, v);
+      sf.base_color = V3{saturate(c.x), saturate(c.y), saturate(c.z)};
     }
   }
-  sf.base_color = V3{word_float(m, 12), word_float(m, 13), word_float(m, 14)};
   sf.roughness = word_float(m, 15);
   sf.emission = V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)};
   sf.metallic = word_float(m, 19);
   V3 specular = V3{word_float(m, 20), word_float(m, 21), word_float(m, 22)};
+  const uint32_t* normal_input = nullptr;
   if (kExtras && (m[2] & kPreviewLookupSlots)) {
     // a lookup replaces its input's constant, clamped into its range
     float s, tc;
-    hit_texcoords(S, m, instance, __float_as_uint(t.c.w), u, v, s, tc);
+    hit_texcoords(S, m, instance, primitive, u, v, s, tc);
     if (const uint32_t* b = material_input(S, m, kBaseColorSlot)) {
       const float4 c = texture_lookup(S, b, s, tc);
       sf.base_color = V3{saturate(c.x), saturate(c.y), saturate(c.z)};
@@ -241,10 +327,36 @@ __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t 
     if (const uint32_t* b = material_input(S, m, kMetallicSlot))
       sf.metallic = saturate(channel_of(texture_lookup(S, b, s, tc), b[2]));
     sf.emission = preview_emission(S, m, s, tc);
+    if (const uint32_t* b = material_input(S, m, kNormalSlot)) {
+      const float4 c = texture_lookup(S, b, s, tc);
+      tn = V3{minf(maxf(c.x, -1.0f), 1.0f), minf(maxf(c.y, -1.0f), 1.0f), minf(maxf(c.z, -1.0f), 1.0f)};
+      normal_input = b;
+    }
     if (const uint32_t* b = material_input(S, m, kSpecularColorSlot)) {
       const float4 c = texture_lookup(S, b, s, tc);
       const V3 value{c.x, c.y, c.z};
       if (finite3(value)) specular = V3{saturate(value.x), saturate(value.y), saturate(value.z)};
+    }
+  }
+  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f)) {
+    // MappedNormal (scene.slang:541-553) in the tangent frame
+    const V3 n = sf.shading_normal;
+    V3 bt, bb;
+    if (kExtras) {
+      tangent_frame(S, normal_input, instance, primitive, p1 - p0, p2 - p0, n, bt, bb);
+    } else {
+      const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+      const float a = -1.0f / (sign + n.z);
+      const float b = n.x * n.y * a;
+      bt = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+      bb = V3{b, sign + n.y * n.y * a, -n.y};
+    }
+    const float mag = maxf(maxf(fabsf(tn.x), fabsf(tn.y)), fabsf(tn.z));
+    if (mag > 0.0f && finite3(tn)) {
+      const V3 q = tn / mag;
+      V3 sh = normalize3(((q.x * bt) + (q.y * bb)) + (q.z * n));
+      if (dot3(sh, sf.normal) < 0.0f) sh = -sh;
+      sf.shading_normal = dot3(sh, dir) < 0.0f ? sh : n;
     }
   }
   const float ior = word_float(m, 23);

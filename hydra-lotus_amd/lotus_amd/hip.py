@@ -79,6 +79,12 @@ class ScenePlan(C.Structure):
         ("bindings", C.c_void_p),
         ("texcoords", C.c_void_p),
         ("geometry_texcoord_offset", C.c_void_p),
+        ("instance_texcoord_offset", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("geometry_normal_offset", C.c_void_p),
+        ("colors", C.c_void_p),
+        ("instance_color_offset", C.c_void_p),
+        ("instance_color_constant", C.c_void_p),
     ]
 
 
@@ -108,8 +114,17 @@ def set_texture_fields(desc, p):
     desc.binding_count = 0 if b is None else len(b) // 24
     desc.bindings = b.ctypes.data if desc.binding_count else None
     off = p.get("geometry_texcoord_offset")
-    desc.texcoords = p["texcoords"].ctypes.data if off is not None else None
+    ioff = p.get("instance_texcoord_offset")
+    desc.texcoords = p["texcoords"].ctypes.data if off is not None or ioff is not None else None
     desc.geometry_texcoord_offset = off.ctypes.data if off is not None else None
+    desc.instance_texcoord_offset = ioff.ctypes.data if ioff is not None else None
+    noff = p.get("geometry_normal_offset")
+    desc.normals = p["normals"].ctypes.data if noff is not None else None
+    desc.geometry_normal_offset = noff.ctypes.data if noff is not None else None
+    coff = p.get("instance_color_offset")
+    desc.colors = p["colors"].ctypes.data if coff is not None else None
+    desc.instance_color_offset = coff.ctypes.data if coff is not None else None
+    desc.instance_color_constant = p["instance_color_constant"].ctypes.data if coff is not None else None
     return tex
 
 

@@ -133,6 +133,13 @@ class Scene:
     textures: List[dict] = dataclasses.field(default_factory=list)
     bindings: List[np.ndarray] = dataclasses.field(default_factory=list)
     texcoords: Dict[int, np.ndarray] = dataclasses.field(default_factory=dict)
+    # per instance the texture-coordinate set its material reads, when not
+    # its geometry's ([triangles * 3, 2]); per geometry authored shading
+    # normals ([triangles * 3, 3]); per instance its colour set, as
+    # (colours [n, 3], constant) with n = 1 when constant, else one per corner
+    instance_texcoords: Dict[int, np.ndarray] = dataclasses.field(default_factory=dict)
+    normals: Dict[int, np.ndarray] = dataclasses.field(default_factory=dict)
+    colors: Dict[int, Tuple[np.ndarray, bool]] = dataclasses.field(default_factory=dict)
     view: np.ndarray = dataclasses.field(default_factory=identity)
     projection: np.ndarray = dataclasses.field(default_factory=identity)
     name: str = "scene"
@@ -164,6 +171,31 @@ class Scene:
         self.texcoords[geometry] = c
         self._packed = None
 
+    def set_normals(self, geometry: int, corners) -> None:
+        """Authored object-space shading normals of a geometry, 3 per triangle."""
+        n = np.ascontiguousarray(corners, F32).reshape(-1, 3)
+        assert n.shape[0] == 3 * self.geometries[geometry][1].shape[0]
+        self.normals[geometry] = n
+        self._packed = None
+
+    def set_colors(self, instance: int, colors, constant: bool = False) -> None:
+        """The colour set an instance's base colour reads: one colour, or one
+        per triangle corner of its geometry."""
+        c = np.ascontiguousarray(colors, F32).reshape(-1, 3)
+        g = self.instances[instance][0]
+        assert c.shape[0] == (1 if constant else 3 * self.geometries[g][1].shape[0])
+        self.colors[instance] = (c, bool(constant))
+        self._packed = None
+
+    def set_instance_texcoords(self, instance: int, corners) -> None:
+        """The texture coordinates an instance's material reads (one set of
+        its geometry's), 3 per triangle."""
+        c = np.ascontiguousarray(corners, F32).reshape(-1, 2)
+        g = self.instances[instance][0]
+        assert c.shape[0] == 3 * self.geometries[g][1].shape[0]
+        self.instance_texcoords[instance] = c
+        self._packed = None
+
     def add_light(self, record: np.ndarray) -> int:
         self.lights.append(np.array(record, LIGHT_DTYPE))
         self._packed = None
@@ -182,6 +214,9 @@ class Scene:
                      lights=[l.copy() for l in self.lights],
                      textures=[dict(t) for t in self.textures], bindings=[b.copy() for b in self.bindings],
                      texcoords={g: c.copy() for g, c in self.texcoords.items()},
+                     instance_texcoords={i: c.copy() for i, c in self.instance_texcoords.items()},
+                     normals={g: n.copy() for g, n in self.normals.items()},
+                     colors={i: (c.copy(), k) for i, (c, k) in self.colors.items()},
                      environment_map=None if self.environment_map is None else
                      {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
                       for k, v in self.environment_map.items()},
@@ -204,19 +239,63 @@ class Scene:
         self.materials[slot] = np.ascontiguousarray(record, np.uint32)
         self._packed = None
 
-    def _packed_texcoords(self) -> dict:
-        if not self.texcoords:
-            return dict(texcoords=np.zeros(2, F32), geometry_texcoord_offset=None)
-        offset, parts, n = [], [], 0
-        for g in range(len(self.geometries)):
-            if g in self.texcoords:
-                offset.append(n)
-                parts.append(self.texcoords[g])
-                n += self.texcoords[g].shape[0]
-            else:
-                offset.append(0xFFFFFFFF)
-        return dict(texcoords=np.ascontiguousarray(np.concatenate(parts).astype(F32)),
-                    geometry_texcoord_offset=np.array(offset, np.uint32))
+    def _packed_corners(self) -> dict:
+        """Texture coordinates, normals and colours with their first corners:
+        per geometry (coordinates, normals) and per instance (coordinate
+        overrides, colour sets); None where the scene has none."""
+        none = 0xFFFFFFFF
+        out = dict(texcoords=np.zeros(2, F32), geometry_texcoord_offset=None, instance_texcoord_offset=None,
+                   normals=np.zeros(3, F32), geometry_normal_offset=None, colors=np.zeros(3, F32),
+                   instance_color_offset=None, instance_color_constant=None)
+        if self.texcoords or self.instance_texcoords:
+            parts, n, goff = [], 0, []
+            for g in range(len(self.geometries)):
+                if g in self.texcoords:
+                    goff.append(n)
+                    parts.append(self.texcoords[g])
+                    n += self.texcoords[g].shape[0]
+                else:
+                    goff.append(none)
+            ioff = None
+            if self.instance_texcoords:
+                ioff = []
+                for i, (g, _, _) in enumerate(self.instances):
+                    if i in self.instance_texcoords:
+                        ioff.append(n)
+                        parts.append(self.instance_texcoords[i])
+                        n += self.instance_texcoords[i].shape[0]
+                    else:
+                        ioff.append(goff[g])
+            out["texcoords"] = np.ascontiguousarray(np.concatenate(parts).astype(F32))
+            out["geometry_texcoord_offset"] = np.array(goff, np.uint32)
+            out["instance_texcoord_offset"] = None if ioff is None else np.array(ioff, np.uint32)
+        if self.normals:
+            parts, n, goff = [], 0, []
+            for g in range(len(self.geometries)):
+                if g in self.normals:
+                    goff.append(n)
+                    parts.append(self.normals[g])
+                    n += self.normals[g].shape[0]
+                else:
+                    goff.append(none)
+            out["normals"] = np.ascontiguousarray(np.concatenate(parts).astype(F32))
+            out["geometry_normal_offset"] = np.array(goff, np.uint32)
+        if self.colors:
+            parts, n, ioff, const = [], 0, [], []
+            for i in range(len(self.instances)):
+                if i in self.colors:
+                    c, k = self.colors[i]
+                    ioff.append(n)
+                    const.append(1 if k else 0)
+                    parts.append(c)
+                    n += c.shape[0]
+                else:
+                    ioff.append(none)
+                    const.append(0)
+            out["colors"] = np.ascontiguousarray(np.concatenate(parts).astype(F32))
+            out["instance_color_offset"] = np.array(ioff, np.uint32)
+            out["instance_color_constant"] = np.array(const, np.uint32)
+        return out
 
     @property
     def world_to_clip(self) -> np.ndarray:
@@ -262,7 +341,7 @@ class Scene:
             lights=np.ascontiguousarray(np.array(self.lights, LIGHT_DTYPE).reshape(-1)),
             textures=list(self.textures),
             bindings=np.ascontiguousarray(np.array(self.bindings, np.uint32).reshape(-1)),
-            **self._packed_texcoords(),
+            **self._packed_corners(),
             environment_map=None if self.environment_map is None else dict(
                 texels=np.ascontiguousarray(self.environment_map["texels"], F32),
                 scale=np.asarray(self.environment_map.get("scale", (1.0, 1.0, 1.0)), F32),

@@ -119,6 +119,25 @@ typedef struct lotus_hip_scene_plan {
   const uint32_t* bindings;
   const float* texcoords;
   const uint32_t* geometry_texcoord_offset; /* per geometry; NULL: no geometry has texture coordinates */
+  /* Per instance the first corner of the texture-coordinate set its material
+   * reads (SceneInstance::texcoords, include/lotus/extraction.hpp:27-36),
+   * 0xFFFFFFFF for none; NULL: each instance reads its geometry's
+   * (geometry_texcoord_offset). */
+  const uint32_t* instance_texcoord_offset;
+  /* Authored shading normals (MeshGeometry::normals, render_world.hpp:43-55;
+   * ShadingNormal, scene.slang:467-488): object-space normals, 3 floats per
+   * triangle corner, and each geometry's first corner, 0xFFFFFFFF for a
+   * geometry without them; NULL: none. */
+  const float* normals;
+  const uint32_t* geometry_normal_offset;
+  /* Vertex colours (MeshGeometry::colors; HitColor, scene.slang:376-385),
+   * 3 floats per entry, and per instance the first entry of the colour set
+   * its material's base colour reads (SceneInstance::colors), 0xFFFFFFFF for
+   * none (at most 2^31 - 1), with whether that set is one colour for the
+   * whole mesh (1) or one per triangle corner (0); NULL: none. */
+  const float* colors;
+  const uint32_t* instance_color_offset;
+  const uint32_t* instance_color_constant;
 } lotus_hip_scene_plan;
 
 /* What a lotus_hip_scene_apply did, the counterpart of GpuScene::Apply's

@@ -58,6 +58,12 @@ class SceneDesc(C.Structure):
         ("bindings", C.c_void_p),
         ("texcoords", C.c_void_p),
         ("geometry_texcoord_offset", C.c_void_p),
+        ("instance_texcoord_offset", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("geometry_normal_offset", C.c_void_p),
+        ("colors", C.c_void_p),
+        ("instance_color_offset", C.c_void_p),
+        ("instance_color_constant", C.c_void_p),
     ]
 
 
@@ -170,6 +176,8 @@ def ref():
         R.ref_reproject.restype = None
         R.ref_reproject.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, _f32p,
                                     _u32p]
+        R.ref_material_fixture.restype = C.c_size_t
+        R.ref_material_fixture.argtypes = [C.c_char_p, C.c_int, _u32p, C.c_size_t]
         R.ref_reuse_normalization.restype = C.c_double
         R.ref_reuse_normalization.argtypes = [_u32p, C.c_uint32]
         _ref = R

@@ -584,6 +584,11 @@ struct Textures {
   std::vector<uint32_t> bindings;
   std::vector<float> texcoords;
   std::vector<uint32_t> instance_texcoords;
+  // authored shading normals (3 per corner) and vertex colours (3 per
+  // entry): per instance its first corner or 0xFFFFFFFF, and whether its
+  // colour set is one colour for the whole mesh
+  std::vector<float> normals, colors;
+  std::vector<uint32_t> instance_normals, instance_colors, instance_color_constant;
 };
 
 struct Scene {
@@ -1044,6 +1049,86 @@ F3 PreviewEmission(const Scene& s, const Material& m, float st_s, float st_t) {
   return F3{std::max(e[0], 0.0f), std::max(e[1], 0.0f), std::max(e[2], 0.0f)};
 }
 
+// scene.slang:467-488 (ShadingNormal): the instance's corner normals
+// interpolated at the hit, in world space by the cofactor matrix of the
+// linear part, on the geometric normal's side; the geometric normal where
+// that vanishes or faces away from the ray.
+F3 AuthoredNormal(const Scene& s, const Hit& hit, F3 geometric, F3 dir) {
+  const float* n = &s.tex.normals[3 * (size_t(s.tex.instance_normals[hit.instance]) + 3 * size_t(hit.primitive))];
+  const F3 n0{n[0], n[1], n[2]}, n1{n[3], n[4], n[5]}, n2{n[6], n[7], n[8]};
+  const F3 object = (n0 + hit.u * (n1 - n0)) + hit.v * (n2 - n0);
+  const float* t = &s.instance_transform[16 * size_t(hit.instance)];
+  const F3 c0 = TransformVector(t, F3{1, 0, 0});
+  const F3 c1 = TransformVector(t, F3{0, 1, 0});
+  const F3 c2 = TransformVector(t, F3{0, 0, 1});
+  const F3 world = (Cross(c1, c2) * object.x + Cross(c2, c0) * object.y) + Cross(c0, c1) * object.z;
+  const float length2 = Dot(world, world);
+  if (!(length2 > 0.0f) || std::isinf(length2)) return geometric;
+  F3 shading = world * Rsqrt(length2);
+  if (Dot(shading, geometric) < 0.0f) shading = -shading;
+  return Dot(shading, dir) < 0.0f ? shading : geometric;
+}
+
+// scene.slang:376-385 (HitColor).
+F3 HitColor(const Scene& s, const Hit& hit) {
+  const size_t first = s.tex.instance_colors[hit.instance];
+  if (s.tex.instance_color_constant[hit.instance]) {
+    const float* c = &s.tex.colors[3 * first];
+    return F3{c[0], c[1], c[2]};
+  }
+  const float* c = &s.tex.colors[3 * (first + 3 * size_t(hit.primitive))];
+  const F3 c0{c[0], c[1], c[2]}, c1{c[3], c[4], c[5]}, c2{c[6], c[7], c[8]};
+  return (c0 + hit.u * (c1 - c0)) + hit.v * (c2 - c0);
+}
+
+// scene.slang:490-537 (TangentFrame): the frame about `normal` whose tangent
+// follows the texture coordinates' s direction, the normal map's coordinate
+// transform applied to their differentials; the normal-only frame without
+// usable coordinates.
+Frame TangentFrame(const Scene& s, const uint32_t* input, const Hit& hit, F3 edge1, F3 edge2, F3 normal) {
+  Frame basis = MakeShadingFrame(normal);
+  const uint32_t first = s.tex.instance_texcoords.empty() ? 0xFFFFFFFFu : s.tex.instance_texcoords[hit.instance];
+  if (first == 0xFFFFFFFFu) return basis;
+  const float* t = &s.tex.texcoords[2 * (size_t(first) + 3 * size_t(hit.primitive))];
+  float u1x = t[2] - t[0], u1y = t[3] - t[1], u2x = t[4] - t[0], u2y = t[5] - t[1];
+  if (input) {
+    const float a1 = WordFloat(input, 16) * u1x + WordFloat(input, 17) * u1y;
+    const float b1 = WordFloat(input, 20) * u1x + WordFloat(input, 21) * u1y;
+    const float a2 = WordFloat(input, 16) * u2x + WordFloat(input, 17) * u2y;
+    const float b2 = WordFloat(input, 20) * u2x + WordFloat(input, 21) * u2y;
+    u1x = a1;
+    u1y = b1;
+    u2x = a2;
+    u2y = b2;
+  }
+  const float uv_scale = std::max(std::max(std::fabs(u1x), std::fabs(u1y)), std::max(std::fabs(u2x), std::fabs(u2y)));
+  const float edge_scale =
+      std::max(std::max(std::max(std::fabs(edge1.x), std::fabs(edge1.y)), std::fabs(edge1.z)),
+               std::max(std::max(std::fabs(edge2.x), std::fabs(edge2.y)), std::fabs(edge2.z)));
+  if (!(uv_scale > 0.0f) || std::isinf(uv_scale) || !(edge_scale > 0.0f) || std::isinf(edge_scale)) return basis;
+  u1x = u1x / uv_scale;
+  u1y = u1y / uv_scale;
+  u2x = u2x / uv_scale;
+  u2y = u2y / uv_scale;
+  edge1 = edge1 / edge_scale;
+  edge2 = edge2 / edge_scale;
+  const float determinant = u1x * u2y - u1y * u2x;
+  if (!(std::fabs(determinant) > 1.0e-8f)) return basis;
+  const float orientation = determinant < 0.0f ? -1.0f : 1.0f;
+  F3 tangent = orientation * (edge1 * u2y - edge2 * u1y);
+  const F3 bitangent = orientation * (edge2 * u1x - edge1 * u2x);
+  tangent = tangent - normal * Dot(normal, tangent);
+  const float length2 = Dot(tangent, tangent);
+  if (!(length2 > 1.0e-16f) || std::isinf(length2)) return basis;
+  basis.t = tangent * Rsqrt(length2);
+  basis.b = Cross(normal, basis.t);
+  if (Dot(basis.b, bitangent) < 0.0f) basis.b = -basis.b;
+  return basis;
+}
+
+// scene.slang:1408-1449 (ReconstructHit) with EvaluatePreview
+// (:1184-1260): authored normals, vertex colours, texture lookups and
+// MappedNormal (:541-553) for the tangent-space normal.
 Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
   const uint32_t g = hit.global;
   const F3 p0 = s.v0[g], p1 = s.v1[g], p2 = s.v2[g];
@@ -1054,17 +1139,10 @@ Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
   sf.area = 0.5f * Length(e);
   sf.normal = Dot(normal, dir) > 0.0f ? -normal : normal;
   sf.shading_normal = sf.normal;
+  if (!s.tex.instance_normals.empty() && s.tex.instance_normals[hit.instance] != 0xFFFFFFFFu)
+    sf.shading_normal = AuthoredNormal(s, hit, sf.normal, dir);
   const Material& m = s.materials[s.instance_material[hit.instance]];
-  const F3 tn = m.tangent_normal;
-  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f)) {
-    const Frame basis = MakeShadingFrame(sf.shading_normal);
-    const float mag = std::max(std::max(std::fabs(tn.x), std::fabs(tn.y)), std::fabs(tn.z));
-    if (mag > 0.0f && IsFinite(tn)) {
-      F3 sh = Normalize(basis.ToWorld(tn / mag));
-      if (Dot(sh, sf.normal) < 0.0f) sh = -sh;
-      sf.shading_normal = Dot(sh, dir) < 0.0f ? sh : basis.n;
-    }
-  }
+  F3 tn = m.tangent_normal;
   bool entering = true;
   if (m.transmission > 0.0f) {
     const float* t = &s.instance_transform[16 * size_t(hit.instance)];
@@ -1074,11 +1152,16 @@ Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
     const F3 outward = Dot(c0, Cross(c1, c2)) < 0.0f ? -normal : normal;
     entering = Dot(outward, dir) < 0.0f;
   }
-  // EvaluatePreview (scene.slang:1197-1221): a lookup replaces its input's
-  // constant, clamped into its range
+  // EvaluatePreview (scene.slang:1187-1221): the instance's colour set, then
+  // a lookup replaces its input's constant, clamped into its range
   F3 base_color = m.base_color, specular = m.specular, emission = m.emission;
+  if (!s.tex.instance_colors.empty() && s.tex.instance_colors[hit.instance] != 0xFFFFFFFFu) {
+    const F3 c = HitColor(s, hit);
+    base_color = F3{Saturate(c.x), Saturate(c.y), Saturate(c.z)};
+  }
   float roughness = m.roughness, metallic = m.metallic;
-  if ((m.textured & 0x7Cu) != 0) {
+  const uint32_t* normal_input = nullptr;
+  if ((m.textured & 0x7Eu) != 0) {
     float st_s, st_t, v[4];
     HitTexcoords(s, m, hit.instance, hit.primitive, hit.u, hit.v, st_s, st_t);
     if (const uint32_t* b = MaterialInput(s.tex, m, 2)) {
@@ -1094,10 +1177,25 @@ Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
       metallic = Saturate(v[b[2]]);
     }
     emission = PreviewEmission(s, m, st_s, st_t);
+    if (const uint32_t* b = MaterialInput(s.tex, m, 1)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      tn = F3{std::min(std::max(v[0], -1.0f), 1.0f), std::min(std::max(v[1], -1.0f), 1.0f),
+              std::min(std::max(v[2], -1.0f), 1.0f)};
+      normal_input = b;
+    }
     if (const uint32_t* b = MaterialInput(s.tex, m, 6)) {
       Lookup(s.tex, b, st_s, st_t, v);
       const F3 value{v[0], v[1], v[2]};
       if (IsFinite(value)) specular = F3{Saturate(v[0]), Saturate(v[1]), Saturate(v[2])};
+    }
+  }
+  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f)) {
+    const Frame basis = TangentFrame(s, normal_input, hit, p1 - p0, p2 - p0, sf.shading_normal);
+    const float mag = std::max(std::max(std::fabs(tn.x), std::fabs(tn.y)), std::fabs(tn.z));
+    if (mag > 0.0f && IsFinite(tn)) {
+      F3 sh = Normalize(basis.ToWorld(tn / mag));
+      if (Dot(sh, sf.normal) < 0.0f) sh = -sh;
+      sf.shading_normal = Dot(sh, dir) < 0.0f ? sh : basis.n;
     }
   }
   sf.bsdf = MakePreviewClosure(base_color, roughness, metallic, m.ior,
@@ -2181,14 +2279,32 @@ oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
     }
   }
   if (d->binding_count) s.tex.bindings.assign(d->bindings, d->bindings + 24 * size_t(d->binding_count));
-  if (d->geometry_texcoord_offset) {
-    uint64_t corners = 0;
-    for (uint32_t g = 0; g < d->geometry_count; ++g)
-      if (d->geometry_texcoord_offset[g] != 0xFFFFFFFFu)
-        corners = std::max<uint64_t>(corners, uint64_t(d->geometry_texcoord_offset[g]) + 3ull * d->geometry_triangle_count[g]);
-    s.tex.texcoords.assign(d->texcoords, d->texcoords + 2 * corners);
-    for (uint32_t i = 0; i < d->instance_count; ++i)
-      s.tex.instance_texcoords.push_back(d->geometry_texcoord_offset[d->instance_geometry[i]]);
+  // Per instance its texture coordinates' (the set its material reads),
+  // normals' and colours' first corner (extraction.hpp:27-36).
+  {
+    uint64_t tcs = 0, nms = 0, cols = 0;
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+      const uint32_t g = d->instance_geometry[i];
+      const uint64_t corners = 3ull * d->geometry_triangle_count[g];
+      uint32_t tc = 0xFFFFFFFFu, nm = 0xFFFFFFFFu, col = 0xFFFFFFFFu, constant = 0;
+      if (d->instance_texcoord_offset) tc = d->instance_texcoord_offset[i];
+      else if (d->geometry_texcoord_offset) tc = d->geometry_texcoord_offset[g];
+      if (d->geometry_normal_offset) nm = d->geometry_normal_offset[g];
+      if (d->instance_color_offset) {
+        col = d->instance_color_offset[i];
+        constant = d->instance_color_constant && d->instance_color_constant[i] ? 1u : 0u;
+      }
+      if (tc != 0xFFFFFFFFu) tcs = std::max<uint64_t>(tcs, tc + corners);
+      if (nm != 0xFFFFFFFFu) nms = std::max<uint64_t>(nms, nm + corners);
+      if (col != 0xFFFFFFFFu) cols = std::max<uint64_t>(cols, col + (constant ? 1 : corners));
+      s.tex.instance_texcoords.push_back(tc);
+      s.tex.instance_normals.push_back(nm);
+      s.tex.instance_colors.push_back(col);
+      s.tex.instance_color_constant.push_back(constant);
+    }
+    if (tcs) s.tex.texcoords.assign(d->texcoords, d->texcoords + 2 * tcs);
+    if (nms) s.tex.normals.assign(d->normals, d->normals + 3 * nms);
+    if (cols) s.tex.colors.assign(d->colors, d->colors + 3 * cols);
   }
   // Light table (gpu_scene.cpp:512-609), emissive triangles only.
   s.light_offsets.assign(d->instance_count, kNoLights);

@@ -71,6 +71,12 @@ typedef struct oracle_scene_desc {
   const uint32_t* bindings;                 /* 24 words per GpuTextureInputRecord */
   const float* texcoords;                   /* 2 per triangle corner */
   const uint32_t* geometry_texcoord_offset; /* per geometry: first corner or 0xFFFFFFFF; NULL: none */
+  const uint32_t* instance_texcoord_offset; /* per instance (overrides the geometry's); NULL: none */
+  const float* normals;                     /* authored shading normals, 3 per triangle corner */
+  const uint32_t* geometry_normal_offset;   /* per geometry: first corner or 0xFFFFFFFF; NULL: none */
+  const float* colors;                      /* vertex colours, 3 per entry */
+  const uint32_t* instance_color_offset;    /* per instance: first entry or 0xFFFFFFFF; NULL: none */
+  const uint32_t* instance_color_constant;  /* per instance: 1 one colour for the mesh, 0 per corner */
 } oracle_scene_desc;
 
 typedef struct oracle_scene oracle_scene;

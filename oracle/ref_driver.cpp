@@ -6,12 +6,16 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <map>
 #include <string>
+#include <string_view>
 #include <vector>
 
+#include <lotus/extraction.hpp>
 #include <lotus/render_world.hpp>
 #include <lotus/temporal.hpp>
 
+#include "material_fixtures.hpp"
 #include "reference.hpp"
 #include "reservoir.hpp"
 
@@ -105,6 +109,152 @@ size_t ref_cornell_scene(float red_scale, uint32_t* out, size_t capacity,
   if (out && capacity >= words.size())
     std::memcpy(out, words.data(), words.size() * 4);
   return words.size();
+}
+
+// A material fixture (adapters/common/material_fixtures.cpp:2402-2435,
+// MaterialFixtures) as the reference's SceneExtraction turns it into a
+// SceneUpdate (core/render-extraction/extraction.cpp), the plan its
+// GpuScene::Apply receives, written as a word stream of this driver's own
+// layout (floats as their bits):
+//   size, camera view[16], projection[16], environment[3];
+//   lights: count, then per light shape, normalize, world_from_light[16],
+//     radiance[3], radius, width, height, angle;
+//   textures: count, then per texture width, height, format, byte count and
+//     the texel bytes padded to whole words;
+//   geometries: count, then per geometry vertex count, triangle count,
+//     positions, triangles, normal count and normals (3 per corner),
+//     texture-coordinate sets (count, then per set its length and 2 floats
+//     per entry), colour sets (count, then per set its length and 3 floats
+//     per entry);
+//   materials: count, then per material model, the PreviewSurface fields
+//     (base colour[3], roughness, metallic, ior, specular workflow,
+//     specular colour[3], emission[3], transmission, transmission
+//     colour[3], thin walled), the MaterialGeometry fields (texcoord
+//     fallback[2], opacity, opacity threshold, normal[3]) and 14 texture
+//     input slots (present, texture index, channel, wrap s, wrap t,
+//     scale[4], bias[4], fallback[4], st transform[6]);
+//   instances: count, then per instance geometry index,
+//     world_from_object[16], material slot, texture-coordinate set, colour
+//     set (0xFFFFFFFF for none).
+// `control` applies the fixture's negative control first. Returns the word
+// count (the needed size when `out` is null or too small), 0 for an unknown
+// fixture.
+size_t ref_material_fixture(const char* id, int control, uint32_t* out, size_t capacity) {
+  const LotusHeadless::MaterialFixture* fixture = nullptr;
+  for (const auto& f : LotusHeadless::MaterialFixtures())
+    if (f.id == std::string_view(id)) fixture = &f;
+  if (!fixture) return 0;
+  Lotus::RenderWorld world;
+  fixture->set(world);
+  if (control) fixture->apply_control(world);
+  const Lotus::FrameSnapshot snap = world.Commit();
+  Lotus::SceneExtraction extraction;
+  const Lotus::SceneUpdate update = extraction.Update(snap);
+  std::vector<uint32_t> w;
+  auto push_f = [&](float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    w.push_back(b);
+  };
+  auto push_fs = [&](const float* f, size_t n) {
+    for (size_t i = 0; i < n; ++i) push_f(f[i]);
+  };
+  w.push_back(fixture->size);
+  push_fs(snap.camera.view.data(), 16);
+  push_fs(snap.camera.projection.data(), 16);
+  push_fs(update.environment.data(), 3);
+  w.push_back(static_cast<uint32_t>(update.lights.size()));
+  for (const Lotus::Light& l : update.lights) {
+    w.push_back(static_cast<uint32_t>(l.shape));
+    w.push_back(l.normalize ? 1u : 0u);
+    push_fs(l.world_from_light.data(), 16);
+    push_fs(l.radiance.data(), 3);
+    push_f(l.radius);
+    push_f(l.width);
+    push_f(l.height);
+    push_f(l.angle);
+  }
+  std::map<const Lotus::Texture*, uint32_t> texture_index;
+  w.push_back(static_cast<uint32_t>(update.texture_uploads.size()));
+  for (const auto& t : update.texture_uploads) {
+    texture_index[t.get()] = static_cast<uint32_t>(texture_index.size());
+    w.push_back(t->width);
+    w.push_back(t->height);
+    w.push_back(static_cast<uint32_t>(t->format));
+    w.push_back(static_cast<uint32_t>(t->texels.size()));
+    const size_t words = (t->texels.size() + 3) / 4;
+    const size_t at = w.size();
+    w.resize(at + words, 0u);
+    std::memcpy(w.data() + at, t->texels.data(), t->texels.size());
+  }
+  std::map<const Lotus::MeshGeometry*, uint32_t> geometry_index;
+  w.push_back(static_cast<uint32_t>(update.geometry_uploads.size()));
+  for (const auto& g : update.geometry_uploads) {
+    geometry_index[g.get()] = static_cast<uint32_t>(geometry_index.size());
+    w.push_back(static_cast<uint32_t>(g->positions.size()));
+    w.push_back(static_cast<uint32_t>(g->triangles.size()));
+    for (const auto& p : g->positions) push_fs(p.data(), 3);
+    for (const auto& t : g->triangles)
+      for (uint32_t v : t) w.push_back(v);
+    w.push_back(static_cast<uint32_t>(g->normals.size()));
+    for (const auto& n : g->normals) push_fs(n.data(), 3);
+    w.push_back(static_cast<uint32_t>(g->texcoords.size()));
+    for (const auto& [name, set] : g->texcoords) {
+      w.push_back(static_cast<uint32_t>(set.size()));
+      for (const auto& c : set) push_fs(c.data(), 2);
+    }
+    w.push_back(static_cast<uint32_t>(g->colors.size()));
+    for (const auto& [name, set] : g->colors) {
+      w.push_back(static_cast<uint32_t>(set.size()));
+      for (const auto& c : set) push_fs(c.data(), 3);
+    }
+  }
+  w.push_back(static_cast<uint32_t>(update.materials.size()));
+  for (const Lotus::SceneMaterial& entry : update.materials) {
+    const Lotus::Material& m = entry.material;
+    w.push_back(static_cast<uint32_t>(m.Model()));
+    Lotus::PreviewSurface p;
+    if (const auto* preview = std::get_if<Lotus::PreviewSurface>(&m.surface)) p = *preview;
+    push_fs(p.base_color.data(), 3);
+    push_f(p.roughness);
+    push_f(p.metallic);
+    push_f(p.ior);
+    w.push_back(p.use_specular_workflow ? 1u : 0u);
+    push_fs(p.specular_color.data(), 3);
+    push_fs(p.emission.data(), 3);
+    push_f(p.transmission);
+    push_fs(p.transmission_color.data(), 3);
+    w.push_back(p.thin_walled ? 1u : 0u);
+    push_fs(m.geometry.texcoord_fallback.data(), 2);
+    push_f(m.geometry.opacity);
+    push_f(m.geometry.opacity_threshold);
+    push_fs(m.geometry.normal.data(), 3);
+    const auto inputs = Lotus::TextureInputs(m);
+    for (size_t slot = 0; slot < inputs.size(); ++slot) {
+      const auto& input = *inputs[slot];
+      w.push_back(input.has_value() ? 1u : 0u);
+      if (!input.has_value()) continue;
+      const auto found = texture_index.find(entry.textures[slot]);
+      w.push_back(found == texture_index.end() ? 0xFFFFFFFFu : found->second);
+      w.push_back(input->channel);
+      w.push_back(static_cast<uint32_t>(input->wrap_s));
+      w.push_back(static_cast<uint32_t>(input->wrap_t));
+      push_fs(input->scale.data(), 4);
+      push_fs(input->bias.data(), 4);
+      push_fs(input->fallback.data(), 4);
+      push_fs(input->st_transform.data(), 6);
+    }
+  }
+  w.push_back(static_cast<uint32_t>(update.instances.size()));
+  for (const Lotus::SceneInstance& i : update.instances) {
+    w.push_back(geometry_index.at(i.geometry));
+    push_fs(i.world_from_object.data(), 16);
+    w.push_back(i.material);
+    w.push_back(i.texcoords);
+    w.push_back(i.colors);
+  }
+  if (out && capacity >= w.size()) std::memcpy(out, w.data(), w.size() * 4);
+  return w.size();
 }
 
 // backend/vulkan/reservoir.cpp: StreamReservoirCandidate. Reservoir as 4
