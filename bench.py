@@ -33,6 +33,9 @@ import numpy as np  # noqa: E402
 from lotus_amd import hip, scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue capacity: 256 CUs x 2.4 GHz, one wave64 VALU instruction per CU
+# per clock (4 SIMDs, 4 clocks each; packed FP32 is compiled out)
+VALU_ISSUE_PER_S = 256 * 2.4e9
 NODE_BYTES = 128
 TRI_BYTES = 48
 # Queue bytes per ray of each traversal kernel: camera writes a hit record
@@ -469,7 +472,7 @@ def main() -> int:
         sh * QUEUE_BYTES["shadow"]
     ms = tot["trace_ms"]
     achieved = bytes_total / (ms / 1e3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    roofline = {"bound": None, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_trace",
                 "algorithmic_bytes_per_launch": int(bytes_total / max(trace_launches, 1)),
                 "avg_launch_ms": round(ms / max(trace_launches, 1), 4),
@@ -481,12 +484,28 @@ def main() -> int:
                                  "extension": round(tris[1] / max(ext, 1), 2),
                                  "shadow": round(tris[2] / max(sh, 1), 2),
                                  "tail": round(tris[3] / max(walked[3], 1), 2)}}
+    # What bounds it, from the committed counter summary of the same kernel
+    # on the same workload (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE
+    # bytes, SQ_INSTS_VALU and active lanes per launch): the measured HBM
+    # fraction against the VALU issue fraction over this run's launch time.
+    # Neither near its peak means the kernel is bound by memory latency (a
+    # walk's dependent node fetches) and lane divergence.
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             t = json.load(f)
         if t.get("kernel") == roofline["kernel"]:
+            launch_s = roofline["avg_launch_ms"] / 1e3
             roofline["traffic"] = t.get("hbm_bytes_per_launch")
+            hbm_frac = t["hbm_bytes_per_launch"] / launch_s / 1e9 / HBM_PEAK_GBS
+            valu_frac = t["valu_insts_per_launch"] / (VALU_ISSUE_PER_S * launch_s)
+            lanes = t["valu_active_lanes"]
+            bound = "latency" if max(hbm_frac, valu_frac) < 0.5 else ("hbm" if hbm_frac >= valu_frac else "valu")
+            roofline["bound"] = bound
+            roofline["measured"] = {"hbm_frac": round(hbm_frac, 4), "valu_issue_frac": round(valu_frac, 4),
+                                    "valu_active_lanes_of_64": round(lanes, 1), "l2_hit": t.get("l2_hit"),
+                                    "achieved_is": "demand bytes of node, triangle and queue touches, most "
+                                                   "served by L2 and the Infinity Cache"}
     line = {
         "metric": "Mrays/s by ray kind + GPU ms to reach equal relMSE (plain vs MLP cache), 1080p",
         "value": round(value, 2),

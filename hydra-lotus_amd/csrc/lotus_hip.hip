@@ -1797,6 +1797,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
     r->key_valid = false;  // every temporal call traces a fresh batch
   }
   uint64_t cache_queries = 0, cache_samples = 0, cache_used = 0;
+  uint32_t last_training = 0;  // the last sample's training records
   double cache_loss_sum = 0.0;
   const uint32_t npix = P->width * P->height;
   Launch L = launch_for(r, r->stream);
@@ -1983,6 +1984,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
       }
       if (cache_frame && add) {
         const uint32_t total = std::min(host[counter_word(1, 3)], r->sample_capacity);
+        last_training = total;
         const uint32_t steps = std::max(1u, r->cache_config.train_steps);
         for (uint32_t k = 0; k < steps; ++k)
           if (total > k) cache_used += std::min((total - k + steps - 1) / steps, r->cache_config.batch_capacity);
@@ -2082,6 +2084,17 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
     res->cache_infer_gpu_ms = t_ms[kTCacheInfer];
     res->cache_train_gpu_ms = t_ms[kTCacheTrain];
     res->restir_gpu_ms = t_ms[kTRestir];
+    res->cache_training_count = 0;
+    if (res->cache_training && cache_frame && last_training) {
+      const uint32_t n = std::min(last_training, res->cache_training_capacity);
+      const hipError_t ce = hipMemcpy(res->cache_training, r->q.samples, size_t(n) * kSampleFloats * 4,
+                                      hipMemcpyDeviceToHost);
+      if (ce != hipSuccess) {
+        set_detail(detail, detail_size, latch(r, "training record readback", ce));
+        return LOTUS_HIP_FAIL;
+      }
+      res->cache_training_count = n;
+    }
     if (tf) {
       hipEvent_t* ev = r->events.data() + r->events.size() - 4;
       float a = 0.0f, b = 0.0f;

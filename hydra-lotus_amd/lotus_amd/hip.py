@@ -252,6 +252,9 @@ class FrameResult(C.Structure):
         ("cache_train_gpu_ms", C.c_double),
         ("restir_records", _u32p),
         ("restir_gpu_ms", C.c_double),
+        ("cache_training", _f32p),
+        ("cache_training_capacity", C.c_uint32),
+        ("cache_training_count", C.c_uint32),
     ]
 
 
@@ -515,7 +518,7 @@ class Renderer:
                light_sampling=LIGHT_MIS, output=OUTPUT_RADIANCE, clear_color=(0.0, 0.0, 0.0, 0.0),
                data_window=(0, 0, 0, 0), display_window=(0.0, 0.0, 0.0, 0.0), max_samples=0,
                want_color=True, want_depth=False, count_capacity=None, split_kinds=False, cache=False,
-               restir_candidates=8, restir_proposal=0, want_restir=False, temporal=None):
+               restir_candidates=8, restir_proposal=0, want_restir=False, temporal=None, want_training=0):
         """One RenderScene call; with `temporal` (a TemporalSettings), one
         RenderTemporalScene frame."""
         fp = frame_params(world_to_clip, width, height, frame_count, sample_index, max_bounces, light_sampling,
@@ -535,6 +538,10 @@ class Renderer:
         if restir is not None:
             res.restir_records = _p(restir, C.c_uint32)
         res.path_counts_capacity = cap
+        training = np.zeros((want_training, 20), np.float32) if want_training else None
+        if training is not None:
+            res.cache_training = _p(training, C.c_float)
+            res.cache_training_capacity = want_training
         buf = C.create_string_buffer(512)
         if temporal is None:
             s = lib().lotus_hip_render(self.handle, C.byref(fp), C.byref(res), buf, 512)
@@ -553,7 +560,8 @@ class Renderer:
                     walked=list(res.traversal_rays), trace_launches=res.trace_launches,
                     cache_queries=res.cache_queries, cache_samples=res.cache_samples, cache_loss=res.cache_loss,
                     cache_infer_ms=res.cache_infer_gpu_ms, cache_train_ms=res.cache_train_gpu_ms,
-                    restir=restir, restir_ms=res.restir_gpu_ms)
+                    restir=restir, restir_ms=res.restir_gpu_ms,
+                    training=None if training is None else training[:res.cache_training_count].copy())
 
     def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

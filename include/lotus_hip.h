@@ -111,8 +111,9 @@ typedef struct lotus_hip_scene_plan {
    * material record's words 1-3 give its first binding and its textured
    * slots), and per-corner texture coordinates (2 floats per triangle
    * corner) with each geometry's first corner, 0xFFFFFFFF for a geometry
-   * without them. Base colour, roughness, metallic, emission and specular
-   * colour take lookups; normal and opacity lookups are rejected. */
+   * without them. Normal (tangent space), base colour, roughness, metallic,
+   * emission and specular colour take lookups; opacity lookups are
+   * rejected. */
   uint32_t texture_count;
   const lotus_hip_texture* textures;
   uint32_t binding_count;
@@ -272,6 +273,13 @@ typedef struct lotus_hip_frame_result {
    * optional, and the reservoir passes' GPU time. */
   uint32_t* restir_records;
   double restir_gpu_ms;
+  /* Radiance cache frames: the last sample's training records (20 floats
+   * each, the layout lotus_hip_cache_train takes), optional: at most
+   * cache_training_capacity of them are written, their count in
+   * cache_training_count. */
+  float* cache_training;
+  uint32_t cache_training_capacity;
+  uint32_t cache_training_count;
 } lotus_hip_frame_result;
 
 /* ProbeVulkanBackend (vulkan_backend.hpp:742): PASS when a gfx950 device is
