@@ -537,8 +537,17 @@ __device__ __forceinline__ bool spatial_compatible(const ReuseSurface& receiver,
          dot3(receiver.normal, source.normal) >= 0.9f && dot3(receiver.shading_normal, source.shading_normal) >= 0.9f;
 }
 
+// The disk mode's random stream: neighbour offsets are drawn apart from the
+// merges' stream (seeded with 0x369DEA0F), so both modes merge with the same
+// random numbers.
+constexpr uint32_t kSpatialDiskSeed = 0x5BD1E995u;
+constexpr int kMaxSpatialSources = 5;
+
 // restir.slang:402-491 (WriteSpatialReservoir): the four axis neighbours at
-// the radius.
+// the radius (restir.slang:408-410), or, in the disk mode, five neighbours
+// drawn uniformly within the radius (a draw that lands on the pixel itself
+// is no neighbour; draws may repeat a neighbour, whose candidates then count
+// twice in M and in the supporting candidates alike).
 template <bool kLights>  // the scene has analytic lights or an environment image
 __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, FrameConst F, RestirReuseArgs A,
                                                                 uint32_t* counters, uint2* spill, int spill_cap,
@@ -548,19 +557,33 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
   st.init(ring, spill, spill_cap);
   const uint32_t pixels = F.width * F.height;
   const int radius = static_cast<int>(A.radius);
+  const int sources = A.disk ? kMaxSpatialSources : 4;
   uint32_t visits = 0, tests = 0;
   for (uint32_t slot = gtid(); slot < pixels; slot += gstride()) {
     const ReuseRecord centre = load_reuse(A.temporal_out + 4ull * slot);
     ReuseRecord result = centre;
-    uint32_t src_index[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    uint32_t src_index[kMaxSpatialSources] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
     uint32_t rays = 0;
     if ((centre.flags & 3u) == 1u) {
       const ReuseSurface receiver = load_reuse_surface(A.surf + 8ull * slot);
       const int px = static_cast<int>(slot % F.width), py = static_cast<int>(slot / F.width);
-      const int ox[4] = {-radius, radius, 0, 0}, oy[4] = {0, 0, -radius, radius};
+      uint32_t disk_rng = seed_random(slot, F.sample_index ^ kSpatialDiskSeed);
       uint32_t accepted = 0;
-      for (int i = 0; i < 4; ++i) {
-        const int nx = px + ox[i], ny = py + oy[i];
+      for (int i = 0; i < sources; ++i) {
+        int ox, oy;
+        if (A.disk) {
+          const float u = next_random(disk_rng), v = next_random(disk_rng);
+          const float rr = static_cast<float>(radius) * sqrtf(u);
+          float sn, cs;
+          sincos_turns(v, &sn, &cs);
+          ox = static_cast<int>(floorf(rr * cs + 0.5f));
+          oy = static_cast<int>(floorf(rr * sn + 0.5f));
+          if (ox == 0 && oy == 0) continue;
+        } else {
+          ox = i == 0 ? -radius : i == 1 ? radius : 0;
+          oy = i == 2 ? -radius : i == 3 ? radius : 0;
+        }
+        const int nx = px + ox, ny = py + oy;
         if (nx < 0 || ny < 0 || nx >= static_cast<int>(F.width) || ny >= static_cast<int>(F.height)) continue;
         const uint32_t index = static_cast<uint32_t>(ny) * F.width + static_cast<uint32_t>(nx);
         const uint32_t nflags = __float_as_uint(A.temporal_out[4ull * index + 1].w);
@@ -576,7 +599,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
                                selected))
           result.flags |= 2u;
         uint32_t rng = seed_random(slot, F.sample_index ^ 0x369DEA0Fu);
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < sources; ++i) {
           if (src_index[i] == 0xFFFFFFFFu) continue;
           const ReuseRecord src = load_reuse(A.temporal_out + 4ull * src_index[i]);
           V3 received = V3{0.0f, 0.0f, 0.0f};
@@ -597,7 +620,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
         result.contribution = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if ((result.flags & 2u) == 0u && result.r.weight_sum > 0.0f) {
           result.supporting = centre.r.count;
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < sources; ++i) {
             if (src_index[i] == 0xFFFFFFFFu) continue;
             const ReuseRecord src = load_reuse(A.temporal_out + 4ull * src_index[i]);
             bool valid = true;
@@ -613,7 +636,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
     }
     wave_count(restir_ray_count(counters), rays != 0u);
     store_reuse(A.spatial_out + 4ull * slot, result);
-    A.spatial_sources[slot] = make_uint4(src_index[0], src_index[1], src_index[2], src_index[3]);
+    A.spatial_sources[2ull * slot] = make_uint4(src_index[0], src_index[1], src_index[2], src_index[3]);
+    A.spatial_sources[2ull * slot + 1] = make_uint4(src_index[4], 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
   }
   flush_stats(stats, 2, visits, tests);
 }

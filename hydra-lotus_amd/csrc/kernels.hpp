@@ -119,12 +119,13 @@ struct RestirReuseArgs {
   const float4* prev_hist;     // the previous frame's final records
   float4* temporal_out;
   float4* spatial_out;
-  uint4* spatial_sources;      // the spatial pass's accepted neighbours
+  uint4* spatial_sources;      // the spatial pass's accepted neighbours, 2 uint4 per pixel
   const TemporalPixel* tsurf;  // this frame's pixel-centre surfaces
   const TemporalValidityPixel* validity;
   uint32_t history;            // the previous frame's records are usable
   uint32_t max_history;        // restir_max_history_candidates
   uint32_t radius;             // spatial radius in pixels
+  uint32_t disk;               // 0: the four axis neighbours at the radius; 1: five drawn within it
 };
 void launch_restir_clear(const Launch& L, float4* restir, float4* surf, uint32_t pixels);
 void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,

@@ -491,7 +491,7 @@ hipError_t ensure_temporal(lotus_hip_renderer* r, uint32_t width, uint32_t heigh
   if (e == hipSuccess) e = dev_alloc(t.validity, P);
   if (e == hipSuccess) e = dev_alloc(t.filter, 3 * P);
   if (e == hipSuccess) e = dev_alloc(t.rspin, 4 * P);
-  if (e == hipSuccess) e = dev_alloc(t.rsources, P);
+  if (e == hipSuccess) e = dev_alloc(t.rsources, 2 * P);
   if (e != hipSuccess) {
     free_temporal(r);
     return e;
@@ -2483,7 +2483,8 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   }
   if (ts->restir_spatial && (!restir || P->frame_count != 1 || ts->restir_max_history_candidates == 0 ||
                              ts->restir_max_history_candidates > 65536 || ts->restir_spatial_radius == 0 ||
-                             ts->restir_spatial_radius > 32)) {
+                             ts->restir_spatial_radius > 32 ||
+                             ts->restir_spatial_neighbours > LOTUS_HIP_SPATIAL_DISK)) {
     set_detail(detail, detail_size,
                "spatial reservoirs require RenderTemporalScene, RestirInitial, one sample, history candidates in [1, 65536] and radius in [1, 32]");
     return LOTUS_HIP_FAIL;
@@ -2553,6 +2554,7 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   A.history = ts->restir_temporal && available && t.reservoirs ? 1u : 0u;
   A.max_history = ts->restir_max_history_candidates;
   A.radius = ts->restir_spatial_radius;
+  A.disk = ts->restir_spatial_neighbours == LOTUS_HIP_SPATIAL_DISK ? 1u : 0u;
   if (tf.accumulate && !(available && t.radiance))
     launch_fill4(reinterpret_cast<float4*>(t.rad[prev]), 2u * P->width * P->height,
                  make_float4(0.0f, 0.0f, 0.0f, 0.0f), r->stream);
@@ -2572,7 +2574,9 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   if (e == hipSuccess && ts->restir_spatial_records && tf.spatial)
     e = hipMemcpy(ts->restir_spatial_records, t.rhist[cur], npix * 4 * sizeof(float4), hipMemcpyDeviceToHost);
   if (e == hipSuccess && ts->restir_spatial_sources && tf.spatial)
-    e = hipMemcpy(ts->restir_spatial_sources, t.rsources, npix * sizeof(uint4), hipMemcpyDeviceToHost);
+    e = A.disk ? hipMemcpy(ts->restir_spatial_sources, t.rsources, npix * 2 * sizeof(uint4), hipMemcpyDeviceToHost)
+               : hipMemcpy2D(ts->restir_spatial_sources, sizeof(uint4), t.rsources, 2 * sizeof(uint4),
+                             sizeof(uint4), npix, hipMemcpyDeviceToHost);
   if (e == hipSuccess && ts->restir_surfaces && tf.reuse)
     e = hipMemcpy(ts->restir_surfaces, t.rsurf[cur], npix * 8 * sizeof(float4), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {

@@ -35,6 +35,8 @@ SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hi
            "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune", "lotus_hip_readback_scene",
            "lotus_hip_write_render_buffer", "lotus_hip_write_ids"]
 BATCH_GRAPH = 1
+# Spatial reuse neighbours (lotus_hip_temporal_settings.restir_spatial_neighbours).
+SPATIAL_AXES, SPATIAL_DISK = 0, 1
 # Render-buffer formats (HdFormat UNorm8Vec4, Float32Vec4, Float32, Int32).
 BUFFER_UNORM8X4, BUFFER_FLOAT32X4, BUFFER_FLOAT32, BUFFER_INT32 = range(4)
 # What a scene apply did (lotus_hip_scene_evidence.update).
@@ -285,6 +287,7 @@ class TemporalSettings(C.Structure):
         ("restir_spatial_records", C.c_void_p),
         ("restir_spatial_sources", C.c_void_p),
         ("restir_surfaces", C.c_void_p),
+        ("restir_spatial_neighbours", C.c_uint32),
     ]
 
 
@@ -716,7 +719,7 @@ class Renderer:
     def render_temporal(self, world_to_clip, view, width, height, frame_index, sample_index=None,
                         accumulate=False, max_history=32, filter_iterations=0, restir_temporal=False,
                         restir_spatial=False, radius=30, max_history_candidates=20, history_key=0,
-                        camera_cut=False, want_products=False, **kw):
+                        camera_cut=False, want_products=False, spatial_disk=False, **kw):
         """One temporal frame; returns render()'s dict plus the reset reason,
         history availability and, with want_products, the surface, validity
         and radiance records."""
@@ -729,6 +732,7 @@ class Renderer:
             t.view[i] = float(v[i])
         t.restir_temporal, t.restir_spatial = int(restir_temporal), int(restir_spatial)
         t.restir_spatial_radius, t.restir_max_history_candidates = radius, max_history_candidates
+        t.restir_spatial_neighbours = SPATIAL_DISK if spatial_disk else SPATIAL_AXES
         prod = {}
         if want_products:
             prod = dict(surface=np.zeros((height, width), TEMPORAL_PIXEL),
@@ -736,7 +740,7 @@ class Renderer:
                         radiance=np.zeros((height, width), TEMPORAL_RADIANCE),
                         restir_temporal=np.zeros((height, width, 16), np.uint32),
                         restir_spatial=np.zeros((height, width, 16), np.uint32),
-                        restir_sources=np.zeros((height, width, 4), np.uint32),
+                        restir_sources=np.zeros((height, width, 8 if spatial_disk else 4), np.uint32),
                         restir_surfaces=np.zeros((height, width, 32), np.float32))
             t.surface = prod["surface"].ctypes.data
             t.validity = prod["validity"].ctypes.data

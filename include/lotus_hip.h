@@ -481,7 +481,8 @@ typedef struct lotus_hip_temporal_settings {
    * host buffers of width*height records: the temporal pass's and the
    * spatial pass's RestirReusePixel (64 B: the RestirPixel, normalization,
    * supporting candidates, previous index, age) and the spatial pass's
-   * accepted neighbours (4 pixel indices, 0xFFFFFFFF for none). Written when
+   * accepted neighbours (4 pixel indices, 0xFFFFFFFF for none; see
+   * restir_spatial_neighbours for the disk mode). Written when
    * the frame ran that pass. */
   void* restir_temporal_records;
   void* restir_spatial_records;
@@ -495,7 +496,17 @@ typedef struct lotus_hip_temporal_settings {
    * low, high bits}; zero where the camera ray missed), width*height of
    * them. Written when the frame ran a reuse pass. */
   void* restir_surfaces;
+  /* The spatial pass's neighbours: LOTUS_HIP_SPATIAL_AXES, the reference's
+   * four at restir_spatial_radius along the axes (restir.slang:408-410), or
+   * LOTUS_HIP_SPATIAL_DISK, five drawn uniformly within the radius from a
+   * random stream of their own (no reference counterpart). In the disk mode
+   * restir_spatial_sources holds 8 words per pixel, the five draws in order
+   * and three 0xFFFFFFFF. */
+  uint32_t restir_spatial_neighbours;
 } lotus_hip_temporal_settings;
+
+#define LOTUS_HIP_SPATIAL_AXES 0u
+#define LOTUS_HIP_SPATIAL_DISK 1u
 
 int lotus_hip_render_temporal(lotus_hip_renderer* renderer, const lotus_hip_frame_params* params,
                               lotus_hip_temporal_settings* temporal, lotus_hip_frame_result* result, char* detail,

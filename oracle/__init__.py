@@ -142,7 +142,7 @@ def lib():
                                              C.c_uint64, _u32p, _f32p, C.c_uint32, _u32p, _f32p]
         L.oracle_restir_spatial.restype = None
         L.oracle_restir_spatial.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                            C.c_uint32, C.c_uint32, _u32p, _f32p, _u32p, _u32p]
+                                            C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _f32p, _u32p, _u32p]
         _lib = L
     return _lib
 
@@ -303,14 +303,15 @@ class OracleScene:
         return rec, surf, bool(hit)
 
     def restir_spatial(self, width: int, height: int, x: int, y: int, sample_index: int, radius: int,
-                       max_history: int, records: np.ndarray, surfaces: np.ndarray):
+                       max_history: int, records: np.ndarray, surfaces: np.ndarray, disk: bool = False):
         """WriteSpatialReservoir at pixel (x, y) over the temporal pass's
-        records [h*w, 16] and surfaces [h*w, 32]: (record[16], sources[4])."""
+        records [h*w, 16] and surfaces [h*w, 32]: (record[16], sources[5]),
+        the four axis neighbours or, with disk, five drawn within the radius."""
         rec = np.zeros(16, np.uint32)
-        src = np.zeros(4, np.uint32)
+        src = np.zeros(5, np.uint32)
         lib().oracle_restir_spatial(self.handle, width, height, x, y, sample_index, radius, max_history,
-                                    ptr(records, C.c_uint32), ptr(surfaces, C.c_float), ptr(rec, C.c_uint32),
-                                    ptr(src, C.c_uint32))
+                                    int(disk), ptr(records, C.c_uint32), ptr(surfaces, C.c_float),
+                                    ptr(rec, C.c_uint32), ptr(src, C.c_uint32))
         return rec, src
 
 

@@ -2844,22 +2844,39 @@ int oracle_restir_temporal(const oracle_scene* scene, const oracle_frame* frame,
 
 // restir.slang:402-491 (WriteSpatialReservoir) for pixel (x, y) over the
 // temporal pass's records and receiver surfaces of the whole target (16
-// words and 32 floats per pixel): the four axis neighbours at `radius`.
+// words and 32 floats per pixel): the four axis neighbours at `radius`, or
+// with `disk` five drawn uniformly within it from a stream of their own
+// (seeded with sample ^ 0x5BD1E995; radius r sqrt(u), angle 2 pi v, rounded
+// to the nearest pixel; the pixel itself is no neighbour).
 void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
-                           uint32_t sample_index, uint32_t radius, uint32_t max_history, const uint32_t* records,
-                           const float* surfaces, uint32_t* record, uint32_t* sources) {
+                           uint32_t sample_index, uint32_t radius, uint32_t max_history, uint32_t disk,
+                           const uint32_t* records, const float* surfaces, uint32_t* record, uint32_t* sources) {
   const Scene& s = scene->s;
   const uint32_t slot = y * width + x;
   const ReuseRecord centre = LoadReuse(records + 16ull * slot);
   const ReuseSurface receiver = LoadSurface(surfaces + 32ull * slot);
   ReuseRecord result = centre;
-  uint32_t src[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+  uint32_t src[5] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
   const int r = static_cast<int>(radius);
-  const int ox[4] = {-r, r, 0, 0}, oy[4] = {0, 0, -r, r};
+  const int count = disk ? 5 : 4;
   if ((centre.px.flags & 3u) == 1u) {
     uint32_t accepted = 0;
-    for (int i = 0; i < 4; ++i) {
-      const int nx = static_cast<int>(x) + ox[i], ny = static_cast<int>(y) + oy[i];
+    uint32_t disk_rng = SeedRandom(slot, sample_index ^ 0x5BD1E995u);
+    for (int i = 0; i < count; ++i) {
+      int ox, oy;
+      if (disk) {
+        const float u = NextRandom(disk_rng), v = NextRandom(disk_rng);
+        const float rr = static_cast<float>(r) * Sqrt(u);
+        float sn, cs;
+        SinCosTurns(v, &sn, &cs);
+        ox = static_cast<int>(std::floor(rr * cs + 0.5f));
+        oy = static_cast<int>(std::floor(rr * sn + 0.5f));
+        if (ox == 0 && oy == 0) continue;
+      } else {
+        ox = i == 0 ? -r : i == 1 ? r : 0;
+        oy = i == 2 ? -r : i == 3 ? r : 0;
+      }
+      const int nx = static_cast<int>(x) + ox, ny = static_cast<int>(y) + oy;
       if (nx < 0 || ny < 0 || nx >= static_cast<int>(width) || ny >= static_cast<int>(height)) continue;
       const uint32_t index = static_cast<uint32_t>(ny) * width + static_cast<uint32_t>(nx);
       if (SpatialCompatible(receiver, records[16ull * index + 7], LoadSurface(surfaces + 32ull * index))) {
@@ -2874,7 +2891,7 @@ void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t h
                             0xFFFFFFFFu, 0.0f, selected))
         result.px.flags |= 2u;
       uint32_t rng = SeedRandom(slot, sample_index ^ 0x369DEA0Fu);
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < count; ++i) {
         if (src[i] == 0xFFFFFFFFu) continue;
         const ReuseRecord source = LoadReuse(records + 16ull * src[i]);
         F3 received{0, 0, 0};
@@ -2898,7 +2915,7 @@ void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t h
       result.px.visibility = 0.0f;
       if ((result.px.flags & 2u) == 0 && result.px.r.weight_sum > 0.0f) {
         result.supporting = centre.px.r.count;
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < count; ++i) {
           if (src[i] == 0xFFFFFFFFu) continue;
           const ReuseRecord source = LoadReuse(records + 16ull * src[i]);
           bool valid = true;
@@ -2913,7 +2930,7 @@ void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t h
     }
   }
   StoreReuse(result, record);
-  for (int i = 0; i < 4; ++i) sources[i] = src[i];
+  for (int i = 0; i < 5; ++i) sources[i] = src[i];
 }
 
 void oracle_camera_ray(const float* clip_to_world, float nx, float ny, float* ray) {

@@ -166,11 +166,12 @@ int oracle_restir_temporal(const oracle_scene* scene, const oracle_frame* frame,
                            const uint32_t* previous_record, const float* previous_surface, uint32_t max_history,
                            uint32_t* record, float* surface);
 /* Spatial pass at pixel (x, y) over the temporal pass's records and surfaces
- * of the whole width x height target: writes the record and the four
- * accepted neighbours (0xFFFFFFFF: none). */
+ * of the whole width x height target: writes the record and the accepted
+ * neighbours (5 words, 0xFFFFFFFF: none): the four axis neighbours at the
+ * radius, or with `disk` five drawn within it. */
 void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
-                           uint32_t sample_index, uint32_t radius, uint32_t max_history, const uint32_t* records,
-                           const float* surfaces, uint32_t* record, uint32_t* sources);
+                           uint32_t sample_index, uint32_t radius, uint32_t max_history, uint32_t disk,
+                           const uint32_t* records, const float* surfaces, uint32_t* record, uint32_t* sources);
 
 /* The camera ray through a film position (reference CameraRay). */
 void oracle_camera_ray(const float* clip_to_world, float ndc_x, float ndc_y,

@@ -58,16 +58,17 @@ def oracle_temporal_frame(o, fr, sample, view, records_prev, surfaces_prev, vali
     return rec, surf, hits
 
 
-def oracle_spatial_frame(o, w, h, sample, radius, cap, records, surfaces):
+def oracle_spatial_frame(o, w, h, sample, radius, cap, records, surfaces, disk=False):
     """The oracle's spatial pass over every pixel of the temporal pass's
-    records and surfaces: records [h, w, 16] and sources [h, w, 4]."""
+    records and surfaces: records [h, w, 16] and sources [h, w, 5] (the four
+    axis neighbours and an unused fifth, or the disk mode's five)."""
     records = np.ascontiguousarray(records, np.uint32).reshape(-1, 16)
     surfaces = np.ascontiguousarray(surfaces, np.float32).reshape(-1, 32)
     rec = np.zeros((h, w, 16), np.uint32)
-    src = np.zeros((h, w, 4), np.uint32)
+    src = np.zeros((h, w, 5), np.uint32)
     for y in range(h):
         for x in range(w):
-            rec[y, x], src[y, x] = o.restir_spatial(w, h, x, y, sample, radius, cap, records, surfaces)
+            rec[y, x], src[y, x] = o.restir_spatial(w, h, x, y, sample, radius, cap, records, surfaces, disk)
     return rec, src
 
 

@@ -41,7 +41,8 @@ def _compare_surfaces(got, want, hits, what):
     return float(np.mean(np.all(gb == wb, axis=1)))
 
 
-def _run(renderer, sc, w, h, frames, views, *, radius, cap, candidates, max_bounces, spatial, first_sample):
+def _run(renderer, sc, w, h, frames, views, *, radius, cap, candidates, max_bounces, spatial, first_sample,
+         disk=False):
     renderer.apply(sc)
     o = oracle.OracleScene(sc)
     prev_final = prev_surf = None
@@ -56,7 +57,7 @@ def _run(renderer, sc, w, h, frames, views, *, radius, cap, candidates, max_boun
                                        restir_temporal=True, restir_spatial=spatial, radius=radius,
                                        max_history_candidates=cap, max_bounces=max_bounces,
                                        light_sampling=hip.LIGHT_RESTIR, restir_candidates=candidates,
-                                       want_products=True)
+                                       want_products=True, spatial_disk=disk)
         fr = oracle.make_frame(w2c, w, h, max_bounces=max_bounces, light_sampling=2, restir_candidates=candidates)
         vd = out["validity"]
         validity = np.stack([vd["valid"], vd["previous_index"]], -1).astype(np.uint32)
@@ -71,8 +72,11 @@ def _run(renderer, sc, w, h, frames, views, *, radius, cap, candidates, max_boun
             assert (gtemp[..., 14] != rr.NO).mean() > 0.5  # most pixels merged their history
         final = gtemp
         if spatial:
-            srec, src = rr.oracle_spatial_frame(o, w, h, sample, radius, cap, gtemp, gsurf)
-            assert np.array_equal(out["restir_sources"], src), f"spatial sources, frame {f}"
+            srec, src = rr.oracle_spatial_frame(o, w, h, sample, radius, cap, gtemp, gsurf, disk)
+            gsrc = out["restir_sources"]
+            assert np.array_equal(gsrc[..., :5 if disk else 4], src[..., :5 if disk else 4]), f"sources, frame {f}"
+            if disk:
+                assert np.all(gsrc[..., 5:] == rr.NO)
             bitwise.append(rr.compare_records(out["restir_spatial"], srec, f"spatial records, frame {f}"))
             assert (src != rr.NO).any()
             final = out["restir_spatial"]
@@ -103,3 +107,26 @@ def test_reuse_on_the_many_light_workload(renderer):
     sc = scenes.cornell_many_lights()
     _run(renderer, sc, 160, 90, 4, lambda f: scenes.camera_spline(f), radius=30, cap=20, candidates=32,
          max_bounces=8, spatial=True, first_sample=40)
+
+
+def test_reuse_five_neighbours_within_the_radius(renderer):
+    # The many-light workload's spatial mode: five neighbours drawn uniformly
+    # within r = 30 px (the reference's four axis neighbours at exactly r stay
+    # the default); records, sources and surfaces bitwise as above (the
+    # oracle's draws are pinned on the plane in test_oracle_restir_reuse.py),
+    # and every accepted neighbour lies within the radius.
+    sc = scenes.cornell_many_lights()
+    _run(renderer, sc, 160, 90, 3, lambda f: scenes.camera_spline(f), radius=30, cap=20, candidates=32,
+         max_bounces=8, spatial=True, first_sample=60, disk=True)
+    out = renderer.render_temporal(scenes.multiply(sc.projection, sc.view), sc.view, 160, 90,
+                                   frame_index=next(_FRAME), camera_cut=True, restir_spatial=True, radius=30,
+                                   max_history_candidates=20, max_bounces=8, light_sampling=hip.LIGHT_RESTIR,
+                                   restir_candidates=32, want_products=True, spatial_disk=True)
+    src = out["restir_sources"][..., :5].reshape(-1, 5)
+    ys, xs = np.divmod(np.arange(160 * 90), 160)
+    ok = src != rr.NO
+    dx = (src % 160).astype(np.int64) - xs[:, None]
+    dy = (src // 160).astype(np.int64) - ys[:, None]
+    d2 = (dx * dx + dy * dy)[ok]
+    print(f"accepted neighbours {ok.sum()}, mean squared distance {d2.mean():.1f}")
+    assert ok.sum() > 2000 and d2.max() <= 31 * 31 and d2.min() >= 1

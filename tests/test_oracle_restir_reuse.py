@@ -113,7 +113,7 @@ def test_spatial_reuse_rederived_by_the_reference(temporal):
             rng = rr.Rng(slot, sample ^ rr.SPATIAL_SEED)
             neighbours = [slot - 1 if x else rr.NO, slot + 1 if x != SIZE - 1 else rr.NO,
                           slot - SIZE if y else rr.NO, slot + SIZE if y != SIZE - 1 else rr.NO]
-            assert list(src[y, x]) == neighbours, (frame, slot)
+            assert list(src[y, x]) == neighbours + [rr.NO], (frame, slot)
             supporting = int(centre_rec[3])
             for n in neighbours:
                 if n == rr.NO:
@@ -174,3 +174,38 @@ def test_temporal_support_counts_only_sources_that_can_hold_the_selection():
         kept += supporting > int(initial[3])
         dropped += 0 < supporting == int(initial[3])
     assert kept > 100 and dropped > 100
+
+
+def test_disk_neighbours_cover_the_disk_uniformly():
+    # The disk mode (no reference counterpart) on the plane, where every
+    # neighbour is compatible (one material, one view depth): the accepted
+    # neighbours of each pixel are its five draws of radius r sqrt(u) and
+    # angle 2 pi v rounded to pixels, so their mean squared distance is that
+    # of the disk clipped by the image (a simulation of the same rule in
+    # numpy), their angles are uniform and the stream is the pixel's own.
+    colors = ((2, 0.5, 1), (0.2, 2, 1.5), (1, 1, 1))
+    sc, o, fr = _plane(colors)
+    radius = 12
+    validity, centre = _still(SIZE, SIZE, 0)
+    trec, tsurf, _ = rr.oracle_temporal_frame(o, fr, 17, sc.view, None, None, validity, centre, False, CAP)
+    _, src = rr.oracle_spatial_frame(o, SIZE, SIZE, 17, radius, CAP, trec, tsurf, disk=True)
+    src = src.reshape(-1, 5)
+    ys, xs = np.divmod(np.arange(SIZE * SIZE), SIZE)
+    ok = src != rr.NO
+    dx = (src % SIZE).astype(np.int64) - xs[:, None]
+    dy = (src // SIZE).astype(np.int64) - ys[:, None]
+    d2 = (dx * dx + dy * dy)[ok]
+    rng = np.random.default_rng(3)
+    n = 400000
+    px, py = rng.integers(0, SIZE, n), rng.integers(0, SIZE, n)
+    rad = radius * np.sqrt(rng.random(n))
+    ang = 2 * np.pi * rng.random(n)
+    sx, sy = np.floor(rad * np.cos(ang) + 0.5), np.floor(rad * np.sin(ang) + 0.5)
+    inside = (px + sx >= 0) & (px + sx < SIZE) & (py + sy >= 0) & (py + sy < SIZE) & ((sx != 0) | (sy != 0))
+    expect = (sx * sx + sy * sy)[inside].mean()
+    print(f"accepted {ok.sum()} of {ok.size}, mean d^2 {d2.mean():.2f}, simulated {expect:.2f}")
+    assert abs(ok.mean() - inside.mean()) < 0.02 and d2.min() >= 1 and d2.max() <= (radius + 1) ** 2
+    assert abs(d2.mean() - expect) < 0.02 * expect
+    angles = np.arctan2(dy[ok], dx[ok])
+    hist, _ = np.histogram(angles, bins=8, range=(-np.pi, np.pi))
+    assert hist.min() > 0.8 * hist.mean()
