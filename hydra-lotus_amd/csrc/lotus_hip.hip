@@ -267,6 +267,7 @@ struct lotus_hip_renderer {
     bool previous = false;      // the last successful temporal frame's state below
     uint64_t frame_index = 0, history_key = 0, scene_generation = 0, scene_epoch = 0;
     float view[16] = {}, world_to_clip[16] = {}, display[4] = {};
+    float eye[3] = {};  // the camera's world-space position
     uint32_t data[4] = {};
     uint32_t light_sampling = 0, max_bounces = 0, candidates = 0, proposal = 0;
     bool reservoirs = false, radiance = false;
@@ -2443,6 +2444,36 @@ void resolved_windows(const lotus_hip_frame_params* P, float* display, uint32_t*
   }
 }
 
+// The camera's world-space position: the view matrix's inverse applied to
+// the origin, in double (Gauss-Jordan with partial pivoting).
+void camera_eye(const float* view, float* eye) {
+  double a[4][8];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      a[r][c] = view[c * 4 + r];
+      a[r][4 + c] = r == c ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 4; ++c) {
+    int pivot = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (std::fabs(a[r][c]) > std::fabs(a[pivot][c])) pivot = r;
+    if (a[pivot][c] == 0.0) {
+      eye[0] = eye[1] = eye[2] = 0.0f;
+      return;
+    }
+    for (int k = 0; k < 8; ++k) std::swap(a[c][k], a[pivot][k]);
+    const double d = a[c][c];
+    for (int k = 0; k < 8; ++k) a[c][k] /= d;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const double f = a[r][c];
+      for (int k = 0; k < 8; ++k) a[r][k] -= f * a[c][k];
+    }
+  }
+  // column 3 of the inverse, divided by its w
+  for (int r = 0; r < 3; ++r) eye[r] = static_cast<float>(a[r][4 + 3] / a[3][4 + 3]);
+}
+
 void default_validation(const float* in, float* out) {
   const bool given = in[0] != 0.0f || in[1] != 0.0f || in[2] != 0.0f || in[3] != 0.0f;
   const float defaults[4] = {0.01f, 0.01f, 0.9f, 32.0f};  // temporal.hpp:28-33
@@ -2472,6 +2503,12 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   }
   if (ts->filter_iterations > 8 || (ts->filter_iterations != 0 && !ts->accumulate_radiance)) {
     set_detail(detail, detail_size, "filter_iterations needs accumulate_radiance and is at most 8");
+    return LOTUS_HIP_FAIL;
+  }
+  // vulkan_backend.cpp:1752-1755
+  if (ts->history_rectification != 0 &&
+      (!ts->accumulate_radiance || ts->history_rectification > LOTUS_HIP_RECTIFY_REFLECTION)) {
+    set_detail(detail, detail_size, "history rectification requires radiance accumulation and a known mode");
     return LOTUS_HIP_FAIL;
   }
   const bool restir = P->light_sampling == LOTUS_HIP_LIGHT_RESTIR;
@@ -2532,6 +2569,9 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   T.fresh_samples = std::max(P->frame_count, 1u);
   T.cap = ts->accumulate_radiance ? ts->max_history_samples : 1u;
   T.iterations = ts->accumulate_radiance ? ts->filter_iterations : 0u;
+  T.rectification = ts->history_rectification;
+  camera_eye(ts->view, T.current_eye);
+  std::memcpy(T.previous_eye, t.eye, sizeof t.eye);
   tf.I = TemporalInstances{ts->previous_from_current, ts->instance_ids, ts->material_ids, ts->instance_flags};
   tf.surface = t.surf[cur];
   tf.previous_surface = t.surf[prev];
@@ -2589,6 +2629,7 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   t.scene_generation = r->scene_generation;
   t.scene_epoch = r->scene_epoch;
   std::memcpy(t.view, ts->view, 64);
+  std::memcpy(t.eye, T.current_eye, sizeof t.eye);
   vulkan_world_to_clip(P->world_to_clip, t.world_to_clip);
   resolved_windows(P, t.display, t.data);
   t.light_sampling = P->light_sampling;
@@ -2673,6 +2714,43 @@ int lotus_hip_svgf(lotus_hip_renderer* r, const void* surface, void* radiance, u
   }
   return svgf_run(r, surface, radiance, width, height, batch_samples, max_history_samples, iterations, color,
                   r->svgf_filter, detail, detail_size);
+}
+
+int lotus_hip_accumulate(lotus_hip_renderer* r, const void* batch, const void* surface, const void* validity,
+                         const void* old_radiance, uint32_t width, uint32_t height, uint32_t fresh_samples,
+                         uint32_t max_history_samples, uint32_t rectification, void* radiance, char* detail,
+                         size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (!batch || !surface || !validity || !old_radiance || !radiance || width == 0 || height == 0 ||
+      fresh_samples == 0 || max_history_samples == 0 || max_history_samples > 65536 ||
+      rectification > LOTUS_HIP_RECTIFY_REFLECTION || uint64_t(width) * height > 0xFFFFFFFFull) {
+    set_detail(detail, detail_size, "invalid accumulation input");
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  TemporalConst T{};
+  T.width = width;
+  T.height = height;
+  T.fresh_samples = fresh_samples;
+  T.cap = max_history_samples;
+  T.rectification = rectification;
+  const Launch L = launch_for(r, r->stream);
+  launch_temporal_accumulate(L, T, static_cast<const float4*>(batch), static_cast<const TemporalPixel*>(surface),
+                             static_cast<const TemporalValidityPixel*>(validity),
+                             static_cast<const TemporalRadiancePixel*>(old_radiance),
+                             static_cast<TemporalRadiancePixel*>(radiance));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, "accumulation", e));
+    return LOTUS_HIP_FAIL;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
 }
 
 int lotus_hip_svgf_static(lotus_hip_renderer* r, const void* surface, void* radiance, uint32_t width,

@@ -140,8 +140,21 @@ __global__ void __launch_bounds__(kTraceBlock) k_temporal_surface(SceneView S, F
                           isfinite(clip[0]) && isfinite(clip[1]) && isfinite(clip[2]) && isfinite(clip[3]) &&
                           isfinite(qx) && isfinite(qy) && isfinite(depth);
       if (finite && pv[3] > 0.0f && clip[3] > 0.0f && depth > 0.0f && pz >= 0.0f && pz <= 1.0f && px >= -1.0f &&
-          py >= -1.0f && px < 1.0f && py < 1.0f && qx >= bx && qy >= by && qx < ex && qy < ey)
+          py >= -1.0f && px < 1.0f && py < 1.0f && qx >= bx && qy >= by && qx < ex && qy < ey) {
         o.motion_depth = make_float4(qx - fx, qy - fy, depth, 1.0f);
+        // temporal.slang:611-622 (ReflectionWeight): the share of history to
+        // keep as float bits in the reserved word
+        if (T.rectification == kRectifyReflection) {
+          const V3 to_current = V3{T.current_eye[0], T.current_eye[1], T.current_eye[2]} - world;
+          const V3 to_previous = V3{T.previous_eye[0], T.previous_eye[1], T.previous_eye[2]} - prev;
+          const float cosine = dot3(to_current, to_previous) / maxf(length3(to_current) * length3(to_previous), 1.0e-20f);
+          const float theta = acos_pi(minf(maxf(cosine, -1.0f), 1.0f)) * 3.14159265358979323846f;
+          const float lobe = maxf(sf.roughness * sf.roughness, 0.001f);
+          const float share = sf.metallic + 0.04f * (1.0f - sf.metallic);
+          const float ratio = theta / lobe;
+          o.reserved = __float_as_uint((1.0f - share) + share / (1.0f + ratio * ratio));
+        }
+      }
     }
     out[slot] = o;
   }
@@ -197,7 +210,83 @@ __global__ void __launch_bounds__(256) k_temporal_reproject(TemporalConst T, con
   }
 }
 
-// temporal.slang:229-267 (AccumulateTemporalRadiance) without rectification.
+// temporal.slang:143-150 (RectifyNeighbour): a fresh-batch neighbour inside
+// the image, a primary hit on the same placement.
+__device__ __forceinline__ bool rectify_neighbour(const TemporalConst& T, const TemporalPixel* surface, int x, int y,
+                                                  uint64_t instance, uint32_t& q) {
+  if (x < 0 || y < 0 || x >= static_cast<int>(T.width) || y >= static_cast<int>(T.height)) return false;
+  q = static_cast<uint32_t>(y) * T.width + static_cast<uint32_t>(x);
+  return (surface[q].flags & 1u) != 0u && surface[q].instance_id == instance;
+}
+
+// temporal.slang:152-172 (ClampHistory): the reprojected illumination
+// clamped per channel to the 3x3 fresh neighbourhood's mean plus or minus one
+// standard deviation.
+__device__ float4 clamp_history(const TemporalConst& T, const float4* batch, const TemporalPixel* surface, int x,
+                                int y, float4 old_color) {
+  const TemporalPixel s = surface[static_cast<uint32_t>(y) * T.width + static_cast<uint32_t>(x)];
+  V3 sum = V3{0.0f, 0.0f, 0.0f}, squares = V3{0.0f, 0.0f, 0.0f};
+  float count = 0.0f;
+  for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+      uint32_t q;
+      if (!rectify_neighbour(T, surface, x + dx, y + dy, s.instance_id, q)) continue;
+      const V3 v = demodulate(xyz(batch[q]), surface[q].albedo);
+      sum = sum + v;
+      squares = squares + v * v;
+      count += 1.0f;
+    }
+  const V3 mean = sum / count;
+  const V3 dev = V3{sqrtf(maxf(squares.x / count - mean.x * mean.x, 0.0f)),
+                    sqrtf(maxf(squares.y / count - mean.y * mean.y, 0.0f)),
+                    sqrtf(maxf(squares.z / count - mean.z * mean.z, 0.0f))};
+  const V3 o = demodulate(xyz(old_color), s.albedo);
+  const V3 kept = V3{minf(maxf(o.x, mean.x - dev.x), mean.x + dev.x), minf(maxf(o.y, mean.y - dev.y), mean.y + dev.y),
+                     minf(maxf(o.z, mean.z - dev.z), mean.z + dev.z)};
+  const V3 c = remodulate(kept, s.albedo);
+  return make_float4(c.x, c.y, c.z, old_color.w);
+}
+
+// temporal.slang:177-212 (AntiLagDrop): the share of history to drop from
+// the 5x5 neighbourhood's fresh and reprojected mean luminance against the
+// fresh mean's standard error.
+__device__ float anti_lag_drop(const TemporalConst& T, const float4* batch, const TemporalPixel* surface,
+                               const TemporalValidityPixel* validity, const TemporalRadiancePixel* old_radiance, int x,
+                               int y) {
+  const uint64_t instance = surface[static_cast<uint32_t>(y) * T.width + static_cast<uint32_t>(x)].instance_id;
+  float fresh = 0.0f, squares = 0.0f, count = 0.0f, history = 0.0f, histories = 0.0f;
+  for (int dy = -2; dy <= 2; ++dy)
+    for (int dx = -2; dx <= 2; ++dx) {
+      uint32_t q;
+      if (!rectify_neighbour(T, surface, x + dx, y + dy, instance, q)) continue;
+      const uint32_t albedo = surface[q].albedo;
+      const V3 d = demodulate(xyz(batch[q]), albedo);
+      const float l = luminance(d.x, d.y, d.z);
+      fresh += l;
+      squares += l * l;
+      count += 1.0f;
+      const TemporalValidityPixel v = validity[q];
+      if (v.valid == 0u) continue;
+      const TemporalRadiancePixel old = old_radiance[v.previous_index];
+      if (old.sample_count == 0u || !finite4(old.color)) continue;
+      const V3 h = demodulate(xyz(old.color), albedo);
+      history += luminance(h.x, h.y, h.z);
+      histories += 1.0f;
+    }
+  if (histories == 0.0f || count < 2.0f) return 0.0f;
+  const float mean = fresh / count;
+  const float variance = maxf(squares - count * mean * mean, 0.0f) / (count - 1.0f);
+  const float z = fabsf(history / histories - mean) / sqrtf(variance / count + 1.0e-12f);
+  return minf(maxf((z - 2.0f) * 0.5f, 0.0f), 1.0f);
+}
+
+// temporal.slang:214-217 (ScaleKept): a kept count scaled, to the nearest sample.
+__device__ __forceinline__ uint32_t scale_kept(uint32_t kept, float scale) {
+  return static_cast<uint32_t>(rintf(static_cast<float>(kept) * minf(maxf(scale, 0.0f), 1.0f)));
+}
+
+// temporal.slang:219-267 (AccumulateTemporalRadiance) with the history
+// rectification modes.
 __global__ void __launch_bounds__(256) k_temporal_accumulate(TemporalConst T, const float4* batch,
                                                              const TemporalPixel* surface,
                                                              const TemporalValidityPixel* validity,
@@ -216,14 +305,25 @@ __global__ void __launch_bounds__(256) k_temporal_accumulate(TemporalConst T, co
       if (v.valid != 0u) {
         const TemporalRadiancePixel old = old_radiance[v.previous_index];
         const uint32_t room = T.cap - o.sample_count;
-        const uint32_t kept = old.sample_count < room ? old.sample_count : room;
-        if (kept != 0u && finite4(old.color) && isfinite(old.first_moment) && isfinite(old.second_moment)) {
+        uint32_t kept = old.sample_count < room ? old.sample_count : room;
+        float4 old_color = old.color;
+        const bool finite_old = finite4(old.color) && isfinite(old.first_moment) && isfinite(old.second_moment);
+        if (kept != 0u && finite_old && T.rectification != 0u) {
+          const int x = static_cast<int>(i % T.width), y = static_cast<int>(i / T.width);
+          if (T.rectification == kRectifyClamp)
+            old_color = clamp_history(T, batch, surface, x, y, old_color);
+          else if (T.rectification == kRectifyAntiLag)
+            kept = scale_kept(kept, 1.0f - anti_lag_drop(T, batch, surface, validity, old_radiance, x, y));
+          else if (T.rectification == kRectifyReflection)
+            kept = scale_kept(kept, __uint_as_float(surface[i].reserved));
+        }
+        if (kept != 0u && finite_old) {
           const uint32_t count = kept + o.sample_count;
           const float blend = static_cast<float>(kept) / static_cast<float>(count);
-          o.color.x = o.color.x + (old.color.x - o.color.x) * blend;
-          o.color.y = o.color.y + (old.color.y - o.color.y) * blend;
-          o.color.z = o.color.z + (old.color.z - o.color.z) * blend;
-          o.color.w = o.color.w + (old.color.w - o.color.w) * blend;
+          o.color.x = o.color.x + (old_color.x - o.color.x) * blend;
+          o.color.y = o.color.y + (old_color.y - o.color.y) * blend;
+          o.color.z = o.color.z + (old_color.z - o.color.z) * blend;
+          o.color.w = o.color.w + (old_color.w - o.color.w) * blend;
           m1 = m1 + (old.first_moment - m1) * blend;
           m2 = m2 + (old.second_moment - m2) * blend;
           o.sample_count = count;

@@ -42,7 +42,13 @@ struct TemporalConst {
   uint32_t available;               // a compatible previous frame exists
   uint32_t width, height;
   uint32_t fresh_samples, cap, iterations;
+  // TemporalHistoryRectification (include/lotus/temporal.hpp:35-57): 0 none,
+  // 1 neighbourhood clamp, 2 anti-lag, 3 reflection weight; the world-space
+  // camera positions the reflection weight compares.
+  uint32_t rectification;
+  float current_eye[3], previous_eye[3];
 };
+constexpr uint32_t kRectifyClamp = 1, kRectifyAntiLag = 2, kRectifyReflection = 3;
 
 // Per-instance temporal identity (temporal.slang:32-40): previous placement
 // relative to the current one, stable ids, flags (bit 1 rigid

@@ -31,12 +31,14 @@ SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hi
            "lotus_hip_read_accumulation", "lotus_hip_set_tuning", "lotus_hip_get_tuning",
            "lotus_hip_cache_configure", "lotus_hip_cache_infer", "lotus_hip_cache_gradients",
            "lotus_hip_cache_train", "lotus_hip_cache_parameters", "lotus_hip_render_temporal", "lotus_hip_svgf",
-           "lotus_hip_svgf_static",
+           "lotus_hip_svgf_static", "lotus_hip_accumulate",
            "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune", "lotus_hip_readback_scene",
            "lotus_hip_write_render_buffer", "lotus_hip_write_ids"]
 BATCH_GRAPH = 1
 # Spatial reuse neighbours (lotus_hip_temporal_settings.restir_spatial_neighbours).
 SPATIAL_AXES, SPATIAL_DISK = 0, 1
+# History rectification (lotus_hip_temporal_settings.history_rectification).
+RECTIFY_NONE, RECTIFY_CLAMP, RECTIFY_ANTI_LAG, RECTIFY_REFLECTION = range(4)
 # Render-buffer formats (HdFormat UNorm8Vec4, Float32Vec4, Float32, Int32).
 BUFFER_UNORM8X4, BUFFER_FLOAT32X4, BUFFER_FLOAT32, BUFFER_INT32 = range(4)
 # What a scene apply did (lotus_hip_scene_evidence.update).
@@ -288,6 +290,7 @@ class TemporalSettings(C.Structure):
         ("restir_spatial_sources", C.c_void_p),
         ("restir_surfaces", C.c_void_p),
         ("restir_spatial_neighbours", C.c_uint32),
+        ("history_rectification", C.c_uint32),
     ]
 
 
@@ -365,6 +368,10 @@ def lib():
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_uint32, C.c_uint32, C.c_void_p, C.c_char_p, C.c_size_t]
+        L.lotus_hip_accumulate.restype = C.c_int
+        L.lotus_hip_accumulate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                           C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_char_p,
+                                           C.c_size_t]
         L.lotus_hip_reproject.restype = C.c_int
         L.lotus_hip_reproject.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, _f32p,
                                           C.c_void_p, C.c_char_p, C.c_size_t]
@@ -719,7 +726,7 @@ class Renderer:
     def render_temporal(self, world_to_clip, view, width, height, frame_index, sample_index=None,
                         accumulate=False, max_history=32, filter_iterations=0, restir_temporal=False,
                         restir_spatial=False, radius=30, max_history_candidates=20, history_key=0,
-                        camera_cut=False, want_products=False, spatial_disk=False, **kw):
+                        camera_cut=False, want_products=False, spatial_disk=False, rectification=0, **kw):
         """One temporal frame; returns render()'s dict plus the reset reason,
         history availability and, with want_products, the surface, validity
         and radiance records."""
@@ -733,6 +740,7 @@ class Renderer:
         t.restir_temporal, t.restir_spatial = int(restir_temporal), int(restir_spatial)
         t.restir_spatial_radius, t.restir_max_history_candidates = radius, max_history_candidates
         t.restir_spatial_neighbours = SPATIAL_DISK if spatial_disk else SPATIAL_AXES
+        t.history_rectification = rectification
         prod = {}
         if want_products:
             prod = dict(surface=np.zeros((height, width), TEMPORAL_PIXEL),
@@ -764,6 +772,16 @@ class Renderer:
         fn = lib().lotus_hip_svgf_static if static else lib().lotus_hip_svgf
         s = fn(self.handle, C.c_void_p(surface_ptr), C.c_void_p(radiance_ptr), width, height,
                batch_samples, max_history, iterations, C.c_void_p(color_ptr), buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    def accumulate(self, batch_ptr, surface_ptr, validity_ptr, old_ptr, width, height, fresh_samples, max_history,
+                   rectification, radiance_ptr):
+        """AccumulateTemporalRadiance over device buffers (RECTIFY_* modes)."""
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_accumulate(self.handle, C.c_void_p(batch_ptr), C.c_void_p(surface_ptr),
+                                       C.c_void_p(validity_ptr), C.c_void_p(old_ptr), width, height, fresh_samples,
+                                       max_history, rectification, C.c_void_p(radiance_ptr), buf, 256)
         if s != PASS:
             raise LotusError(s, buf.value.decode())
 

@@ -503,7 +503,21 @@ typedef struct lotus_hip_temporal_settings {
    * restir_spatial_sources holds 8 words per pixel, the five draws in order
    * and three 0xFFFFFFFF. */
   uint32_t restir_spatial_neighbours;
+  /* TemporalHistoryRectification (include/lotus/temporal.hpp:35-57), which
+   * needs accumulate_radiance: LOTUS_HIP_RECTIFY_NONE, _CLAMP (the
+   * reprojected illumination clamped to the 3x3 fresh neighbourhood's mean
+   * plus or minus one standard deviation), _ANTI_LAG (the kept history
+   * shrunk by saturate((z - 2) / 2) of the 5x5 neighbourhood's luminance
+   * difference in standard errors) or _REFLECTION (the kept history scaled
+   * by 1 - s + s / (1 + (theta / a)^2) of the point's view-direction
+   * change). Changing it resets no history. */
+  uint32_t history_rectification;
 } lotus_hip_temporal_settings;
+
+#define LOTUS_HIP_RECTIFY_NONE 0u
+#define LOTUS_HIP_RECTIFY_CLAMP 1u
+#define LOTUS_HIP_RECTIFY_ANTI_LAG 2u
+#define LOTUS_HIP_RECTIFY_REFLECTION 3u
 
 #define LOTUS_HIP_SPATIAL_AXES 0u
 #define LOTUS_HIP_SPATIAL_DISK 1u
@@ -520,6 +534,18 @@ int lotus_hip_render_temporal(lotus_hip_renderer* renderer, const lotus_hip_fram
 int lotus_hip_svgf(lotus_hip_renderer* renderer, const void* surface_device, void* radiance_device, uint32_t width,
                    uint32_t height, uint32_t batch_samples, uint32_t max_history_samples, uint32_t iterations,
                    void* color_device, char* detail, size_t detail_size);
+
+/* AccumulateTemporalRadiance (temporal.slang:219-267) over caller device
+ * buffers of width*height records: each pixel's fresh batch colour (RGBA32F),
+ * TemporalPixel, TemporalValidityPixel and the previous frame's
+ * TemporalRadiancePixel records into this frame's TemporalRadiancePixel,
+ * with fresh_samples samples in the batch, at most max_history_samples kept,
+ * under a LOTUS_HIP_RECTIFY_* mode (the reflection weight reads each
+ * surface's reserved word). */
+int lotus_hip_accumulate(lotus_hip_renderer* renderer, const void* batch_device, const void* surface_device,
+                         const void* validity_device, const void* old_radiance_device, uint32_t width,
+                         uint32_t height, uint32_t fresh_samples, uint32_t max_history_samples,
+                         uint32_t rectification, void* radiance_device, char* detail, size_t detail_size);
 
 /* lotus_hip_svgf without allocating: no counterpart in the reference. For
  * callers that cannot allocate device memory (code under HIP graph capture,

@@ -276,3 +276,63 @@ def test_temporal_settings_errors(renderer):
                                  light_sampling=hip.LIGHT_RESTIR, radius=0)
     with pytest.raises(hip.LotusError, match="filter_iterations"):
         renderer.render_temporal(sc.world_to_clip, sc.view, 16, 16, frame_index=902, filter_iterations=3)
+
+
+@pytest.mark.parametrize("mode", [hip.RECTIFY_NONE, hip.RECTIFY_CLAMP, hip.RECTIFY_ANTI_LAG, hip.RECTIFY_REFLECTION])
+def test_accumulation_with_rectification_matches_oracle(renderer, mode):
+    # AccumulateTemporalRadiance (temporal.slang:219-267) under each history
+    # rectification mode against the oracle's float32 restatement (pinned by
+    # closed forms in test_oracle_temporal.py): sample counts exact, colours
+    # and moments to 1e-6.
+    from oracle import temporal as ot
+    w, h = 57, 43
+    surf, old = _synthetic(w, h, 21 + mode)
+    rng = np.random.default_rng(mode)
+    surf["reserved"] = rng.uniform(0, 1, (h, w)).astype(np.float32).view(np.uint32)
+    surf["flags"][rng.random((h, w)) < 0.05] = 0  # some misses
+    batch = np.ones((h, w, 4), np.float32)
+    batch[..., :3] = rng.lognormal(-1, 0.8, (h, w, 3))
+    valid = np.zeros((h, w), hip.TEMPORAL_VALIDITY)
+    valid["valid"] = rng.random((h, w)) < 0.85
+    valid["previous_index"] = rng.integers(0, w * h, (h, w))
+    old["color"][rng.random((h, w)) < 0.2, :3] *= 20.0  # some far histories
+    out = torch.zeros(w * h * 32, dtype=torch.uint8, device="cuda")
+    db = torch.from_numpy(batch.reshape(-1).view(np.uint8).copy()).cuda()
+    ds, dv, do = _dev(surf), _dev(valid), _dev(old)
+    renderer.accumulate(db.data_ptr(), ds.data_ptr(), dv.data_ptr(), do.data_ptr(), w, h, 4, 24, mode,
+                        out.data_ptr())
+    got = _host(out, hip.TEMPORAL_RADIANCE, (h, w))
+    ref = ot.accumulate(batch, surf, valid, old, 4, 24, mode)
+    assert np.array_equal(got["sample_count"], ref["sample_count"])
+    for field in ("color", "first_moment", "second_moment"):
+        a, b = got[field].astype(np.float64), ref[field].astype(np.float64)
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-12)
+        assert rel.max() < 1e-6, (field, rel.max())
+    print(f"mode {mode}: kept histories {np.mean(got['sample_count'] > 4):.3f}, "
+          f"bit-exact colours {np.mean(np.all(got['color'] == ref['color'], -1)):.4f}")
+
+
+def test_reflection_weight_of_a_still_camera_keeps_everything(renderer):
+    # ReflectionWeight (temporal.slang:611-622): theta = 0 under a still
+    # camera, so 1 - s + s / (1 + 0) = 1 for every surface with a valid
+    # projection (up to the cosine's rounding, dot(v, v) against |v| |v|);
+    # a moved camera keeps less of the glossy metal's history.
+    sc = scenes.cornell_v1()
+    renderer.apply(sc)
+    kw = dict(accumulate=True, max_history=16, want_products=True, rectification=hip.RECTIFY_REFLECTION,
+              max_bounces=2)
+    renderer.render_temporal(sc.world_to_clip, sc.view, 64, 64, frame_index=9000, camera_cut=True, **kw)
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, 64, 64, frame_index=9001, **kw)
+    s = out["surface"]
+    moving = s["motion_depth"][..., 3] == 1.0
+    assert moving.mean() > 0.9
+    kept = s["reserved"][moving].view(np.float32)
+    print(f"still camera: kept share {kept.min():.7f}..{kept.max():.7f}")
+    assert np.all((kept > 1.0 - 1e-4) & (kept <= 1.0))
+    view = scenes.look_at((0.4, 0.2, 3.0), (0.0, 0.0, 0.0))
+    w2c = scenes.multiply(sc.projection, view)
+    out = renderer.render_temporal(w2c, view, 64, 64, frame_index=9002, **kw)
+    s = out["surface"]
+    keep = s["reserved"].view(np.float32)[s["motion_depth"][..., 3] == 1.0]
+    print(f"moved camera: kept share {keep.min():.3f}..{keep.max():.3f}")
+    assert keep.min() < 0.9 and keep.max() <= 1.0 and keep.min() > 0.0
