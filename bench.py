@@ -148,7 +148,7 @@ def restir_svgf_run(r, width, height, frames, depth=8, candidates=32, history=20
     for f in range(frames):
         view = scenes.camera_spline(f, frames)
         w2c = scenes.multiply(sc.projection, view)
-        out = r.render_temporal(w2c, view, width, height, frame_index=f + 1, sample_index=f, accumulate=True,
+        out = r.render_temporal(w2c, view, width, height, frame_index=f + 1, accumulate=True,
                                 max_history=32, filter_iterations=5, restir_temporal=True, restir_spatial=True,
                                 radius=radius, max_history_candidates=history, max_bounces=depth,
                                 light_sampling=hip.LIGHT_RESTIR, restir_candidates=candidates)

@@ -141,6 +141,11 @@ struct TemporalFrameState {
   TemporalInstances I{};
   float view[16];
   bool reuse = false, spatial = false, accumulate = false;
+  bool fresh = false;              // a fresh estimator batch (accumulation or reservoir reuse)
+  bool settling = false;           // SettleTemporalRadiance instead of the accumulation and filter
+  const float4* prior = nullptr;   // the last changing frame's colour
+  uint32_t max_samples = 0;
+  float* batch_host = nullptr;     // the batch product (resolved colour before the SVGF passes)
   RestirReuseArgs A{};
   TemporalPixel* surface = nullptr;
   const TemporalPixel* previous_surface = nullptr;
@@ -263,7 +268,11 @@ struct lotus_hip_renderer {
     float4* rhist[2] = {nullptr, nullptr};
     float4* rspin = nullptr;
     uint4* rsources = nullptr;
+    float4* prior = nullptr;    // the last changing frame's colour, for settling
+    uint32_t sample_offset = 0; // the fresh batches' samples since history was last dropped
     uint32_t cur = 0;
+    uint32_t rcur = 0;          // the radiance history's write index (settling frames keep it)
+    bool prior_valid = false;   // the last frame rendered with settle
     bool previous = false;      // the last successful temporal frame's state below
     uint64_t frame_index = 0, history_key = 0, scene_generation = 0, scene_epoch = 0;
     float view[16] = {}, world_to_clip[16] = {}, display[4] = {};
@@ -473,8 +482,10 @@ void free_temporal(lotus_hip_renderer* r) {
   dev_free(t.filter);
   dev_free(t.rspin);
   dev_free(t.rsources);
+  dev_free(t.prior);
   t.width = t.height = 0;
   t.previous = false;
+  t.prior_valid = false;
 }
 
 hipError_t ensure_temporal(lotus_hip_renderer* r, uint32_t width, uint32_t height) {
@@ -493,6 +504,7 @@ hipError_t ensure_temporal(lotus_hip_renderer* r, uint32_t width, uint32_t heigh
   if (e == hipSuccess) e = dev_alloc(t.filter, 3 * P);
   if (e == hipSuccess) e = dev_alloc(t.rspin, 4 * P);
   if (e == hipSuccess) e = dev_alloc(t.rsources, 2 * P);
+  if (e == hipSuccess) e = dev_alloc(t.prior, P);
   if (e != hipSuccess) {
     free_temporal(r);
     return e;
@@ -1795,7 +1807,10 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
     F.restir_surf = tf->reuse ? const_cast<float4*>(tf->A.surf) : nullptr;
     F.instance_ids = tf->I.instance_id;
     F.material_ids = tf->I.material_id;
-    r->key_valid = false;  // every temporal call traces a fresh batch
+    // a fresh batch never continues a progressive mean; a settling frame
+    // continues the still frames' progressive accumulation, and a frame
+    // without accumulation or reuse the ordinary one
+    if (tf->fresh && !tf->settling) r->key_valid = false;
   }
   uint64_t cache_queries = 0, cache_samples = 0, cache_used = 0;
   uint32_t last_training = 0;  // the last sample's training records
@@ -2019,14 +2034,21 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
       return LOTUS_HIP_FAIL;
     }
   }
+  if (tf && tf->batch_host && e == hipSuccess)
+    e = hipMemcpyAsync(tf->batch_host, r->q.color, npix * 16ull, hipMemcpyDeviceToHost, r->stream);
   if (tf && tf->accumulate) {
     // SVGF over this call's batch (the resolved colour): accumulation,
-    // variance, a-trous iterations and the composite back into the colour
+    // variance, a-trous iterations and the composite back into the colour;
+    // a settling frame crossfades its progressive mean from the prior instead
     hipEvent_t* ev = r->events.data() + r->events.size() - 4;
     (void)hipEventRecord(ev[2], r->stream);
-    launch_temporal_accumulate(L, tf->T, r->q.color, tf->surface, tf->validity, tf->old_radiance, tf->radiance);
-    launch_temporal_variance(L, tf->T, tf->surface, tf->radiance, tf->filter);
-    launch_temporal_filter(L, tf->T, tf->surface, tf->radiance, tf->filter, r->q.color);
+    if (tf->settling) {
+      launch_temporal_settle(L, tf->T, tf->prior, tf->old_radiance, r->q.color, r->accumulated, tf->max_samples);
+    } else {
+      launch_temporal_accumulate(L, tf->T, r->q.color, tf->surface, tf->validity, tf->old_radiance, tf->radiance);
+      launch_temporal_variance(L, tf->T, tf->surface, tf->radiance, tf->filter);
+      launch_temporal_filter(L, tf->T, tf->surface, tf->radiance, tf->filter, r->q.color);
+    }
     (void)hipEventRecord(ev[3], r->stream);
   }
   // The pixel-centre camera pass: depth, and the colour of the other outputs.
@@ -2505,6 +2527,10 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
     set_detail(detail, detail_size, "filter_iterations needs accumulate_radiance and is at most 8");
     return LOTUS_HIP_FAIL;
   }
+  if (ts->settle && !ts->accumulate_radiance) {  // vulkan_backend.cpp:1750-1751
+    set_detail(detail, detail_size, "settling requires radiance accumulation");
+    return LOTUS_HIP_FAIL;
+  }
   // vulkan_backend.cpp:1752-1755
   if (ts->history_rectification != 0 &&
       (!ts->accumulate_radiance || ts->history_rectification > LOTUS_HIP_RECTIFY_REFLECTION)) {
@@ -2554,6 +2580,23 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   }
   const bool available = reason == 0 && t.previous;
   const uint32_t cur = t.cur, prev = 1u - t.cur;
+  const uint32_t rcur = t.rcur, rprev = 1u - t.rcur;
+  const bool fresh = ts->accumulate_radiance || (restir && (ts->restir_temporal || ts->restir_spatial));
+  // vulkan_backend.cpp:1790-1795: settling needs nothing this frame depends
+  // on to differ from the preceding frame, which left its colour as the
+  // prior; reservoir reuse traces a fresh batch every frame and never settles
+  float clip[16];
+  vulkan_world_to_clip(P->world_to_clip, clip);
+  const bool settling = ts->settle && !(restir && (ts->restir_temporal || ts->restir_spatial)) && t.prior_valid &&
+                        available && r->scene_generation == t.scene_generation &&
+                        std::memcmp(ts->view, t.view, 64) == 0 && std::memcmp(clip, t.world_to_clip, 64) == 0;
+  // vulkan_backend.cpp:1805-1813: a fresh batch continues the stream's
+  // samples; a settling frame continues the still accumulation from the
+  // host's sample index
+  const uint32_t sample_offset = available ? t.sample_offset : 0u;
+  const uint32_t settled_before = settling ? r->accumulated : 0u;
+  lotus_hip_frame_params B = *P;
+  if (fresh && !settling) B.sample_index += sample_offset;
   TemporalFrameState tf;
   std::memcpy(tf.view, ts->view, 64);
   TemporalConst& T = tf.T;
@@ -2576,10 +2619,15 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   tf.surface = t.surf[cur];
   tf.previous_surface = t.surf[prev];
   tf.validity = t.validity;
-  tf.radiance = t.rad[cur];
-  tf.old_radiance = t.rad[prev];
+  tf.radiance = t.rad[rcur];
+  tf.old_radiance = t.rad[rprev];
   tf.filter = t.filter;
   tf.accumulate = ts->accumulate_radiance != 0;
+  tf.fresh = fresh;
+  tf.settling = settling;
+  tf.prior = t.prior;
+  tf.max_samples = P->max_samples;
+  tf.batch_host = ts->batch;
   tf.reuse = restir && (ts->restir_temporal || ts->restir_spatial);
   tf.spatial = restir && ts->restir_spatial;
   RestirReuseArgs& A = tf.A;
@@ -2596,16 +2644,23 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   A.radius = ts->restir_spatial_radius;
   A.disk = ts->restir_spatial_neighbours == LOTUS_HIP_SPATIAL_DISK ? 1u : 0u;
   if (tf.accumulate && !(available && t.radiance))
-    launch_fill4(reinterpret_cast<float4*>(t.rad[prev]), 2u * P->width * P->height,
+    launch_fill4(reinterpret_cast<float4*>(t.rad[rprev]), 2u * P->width * P->height,
                  make_float4(0.0f, 0.0f, 0.0f, 0.0f), r->stream);
-  const int status = render_impl(r, P, res, detail, detail_size, &tf);
+  const int status = render_impl(r, &B, res, detail, detail_size, &tf);
   if (status != LOTUS_HIP_PASS) return status;
   const size_t npix = size_t(P->width) * P->height;
-  if (ts->surface) e = hipMemcpy(ts->surface, t.surf[cur], npix * sizeof(TemporalPixel), hipMemcpyDeviceToHost);
+  const uint32_t samples = r->accumulated;  // this frame's samples per pixel
+  // the changing frame's colour becomes the next settling frames' prior
+  if (ts->settle && !settling)
+    e = hipMemcpy(t.prior, r->q.color, npix * sizeof(float4), hipMemcpyDeviceToDevice);
+  if (e == hipSuccess && ts->surface)
+    e = hipMemcpy(ts->surface, t.surf[cur], npix * sizeof(TemporalPixel), hipMemcpyDeviceToHost);
   if (e == hipSuccess && ts->validity)
     e = hipMemcpy(ts->validity, t.validity, npix * sizeof(TemporalValidityPixel), hipMemcpyDeviceToHost);
+  // a settling frame writes no history: its product is the retained one
   if (e == hipSuccess && ts->radiance && tf.accumulate)
-    e = hipMemcpy(ts->radiance, t.rad[cur], npix * sizeof(TemporalRadiancePixel), hipMemcpyDeviceToHost);
+    e = hipMemcpy(ts->radiance, t.rad[settling ? rprev : rcur], npix * sizeof(TemporalRadiancePixel),
+                  hipMemcpyDeviceToHost);
   // the reuse passes' records: the temporal pass writes the final records
   // alone, or the spatial pass's input when that pass follows
   if (e == hipSuccess && ts->restir_temporal_records && restir && ts->restir_temporal)
@@ -2639,8 +2694,22 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   t.reservoirs = tf.reuse;
   t.radiance = tf.accumulate;
   t.cur = prev;
+  // vulkan_backend.cpp:2085-2097
+  if (!settling) {
+    t.sample_offset = fresh ? sample_offset + samples : 0u;
+    t.rcur = rprev;
+  }
+  if (fresh && !settling) {  // the next progressive mean starts over
+    r->key_valid = false;
+    r->accumulated = 0;
+  }
+  t.prior_valid = ts->settle != 0;
   ts->reset_reason = reason;
   ts->history_available = available ? 1u : 0u;
+  ts->settling = settling ? 1u : 0u;
+  ts->settled_samples = settling ? samples : 0u;
+  ts->batch_samples = samples - std::min(samples, settled_before);
+  ts->batch_sample_index = B.sample_index + settled_before;
   return LOTUS_HIP_PASS;
 }
 

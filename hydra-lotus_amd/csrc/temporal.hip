@@ -336,6 +336,27 @@ __global__ void __launch_bounds__(256) k_temporal_accumulate(TemporalConst T, co
   }
 }
 
+// temporal.slang:480-493 (SettleTemporalRadiance; CPU temporal.cpp:514-528):
+// a settling frame's colour, the still frames' progressive mean `color`
+// crossfaded from the last changing frame's colour `prior` over the history
+// cap (at most max_samples) where the retained history, read in place since
+// nothing moved, has samples.
+__global__ void __launch_bounds__(256) k_temporal_settle(TemporalConst T, const float4* prior,
+                                                         const TemporalRadiancePixel* old_radiance, float4* color,
+                                                         uint32_t samples, uint32_t max_samples) {
+  const uint32_t total = T.width * T.height;
+  for (uint32_t i = tid(); i < total; i += tstride()) {
+    const float4 still = color[i];
+    const float4 p = prior[i];
+    uint32_t span = old_radiance[i].sample_count != 0u ? T.cap : 0u;
+    if (max_samples != 0u && max_samples < span) span = max_samples;
+    if (samples >= span || !finite4(p)) continue;
+    const float t = static_cast<float>(samples) / static_cast<float>(span);
+    color[i] = make_float4(p.x + (still.x - p.x) * t, p.y + (still.y - p.y) * t, p.z + (still.z - p.z) * t,
+                           p.w + (still.w - p.w) * t);
+  }
+}
+
 // A usable neighbour: inside, with a history, on the same placement.
 __device__ __forceinline__ bool usable(const TemporalConst& T, const TemporalPixel* surface,
                                        const TemporalRadiancePixel* radiance, int x, int y, uint64_t instance) {
@@ -587,6 +608,13 @@ void launch_temporal_accumulate(const Launch& L, const TemporalConst& T, const f
                                 const TemporalRadiancePixel* old_radiance, TemporalRadiancePixel* radiance) {
   k_temporal_accumulate<<<pixel_blocks(T.width * T.height), 256, 0, L.stream>>>(T, batch, surface, validity,
                                                                                old_radiance, radiance);
+}
+
+void launch_temporal_settle(const Launch& L, const TemporalConst& T, const float4* prior,
+                            const TemporalRadiancePixel* old_radiance, float4* color, uint32_t samples,
+                            uint32_t max_samples) {
+  k_temporal_settle<<<pixel_blocks(T.width * T.height), 256, 0, L.stream>>>(T, prior, old_radiance, color, samples,
+                                                                           max_samples);
 }
 
 void launch_temporal_variance(const Launch& L, const TemporalConst& T, const TemporalPixel* surface,

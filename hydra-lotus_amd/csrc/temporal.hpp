@@ -65,6 +65,9 @@ void launch_temporal_surface(const Launch& L, const SceneView& S, const FrameCon
                              const TemporalInstances& I, TemporalPixel* out);
 void launch_temporal_reproject(const Launch& L, const TemporalConst& T, const TemporalPixel* current,
                                const TemporalPixel* previous, TemporalValidityPixel* validity);
+void launch_temporal_settle(const Launch& L, const TemporalConst& T, const float4* prior,
+                            const TemporalRadiancePixel* old_radiance, float4* color, uint32_t samples,
+                            uint32_t max_samples);
 void launch_temporal_accumulate(const Launch& L, const TemporalConst& T, const float4* batch,
                                 const TemporalPixel* surface, const TemporalValidityPixel* validity,
                                 const TemporalRadiancePixel* old_radiance, TemporalRadiancePixel* radiance);

@@ -291,6 +291,12 @@ class TemporalSettings(C.Structure):
         ("restir_surfaces", C.c_void_p),
         ("restir_spatial_neighbours", C.c_uint32),
         ("history_rectification", C.c_uint32),
+        ("settle", C.c_uint32),
+        ("settling", C.c_uint32),
+        ("settled_samples", C.c_uint32),
+        ("batch_sample_index", C.c_uint32),
+        ("batch_samples", C.c_uint32),
+        ("batch", _f32p),
     ]
 
 
@@ -723,14 +729,19 @@ class Renderer:
             raise LotusError(s, buf.value.decode())
 
     # temporal frames ------------------------------------------------------
-    def render_temporal(self, world_to_clip, view, width, height, frame_index, sample_index=None,
+    def render_temporal(self, world_to_clip, view, width, height, frame_index, sample_index=0,
                         accumulate=False, max_history=32, filter_iterations=0, restir_temporal=False,
                         restir_spatial=False, radius=30, max_history_candidates=20, history_key=0,
-                        camera_cut=False, want_products=False, spatial_disk=False, rectification=0, **kw):
+                        camera_cut=False, want_products=False, spatial_disk=False, rectification=0, settle=False,
+                        **kw):
         """One temporal frame; returns render()'s dict plus the reset reason,
-        history availability and, with want_products, the surface, validity
-        and radiance records."""
+        history availability, whether it settled and its still samples, the
+        first index and count of the samples it added and, with
+        want_products, the surface, validity and radiance records and the
+        batch (the resolved colour before accumulation). `sample_index` is
+        the stream's base: fresh batches continue from it by themselves."""
         t = TemporalSettings()
+        t.settle = int(settle)
         t.frame_index, t.history_key, t.camera_cut = frame_index, history_key, int(camera_cut)
         t.accumulate_radiance, t.max_history_samples, t.filter_iterations = int(accumulate), max_history, \
             filter_iterations
@@ -757,11 +768,16 @@ class Renderer:
             t.restir_spatial_records = prod["restir_spatial"].ctypes.data
             t.restir_spatial_sources = prod["restir_sources"].ctypes.data
             t.restir_surfaces = prod["restir_surfaces"].ctypes.data
-        out = self.render(world_to_clip, width, height,
-                          sample_index=frame_index if sample_index is None else sample_index, temporal=t, **kw)
+            prod["batch"] = np.zeros((height, width, 4), np.float32)
+            t.batch = _p(prod["batch"], C.c_float)
+        out = self.render(world_to_clip, width, height, sample_index=sample_index, temporal=t, **kw)
         out.update(prod)
         out["reset_reason"] = t.reset_reason
         out["history_available"] = t.history_available
+        out["settling"] = bool(t.settling)
+        out["settled_samples"] = t.settled_samples
+        out["batch_sample_index"] = t.batch_sample_index
+        out["batch_samples"] = t.batch_samples
         return out
 
     def svgf(self, surface_ptr, radiance_ptr, width, height, batch_samples, max_history, iterations, color_ptr=0,

@@ -512,6 +512,30 @@ typedef struct lotus_hip_temporal_settings {
    * by 1 - s + s / (1 + (theta / a)^2) of the point's view-direction
    * change). Changing it resets no history. */
   uint32_t history_rectification;
+  /* TemporalFrameSettings::settle (temporal.hpp:77-88), which needs
+   * accumulate_radiance: once nothing changes (no reset, the same scene
+   * generation, view and world_to_clip, no reservoir reuse, and the
+   * preceding frame also asked to settle), frames stop accumulating history
+   * and filtering and continue a progressive accumulation of still samples
+   * from the host's sample_index (at most max_samples of them), whose mean
+   * the output crossfades into from the last changing frame's colour over
+   * max_history_samples (capped by max_samples) samples where that frame's
+   * history had samples (SettleTemporalRadiance, temporal.slang:477-493). */
+  uint32_t settle;
+  /* Written (TemporalProducts, vulkan_backend.hpp:120-146): whether the
+   * frame settled and its still samples, the first sample index of the
+   * samples it added and their count. Frames with a fresh batch (radiance
+   * accumulation or reservoir reuse) that do not settle draw their batch
+   * from sample_index plus the samples of the preceding frames since the
+   * history was last dropped (vulkan_backend.cpp:1805-1813). */
+  uint32_t settling;
+  uint32_t settled_samples;
+  uint32_t batch_sample_index;
+  uint32_t batch_samples;
+  /* Optional host buffer of width*height float4: the frame's resolved
+   * colour before accumulation (the fresh batch), or on a settling frame
+   * the still samples' mean. */
+  float* batch;
 } lotus_hip_temporal_settings;
 
 #define LOTUS_HIP_RECTIFY_NONE 0u

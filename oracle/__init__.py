@@ -178,6 +178,8 @@ def ref():
                                     _u32p]
         R.ref_material_fixture.restype = C.c_size_t
         R.ref_material_fixture.argtypes = [C.c_char_p, C.c_int, _u32p, C.c_size_t]
+        R.ref_settle.restype = None
+        R.ref_settle.argtypes = [_f32p, _f32p, _u32p, C.c_size_t, C.c_uint32, C.c_uint32, _f32p]
         R.ref_reuse_normalization.restype = C.c_double
         R.ref_reuse_normalization.argtypes = [_u32p, C.c_uint32]
         _ref = R

@@ -353,4 +353,17 @@ void ref_reproject(const void* current, uint32_t x, uint32_t y, uint32_t width, 
   out[3] = r.reserved;
 }
 
+// temporal.cpp:514-528 (SettleTemporalRadiance) over `count` pixels: prior
+// and still colours (4 floats each), each pixel's prior sample count.
+void ref_settle(const float* prior, const float* still, const uint32_t* prior_samples, size_t count,
+                uint32_t still_samples, uint32_t max_samples, float* out) {
+  for (size_t i = 0; i < count; ++i) {
+    std::array<float, 4> p, s;
+    std::memcpy(p.data(), prior + 4 * i, 16);
+    std::memcpy(s.data(), still + 4 * i, 16);
+    const auto v = Lotus::SettleTemporalRadiance(p, s, prior_samples[i], still_samples, max_samples);
+    std::memcpy(out + 4 * i, v.data(), 16);
+  }
+}
+
 }  // extern "C"

@@ -9,6 +9,10 @@ history to its value, a history equal to the fresh mean keeps its anti-lag
 count, a far one drops it, a reflection weight of one half halves the kept
 count, and no rectification is the plain blend.
 
+SettleTemporalRadiance (temporal.slang:477-493) is restated as `settle`
+and checked against the reference's CPU form (backend/vulkan/temporal.cpp:
+514-528, built into oracle/_ref) and the closed forms of its ends.
+
 Records are numpy structured arrays in the reference's layouts (the
 TEMPORAL_PIXEL, TEMPORAL_VALIDITY and TEMPORAL_RADIANCE dtypes of
 lotus_amd.hip), rows from the top.
@@ -153,4 +157,35 @@ def accumulate(batch, surface, validity, old, fresh_samples, cap, rectification=
             o["color"] = color
             o["sample_count"] = count
             out[y, x] = o
+    return out
+
+
+def settle(prior, still, prior_samples, still_samples, max_samples):
+    """temporal.slang:481-493 over whole images: prior and still [..., 4]
+    float32 colours, prior_samples the per-pixel span before the cap (the
+    history cap where the last changing frame had history, else 0)."""
+    prior = np.asarray(prior, F)
+    still = np.asarray(still, F)
+    span = np.asarray(prior_samples, np.uint64)
+    if max_samples:
+        span = np.minimum(span, max_samples)
+    keep = (still_samples >= span) | ~np.all(np.isfinite(prior), -1)
+    t = (F(still_samples) / np.maximum(span, 1).astype(F)).astype(F)
+    mixed = (prior + ((still - prior).astype(F) * t[..., None]).astype(F)).astype(F)
+    return np.where(keep[..., None], still, mixed).astype(F)
+
+
+def ref_settle(prior, still, prior_samples, still_samples, max_samples):
+    """The reference's SettleTemporalRadiance (temporal.cpp:514-528) through
+    oracle/_ref, pixel by pixel."""
+    import ctypes as C
+    import oracle
+    R = oracle.ref()
+    p = np.ascontiguousarray(prior, F)
+    s = np.ascontiguousarray(still, F)
+    n = np.ascontiguousarray(prior_samples, np.uint32)
+    out = np.empty_like(s)
+    fp, up = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+    R.ref_settle(p.ctypes.data_as(fp), s.ctypes.data_as(fp), n.ctypes.data_as(up), n.size, still_samples,
+                 max_samples, out.ctypes.data_as(fp))
     return out

@@ -50,14 +50,15 @@ def _run(renderer, sc, w, h, frames, views, *, radius, cap, candidates, max_boun
     for f in range(frames):
         view = views(f)
         w2c = scenes.multiply(sc.projection, view)
-        sample = first_sample + f
-        print(f"frame {f} sample {sample} {w}x{h} spatial={spatial}", flush=True)
-        out = renderer.render_temporal(w2c, view, w, h, frame_index=next(_FRAME), sample_index=sample,
+        print(f"frame {f} {w}x{h} spatial={spatial}", flush=True)
+        out = renderer.render_temporal(w2c, view, w, h, frame_index=next(_FRAME), sample_index=first_sample,
                                        camera_cut=f == 0,
                                        restir_temporal=True, restir_spatial=spatial, radius=radius,
                                        max_history_candidates=cap, max_bounces=max_bounces,
                                        light_sampling=hip.LIGHT_RESTIR, restir_candidates=candidates,
                                        want_products=True, spatial_disk=disk)
+        sample = out["batch_sample_index"]  # the stream continues from the base by itself
+        assert sample == first_sample + f and out["batch_samples"] == 1
         fr = oracle.make_frame(w2c, w, h, max_bounces=max_bounces, light_sampling=2, restir_candidates=candidates)
         vd = out["validity"]
         validity = np.stack([vd["valid"], vd["previous_index"]], -1).astype(np.uint32)

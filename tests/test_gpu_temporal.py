@@ -151,9 +151,10 @@ def test_static_temporal_accumulation_is_the_progressive_mean(renderer):
     w = h = 64
     n = 12
     for f in range(n):
-        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=100 + f, sample_index=f,
+        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=100 + f,
                                        accumulate=True, max_history=64, max_bounces=8, want_products=True)
         assert out["reset_reason"] == (1 if f == 0 else 0)
+        assert out["batch_sample_index"] == f and out["batch_samples"] == 1  # the stream continues
     hit = (out["surface"]["flags"] & 1) != 0
     assert hit.mean() > 0.9
     assert np.all(out["validity"]["valid"][hit] == 1)
@@ -207,7 +208,7 @@ def test_restir_reuse_is_consistent_and_reduces_error(renderer):
     kw = dict(max_bounces=8, light_sampling=hip.LIGHT_RESTIR, restir_candidates=8, want_restir=True)
     initial_err, reuse_err, means = [], [], []
     for f in range(24):
-        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=300 + f, sample_index=f,
+        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=300 + f,
                                        restir_temporal=True, restir_spatial=True, radius=8,
                                        max_history_candidates=20, **kw)
         rec = out["restir"]
@@ -239,7 +240,7 @@ def test_restir_reuse_products(renderer):
     renderer.apply(sc)
     w, h, radius = 48, 40, 6
     for f in range(4):
-        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=700 + f, sample_index=f,
+        out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=700 + f,
                                        restir_temporal=True, restir_spatial=True, radius=radius,
                                        max_history_candidates=20, max_bounces=8, light_sampling=hip.LIGHT_RESTIR,
                                        restir_candidates=8, want_products=True)
@@ -336,3 +337,111 @@ def test_reflection_weight_of_a_still_camera_keeps_everything(renderer):
     keep = s["reserved"].view(np.float32)[s["motion_depth"][..., 3] == 1.0]
     print(f"moved camera: kept share {keep.min():.3f}..{keep.max():.3f}")
     assert keep.min() < 0.9 and keep.max() <= 1.0 and keep.min() > 0.0
+
+
+def _settle_card():
+    # temporal-gpu-test.cpp:918-930: a metal card (GGX, roughness 0.5) with
+    # misses around it under a constant environment.
+    sc = scenes.Scene(name="settle-card")
+    sc.projection = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, -1.2, -1, 0, 0, -2.2, 0], np.float32)
+    m = sc.add_material(scenes.preview_material(base_color=(0.6, 0.4, 0.2), metallic=1.0, roughness=0.5))
+    sc.add_mesh([[-2, -2, -4], [2, -2, -4], [2, 2, -4], [-2, 2, -4]], [[0, 1, 2], [0, 2, 3]], m)
+    sc.environment = (1.0, 2.0, 3.0)
+    return sc
+
+
+def test_settling_sequence(renderer):
+    # SettlingSequence (temporal-gpu-test.cpp:911-1076): while the camera
+    # moves, frames accumulate and filter; held still, frames settle: their
+    # still samples are the progressive render of the host's sample index
+    # bit for bit (a second renderer's), the retained history is untouched,
+    # and the output is the reference's SettleTemporalRadiance
+    # (temporal.cpp:514-528, oracle/_ref) of the last changing frame's colour
+    # and the still mean, bit for bit, until it is the progressive image and
+    # stops. Switching settle off and on resets nothing and resumes the
+    # stream's samples; a change ends settling at once; a cut drops history.
+    from oracle import temporal as ot
+    sc = _settle_card()
+    renderer.apply(sc)
+    progressive = hip.Renderer(0)
+    try:
+        progressive.apply(sc)
+        w, h, max_samples, cap = 64, 48, 40, 32
+        frames = iter(range(20000, 30000))
+        base = dict(accumulate=True, max_history=cap, filter_iterations=5, max_bounces=1, sample_index=31,
+                    max_samples=max_samples, want_products=True)
+
+        def render(view, settle=True, **extra):
+            return renderer.render_temporal(scenes.multiply(sc.projection, view), view, w, h,
+                                            frame_index=next(frames), settle=settle, **base, **extra)
+
+        last = None
+        for k in range(4):
+            view = scenes.translation(-0.05 * k, 0.0, 0.0)
+            last = render(view, camera_cut=k == 0)
+            assert not last["settling"] and last["settled_samples"] == 0, k
+            assert k == 0 or last["history_available"], k
+            assert last["batch_sample_index"] == 31 + k and last["batch_samples"] == 1, k
+        misses = (last["surface"]["flags"] & 1) == 0
+        assert 0 < misses.sum() < w * h
+        prior = last["color"].copy()
+        history = last["radiance"].copy()
+        resume = last["batch_sample_index"] + last["batch_samples"]
+        w2c = scenes.multiply(sc.projection, view)
+        weight = np.where(history["sample_count"] != 0, cap, 0).astype(np.uint32)
+        span = np.minimum(weight, max_samples)
+        converged = still_ref = None
+        crossfaded = 0
+        for n in range(1, max_samples + 3):
+            out = render(view)
+            samples, added = min(n, max_samples), int(n <= max_samples)
+            assert out["settling"] and out["history_available"] and out["reset_reason"] == 0, n
+            assert out["settled_samples"] == samples and out["spp"] == samples, n
+            assert out["batch_samples"] == added and out["batch_sample_index"] == 31 + samples - added, n
+            assert out["radiance"].tobytes() == history.tobytes(), f"settling changed the history at {n}"
+            still = out["batch"]
+            if added:
+                if n == 1:
+                    progressive.render(w2c, w, h, sample_index=31 ^ 0x80000000, max_bounces=1,
+                                       max_samples=max_samples)
+                still_ref = progressive.render(w2c, w, h, sample_index=31, max_bounces=1,
+                                               max_samples=max_samples)["color"]
+            assert still.tobytes() == still_ref.tobytes(), f"still mean differs from the progressive image at {n}"
+            expected = ot.ref_settle(prior.reshape(-1, 4), still.reshape(-1, 4), weight.reshape(-1), samples,
+                                     max_samples).reshape(h, w, 4)
+            got = out["color"]
+            assert got.tobytes() == expected.tobytes(), f"crossfade differs from SettleTemporalRadiance at {n}"
+            done = samples >= span
+            assert got[done].tobytes() == still[done].tobytes()
+            crossfaded += int((~done).sum())
+            if n == max_samples:
+                converged = got.copy()
+            elif n > max_samples:
+                assert got.tobytes() == converged.tobytes(), "a converged view changed"
+        assert crossfaded != 0
+        assert converged.tobytes() == still_ref.tobytes(), "the settled image is not the progressive image"
+
+        # settle off: nothing resets, the stream resumes where it stopped
+        out = render(view, settle=False)
+        assert not out["settling"] and out["history_available"] and out["reset_reason"] == 0
+        assert out["batch_sample_index"] == resume
+        # settle on again: the first frame leaves the prior, the next settles
+        out = render(view)
+        assert not out["settling"] and out["history_available"] and out["reset_reason"] == 0
+        out = render(view)
+        assert out["settling"] and out["settled_samples"] == 1
+        kept = out["radiance"].copy()
+        # a change ends settling at once and resumes the retained history
+        moved = scenes.translation(0.05, 0.0, 0.0)
+        out = render(moved)
+        assert not out["settling"] and out["history_available"] and out["reset_reason"] == 0
+        hit = (out["validity"]["valid"] != 0)
+        assert hit.mean() > 0.2  # the card covers a quarter of the target
+        assert np.all(out["radiance"]["sample_count"][hit] > 1)  # blended the kept history
+        assert kept["sample_count"].max() > 1
+        out = render(moved, camera_cut=True)
+        assert not out["settling"] and not out["history_available"] and out["reset_reason"] == 3
+        with pytest.raises(hip.LotusError, match="settling requires radiance accumulation"):
+            renderer.render_temporal(w2c, view, w, h, frame_index=next(frames), settle=True, max_bounces=1)
+    finally:
+        progressive.close()

@@ -81,3 +81,31 @@ def test_reflection_weight_scales_the_kept_count():
     surf["reserved"] = np.array([1.0], np.float32).view(np.uint32)[0]
     assert np.array_equal(ot.accumulate(batch, surf, valid, old, 4, 32, ot.REFLECTION),
                           ot.accumulate(batch, surf, valid, old, 4, 32))
+
+
+def test_settle_closed_forms_and_reference():
+    # SettleTemporalRadiance (temporal.slang:481-493): half way through its
+    # span the output is the midpoint of prior and still; from the span on,
+    # on a miss (span 0) and where the prior is not finite it is `still` bit
+    # for bit; max_samples caps the span. The restatement equals the
+    # reference's CPU form (temporal.cpp:514-528) bit for bit.
+    import oracle
+    rng = np.random.default_rng(4)
+    n = 4096
+    prior = rng.lognormal(-1, 1, (n, 4)).astype(np.float32)
+    still = rng.lognormal(-1, 1, (n, 4)).astype(np.float32)
+    spans = np.where(rng.random(n) < 0.2, 0, 32).astype(np.uint32)
+    prior[:50, 1] = np.inf
+    prior[50:60, 2] = np.nan
+    for samples, cap in ((16, 0), (16, 40), (7, 10), (32, 0), (40, 40), (1, 0)):
+        got = ot.settle(prior, still, spans, samples, cap)
+        span = np.minimum(spans, cap) if cap else spans
+        ends = (samples >= span) | ~np.all(np.isfinite(prior), -1)
+        assert np.array_equal(got[ends], still[ends])
+        if samples == 16 and cap == 0:
+            mid = ~ends
+            assert mid.sum() > 1000
+            assert np.allclose(got[mid], (prior[mid] + still[mid]) / 2, rtol=1e-6)
+        if oracle.ref() is not None:
+            ref = ot.ref_settle(prior, still, spans, samples, cap)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (samples, cap)
