@@ -66,7 +66,8 @@ __device__ __forceinline__ Tri world_triangle(const BuildInput& in, uint32_t g) 
   Tri t;
   t.a = make_float4(w[0][0], w[0][1], w[0][2], __uint_as_float(g));
   t.b = make_float4(w[1][0], w[1][1], w[1][2], __uint_as_float(inst));
-  t.c = make_float4(w[2][0], w[2][1], w[2][2], __uint_as_float(prim));
+  const uint32_t covered = (in.instance_covered && in.instance_covered[inst]) ? kCoveredBit : 0u;
+  t.c = make_float4(w[2][0], w[2][1], w[2][2], __uint_as_float(prim | covered));
   return t;
 }
 
@@ -485,6 +486,15 @@ __global__ void __launch_bounds__(kBlock) k_reflatten(BuildInput in, const uint3
   tris[tri_of_global[g]] = world_triangle(in, g);
 }
 
+// The covered bits of the triangles after the instances' materials changed.
+__global__ void __launch_bounds__(kBlock) k_mark_covered(Tri* tris, uint32_t n, const uint32_t* instance_covered) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t inst = __float_as_uint(tris[i].b.w);
+  const uint32_t prim = __float_as_uint(tris[i].c.w) & ~kCoveredBit;
+  tris[i].c.w = __uint_as_float(prim | (instance_covered && instance_covered[inst] ? kCoveredBit : 0u));
+}
+
 // One level's nodes [first, first + count): each child's box from its
 // triangles (a leaf) or from the exact box its node wrote one launch earlier,
 // with the build's margin; the node's exact box is their union.
@@ -748,6 +758,12 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   out.bytes += uint64_t(max_nodes) * sizeof(WideNode);
   release();
   return true;
+}
+
+hipError_t mark_covered(SceneBuffers& s, uint32_t tri_count, const uint32_t* instance_covered, hipStream_t stream) {
+  if (tri_count == 0) return hipSuccess;
+  k_mark_covered<<<blocks_for(tri_count, kBlock), kBlock, 0, stream>>>(s.tris, tri_count, instance_covered);
+  return hipGetLastError();
 }
 
 bool refit_bvh(const BuildInput& in, SceneBuffers& s, hipStream_t stream, double& gpu_ms, std::string& error) {

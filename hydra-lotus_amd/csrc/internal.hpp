@@ -102,6 +102,9 @@ struct SceneView {
   const uint32_t* instance_normals;
   const float* colors;
   const uint32_t* instance_colors;
+  // Instances whose material's coverage is tested (their triangles carry
+  // kCoveredBit); 0: every surface is opaque.
+  uint32_t covered_instances;
 };
 
 // The light-table entry of the environment image (gpu_scene.hpp:132-134).
@@ -112,7 +115,7 @@ constexpr uint32_t kEnvironmentLight = 0xFFFFFFFFu;
 // colours run (the others compile none of that code).
 __host__ __device__ inline bool scene_has_extras(const SceneView& S) {
   return S.analytic_count != 0 || S.env_width != 0 || S.bindings != nullptr || S.texcoords != nullptr ||
-         S.instance_normals != nullptr || S.instance_colors != nullptr;
+         S.instance_normals != nullptr || S.instance_colors != nullptr || S.covered_instances != 0;
 }
 
 // Device memory of the resident scene, kept until a plan needs a new BVH.
@@ -145,6 +148,11 @@ struct SceneBuffers {
   uint32_t* instance_normals = nullptr;
   float* colors = nullptr;
   uint32_t* instance_colors = nullptr;
+  // Per instance 1 when its material's coverage is tested (the host's copy
+  // beside it); covered_instances counts them.
+  uint32_t* instance_covered = nullptr;
+  std::vector<uint32_t> covered_host;
+  uint32_t covered_instances = 0;
   uint64_t normal_count = 0, color_count = 0;
   uint32_t texture_count = 0, binding_count = 0;
   uint64_t texel_count = 0, texcoord_count = 0;
@@ -178,6 +186,7 @@ struct BuildInput {
   const uint32_t* instance_geometry;
   const float* instance_transform;
   const uint32_t* instance_base;  // prefix sum of instance triangle counts, instance_count + 1
+  const uint32_t* instance_covered;  // per instance 1 when its coverage is tested; null: none is
   uint32_t instance_count;
   uint32_t tri_count;
   uint32_t leaf_size;
@@ -192,6 +201,10 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
 // level by level from the leaves up. The boxes equal those a build of the
 // same topology writes (bvh_build.hip).
 bool refit_bvh(const BuildInput& in, SceneBuffers& s, hipStream_t stream, double& gpu_ms, std::string& error);
+
+// Rewrites the triangles' covered bits from instance_covered (null: none)
+// after the instances' materials changed (bvh_build.hip).
+hipError_t mark_covered(SceneBuffers& s, uint32_t tri_count, const uint32_t* instance_covered, hipStream_t stream);
 
 // Test hook of the sort (bvh_build.hip): stable sort of keys with values.
 bool radix_sort_pairs(uint32_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits,

@@ -90,10 +90,11 @@ __device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
 // the wave (or all of it) is ready, the ready lanes hand their results to
 // finish(item, walk) together and start their next items. No atomics: the
 // sequences are fixed, so neighbouring lanes keep neighbouring items.
-template <class Fetch, class Finish>
+template <class Cov, class Fetch, class Finish>
 __device__ __forceinline__ void sequence_walks(const SceneView& S, TraceStack& st, uint32_t total,
-                                               Fetch fetch, Finish finish) {
-  Walk w;
+                                               Fetch fetch, Finish finish, const Cov& cov) {
+  WalkT<Cov> w;
+  w.cov = cov;
   bool working = false;  // a walk in progress
   bool holding = false;  // a finished walk whose result awaits hand-over
   uint32_t item = 0;
@@ -132,11 +133,12 @@ __device__ __forceinline__ void sequence_walks(const SceneView& S, TraceStack& s
 
 // The same walks without hand-over: each lane follows one ray to its end,
 // striding over the queue.
-template <class Fetch, class Finish>
+template <class Cov, class Fetch, class Finish>
 __device__ __forceinline__ void strided_walks(const SceneView& S, TraceStack& st, uint32_t total, Fetch fetch,
-                                              Finish finish) {
+                                              Finish finish, const Cov& cov) {
   for (uint32_t i = gtid(); i < total; i += gstride()) {
-    Walk w;
+    WalkT<Cov> w;
+    w.cov = cov;
     fetch(i, w);
     st.clear();
     w.run(S.nodes, S.tris, st);
@@ -144,11 +146,42 @@ __device__ __forceinline__ void strided_walks(const SceneView& S, TraceStack& st
   }
 }
 
-template <bool kDynamic, class Fetch, class Finish>
+template <bool kDynamic, class Cov, class Fetch, class Finish>
 __device__ __forceinline__ void walks(const SceneView& S, TraceStack& st, uint32_t total, Fetch fetch,
-                                      Finish finish) {
-  if constexpr (kDynamic) sequence_walks(S, st, total, fetch, finish);
-  else strided_walks(S, st, total, fetch, finish);
+                                      Finish finish, const Cov& cov) {
+  if constexpr (kDynamic) sequence_walks(S, st, total, fetch, finish, cov);
+  else strided_walks(S, st, total, fetch, finish, cov);
+}
+
+// The coverage policy of a kernel variant: the scene's when kCov, else none.
+template <bool kCov>
+using CoverageOf = typename std::conditional<kCov, SceneCoverage, NoCoverage>::type;
+template <bool kCov>
+__device__ __forceinline__ CoverageOf<kCov> coverage_of(const SceneView& S) {
+  if constexpr (kCov) return SceneCoverage{&S};
+  else return NoCoverage{};
+}
+
+// A shadow ray's pending light once its walk ended (ShadowedLight's end,
+// transport.slang:105-114): without coverage, the contribution shading
+// stored, when nothing blocked the ray; with coverage, f * radiance *
+// (transmittance * weight / pdf) from the factors shading stored, times the
+// path's throughput, when the transmittance is positive.
+template <class Cov>
+__device__ __forceinline__ bool shadow_light(const Queues& q, uint32_t j, const WalkT<Cov>& w, float4& nee) {
+  if constexpr (!Cov::kEnabled) {
+    if (w.hit()) return false;
+    const float4 c4 = q.sh_c[j];
+    nee = make_float4(c4.x, c4.y, c4.z, 1.0f);
+    return true;
+  } else {
+    const float T = w.transmittance();
+    if (!(T > 0.0f)) return false;
+    const float4 f4 = q.sh_f[j], t4 = q.sh_t[j];
+    const V3 d = shadowed_delta(V3{f4.x, f4.y, f4.z}, f4.w, t4.w, V3{t4.x, t4.y, t4.z}, T);
+    nee = make_float4(d.x, d.y, d.z, 1.0f);
+    return true;
+  }
 }
 
 __device__ __forceinline__ uint32_t* ray_count(uint32_t* c, uint32_t b) { return c + counter_word(b, 0); }
@@ -165,7 +198,7 @@ __device__ __forceinline__ void push_hit(const Queues& q, uint32_t k, uint32_t s
 // wavefront.slang:206-225: a camera path per pixel of the data window, its
 // camera ray traced at once; a hit is queued for shading, a miss leaves no
 // sample. Clears the slot's pending terms.
-template <bool kDynamic>
+template <bool kDynamic, bool kCov>  // kCov: the scene has covered surfaces
 __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                         uint2* spill, int spill_cap, unsigned long long* stats) {
   __shared__ uint2 ring[(kRing + 1) * kTraceBlock];  // + the lanes' traversal counts
@@ -189,7 +222,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   walks<kDynamic>(
       S, st, tiles * 64,
-      [&](uint32_t i, Walk& w) {
+      [&](uint32_t i, auto& w) {
         uint32_t rng, x, y;
         if (!pixel(i, x, y)) {
           w.start(Ray{V3{0.0f, 0.0f, 0.0f}, 0.0f, V3{0.0f, 0.0f, 1.0f}, 0.0f}, false);
@@ -198,7 +231,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
         }
         w.start(start_path(F, F.rect[0] + x, F.rect[1] + y, F.sample_index, rng), false);
       },
-      [&](uint32_t i, const Walk& w) {
+      [&](uint32_t i, const auto& w) {
         uint32_t x, y;
         if (!pixel(i, x, y)) return;
         x += F.rect[0];
@@ -225,7 +258,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
           q.restir[4ull * slot + 1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(failed));
           q.restir[4ull * slot + 2] = zero;
         }
-      });
+      },
+      coverage_of<kCov>(S));
   ls.flush(stats, 0);
 }
 
@@ -292,9 +326,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
     if (trace) {
       const LightSample ls = reconstruct_reservoir_light<kLights>(S, r.sample, ru, rv, origin);
       uint32_t visits = 0, tests = 0;
-      const float visibility =
-          walk_occluded(S.nodes, S.tris, st, Ray{origin, 0.0f, ls.direction, ls.distance}, visits, tests) ? 0.0f
-                                                                                                          : 1.0f;
+      const float visibility = walk_transmittance(S.nodes, S.tris, st, Ray{origin, 0.0f, ls.direction, ls.distance},
+                                                  visits, tests, coverage_of<kLights>(S));
       ls_counts.add(visits, tests);
       const float normalization = reservoir_normalization(r);
       const V3 estimate = candidate_contribution(sf, ft, fb, wo, ls, S.lights[r.sample].pdf) *
@@ -444,10 +477,8 @@ __device__ __forceinline__ void resolve_estimate(const SceneView& S, TraceStack&
     visibility = own.contribution.w;
   } else {
     ++rays;
-    visibility = walk_occluded(S.nodes, S.tris, st, Ray{receiver.origin, 0.0f, ls.direction, ls.distance}, visits,
-                               tests)
-                     ? 0.0f
-                     : 1.0f;
+    visibility = walk_transmittance(S.nodes, S.tris, st, Ray{receiver.origin, 0.0f, ls.direction, ls.distance},
+                                    visits, tests, coverage_of<kLights>(S));
   }
   const V3 estimate = reuse_contribution(S, receiver, result.r.sample, ls) * (visibility * result.normalization);
   if (!isfinite(result.normalization) || result.normalization <= 0.0f || !finite3(estimate))
@@ -661,7 +692,7 @@ __device__ __forceinline__ void cache_query(const FrameConst& F, const Surface& 
 // its emission term, leaving a cache query with its throughput, and a
 // training path records each vertex (query, throughput on arrival, radiance
 // including the vertex's emission).
-template <bool kAnalytic>  // the scene has analytic lights
+template <bool kAnalytic, bool kCov>  // the scene has analytic lights (extras); covered surfaces
 __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                        uint32_t bounce, uint32_t link) {
   const uint32_t count = *hit_count(counters, bounce);
@@ -700,14 +731,15 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
     const V3 arrival = p.throughput, dir = p.d;
     Scatter s;
     Surface sf;
+    ShadowFactors factors;
     if (query) {
       sf = reconstruct_hit<kAnalytic>(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z), __uint_as_float(h4.w), p.d);
       add_emission(S, F, p, sf, h4.x, h4.y);
       s.continues = false;
       s.shadow = false;
     } else {
-      s = scatter_path<kAnalytic>(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z), __uint_as_float(h4.w),
-                       train ? &sf : nullptr, slot);
+      s = scatter_path<kAnalytic, kCov>(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z), __uint_as_float(h4.w),
+                       train ? &sf : nullptr, slot, kCov ? &factors : nullptr);
     }
     const uint32_t kq = wave_append(cache_query_count(counters), query);
     if (query) {
@@ -727,6 +759,10 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
       q.sh_d[ks] = make_float4(s.shadow_ray.d.x, s.shadow_ray.d.y, s.shadow_ray.d.z, s.shadow_ray.tmax);
       // w: the tail walks this ray with its path (1) or on its own (0)
       q.sh_c[ks] = make_float4(s.delta.x, s.delta.y, s.delta.z, __uint_as_float(link && s.continues ? 1u : 0u));
+      if constexpr (kCov) {
+        q.sh_f[ks] = make_float4(factors.fl.x, factors.fl.y, factors.fl.z, factors.weight);
+        q.sh_t[ks] = make_float4(factors.thr.x, factors.thr.y, factors.thr.z, factors.pdf);
+      }
     }
     const uint32_t kr = wave_append(ray_count(counters, bounce + 1), s.continues);
     if (s.continues) {
@@ -796,7 +832,7 @@ __global__ void __launch_bounds__(256) k_cache_samples(FrameConst F, Queues q, u
 // for shading, a miss leaves its environment term pending, transport.slang:
 // 436-448), items [n_ext, n_ext + n_shadow) the shadow rays of bounce b - 1
 // (an unblocked ray leaves its light pending).
-template <bool kDynamic>
+template <bool kDynamic, bool kCov>  // kCov: the scene has covered surfaces
 __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, uint32_t* counters, uint2* spill,
                                                        int spill_cap, uint32_t bounce, uint32_t kinds,
                                                        unsigned long long* stats) {
@@ -811,14 +847,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
   bs.init(counts);
   walks<kDynamic>(
       S, st, n_ext + n_sh,
-      [&](uint32_t i, Walk& w) {
+      [&](uint32_t i, auto& w) {
         const bool ext = i < n_ext;
         const uint32_t j = ext ? i : i - n_ext;
         const float4 o4 = ext ? q.ray_o[j] : q.sh_o[j];
         const float4 d4 = ext ? q.ray_d[j] : q.sh_d[j];
         w.start(Ray{xyz(o4), 0.0f, xyz(d4), d4.w}, !ext);
       },
-      [&](uint32_t i, const Walk& w) {
+      [&](uint32_t i, const auto& w) {
         const bool ext = i < n_ext;
         const uint32_t j = ext ? i : i - n_ext;
         const uint32_t slot = __float_as_uint(ext ? q.ray_o[j].w : q.sh_o[j].w);
@@ -839,11 +875,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(SceneView S, Queues q, ui
           float4 r4 = q.rad[slot];
           r4.w = 1.0f;
           q.rad[slot] = r4;
-        } else if (!hit) {
-          const float4 c4 = q.sh_c[j];
-          q.nee[slot] = make_float4(c4.x, c4.y, c4.z, 1.0f);
+        } else {
+          float4 nee;
+          if (shadow_light(q, j, w, nee)) q.nee[slot] = nee;
         }
-      });
+      },
+      coverage_of<kCov>(S));
   bs.flush(stats, 1);
 }
 
@@ -877,7 +914,9 @@ __global__ void __launch_bounds__(256) k_escape(SceneView S, Queues q, uint32_t*
 // walk, so radiance is summed in the reference's order.
 __device__ __forceinline__ float pair_float(float v) { return __shfl_xor(v, 1); }
 
-template <bool kCache, bool kAnalytic>  // a radiance cache frame (training vertices recorded); analytic lights
+// a radiance cache frame (training vertices recorded); analytic lights
+// (extras); covered surfaces
+template <bool kCache, bool kAnalytic, bool kCov>
 __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                       uint2* spill, int spill_cap, uint32_t bounce,
                                                       unsigned long long* stats) {
@@ -898,7 +937,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
       const uint32_t j = i - count;
       const float4 c4 = q.sh_c[j];
       const bool alone = __float_as_uint(c4.w) == 0u;
-      Walk w;
+      WalkT<CoverageOf<kCov>> w;
+      w.cov = coverage_of<kCov>(S);
       uint32_t sslot = 0;
       if (!path_lane && alone) {
         const float4 o4 = q.sh_o[j], d4 = q.sh_d[j];
@@ -912,7 +952,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
       w.run(S.nodes, S.tris, st);
       visits += w.visits;
       tests += w.tests;
-      if (!path_lane && alone && !w.hit()) q.nee[sslot] = make_float4(c4.x, c4.y, c4.z, 1.0f);
+      if constexpr (kCov) {
+        float4 nee;
+        if (!path_lane && alone && shadow_light(q, j, w, nee)) q.nee[sslot] = nee;
+      } else {
+        if (!path_lane && alone && !w.hit()) q.nee[sslot] = make_float4(c4.x, c4.y, c4.z, 1.0f);
+      }
       continue;
     }
     PathRegs p{};
@@ -947,6 +992,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
     bool shadow = false;      // a shadow ray of the previous bounce is pending
     Ray sray{};               // that ray (sent to the odd lane)
     V3 delta{};               // its contribution (kept in the path lane)
+    ShadowFactors sf4{};      // with covered surfaces: its factors
     if (path_lane) {
       // the shadow ray the last round's shading linked to this path
       const uint32_t link = __float_as_uint(q.ray_d[i].w);
@@ -955,6 +1001,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
         shadow = true;
         sray = Ray{xyz(o4), 0.0f, xyz(d4), d4.w};
         delta = xyz(c4);
+        if constexpr (kCov) {
+          const float4 f4 = q.sh_f[link], t4 = q.sh_t[link];
+          sf4 = ShadowFactors{xyz(f4), f4.w, t4.w, xyz(t4)};
+        }
       }
     }
     while (true) {
@@ -965,7 +1015,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
       // the odd lane receives the shadow ray
       const Ray sr{V3{pair_float(sray.o.x), pair_float(sray.o.y), pair_float(sray.o.z)}, pair_float(sray.tmin),
                    V3{pair_float(sray.d.x), pair_float(sray.d.y), pair_float(sray.d.z)}, pair_float(sray.tmax)};
-      Walk w;
+      WalkT<CoverageOf<kCov>> w;
+      w.cov = coverage_of<kCov>(S);
       if (path_lane) {
         w.start(Ray{p.o, 0.0f, p.d, kFloatMax}, false);
         if (!extend) w.cur = kWalkDone;
@@ -977,10 +1028,19 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
       w.run(S.nodes, S.tris, st);
       visits += w.visits;
       tests += w.tests;
-      const bool blocked = __shfl_xor(static_cast<int>(w.hit()), 1) != 0;  // the odd lane's answer
-      if (path_lane && shadow && !blocked) p.radiance = p.radiance + delta;
+      // the odd lane's answer: blocked, or with covered surfaces its
+      // transmittance (0: blocked)
+      if constexpr (kCov) {
+        const float transmittance = pair_float(w.transmittance());
+        if (path_lane && shadow && transmittance > 0.0f)
+          p.radiance = p.radiance + shadowed_delta(sf4.fl, sf4.weight, sf4.pdf, sf4.thr, transmittance);
+      } else {
+        const bool blocked = __shfl_xor(static_cast<int>(w.hit()), 1) != 0;
+        if (path_lane && shadow && !blocked) p.radiance = p.radiance + delta;
+      }
       shadow = false;
       Scatter sc{};
+      ShadowFactors sc4;
       bool hit = false;
       if (path_lane && extend) {
         hit = w.hit();
@@ -994,8 +1054,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
           const V3 arrival = p.throughput, dir = p.d;
           const uint32_t b = p.depth;
           Surface sf;
-          sc = scatter_path<kAnalytic>(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v, kCache && train ? &sf : nullptr,
-                            slot);
+          sc = scatter_path<kAnalytic, kCov>(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v, kCache && train ? &sf : nullptr,
+                            slot, kCov ? &sc4 : nullptr);
           if (kCache && train && b < kCacheRecordDepth && path < F.train_capacity) {
             float4* rec = q.crec + 6ull * (static_cast<size_t>(b) * F.train_capacity + path);
             cache_query(F, sf, dir, rec);
@@ -1012,6 +1072,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
         shadow = true;
         sray = sc.shadow_ray;
         delta = sc.delta;
+        if constexpr (kCov) sf4 = sc4;
       }
       extend = shaded && sc.continues;
     }
@@ -1070,7 +1131,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_pass(SceneView S, FrameC
     const Ray r = camera_ray(F.clip_to_world, fx * F.ndc_from_pixel[0] + F.ndc_from_pixel[2],
                              fy * F.ndc_from_pixel[1] + F.ndc_from_pixel[3]);
     HitRecord h;
-    if (!walk_closest(S.nodes, S.tris, st, r, h, visits, tests)) continue;
+    if (!walk_closest(S.nodes, S.tris, st, r, h, visits, tests, SceneCoverage{&S})) continue;
     const V3 pos = r.o + r.d * h.t;
     float clip[4];
     mul_point(F.world_to_clip, pos.x, pos.y, pos.z, 1.0f, clip);
@@ -1097,13 +1158,17 @@ __global__ void __launch_bounds__(kTraceBlock) k_rays(SceneView S, const float4*
   st.init(ring, spill, spill_cap);
   strided_walks(
       S, st, count,
-      [&](uint32_t i, Walk& w) {
+      [&](uint32_t i, auto& w) {
         const float4 o4 = rays[2 * i], d4 = rays[2 * i + 1];
         w.start(Ray{xyz(o4), o4.w, xyz(d4), d4.w}, any_hit != 0);
       },
-      [&](uint32_t i, const Walk& w) {
+      [&](uint32_t i, const auto& w) {
+        if (any_hit == 2u) {
+          hits[i] = __float_as_uint(w.transmittance());
+          return;
+        }
         if (any_hit) {
-          hits[i] = w.hit() ? 1u : 0u;
+          hits[i] = w.transmittance() > 0.0f ? 0u : 1u;
           return;
         }
         uint32_t* o = hits + 5ull * i;
@@ -1121,7 +1186,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_rays(SceneView S, const float4*
           o[3] = 0u;
           o[4] = 0u;
         }
-      });
+      },
+      SceneCoverage{&S});
 }
 
 __global__ void k_zero(uint32_t* p, uint32_t n) {
@@ -1151,16 +1217,16 @@ uint32_t small_grid(uint32_t n) { return (n + 255) / 256 < 1024 ? (n + 255) / 25
 
 GridShape grid_shape(uint32_t cus) {
   GridShape g;
-  g.camera = persistent_blocks(k_camera<false>, kTraceBlock, cus);
-  g.camera_dynamic = persistent_blocks(k_camera<true>, kTraceBlock, cus);
-  // sized for the variants of scenes without analytic lights
-  g.trace = persistent_blocks(k_trace<false>, kTraceBlock, cus);
-  g.trace_dynamic = persistent_blocks(k_trace<true>, kTraceBlock, cus);
-  g.tail = persistent_blocks(k_tail<false, false>, kTraceBlock, cus);
-  g.tail_cache = persistent_blocks(k_tail<true, false>, kTraceBlock, cus);
+  g.camera = persistent_blocks(k_camera<false, false>, kTraceBlock, cus);
+  g.camera_dynamic = persistent_blocks(k_camera<true, false>, kTraceBlock, cus);
+  // sized for the variants of scenes without analytic lights or covered surfaces
+  g.trace = persistent_blocks(k_trace<false, false>, kTraceBlock, cus);
+  g.trace_dynamic = persistent_blocks(k_trace<true, false>, kTraceBlock, cus);
+  g.tail = persistent_blocks(k_tail<false, false, false>, kTraceBlock, cus);
+  g.tail_cache = persistent_blocks(k_tail<true, false, false>, kTraceBlock, cus);
   g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
   g.rays = persistent_blocks(k_rays, kTraceBlock, cus);
-  g.shade = persistent_blocks(k_shade<false>, kShadeBlock, cus);
+  g.shade = persistent_blocks(k_shade<false, false>, kShadeBlock, cus);
   g.resolve = cus * 4;
   g.max_trace_lanes =
       std::max({g.camera, g.camera_dynamic, g.trace, g.trace_dynamic, g.tail, g.tail_cache, g.camera_pass, g.rays}) * kTraceBlock;
@@ -1171,44 +1237,61 @@ void launch_zero(uint32_t* p, uint32_t n, hipStream_t s) { k_zero<<<small_grid(n
 void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s) { k_fill4<<<small_grid(n), 256, 0, s>>>(p, n, v); }
 void launch_fill1(float* p, uint32_t n, float v, hipStream_t s) { k_fill1<<<small_grid(n), 256, 0, s>>>(p, n, v); }
 void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
-  if (L.dynamic_camera)
-    k_camera<true><<<L.grid.camera_dynamic, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+  const bool cov = S.covered_instances != 0;
+  if (L.dynamic_camera && cov)
+    k_camera<true, true><<<L.grid.camera_dynamic, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+  else if (L.dynamic_camera)
+    k_camera<true, false><<<L.grid.camera_dynamic, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap,
+                                                                              L.stats);
+  else if (cov)
+    k_camera<false, true><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
   else
-    k_camera<false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+    k_camera<false, false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce, bool link) {
-  if (scene_has_extras(S))
-    k_shade<true><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
+  if (S.covered_instances != 0)
+    k_shade<true, true><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
+  else if (scene_has_extras(S))
+    k_shade<true, false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
   else
-    k_shade<false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
+    k_shade<false, false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
 }
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds) {
-  if (L.dynamic_trace)
-    k_trace<true><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
-                                                                L.stats);
+  const bool cov = S.covered_instances != 0;
+  if (L.dynamic_trace && cov)
+    k_trace<true, true><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce,
+                                                                          kinds, L.stats);
+  else if (L.dynamic_trace)
+    k_trace<true, false><<<L.grid.trace_dynamic, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce,
+                                                                           kinds, L.stats);
+  else if (cov)
+    k_trace<false, true><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
+                                                                    L.stats);
   else
-    k_trace<false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
-                                                                 L.stats);
+    k_trace<false, false><<<L.grid.trace, kTraceBlock, 0, L.stream>>>(S, q, c, L.spill, L.spill_cap, bounce, kinds,
+                                                                     L.stats);
 }
 void launch_escape(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce, bool weigh) {
   k_escape<<<L.grid.resolve, 256, 0, L.stream>>>(S, q, c, bounce, weigh ? 1u : 0u);
 }
 void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                  uint32_t bounce) {
-  const bool analytic = scene_has_extras(S);
-  if (F.cache_depth != 0 && analytic)
-    k_tail<true, true><<<L.grid.tail_cache, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
-                                                                       L.stats);
+  const bool analytic = scene_has_extras(S), cov = S.covered_instances != 0;
+  const uint32_t grid = F.cache_depth != 0 ? L.grid.tail_cache : L.grid.tail;
+  if (F.cache_depth != 0 && cov)
+    k_tail<true, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+  else if (F.cache_depth != 0 && analytic)
+    k_tail<true, true, false><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
   else if (F.cache_depth != 0)
-    k_tail<true, false><<<L.grid.tail_cache, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
-                                                                        L.stats);
+    k_tail<true, false, false><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+  else if (cov)
+    k_tail<false, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
   else if (analytic)
-    k_tail<false, true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+    k_tail<false, true, false><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
   else
-    k_tail<false, false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
-                                                                    L.stats);
+    k_tail<false, false, false><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
 }
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample) {
   k_resolve<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, add_sample);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "nrc.hpp"
 #include "transport.hpp"
@@ -34,6 +35,8 @@ struct Queues {
   float4* sh_o;   // shadow queue: origin, slot
   float4* sh_d;   //               direction, distance
   float4* sh_c;   //               radiance delta if unblocked
+  float4* sh_f;   //               with covered surfaces: f * radiance, MIS weight
+  float4* sh_t;   //                 and the path's throughput, the light's pdf
   float4* accum;  // per pixel: radiance sum, hit samples
   float4* color;  // per pixel: mean (RGBA32F product)
   // Radiance cache frames.

@@ -322,6 +322,8 @@ void free_target(lotus_hip_renderer* r) {
   dev_free(q.sh_o);
   dev_free(q.sh_d);
   dev_free(q.sh_c);
+  dev_free(q.sh_f);
+  dev_free(q.sh_t);
   dev_free(q.accum);
   dev_free(q.color);
   dev_free(q.cq);
@@ -358,6 +360,9 @@ void free_scene(lotus_hip_renderer* r) {
   dev_free(s.instance_texcoords);
   dev_free(s.normals);
   dev_free(s.instance_normals);
+  dev_free(s.instance_covered);
+  s.covered_host.clear();
+  s.covered_instances = 0;
   dev_free(s.colors);
   dev_free(s.instance_colors);
   dev_free(s.positions);
@@ -394,8 +399,8 @@ hipError_t ensure_target(lotus_hip_renderer* r, uint32_t width, uint32_t height)
   const size_t P = size_t(width) * height;
   Queues& q = r->q;
   hipError_t e = hipSuccess;
-  for (float4** p : {&q.thr, &q.rad, &q.nee, &q.esc, &q.esc2, &q.esc_o, &q.esc_d, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.accum,
-                     &q.color})
+  for (float4** p : {&q.thr, &q.rad, &q.nee, &q.esc, &q.esc2, &q.esc_o, &q.esc_d, &q.ray_o, &q.ray_d, &q.hit_o, &q.hit_d, &q.sh_o, &q.sh_d, &q.sh_c, &q.sh_f,
+                     &q.sh_t, &q.accum, &q.color})
     if (e == hipSuccess) e = dev_alloc(*p, P);
   if (e == hipSuccess) e = dev_alloc(q.hit_h, P);
   if (e == hipSuccess) e = dev_alloc(r->depth, P);
@@ -1050,15 +1055,34 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   for (uint32_t m = 0; m < p->material_count; ++m) {
     const uint32_t* w = p->materials + 36ull * m;
     if ((w[0] & 0xFFu) != 0) return "material " + std::to_string(m) + ": only UsdPreviewSurface is supported";
-    if (w[3] != 0 || (w[2] & ~0x7Eu) != 0)
-      return "material " + std::to_string(m) + ": only normal, base colour, roughness, metallic, emission and specular colour lookups are supported";
+    if (w[3] != 0 || (w[2] & ~0x7Fu) != 0)
+      return "material " + std::to_string(m) + ": only opacity, normal, base colour, roughness, metallic, emission and specular colour lookups are supported";
     if (w[2] != 0 && (!p->bindings || uint64_t(w[1]) + __builtin_popcount(w[2]) > p->binding_count))
       return "material " + std::to_string(m) + ": its lookups lie outside the binding table";
-    float opacity;
+    float opacity, threshold;
     std::memcpy(&opacity, w + 6, 4);
-    if (opacity != 1.0f) return "material " + std::to_string(m) + ": fractional opacity is not supported";
+    std::memcpy(&threshold, w + 7, 4);
+    // ValidGeometry (gpu_scene.cpp:821-831)
+    if (!(opacity >= 0.0f && opacity <= 1.0f) || !(threshold >= 0.0f && threshold <= 1.0f))
+      return "material " + std::to_string(m) + ": opacity and its threshold must lie in [0, 1]";
   }
   return {};
+}
+
+// Per instance 1 when a ray tests its material's coverage (scene.slang:
+// 393-406): an opacity lookup, or a constant coverage below one (opacity
+// below one without a threshold, or below a positive threshold).
+std::vector<uint32_t> covered_flags(const lotus_hip_scene_plan* p) {
+  std::vector<uint32_t> covered(p->instance_count, 0u);
+  for (uint32_t i = 0; i < p->instance_count; ++i) {
+    const uint32_t* w = p->materials + 36ull * p->instance_material[i];
+    float opacity, threshold;
+    std::memcpy(&opacity, w + 6, 4);
+    std::memcpy(&threshold, w + 7, 4);
+    const float constant = threshold > 0.0f ? (opacity >= threshold ? 1.0f : 0.0f) : opacity;
+    covered[i] = ((w[2] & 1u) != 0 || !(constant >= 1.0f)) ? 1u : 0u;
+  }
+  return covered;
 }
 
 // The extents of a plan's arrays.
@@ -1273,6 +1297,7 @@ void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environme
   v.instance_normals = s.normal_count ? s.instance_normals : nullptr;
   v.colors = s.colors;
   v.instance_colors = s.color_count ? s.instance_colors : nullptr;
+  v.covered_instances = s.covered_instances;
   v.tri_count = sh.tri_count;
   v.instance_count = sh.instances;
   for (int c = 0; c < 3; ++c) v.environment[c] = environment[c];
@@ -1374,6 +1399,7 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   if (e == hipSuccess) e = dev_alloc(s.instance_geometry, I);
   if (e == hipSuccess) e = dev_alloc(s.instance_base, I + 1);
   if (e == hipSuccess) e = dev_alloc(s.instance_material, I);
+  if (e == hipSuccess) e = dev_alloc(s.instance_covered, I);
   if (e == hipSuccess) e = dev_alloc(s.instance_transform, 16ull * I);
   if (e == hipSuccess) e = dev_alloc(s.materials, 36ull * M);
   if (e == hipSuccess) e = dev_alloc(s.light_offsets, I);
@@ -1391,6 +1417,9 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   up(s.instance_geometry, p->instance_geometry, I * 4ull);
   up(s.instance_base, sh.base.data(), sh.base.size() * 4);
   up(s.instance_material, p->instance_material, I * 4ull);
+  s.covered_host = covered_flags(p);
+  s.covered_instances = static_cast<uint32_t>(std::count(s.covered_host.begin(), s.covered_host.end(), 1u));
+  up(s.instance_covered, s.covered_host.data(), I * 4ull);
   up(s.instance_transform, p->instance_transform, 64ull * I);
   up(s.materials, p->materials, 144ull * M);
   if (e == hipSuccess) e = upload_light_table(s, table, up);
@@ -1455,7 +1484,8 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   s.index_words = sh.index_words;
   s.material_count = M;
   const BuildInput in{s.positions, s.triangles, s.geometry_vertex_offset, s.geometry_triangle_offset,
-                      s.instance_geometry, s.instance_transform, s.instance_base, I, sh.tri_count, sh.leaf_size};
+                      s.instance_geometry, s.instance_transform, s.instance_base,
+                      s.covered_instances ? s.instance_covered : nullptr, I, sh.tri_count, sh.leaf_size};
   if (!build_bvh(in, s, r->stream, build_ms, error)) return false;
   e = ensure_launch(r);
   if (e != hipSuccess) {
@@ -1496,6 +1526,17 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
   }
   if (c.transforms) up(s.instance_transform, p->instance_transform, 64ull * I);
   if (c.instance_materials) up(s.instance_material, p->instance_material, 4ull * I);
+  // the instances whose coverage is tested, when their materials changed
+  bool covered_changed = false;
+  if (c.materials || c.instance_materials) {
+    std::vector<uint32_t> covered = covered_flags(p);
+    covered_changed = covered != s.covered_host;
+    if (covered_changed) {
+      s.covered_host = std::move(covered);
+      s.covered_instances = static_cast<uint32_t>(std::count(s.covered_host.begin(), s.covered_host.end(), 1u));
+      up(s.instance_covered, s.covered_host.data(), 4ull * I);
+    }
+  }
   if (c.materials) {
     if (s.material_count != M) {
       dev_free(s.materials);
@@ -1513,9 +1554,17 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
   }
   if (c.refit) {
     const BuildInput in{s.positions, s.triangles, s.geometry_vertex_offset, s.geometry_triangle_offset,
-                        s.instance_geometry, s.instance_transform, s.instance_base, I, sh.tri_count, sh.leaf_size};
+                        s.instance_geometry, s.instance_transform, s.instance_base,
+                        s.covered_instances ? s.instance_covered : nullptr, I, sh.tri_count, sh.leaf_size};
     if (!refit_bvh(in, s, r->stream, build_ms, error)) return false;
     ++r->bvh_refits;
+  } else if (covered_changed) {
+    e = mark_covered(s, sh.tri_count, s.covered_instances ? s.instance_covered : nullptr, r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) {
+      error = std::string("coverage marks: ") + hipGetErrorString(e);
+      return false;
+    }
   }
   if (c.materials) ++r->material_writes;
   if (lights_changed) ++r->light_writes;
@@ -2152,7 +2201,7 @@ int lotus_hip_trace_device(lotus_hip_renderer* r, uint32_t count, const void* ra
   (void)hipSetDevice(r->device);
   Launch L = launch_for(r, stream ? static_cast<hipStream_t>(stream) : r->stream);
   launch_rays(L, r->view, static_cast<const float*>(rays), static_cast<uint32_t*>(hits), count,
-               any_hit ? 1u : 0u);
+               any_hit == 2 ? 2u : any_hit ? 1u : 0u);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "trace launch", e));

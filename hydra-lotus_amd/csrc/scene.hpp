@@ -9,7 +9,8 @@
 //              16-byte loads and four slab tests share every load.
 //   tris     : Tri[tri_count], 48 B, in BVH leaf order: the three world-space
 //              corners, with the flattened triangle id, instance and
-//              primitive in the w lanes.
+//              primitive in the w lanes; the primitive's bit 31 marks a
+//              triangle whose material's coverage is tested.
 // A child word is a node index, or (bit 31) a leaf: first triangle in bits
 // 0-26, count - 1 in bits 27-30. An empty child has an inverted box.
 #pragma once
@@ -51,6 +52,38 @@ struct Ray {
   V3 d;
   float tmax;
 };
+
+// Bit 31 of a Tri's primitive word: the triangle's instance has a material
+// whose coverage (scene.slang:388-406) may be below one, so a ray tests it.
+constexpr uint32_t kCoveredBit = 0x80000000u;
+
+// Coverage policies of a walk. NoCoverage: every triangle is opaque (no
+// code for the test is compiled). A policy with kEnabled answers the
+// coverage of candidate (instance, primitive, u, v) of a covered triangle.
+struct NoCoverage {
+  static constexpr bool kEnabled = false;
+  __device__ __forceinline__ float operator()(uint32_t, uint32_t, float, float) const { return 1.0f; }
+};
+
+// AcceptSurface (scene.slang:408-413) draws a candidate's uniform number
+// from the path's stream in the order the hardware reports candidates, an
+// order no other traversal reproduces. Here the number is a hash of the
+// ray (its origin and direction bits) and the candidate's flattened id: the
+// same law (each candidate accepted with probability equal to its coverage,
+// independently), decided the same way whatever order candidates are met
+// in, and by the oracle's brute-force trace alike. The path's stream is not
+// advanced.
+__device__ __forceinline__ uint32_t coverage_ray_key(V3 o, V3 d) {
+  uint32_t h = pcg_hash(__float_as_uint(d.z));
+  h = pcg_hash(__float_as_uint(d.y) ^ h);
+  h = pcg_hash(__float_as_uint(d.x) ^ h);
+  h = pcg_hash(__float_as_uint(o.z) ^ h);
+  h = pcg_hash(__float_as_uint(o.y) ^ h);
+  return pcg_hash(__float_as_uint(o.x) ^ h);
+}
+__device__ __forceinline__ float coverage_uniform(uint32_t key, uint32_t g) {
+  return static_cast<float>(pcg_hash(key ^ pcg_hash(g)) >> 8u) * (1.0f / 16777216.0f);
+}
 
 struct HitRecord {
   uint32_t instance, primitive, tri;  // tri: index in BVH order
@@ -274,9 +307,20 @@ constexpr uint32_t kWalkDone = 0xFFFFFFFEu;  // no node left (never a node or le
 // One ray's resumable walk of the wide BVH: each step tests one node's four
 // children (nearest-first descent) or one leaf's triangles. Closest hit:
 // ties in t go to the lowest flattened triangle id, so the result does not
-// depend on the order nodes are visited. Any hit (ShadowTransmittance for
-// opaque surfaces): the walk ends at the first triangle in [tmin, tmax].
-struct Walk {
+// depend on the order nodes are visited. Any hit (ShadowTransmittance): the
+// walk ends at the first opaque triangle in [tmin, tmax].
+//
+// Under a coverage policy (scene.slang:388-460), a covered triangle's
+// candidate counts only if accepted: a closest-hit walk accepts it with
+// probability equal to its coverage (decided by coverage_uniform), and an
+// any-hit walk ends at it only at coverage one, otherwise multiplying its
+// transmittance by 1 - coverage. The product is kept in double, where the
+// factors of up to two surfaces multiply exactly and more round far below
+// float's precision, so its float value does not depend on the order
+// candidates are met in (the reference multiplies in float in the
+// hardware's order).
+template <class Cov>
+struct WalkT {
   Ray r;
   RayPre pre;
   V3 inv;
@@ -287,6 +331,9 @@ struct Walk {
   uint32_t steps;
   uint32_t visits;  // wide nodes tested
   uint32_t tests;   // triangles tested
+  Cov cov;
+  uint32_t key;     // the ray's coverage key (closest hit)
+  double trans;     // transmittance so far (any hit)
 
   __device__ __forceinline__ void start(const Ray& ray, bool any_hit) {
     r = ray;
@@ -300,9 +347,20 @@ struct Walk {
     steps = 0;
     visits = 0;
     tests = 0;
+    if constexpr (Cov::kEnabled) {
+      key = any_hit ? 0u : coverage_ray_key(ray.o, ray.d);
+      trans = 1.0;
+    }
   }
   __device__ __forceinline__ bool done() const { return cur == kWalkDone; }
   __device__ __forceinline__ bool hit() const { return best_g != kMiss; }
+  // An any-hit walk's result: 0 at an opaque hit, else the product of the
+  // covered surfaces' transparencies (1 without a coverage policy).
+  __device__ __forceinline__ float transmittance() const {
+    if (hit()) return 0.0f;
+    if constexpr (Cov::kEnabled) return static_cast<float>(trans);
+    return 1.0f;
+  }
 
   // The next entry no farther than the best hit (any entry for any hit).
   template <int S, int W>
@@ -346,6 +404,22 @@ struct Walk {
       ++tests;
       if (intersect_triangle(r, pre, xyz(t4.a), xyz(t4.b), xyz(t4.c), t, u, v)) {
         const uint32_t g = __float_as_uint(t4.a.w);
+        if constexpr (Cov::kEnabled) {
+          const uint32_t pw = __float_as_uint(t4.c.w);
+          if (pw & kCoveredBit) {
+            // only a candidate that could become the closest hit is tested
+            if (!any && !(t < best_t || (t == best_t && g < best_g))) continue;
+            const float c = cov(__float_as_uint(t4.b.w), pw & ~kCoveredBit, u, v);
+            if (any) {
+              if (!(c >= 1.0f)) {
+                if (c > 0.0f) trans *= static_cast<double>(1.0f - c);
+                continue;
+              }
+            } else if (!(c > 0.0f) || (c < 1.0f && !(coverage_uniform(key, g) < c))) {
+              continue;
+            }
+          }
+        }
         if (any) {
           best_g = g;
           best_tri = first + k;
@@ -424,7 +498,7 @@ struct Walk {
     const Tri t4 = tris[best_tri];
     HitRecord h;
     h.instance = __float_as_uint(t4.b.w);
-    h.primitive = __float_as_uint(t4.c.w);
+    h.primitive = __float_as_uint(t4.c.w) & ~kCoveredBit;
     h.tri = best_tri;
     float t = best_t;
     h.u = h.v = 0.0f;
@@ -434,11 +508,15 @@ struct Walk {
   }
 };
 
+using Walk = WalkT<NoCoverage>;
+
 // A walk followed to its end by one lane.
-template <int S, int W>
+template <class Cov = NoCoverage, int S, int W>
 __device__ __forceinline__ bool walk_closest(const WideNode* nodes, const Tri* tris, LdsStack<S, W>& st,
-                                             const Ray& r, HitRecord& h, uint32_t& visits, uint32_t& tests) {
-  Walk w;
+                                             const Ray& r, HitRecord& h, uint32_t& visits, uint32_t& tests,
+                                             const Cov& cov = Cov{}) {
+  WalkT<Cov> w;
+  w.cov = cov;
   w.start(r, false);
   st.clear();
   w.run(nodes, tris, st);
@@ -448,16 +526,19 @@ __device__ __forceinline__ bool walk_closest(const WideNode* nodes, const Tri* t
   h = w.record(tris);
   return true;
 }
-template <int S, int W>
-__device__ __forceinline__ bool walk_occluded(const WideNode* nodes, const Tri* tris, LdsStack<S, W>& st,
-                                              const Ray& r, uint32_t& visits, uint32_t& tests) {
-  Walk w;
+// ShadowTransmittance (scene.slang:445-460) followed to its end by one lane.
+template <class Cov = NoCoverage, int S, int W>
+__device__ __forceinline__ float walk_transmittance(const WideNode* nodes, const Tri* tris, LdsStack<S, W>& st,
+                                                   const Ray& r, uint32_t& visits, uint32_t& tests,
+                                                   const Cov& cov = Cov{}) {
+  WalkT<Cov> w;
+  w.cov = cov;
   w.start(r, true);
   st.clear();
   w.run(nodes, tris, st);
   visits += w.visits;
   tests += w.tests;
-  return w.hit();
+  return w.transmittance();
 }
 
 }  // namespace lotus

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_temporal_surface(SceneView S, F
     const Ray r = camera_ray(F.clip_to_world, fx * F.ndc_from_pixel[0] + F.ndc_from_pixel[2],
                              fy * F.ndc_from_pixel[1] + F.ndc_from_pixel[3]);
     HitRecord h;
-    if (!walk_closest(S.nodes, S.tris, st, r, h, visits, tests)) continue;
+    if (!walk_closest(S.nodes, S.tris, st, r, h, visits, tests, SceneCoverage{&S})) continue;
     const Tri t = S.tris[h.tri];
     const V3 p0 = xyz(t.a), p1 = xyz(t.b), p2 = xyz(t.c);
     const V3 world = (p0 + h.u * (p1 - p0)) + h.v * (p2 - p0);

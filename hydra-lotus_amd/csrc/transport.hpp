@@ -101,7 +101,7 @@ __device__ __forceinline__ Ray start_path(const FrameConst& f, uint32_t x, uint3
 // ---------------------------------------------------- material textures
 // Texture-input slots (scene.slang:59-67), lookup modes (gpu_scene.hpp:56-59)
 // and wrap modes (TextureWrap, material.hpp:43-50).
-constexpr uint32_t kNormalSlot = 1, kBaseColorSlot = 2, kRoughnessSlot = 3, kMetallicSlot = 4, kEmissionSlot = 5,
+constexpr uint32_t kOpacitySlot = 0, kNormalSlot = 1, kBaseColorSlot = 2, kRoughnessSlot = 3, kMetallicSlot = 4, kEmissionSlot = 5,
                    kSpecularColorSlot = 6;
 constexpr uint32_t kPreviewLookupSlots = (1u << kNormalSlot) | (1u << kBaseColorSlot) | (1u << kRoughnessSlot) |
                                          (1u << kMetallicSlot) | (1u << kEmissionSlot) | (1u << kSpecularColorSlot);
@@ -182,6 +182,34 @@ __device__ __forceinline__ void hit_texcoords(const SceneView& S, const uint32_t
 }
 
 __device__ __forceinline__ float channel_of(float4 v, uint32_t c) { return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w; }
+
+// scene.slang:393-406 (Coverage): the probability that a ray accepts a
+// candidate of `instance` at barycentrics (u, v): its material's opacity,
+// constant or looked up (clamped to [0, 1], 0 where the lookup is not
+// finite), or 0 or 1 against a positive threshold.
+__device__ __forceinline__ float surface_coverage(const SceneView& S, uint32_t instance, uint32_t primitive, float u,
+                                                  float v) {
+  const uint32_t* m = S.materials + 36ull * S.instance_material[instance];
+  float opacity = word_float(m, 6);
+  if (const uint32_t* b = material_input(S, m, kOpacitySlot)) {
+    float s, t;
+    hit_texcoords(S, m, instance, primitive, u, v, s, t);
+    const float o = channel_of(texture_lookup(S, b, s, t), b[2]);
+    opacity = isfinite(o) ? saturate(o) : 0.0f;
+  }
+  const float threshold = word_float(m, 7);
+  if (threshold > 0.0f) return opacity >= threshold ? 1.0f : 0.0f;
+  return opacity;
+}
+
+// The coverage policy of walks over a scene with covered triangles.
+struct SceneCoverage {
+  static constexpr bool kEnabled = true;
+  const SceneView* S;
+  __device__ __forceinline__ float operator()(uint32_t instance, uint32_t primitive, float u, float v) const {
+    return surface_coverage(*S, instance, primitive, u, v);
+  }
+};
 
 // scene.slang:572-577 (PreviewEmission) at texture coordinates (s, t).
 __device__ __forceinline__ V3 preview_emission(const SceneView& S, const uint32_t* m, float s, float t) {
@@ -292,7 +320,7 @@ __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t 
   sf.area = 0.5f * length3(e);
   sf.normal = dot3(normal, dir) > 0.0f ? -normal : normal;
   sf.shading_normal = sf.normal;
-  const uint32_t primitive = __float_as_uint(t.c.w);
+  const uint32_t primitive = __float_as_uint(t.c.w) & ~kCoveredBit;
   if (kExtras && S.instance_normals) {
     const uint32_t first = S.instance_normals[instance];
     if (first != kNoCorner)
@@ -396,8 +424,9 @@ struct LightSample {
 };
 
 // scene.slang:632-677 (SampleTriangle): a point uniformly by area on light
-// `light` of the table, (uz, uw) its unit-square coordinates.
-template <bool kExtras = true>
+// `light` of the table, (uz, uw) its unit-square coordinates; kCov compiles
+// in the coverage of covered emitters.
+template <bool kExtras = true, bool kCov = kExtras>
 __device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S, uint32_t light, float uz, float uw,
                                                              V3 origin) {
   LightSample none{V3{0.0f, 0.0f, 1.0f}, 0.0f, V3{0.0f, 0.0f, 0.0f}, 0.0f};
@@ -419,7 +448,11 @@ __device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S,
     hit_texcoords(S, m, chosen.instance, prim, bu, bv, s, tc);
     emission = preview_emission(S, m, s, tc);
   }
-  const V3 radiance = emission * 1.0f;
+  // an emitter's radiance times its coverage, the probability that a ray
+  // reaching it stops there (scene.slang:608-611, 656-657)
+  const V3 radiance =
+      emission * ((kCov && (__float_as_uint(t.c.w) & kCoveredBit)) ? surface_coverage(S, chosen.instance, prim, bu, bv)
+                                                                    : 1.0f);
   const V3 toward = position - origin;
   const float d2 = dot3(toward, toward);
   if (!(area > 0.0f) || !(d2 > 0.0f) || max3(radiance) <= 0.0f) return none;
@@ -675,7 +708,7 @@ __device__ __forceinline__ V3 environment_radiance(const SceneView& S, V3 d, boo
 // scene.slang:894-909 (SampleLight): the alias table's column and threshold
 // choose the light, an emissive triangle or (kAnalytic) an analytic light.
 // Kernels of scenes without analytic lights compile the triangle case alone.
-template <bool kAnalytic = true>
+template <bool kAnalytic = true, bool kCov = kAnalytic>
 __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux, float uy, float uz, float uw,
                                                     V3 origin) {
   const uint32_t count = S.light_count;
@@ -689,19 +722,19 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& S, float ux
     if (chosen.instance & kAnalyticLight)
       return sample_analytic(S.analytic[chosen.instance & ~kAnalyticLight], chosen.pdf, uz, uw, origin);
   }
-  return sample_triangle_light<kAnalytic>(S, light, uz, uw, origin);
+  return sample_triangle_light<kAnalytic, kCov>(S, light, uz, uw, origin);
 }
 
 // scene.slang:939-971 (ReconstructReservoirLight): a light-table entry and
 // its unit-square coordinates as a sample; a sphere's candidate is a point
 // uniformly by area over the whole sphere (its far side contributing 0).
-template <bool kAnalytic = true>
+template <bool kAnalytic = true, bool kCov = kAnalytic>
 __device__ __forceinline__ LightSample reconstruct_reservoir_light(const SceneView& S, uint32_t index, float u,
                                                                    float v, V3 origin) {
-  if constexpr (!kAnalytic) return sample_triangle_light<false>(S, index, u, v, origin);
+  if constexpr (!kAnalytic) return sample_triangle_light<false, false>(S, index, u, v, origin);
   const LightRecord chosen = S.lights[index];
   if (chosen.instance == kEnvironmentLight) return sample_environment(S, chosen.pdf, u, v);
-  if ((chosen.instance & kAnalyticLight) == 0) return sample_triangle_light(S, index, u, v, origin);
+  if ((chosen.instance & kAnalyticLight) == 0) return sample_triangle_light<true, kCov>(S, index, u, v, origin);
   const AnalyticRecord& L = S.analytic[chosen.instance & ~kAnalyticLight];
   if (L.shape != kSphereLight) return sample_analytic(L, chosen.pdf, u, v, origin);
   const LightSample none{V3{0.0f, 0.0f, 1.0f}, 0.0f, V3{0.0f, 0.0f, 0.0f}, 0.0f, false};
@@ -943,6 +976,22 @@ struct Scatter {
   V3 delta;            // throughput * contribution, added when unoccluded
 };
 
+// A shadow ray's contribution in factors, for a ray through covered
+// surfaces, whose transmittance multiplies inside the product.
+struct ShadowFactors {
+  V3 fl;               // f * radiance
+  float weight, pdf;   // MIS weight, the light's pdf
+  V3 thr;              // the path's throughput at the vertex
+};
+
+// ShadowedLight's contribution (transport.slang:109-114) once the shadow
+// ray's transmittance is known, times the path's throughput: f * radiance *
+// (transmittance * weight / pdf), zero where not finite.
+__device__ __forceinline__ V3 shadowed_delta(V3 fl, float weight, float pdf, V3 throughput, float transmittance) {
+  const V3 c = fl * (transmittance * weight / pdf);
+  return throughput * (finite3(c) ? c : V3{0.0f, 0.0f, 0.0f});
+}
+
 // transport.slang:164-170 with EmissionWeight (:150-158): the surface's emission, MIS-weighted against
 // light sampling unless the path started here or arrived by a delta or
 // transmitted direction, added to the path's radiance.
@@ -1037,7 +1086,7 @@ __device__ __forceinline__ V3 reuse_contribution(const SceneView& S, const Reuse
 // restir.slang:235-254 (InitialDirectLight): the first hit's direct light
 // from its reservoir record, weighted against BSDF sampling by the power
 // heuristic of the light table's density.
-template <bool kAnalytic>
+template <bool kAnalytic, bool kCov = kAnalytic>
 __device__ __forceinline__ V3 initial_direct_light(const SceneView& S, const float4* rec, const Surface& sf,
                                                    const V3& ft, const V3& fb, V3 wo) {
   const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
@@ -1045,7 +1094,7 @@ __device__ __forceinline__ V3 initial_direct_light(const SceneView& S, const flo
   if ((__float_as_uint(r1.w) & 2u) != 0u || !(r0.y > 0.0f) || light >= S.light_count)
     return V3{0.0f, 0.0f, 0.0f};
   const V3 origin = offset_ray_origin(sf.position, sf.normal);
-  const LightSample ls = reconstruct_reservoir_light<kAnalytic>(S, light, r1.x, r1.y, origin);
+  const LightSample ls = reconstruct_reservoir_light<kAnalytic, kCov>(S, light, r1.x, r1.y, origin);
   const V3 n = sf.shading_normal;
   const V3 wi = V3{dot3(ls.direction, ft), dot3(ls.direction, fb), dot3(ls.direction, n)};
   const float bsdf = wi.z > 0.0f ? bsdf_pdf(sf.bsdf, wo, wi) : 0.0f;
@@ -1071,11 +1120,13 @@ __device__ __forceinline__ void escape_terms(const SceneView& S, bool sample_lig
 // transport.slang:164-230 (ScatterPath) with DirectLight (:123-135) and
 // ShadowedLight (:82-115) up to the shadow ray, which the caller traces.
 // `keep`, when given, receives the reconstructed surface; `slot` indexes the
-// ReSTIR record of the first hit.
-template <bool kAnalytic>
+// ReSTIR record of the first hit; `factors`, when given, receives the shadow
+// ray's contribution in factors.
+template <bool kAnalytic, bool kCov = kAnalytic>  // kCov: covered emitters' coverage compiled in
 __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameConst& F, PathRegs& p,
                                                 uint32_t tri, uint32_t instance, uint32_t primitive, float u,
-                                                float v, Surface* keep = nullptr, uint32_t slot = 0) {
+                                                float v, Surface* keep = nullptr, uint32_t slot = 0,
+                                                ShadowFactors* factors = nullptr) {
   Scatter out;
   out.continues = false;
   out.shadow = false;
@@ -1104,7 +1155,7 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
   const V3 md = -p.d;
   const V3 wo = V3{dot3(md, ft), dot3(md, fb), dot3(md, n)};
   if (F.light_sampling == kLightRestir && p.depth == 0) {
-    p.radiance = p.radiance + p.throughput * initial_direct_light<kAnalytic>(S, F.restir + 4ull * slot, sf, ft, fb, wo);
+    p.radiance = p.radiance + p.throughput * initial_direct_light<kAnalytic, kCov>(S, F.restir + 4ull * slot, sf, ft, fb, wo);
   } else if ((sample_lights && S.light_count != 0) || (kAnalytic && !sample_lights && S.parallel != 0)) {
     // next event estimation (transport.slang:123-135), or under BSDF sampling
     // the parallel light no sampled direction finds (:139-146; one such
@@ -1114,7 +1165,7 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
     if (sample_lights) {
       const float ux = next_random(p.rng), uy = next_random(p.rng), uz = next_random(p.rng),
                   uw = next_random(p.rng);
-      ls = sample_light<kAnalytic>(S, ux, uy, uz, uw, origin);
+      ls = sample_light<kAnalytic, kCov>(S, ux, uy, uz, uw, origin);
     } else if constexpr (kAnalytic) {
       const AnalyticRecord& L = S.analytic[S.parallel - 1];
       ls = LightSample{V3{L.axis_u[0], L.axis_u[1], L.axis_u[2]}, kFloatMax,
@@ -1126,10 +1177,11 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
         const V3 f = bsdf_evaluate(sf.bsdf, wo, wi);
         if (!(max3(f) <= 0.0f)) {
           const float weight = ls.delta ? 1.0f : power_heuristic(ls.pdf, bsdf_pdf(sf.bsdf, wo, wi));
-          const V3 c = f * ls.radiance * (1.0f * weight / ls.pdf);
           out.shadow = true;
           out.shadow_ray = Ray{origin, 0.0f, ls.direction, ls.distance};
-          out.delta = p.throughput * (finite3(c) ? c : V3{0.0f, 0.0f, 0.0f});
+          const V3 fl = f * ls.radiance;
+          if (factors) *factors = ShadowFactors{fl, weight, ls.pdf, p.throughput};
+          out.delta = shadowed_delta(fl, weight, ls.pdf, p.throughput, 1.0f);
         }
       }
     }

@@ -590,6 +590,10 @@ class Renderer:
             raise LotusError(s, buf.value.decode())
         return hits
 
+    def transmittance(self, rays: np.ndarray) -> np.ndarray:
+        """ShadowTransmittance per ray (any_hit 2)."""
+        return self.trace(rays, any_hit=2).view(np.float32)
+
     def trace_device(self, n: int, rays_ptr: int, hits_ptr: int, any_hit: bool, stream_ptr: int = 0):
         buf = C.create_string_buffer(512)
         s = lib().lotus_hip_trace_device(self.handle, n, C.c_void_p(rays_ptr), C.c_void_p(hits_ptr),

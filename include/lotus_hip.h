@@ -111,9 +111,8 @@ typedef struct lotus_hip_scene_plan {
    * material record's words 1-3 give its first binding and its textured
    * slots), and per-corner texture coordinates (2 floats per triangle
    * corner) with each geometry's first corner, 0xFFFFFFFF for a geometry
-   * without them. Normal (tangent space), base colour, roughness, metallic,
-   * emission and specular colour take lookups; opacity lookups are
-   * rejected. */
+   * without them. Opacity, normal (tangent space), base colour, roughness,
+   * metallic, emission and specular colour take lookups. */
   uint32_t texture_count;
   const lotus_hip_texture* textures;
   uint32_t binding_count;
@@ -313,8 +312,14 @@ int lotus_hip_render(lotus_hip_renderer* renderer,
  * caller's rays: origin xyz, tmin, direction xyz, tmax (8 floats per ray).
  * Closest hits are instance, primitive (LOTUS_HIP_MISS on a miss),
  * barycentric weights of corners 1 and 2, t (5 words); any-hit results
- * are 1 (blocked) or 0. Host pointers. Ties in t resolve to the lowest
- * (instance, primitive). */
+ * (any_hit 1) are 1 (blocked: transmittance 0) or 0, and with any_hit 2
+ * each ray's transmittance as a float's bits (the product of 1 - coverage
+ * over the covered surfaces it crosses, 0 at an opaque one). Host
+ * pointers. Ties in t resolve to the lowest flattened (instance,
+ * primitive). A covered surface's candidate counts for a closest hit
+ * with probability equal to its coverage, decided by a hash of the ray
+ * and the candidate (the reference draws from the path's random stream in
+ * the hardware's candidate order, which no other traversal reproduces). */
 int lotus_hip_trace(lotus_hip_renderer* renderer, uint32_t count,
                     const float* rays, uint32_t* hits, int any_hit,
                     char* detail, size_t detail_size);

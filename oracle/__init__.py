@@ -104,6 +104,7 @@ def lib():
         L.oracle_light_table.argtypes = [C.c_void_p, _u32p, _u32p]
         L.oracle_trace_closest.argtypes = [C.c_void_p, C.c_uint32, _f32p, _u32p, C.c_int, C.c_int]
         L.oracle_trace_occluded.argtypes = [C.c_void_p, C.c_uint32, _f32p, _u32p, C.c_int, C.c_int]
+        L.oracle_trace_transmittance.argtypes = [C.c_void_p, C.c_uint32, _f32p, _f32p, C.c_int, C.c_int]
         L.oracle_render.argtypes = [C.c_void_p, C.POINTER(Frame), _f64p, _u32p, _u64p, C.c_int]
         L.oracle_trace_path.restype = C.c_uint32
         L.oracle_trace_path.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_uint32, C.c_uint32,
@@ -259,6 +260,14 @@ class OracleScene:
         out = np.zeros(rays.shape[0], np.uint32)
         lib().oracle_trace_occluded(self.handle, rays.shape[0], ptr(rays, C.c_float),
                                     ptr(out, C.c_uint32), int(brute_force), threads)
+        return out
+
+    def transmittance(self, rays: np.ndarray, brute_force=False, threads=8) -> np.ndarray:
+        """ShadowTransmittance per ray (scene.slang:445-460)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.float32)
+        lib().oracle_trace_transmittance(self.handle, rays.shape[0], ptr(rays, C.c_float),
+                                         ptr(out, C.c_float), int(brute_force), threads)
         return out
 
     def render(self, frame: Frame, threads=8):

@@ -498,6 +498,7 @@ struct Material {
   F3 tangent_normal;
   uint32_t lookup_base, textured;  // the binding table's first entry, the slots with lookups
   float fallback_st[2];            // texture coordinates without a set
+  float opacity, opacity_threshold;
 };
 inline float WordFloat(const uint32_t* w, int i) {
   float f;
@@ -520,7 +521,16 @@ Material DecodeMaterial(const uint32_t* w) {
   m.textured = w[2];
   m.fallback_st[0] = WordFloat(w, 4);
   m.fallback_st[1] = WordFloat(w, 5);
+  m.opacity = WordFloat(w, 6);
+  m.opacity_threshold = WordFloat(w, 7);
   return m;
+}
+// Whether a ray tests the material's coverage: an opacity lookup, or a
+// constant coverage below one.
+bool Covered(const Material& m) {
+  const float constant =
+      m.opacity_threshold > 0.0f ? (m.opacity >= m.opacity_threshold ? 1.0f : 0.0f) : m.opacity;
+  return (m.textured & 1u) != 0 || !(constant >= 1.0f);
 }
 constexpr uint32_t kSpecularWorkflow = 1u;
 constexpr uint32_t kThinWalled = 2u;
@@ -600,6 +610,7 @@ struct Scene {
   std::vector<F3> v0, v1, v2;           // world corners per flattened triangle
   std::vector<uint32_t> tri_instance;   // per flattened triangle
   std::vector<uint32_t> tri_primitive;
+  std::vector<uint8_t> tri_covered;     // per flattened triangle: its coverage is tested
   std::vector<uint32_t> instance_base;  // first flattened triangle
   std::vector<uint32_t> instance_material;
   std::vector<float> instance_transform;  // 16 per instance
@@ -681,6 +692,38 @@ inline bool IntersectBox(const Ray& r, const float* inv, const float* lo,
   }
   tnear = t0;
   return true;
+}
+
+// scene.slang:393-406 (Coverage), defined with the texture lookups below.
+float SurfaceCoverage(const Scene& s, uint32_t instance, uint32_t primitive, float u, float v);
+
+// AcceptSurface (scene.slang:408-413) with the candidate's uniform number a
+// hash of the ray and the flattened triangle id (the reference draws it
+// from the path's stream in its hardware's candidate order, which no other
+// traversal reproduces): the same acceptance law, decided alike whatever
+// order candidates are met in.
+uint32_t CoverageRayKey(const Ray& r) {
+  auto bits = [](float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+  };
+  uint32_t h = PcgHash(bits(r.d.z));
+  h = PcgHash(bits(r.d.y) ^ h);
+  h = PcgHash(bits(r.d.x) ^ h);
+  h = PcgHash(bits(r.o.z) ^ h);
+  h = PcgHash(bits(r.o.y) ^ h);
+  return PcgHash(bits(r.o.x) ^ h);
+}
+float CoverageUniform(uint32_t key, uint32_t g) {
+  return static_cast<float>(PcgHash(key ^ PcgHash(g)) >> 8) * (1.0f / 16777216.0f);
+}
+bool AcceptSurface(const Scene& s, uint32_t key, uint32_t g, float u, float v) {
+  if (!s.tri_covered[g]) return true;
+  const float c = SurfaceCoverage(s, s.tri_instance[g], s.tri_primitive[g], u, v);
+  if (!(c > 0.0f)) return false;
+  if (c >= 1.0f) return true;
+  return CoverageUniform(key, g) < c;
 }
 
 inline bool Closer(float t, uint32_t g, const Hit& best) {
@@ -781,10 +824,11 @@ bool TraceClosest(const Scene& s, const Ray& r, Hit& best, bool brute) {
   best.primitive = kMiss;
   best.u = best.v = 0.0f;
   const RayPre p = Precompute(r);
+  const uint32_t key = CoverageRayKey(r);
   auto test = [&](uint32_t g) {
     float t, u, v;
     if (IntersectTriangle(r, p, s.v0[g], s.v1[g], s.v2[g], t, u, v) &&
-        Closer(t, g, best)) {
+        Closer(t, g, best) && AcceptSurface(s, key, g, u, v)) {
       best.t = t;
       best.global = g;
       best.u = u;
@@ -818,15 +862,30 @@ bool TraceClosest(const Scene& s, const Ray& r, Hit& best, bool brute) {
   return true;
 }
 
-bool Occluded(const Scene& s, const Ray& r, bool brute) {
+// scene.slang:445-460 (ShadowTransmittance): 0 when an opaque surface lies
+// on the segment, else the product of 1 - coverage over the covered
+// surfaces it crosses, kept in double so its float value does not depend on
+// the order the candidates are met in (the reference multiplies in float in
+// its hardware's order).
+float ShadowTransmittance(const Scene& s, const Ray& r, bool brute) {
   const RayPre p = Precompute(r);
-  float t, u, v;
+  double transmittance = 1.0;
+  // false: an opaque candidate ends the walk
+  auto candidate = [&](uint32_t g) {
+    float t, u, v;
+    if (!IntersectTriangle(r, p, s.v0[g], s.v1[g], s.v2[g], t, u, v)) return true;
+    if (!s.tri_covered[g]) return false;
+    const float c = SurfaceCoverage(s, s.tri_instance[g], s.tri_primitive[g], u, v);
+    if (c >= 1.0f) return false;
+    if (c > 0.0f) transmittance *= static_cast<double>(1.0f - c);
+    return true;
+  };
   if (brute) {
     for (uint32_t g = 0; g < s.v0.size(); ++g)
-      if (IntersectTriangle(r, p, s.v0[g], s.v1[g], s.v2[g], t, u, v)) return true;
-    return false;
+      if (!candidate(g)) return 0.0f;
+    return static_cast<float>(transmittance);
   }
-  if (s.nodes.empty()) return false;
+  if (s.nodes.empty()) return 1.0f;
   float inv[3];
   InverseDirection(r, inv);
   uint32_t stack[128];
@@ -837,17 +896,16 @@ bool Occluded(const Scene& s, const Ray& r, bool brute) {
     float tn;
     if (!IntersectBox(r, inv, node.lo, node.hi, r.tmax, tn)) continue;
     if (node.count > 0) {
-      for (uint32_t k = 0; k < node.count; ++k) {
-        const uint32_t g = s.order[node.left_or_first + k];
-        if (IntersectTriangle(r, p, s.v0[g], s.v1[g], s.v2[g], t, u, v)) return true;
-      }
+      for (uint32_t k = 0; k < node.count; ++k)
+        if (!candidate(s.order[node.left_or_first + k])) return 0.0f;
     } else {
       stack[top++] = node.right;
       stack[top++] = node.left_or_first;
     }
   }
-  return false;
+  return static_cast<float>(transmittance);
 }
+bool Occluded(const Scene& s, const Ray& r, bool brute) { return !(ShadowTransmittance(s, r, brute) > 0.0f); }
 
 // ------------------------------------------------------------ light table --
 // gpu_scene.cpp:144-203, 512-609 (emissive triangles only).
@@ -1041,6 +1099,21 @@ void HitTexcoords(const Scene& s, const Material& m, uint32_t instance, uint32_t
 }
 
 // scene.slang:572-577 (PreviewEmission).
+float SurfaceCoverage(const Scene& s, uint32_t instance, uint32_t primitive, float u, float v) {
+  const Material& m = s.materials[s.instance_material[instance]];
+  float opacity = m.opacity;
+  if (const uint32_t* b = MaterialInput(s.tex, m, 0)) {
+    float st_s, st_t, o[4];
+    HitTexcoords(s, m, instance, primitive, u, v, st_s, st_t);
+    Lookup(s.tex, b, st_s, st_t, o);
+    const float value = o[b[2] & 3u];
+    // Lookup arithmetic can overflow even with finite float texels.
+    opacity = std::isfinite(value) ? std::min(std::max(value, 0.0f), 1.0f) : 0.0f;
+  }
+  if (m.opacity_threshold > 0.0f) return opacity >= m.opacity_threshold ? 1.0f : 0.0f;
+  return opacity;
+}
+
 F3 PreviewEmission(const Scene& s, const Material& m, float st_s, float st_t) {
   const uint32_t* b = MaterialInput(s.tex, m, 5);
   if (!b) return m.emission;
@@ -1234,7 +1307,8 @@ LightSample SampleTriangleLight(const Scene& s, uint32_t light, float uz, float 
     HitTexcoords(s, m, chosen.instance, prim, bu, bv, st_s, st_t);
     emission = PreviewEmission(s, m, st_s, st_t);
   }
-  const F3 radiance = emission * 1.0f;
+  // an emitter's radiance times its coverage (scene.slang:608-611, 656-657)
+  const F3 radiance = emission * (s.tri_covered[g] ? SurfaceCoverage(s, chosen.instance, prim, bu, bv) : 1.0f);
   const F3 toward = position - origin;
   const float d2 = Dot(toward, toward);
   if (!(area > 0.0f) || !(d2 > 0.0f) || MaxComponent(radiance) <= 0.0f) return none;
@@ -1747,7 +1821,7 @@ F3 ShadowedLight(const Scene& s, const Surface& sf, const Frame& basis, F3 wo, F
   if (MaxComponent(f) <= 0.0f) return F3{0, 0, 0};
   Ray shadow{origin, 0.0f, ls.direction, ls.distance};
   ++k.shadow;
-  const float transmittance = Occluded(s, shadow, brute) ? 0.0f : 1.0f;
+  const float transmittance = ShadowTransmittance(s, shadow, brute);
   if (transmittance <= 0.0f) return F3{0, 0, 0};
   const float weight = ls.delta ? 1.0f : PowerHeuristic(ls.pdf, BsdfPdf(sf.bsdf, wo, wi));
   const F3 c = f * ls.radiance * (transmittance * weight / ls.pdf);
@@ -1815,7 +1889,7 @@ RestirPixel InitialReservoir(const Scene& s, const oracle_frame& fr, uint32_t sl
   if ((px.flags & 2) == 0 && px.r.weight_sum > 0.0f) {
     const LightSample ls = ReconstructReservoirLight(s, px.r.sample, px.u, px.v, origin);
     ++k.shadow;
-    const float visibility = Occluded(s, Ray{origin, 0.0f, ls.direction, ls.distance}, brute) ? 0.0f : 1.0f;
+    const float visibility = ShadowTransmittance(s, Ray{origin, 0.0f, ls.direction, ls.distance}, brute);
     const float normalization = ReservoirNormalization(px.r);
     const F3 estimate = CandidateContribution(sf, basis, wo, ls, s.lights[px.r.sample].pdf) *
                         (visibility * normalization);
@@ -2258,6 +2332,7 @@ oracle_scene* oracle_scene_create(const oracle_scene_desc* d, char* error,
       s.v2.push_back(w[2]);
       s.tri_instance.push_back(i);
       s.tri_primitive.push_back(k);
+      s.tri_covered.push_back(Covered(s.materials[d->instance_material[i]]) ? 1 : 0);
     }
   }
   // texture lookups: texels decoded to linear RGBA as the samplers decode
@@ -2466,6 +2541,13 @@ void oracle_trace_occluded(const oracle_scene* scene, uint32_t count, const floa
                            uint32_t* occluded, int brute_force, int threads) {
   ParallelFor(count, threads, [&](uint32_t i, int) {
     occluded[i] = Occluded(scene->s, LoadRay(rays + 8 * size_t(i)), brute_force != 0) ? 1u : 0u;
+  });
+}
+
+void oracle_trace_transmittance(const oracle_scene* scene, uint32_t count, const float* rays,
+                                float* transmittance, int brute_force, int threads) {
+  ParallelFor(count, threads, [&](uint32_t i, int) {
+    transmittance[i] = ShadowTransmittance(scene->s, LoadRay(rays + 8 * size_t(i)), brute_force != 0);
   });
 }
 
@@ -2684,7 +2766,7 @@ float HistoryTarget(const Scene& s, const ReuseRecord& source, const ReuseSurfac
 float ReceiverVisibility(const Scene& s, const RestirPixel& selected, const RestirPixel& current, F3 origin,
                          const LightSample& ls) {
   if (SameSample(selected, current)) return current.visibility;
-  return Occluded(s, Ray{origin, 0.0f, ls.direction, ls.distance}, false) ? 0.0f : 1.0f;
+  return ShadowTransmittance(s, Ray{origin, 0.0f, ls.direction, ls.distance}, false);
 }
 
 // restir.slang:372-385, 475-485: the selection's estimate at the receiver.

@@ -102,6 +102,10 @@ void oracle_trace_closest(const oracle_scene* scene, uint32_t count,
 void oracle_trace_occluded(const oracle_scene* scene, uint32_t count,
                            const float* rays, uint32_t* occluded,
                            int brute_force, int threads);
+/* ShadowTransmittance per ray (scene.slang:445-460). */
+void oracle_trace_transmittance(const oracle_scene* scene, uint32_t count,
+                                const float* rays, float* transmittance,
+                                int brute_force, int threads);
 
 typedef struct oracle_frame {
   float clip_to_world[16];  /* column-major */

@@ -129,10 +129,17 @@ def test_texture_edits_and_rejected_slots(renderer):
     ev = renderer.apply(brighter)
     assert ev.update in (hip.SCENE_TABLES, hip.SCENE_BUILD) and ev.bvh_refits == 0
     assert np.array_equal(renderer.light_table()[1], oracle.OracleScene(brighter).light_table()[1])
-    # normal and opacity lookups are not supported: the plan is refused
+    # a lookup in a slot the preview surface does not read is refused
     bad = sc.copy()
     rec = np.array(bad.materials[1], np.uint32)
     rec[2] |= 1 << 7
     bad.set_material(1, rec)
     with pytest.raises(hip.LotusError, match="lookups are supported"):
+        renderer.apply(bad)
+    # opacity and its threshold lie in [0, 1] (ValidGeometry, gpu_scene.cpp:821-831)
+    bad = sc.copy()
+    rec = np.array(bad.materials[1], np.uint32)
+    rec.view(np.float32)[6] = 1.5
+    bad.set_material(1, rec)
+    with pytest.raises(hip.LotusError, match="opacity and its threshold"):
         renderer.apply(bad)

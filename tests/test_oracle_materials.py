@@ -10,7 +10,10 @@ fails it at 256. These pin the preview closure's transmission, interface
 and thin-walled lobes (material-transmission-v1), the specular workflow and
 metallic blends (material-basics-v1), authored shading normals, normal maps
 and tangent frames (material-normals-v1), texture lookups
-(material-textures-v1) and emission maps (material-emission-v1)."""
+(material-textures-v1), emission maps (material-emission-v1), and coverage
+(scene.slang:388-460: fractional, thresholded and looked-up opacity, covered
+emitters and shadow transmittance; material-alpha-v1, with the primvar,
+subset and transform fixtures that also carry covered cards)."""
 import numpy as np
 import pytest
 
@@ -20,7 +23,8 @@ from oracle import desq5
 import material_fixtures as mf
 
 FIXTURES = ["material-basics-v1", "material-transmission-v1", "material-textures-v1", "material-normals-v1",
-            "material-emission-v1"]
+            "material-emission-v1", "material-alpha-v1", "material-primvars-v1", "material-subsets-v1",
+            "material-transforms-v1"]
 
 
 def card_pixels(sc, size, centre, half):
