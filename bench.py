@@ -141,7 +141,7 @@ def restir_svgf_run(r, width, height, frames, depth=8, candidates=32, history=20
     and, at the last camera, relMSE of the output and of a plain NEE+MIS
     frame against a 512-spp plain reference."""
     sc = scenes.cornell_many_lights(camera_aspect=width / height)
-    r.apply(sc)
+    r.apply(sc, builder=BUILDER)
     tot = {"frame_ms": 0.0, "restir_ms": 0.0, "trace_ms": 0.0, "shade_ms": 0.0, "camera_ms": 0.0}
     rays = 0
     out = None
@@ -171,6 +171,9 @@ def restir_svgf_run(r, width, height, frames, depth=8, candidates=32, history=20
             "reference_spp": 512}
 
 
+BUILDER = hip.BVH_SAH  # the BVH topology every workload's scene is built with (--builder)
+
+
 def scene_update_run(r, scene, width, height, depth, steps, frames=4):
     """The heightfield animated (scenes.heightfield_wave): every step's plan
     keeps each mesh's vertex count and indices, so the renderer re-places the
@@ -183,7 +186,7 @@ def scene_update_run(r, scene, width, height, depth, steps, frames=4):
     moved = scene
     for k in range(steps):
         moved = scenes.heightfield_wave(scene, 0.35 * (k + 1))
-        before = r.apply(moved)
+        before = r.apply(moved, builder=BUILDER)
         if before.update != hip.SCENE_REFIT:
             raise RuntimeError(f"animated step {k} was applied as update {before.update}, not as a refit")
         refit_ms.append(before.build_gpu_ms)
@@ -201,14 +204,17 @@ def scene_update_run(r, scene, width, height, depth, steps, frames=4):
     refit_frame = frame_ms(r)
     fresh = hip.Renderer(0)
     try:
-        ev = fresh.apply(moved)
+        ev = fresh.apply(moved, builder=BUILDER)
         built_frame = frame_ms(fresh)
+        # the GPU build (Morton order), the fast one beside the host SAH topology
+        gpu_build = fresh.apply(moved, builder=hip.BVH_LBVH).build_gpu_ms
     finally:
         fresh.close()
-    r.apply(scene)  # back to the still heightfield (a refit)
+    r.apply(scene, builder=BUILDER)  # back to the still heightfield (a refit)
     return {"workload": f"heightfield {ev.triangle_count} tris, {steps} animated steps "
                         f"(heights + 0.05 sin(4x + 3z + t)), {width}x{height}, depth {depth}",
-            "build_gpu_ms": round(ev.build_gpu_ms, 3),
+            "build_ms": round(ev.build_gpu_ms, 3),
+            "build_gpu_ms_lbvh": round(gpu_build, 3),
             "refit_gpu_ms": round(float(np.mean(refit_ms)), 3),
             "refit_upload_wall_ms": round(float(np.mean(upload_ms)), 2),
             "refit_uploaded_mb": round((uploaded[-1] - uploaded[0]) / max(len(uploaded) - 1, 1) / 1e6, 1),
@@ -231,7 +237,7 @@ def lights_run(r, scene, width, height, depth, frames):
     sun[8:11] = (0.3, 1.0, 0.4)  # the frame's Z: towards the sun
     lit.add_light(scenes.analytic_light(scenes.LIGHT_DISTANT, sun, radiance=(3.0, 2.8, 2.5), angle=0.53,
                                         normalize=True))
-    r.apply(lit)
+    r.apply(lit, builder=BUILDER)
     kw = dict(frame_count=1, max_bounces=depth, light_sampling=hip.LIGHT_MIS, want_color=False)
     r.render(lit.world_to_clip, width, height, **kw)
     tot = {"rays": 0, "frame_ms": 0.0, "shade_ms": 0.0, "trace_ms": 0.0, "tail_ms": 0.0, "camera_ms": 0.0}
@@ -248,7 +254,7 @@ def lights_run(r, scene, width, height, depth, frames):
     # the same heightfield under a 1024 x 512 sky image instead of the sun
     sky = scene.copy()
     sky.environment_map = scenes.sky_image(1024, 512, sun=(0.3, 0.3), sun_radiance=200.0)
-    r.apply(sky)
+    r.apply(sky, builder=BUILDER)
     r.render(sky.world_to_clip, width, height, **kw)
     rays, ms = 0, 0.0
     for _ in range(frames):
@@ -274,7 +280,7 @@ def lights_run(r, scene, width, height, depth, frames):
         pos, tri = tex.geometries[g]
         tex.set_texcoords(g, pos[tri.reshape(-1)][:, [0, 2]] * 0.5 + 0.5)
     tex._packed = None
-    r.apply(tex)
+    r.apply(tex, builder=BUILDER)
     r.render(tex.world_to_clip, width, height, **kw)
     rays, ms = 0, 0.0
     for _ in range(frames):
@@ -284,7 +290,7 @@ def lights_run(r, scene, width, height, depth, frames):
     result["textured"] = {"workload": "the heightfield with base colour and roughness looked up in a 1024x1024 "
                                       "sRGB texture (per-corner coordinates), NEE+MIS",
                           "mrays_per_s_gpu": round(rays / ms / 1e3, 1), "gpu_ms_per_frame": round(ms / frames, 3)}
-    r.apply(scene)  # back to the untextured scene (a rebuild)
+    r.apply(scene, builder=BUILDER)  # back to the untextured scene (a rebuild)
     return result
 
 
@@ -335,7 +341,7 @@ def batch_run(views, frames):
     t_build = time.perf_counter()
     for v in views:
         r = hip.Renderer(0)
-        r.apply(scenes.icosphere_rig(v["scene_triangles"], v["scene_seed"]))
+        r.apply(scenes.icosphere_rig(v["scene_triangles"], v["scene_seed"]), builder=BUILDER)
         rs.append(r)
     build_s = time.perf_counter() - t_build
     kws = [dict(world_to_clip=v["world_to_clip"], width=v["size"], height=v["size"], max_bounces=v["depth"],
@@ -379,6 +385,8 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=6)
+    ap.add_argument("--builder", choices=("lbvh", "sah"), default="sah",
+                    help="BVH topology: Morton order on the GPU, or binned SAH on the host")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kind-frames", type=int, default=16,
@@ -416,9 +424,11 @@ def main() -> int:
     log(f"building heightfield {args.cells}^2")
     scene = scenes.heightfield(cells=args.cells)
     r = hip.Renderer(0)
-    ev = r.apply(scene)
+    global BUILDER
+    BUILDER = hip.BVH_SAH if args.builder == "sah" else hip.BVH_LBVH
+    ev = r.apply(scene, builder=BUILDER)
     log(f"scene: {ev.triangle_count} tris, {ev.node_count} nodes, depth {ev.bvh_depth}, "
-        f"build {ev.build_gpu_ms:.2f} ms")
+        f"build {ev.build_gpu_ms:.2f} ms ({args.builder})")
     w2c = scene.world_to_clip
     kw = dict(frame_count=1, max_bounces=args.depth, light_sampling=hip.LIGHT_MIS, want_color=False)
     for _ in range(args.warmup):
@@ -522,7 +532,8 @@ def main() -> int:
         "config": {"workload": f"wavefront PT, {args.cells}x{args.cells} heightfield "
                                f"({ev.triangle_count} tris), {args.width}x{args.height}, 1 spp/frame, "
                                f"depth {args.depth}, NEE+MIS",
-                   "frames": args.steps, "bvh": f"4-wide, {ev.node_count} nodes, depth {ev.bvh_depth}"},
+                   "frames": args.steps, "bvh": f"4-wide, {ev.node_count} nodes, depth {ev.bvh_depth}, "
+                                                f"{args.builder} topology, build {ev.build_gpu_ms:.1f} ms"},
         "mrays_by_kind": {k: (round(v, 1) if v is not None else None) for k, v in by_kind.items()},
         "rays_per_frame": {"camera": tot["camera_rays"] / args.steps,
                            "extension": tot["extension_rays"] / args.steps,
@@ -546,7 +557,7 @@ def main() -> int:
                                                  args.race_reference_spp, cache_cfg)}
         log("equal-relMSE race: Cornell box ...")
         box = scenes.cornell_lambert()
-        r.apply(box)
+        r.apply(box, builder=BUILDER)
         race["cornell_1024"] = relmse_race(r, box, 1024, 1024, 8, args.race_frames, args.race_reference_spp,
                                            cache_cfg)
         line["equal_relmse"] = race

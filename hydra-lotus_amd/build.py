@@ -19,12 +19,12 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblotus_hip.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["bvh_build.hip", "kernels.hip", "nrc.hip", "temporal.hip", "products.hip", "lotus_hip.hip"]
+SOURCES = ["bvh_build.hip", "sah_build.hip", "kernels.hip", "nrc.hip", "temporal.hip", "products.hip", "lotus_hip.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-I" + os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-result"]
+         "-I" + os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-result", "-pthread"]
 
 
 def _compile(src: str) -> str:
@@ -41,7 +41,7 @@ def build() -> str:
     with concurrent.futures.ThreadPoolExecutor(len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    r = subprocess.run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs],
+    r = subprocess.run([HIPCC, "-shared", "-pthread", f"--offload-arch={ARCH}", "-o", tmp, *objs],
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

@@ -11,6 +11,9 @@
 //      device-wide fences);
 //   4. top-down collapse into 4-wide nodes, opening the child of largest
 //      surface area first, leaves of at most `leaf_size` triangles.
+// With LOTUS_HIP_BVH_SAH, steps 2 and 3 are replaced by a binned-SAH
+// topology built on the host (sah_build.hip); the boxes and the collapse
+// stay here.
 #include "internal.hpp"
 
 #include <algorithm>
@@ -684,7 +687,7 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   hipError_t e = hipMemcpyAsync(bounds, init, sizeof init, hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) {
     k_flatten<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(in, world, centroid, bounds);
-    k_morton<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(centroid, bounds, n, keys, vals);
+    if (in.builder != LOTUS_HIP_BVH_SAH) k_morton<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(centroid, bounds, n, keys, vals);
     e = hipGetLastError();
   }
   if (e != hipSuccess) {
@@ -692,7 +695,27 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
     release();
     return false;
   }
-  if (!radix_sort_pairs(keys, vals, n, 30, stream, error)) {
+  const bool sah = in.builder == LOTUS_HIP_BVH_SAH && n > 1;
+  if (sah) {
+    // the topology on the host from the world triangles: leaf order, children, ranges
+    std::vector<Tri> host(n);
+    std::vector<uint32_t> order(n), kids(2ull * (n - 1)), ranges(2ull * (n - 1));
+    e = hipMemcpyAsync(host.data(), world, n * sizeof(Tri), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess && !sah_topology(host.data(), n, order.data(), kids.data(), ranges.data(), error)) {
+      release();
+      return false;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(vals, order.data(), 4ull * n, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(child, kids.data(), 4ull * kids.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(range, ranges.data(), 4ull * ranges.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);  // the host vectors go out of scope
+    if (e != hipSuccess) {
+      error = std::string("SAH topology upload: ") + hipGetErrorString(e);
+      release();
+      return false;
+    }
+  } else if (!radix_sort_pairs(keys, vals, n, 30, stream, error)) {
     release();
     return false;
   }
@@ -702,7 +725,7 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   e = hipMemcpyAsync(tasks_a, &root, sizeof root, hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipMemcpyAsync(counters, init_counts, sizeof init_counts, hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) {
-    if (n > 1) k_karras<<<blocks_for(n - 1, kBlock), kBlock, 0, stream>>>(keys, static_cast<int>(n), child, range);
+    if (n > 1 && !sah) k_karras<<<blocks_for(n - 1, kBlock), kBlock, 0, stream>>>(keys, static_cast<int>(n), child, range);
     k_leaf_boxes<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(world, vals, n, table, out.tris, out.tri_of_global);
     for (uint32_t j = 1; j < levels; ++j) {
       const uint32_t half = 1u << (j - 1);

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "lotus_hip.h"
 #include "scene.hpp"
 
 namespace lotus {
@@ -190,7 +191,13 @@ struct BuildInput {
   uint32_t instance_count;
   uint32_t tri_count;
   uint32_t leaf_size;
+  uint32_t builder;  // LOTUS_HIP_BVH_LBVH or LOTUS_HIP_BVH_SAH
 };
+
+// The binary tree's topology by binned SAH on the host (sah_build.hip):
+// leaf order, children and leaf ranges in bvh_build.hip's format.
+bool sah_topology(const Tri* world, uint32_t n, uint32_t* order, uint32_t* child, uint32_t* range,
+                  std::string& error);
 
 // Builds tris, tri_of_global and the wide BVH into `out` (bvh_build.hip).
 bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, double& gpu_ms,

@@ -235,6 +235,7 @@ struct lotus_hip_renderer {
     float env_scale[3] = {0, 0, 0}, env_frame[16] = {};
     float environment[3] = {0, 0, 0};
     uint32_t leaf_size = 0;
+    uint32_t builder = 0;
   } plan;
   // Lifetime scene counts (lotus_hip_scene_evidence).
   uint64_t bvh_builds = 0, bvh_refits = 0, material_writes = 0, light_writes = 0, uploaded_bytes = 0;
@@ -971,6 +972,7 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   if (!p) return "no scene plan";
   if (p->material_count == 0 || !p->materials) return "the material table is empty";
   if (p->bvh_leaf_size > 8) return "BVH leaves hold at most 8 triangles";
+  if (p->bvh_builder > LOTUS_HIP_BVH_SAH) return "unknown BVH builder";
   uint64_t tris = 0;
   for (uint32_t g = 0; g < p->geometry_count; ++g) {
     const uint32_t vc = p->geometry_vertex_count[g];
@@ -1087,7 +1089,7 @@ std::vector<uint32_t> covered_flags(const lotus_hip_scene_plan* p) {
 
 // The extents of a plan's arrays.
 struct PlanShape {
-  uint32_t geometries = 0, instances = 0, materials = 0, tri_count = 0, leaf_size = 2;
+  uint32_t geometries = 0, instances = 0, materials = 0, tri_count = 0, leaf_size = 2, builder = 0;
   uint64_t vertex_words = 0, index_words = 0;
   std::vector<uint32_t> base;  // first flattened triangle per instance, instances + 1
 };
@@ -1098,6 +1100,7 @@ PlanShape plan_shape(const lotus_hip_scene_plan* p) {
   sh.instances = p->instance_count;
   sh.materials = p->material_count;
   sh.leaf_size = p->bvh_leaf_size ? p->bvh_leaf_size : 2u;
+  sh.builder = p->bvh_builder;
   sh.base.assign(sh.instances + 1, 0);
   for (uint32_t i = 0; i < sh.instances; ++i)
     sh.base[i + 1] = sh.base[i] + p->geometry_triangle_count[p->instance_geometry[i]];
@@ -1154,7 +1157,7 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
   // A retained geometry keeps its vertex count and exactly its triangle
   // indices (gpu_scene.cpp:1651-1656); every instance keeps its geometry.
   const bool topology =
-      r->has_scene && old.leaf_size == sh.leaf_size && same_words(old.gvo, p->geometry_vertex_offset, sh.geometries) &&
+      r->has_scene && old.leaf_size == sh.leaf_size && old.builder == sh.builder && same_words(old.gvo, p->geometry_vertex_offset, sh.geometries) &&
       same_words(old.gvc, p->geometry_vertex_count, sh.geometries) &&
       same_words(old.gto, p->geometry_triangle_offset, sh.geometries) &&
       same_words(old.gtc, p->geometry_triangle_count, sh.geometries) &&
@@ -1257,6 +1260,7 @@ void record_plan(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   if (all || c.instance_materials) d.imat.assign(p->instance_material, p->instance_material + sh.instances);
   std::memcpy(d.environment, p->environment, sizeof d.environment);
   d.leaf_size = sh.leaf_size;
+  d.builder = sh.builder;
 }
 
 void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environment) {
@@ -1485,7 +1489,8 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   s.material_count = M;
   const BuildInput in{s.positions, s.triangles, s.geometry_vertex_offset, s.geometry_triangle_offset,
                       s.instance_geometry, s.instance_transform, s.instance_base,
-                      s.covered_instances ? s.instance_covered : nullptr, I, sh.tri_count, sh.leaf_size};
+                      s.covered_instances ? s.instance_covered : nullptr, I, sh.tri_count, sh.leaf_size,
+                      sh.builder};
   if (!build_bvh(in, s, r->stream, build_ms, error)) return false;
   e = ensure_launch(r);
   if (e != hipSuccess) {
@@ -1555,7 +1560,8 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
   if (c.refit) {
     const BuildInput in{s.positions, s.triangles, s.geometry_vertex_offset, s.geometry_triangle_offset,
                         s.instance_geometry, s.instance_transform, s.instance_base,
-                        s.covered_instances ? s.instance_covered : nullptr, I, sh.tri_count, sh.leaf_size};
+                        s.covered_instances ? s.instance_covered : nullptr, I, sh.tri_count, sh.leaf_size,
+                        sh.builder};
     if (!refit_bvh(in, s, r->stream, build_ms, error)) return false;
     ++r->bvh_refits;
   } else if (covered_changed) {

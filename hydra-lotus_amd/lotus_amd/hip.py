@@ -43,6 +43,8 @@ RECTIFY_NONE, RECTIFY_CLAMP, RECTIFY_ANTI_LAG, RECTIFY_REFLECTION = range(4)
 BUFFER_UNORM8X4, BUFFER_FLOAT32X4, BUFFER_FLOAT32, BUFFER_INT32 = range(4)
 # What a scene apply did (lotus_hip_scene_evidence.update).
 SCENE_UNCHANGED, SCENE_TABLES, SCENE_REFIT, SCENE_BUILD = range(4)
+# BVH topology builders (lotus_hip_scene_plan.bvh_builder).
+BVH_LBVH, BVH_SAH = 0, 1
 
 # The reference's temporal records (include/lotus/temporal.hpp:193-245).
 TEMPORAL_PIXEL = np.dtype([("normal_depth", "<f4", 4), ("motion_depth", "<f4", 4), ("instance_id", "<u8"),
@@ -89,6 +91,7 @@ class ScenePlan(C.Structure):
         ("colors", C.c_void_p),
         ("instance_color_offset", C.c_void_p),
         ("instance_color_constant", C.c_void_p),
+        ("bvh_builder", C.c_uint32),
     ]
 
 
@@ -497,7 +500,8 @@ class Renderer:
     def __del__(self):
         self.close()
 
-    def apply(self, scene, leaf_size: int = 0) -> SceneEvidence:
+    def apply(self, scene, leaf_size: int = 0, builder: int = 0) -> SceneEvidence:
+        """UpdateScene with a whole plan; `builder` is BVH_LBVH or BVH_SAH."""
         p = scene.packed()
         plan = ScenePlan()
         plan.geometry_count = p["geometry_count"]
@@ -516,6 +520,7 @@ class Renderer:
         for i in range(3):
             plan.environment[i] = float(p["environment"][i])
         plan.bvh_leaf_size = leaf_size
+        plan.bvh_builder = builder
         plan.light_count = len(p["lights"])
         plan.lights = p["lights"].ctypes.data if len(p["lights"]) else None
         env = environment_struct(p.get("environment_map"))

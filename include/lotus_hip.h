@@ -138,7 +138,18 @@ typedef struct lotus_hip_scene_plan {
   const float* colors;
   const uint32_t* instance_color_offset;
   const uint32_t* instance_color_constant;
+  /* How the BVH's topology is built (no reference counterpart: the
+   * reference's driver builds its BLAS, gpu_scene.cpp:1899-1950):
+   * LOTUS_HIP_BVH_LBVH (0), Morton order and radix tree on the GPU, the
+   * fastest build; or LOTUS_HIP_BVH_SAH (1), binned SAH on the host's
+   * threads, a slower build whose tree a ray walks through fewer nodes.
+   * Boxes, the 4-wide collapse and refits run on the GPU either way. A
+   * change of builder rebuilds. */
+  uint32_t bvh_builder;
 } lotus_hip_scene_plan;
+
+#define LOTUS_HIP_BVH_LBVH 0u
+#define LOTUS_HIP_BVH_SAH 1u
 
 /* What a lotus_hip_scene_apply did, the counterpart of GpuScene::Apply's
  * retain / refit / rebuild decisions (gpu_scene.cpp:1638-1725; rules in

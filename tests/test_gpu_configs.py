@@ -40,10 +40,11 @@ def heightfield():
 
 
 def test_heightfield_1080p_frame_matches_oracle(heightfield):
+    # on the SAH topology the benchmark builds
     sc = heightfield
     r = hip.Renderer(0)
     try:
-        ev = r.apply(sc)
+        ev = r.apply(sc, builder=hip.BVH_SAH)
         _say(f"heightfield: {sc.triangle_count} triangles")
         out = r.render(sc.world_to_clip, W, H, frame_count=1, sample_index=5, max_bounces=6,
                        light_sampling=hip.LIGHT_MIS)

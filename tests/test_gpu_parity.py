@@ -59,11 +59,14 @@ def test_probe_reports_gfx950():
     assert status == hip.PASS and "gfx950" in detail
 
 
+@pytest.mark.parametrize("builder", [hip.BVH_LBVH, hip.BVH_SAH])
 @pytest.mark.parametrize("count", [1, 7, 300, 20000])
-def test_trace_random_triangles(renderer, count):
-    _say(f"trace random triangles {count}")
+def test_trace_random_triangles(renderer, count, builder):
+    # closest hits and any-hit results do not depend on the tree: both
+    # topologies give the oracle's ids, barycentrics and t bit for bit
+    _say(f"trace random triangles {count}, builder {builder}")
     sc = scenes.random_triangles(count, seed=count)
-    ev = renderer.apply(sc)
+    ev = renderer.apply(sc, builder=builder)
     assert ev.triangle_count == count
     o = oracle.OracleScene(sc)
     rays = _random_rays(20000, seed=count)
@@ -195,12 +198,14 @@ def test_closed_forms(renderer):
         assert list(out["counts"]) == [1024] * (bounces + 1)
 
 
-def test_heightfield_counts_and_primary_hits(renderer):
+@pytest.mark.parametrize("builder", [hip.BVH_LBVH, hip.BVH_SAH])
+def test_heightfield_counts_and_primary_hits(renderer, builder):
     # The 2.1M-triangle heightfield at reduced resolution: queue counts per
-    # bounce exact, per-pixel one-sample radiance, pixel-centre hits exact.
-    _say("heightfield 1024 parity")
+    # bounce exact, per-pixel one-sample radiance, pixel-centre hits exact,
+    # on the GPU-built tree and on the SAH topology.
+    _say(f"heightfield 1024 parity, builder {builder}")
     sc = scenes.heightfield(cells=1024)
-    ev = renderer.apply(sc)
+    ev = renderer.apply(sc, builder=builder)
     assert ev.triangle_count == 2 * 1024 * 1024
     o = oracle.OracleScene(sc)
     w, h = 192, 108

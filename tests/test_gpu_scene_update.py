@@ -70,10 +70,11 @@ def _frame(r, sc):
     return r.render(sc.world_to_clip, 96, 54, frame_count=1, max_bounces=6, light_sampling=hip.LIGHT_MIS)
 
 
-def test_refit_matches_oracle_and_a_fresh_build(renderer):
-    _say("refit: heightfield 64 cells, wave deformation and a moved light")
+@pytest.mark.parametrize("builder", [hip.BVH_LBVH, hip.BVH_SAH])
+def test_refit_matches_oracle_and_a_fresh_build(renderer, builder):
+    _say(f"refit: heightfield 64 cells, wave deformation and a moved light, builder {builder}")
     a = scenes.heightfield(cells=64, emissive=16, tiles=4)
-    ev = renderer.apply(a)
+    ev = renderer.apply(a, builder=builder)
     assert ev.update == hip.SCENE_BUILD
     builds, refits = ev.bvh_builds, ev.bvh_refits
     first = renderer.readback_scene()
@@ -87,7 +88,7 @@ def test_refit_matches_oracle_and_a_fresh_build(renderer):
     t = scenes.identity()
     t[13] = 0.03  # column-major translation y
     b.set_transform(light, t)
-    ev = renderer.apply(b)
+    ev = renderer.apply(b, builder=builder)
     assert ev.update == hip.SCENE_REFIT
     assert ev.bvh_builds == builds and ev.bvh_refits == refits + 1
     assert ev.moved_geometries == len(b.geometries)
@@ -106,7 +107,7 @@ def test_refit_matches_oracle_and_a_fresh_build(renderer):
     refit_frame = _frame(renderer, b)
     fresh = hip.Renderer(0)
     try:
-        ev_fresh = fresh.apply(b)
+        ev_fresh = fresh.apply(b, builder=builder)
         assert ev_fresh.update == hip.SCENE_BUILD
         built_frame = _frame(fresh, b)
     finally:
@@ -116,7 +117,7 @@ def test_refit_matches_oracle_and_a_fresh_build(renderer):
     assert np.array_equal(refit_frame["color"], built_frame["color"])
 
     # back to the first positions and transforms: the first build, bit for bit
-    ev = renderer.apply(a)
+    ev = renderer.apply(a, builder=builder)
     assert ev.update == hip.SCENE_REFIT and ev.bvh_refits == refits + 2
     back = renderer.readback_scene()
     assert np.array_equal(back["bvh_nodes"], first["bvh_nodes"])
@@ -234,6 +235,11 @@ def test_topology_changes_rebuild(renderer):
     ev = renderer.apply(d)
     assert ev.update == hip.SCENE_BUILD and ev.instance_count == len(sc.instances) - 1
     _check_against_oracle(renderer, d, seed=9, box=True)
+    # another builder is another tree; the same builder again changes nothing
+    ev = renderer.apply(d, builder=hip.BVH_SAH)
+    assert ev.update == hip.SCENE_BUILD
+    assert renderer.apply(d, builder=hip.BVH_SAH).update == hip.SCENE_UNCHANGED
+    _check_against_oracle(renderer, d, seed=10, box=True)
 
 
 def test_temporal_history_across_scene_updates(renderer):
