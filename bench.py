@@ -142,7 +142,8 @@ def restir_svgf_run(r, width, height, frames, depth=8, candidates=32, history=20
     frame against a 512-spp plain reference."""
     sc = scenes.cornell_many_lights(camera_aspect=width / height)
     r.apply(sc, builder=BUILDER)
-    tot = {"frame_ms": 0.0, "restir_ms": 0.0, "trace_ms": 0.0, "shade_ms": 0.0, "camera_ms": 0.0}
+    tot = {"frame_ms": 0.0, "restir_ms": 0.0, "trace_ms": 0.0, "shade_ms": 0.0, "camera_ms": 0.0,
+           "temporal_surface_ms": 0.0, "svgf_ms": 0.0}
     rays = 0
     out = None
     for f in range(frames):
@@ -387,6 +388,7 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--builder", choices=("lbvh", "sah"), default="sah",
                     help="BVH topology: Morton order on the GPU, or binned SAH on the host")
+    ap.add_argument("--leaf-size", type=int, default=0, help="BVH leaf size (0: the default, 2)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kind-frames", type=int, default=16,
@@ -426,7 +428,7 @@ def main() -> int:
     r = hip.Renderer(0)
     global BUILDER
     BUILDER = hip.BVH_SAH if args.builder == "sah" else hip.BVH_LBVH
-    ev = r.apply(scene, builder=BUILDER)
+    ev = r.apply(scene, builder=BUILDER, leaf_size=args.leaf_size)
     log(f"scene: {ev.triangle_count} tris, {ev.node_count} nodes, depth {ev.bvh_depth}, "
         f"build {ev.build_gpu_ms:.2f} ms ({args.builder})")
     w2c = scene.world_to_clip

@@ -507,7 +507,7 @@ hipError_t ensure_temporal(lotus_hip_renderer* r, uint32_t width, uint32_t heigh
     if (e == hipSuccess) e = dev_alloc(t.rhist[k], 4 * P);
   }
   if (e == hipSuccess) e = dev_alloc(t.validity, P);
-  if (e == hipSuccess) e = dev_alloc(t.filter, 3 * P);
+  if (e == hipSuccess) e = dev_alloc(t.filter, 4 * P);
   if (e == hipSuccess) e = dev_alloc(t.rspin, 4 * P);
   if (e == hipSuccess) e = dev_alloc(t.rsources, 2 * P);
   if (e == hipSuccess) e = dev_alloc(t.prior, P);
@@ -1585,15 +1585,15 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
 namespace {
 
 // lotus_hip_svgf_static's scratch: SVGF's filter buffers (the a-trous
-// iterations' two halves and the per-pixel guides, 48 B per pixel) for at
-// most kStaticSvgfPixels pixels, 96 MiB in all. The array lives in this
+// iterations' two halves, the per-pixel guides and unpacked normals, 64 B
+// per pixel) for at most kStaticSvgfPixels pixels, 128 MiB in all. The array lives in this
 // library's code object: each GPU the library runs on holds its own copy
 // until the process ends. Host code never passes its name to a kernel: the
 // name is a host-side stand-in, and a kernel given it would fault. Each
 // device's address comes from hipGetSymbolAddress, called once per device
 // with that device current, and only that address is used.
 constexpr uint32_t kStaticSvgfPixels = 1u << 21;  // 2,097,152: 1920 x 1080 fits
-static __device__ __align__(256) float4 scratch[3ull * kStaticSvgfPixels];
+static __device__ __align__(256) float4 scratch[4ull * kStaticSvgfPixels];
 
 std::mutex g_scratch_mutex;
 std::map<int, std::pair<float4*, std::string>> g_scratch;  // device -> (address, error)
@@ -2179,6 +2179,11 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
       (void)hipEventElapsedTime(&a, ev[0], ev[1]);
       if (tf->accumulate) (void)hipEventElapsedTime(&b, ev[2], ev[3]);
       res->frame_gpu_ms += a + b;
+      res->temporal_surface_gpu_ms = a;
+      res->svgf_gpu_ms = b;
+    } else {
+      res->temporal_surface_gpu_ms = 0.0;
+      res->svgf_gpu_ms = 0.0;
     }
   }
   set_detail(detail, detail_size, "");
@@ -2829,7 +2834,7 @@ int lotus_hip_svgf(lotus_hip_renderer* r, const void* surface, void* radiance, u
   if (r->svgf_pixels < P) {
     dev_free(r->svgf_filter);
     r->svgf_pixels = 0;
-    const hipError_t e = dev_alloc(r->svgf_filter, 3ull * P);
+    const hipError_t e = dev_alloc(r->svgf_filter, 4ull * P);
     if (e != hipSuccess) {
       set_detail(detail, detail_size, latch(r, "filter", e));
       return LOTUS_HIP_FAIL;

@@ -480,6 +480,9 @@ __global__ void __launch_bounds__(256) k_temporal_variance(TemporalConst T, cons
     }
     filter[i] = in;
     reinterpret_cast<uint4*>(filter)[2ull * total + i] = guide;
+    // the guide's shading normal unpacked once, for every tap of every iteration
+    const V3 un = c.sample_count != 0u ? unpack_normal(s.shading_normal) : V3{0.0f, 0.0f, 0.0f};
+    filter[3ull * total + i] = make_float4(un.x, un.y, un.z, 0.0f);
   }
 }
 
@@ -505,8 +508,8 @@ __device__ __forceinline__ float filter_depth_step(const TemporalConst& T, const
 
 // temporal.slang:405-454 (FilterIteration; CPU temporal.cpp:424-512): one
 // a-trous iteration at pixel (x, y) over input half `in`.
-__device__ float4 filter_iteration(const TemporalConst& T, const float4* in, const uint4* guides, int x, int y,
-                                   uint32_t iteration) {
+__device__ float4 filter_iteration(const TemporalConst& T, const float4* in, const uint4* guides,
+                                   const float4* normals, int x, int y, uint32_t iteration) {
   const uint32_t index = static_cast<uint32_t>(y) * T.width + static_cast<uint32_t>(x);
   const uint4 center = guides[index];
   if (center.z == kNoGuide) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -525,13 +528,15 @@ __device__ float4 filter_iteration(const TemporalConst& T, const float4* in, con
   const float phi_z = maxf(filter_depth_step(T, guides, x, y, 1, 0, depth, inst) +
                                filter_depth_step(T, guides, x, y, 0, 1, depth, inst),
                            1.0e-8f);
-  const V3 normal = unpack_normal(center.w);
+  const V3 normal = xyz(normals[index]);
   const float4 cv = in[index];
   const float lum = luminance(cv.x, cv.y, cv.z);
   const int step = 1 << iteration;
   const float kernel[3] = {0.375f, 0.25f, 0.0625f};
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, variance = 0.0f, ws = 0.0f;
+#pragma unroll
   for (int ty = -2; ty <= 2; ++ty)
+#pragma unroll
     for (int tx = -2; tx <= 2; ++tx) {
       const int qx = x + tx * step, qy = y + ty * step;
       if (!filter_neighbour(T, guides, qx, qy, inst, g)) continue;
@@ -541,7 +546,7 @@ __device__ float4 filter_iteration(const TemporalConst& T, const float4* in, con
         const float distance = static_cast<float>(step) * sqrtf(static_cast<float>(tx * tx + ty * ty));
         const float wz = fabsf(depth - __uint_as_float(g.z)) / (phi_z * distance);
         const float wl = fabsf(lum - luminance(v.x, v.y, v.z)) / phi_l;
-        const V3 other = unpack_normal(g.w);
+        const V3 other = xyz(normals[static_cast<uint32_t>(qy) * T.width + static_cast<uint32_t>(qx)]);
         const float cosine = (normal.x * other.x + normal.y * other.y) + normal.z * other.z;
         const float wn = pow128(minf(maxf(cosine, 0.0f), 1.0f));
         w = w * expf(-(wl + wz)) * wn;
@@ -555,13 +560,141 @@ __device__ float4 filter_iteration(const TemporalConst& T, const float4* in, con
   return make_float4(sx / ws, sy / ws, sz / ws, variance / (ws * ws));
 }
 
+// One a-trous iteration over LDS tiles, for steps of at most 4: a 16 x 16
+// block of pixels stages the guides and inputs of the block and its apron
+// of 2 s pixels (out of the image: no guide), (16 + 4 s)^2 entries of 32
+// bytes, so the 3 x 3 variance Gaussian, the depth steps and the 25 taps read
+// LDS instead of 16-byte global loads. The arithmetic is filter_iteration's,
+// in the same order, so every result is the same. (Larger steps would need
+// more LDS than a CU holds; they use k_temporal_filter.) kComposite: the
+// last iteration, written into the colour as k_temporal_composite writes it.
+template <int kStep, bool kComposite>
+__global__ void __launch_bounds__(256) k_temporal_filter_tiled(TemporalConst T, float4* filter,
+                                                               const TemporalPixel* surface,
+                                                               const TemporalRadiancePixel* radiance, float4* color) {
+  constexpr int kTile = 16, kApron = 2 * kStep, kSpan = kTile + 2 * kApron;
+  constexpr uint32_t kIteration = kStep == 1 ? 0u : kStep == 2 ? 1u : 2u;
+  __shared__ uint4 lg[kSpan * kSpan];
+  __shared__ float4 lv[kSpan * kSpan];
+  __shared__ float4 ln[kSpan * kSpan];  // each entry's unpacked normal and luminance
+  const uint32_t total = T.width * T.height;
+  const int W = static_cast<int>(T.width), H = static_cast<int>(T.height);
+  const float4* in = filter + static_cast<size_t>(kIteration & 1u) * total;
+  float4* out = filter + static_cast<size_t>((kIteration + 1u) & 1u) * total;
+  const uint4* guides = reinterpret_cast<const uint4*>(filter) + 2ull * total;
+  const float4* normals = filter + 3ull * total;
+  const uint32_t tiles_x = (T.width + kTile - 1) / kTile;
+  const uint32_t tiles = tiles_x * ((T.height + kTile - 1) / kTile);
+  const int lx = static_cast<int>(threadIdx.x % kTile), ly = static_cast<int>(threadIdx.x / kTile);
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const int x0 = static_cast<int>(t % tiles_x) * kTile - kApron;
+    const int y0 = static_cast<int>(t / tiles_x) * kTile - kApron;
+    __syncthreads();  // the previous tile's taps are done
+    for (int k = static_cast<int>(threadIdx.x); k < kSpan * kSpan; k += static_cast<int>(blockDim.x)) {
+      const int gx = x0 + k % kSpan, gy = y0 + k / kSpan;
+      if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+        const uint32_t q = static_cast<uint32_t>(gy) * T.width + static_cast<uint32_t>(gx);
+        const float4 v = in[q];
+        const float4 n = normals[q];
+        lg[k] = guides[q];
+        lv[k] = v;
+        ln[k] = make_float4(n.x, n.y, n.z, luminance(v.x, v.y, v.z));
+      } else {
+        lg[k] = make_uint4(0u, 0u, kNoGuide, 0u);
+        lv[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ln[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+    }
+    __syncthreads();
+    const int x = x0 + kApron + lx, y = y0 + kApron + ly;
+    if (x >= W || y >= H) continue;
+    const uint32_t index = static_cast<uint32_t>(y) * T.width + static_cast<uint32_t>(x);
+    if constexpr (kComposite) {
+      const TemporalRadiancePixel r = radiance[index];
+      if (r.sample_count == 0u || T.iterations == 0u) {
+        color[index] = r.color;
+        continue;
+      }
+    }
+    const int centre = (ly + kApron) * kSpan + (lx + kApron);
+    const uint4 center = lg[centre];
+    float4 result = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (center.z != kNoGuide) {
+      const uint2 inst = make_uint2(center.x, center.y);
+      const auto match = [&](const uint4& g) { return g.z != kNoGuide && g.x == inst.x && g.y == inst.y; };
+      float gaussian = 0.0f, gaussian_weight = 0.0f;
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int k = centre + dy * kSpan + dx;
+          if (!match(lg[k])) continue;
+          const float kw = (dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f);
+          gaussian += kw * lv[k].w;
+          gaussian_weight += kw;
+        }
+      const float phi_l = 4.0f * sqrtf(maxf(gaussian / gaussian_weight, 0.0f) + 1.0e-10f);
+      const float depth = __uint_as_float(center.z);
+      // filter_depth_step along x, then y
+      const auto depth_step = [&](int off) {
+        float st = -1.0f;
+        const uint4 a = lg[centre - off], b = lg[centre + off];
+        if (match(a)) st = fabsf(depth - __uint_as_float(a.z));
+        if (match(b)) {
+          const float other = fabsf(depth - __uint_as_float(b.z));
+          st = st < 0.0f ? other : minf(st, other);
+        }
+        return maxf(st, 0.0f);
+      };
+      const float phi_z = maxf(depth_step(1) + depth_step(kSpan), 1.0e-8f);
+      const float4 cn = ln[centre];
+      const V3 normal = V3{cn.x, cn.y, cn.z};
+      const float lum = cn.w;
+      const float kernel[3] = {0.375f, 0.25f, 0.0625f};
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f, variance = 0.0f, ws = 0.0f;
+#pragma unroll
+      for (int ty = -2; ty <= 2; ++ty)
+#pragma unroll
+        for (int tx = -2; tx <= 2; ++tx) {
+          const int k = centre + ty * kStep * kSpan + tx * kStep;
+          const uint4 g = lg[k];
+          if (!match(g)) continue;
+          const float4 v = lv[k];
+          float w = kernel[tx < 0 ? -tx : tx] * kernel[ty < 0 ? -ty : ty];
+          if (tx != 0 || ty != 0) {
+            const float4 on = ln[k];
+            const float distance = static_cast<float>(kStep) * sqrtf(static_cast<float>(tx * tx + ty * ty));
+            const float wz = fabsf(depth - __uint_as_float(g.z)) / (phi_z * distance);
+            const float wl = fabsf(lum - on.w) / phi_l;
+            const V3 other = V3{on.x, on.y, on.z};
+            const float cosine = (normal.x * other.x + normal.y * other.y) + normal.z * other.z;
+            const float wn = pow128(minf(maxf(cosine, 0.0f), 1.0f));
+            w = w * expf(-(wl + wz)) * wn;
+          }
+          sx += w * v.x;
+          sy += w * v.y;
+          sz += w * v.z;
+          variance += w * w * v.w;
+          ws += w;
+        }
+      result = make_float4(sx / ws, sy / ws, sz / ws, variance / (ws * ws));
+    }
+    if constexpr (kComposite) {
+      const V3 c = remodulate(V3{result.x, result.y, result.z}, surface[index].albedo);
+      color[index] = make_float4(c.x, c.y, c.z, radiance[index].color.w);
+    } else {
+      out[index] = result;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_temporal_filter(TemporalConst T, float4* filter, uint32_t iteration) {
   const uint32_t total = T.width * T.height;
   const float4* in = filter + static_cast<size_t>(iteration & 1u) * total;
   float4* out = filter + static_cast<size_t>((iteration + 1u) & 1u) * total;
   const uint4* guides = reinterpret_cast<const uint4*>(filter) + 2ull * total;
+  const float4* normals = filter + 3ull * total;
   for (uint32_t i = tid(); i < total; i += tstride())
-    out[i] = filter_iteration(T, in, guides, static_cast<int>(i % T.width), static_cast<int>(i / T.width), iteration);
+    out[i] = filter_iteration(T, in, guides, normals, static_cast<int>(i % T.width), static_cast<int>(i / T.width),
+                              iteration);
 }
 
 // temporal.slang:468-478 (CompositeTemporalRadiance): the last iteration,
@@ -574,13 +707,15 @@ __global__ void __launch_bounds__(256) k_temporal_composite(TemporalConst T, con
   const uint32_t last = T.iterations ? T.iterations - 1u : 0u;
   const float4* in = filter + static_cast<size_t>(last & 1u) * total;
   const uint4* guides = reinterpret_cast<const uint4*>(filter) + 2ull * total;
+  const float4* normals = filter + 3ull * total;
   for (uint32_t i = tid(); i < total; i += tstride()) {
     const TemporalRadiancePixel r = radiance[i];
     if (r.sample_count == 0u || T.iterations == 0u) {
       color[i] = r.color;
       continue;
     }
-    const float4 f = filter_iteration(T, in, guides, static_cast<int>(i % T.width), static_cast<int>(i / T.width), last);
+    const float4 f =
+        filter_iteration(T, in, guides, normals, static_cast<int>(i % T.width), static_cast<int>(i / T.width), last);
     const V3 c = remodulate(V3{f.x, f.y, f.z}, surface[i].albedo);
     color[i] = make_float4(c.x, c.y, c.z, r.color.w);
   }
@@ -624,9 +759,32 @@ void launch_temporal_variance(const Launch& L, const TemporalConst& T, const Tem
 
 void launch_temporal_filter(const Launch& L, const TemporalConst& T, const TemporalPixel* surface,
                             const TemporalRadiancePixel* radiance, float4* filter, float4* color) {
-  for (uint32_t it = 0; it + 1u < T.iterations; ++it)
-    k_temporal_filter<<<pixel_blocks(T.width * T.height), 256, 0, L.stream>>>(T, filter, it);
-  k_temporal_composite<<<pixel_blocks(T.width * T.height), 256, 0, L.stream>>>(T, surface, radiance, filter, color);
+  const uint32_t blocks = pixel_blocks(T.width * T.height);
+  // iterations 0-2 (steps 1, 2, 4) over LDS tiles, later ones from global memory
+  const auto iteration = [&](uint32_t it, bool composite) {
+    if (it == 0 && composite)
+      k_temporal_filter_tiled<1, true><<<blocks, 256, 0, L.stream>>>(T, filter, surface, radiance, color);
+    else if (it == 0)
+      k_temporal_filter_tiled<1, false><<<blocks, 256, 0, L.stream>>>(T, filter, surface, radiance, color);
+    else if (it == 1 && composite)
+      k_temporal_filter_tiled<2, true><<<blocks, 256, 0, L.stream>>>(T, filter, surface, radiance, color);
+    else if (it == 1)
+      k_temporal_filter_tiled<2, false><<<blocks, 256, 0, L.stream>>>(T, filter, surface, radiance, color);
+    else if (it == 2 && composite)
+      k_temporal_filter_tiled<4, true><<<blocks, 256, 0, L.stream>>>(T, filter, surface, radiance, color);
+    else if (it == 2)
+      k_temporal_filter_tiled<4, false><<<blocks, 256, 0, L.stream>>>(T, filter, surface, radiance, color);
+    else if (composite)
+      k_temporal_composite<<<blocks, 256, 0, L.stream>>>(T, surface, radiance, filter, color);
+    else
+      k_temporal_filter<<<blocks, 256, 0, L.stream>>>(T, filter, it);
+  };
+  if (T.iterations == 0u) {
+    k_temporal_composite<<<blocks, 256, 0, L.stream>>>(T, surface, radiance, filter, color);
+    return;
+  }
+  for (uint32_t it = 0; it + 1u < T.iterations; ++it) iteration(it, false);
+  iteration(T.iterations - 1u, true);
 }
 
 }  // namespace lotus

@@ -262,6 +262,8 @@ class FrameResult(C.Structure):
         ("cache_training", _f32p),
         ("cache_training_capacity", C.c_uint32),
         ("cache_training_count", C.c_uint32),
+        ("temporal_surface_gpu_ms", C.c_double),
+        ("svgf_gpu_ms", C.c_double),
     ]
 
 
@@ -582,6 +584,7 @@ class Renderer:
                     cache_queries=res.cache_queries, cache_samples=res.cache_samples, cache_loss=res.cache_loss,
                     cache_infer_ms=res.cache_infer_gpu_ms, cache_train_ms=res.cache_train_gpu_ms,
                     restir=restir, restir_ms=res.restir_gpu_ms,
+                    temporal_surface_ms=res.temporal_surface_gpu_ms, svgf_ms=res.svgf_gpu_ms,
                     training=None if training is None else training[:res.cache_training_count].copy())
 
     def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:

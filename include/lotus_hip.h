@@ -290,6 +290,13 @@ typedef struct lotus_hip_frame_result {
   float* cache_training;
   uint32_t cache_training_capacity;
   uint32_t cache_training_count;
+  /* Temporal frames (TemporalProducts' pass timings,
+   * vulkan_backend.hpp:120-146), hipEvent: the pixel-centre surface pass
+   * with its reprojection, and the radiance accumulation, variance and
+   * a-trous iterations (or a settling frame's crossfade); both are part of
+   * frame_gpu_ms. */
+  double temporal_surface_gpu_ms;
+  double svgf_gpu_ms;
 } lotus_hip_frame_result;
 
 /* ProbeVulkanBackend (vulkan_backend.hpp:742): PASS when a gfx950 device is
@@ -590,7 +597,7 @@ int lotus_hip_accumulate(lotus_hip_renderer* renderer, const void* batch_device,
 /* lotus_hip_svgf without allocating: no counterpart in the reference. For
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc). The same kernels and
- * results as lotus_hip_svgf, with the filter's scratch (48 B per pixel) in a
+ * results as lotus_hip_svgf, with the filter's scratch (64 B per pixel) in a
  * buffer of the library's own, compiled into it, of at most 2,097,152 pixels
  * (1920 x 1080 fits); a larger image fails with a message and nothing
  * written. Calls no hipMalloc, hipFree or allocating routine. Every call on
