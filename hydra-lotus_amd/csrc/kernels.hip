@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
           w.cur = kWalkDone;
           return;
         }
-        w.start(start_path(F, F.rect[0] + x, F.rect[1] + y, F.sample_index, rng), false);
+        w.start(start_path(F, F.rect[0] + x, F.rect[1] + y, frame_sample_index(F), rng), false);
       },
       [&](uint32_t i, const auto& w) {
         uint32_t x, y;
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
         const uint32_t k = wave_append(hit_count(counters, 0), hit);
         if (hit) {
           uint32_t rng;
-          const Ray r = start_path(F, x, y, F.sample_index, rng);  // again: the ray, and the path's random state
+          const Ray r = start_path(F, x, y, frame_sample_index(F), rng);  // again: the ray, and the path's random state
           push_hit(q, k, slot, r.o, r.d, w.record(S.tris));
           q.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng));
         } else {
@@ -1085,12 +1085,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
 // then the pending direct light, then the pending environment) added to the
 // pixel's accumulation, and the mean written, a missed sample counting as the
 // background; pixels no sample has hit keep the clear colour.
-__global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_t add_sample) {
+__global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_t add_arg) {
   const uint32_t rw = F.rect[2], total = F.rect[2] * F.rect[3];
+  const uint32_t accumulated = frame_accumulated(F);
+  const uint32_t add_sample = F.vars ? F.vars[2] : add_arg;
   for (uint32_t i = gtid(); i < total; i += gstride()) {
     const uint32_t x = F.rect[0] + i % rw, y = F.rect[1] + i / rw;
     const uint32_t slot = y * F.width + x;
-    float4 sum = F.accumulated != 0 ? q.accum[slot] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 sum = accumulated != 0 ? q.accum[slot] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (add_sample) {
       const float4 s = q.rad[slot], n = q.nee[slot], e = q.esc[slot];
       V3 v = xyz(s);
@@ -1103,7 +1105,7 @@ __global__ void __launch_bounds__(256) k_resolve(FrameConst F, Queues q, uint32_
       sum.w = sum.w + s.w;
       q.accum[slot] = sum;
     }
-    const float count = static_cast<float>(F.accumulated + add_sample);
+    const float count = static_cast<float>(accumulated + add_sample);
     if (sum.w == 0.0f || count == 0.0f) {
       q.color[slot] = make_float4(F.background[0], F.background[1], F.background[2], F.background[3]);
       continue;

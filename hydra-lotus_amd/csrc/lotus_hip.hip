@@ -290,6 +290,22 @@ struct lotus_hip_renderer {
     uint64_t launches = 0;  // its traversal launches
   } deferred;
   hipEvent_t join = nullptr;       // this renderer's branch of a batch ends
+  // The executable graph of the last batch this renderer led (its first
+  // view), for the views in `batch_views`: a later batch of the same views
+  // updates it in place (hipGraphExecUpdate) instead of instantiating anew.
+  hipGraphExec_t batch_exec = nullptr;
+  std::vector<const lotus_hip_renderer*> batch_views;
+  bool graph_batch = false;  // the batch this renderer's sample is enqueued in is captured
+  // A captured sample's frame variables: their device words (in the batch
+  // leader's array), this frame's values, and the sample's deferred record
+  // and parameters, which a replay of the same graph reuses.
+  uint32_t* graph_vars = nullptr;
+  uint32_t graph_host_vars[4] = {0, 0, 0, 0};
+  Deferred graph_deferred;
+  std::vector<uint8_t> graph_signature;
+  uint32_t* batch_vars = nullptr;  // the batch leader's frame variables, 4 words per view
+  uint32_t batch_vars_count = 0;
+  uint64_t batch_instantiations = 0, batch_updates = 0, batch_replays = 0;
   float4* svgf_filter = nullptr;   // lotus_hip_svgf's scratch
   uint32_t svgf_pixels = 0;
   float4* static_scratch = nullptr;  // this device's copy of `scratch` (lotus_hip_svgf_static)
@@ -1695,6 +1711,8 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   for (auto& ev : r->events)
     if (ev) (void)hipEventDestroy(ev);
   if (r->join) (void)hipEventDestroy(r->join);
+  if (r->batch_exec) (void)hipGraphExecDestroy(r->batch_exec);
+  dev_free(r->batch_vars);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
 }
@@ -1755,6 +1773,30 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
 }  // extern "C"
 
 namespace {
+// AccumulationKey and restart (vulkan_backend.cpp:419-434, 1186-1205): the
+// accumulation starts over when anything it depends on changed.
+void restart_if_changed(lotus_hip_renderer* r, const lotus_hip_frame_params* P, const float* vk_world_to_clip,
+                        bool cache_frame, bool restir) {
+  AccumulationKey key;
+  std::memcpy(key.world_to_clip.data(), vk_world_to_clip, 64);
+  key.width = P->width;
+  key.height = P->height;
+  std::memcpy(key.display.data(), P->display_window, 16);
+  std::memcpy(key.data.data(), P->data_window, 16);
+  key.light_sampling = P->light_sampling;
+  key.sample_index = P->sample_index;
+  key.max_bounces = P->max_bounces;
+  key.scene_generation = r->scene_generation;
+  key.cache_frames = cache_frame ? 1u : 0u;
+  key.restir_candidates = restir ? P->restir_candidates : 0u;
+  key.restir_proposal = restir ? P->restir_proposal : 0u;
+  if (!r->key_valid || !(r->key == key)) {
+    r->key = key;
+    r->key_valid = true;
+    r->accumulated = 0;
+  }
+}
+
 // `deferred` (batched frames): one sample is enqueued on the renderer's
 // stream with the previous sample's rounds, without events, readbacks or a
 // synchronisation; batch_finish reads its counters afterwards.
@@ -1897,24 +1939,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
   uint64_t trace_launches = 0;
   uint32_t rounds_used = 0;
   if (P->output == LOTUS_HIP_OUTPUT_RADIANCE) {
-    AccumulationKey key;
-    std::memcpy(key.world_to_clip.data(), F.world_to_clip, 64);
-    key.width = P->width;
-    key.height = P->height;
-    std::memcpy(key.display.data(), P->display_window, 16);
-    std::memcpy(key.data.data(), P->data_window, 16);
-    key.light_sampling = P->light_sampling;
-    key.sample_index = P->sample_index;
-    key.max_bounces = P->max_bounces;
-    key.scene_generation = r->scene_generation;
-    key.cache_frames = cache_frame ? 1u : 0u;
-    key.restir_candidates = restir ? P->restir_candidates : 0u;
-    key.restir_proposal = restir ? P->restir_proposal : 0u;
-    if (!r->key_valid || !(r->key == key)) {
-      r->key = key;
-      r->key_valid = true;
-      r->accumulated = 0;
-    }
+    restart_if_changed(r, P, F.world_to_clip, cache_frame, restir);
     const uint32_t room = P->max_samples == 0 ? P->frame_count
                                               : P->max_samples - std::min(P->max_samples, r->accumulated);
     const uint32_t to_add = std::min(P->frame_count, room);
@@ -1925,7 +1950,20 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
       const uint32_t add = sample < to_add ? 1u : 0u;
       F.sample_index = P->sample_index + r->accumulated;
       F.accumulated = r->accumulated;
+      if (deferred && r->graph_batch && r->graph_vars) {
+        // a captured sample reads its frame variables from device memory,
+        // so a replay of the graph takes another frame's values
+        F.vars = r->graph_vars;
+        r->graph_host_vars[0] = F.sample_index;
+        r->graph_host_vars[1] = F.accumulated;
+        r->graph_host_vars[2] = add;
+        r->graph_host_vars[3] = 0u;
+      }
       uint32_t rounds = r->rounds_hint == 0 ? std::min(bounces, kFirstRounds) : std::min(r->rounds_hint, bounces);
+      // a graphed batch runs every bounce as a round: its launches, and so
+      // its graph's topology, stay the same from frame to frame (results do
+      // not depend on the rounds)
+      if (deferred && r->graph_batch) rounds = std::min(bounces, kFirstRounds);
       rounds = std::min(std::max(rounds, 1u), kMaxRounds);
       if (cache_frame) {
         rounds = std::min(std::max(rounds, F.cache_depth + 1), std::min(bounces, kMaxRounds));
@@ -3008,19 +3046,66 @@ namespace {
 
 // The bookkeeping of a deferred sample once its batch has completed: ray
 // counts, the next sample's rounds, the accumulation count, the colour.
+// What a graphed batch's captured launches of one view depend on besides
+// its frame variables: the parameters (but the sample index), the scene's
+// generation, the launch tuning and the view's buffers.
+std::vector<uint8_t> graph_signature(const lotus_hip_renderer* r, const lotus_hip_frame_params* P) {
+  lotus_hip_frame_params p = *P;
+  p.sample_index = 0;
+  std::vector<uint8_t> out(sizeof p + sizeof r->scene_generation + sizeof r->tuning + 4 * sizeof(void*));
+  uint8_t* o = out.data();
+  std::memcpy(o, &p, sizeof p);
+  o += sizeof p;
+  std::memcpy(o, &r->scene_generation, sizeof r->scene_generation);
+  o += sizeof r->scene_generation;
+  std::memcpy(o, &r->tuning, sizeof r->tuning);
+  o += sizeof r->tuning;
+  const void* buffers[4] = {r->q.thr, r->q.color, r->counters, r->stats};
+  std::memcpy(o, buffers, sizeof buffers);
+  return out;
+}
+
+// A replayed view's host side of render_impl's deferred sample: the
+// accumulation's restart rule, and the frame variables of a sample that
+// adds (false when none would: the captured launches no longer apply).
+bool replay_prelude(lotus_hip_renderer* r, const lotus_hip_frame_params* P, uint32_t* vars) {
+  float vk[16];
+  vulkan_world_to_clip(P->world_to_clip, vk);
+  restart_if_changed(r, P, vk, false, false);
+  if (P->max_samples != 0 && r->accumulated >= P->max_samples) return false;
+  if (!r->graph_deferred.add) return false;
+  vars[0] = P->sample_index + r->accumulated;
+  vars[1] = r->accumulated;
+  vars[2] = 1u;
+  vars[3] = 0u;
+  r->deferred = r->graph_deferred;
+  r->deferred.pending = true;
+  return true;
+}
+
+// The readbacks of a completed batch, every view's counters and colour
+// enqueued on the leader's stream with one synchronisation for all.
+hipError_t batch_readback(lotus_hip_renderer* const* renderers, const lotus_hip_frame_params* params,
+                          lotus_hip_frame_result* results, uint32_t count, hipStream_t stream,
+                          std::vector<std::vector<uint32_t>>& hosts) {
+  hosts.resize(count);
+  hipError_t e = hipSuccess;
+  for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+    lotus_hip_renderer* r = renderers[i];
+    hosts[i].assign(frame_counter_words(r->deferred.stride), 0u);
+    e = hipMemcpyAsync(hosts[i].data(), r->counters, hosts[i].size() * 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess && results && results[i].color)
+      e = hipMemcpyAsync(results[i].color, r->q.color, 16ull * params[i].width * params[i].height,
+                         hipMemcpyDeviceToHost, stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  return e;
+}
+
 int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hip_frame_result* res, double gpu_ms,
-                 char* detail, size_t detail_size) {
+                 const std::vector<uint32_t>& host) {
   auto& d = r->deferred;
   d.pending = false;
-  std::vector<uint32_t> host(frame_counter_words(d.stride));
-  hipError_t e = hipMemcpy(host.data(), r->counters, host.size() * 4, hipMemcpyDeviceToHost);
-  const uint32_t npix = P->width * P->height;
-  if (e == hipSuccess && res && res->color)
-    e = hipMemcpy(res->color, r->q.color, npix * 16ull, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    set_detail(detail, detail_size, latch(r, "batch readback", e));
-    return LOTUS_HIP_FAIL;
-  }
   std::vector<uint32_t> counts;
   uint64_t ext = 0, sh = 0, round_ext = 0, round_sh = 0;
   if (d.add) {
@@ -3106,29 +3191,124 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
   const bool graph = (flags & LOTUS_HIP_BATCH_GRAPH) != 0;
   hipEvent_t start = root->events[root->events.size() - 2], stop = root->events[root->events.size() - 1];
   hipEvent_t fork = root->join;
+  // a graphed batch's frame variables (4 words per view) in device memory
+  if (graph && root->batch_vars_count < count) {
+    dev_free(root->batch_vars);
+    root->batch_vars_count = 0;
+    e = dev_alloc(root->batch_vars, 4ull * count);
+    if (e != hipSuccess) {
+      set_detail(detail, detail_size, latch(root, "batch allocation", e));
+      return LOTUS_HIP_FAIL;
+    }
+    root->batch_vars_count = count;
+  }
+  const std::vector<const lotus_hip_renderer*> views(renderers, renderers + count);
+  std::vector<uint32_t> vars(4ull * count, 0u);
+  std::vector<std::vector<uint8_t>> signatures(count);
+  if (graph) {
+    for (uint32_t i = 0; i < count; ++i) {
+      renderers[i]->graph_vars = root->batch_vars + 4ull * i;
+      signatures[i] = graph_signature(renderers[i], &params[i]);
+    }
+  }
+  // Replay: the same views with the same parameters (but the sample index)
+  // on the same scenes and buffers as the captured graph, each adding a
+  // sample: the graph runs again with this frame's variables, uncaptured.
+  bool replay = graph && root->batch_exec && root->batch_views == views;
+  for (uint32_t i = 0; i < count && replay; ++i)
+    replay = renderers[i]->graph_signature == signatures[i] && replay_prelude(renderers[i], &params[i], &vars[4ull * i]);
+  if (replay) {
+    e = hipMemcpyAsync(root->batch_vars, vars.data(), vars.size() * 4, hipMemcpyHostToDevice, root->stream);
+    if (e == hipSuccess) e = hipEventRecord(start, root->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(root->batch_exec, root->stream);
+    if (e == hipSuccess) e = hipEventRecord(stop, root->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(root->stream);
+    if (e != hipSuccess) {
+      for (uint32_t i = 0; i < count; ++i) renderers[i]->deferred.pending = false;
+      set_detail(detail, detail_size, latch(root, "batched frame", e));
+      return LOTUS_HIP_FAIL;
+    }
+    ++root->batch_replays;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, start, stop);
+    if (gpu_ms) *gpu_ms = ms;
+    std::vector<std::vector<uint32_t>> hosts;
+    e = batch_readback(renderers, params, results, count, root->stream, hosts);
+    if (e != hipSuccess) {
+      for (uint32_t i = 0; i < count; ++i) renderers[i]->deferred.pending = false;
+      set_detail(detail, detail_size, latch(root, "batch readback", e));
+      return LOTUS_HIP_FAIL;
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+      (void)batch_finish(renderers[i], &params[i], results ? &results[i] : nullptr, ms, hosts[i]);
+      if (results) results[i].batch_graph = 3u;
+    }
+    set_detail(detail, detail_size, "");
+    return LOTUS_HIP_PASS;
+  }
   if (graph) e = hipStreamBeginCapture(root->stream, hipStreamCaptureModeRelaxed);
   if (!graph && e == hipSuccess) e = hipEventRecord(start, root->stream);
   if (e == hipSuccess) e = hipEventRecord(fork, root->stream);
   for (uint32_t i = 1; i < count && e == hipSuccess; ++i) e = hipStreamWaitEvent(renderers[i]->stream, fork, 0);
   int status = LOTUS_HIP_PASS;
-  for (uint32_t i = 0; i < count && e == hipSuccess && status == LOTUS_HIP_PASS; ++i)
+  for (uint32_t i = 0; i < count && e == hipSuccess && status == LOTUS_HIP_PASS; ++i) {
+    renderers[i]->graph_batch = graph;
     status = render_impl(renderers[i], &params[i], nullptr, detail, detail_size, nullptr, true);
+    renderers[i]->graph_batch = false;
+    if (graph && status == LOTUS_HIP_PASS) {
+      std::memcpy(&vars[4ull * i], renderers[i]->graph_host_vars, 16);
+      renderers[i]->graph_deferred = renderers[i]->deferred;
+    }
+  }
   for (uint32_t i = 1; i < count && e == hipSuccess; ++i) {
     e = hipEventRecord(renderers[i]->join, renderers[i]->stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(root->stream, renderers[i]->join, 0);
   }
   hipGraph_t g = nullptr;
-  hipGraphExec_t exec = nullptr;
+  bool updated_graph = false;
   if (graph) {
     const hipError_t end = hipStreamEndCapture(root->stream, &g);
     if (e == hipSuccess) e = end;
-    if (e == hipSuccess && status == LOTUS_HIP_PASS) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (e == hipSuccess && status == LOTUS_HIP_PASS) {
+      // the same views as the last batch: the executable graph takes this
+      // frame's parameters in place when the topology is unchanged
+      const std::vector<const lotus_hip_renderer*> views(renderers, renderers + count);
+      bool updated = false;
+      if (root->batch_exec && root->batch_views == views) {
+        hipGraphNode_t failed = nullptr;
+        hipGraphExecUpdateResult result = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(root->batch_exec, g, &failed, &result) == hipSuccess &&
+                  result == hipGraphExecUpdateSuccess;
+        if (!updated) (void)hipGetLastError();  // a refused update leaves no error behind
+      }
+      updated_graph = updated;
+      if (updated) {
+        ++root->batch_updates;
+      } else {
+        if (root->batch_exec) (void)hipGraphExecDestroy(root->batch_exec);
+        root->batch_exec = nullptr;
+        root->batch_views.clear();
+        e = hipGraphInstantiate(&root->batch_exec, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) {
+          root->batch_views = views;
+          ++root->batch_instantiations;
+        } else {
+          root->batch_exec = nullptr;
+        }
+      }
+    }
+    if (e == hipSuccess && status == LOTUS_HIP_PASS)
+      e = hipMemcpyAsync(root->batch_vars, vars.data(), vars.size() * 4, hipMemcpyHostToDevice, root->stream);
     if (e == hipSuccess && status == LOTUS_HIP_PASS) e = hipEventRecord(start, root->stream);
-    if (e == hipSuccess && status == LOTUS_HIP_PASS) e = hipGraphLaunch(exec, root->stream);
+    if (e == hipSuccess && status == LOTUS_HIP_PASS) e = hipGraphLaunch(root->batch_exec, root->stream);
+    // the views' signatures, for replays of this graph
+    for (uint32_t i = 0; i < count; ++i)
+      renderers[i]->graph_signature = (e == hipSuccess && status == LOTUS_HIP_PASS && root->batch_exec)
+                                          ? std::move(signatures[i])
+                                          : std::vector<uint8_t>{};
   }
   if (e == hipSuccess) e = hipEventRecord(stop, root->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(root->stream);
-  if (exec) (void)hipGraphExecDestroy(exec);
   if (g) (void)hipGraphDestroy(g);
   if (e != hipSuccess || status != LOTUS_HIP_PASS) {
     for (uint32_t i = 0; i < count; ++i) renderers[i]->deferred.pending = false;
@@ -3138,9 +3318,17 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, start, stop);
   if (gpu_ms) *gpu_ms = ms;
+  const uint32_t graph_kind = !graph ? 0u : updated_graph ? 2u : 1u;
+  std::vector<std::vector<uint32_t>> hosts;
+  e = batch_readback(renderers, params, results, count, root->stream, hosts);
+  if (e != hipSuccess) {
+    for (uint32_t i = 0; i < count; ++i) renderers[i]->deferred.pending = false;
+    set_detail(detail, detail_size, latch(root, "batch readback", e));
+    return LOTUS_HIP_FAIL;
+  }
   for (uint32_t i = 0; i < count; ++i) {
-    status = batch_finish(renderers[i], &params[i], results ? &results[i] : nullptr, ms, detail, detail_size);
-    if (status != LOTUS_HIP_PASS) return status;
+    (void)batch_finish(renderers[i], &params[i], results ? &results[i] : nullptr, ms, hosts[i]);
+    if (results) results[i].batch_graph = graph_kind;
   }
   set_detail(detail, detail_size, "");
   return LOTUS_HIP_PASS;

@@ -43,7 +43,18 @@ struct FrameConst {
   float view[16];
   const uint64_t* instance_ids;  // stable ids (null: instance + 1, material slot + 1)
   const uint64_t* material_ids;
+  // A replayed graph's frame variables in device memory (null: the fields
+  // above): the sample index, the samples already accumulated and whether
+  // this sample adds, written before each launch of the graph.
+  const uint32_t* vars;
 };
+
+__device__ __forceinline__ uint32_t frame_sample_index(const FrameConst& F) {
+  return F.vars ? F.vars[0] : F.sample_index;
+}
+__device__ __forceinline__ uint32_t frame_accumulated(const FrameConst& F) {
+  return F.vars ? F.vars[1] : F.accumulated;
+}
 
 __device__ __forceinline__ uint64_t stable_instance_id(const FrameConst& F, uint32_t instance) {
   return F.instance_ids ? F.instance_ids[instance] : static_cast<uint64_t>(instance) + 1u;

@@ -264,6 +264,7 @@ class FrameResult(C.Structure):
         ("cache_training_count", C.c_uint32),
         ("temporal_surface_gpu_ms", C.c_double),
         ("svgf_gpu_ms", C.c_double),
+        ("batch_graph", C.c_uint32),
     ]
 
 
@@ -450,6 +451,41 @@ def frame_params(world_to_clip, width, height, frame_count=1, sample_index=0, ma
     return fp
 
 
+class Batch:
+    """A batched frame whose parameter and result records are built once:
+    run() renders one frame of every view (optionally each at a new sample
+    index) and returns the rays the views traced and the batch's GPU ms,
+    with little host work, as a serving loop would call it."""
+
+    def __init__(self, renderers, views):
+        n = len(renderers)
+        self.n = n
+        self.handles = (C.c_void_p * n)(*[r.handle for r in renderers])
+        self.ps = (FrameParams * n)()
+        self.rs = (FrameResult * n)()
+        self._counts = []
+        for i, v in enumerate(views):
+            self.ps[i] = frame_params(**v)
+            cap = v.get("max_bounces", 8) + 2
+            k = np.zeros(cap, np.uint32)
+            self.rs[i].path_counts = _p(k, C.c_uint32)
+            self.rs[i].path_counts_capacity = cap
+            self._counts.append(k)
+        self.ms = C.c_double(0.0)
+        self.buf = C.create_string_buffer(512)
+
+    def run(self, graph=False, sample_index=None):
+        if sample_index is not None:
+            for i in range(self.n):
+                self.ps[i].sample_index = sample_index
+        s = lib().lotus_hip_render_batch(self.handles, self.ps, self.rs, self.n, BATCH_GRAPH if graph else 0,
+                                         C.byref(self.ms), self.buf, 512)
+        if s != PASS:
+            raise LotusError(s, self.buf.value.decode())
+        rays = sum(r.camera_rays + r.extension_rays + r.shadow_rays for r in self.rs)
+        return rays, self.ms.value, self.rs[0].batch_graph
+
+
 def render_batch(renderers, views, graph=False, want_color=True):
     """One batched frame: views[i] (frame_params() keyword dicts) through
     renderers[i]. Returns (per-view results, GPU ms)."""
@@ -478,7 +514,8 @@ def render_batch(renderers, views, graph=False, want_color=True):
     for i in range(n):
         out.append(dict(color=colors[i], counts=counts[i][:rs[i].path_counts_size].copy(), spp=rs[i].samples_per_pixel,
                         camera_rays=rs[i].camera_rays, extension_rays=rs[i].extension_rays,
-                        shadow_rays=rs[i].shadow_rays, rounds=rs[i].wavefront_rounds))
+                        shadow_rays=rs[i].shadow_rays, rounds=rs[i].wavefront_rounds,
+                        batch_graph=rs[i].batch_graph))
     return out, ms.value
 
 

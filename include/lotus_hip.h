@@ -297,6 +297,11 @@ typedef struct lotus_hip_frame_result {
    * frame_gpu_ms. */
   double temporal_surface_gpu_ms;
   double svgf_gpu_ms;
+  /* Batched frames (lotus_hip_render_batch): 0 eager; with
+   * LOTUS_HIP_BATCH_GRAPH, 1 when this frame's graph was instantiated, 2
+   * when the last batch's executable graph of the same views took this
+   * frame's parameters in place (hipGraphExecUpdate). */
+  uint32_t batch_graph;
 } lotus_hip_frame_result;
 
 /* ProbeVulkanBackend (vulkan_backend.hpp:742): PASS when a gfx950 device is

@@ -26,31 +26,44 @@ def _renderers(vs):
     return out
 
 
-def _kw(v, sample_index=0):
-    return dict(world_to_clip=v["world_to_clip"], width=v["size"], height=v["size"], max_bounces=v["depth"],
+def _kw(v, sample_index=0, nudge=0.0):
+    w2c = np.array(v["world_to_clip"], np.float32).copy()
+    w2c[12] += np.float32(nudge)  # a camera moved sideways
+    return dict(world_to_clip=w2c, width=v["size"], height=v["size"], max_bounces=v["depth"],
                 sample_index=sample_index, light_sampling=hip.LIGHT_MIS)
 
 
 def test_batched_views_match_unbatched():
     vs = _views()
     alone, eager, graph = _renderers(vs), _renderers(vs), _renderers(vs)
+    kinds = []
     try:
-        for frame in range(4):
-            ref = [r.render(**_kw(v)) for r, v in zip(alone, vs)]
-            out_e, ms_e = hip.render_batch(eager, [_kw(v) for v in vs], graph=False)
-            # graph replays on frames 0, 2, 3 with an eager frame between
-            out_g, ms_g = hip.render_batch(graph, [_kw(v) for v in vs], graph=frame != 1)
-            print(f"frame {frame}: eager {ms_e:.3f} ms, {'graph' if frame != 1 else 'eager'} {ms_g:.3f} ms", flush=True)
+        for frame in range(8):
+            # the cameras move at frame 5: every accumulation starts over
+            nudge = 0.01 if frame >= 5 else 0.0
+            ref = [r.render(**_kw(v, nudge=nudge)) for r, v in zip(alone, vs)]
+            out_e, ms_e = hip.render_batch(eager, [_kw(v, nudge=nudge) for v in vs], graph=False)
+            # graph frames on 0, 2, 3, ... with an eager frame between: the
+            # first is captured and instantiated (batch_graph 1), the next
+            # replay it with their frame variables (3); the moved cameras
+            # change kernel arguments, so that frame is captured and updates
+            # the executable graph in place (2), and later ones replay it
+            out_g, ms_g = hip.render_batch(graph, [_kw(v, nudge=nudge) for v in vs], graph=frame != 1)
+            kinds.append(out_g[0]["batch_graph"])
+            assert all(o["batch_graph"] == kinds[-1] for o in out_g)
+            print(f"frame {frame}: eager {ms_e:.3f} ms, {'graph' if frame != 1 else 'eager'} {ms_g:.3f} ms "
+                  f"(graph kind {kinds[-1]})", flush=True)
             for i, v in enumerate(vs):
                 a, _ = alone[i].accumulation(v["size"], v["size"])
                 b, nb = eager[i].accumulation(v["size"], v["size"])
                 c, nc = graph[i].accumulation(v["size"], v["size"])
-                assert nb == nc == frame + 1
+                assert nb == nc == (frame + 1 if frame < 5 else frame - 4)
                 assert np.array_equal(a, b), (frame, i)
                 assert np.array_equal(a, c), (frame, i)
                 assert np.array_equal(ref[i]["color"], out_e[i]["color"])
                 assert np.array_equal(out_e[i]["counts"], ref[i]["counts"])
                 assert out_g[i]["shadow_rays"] == ref[i]["shadow_rays"]
+        assert kinds == [1, 0, 3, 3, 3, 2, 3, 3], kinds
     finally:
         for r in alone + eager + graph:
             r.close()
