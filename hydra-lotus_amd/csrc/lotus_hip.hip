@@ -217,6 +217,7 @@ struct lotus_hip_renderer {
   bool has_scene = false;
   uint64_t scene_generation = 0;  // any change: restarts the accumulation
   uint64_t scene_epoch = 0;       // a change temporal history cannot carry over
+  uint64_t emissive_triangles = 0;  // the light table's triangle entries
   LightTableHost light_table;     // as uploaded
   // The applied plan, to find what the next one changes.
   struct PlanRecord {
@@ -281,6 +282,13 @@ struct lotus_hip_renderer {
     uint32_t data[4] = {};
     uint32_t light_sampling = 0, max_bounces = 0, candidates = 0, proposal = 0;
     bool reservoirs = false, radiance = false;
+    // the windows as the caller gave them (TemporalHistory compares these)
+    float raw_display[4] = {};
+    int32_t raw_data[4] = {};
+    // the last fresh batch's transport settings (TemporalTransportKey,
+    // vulkan_backend.cpp:295-310); transport_valid: the last frame had one
+    std::array<uint32_t, 14> transport{};
+    bool transport_valid = false;
   } temporal;
   // A batched frame's sample whose counters are read after the batch.
   struct Deferred {
@@ -1515,6 +1523,10 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   }
   set_view(r, sh, p->environment);
   r->light_table = std::move(table);
+  r->emissive_triangles = static_cast<uint64_t>(
+      std::count_if(r->light_table.lights.begin(), r->light_table.lights.end(), [](const LightRecord& l) {
+        return l.instance != kEnvironmentLight && (l.instance & kAnalyticLight) == 0;
+      }));
   r->has_scene = true;
   ++r->bvh_builds;
   ++r->material_writes;
@@ -1592,6 +1604,10 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
   if (lights_changed) ++r->light_writes;
   r->uploaded_bytes += bytes;
   r->light_table = std::move(table);
+  r->emissive_triangles = static_cast<uint64_t>(
+      std::count_if(r->light_table.lights.begin(), r->light_table.lights.end(), [](const LightRecord& l) {
+        return l.instance != kEnvironmentLight && (l.instance & kAnalyticLight) == 0;
+      }));
   set_view(r, sh, p->environment);
   return true;
 }
@@ -1773,6 +1789,37 @@ int lotus_hip_render(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lot
 }  // extern "C"
 
 namespace {
+// DisplayViewport and DataScissor (vulkan_backend.cpp:651-677): a display
+// window of positive size, else the whole image; the data window clipped to
+// the image (an empty one renders nothing), or the whole image when its size
+// is not positive.
+void resolved_windows(const lotus_hip_frame_params* P, float* display, uint32_t* data) {
+  const float* dw = P->display_window;
+  const bool whole = dw[2] <= 0.0f || dw[3] <= 0.0f;
+  display[0] = whole ? 0.0f : dw[0];
+  display[1] = whole ? 0.0f : dw[1];
+  display[2] = whole ? static_cast<float>(P->width) : dw[2];
+  display[3] = whole ? static_cast<float>(P->height) : dw[3];
+  data[0] = 0;
+  data[1] = 0;
+  data[2] = P->width;
+  data[3] = P->height;
+  const int32_t* w = P->data_window;
+  if (w[2] > 0 && w[3] > 0) {
+    const int64_t x0 = std::max<int64_t>(w[0], 0), y0 = std::max<int64_t>(w[1], 0);
+    const int64_t x1 = std::min<int64_t>(int64_t{w[0]} + w[2], P->width);
+    const int64_t y1 = std::min<int64_t>(int64_t{w[1]} + w[3], P->height);
+    if (x1 <= x0 || y1 <= y0) {
+      data[0] = data[1] = data[2] = data[3] = 0;
+    } else {
+      data[0] = static_cast<uint32_t>(x0);
+      data[1] = static_cast<uint32_t>(y0);
+      data[2] = static_cast<uint32_t>(x1 - x0);
+      data[3] = static_cast<uint32_t>(y1 - y0);
+    }
+  }
+}
+
 // AccumulationKey and restart (vulkan_backend.cpp:419-434, 1186-1205): the
 // accumulation starts over when anything it depends on changed.
 void restart_if_changed(lotus_hip_renderer* r, const lotus_hip_frame_params* P, const float* vk_world_to_clip,
@@ -1852,23 +1899,15 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
     return LOTUS_HIP_FAIL;
   }
   (void)hipSetDevice(r->device);
-  // DisplayViewport / DataScissor: empty windows mean the whole image.
-  const bool whole_display = P->display_window[2] == 0.0f || P->display_window[3] == 0.0f;
-  const float vx = whole_display ? 0.0f : P->display_window[0], vy = whole_display ? 0.0f : P->display_window[1];
-  const float vw = whole_display ? static_cast<float>(P->width) : P->display_window[2];
-  const float vh = whole_display ? static_cast<float>(P->height) : P->display_window[3];
+  // DisplayViewport / DataScissor, and ndc_from_pixel (vulkan_backend.cpp:2408-2411)
+  float viewport[4];
+  uint32_t rect[4];
+  resolved_windows(P, viewport, rect);
+  const float vx = viewport[0], vy = viewport[1], vw = viewport[2], vh = viewport[3];
   F.ndc_from_pixel[0] = 2.0f / vw;
   F.ndc_from_pixel[1] = 2.0f / vh;
   F.ndc_from_pixel[2] = -2.0f * vx / vw - 1.0f;
   F.ndc_from_pixel[3] = -2.0f * vy / vh - 1.0f;
-  uint32_t rect[4] = {0, 0, P->width, P->height};
-  if (P->data_window[2] != 0 && P->data_window[3] != 0) {
-    const uint32_t x0 = std::min(P->data_window[0], P->width), y0 = std::min(P->data_window[1], P->height);
-    rect[0] = x0;
-    rect[1] = y0;
-    rect[2] = std::min(P->data_window[2], P->width - x0);
-    rect[3] = std::min(P->data_window[3], P->height - y0);
-  }
   std::memcpy(F.rect, rect, sizeof rect);
   std::memcpy(F.background, P->clear_color, 16);
   F.width = P->width;
@@ -2546,23 +2585,6 @@ int lotus_hip_cache_parameters(lotus_hip_renderer* r, float* weights, float* gri
 
 namespace {
 
-void resolved_windows(const lotus_hip_frame_params* P, float* display, uint32_t* data) {
-  const bool whole = P->display_window[2] == 0.0f || P->display_window[3] == 0.0f;
-  display[0] = whole ? 0.0f : P->display_window[0];
-  display[1] = whole ? 0.0f : P->display_window[1];
-  display[2] = whole ? static_cast<float>(P->width) : P->display_window[2];
-  display[3] = whole ? static_cast<float>(P->height) : P->display_window[3];
-  data[0] = 0;
-  data[1] = 0;
-  data[2] = P->width;
-  data[3] = P->height;
-  if (P->data_window[2] != 0 && P->data_window[3] != 0) {
-    data[0] = std::min(P->data_window[0], P->width);
-    data[1] = std::min(P->data_window[1], P->height);
-    data[2] = std::min(P->data_window[2], P->width - data[0]);
-    data[3] = std::min(P->data_window[3], P->height - data[1]);
-  }
-}
 
 // The camera's world-space position: the view matrix's inverse applied to
 // the origin, in double (Gauss-Jordan with partial pivoting).
@@ -2651,7 +2673,13 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
     return LOTUS_HIP_FAIL;
   }
   auto& t = r->temporal;
-  if (t.previous && !ts->camera_cut && ts->frame_index <= t.frame_index) {
+  if (P->width == 0 || P->height == 0 || !std::isfinite(P->display_window[0]) ||
+      !std::isfinite(P->display_window[1]) || !std::isfinite(P->display_window[2]) ||
+      !std::isfinite(P->display_window[3])) {
+    set_detail(detail, detail_size, "temporal frame extent must be nonempty");
+    return LOTUS_HIP_FAIL;
+  }
+  if (t.previous && ts->frame_index <= t.frame_index) {
     set_detail(detail, detail_size, "temporal frame indices must increase");
     return LOTUS_HIP_FAIL;
   }
@@ -2659,17 +2687,29 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   uint32_t reason = 0;
   if (!t.previous) reason = 1;                                    // FirstFrame
   else if (ts->camera_cut) reason = 3;                            // CameraCut
-  else if (ts->frame_index != t.frame_index + 1) reason = 4;      // FrameGap
-  else if (P->width != t.width || P->height != t.height) reason = 5;  // TargetChanged
+  else if (ts->frame_index - t.frame_index != 1) reason = 4;      // FrameGap
+  else if (P->width != t.width || P->height != t.height ||         // TargetChanged
+           std::memcmp(P->display_window, t.raw_display, 16) != 0 || std::memcmp(P->data_window, t.raw_data, 16) != 0)
+    reason = 5;
   else if (ts->history_key != t.history_key) reason = 6;          // HistoryKeyChanged
-  else if (P->light_sampling != t.light_sampling || P->max_bounces != t.max_bounces ||
-           (restir && (P->restir_candidates != t.candidates || P->restir_proposal != t.proposal)))
-    reason = 7;                                                   // RenderSettingsChanged
-  // SceneChanged (vulkan_backend.cpp:1782-1788): an incompatible scene change,
-  // or any change while temporal reservoirs reuse light samples of an emitting scene.
-  else if (r->scene_epoch != t.scene_epoch ||
-           (r->scene_generation != t.scene_generation && restir && ts->restir_temporal && r->view.light_count != 0))
-    reason = 8;
+  // vulkan_backend.cpp:1770-1789: a frame with a fresh batch drops the history
+  // its transport settings no longer match (RenderSettingsChanged), or a
+  // scene change temporal history cannot carry: an incompatible one, or any
+  // while reservoirs reuse light samples of emissive triangles (SceneChanged)
+  const bool reuse = restir && (ts->restir_temporal || ts->restir_spatial);
+  const bool fresh = ts->accumulate_radiance || reuse;
+  std::array<uint32_t, 14> transport{P->light_sampling, P->restir_proposal, P->restir_candidates,
+                                     ts->restir_temporal ? 1u : 0u, ts->restir_max_history_candidates,
+                                     ts->restir_spatial ? 1u : 0u, ts->restir_spatial_radius,
+                                     ts->accumulate_radiance ? 1u : 0u, P->sample_index, P->max_bounces};
+  std::memcpy(transport.data() + 10, P->clear_color, 16);
+  if (reason == 0 && fresh) {
+    if (!t.transport_valid || transport != t.transport)
+      reason = 7;
+    else if (r->scene_generation != t.scene_generation &&
+             (r->scene_epoch != t.scene_epoch || (reuse && r->emissive_triangles != 0)))
+      reason = 8;
+  }
   (void)hipSetDevice(r->device);
   hipError_t e = ensure_temporal(r, P->width, P->height);
   if (e != hipSuccess) {
@@ -2679,7 +2719,6 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   const bool available = reason == 0 && t.previous;
   const uint32_t cur = t.cur, prev = 1u - t.cur;
   const uint32_t rcur = t.rcur, rprev = 1u - t.rcur;
-  const bool fresh = ts->accumulate_radiance || (restir && (ts->restir_temporal || ts->restir_spatial));
   // vulkan_backend.cpp:1790-1795: settling needs nothing this frame depends
   // on to differ from the preceding frame, which left its colour as the
   // prior; reservoir reuse traces a fresh batch every frame and never settles
@@ -2785,6 +2824,10 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   std::memcpy(t.eye, T.current_eye, sizeof t.eye);
   vulkan_world_to_clip(P->world_to_clip, t.world_to_clip);
   resolved_windows(P, t.display, t.data);
+  std::memcpy(t.raw_display, P->display_window, 16);
+  std::memcpy(t.raw_data, P->data_window, 16);
+  t.transport = transport;
+  t.transport_valid = fresh;
   t.light_sampling = P->light_sampling;
   t.max_bounces = P->max_bounces;
   t.candidates = P->restir_candidates;

@@ -213,7 +213,7 @@ class FrameParams(C.Structure):
         ("width", C.c_uint32),
         ("height", C.c_uint32),
         ("display_window", C.c_float * 4),
-        ("data_window", C.c_uint32 * 4),
+        ("data_window", C.c_int32 * 4),
         ("clear_color", C.c_float * 4),
         ("frame_count", C.c_uint32),
         ("output", C.c_uint32),

@@ -217,8 +217,14 @@ typedef struct lotus_hip_scene_evidence {
 typedef struct lotus_hip_frame_params {
   float world_to_clip[16];    /* OpenGL clip space, column-major */
   uint32_t width, height;     /* target */
-  float display_window[4];    /* x, y, w, h; all zero = the whole image */
-  uint32_t data_window[4];    /* x, y, w, h; all zero = the whole image */
+  /* OffscreenTarget's windows (vulkan_backend.hpp:253-262, resolved as
+   * DisplayViewport and DataScissor, vulkan_backend.cpp:651-677): the
+   * display window (x, y, w, h) maps pixels to NDC, the whole image when w
+   * or h is not positive; the data window (x, y, w, h) is the pixels that
+   * render, clipped to the image (outside it the clear colour stays), the
+   * whole image when w or h is not positive. */
+  float display_window[4];
+  int32_t data_window[4];
   float clear_color[4];
   uint32_t frame_count;       /* samples this call adds */
   uint32_t output;            /* LOTUS_HIP_OUTPUT_* */

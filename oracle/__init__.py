@@ -179,6 +179,9 @@ def ref():
                                     _u32p]
         R.ref_material_fixture.restype = C.c_size_t
         R.ref_material_fixture.argtypes = [C.c_char_p, C.c_int, _u32p, C.c_size_t]
+        R.ref_temporal_advance.restype = None
+        R.ref_temporal_advance.argtypes = [C.c_uint32, C.POINTER(C.c_uint64), _u32p, _u32p, _f32p,
+                                           C.POINTER(C.c_int32), C.POINTER(C.c_uint64), _u32p]
         R.ref_settle.restype = None
         R.ref_settle.argtypes = [_f32p, _f32p, _u32p, C.c_size_t, C.c_uint32, C.c_uint32, _f32p]
         R.ref_reuse_normalization.restype = C.c_double
@@ -338,17 +341,21 @@ def camera_constants(world_to_clip_gl):
 
 
 def make_frame(world_to_clip_gl, width, height, spp=1, sample_index=0, max_bounces=8,
-               light_sampling=1, rect=None, restir_candidates=8, restir_proposal=0) -> Frame:
+               light_sampling=1, rect=None, restir_candidates=8, restir_proposal=0, display=None) -> Frame:
+    """`display`: the display window (x, y, w, h), the whole image when
+    None; `rect`: the data window as DataScissor resolves it."""
     _, inv = camera_constants(world_to_clip_gl)
     f = Frame()
     for i in range(16):
         f.clip_to_world[i] = float(inv[i])
-    # ndc = pixel * (2/w, 2/h) + (-1, -1) for a display window equal to the image
-    # (vulkan_backend.cpp:2408-2411 with viewport x = y = 0).
-    f.ndc_from_pixel[0] = np.float32(2.0) / np.float32(width)
-    f.ndc_from_pixel[1] = np.float32(2.0) / np.float32(height)
-    f.ndc_from_pixel[2] = -1.0
-    f.ndc_from_pixel[3] = -1.0
+    # ndc = pixel * (2/w, 2/h) + (-2x/w - 1, -2y/h - 1) over the display
+    # window (vulkan_backend.cpp:2408-2411), in float
+    F = np.float32
+    vx, vy, vw, vh = (F(0), F(0), F(width), F(height)) if display is None else tuple(F(v) for v in display)
+    f.ndc_from_pixel[0] = F(2.0) / vw
+    f.ndc_from_pixel[1] = F(2.0) / vh
+    f.ndc_from_pixel[2] = F(F(F(-2.0) * vx) / vw) - F(1.0)
+    f.ndc_from_pixel[3] = F(F(F(-2.0) * vy) / vh) - F(1.0)
     f.width, f.height = width, height
     r = rect or (0, 0, width, height)
     for i in range(4):

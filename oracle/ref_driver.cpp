@@ -353,6 +353,32 @@ void ref_reproject(const void* current, uint32_t x, uint32_t y, uint32_t width, 
   out[3] = r.reserved;
 }
 
+// temporal.cpp:114-147 (TemporalHistory::Advance) over n frames' settings:
+// each frame's TemporalResetReason, or 0xFFFFFFFF where Advance refused it.
+void ref_temporal_advance(uint32_t n, const uint64_t* frame_index, const uint32_t* camera_cut,
+                          const uint32_t* extent, const float* display, const int32_t* data,
+                          const uint64_t* history_key, uint32_t* reasons) {
+  Lotus::TemporalHistory history;
+  const Lotus::FrameSnapshot snapshot{};
+  for (uint32_t i = 0; i < n; ++i) {
+    Lotus::TemporalFrameSettings s;
+    s.frame_index = frame_index[i];
+    s.camera_cut = camera_cut[i] != 0;
+    s.width = extent[2 * i];
+    s.height = extent[2 * i + 1];
+    for (int k = 0; k < 4; ++k) {
+      s.display_window[k] = display[4 * i + k];
+      s.data_window[k] = data[4 * i + k];
+    }
+    s.history_key = history_key[i];
+    try {
+      reasons[i] = static_cast<uint32_t>(history.Advance(snapshot, s).reset_reason);
+    } catch (const std::exception&) {
+      reasons[i] = 0xFFFFFFFFu;
+    }
+  }
+}
+
 // temporal.cpp:514-528 (SettleTemporalRadiance) over `count` pixels: prior
 // and still colours (4 floats each), each pixel's prior sample count.
 void ref_settle(const float* prior, const float* still, const uint32_t* prior_samples, size_t count,

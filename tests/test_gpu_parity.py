@@ -372,3 +372,54 @@ def test_tuning_does_not_change_results(renderer):
         assert np.array_equal(o["counts"], outs[0]["counts"])
         assert np.array_equal(o["color"], outs[0]["color"])
     assert outs[2]["rounds"] <= outs[0]["rounds"] <= outs[1]["rounds"] and outs[2]["rounds"] < outs[1]["rounds"]
+
+
+def _data_scissor(data, w, h):
+    # DataScissor (vulkan_backend.cpp:660-677)
+    if data[2] <= 0 or data[3] <= 0:
+        return (0, 0, w, h)
+    x0, y0 = max(data[0], 0), max(data[1], 0)
+    x1, y1 = min(data[0] + data[2], w), min(data[1] + data[3], h)
+    return (0, 0, 0, 0) if x1 <= x0 or y1 <= y0 else (x0, y0, x1 - x0, y1 - y0)
+
+
+@pytest.mark.parametrize("display,data", [
+    ((-8.5, 4.25, 80.0, 60.0), (5, 7, 40, 30)),   # an offset, scaled display window; an inner data window
+    ((0.0, 0.0, 0.0, 0.0), (50, -4, 40, 30)),     # the data window clipped at the right and the top
+    ((3.0, -2.0, 58.0, 44.0), (0, 0, -1, 5)),     # no data window (a size that is not positive)
+    ((0.0, 0.0, 0.0, 0.0), (100, 0, 10, 10)),     # a data window outside the image: nothing renders
+])
+def test_display_and_data_windows(renderer, display, data):
+    # OffscreenTarget's windows (DisplayViewport, DataScissor, the
+    # ndc_from_pixel of vulkan_backend.cpp:2408-2411) against the oracle
+    # given the same windows: counts and shadow rays exact, one-sample
+    # radiance bit-exact inside the data window, the clear colour outside.
+    _say(f"windows display={display} data={data}")
+    sc = scenes.cornell_lambert()
+    renderer.apply(sc)
+    w, h = 64, 48
+    clear = (0.25, 0.5, 0.75, 1.0)
+    out = renderer.render(sc.world_to_clip, w, h, frame_count=1, sample_index=3, max_bounces=6,
+                          light_sampling=hip.LIGHT_MIS, display_window=display, data_window=data, clear_color=clear)
+    rect = _data_scissor(data, w, h)
+    inside = np.zeros((h, w), bool)
+    inside[rect[1]:rect[1] + rect[3], rect[0]:rect[0] + rect[2]] = True
+    color = out["color"]
+    assert np.all(color[~inside] == np.array(clear, np.float32))
+    if rect[2] == 0:
+        assert out["camera_rays"] == 0 and len(out["counts"]) == 0
+        return
+    assert out["camera_rays"] == rect[2] * rect[3]
+    disp = None if display[2] <= 0 or display[3] <= 0 else display
+    fr = oracle.make_frame(sc.world_to_clip, w, h, spp=1, sample_index=3, max_bounces=6, light_sampling=1, rect=rect,
+                           display=disp)
+    acc, counts, shadow = oracle.OracleScene(sc).render(fr)
+    n = len(out["counts"])
+    assert np.array_equal(out["counts"], counts[0][:n]) and not counts[0][n:].any(), (out["counts"], counts[0])
+    assert out["shadow_rays"] == shadow[0]
+    hit = inside & (acc[..., 3] > 0)
+    miss = inside & (acc[..., 3] == 0)
+    assert np.all(color[miss] == np.array(clear, np.float32))
+    exact = np.mean(np.all(color[hit][:, :3] == acc[hit][:, :3].astype(np.float32), -1))
+    _say(f"  hits {hit.sum()}, misses {miss.sum()}, bit-exact {exact:.6f}")
+    assert hit.sum() > 100 and exact > 0.999

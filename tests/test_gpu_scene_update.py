@@ -245,30 +245,31 @@ def test_topology_changes_rebuild(renderer):
 def test_temporal_history_across_scene_updates(renderer):
     # TemporalSceneCompatible (vulkan_backend.cpp:262-289): moving a block that
     # does not emit keeps the history; moving the light or editing a material
-    # drops it with SceneChanged (8).
+    # drops it with SceneChanged (8), for frames with a fresh batch (radiance
+    # accumulation here, vulkan_backend.cpp:1778-1789).
     _say("temporal history across updates")
     sc = scenes.cornell_lambert()
     renderer.apply(sc)
     w, h = 32, 24
     f = 5000
-    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f, camera_cut=True, max_bounces=3)
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f, camera_cut=True, max_bounces=3, accumulate=True)
     moved = sc.copy()
     t = scenes.block((0.6, 0.6, 0.6), -10.0, (0.38, -0.699, 0.3))
     moved.set_transform(6, t)
     assert renderer.apply(moved).update == hip.SCENE_REFIT
-    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 1, max_bounces=3)
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 1, max_bounces=3, accumulate=True)
     assert out["reset_reason"] == 0 and out["history_available"]
     light = moved.copy()
     lt = scenes.identity()
     lt[12] = 0.1
     light.set_transform(4, lt)
     assert renderer.apply(light).update == hip.SCENE_REFIT
-    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 2, max_bounces=3)
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 2, max_bounces=3, accumulate=True)
     assert out["reset_reason"] == 8
-    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 3, max_bounces=3)
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 3, max_bounces=3, accumulate=True)
     assert out["reset_reason"] == 0
     mat = light.copy()
     mat.set_material(1, scenes.preview_material(base_color=(0.3, 0.3, 0.9), ior=1.0))
     assert renderer.apply(mat).update == hip.SCENE_TABLES
-    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 4, max_bounces=3)
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 4, max_bounces=3, accumulate=True)
     assert out["reset_reason"] == 8

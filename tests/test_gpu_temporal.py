@@ -445,3 +445,76 @@ def test_settling_sequence(renderer):
             renderer.render_temporal(w2c, view, w, h, frame_index=next(frames), settle=True, max_bounces=1)
     finally:
         progressive.close()
+
+
+def test_reset_reasons_follow_the_reference(renderer):
+    # TemporalHistory::Advance (temporal.cpp:114-147, built from the
+    # reference into oracle/_ref) decides first frame, camera cut, frame
+    # gap, target (extent and both windows) and history key; the renderer's
+    # reasons over the same sequence equal its, including the frames it
+    # refuses (a non-increasing index, an empty extent) without touching the
+    # history. Then vulkan_backend.cpp:1770-1789: with a fresh batch, a
+    # change of transport settings (bounces, the sample index, the clear
+    # colour) is RenderSettingsChanged; without one (surfaces only) the
+    # history carries over them.
+    import ctypes as C
+    R = oracle.ref()
+    sc = _cornell(renderer)
+    seq = [  # frame index, cut, width, height, display window, data window, history key
+        (40000, 0, 32, 24, (0, 0, 0, 0), (0, 0, 0, 0), 0),
+        (40001, 0, 32, 24, (0, 0, 0, 0), (0, 0, 0, 0), 0),
+        (40002, 1, 32, 24, (0, 0, 0, 0), (0, 0, 0, 0), 0),      # cut
+        (40004, 0, 32, 24, (0, 0, 0, 0), (0, 0, 0, 0), 0),      # gap
+        (40005, 0, 32, 24, (0, 0, 0, 0), (0, 0, 0, 0), 0),
+        (40006, 0, 40, 24, (0, 0, 0, 0), (0, 0, 0, 0), 0),      # extent
+        (40007, 0, 40, 24, (1.5, 0, 40, 24), (0, 0, 0, 0), 0),  # display window
+        (40008, 0, 40, 24, (1.5, 0, 40, 24), (2, 3, 30, 20), 0),  # data window
+        (40009, 0, 40, 24, (1.5, 0, 40, 24), (2, 3, 30, 20), 7),  # history key
+        (40009, 0, 40, 24, (1.5, 0, 40, 24), (2, 3, 30, 20), 7),  # not increasing: refused
+        (40010, 0, 0, 24, (1.5, 0, 40, 24), (2, 3, 30, 20), 7),   # empty: refused
+        (40010, 0, 40, 24, (1.5, 0, 40, 24), (2, 3, 30, 20), 7),
+        (40011, 1, 40, 24, (0, 0, 0, 0), (0, 0, 0, 0), 7),        # a cut with a changed target: the cut
+    ]
+    n = len(seq)
+    fi = np.array([s[0] for s in seq], np.uint64)
+    cut = np.array([s[1] for s in seq], np.uint32)
+    ext = np.array([(s[2], s[3]) for s in seq], np.uint32)
+    disp = np.array([s[4] for s in seq], np.float32)
+    data = np.array([s[5] for s in seq], np.int32)
+    key = np.array([s[6] for s in seq], np.uint64)
+    want = np.zeros(n, np.uint32)
+    R.ref_temporal_advance(n, fi.ctypes.data_as(C.POINTER(C.c_uint64)), oracle.ptr(cut, C.c_uint32),
+                           oracle.ptr(ext, C.c_uint32), oracle.ptr(disp, C.c_float),
+                           data.ctypes.data_as(C.POINTER(C.c_int32)), key.ctypes.data_as(C.POINTER(C.c_uint64)),
+                           oracle.ptr(want, C.c_uint32))
+    renderer.render_temporal(sc.world_to_clip, sc.view, 8, 8, frame_index=39990, camera_cut=True, max_bounces=2)
+    got = []
+    for k, (f, c, w, h, dw, dd, hk) in enumerate(seq):
+        try:
+            out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f, camera_cut=bool(c),
+                                           history_key=hk, display_window=dw, data_window=dd, max_bounces=2)
+            got.append(out["reset_reason"])
+        except hip.LotusError as err:
+            assert ("must increase" in str(err)) or ("extent must be nonempty" in str(err)), err
+            got.append(0xFFFFFFFF)
+    # the renderer's own history before the sequence makes its first frame a
+    # continuation or a gap, not a first frame: compare from the second on
+    print("reference", list(want), "renderer", got)
+    assert want[0] == 1 and got[0] == 4
+    assert list(want[1:]) == got[1:]
+    assert set(want) >= {0, 1, 3, 4, 5, 6, 0xFFFFFFFF}
+    # transport settings: only frames with a fresh batch drop history over them
+    kw = dict(history_key=7, max_bounces=2)
+    f = 40100
+    renderer.render_temporal(sc.world_to_clip, sc.view, 32, 24, frame_index=f, camera_cut=True, accumulate=True, **kw)
+    for change, expect in ((dict(), 0), (dict(max_bounces=3), 7), (dict(sample_index=9), 7),
+                           (dict(clear_color=(0.5, 0.0, 0.0, 1.0)), 7), (dict(), 0)):
+        f += 1
+        args = dict(kw, **change)
+        out = renderer.render_temporal(sc.world_to_clip, sc.view, 32, 24, frame_index=f, accumulate=True, **args)
+        assert out["reset_reason"] == expect, (change, out["reset_reason"])
+        f += 1  # back to the settings before the change (itself a change when the last was one)
+        renderer.render_temporal(sc.world_to_clip, sc.view, 32, 24, frame_index=f, accumulate=True, **kw)
+    f += 1
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, 32, 24, frame_index=f, **dict(kw, max_bounces=5))
+    assert out["reset_reason"] == 0  # no fresh batch: the surfaces' history carries over
