@@ -74,7 +74,8 @@ struct SceneView {
   uint32_t tri_count;
   uint32_t instance_count;
   uint32_t analytic_count, light_node_count, distant_count;
-  uint32_t parallel;                // the parallel light's index + 1 (BSDF sampling traces it), 0 without one
+  uint32_t parallel;                // the first parallel light's index + 1 (BSDF sampling traces it), 0 without one
+  uint32_t parallel_count;          // parallel lights (more than one: BSDF sampling walks them inline)
   float environment[3];
   // The environment image (env_width 0: none): texels, then its sampling
   // distribution's floats (cell densities, the rows' cumulative

@@ -686,15 +686,64 @@ __device__ __forceinline__ void cache_query(const FrameConst& F, const Surface& 
   out[3] = make_float4(sf.base_color.z, sf.metallic, 0.0f, 0.0f);
 }
 
+// Under BSDF sampling with several parallel lights, ParallelLights
+// (transport.slang:139-146) at the shading lane itself: each parallel light's
+// ShadowedLight (:82-115) with a shadow ray the lane walks at once, summed in
+// light order (a blocked or non-finite term adds nothing), the rays counted
+// in parallel_ray_count.
+template <bool kCov>
+struct ParallelWalk {
+  static constexpr bool kEnabled = true;
+  TraceStack* st;
+  uint32_t* rays;
+  __device__ V3 operator()(const SceneView& S, const Surface& sf, const V3& ft, const V3& fb, const V3& wo) const {
+    const V3 n = sf.shading_normal;
+    const V3 origin = offset_ray_origin(sf.position, sf.normal);
+    V3 sum{0.0f, 0.0f, 0.0f};
+    uint32_t traced = 0;
+    for (uint32_t a = 0; a < S.analytic_count; ++a) {
+      const AnalyticRecord& L = S.analytic[a];
+      if (L.shape != kParallelLight) continue;  // ParallelLight gives NoLight for the others
+      const V3 dir{L.axis_u[0], L.axis_u[1], L.axis_u[2]}, rad{L.radiance[0], L.radiance[1], L.radiance[2]};
+      if (max3(rad) <= 0.0f || dot3(dir, sf.normal) <= 0.0f) continue;
+      const V3 wi{dot3(dir, ft), dot3(dir, fb), dot3(dir, n)};
+      if (wi.z <= 0.0f) continue;
+      const V3 f = bsdf_evaluate(sf.bsdf, wo, wi);
+      if (max3(f) <= 0.0f) continue;
+      ++traced;
+      uint32_t visits = 0, tests = 0;
+      const float t = walk_transmittance(S.nodes, S.tris, *st, Ray{origin, 0.0f, dir, kFloatMax}, visits, tests,
+                                         coverage_of<kCov>(S));
+      if (t <= 0.0f) continue;
+      // f * radiance * (transmittance * weight / pdf), weight and pdf 1
+      const V3 c = f * rad * ((t * 1.0f) / 1.0f);
+      if (finite3(c)) sum = sum + c;
+    }
+    if (traced != 0) atomicAdd(rays, traced);
+    return sum;
+  }
+};
+
 // wavefront.slang:264-282: one scattering event per queued hit. The light a
 // shadow ray of the previous bounce found is added first. In radiance cache
 // frames a path other than a training path ends at bounce F.cache_depth after
 // its emission term, leaving a cache query with its throughput, and a
 // training path records each vertex (query, throughput on arrival, radiance
 // including the vertex's emission).
-template <bool kAnalytic, bool kCov>  // the scene has analytic lights (extras); covered surfaces
+// kParallel: BSDF sampling walks several parallel lights (ParallelWalk), with
+// traversal stacks in LDS.
+template <bool kAnalytic, bool kCov, bool kParallel = false>  // analytic lights (extras); covered surfaces
 __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                       uint32_t bounce, uint32_t link) {
+                                                       uint32_t bounce, uint32_t link, uint2* spill, int spill_cap) {
+  using Walk = std::conditional_t<kParallel, ParallelWalk<kCov>, NoParallelWalk>;
+  Walk walk{};
+  __shared__ uint2 ring[kParallel ? kRing * kTraceBlock : 1];
+  TraceStack st;
+  if constexpr (kParallel) {
+    st.init(ring, spill, spill_cap);
+    walk.st = &st;
+    walk.rays = parallel_ray_count(counters);
+  }
   const uint32_t count = *hit_count(counters, bounce);
   for (uint32_t i = gtid(); i < count; i += gstride()) {
     const float4 o4 = q.hit_o[i], d4 = q.hit_d[i];
@@ -738,8 +787,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
       s.continues = false;
       s.shadow = false;
     } else {
-      s = scatter_path<kAnalytic, kCov>(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z), __uint_as_float(h4.w),
-                       train ? &sf : nullptr, slot, kCov ? &factors : nullptr);
+      s = scatter_path<kAnalytic, kCov, Walk>(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z),
+                                              __uint_as_float(h4.w), train ? &sf : nullptr, slot,
+                                              kCov ? &factors : nullptr, walk);
     }
     const uint32_t kq = wave_append(cache_query_count(counters), query);
     if (query) {
@@ -916,13 +966,22 @@ __device__ __forceinline__ float pair_float(float v) { return __shfl_xor(v, 1); 
 
 // a radiance cache frame (training vertices recorded); analytic lights
 // (extras); covered surfaces
-template <bool kCache, bool kAnalytic, bool kCov>
+// (extras); covered surfaces; BSDF sampling walks several parallel lights
+// (ParallelWalk, on the stack the lane's walks use, which is free while it
+// shades)
+template <bool kCache, bool kAnalytic, bool kCov, bool kParallel = false>
 __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                       uint2* spill, int spill_cap, uint32_t bounce,
                                                       unsigned long long* stats) {
   __shared__ uint2 ring[kRing * kTraceBlock];
   TraceStack st;
   st.init(ring, spill, spill_cap);
+  using Walk = std::conditional_t<kParallel, ParallelWalk<kCov>, NoParallelWalk>;
+  Walk walk{};
+  if constexpr (kParallel) {
+    walk.st = &st;
+    walk.rays = parallel_ray_count(counters);
+  }
   const uint32_t count = *ray_count(counters, bounce);
   const uint32_t n_shadow = *shadow_count(counters, bounce - 1);  // the last round's, linked or on their own
   const bool path_lane = (threadIdx.x & 1u) == 0;
@@ -1054,8 +1113,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
           const V3 arrival = p.throughput, dir = p.d;
           const uint32_t b = p.depth;
           Surface sf;
-          sc = scatter_path<kAnalytic, kCov>(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v, kCache && train ? &sf : nullptr,
-                            slot, kCov ? &sc4 : nullptr);
+          sc = scatter_path<kAnalytic, kCov, Walk>(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v,
+                                                   kCache && train ? &sf : nullptr, slot, kCov ? &sc4 : nullptr, walk);
           if (kCache && train && b < kCacheRecordDepth && path < F.train_capacity) {
             float4* rec = q.crec + 6ull * (static_cast<size_t>(b) * F.train_capacity + path);
             cache_query(F, sf, dir, rec);
@@ -1229,9 +1288,11 @@ GridShape grid_shape(uint32_t cus) {
   g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
   g.rays = persistent_blocks(k_rays, kTraceBlock, cus);
   g.shade = persistent_blocks(k_shade<false, false>, kShadeBlock, cus);
+  g.shade_parallel = persistent_blocks(k_shade<true, false, true>, kShadeBlock, cus);
   g.resolve = cus * 4;
   g.max_trace_lanes =
-      std::max({g.camera, g.camera_dynamic, g.trace, g.trace_dynamic, g.tail, g.tail_cache, g.camera_pass, g.rays}) * kTraceBlock;
+      std::max({g.camera, g.camera_dynamic, g.trace, g.trace_dynamic, g.tail, g.tail_cache, g.camera_pass, g.rays,
+                g.shade_parallel}) * kTraceBlock;
   return g;
 }
 
@@ -1252,12 +1313,21 @@ void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, con
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce, bool link) {
-  if (S.covered_instances != 0)
-    k_shade<true, true><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
-  else if (scene_has_extras(S))
-    k_shade<true, false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
-  else
-    k_shade<false, false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, link ? 1u : 0u);
+  const uint32_t l = link ? 1u : 0u;
+  if (parallel_walks(S, F)) {
+    if (S.covered_instances != 0)
+      k_shade<true, true, true><<<L.grid.shade_parallel, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, L.spill,
+                                                                                     L.spill_cap);
+    else
+      k_shade<true, false, true><<<L.grid.shade_parallel, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, L.spill,
+                                                                                      L.spill_cap);
+  } else if (S.covered_instances != 0) {
+    k_shade<true, true><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, nullptr, 0);
+  } else if (scene_has_extras(S)) {
+    k_shade<true, false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, nullptr, 0);
+  } else {
+    k_shade<false, false><<<L.grid.shade, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, nullptr, 0);
+  }
 }
 void launch_trace(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce,
                   uint32_t kinds) {
@@ -1282,7 +1352,20 @@ void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const
                  uint32_t bounce) {
   const bool analytic = scene_has_extras(S), cov = S.covered_instances != 0;
   const uint32_t grid = F.cache_depth != 0 ? L.grid.tail_cache : L.grid.tail;
-  if (F.cache_depth != 0 && cov)
+  if (parallel_walks(S, F)) {
+    if (F.cache_depth != 0 && cov)
+      k_tail<true, true, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
+                                                                         L.stats);
+    else if (F.cache_depth != 0)
+      k_tail<true, true, false, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
+                                                                          L.stats);
+    else if (cov)
+      k_tail<false, true, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
+                                                                          L.stats);
+    else
+      k_tail<false, true, false, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
+                                                                           L.stats);
+  } else if (F.cache_depth != 0 && cov)
     k_tail<true, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
   else if (F.cache_depth != 0 && analytic)
     k_tail<true, true, false><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);

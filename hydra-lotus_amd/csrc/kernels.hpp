@@ -63,10 +63,18 @@ constexpr float kCacheMinThroughput = 1e-3f;
 // occupancy (a grid sized for another variant's occupancy would leave
 // workgroups waiting for a CU, and a strided launch then ends a pass late).
 struct GridShape {
-  uint32_t camera, camera_dynamic, trace, trace_dynamic, tail, tail_cache, camera_pass, rays, shade, resolve;
+  uint32_t camera, camera_dynamic, trace, trace_dynamic, tail, tail_cache, camera_pass, rays, shade, shade_parallel,
+      resolve;
   uint32_t max_trace_lanes;  // the most traversal lanes any grid holds
 };
 GridShape grid_shape(uint32_t cus);
+
+// BSDF sampling in a scene with several parallel lights: the shading lanes
+// walk every parallel light's shadow ray themselves (the kernels' kParallel
+// variants); with one, it is queued as next event estimation's ray is.
+inline bool parallel_walks(const SceneView& S, const FrameConst& F) {
+  return F.light_sampling == 0 && S.parallel_count > 1;
+}
 
 struct Launch {
   hipStream_t stream;
@@ -155,6 +163,12 @@ __host__ __device__ inline uint32_t* cache_query_count(uint32_t* c) { return c +
 __host__ __device__ inline uint32_t* cache_sample_count(uint32_t* c) { return c + counter_word(1, 3); }
 // ReSTIR visibility rays traced inline by the reservoir passes (not queued).
 __host__ __device__ inline uint32_t* restir_ray_count(uint32_t* c) { return c + counter_word(2, 3); }
+// Shadow rays the shading lanes walk themselves (several parallel lights
+// under BSDF sampling): the second half of the camera-ray counter's line,
+// whose only other atomics come from ray generation, before any shading.
+__host__ __device__ inline uint32_t* parallel_ray_count(uint32_t* c) {
+  return c + counter_word(0, 0) + kCounterSpread / 2;
+}
 // Extension rays of bounce b (b >= 1) that escaped a scene with analytic
 // lights or an environment image: kind 3 of bounce b + 2.
 __host__ __device__ inline uint32_t* escape_count(uint32_t* c, uint32_t bounce) { return c + counter_word(bounce + 2, 3); }

@@ -217,6 +217,10 @@ struct lotus_hip_renderer {
   bool has_scene = false;
   uint64_t scene_generation = 0;  // any change: restarts the accumulation
   uint64_t scene_epoch = 0;       // a change temporal history cannot carry over
+  // The caller's per-instance temporal identities copied to the device:
+  // previous_from_current (16 floats), instance and material ids, flags.
+  unsigned char* temporal_ids = nullptr;
+  size_t temporal_ids_bytes = 0;
   uint64_t emissive_triangles = 0;  // the light table's triangle entries
   LightTableHost light_table;     // as uploaded
   // The applied plan, to find what the next one changes.
@@ -1307,6 +1311,7 @@ void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environme
   v.light_node_count = s.light_node_count;
   v.distant_count = s.distant_count;
   v.parallel = s.parallel;
+  v.parallel_count = s.parallel_count;
   v.env_texels = s.env_texels;
   v.env_dist = s.env_dist;
   v.env_width = s.env_width;
@@ -1720,6 +1725,7 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   free_scene(r);
   cache_free(r->cache);
   free_temporal(r);
+  dev_free(r->temporal_ids);
   dev_free(r->svgf_filter);
   dev_free(r->counters);
   dev_free(r->stats);
@@ -1865,10 +1871,6 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
   }
   if (P->light_sampling > LOTUS_HIP_LIGHT_RESTIR) {
     set_detail(detail, detail_size, "unknown light sampling");
-    return LOTUS_HIP_FAIL;
-  }
-  if (P->light_sampling == LOTUS_HIP_LIGHT_BSDF && r->scene.parallel_count > 1) {
-    set_detail(detail, detail_size, "BSDF sampling traces at most one parallel light");
     return LOTUS_HIP_FAIL;
   }
   if (P->light_sampling == LOTUS_HIP_LIGHT_RESTIR && (P->restir_candidates == 0 || P->restir_candidates > 64)) {
@@ -2152,7 +2154,7 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
           if (b + 1 < rounds || (b < rounds && rounds >= bounces)) round_shadow_rays += host[counter_word(b, 2)];
         }
         camera_rays += host[0];
-        shadow_rays += *restir_ray_count(host.data());
+        shadow_rays += *restir_ray_count(host.data()) + *parallel_ray_count(host.data());
         restir_rays += *restir_ray_count(host.data());
         counts.resize(host[0] ? last + 1 : 0);
         r->rounds_hint = wavefront_rounds(counts, r->tuning.tail_divisor);
@@ -2752,7 +2754,36 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
   T.rectification = ts->history_rectification;
   camera_eye(ts->view, T.current_eye);
   std::memcpy(T.previous_eye, t.eye, sizeof t.eye);
-  tf.I = TemporalInstances{ts->previous_from_current, ts->instance_ids, ts->material_ids, ts->instance_flags};
+  // the per-instance identities are the caller's host arrays: copied to the
+  // device, which the surface pass and the reuse passes read them on
+  tf.I = TemporalInstances{nullptr, nullptr, nullptr, nullptr};
+  if (ts->previous_from_current || ts->instance_ids || ts->material_ids || ts->instance_flags) {
+    const size_t n = r->view.instance_count;
+    const size_t xf_bytes = 64 * n, id_bytes = 8 * n, flag_bytes = 4 * n;
+    const size_t need = xf_bytes + 2 * id_bytes + flag_bytes;
+    if (need > r->temporal_ids_bytes) {
+      dev_free(r->temporal_ids);
+      r->temporal_ids_bytes = 0;
+      e = dev_alloc(r->temporal_ids, need);
+      if (e == hipSuccess) r->temporal_ids_bytes = need;
+    }
+    unsigned char* d = r->temporal_ids;
+    auto upload = [&](const void* host, size_t offset, size_t bytes) -> void* {
+      if (!host || e != hipSuccess) return nullptr;
+      e = hipMemcpyAsync(d + offset, host, bytes, hipMemcpyHostToDevice, r->stream);
+      return d + offset;
+    };
+    if (e == hipSuccess && n != 0) {
+      tf.I.previous_from_current = static_cast<const float*>(upload(ts->previous_from_current, 0, xf_bytes));
+      tf.I.instance_id = static_cast<const uint64_t*>(upload(ts->instance_ids, xf_bytes, id_bytes));
+      tf.I.material_id = static_cast<const uint64_t*>(upload(ts->material_ids, xf_bytes + id_bytes, id_bytes));
+      tf.I.flags = static_cast<const uint32_t*>(upload(ts->instance_flags, xf_bytes + 2 * id_bytes, flag_bytes));
+    }
+    if (e != hipSuccess) {
+      set_detail(detail, detail_size, latch(r, "temporal identity upload", e));
+      return LOTUS_HIP_FAIL;
+    }
+  }
   tf.surface = t.surf[cur];
   tf.previous_surface = t.surf[prev];
   tf.validity = t.validity;
@@ -3052,6 +3083,93 @@ int lotus_hip_write_ids(lotus_hip_renderer* r, void* buffer, uint32_t buffer_for
   return LOTUS_HIP_PASS;
 }
 
+namespace {
+// The checks WriteRowsFlippedLocked makes before writing (adapter.cpp:
+// 2713-2720): the buffer's format is the AOV's and its size the source's.
+bool aov_target_ok(uint32_t aov_format, uint32_t source_width, uint32_t source_height, const void* buffer,
+                   uint32_t buffer_format, uint32_t buffer_width, uint32_t buffer_height, char* detail,
+                   size_t detail_size) {
+  if (buffer_format != aov_format) {
+    set_detail(detail, detail_size, "the AOV's format differs from the render buffer's");
+    return false;
+  }
+  if (source_width != buffer_width || source_height != buffer_height) {
+    set_detail(detail, detail_size, "the product's size differs from the render buffer's");
+    return false;
+  }
+  if (uint64_t(source_width) * source_height != 0 && !buffer) {
+    set_detail(detail, detail_size, "invalid render buffer input");
+    return false;
+  }
+  return true;
+}
+
+int finish_aov_write(lotus_hip_renderer* r, const char* what, char* detail, size_t detail_size) {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) {
+    set_detail(detail, detail_size, latch(r, what, e));
+    return LOTUS_HIP_FAIL;
+  }
+  set_detail(detail, detail_size, "");
+  return LOTUS_HIP_PASS;
+}
+}  // namespace
+
+int lotus_hip_write_temporal_aov(lotus_hip_renderer* r, uint32_t aov, const void* surface, const void* validity,
+                                 const void* radiance, uint32_t source_width, uint32_t source_height, void* buffer,
+                                 uint32_t buffer_format, uint32_t buffer_width, uint32_t buffer_height, char* detail,
+                                 size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (aov > LOTUS_HIP_AOV_HISTORY_REJECTION) {
+    set_detail(detail, detail_size, "not a temporal AOV");
+    return LOTUS_HIP_FAIL;
+  }
+  if (!aov_target_ok(temporal_aov_format(aov), source_width, source_height, buffer, buffer_format, buffer_width,
+                     buffer_height, detail, detail_size))
+    return LOTUS_HIP_FAIL;
+  if (uint64_t(source_width) * source_height == 0) {
+    set_detail(detail, detail_size, "");
+    return LOTUS_HIP_PASS;
+  }
+  if ((aov == LOTUS_HIP_AOV_MOTION && !surface) ||
+      ((aov == LOTUS_HIP_AOV_HISTORY_VALIDITY || aov == LOTUS_HIP_AOV_HISTORY_REJECTION) && !validity)) {
+    set_detail(detail, detail_size, "the AOV's temporal records are missing");
+    return LOTUS_HIP_FAIL;
+  }
+  (void)hipSetDevice(r->device);
+  launch_temporal_aov(aov, surface, validity, radiance, buffer, source_width, source_height, r->stream);
+  return finish_aov_write(r, "temporal AOV write", detail, detail_size);
+}
+
+int lotus_hip_write_restir_aov(lotus_hip_renderer* r, uint32_t aov, const void* initial, const void* resolved,
+                               uint32_t source_width, uint32_t source_height, void* buffer, uint32_t buffer_format,
+                               uint32_t buffer_width, uint32_t buffer_height, char* detail, size_t detail_size) {
+  if (!r) return LOTUS_HIP_FAIL;
+  if (!r->failure.empty()) {
+    set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
+    return LOTUS_HIP_FAIL;
+  }
+  if (aov > LOTUS_HIP_AOV_RESTIR_DIRECT) {
+    set_detail(detail, detail_size, "not a ReSTIR AOV");
+    return LOTUS_HIP_FAIL;
+  }
+  if (!aov_target_ok(restir_aov_format(aov), source_width, source_height, buffer, buffer_format, buffer_width,
+                     buffer_height, detail, detail_size))
+    return LOTUS_HIP_FAIL;
+  if (uint64_t(source_width) * source_height == 0) {
+    set_detail(detail, detail_size, "");
+    return LOTUS_HIP_PASS;
+  }
+  (void)hipSetDevice(r->device);
+  launch_restir_aov(aov, initial, resolved, buffer, source_width, source_height, r->stream);
+  return finish_aov_write(r, "ReSTIR AOV write", detail, detail_size);
+}
+
 int lotus_hip_reproject(lotus_hip_renderer* r, const void* current, const void* previous, uint32_t width,
                         uint32_t height, const float* validation, void* validity, char* detail, size_t detail_size) {
   if (!r) return LOTUS_HIP_FAIL;
@@ -3174,7 +3292,7 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
     res->frames_rendered = 1;
     res->camera_rays = d.add ? host[0] : 0;
     res->extension_rays = ext;
-    res->shadow_rays = sh;
+    res->shadow_rays = sh + host[counter_word(0, 0) + kCounterSpread / 2];  // + parallel_ray_count
     res->trace_launches = d.launches;
     res->traversal_rays[0] = d.add ? host[0] : 0;
     res->traversal_rays[1] = round_ext;

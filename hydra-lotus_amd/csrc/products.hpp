@@ -22,6 +22,30 @@ inline uint32_t buffer_pixel_bytes(uint32_t f) {
 // be one format. Returns false for a pair it does not convert.
 bool launch_write_rows_flipped(const void* src, uint32_t src_format, void* dst, uint32_t dst_format, uint32_t width,
                                uint32_t height, hipStream_t s);
+// The named temporal and ReSTIR AOVs (adapter.cpp:256-285).
+enum TemporalAov : uint32_t { kAovMotion = 0, kAovHistoryLength = 1, kAovHistoryValidity = 2, kAovHistoryRejection = 3 };
+enum RestirAov : uint32_t {
+  kAovRestirLight = 0,
+  kAovRestirCandidates = 1,
+  kAovRestirSupport = 2,
+  kAovRestirAge = 3,
+  kAovRestirWeight = 4,
+  kAovRestirDirect = 5
+};
+inline uint32_t temporal_aov_format(uint32_t aov) { return aov == kAovMotion ? kFloat32x4 : kInt32; }
+inline uint32_t restir_aov_format(uint32_t aov) {
+  return aov == kAovRestirWeight || aov == kAovRestirDirect ? kFloat32x4 : kInt32;
+}
+
+// Writes a temporal AOV of width x height top-down records (TemporalPixel,
+// TemporalValidityPixel, TemporalRadiancePixel or null) into a bottom-up
+// buffer of the AOV's format.
+void launch_temporal_aov(uint32_t aov, const void* surface, const void* validity, const void* radiance, void* dst,
+                         uint32_t width, uint32_t height, hipStream_t s);
+// Writes a ReSTIR AOV from the initial RestirPixel records and the resolved
+// RestirReusePixel records (either may be null) into a bottom-up buffer.
+void launch_restir_aov(uint32_t aov, const void* initial, const void* resolved, void* dst, uint32_t width,
+                       uint32_t height, hipStream_t s);
 // Fills `count` Int32 pixels with `value`.
 void launch_fill_ids(int32_t* dst, uint64_t count, int32_t value, hipStream_t s);
 

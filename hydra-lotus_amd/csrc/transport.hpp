@@ -1133,11 +1133,21 @@ __device__ __forceinline__ void escape_terms(const SceneView& S, bool sample_lig
 // `keep`, when given, receives the reconstructed surface; `slot` indexes the
 // ReSTIR record of the first hit; `factors`, when given, receives the shadow
 // ray's contribution in factors.
-template <bool kAnalytic, bool kCov = kAnalytic>  // kCov: covered emitters' coverage compiled in
+// Under BSDF sampling with more than one parallel light, `walk` (kEnabled)
+// sums every parallel light's ShadowedLight through shadow rays it walks
+// itself (transport.slang:139-146); the default walker has none.
+struct NoParallelWalk {
+  static constexpr bool kEnabled = false;
+  __device__ V3 operator()(const SceneView&, const Surface&, const V3&, const V3&, const V3&) const {
+    return V3{0.0f, 0.0f, 0.0f};
+  }
+};
+
+template <bool kAnalytic, bool kCov = kAnalytic, class Walk = NoParallelWalk>  // kCov: covered emitters' coverage
 __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameConst& F, PathRegs& p,
                                                 uint32_t tri, uint32_t instance, uint32_t primitive, float u,
                                                 float v, Surface* keep = nullptr, uint32_t slot = 0,
-                                                ShadowFactors* factors = nullptr) {
+                                                ShadowFactors* factors = nullptr, const Walk& walk = Walk{}) {
   Scatter out;
   out.continues = false;
   out.shadow = false;
@@ -1167,10 +1177,14 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
   const V3 wo = V3{dot3(md, ft), dot3(md, fb), dot3(md, n)};
   if (F.light_sampling == kLightRestir && p.depth == 0) {
     p.radiance = p.radiance + p.throughput * initial_direct_light<kAnalytic, kCov>(S, F.restir + 4ull * slot, sf, ft, fb, wo);
+  } else if (Walk::kEnabled && !sample_lights && S.parallel_count > 1) {
+    // ParallelLights (transport.slang:139-146): the lights' sum, then the
+    // throughput, as path.radiance += throughput * ParallelLights(...)
+    p.radiance = p.radiance + p.throughput * walk(S, sf, ft, fb, wo);
   } else if ((sample_lights && S.light_count != 0) || (kAnalytic && !sample_lights && S.parallel != 0)) {
     // next event estimation (transport.slang:123-135), or under BSDF sampling
-    // the parallel light no sampled direction finds (:139-146; one such
-    // light, the renderer checks), through ShadowedLight (:82-115)
+    // the one parallel light no sampled direction finds (:139-146), through
+    // ShadowedLight (:82-115)
     const V3 origin = offset_ray_origin(sf.position, sf.normal);
     LightSample ls{};
     if (sample_lights) {

@@ -33,7 +33,12 @@ SYMBOLS = ["lotus_hip_probe", "lotus_hip_create", "lotus_hip_destroy", "lotus_hi
            "lotus_hip_cache_train", "lotus_hip_cache_parameters", "lotus_hip_render_temporal", "lotus_hip_svgf",
            "lotus_hip_svgf_static", "lotus_hip_accumulate",
            "lotus_hip_reproject", "lotus_hip_render_batch", "lotus_hip_autotune", "lotus_hip_readback_scene",
-           "lotus_hip_write_render_buffer", "lotus_hip_write_ids"]
+           "lotus_hip_write_render_buffer", "lotus_hip_write_ids", "lotus_hip_write_temporal_aov",
+           "lotus_hip_write_restir_aov"]
+# Named AOVs (lotus_hip_write_temporal_aov, lotus_hip_write_restir_aov).
+AOV_MOTION, AOV_HISTORY_LENGTH, AOV_HISTORY_VALIDITY, AOV_HISTORY_REJECTION = range(4)
+AOV_RESTIR_LIGHT, AOV_RESTIR_CANDIDATES, AOV_RESTIR_SUPPORT, AOV_RESTIR_AGE, AOV_RESTIR_WEIGHT, \
+    AOV_RESTIR_DIRECT = range(6)
 BATCH_GRAPH = 1
 # Spatial reuse neighbours (lotus_hip_temporal_settings.restir_spatial_neighbours).
 SPATIAL_AXES, SPATIAL_DISK = 0, 1
@@ -373,6 +378,14 @@ def lib():
         L.lotus_hip_write_render_buffer.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                                     C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_char_p,
                                                     C.c_size_t]
+        L.lotus_hip_write_temporal_aov.restype = C.c_int
+        L.lotus_hip_write_temporal_aov.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                   C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32,
+                                                   C.c_uint32, C.c_char_p, C.c_size_t]
+        L.lotus_hip_write_restir_aov.restype = C.c_int
+        L.lotus_hip_write_restir_aov.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                 C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 C.c_char_p, C.c_size_t]
         L.lotus_hip_write_ids.restype = C.c_int
         L.lotus_hip_write_ids.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32,
                                           C.c_char_p, C.c_size_t]
@@ -782,6 +795,7 @@ class Renderer:
                         accumulate=False, max_history=32, filter_iterations=0, restir_temporal=False,
                         restir_spatial=False, radius=30, max_history_candidates=20, history_key=0,
                         camera_cut=False, want_products=False, spatial_disk=False, rectification=0, settle=False,
+                        instance_ids=None, material_ids=None, instance_flags=None, previous_from_current=None,
                         **kw):
         """One temporal frame; returns render()'s dict plus the reset reason,
         history availability, whether it settled and its still samples, the
@@ -801,6 +815,20 @@ class Renderer:
         t.restir_spatial_radius, t.restir_max_history_candidates = radius, max_history_candidates
         t.restir_spatial_neighbours = SPATIAL_DISK if spatial_disk else SPATIAL_AXES
         t.history_rectification = rectification
+        # per-instance identities: host arrays the call copies to the device
+        ident = {}
+        if instance_ids is not None:
+            ident["i"] = np.ascontiguousarray(instance_ids, np.uint64)
+            t.instance_ids = ident["i"].ctypes.data_as(C.POINTER(C.c_uint64))
+        if material_ids is not None:
+            ident["m"] = np.ascontiguousarray(material_ids, np.uint64)
+            t.material_ids = ident["m"].ctypes.data_as(C.POINTER(C.c_uint64))
+        if instance_flags is not None:
+            ident["f"] = np.ascontiguousarray(instance_flags, np.uint32)
+            t.instance_flags = _p(ident["f"], C.c_uint32)
+        if previous_from_current is not None:
+            ident["x"] = np.ascontiguousarray(previous_from_current, np.float32).reshape(-1)
+            t.previous_from_current = _p(ident["x"], C.c_float)
         prod = {}
         if want_products:
             prod = dict(surface=np.zeros((height, width), TEMPORAL_PIXEL),
@@ -865,6 +893,29 @@ class Renderer:
     def write_ids(self, buffer_ptr, buffer_format, width, height, value):
         buf = C.create_string_buffer(256)
         s = lib().lotus_hip_write_ids(self.handle, C.c_void_p(buffer_ptr), buffer_format, width, height, value, buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    def write_temporal_aov(self, aov, surface_ptr, validity_ptr, radiance_ptr, width, height, buffer_ptr,
+                           buffer_format, buffer_width=None, buffer_height=None):
+        """A named temporal AOV from device records into a bottom-up render buffer."""
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_write_temporal_aov(self.handle, aov, C.c_void_p(surface_ptr or None),
+                                               C.c_void_p(validity_ptr or None), C.c_void_p(radiance_ptr or None),
+                                               width, height, C.c_void_p(buffer_ptr), buffer_format,
+                                               width if buffer_width is None else buffer_width,
+                                               height if buffer_height is None else buffer_height, buf, 256)
+        if s != PASS:
+            raise LotusError(s, buf.value.decode())
+
+    def write_restir_aov(self, aov, initial_ptr, resolved_ptr, width, height, buffer_ptr, buffer_format,
+                         buffer_width=None, buffer_height=None):
+        """A named ReSTIR AOV from device records into a bottom-up render buffer."""
+        buf = C.create_string_buffer(256)
+        s = lib().lotus_hip_write_restir_aov(self.handle, aov, C.c_void_p(initial_ptr or None),
+                                             C.c_void_p(resolved_ptr or None), width, height, C.c_void_p(buffer_ptr),
+                                             buffer_format, width if buffer_width is None else buffer_width,
+                                             height if buffer_height is None else buffer_height, buf, 256)
         if s != PASS:
             raise LotusError(s, buf.value.decode())
 

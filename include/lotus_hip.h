@@ -497,8 +497,10 @@ typedef struct lotus_hip_temporal_settings {
   uint32_t restir_spatial;
   uint32_t restir_spatial_radius;          /* [1, 32] */
   uint32_t restir_max_history_candidates;  /* [1, 65536] */
-  /* Optional per-instance identity (temporal.slang:32-40); NULL: instance + 1,
-   * material slot + 1, rigid and unchanged with a static placement. */
+  /* Optional per-instance identity (temporal.slang:32-40), host arrays of
+   * the scene's instance_count entries, copied to the device each frame;
+   * NULL: instance + 1, material slot + 1, rigid and unchanged with a static
+   * placement. */
   const uint64_t* instance_ids;
   const uint64_t* material_ids;
   const uint32_t* instance_flags;          /* bit 1 rigid correspondence, bit 2 material unchanged */
@@ -650,7 +652,9 @@ int lotus_hip_autotune(lotus_hip_renderer* renderer, const lotus_hip_frame_param
                        lotus_hip_tuning* chosen, char* detail, size_t detail_size);
 
 /* Render-buffer writes of the products, on the device. Formats of a product
- * and of a host's render buffer, as HdFormat names them. */
+ * and of a host's render buffer, as HdFormat names them. The writes run on
+ * the renderer's own stream, which does not wait for other streams: work that
+ * fills a device buffer a call reads must have completed before the call. */
 #define LOTUS_HIP_BUFFER_UNORM8X4 0u  /* HdFormatUNorm8Vec4 ("rgba8-unorm") */
 #define LOTUS_HIP_BUFFER_FLOAT32X4 1u /* HdFormatFloat32Vec4 ("rgba32-sfloat") */
 #define LOTUS_HIP_BUFFER_FLOAT32 2u   /* HdFormatFloat32 (depth) */
@@ -673,6 +677,52 @@ int lotus_hip_write_render_buffer(lotus_hip_renderer* renderer, const void* prod
  * Int32 render buffer of width x height pixels with `value`. */
 int lotus_hip_write_ids(lotus_hip_renderer* renderer, void* buffer_device, uint32_t buffer_format, uint32_t width,
                         uint32_t height, int32_t value, char* detail, size_t detail_size);
+
+/* The named temporal AOVs (adapter.cpp:275-285): lotus:motion (Float32Vec4,
+ * the surface record's motion and depth), lotus:historyLength (Int32, the
+ * radiance record's sample count, 0 without radiance records),
+ * lotus:historyValidity and lotus:historyRejection (Int32, the validity
+ * record's valid and rejection words). */
+#define LOTUS_HIP_AOV_MOTION 0u
+#define LOTUS_HIP_AOV_HISTORY_LENGTH 1u
+#define LOTUS_HIP_AOV_HISTORY_VALIDITY 2u
+#define LOTUS_HIP_AOV_HISTORY_REJECTION 3u
+
+/* Replaces HdLotusRenderBuffer::WriteTemporal (adapter.cpp:2598-2630) on the
+ * device: from source_width x source_height top-down device records,
+ * TemporalPixel (64 B), TemporalValidityPixel (16 B) and
+ * TemporalRadiancePixel (32 B, may be NULL), writes the AOV into a bottom-up
+ * render buffer of its format (Float32Vec4 for motion, else Int32). A buffer
+ * of another format or size fails without writing. Returns once written. */
+int lotus_hip_write_temporal_aov(lotus_hip_renderer* renderer, uint32_t aov, const void* surface_device,
+                                 const void* validity_device, const void* radiance_device, uint32_t source_width,
+                                 uint32_t source_height, void* buffer_device, uint32_t buffer_format,
+                                 uint32_t buffer_width, uint32_t buffer_height, char* detail, size_t detail_size);
+
+/* The named ReSTIR AOVs (adapter.cpp:256-268): lotus:restirLight,
+ * lotus:restirCandidates, lotus:restirSupport, lotus:restirAge (Int32),
+ * lotus:restirWeight and lotus:restirDirect (Float32Vec4). */
+#define LOTUS_HIP_AOV_RESTIR_LIGHT 0u
+#define LOTUS_HIP_AOV_RESTIR_CANDIDATES 1u
+#define LOTUS_HIP_AOV_RESTIR_SUPPORT 2u
+#define LOTUS_HIP_AOV_RESTIR_AGE 3u
+#define LOTUS_HIP_AOV_RESTIR_WEIGHT 4u
+#define LOTUS_HIP_AOV_RESTIR_DIRECT 5u
+
+/* Replaces HdLotusRenderBuffer::WriteRestir (adapter.cpp:2632-2672) on the
+ * device: per pixel the resolved RestirReusePixel (64 B; the spatial pass's
+ * records, else the temporal pass's; may be NULL), else the initial
+ * RestirPixel (48 B; may be NULL), else an empty record (light 0xFFFFFFFF,
+ * zeros). lotus:restirWeight writes {weight_sum, selected_target,
+ * normalization, proposal}, the normalization the reuse record's, else
+ * (weight_sum / M) / selected_target when M > 0 and the target is positive,
+ * else 0; lotus:restirSupport the reuse record's supporting candidates, else
+ * M; lotus:restirAge the reuse record's age, else 0. A buffer of another
+ * format or size fails without writing. Returns once written. */
+int lotus_hip_write_restir_aov(lotus_hip_renderer* renderer, uint32_t aov, const void* initial_device,
+                               const void* resolved_device, uint32_t source_width, uint32_t source_height,
+                               void* buffer_device, uint32_t buffer_format, uint32_t buffer_width,
+                               uint32_t buffer_height, char* detail, size_t detail_size);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@ def build(reference: bool = True) -> None:
     subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
     if reference and os.path.isdir(REFERENCE_ROOT):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "hydra-lotus_amd", "liblotus_hip.so")):
+            subprocess.run(["make", "-s", "-C", HERE, "facade"], check=True)
 
 
 class SceneDesc(C.Structure):

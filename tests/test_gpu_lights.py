@@ -115,13 +115,37 @@ def test_floor_closed_forms_on_gpu(renderer, shape, light_sampling):
     assert np.all(np.abs(px.mean(0) - expect) <= 4 * se + 1e-4 * expect), (shape, px.mean(0), expect, se)
 
 
-def test_bsdf_sampling_takes_one_parallel_light(renderer):
-    sc = lit_cornell(parallel=2)
+@pytest.mark.parametrize("parallel", [2, 3])
+def test_bsdf_sampling_walks_every_parallel_light(renderer, parallel):
+    # ParallelLights (transport.slang:139-146) with several parallel lights:
+    # the shading lanes (k_shade and, past the rounds, k_tail) walk each
+    # light's shadow ray themselves. Paths of up to 24 bounces reach the tail.
+    _say(f"bsdf sampling, {parallel} parallel lights")
+    sc = lit_cornell(parallel=parallel)
     renderer.apply(sc)
-    with pytest.raises(hip.LotusError, match="at most one parallel light"):
-        renderer.render(sc.world_to_clip, 16, 16, light_sampling=hip.LIGHT_BSDF)
-    out = renderer.render(sc.world_to_clip, 16, 16, light_sampling=hip.LIGHT_MIS)  # not latched
-    assert out["spp"] == 1
+    w = h = 64
+    spp, bounces = 4, 24
+    o = oracle.OracleScene(sc)
+    acc, counts, shadow = o.render(oracle.make_frame(sc.world_to_clip, w, h, spp=spp, max_bounces=bounces,
+                                                     light_sampling=0))
+    first, tails = None, 0
+    for k in range(spp):
+        out = renderer.render(sc.world_to_clip, w, h, frame_count=1, max_bounces=bounces,
+                              light_sampling=hip.LIGHT_BSDF)
+        n = len(out["counts"])
+        assert np.array_equal(out["counts"], counts[k][:n]) and not counts[k][n:].any(), (k, out["counts"], counts[k])
+        assert out["shadow_rays"] == shadow[k], (k, out["shadow_rays"], shadow[k])
+        tails += out["rounds"] < n
+        if first is None:
+            first = out["color"]
+    assert tails > 0  # the tail shaded some of the paths
+    acc1, _, _ = o.render(oracle.make_frame(sc.world_to_clip, w, h, spp=1, max_bounces=bounces, light_sampling=0))
+    ref1 = acc1[..., :3].astype(np.float32)
+    hit = acc1[..., 3] > 0
+    exact = np.mean(np.all(first[..., :3][hit] == ref1[hit], -1))
+    _say(f"  1-spp radiance bit-exact fraction {exact:.6f}")
+    assert exact > 0.999
+    assert np.allclose(first[..., :3], ref1, rtol=1e-4, atol=1e-5)
 
 
 def sky_cornell():

@@ -47,6 +47,7 @@ def test_writes_match_oracle(renderer, size, pair):
     expect = oracle.write_render_buffer(prod, src_fmt, dst_fmt)
     src = _device(prod)
     dst = torch.zeros(expect.size, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()  # the renderer's stream does not wait for torch's
     renderer.write_render_buffer(src.data_ptr(), src_fmt, w, h, dst.data_ptr(), dst_fmt)
     assert np.array_equal(dst.cpu().numpy(), expect)
 
@@ -60,14 +61,17 @@ def test_frame_products_and_ids(renderer):
     color = out["color"].astype(F32)
     src = _device(color)
     dst = torch.zeros(w * h * 4, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()  # the renderer's stream does not wait for torch's
     renderer.write_render_buffer(src.data_ptr(), hip.BUFFER_FLOAT32X4, w, h, dst.data_ptr(), hip.BUFFER_UNORM8X4)
     assert np.array_equal(dst.cpu().numpy(), oracle.write_render_buffer(color, 1, 0))
     depth = out["depth"].astype(F32)
     dsrc = _device(depth)
     ddst = torch.zeros(w * h * 4, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()  # the renderer's stream does not wait for torch's
     renderer.write_render_buffer(dsrc.data_ptr(), hip.BUFFER_FLOAT32, w, h, ddst.data_ptr(), hip.BUFFER_FLOAT32)
     assert np.array_equal(ddst.cpu().numpy().view(F32).reshape(h, w), np.flipud(depth))
     ids = torch.zeros(w * h, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()  # the renderer's stream does not wait for torch's
     renderer.write_ids(ids.data_ptr(), hip.BUFFER_INT32, w, h, -1)
     assert bool((ids == -1).all())
 
@@ -82,4 +86,5 @@ def test_refusals_do_not_latch(renderer):
         renderer.write_render_buffer(a.data_ptr(), hip.BUFFER_FLOAT32, 8, 8, b.data_ptr(), hip.BUFFER_INT32)
     with pytest.raises(hip.LotusError, match="Int32"):
         renderer.write_ids(b.data_ptr(), hip.BUFFER_FLOAT32, 8, 8, 3)
+    torch.cuda.synchronize()  # the renderer's stream does not wait for torch's
     renderer.write_render_buffer(a.data_ptr(), hip.BUFFER_FLOAT32X4, 8, 8, b.data_ptr(), hip.BUFFER_FLOAT32X4)

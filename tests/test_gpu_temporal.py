@@ -518,3 +518,43 @@ def test_reset_reasons_follow_the_reference(renderer):
     f += 1
     out = renderer.render_temporal(sc.world_to_clip, sc.view, 32, 24, frame_index=f, **dict(kw, max_bounces=5))
     assert out["reset_reason"] == 0  # no fresh batch: the surfaces' history carries over
+
+
+def test_explicit_instance_identities_reach_the_device(renderer):
+    # The per-instance identities are host arrays the call copies to the
+    # device: explicit arrays equal to the defaults (instance + 1, material
+    # slot + 1, rigid and unchanged, static placement) give the default
+    # frames bit for bit, and a placement map moves every motion vector.
+    sc = _cornell(renderer)
+    n = len(sc.instances)
+    ids = np.arange(1, n + 1, dtype=np.uint64)
+    mids = np.array([m + 1 for (_, _, m) in sc.instances], np.uint64)
+    flags = np.full(n, 6, np.uint32)
+    still = np.tile(np.eye(4, dtype=np.float32).reshape(16), n)
+    w = h = 48
+    outs = []
+    for k, explicit in enumerate([False, True]):
+        extra = dict(instance_ids=ids, material_ids=mids, instance_flags=flags,
+                     previous_from_current=still) if explicit else {}
+        for f in range(2):
+            print(f"identities explicit={explicit} frame {f}", flush=True)
+            out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=7000 + 10 * k + f,
+                                           history_key=300 + k, accumulate=True, max_history=8, max_bounces=4,
+                                           want_products=True, **extra)
+        outs.append(out)
+    assert np.array_equal(outs[0]["surface"].view(np.uint8), outs[1]["surface"].view(np.uint8))
+    assert np.array_equal(outs[0]["validity"].view(np.uint8), outs[1]["validity"].view(np.uint8))
+    assert np.array_equal(outs[0]["color"].view(np.uint32), outs[1]["color"].view(np.uint32))
+    moved = still.reshape(n, 16).copy()
+    moved[:, 12] = 0.05  # each point's previous position 0.05 further along +x
+    # the explicit stream's next frame, whose previous frame has the same view
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=7012, history_key=301,
+                                   accumulate=True, max_history=8, max_bounces=4, want_products=True,
+                                   instance_ids=ids, material_ids=mids, instance_flags=flags,
+                                   previous_from_current=moved.reshape(-1))
+    hit = (out["surface"]["flags"] & 1) != 0
+    assert hit.mean() > 0.9
+    mx = np.abs(out["surface"]["motion_depth"][hit][:, 0])
+    # 0.05 along x moves the box's points 0.72 to 1.19 px at 48 x 48 (5th to
+    # 95th percentile); points that leave the view keep no motion
+    assert np.mean(mx > 0.5) > 0.9 and mx.max() < 1.5

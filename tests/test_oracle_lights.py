@@ -74,6 +74,29 @@ def test_floor_under_analytic_light_closed_form(shape, light_sampling):
         assert shadow.sum() == 0
 
 
+def _parallel(tilt_degrees, radiance):
+    t = math.radians(tilt_degrees)
+    frame = scenes.identity()
+    frame[8:11] = (math.sin(t), math.cos(t), 0.0)  # towards the light; it shines along -Z
+    return scenes.analytic_light(scenes.LIGHT_DISTANT, frame, radiance=radiance, angle=0.0)
+
+
+def test_floor_under_several_parallel_lights_bsdf_sampling():
+    # BSDF sampling finds no parallel light: ParallelLights (transport.slang:
+    # 139-146) adds each one's exact term through a shadow ray of its own, so
+    # every sample is a/pi * sum_j L_j cos(theta_j) and each hit sends one
+    # shadow ray per light.
+    lights = [(0.0, L), (50.0, np.array([1.0, 3.0, 0.5])), (-70.0, np.array([0.5, 0.5, 2.0]))]
+    sc = scenes.lit_floor(_parallel(*lights[0]))
+    for tilt, radiance in lights[1:]:
+        sc.add_light(_parallel(tilt, radiance))
+    o = oracle.OracleScene(sc)
+    px, shadow = _pixel_means(o, sc, 0, spp=16)
+    expect = ALBEDO / math.pi * sum(r * math.cos(math.radians(t)) for t, r in lights)
+    assert np.allclose(px, expect, rtol=1e-5)
+    assert shadow.sum() == len(lights) * px.shape[0] * 16
+
+
 def test_many_sphere_lights_estimators_agree():
     # Seven spheres over the floor: the crossing hierarchy (three levels) must
     # find every one that BSDF sampling hits, as light sampling does.
