@@ -32,6 +32,20 @@ namespace {
     }                                                                      \
   } while (0)
 
+// A start/stop pair of timing events, destroyed on every way out of a build.
+struct EventPair {
+  hipEvent_t start = nullptr, stop = nullptr;
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&start);
+    if (e == hipSuccess) e = hipEventCreate(&stop);
+    return e;
+  }
+  ~EventPair() {
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+  }
+};
+
 constexpr int kBlock = 256;
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per_block) {
@@ -625,9 +639,9 @@ bool radix_sort_pairs(uint32_t* keys, uint32_t* values, uint32_t n, uint32_t key
 bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, double& gpu_ms,
                std::string& error) {
   const uint32_t n = in.tri_count;
-  hipEvent_t start, stop;
-  LOTUS_CHECK(hipEventCreate(&start));
-  LOTUS_CHECK(hipEventCreate(&stop));
+  EventPair events;
+  LOTUS_CHECK(events.create());
+  const hipEvent_t start = events.start, stop = events.stop;
   LOTUS_CHECK(hipEventRecord(start, stream));
   if (!alloc(out.tris, n, error) || !alloc(out.tri_of_global, n, error)) return false;
   out.bytes += uint64_t(n) * (sizeof(Tri) + 4);
@@ -767,8 +781,6 @@ bool build_bvh(const BuildInput& in, SceneBuffers& out, hipStream_t stream, doub
   }
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, start, stop);
-  (void)hipEventDestroy(start);
-  (void)hipEventDestroy(stop);
   gpu_ms = ms;
   if (final_counts[4] != 0 || pending > 0 || allocated > max_nodes) {
     error = "BVH build produced an invalid tree (code " + std::to_string(final_counts[4]) + ")";
@@ -803,9 +815,9 @@ bool refit_bvh(const BuildInput& in, SceneBuffers& s, hipStream_t stream, double
     if (!alloc(s.exact, 2ull * s.node_count, error)) return false;
     s.bytes += 2ull * s.node_count * sizeof(float4);
   }
-  hipEvent_t start, stop;
-  LOTUS_CHECK(hipEventCreate(&start));
-  LOTUS_CHECK(hipEventCreate(&stop));
+  EventPair events;
+  LOTUS_CHECK(events.create());
+  const hipEvent_t start = events.start, stop = events.stop;
   hipError_t e = hipEventRecord(start, stream);
   if (e == hipSuccess) {
     k_reflatten<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(in, s.tri_of_global, s.tris);
@@ -822,8 +834,6 @@ bool refit_bvh(const BuildInput& in, SceneBuffers& s, hipStream_t stream, double
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   float ms = 0.0f;
   if (e == hipSuccess) (void)hipEventElapsedTime(&ms, start, stop);
-  (void)hipEventDestroy(start);
-  (void)hipEventDestroy(stop);
   if (e != hipSuccess) {
     error = std::string("BVH refit: ") + hipGetErrorString(e);
     return false;

@@ -1022,12 +1022,16 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   if (p->light_count != 0 && !p->lights) return "the plan names lights but holds none";
   if ((p->texture_count != 0 && !p->textures) || (p->binding_count != 0 && !p->bindings))
     return "the plan names textures or bindings but holds none";
+  uint64_t texel_total = 0;
   for (uint32_t k = 0; k < p->texture_count; ++k) {
     const lotus_hip_texture& x = p->textures[k];
     if (x.width == 0 || x.height == 0 || x.width > 16384 || x.height > 16384 || !x.texels ||
         x.format > LOTUS_HIP_TEXTURE_RGBA32F)
       return "texture " + std::to_string(k) + " must be 1 to 16384 texels on a side in a known format";
+    texel_total += uint64_t(x.width) * x.height;
   }
+  // each texture's first texel is a 32-bit offset into the shared texel array
+  if (texel_total > 0xFFFFFFFFull) return "the plan's textures hold more than 2^32 - 1 texels";
   for (uint32_t k = 0; k < p->binding_count; ++k) {
     const uint32_t* b = p->bindings + 24ull * k;
     if (b[3] > 2u || b[2] > 3u || b[1] > 15u || (b[3] == 1u && b[0] >= p->texture_count))

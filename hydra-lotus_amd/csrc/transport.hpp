@@ -139,6 +139,13 @@ __device__ __forceinline__ int wrap_texel(int i, int n, uint32_t mode, bool& bla
   return i;
 }
 
+// A texel coordinate kept within +-2^24, where floats are still integers
+// and the corner indices below fit an int; NaN (coordinates that overflowed
+// on the way) reads texel 0. Coordinates of real lookups lie far inside.
+__device__ __forceinline__ float lookup_coordinate(float v) {
+  return v == v ? minf(maxf(v, -16777216.0f), 16777216.0f) : 0.0f;
+}
+
 // scene.slang:318-329 (Lookup): the bilinear texel at (s', 1 - t') of the
 // binding's coordinate transform, times its scale plus its bias; or its
 // fallback. Binding words: texture, sampler (wrap_s * 4 + wrap_t), channel,
@@ -150,7 +157,8 @@ __device__ __forceinline__ float4 texture_lookup(const SceneView& S, const uint3
   const float tt = 1.0f - ((f[20] * s + f[21] * t) + f[22]);
   const uint4 tex = S.textures[b[0]];
   const int W = static_cast<int>(tex.y), H = static_cast<int>(tex.z);
-  const float x = ss * static_cast<float>(W) - 0.5f, y = tt * static_cast<float>(H) - 0.5f;
+  const float x = lookup_coordinate(ss * static_cast<float>(W) - 0.5f),
+              y = lookup_coordinate(tt * static_cast<float>(H) - 0.5f);
   const float x0 = floorf(x), y0 = floorf(y);
   const float fx = x - x0, fy = y - y0;
   const uint32_t ws = b[1] >> 2, wt = b[1] & 3u;

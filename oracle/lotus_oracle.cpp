@@ -1058,7 +1058,10 @@ void Lookup(const Textures& T, const uint32_t* b, float s, float t, float out[4]
   const float tt = 1.0f - ((WordFloat(b, 20) * s + WordFloat(b, 21) * t) + WordFloat(b, 22));
   const uint32_t* rec = T.records.data() + 3 * size_t(b[0]);
   const int W = static_cast<int>(rec[1]), H = static_cast<int>(rec[2]);
-  const float x = ss * static_cast<float>(W) - 0.5f, y = tt * static_cast<float>(H) - 0.5f;
+  // texel coordinates kept within +-2^24 (NaN reads texel 0), as the
+  // kernels keep them: the corner indices then fit an int
+  auto keep = [](float v) { return v == v ? std::min(std::max(v, -16777216.0f), 16777216.0f) : 0.0f; };
+  const float x = keep(ss * static_cast<float>(W) - 0.5f), y = keep(tt * static_cast<float>(H) - 0.5f);
   const float x0 = std::floor(x), y0 = std::floor(y);
   const float fx = x - x0, fy = y - y0;
   const uint32_t ws = b[1] >> 2, wt = b[1] & 3u;
