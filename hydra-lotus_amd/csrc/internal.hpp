@@ -76,6 +76,8 @@ struct SceneView {
   uint32_t analytic_count, light_node_count, distant_count;
   uint32_t parallel;                // the first parallel light's index + 1 (BSDF sampling traces it), 0 without one
   uint32_t parallel_count;          // parallel lights (more than one: BSDF sampling walks them inline)
+  uint32_t gltf_materials;          // materials of the glTF PBR model
+  const float* albedo_tables;       // the glTF closure's directional albedo tables (albedo_tables.inc)
   float environment[3];
   // The environment image (env_width 0: none): texels, then its sampling
   // distribution's floats (cell densities, the rows' cumulative
@@ -117,7 +119,8 @@ constexpr uint32_t kEnvironmentLight = 0xFFFFFFFFu;
 // colours run (the others compile none of that code).
 __host__ __device__ inline bool scene_has_extras(const SceneView& S) {
   return S.analytic_count != 0 || S.env_width != 0 || S.bindings != nullptr || S.texcoords != nullptr ||
-         S.instance_normals != nullptr || S.instance_colors != nullptr || S.covered_instances != 0;
+         S.instance_normals != nullptr || S.instance_colors != nullptr || S.covered_instances != 0 ||
+         S.gltf_materials != 0;
 }
 
 // Device memory of the resident scene, kept until a plan needs a new BVH.
@@ -135,6 +138,7 @@ struct SceneBuffers {
   LightNode* light_nodes = nullptr;
   uint32_t* distant = nullptr;
   uint32_t analytic_count = 0, light_node_count = 0, distant_count = 0, parallel_count = 0, parallel = 0;
+  uint32_t gltf_materials = 0;  // materials of the glTF PBR model
   float4* env_texels = nullptr;
   float* env_dist = nullptr;
   uint32_t env_width = 0, env_height = 0, env_columns = 0, env_rows = 0;

@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst 
 // streams them by the maximum component of their unshadowed contribution,
 // and traces one visibility ray for the selection (counted apart from the
 // queued shadow rays).
-template <bool kLights>  // the scene has analytic lights or an environment image
+template <bool kLights, bool kGltf = false>  // extras (lights, images, lookups); glTF PBR surfaces
 __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, FrameConst F, Queues q,
                                                                 uint32_t* counters, uint2* spill, int spill_cap,
                                                                 unsigned long long* stats) {
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
     const uint4 h4 = q.hit_h[i];
     const uint32_t slot = __float_as_uint(o4.w);
     const V3 dir = xyz(d4);
-    const Surface sf = reconstruct_hit<kLights>(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z),
+    const Surface sf = reconstruct_hit<kLights, kGltf>(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z),
                                                 __uint_as_float(h4.w), dir);
     const V3 n = sf.shading_normal;
     const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
@@ -732,7 +732,8 @@ struct ParallelWalk {
 // including the vertex's emission).
 // kParallel: BSDF sampling walks several parallel lights (ParallelWalk), with
 // traversal stacks in LDS.
-template <bool kAnalytic, bool kCov, bool kParallel = false>  // analytic lights (extras); covered surfaces
+template <bool kAnalytic, bool kCov, bool kParallel = false,
+          bool kGltf = false>  // analytic lights (extras); covered surfaces; glTF PBR surfaces
 __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                        uint32_t bounce, uint32_t link, uint2* spill, int spill_cap) {
   using Walk = std::conditional_t<kParallel, ParallelWalk<kCov>, NoParallelWalk>;
@@ -782,12 +783,13 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(SceneView S, FrameConst F
     Surface sf;
     ShadowFactors factors;
     if (query) {
-      sf = reconstruct_hit<kAnalytic>(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z), __uint_as_float(h4.w), p.d);
+      sf = reconstruct_hit<kAnalytic, kGltf>(S, __float_as_uint(d4.w), h4.x, __uint_as_float(h4.z),
+                                             __uint_as_float(h4.w), p.d);
       add_emission(S, F, p, sf, h4.x, h4.y);
       s.continues = false;
       s.shadow = false;
     } else {
-      s = scatter_path<kAnalytic, kCov, Walk>(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z),
+      s = scatter_path<kAnalytic, kCov, Walk, kGltf>(S, F, p, __float_as_uint(d4.w), h4.x, h4.y, __uint_as_float(h4.z),
                                               __uint_as_float(h4.w), train ? &sf : nullptr, slot,
                                               kCov ? &factors : nullptr, walk);
     }
@@ -969,7 +971,7 @@ __device__ __forceinline__ float pair_float(float v) { return __shfl_xor(v, 1); 
 // (extras); covered surfaces; BSDF sampling walks several parallel lights
 // (ParallelWalk, on the stack the lane's walks use, which is free while it
 // shades)
-template <bool kCache, bool kAnalytic, bool kCov, bool kParallel = false>
+template <bool kCache, bool kAnalytic, bool kCov, bool kParallel = false, bool kGltf = false>
 __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F, Queues q, uint32_t* counters,
                                                       uint2* spill, int spill_cap, uint32_t bounce,
                                                       unsigned long long* stats) {
@@ -1113,7 +1115,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
           const V3 arrival = p.throughput, dir = p.d;
           const uint32_t b = p.depth;
           Surface sf;
-          sc = scatter_path<kAnalytic, kCov, Walk>(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v,
+          sc = scatter_path<kAnalytic, kCov, Walk, kGltf>(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v,
                                                    kCache && train ? &sf : nullptr, slot, kCov ? &sc4 : nullptr, walk);
           if (kCache && train && b < kCacheRecordDepth && path < F.train_capacity) {
             float4* rec = q.crec + 6ull * (static_cast<size_t>(b) * F.train_capacity + path);
@@ -1314,7 +1316,19 @@ void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, con
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce, bool link) {
   const uint32_t l = link ? 1u : 0u;
-  if (parallel_walks(S, F)) {
+  if (S.gltf_materials != 0) {
+    // glTF PBR scenes: the variants with its closure
+    const bool cov = S.covered_instances != 0, par = parallel_walks(S, F);
+    const uint32_t grid = par ? L.grid.shade_parallel : L.grid.shade;
+    if (par && cov)
+      k_shade<true, true, true, true><<<grid, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, L.spill, L.spill_cap);
+    else if (par)
+      k_shade<true, false, true, true><<<grid, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, L.spill, L.spill_cap);
+    else if (cov)
+      k_shade<true, true, false, true><<<grid, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, nullptr, 0);
+    else
+      k_shade<true, false, false, true><<<grid, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, nullptr, 0);
+  } else if (parallel_walks(S, F)) {
     if (S.covered_instances != 0)
       k_shade<true, true, true><<<L.grid.shade_parallel, kShadeBlock, 0, L.stream>>>(S, F, q, c, bounce, l, L.spill,
                                                                                      L.spill_cap);
@@ -1352,7 +1366,29 @@ void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const
                  uint32_t bounce) {
   const bool analytic = scene_has_extras(S), cov = S.covered_instances != 0;
   const uint32_t grid = F.cache_depth != 0 ? L.grid.tail_cache : L.grid.tail;
-  if (parallel_walks(S, F)) {
+  if (S.gltf_materials != 0) {
+    // glTF PBR scenes: the variants with its closure
+    const bool cache = F.cache_depth != 0, par = parallel_walks(S, F);
+    if (cache) {
+      if (par && cov)
+        k_tail<true, true, true, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+      else if (par)
+        k_tail<true, true, false, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+      else if (cov)
+        k_tail<true, true, true, false, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+      else
+        k_tail<true, true, false, false, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+    } else {
+      if (par && cov)
+        k_tail<false, true, true, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+      else if (par)
+        k_tail<false, true, false, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+      else if (cov)
+        k_tail<false, true, true, false, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+      else
+        k_tail<false, true, false, false, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
+    }
+  } else if (parallel_walks(S, F)) {
     if (F.cache_depth != 0 && cov)
       k_tail<true, true, true, true><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce,
                                                                          L.stats);
@@ -1387,7 +1423,9 @@ void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F
                                                                    L.spill_cap);
 }
 void launch_restir_initial(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
-  if (scene_has_extras(S))
+  if (S.gltf_materials != 0)
+    k_restir_initial<true, true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+  else if (scene_has_extras(S))
     k_restir_initial<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
   else
     k_restir_initial<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);

@@ -173,9 +173,22 @@ double emission_estimate(const uint32_t* m, const uint32_t* bindings) {
     for (int c = 0; c < 3; ++c)
       e[c] = b[3] == 2u ? std::max(0.0f, f[12 + c]) : double(std::max(0.0f, f[4 + c])) + std::max(0.0f, f[8 + c]);
   }
-  const double mean = (e[0] + e[1] + e[2]) / 3.0;
+  // glTF PBR's emissive strength is a constant factor on the estimate
+  double factor = 1.0;
+  if ((m[0] & 0xFFu) == 1u) {
+    float strength;
+    std::memcpy(&strength, m + 20, 4);
+    factor = strength;
+  }
+  const double mean = factor * (e[0] + e[1] + e[2]) / 3.0;
   return std::isfinite(mean) && mean > 0.0 ? mean : 0.0;
 }
+
+// The glTF closure's directional albedo tables (generated by
+// hydra-lotus_amd/tables/make_albedo_tables.cpp in albedo.slang's layout).
+const float kAlbedoTables[] = {
+#include "albedo_tables.inc"
+};
 
 // The light table and the analytic lights as the device holds them.
 struct LightTableHost {
@@ -221,6 +234,7 @@ struct lotus_hip_renderer {
   // previous_from_current (16 floats), instance and material ids, flags.
   unsigned char* temporal_ids = nullptr;
   size_t temporal_ids_bytes = 0;
+  float* albedo_tables = nullptr;  // the glTF closure's albedo tables
   uint64_t emissive_triangles = 0;  // the light table's triangle entries
   LightTableHost light_table;     // as uploaded
   // The applied plan, to find what the next one changes.
@@ -1088,9 +1102,24 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
   }
   for (uint32_t m = 0; m < p->material_count; ++m) {
     const uint32_t* w = p->materials + 36ull * m;
-    if ((w[0] & 0xFFu) != 0) return "material " + std::to_string(m) + ": only UsdPreviewSurface is supported";
-    if (w[3] != 0 || (w[2] & ~0x7Fu) != 0)
-      return "material " + std::to_string(m) + ": only opacity, normal, base colour, roughness, metallic, emission and specular colour lookups are supported";
+    const uint32_t model = w[0] & 0xFFu;
+    if (model > 1u) return "material " + std::to_string(m) + ": only UsdPreviewSurface and glTF PBR are supported";
+    // UsdPreviewSurface reads slots 0-6, glTF PBR slots 0-13 (scene.slang:59-79)
+    if (w[3] != 0 || (w[2] & ~(model == 1u ? 0x3FFFu : 0x7Fu)) != 0)
+      return "material " + std::to_string(m) + ": a lookup in a slot its model does not read";
+    if (model == 1u) {
+      // ValidGltfPbr (gpu_scene.cpp:853-881)
+      float f[36];
+      std::memcpy(f, w, sizeof f);
+      auto unit = [](float x) { return x >= 0.0f && x <= 1.0f; };
+      auto radiance = [](float x) { return std::isfinite(x) && x >= 0.0f; };
+      const bool ok = unit(f[12]) && unit(f[13]) && unit(f[14]) && unit(f[15]) && unit(f[19]) && radiance(f[16]) &&
+                      radiance(f[17]) && radiance(f[18]) && radiance(f[20]) && unit(f[21]) && unit(f[22]) &&
+                      unit(f[23]) && radiance(f[24]) && radiance(f[25]) && radiance(f[26]) && unit(f[27]) &&
+                      unit(f[28]) && unit(f[29]) && unit(f[30]) && unit(f[31]) && unit(f[32]) &&
+                      std::isfinite(f[33]) && f[33] >= 1.0f;
+      if (!ok) return "material " + std::to_string(m) + ": a glTF PBR parameter lies outside its range";
+    }
     if (w[2] != 0 && (!p->bindings || uint64_t(w[1]) + __builtin_popcount(w[2]) > p->binding_count))
       return "material " + std::to_string(m) + ": its lookups lie outside the binding table";
     float opacity, threshold;
@@ -1101,6 +1130,13 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
       return "material " + std::to_string(m) + ": opacity and its threshold must lie in [0, 1]";
   }
   return {};
+}
+
+// The plan's materials of the glTF PBR model.
+uint32_t count_gltf_materials(const lotus_hip_scene_plan* p) {
+  uint32_t n = 0;
+  for (uint32_t m = 0; m < p->material_count; ++m) n += (p->materials[36ull * m] & 0xFFu) == 1u ? 1u : 0u;
+  return n;
 }
 
 // Per instance 1 when a ray tests its material's coverage (scene.slang:
@@ -1316,6 +1352,8 @@ void set_view(lotus_hip_renderer* r, const PlanShape& sh, const float* environme
   v.distant_count = s.distant_count;
   v.parallel = s.parallel;
   v.parallel_count = s.parallel_count;
+  v.gltf_materials = s.gltf_materials;
+  v.albedo_tables = r->albedo_tables;
   v.env_texels = s.env_texels;
   v.env_dist = s.env_dist;
   v.env_width = s.env_width;
@@ -1459,6 +1497,7 @@ bool apply_build(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
   up(s.instance_covered, s.covered_host.data(), I * 4ull);
   up(s.instance_transform, p->instance_transform, 64ull * I);
   up(s.materials, p->materials, 144ull * M);
+  s.gltf_materials = count_gltf_materials(p);
   if (e == hipSuccess) e = upload_light_table(s, table, up);
   // texture lookups: texels decoded to linear RGBA, {first texel, width,
   // height} per texture, the binding table, texture coordinates
@@ -1586,6 +1625,7 @@ bool apply_changes(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const P
       s.material_count = M;
     }
     up(s.materials, p->materials, 144ull * M);
+    s.gltf_materials = count_gltf_materials(p);
   }
   if (lights_changed && e == hipSuccess) e = upload_light_table(s, table, up);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
@@ -1708,6 +1748,8 @@ lotus_hip_renderer* lotus_hip_create(int device, int* status, char* detail, size
       if (e == hipSuccess) e = hipEventCreate(&ev);
   }
   if (e == hipSuccess) e = dev_alloc(r->counters, frame_counter_words(LOTUS_HIP_MAX_BOUNCES + 4));
+  if (e == hipSuccess) e = dev_alloc(r->albedo_tables, sizeof kAlbedoTables / sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(r->albedo_tables, kAlbedoTables, sizeof kAlbedoTables, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = dev_alloc(r->stats, 8);
   if (e == hipSuccess) resolve_scratch(device, r->static_scratch, r->static_scratch_error);
   if (e != hipSuccess) {
@@ -1730,6 +1772,7 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   cache_free(r->cache);
   free_temporal(r);
   dev_free(r->temporal_ids);
+  dev_free(r->albedo_tables);
   dev_free(r->svgf_filter);
   dev_free(r->counters);
   dev_free(r->stats);
@@ -2676,6 +2719,13 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
                              ts->restir_spatial_neighbours > LOTUS_HIP_SPATIAL_DISK)) {
     set_detail(detail, detail_size,
                "spatial reservoirs require RenderTemporalScene, RestirInitial, one sample, history candidates in [1, 65536] and radius in [1, 32]");
+    return LOTUS_HIP_FAIL;
+  }
+  // Reservoir reuse keeps each receiver's closure in a stored surface; this
+  // renderer stores UsdPreviewSurface's (restir.slang:72-98) and not yet glTF
+  // PBR's 22 terms (:80-93)
+  if ((ts->restir_temporal || ts->restir_spatial) && r->view.gltf_materials != 0) {
+    set_detail(detail, detail_size, "reservoir reuse over glTF PBR materials is not supported by this renderer");
     return LOTUS_HIP_FAIL;
   }
   auto& t = r->temporal;

@@ -114,11 +114,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_temporal_surface(SceneView S, F
     const uint32_t iflags = I.flags ? I.flags[h.instance] : 6u;
     o.flags = 1u | iflags;
     const Surface sf = reconstruct_hit(S, h.tri, h.instance, h.u, h.v, r.d);
-    // scene.slang:1254-1257: transmitted light is tinted by the transmission colour
-    const V3 albedo = sf.bsdf.transmission > 0.0f
-                          ? sf.base_color + (sf.bsdf.transmission_color - sf.base_color) * sf.bsdf.transmission
-                          : sf.base_color;
-    o.albedo = pack_albedo(albedo);
+    // scene.slang:1254-1257, 1396: the model's surface summary
+    o.albedo = pack_albedo(surface_albedo(sf));
     o.shading_normal = pack_normal(sf.shading_normal);
     if ((iflags & 2u) != 0u) {
       V3 prev = world;
@@ -150,7 +147,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_temporal_surface(SceneView S, F
           const float cosine = dot3(to_current, to_previous) / maxf(length3(to_current) * length3(to_previous), 1.0e-20f);
           const float theta = acos_pi(minf(maxf(cosine, -1.0f), 1.0f)) * 3.14159265358979323846f;
           const float lobe = maxf(sf.roughness * sf.roughness, 0.001f);
-          const float share = sf.metallic + 0.04f * (1.0f - sf.metallic);
+          const float share = surface_specular_share(sf);
           const float ratio = theta / lobe;
           o.reserved = __float_as_uint((1.0f - share) + share / (1.0f + ratio * ratio));
         }

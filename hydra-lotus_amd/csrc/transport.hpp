@@ -5,6 +5,7 @@
 #pragma once
 
 #include "bsdf.hpp"
+#include "gltf.hpp"
 #include "internal.hpp"
 
 namespace lotus {
@@ -74,14 +75,49 @@ struct PathRegs {
   uint32_t depth;
 };
 
+// A hit's closure, whatever its model (bsdf.slang:23-90): UsdPreviewSurface's
+// own or glTF PBR's composed one.
+constexpr uint32_t kPreviewModel = 0u, kGltfModel = 1u;
+struct SurfaceBsdf {
+  uint32_t model;
+  Closure preview;
+  GltfClosure gltf;
+};
+__device__ __forceinline__ V3 bsdf_evaluate(const SurfaceBsdf& b, V3 wo, V3 wi) {
+  return b.model == kGltfModel ? gltf_evaluate(b.gltf, wo, wi) : bsdf_evaluate(b.preview, wo, wi);
+}
+__device__ __forceinline__ float bsdf_pdf(const SurfaceBsdf& b, V3 wo, V3 wi) {
+  return b.model == kGltfModel ? gltf_pdf(b.gltf, wo, wi) : bsdf_pdf(b.preview, wo, wi);
+}
+__device__ __forceinline__ bool bsdf_sample(const SurfaceBsdf& b, V3 wo, float ux, float uy, float uz,
+                                            BsdfSample& s) {
+  return b.model == kGltfModel ? gltf_sample(b.gltf, wo, ux, uy, uz, s) : bsdf_sample(b.preview, wo, ux, uy, uz, s);
+}
+
 struct Surface {
   V3 position, normal, shading_normal;
   float area;
   V3 base_color;
   float roughness, metallic;
-  Closure bsdf;
+  SurfaceBsdf bsdf;
   V3 emission;
 };
+
+// The surface summary of temporal products (scene.slang:1251-1258, 1396-1401):
+// the albedo that demodulates its light (UsdPreviewSurface's transmitted
+// share tinted by the transmission colour) and the principal lobe's
+// roughness and share of the reflection.
+__device__ __forceinline__ V3 surface_albedo(const Surface& sf) {
+  if (sf.bsdf.model == kGltfModel) return sf.base_color;
+  const Closure& b = sf.bsdf.preview;
+  return b.transmission > 0.0f ? sf.base_color + (b.transmission_color - sf.base_color) * b.transmission
+                               : sf.base_color;
+}
+__device__ __forceinline__ float surface_specular_share(const Surface& sf) {
+  if (sf.bsdf.model == kGltfModel)
+    return sf.metallic + max3(sf.bsdf.gltf.t.f0) * (1.0f - sf.metallic);
+  return sf.metallic + 0.04f * (1.0f - sf.metallic);
+}
 
 __device__ __forceinline__ float word_float(const uint32_t* w, int i) { return __uint_as_float(w[i]); }
 
@@ -324,10 +360,154 @@ __device__ __forceinline__ void tangent_frame(const SceneView& S, const uint32_t
   if (dot3(fb, bitangent) < 0.0f) fb = -fb;
 }
 
+// MakeShadingFrame (sampling.slang:45-56, Duff et al. 2017): tangent and
+// bitangent of unit n.
+__device__ __forceinline__ void shading_frame(V3 n, V3& ft, V3& fb) {
+  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  ft = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+  fb = V3{b, sign + n.y * n.y * a, -n.y};
+}
+
+// scene.slang:541-553 (MappedNormal) of tangent-space normal `tn` in the
+// tangent frame about `base` (TangentFrame with the map's binding `input`,
+// null for a constant normal): the shading normal, or `base` where the mapped
+// one faces away from the ray.
+template <bool kExtras>
+__device__ __forceinline__ V3 map_normal(const SceneView& S, V3 tn, const uint32_t* input, uint32_t instance,
+                                         uint32_t primitive, V3 e1, V3 e2, V3 base, V3 geometric, V3 dir) {
+  V3 bt, bb;
+  if (kExtras) {
+    tangent_frame(S, input, instance, primitive, e1, e2, base, bt, bb);
+  } else {
+    shading_frame(base, bt, bb);
+  }
+  const float mag = maxf(maxf(fabsf(tn.x), fabsf(tn.y)), fabsf(tn.z));
+  if (!(mag > 0.0f) || !finite3(tn)) return base;
+  const V3 q = tn / mag;
+  V3 sh = normalize3(((q.x * bt) + (q.y * bb)) + (q.z * base));
+  if (dot3(sh, geometric) < 0.0f) sh = -sh;
+  return dot3(sh, dir) < 0.0f ? sh : base;
+}
+
+// A scalar glTF input (scene.slang:1262-1274): its lookup's channel, clamped,
+// where the slot has one, else the constant.
+__device__ __forceinline__ float gltf_scalar(const SceneView& S, const uint32_t* m, uint32_t slot, float s, float t,
+                                             float constant) {
+  const uint32_t* b = material_input(S, m, slot);
+  return b ? saturate(channel_of(texture_lookup(S, b, s, t), b[2])) : constant;
+}
+
+__device__ __forceinline__ V3 clamp_normal(float4 c) {
+  return V3{minf(maxf(c.x, -1.0f), 1.0f), minf(maxf(c.y, -1.0f), 1.0f), minf(maxf(c.z, -1.0f), 1.0f)};
+}
+
+// scene.slang:1276-1403 (EvaluateGltfPbr) into `sf`, whose geometry and
+// shading normal are set: the inputs (material words 12-33,
+// gpu_scene.cpp:2050-2071) and their lookups (slots: normal 1, base colour
+// 2, metallic 3, roughness 4, emissive 5, specular 6, specular colour 7,
+// transmission 8, clearcoat 9, its roughness 10, its normal 11, sheen colour
+// 12, sheen roughness 13), the base's and the coat's normal maps, the
+// closure's outgoing direction (the shading frame's x-z plane without a
+// coat), its terms and its closure.
+__device__ __forceinline__ void gltf_surface(const SceneView& S, const uint32_t* m, uint32_t instance,
+                                             uint32_t primitive, float u, float v, V3 e1, V3 e2, V3 dir,
+                                             Surface& sf) {
+  V3 base = V3{word_float(m, 12), word_float(m, 13), word_float(m, 14)};
+  float roughness = word_float(m, 15), metallic = word_float(m, 19);
+  const float strength = word_float(m, 20);
+  V3 emission = V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)} * strength;
+  V3 tn = V3{word_float(m, 8), word_float(m, 9), word_float(m, 10)};
+  float specular = word_float(m, 22), transmission = word_float(m, 23), clearcoat = word_float(m, 27);
+  float sheen_roughness = word_float(m, 31), clearcoat_roughness = word_float(m, 32);
+  V3 specular_color = V3{word_float(m, 24), word_float(m, 25), word_float(m, 26)};
+  V3 sheen_color = V3{word_float(m, 28), word_float(m, 29), word_float(m, 30)};
+  const V3 mesh_normal = sf.shading_normal;
+  V3 coat_tn{0.0f, 0.0f, 1.0f};
+  const uint32_t* normal_input = nullptr;
+  const uint32_t* coat_input = nullptr;
+  float s = 0.0f, tc = 0.0f;
+  if (m[2] & 0x3FFEu) hit_texcoords(S, m, instance, primitive, u, v, s, tc);
+  if (m[2] & 0x3Eu) {
+    if (const uint32_t* b = material_input(S, m, 2)) {
+      const float4 c = texture_lookup(S, b, s, tc);
+      base = V3{saturate(c.x), saturate(c.y), saturate(c.z)};
+    }
+    if (const uint32_t* b = material_input(S, m, 4)) roughness = saturate(channel_of(texture_lookup(S, b, s, tc), b[2]));
+    if (const uint32_t* b = material_input(S, m, 3)) metallic = saturate(channel_of(texture_lookup(S, b, s, tc), b[2]));
+    if (const uint32_t* b = material_input(S, m, 5)) {
+      const float4 c = texture_lookup(S, b, s, tc);
+      emission = V3{maxf(c.x, 0.0f), maxf(c.y, 0.0f), maxf(c.z, 0.0f)} * strength;
+    }
+    if (const uint32_t* b = material_input(S, m, 1)) {
+      tn = clamp_normal(texture_lookup(S, b, s, tc));
+      normal_input = b;
+    }
+  }
+  if (m[2] & 0x3FC0u) {
+    specular = gltf_scalar(S, m, 6, s, tc, specular);
+    transmission = gltf_scalar(S, m, 8, s, tc, transmission);
+    clearcoat = gltf_scalar(S, m, 9, s, tc, clearcoat);
+    clearcoat_roughness = gltf_scalar(S, m, 10, s, tc, clearcoat_roughness);
+    sheen_roughness = gltf_scalar(S, m, 13, s, tc, sheen_roughness);
+    if (const uint32_t* b = material_input(S, m, 7)) {
+      const float4 c = texture_lookup(S, b, s, tc);
+      specular_color = V3{maxf(c.x, 0.0f), maxf(c.y, 0.0f), maxf(c.z, 0.0f)};
+    }
+    if (const uint32_t* b = material_input(S, m, 12)) {
+      const float4 c = texture_lookup(S, b, s, tc);
+      sheen_color = V3{saturate(c.x), saturate(c.y), saturate(c.z)};
+    }
+    if (const uint32_t* b = material_input(S, m, 11)) {
+      coat_tn = clamp_normal(texture_lookup(S, b, s, tc));
+      coat_input = b;
+    }
+  }
+  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f))
+    sf.shading_normal = map_normal<true>(S, tn, normal_input, instance, primitive, e1, e2, sf.shading_normal,
+                                         sf.normal, dir);
+  const float cos_o = dot3(sf.shading_normal, -dir);
+  V3 wo{sqrtf(maxf(1.0f - cos_o * cos_o, 0.0f)), 0.0f, cos_o};
+  GltfInputs in;
+  in.base_color = base;
+  in.roughness = roughness;
+  in.metallic = metallic;
+  in.dielectric_f0 = word_float(m, 21);
+  in.specular = specular;
+  in.specular_color = specular_color;
+  in.transmission = transmission;
+  in.clearcoat = clearcoat;
+  in.clearcoat_roughness = clearcoat_roughness;
+  in.clearcoat_normal = V3{0.0f, 0.0f, 1.0f};
+  in.sheen_color = sheen_color;
+  in.sheen_roughness = sheen_roughness;
+  if (clearcoat > 0.0f) {
+    V3 coat_normal = mesh_normal;
+    if (!(coat_tn.x == 0.0f && coat_tn.y == 0.0f && coat_tn.z == 1.0f))
+      coat_normal = map_normal<true>(S, coat_tn, coat_input, instance, primitive, e1, e2, mesh_normal, sf.normal, dir);
+    V3 ft, fb;
+    shading_frame(sf.shading_normal, ft, fb);
+    const V3 md = -dir;
+    wo.x = dot3(md, ft);
+    wo.y = dot3(md, fb);
+    in.clearcoat_normal = normalize3(V3{dot3(coat_normal, ft), dot3(coat_normal, fb), dot3(coat_normal, sf.shading_normal)});
+  }
+  sf.bsdf.model = kGltfModel;
+  sf.bsdf.gltf = make_gltf_closure(make_gltf_terms(S.albedo_tables, in, wo), wo);
+  sf.emission = emission;
+  sf.base_color = base;
+  sf.roughness = roughness;
+  sf.metallic = metallic;
+}
+
 // scene.slang:1408-1449 with EvaluatePreview (:1184-1260): authored shading
 // normals, vertex colours, texture lookups (with kExtras) and MappedNormal
-// (:541-553) for the tangent-space normal, constant or mapped.
-template <bool kExtras = true>  // normals, colours and texture lookups compiled in
+// (:541-553) for the tangent-space normal, constant or mapped; glTF PBR
+// materials (with kExtras) through gltf_surface.
+// kGltf: glTF PBR's closure compiled in (a variant of its own: its closure
+// needs far more registers than UsdPreviewSurface's)
+template <bool kExtras = true, bool kGltf = kExtras>  // normals, colours and lookups compiled in
 __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t tri, uint32_t instance,
                                                    float u, float v, V3 dir) {
   const Tri t = S.tris[tri];
@@ -346,6 +526,11 @@ __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t 
       sf.shading_normal = authored_normal(S, first + 3u * primitive, u, v, instance, sf.normal, dir);
   }
   const uint32_t* m = S.materials + 36ull * S.instance_material[instance];
+  sf.bsdf.model = kPreviewModel;
+  if (kGltf && (m[0] & 0xFFu) == kGltfModel) {
+    gltf_surface(S, m, instance, primitive, u, v, p1 - p0, p2 - p0, dir, sf);
+    return sf;
+  }
   const uint32_t flags = m[0] >> 8;
   V3 tn = V3{word_float(m, 8), word_float(m, 9), word_float(m, 10)};
   sf.base_color = V3{word_float(m, 12), word_float(m, 13), word_float(m, 14)};
@@ -385,27 +570,9 @@ __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t 
       if (finite3(value)) specular = V3{saturate(value.x), saturate(value.y), saturate(value.z)};
     }
   }
-  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f)) {
-    // MappedNormal (scene.slang:541-553) in the tangent frame
-    const V3 n = sf.shading_normal;
-    V3 bt, bb;
-    if (kExtras) {
-      tangent_frame(S, normal_input, instance, primitive, p1 - p0, p2 - p0, n, bt, bb);
-    } else {
-      const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
-      const float a = -1.0f / (sign + n.z);
-      const float b = n.x * n.y * a;
-      bt = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
-      bb = V3{b, sign + n.y * n.y * a, -n.y};
-    }
-    const float mag = maxf(maxf(fabsf(tn.x), fabsf(tn.y)), fabsf(tn.z));
-    if (mag > 0.0f && finite3(tn)) {
-      const V3 q = tn / mag;
-      V3 sh = normalize3(((q.x * bt) + (q.y * bb)) + (q.z * n));
-      if (dot3(sh, sf.normal) < 0.0f) sh = -sh;
-      sf.shading_normal = dot3(sh, dir) < 0.0f ? sh : n;
-    }
-  }
+  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f))
+    sf.shading_normal = map_normal<kExtras>(S, tn, normal_input, instance, primitive, p1 - p0, p2 - p0,
+                                            sf.shading_normal, sf.normal, dir);
   const float ior = word_float(m, 23);
   const V3 tcolor = V3{word_float(m, 24), word_float(m, 25), word_float(m, 26)};
   const float transmission = word_float(m, 27);
@@ -421,7 +588,7 @@ __device__ __forceinline__ Surface reconstruct_hit(const SceneView& S, uint32_t 
                            : normal;
     entering = dot3(outward, dir) < 0.0f;
   }
-  sf.bsdf = make_preview_closure(sf.base_color, sf.roughness, sf.metallic, ior, (flags & 1u) != 0, specular,
+  sf.bsdf.preview = make_preview_closure(sf.base_color, sf.roughness, sf.metallic, ior, (flags & 1u) != 0, specular,
                                  transmission, tcolor, (flags & 2u) != 0, entering);
   return sf;
 }
@@ -461,11 +628,15 @@ __device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S,
   const V3 e = cross3(p1 - p0, p2 - p0);
   const V3 normal = normalize3(e);
   const float area = 0.5f * length3(e);
+  // Emission (scene.slang:591-600): glTF PBR's emissive times its strength
+  const bool gltf = kExtras && (m[0] & 0xFFu) == kGltfModel;
   V3 emission = V3{word_float(m, 16), word_float(m, 17), word_float(m, 18)};
+  if (gltf) emission = emission * word_float(m, 20);
   if (kExtras && (m[2] & (1u << kEmissionSlot))) {
     float s, tc;
     hit_texcoords(S, m, chosen.instance, prim, bu, bv, s, tc);
     emission = preview_emission(S, m, s, tc);
+    if (gltf) emission = emission * word_float(m, 20);
   }
   // an emitter's radiance times its coverage, the probability that a ray
   // reaching it stops there (scene.slang:608-611, 656-657)
@@ -487,15 +658,6 @@ __device__ __forceinline__ LightSample sample_triangle_light(const SceneView& S,
   return ls;
 }
 
-// MakeShadingFrame (sampling.slang:45-56, Duff et al. 2017): tangent and
-// bitangent of unit n.
-__device__ __forceinline__ void shading_frame(V3 n, V3& ft, V3& fb) {
-  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
-  const float a = -1.0f / (sign + n.z);
-  const float b = n.x * n.y * a;
-  ft = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
-  fb = V3{b, sign + n.y * n.y * a, -n.y};
-}
 __device__ __forceinline__ V3 to_world(V3 v, V3 ft, V3 fb, V3 n) { return ((v.x * ft) + (v.y * fb)) + (v.z * n); }
 
 // scene.slang:682-689: the solid-angle density of a sphere's cone, given
@@ -1038,7 +1200,7 @@ __device__ __forceinline__ void add_emission(const SceneView& S, const FrameCons
 // eta (negated: smooth), material id low, high}.
 __device__ __forceinline__ void store_reuse_surface(float4* out, const Surface& sf, V3 origin, float depth, V3 wo,
                                                     uint64_t instance_id, uint64_t material_id) {
-  const Closure& b = sf.bsdf;
+  const Closure& b = sf.bsdf.preview;  // glTF PBR surfaces take no part in reuse (the renderer refuses them)
   out[0] = make_float4(origin.x, origin.y, origin.z, depth);
   out[1] = make_float4(sf.normal.x, sf.normal.y, sf.normal.z, __uint_as_float(static_cast<uint32_t>(instance_id)));
   out[2] = make_float4(sf.shading_normal.x, sf.shading_normal.y, sf.shading_normal.z,
@@ -1151,7 +1313,8 @@ struct NoParallelWalk {
   }
 };
 
-template <bool kAnalytic, bool kCov = kAnalytic, class Walk = NoParallelWalk>  // kCov: covered emitters' coverage
+template <bool kAnalytic, bool kCov = kAnalytic, class Walk = NoParallelWalk,
+          bool kGltf = false>  // kCov: covered emitters' coverage; kGltf: glTF PBR surfaces
 __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameConst& F, PathRegs& p,
                                                 uint32_t tri, uint32_t instance, uint32_t primitive, float u,
                                                 float v, Surface* keep = nullptr, uint32_t slot = 0,
@@ -1160,7 +1323,7 @@ __device__ __forceinline__ Scatter scatter_path(const SceneView& S, const FrameC
   out.continues = false;
   out.shadow = false;
   const bool sample_lights = F.light_sampling != 0;
-  const Surface sf = reconstruct_hit<kAnalytic>(S, tri, instance, u, v, p.d);
+  const Surface sf = reconstruct_hit<kAnalytic, kGltf>(S, tri, instance, u, v, p.d);
   if (keep) *keep = sf;
   add_emission(S, F, p, sf, instance, primitive);
   // the analytic lights the continuation ray crossed before the hit

@@ -64,6 +64,29 @@ def preview_material(base_color=(0.18, 0.18, 0.18), roughness=0.5, metallic=0.0,
     return w
 
 
+def gltf_material(base_color=(1.0, 1.0, 1.0), roughness=1.0, metallic=1.0, emissive=(0.0, 0.0, 0.0),
+                  emissive_strength=1.0, ior=1.5, specular=1.0, specular_color=(1.0, 1.0, 1.0), transmission=0.0,
+                  clearcoat=0.0, clearcoat_roughness=0.0, sheen_color=(0.0, 0.0, 0.0), sheen_roughness=0.0,
+                  opacity=1.0, opacity_threshold=0.0, normal=(0.0, 0.0, 1.0)) -> np.ndarray:
+    """A glTF PBR GpuMaterialRecord (gpu_scene.hpp:94-113) as 36 words, filled
+    as gpu_scene.cpp:2050-2071 fills it: the dielectric's reflectance at
+    normal incidence ((1 - ior) / (1 + ior))^2 computed in double. Defaults
+    are GltfPbr's (include/lotus/material.hpp:180-219)."""
+    w = np.zeros(MATERIAL_WORDS, np.uint32)
+    f = w.view(np.float32)
+    w[0] = 1  # model kGltfPbr
+    f[4:8] = (0.0, 0.0, opacity, opacity_threshold)
+    f[8:12] = (*normal, 0.0)
+    root = (1.0 - float(np.float32(ior))) / (1.0 + float(np.float32(ior)))
+    f[12:16] = (*base_color, roughness)
+    f[16:20] = (*emissive, metallic)
+    f[20:24] = (emissive_strength, np.float32(root * root), specular, transmission)
+    f[24:28] = (*specular_color, clearcoat)
+    f[28:32] = (*sheen_color, sheen_roughness)
+    f[32:34] = (clearcoat_roughness, ior)
+    return w
+
+
 # ----------------------------------------------------------------- scene --
 # lotus_hip_light (Light, include/lotus/render_world.hpp:89-114).
 LIGHT_SPHERE, LIGHT_RECT, LIGHT_DISK, LIGHT_DISTANT = range(4)

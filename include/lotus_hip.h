@@ -100,7 +100,9 @@ typedef struct lotus_hip_scene_plan {
   const float* instance_transform;          /* 16 floats per instance, column-major */
   const uint32_t* instance_material;        /* per instance: material table slot */
   uint32_t material_count;
-  const uint32_t* materials;                /* 36 words per material */
+  const uint32_t* materials;                /* 36 words per material: UsdPreviewSurface
+                                               (model 0) or glTF PBR (model 1), as
+                                               FlattenMaterial writes them */
   float environment[3];                     /* constant environment radiance */
   uint32_t bvh_leaf_size;                   /* 0: default (2); at most 8 */
   uint32_t light_count;                     /* analytic lights, in the light table after the triangles */
@@ -111,8 +113,9 @@ typedef struct lotus_hip_scene_plan {
    * material record's words 1-3 give its first binding and its textured
    * slots), and per-corner texture coordinates (2 floats per triangle
    * corner) with each geometry's first corner, 0xFFFFFFFF for a geometry
-   * without them. Opacity, normal (tangent space), base colour, roughness,
-   * metallic, emission and specular colour take lookups. */
+   * without them. Under UsdPreviewSurface opacity, normal (tangent space),
+   * base colour, roughness, metallic, emission and specular colour take
+   * lookups; under glTF PBR its fourteen slots do. */
   uint32_t texture_count;
   const lotus_hip_texture* textures;
   uint32_t binding_count;

@@ -485,6 +485,513 @@ bool BsdfSample(const Closure& b, F3 wo, float ux, float uy, float uz,
 // GpuMaterialRecord (gpu_scene.hpp:94-113): words 0 model_flags,
 // 1 lookup_base, 2-3 textured, 4-7 geometry_coverage, 8-11 geometry_normal,
 // 12-35 parameters[6][4].
+// ------------------------------------------------------------- glTF PBR --
+// The portable logarithm and exponential the kernels use for the albedo
+// tables' axes and the sheen (the same operations in the same order): the
+// atanh series of the mantissa reduced to [sqrt(1/2), sqrt(2)), and 2^k
+// times a degree-7 polynomial of the remainder.
+inline float LogPositive(float x) {
+  int e;
+  float m = std::frexp(x, &e);
+  if (m < 0.70710678f) {
+    m = m * 2.0f;
+    e -= 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  const float p = s2 * (0.333333343f + s2 * (0.2f + s2 * (0.142857149f + s2 * 0.111111112f)));
+  return static_cast<float>(e) * 0.693147182f + (2.0f * s + (2.0f * s) * p);
+}
+inline float ExpNonpositive(float t) {
+  if (!(t > -87.0f)) return 0.0f;
+  const float k = std::floor(t * 1.44269504f + 0.5f);
+  const float r = (t - k * 0.693145752f) - k * 1.42860677e-06f;
+  const float p =
+      1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f + r * (0.00833333377f +
+                                                                                   r * (0.00138888892f + r * 0.000198412701f))))));
+  return std::ldexp(p, static_cast<int>(k));
+}
+inline float PowUnit(float x, float y) {
+  if (!(x > 0.0f)) return 0.0f;
+  if (x >= 1.0f) return 1.0f;
+  return ExpNonpositive(y * LogPositive(x));
+}
+
+// The directional albedo tables (hydra-lotus_amd/csrc/albedo_tables.inc, the
+// data the renderer uploads, in albedo.slang's layout; pinned against the
+// reference's own table, brute-force quadrature and closed forms by
+// tests/test_oracle_gltf.py) and their lookups, albedo.slang:22-410.
+const float kAlbedoTables[] = {
+#include "../hydra-lotus_amd/csrc/albedo_tables.inc"
+};
+constexpr int kGgxCosinesT = 32, kDielectricCosinesT = 48, kAlphasT = 32, kIndicesT = 16;
+constexpr int kCriticalNodeT = kDielectricCosinesT / 2;
+constexpr float kCriticalCoordinateT = float(kCriticalNodeT) / float(kDielectricCosinesT - 1);
+
+float AxisScale(float v, float slope) { return LogPositive(v) + slope * v; }
+float AxisCoordinate(float v, float low, float high, float slope) {
+  const float c = std::min(std::max(v, low), high);
+  const float l = AxisScale(low, slope);
+  return (AxisScale(c, slope) - l) / (AxisScale(high, slope) - l);
+}
+float CosineCoordinate(float c) { return AxisCoordinate(c, 2.0e-4f, 1.0f, 4.0f); }
+float AlphaCoordinate(float a) { return AxisCoordinate(a, kMinAlpha, 1.0f, 2.0f); }
+float SheenCoordinate(float r) { return AxisCoordinate(r, 0.005f, 1.0f, 2.0f); }
+float IndexCoordinate(float i) { return AxisCoordinate(i - 1.0f, 1.01f - 1.0f, 3.0f - 1.0f, 0.5f); }
+float CriticalCosine(float index) { return Sqrt(std::max(1.0f - 1.0f / (index * index), 0.0f)); }
+float EvanescentScale(float mu, float index, float critical) {
+  const float tau = std::max(index * Sqrt(std::max(critical * critical - mu * mu, 0.0f)), 2.0e-3f);
+  return (LogPositive(mu) - LogPositive(tau)) + 4.0f * mu;
+}
+float LeavingCoordinate(float c, float index) {
+  const float critical = CriticalCosine(index);
+  if (c < critical) {
+    const float low = EvanescentScale(2.0e-4f, index, critical), high = EvanescentScale(critical, index, critical);
+    return kCriticalCoordinateT *
+           Saturate((EvanescentScale(std::max(c, 2.0e-4f), index, critical) - low) / (high - low));
+  }
+  const float refracted = Sqrt(std::max(1.0f - index * index * (1.0f - c * c), 0.0f));
+  return kCriticalCoordinateT + CosineCoordinate(refracted) * (1.0f - kCriticalCoordinateT);
+}
+void CubicWeights(float t, float w[4]) {
+  const float t2 = t * t, t3 = t2 * t;
+  w[0] = ((-0.5f * t3) + t2) - 0.5f * t;
+  w[1] = ((1.5f * t3) - 2.5f * t2) + 1.0f;
+  w[2] = ((-1.5f * t3) + 2.0f * t2) + 0.5f * t;
+  w[3] = (0.5f * t3) - 0.5f * t2;
+}
+float FetchNode(int base, int cosines, int c, int a, int first, int last) {
+  const int ac = std::min(std::max(a, 0), kAlphasT - 1), cc = std::min(std::max(c, first), last);
+  const int dc = c < first ? 1 : (c > last ? -1 : 0), da = a < 0 ? 1 : (a > kAlphasT - 1 ? -1 : 0);
+  const float at = kAlbedoTables[base + ac * cosines + cc];
+  if (dc == 0 && da == 0) return at;
+  return 2.0f * at - kAlbedoTables[base + (ac + da) * cosines + cc + dc];
+}
+float FetchSlice(int base, int cosines, float xc, float xa, bool split) {
+  const float fc = Saturate(xc) * float(cosines - 1), fa = Saturate(xa) * float(kAlphasT - 1);
+  const int c0 = int(std::min(uint32_t(fc), uint32_t(cosines - 2)));
+  const int a0 = int(std::min(uint32_t(fa), uint32_t(kAlphasT - 2)));
+  int first = 0, last = cosines - 1;
+  if (split) (c0 < kCriticalNodeT ? last : first) = kCriticalNodeT;
+  float wc[4], wa[4];
+  CubicWeights(fc - float(c0), wc);
+  CubicWeights(fa - float(a0), wa);
+  float value = 0.0f;
+  for (int j = 0; j < 4; ++j) {
+    float row = 0.0f;
+    for (int i = 0; i < 4; ++i) row = row + wc[i] * FetchNode(base, cosines, c0 + i - 1, a0 + j - 1, first, last);
+    value = value + wa[j] * row;
+  }
+  return value;
+}
+float AlbedoGgx(float c, float alpha) {
+  return FetchSlice(0, kGgxCosinesT, CosineCoordinate(c), AlphaCoordinate(alpha), false);
+}
+float AlbedoSchlickShare(float c, float alpha) {
+  return FetchSlice(1024, kGgxCosinesT, CosineCoordinate(c), AlphaCoordinate(alpha), false);
+}
+float AlbedoSheen(float c, float roughness) {
+  return std::max(FetchSlice(75776, kGgxCosinesT, CosineCoordinate(c), SheenCoordinate(roughness), false), 0.0f);
+}
+float AlbedoDielectric(float c, float alpha, float eta, bool thin) {
+  int sheet = thin ? 2 : 0;
+  float index = eta;
+  if (eta < 1.0f) {
+    index = 1.0f / eta;
+    sheet = thin ? 2 : 1;
+  }
+  index = std::min(std::max(index, 1.01f), 3.0f);
+  const float fi = IndexCoordinate(index) * float(kIndicesT - 1);
+  const int i0 = int(std::min(uint32_t(fi), uint32_t(kIndicesT - 2)));
+  const bool split = sheet == 1;
+  const float xc = split ? LeavingCoordinate(c, index) : CosineCoordinate(c), xa = AlphaCoordinate(alpha);
+  const int slice_size = kDielectricCosinesT * kAlphasT, base = 2048 + sheet * kIndicesT * slice_size;
+  float slices[4];
+  for (int j = 0; j < 4; ++j) {
+    const int slice = std::min(std::max(i0 + j - 1, 0), kIndicesT - 1);
+    slices[j] = FetchSlice(base + slice * slice_size, kDielectricCosinesT, xc, xa, split);
+  }
+  if (i0 == 0) slices[0] = 2.0f * slices[1] - slices[2];
+  if (i0 == kIndicesT - 2) slices[3] = 2.0f * slices[2] - slices[1];
+  float w[4];
+  CubicWeights(fi - float(i0), w);
+  return ((w[0] * slices[0] + w[1] * slices[1]) + w[2] * slices[2]) + w[3] * slices[3];
+}
+
+// lobes.slang: Lambert, GGX with Schlick's Fresnel (reflected, or mirrored
+// through the tangent plane), the GGX dielectric interface in its reflecting
+// mode, and the sheen; each about its own normal.
+enum LobeType { kLambertLobe = 0, kGgxSchlickLobe = 1, kGgxDielectricLobe = 2, kSheenLobe = 3 };
+struct Lobe {
+  int type = kLambertLobe;
+  F3 normal{0, 0, 1};
+  F3 color{1, 1, 1};
+  F3 f90{1, 1, 1};
+  float alpha = 1.0f, eta = 1.0f;
+  bool transmit = false, compensate = false;
+  F3 compensation{1, 1, 1};
+};
+Lobe MakeLambertLobe(F3 color) {
+  Lobe l;
+  l.type = kLambertLobe;
+  l.color = color;
+  return l;
+}
+Lobe MakeGgxSchlickLobe(F3 f0, F3 f90, float alpha, bool compensate) {
+  Lobe l;
+  l.type = kGgxSchlickLobe;
+  l.color = f0;
+  l.f90 = f90;
+  l.alpha = std::max(alpha, kMinAlpha);
+  l.compensate = compensate;
+  return l;
+}
+Lobe MakeSheenLobe(float roughness) {
+  Lobe l;
+  l.type = kSheenLobe;
+  l.alpha = std::max(roughness, 0.005f);
+  return l;
+}
+// The coat: a reflecting dielectric of index 1.5 about its own normal
+// (never smooth: its alpha is at least the floor).
+Lobe MakeCoatLobe(float alpha, F3 normal) {
+  Lobe l;
+  l.type = kGgxDielectricLobe;
+  l.alpha = std::max(alpha, kMinAlpha);
+  l.eta = 1.5f;
+  l.normal = normal;
+  return l;
+}
+Frame LobeFrame(F3 n) { return MakeShadingFrame(n); }
+float GgxReflection(F3 wo, F3 wi, F3 h, float alpha) {
+  const float g2 = 1.0f / (1.0f + GgxLambda(wo, alpha) + GgxLambda(wi, alpha));
+  return GgxD(h, alpha) * g2 / (4.0f * wo.z);
+}
+float GgxReflectionPdf(F3 wo, F3 h, float alpha) {
+  const float g1 = 1.0f / (1.0f + GgxLambda(wo, alpha));
+  return GgxD(h, alpha) * g1 / (4.0f * wo.z);
+}
+F3 SchlickSpectrum(F3 f0, F3 f90, float c) {
+  const float w = Pow5(Saturate(1.0f - c));
+  return f0 * (1.0f - w) + f90 * w;
+}
+bool LobeScattersToward(const Lobe& l, F3 wi) { return l.transmit ? wi.z < 0.0f : wi.z > 0.0f; }
+F3 LobeValue(const Lobe& l, F3 wo, F3 wi) {
+  const F3 zero{0, 0, 0};
+  if (!(wo.z > 0.0f)) return zero;
+  switch (l.type) {
+    case kLambertLobe:
+      return wi.z > 0.0f ? l.color * (wi.z / kPi) : zero;
+    case kGgxSchlickLobe: {
+      const F3 wr = l.transmit ? F3{wi.x, wi.y, -wi.z} : wi;
+      if (!(wr.z > 0.0f)) return zero;
+      const F3 h = Normalize(wo + wr);
+      return (SchlickSpectrum(l.color, l.f90, Dot(wo, h)) * l.compensation) * GgxReflection(wo, wr, h, l.alpha);
+    }
+    case kSheenLobe: {
+      if (!(wi.z > 0.0f)) return zero;
+      const F3 h = Normalize(wo + wi);
+      const float inverse = 1.0f / l.alpha;
+      const float sin2 = std::max(1.0f - h.z * h.z, 0.0f);
+      const float d = (2.0f + inverse) * PowUnit(sin2, 0.5f * inverse) / (2.0f * kPi);
+      return (l.color * l.compensation) * (d * wi.z / (4.0f * ((wi.z + wo.z) - wi.z * wo.z)));
+    }
+    default: {
+      if (!(wi.z > 0.0f)) return zero;
+      const F3 h = Normalize(wo + wi);
+      return (l.color * l.compensation) * (FresnelDielectric(Dot(wo, h), l.eta) * GgxReflection(wo, wi, h, l.alpha));
+    }
+  }
+}
+float LobePdf(const Lobe& l, F3 wo, F3 wi) {
+  if (!(wo.z > 0.0f)) return 0.0f;
+  switch (l.type) {
+    case kLambertLobe:
+      return wi.z > 0.0f ? wi.z / kPi : 0.0f;
+    case kGgxSchlickLobe: {
+      const F3 wr = l.transmit ? F3{wi.x, wi.y, -wi.z} : wi;
+      return wr.z > 0.0f ? GgxReflectionPdf(wo, Normalize(wo + wr), l.alpha) : 0.0f;
+    }
+    case kSheenLobe:
+      return wi.z > 0.0f ? 1.0f / (2.0f * kPi) : 0.0f;
+    default:
+      return wi.z > 0.0f ? GgxReflectionPdf(wo, Normalize(wo + wi), l.alpha) : 0.0f;
+  }
+}
+bool LobeSample(const Lobe& l, F3 wo, float ux, float uy, F3& wi) {
+  if (!(wo.z > 0.0f)) return false;
+  switch (l.type) {
+    case kLambertLobe:
+      wi = SampleCosineHemisphere(ux, uy);
+      return wi.z > 0.0f;
+    case kGgxSchlickLobe: {
+      const F3 h = SampleGgxVisibleNormal(wo, l.alpha, ux, uy);
+      wi = (2.0f * Dot(wo, h)) * h - wo;
+      if (!(wi.z > 0.0f)) return false;
+      if (l.transmit) wi.z = -wi.z;
+      return true;
+    }
+    case kSheenLobe: {
+      const float sin_theta = Sqrt(std::max(1.0f - ux * ux, 0.0f));
+      float s, c;
+      SinCosTurns(uy, &s, &c);
+      wi = F3{sin_theta * c, sin_theta * s, ux};
+      return wi.z > 0.0f;
+    }
+    default: {
+      const F3 h = SampleGgxVisibleNormal(wo, l.alpha, ux, uy);
+      const float cos_oh = Dot(wo, h);
+      if (!(cos_oh > 0.0f)) return false;
+      wi = (2.0f * cos_oh) * h - wo;
+      return wi.z > 0.0f;
+    }
+  }
+}
+
+// composed.slang: a list of lobes with weights, built by a model's builder
+// with the albedo and estimate of each lobe toward wo beside it.
+constexpr int kMaxLobes = 6;
+struct Composed {
+  int count = 0;
+  Lobe lobes[kMaxLobes];
+  F3 weights[kMaxLobes];
+  float probabilities[kMaxLobes] = {};
+};
+struct ComposedBuilder {
+  Composed c;
+  F3 albedos[kMaxLobes];
+  F3 estimates[kMaxLobes];
+};
+int ComposedPush(ComposedBuilder& b, const Lobe& lobe, F3 albedo, F3 estimate) {
+  const int k = b.c.count;
+  if (k >= kMaxLobes) return k;
+  b.c.count = k + 1;
+  b.c.weights[k] = F3{1, 1, 1};
+  b.c.lobes[k] = lobe;
+  b.albedos[k] = albedo;
+  b.estimates[k] = estimate;
+  return k;
+}
+int ComposedAddLambert(ComposedBuilder& b, const Lobe& lobe, F3 wo) {
+  const F3 a = Dot(wo, lobe.normal) > 0.0f ? lobe.color : F3{0, 0, 0};
+  return ComposedPush(b, lobe, a, a);
+}
+int ComposedAddSchlick(ComposedBuilder& b, Lobe lobe, F3 wo, float e1, float share) {
+  F3 albedo{0, 0, 0};
+  if (Dot(wo, lobe.normal) > 0.0f) {
+    if (lobe.compensate && e1 > 0.0f) lobe.compensation = F3{1, 1, 1} + lobe.color * ((1.0f - e1) / e1);
+    albedo = (lobe.color * (e1 - share) + lobe.f90 * share) * lobe.compensation;
+  }
+  return ComposedPush(b, lobe, lobe.transmit ? F3{0, 0, 0} : albedo, albedo);
+}
+int ComposedAddWeighted(ComposedBuilder& b, const Lobe& lobe, F3 weight, F3 albedo) {
+  const int k = ComposedPush(b, lobe, albedo, albedo * lobe.color);
+  if (k < kMaxLobes) b.c.weights[k] = weight;
+  return k;
+}
+void ComposedScale(ComposedBuilder& b, int begin, int end, F3 s) {
+  for (int k = begin; k < std::min(end, b.c.count); ++k) b.c.weights[k] = b.c.weights[k] * s;
+}
+void ComposedMix(ComposedBuilder& b, int begin, int split, int end, float mix) {
+  ComposedScale(b, begin, split, F3{1.0f - mix, 1.0f - mix, 1.0f - mix});
+  ComposedScale(b, split, end, F3{mix, mix, mix});
+}
+F3 ComposedAlbedo(const ComposedBuilder& b, int begin, int end) {
+  F3 a{0, 0, 0};
+  for (int k = begin; k < std::min(end, b.c.count); ++k) a = a + b.c.weights[k] * b.albedos[k];
+  return a;
+}
+void ComposedLayer(ComposedBuilder& b, int begin, int split, int end) {
+  const F3 top = ComposedAlbedo(b, begin, split);
+  ComposedScale(b, split, end, F3{Saturate(1.0f - top.x), Saturate(1.0f - top.y), Saturate(1.0f - top.z)});
+}
+void ComposedLayerScalar(ComposedBuilder& b, int begin, int split, int end) {
+  const float s = Saturate(1.0f - MaxComponent(ComposedAlbedo(b, begin, split)));
+  ComposedScale(b, split, end, F3{s, s, s});
+}
+Composed ComposedFinalize(const ComposedBuilder& b) {
+  Composed c = b.c;
+  float total = 0.0f;
+  for (int k = 0; k < c.count; ++k) {
+    const float e = MaxComponent(c.weights[k] * b.estimates[k]);
+    c.probabilities[k] = e > 0.0f ? e : 0.0f;
+    total += c.probabilities[k];
+  }
+  if (!(total > 0.0f)) return c;
+  float floored = 0.0f;
+  for (int k = 0; k < c.count; ++k)
+    if (c.probabilities[k] > 0.0f) {
+      c.probabilities[k] = std::max(c.probabilities[k] / total, 0.05f);
+      floored += c.probabilities[k];
+    }
+  for (int k = 0; k < c.count; ++k) c.probabilities[k] /= floored;
+  return c;
+}
+F3 ComposedEvaluate(const Composed& c, F3 wo, F3 wi) {
+  F3 v{0, 0, 0};
+  for (int k = 0; k < c.count; ++k) {
+    const Lobe& l = c.lobes[k];
+    if (!LobeScattersToward(l, wi)) continue;
+    const Frame f = LobeFrame(l.normal);
+    v = v + c.weights[k] * LobeValue(l, f.ToLocal(wo), f.ToLocal(wi));
+  }
+  return v;
+}
+float ComposedPdf(const Composed& c, F3 wo, F3 wi) {
+  float pdf = 0.0f;
+  for (int k = 0; k < c.count; ++k) {
+    const Lobe& l = c.lobes[k];
+    if (!(c.probabilities[k] > 0.0f) || !LobeScattersToward(l, wi)) continue;
+    const Frame f = LobeFrame(l.normal);
+    pdf += c.probabilities[k] * LobePdf(l, f.ToLocal(wo), f.ToLocal(wi));
+  }
+  return pdf;
+}
+bool ComposedSample(const Composed& c, F3 wo, float ux, float uy, float uz, SampleResult& s) {
+  s.wi = F3{0, 0, 1};
+  s.weight = F3{0, 0, 0};
+  s.pdf = 0.0f;
+  s.delta = false;
+  s.transmitted = false;
+  int chosen = kMaxLobes;
+  float end = 0.0f;
+  for (int k = 0; k < c.count; ++k) {
+    const float p = c.probabilities[k];
+    if (!(p > 0.0f)) continue;
+    chosen = k;
+    end += p;
+    if (uz < end) break;
+  }
+  if (chosen == kMaxLobes) return false;
+  const Lobe& l = c.lobes[chosen];
+  const Frame f = LobeFrame(l.normal);
+  F3 local;
+  if (!LobeSample(l, f.ToLocal(wo), ux, uy, local)) return false;
+  s.wi = f.ToWorld(local);
+  if (!LobeScattersToward(l, s.wi)) return false;
+  s.transmitted = s.wi.z < 0.0f;
+  s.pdf = ComposedPdf(c, wo, s.wi);
+  if (!(s.pdf > 0.0f)) return false;
+  s.weight = ComposedEvaluate(c, wo, s.wi) / s.pdf;
+  return IsFinite(s.weight);
+}
+
+// gltf_pbr.slang: the model's terms (MakeGltfPbrTerms, :99-155) and its
+// closure (MakeGltfPbrClosure, :218-276).
+struct GltfInputs {
+  F3 base_color;
+  float roughness, metallic, dielectric_f0, specular;
+  F3 specular_color;
+  float transmission, clearcoat, clearcoat_roughness;
+  F3 clearcoat_normal{0, 0, 1};
+  F3 sheen_color;
+  float sheen_roughness;
+};
+struct GltfTerms {
+  F3 base_color;
+  float alpha, metallic, e1, share;
+  F3 f0;
+  float f90, transmission, coat_scale, coat_dim, coat_alpha;
+  F3 coat_normal;
+  F3 sheen_color;
+  float sheen_roughness, sheen_dim;
+};
+GltfTerms MakeGltfTerms(const GltfInputs& in, F3 wo) {
+  GltfTerms t;
+  t.base_color = in.base_color;
+  t.alpha = std::max(in.roughness * in.roughness, kMinAlpha);
+  t.metallic = in.metallic;
+  t.e1 = 0.0f;
+  t.share = 0.0f;
+  if (wo.z > 0.0f) {
+    t.e1 = AlbedoGgx(wo.z, t.alpha);
+    t.share = AlbedoSchlickShare(wo.z, t.alpha);
+  }
+  const F3 f0 = in.specular_color * in.dielectric_f0;
+  t.f0 = F3{Saturate(f0.x), Saturate(f0.y), Saturate(f0.z)} * in.specular;
+  t.f90 = in.specular;
+  t.transmission = in.transmission;
+  t.coat_scale = 0.0f;
+  t.coat_dim = 0.0f;
+  t.coat_alpha = std::max(in.clearcoat_roughness * in.clearcoat_roughness, kMinAlpha);
+  t.coat_normal = in.clearcoat_normal.z > 0.0f ? in.clearcoat_normal : F3{0, 0, 1};
+  const float coat_cos = Dot(wo, t.coat_normal);
+  if (in.clearcoat > 0.0f && coat_cos > 0.0f) {
+    const float reflected = AlbedoDielectric(coat_cos, t.coat_alpha, 1.5f, false);
+    const float coat_e1 = AlbedoGgx(coat_cos, t.coat_alpha);
+    const float root = (1.0f - 1.5f) / (1.0f + 1.5f);
+    float compensation = 1.0f;
+    if (coat_e1 > 0.0f) compensation += root * root * (1.0f - coat_e1) / coat_e1;
+    t.coat_scale = in.clearcoat * compensation;
+    t.coat_dim = Saturate(t.coat_scale * reflected);
+  }
+  t.sheen_color = in.sheen_color;
+  t.sheen_roughness = std::max(in.sheen_roughness * in.sheen_roughness, 0.005f);
+  t.sheen_dim = 0.0f;
+  const float sheen = MaxComponent(in.sheen_color);
+  if (sheen > 0.0f && wo.z > 0.0f) {
+    const float e = AlbedoSheen(wo.z, t.sheen_roughness);
+    if (e > 1.0f) t.sheen_color = in.sheen_color * (1.0f / e);
+    t.sheen_dim = Saturate(sheen * std::min(e, 1.0f));
+  }
+  return t;
+}
+Composed MakeGltfClosure(const GltfTerms& t, F3 wo) {
+  const F3 white{1, 1, 1};
+  ComposedBuilder b;
+  const int coat = b.c.count;
+  if (t.coat_scale > 0.0f) {
+    const float a = t.coat_dim / t.coat_scale;
+    ComposedAddWeighted(b, MakeCoatLobe(t.coat_alpha, t.coat_normal), F3{t.coat_scale, t.coat_scale, t.coat_scale},
+                        F3{a, a, a});
+  }
+  const int sheen = b.c.count;
+  if (t.sheen_dim > 0.0f) {
+    const float a = t.sheen_dim / MaxComponent(t.sheen_color);
+    ComposedAddWeighted(b, MakeSheenLobe(t.sheen_roughness), t.sheen_color, F3{a, a, a});
+  }
+  const bool dielectric = t.metallic < 1.0f;
+  const int top = b.c.count;
+  if (dielectric && t.f90 > 0.0f)
+    ComposedAddSchlick(b, MakeGgxSchlickLobe(t.f0, F3{t.f90, t.f90, t.f90}, t.alpha, true), wo, t.e1, t.share);
+  const int diffuse = b.c.count;
+  if (dielectric && t.transmission < 1.0f) ComposedAddLambert(b, MakeLambertLobe(t.base_color), wo);
+  const int transmitted = b.c.count;
+  if (dielectric && t.transmission > 0.0f) {
+    Lobe lobe = MakeGgxSchlickLobe(white, white, t.alpha, true);
+    lobe.transmit = true;
+    const int k = ComposedAddSchlick(b, lobe, wo, t.e1, t.share);
+    ComposedScale(b, k, k + 1, t.base_color);
+  }
+  const int metal = b.c.count;
+  if (t.transmission > 0.0f) ComposedMix(b, diffuse, transmitted, metal, t.transmission);
+  ComposedLayerScalar(b, top, diffuse, metal);
+  if (t.metallic > 0.0f)
+    ComposedAddSchlick(b, MakeGgxSchlickLobe(t.base_color, white, t.alpha, true), wo, t.e1, t.share);
+  const int end = b.c.count;
+  ComposedMix(b, top, metal, end, t.metallic);
+  ComposedLayerScalar(b, sheen, top, end);
+  ComposedLayer(b, coat, sheen, end);
+  return ComposedFinalize(b);
+}
+
+// The closure of a hit, whatever its model (bsdf.slang:23-90).
+constexpr uint32_t kPreviewModel = 0, kGltfModel = 1;
+struct Bsdf {
+  uint32_t model = kPreviewModel;
+  Closure preview{};
+  Composed composed{};
+};
+inline F3 BsdfEvaluate(const Bsdf& b, F3 wo, F3 wi) {
+  return b.model == kGltfModel ? ComposedEvaluate(b.composed, wo, wi) : BsdfEvaluate(b.preview, wo, wi);
+}
+inline float BsdfPdf(const Bsdf& b, F3 wo, F3 wi) {
+  return b.model == kGltfModel ? ComposedPdf(b.composed, wo, wi) : BsdfPdf(b.preview, wo, wi);
+}
+inline bool BsdfSample(const Bsdf& b, F3 wo, float ux, float uy, float uz, SampleResult& s) {
+  return b.model == kGltfModel ? ComposedSample(b.composed, wo, ux, uy, uz, s) : BsdfSample(b.preview, wo, ux, uy, uz, s);
+}
+
 struct Material {
   uint32_t flags;
   F3 base_color;
@@ -499,6 +1006,14 @@ struct Material {
   uint32_t lookup_base, textured;  // the binding table's first entry, the slots with lookups
   float fallback_st[2];            // texture coordinates without a set
   float opacity, opacity_threshold;
+  // ShadingModel (gpu_scene.hpp:82-92); glTF PBR's parameters
+  // (gpu_scene.cpp:2050-2071): emissive strength, the dielectric's
+  // reflectance at normal incidence, specular, transmission, specular colour,
+  // clearcoat, sheen colour and roughness, clearcoat roughness
+  uint32_t model = 0;
+  float g_strength = 1.0f, g_f0 = 0.04f, g_specular = 1.0f, g_transmission = 0.0f, g_clearcoat = 0.0f;
+  F3 g_specular_color{1, 1, 1}, g_sheen_color{0, 0, 0};
+  float g_sheen_roughness = 0.0f, g_clearcoat_roughness = 0.0f;
 };
 inline float WordFloat(const uint32_t* w, int i) {
   float f;
@@ -523,6 +1038,20 @@ Material DecodeMaterial(const uint32_t* w) {
   m.fallback_st[1] = WordFloat(w, 5);
   m.opacity = WordFloat(w, 6);
   m.opacity_threshold = WordFloat(w, 7);
+  m.model = w[0] & 0xFFu;
+  if (m.model == 1u) {
+    m.flags = 0;
+    m.g_strength = WordFloat(w, 20);
+    m.g_f0 = WordFloat(w, 21);
+    m.g_specular = WordFloat(w, 22);
+    m.g_transmission = WordFloat(w, 23);
+    m.g_specular_color = F3{WordFloat(w, 24), WordFloat(w, 25), WordFloat(w, 26)};
+    m.g_clearcoat = WordFloat(w, 27);
+    m.g_sheen_color = F3{WordFloat(w, 28), WordFloat(w, 29), WordFloat(w, 30)};
+    m.g_sheen_roughness = WordFloat(w, 31);
+    m.g_clearcoat_roughness = WordFloat(w, 32);
+    m.transmission = 0.0f;  // the preview words here hold glTF's parameters
+  }
   return m;
 }
 // Whether a ray tests the material's coverage: an opacity lookup, or a
@@ -919,7 +1448,9 @@ double EmissionEstimate(const Material& m, const std::vector<uint32_t>& bindings
       e[c] = b[3] == 2u ? std::max(0.0f, WordFloat(b, 12 + c))
                         : double(std::max(0.0f, WordFloat(b, 4 + c))) + std::max(0.0f, WordFloat(b, 8 + c));
   }
-  const double mean = (e[0] + e[1] + e[2]) / 3.0;
+  // glTF PBR's emissive strength is a constant factor on the estimate
+  const double factor = m.model == 1u ? double(m.g_strength) : 1.0;
+  const double mean = factor * (e[0] + e[1] + e[2]) / 3.0;
   return std::isfinite(mean) && mean > 0.0 ? mean : 0.0;
 }
 double WorldTriangleArea(const F3* object, const float* t) {
@@ -1021,8 +1552,9 @@ F3 OffsetRayOrigin(F3 p, F3 n) {
 struct Surface {
   F3 position, normal, shading_normal;
   float area;
-  Closure bsdf;
+  Bsdf bsdf;
   F3 emission;
+  GltfTerms gltf{};  // what a glTF PBR closure was built from
 };
 
 // scene.slang:1408-1449 with EvaluatePreview (:1184-1260) for a constant
@@ -1124,6 +1656,18 @@ F3 PreviewEmission(const Scene& s, const Material& m, float st_s, float st_t) {
   Lookup(s.tex, b, st_s, st_t, e);
   return F3{std::max(e[0], 0.0f), std::max(e[1], 0.0f), std::max(e[2], 0.0f)};
 }
+// scene.slang:579-589 (GltfEmission): emissive times its strength.
+F3 GltfEmission(const Scene& s, const Material& m, float st_s, float st_t) {
+  const uint32_t* b = MaterialInput(s.tex, m, 5);
+  if (!b) return m.emission * m.g_strength;
+  float e[4];
+  Lookup(s.tex, b, st_s, st_t, e);
+  return F3{std::max(e[0], 0.0f), std::max(e[1], 0.0f), std::max(e[2], 0.0f)} * m.g_strength;
+}
+// scene.slang:591-600 (Emission) for a constant emission, or at (st_s, st_t).
+F3 MaterialEmission(const Scene& s, const Material& m, float st_s, float st_t) {
+  return m.model == 1u ? GltfEmission(s, m, st_s, st_t) : PreviewEmission(s, m, st_s, st_t);
+}
 
 // scene.slang:467-488 (ShadingNormal): the instance's corner normals
 // interpolated at the hit, in world space by the cofactor matrix of the
@@ -1202,9 +1746,125 @@ Frame TangentFrame(const Scene& s, const uint32_t* input, const Hit& hit, F3 edg
   return basis;
 }
 
+// scene.slang:541-553 (MappedNormal) of tangent-space normal `tn` in the
+// tangent frame about `base` (TangentFrame with the normal map's lookup
+// `input`): the shading normal, or `base` where the mapped one faces away.
+F3 MapNormal(const Scene& s, const Hit& hit, F3 tn, const uint32_t* input, F3 edge1, F3 edge2, F3 base,
+             F3 geometric, F3 dir) {
+  const Frame basis = TangentFrame(s, input, hit, edge1, edge2, base);
+  const float mag = std::max(std::max(std::fabs(tn.x), std::fabs(tn.y)), std::fabs(tn.z));
+  if (!(mag > 0.0f) || !IsFinite(tn)) return base;
+  F3 sh = Normalize(basis.ToWorld(tn / mag));
+  if (Dot(sh, geometric) < 0.0f) sh = -sh;
+  return Dot(sh, dir) < 0.0f ? sh : basis.n;
+}
+
+// A scalar glTF input (scene.slang:1262-1274): its lookup's channel,
+// clamped, where the slot has one, else the constant.
+float GltfScalar(const Scene& s, const Material& m, uint32_t slot, float st_s, float st_t, float constant) {
+  const uint32_t* b = MaterialInput(s.tex, m, slot);
+  if (!b) return constant;
+  float v[4];
+  Lookup(s.tex, b, st_s, st_t, v);
+  return Saturate(v[b[2]]);
+}
+
+// scene.slang:1276-1403 (EvaluateGltfPbr) into `sf`, whose geometry and
+// shading normal are set: the inputs and their lookups, the base's and the
+// coat's normal maps, the closure's outgoing direction, terms and closure.
+void EvaluateGltf(const Scene& s, const Material& m, const Hit& hit, F3 edge1, F3 edge2, F3 dir, Surface& sf) {
+  F3 base_color = m.base_color, emission = m.emission * m.g_strength, tn = m.tangent_normal;
+  float roughness = m.roughness, metallic = m.metallic;
+  float specular = m.g_specular, transmission = m.g_transmission, clearcoat = m.g_clearcoat;
+  F3 specular_color = m.g_specular_color, sheen_color = m.g_sheen_color;
+  float sheen_roughness = m.g_sheen_roughness, clearcoat_roughness = m.g_clearcoat_roughness;
+  const F3 mesh_normal = sf.shading_normal;
+  F3 coat_tn{0.0f, 0.0f, 1.0f};
+  const uint32_t* normal_input = nullptr;
+  const uint32_t* coat_input = nullptr;
+  float st_s = 0.0f, st_t = 0.0f;
+  if ((m.textured & 0x3FFEu) != 0) HitTexcoords(s, m, hit.instance, hit.primitive, hit.u, hit.v, st_s, st_t);
+  if ((m.textured & 0x3Eu) != 0) {
+    float v[4];
+    if (const uint32_t* b = MaterialInput(s.tex, m, 2)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      base_color = F3{Saturate(v[0]), Saturate(v[1]), Saturate(v[2])};
+    }
+    if (const uint32_t* b = MaterialInput(s.tex, m, 4)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      roughness = Saturate(v[b[2]]);
+    }
+    if (const uint32_t* b = MaterialInput(s.tex, m, 3)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      metallic = Saturate(v[b[2]]);
+    }
+    emission = GltfEmission(s, m, st_s, st_t);
+    if (const uint32_t* b = MaterialInput(s.tex, m, 1)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      tn = F3{std::min(std::max(v[0], -1.0f), 1.0f), std::min(std::max(v[1], -1.0f), 1.0f),
+              std::min(std::max(v[2], -1.0f), 1.0f)};
+      normal_input = b;
+    }
+  }
+  if ((m.textured & 0x3FC0u) != 0) {
+    float v[4];
+    specular = GltfScalar(s, m, 6, st_s, st_t, specular);
+    transmission = GltfScalar(s, m, 8, st_s, st_t, transmission);
+    clearcoat = GltfScalar(s, m, 9, st_s, st_t, clearcoat);
+    clearcoat_roughness = GltfScalar(s, m, 10, st_s, st_t, clearcoat_roughness);
+    sheen_roughness = GltfScalar(s, m, 13, st_s, st_t, sheen_roughness);
+    if (const uint32_t* b = MaterialInput(s.tex, m, 7)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      specular_color = F3{std::max(v[0], 0.0f), std::max(v[1], 0.0f), std::max(v[2], 0.0f)};
+    }
+    if (const uint32_t* b = MaterialInput(s.tex, m, 12)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      sheen_color = F3{Saturate(v[0]), Saturate(v[1]), Saturate(v[2])};
+    }
+    if (const uint32_t* b = MaterialInput(s.tex, m, 11)) {
+      Lookup(s.tex, b, st_s, st_t, v);
+      coat_tn = F3{std::min(std::max(v[0], -1.0f), 1.0f), std::min(std::max(v[1], -1.0f), 1.0f),
+                   std::min(std::max(v[2], -1.0f), 1.0f)};
+      coat_input = b;
+    }
+  }
+  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f))
+    sf.shading_normal = MapNormal(s, hit, tn, normal_input, edge1, edge2, sf.shading_normal, sf.normal, dir);
+  // the closure's outgoing direction: without a coat only its cosine counts
+  const float cos_o = Dot(sf.shading_normal, -dir);
+  F3 wo{Sqrt(std::max(1.0f - cos_o * cos_o, 0.0f)), 0.0f, cos_o};
+  GltfInputs in;
+  in.base_color = base_color;
+  in.roughness = roughness;
+  in.metallic = metallic;
+  in.dielectric_f0 = m.g_f0;
+  in.specular = specular;
+  in.specular_color = specular_color;
+  in.transmission = transmission;
+  in.clearcoat = clearcoat;
+  in.clearcoat_roughness = clearcoat_roughness;
+  in.sheen_color = sheen_color;
+  in.sheen_roughness = sheen_roughness;
+  if (clearcoat > 0.0f) {
+    F3 coat_normal = mesh_normal;
+    if (!(coat_tn.x == 0.0f && coat_tn.y == 0.0f && coat_tn.z == 1.0f))
+      coat_normal = MapNormal(s, hit, coat_tn, coat_input, edge1, edge2, mesh_normal, sf.normal, dir);
+    const Frame basis = MakeShadingFrame(sf.shading_normal);
+    const F3 local = basis.ToLocal(-dir);
+    wo.x = local.x;
+    wo.y = local.y;
+    in.clearcoat_normal = Normalize(basis.ToLocal(coat_normal));
+  }
+  sf.bsdf.model = kGltfModel;
+  sf.gltf = MakeGltfTerms(in, wo);
+  sf.bsdf.composed = MakeGltfClosure(sf.gltf, wo);
+  sf.emission = emission;
+}
+
 // scene.slang:1408-1449 (ReconstructHit) with EvaluatePreview
 // (:1184-1260): authored normals, vertex colours, texture lookups and
-// MappedNormal (:541-553) for the tangent-space normal.
+// MappedNormal (:541-553) for the tangent-space normal; glTF PBR through
+// EvaluateGltf.
 Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
   const uint32_t g = hit.global;
   const F3 p0 = s.v0[g], p1 = s.v1[g], p2 = s.v2[g];
@@ -1218,6 +1878,10 @@ Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
   if (!s.tex.instance_normals.empty() && s.tex.instance_normals[hit.instance] != 0xFFFFFFFFu)
     sf.shading_normal = AuthoredNormal(s, hit, sf.normal, dir);
   const Material& m = s.materials[s.instance_material[hit.instance]];
+  if (m.model == 1u) {
+    EvaluateGltf(s, m, hit, p1 - p0, p2 - p0, dir, sf);
+    return sf;
+  }
   F3 tn = m.tangent_normal;
   bool entering = true;
   if (m.transmission > 0.0f) {
@@ -1265,16 +1929,9 @@ Surface ReconstructHit(const Scene& s, const Hit& hit, F3 dir) {
       if (IsFinite(value)) specular = F3{Saturate(v[0]), Saturate(v[1]), Saturate(v[2])};
     }
   }
-  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f)) {
-    const Frame basis = TangentFrame(s, normal_input, hit, p1 - p0, p2 - p0, sf.shading_normal);
-    const float mag = std::max(std::max(std::fabs(tn.x), std::fabs(tn.y)), std::fabs(tn.z));
-    if (mag > 0.0f && IsFinite(tn)) {
-      F3 sh = Normalize(basis.ToWorld(tn / mag));
-      if (Dot(sh, sf.normal) < 0.0f) sh = -sh;
-      sf.shading_normal = Dot(sh, dir) < 0.0f ? sh : basis.n;
-    }
-  }
-  sf.bsdf = MakePreviewClosure(base_color, roughness, metallic, m.ior,
+  if (!(tn.x == 0.0f && tn.y == 0.0f && tn.z == 1.0f))
+    sf.shading_normal = MapNormal(s, hit, tn, normal_input, p1 - p0, p2 - p0, sf.shading_normal, sf.normal, dir);
+  sf.bsdf.preview = MakePreviewClosure(base_color, roughness, metallic, m.ior,
                                (m.flags & kSpecularWorkflow) != 0, specular,
                                m.transmission, m.transmission_color,
                                (m.flags & kThinWalled) != 0, entering);
@@ -1304,11 +1961,11 @@ LightSample SampleTriangleLight(const Scene& s, uint32_t light, float uz, float 
   const F3 e = Cross(p1 - p0, p2 - p0);
   const F3 normal = Normalize(e);
   const float area = 0.5f * Length(e);
-  F3 emission = m.emission;
+  F3 emission = m.model == 1u ? m.emission * m.g_strength : m.emission;
   if ((m.textured & (1u << 5)) != 0) {
     float st_s, st_t;
     HitTexcoords(s, m, chosen.instance, prim, bu, bv, st_s, st_t);
-    emission = PreviewEmission(s, m, st_s, st_t);
+    emission = MaterialEmission(s, m, st_s, st_t);
   }
   // an emitter's radiance times its coverage (scene.slang:608-611, 656-657)
   const F3 radiance = emission * (s.tri_covered[g] ? SurfaceCoverage(s, chosen.instance, prim, bu, bv) : 1.0f);
@@ -1789,8 +2446,8 @@ float ReservoirProposalPdf(const Scene& s, uint32_t mode, uint32_t index) {
 }
 
 // restir.slang:66-74: reflection-side BSDF values only.
-F3 RestirEvaluate(const Closure& b, F3 wo, F3 wi) { return wi.z > 0.0f ? BsdfEvaluate(b, wo, wi) : F3{0, 0, 0}; }
-float RestirPdf(const Closure& b, F3 wo, F3 wi) { return wi.z > 0.0f ? BsdfPdf(b, wo, wi) : 0.0f; }
+F3 RestirEvaluate(const Bsdf& b, F3 wo, F3 wi) { return wi.z > 0.0f ? BsdfEvaluate(b, wo, wi) : F3{0, 0, 0}; }
+float RestirPdf(const Bsdf& b, F3 wo, F3 wi) { return wi.z > 0.0f ? BsdfPdf(b, wo, wi) : 0.0f; }
 
 // restir.slang:157-161 (CandidateContribution): f in the light's unit-square
 // measure, the power selection `selection` cancelled.
@@ -2680,7 +3337,7 @@ struct ReuseSurface {
   float depth = 0.0f;
   F3 normal{0, 0, 0}, shading_normal{0, 0, 0}, wo{0, 0, 0};
   uint64_t instance_id = 0, material_id = 0;
-  Closure bsdf{};
+  Bsdf bsdf{};
 };
 
 inline float BitsFloat(uint32_t w) {
@@ -2695,7 +3352,7 @@ inline uint32_t FloatBits(float f) {
 }
 
 void StoreSurface(const ReuseSurface& r, float* out) {
-  const Closure& b = r.bsdf;
+  const Closure& b = r.bsdf.preview;
   const float v[32] = {r.origin.x, r.origin.y, r.origin.z, r.depth,
                        r.normal.x, r.normal.y, r.normal.z, BitsFloat(static_cast<uint32_t>(r.instance_id)),
                        r.shading_normal.x, r.shading_normal.y, r.shading_normal.z,
@@ -2719,7 +3376,7 @@ ReuseSurface LoadSurface(const float* s) {
   r.wo = F3{s[12], s[13], s[14]};
   r.instance_id = static_cast<uint64_t>(FloatBits(s[7])) | (static_cast<uint64_t>(FloatBits(s[11])) << 32);
   r.material_id = static_cast<uint64_t>(FloatBits(s[30])) | (static_cast<uint64_t>(FloatBits(s[31])) << 32);
-  Closure& b = r.bsdf;
+  Closure& b = r.bsdf.preview;
   b.dielectric_f90 = s[15];
   b.base_color = F3{s[16], s[17], s[18]};
   b.alpha = s[19];

@@ -229,6 +229,22 @@ size_t ref_material_fixture(const char* id, int control, uint32_t* out, size_t c
     push_f(m.geometry.opacity);
     push_f(m.geometry.opacity_threshold);
     push_fs(m.geometry.normal.data(), 3);
+    // glTF PBR's parameters (material.hpp:180-219), defaults for other models
+    Lotus::GltfPbr g;
+    if (const auto* gltf = std::get_if<Lotus::GltfPbr>(&m.surface)) g = *gltf;
+    push_fs(g.base_color.value.data(), 3);
+    push_f(g.roughness.value);
+    push_f(g.metallic.value);
+    push_fs(g.emissive.value.data(), 3);
+    push_f(g.emissive_strength);
+    push_f(g.ior);
+    push_f(g.specular.value);
+    push_fs(g.specular_color.value.data(), 3);
+    push_f(g.transmission.value);
+    push_f(g.clearcoat.value);
+    push_f(g.clearcoat_roughness.value);
+    push_fs(g.sheen_color.value.data(), 3);
+    push_f(g.sheen_roughness.value);
     const auto inputs = Lotus::TextureInputs(m);
     for (size_t slot = 0; slot < inputs.size(); ++slot) {
       const auto& input = *inputs[slot];

@@ -23,7 +23,8 @@ import oracle  # noqa: E402
 
 FIXTURES = ["material-basics-v1", "material-transmission-v1", "material-primvars-v1", "material-normals-v1",
             "material-textures-v1", "material-emission-v1", "material-alpha-v1", "material-transforms-v1",
-            "material-subsets-v1"]
+            "material-subsets-v1", "material-mtlx-v1", "material-graphs-v1", "material-gltf-v1",
+            "material-gltf-extensions-v1"]
 
 
 def words(R, fixture, control):

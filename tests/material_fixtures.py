@@ -89,7 +89,24 @@ def scene_from_words(words, name="material-fixture"):
         fallback_st = r.f(2)
         opacity, threshold = r.f(), r.f()
         normal = r.f(3)
-        rec = scenes.preview_material(base_color=tuple(base), roughness=roughness, metallic=metallic, ior=ior,
+        g_base = r.f(3)
+        g_roughness, g_metallic = r.f(), r.f()
+        g_emissive = r.f(3)
+        g_strength, g_ior, g_specular = r.f(), r.f(), r.f()
+        g_specular_color = r.f(3)
+        g_transmission, g_clearcoat, g_cc_roughness = r.f(), r.f(), r.f()
+        g_sheen_color = r.f(3)
+        g_sheen_roughness = r.f()
+        if model == 1:
+            rec = scenes.gltf_material(tuple(g_base), g_roughness, g_metallic, tuple(g_emissive), g_strength, g_ior,
+                                       g_specular, tuple(g_specular_color), g_transmission, g_clearcoat,
+                                       g_cc_roughness, tuple(g_sheen_color), g_sheen_roughness, opacity=opacity,
+                                       opacity_threshold=threshold, normal=tuple(normal))
+            rec.view(np.float32)[4:6] = fallback_st
+            model = 0  # the record holds its model
+        else:
+            rec = None
+        rec = rec if rec is not None else scenes.preview_material(base_color=tuple(base), roughness=roughness, metallic=metallic, ior=ior,
                                       emission=tuple(emission), specular_workflow=bool(specular_workflow),
                                       specular_color=tuple(specular), transmission=transmission,
                                       transmission_color=tuple(transmission_color), thin_walled=bool(thin),

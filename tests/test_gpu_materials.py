@@ -82,3 +82,54 @@ def test_fixture_wavefront_frames_match_reference_images(renderer, fixture):
                       mf.REFERENCE_SAMPLES)
     _say(f"{fixture} negative control: max tile |z| {c.max_tile_z:.2f}")
     assert c.failure != ""
+
+
+GLTF = ["material-gltf-v1", "material-gltf-extensions-v1"]
+
+
+@pytest.mark.parametrize("fixture", GLTF)
+def test_gltf_restir_initial_records_match_oracle(renderer, fixture):
+    # restir.slang:163-233 over glTF PBR receivers: the target function is the
+    # composed closure (bsdf/gltf_pbr.slang) at each camera hit
+    sc, size = mf.load(fixture)
+    renderer.apply(sc)
+    o = oracle.OracleScene(sc)
+    _say(f"{fixture}: restir initial records")
+    out = renderer.render(sc.world_to_clip, size, size, frame_count=1, sample_index=5, max_bounces=8,
+                          light_sampling=hip.LIGHT_RESTIR, restir_candidates=8, want_restir=True)
+    f = oracle.make_frame(sc.world_to_clip, size, size, spp=1, sample_index=5, max_bounces=8, light_sampling=2,
+                          restir_candidates=8)
+    g = out["restir"]
+    exact = hits = 0
+    for y in range(size):
+        for x in range(size):
+            ref = o.restir_initial(f, x, y, 5)
+            if ref is None:
+                continue
+            hits += 1
+            rec = g[y, x]
+            assert rec[0] == ref[0] and rec[3] == ref[3] and rec[7] == ref[7], (x, y, rec, ref)
+            exact += np.array_equal(rec, ref)
+    _say(f"  bit-exact records {exact}/{hits}")
+    assert hits > 0 and exact >= 0.999 * hits
+    acc, counts, shadow = o.render(f)
+    assert out["shadow_rays"] == shadow[0]
+    assert np.mean(np.all(out["color"][..., :3] == acc[..., :3].astype(np.float32), -1)) > 0.999
+
+
+def test_gltf_temporal_frames_and_reuse_refusal(renderer):
+    # temporal accumulation runs over glTF receivers; reservoir reuse, whose
+    # stored surface holds UsdPreviewSurface's terms only, is refused without
+    # latching
+    sc, size = mf.load("material-gltf-extensions-v1")
+    renderer.apply(sc)
+    for i in range(2):
+        out = renderer.render_temporal(sc.world_to_clip, sc.view, size, size, frame_index=i + 1, accumulate=True,
+                                       max_history=8, max_bounces=8)
+        assert np.isfinite(out["color"]).all()
+    assert out["history_available"]
+    with pytest.raises(hip.LotusError, match="reservoir reuse over glTF PBR"):
+        renderer.render_temporal(sc.world_to_clip, sc.view, size, size, frame_index=3, restir_temporal=True,
+                                 light_sampling=hip.LIGHT_RESTIR, max_bounces=8)
+    out = renderer.render(sc.world_to_clip, size, size, max_bounces=8)
+    assert out["spp"] == 1

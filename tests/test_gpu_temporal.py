@@ -538,7 +538,7 @@ def test_explicit_instance_identities_reach_the_device(renderer):
                      previous_from_current=still) if explicit else {}
         for f in range(2):
             print(f"identities explicit={explicit} frame {f}", flush=True)
-            out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=7000 + 10 * k + f,
+            out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=50000 + 10 * k + f,
                                            history_key=300 + k, accumulate=True, max_history=8, max_bounces=4,
                                            want_products=True, **extra)
         outs.append(out)
@@ -548,7 +548,7 @@ def test_explicit_instance_identities_reach_the_device(renderer):
     moved = still.reshape(n, 16).copy()
     moved[:, 12] = 0.05  # each point's previous position 0.05 further along +x
     # the explicit stream's next frame, whose previous frame has the same view
-    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=7012, history_key=301,
+    out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=50012, history_key=301,
                                    accumulate=True, max_history=8, max_bounces=4, want_products=True,
                                    instance_ids=ids, material_ids=mids, instance_flags=flags,
                                    previous_from_current=moved.reshape(-1))

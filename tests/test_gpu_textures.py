@@ -134,7 +134,7 @@ def test_texture_edits_and_rejected_slots(renderer):
     rec = np.array(bad.materials[1], np.uint32)
     rec[2] |= 1 << 7
     bad.set_material(1, rec)
-    with pytest.raises(hip.LotusError, match="lookups are supported"):
+    with pytest.raises(hip.LotusError, match="a slot its model does not read"):
         renderer.apply(bad)
     # opacity and its threshold lie in [0, 1] (ValidGeometry, gpu_scene.cpp:821-831)
     bad = sc.copy()

@@ -1,0 +1,161 @@
+"""Pins the glTF closure's directional albedo tables
+(hydra-lotus_amd/csrc/albedo_tables.inc, written by
+hydra-lotus_amd/tables/make_albedo_tables.cpp in the layout of
+backend/vulkan/shaders/bsdf/albedo.slang), which the oracle and the kernels
+both read:
+* the committed data is the generator's output (nodes recomputed);
+* node for node against the reference's own committed table
+  (backend/vulkan/bsdf-albedo-tables.inc) where the reference is present;
+* independent brute-force quadrature (a midpoint grid in numpy, in double
+  precision) of E1 and Esheen at nodes where the two tables disagree most:
+  GGX's tail of microfacets tilted past 45 degrees sends about alpha^2 of
+  normal-incidence light below the horizon, which the generator's adaptive
+  quadrature resolves and the reference's table misses;
+* closed forms: near alpha = 0 the rough lobes become mirrors, so E1 -> 1,
+  Eschlick -> (1 - cos)^5 and Edielectric -> the Fresnel reflectance (a thin
+  wall's 2R / (1 + R)).
+The closure built from them is pinned by the reference's glTF fixture images
+(test_oracle_materials.py)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "hydra-lotus_amd", "csrc", "albedo_tables.inc")
+GEN = os.path.join(ROOT, "hydra-lotus_amd", "tables", "make_albedo_tables.cpp")
+REF_INC = os.path.join(oracle.REFERENCE_ROOT, "backend", "vulkan", "bsdf-albedo-tables.inc")
+# offsets and shapes (cosines, alphas, layers) of E1, Eschlick, Edielectric, Esheen
+TABLES = {"ggx": (0, 32, 1), "schlick": (1024, 32, 1), "dielectric": (2048, 48, 48), "sheen": (75776, 32, 1)}
+
+
+def read_inc(path):
+    vals = []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("//"):
+                continue
+            vals += [float(x) for x in line.strip().split(",") if x.strip()]
+    return np.array(vals, np.float64)
+
+
+@pytest.fixture(scope="module")
+def table():
+    t = read_inc(INC)
+    assert t.size == 76800
+    return t
+
+
+def node(table_name, cosine, alpha, layer=0):
+    off, cosines, _ = TABLES[table_name]
+    return off + (layer * 32 + alpha) * cosines + cosine
+
+
+def axis(x, low, high, slope):
+    """The node value at coordinate x of albedo.slang's log(v) + k v axis."""
+    scale = lambda v: np.log(v) + slope * v  # noqa: E731
+    target = scale(low) + (scale(high) - scale(low)) * x
+    v = low
+    for _ in range(200):
+        v = min(max(v - (scale(v) - target) / (1.0 / v + slope), low), high)
+    return v
+
+
+def ggx_albedo_brute(mu, alpha, fresnel=None, n=3000):
+    """E1 (or the albedo with `fresnel(cos_oh)`) by a midpoint grid over the
+    visible-normal sampler's two numbers, in double precision."""
+    u1, u2 = np.meshgrid((np.arange(n) + 0.5) / n, (np.arange(n) + 0.5) / n, indexing="ij")
+    wo = np.array([np.sqrt(1 - mu * mu), 0.0, mu])
+    s = np.array([wo[0] * alpha, 0.0, wo[2]])
+    s /= np.linalg.norm(s)
+    phi = 2 * np.pi * u1
+    z = (1 - u2) * (1 + s[2]) - s[2]
+    st = np.sqrt(np.clip(1 - z * z, 0, 1))
+    h = np.stack([(st * np.cos(phi) + s[0]) * alpha, st * np.sin(phi) * alpha, z + s[2]], -1)
+    h /= np.linalg.norm(h, axis=-1, keepdims=True)
+    c = h @ wo
+    wi_z = 2 * c * h[..., 2] - wo[2]
+    lam = lambda zz: 0.5 * (np.sqrt(1 + alpha * alpha * np.maximum(0, 1 - zz * zz) / (zz * zz)) - 1)  # noqa: E731
+    ok = (c > 0) & (wi_z > 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        w = np.where(ok, (1 + lam(wo[2])) / (1 + lam(wo[2]) + lam(np.where(ok, wi_z, 1.0))), 0.0)
+    if fresnel is not None:
+        w = w * np.where(ok, fresnel(np.clip(c, 0, 1)), 0.0)
+    return float(w.mean())
+
+
+def fresnel_dielectric(c, eta):
+    s2 = (1 - c * c) / (eta * eta)
+    if s2 >= 1:
+        return 1.0
+    ct = np.sqrt(1 - s2)
+    par = (eta * c - ct) / (eta * c + ct)
+    per = (c - eta * ct) / (c + eta * ct)
+    return 0.5 * (par * par + per * per)
+
+
+def test_committed_table_is_the_generators_output(table, tmp_path):
+    exe = str(tmp_path / "make_albedo_tables")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", GEN, "-o", exe], check=True)
+    for args, index in [((0, 13, 30, 0), node("ggx", 13, 30)), ((1, 5, 7, 0), node("schlick", 5, 7)),
+                        ((2, 30, 10, 20), node("dielectric", 30, 10, 20)), ((3, 5, 3, 0), node("sheen", 5, 3)),
+                        ((2, 40, 20, 40), node("dielectric", 40, 20, 40))]:
+        out = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, check=True).stdout
+        value = float(out.split(":")[1].split()[0])
+        assert abs(value - table[index]) <= 1e-6 * max(1.0, abs(value)), (args, value, table[index])
+
+
+def test_table_against_the_reference_table(table):
+    if not os.path.exists(REF_INC):
+        pytest.skip("the reference's table is not present")
+    ref = read_inc(REF_INC)
+    assert ref.size == table.size
+    d = np.abs(table - ref)
+    # bounds per table (max, 99th percentile): the two quadratures differ
+    # most where one misses GGX's tail below the horizon (E1 near normal
+    # incidence) and at the leaving sheet's critical angle
+    bounds = {"ggx": (1.5e-3, 6e-4), "schlick": (1.5e-4, 3e-5), "entering": (6e-4, 6e-5), "leaving": (3e-3, 1.2e-3),
+              "thin": (1.5e-3, 1e-4), "sheen": (5e-5, 1e-5)}
+    parts = {"ggx": (0, 1024), "schlick": (1024, 2048), "entering": (2048, 26624), "leaving": (26624, 51200),
+             "thin": (51200, 75776), "sheen": (75776, 76800)}
+    for k, (a, b) in parts.items():
+        mx, p99 = bounds[k]
+        assert d[a:b].max() <= mx and np.percentile(d[a:b], 99) <= p99, (k, d[a:b].max(), np.percentile(d[a:b], 99))
+
+
+def test_table_against_brute_force(table):
+    # E1 at normal incidence, alpha = AlphaAt(12/31): the reference's table
+    # reads 0.999994; the integral leaves out about alpha^2 below the horizon
+    alpha = axis(12 / 31, 1e-3, 1.0, 2.0)
+    e1 = ggx_albedo_brute(1.0, alpha, n=6000)
+    assert abs(table[node("ggx", 31, 12)] - e1) < 1.5e-4 and e1 < 1 - 0.8 * alpha * alpha
+    # E1 at a grazing cosine and a rough alpha
+    mu, alpha = axis(13 / 31, 2e-4, 1.0, 4.0), axis(30 / 31, 1e-3, 1.0, 2.0)
+    assert abs(table[node("ggx", 13, 30)] - ggx_albedo_brute(mu, alpha)) < 2e-4
+    # Eschlick and Edielectric (entering, index 1.5-ish node) at a middle node
+    mu, alpha = axis(20 / 31, 2e-4, 1.0, 4.0), axis(16 / 31, 1e-3, 1.0, 2.0)
+    schlick = ggx_albedo_brute(mu, alpha, fresnel=lambda c: (1 - c) ** 5)
+    assert abs(table[node("schlick", 20, 16)] - schlick) < 2e-4
+    index = 1 + axis(8 / 15, 0.01, 2.0, 0.5)
+    mu_d = axis(30 / 47, 2e-4, 1.0, 4.0)
+    fr = np.vectorize(lambda c: fresnel_dielectric(c, index))
+    assert abs(table[node("dielectric", 30, 16, 8)] - ggx_albedo_brute(mu_d, alpha, fresnel=fr, n=2000)) < 3e-4
+
+
+def test_tables_reduce_to_mirrors_near_zero_alpha(table):
+    # alpha node 0 is 1e-3: a mirror up to terms of order alpha^2, except at
+    # grazing cosines where the microfacets' masking still counts
+    for c in range(16, 32):
+        mu = axis(c / 31, 2e-4, 1.0, 4.0)
+        assert abs(table[node("ggx", c, 0)] - 1.0) < 2e-4
+        assert abs(table[node("schlick", c, 0)] - (1 - mu) ** 5) < 3e-4
+        for layer in (0, 7, 15):
+            index = 1 + axis(layer / 15, 0.01, 2.0, 0.5)
+            cc = int(round(c * 47 / 31))
+            mu_d = axis(cc / 47, 2e-4, 1.0, 4.0)
+            r = fresnel_dielectric(mu_d, index)
+            assert abs(table[node("dielectric", cc, 0, layer)] - r) < 3e-4, (cc, layer)
+            assert abs(table[node("dielectric", cc, 0, 32 + layer)] - 2 * r / (1 + r)) < 3e-4, (cc, layer)

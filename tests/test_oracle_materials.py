@@ -13,7 +13,13 @@ and tangent frames (material-normals-v1), texture lookups
 (material-textures-v1), emission maps (material-emission-v1), and coverage
 (scene.slang:388-460: fractional, thresholded and looked-up opacity, covered
 emitters and shadow transmittance; material-alpha-v1, with the primvar,
-subset and transform fixtures that also carry covered cards)."""
+subset and transform fixtures that also carry covered cards), MaterialX
+graphs reduced to UsdPreviewSurface (material-mtlx-v1, material-graphs-v1),
+and glTF PBR's composed closure with its albedo tables: the core's
+dielectric, diffuse and metal lobes, packed metallic-roughness lookups and
+emission (material-gltf-v1), and the extensions' specular, specular colour,
+thin transmission, clearcoat with its own normal map, and sheen
+(material-gltf-extensions-v1)."""
 import numpy as np
 import pytest
 
@@ -24,7 +30,8 @@ import material_fixtures as mf
 
 FIXTURES = ["material-basics-v1", "material-transmission-v1", "material-textures-v1", "material-normals-v1",
             "material-emission-v1", "material-alpha-v1", "material-primvars-v1", "material-subsets-v1",
-            "material-transforms-v1"]
+            "material-transforms-v1", "material-mtlx-v1", "material-graphs-v1", "material-gltf-v1",
+            "material-gltf-extensions-v1"]
 
 
 def card_pixels(sc, size, centre, half):
