@@ -78,16 +78,92 @@ def relmse(img, ref):
     return float(np.mean((img[..., :3].astype(np.float64) - ref) ** 2 / (ref * ref + 1e-2)))
 
 
-def relmse_race(r, scene, width, height, depth, frames, ref_frames, cache_cfg, checkpoints=(4, 16, 64)):
-    """Plain frames and radiance-cache frames, each accumulated from the start
-    (the cache from a fresh initialisation, its training inside every frame's
-    GPU time), scored against a ref_frames-spp plain reference."""
+def centred_crop(width, height, fraction):
+    """A window of `fraction` of each side, centred, so its rows are the same
+    counted from the top or from the bottom."""
+    cw, ch = max(1, int(width * fraction)), max(1, int(height * fraction))
+    x, y = (width - cw) // 2, (height - ch) // 2
+    if (height - ch) % 2:
+        ch += 1
+    return x, y, cw, ch
+
+
+def fp64_reference(scene, width, height, spp, crop, views=8, **kw):
+    """The mean of `spp` samples per pixel over the crop (a data window of
+    the full-size target). `views` renderers of the scene render it as the
+    views of one replayed hipGraph batch per sample, each on its own sample
+    stream; every 1024 samples each renderer's float32 sums are added in
+    float64 on the host and its accumulation restarts (as the reference's
+    --write-reference sums its batches, main.cpp:3406-3452)."""
+    x, y, cw, ch = crop
+    acc = np.zeros((ch, cw, 3), np.float64)
+    rs = [hip.Renderer(0) for _ in range(views)]
+    try:
+        for v in rs:
+            v.apply(scene, builder=BUILDER)
+        done = 0
+        while done < spp:
+            k = min(1024, (spp - done) // views)
+            if k == 0:
+                raise ValueError("spp must be a multiple of the view count")
+            params = [dict(world_to_clip=scene.world_to_clip, width=width, height=height, data_window=crop,
+                           sample_index=(1 << 24) + done + j * k, **kw) for j in range(views)]
+            for _ in range(k):
+                hip.render_batch(rs, params, graph=True, want_color=False)
+            for v in rs:
+                sums, n = v.accumulation(width, height)
+                assert n == k, (n, k)
+                acc += sums[y:y + ch, x:x + cw, :3].astype(np.float64)
+            done += views * k
+    finally:
+        for v in rs:
+            v.close()
+    return acc / spp
+
+
+def variance_share(r, scene, width, height, depth, crop, ref, frames=256, **kw):
+    """The share of a 1-spp frame's relMSE that survives when everything
+    after the second vertex's emission is replaced by its mean: per-pixel
+    sample variances of the full estimator and of the estimator stopped
+    there (max_bounces 1: emission, next event estimation at the camera
+    hit, the BSDF-sampled emission at the second vertex), weighted as
+    relMSE weighs them. A radiance cache queried at the second vertex can
+    at best remove the rest."""
+    x, y, cw, ch = crop
+    wgt = 1.0 / (ref * ref + 1e-2)
+    out = {}
+    for name, bounces in (("full", depth), ("first_two_vertices", 1)):
+        s1 = np.zeros((ch, cw, 3))
+        s2 = np.zeros((ch, cw, 3))
+        for k in range(frames):
+            c = r.render(scene.world_to_clip, width, height, sample_index=(1 << 25) + k, max_bounces=bounces,
+                         data_window=crop, **kw)["color"][y:y + ch, x:x + cw, :3].astype(np.float64)
+            s1 += c
+            s2 += c * c
+        var = (s2 - s1 * s1 / frames) / (frames - 1)
+        out[name] = float(np.mean(var * wgt))
+        if name == "full":
+            per_pixel = np.sort((var * wgt).sum(-1).ravel())[::-1]
+            top = per_pixel[: max(1, per_pixel.size // 1000)].sum() / max(per_pixel.sum(), 1e-30)
+    return out["first_two_vertices"] / out["full"], dict(out, frames=frames,
+                                                         share_of_the_top_tenth_percent_of_pixels=float(top))
+
+
+def relmse_race(r, scene, width, height, depth, frames, ref_spp, cache_cfg, checkpoints=(4, 16, 64),
+                crop_fraction=0.25):
+    """Plain frames and radiance-cache frames of the full target, each
+    accumulated from the start (the cache from a fresh initialisation, its
+    training inside every frame's GPU time), scored over a centred crop
+    against a ref_spp-sample float64 reference of that crop."""
     w2c = scene.world_to_clip
     kw = dict(max_bounces=depth, light_sampling=hip.LIGHT_MIS)
-    for _ in range(ref_frames):
-        r.render(w2c, width, height, sample_index=1 << 24, want_color=False, **kw)
-    ref = r.render(w2c, width, height, sample_index=1 << 24, frame_count=0, **kw)["color"][..., :3]
-    ref = ref.astype(np.float64)
+    crop = centred_crop(width, height, crop_fraction)
+    x, y, cw, ch = crop
+    t0 = time.perf_counter()
+    ref = fp64_reference(scene, width, height, ref_spp, crop, **kw)
+    ref_s = time.perf_counter() - t0
+    share, variances = variance_share(r, scene, width, height, depth, crop, ref,
+                                      light_sampling=hip.LIGHT_MIS)
 
     def curve(cache):
         t, out_c = 0.0, []
@@ -95,7 +171,7 @@ def relmse_race(r, scene, width, height, depth, frames, ref_frames, cache_cfg, c
         for f in range(frames):
             out = r.render(w2c, width, height, sample_index=0, cache=cache, **kw)
             t += out["frame_ms"]
-            out_c.append((t, relmse(out["color"], ref)))
+            out_c.append((t, relmse(out["color"][y:y + ch, x:x + cw], ref)))
             last = out
         return out_c, last
 
@@ -109,10 +185,15 @@ def relmse_race(r, scene, width, height, depth, frames, ref_frames, cache_cfg, c
         t_plain, level = plain[k - 1]
         hit = next((i for i, (_, e) in enumerate(cached) if e <= level), None)
         race[f"plain_{k}_frames"] = {
-            "relmse": round(level, 6), "plain_gpu_ms": round(t_plain, 2),
+            "relmse": round(level, 7), "plain_gpu_ms": round(t_plain, 2),
             "cache_gpu_ms": round(cached[hit][0], 2) if hit is not None else None,
             "cache_frames": hit + 1 if hit is not None else None,
             "ratio": round(cached[hit][0] / t_plain, 3) if hit is not None else None}
+    cost = (cached[-1][0] / frames) / (plain[-1][0] / frames)
+    bound = {"relmse_share_of_first_two_vertices": round(share, 4),
+             "variance_weighted": {k: float(f"{v:.4g}") for k, v in variances.items()},
+             "cache_over_plain_frame_cost": round(cost, 3),
+             "best_possible_ratio": round(share * cost, 3)}
     q, smp = last["cache_queries"], last["cache_samples"]
     steps = cache_cfg.get("train_steps", hip.CACHE_DEFAULTS["train_steps"])
     # MFMA work: 2 x 16384 multiply-adds per query forwards; a training
@@ -120,9 +201,11 @@ def relmse_race(r, scene, width, height, depth, frames, ref_frames, cache_cfg, c
     infer_tflops = 2 * 16384 * q / (last["cache_infer_ms"] * 1e-3) / 1e12 if last["cache_infer_ms"] else None
     train_tflops = 3 * 2 * 16384 * min(smp, 4 * cache_cfg.get("batch_capacity", 65536)) / \
         (last["cache_train_ms"] * 1e-3) / 1e12 if last["cache_train_ms"] else None
-    return {"reference_spp": ref_frames, "frames": frames, "race": race,
-            "plain_relmse": [round(e, 6) for _, e in plain[:: max(1, frames // 16)]],
-            "cache_relmse": [round(e, 6) for _, e in cached[:: max(1, frames // 16)]],
+    return {"reference": {"spp": ref_spp, "accumulation": "float64 sums of float32 batches of 1024",
+                          "crop": {"x": x, "y": y, "width": cw, "height": ch}, "seconds": round(ref_s, 1)},
+            "frames": frames, "race": race, "cache_bound": bound,
+            "plain_relmse": [float(f"{e:.4g}") for _, e in plain[:: max(1, frames // 16)]],
+            "cache_relmse": [float(f"{e:.4g}") for _, e in cached[:: max(1, frames // 16)]],
             "plain_ms_per_frame": round(plain[-1][0] / frames, 3),
             "cache_ms_per_frame": round(cached[-1][0] / frames, 3),
             "cache_last_frame": {"queries": q, "samples": smp, "train_steps": steps,
@@ -401,7 +484,7 @@ def main() -> int:
                     help="frames rendered after the timed ones with each ray kind in launches of its own")
     ap.add_argument("--race-frames", type=int, default=64,
                     help="frames of each form in the equal-relMSE race (0 skips it)")
-    ap.add_argument("--race-reference-spp", type=int, default=4096)
+    ap.add_argument("--race-reference-spp", type=int, default=65536)
     ap.add_argument("--batch-frames", type=int, default=16,
                     help="frames of the 64-view batched workload (0 skips it)")
     ap.add_argument("--restir-frames", type=int, default=120,
