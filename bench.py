@@ -463,6 +463,7 @@ def batch_run(views, frames):
                         f"{max(v['scene_triangles'] for v in views)} tris), depth 2..12, NEE+MIS",
             "frames": frames, "scene_build_s": round(build_s, 2),
             "autotuned_dynamic_trace": sum(t["trace_dynamic"] for t in tunings),
+            "autotuned_half_resident": sum(t["resident_percent"] == 50 for t in tunings),
             "graph_frames_replayed": sum(k == 3 for k in graph_kinds), **res}
 
 

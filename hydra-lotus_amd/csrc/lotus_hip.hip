@@ -431,8 +431,17 @@ hipError_t ensure_launch(lotus_hip_renderer* r) {
   return dev_alloc(r->spill, need * size_t(r->grid.max_trace_lanes));
 }
 
+// The launch shapes under the renderer's tuning: the persistent traversal
+// grids scaled down to the resident share (never up, so the spill area sized
+// for the full grids holds every lane).
 Launch launch_for(lotus_hip_renderer* r, hipStream_t stream) {
-  return Launch{stream, r->grid, r->spill, static_cast<int>(r->spill_entries), r->stats,
+  GridShape g = r->grid;
+  const uint32_t pct = r->tuning.resident_percent;
+  if (pct != 0 && pct < 100) {
+    for (uint32_t* n : {&g.camera, &g.camera_dynamic, &g.trace, &g.trace_dynamic, &g.tail, &g.tail_cache})
+      *n = std::max(1u, static_cast<uint32_t>((uint64_t{*n} * pct) / 100));
+  }
+  return Launch{stream, g, r->spill, static_cast<int>(r->spill_entries), r->stats,
                 r->tuning.camera_dynamic != 0, r->tuning.trace_dynamic != 0};
 }
 
@@ -786,7 +795,7 @@ double environment_distribution(const lotus_hip_environment_map& e, uint32_t& co
     const size_t row = uint64_t(y) * rows / e.height;
     for (uint32_t x = 0; x < e.width; ++x) {
       const size_t column = uint64_t(x) * columns / e.width;
-      const float* t = e.texels + 4 * (size_t(y) * e.width + x);
+      const float* t = static_cast<const float*>(e.texels) + 4 * (size_t(y) * e.width + x);
       double radiance = 0.0;
       for (int c = 0; c < 3; ++c) radiance += std::max(double(t[c]), 0.0) * e.scale[c];
       sums[row * columns + column] += radiance / 3.0;
@@ -1080,8 +1089,10 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
     const lotus_hip_environment_map& e = *p->environment_map;
     if (e.width == 0 || e.height == 0 || e.width > 16384 || e.height > 16384 || !e.texels)
       return "the environment image must be 1 to 16384 texels on a side";
+    if (e.format != LOTUS_HIP_TEXTURE_RGBA32F) return "unknown environment texel format";
+    const float* texels = static_cast<const float*>(e.texels);
     for (size_t k = 0; k < 4ull * e.width * e.height; ++k)
-      if (!std::isfinite(e.texels[k])) return "the environment image has a non-finite texel";
+      if (!std::isfinite(texels[k])) return "the environment image has a non-finite texel";
     for (int c = 0; c < 3; ++c)
       if (!(e.scale[c] >= 0.0f) || !std::isfinite(e.scale[c])) return "the environment scale must be finite and non-negative";
     double axes[3][3];
@@ -1256,7 +1267,7 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
       (!e || (old.env_width == e->width && old.env_height == e->height &&
               std::memcmp(old.env_scale, e->scale, sizeof old.env_scale) == 0 &&
               std::memcmp(old.env_frame, e->world_from_environment, sizeof old.env_frame) == 0 &&
-              same_words(old.env_texels, e->texels, 4ull * e->width * e->height)));
+              same_words(old.env_texels, static_cast<const float*>(e->texels), 4ull * e->width * e->height)));
   c.lights = c.lights || !env_same;
   c.refit = !c.moved.empty() || c.transforms;
   if (!c.refit && !c.materials && !c.instance_materials && !c.environment && !c.lights) {
@@ -1318,7 +1329,8 @@ void record_plan(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const Pla
     d.env_width = e ? e->width : 0;
     d.env_height = e ? e->height : 0;
     if (e) {
-      d.env_texels.assign(e->texels, e->texels + 4ull * e->width * e->height);
+      const float* texels = static_cast<const float*>(e->texels);
+      d.env_texels.assign(texels, texels + 4ull * e->width * e->height);
       std::memcpy(d.env_scale, e->scale, sizeof d.env_scale);
       std::memcpy(d.env_frame, e->world_from_environment, sizeof d.env_frame);
     } else {
@@ -1786,12 +1798,56 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   delete r;
 }
 
+// An 8-bit environment image's texels decoded to RGBA32F as TexelRgb decodes
+// them (gpu_scene.cpp:612-629): channel / 255, then for sRGB the colour
+// channels' transfer function, in double, rounded once to float. Alpha is
+// not read. Returns a refusal, or "" with `out` describing the decoded copy.
+std::string decode_environment(const lotus_hip_environment_map& e, lotus_hip_environment_map& out,
+                               std::vector<float>& texels) {
+  if (e.format != LOTUS_HIP_TEXTURE_RGBA8_SRGB && e.format != LOTUS_HIP_TEXTURE_RGBA8_UNORM)
+    return "unknown environment texel format";
+  if (e.width == 0 || e.height == 0 || e.width > 16384 || e.height > 16384 || !e.texels)
+    return "the environment image must be 1 to 16384 texels on a side";
+  const size_t n = size_t(e.width) * e.height;
+  const uint8_t* b = static_cast<const uint8_t*>(e.texels);
+  float lut[256];
+  for (int v = 0; v < 256; ++v) {
+    const double x = v / 255.0;
+    lut[v] = static_cast<float>(e.format == LOTUS_HIP_TEXTURE_RGBA8_SRGB
+                                    ? (x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4))
+                                    : x);
+  }
+  texels.resize(4 * n);
+  for (size_t i = 0; i < n; ++i) {
+    for (int c = 0; c < 3; ++c) texels[4 * i + c] = lut[b[4 * i + c]];
+    texels[4 * i + 3] = static_cast<float>(b[4 * i + 3] / 255.0);
+  }
+  out = e;
+  out.texels = texels.data();
+  out.format = LOTUS_HIP_TEXTURE_RGBA32F;
+  return "";
+}
+
 int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, lotus_hip_scene_evidence* ev,
                           char* detail, size_t detail_size) {
   if (!r) return LOTUS_HIP_FAIL;
   if (!r->failure.empty()) {
     set_detail(detail, detail_size, "the renderer failed earlier: " + r->failure);
     return LOTUS_HIP_FAIL;
+  }
+  // an 8-bit environment image enters as its decoded RGBA32F texels
+  lotus_hip_scene_plan decoded_plan;
+  lotus_hip_environment_map decoded_map;
+  std::vector<float> decoded_texels;
+  if (p && p->environment_map && p->environment_map->format != LOTUS_HIP_TEXTURE_RGBA32F) {
+    const std::string bad = decode_environment(*p->environment_map, decoded_map, decoded_texels);
+    if (!bad.empty()) {
+      set_detail(detail, detail_size, bad);
+      return LOTUS_HIP_FAIL;
+    }
+    decoded_plan = *p;
+    decoded_plan.environment_map = &decoded_map;
+    p = &decoded_plan;
   }
   const std::string invalid = validate_plan(p);
   if (!invalid.empty()) {
@@ -2381,6 +2437,10 @@ int lotus_hip_trace(lotus_hip_renderer* r, uint32_t count, const float* rays, ui
 int lotus_hip_set_tuning(lotus_hip_renderer* r, const lotus_hip_tuning* tuning, char* detail, size_t detail_size) {
   if (!r || !tuning) {
     set_detail(detail, detail_size, "no renderer or tuning");
+    return LOTUS_HIP_FAIL;
+  }
+  if (tuning->resident_percent > 100) {
+    set_detail(detail, detail_size, "resident_percent must be in [1, 100] (0 for 100)");
     return LOTUS_HIP_FAIL;
   }
   r->tuning = *tuning;
@@ -3552,8 +3612,11 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
 int lotus_hip_autotune(lotus_hip_renderer* r, const lotus_hip_frame_params* params, uint32_t trials,
                        lotus_hip_tuning* chosen, char* detail, size_t detail_size) {
   if (!r || !params) return LOTUS_HIP_FAIL;
-  const lotus_hip_tuning candidates[8] = {{0u, 0u, 16u}, {1u, 1u, 16u}, {0u, 1u, 16u}, {1u, 0u, 16u},
-                                          {0u, 0u, 0u},  {1u, 1u, 0u},  {0u, 1u, 0u},  {1u, 0u, 0u}};
+  std::vector<lotus_hip_tuning> candidates;
+  for (uint32_t pct : {100u, 50u})
+    for (uint32_t div : {16u, 0u})
+      for (uint32_t cam : {0u, 1u})
+        for (uint32_t tr : {0u, 1u}) candidates.push_back(lotus_hip_tuning{cam, tr, div, pct});
   double best = 1e300;
   lotus_hip_tuning pick = r->tuning;
   lotus_hip_frame_params P = *params;

@@ -736,13 +736,11 @@ class HipOffscreenRenderer final : public OffscreenRenderer {
     p.lights = out.lights.empty() ? nullptr : out.lights.data();
     if (environment_map_ && environment_map_->texture) {
       const Texture& t = *environment_map_->texture;
-      if (t.format != TextureFormat::Rgba32Float) {
-        error = "the environment image must be RGBA32F";
-        return false;
-      }
       out.environment_map.width = t.width;
       out.environment_map.height = t.height;
-      out.environment_map.texels = reinterpret_cast<const float*>(t.texels.data());
+      out.environment_map.texels = t.texels.data();
+      // TextureFormat's values are LOTUS_HIP_TEXTURE_*'s, as for the bindings' textures
+      out.environment_map.format = static_cast<std::uint32_t>(t.format);
       std::memcpy(out.environment_map.scale, environment_map_->scale.data(), sizeof out.environment_map.scale);
       std::memcpy(out.environment_map.world_from_environment, environment_map_->world_from_environment.data(),
                   sizeof out.environment_map.world_from_environment);

@@ -142,16 +142,19 @@ def set_texture_fields(desc, p):
 
 class EnvironmentMap(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p), ("scale", C.c_float * 3),
-                ("world_from_environment", C.c_float * 16)]
+                ("world_from_environment", C.c_float * 16), ("format", C.c_uint32)]
 
 
 def environment_struct(env):
-    """The C record of a packed environment image (None: NULL)."""
+    """The C record of a packed environment image (None: NULL): float texels
+    are RGBA32F, 8-bit ones take env["format"] (sRGB unless it says
+    otherwise)."""
     if env is None:
         return None
     m = EnvironmentMap()
     m.height, m.width = env["texels"].shape[:2]
     m.texels = env["texels"].ctypes.data
+    m.format = 2 if env["texels"].dtype == np.float32 else env.get("format", 1)  # RGBA32F; 8-bit: sRGB unless stated
     for i in range(3):
         m.scale[i] = float(env["scale"][i])
     for i in range(16):
@@ -209,7 +212,8 @@ class SceneContents(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("camera_dynamic", C.c_uint32), ("trace_dynamic", C.c_uint32), ("tail_divisor", C.c_uint32)]
+    _fields_ = [("camera_dynamic", C.c_uint32), ("trace_dynamic", C.c_uint32), ("tail_divisor", C.c_uint32),
+                ("resident_percent", C.c_uint32)]
 
 
 class FrameParams(C.Structure):
@@ -701,7 +705,7 @@ class Renderer:
         t = Tuning()
         lib().lotus_hip_get_tuning(self.handle, C.byref(t))
         return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic,
-                "tail_divisor": t.tail_divisor}
+                "tail_divisor": t.tail_divisor, "resident_percent": t.resident_percent}
 
     def set_tuning(self, **kw):
         t = Tuning()
@@ -937,4 +941,4 @@ class Renderer:
         if s != PASS:
             raise LotusError(s, buf.value.decode())
         return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic,
-                "tail_divisor": t.tail_divisor}
+                "tail_divisor": t.tail_divisor, "resident_percent": t.resident_percent}

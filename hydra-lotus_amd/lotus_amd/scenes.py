@@ -366,7 +366,9 @@ class Scene:
             bindings=np.ascontiguousarray(np.array(self.bindings, np.uint32).reshape(-1)),
             **self._packed_corners(),
             environment_map=None if self.environment_map is None else dict(
-                texels=np.ascontiguousarray(self.environment_map["texels"], F32),
+                texels=np.ascontiguousarray(self.environment_map["texels"],
+                                            np.uint8 if self.environment_map["texels"].dtype == np.uint8 else F32),
+                format=self.environment_map.get("format", TEXTURE_RGBA8_SRGB),
                 scale=np.asarray(self.environment_map.get("scale", (1.0, 1.0, 1.0)), F32),
                 transform=np.asarray(self.environment_map.get("transform", identity()), F32).reshape(16)),
         )

@@ -64,24 +64,30 @@ typedef struct lotus_hip_light {
   float angle;                /* distant, degrees */
 } lotus_hip_light;
 
+/* Texel formats of textures and environment images (TextureFormat,
+ * include/lotus/material.hpp:15-41). */
+#define LOTUS_HIP_TEXTURE_RGBA8_UNORM 0u
+#define LOTUS_HIP_TEXTURE_RGBA8_SRGB 1u
+#define LOTUS_HIP_TEXTURE_RGBA32F 2u
+
 /* A latitude-longitude environment image (EnvironmentMap,
- * include/lotus/render_world.hpp:116-133): RGBA32F texels, rows from the
- * top (+Y) to the bottom (-Y), column 0 at longitude +pi; its radiance is
- * the texels' RGB times `scale`, and only the frame's orientation counts
- * (its axes orthonormalized in X, Y, Z order). */
+ * include/lotus/render_world.hpp:116-133, its texture any Texture format):
+ * texels in `format` (LOTUS_HIP_TEXTURE_*, above; 8-bit sRGB channels
+ * decode to linear values as TexelRgb does, gpu_scene.cpp:612-629), rows
+ * from the top (+Y) to the bottom (-Y), column 0 at longitude +pi; its
+ * radiance is the texels' RGB times `scale`, and only the frame's
+ * orientation counts (its axes orthonormalized in X, Y, Z order). */
 typedef struct lotus_hip_environment_map {
   uint32_t width, height;
-  const float* texels;          /* 4 floats per texel */
+  const void* texels;           /* 4 floats or 4 bytes per texel */
   float scale[3];
   float world_from_environment[16]; /* column-major */
+  uint32_t format;              /* LOTUS_HIP_TEXTURE_RGBA32F, _RGBA8_SRGB or _RGBA8_UNORM */
 } lotus_hip_environment_map;
 
 /* A texture of the material bindings (Texture, include/lotus/material.hpp:
  * 15-41): texels row-major from the image's top; 8-bit sRGB channels are
  * decoded to linear values (alpha stays linear). */
-#define LOTUS_HIP_TEXTURE_RGBA8_UNORM 0u
-#define LOTUS_HIP_TEXTURE_RGBA8_SRGB 1u
-#define LOTUS_HIP_TEXTURE_RGBA32F 2u
 typedef struct lotus_hip_texture {
   uint32_t width, height, format;
   const void* texels; /* 4 bytes or 4 floats per texel */
@@ -375,6 +381,9 @@ typedef struct lotus_hip_tuning {
                               previous sample run as wavefront rounds, the megakernel tail follows the rest
                               (vulkan_backend.cpp:369-385, whose divisor is 16, the default); 0: every bounce
                               that traced a ray runs as a round */
+  uint32_t resident_percent; /* the persistent traversal kernels' (camera, trace, tail) workgroups as a
+                                percentage of what their occupancy lets the device hold at once, in
+                                [1, 100]: fewer workgroups each walk more rays; 0 means 100 */
 } lotus_hip_tuning;
 
 int lotus_hip_set_tuning(lotus_hip_renderer* renderer, const lotus_hip_tuning* tuning, char* detail,
@@ -648,8 +657,9 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
                            char* detail, size_t detail_size);
 
 /* Autotuner (no reference counterpart): times trials frames of `params`
- * under each traversal driver choice of lotus_hip_tuning and keeps the
- * fastest; writes it to `chosen`. Trial frames do not join an
+ * under each combination of lotus_hip_tuning's choices (the traversal
+ * drivers, the tail rule and 100 or 50 percent resident workgroups) and
+ * keeps the fastest; writes it to `chosen`. Trial frames do not join an
  * accumulation, and the next frame restarts it. */
 int lotus_hip_autotune(lotus_hip_renderer* renderer, const lotus_hip_frame_params* params, uint32_t trials,
                        lotus_hip_tuning* chosen, char* detail, size_t detail_size);

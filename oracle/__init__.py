@@ -196,6 +196,17 @@ def ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
+def decode_environment(texels: np.ndarray, fmt: int) -> np.ndarray:
+    """TexelRgb (backend/vulkan/gpu_scene.cpp:612-629) over an 8-bit image:
+    channel / 255, and for sRGB (format 1) the colour channels' transfer
+    function, in float64, rounded once to float32; alpha stays linear."""
+    v = texels.astype(np.float64) / 255.0
+    if fmt == 1:
+        rgb = v[..., :3]
+        v[..., :3] = np.where(rgb <= 0.04045, rgb / 12.92, ((rgb + 0.055) / 1.055) ** 2.4)
+    return np.ascontiguousarray(v.astype(np.float32))
+
+
 class OracleScene:
     """A scene (see ``lotus_amd.scenes.Scene``) loaded into the oracle."""
 
@@ -222,7 +233,10 @@ class OracleScene:
         d.light_count = 0 if lights is None else len(lights)
         d.lights = lights.ctypes.data if d.light_count else None
         from lotus_amd.hip import environment_struct
-        self._env = environment_struct(p.get("environment_map"))
+        env = p.get("environment_map")
+        if env is not None and env["texels"].dtype == np.uint8:
+            env = dict(env, texels=decode_environment(env["texels"], env.get("format", 1)))
+        self._env = environment_struct(env)
         d.environment_map = C.addressof(self._env) if self._env is not None else None
         from lotus_amd.hip import set_texture_fields
         self._tex = set_texture_fields(d, p)

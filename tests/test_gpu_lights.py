@@ -221,3 +221,44 @@ def test_floor_under_constant_environment_on_gpu(renderer, light_sampling):
     else:
         se = px.std(0) / math.sqrt(px.shape[0])
         assert np.all(np.abs(px.mean(0) - expect) <= 4 * se + 1e-6), (px.mean(0), expect, se)
+
+
+def _srgb_bytes(linear):
+    """IEC 61966-2-1 encoding of linear values in [0, 1] to bytes."""
+    v = np.clip(linear, 0.0, 1.0)
+    e = np.where(v <= 0.0031308, v * 12.92, 1.055 * v ** (1.0 / 2.4) - 0.055)
+    return np.round(e * 255.0).astype(np.uint8)
+
+
+def test_srgb_environment_image_matches_oracle(renderer):
+    # an 8-bit sRGB sky (EnvironmentMap over an Rgba8Srgb Texture): decoded as
+    # TexelRgb decodes it; its light table, counts and radiance against the
+    # oracle, and the same scene as its decoded RGBA32F image
+    sc = sky_cornell()
+    tex = sc.environment_map["texels"]
+    texels8 = np.full(tex.shape, 255, np.uint8)
+    texels8[..., :3] = _srgb_bytes(tex[..., :3] / 64.0)
+    sc.environment_map = dict(texels=texels8, format=scenes.TEXTURE_RGBA8_SRGB, scale=(64.0, 64.0, 64.0),
+                              transform=scenes.identity())
+    renderer.apply(sc)
+    o = oracle.OracleScene(sc)
+    assert np.array_equal(renderer.light_table()[1], o.light_table()[1])
+    w = h = 96
+    out = renderer.render(sc.world_to_clip, w, h, frame_count=1, max_bounces=8, light_sampling=hip.LIGHT_MIS)
+    acc, counts, shadow = o.render(oracle.make_frame(sc.world_to_clip, w, h, spp=1, max_bounces=8,
+                                                     light_sampling=hip.LIGHT_MIS))
+    n = len(out["counts"])
+    assert np.array_equal(out["counts"], counts[0][:n]) and out["shadow_rays"] == shadow[0]
+    hit = acc[..., 3] > 0
+    exact = np.mean(np.all(out["color"][..., :3][hit] == acc[..., :3].astype(np.float32)[hit], -1))
+    _say(f"  sRGB sky 1-spp radiance bit-exact fraction {exact:.6f}")
+    assert exact > 0.999
+    decoded = sc.copy()
+    decoded.environment_map = dict(sc.environment_map, texels=oracle.decode_environment(texels8, 1))
+    decoded._packed = None
+    assert renderer.apply(decoded).update == hip.SCENE_UNCHANGED
+    unorm = sc.copy()
+    unorm.environment_map = dict(sc.environment_map, format=scenes.TEXTURE_RGBA8_UNORM)
+    unorm._packed = None
+    assert renderer.apply(unorm).update == hip.SCENE_TABLES
+    assert np.array_equal(renderer.light_table()[1], oracle.OracleScene(unorm).light_table()[1])

@@ -217,3 +217,17 @@ def test_environment_light_table_entry():
     assert records[-1, 3] == 0xFFFFFFFF
     pdf = records[:, 0].view(np.float32)
     assert np.allclose(pdf, np.array([3.0, w_env]) / (3.0 + w_env), rtol=2e-6)
+
+
+def test_environment_texel_decode_known_answers():
+    # TexelRgb (gpu_scene.cpp:612-629): channel / 255; sRGB colour channels
+    # through IEC 61966-2-1's decoding, alpha linear; UNorm linear
+    b = np.array([[[0, 10, 128, 128], [255, 11, 200, 0]]], np.uint8)
+    s = oracle.decode_environment(b, 1)
+    assert s[0, 0, 0] == 0.0 and s[0, 1, 0] == 1.0
+    assert s[0, 0, 1] == np.float32(10 / 255 / 12.92)  # the linear segment (10/255 <= 0.04045)
+    assert abs(float(s[0, 0, 2]) - 0.21586050011389926) < 1e-7  # ((128/255 + 0.055) / 1.055)^2.4
+    assert s[0, 0, 3] == np.float32(128 / 255) and s[0, 1, 3] == 0.0
+    assert s[0, 1, 1] == np.float32(((11 / 255 + 0.055) / 1.055) ** 2.4)  # 11/255 > 0.04045
+    u = oracle.decode_environment(b, 0)
+    assert np.array_equal(u, (b / 255.0).astype(np.float32))
