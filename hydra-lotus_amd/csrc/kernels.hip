@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_initial(SceneView S, Fra
     if (F.restir_surf) {
       float view[4];
       mul_point(F.view, sf.position.x, sf.position.y, sf.position.z, 1.0f, view);
-      store_reuse_surface(F.restir_surf + 8ull * slot, sf, origin, -view[2] / view[3], wo,
+      store_reuse_surface(F.restir_surf + kReuseSurfaceWords * slot, sf, origin, -view[2] / view[3], wo,
                           stable_instance_id(F, h4.x), stable_material_id(F, S.instance_material, h4.x));
     }
   }
@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(256) k_restir_clear(float4* restir, float4* su
     restir[4ull * i + 2] = zero;
     restir[4ull * i + 3] = zero;
     if (surf)
-      for (int k = 0; k < 8; ++k) surf[8ull * i + k] = zero;
+      for (uint32_t k = 0; k < kReuseSurfaceWords; ++k) surf[kReuseSurfaceWords * i + k] = zero;
   }
 }
 
@@ -488,7 +488,7 @@ __device__ __forceinline__ void resolve_estimate(const SceneView& S, TraceStack&
 }
 
 // restir.slang:302-387 (WriteTemporalReservoir) per pixel of the target.
-template <bool kLights>  // the scene has analytic lights or an environment image
+template <bool kLights, bool kGltf = false>  // analytic lights or an environment image; glTF PBR receivers
 __global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, FrameConst F, RestirReuseArgs A,
                                                                  uint32_t* counters, uint2* spill, int spill_cap,
                                                                  unsigned long long* stats) {
@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, Fr
     result.supporting = init.r.count;
     uint32_t rays = 0;
     if ((init.flags & 1u) != 0u) {
-      const ReuseSurface receiver = load_reuse_surface(A.surf + 8ull * slot);
+      const ReuseSurface receiver = load_reuse_surface<kGltf>(A.surf + kReuseSurfaceWords * slot);
       // TemporalSampleHistory (temporal.slang:73-79): a valid reprojection
       // onto the same placement as this receiver
       const bool history = A.history != 0u && (result.flags & 2u) == 0u && A.validity[slot].valid != 0u &&
@@ -516,7 +516,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_temporal(SceneView S, Fr
         const TemporalValidityPixel vd = A.validity[slot];
         const uint32_t pi = vd.previous_index;
         const ReuseRecord src = load_reuse(A.prev_hist + 4ull * pi);
-        const ReuseSurface psurf = load_reuse_surface(A.prev_surf + 8ull * pi);
+        const ReuseSurface psurf = load_reuse_surface<kGltf>(A.prev_surf + kReuseSurfaceWords * pi);
         if ((src.flags & 1u) != 0u && (src.flags & 2u) == 0u && psurf.instance_id == receiver.instance_id &&
             psurf.material_id == receiver.material_id) {
           const uint32_t count = src.r.count < A.max_history ? src.r.count : A.max_history;
@@ -579,7 +579,7 @@ constexpr int kMaxSpatialSources = 5;
 // drawn uniformly within the radius (a draw that lands on the pixel itself
 // is no neighbour; draws may repeat a neighbour, whose candidates then count
 // twice in M and in the supporting candidates alike).
-template <bool kLights>  // the scene has analytic lights or an environment image
+template <bool kLights, bool kGltf = false>  // analytic lights or an environment image; glTF PBR receivers
 __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, FrameConst F, RestirReuseArgs A,
                                                                 uint32_t* counters, uint2* spill, int spill_cap,
                                                                 unsigned long long* stats) {
@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
     uint32_t src_index[kMaxSpatialSources] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
     uint32_t rays = 0;
     if ((centre.flags & 3u) == 1u) {
-      const ReuseSurface receiver = load_reuse_surface(A.surf + 8ull * slot);
+      const ReuseSurface receiver = load_reuse_surface<kGltf>(A.surf + kReuseSurfaceWords * slot);
       const int px = static_cast<int>(slot % F.width), py = static_cast<int>(slot / F.width);
       uint32_t disk_rng = seed_random(slot, F.sample_index ^ kSpatialDiskSeed);
       uint32_t accepted = 0;
@@ -618,7 +618,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
         if (nx < 0 || ny < 0 || nx >= static_cast<int>(F.width) || ny >= static_cast<int>(F.height)) continue;
         const uint32_t index = static_cast<uint32_t>(ny) * F.width + static_cast<uint32_t>(nx);
         const uint32_t nflags = __float_as_uint(A.temporal_out[4ull * index + 1].w);
-        if (spatial_compatible(receiver, nflags, load_reuse_surface(A.surf + 8ull * index))) {
+        if (spatial_compatible(receiver, nflags, load_reuse_surface<kGltf>(A.surf + kReuseSurfaceWords * index))) {
           src_index[i] = index;
           ++accepted;
         }
@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_restir_spatial(SceneView S, Fra
             const ReuseRecord src = load_reuse(A.temporal_out + 4ull * src_index[i]);
             bool valid = true;
             const float support =
-                history_target<kLights>(S, src, load_reuse_surface(A.surf + 8ull * src_index[i]), result, valid);
+                history_target<kLights>(S, src, load_reuse_surface<kGltf>(A.surf + kReuseSurfaceWords * src_index[i]), result, valid);
             if (!valid) result.flags |= 2u;
             if (support > 0.0f) result.supporting += src.r.count < A.max_history ? src.r.count : A.max_history;
           }
@@ -1435,14 +1435,20 @@ void launch_restir_clear(const Launch& L, float4* restir, float4* surf, uint32_t
 }
 void launch_restir_temporal(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                             uint32_t* c) {
-  if (scene_has_extras(S))
+  if (S.gltf_materials != 0)
+    k_restir_temporal<true, true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap,
+                                                                            L.stats);
+  else if (scene_has_extras(S))
     k_restir_temporal<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
   else
     k_restir_temporal<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_restir_spatial(const Launch& L, const SceneView& S, const FrameConst& F, const RestirReuseArgs& A,
                            uint32_t* c) {
-  if (scene_has_extras(S))
+  if (S.gltf_materials != 0)
+    k_restir_spatial<true, true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap,
+                                                                           L.stats);
+  else if (scene_has_extras(S))
     k_restir_spatial<true><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);
   else
     k_restir_spatial<false><<<L.grid.tail, kTraceBlock, 0, L.stream>>>(S, F, A, c, L.spill, L.spill_cap, L.stats);

@@ -554,7 +554,7 @@ hipError_t ensure_temporal(lotus_hip_renderer* r, uint32_t width, uint32_t heigh
   for (int k = 0; k < 2 && e == hipSuccess; ++k) {
     e = dev_alloc(t.surf[k], P);
     if (e == hipSuccess) e = dev_alloc(t.rad[k], P);
-    if (e == hipSuccess) e = dev_alloc(t.rsurf[k], 8 * P);
+    if (e == hipSuccess) e = dev_alloc(t.rsurf[k], kReuseSurfaceWords * P);
     if (e == hipSuccess) e = dev_alloc(t.rhist[k], 4 * P);
   }
   if (e == hipSuccess) e = dev_alloc(t.validity, P);
@@ -2781,13 +2781,6 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
                "spatial reservoirs require RenderTemporalScene, RestirInitial, one sample, history candidates in [1, 65536] and radius in [1, 32]");
     return LOTUS_HIP_FAIL;
   }
-  // Reservoir reuse keeps each receiver's closure in a stored surface; this
-  // renderer stores UsdPreviewSurface's (restir.slang:72-98) and not yet glTF
-  // PBR's 22 terms (:80-93)
-  if ((ts->restir_temporal || ts->restir_spatial) && r->view.gltf_materials != 0) {
-    set_detail(detail, detail_size, "reservoir reuse over glTF PBR materials is not supported by this renderer");
-    return LOTUS_HIP_FAIL;
-  }
   auto& t = r->temporal;
   if (P->width == 0 || P->height == 0 || !std::isfinite(P->display_window[0]) ||
       !std::isfinite(P->display_window[1]) || !std::isfinite(P->display_window[2]) ||
@@ -2955,7 +2948,7 @@ int lotus_hip_render_temporal(lotus_hip_renderer* r, const lotus_hip_frame_param
                : hipMemcpy2D(ts->restir_spatial_sources, sizeof(uint4), t.rsources, 2 * sizeof(uint4),
                              sizeof(uint4), npix, hipMemcpyDeviceToHost);
   if (e == hipSuccess && ts->restir_surfaces && tf.reuse)
-    e = hipMemcpy(ts->restir_surfaces, t.rsurf[cur], npix * 8 * sizeof(float4), hipMemcpyDeviceToHost);
+    e = hipMemcpy(ts->restir_surfaces, t.rsurf[cur], npix * kReuseSurfaceWords * sizeof(float4), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     set_detail(detail, detail_size, latch(r, "temporal readback", e));
     return LOTUS_HIP_FAIL;

@@ -1193,36 +1193,60 @@ __device__ __forceinline__ void add_emission(const SceneView& S, const FrameCons
 }
 
 // A receiver surface for reuse (restir.slang:30-62, RestirSurface): what a
-// reservoir's target needs at it, 8 float4 = {origin, view depth},
-// {geometric normal, instance id low}, {shading normal, instance id high},
-// {wo in the shading frame, dielectric f90}, {base colour, alpha},
-// {dielectric f0, metallic}, {f0, f90}, {transmission (negated: thin),
-// eta (negated: smooth), material id low, high}.
+// reservoir's target needs at it, kReuseSurfaceWords float4 =
+// {origin, view depth}, {geometric normal, instance id low},
+// {shading normal, instance id high}, {wo in the shading frame, c0},
+// {c1..c4}, {c5..c8}, {c9..c12}, {c13, c14, material id low, high},
+// {c15..c18}, {c19, c20, c21, model}. The closure words c: UsdPreviewSurface
+// (model 0) keeps {dielectric f90; base colour, alpha; dielectric f0,
+// metallic; f0, f90; transmission (negated: thin), eta (negated: smooth)}
+// in c0..c14 and zeros after; glTF PBR (model 1) keeps its 22 terms as
+// PackGltfPbrTerms orders them (gltf_pbr.slang:159-190) in c0..c21
+// (StoreClosure, restir.slang:80-112, with this layout's words).
+constexpr uint32_t kReuseSurfaceWords = 10;
 __device__ __forceinline__ void store_reuse_surface(float4* out, const Surface& sf, V3 origin, float depth, V3 wo,
                                                     uint64_t instance_id, uint64_t material_id) {
-  const Closure& b = sf.bsdf.preview;  // glTF PBR surfaces take no part in reuse (the renderer refuses them)
   out[0] = make_float4(origin.x, origin.y, origin.z, depth);
   out[1] = make_float4(sf.normal.x, sf.normal.y, sf.normal.z, __uint_as_float(static_cast<uint32_t>(instance_id)));
   out[2] = make_float4(sf.shading_normal.x, sf.shading_normal.y, sf.shading_normal.z,
                        __uint_as_float(static_cast<uint32_t>(instance_id >> 32)));
+  const float mlo = __uint_as_float(static_cast<uint32_t>(material_id));
+  const float mhi = __uint_as_float(static_cast<uint32_t>(material_id >> 32));
+  if (sf.bsdf.model == kGltfModel) {
+    float l[22];
+    pack_gltf_terms(sf.bsdf.gltf.t, l);
+    out[3] = make_float4(wo.x, wo.y, wo.z, l[0]);
+    out[4] = make_float4(l[1], l[2], l[3], l[4]);
+    out[5] = make_float4(l[5], l[6], l[7], l[8]);
+    out[6] = make_float4(l[9], l[10], l[11], l[12]);
+    out[7] = make_float4(l[13], l[14], mlo, mhi);
+    out[8] = make_float4(l[15], l[16], l[17], l[18]);
+    out[9] = make_float4(l[19], l[20], l[21], 1.0f);
+    return;
+  }
+  const Closure& b = sf.bsdf.preview;
   out[3] = make_float4(wo.x, wo.y, wo.z, b.dielectric_f90);
   out[4] = make_float4(b.base_color.x, b.base_color.y, b.base_color.z, b.alpha);
   out[5] = make_float4(b.dielectric_f0.x, b.dielectric_f0.y, b.dielectric_f0.z, b.metallic);
   out[6] = make_float4(b.f0.x, b.f0.y, b.f0.z, b.f90);
-  out[7] = make_float4(b.thin_walled ? -b.transmission : b.transmission, b.smooth ? -b.eta : b.eta,
-                       __uint_as_float(static_cast<uint32_t>(material_id)),
-                       __uint_as_float(static_cast<uint32_t>(material_id >> 32)));
+  out[7] = make_float4(b.thin_walled ? -b.transmission : b.transmission, b.smooth ? -b.eta : b.eta, mlo, mhi);
+  out[8] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  out[9] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 struct ReuseSurface {
   V3 origin, normal, shading_normal, wo;
   float depth;
   uint64_t instance_id, material_id;
-  Closure bsdf;
+  SurfaceBsdf bsdf;
 };
 
 // restir.slang:117-141 (LoadClosure): targets are reflection-side, so the
-// transmission colour is white.
+// preview closure's transmission colour is white; a glTF PBR closure is
+// rebuilt from its terms for the stored wo (MakeGltfPbrClosure). Kernels
+// of scenes without glTF PBR materials (kGltf false) read the preview
+// words only.
+template <bool kGltf = false>
 __device__ __forceinline__ ReuseSurface load_reuse_surface(const float4* in) {
   const float4 s0 = in[0], s1 = in[1], s2 = in[2], s3 = in[3], s4 = in[4], s5 = in[5], s6 = in[6], s7 = in[7];
   ReuseSurface r;
@@ -1233,7 +1257,18 @@ __device__ __forceinline__ ReuseSurface load_reuse_surface(const float4* in) {
   r.wo = xyz(s3);
   r.instance_id = static_cast<uint64_t>(__float_as_uint(s1.w)) | (static_cast<uint64_t>(__float_as_uint(s2.w)) << 32);
   r.material_id = static_cast<uint64_t>(__float_as_uint(s7.z)) | (static_cast<uint64_t>(__float_as_uint(s7.w)) << 32);
-  Closure& b = r.bsdf;
+  if constexpr (kGltf) {
+    const float4 s8 = in[8], s9 = in[9];
+    if (s9.w != 0.0f) {
+      const float l[22] = {s3.w, s4.x, s4.y, s4.z, s4.w, s5.x, s5.y, s5.z, s5.w, s6.x, s6.y,
+                           s6.z, s6.w, s7.x, s7.y, s8.x, s8.y, s8.z, s8.w, s9.x, s9.y, s9.z};
+      r.bsdf.model = kGltfModel;
+      r.bsdf.gltf = make_gltf_closure(unpack_gltf_terms(l), r.wo);
+      return r;
+    }
+  }
+  r.bsdf.model = kPreviewModel;
+  Closure& b = r.bsdf.preview;
   b.base_color = xyz(s4);
   b.alpha = s4.w;
   b.dielectric_f0 = xyz(s5);

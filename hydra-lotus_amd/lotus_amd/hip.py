@@ -841,7 +841,7 @@ class Renderer:
                         restir_temporal=np.zeros((height, width, 16), np.uint32),
                         restir_spatial=np.zeros((height, width, 16), np.uint32),
                         restir_sources=np.zeros((height, width, 8 if spatial_disk else 4), np.uint32),
-                        restir_surfaces=np.zeros((height, width, 32), np.float32))
+                        restir_surfaces=np.zeros((height, width, 40), np.float32))
             t.surface = prod["surface"].ctypes.data
             t.validity = prod["validity"].ctypes.data
             t.radiance = prod["radiance"].ctypes.data

@@ -536,13 +536,16 @@ typedef struct lotus_hip_temporal_settings {
   void* restir_spatial_records;
   void* restir_spatial_sources;
   /* Optional: the reuse passes' receiver surfaces (RestirSurface,
-   * restir.slang:30-62, in this renderer's 128-byte form: {offset origin,
-   * view depth}, {geometric normal, instance id low bits}, {shading normal,
-   * instance id high bits}, {wo in the shading frame, dielectric f90},
-   * {base colour, alpha}, {dielectric f0, metallic}, {f0, f90},
-   * {transmission (negated: thin wall), eta (negated: smooth), material id
-   * low, high bits}; zero where the camera ray missed), width*height of
-   * them. Written when the frame ran a reuse pass. */
+   * restir.slang:30-62, in this renderer's 160-byte form of 10 float4:
+   * {offset origin, view depth}, {geometric normal, instance id low bits},
+   * {shading normal, instance id high bits}, {wo in the shading frame, c0},
+   * {c1..c4}, {c5..c8}, {c9..c12}, {c13, c14, material id low, high bits},
+   * {c15..c18}, {c19, c20, c21, model}; zero where the camera ray missed),
+   * width*height of them. The closure words c of UsdPreviewSurface (model
+   * 0): dielectric f90; base colour, alpha; dielectric f0, metallic; f0,
+   * f90; transmission (negated: thin wall), eta (negated: smooth); zeros.
+   * Of glTF PBR (model 1): its 22 terms in PackGltfPbrTerms' order
+   * (gltf_pbr.slang:159-190). Written when the frame ran a reuse pass. */
   void* restir_surfaces;
   /* The spatial pass's neighbours: LOTUS_HIP_SPATIAL_AXES, the reference's
    * four at restir_spatial_radius along the axes (restir.slang:408-410), or

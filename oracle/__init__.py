@@ -317,12 +317,12 @@ class OracleScene:
                         previous_index: int, centre_ids, previous_record, previous_surface, max_history: int,
                         instance_ids=None, material_ids=None):
         """WriteTemporalReservoir at pixel (x, y): (record[16] words,
-        receiver surface[32] floats, hit)."""
+        receiver surface[40] floats, hit)."""
         view = np.ascontiguousarray(view, np.float32)
         prec = np.ascontiguousarray(previous_record if previous_record is not None else np.zeros(16), np.uint32)
-        psurf = np.ascontiguousarray(previous_surface if previous_surface is not None else np.zeros(32), np.float32)
+        psurf = np.ascontiguousarray(previous_surface if previous_surface is not None else np.zeros(40), np.float32)
         rec = np.zeros(16, np.uint32)
-        surf = np.zeros(32, np.float32)
+        surf = np.zeros(40, np.float32)
         iid = None if instance_ids is None else np.ascontiguousarray(instance_ids, np.uint64)
         mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.uint64)
         hit = lib().oracle_restir_temporal(

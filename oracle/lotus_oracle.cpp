@@ -3324,20 +3324,27 @@ void StoreReuse(const ReuseRecord& x, uint32_t* w) {
   w[15] = x.age;
 }
 
-// restir.slang:30-62 (RestirSurface) in the renderer's 32-float form:
+// restir.slang:30-62 (RestirSurface) in the renderer's 40-float form:
 // {offset origin, view depth}, {geometric normal, instance id low},
-// {shading normal, instance id high}, {wo in the shading frame, dielectric
-// f90}, {base colour, alpha}, {dielectric f0, metallic}, {f0, f90},
-// {transmission (negated: thin wall), eta (negated: smooth), material id
-// low, high}. StoreClosure / LoadClosure (restir.slang:80-141) for the
-// preview closure: targets are reflection-side, so a loaded closure's
-// transmission colour is white.
+// {shading normal, instance id high}, {wo in the shading frame, c0},
+// {c1..c4}, {c5..c8}, {c9..c12}, {c13, c14, material id low, high},
+// {c15..c18}, {c19, c20, c21, model}, where the closure words c are, for
+// UsdPreviewSurface (model 0), {dielectric f90; base colour, alpha;
+// dielectric f0, metallic; f0, f90; transmission (negated: thin wall), eta
+// (negated: smooth)} and zeros, and for glTF PBR (model 1) the 22 lanes of
+// PackGltfPbrTerms (gltf_pbr.slang:164-190). StoreClosure / LoadClosure
+// (restir.slang:80-141): targets are reflection-side, so a loaded preview
+// closure's transmission colour is white, and a glTF PBR closure is rebuilt
+// from its unpacked terms (UnpackGltfPbrTerms, gltf_pbr.slang:192-212: the
+// coat normal from its tangent components) for the stored wo.
+constexpr int kReuseSurfaceFloats = 40;
 struct ReuseSurface {
   F3 origin{0, 0, 0};
   float depth = 0.0f;
   F3 normal{0, 0, 0}, shading_normal{0, 0, 0}, wo{0, 0, 0};
   uint64_t instance_id = 0, material_id = 0;
   Bsdf bsdf{};
+  GltfTerms gltf{};  // a glTF PBR closure's terms
 };
 
 inline float BitsFloat(uint32_t w) {
@@ -3352,18 +3359,33 @@ inline uint32_t FloatBits(float f) {
 }
 
 void StoreSurface(const ReuseSurface& r, float* out) {
-  const Closure& b = r.bsdf.preview;
-  const float v[32] = {r.origin.x, r.origin.y, r.origin.z, r.depth,
-                       r.normal.x, r.normal.y, r.normal.z, BitsFloat(static_cast<uint32_t>(r.instance_id)),
-                       r.shading_normal.x, r.shading_normal.y, r.shading_normal.z,
-                       BitsFloat(static_cast<uint32_t>(r.instance_id >> 32)),
-                       r.wo.x, r.wo.y, r.wo.z, b.dielectric_f90,
-                       b.base_color.x, b.base_color.y, b.base_color.z, b.alpha,
-                       b.dielectric_f0.x, b.dielectric_f0.y, b.dielectric_f0.z, b.metallic,
-                       b.f0.x, b.f0.y, b.f0.z, b.f90,
-                       b.thin_walled ? -b.transmission : b.transmission, b.smooth ? -b.eta : b.eta,
-                       BitsFloat(static_cast<uint32_t>(r.material_id)),
-                       BitsFloat(static_cast<uint32_t>(r.material_id >> 32))};
+  float c[22] = {};
+  if (r.bsdf.model == kGltfModel) {
+    const GltfTerms& t = r.gltf;
+    const float lanes[22] = {t.base_color.x, t.base_color.y, t.base_color.z, t.alpha, t.metallic, t.e1, t.share,
+                             t.f0.x, t.f0.y, t.f0.z, t.f90, t.transmission, t.coat_scale, t.coat_dim, t.coat_alpha,
+                             t.coat_normal.x, t.coat_normal.y, t.sheen_color.x, t.sheen_color.y, t.sheen_color.z,
+                             t.sheen_roughness, t.sheen_dim};
+    std::memcpy(c, lanes, sizeof c);
+  } else {
+    const Closure& b = r.bsdf.preview;
+    const float lanes[15] = {b.dielectric_f90, b.base_color.x, b.base_color.y, b.base_color.z, b.alpha,
+                             b.dielectric_f0.x, b.dielectric_f0.y, b.dielectric_f0.z, b.metallic,
+                             b.f0.x, b.f0.y, b.f0.z, b.f90,
+                             b.thin_walled ? -b.transmission : b.transmission, b.smooth ? -b.eta : b.eta};
+    std::memcpy(c, lanes, sizeof lanes);
+  }
+  const float v[kReuseSurfaceFloats] = {
+      r.origin.x, r.origin.y, r.origin.z, r.depth,
+      r.normal.x, r.normal.y, r.normal.z, BitsFloat(static_cast<uint32_t>(r.instance_id)),
+      r.shading_normal.x, r.shading_normal.y, r.shading_normal.z, BitsFloat(static_cast<uint32_t>(r.instance_id >> 32)),
+      r.wo.x, r.wo.y, r.wo.z, c[0],
+      c[1], c[2], c[3], c[4],
+      c[5], c[6], c[7], c[8],
+      c[9], c[10], c[11], c[12],
+      c[13], c[14], BitsFloat(static_cast<uint32_t>(r.material_id)), BitsFloat(static_cast<uint32_t>(r.material_id >> 32)),
+      c[15], c[16], c[17], c[18],
+      c[19], c[20], c[21], r.bsdf.model == kGltfModel ? 1.0f : 0.0f};
   std::memcpy(out, v, sizeof v);
 }
 
@@ -3376,18 +3398,41 @@ ReuseSurface LoadSurface(const float* s) {
   r.wo = F3{s[12], s[13], s[14]};
   r.instance_id = static_cast<uint64_t>(FloatBits(s[7])) | (static_cast<uint64_t>(FloatBits(s[11])) << 32);
   r.material_id = static_cast<uint64_t>(FloatBits(s[30])) | (static_cast<uint64_t>(FloatBits(s[31])) << 32);
+  const float c[22] = {s[15], s[16], s[17], s[18], s[19], s[20], s[21], s[22], s[23], s[24], s[25],
+                       s[26], s[27], s[28], s[29], s[32], s[33], s[34], s[35], s[36], s[37], s[38]};
+  if (s[39] != 0.0f) {
+    GltfTerms& t = r.gltf;
+    t.base_color = F3{c[0], c[1], c[2]};
+    t.alpha = c[3];
+    t.metallic = c[4];
+    t.e1 = c[5];
+    t.share = c[6];
+    t.f0 = F3{c[7], c[8], c[9]};
+    t.f90 = c[10];
+    t.transmission = c[11];
+    t.coat_scale = c[12];
+    t.coat_dim = c[13];
+    t.coat_alpha = c[14];
+    t.coat_normal = F3{c[15], c[16], std::sqrt(std::max(1.0f - (c[15] * c[15] + c[16] * c[16]), 0.0f))};
+    t.sheen_color = F3{c[17], c[18], c[19]};
+    t.sheen_roughness = c[20];
+    t.sheen_dim = c[21];
+    r.bsdf.model = kGltfModel;
+    r.bsdf.composed = MakeGltfClosure(t, r.wo);
+    return r;
+  }
   Closure& b = r.bsdf.preview;
-  b.dielectric_f90 = s[15];
-  b.base_color = F3{s[16], s[17], s[18]};
-  b.alpha = s[19];
-  b.dielectric_f0 = F3{s[20], s[21], s[22]};
-  b.metallic = s[23];
-  b.f0 = F3{s[24], s[25], s[26]};
-  b.f90 = s[27];
-  b.transmission = std::fabs(s[28]);
-  b.thin_walled = s[28] < 0.0f;
-  b.eta = std::fabs(s[29]);
-  b.smooth = s[29] < 0.0f;
+  b.dielectric_f90 = c[0];
+  b.base_color = F3{c[1], c[2], c[3]};
+  b.alpha = c[4];
+  b.dielectric_f0 = F3{c[5], c[6], c[7]};
+  b.metallic = c[8];
+  b.f0 = F3{c[9], c[10], c[11]};
+  b.f90 = c[12];
+  b.transmission = std::fabs(c[13]);
+  b.thin_walled = c[13] < 0.0f;
+  b.eta = std::fabs(c[14]);
+  b.smooth = c[14] < 0.0f;
   b.transmission_color = F3{1, 1, 1};
   return r;
 }
@@ -3533,10 +3578,16 @@ int oracle_restir_temporal(const oracle_scene* scene, const oracle_frame* frame,
     receiver.shading_normal = sf.shading_normal;
     receiver.wo = basis.ToLocal(-path.ray.d);
     receiver.bsdf = sf.bsdf;
+    receiver.gltf = sf.gltf;
     // temporal.slang:67-72 (TemporalInstanceIdentity)
     receiver.instance_id = instance_ids ? instance_ids[hit.instance] : uint64_t(hit.instance) + 1u;
     receiver.material_id =
         material_ids ? material_ids[hit.instance] : uint64_t(s.instance_material[hit.instance]) + 1u;
+    // targets read the closure as stored (HistoryContribution of the stored
+    // receiver: LoadClosure after StoreClosure)
+    float stored[kReuseSurfaceFloats];
+    StoreSurface(receiver, stored);
+    receiver = LoadSurface(stored);
     // temporal.slang:74-79 (TemporalSampleHistory)
     const bool history = history_valid != 0 && centre_instance_id == receiver.instance_id &&
                          centre_material_id == receiver.material_id;
@@ -3586,7 +3637,7 @@ int oracle_restir_temporal(const oracle_scene* scene, const oracle_frame* frame,
 
 // restir.slang:402-491 (WriteSpatialReservoir) for pixel (x, y) over the
 // temporal pass's records and receiver surfaces of the whole target (16
-// words and 32 floats per pixel): the four axis neighbours at `radius`, or
+// words and 40 floats per pixel): the four axis neighbours at `radius`, or
 // with `disk` five drawn uniformly within it from a stream of their own
 // (seeded with sample ^ 0x5BD1E995; radius r sqrt(u), angle 2 pi v, rounded
 // to the nearest pixel; the pixel itself is no neighbour).
@@ -3596,7 +3647,7 @@ void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t h
   const Scene& s = scene->s;
   const uint32_t slot = y * width + x;
   const ReuseRecord centre = LoadReuse(records + 16ull * slot);
-  const ReuseSurface receiver = LoadSurface(surfaces + 32ull * slot);
+  const ReuseSurface receiver = LoadSurface(surfaces + uint64_t(kReuseSurfaceFloats) * slot);
   ReuseRecord result = centre;
   uint32_t src[5] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
   const int r = static_cast<int>(radius);
@@ -3621,7 +3672,7 @@ void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t h
       const int nx = static_cast<int>(x) + ox, ny = static_cast<int>(y) + oy;
       if (nx < 0 || ny < 0 || nx >= static_cast<int>(width) || ny >= static_cast<int>(height)) continue;
       const uint32_t index = static_cast<uint32_t>(ny) * width + static_cast<uint32_t>(nx);
-      if (SpatialCompatible(receiver, records[16ull * index + 7], LoadSurface(surfaces + 32ull * index))) {
+      if (SpatialCompatible(receiver, records[16ull * index + 7], LoadSurface(surfaces + uint64_t(kReuseSurfaceFloats) * index))) {
         src[i] = index;
         ++accepted;
       }
@@ -3662,7 +3713,7 @@ void oracle_restir_spatial(const oracle_scene* scene, uint32_t width, uint32_t h
           const ReuseRecord source = LoadReuse(records + 16ull * src[i]);
           bool valid = true;
           const float support =
-              HistoryTarget(s, source, LoadSurface(surfaces + 32ull * src[i]), result.px, valid);
+              HistoryTarget(s, source, LoadSurface(surfaces + uint64_t(kReuseSurfaceFloats) * src[i]), result.px, valid);
           if (!valid) result.px.flags |= 2u;
           if (support > 0.0f) result.supporting += std::min(source.px.r.count, max_history);
         }

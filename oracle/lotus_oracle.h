@@ -150,11 +150,14 @@ int oracle_restir_initial(const oracle_scene* scene, const oracle_frame* frame, 
 
 /* ReSTIR DI reuse (restir.slang:302-491). A reuse record is RestirReusePixel's
  * 16 words {RestirPixel's 12, normalization, supporting candidates, previous
- * index, age}; a receiver surface is 32 floats: {offset origin, view depth},
+ * index, age}; a receiver surface is 40 floats: {offset origin, view depth},
  * {geometric normal, instance id low bits}, {shading normal, instance id high
- * bits}, {wo in the shading frame, dielectric f90}, {base colour, alpha},
- * {dielectric f0, metallic}, {f0, f90}, {transmission (negated: thin wall),
- * eta (negated: smooth), material id low, high}.
+ * bits}, {wo in the shading frame, c0}, {c1..c4}, {c5..c8}, {c9..c12},
+ * {c13, c14, material id low, high}, {c15..c18}, {c19, c20, c21, model},
+ * the closure words c being UsdPreviewSurface's {dielectric f90; base
+ * colour, alpha; dielectric f0, metallic; f0, f90; transmission (negated:
+ * thin wall), eta (negated: smooth)} and zeros (model 0), or glTF PBR's 22
+ * packed terms (model 1).
  *
  * Temporal pass at pixel (x, y): the oracle's own initial reservoir and
  * receiver surface of the frame's camera ray, merged with the previous frame's

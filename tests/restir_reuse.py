@@ -45,14 +45,14 @@ def oracle_temporal_frame(o, fr, sample, view, records_prev, surfaces_prev, vali
     id), both the renderer's products."""
     h, w = fr.height, fr.width
     rec = np.zeros((h, w, 16), np.uint32)
-    surf = np.zeros((h, w, 32), np.float32)
+    surf = np.zeros((h, w, 40), np.float32)
     hits = np.zeros((h, w), bool)
     for y in range(h):
         for x in range(w):
             valid, pi = (int(validity[y, x, 0]), int(validity[y, x, 1])) if history else (0, NO)
             use = valid != 0 and pi < w * h and records_prev is not None
             prev_r = records_prev.reshape(-1, 16)[pi] if use else None
-            prev_s = surfaces_prev.reshape(-1, 32)[pi] if use else None
+            prev_s = surfaces_prev.reshape(-1, 40)[pi] if use else None
             r, s, hit = o.restir_temporal(fr, x, y, sample, view, use, pi, centre[y, x], prev_r, prev_s, cap)
             rec[y, x], surf[y, x], hits[y, x] = r, s, hit
     return rec, surf, hits
@@ -63,7 +63,7 @@ def oracle_spatial_frame(o, w, h, sample, radius, cap, records, surfaces, disk=F
     records and surfaces: records [h, w, 16] and sources [h, w, 5] (the four
     axis neighbours and an unused fifth, or the disk mode's five)."""
     records = np.ascontiguousarray(records, np.uint32).reshape(-1, 16)
-    surfaces = np.ascontiguousarray(surfaces, np.float32).reshape(-1, 32)
+    surfaces = np.ascontiguousarray(surfaces, np.float32).reshape(-1, 40)
     rec = np.zeros((h, w, 16), np.uint32)
     src = np.zeros((h, w, 5), np.uint32)
     for y in range(h):

@@ -117,10 +117,9 @@ def test_gltf_restir_initial_records_match_oracle(renderer, fixture):
     assert np.mean(np.all(out["color"][..., :3] == acc[..., :3].astype(np.float32), -1)) > 0.999
 
 
-def test_gltf_temporal_frames_and_reuse_refusal(renderer):
-    # temporal accumulation runs over glTF receivers; reservoir reuse, whose
-    # stored surface holds UsdPreviewSurface's terms only, is refused without
-    # latching
+def test_gltf_temporal_frames(renderer):
+    # temporal accumulation over glTF receivers (reservoir reuse over them is
+    # re-derived in test_gpu_restir_reuse.py)
     sc, size = mf.load("material-gltf-extensions-v1")
     renderer.apply(sc)
     for i in range(2):
@@ -128,8 +127,5 @@ def test_gltf_temporal_frames_and_reuse_refusal(renderer):
                                        max_history=8, max_bounces=8)
         assert np.isfinite(out["color"]).all()
     assert out["history_available"]
-    with pytest.raises(hip.LotusError, match="reservoir reuse over glTF PBR"):
-        renderer.render_temporal(sc.world_to_clip, sc.view, size, size, frame_index=3, restir_temporal=True,
-                                 light_sampling=hip.LIGHT_RESTIR, max_bounces=8)
     out = renderer.render(sc.world_to_clip, size, size, max_bounces=8)
     assert out["spp"] == 1

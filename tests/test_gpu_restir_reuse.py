@@ -15,6 +15,7 @@ import pytest
 import oracle
 from lotus_amd import hip, scenes
 
+import material_fixtures as mf
 import restir_reuse as rr
 
 pytestmark = [pytest.mark.gpu]
@@ -30,8 +31,8 @@ def renderer():
 
 
 def _compare_surfaces(got, want, hits, what):
-    g = got.reshape(-1, 32)[hits.reshape(-1)]
-    w = want.reshape(-1, 32)[hits.reshape(-1)]
+    g = got.reshape(-1, 40)[hits.reshape(-1)]
+    w = want.reshape(-1, 40)[hits.reshape(-1)]
     gb, wb = g.view(np.uint32), w.view(np.uint32)
     ids = [7, 11, 30, 31]  # instance and material id words
     assert np.array_equal(gb[:, ids], wb[:, ids]), what
@@ -131,3 +132,14 @@ def test_reuse_five_neighbours_within_the_radius(renderer):
     d2 = (dx * dx + dy * dy)[ok]
     print(f"accepted neighbours {ok.sum()}, mean squared distance {d2.mean():.1f}")
     assert ok.sum() > 2000 and d2.max() <= 31 * 31 and d2.min() >= 1
+
+
+@pytest.mark.parametrize("fixture", ["material-gltf-v1", "material-gltf-extensions-v1"])
+def test_reuse_over_gltf_receivers(renderer, fixture):
+    # glTF PBR receivers keep their closure's 22 terms in the stored surface
+    # (StoreClosure / LoadClosure, restir.slang:80-141): coat, sheen,
+    # specular, thin transmission and metal lobes re-evaluated at the
+    # neighbours' and the previous frame's samples.
+    sc, _ = mf.load(fixture)
+    _run(renderer, sc, 64, 64, 4, lambda f: sc.view, radius=4, cap=20, candidates=8, max_bounces=4,
+         spatial=True, first_sample=9)
