@@ -420,7 +420,9 @@ def render_buffer_run(r, width, height, calls=200):
 def batch_run(views, frames):
     """64 seeded views of 64^2..512^2 over their own icosphere rigs (1k..200k
     triangles, depth 2..12): each view autotuned, then frames rendered view
-    by view (one call each), as one batch, and as one hipGraph per frame."""
+    by view (one call each), as one batch, as one hipGraph per frame, and as
+    one fused batch (every view's paths in one path-megakernel launch fed by
+    one device work queue)."""
     rs = []
     t_build = time.perf_counter()
     for v in views:
@@ -434,7 +436,10 @@ def batch_run(views, frames):
     res = {}
     batch = hip.Batch(rs, kws)  # the views' records built once, as a serving loop keeps them
     graph_kinds = []
-    for mode in ("per_view", "batch", "graph"):
+    for mode in ("per_view", "batch", "graph", "fused"):
+        if mode == "fused":
+            for r in rs:
+                r.set_tuning(megakernel=1)
         for r, kw in zip(rs, kws):  # warm-up frame: rounds and targets settle
             r.render(want_color=False, sample_index=0, **kw)
         if mode == "graph":
@@ -448,7 +453,7 @@ def batch_run(views, frames):
                     rays += out["camera_rays"] + out["extension_rays"] + out["shadow_rays"]
                     gpu_ms += out["frame_ms"]
             else:
-                n, ms, kind = batch.run(graph=mode == "graph", sample_index=f + 1)
+                n, ms, kind = batch.run(graph=mode == "graph", sample_index=f + 1, fused=mode == "fused")
                 rays += n
                 gpu_ms += ms
                 if mode == "graph":
@@ -464,6 +469,7 @@ def batch_run(views, frames):
             "frames": frames, "scene_build_s": round(build_s, 2),
             "autotuned_dynamic_trace": sum(t["trace_dynamic"] for t in tunings),
             "autotuned_half_resident": sum(t["resident_percent"] == 50 for t in tunings),
+            "autotuned_megakernel": sum(t["megakernel"] for t in tunings),
             "graph_frames_replayed": sum(k == 3 for k in graph_kinds), **res}
 
 

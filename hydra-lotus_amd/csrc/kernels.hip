@@ -1142,6 +1142,173 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(SceneView S, FrameConst F,
   flush_stats(stats, 3, visits, tests);
 }
 
+// A whole path in one lane pair, from its camera ray to its end: the path
+// lane walks the camera ray and then each extension ray and shades every
+// hit (ScatterPath, transport.slang:164-230); the odd lane walks the shadow
+// ray of the previous bounce beside it, as the tail does. Path i of the
+// data window is pixel (i mod 32) of its 8 x 4 tile i / 32, so a wave's 32
+// pairs start from neighbouring pixels. A camera miss leaves no sample, as
+// wavefront.slang:206-225 does; counts are those of the rounds (each ray at
+// its bounce of departure), and the sample the resolve reads is complete,
+// with no light or environment term pending. Both lanes of a pair call this
+// with the same i.
+template <bool kAnalytic, bool kCov>
+__device__ __forceinline__ void camera_path(const SceneView& S, const FrameConst& F, const Queues& q,
+                                            uint32_t* counters, TraceStack& st, uint32_t i, uint32_t& visits,
+                                            uint32_t& tests) {
+  const bool path_lane = (threadIdx.x & 1u) == 0;
+  const uint32_t rw = F.rect[2], rh = F.rect[3];
+  const uint32_t tiles_x = (rw + 7) / 8, tile = i >> 5, within = i & 31u;
+  const uint32_t lx = (tile % tiles_x) * 8 + (within & 7u), ly = (tile / tiles_x) * 4 + (within >> 3);
+  if (lx >= rw || ly >= rh) return;
+  const uint32_t x = F.rect[0] + lx, y = F.rect[1] + ly, slot = y * F.width + x;
+  PathRegs p{};
+  Ray camera{};
+  if (path_lane) {
+    uint32_t rng;
+    camera = start_path(F, x, y, frame_sample_index(F), rng);
+    p.o = camera.o;
+    p.d = camera.d;
+    p.depth = 0;
+    p.throughput = V3{1.0f, 1.0f, 1.0f};
+    p.last_pdf = 0.0f;
+    p.radiance = V3{0.0f, 0.0f, 0.0f};
+    p.rng = rng;
+  }
+  bool first = true, missed = false;
+  bool extend = path_lane;  // the path lane has a ray to walk
+  bool shadow = false;      // a shadow ray of the previous bounce is pending
+  Ray sray{};
+  V3 delta{};
+  ShadowFactors sf4{};
+  while (true) {
+    const bool pair_extend = __shfl_xor(static_cast<int>(extend), 1) != 0 || extend;
+    const bool pair_shadow = __shfl_xor(static_cast<int>(shadow), 1) != 0 || shadow;
+    if (!pair_extend && !pair_shadow) break;
+    const Ray sr{V3{pair_float(sray.o.x), pair_float(sray.o.y), pair_float(sray.o.z)}, pair_float(sray.tmin),
+                 V3{pair_float(sray.d.x), pair_float(sray.d.y), pair_float(sray.d.z)}, pair_float(sray.tmax)};
+    WalkT<CoverageOf<kCov>> w;
+    w.cov = coverage_of<kCov>(S);
+    if (path_lane) {
+      w.start(first ? camera : Ray{p.o, 0.0f, p.d, kFloatMax}, false);
+      if (!extend) w.cur = kWalkDone;
+    } else {
+      w.start(sr, true);
+      if (!pair_shadow) w.cur = kWalkDone;
+    }
+    st.clear();
+    w.run(S.nodes, S.tris, st);
+    visits += w.visits;
+    tests += w.tests;
+    if constexpr (kCov) {
+      const float transmittance = pair_float(w.transmittance());
+      if (path_lane && shadow && transmittance > 0.0f)
+        p.radiance = p.radiance + shadowed_delta(sf4.fl, sf4.weight, sf4.pdf, sf4.thr, transmittance);
+    } else {
+      const bool blocked = __shfl_xor(static_cast<int>(w.hit()), 1) != 0;
+      if (path_lane && shadow && !blocked) p.radiance = p.radiance + delta;
+    }
+    shadow = false;
+    Scatter sc{};
+    ShadowFactors sc4;
+    bool hit = false;
+    if (path_lane && extend) {
+      hit = w.hit();
+      if (!hit) {
+        if (first) {
+          missed = true;
+        } else {
+          V3 e, crossed;
+          escape_terms<kAnalytic>(S, F.light_sampling != 0, p.o, p.d, p.throughput, p.last_pdf, e, crossed);
+          p.radiance = p.radiance + e;
+          if (kAnalytic && S.analytic_count != 0) p.radiance = p.radiance + crossed;
+        }
+      } else {
+        const HitRecord h = w.record(S.tris);
+        sc = scatter_path<kAnalytic, kCov>(S, F, p, h.tri, h.instance, h.primitive, h.u, h.v, nullptr, slot,
+                                           kCov ? &sc4 : nullptr);
+      }
+    }
+    const bool shaded = path_lane && extend && hit;
+    wave_count_each(shadow_count(counters, p.depth - (sc.continues ? 1u : 0u)), shaded && sc.shadow);
+    wave_count_each(ray_count(counters, p.depth), shaded && sc.continues);
+    if (shaded && sc.shadow) {
+      shadow = true;
+      sray = sc.shadow_ray;
+      delta = sc.delta;
+      if constexpr (kCov) sf4 = sc4;
+    }
+    extend = shaded && sc.continues;
+    first = false;
+  }
+  if (path_lane) {
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q.rad[slot] = missed ? zero : make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 1.0f);
+    q.nee[slot] = zero;
+    q.esc[slot] = zero;
+  }
+}
+
+// Paths of a data window of 8 x 4 tiles.
+__host__ __device__ inline uint32_t window_paths(uint32_t rw, uint32_t rh) {
+  return ((rw + 7) / 8) * ((rh + 3) / 4) * 32u;
+}
+
+// The path megakernel of one view: every path of the data window from its
+// camera ray to its end (camera_path), the pairs striding over the paths.
+template <bool kAnalytic, bool kCov>
+__global__ void __launch_bounds__(kTraceBlock) k_paths(SceneView S, FrameConst F, Queues q, uint32_t* counters,
+                                                       uint2* spill, int spill_cap, unsigned long long* stats) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  TraceStack st;
+  st.init(ring, spill, spill_cap);
+  const uint32_t rw = F.rect[2], rh = F.rect[3];
+  if (gtid() == 0) *ray_count(counters, 0) = rw * rh;
+  const uint32_t n = window_paths(rw, rh);
+  uint32_t visits = 0, tests = 0;
+  for (uint32_t i = gtid() >> 1; i < n; i += gstride() >> 1)
+    camera_path<kAnalytic, kCov>(S, F, q, counters, st, i, visits, tests);
+  flush_stats(stats, 3, visits, tests);
+}
+
+// The path megakernel of a batch of views fed by one device work queue: each
+// wave takes the next chunk of kPathChunk paths from a global counter (one
+// atomic per chunk), finds the view that holds it in the view table (views
+// in order, each with its first chunk) and runs its paths with that view's
+// scene, frame and buffers. The grid drains when the counter passes the
+// last chunk.
+template <bool kAnalytic, bool kCov>
+__global__ void __launch_bounds__(kTraceBlock) k_paths_views(const PathView* __restrict__ views, uint32_t view_count,
+                                                             uint32_t* next_chunk, uint32_t total_chunks,
+                                                             uint2* spill, int spill_cap) {
+  __shared__ uint2 ring[kRing * kTraceBlock];
+  TraceStack st;
+  st.init(ring, spill, spill_cap);
+  const uint32_t lane = __lane_id();
+  while (true) {
+    uint32_t c = 0;
+    if (lane == 0) c = atomicAdd(next_chunk, 1u);
+    c = __shfl(c, 0);
+    if (c >= total_chunks) break;
+    // the last view whose first chunk is at most c (binary search)
+    uint32_t lo = 0, hi = view_count;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (views[mid].first_chunk <= c) lo = mid; else hi = mid;
+    }
+    const PathView& V = views[lo];
+    const uint32_t base = (c - V.first_chunk) * kPathChunk;
+    uint32_t visits = 0, tests = 0;
+    for (uint32_t j = lane >> 1; j < kPathChunk; j += 32) {
+      const uint32_t i = base + j;
+      if (i >= V.paths) break;
+      if (i == 0 && (lane & 1u) == 0) *ray_count(V.counters, 0) = V.F.rect[2] * V.F.rect[3];
+      camera_path<kAnalytic, kCov>(V.S, V.F, V.q, V.counters, st, i, visits, tests);
+    }
+    flush_stats(V.stats, 3, visits, tests);
+  }
+}
+
 // path_trace.slang:106-139 (kResolvePass): each slot's sample (its radiance,
 // then the pending direct light, then the pending environment) added to the
 // pixel's accumulation, and the mean written, a missed sample counting as the
@@ -1288,13 +1455,14 @@ GridShape grid_shape(uint32_t cus) {
   g.tail = persistent_blocks(k_tail<false, false, false>, kTraceBlock, cus);
   g.tail_cache = persistent_blocks(k_tail<true, false, false>, kTraceBlock, cus);
   g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
+  g.paths = persistent_blocks(k_paths_views<true, true>, kTraceBlock, cus);
   g.rays = persistent_blocks(k_rays, kTraceBlock, cus);
   g.shade = persistent_blocks(k_shade<false, false>, kShadeBlock, cus);
   g.shade_parallel = persistent_blocks(k_shade<true, false, true>, kShadeBlock, cus);
   g.resolve = cus * 4;
   g.max_trace_lanes =
       std::max({g.camera, g.camera_dynamic, g.trace, g.trace_dynamic, g.tail, g.tail_cache, g.camera_pass, g.rays,
-                g.shade_parallel}) * kTraceBlock;
+                g.shade_parallel, g.paths}) * kTraceBlock;
   return g;
 }
 
@@ -1414,6 +1582,33 @@ void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const
   else
     k_tail<false, false, false><<<grid, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, bounce, L.stats);
 }
+bool paths_eligible(const SceneView& S, const FrameConst& F) {
+  return F.light_sampling <= 1u && F.cache_depth == 0 && F.restir_surf == nullptr && S.gltf_materials == 0 &&
+         !parallel_walks(S, F);
+}
+void launch_paths(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
+  const bool analytic = scene_has_extras(S), cov = S.covered_instances != 0;
+  if (cov)
+    k_paths<true, true><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+  else if (analytic)
+    k_paths<true, false><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+  else
+    k_paths<false, false><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+}
+void launch_paths_views(const Launch& L, const PathView* views, uint32_t view_count, bool analytic, bool cov,
+                        uint32_t* next_chunk, uint32_t total_chunks) {
+  if (cov)
+    k_paths_views<true, true><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk,
+                                                                          total_chunks, L.spill, L.spill_cap);
+  else if (analytic)
+    k_paths_views<true, false><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk,
+                                                                           total_chunks, L.spill, L.spill_cap);
+  else
+    k_paths_views<false, false><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk,
+                                                                            total_chunks, L.spill, L.spill_cap);
+}
+uint32_t path_count(const FrameConst& F) { return window_paths(F.rect[2], F.rect[3]); }
+uint32_t path_chunks(const FrameConst& F) { return (path_count(F) + kPathChunk - 1) / kPathChunk; }
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample) {
   k_resolve<<<L.grid.resolve, 256, 0, L.stream>>>(F, q, add_sample);
 }

@@ -64,7 +64,7 @@ constexpr float kCacheMinThroughput = 1e-3f;
 // workgroups waiting for a CU, and a strided launch then ends a pass late).
 struct GridShape {
   uint32_t camera, camera_dynamic, trace, trace_dynamic, tail, tail_cache, camera_pass, rays, shade, shade_parallel,
-      resolve;
+      resolve, paths;
   uint32_t max_trace_lanes;  // the most traversal lanes any grid holds
 };
 GridShape grid_shape(uint32_t cus);
@@ -112,6 +112,29 @@ void launch_tail(const Launch& L, const SceneView& S, const FrameConst& F, const
 // escaped rays (scenes with lights; `weigh`: light sampling is on).
 void launch_escape(const Launch& L, const SceneView& S, const Queues& q, uint32_t* c, uint32_t bounce, bool weigh);
 void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint32_t add_sample);
+
+// The path megakernel: every path of a frame from its camera ray to its end
+// in one launch, for frames of BSDF or MIS light sampling without the
+// radiance cache, reservoirs, glTF PBR or several parallel lights under BSDF
+// sampling (paths_eligible); its results equal the wavefront's bit for bit.
+// One view per launch, or a batch of views fed by one device work queue of
+// kPathChunk-path chunks (a PathView table on the device, views in order).
+constexpr uint32_t kPathChunk = 256;
+struct PathView {
+  SceneView S;
+  FrameConst F;
+  Queues q;
+  uint32_t* counters;
+  unsigned long long* stats;
+  uint32_t first_chunk;  // the view's first chunk of the batch's queue
+  uint32_t paths;        // its data window's paths, 8 x 4 tiles (path_chunks of them in chunks)
+};
+bool paths_eligible(const SceneView& S, const FrameConst& F);
+uint32_t path_count(const FrameConst& F);   // paths of the data window (8 x 4 tiles, whole)
+uint32_t path_chunks(const FrameConst& F);  // their chunks
+void launch_paths(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c);
+void launch_paths_views(const Launch& L, const PathView* views, uint32_t view_count, bool analytic, bool cov,
+                        uint32_t* next_chunk, uint32_t total_chunks);
 void launch_camera_pass(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, float* depth,
                         uint32_t write_color);
 void launch_rays(const Launch& L, const SceneView& S, const float* rays, uint32_t* hits, uint32_t count,

@@ -267,7 +267,7 @@ struct lotus_hip_renderer {
   uint2* spill = nullptr;
   size_t spill_entries = 0;  // per lane
   GridShape grid{};
-  lotus_hip_tuning tuning{0u, 0u, 16u};
+  lotus_hip_tuning tuning{0u, 0u, 16u, 0u, 0u};
 
   AccumulationKey key;
   bool key_valid = false;
@@ -331,6 +331,19 @@ struct lotus_hip_renderer {
   std::vector<uint8_t> graph_signature;
   uint32_t* batch_vars = nullptr;  // the batch leader's frame variables, 4 words per view
   uint32_t batch_vars_count = 0;
+  // A fused batch's view (render_impl stops before the paths and records
+  // them here), and, held by the batch leader, the device view table, its
+  // work-queue counters (one per kernel variant) and a traversal spill area
+  // for the fused launches.
+  bool fuse = false;
+  bool fused_pending = false;
+  PathView fused_view{};
+  uint32_t fused_add = 0;
+  PathView* fused_table = nullptr;
+  uint32_t fused_table_count = 0;
+  uint32_t* fused_queue = nullptr;  // 3 words
+  uint2* fused_spill = nullptr;
+  size_t fused_spill_entries = 0;
   uint64_t batch_instantiations = 0, batch_updates = 0, batch_replays = 0;
   float4* svgf_filter = nullptr;   // lotus_hip_svgf's scratch
   uint32_t svgf_pixels = 0;
@@ -1794,6 +1807,9 @@ void lotus_hip_destroy(lotus_hip_renderer* r) {
   if (r->join) (void)hipEventDestroy(r->join);
   if (r->batch_exec) (void)hipGraphExecDestroy(r->batch_exec);
   dev_free(r->batch_vars);
+  dev_free(r->fused_table);
+  dev_free(r->fused_queue);
+  dev_free(r->fused_spill);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
 }
@@ -2128,7 +2144,24 @@ int render_impl(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_hi
         launch_zero(r->q.crec_n, r->train_capacity, r->stream);
         launch_fill1(r->cache.loss, 1, 0.0f, r->stream);
       }
-      if (add) {
+      // the path megakernel: each path from its camera ray to its end in one
+      // launch (its timer is the tail's)
+      const bool mega = add && r->tuning.megakernel != 0 && !split && !tf && paths_eligible(r->view, F);
+      if (mega && r->fuse && deferred) {
+        // a fused batch launches the paths of all its views itself, then
+        // each view's resolve
+        r->fused_view = PathView{r->view, F, r->q, r->counters, r->stats, 0u, path_count(F)};
+        r->fused_add = add;
+        r->fused_pending = true;
+        r->deferred = {true, add != 0, stride, 0u, 0u};
+        return LOTUS_HIP_PASS;
+      }
+      if (mega) {
+        mark(kTTail);
+        launch_paths(L, r->view, F, r->q, r->counters);
+        done();
+        rounds = 0;
+      } else if (add) {
         if (tf && tf->reuse) launch_restir_clear(L, r->q.restir, const_cast<float4*>(tf->A.surf), npix);
         mark(kTCamera);
         launch_camera(L, r->view, F, r->q, r->counters);
@@ -3414,6 +3447,78 @@ int batch_finish(lotus_hip_renderer* r, const lotus_hip_frame_params* P, lotus_h
 
 extern "C" {
 
+// A fused batch's paths, enqueued on the leader's stream once every view's
+// stream has joined it: the views whose frames the megakernel runs (those
+// render_impl left pending) go into one device table, grouped by kernel
+// variant (plain, extras, covered surfaces), each group one launch fed by
+// its own work-queue counter; then each such view's resolve on its own
+// stream, joined back. Views render_impl ran in full are already done.
+hipError_t fused_paths(lotus_hip_renderer* root, lotus_hip_renderer* const* renderers, uint32_t count) {
+  std::vector<PathView> groups[3];
+  size_t need = 4;  // spill entries per lane: the deepest view's (as ensure_launch sizes them)
+  for (uint32_t i = 0; i < count; ++i) {
+    lotus_hip_renderer* r = renderers[i];
+    if (!r->fused_pending) continue;
+    const SceneView& S = r->fused_view.S;
+    const int g = S.covered_instances != 0 ? 2 : scene_has_extras(S) ? 1 : 0;
+    PathView v = r->fused_view;
+    const uint32_t first = groups[g].empty() ? 0u : groups[g].back().first_chunk + path_chunks(groups[g].back().F);
+    v.first_chunk = first;
+    groups[g].push_back(v);
+    const size_t stack = 3 * size_t(r->scene.depth) + 4;
+    need = std::max(need, stack > size_t(kRing) ? stack - kRing : size_t(0));
+  }
+  std::vector<PathView> table;
+  for (const auto& g : groups) table.insert(table.end(), g.begin(), g.end());
+  hipError_t e = hipSuccess;
+  if (!table.empty()) {
+    if (root->fused_table_count < table.size()) {
+      dev_free(root->fused_table);
+      root->fused_table_count = 0;
+      e = dev_alloc(root->fused_table, table.size());
+      if (e == hipSuccess) root->fused_table_count = static_cast<uint32_t>(table.size());
+    }
+    if (e == hipSuccess && !root->fused_queue) e = dev_alloc(root->fused_queue, 3);
+    if (e == hipSuccess && (!root->fused_spill || root->fused_spill_entries < need)) {
+      dev_free(root->fused_spill);
+      root->fused_spill_entries = 0;
+      e = dev_alloc(root->fused_spill, need * size_t(root->grid.max_trace_lanes));
+      if (e == hipSuccess) root->fused_spill_entries = need;
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(root->fused_table, table.data(), table.size() * sizeof(PathView), hipMemcpyHostToDevice,
+                         root->stream);
+    if (e == hipSuccess) launch_zero(root->fused_queue, 3, root->stream);
+    Launch L = launch_for(root, root->stream);
+    L.spill = root->fused_spill;
+    L.spill_cap = static_cast<int>(root->fused_spill_entries);
+    size_t offset = 0;
+    for (int g = 0; g < 3 && e == hipSuccess; ++g) {
+      if (groups[g].empty()) continue;
+      const uint32_t chunks = groups[g].back().first_chunk + path_chunks(groups[g].back().F);
+      if (chunks != 0)
+        launch_paths_views(L, root->fused_table + offset, static_cast<uint32_t>(groups[g].size()), g >= 1, g == 2,
+                           root->fused_queue + g, chunks);
+      offset += groups[g].size();
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+  }
+  // every view's resolve after the paths, each on its own stream
+  if (e == hipSuccess) e = hipEventRecord(root->join, root->stream);
+  for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+    lotus_hip_renderer* r = renderers[i];
+    if (!r->fused_pending) continue;
+    if (i != 0) e = hipStreamWaitEvent(r->stream, root->join, 0);
+    if (e == hipSuccess) launch_resolve(launch_for(r, r->stream), r->fused_view.F, r->q, r->fused_add);
+    r->fused_pending = false;
+  }
+  for (uint32_t i = 1; i < count && e == hipSuccess; ++i) {
+    e = hipEventRecord(renderers[i]->join, renderers[i]->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(root->stream, renderers[i]->join, 0);
+  }
+  return e;
+}
+
 int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip_frame_params* params,
                            lotus_hip_frame_result* results, uint32_t count, uint32_t flags, double* gpu_ms,
                            char* detail, size_t detail_size) {
@@ -3443,6 +3548,11 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
       set_detail(detail, detail_size, "batched views render one radiance sample with BSDF or MIS light sampling");
       return LOTUS_HIP_FAIL;
     }
+  }
+  const bool fused = (flags & LOTUS_HIP_BATCH_FUSED) != 0;
+  if (fused && (flags & LOTUS_HIP_BATCH_GRAPH) != 0) {
+    set_detail(detail, detail_size, "fused batches run eagerly, not as a graph");
+    return LOTUS_HIP_FAIL;
   }
   lotus_hip_renderer* root = renderers[0];
   (void)hipSetDevice(root->device);
@@ -3521,8 +3631,11 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
   int status = LOTUS_HIP_PASS;
   for (uint32_t i = 0; i < count && e == hipSuccess && status == LOTUS_HIP_PASS; ++i) {
     renderers[i]->graph_batch = graph;
+    renderers[i]->fuse = fused;
+    renderers[i]->fused_pending = false;
     status = render_impl(renderers[i], &params[i], nullptr, detail, detail_size, nullptr, true);
     renderers[i]->graph_batch = false;
+    renderers[i]->fuse = false;
     if (graph && status == LOTUS_HIP_PASS) {
       std::memcpy(&vars[4ull * i], renderers[i]->graph_host_vars, 16);
       renderers[i]->graph_deferred = renderers[i]->deferred;
@@ -3532,6 +3645,8 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
     e = hipEventRecord(renderers[i]->join, renderers[i]->stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(root->stream, renderers[i]->join, 0);
   }
+  if (fused && e == hipSuccess && status == LOTUS_HIP_PASS) e = fused_paths(root, renderers, count);
+  for (uint32_t i = 0; i < count; ++i) renderers[i]->fused_pending = false;
   hipGraph_t g = nullptr;
   bool updated_graph = false;
   if (graph) {
@@ -3606,10 +3721,12 @@ int lotus_hip_autotune(lotus_hip_renderer* r, const lotus_hip_frame_params* para
                        lotus_hip_tuning* chosen, char* detail, size_t detail_size) {
   if (!r || !params) return LOTUS_HIP_FAIL;
   std::vector<lotus_hip_tuning> candidates;
-  for (uint32_t pct : {100u, 50u})
+  for (uint32_t pct : {100u, 50u}) {
     for (uint32_t div : {16u, 0u})
       for (uint32_t cam : {0u, 1u})
-        for (uint32_t tr : {0u, 1u}) candidates.push_back(lotus_hip_tuning{cam, tr, div, pct});
+        for (uint32_t tr : {0u, 1u}) candidates.push_back(lotus_hip_tuning{cam, tr, div, pct, 0u});
+    candidates.push_back(lotus_hip_tuning{0u, 0u, 16u, pct, 1u});  // the path megakernel
+  }
   double best = 1e300;
   lotus_hip_tuning pick = r->tuning;
   lotus_hip_frame_params P = *params;

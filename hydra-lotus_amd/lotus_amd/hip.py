@@ -40,6 +40,7 @@ AOV_MOTION, AOV_HISTORY_LENGTH, AOV_HISTORY_VALIDITY, AOV_HISTORY_REJECTION = ra
 AOV_RESTIR_LIGHT, AOV_RESTIR_CANDIDATES, AOV_RESTIR_SUPPORT, AOV_RESTIR_AGE, AOV_RESTIR_WEIGHT, \
     AOV_RESTIR_DIRECT = range(6)
 BATCH_GRAPH = 1
+BATCH_FUSED = 2
 # Spatial reuse neighbours (lotus_hip_temporal_settings.restir_spatial_neighbours).
 SPATIAL_AXES, SPATIAL_DISK = 0, 1
 # History rectification (lotus_hip_temporal_settings.history_rectification).
@@ -213,7 +214,7 @@ class SceneContents(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("camera_dynamic", C.c_uint32), ("trace_dynamic", C.c_uint32), ("tail_divisor", C.c_uint32),
-                ("resident_percent", C.c_uint32)]
+                ("resident_percent", C.c_uint32), ("megakernel", C.c_uint32)]
 
 
 class FrameParams(C.Structure):
@@ -491,21 +492,24 @@ class Batch:
         self.ms = C.c_double(0.0)
         self.buf = C.create_string_buffer(512)
 
-    def run(self, graph=False, sample_index=None):
+    def run(self, graph=False, sample_index=None, fused=False):
         if sample_index is not None:
             for i in range(self.n):
                 self.ps[i].sample_index = sample_index
-        s = lib().lotus_hip_render_batch(self.handles, self.ps, self.rs, self.n, BATCH_GRAPH if graph else 0,
-                                         C.byref(self.ms), self.buf, 512)
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_FUSED if fused else 0)
+        s = lib().lotus_hip_render_batch(self.handles, self.ps, self.rs, self.n, flags, C.byref(self.ms), self.buf,
+                                         512)
         if s != PASS:
             raise LotusError(s, self.buf.value.decode())
         rays = sum(r.camera_rays + r.extension_rays + r.shadow_rays for r in self.rs)
         return rays, self.ms.value, self.rs[0].batch_graph
 
 
-def render_batch(renderers, views, graph=False, want_color=True):
+def render_batch(renderers, views, graph=False, want_color=True, fused=False):
     """One batched frame: views[i] (frame_params() keyword dicts) through
-    renderers[i]. Returns (per-view results, GPU ms)."""
+    renderers[i]; `fused` runs the views' paths in shared path-megakernel
+    launches fed by one device work queue. Returns (per-view results, GPU
+    ms)."""
     n = len(renderers)
     handles = (C.c_void_p * n)(*[r.handle for r in renderers])
     ps = (FrameParams * n)()
@@ -524,7 +528,8 @@ def render_batch(renderers, views, graph=False, want_color=True):
         counts.append(k)
     ms = C.c_double(0.0)
     buf = C.create_string_buffer(512)
-    s = lib().lotus_hip_render_batch(handles, ps, rs, n, BATCH_GRAPH if graph else 0, C.byref(ms), buf, 512)
+    flags = (BATCH_GRAPH if graph else 0) | (BATCH_FUSED if fused else 0)
+    s = lib().lotus_hip_render_batch(handles, ps, rs, n, flags, C.byref(ms), buf, 512)
     if s != PASS:
         raise LotusError(s, buf.value.decode())
     out = []
@@ -705,7 +710,8 @@ class Renderer:
         t = Tuning()
         lib().lotus_hip_get_tuning(self.handle, C.byref(t))
         return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic,
-                "tail_divisor": t.tail_divisor, "resident_percent": t.resident_percent}
+                "tail_divisor": t.tail_divisor, "resident_percent": t.resident_percent,
+                "megakernel": t.megakernel}
 
     def set_tuning(self, **kw):
         t = Tuning()
@@ -941,4 +947,5 @@ class Renderer:
         if s != PASS:
             raise LotusError(s, buf.value.decode())
         return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic,
-                "tail_divisor": t.tail_divisor, "resident_percent": t.resident_percent}
+                "tail_divisor": t.tail_divisor, "resident_percent": t.resident_percent,
+                "megakernel": t.megakernel}

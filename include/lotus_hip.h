@@ -384,6 +384,11 @@ typedef struct lotus_hip_tuning {
   uint32_t resident_percent; /* the persistent traversal kernels' (camera, trace, tail) workgroups as a
                                 percentage of what their occupancy lets the device hold at once, in
                                 [1, 100]: fewer workgroups each walk more rays; 0 means 100 */
+  uint32_t megakernel;       /* 1: a frame of BSDF or MIS light sampling runs each path from its camera ray
+                                to its end in one persistent launch (a lane pair per path, the tail's form)
+                                instead of wavefront rounds; frames with the radiance cache, reservoirs,
+                                temporal products, glTF PBR or several parallel lights under BSDF sampling
+                                keep the rounds */
 } lotus_hip_tuning;
 
 int lotus_hip_set_tuning(lotus_hip_renderer* renderer, const lotus_hip_tuning* tuning, char* detail,
@@ -655,6 +660,9 @@ int lotus_hip_reproject(lotus_hip_renderer* renderer, const void* current_device
  * (optional) receive each view's counts and colour; gpu_ms the frame's GPU
  * time. Results are the same as unbatched frames. */
 #define LOTUS_HIP_BATCH_GRAPH 1u
+#define LOTUS_HIP_BATCH_FUSED 2u /* one path megakernel for every view, fed by one device work queue of
+                                    256-path chunks (views of the same kernel variant share a launch);
+                                    every view's frame must be one the megakernel runs; eager only */
 int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip_frame_params* params,
                            lotus_hip_frame_result* results, uint32_t count, uint32_t flags, double* gpu_ms,
                            char* detail, size_t detail_size);

@@ -90,7 +90,7 @@ def test_autotune_keeps_results():
     a, b = _renderers(vs + vs)
     try:
         chosen = b.autotune(trials=1, **_kw(vs[0]))
-        assert set(chosen) == {"camera_dynamic", "trace_dynamic", "tail_divisor", "resident_percent"}
+        assert set(chosen) == {"camera_dynamic", "trace_dynamic", "tail_divisor", "resident_percent", "megakernel"}
         assert chosen["tail_divisor"] in (0, 16) and chosen["resident_percent"] in (50, 100)
         x = a.render(**_kw(vs[0], 5))["color"]
         y = b.render(**_kw(vs[0], 5))["color"]
@@ -98,3 +98,49 @@ def test_autotune_keeps_results():
     finally:
         a.close()
         b.close()
+
+
+def _lit_view():
+    """The lit Cornell box under a sky image (analytic lights and an
+    environment: the megakernel's variant with extras) as a batch view."""
+    from test_gpu_lights import lit_cornell
+    sc = lit_cornell()
+    sc.environment_map = scenes.sky_image()
+    return sc, dict(world_to_clip=sc.world_to_clip, width=48, height=40, max_bounces=6, sample_index=0,
+                    light_sampling=hip.LIGHT_MIS)
+
+
+def test_fused_batch_matches_unbatched():
+    # one path megakernel per kernel variant for every view, fed by one
+    # device work queue: each view accumulates what the wavefront rounds
+    # accumulate rendered alone (colour, counts, shadow rays bit for bit)
+    vs = _views()
+    alone, fused = _renderers(vs), _renderers(vs)
+    lit, lit_kw = _lit_view()
+    alone_lit, fused_lit = hip.Renderer(0), hip.Renderer(0)
+    alone_lit.apply(lit)
+    fused_lit.apply(lit)
+    alone.append(alone_lit)
+    fused.append(fused_lit)
+    kws = [_kw(v) for v in vs] + [lit_kw]
+    sizes = [(v["size"], v["size"]) for v in vs] + [(lit_kw["width"], lit_kw["height"])]
+    try:
+        for r in fused:
+            r.set_tuning(megakernel=1)
+        with pytest.raises(hip.LotusError, match="eagerly"):
+            hip.render_batch(fused, kws, graph=True, fused=True)
+        for frame in range(3):
+            ref = [r.render(**kw) for r, kw in zip(alone, kws)]
+            out, ms = hip.render_batch(fused, kws, fused=True)
+            print(f"fused frame {frame}: {ms:.3f} ms", flush=True)
+            for i, (w, h) in enumerate(sizes):
+                a, na = alone[i].accumulation(w, h)
+                b, nb = fused[i].accumulation(w, h)
+                assert na == nb == frame + 1
+                assert np.array_equal(a, b), (frame, i)
+                assert np.array_equal(ref[i]["color"], out[i]["color"]), (frame, i)
+                assert np.array_equal(out[i]["counts"], ref[i]["counts"]), (frame, i, out[i]["counts"], ref[i]["counts"])
+                assert out[i]["shadow_rays"] == ref[i]["shadow_rays"]
+    finally:
+        for r in alone + fused:
+            r.close()

@@ -426,3 +426,32 @@ def test_display_and_data_windows(renderer, display, data):
     exact = np.mean(np.all(color[hit][:, :3] == acc[hit][:, :3].astype(np.float32), -1))
     _say(f"  hits {hit.sum()}, misses {miss.sum()}, bit-exact {exact:.6f}")
     assert hit.sum() > 100 and exact > 0.999
+
+
+@pytest.mark.parametrize("light", [hip.LIGHT_BSDF, hip.LIGHT_MIS])
+@pytest.mark.parametrize("which", ["lambert", "lit_sky", "covered"])
+def test_path_megakernel_equals_the_rounds(renderer, light, which):
+    # each path from its camera ray to its end in one launch (lane pairs, as
+    # the tail walks them) gives the rounds' frame: counts, shadow rays and
+    # colour bit for bit, in the plain, extras and covered-surface variants
+    if which == "lambert":
+        sc = scenes.cornell_lambert()
+    elif which == "lit_sky":
+        from test_gpu_lights import lit_cornell
+        sc = lit_cornell()
+        sc.environment_map = scenes.sky_image()
+    else:
+        import material_fixtures as mf
+        sc, _ = mf.load("material-alpha-v1")
+    renderer.apply(sc)
+    outs = []
+    for mega in (0, 1):
+        renderer.set_tuning(megakernel=mega)
+        renderer.render(sc.world_to_clip, 80, 64, frame_count=1, max_bounces=8, sample_index=90)  # restart
+        outs.append(renderer.render(sc.world_to_clip, 80, 64, frame_count=2, max_bounces=8, sample_index=4,
+                                    light_sampling=light))
+    renderer.set_tuning(megakernel=0)
+    assert outs[1]["rounds"] == 0 and outs[0]["rounds"] > 0
+    assert np.array_equal(outs[0]["counts"], outs[1]["counts"]), (outs[0]["counts"], outs[1]["counts"])
+    assert outs[0]["shadow_rays"] == outs[1]["shadow_rays"]
+    assert np.array_equal(outs[0]["color"], outs[1]["color"])
