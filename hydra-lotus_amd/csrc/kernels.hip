@@ -1271,12 +1271,16 @@ __global__ void __launch_bounds__(kTraceBlock) k_paths(SceneView S, FrameConst F
   flush_stats(stats, 3, visits, tests);
 }
 
-// The path megakernel of a batch of views fed by one device work queue: each
-// wave takes the next chunk of kPathChunk paths from a global counter (one
-// atomic per chunk), finds the view that holds it in the view table (views
-// in order, each with its first chunk) and runs its paths with that view's
-// scene, frame and buffers. The grid drains when the counter passes the
-// last chunk.
+// The path megakernel of a batch of views fed by one device work queue, cut
+// for the device's eight XCDs (each with its own L2): the batch's chunks of
+// kPathChunk paths (views in table order) split into eight contiguous
+// ranges, one counter each (next_chunk[0..7]). Workgroups are dispatched to
+// the XCDs in turn (XCD = workgroup index mod 8), so a wave starts on its
+// own XCD's range, which holds a few neighbouring views, and so their trees
+// stay in that L2; once that range is drained it helps the next ranges in
+// turn. Each grab is one atomic; the wave finds the chunk's view in the view
+// table (each with its first chunk) and runs its paths with that view's
+// scene, frame and buffers. The grid drains when every range has.
 template <bool kAnalytic, bool kCov>
 __global__ void __launch_bounds__(kTraceBlock) k_paths_views(const PathView* __restrict__ views, uint32_t view_count,
                                                              uint32_t* next_chunk, uint32_t total_chunks,
@@ -1285,11 +1289,19 @@ __global__ void __launch_bounds__(kTraceBlock) k_paths_views(const PathView* __r
   TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t lane = __lane_id();
-  while (true) {
-    uint32_t c = 0;
-    if (lane == 0) c = atomicAdd(next_chunk, 1u);
-    c = __shfl(c, 0);
-    if (c >= total_chunks) break;
+  uint32_t part = blockIdx.x % kXcds, drained = 0;
+  while (drained < kXcds) {
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(next_chunk + part, 1u);
+    k = __shfl(k, 0);
+    const uint32_t first = static_cast<uint32_t>((uint64_t{total_chunks} * part) / kXcds);
+    const uint32_t end = static_cast<uint32_t>((uint64_t{total_chunks} * (part + 1)) / kXcds);
+    if (first + k >= end) {
+      part = part + 1 == kXcds ? 0 : part + 1;
+      ++drained;
+      continue;
+    }
+    const uint32_t c = first + k;
     // the last view whose first chunk is at most c (binary search)
     uint32_t lo = 0, hi = view_count;
     while (hi - lo > 1) {
@@ -1455,7 +1467,10 @@ GridShape grid_shape(uint32_t cus) {
   g.tail = persistent_blocks(k_tail<false, false, false>, kTraceBlock, cus);
   g.tail_cache = persistent_blocks(k_tail<true, false, false>, kTraceBlock, cus);
   g.camera_pass = persistent_blocks(k_camera_pass, kTraceBlock, cus);
-  g.paths = persistent_blocks(k_paths_views<true, true>, kTraceBlock, cus);
+  g.paths_variant[0] = persistent_blocks(k_paths_views<false, false>, kTraceBlock, cus);
+  g.paths_variant[1] = persistent_blocks(k_paths_views<true, false>, kTraceBlock, cus);
+  g.paths_variant[2] = persistent_blocks(k_paths_views<true, true>, kTraceBlock, cus);
+  g.paths = std::max({g.paths_variant[0], g.paths_variant[1], g.paths_variant[2]});
   g.rays = persistent_blocks(k_rays, kTraceBlock, cus);
   g.shade = persistent_blocks(k_shade<false, false>, kShadeBlock, cus);
   g.shade_parallel = persistent_blocks(k_shade<true, false, true>, kShadeBlock, cus);
@@ -1588,24 +1603,30 @@ bool paths_eligible(const SceneView& S, const FrameConst& F) {
 }
 void launch_paths(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
   const bool analytic = scene_has_extras(S), cov = S.covered_instances != 0;
+  const uint32_t* g = L.grid.paths_variant;
   if (cov)
-    k_paths<true, true><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+    k_paths<true, true><<<g[2], kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
   else if (analytic)
-    k_paths<true, false><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+    k_paths<true, false><<<g[1], kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
   else
-    k_paths<false, false><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+    k_paths<false, false><<<g[0], kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
 }
 void launch_paths_views(const Launch& L, const PathView* views, uint32_t view_count, bool analytic, bool cov,
                         uint32_t* next_chunk, uint32_t total_chunks) {
+  // every variant with the covered variant's grid, the smallest: more
+  // resident waves hold chunks of more scenes at once and thrash the L2s
+  // (measured: 3 workgroups per CU instead of 2 made the 64-view batch 22 %
+  // slower)
+  const uint32_t g = L.grid.paths_variant[2];
   if (cov)
-    k_paths_views<true, true><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk,
-                                                                          total_chunks, L.spill, L.spill_cap);
+    k_paths_views<true, true><<<g, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk, total_chunks,
+                                                               L.spill, L.spill_cap);
   else if (analytic)
-    k_paths_views<true, false><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk,
-                                                                           total_chunks, L.spill, L.spill_cap);
+    k_paths_views<true, false><<<g, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk, total_chunks,
+                                                                L.spill, L.spill_cap);
   else
-    k_paths_views<false, false><<<L.grid.paths, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk,
-                                                                            total_chunks, L.spill, L.spill_cap);
+    k_paths_views<false, false><<<g, kTraceBlock, 0, L.stream>>>(views, view_count, next_chunk, total_chunks,
+                                                                 L.spill, L.spill_cap);
 }
 uint32_t path_count(const FrameConst& F) { return window_paths(F.rect[2], F.rect[3]); }
 uint32_t path_chunks(const FrameConst& F) { return (path_count(F) + kPathChunk - 1) / kPathChunk; }

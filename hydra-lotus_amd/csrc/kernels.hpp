@@ -65,6 +65,7 @@ constexpr float kCacheMinThroughput = 1e-3f;
 struct GridShape {
   uint32_t camera, camera_dynamic, trace, trace_dynamic, tail, tail_cache, camera_pass, rays, shade, shade_parallel,
       resolve, paths;
+  uint32_t paths_variant[3];  // the path megakernel's grids: plain, extras, covered surfaces
   uint32_t max_trace_lanes;  // the most traversal lanes any grid holds
 };
 GridShape grid_shape(uint32_t cus);
@@ -120,6 +121,7 @@ void launch_resolve(const Launch& L, const FrameConst& F, const Queues& q, uint3
 // One view per launch, or a batch of views fed by one device work queue of
 // kPathChunk-path chunks (a PathView table on the device, views in order).
 constexpr uint32_t kPathChunk = 256;
+constexpr uint32_t kXcds = 8;  // a fused launch's work-queue ranges: one per XCD (8 counters per launch)
 struct PathView {
   SceneView S;
   FrameConst F;

@@ -341,7 +341,7 @@ struct lotus_hip_renderer {
   uint32_t fused_add = 0;
   PathView* fused_table = nullptr;
   uint32_t fused_table_count = 0;
-  uint32_t* fused_queue = nullptr;  // 3 words
+  uint32_t* fused_queue = nullptr;  // kXcds words per kernel variant
   uint2* fused_spill = nullptr;
   size_t fused_spill_entries = 0;
   uint64_t batch_instantiations = 0, batch_updates = 0, batch_replays = 0;
@@ -451,7 +451,8 @@ Launch launch_for(lotus_hip_renderer* r, hipStream_t stream) {
   GridShape g = r->grid;
   const uint32_t pct = r->tuning.resident_percent;
   if (pct != 0 && pct < 100) {
-    for (uint32_t* n : {&g.camera, &g.camera_dynamic, &g.trace, &g.trace_dynamic, &g.tail, &g.tail_cache})
+    for (uint32_t* n : {&g.camera, &g.camera_dynamic, &g.trace, &g.trace_dynamic, &g.tail, &g.tail_cache,
+                        &g.paths_variant[0], &g.paths_variant[1], &g.paths_variant[2]})
       *n = std::max(1u, static_cast<uint32_t>((uint64_t{*n} * pct) / 100));
   }
   return Launch{stream, g, r->spill, static_cast<int>(r->spill_entries), r->stats,
@@ -3478,7 +3479,7 @@ hipError_t fused_paths(lotus_hip_renderer* root, lotus_hip_renderer* const* rend
       e = dev_alloc(root->fused_table, table.size());
       if (e == hipSuccess) root->fused_table_count = static_cast<uint32_t>(table.size());
     }
-    if (e == hipSuccess && !root->fused_queue) e = dev_alloc(root->fused_queue, 3);
+    if (e == hipSuccess && !root->fused_queue) e = dev_alloc(root->fused_queue, 3 * kXcds);
     if (e == hipSuccess && (!root->fused_spill || root->fused_spill_entries < need)) {
       dev_free(root->fused_spill);
       root->fused_spill_entries = 0;
@@ -3488,7 +3489,7 @@ hipError_t fused_paths(lotus_hip_renderer* root, lotus_hip_renderer* const* rend
     if (e == hipSuccess)
       e = hipMemcpyAsync(root->fused_table, table.data(), table.size() * sizeof(PathView), hipMemcpyHostToDevice,
                          root->stream);
-    if (e == hipSuccess) launch_zero(root->fused_queue, 3, root->stream);
+    if (e == hipSuccess) launch_zero(root->fused_queue, 3 * kXcds, root->stream);
     Launch L = launch_for(root, root->stream);
     L.spill = root->fused_spill;
     L.spill_cap = static_cast<int>(root->fused_spill_entries);
@@ -3498,7 +3499,7 @@ hipError_t fused_paths(lotus_hip_renderer* root, lotus_hip_renderer* const* rend
       const uint32_t chunks = groups[g].back().first_chunk + path_chunks(groups[g].back().F);
       if (chunks != 0)
         launch_paths_views(L, root->fused_table + offset, static_cast<uint32_t>(groups[g].size()), g >= 1, g == 2,
-                           root->fused_queue + g, chunks);
+                           root->fused_queue + kXcds * g, chunks);
       offset += groups[g].size();
     }
     if (e == hipSuccess) e = hipGetLastError();
