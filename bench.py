@@ -267,10 +267,21 @@ def scene_update_run(r, scene, width, height, depth, steps, frames=4):
     tree built for the last step's heights (the refit's quality cost)."""
     kw = dict(frame_count=1, max_bounces=depth, light_sampling=hip.LIGHT_MIS, want_color=False)
     refit_ms, upload_ms, uploaded = [], [], []
+    apply_wall = {"compared": [], "vouched": []}
     moved = scene
+    every = hip.UNCHANGED_TOPOLOGY | hip.UNCHANGED_POSITIONS | hip.UNCHANGED_TRANSFORMS | \
+        hip.UNCHANGED_MATERIALS | hip.UNCHANGED_LIGHTS
     for k in range(steps):
         moved = scenes.heightfield_wave(scene, 0.35 * (k + 1))
-        before = r.apply(moved, builder=BUILDER)
+        moved.packed()  # the host's plan, built before the call is timed
+        # odd steps vouch for everything but the moved positions, as a host
+        # that knows its SceneUpdate's flags does; even steps let the
+        # library compare the whole plan
+        vouch = k % 2 == 1
+        t0 = time.perf_counter()
+        before = r.apply(moved, builder=BUILDER, unchanged=every if vouch else 0,
+                         moved_geometries=np.arange(len(moved.geometries)) if vouch else None)
+        apply_wall["vouched" if vouch else "compared"].append((time.perf_counter() - t0) * 1e3)
         if before.update != hip.SCENE_REFIT:
             raise RuntimeError(f"animated step {k} was applied as update {before.update}, not as a refit")
         refit_ms.append(before.build_gpu_ms)
@@ -298,6 +309,7 @@ def scene_update_run(r, scene, width, height, depth, steps, frames=4):
     return {"workload": f"heightfield {ev.triangle_count} tris, {steps} animated steps "
                         f"(heights + 0.05 sin(4x + 3z + t)), {width}x{height}, depth {depth}",
             "build_ms": round(ev.build_gpu_ms, 3),
+            "apply_wall_ms": {k: round(float(np.mean(v)), 3) for k, v in apply_wall.items() if v},
             "build_gpu_ms_lbvh": round(gpu_build, 3),
             "refit_gpu_ms": round(float(np.mean(refit_ms)), 3),
             "refit_upload_wall_ms": round(float(np.mean(upload_ms)), 2),

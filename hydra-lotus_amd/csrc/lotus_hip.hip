@@ -1033,21 +1033,41 @@ void corner_extents(const lotus_hip_scene_plan* p, const std::vector<uint32_t>& 
   }
 }
 
-std::string validate_plan(const lotus_hip_scene_plan* p) {
+bool material_emits(const uint32_t* m, const uint32_t* bindings);
+
+// `trusted`: the LOTUS_HIP_UNCHANGED_* bits honoured (the plan's, when a
+// scene is resident); vouched parts are not validated again.
+std::string validate_plan(const lotus_hip_scene_plan* p, uint32_t trusted = 0) {
   if (!p) return "no scene plan";
   if (p->material_count == 0 || !p->materials) return "the material table is empty";
   if (p->bvh_leaf_size > 8) return "BVH leaves hold at most 8 triangles";
   if (p->bvh_builder > LOTUS_HIP_BVH_SAH) return "unknown BVH builder";
+  if ((trusted & LOTUS_HIP_UNCHANGED_POSITIONS) && p->moved_geometry_count != 0 && !p->moved_geometries)
+    return "the plan names moved geometries but holds none";
+  for (uint32_t k = 0; (trusted & LOTUS_HIP_UNCHANGED_POSITIONS) && k < p->moved_geometry_count; ++k)
+    if (p->moved_geometries[k] >= p->geometry_count) return "a moved geometry is not in the plan";
+  const auto check_positions = [&](uint32_t g) -> std::string {
+    const float* pos = p->positions + 3ull * p->geometry_vertex_offset[g];
+    for (uint64_t k = 0; k < 3ull * p->geometry_vertex_count[g]; ++k)
+      if (!std::isfinite(pos[k])) return "geometry " + std::to_string(g) + " has a non-finite position";
+    return "";
+  };
   uint64_t tris = 0;
   for (uint32_t g = 0; g < p->geometry_count; ++g) {
     const uint32_t vc = p->geometry_vertex_count[g];
     const uint32_t* tri = p->triangles + 3ull * p->geometry_triangle_offset[g];
-    for (uint64_t k = 0; k < 3ull * p->geometry_triangle_count[g]; ++k)
+    for (uint64_t k = 0; !(trusted & LOTUS_HIP_UNCHANGED_TOPOLOGY) && k < 3ull * p->geometry_triangle_count[g]; ++k)
       if (tri[k] >= vc) return "geometry " + std::to_string(g) + " indexes a missing vertex";
-    const float* pos = p->positions + 3ull * p->geometry_vertex_offset[g];
-    for (uint64_t k = 0; k < 3ull * vc; ++k)
-      if (!std::isfinite(pos[k])) return "geometry " + std::to_string(g) + " has a non-finite position";
+    if (!(trusted & LOTUS_HIP_UNCHANGED_POSITIONS)) {
+      const std::string bad = check_positions(g);
+      if (!bad.empty()) return bad;
+    }
   }
+  for (uint32_t k = 0; (trusted & LOTUS_HIP_UNCHANGED_POSITIONS) && k < p->moved_geometry_count; ++k) {
+    const std::string bad = check_positions(p->moved_geometries[k]);
+    if (!bad.empty()) return bad;
+  }
+
   for (uint32_t i = 0; i < p->instance_count; ++i) {
     if (p->instance_geometry[i] >= p->geometry_count) return "an instance names a missing geometry";
     if (p->instance_material[i] >= p->material_count) return "an instance names a missing material slot";
@@ -1154,6 +1174,18 @@ std::string validate_plan(const lotus_hip_scene_plan* p) {
     if (!(opacity >= 0.0f && opacity <= 1.0f) || !(threshold >= 0.0f && threshold <= 1.0f))
       return "material " + std::to_string(m) + ": opacity and its threshold must lie in [0, 1]";
   }
+  // with the topology vouched for, the light table still reads the emitting
+  // instances' triangles from the plan: their indices are checked
+  if (trusted & LOTUS_HIP_UNCHANGED_TOPOLOGY) {
+    for (uint32_t i = 0; i < p->instance_count; ++i) {
+      const uint32_t g = p->instance_geometry[i];
+      if (!material_emits(p->materials + 36ull * p->instance_material[i], p->bindings)) continue;
+      const uint32_t vc = p->geometry_vertex_count[g];
+      const uint32_t* tri = p->triangles + 3ull * p->geometry_triangle_offset[g];
+      for (uint64_t k = 0; k < 3ull * p->geometry_triangle_count[g]; ++k)
+        if (tri[k] >= vc) return "geometry " + std::to_string(g) + " indexes a missing vertex";
+    }
+  }
   return {};
 }
 
@@ -1247,37 +1279,57 @@ struct Changes {
 Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p, const PlanShape& sh) {
   Changes c;
   const auto& old = r->plan;
+  const uint32_t u = r->has_scene ? p->unchanged : 0u;  // vouched-for parts
   // A retained geometry keeps its vertex count and exactly its triangle
   // indices (gpu_scene.cpp:1651-1656); every instance keeps its geometry.
+  const bool shapes = r->has_scene && old.leaf_size == sh.leaf_size && old.builder == sh.builder &&
+                      old.gvo.size() == sh.geometries && old.igeo.size() == sh.instances &&
+                      old.positions.size() == sh.vertex_words && old.triangles.size() == sh.index_words;
   const bool topology =
-      r->has_scene && old.leaf_size == sh.leaf_size && old.builder == sh.builder && same_words(old.gvo, p->geometry_vertex_offset, sh.geometries) &&
-      same_words(old.gvc, p->geometry_vertex_count, sh.geometries) &&
-      same_words(old.gto, p->geometry_triangle_offset, sh.geometries) &&
-      same_words(old.gtc, p->geometry_triangle_count, sh.geometries) &&
-      same_words(old.igeo, p->instance_geometry, sh.instances) && old.positions.size() == sh.vertex_words &&
-      same_words(old.triangles, p->triangles, sh.index_words);
+      shapes && ((u & LOTUS_HIP_UNCHANGED_TOPOLOGY) ||
+                 (same_words(old.gvo, p->geometry_vertex_offset, sh.geometries) &&
+                  same_words(old.gvc, p->geometry_vertex_count, sh.geometries) &&
+                  same_words(old.gto, p->geometry_triangle_offset, sh.geometries) &&
+                  same_words(old.gtc, p->geometry_triangle_count, sh.geometries) &&
+                  same_words(old.igeo, p->instance_geometry, sh.instances) &&
+                  same_words(old.triangles, p->triangles, sh.index_words)));
   if (!topology) return c;
   // texture lookups are uploaded with a new scene (a change of them builds)
-  const TextureData tex = texture_data(p);
-  if (tex.shape != old.texture_shape || tex.bindings != old.bindings ||
-      tex.instance_texcoords != old.instance_texcoords || tex.instance_normals != old.instance_normals ||
-      tex.instance_colors != old.instance_colors || tex.texel_bytes != old.texel_bytes ||
-      tex.texcoords != old.texcoords || tex.normals != old.normals || tex.colors != old.colors)
-    return c;
-  if (!same_words(old.positions, p->positions, sh.vertex_words)) {
+  if (!(u & LOTUS_HIP_UNCHANGED_TOPOLOGY)) {
+    const TextureData tex = texture_data(p);
+    if (tex.shape != old.texture_shape || tex.bindings != old.bindings ||
+        tex.instance_texcoords != old.instance_texcoords || tex.instance_normals != old.instance_normals ||
+        tex.instance_colors != old.instance_colors || tex.texel_bytes != old.texel_bytes ||
+        tex.texcoords != old.texcoords || tex.normals != old.normals || tex.colors != old.colors)
+      return c;
+  }
+  if (u & LOTUS_HIP_UNCHANGED_POSITIONS) {
+    std::vector<char> seen(sh.geometries, 0);
+    for (uint32_t k = 0; k < p->moved_geometry_count; ++k) {
+      const uint32_t g = p->moved_geometries[k];
+      if (seen[g]) continue;
+      seen[g] = 1;
+      const uint64_t first = 3ull * p->geometry_vertex_offset[g], words = 3ull * p->geometry_vertex_count[g];
+      if (words && std::memcmp(old.positions.data() + first, p->positions + first, words * 4) != 0) c.moved.push_back(g);
+    }
+    std::sort(c.moved.begin(), c.moved.end());
+  } else if (!same_words(old.positions, p->positions, sh.vertex_words)) {
     for (uint32_t g = 0; g < sh.geometries; ++g) {
       const uint64_t first = 3ull * p->geometry_vertex_offset[g], words = 3ull * p->geometry_vertex_count[g];
       if (words && std::memcmp(old.positions.data() + first, p->positions + first, words * 4) != 0) c.moved.push_back(g);
     }
   }
-  c.transforms = !same_words(old.transforms, p->instance_transform, 16ull * sh.instances);
-  c.materials = !same_words(old.materials, p->materials, 36ull * sh.materials);
-  c.instance_materials = !same_words(old.imat, p->instance_material, sh.instances);
-  c.environment = std::memcmp(old.environment, p->environment, sizeof old.environment) != 0;
-  c.lights = !same_words(old.lights, p->lights, p->light_count);
+  c.transforms = !(u & LOTUS_HIP_UNCHANGED_TRANSFORMS) &&
+                 !same_words(old.transforms, p->instance_transform, 16ull * sh.instances);
+  c.materials = !(u & LOTUS_HIP_UNCHANGED_MATERIALS) && !same_words(old.materials, p->materials, 36ull * sh.materials);
+  c.instance_materials =
+      !(u & LOTUS_HIP_UNCHANGED_MATERIALS) && !same_words(old.imat, p->instance_material, sh.instances);
+  const bool lights_vouched = (u & LOTUS_HIP_UNCHANGED_LIGHTS) != 0;
+  c.environment = !lights_vouched && std::memcmp(old.environment, p->environment, sizeof old.environment) != 0;
+  c.lights = !lights_vouched && !same_words(old.lights, p->lights, p->light_count);
   const lotus_hip_environment_map* e = p->environment_map;
   const bool env_same =
-      old.has_env == (e != nullptr) &&
+      lights_vouched || old.has_env == (e != nullptr) &&
       (!e || (old.env_width == e->width && old.env_height == e->height &&
               std::memcmp(old.env_scale, e->scale, sizeof old.env_scale) == 0 &&
               std::memcmp(old.env_frame, e->world_from_environment, sizeof old.env_frame) == 0 &&
@@ -1866,7 +1918,7 @@ int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, 
     decoded_plan.environment_map = &decoded_map;
     p = &decoded_plan;
   }
-  const std::string invalid = validate_plan(p);
+  const std::string invalid = validate_plan(p, r->has_scene ? p->unchanged : 0u);
   if (!invalid.empty()) {
     set_detail(detail, detail_size, invalid);
     return LOTUS_HIP_FAIL;
@@ -1874,6 +1926,14 @@ int lotus_hip_scene_apply(lotus_hip_renderer* r, const lotus_hip_scene_plan* p, 
   (void)hipSetDevice(r->device);
   const PlanShape shape = plan_shape(p);
   const Changes c = compare_plan(r, p, shape);
+  if (c.update == LOTUS_HIP_SCENE_BUILD && r->has_scene && p->unchanged != 0) {
+    // a plan that rebuilds is read whole: it is validated whole
+    const std::string whole = validate_plan(p);
+    if (!whole.empty()) {
+      set_detail(detail, detail_size, whole);
+      return LOTUS_HIP_FAIL;
+    }
+  }
   if (c.update == LOTUS_HIP_SCENE_UNCHANGED) {
     fill_evidence(r, ev, c.update, 0, 0.0, 0.0);
     set_detail(detail, detail_size, "");

@@ -12,6 +12,7 @@
 #include <optional>
 #include <stdexcept>
 #include <unordered_map>
+#include <unordered_set>
 #include <utility>
 #include <variant>
 #include <vector>
@@ -320,16 +321,43 @@ class HipOffscreenRenderer final : public OffscreenRenderer {
   }
 
   // vulkan_backend.hpp:816-817, applied as GpuScene::Apply applies a plan
-  // (gpu_scene.cpp:1606-2450): releases, then uploads, then the
-  // replacements the plan marks.
+  // (gpu_scene.cpp:1606-2450): a replacement takes its previous geometry's
+  // slot (gpu_scene.cpp:1651-1671), then the other releases, then the other
+  // uploads. The plan vouches for what the update leaves unchanged
+  // (lotus_hip_scene_plan.unchanged), so the library compares only the rest.
   GpuSceneEvidence UpdateScene(const SceneUpdate& update) override {
     GpuSceneEvidence ev;
+    std::unordered_set<const MeshGeometry*> replaced_sources, replaced_targets;
+    std::vector<std::uint32_t> replaced_slots;
+    bool same_topology = !update.instances_changed && !update.materials_changed && update.texture_uploads.empty() &&
+                         update.texture_releases.empty() && has_plan_;
+    for (const auto& rep : update.geometry_replacements) {
+      const auto held = std::find_if(geometries_.begin(), geometries_.end(),
+                                     [&](const auto& g) { return g.get() == rep.previous; });
+      const auto next = std::find_if(update.geometry_uploads.begin(), update.geometry_uploads.end(),
+                                     [&](const auto& g) { return g.get() == rep.replacement; });
+      if (held == geometries_.end() || next == update.geometry_uploads.end()) continue;
+      const MeshGeometry& a = **held;
+      const MeshGeometry& b = **next;
+      same_topology = same_topology && a.positions.size() == b.positions.size() && a.triangles == b.triangles &&
+                      a.normals == b.normals && a.texcoords == b.texcoords && a.colors == b.colors;
+      replaced_slots.push_back(static_cast<std::uint32_t>(held - geometries_.begin()));
+      *held = *next;
+      replaced_sources.insert(rep.previous);
+      replaced_targets.insert(rep.replacement);
+    }
     for (const MeshGeometry* g : update.geometry_releases) {
+      if (replaced_sources.count(g)) continue;
+      same_topology = false;
       geometries_.erase(std::remove_if(geometries_.begin(), geometries_.end(),
                                        [g](const auto& held) { return held.get() == g; }),
                         geometries_.end());
     }
-    for (const auto& g : update.geometry_uploads) geometries_.push_back(g);
+    for (const auto& g : update.geometry_uploads) {
+      if (replaced_targets.count(g.get())) continue;
+      same_topology = false;
+      geometries_.push_back(g);
+    }
     for (const Texture* t : update.texture_releases) {
       textures_.erase(std::remove_if(textures_.begin(), textures_.end(),
                                      [t](const auto& held) { return held.get() == t; }),
@@ -352,9 +380,20 @@ class HipOffscreenRenderer final : public OffscreenRenderer {
       ev.detail = error;
       return ev;
     }
+    if (same_topology) {
+      plan.plan.unchanged |= LOTUS_HIP_UNCHANGED_TOPOLOGY | LOTUS_HIP_UNCHANGED_POSITIONS;
+      plan.plan.moved_geometry_count = static_cast<std::uint32_t>(replaced_slots.size());
+      plan.plan.moved_geometries = replaced_slots.empty() ? nullptr : replaced_slots.data();
+    }
+    if (has_plan_ && !update.instances_changed) plan.plan.unchanged |= LOTUS_HIP_UNCHANGED_TRANSFORMS;
+    if (has_plan_ && !update.instances_changed && !update.materials_changed)
+      plan.plan.unchanged |= LOTUS_HIP_UNCHANGED_MATERIALS;
+    if (has_plan_ && !update.lights_changed && !update.environment_changed)
+      plan.plan.unchanged |= LOTUS_HIP_UNCHANGED_LIGHTS;
     lotus_hip_scene_evidence se{};
     char detail[1024] = {};
     const int status = lotus_hip_scene_apply(r_, &plan.plan, &se, detail, sizeof detail);
+    has_plan_ = status == LOTUS_HIP_PASS;
     ev.status = StatusOf(status);
     ev.detail = detail;
     instance_count_ = plan.instance_count;
@@ -769,6 +808,7 @@ class HipOffscreenRenderer final : public OffscreenRenderer {
   std::vector<SceneInstance> instances_;
   std::vector<SceneMaterial> materials_;
   std::array<float, 3> environment_{};
+  bool has_plan_ = false;  // the last plan applied: the next may vouch for parts of it
   std::optional<SceneEnvironmentMap> environment_map_;
   std::vector<Light> lights_;
   std::shared_ptr<const LotusScene> uploaded_scene_;

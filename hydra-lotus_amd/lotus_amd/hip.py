@@ -98,7 +98,14 @@ class ScenePlan(C.Structure):
         ("instance_color_offset", C.c_void_p),
         ("instance_color_constant", C.c_void_p),
         ("bvh_builder", C.c_uint32),
+        ("unchanged", C.c_uint32),
+        ("moved_geometry_count", C.c_uint32),
+        ("moved_geometries", C.c_void_p),
     ]
+
+
+# lotus_hip_scene_plan.unchanged bits: what a plan vouches has not changed
+UNCHANGED_TOPOLOGY, UNCHANGED_POSITIONS, UNCHANGED_TRANSFORMS, UNCHANGED_MATERIALS, UNCHANGED_LIGHTS = 1, 2, 4, 8, 16
 
 
 class Texture(C.Structure):
@@ -561,8 +568,12 @@ class Renderer:
     def __del__(self):
         self.close()
 
-    def apply(self, scene, leaf_size: int = 0, builder: int = 0) -> SceneEvidence:
-        """UpdateScene with a whole plan; `builder` is BVH_LBVH or BVH_SAH."""
+    def apply(self, scene, leaf_size: int = 0, builder: int = 0, unchanged: int = 0,
+              moved_geometries=None) -> SceneEvidence:
+        """UpdateScene with a whole plan; `builder` is BVH_LBVH or BVH_SAH;
+        `unchanged` (UNCHANGED_* bits) vouches for parts of it, with
+        UNCHANGED_POSITIONS `moved_geometries` the geometries whose
+        positions may have changed."""
         p = scene.packed()
         plan = ScenePlan()
         plan.geometry_count = p["geometry_count"]
@@ -582,6 +593,10 @@ class Renderer:
             plan.environment[i] = float(p["environment"][i])
         plan.bvh_leaf_size = leaf_size
         plan.bvh_builder = builder
+        plan.unchanged = unchanged
+        moved = np.ascontiguousarray(moved_geometries if moved_geometries is not None else [], np.uint32)
+        plan.moved_geometry_count = len(moved)
+        plan.moved_geometries = moved.ctypes.data if len(moved) else None
         plan.light_count = len(p["lights"])
         plan.lights = p["lights"].ctypes.data if len(p["lights"]) else None
         env = environment_struct(p.get("environment_map"))

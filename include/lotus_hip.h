@@ -155,7 +155,27 @@ typedef struct lotus_hip_scene_plan {
    * Boxes, the 4-wide collapse and refits run on the GPU either way. A
    * change of builder rebuilds. */
   uint32_t bvh_builder;
+  /* What the caller vouches has not changed since the last plan it applied
+   * (SceneUpdate's instances_changed, materials_changed, lights_changed,
+   * environment_changed and its geometry and texture lists,
+   * include/lotus/extraction.hpp:91-132): LOTUS_HIP_UNCHANGED_* bits. The
+   * library then neither validates nor compares those parts again and keeps
+   * what it holds for them; counts are always compared, and a renderer
+   * without a scene ignores the bits. 0 checks everything. A plan whose
+   * vouched parts did change renders the resident ones. */
+  uint32_t unchanged;
+  /* With LOTUS_HIP_UNCHANGED_POSITIONS: the geometries whose positions may
+   * have changed (the others' are vouched for). */
+  uint32_t moved_geometry_count;
+  const uint32_t* moved_geometries;
 } lotus_hip_scene_plan;
+
+#define LOTUS_HIP_UNCHANGED_TOPOLOGY 1u   /* geometry offsets and counts, triangle indices, the instances'
+                                             geometries, texture lookups (textures, bindings, corner sets) */
+#define LOTUS_HIP_UNCHANGED_POSITIONS 2u  /* positions, but moved_geometries' */
+#define LOTUS_HIP_UNCHANGED_TRANSFORMS 4u /* instance transforms */
+#define LOTUS_HIP_UNCHANGED_MATERIALS 8u  /* the material table and the instances' slots */
+#define LOTUS_HIP_UNCHANGED_LIGHTS 16u    /* analytic lights, the constant environment and the image */
 
 #define LOTUS_HIP_BVH_LBVH 0u
 #define LOTUS_HIP_BVH_SAH 1u

@@ -273,3 +273,41 @@ def test_temporal_history_across_scene_updates(renderer):
     assert renderer.apply(mat).update == hip.SCENE_TABLES
     out = renderer.render_temporal(sc.world_to_clip, sc.view, w, h, frame_index=f + 4, max_bounces=3, accumulate=True)
     assert out["reset_reason"] == 8
+
+
+def test_vouched_plans_compare_only_what_changed(renderer):
+    # lotus_hip_scene_plan.unchanged: a host that knows its SceneUpdate's
+    # flags vouches for the rest, and the library validates and compares only
+    # the moved geometries' positions (counts are always compared)
+    every = hip.UNCHANGED_TOPOLOGY | hip.UNCHANGED_POSITIONS | hip.UNCHANGED_TRANSFORMS | \
+        hip.UNCHANGED_MATERIALS | hip.UNCHANGED_LIGHTS
+    a = scenes.heightfield(cells=64, emissive=16, tiles=4)
+    assert renderer.apply(a).update == hip.SCENE_BUILD
+    b = scenes.heightfield_wave(a, 0.5)
+    moved = [g for g in range(len(b.geometries))
+             if not np.array_equal(b.geometries[g][0], a.geometries[g][0])]
+    assert moved
+    ev = renderer.apply(b, unchanged=every, moved_geometries=moved)
+    assert ev.update == hip.SCENE_REFIT and ev.moved_geometries == len(moved)
+    vouched = _frame(renderer, b)
+    fresh = hip.Renderer(0)
+    try:
+        fresh.apply(b)
+        built = _frame(fresh, b)
+    finally:
+        fresh.close()
+    assert np.array_equal(vouched["counts"], built["counts"]) and np.array_equal(vouched["color"], built["color"])
+    # vouched-for parts are not read again: positions of a geometry left out
+    # of the moved list keep the resident ones
+    c = scenes.heightfield_wave(a, 0.9)
+    assert renderer.apply(c, unchanged=every, moved_geometries=[]).update == hip.SCENE_UNCHANGED
+    got = renderer.readback_scene()
+    assert np.array_equal(got["positions"], b.packed()["positions"].reshape(-1))
+    # a moved geometry outside the plan is refused, without latching
+    with pytest.raises(hip.LotusError, match="moved geometry is not in the plan"):
+        renderer.apply(c, unchanged=every, moved_geometries=[len(c.geometries)])
+    # other counts: the vouched topology cannot hold, and the plan is
+    # validated and built whole
+    d = scenes.heightfield(cells=48, emissive=8, tiles=4)
+    assert renderer.apply(d, unchanged=every, moved_geometries=[0]).update == hip.SCENE_BUILD
+    assert _frame(renderer, d)["spp"] == 1
