@@ -1309,6 +1309,12 @@ Changes compare_plan(const lotus_hip_renderer* r, const lotus_hip_scene_plan* p,
       const uint32_t g = p->moved_geometries[k];
       if (seen[g]) continue;
       seen[g] = 1;
+      // a moved geometry's positions are uploaded at its place: that place
+      // must be the resident one, or the plan builds (validated whole)
+      if (p->geometry_vertex_offset[g] != old.gvo[g] || p->geometry_vertex_count[g] != old.gvc[g]) {
+        c.moved.clear();
+        return c;
+      }
       const uint64_t first = 3ull * p->geometry_vertex_offset[g], words = 3ull * p->geometry_vertex_count[g];
       if (words && std::memcmp(old.positions.data() + first, p->positions + first, words * 4) != 0) c.moved.push_back(g);
     }
