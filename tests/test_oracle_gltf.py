@@ -159,3 +159,39 @@ def test_tables_reduce_to_mirrors_near_zero_alpha(table):
             r = fresnel_dielectric(mu_d, index)
             assert abs(table[node("dielectric", cc, 0, layer)] - r) < 3e-4, (cc, layer)
             assert abs(table[node("dielectric", cc, 0, 32 + layer)] - 2 * r / (1 + r)) < 3e-4, (cc, layer)
+
+
+@pytest.mark.parametrize("fixture", ["material-gltf-v1", "material-gltf-extensions-v1"])
+def test_stored_receiver_surfaces_keep_the_gltf_terms(fixture):
+    # StoreClosure (restir.slang:80-112) with PackGltfPbrTerms' lane order
+    # (gltf_pbr.slang:164-190) in the renderer's 40-float surface: at
+    # receivers of materials without lookups, the lanes hold the material's
+    # own inputs as MakeGltfPbrTerms takes them (base colour, alpha =
+    # max(roughness^2, 1e-3), metallic, transmission, the coat's alpha, the
+    # sheen's roughness = max(r^2, 0.005)); a lane out of order fails
+    import material_fixtures as mf
+    sc, size = mf.load(fixture)
+    o = oracle.OracleScene(sc)
+    mats = np.ascontiguousarray(sc.packed()["materials"], np.uint32).reshape(-1, 36)
+    fr = oracle.make_frame(sc.world_to_clip, size, size, max_bounces=2, light_sampling=2, restir_candidates=4)
+    f32 = np.float32
+    checked = 0
+    for y in range(0, size, 3):
+        for x in range(0, size, 3):
+            _, s, hit = o.restir_temporal(fr, x, y, 3, sc.view, False, 0xFFFFFFFF, (0, 0), None, None, 20)
+            if not hit or s[39] != 1.0:
+                continue
+            ids = s.view(np.uint32)
+            slot = int(ids[30]) + (int(ids[31]) << 32) - 1
+            w = mats[slot]
+            if w[2] != 0:  # lookups: the lanes hold looked-up inputs
+                continue
+            m = w.view(np.float32)
+            assert np.array_equal(s[15:18], m[12:15]), (x, y)
+            assert s[18] == max(f32(m[15]) * f32(m[15]), f32(1e-3))
+            assert s[19] == m[19]
+            assert s[26] == m[23]
+            assert s[29] == max(f32(m[32]) * f32(m[32]), f32(1e-3))
+            assert s[37] == max(f32(m[31]) * f32(m[31]), f32(0.005))
+            checked += 1
+    assert checked >= 10, checked
