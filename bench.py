@@ -482,6 +482,8 @@ def batch_run(views, frames):
             "autotuned_dynamic_trace": sum(t["trace_dynamic"] for t in tunings),
             "autotuned_half_resident": sum(t["resident_percent"] == 50 for t in tunings),
             "autotuned_megakernel": sum(t["megakernel"] for t in tunings),
+            "autotuned_camera_tiles": {n: sum(t["camera_tile"] == k and not t["megakernel"] for t in tunings)
+                                       for k, n in enumerate(("8x8", "16x4", "4x16"))},
             "graph_frames_replayed": sum(k == 3 for k in graph_kinds), **res}
 
 

@@ -200,21 +200,23 @@ __device__ __forceinline__ void push_hit(const Queues& q, uint32_t k, uint32_t s
 // sample. Clears the slot's pending terms.
 template <bool kDynamic, bool kCov>  // kCov: the scene has covered surfaces
 __global__ void __launch_bounds__(kTraceBlock) k_camera(SceneView S, FrameConst F, Queues q, uint32_t* counters,
-                                                        uint2* spill, int spill_cap, unsigned long long* stats) {
+                                                        uint2* spill, int spill_cap, unsigned long long* stats,
+                                                        uint32_t tile_shift) {
   __shared__ uint2 ring[(kRing + 1) * kTraceBlock];  // + the lanes' traversal counts
   TraceStack st;
   st.init(ring, spill, spill_cap);
   const uint32_t rw = F.rect[2], rh = F.rect[3];
   if (gtid() == 0) *ray_count(counters, 0) = rw * rh;
-  // Items walk the data window in 8 x 8 pixel tiles, a wave's 64 rays one
-  // tile, so a wave's rays stay close together through the tree (and so do
-  // the hits and the next rays they queue). Items past the window's edge in
-  // its last tiles are skipped.
-  const uint32_t tiles_x = (rw + 7) / 8, tiles = tiles_x * ((rh + 7) / 8);
+  // Items walk the data window in pixel tiles of 64, a wave's rays one tile
+  // (2^tile_shift wide: 8 x 8, 16 x 4 or 4 x 16), so a wave's rays stay
+  // close together through the tree (and so do the hits and the next rays
+  // they queue). Items past the window's edge in its last tiles are skipped.
+  const uint32_t tw = 1u << tile_shift, th = 64u >> tile_shift;
+  const uint32_t tiles_x = (rw + tw - 1) / tw, tiles = tiles_x * ((rh + th - 1) / th);
   const auto pixel = [&](uint32_t i, uint32_t& x, uint32_t& y) {
     const uint32_t tile = i >> 6, within = i & 63u;
-    x = (tile % tiles_x) * 8 + (within & 7u);
-    y = (tile / tiles_x) * 8 + (within >> 3);
+    x = (tile % tiles_x) * tw + (within & (tw - 1u));
+    y = (tile / tiles_x) * th + (within >> tile_shift);
     return x < rw && y < rh;
   };
   LaneStats ls;
@@ -1486,15 +1488,17 @@ void launch_fill4(float4* p, uint32_t n, float4 v, hipStream_t s) { k_fill4<<<sm
 void launch_fill1(float* p, uint32_t n, float v, hipStream_t s) { k_fill1<<<small_grid(n), 256, 0, s>>>(p, n, v); }
 void launch_camera(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c) {
   const bool cov = S.covered_instances != 0;
+  const uint32_t t = L.camera_tile_shift;
   if (L.dynamic_camera && cov)
-    k_camera<true, true><<<L.grid.camera_dynamic, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+    k_camera<true, true><<<L.grid.camera_dynamic, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats,
+                                                                             t);
   else if (L.dynamic_camera)
     k_camera<true, false><<<L.grid.camera_dynamic, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap,
-                                                                              L.stats);
+                                                                              L.stats, t);
   else if (cov)
-    k_camera<false, true><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+    k_camera<false, true><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats, t);
   else
-    k_camera<false, false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats);
+    k_camera<false, false><<<L.grid.camera, kTraceBlock, 0, L.stream>>>(S, F, q, c, L.spill, L.spill_cap, L.stats, t);
 }
 void launch_shade(const Launch& L, const SceneView& S, const FrameConst& F, const Queues& q, uint32_t* c,
                   uint32_t bounce, bool link) {

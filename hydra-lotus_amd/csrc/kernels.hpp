@@ -85,6 +85,7 @@ struct Launch {
   unsigned long long* stats;  // node visits, triangle tests: camera, extension, shadow, tail
   bool dynamic_camera;        // lanes hand over finished rays (else one ray per lane per pass)
   bool dynamic_trace;
+  uint32_t camera_tile_shift = 3;  // camera tiles 2^shift pixels wide, 64 pixels each
 };
 
 // k_trace item kinds.

@@ -267,7 +267,7 @@ struct lotus_hip_renderer {
   uint2* spill = nullptr;
   size_t spill_entries = 0;  // per lane
   GridShape grid{};
-  lotus_hip_tuning tuning{0u, 0u, 16u, 0u, 0u};
+  lotus_hip_tuning tuning{0u, 0u, 16u, 0u, 0u, 0u};
 
   AccumulationKey key;
   bool key_valid = false;
@@ -455,8 +455,10 @@ Launch launch_for(lotus_hip_renderer* r, hipStream_t stream) {
                         &g.paths_variant[0], &g.paths_variant[1], &g.paths_variant[2]})
       *n = std::max(1u, static_cast<uint32_t>((uint64_t{*n} * pct) / 100));
   }
+  // camera tiles: 8 x 8 (0), 16 x 4 (1) or 4 x 16 (2)
+  const uint32_t shift = r->tuning.camera_tile == 1 ? 4u : r->tuning.camera_tile == 2 ? 2u : 3u;
   return Launch{stream, g, r->spill, static_cast<int>(r->spill_entries), r->stats,
-                r->tuning.camera_dynamic != 0, r->tuning.trace_dynamic != 0};
+                r->tuning.camera_dynamic != 0, r->tuning.trace_dynamic != 0, shift};
 }
 
 hipError_t ensure_target(lotus_hip_renderer* r, uint32_t width, uint32_t height) {
@@ -2543,6 +2545,10 @@ int lotus_hip_set_tuning(lotus_hip_renderer* r, const lotus_hip_tuning* tuning, 
     set_detail(detail, detail_size, "resident_percent must be in [1, 100] (0 for 100)");
     return LOTUS_HIP_FAIL;
   }
+  if (tuning->camera_tile > 2) {
+    set_detail(detail, detail_size, "camera_tile must be 0 (8 x 8), 1 (16 x 4) or 2 (4 x 16)");
+    return LOTUS_HIP_FAIL;
+  }
   r->tuning = *tuning;
   r->rounds_hint = 0;  // the next sample sizes the rounds under the new rule
   set_detail(detail, detail_size, "");
@@ -3791,8 +3797,8 @@ int lotus_hip_autotune(lotus_hip_renderer* r, const lotus_hip_frame_params* para
   for (uint32_t pct : {100u, 50u}) {
     for (uint32_t div : {16u, 0u})
       for (uint32_t cam : {0u, 1u})
-        for (uint32_t tr : {0u, 1u}) candidates.push_back(lotus_hip_tuning{cam, tr, div, pct, 0u});
-    candidates.push_back(lotus_hip_tuning{0u, 0u, 16u, pct, 1u});  // the path megakernel
+        for (uint32_t tr : {0u, 1u}) candidates.push_back(lotus_hip_tuning{cam, tr, div, pct, 0u, 0u});
+    candidates.push_back(lotus_hip_tuning{0u, 0u, 16u, pct, 1u, 0u});  // the path megakernel
   }
   double best = 1e300;
   lotus_hip_tuning pick = r->tuning;
@@ -3800,19 +3806,34 @@ int lotus_hip_autotune(lotus_hip_renderer* r, const lotus_hip_frame_params* para
   P.frame_count = 1;
   P.max_samples = 0;
   trials = std::max(trials, 1u);
-  for (const auto& c : candidates) {
+  int status = LOTUS_HIP_PASS;
+  const auto trial = [&](const lotus_hip_tuning& c) {
     r->tuning = c;
     double total = 0.0;
-    for (uint32_t k = 0; k <= trials; ++k) {
+    for (uint32_t k = 0; k <= trials && status == LOTUS_HIP_PASS; ++k) {
       lotus_hip_frame_result res{};
       r->key_valid = false;  // trial frames never join an accumulation
-      const int s = lotus_hip_render(r, &P, &res, detail, detail_size);
-      if (s != LOTUS_HIP_PASS) return s;
+      status = lotus_hip_render(r, &P, &res, detail, detail_size);
       if (k > 0) total += res.frame_gpu_ms;  // the first frame sizes the rounds
     }
-    if (total < best) {
+    if (status == LOTUS_HIP_PASS && total < best) {
       best = total;
       pick = c;
+    }
+  };
+  for (const auto& c : candidates) {
+    trial(c);
+    if (status != LOTUS_HIP_PASS) return status;
+  }
+  // then the camera kernel's tile shapes (the block shape of its waves) for
+  // the chosen wavefront tuning
+  if (pick.megakernel == 0) {
+    const lotus_hip_tuning base = pick;
+    for (uint32_t tile : {1u, 2u}) {
+      lotus_hip_tuning c = base;
+      c.camera_tile = tile;
+      trial(c);
+      if (status != LOTUS_HIP_PASS) return status;
     }
   }
   r->tuning = pick;

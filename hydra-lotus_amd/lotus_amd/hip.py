@@ -221,7 +221,7 @@ class SceneContents(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("camera_dynamic", C.c_uint32), ("trace_dynamic", C.c_uint32), ("tail_divisor", C.c_uint32),
-                ("resident_percent", C.c_uint32), ("megakernel", C.c_uint32)]
+                ("resident_percent", C.c_uint32), ("megakernel", C.c_uint32), ("camera_tile", C.c_uint32)]
 
 
 class FrameParams(C.Structure):
@@ -726,7 +726,7 @@ class Renderer:
         lib().lotus_hip_get_tuning(self.handle, C.byref(t))
         return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic,
                 "tail_divisor": t.tail_divisor, "resident_percent": t.resident_percent,
-                "megakernel": t.megakernel}
+                "megakernel": t.megakernel, "camera_tile": t.camera_tile}
 
     def set_tuning(self, **kw):
         t = Tuning()
@@ -963,4 +963,4 @@ class Renderer:
             raise LotusError(s, buf.value.decode())
         return {"camera_dynamic": t.camera_dynamic, "trace_dynamic": t.trace_dynamic,
                 "tail_divisor": t.tail_divisor, "resident_percent": t.resident_percent,
-                "megakernel": t.megakernel}
+                "megakernel": t.megakernel, "camera_tile": t.camera_tile}

@@ -409,6 +409,7 @@ typedef struct lotus_hip_tuning {
                                 instead of wavefront rounds; frames with the radiance cache, reservoirs,
                                 temporal products, glTF PBR or several parallel lights under BSDF sampling
                                 keep the rounds */
+  uint32_t camera_tile;      /* the camera kernel's pixel tile per wave: 0 8 x 8, 1 16 x 4, 2 4 x 16 */
 } lotus_hip_tuning;
 
 int lotus_hip_set_tuning(lotus_hip_renderer* renderer, const lotus_hip_tuning* tuning, char* detail,
@@ -689,8 +690,9 @@ int lotus_hip_render_batch(lotus_hip_renderer* const* renderers, const lotus_hip
 
 /* Autotuner (no reference counterpart): times trials frames of `params`
  * under each combination of lotus_hip_tuning's choices (the traversal
- * drivers, the tail rule and 100 or 50 percent resident workgroups) and
- * keeps the fastest; writes it to `chosen`. Trial frames do not join an
+ * drivers, the tail rule, the path megakernel, 100 or 50 percent resident
+ * workgroups), then the camera tile shapes for the chosen one, and keeps the
+ * fastest; writes it to `chosen`. Trial frames do not join an
  * accumulation, and the next frame restarts it. */
 int lotus_hip_autotune(lotus_hip_renderer* renderer, const lotus_hip_frame_params* params, uint32_t trials,
                        lotus_hip_tuning* chosen, char* detail, size_t detail_size);

@@ -90,7 +90,8 @@ def test_autotune_keeps_results():
     a, b = _renderers(vs + vs)
     try:
         chosen = b.autotune(trials=1, **_kw(vs[0]))
-        assert set(chosen) == {"camera_dynamic", "trace_dynamic", "tail_divisor", "resident_percent", "megakernel"}
+        assert set(chosen) == {"camera_dynamic", "trace_dynamic", "tail_divisor", "resident_percent", "megakernel",
+                               "camera_tile"}
         assert chosen["tail_divisor"] in (0, 16) and chosen["resident_percent"] in (50, 100)
         x = a.render(**_kw(vs[0], 5))["color"]
         y = b.render(**_kw(vs[0], 5))["color"]

@@ -363,14 +363,17 @@ def test_tuning_does_not_change_results(renderer):
     # drivers, the tail taking every bounce from the first that held fewer
     # rays than the camera's (1), the reference's rule (16), or none (0), and
     # the persistent kernels' resident workgroups (percent of the occupancy's)
-    for cam, tr, div, pct in [(1, 1, 16, 0), (0, 0, 0, 50), (0, 1, 1, 25), (1, 0, 4, 1)]:
-        renderer.set_tuning(camera_dynamic=cam, trace_dynamic=tr, tail_divisor=div, resident_percent=pct)
+    for cam, tr, div, pct, tile in [(1, 1, 16, 0, 0), (0, 0, 0, 50, 1), (0, 1, 1, 25, 2), (1, 0, 4, 1, 1)]:
+        renderer.set_tuning(camera_dynamic=cam, trace_dynamic=tr, tail_divisor=div, resident_percent=pct,
+                            camera_tile=tile)
         renderer.render(sc.world_to_clip, 64, 64, frame_count=1, max_bounces=8, sample_index=50)  # restart
         outs.append(renderer.render(sc.world_to_clip, 64, 64, frame_count=1, max_bounces=8, sample_index=5,
                                     light_sampling=hip.LIGHT_MIS))
     with pytest.raises(hip.LotusError, match="resident_percent"):
         renderer.set_tuning(resident_percent=101)
-    renderer.set_tuning(camera_dynamic=0, trace_dynamic=0, tail_divisor=16, resident_percent=0)
+    with pytest.raises(hip.LotusError, match="camera_tile"):
+        renderer.set_tuning(camera_tile=3)
+    renderer.set_tuning(camera_dynamic=0, trace_dynamic=0, tail_divisor=16, resident_percent=0, camera_tile=0)
     for o in outs[1:]:
         assert np.array_equal(o["counts"], outs[0]["counts"])
         assert np.array_equal(o["color"], outs[0]["color"])
