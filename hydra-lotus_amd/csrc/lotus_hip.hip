@@ -3563,6 +3563,7 @@ hipError_t fused_paths(lotus_hip_renderer* root, lotus_hip_renderer* const* rend
                          root->stream);
     if (e == hipSuccess) launch_zero(root->fused_queue, 3 * kXcds, root->stream);
     Launch L = launch_for(root, root->stream);
+    L.grid = root->grid;  // the batch's launch: not the leader view's resident share
     L.spill = root->fused_spill;
     L.spill_cap = static_cast<int>(root->fused_spill_entries);
     size_t offset = 0;
